@@ -1,0 +1,191 @@
+"""fargo_thorin_amd -- MI355X-native hydro step (AlgoGas) of FARGO_THORIN.
+
+The compute path is libthorin_amd.so (HIP kernels for gfx950 behind the C ABI
+of include/thorin_amd.h).  This module is a thin ctypes binding that mirrors
+the reference's host interface:
+
+    Params      in.par parameters (ReadVariables, src_main/Interpret.c:65-266)
+    Disk        the PolarGrid fields resident on the GPU; Disk.algogas() is
+                one call of AlgoGas (src_main/SourceEuler.c:210-317)
+    Planets     the planetary system (types.h:96-117)
+
+Nothing here computes physics on the CPU; without the built library every
+entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional
+
+import numpy as np
+
+from . import _capi
+from ._capi import FIELD_ID, FIELDS, OPACITY, Planets, State, StepLog, last_error, lib
+from .seed import seeded_disk, write_cfg
+
+__all__ = ["Params", "Disk", "Planets", "State", "StepLog", "seeded_disk", "write_cfg", "FIELDS",
+           "OPACITY", "ThorinError", "make_planets"]
+
+
+class ThorinError(RuntimeError):
+    pass
+
+
+def _check(rc: int, what: str) -> int:
+    if rc < 0:
+        raise ThorinError(f"{what} failed: {last_error()}")
+    return rc
+
+
+class Params:
+    """in.par parameters with the reference's defaults (var.c:19-139)."""
+
+    def __init__(self, path: Optional[str] = None, **keys):
+        L = lib()
+        self._p = L.thorin_params_new()
+        if not self._p:
+            raise ThorinError("out of memory")
+        if path is not None:
+            _check(L.thorin_params_read(path.encode(), self._p), f"reading {path}")
+        for k, v in keys.items():
+            self.set(k, v)
+        self.derive()
+
+    def set(self, key: str, value) -> "Params":
+        _check(lib().thorin_params_set(self._p, key.encode(), str(value).encode()), f"set {key}")
+        return self
+
+    def derive(self) -> "Params":
+        _check(lib().thorin_params_derive(self._p), "parameter check")
+        return self
+
+    def get(self, key: str) -> str:
+        buf = C.create_string_buffer(1024)
+        _check(lib().thorin_params_get(self._p, key.encode(), buf, 1024), f"get {key}")
+        return buf.value.decode()
+
+    def num(self, key: str) -> float:
+        return float(self.get(key))
+
+    def flag(self, name: str) -> int:
+        return lib().thorin_params_flag(self._p, name.encode())
+
+    @property
+    def ptr(self):
+        return self._p
+
+    def __del__(self):
+        try:
+            if getattr(self, "_p", None):
+                lib().thorin_params_free(self._p)
+                self._p = None
+        except Exception:
+            pass
+
+
+def make_planets(masses, radii_au, feel_disk: bool = False) -> Planets:
+    """Circular, coplanar planets at phase 0 (the state REBOUND's
+    reb_tools_orbit_to_particle gives for e = inc = f = 0)."""
+    s = Planets()
+    s.nb = len(masses)
+    for k, (m, a) in enumerate(zip(masses, radii_au)):
+        m = float(np.float32(m))   # the planet file is parsed as float (ReboundInterface.c:162)
+        a = float(np.float32(a))
+        s.mass[k] = m
+        s.x[k] = a
+        s.vy[k] = (1.0 * (1.0 + m) / a) ** 0.5
+        s.feeldisk[k] = int(feel_disk)
+    return s
+
+
+class Disk:
+    """All PolarGrid fields of one disk, resident on the GPU."""
+
+    def __init__(self, params: Params):
+        self.params = params
+        L = lib()
+        h = C.c_void_p()
+        _check(L.thorin_ctx_create(params.ptr, C.byref(h)), "thorin_ctx_create")
+        self._h = h
+        self.nrad = int(params.get("NRAD"))
+        self.nsec = int(params.get("NSEC"))
+        self.log = StepLog()
+        self._nbody = _capi.host_nbody_fn()
+
+    # --- initialisation (Init.c:72-145)
+    def init(self, dens=None, vrad=None, vtheta=None, temper=None, mode: Optional[int] = None) -> None:
+        if dens is None:
+            _check(lib().thorin_init_state(self._h, 0, None, None, None, None), "init")
+            return
+        arrs = [np.ascontiguousarray(a, dtype=np.float64).ravel() for a in (dens, vrad, vtheta, temper)]
+        n = self.nrad * self.nsec
+        for a in arrs:
+            if a.size != n:
+                raise ValueError(f"initial arrays must hold NRAD*NSEC = {n} values")
+        ptrs = [a.ctypes.data_as(C.POINTER(C.c_double)) for a in arrs]
+        _check(lib().thorin_init_state(self._h, 1 if mode is None else mode, *ptrs), "init")
+
+    # --- the hot path
+    def algogas(self, planets: Planets, nbody=None) -> StepLog:
+        """One DT of AlgoGas; returns the step log (substeps, dt, SOR iterations)."""
+        cb = self._nbody if nbody is None else nbody
+        rc = lib().thorin_algogas(self._h, C.byref(planets), cb, None, C.byref(self.log))
+        _check(rc, "thorin_algogas")
+        self.last_rc = rc
+        return self.log
+
+    # --- fields
+    def field(self, name: str, ghost: bool = False) -> np.ndarray:
+        rings = self.nrad + (1 if ghost else 0)
+        out = np.empty((rings, self.nsec), dtype=np.float64)
+        _check(lib().thorin_download_field(self._h, FIELD_ID[name], out.ctypes.data_as(C.POINTER(C.c_double)),
+                                           out.size), f"download {name}")
+        return out
+
+    def upload(self, name: str, arr: np.ndarray) -> None:
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        _check(lib().thorin_upload_field(self._h, FIELD_ID[name], a.ctypes.data_as(C.POINTER(C.c_double)),
+                                         a.size), f"upload {name}")
+
+    def ring_means(self, name: str) -> np.ndarray:
+        out = np.empty(self.nrad)
+        _check(lib().thorin_ring_means(self._h, FIELD_ID[name], out.ctypes.data_as(C.POINTER(C.c_double))),
+               "ring means")
+        return out
+
+    def total_mass(self) -> float:
+        v = C.c_double()
+        _check(lib().thorin_gas_total_mass(self._h, C.byref(v)), "total mass")
+        return v.value
+
+    def state(self) -> State:
+        s = State()
+        _check(lib().thorin_get_state(self._h, C.byref(s)), "state")
+        return s
+
+    def set_state(self, s: State) -> None:
+        _check(lib().thorin_set_state(self._h, C.byref(s)), "state")
+
+    def geometry(self) -> dict:
+        g = _capi.Geometry()
+        _check(lib().thorin_get_geometry(self._h, C.byref(g)), "geometry")
+        n = self.nrad + 1
+        out = {}
+        for name, _ in _capi.Geometry._fields_[2:]:
+            p = getattr(g, name)
+            out[name] = np.ctypeslib.as_array(p, shape=(n,)).copy()
+        return out
+
+    def sync(self) -> None:
+        _check(lib().thorin_sync(self._h), "sync")
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().thorin_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,52 @@
+"""The benchmark and test workloads as in.par key sets.
+
+Every key exists in the reference's registry (src_main/var.c:19-139) except
+POWERLAWOPACITY, the build-only kappa = k0*T^2 option.
+
+  plain_128x256    : 128x256 log grid on [0.4, 2.5], Sigma0 5e-4 r^-0.5, h 0.05,
+                     alpha 1e-3, gamma 1.4, energy equation with the implicit
+                     radiative diffusion (SOR), LP85 opacity, WALL inner
+                     boundary, one 1e-13 planet at r = 1, DT 1e-3, 200 steps
+  damped_1024x1536 : the same physics at 1024x1536 with damping zones at both
+                     radial edges (ZEROVRAD), 100 steps -- the hydro-step
+                     benchmark workload
+"""
+from __future__ import annotations
+
+BASE = {
+    "Sigma0": "5e-4", "SigmaSlope": "0.5", "AspectRatio": "0.05", "FlaringIndex": "0.0",
+    "AlphaViscosity": "1e-3", "Adiabind": "1.4", "EnergyEquation": "YES", "CoolingTime": "-1.0",
+    "StellarIrradiation": "NO", "ThicknessSmoothing": "0.6", "Transport": "FARGO",
+    "RadialSpacing": "LOG", "Rmin": "0.4", "Rmax": "2.5", "OpacityTable": "LP85",
+    "PlanetsFeelDisk": "NO", "WriteTorqueFiles": "YES", "DT": "1e-3", "InitializeFromFile": "YES",
+    "Frame": "FIXED", "OmegaFrame": "0.0", "IndirectTerm": "YES",
+}
+
+PLAIN_128x256 = dict(BASE, Nrad="128", Nsec="256", InnerBoundary="WALL", Ntot="200", Ninterm="200")
+
+DAMPED_1024x1536 = dict(BASE, Nrad="1024", Nsec="1536", InnerBoundary="DAMPING", DampTowards="ZEROVRAD",
+                        Ntot="100", Ninterm="100")
+
+PLANET_ZERO_MASS = [(1e-13, 1.0)]
+
+
+def with_grid(cfg: dict, nrad: int, nsec: int, **extra) -> dict:
+    out = dict(cfg, Nrad=str(nrad), Nsec=str(nsec))
+    out.update({k: str(v) for k, v in extra.items()})
+    return out
+
+
+def inpar_text(cfg: dict, outputdir: str, planetfile: str, cfgdir: str) -> str:
+    """An in.par for the reference's reader (Interpret.c:84-119)."""
+    lines = [f"{k}\t{v}" for k, v in cfg.items()]
+    lines += [f"OutputDir\t{outputdir}", f"PlanetConfig\t{planetfile}",
+              f"DensInfile\t{cfgdir}/gasdens.cfg", f"VradInfile\t{cfgdir}/gasvrad.cfg",
+              f"VthetaInfile\t{cfgdir}/gasvtheta.cfg", f"TemperInfile\t{cfgdir}/gastemper.cfg"]
+    return "\n".join(lines) + "\n"
+
+
+def planet_text(planets) -> str:
+    """PlanetConfig lines: name, mass, gas, solid, a, e, inc, Omega, omega, f
+    (ReboundInterface.c:157-166)."""
+    return "".join(f"Planet{k + 1}\t{m!r}\t0.0\t0.0\t{a!r}\t0.0\t0.0\t0.0\t0.0\t0.0\n"
+                   for k, (m, a) in enumerate(planets))

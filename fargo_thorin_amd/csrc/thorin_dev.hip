@@ -1,0 +1,1894 @@
+// thorin_dev.hip -- MI355X (gfx950) kernels and the device driver of AlgoGas.
+//
+// One DT of the reference's AlgoGas (src_main/SourceEuler.c:210-317) runs as
+// a sequence of HIP kernels on one stream; every PolarGrid lives in HBM for
+// the whole run.  All arithmetic is IEEE FP64 compiled with
+// -ffp-contract=off and keeps the reference's operation order, so every
+// stencil reproduces the reference bit for bit; only OCML's exp/log/pow/tanh
+// differ from glibc by at most a few ulp.  Ring means that feed integer
+// decisions (Vmoy, VMed -> Nshift, globcsvec, globtempervec) are summed in
+// the reference's sequential order (mpiTasks.c:26-32, TransportEuler.c:86-91,
+// SourceEuler.c:609-618); every other reduction is a fixed-order block tree
+// and hence run-to-run deterministic.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <vector>
+
+#include "thorin_internal.h"
+
+#define HCHECK(call)                                                                     \
+  do {                                                                                   \
+    hipError_t e_ = (call);                                                              \
+    if (e_ != hipSuccess) {                                                              \
+      char m_[256];                                                                      \
+      snprintf(m_, sizeof m_, "HIP error %s at %s:%d", hipGetErrorString(e_), __FILE__, \
+               __LINE__);                                                                \
+      thorin_set_error(m_);                                                              \
+      return -1;                                                                         \
+    }                                                                                    \
+  } while (0)
+
+#define DPI 3.14159265358979323844
+#define CFLSECURITY 0.5
+#define CVNR 1.41
+#define SORMAXITERS 100000
+#define SOREPS 1.0e-6
+#define SORMIN 1.0e-15
+#define SORPRECISION 1.0e-15
+#define OMEGAUPPER 1.9
+
+static const int TB = 256;          // threads per block of the cell kernels
+static const int RB = 256;          // threads per block of the ring kernels
+static const int MAXPL = THORIN_MAXPLANETS;
+
+// ---------------------------------------------------------------------------
+// device buffers
+enum {
+  B_DENS = THORIN_F_DENS, B_VRAD = THORIN_F_VRAD, B_VTHETA = THORIN_F_VTHETA,
+  B_ENERGY = THORIN_F_ENERGY, B_LABEL = THORIN_F_LABEL, B_TEMPER = THORIN_F_TEMPER,
+  B_PRESSURE = THORIN_F_PRESSURE, B_CS = THORIN_F_SOUNDSPEED, B_QPLUS = THORIN_F_QPLUS,
+  B_QMINUS = THORIN_F_QMINUS, B_OPACITY = THORIN_F_OPACITY, B_DIVV = THORIN_F_DIVV,
+  B_TAURR = THORIN_F_TAURR, B_TAURP = THORIN_F_TAURP, B_TAUPP = THORIN_F_TAUPP,
+  B_AGR = THORIN_F_AGR, B_AGT = THORIN_F_AGT, B_EINT = THORIN_F_ENERGYINT,
+  B_ENEW = THORIN_F_ENERGYNEW, B_VRADINT = THORIN_F_VRADINT, B_VTHETAINT = THORIN_F_VTHETAINT,
+  B_VRADNEW = THORIN_F_VRADNEW, B_VTHETANEW = THORIN_F_VTHETANEW,
+  B_DIFF = THORIN_NFIELDS, B_ML, B_MLIP, B_MLIM, B_MLJP, B_MLJM, B_RHS, B_TBACK,
+  B_RP, B_RM, B_TP, B_TM, B_XRP, B_XRM, B_XTP, B_XTM, B_XE, B_XRHO, B_ABS, B_ORD,
+  NBUF
+};
+
+struct DevCtl {                 // device-side control block, mirrored to pinned host memory
+  double newdt_raw;             // ConditionCFL minimum before the clamps
+  unsigned long long dtst_bits; // DiffusionTimestep minimum (positive double as bits)
+  int crash_e, crash_rho;
+  int sor_done, sor_niter, sor_iter;
+  unsigned int sor_count;       // blocks finished in the current SOR iteration
+  double sor_normres0, sor_normres;
+  double axstar, aystar;
+  double axpl[MAXPL], aypl[MAXPL];
+};
+
+struct PlanetK {                // per-planet constants of FillForcesArrays (Pframeforce.c:93-121)
+  double x, y, z, m, rhill, rst2, rst3, rst4, rsm2, rmorevert2;
+  int iring;                    // ring of ThicknessSmoothing, -1 outside the grid
+  int pad;
+};
+
+struct Geo {                    // passed by value to every kernel
+  int nr, ns;
+  const double *Rinf, *Rsup, *Rmed, *Surf, *InvRmed, *InvSurf, *InvDiffRmed, *InvDiffRsup, *InvRinf;
+  const double *OmegaInv, *SigmaMed, *SigmaInf, *EnergyMed, *VthetaMed, *WaveKiller;
+  const double *pow_rmed15, *pow_rinf15, *cav_med, *cav_inf, *cfl_pow2, *supp_torque;
+  const int *dampring;
+  double *gcs, *gtemp, *Vmoy, *VMed, *ringmin;
+  int *Nshift;
+  double adiabind, sqrt_adiabind_inv, cv, viscosity, alphavisc, alphamri, alphadead, tmri, twidth;
+  double t2si, opa2cu, rho2cgs, stefan, sqrt2pi_inv, sqrt3, opacitydrop, parametricopacity;
+  double powerlawopacity, gasflux, omega_frame, lr01, lr12, thicksmooth;
+  int visc_alpha, alpha_flock, fast_transport, damp_init, damp_flux, opacity_law, stellar_irr;
+  int indirect_term, exclude_hill;
+  double hillcut;
+  // LP85 thresholds folded on the host (Opacity.c:205-220)
+  double lp_t1, lp_t2, lp_t3a, lp_t4, lp_t5a, lp_t6a;
+};
+
+struct thorin_dev {
+  hipStream_t s;
+  int nr, ns;
+  size_t nf;
+  double *b[NBUF];
+  double *d1base;
+  Geo g;
+  DevCtl *ctl;                  // device
+  DevCtl *hctl;                 // pinned host mirror
+  PlanetK *pk, *hpk;            // device / pinned host planet constants
+  double *red;                  // block partial sums (forces)
+  double *sorp;                 // SOR partial sums (2 passes)
+  int *sorc;                    // SOR partial counts
+  double *ringtmp;              // per-ring scratch
+  int nblk_cells, nblk_sor;
+  int opa_filled;
+};
+
+// ---------------------------------------------------------------------------
+// small device helpers
+
+__device__ __forceinline__ double fvisc_med(const Geo &g, int i) {
+  // FViscosity(Rmed[i]), Viscosity.c:26-54 (the radial search lands on ring i)
+  double v = g.viscosity;
+  if (g.visc_alpha) v = g.alphavisc * g.gcs[i] * g.gcs[i] * g.pow_rmed15[i];
+  if (g.alpha_flock) {
+    double a = (g.alphamri - g.alphadead) * (1.0 - tanh((g.tmri - g.gtemp[i] * g.t2si) / g.twidth)) / 2.0 + g.alphadead;
+    v = a * g.gcs[i] * g.gcs[i] * g.pow_rmed15[i];
+  }
+  return v * g.cav_med[i];
+}
+
+__device__ __forceinline__ double fvisc_inf(const Geo &g, int i) {
+  // FViscosity(Rinf[i])
+  double v = g.viscosity;
+  if (g.visc_alpha) v = g.alphavisc * g.gcs[i] * g.gcs[i] * g.pow_rinf15[i];
+  if (g.alpha_flock) {
+    double a = (g.alphamri - g.alphadead) * (1.0 - tanh((g.tmri - g.gtemp[i] * g.t2si) / g.twidth)) / 2.0 + g.alphadead;
+    v = a * g.gcs[i] * g.gcs[i] * g.pow_rinf15[i];
+  }
+  return v * g.cav_inf[i];
+}
+
+// x^3 rounded once (double-double product): glibc's pow(x,3.0) is correctly
+// rounded in all but pathological cases, so this matches it where OCML's
+// pow would differ by an ulp.
+__device__ __forceinline__ double cube_cr(double x) {
+  double p = x * x;
+  double e1 = __builtin_fma(x, x, -p);
+  double q = p * x;
+  double e2 = __builtin_fma(p, x, -q);
+  return q + (e2 + e1 * x);
+}
+
+__device__ __forceinline__ double opacity_bl94(double rho, double T) {
+  // Opacity.c:31-75
+  const double kappa[8] = {2.0e-4, 2.0e16, 0.1, 2.0e81, 1.0e-8, 1.0e-36, 1.5e20, 0.348};
+  const double a[8] = {0.0, 0.0, 0.0, 1.0, 2.0 / 3.0, 1.0 / 3.0, 1.0, 0.0};
+  const double b[8] = {2.0, -7.0, 0.5, -24.0, 3.0, 10.0, -2.5, 0.0};
+#define TEMP(i, j) pow(kappa[j] / kappa[i] * pow(rho, a[j] - a[i]), 1.0 / (b[i] - b[j]))
+  double k_ = 1.0e-12;
+  if (T <= TEMP(2, 3)) {
+    double k1 = kappa[0] * pow(T, b[0]);
+    double k2 = kappa[1] * pow(T, b[1]);
+    double k3 = kappa[2] * pow(T, b[2]);
+    double t1 = 1.0 / (k1 * k1) + 1.0 / (k2 * k2);
+    t1 = 1.0 / (t1 * t1);
+    double t2 = pow(k3 / (1.0 + 1.0e22 * pow(T, -10.0)), 4.0);
+    k_ = pow(t1 + t2, 0.25);
+  } else if ((T > TEMP(2, 3)) && (T <= TEMP(3, 4))) k_ = kappa[3] * pow(rho, a[3]) * pow(T, b[3]);
+  else if ((T > TEMP(3, 4)) && (T <= TEMP(4, 5))) k_ = kappa[4] * pow(rho, a[4]) * pow(T, b[4]);
+  else if ((T > TEMP(4, 5)) && (T <= TEMP(5, 6))) k_ = kappa[5] * pow(rho, a[5]) * pow(T, b[5]);
+  else if ((T > TEMP(5, 6)) && (T <= TEMP(6, 7))) k_ = kappa[6] * pow(rho, a[6]) * pow(T, b[6]);
+  else k_ = kappa[7];
+#undef TEMP
+  return k_;
+}
+
+__device__ __forceinline__ double opacity_lp85(const Geo &g, double rho, double T) {
+  // Opacity.c:201-227; the constant thresholds come folded from the host
+  if (T < g.lp_t1) return 2.0e-4 * T * T;
+  if (T < g.lp_t2) return 2.0e16 * pow(T, -7.0);
+  if (T < g.lp_t3a * pow(rho, 1.0 / 15.0)) return 5.0e-3 * T;
+  if (T < g.lp_t4) return 2.0e34 * pow(rho, 2.0 / 3.0) * pow(T, -9.0);
+  if (T < g.lp_t5a * pow(rho, 1.0 / 21.0)) return 2.0e-8 * pow(rho, 2.0 / 3.0) * T * T * T;
+  if (T < g.lp_t6a * pow(rho, 4.0 / 75.0)) return 1.0e-36 * pow(rho, 1.0 / 3.0) * pow(T, 10.0);
+  return 1.5e20 * rho * pow(T, -5.0 / 2.0);
+}
+
+__device__ __forceinline__ double opacity_zhu12(double rho, double T) {
+  // Opacity.c:145-196
+  double xlop, xlp, xlt, pre;
+  pre = 8.3144598 * 1.0e7 / 2.4 * rho * T;
+  xlp = log10(pre);
+  xlt = log10(T);
+  if (xlt < 2.23567 + 0.01899 * (xlp - 5.)) xlop = 1.5 * (xlt - 1.16331) - 0.736364;
+  else if (xlt < 2.30713 + 0.01899 * (xlp - 5.)) xlop = -3.53154212 * xlt + 8.767726 - (7.24786 - 8.767726) * (xlp - 5.) / 16.;
+  else if (xlt < (17.7 - 0.62 + 1.5 * 2.30713) / (1.5 + 5.832)) xlop = 1.5 * (xlt - 2.30713) + 0.62;
+  else if (xlt < (5.9398 + 17.7) / (5.832 + 2.129)) xlop = -5.832 * xlt + 17.7;
+  else if (xlt < (129.88071 + 5.9398 + (142.996475 - 129.88071) * 0.1 * (xlp + 4.)) / (2.129 + 42.98075)) xlop = 2.129 * xlt - 5.9398;
+  else if (xlt < (129.88071 + 15.0125 + (142.996475 - 129.88071) * 0.1 * (xlp + 4.)) / (4.0625 + 42.98075))
+    xlop = 129.88071 - 42.98075 * xlt + (142.996475 - 129.88071) * 0.1 * (xlp + 4.0);
+  else if (xlt < 3.28 + xlp / 4. * 0.12) xlop = -15.0125 + 4.0625 * xlt;
+  else if (xlt < 3.41 + 0.03328 * xlp / 4.) xlop = 58.9294 - 18.4808 * xlt + (61.6346 - 58.9294) * xlp / 4.;
+  else if (xlt < 3.76 + (xlp - 4) / 2. * 0.03) xlop = -12.002 + 2.90477 * xlt + (xlp - 4) / 4. * (13.9953 - 12.002);
+  else if (xlt < 4.07 + (xlp - 4) / 2. * 0.08) xlop = -39.4077 + 10.1935 * xlt + (xlp - 4) / 2. * (40.1719 - 39.4077);
+  else if (xlt < 5.3715 + (xlp - 6) / 2. * 0.5594) xlop = 17.5935 - 3.3647 * xlt + (xlp - 6) / 2. * (17.5935 - 15.7376);
+  else xlop = -0.48;
+  if (xlop < 3.586 * xlt - 16.85 && xlt < 4.) xlop = 3.586 * xlt - 16.85;
+  return pow(10., xlop);
+}
+
+__device__ __forceinline__ double opacity_law(const Geo &g, double rho, double T) {
+  switch (g.opacity_law) {
+    case THORIN_OPA_LP85: return opacity_lp85(g, rho, T);
+    case THORIN_OPA_ZHU12: return opacity_zhu12(rho, T);
+    case THORIN_OPA_CONST: return g.parametricopacity;
+    case THORIN_OPA_POWERLAW: return g.powerlawopacity * T * T;
+    default: return opacity_bl94(rho, T);
+  }
+}
+
+// fixed-order block reduction (sum), blockDim.x == n threads, power of two
+template <int N>
+__device__ __forceinline__ double block_sum(double v, double *sh) {
+  int t = threadIdx.x;
+  sh[t] = v;
+  __syncthreads();
+#pragma unroll
+  for (int s = N / 2; s > 0; s >>= 1) {
+    if (t < s) sh[t] = sh[t] + sh[t + s];
+    __syncthreads();
+  }
+  double r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+template <int N>
+__device__ __forceinline__ double block_min(double v, double *sh) {
+  int t = threadIdx.x;
+  sh[t] = v;
+  __syncthreads();
+#pragma unroll
+  for (int s = N / 2; s > 0; s >>= 1) {
+    if (t < s) { double o = sh[t + s]; if (o < sh[t]) sh[t] = o; }
+    __syncthreads();
+  }
+  double r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+// sequential ring sum in the reference's order: thread 0 walks the ring in LDS
+__device__ __forceinline__ double ordered_ring_sum(const double *sh, int ns) {
+  double s = 0.0;
+  for (int j = 0; j < ns; j++) s += sh[j];
+  return s;
+}
+
+// ---------------------------------------------------------------------------
+// ConditionCFL (SourceEuler.c:589-712): per-ring Vmoy (ordered) and per-cell
+// limits; the block minimum goes to ringmin[i].
+__global__ void __launch_bounds__(RB) k_cfl_ring(Geo g, const double *__restrict__ vr,
+                                                 const double *__restrict__ vt,
+                                                 const double *__restrict__ cs, double invdtreb_sq) {
+  extern __shared__ double sh[];       // ns + RB doubles
+  double *red = sh + g.ns;
+  const int i = blockIdx.x, ns = g.ns, nr = g.nr;
+  const double *vti = vt + (size_t)i * ns;
+  for (int j = threadIdx.x; j < ns; j += RB) sh[j] = vti[j];
+  __syncthreads();
+  __shared__ double vmoy;
+  if (threadIdx.x == 0) {
+    double s = ordered_ring_sum(sh, ns);
+    s /= (double)ns;
+    vmoy = s;
+    g.Vmoy[i] = s;
+  }
+  __syncthreads();
+  double m = 1e30;
+  if (i >= 1 && i < nr) {
+    const double dxrad = g.Rsup[i] - g.Rinf[i];
+    const double dxtheta = g.Rmed[i] * 2.0 * DPI / (double)ns;
+    const double mn = (dxtheta < dxrad) ? dxtheta : dxrad;
+    const int viscous = g.visc_alpha || (g.viscosity != 0.0);
+    const double visc = viscous ? fvisc_med(g, i) : 0.0;
+    const double invdt5 = viscous ? visc * 4. / g.cfl_pow2[i] : 1e-30;
+    for (int j = threadIdx.x; j < ns; j += RB) {
+      const size_t l = (size_t)i * ns + j;
+      const int jp = (j == ns - 1) ? 0 : j + 1;
+      const double vres = g.fast_transport ? sh[j] - vmoy : sh[j];
+      const double invdt1 = cs[l] / mn;
+      const double invdt2 = fabs(vr[l]) / dxrad;
+      const double invdt3 = fabs(vres) / dxtheta;
+      double dvr = vr[l + ns] - vr[l];
+      double dvt = sh[jp] - sh[j];
+      if (dvr >= 0.0) dvr = 1e-10; else dvr = -dvr;
+      if (dvt >= 0.0) dvt = 1e-10; else dvt = -dvt;
+      double a = dvr / dxrad, b = dvt / dxtheta;
+      double invdt4 = (b > a) ? b : a;
+      invdt4 *= 4.0 * CVNR * CVNR;
+      const double dt = CFLSECURITY / sqrt(invdt1 * invdt1 + invdt2 * invdt2 + invdt3 * invdt3 + invdt4 * invdt4 +
+                                           invdt5 * invdt5 + invdtreb_sq);
+      if (dt < m) m = dt;
+    }
+  }
+  m = block_min<RB>(m, red);
+  if (threadIdx.x == 0) g.ringmin[i] = m;
+}
+
+// minimum over rings plus the shear limit between rings (SourceEuler.c:673-676)
+__global__ void __launch_bounds__(RB) k_cfl_final(Geo g, DevCtl *ctl) {
+  __shared__ double red[RB];
+  double m = 1e30;
+  for (int i = 1 + threadIdx.x; i < g.nr; i += RB) { double v = g.ringmin[i]; if (v < m) m = v; }
+  for (int i = threadIdx.x; i < g.nr - 1; i += RB) {
+    double dt = 2.0 * DPI * CFLSECURITY / (double)g.ns /
+                fabs(g.Vmoy[i] * g.InvRmed[i] - g.Vmoy[i + 1] * g.InvRmed[i + 1]);
+    if (dt < m) m = dt;
+  }
+  m = block_min<RB>(m, red);
+  if (threadIdx.x == 0) ctl->newdt_raw = m;
+}
+
+// ---------------------------------------------------------------------------
+// FillForcesArrays (Pframeforce.c:43-510), gas part.  One thread per cell;
+// per-block partial sums of the disk->star and disk->planet accelerations.
+__global__ void __launch_bounds__(TB) k_forces(Geo g, const double *__restrict__ rho,
+                                               const double *__restrict__ cs,
+                                               const double *__restrict__ abs_,
+                                               const double *__restrict__ ord,
+                                               double *__restrict__ agr, double *__restrict__ agt,
+                                               const PlanetK *__restrict__ pk, int npl, double mt,
+                                               double *__restrict__ partial) {
+  __shared__ double red[TB];
+  const int ns = g.ns, i = blockIdx.y;
+  const int j = blockIdx.x * TB + threadIdx.x;
+  const bool act = j < ns;
+  double axs = 0.0, ays = 0.0;
+  double axpl_loc[8], aypl_loc[8];
+  for (int k = 0; k < 8; k++) { axpl_loc[k] = 0.0; aypl_loc[k] = 0.0; }
+  if (act) {
+    const size_t l = (size_t)i * ns + j;
+    const double omegainv = g.OmegaInv[i];
+    const double r2 = g.Rmed[i] * g.Rmed[i];
+    double Rsup_Rinf2 = g.Rsup[i] - g.Rinf[i];
+    Rsup_Rinf2 *= Rsup_Rinf2;
+    // ring average (cs_avr equals globcsvec[i]: same sequential sum)
+    const double H_avr = g.gcs[i] * omegainv * g.sqrt_adiabind_inv;
+    double H2 = H_avr * H_avr;
+    double zmax = 3.0 * H_avr;
+    double deltaz = zmax / (double)10;
+    double sigma_avr = 0.0, integstar_avr = 0.0;
+    for (int m = 1; m <= 10; m++) {
+      double znode = ((double)m - 0.5) * deltaz;
+      double znode2 = znode * znode;
+      double sum = exp(-(0.5 * znode2 / H2));
+      sigma_avr += sum;
+      double d2 = r2 + znode2;
+      double d = sqrt(d2);
+      double denom = d * d2;
+      integstar_avr += sum / denom;
+    }
+    int IntegrateLocally = 0, Nvert = 10;
+    double ax = 0.0, ay = 0.0;
+    const double x = abs_[l], y = ord[l];
+    const double mcell = rho[l] * g.Surf[i];
+    const double H = cs[l] * omegainv * g.sqrt_adiabind_inv;
+    H2 = H * H;
+    if (fabs(H - H_avr) / H_avr > 0.05) IntegrateLocally = 1;
+    unsigned int withpl = 0;    // bit k: IntegrateWithPlanet[k] (npl <= 8 on this path)
+    double s2[8], hillcut[8];
+    for (int k = 0; k < npl; k++) {
+      const PlanetK P = pk[k];
+      double smoothing = 0.0;
+      if (P.iring >= 0) smoothing = g.thicksmooth * (g.gcs[P.iring] * g.OmegaInv[P.iring] * g.sqrt_adiabind_inv);
+      const double Hpl = smoothing / g.thicksmooth;
+      s2[k] = (x - P.x) * (x - P.x) + (y - P.y) * (y - P.y);
+      hillcut[k] = 1.0;
+      double r2chk_1 = 10.0 * Hpl;
+      r2chk_1 *= r2chk_1;
+      double r2chk_2 = 5.0 * P.rhill;
+      r2chk_2 *= r2chk_2;
+      if (mt == 1.0 && (s2[k] < r2chk_1 || s2[k] < r2chk_2)) {
+        IntegrateLocally = 1;
+        withpl |= 1u << k;
+        double d2 = s2[k] + P.z * P.z;
+        if (g.exclude_hill) hillcut[k] = 1.0 / (exp(-(sqrt(d2) / P.rhill - g.hillcut) / g.hillcut * 10.0) + 1.0);
+      }
+      if (s2[k] < P.rmorevert2) Nvert = 30;
+    }
+    double sigma, integstar, integpl[8];
+    for (int k = 0; k < 8; k++) integpl[k] = 0.0;
+    if (IntegrateLocally) {
+      zmax = 3.0 * H;
+      deltaz = zmax / (double)Nvert;
+      sigma = 0.0;
+      integstar = 0.0;
+      for (int m = 1; m <= Nvert; m++) {
+        double znode = ((double)m - 0.5) * deltaz;
+        double znode2 = znode * znode;
+        double sum = exp(-(0.5 * znode2 / H2));
+        sigma += sum;
+        double d2 = r2 + znode2;
+        double d = sqrt(d2);
+        double denom = d * d2;
+        integstar += sum / denom;
+        for (int k = 0; k < npl; k++) {
+          if (withpl & (1u << k)) {
+            const PlanetK P = pk[k];
+            double zpl = P.z, taper;
+            d2 = s2[k] + (znode - zpl) * (znode - zpl);
+            if (d2 < P.rst2) {
+              d = sqrt(d2);
+              taper = -3.0 * d2 * d2 / P.rst4 + 4.0 * d * d2 / P.rst3;
+            } else taper = 1.0;
+            d2 += (P.rsm2 > Rsup_Rinf2 ? P.rsm2 : Rsup_Rinf2);
+            d = sqrt(d2);
+            denom = d * d2;
+            integpl[k] += sum / denom * taper;
+            if (zpl < -0.05 * H || zpl > 0.05 * H) {
+              d2 = s2[k] + (-znode - zpl) * (-znode - zpl);
+              if (d2 < P.rst2) {
+                d = sqrt(d2);
+                taper = -3.0 * d2 * d2 / P.rst4 + 4.0 * d * d2 / P.rst3;
+              } else taper = 1.0;
+              d2 += (P.rsm2 > Rsup_Rinf2 ? P.rsm2 : Rsup_Rinf2);
+              d = sqrt(d2);
+              denom = d * d2;
+              integpl[k] += sum / denom * taper;
+            }
+          }
+        }
+      }
+    } else {
+      sigma = sigma_avr;
+      integstar = integstar_avr;
+    }
+    double tmp = integstar / sigma;
+    double ar = -g.Rmed[i] * tmp;
+    double agr_l = 0.0 + ar;
+    axs = mcell * tmp * x;
+    ays = mcell * tmp * y;
+    for (int k = 0; k < npl; k++) {
+      const PlanetK P = pk[k];
+      double mpl = P.m;
+      if (withpl & (1u << k)) {
+        if (P.z < -0.05 * H || P.z > 0.05 * H) tmp = (x - P.x) * integpl[k] / (2.0 * sigma);
+        else tmp = (x - P.x) * integpl[k] / sigma;
+        ax += -mpl * tmp;
+        if (g.exclude_hill) tmp *= hillcut[k];
+        axpl_loc[k] = mcell * tmp;
+        if (P.z < -0.05 * H || P.z > 0.05 * H) tmp = (y - P.y) * integpl[k] / (2.0 * sigma);
+        else tmp = (y - P.y) * integpl[k] / sigma;
+        ay += -mpl * tmp;
+        if (g.exclude_hill) tmp *= hillcut[k];
+        aypl_loc[k] = mcell * tmp;
+      } else {
+        double smoothing = 0.0;
+        if (P.iring >= 0) smoothing = g.thicksmooth * (g.gcs[P.iring] * g.OmegaInv[P.iring] * g.sqrt_adiabind_inv);
+        double smoothing2 = smoothing * smoothing;
+        mpl *= mt;
+        double d2 = s2[k] + P.z * P.z + smoothing2;
+        double d = sqrt(d2);
+        double denom = d * d2;
+        tmp = (x - P.x) / denom;
+        ax += -mpl * tmp;
+        if (g.exclude_hill) tmp *= hillcut[k];
+        axpl_loc[k] = mcell * tmp;
+        tmp = (y - P.y) / denom;
+        ay += -mpl * tmp;
+        if (g.exclude_hill) tmp *= hillcut[k];
+        aypl_loc[k] = mcell * tmp;
+      }
+    }
+    ar = (x * ax + y * ay) * g.InvRmed[i];
+    double at = (x * ay - y * ax) * g.InvRmed[i];
+    agr_l += ar;
+    agr[l] = agr_l;
+    agt[l] = 0.0 + at;
+  }
+  // partial sums of this block: [axstar, aystar, axpl[k], aypl[k]]
+  const int nb = gridDim.x * gridDim.y, bid = blockIdx.y * gridDim.x + blockIdx.x;
+  double v = block_sum<TB>(axs, red);
+  if (threadIdx.x == 0) partial[bid] = v;
+  v = block_sum<TB>(ays, red);
+  if (threadIdx.x == 0) partial[nb + bid] = v;
+  for (int k = 0; k < npl; k++) {
+    v = block_sum<TB>(axpl_loc[k], red);
+    if (threadIdx.x == 0) partial[(2 + 2 * k) * nb + bid] = v;
+    v = block_sum<TB>(aypl_loc[k], red);
+    if (threadIdx.x == 0) partial[(3 + 2 * k) * nb + bid] = v;
+  }
+}
+
+// reduce the partial sums in block order; one block per quantity
+__global__ void __launch_bounds__(RB) k_forces_reduce(const double *__restrict__ partial, int nb, DevCtl *ctl) {
+  __shared__ double red[RB];
+  const int q = blockIdx.x;
+  double s = 0.0;
+  for (int b = threadIdx.x; b < nb; b += RB) s += partial[(size_t)q * nb + b];
+  s = block_sum<RB>(s, red);
+  if (threadIdx.x == 0) {
+    if (q == 0) ctl->axstar = s;
+    else if (q == 1) ctl->aystar = s;
+    else if ((q & 1) == 0) ctl->axpl[(q - 2) / 2] = s;
+    else ctl->aypl[(q - 3) / 2] = s;
+  }
+}
+
+// indirect term (Pframeforce.c:488-509)
+__global__ void __launch_bounds__(TB) k_indirect(Geo g, double *__restrict__ agr, double *__restrict__ agt,
+                                                 const double *__restrict__ abs_, const double *__restrict__ ord,
+                                                 const DevCtl *__restrict__ ctl, double indir_px, double indir_py) {
+  const int ns = g.ns, i = blockIdx.y, j = blockIdx.x * TB + threadIdx.x;
+  if (j >= ns) return;
+  double axindir = -ctl->axstar, ayindir = -ctl->aystar;
+  if (g.indirect_term) { axindir += indir_px; ayindir += indir_py; }
+  const size_t l = (size_t)i * ns + j;
+  const double x = abs_[l], y = ord[l];
+  const double ar = (x * axindir + y * ayindir) * g.InvRmed[i];
+  const double at = (x * ayindir - y * axindir) * g.InvRmed[i];
+  agr[l] += ar;
+  agt[l] += at;
+}
+
+// ---------------------------------------------------------------------------
+// DampingBoundary (SideEuler.c:368-419) on the rings flagged by the host
+__global__ void __launch_bounds__(TB) k_boundary(Geo g, double *__restrict__ vrad, double *__restrict__ vtheta,
+                                                 double *__restrict__ rho, double *__restrict__ energy, double step) {
+  const int ns = g.ns, i = blockIdx.y, j = blockIdx.x * TB + threadIdx.x;
+  if (j >= ns || !g.dampring[i]) return;
+  double vrad0 = 0.0, rho0 = 0.0, energy0 = 0.0, vtheta0 = 0.0;
+  if (g.damp_init) {
+    rho0 = g.SigmaMed[i];
+    energy0 = g.EnergyMed[i];
+    vtheta0 = g.VthetaMed[i] - g.Rmed[i] * g.omega_frame;
+  }
+  if (g.damp_flux) {
+    vrad0 = -g.gasflux / (2.0 * DPI * g.Rinf[i] * g.SigmaInf[i]);
+    rho0 = g.SigmaMed[i];
+  }
+  const double lambda = g.WaveKiller[i] * step;
+  const size_t l = (size_t)i * ns + j;
+  vrad[l] = (vrad[l] + lambda * vrad0) / (1.0 + lambda);
+  if (g.damp_init) {
+    rho[l] = (rho[l] + lambda * rho0) / (1.0 + lambda);
+    vtheta[l] = (vtheta[l] + lambda * vtheta0) / (1.0 + lambda);
+    energy[l] = (energy[l] + lambda * energy0) / (1.0 + lambda);
+  }
+  if (g.damp_flux) rho[l] = (rho[l] + lambda * rho0) / (1.0 + lambda);
+}
+
+// ---------------------------------------------------------------------------
+// SubStep1 source terms (SourceEuler.c:319-406) with P = (gamma-1) e on the
+// fly (ComputePressureField, SourceEuler.c:740-762) and DetectCrash
+// (SourceEuler.c:48-66, 272-273).
+__global__ void __launch_bounds__(TB) k_substep1(Geo g, const double *__restrict__ rho,
+                                                 const double *__restrict__ energy,
+                                                 const double *__restrict__ vrad,
+                                                 const double *__restrict__ vtheta,
+                                                 const double *__restrict__ agr,
+                                                 const double *__restrict__ agt,
+                                                 double *__restrict__ vradint, double *__restrict__ vthetaint,
+                                                 double dt, DevCtl *ctl) {
+  const int ns = g.ns, i = blockIdx.y, j = blockIdx.x * TB + threadIdx.x;
+  if (j >= ns) return;
+  const double gm1 = g.adiabind - 1.0;
+  const size_t l = (size_t)i * ns + j;
+  const size_t ljm = (j == 0) ? (size_t)i * ns + ns - 1 : l - 1;
+  const size_t ljp = (j == ns - 1) ? (size_t)i * ns : l + 1;
+  if (energy[l] < 0.0) ctl->crash_e = 1;
+  if (rho[l] < 0.0) ctl->crash_rho = 1;
+  const double pl = gm1 * energy[l];
+  if (i >= 1) {
+    const size_t lim = l - ns;
+    const double plim = gm1 * energy[lim];
+    double gradp = (pl - plim) * 2.0 / (rho[l] + rho[lim]) * g.InvDiffRmed[i];
+    double vt2 = vtheta[l] + vtheta[ljp] + vtheta[lim] + vtheta[ljp - ns];
+    vt2 = vt2 / 4.0 + g.Rinf[i] * g.omega_frame;
+    vt2 = vt2 * vt2;
+    double tmp = -gradp + 0.5 * (agr[l] + agr[lim]) + vt2 * g.InvRinf[i];
+    vradint[l] = vrad[l] + dt * tmp;
+  }
+  {
+    const double supp_torque = g.supp_torque[i];
+    const double dxtheta = 2.0 * DPI / (double)ns * g.Rmed[i];
+    const double invdxtheta = 1.0 / dxtheta;
+    const double pljm = gm1 * energy[ljm];
+    double gradp = (pl - pljm) * 2.0 / (rho[l] + rho[ljm]) * invdxtheta;
+    double tmp = -gradp + 0.5 * (agt[l] + agt[ljm]);
+    double v = vtheta[l] + dt * tmp;
+    v += dt * supp_torque;
+    vthetaint[l] = v;
+  }
+}
+
+// UpdateDivVelocAndStressTensor (Viscosity.c:85-162); Drr, Dpp, Drp stay in
+// registers.
+__global__ void __launch_bounds__(TB) k_divstress(Geo g, const double *__restrict__ vr,
+                                                  const double *__restrict__ vt,
+                                                  const double *__restrict__ rho,
+                                                  double *__restrict__ divv, double *__restrict__ Trr,
+                                                  double *__restrict__ Trp, double *__restrict__ Tpp) {
+  const int ns = g.ns, i = blockIdx.y, j = blockIdx.x * TB + threadIdx.x;
+  if (j >= ns) return;
+  const size_t l = (size_t)i * ns + j, lip = l + ns;
+  const size_t ljp = (j == ns - 1) ? (size_t)i * ns : l + 1;
+  const size_t ljm = (j == 0) ? (size_t)i * ns + ns - 1 : l - 1;
+  const double dphi = 2.0 * M_PI / (double)ns, invdphi = 1.0 / dphi, onethird = 1.0 / 3.0;
+  const double Drr = (vr[lip] - vr[l]) * g.InvDiffRsup[i];
+  const double Dpp = (vt[ljp] - vt[l]) * invdphi * g.InvRmed[i] + 0.5 * (vr[lip] + vr[l]) * g.InvRmed[i];
+  double dv = (vr[lip] * g.Rsup[i] - vr[l] * g.Rinf[i]) * g.InvDiffRsup[i] * g.InvRmed[i];
+  dv += (vt[ljp] - vt[l]) * invdphi * g.InvRmed[i];
+  divv[l] = dv;
+  const double visc = fvisc_med(g, i);
+  Trr[l] = 2.0 * rho[l] * visc * (Drr - onethird * dv);
+  Tpp[l] = 2.0 * rho[l] * visc * (Dpp - onethird * dv);
+  if (i >= 1) {
+    const size_t lim = l - ns, ljmim = ljm - ns;
+    const double Drp = 0.5 * (g.Rinf[i] * (vt[l] * g.InvRmed[i] - vt[lim] * g.InvRmed[i - 1]) * g.InvDiffRmed[i] +
+                              (vr[l] - vr[ljm]) * invdphi * g.InvRinf[i]);
+    const double vinf = fvisc_inf(g, i);
+    Trp[l] = 2.0 * 0.25 * (rho[l] + rho[lim] + rho[ljm] + rho[ljmim]) * vinf * Drp;
+  }
+}
+
+// UpdateVelocityWithViscousTerms (Viscosity.c:165-218) + ImposeKeplerianEdges
+// (Viscosity.c:221-260, VKepIn/VKepOut from the host)
+__global__ void __launch_bounds__(TB) k_visc_update(Geo g, double *__restrict__ vr, double *__restrict__ vt,
+                                                    const double *__restrict__ rho,
+                                                    const double *__restrict__ Trr,
+                                                    const double *__restrict__ Trp,
+                                                    const double *__restrict__ Tpp, double DeltaT,
+                                                    int kepler, double vkin, double vkout) {
+  const int ns = g.ns, nr = g.nr, i = blockIdx.y, j = blockIdx.x * TB + threadIdx.x;
+  if (j >= ns) return;
+  const size_t l = (size_t)i * ns + j, lip = l + ns;
+  const size_t ljp = (j == ns - 1) ? (size_t)i * ns : l + 1;
+  const size_t ljm = (j == 0) ? (size_t)i * ns + ns - 1 : l - 1;
+  const double invdphi = 1.0 / (2.0 * M_PI / (double)ns);
+  if (i >= 1 && i < nr - 1)
+    vt[l] += DeltaT * g.InvRmed[i] *
+             ((g.Rsup[i] * Trp[lip] - g.Rinf[i] * Trp[l]) * g.InvDiffRsup[i] + (Tpp[l] - Tpp[ljm]) * invdphi +
+              0.5 * (Trp[l] + Trp[lip])) /
+             (0.5 * (rho[l] + rho[ljm]));
+  if (i >= 1) {
+    const size_t lim = l - ns;
+    vr[l] += DeltaT * g.InvRinf[i] *
+             ((g.Rmed[i] * Trr[l] - g.Rmed[i - 1] * Trr[lim]) * g.InvDiffRmed[i] + (Trp[ljp] - Trp[l]) * invdphi -
+              0.5 * (Tpp[l] + Tpp[lim])) /
+             (0.5 * (rho[l] + rho[lim]));
+  }
+  if (kepler) {
+    if (i == 0) vt[l] = vkin - g.Rmed[0] * g.omega_frame;
+    if (i == nr - 1) vt[l] = vkout - g.Rmed[nr - 1] * g.omega_frame;
+  }
+}
+
+// SubStep2 (SourceEuler.c:417-504): artificial viscosity q computed on the fly
+__device__ __forceinline__ double qvnr(double rho, double dv) {
+  return (dv < 0.0) ? CVNR * CVNR * rho * dv * dv : 0.0;
+}
+
+__global__ void __launch_bounds__(TB) k_substep2(Geo g, const double *__restrict__ rho,
+                                                 const double *__restrict__ vrad,
+                                                 const double *__restrict__ vtheta,
+                                                 const double *__restrict__ energy,
+                                                 double *__restrict__ vradnew, double *__restrict__ vthetanew,
+                                                 double *__restrict__ energyint, double dt) {
+  const int ns = g.ns, nr = g.nr, i = blockIdx.y, j = blockIdx.x * TB + threadIdx.x;
+  if (j >= ns) return;
+  const size_t l = (size_t)i * ns + j, lip = l + ns;
+  const size_t ljp = (j == ns - 1) ? (size_t)i * ns : l + 1;
+  const size_t ljm = (j == 0) ? (size_t)i * ns + ns - 1 : l - 1;
+  const double dvr = vrad[lip] - vrad[l];
+  const double dvt = vtheta[ljp] - vtheta[l];
+  const double qr = qvnr(rho[l], dvr);
+  const double qt = qvnr(rho[l], dvt);
+  if (i >= 1) {
+    const size_t lim = l - ns;
+    const double qrim = qvnr(rho[lim], vrad[l] - vrad[lim]);
+    vradnew[l] = vrad[l] - dt * 2.0 / (rho[l] + rho[lim]) * (qr - qrim) * g.InvDiffRmed[i];
+  } else {
+    vradnew[l] = 0.0;                       // never written by the reference: stays 0
+    vradnew[(size_t)nr * ns + j] = 0.0;     // ghost ring nr likewise
+    vthetanew[(size_t)nr * ns + j] = 0.0;
+  }
+  const double dxtheta = 2.0 * DPI / (double)ns * g.Rmed[i];
+  const double invdxtheta = 1.0 / dxtheta;
+  const size_t ljmjp = (size_t)i * ns + j;   // (ljm)+1 == l
+  const double qtjm = qvnr(rho[ljm], vtheta[ljmjp] - vtheta[ljm]);
+  vthetanew[l] = vtheta[l] - dt * 2.0 / (rho[l] + rho[ljm]) * (qt - qtjm) * invdxtheta;
+  energyint[l] = energy[l] - dt * qr * (vrad[lip] - vrad[l]) * g.InvDiffRsup[i] -
+                 dt * qt * (vtheta[ljp] - vtheta[l]) * invdxtheta;
+}
+
+// ---------------------------------------------------------------------------
+// SubStep3 viscous heating Q+ (SourceEuler.c:541-567)
+__device__ __forceinline__ double qplus_cell(const Geo &g, int i, size_t l, const double *rho,
+                                             const double *Trr, const double *Trp, const double *Tpp,
+                                             const double *divv) {
+  const double visc = fvisc_med(g, i);
+  if (visc != 0.0) {
+    double q = 0.5 / visc / rho[l] * (Trr[l] * Trr[l] + 2.0 * Trp[l] * Trp[l] + Tpp[l] * Tpp[l]);
+    q += (2.0 / 9.0) * visc * rho[l] * divv[l] * divv[l];
+    return q;
+  }
+  return 0.0;
+}
+
+__global__ void __launch_bounds__(TB) k_qplus(Geo g, const double *__restrict__ rho,
+                                              const double *__restrict__ Trr, const double *__restrict__ Trp,
+                                              const double *__restrict__ Tpp, const double *__restrict__ divv,
+                                              double *__restrict__ qplus) {
+  const int ns = g.ns, i = blockIdx.y, j = blockIdx.x * TB + threadIdx.x;
+  if (j >= ns) return;
+  const size_t l = (size_t)i * ns + j;
+  if (i >= 1) {
+    qplus[l] = qplus_cell(g, i, l, rho, Trr, Trp, Tpp, divv);
+  } else {
+    const double v0 = fvisc_med(g, 0);
+    const double q1 = qplus_cell(g, 1, l + ns, rho, Trr, Trp, Tpp, divv);
+    const double q2 = qplus_cell(g, 2, l + 2 * ns, rho, Trr, Trp, Tpp, divv);
+    qplus[l] = (v0 != 0.0) ? q1 * exp(log(q1 / q2) * g.lr01 / g.lr12) : 0.0;
+  }
+}
+
+// ComputeTemperatureField + ComputeSoundSpeed (SourceEuler.c:715-792) from an
+// energy field, with the ordered ring means globtempervec / globcsvec.
+__global__ void __launch_bounds__(RB) k_temper_cs(Geo g, const double *__restrict__ rho,
+                                                  const double *__restrict__ e, double *__restrict__ T,
+                                                  double *__restrict__ cs) {
+  extern __shared__ double sh[];      // 2*ns
+  const int i = blockIdx.x, ns = g.ns;
+  const double gam = g.adiabind;
+  for (int j = threadIdx.x; j < ns; j += RB) {
+    const size_t l = (size_t)i * ns + j;
+    const double tmp2 = 1.0 / (rho[l] * 1.0);
+    const double tmp1 = e[l] * 1.0 * (gam - 1.0);
+    const double t = tmp1 * tmp2;
+    const double c = sqrt(gam * (gam - 1.0) * fabs(e[l]) / rho[l]);
+    T[l] = t;
+    cs[l] = c;
+    sh[j] = t;
+    sh[ns + j] = c;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = ordered_ring_sum(sh, ns);
+    s /= (double)ns;
+    g.gtemp[i] = s;
+  } else if (threadIdx.x == 64) {
+    double s = ordered_ring_sum(sh + ns, ns);
+    s /= (double)ns;
+    g.gcs[i] = s;
+  }
+}
+
+// MidplaneVolumeDensity + OpacityProfile + CalculateQminus
+// (EnergySources.c:658-680, Opacity.c:231-265, EnergySources.c:75-99)
+__global__ void __launch_bounds__(TB) k_opacity_qminus(Geo g, const double *__restrict__ rho,
+                                                       const double *__restrict__ cs,
+                                                       const double *__restrict__ T,
+                                                       double *__restrict__ opa, double *__restrict__ qminus,
+                                                       int fill_opacity) {
+  const int ns = g.ns, i = blockIdx.y, j = blockIdx.x * TB + threadIdx.x;
+  if (j >= ns) return;
+  const size_t l = (size_t)i * ns + j;
+  double o;
+  if (g.opacity_law == THORIN_OPA_CONST) {
+    o = fill_opacity ? g.parametricopacity * g.opa2cu : opa[l];
+  } else {
+    const double H = cs[l] * g.OmegaInv[i] * g.sqrt_adiabind_inv;
+    const double voldens = rho[l] / H * g.sqrt2pi_inv;
+    o = opacity_law(g, voldens * g.rho2cgs, T[l] * g.t2si) * g.opa2cu;
+  }
+  opa[l] = o;
+  const double tau = g.opacitydrop * 0.5 * o * rho[l];
+  const double taueff = 0.375 * tau + 0.25 * g.sqrt3 + 0.25 / tau;
+  qminus[l] = 2.0 * g.stefan * cube_cr(T[l]) / taueff;
+}
+
+// TemperatureGradient + DiffusionCoefs + DiffusionTimestep
+// (EnergySources.c:556-653, 833-856); |grad T| is rebuilt from T on the fly.
+__device__ __forceinline__ double gradtmag(const Geo &g, const double *T, int i, int j) {
+  const int ns = g.ns;
+  const size_t l = (size_t)i * ns + j, lip = l + ns, lim = l - ns, lipm = lip - ns;
+  const int jp = (j == ns - 1) ? 0 : j + 1;
+  const int jm = (j == 0) ? ns - 1 : j - 1;
+  const size_t ljp = (size_t)i * ns + jp;
+  const size_t ljm = (size_t)i * ns + jm;
+  const size_t ljpm = (size_t)i * ns + ((jp == 0) ? ns - 1 : jp - 1);
+  const double invdxtheta = 1.0 / (2.0 * DPI / (double)ns * g.Rmed[i]);
+  const double gtr_l = (T[l] - T[lim]) * g.InvDiffRmed[i];
+  const double gtr_lip = (T[lip] - T[lipm]) * g.InvDiffRmed[i + 1];
+  const double gtt_l = (T[l] - T[ljm]) * invdxtheta;
+  const double gtt_ljp = (T[ljp] - T[ljpm]) * invdxtheta;
+  const double r = 0.5 * (gtr_lip + gtr_l);
+  const double t = 0.5 * (gtt_ljp + gtt_l);
+  return sqrt(r * r + t * t);
+}
+
+__device__ __forceinline__ double flux_limiter(double s) {   // EnergySources.c:684-696
+  double tmp;
+  if (s <= 2.0) { tmp = 3.0 + sqrt(9.0 + 10.0 * s * s); return 2.0 / tmp; }
+  tmp = 10.0 * s + 9.0 + sqrt(81.0 + 180.0 * s);
+  return 10.0 / tmp;
+}
+
+__global__ void __launch_bounds__(TB) k_diffcoef(Geo g, const double *__restrict__ rho,
+                                                 const double *__restrict__ cs, const double *__restrict__ T,
+                                                 const double *__restrict__ opa, double *__restrict__ D,
+                                                 DevCtl *ctl) {
+  __shared__ double red[TB];
+  const int ns = g.ns, nr = g.nr, i = blockIdx.y, j = blockIdx.x * TB + threadIdx.x;
+  double dtmin = 1.0e38;
+  if (j < ns) {
+    const size_t l = (size_t)i * ns + j;
+    const int ig = (i == 0) ? 1 : ((i == nr - 1) ? nr - 2 : i);
+    const double gradt = gradtmag(g, T, ig, j);
+    const double H = cs[l] * g.OmegaInv[i] * g.sqrt_adiabind_inv;
+    const double voldens = rho[l] / H * g.sqrt2pi_inv;
+    const double fac = 1.0 / (voldens * opa[l]);
+    const double s = 4.0 * gradt * fac / T[l];
+    const double d = flux_limiter(s) * 16.0 * g.stefan * cube_cr(T[l]) * fac;
+    D[l] = d;
+    double deltar2 = g.Rsup[i] - g.Rinf[i];
+    deltar2 = deltar2 * deltar2 / 2.0;
+    const double dttmp = deltar2 / d;
+    if (dttmp < dtmin) dtmin = dttmp;
+  }
+  dtmin = block_min<TB>(dtmin, red);
+  if (threadIdx.x == 0 && dtmin > 0.0)
+    atomicMin(&ctl->dtst_bits, (unsigned long long)__double_as_longlong(dtmin));
+}
+
+// implicit energy matrix (EnergySources.c:256-297) with A and B rebuilt on the fly
+__device__ __forceinline__ double coefA(const Geo &g, const double *D, int i, size_t l) {
+  return 0.5 * (D[l] + D[l - g.ns]) * g.Rinf[i] * g.InvDiffRmed[i];
+}
+__device__ __forceinline__ double coefB(const Geo &g, const double *D, const double *cs, int i, int j) {
+  const int ns = g.ns;
+  const size_t l = (size_t)i * ns + j;
+  const size_t ljm = (j == 0) ? (size_t)i * ns + ns - 1 : l - 1;
+  const double dxtheta = 2.0 * DPI / (double)ns * g.Rmed[i];
+  const double invdxtheta2 = 1.0 / dxtheta / dxtheta;
+  const double H = cs[l] * g.OmegaInv[i] * g.sqrt_adiabind_inv;
+  return 0.5 * (D[l] + D[ljm]) * invdxtheta2 * 2.0 * H;
+}
+
+__global__ void __launch_bounds__(TB) k_matrix(Geo g, const double *__restrict__ rho,
+                                               const double *__restrict__ cs, const double *__restrict__ D,
+                                               const double *__restrict__ divv,
+                                               const double *__restrict__ qminus,
+                                               const double *__restrict__ qplus, const double *__restrict__ T,
+                                               double *__restrict__ Ml, double *__restrict__ Mlip,
+                                               double *__restrict__ Mlim, double *__restrict__ Mljp,
+                                               double *__restrict__ Mljm, double *__restrict__ RHS, double dt) {
+  const int ns = g.ns, i = blockIdx.y + 1, j = blockIdx.x * TB + threadIdx.x;
+  if (j >= ns) return;   // rings 1..nr-2 (grid.y = nr-2)
+  const size_t l = (size_t)i * ns + j, lip = l + ns;
+  const int jp = (j == ns - 1) ? 0 : j + 1;
+  const double Al = coefA(g, D, i, l), Alip = coefA(g, D, i + 1, lip);
+  const double Bl = coefB(g, D, cs, i, j), Bljp = coefB(g, D, cs, i, jp);
+  const double H = cs[l] * g.OmegaInv[i] * g.sqrt_adiabind_inv;
+  const double bfac = dt / (rho[l] * g.cv);
+  const double afac = bfac * g.InvDiffRsup[i] * 2.0 * H / g.Rmed[i];
+  Ml[l] = 1.0 + divv[l] * (g.adiabind - 1.0) * dt + 4.0 * bfac * qminus[l] + afac * (Alip + Al) + bfac * (Bljp + Bl);
+  Mlip[l] = -afac * Alip;
+  Mlim[l] = -afac * Al;
+  Mljp[l] = -bfac * Bljp;
+  Mljm[l] = -bfac * Bl;
+  RHS[l] = T[l] + bfac * (qplus[l] + 3.0 * qminus[l] * T[l]);
+}
+
+// ---------------------------------------------------------------------------
+// SOR (EnergySources.c:406-511).  Red-black in place: within one pass every
+// updated cell reads only cells of the other colour, so the parallel sweep
+// equals the reference's sequential sweep value for value.  Pass 0 updates
+// cells with (i+j) odd (jchess1st = 0 at i = 1), pass 1 cells with (i+j) even.
+__device__ __forceinline__ double sor_resid(const double *T, const double *Ml, const double *Mlip,
+                                            const double *Mlim, const double *Mljp, const double *Mljm,
+                                            const double *RHS, size_t l, size_t lip, size_t lim, size_t ljp,
+                                            size_t ljm) {
+  return T[l] * Ml[l] + T[lip] * Mlip[l] + T[lim] * Mlim[l] + T[ljp] * Mljp[l] + T[ljm] * Mljm[l] - RHS[l];
+}
+
+__global__ void __launch_bounds__(TB) k_sor_resid0(Geo g, const double *__restrict__ T,
+                                                   const double *__restrict__ Ml, const double *__restrict__ Mlip,
+                                                   const double *__restrict__ Mlim, const double *__restrict__ Mljp,
+                                                   const double *__restrict__ Mljm, const double *__restrict__ RHS,
+                                                   double *__restrict__ partial) {
+  __shared__ double red[TB];
+  const int ns = g.ns, i = blockIdx.y + 1, j = blockIdx.x * TB + threadIdx.x;
+  double r = 0.0;
+  if (j < ns) {
+    const size_t l = (size_t)i * ns + j;
+    const size_t ljp = (j == ns - 1) ? (size_t)i * ns : l + 1;
+    const size_t ljm = (j == 0) ? (size_t)i * ns + ns - 1 : l - 1;
+    r = fabs(sor_resid(T, Ml, Mlip, Mlim, Mljp, Mljm, RHS, l, l + ns, l - ns, ljp, ljm));
+  }
+  r = block_sum<TB>(r, red);
+  if (threadIdx.x == 0) partial[blockIdx.y * gridDim.x + blockIdx.x] = r;
+}
+
+__global__ void __launch_bounds__(RB) k_sum_to(const double *__restrict__ partial, int nb, double *out) {
+  __shared__ double red[RB];
+  double s = 0.0;
+  for (int b = threadIdx.x; b < nb; b += RB) s += partial[b];
+  s = block_sum<RB>(s, red);
+  if (threadIdx.x == 0) *out = s;
+}
+
+// one half-sweep; the block that finishes last in pass 1 evaluates the stop test
+__global__ void __launch_bounds__(TB) k_sor_half(Geo g, double *__restrict__ T, const double *__restrict__ Ml,
+                                                 const double *__restrict__ Mlip, const double *__restrict__ Mlim,
+                                                 const double *__restrict__ Mljp, const double *__restrict__ Mljm,
+                                                 const double *__restrict__ RHS, double omega, int pass,
+                                                 double *__restrict__ psum, int *__restrict__ pcnt,
+                                                 DevCtl *ctl, int maxiter) {
+  __shared__ double red[TB];
+  __shared__ int amlast;
+  if (ctl->sor_done) return;
+  const int ns = g.ns, half = ns / 2, i = blockIdx.y + 1, t = blockIdx.x * TB + threadIdx.x;
+  const int nbx = gridDim.x, nb = gridDim.x * gridDim.y, bid = blockIdx.y * nbx + blockIdx.x;
+  double r = 0.0;
+  int changed = 0;
+  if (t < half) {
+    const int j0 = (pass == 0) ? ((i - 1) & 1) : (i & 1);
+    const int j = 2 * t + j0;
+    const size_t l = (size_t)i * ns + j;
+    const size_t ljp = (j == ns - 1) ? (size_t)i * ns : l + 1;
+    const size_t ljm = (j == 0) ? (size_t)i * ns + ns - 1 : l - 1;
+    const double resid = sor_resid(T, Ml, Mlip, Mlim, Mljp, Mljm, RHS, l, l + ns, l - ns, ljp, ljm);
+    r = fabs(resid);
+    const double dtemp = omega * resid / Ml[l];
+    const double tn = T[l] - dtemp;
+    T[l] = tn;
+    if (dtemp > tn * SORPRECISION) changed = 1;
+  }
+  r = block_sum<TB>(r, red);
+  double c = block_sum<TB>((double)changed, red);
+  if (threadIdx.x == 0) {
+    psum[pass * nb + bid] = r;
+    pcnt[pass * nb + bid] = (int)c;
+  }
+  if (pass == 0) return;
+  // pass 1: last block closes the iteration
+  __threadfence();
+  if (threadIdx.x == 0) amlast = (atomicAdd(&ctl->sor_count, 1u) == (unsigned)(nb - 1));
+  __syncthreads();
+  if (!amlast) return;
+  __threadfence();
+  double s = 0.0;
+  long long cc = 0;
+  for (int b = threadIdx.x; b < nb; b += TB) {
+    s += psum[b];
+    cc += pcnt[b];
+  }
+  for (int b = threadIdx.x; b < nb; b += TB) {
+    s += psum[nb + b];
+    cc += pcnt[nb + b];
+  }
+  s = block_sum<TB>(s, red);
+  double ccd = block_sum<TB>((double)cc, red);
+  if (threadIdx.x == 0) {
+    const int n = ctl->sor_iter + 1;
+    ctl->sor_iter = n;
+    ctl->sor_count = 0;
+    ctl->sor_normres = s;
+    if (s < SOREPS * ctl->sor_normres0 || s < SORMIN || ccd == 0.0) {
+      ctl->sor_done = 1;
+      ctl->sor_niter = n;
+    } else if (n >= maxiter) {
+      ctl->sor_done = 1;
+      ctl->sor_niter = maxiter + 1;
+    }
+  }
+}
+
+// edges of T and EnergyNew = rho*T*CV (EnergySources.c:319-334)
+__global__ void __launch_bounds__(TB) k_energy_new(Geo g, const double *__restrict__ rho, double *__restrict__ T,
+                                                   double *__restrict__ enew) {
+  const int ns = g.ns, nr = g.nr, i = blockIdx.y, j = blockIdx.x * TB + threadIdx.x;
+  if (j >= ns) return;
+  const size_t l = (size_t)i * ns + j;
+  double t = T[l];
+  if (i == 0) { t = T[l + ns]; T[l] = t; }
+  if (i == nr - 1) { t = T[l - ns]; T[l] = t; }
+  enew[l] = rho[l] * t * g.cv;
+  if (i == 0) enew[(size_t)nr * ns + j] = 0.0;
+}
+
+// ---------------------------------------------------------------------------
+// Radial transport: ComputeLRMomenta (TransportEuler.c:318-347) + OneWindRad
+// (49-65) with ComputeStarRad (232-273), VanLeerRadial (384-415) and
+// DivisePolarGrid (SideEuler.c:82-99) for the six conserved quantities at
+// once.  Momenta and Work = Q/(RhoInt+1e-20) are rebuilt on the fly, which
+// gives the same values as the reference's stored arrays.
+struct RadQ {
+  const double *rho, *vr, *vt, *e;
+  double of;
+};
+
+__device__ __forceinline__ double radq_val(const Geo &g, const RadQ &q, int which, int i, int j) {
+  const int ns = g.ns;
+  const size_t l = (size_t)i * ns + j;
+  const double r = q.rho[l];
+  switch (which) {
+    case 0: return r * q.vr[l + ns];                                         // RadMomP
+    case 1: return r * q.vr[l];                                              // RadMomM
+    case 2: {
+      const size_t ljp = (j == ns - 1) ? (size_t)i * ns : l + 1;
+      return r * (q.vt[ljp] + g.Rmed[i] * q.of) * g.Rmed[i];                // ThetaMomP
+    }
+    case 3: return r * (q.vt[l] + g.Rmed[i] * q.of) * g.Rmed[i];            // ThetaMomM
+    case 4: return q.e[l];                                                   // Energy
+    default: return r;                                                       // Rho
+  }
+}
+
+__device__ __forceinline__ double rad_work(const Geo &g, const RadQ &q, int which, int i, int j) {
+  if (which == 5) return q.rho[(size_t)i * g.ns + j];                       // RhoStar: Qbase = Rho
+  return radq_val(g, q, which, i, j) / (q.rho[(size_t)i * g.ns + j] + 1e-20);
+}
+__device__ __forceinline__ double rad_work_rho(const Geo &g, const RadQ &q, int i, int j) {
+  return q.rho[(size_t)i * g.ns + j] / (q.rho[(size_t)i * g.ns + j] + 1e-20);
+}
+
+// dq of ComputeStarRad at ring k for work values w(k-1), w(k), w(k+1)
+__device__ __forceinline__ double rad_dq(const Geo &g, int k, double wm, double w0, double wp) {
+  if (k == 0 || k == g.nr - 1) return 0.0;
+  const double dqm = (w0 - wm) * g.InvDiffRmed[k];
+  const double dqp = (wp - w0) * g.InvDiffRmed[k + 1];
+  return (dqp * dqm > 0.0) ? 2.0 * dqp * dqm / (dqp + dqm) : 0.0;
+}
+
+// star value at interface ring k (1..nr-1) given w(k-2..k+1)
+__device__ __forceinline__ double rad_star(const Geo &g, int k, double vrk, double dt, const double *w /* w[0..3] = k-2..k+1 */) {
+  if (vrk > 0.0) {
+    const double dq = rad_dq(g, k - 1, w[0], w[1], w[2]);
+    return w[1] + (g.Rmed[k] - g.Rmed[k - 1] - vrk * dt) * 0.5 * dq;
+  }
+  const double dq = rad_dq(g, k, w[1], w[2], w[3]);
+  return w[2] - (g.Rmed[k + 1] - g.Rmed[k] + vrk * dt) * 0.5 * dq;
+}
+
+__global__ void __launch_bounds__(TB) k_transport_rad(Geo g, RadQ q, double dt, double *__restrict__ orp,
+                                                      double *__restrict__ orm, double *__restrict__ otp,
+                                                      double *__restrict__ otm, double *__restrict__ oe,
+                                                      double *__restrict__ orho) {
+  const int ns = g.ns, nr = g.nr, i = blockIdx.y, j = blockIdx.x * TB + threadIdx.x;
+  if (j >= ns) return;
+  const size_t l = (size_t)i * ns + j, lip = l + ns;
+  const double vrl = q.vr[l], vrlip = q.vr[lip];
+  const double dtheta = 2.0 * DPI / (double)ns;
+  // ring indices needed: i-2 .. i+2, clamped; values outside [0,nr-1] never used
+  int rk[5];
+  for (int k = 0; k < 5; k++) {
+    int r = i - 2 + k;
+    rk[k] = r < 0 ? 0 : (r > nr - 1 ? nr - 1 : r);
+  }
+  // RhoStar at i and i+1 (ComputeStarRad(Rho): Work = Rho itself)
+  double wr[5];
+  for (int k = 0; k < 5; k++) wr[k] = q.rho[(size_t)rk[k] * ns + j];
+  const double rs_l = (i == 0) ? 0.0 : rad_star(g, i, vrl, dt, wr);
+  const double rs_lip = (i + 1 >= nr) ? 0.0 : rad_star(g, i + 1, vrlip, dt, wr + 1);
+  double *outs[6] = {orp, orm, otp, otm, oe, orho};
+  for (int which = 0; which < 6; which++) {
+    double w[5];
+    for (int k = 0; k < 5; k++) {
+      const int r = rk[k];
+      w[k] = radq_val(g, q, which, r, j) / (q.rho[(size_t)r * ns + j] + 1e-20);
+    }
+    const double qs_l = (i == 0) ? 0.0 : rad_star(g, i, vrl, dt, w);
+    const double qs_lip = (i + 1 >= nr) ? 0.0 : rad_star(g, i + 1, vrlip, dt, w + 1);
+    double varq = dt * dtheta * g.Rinf[i] * qs_l * rs_l * vrl;
+    varq -= dt * dtheta * g.Rsup[i] * qs_lip * rs_lip * vrlip;
+    const double qb = radq_val(g, q, which, i, j);
+    outs[which][l] = qb + varq * g.InvSurf[i];
+  }
+}
+
+// Azimuthal transport, ring-local: ComputeResiduals (TransportEuler.c:70-111),
+// QuantitiesAdvection with the residual and the uniform velocity (159-175,
+// ComputeStarTheta 275-316, VanLeerTheta 417-445) and AdvectSHIFT (113-136),
+// all in LDS, one workgroup per ring.  The rho history (before/after the
+// first pass) and both RhoStar arrays stay in LDS while the five other
+// quantities stream through.
+__device__ __forceinline__ double th_dq(const double *w, const double *rint, int j, int ns, double invdxtheta,
+                                        bool rhow) {
+  const int jm = (j == 0) ? ns - 1 : j - 1, jp = (j == ns - 1) ? 0 : j + 1;
+  double wm, w0, wp;
+  if (rhow) { wm = w[jm]; w0 = w[j]; wp = w[jp]; }
+  else {
+    wm = w[jm] / (rint[jm] + 1e-20);
+    w0 = w[j] / (rint[j] + 1e-20);
+    wp = w[jp] / (rint[jp] + 1e-20);
+  }
+  const double dqm = (w0 - wm);
+  const double dqp = (wp - w0);
+  return ((dqp * dqm > 0.0) && (dqp + dqm != 0.0)) ? dqp * dqm / (dqp + dqm) * invdxtheta : 0.0;
+}
+
+// star value at j; rhow: the base is w itself (RhoStar), else w/(rint+1e-20)
+__device__ __forceinline__ double th_star(const double *w, const double *rint, int j, int ns, double dxtheta,
+                                          double invdxtheta, double ksi, bool rhow) {
+  const int jm = (j == 0) ? ns - 1 : j - 1;
+  if (ksi > 0.0) {
+    const double wjm = rhow ? w[jm] : w[jm] / (rint[jm] + 1e-20);
+    return wjm + (dxtheta - ksi) * th_dq(w, rint, jm, ns, invdxtheta, rhow);
+  }
+  const double wj = rhow ? w[j] : w[j] / (rint[j] + 1e-20);
+  return wj - (dxtheta + ksi) * th_dq(w, rint, j, ns, invdxtheta, rhow);
+}
+
+__global__ void __launch_bounds__(RB) k_transport_theta(Geo g, const double *__restrict__ vt,
+                                                        const double *__restrict__ xrp, const double *__restrict__ xrm,
+                                                        const double *__restrict__ xtp, const double *__restrict__ xtm,
+                                                        const double *__restrict__ xe, const double *__restrict__ xrho,
+                                                        double *__restrict__ orp, double *__restrict__ orm,
+                                                        double *__restrict__ otp, double *__restrict__ otm,
+                                                        double *__restrict__ oe, double *__restrict__ orho,
+                                                        double dt) {
+  extern __shared__ double sh[];       // 6*ns: vtr, rho0, rho1, rs1, rs2, q
+  const int i = blockIdx.x, ns = g.ns;
+  double *s_vtr = sh, *s_r0 = sh + ns, *s_r1 = sh + 2 * ns, *s_rs1 = sh + 3 * ns, *s_rs2 = sh + 4 * ns,
+         *s_q = sh + 5 * ns;
+  const size_t base = (size_t)i * ns;
+  __shared__ double vmed, vtemp;
+  __shared__ int nshift;
+  for (int j = threadIdx.x; j < ns; j += RB) { s_vtr[j] = vt[base + j]; s_r0[j] = xrho[base + j]; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double moy = ordered_ring_sum(s_vtr, ns);
+    const double vm = moy / (double)ns;
+    const double Ntilde = vm * g.InvRmed[i] * dt * (double)ns / 2.0 / DPI;
+    const double Nround = floor(Ntilde + 0.5);
+    vmed = vm;
+    nshift = (int)(long)Nround;
+    vtemp = g.fast_transport ? (Ntilde - Nround) * g.Rmed[i] * (1.0 / dt) * (2.0 * DPI / (double)ns) : 0.0;
+    g.VMed[i] = vm;
+    g.Nshift[i] = nshift;
+  }
+  __syncthreads();
+  const double dxtheta = 2.0 * DPI / (double)ns * g.Rmed[i];
+  const double invdxtheta = 1.0 / dxtheta;
+  const double dxrad = (g.Rsup[i] - g.Rinf[i]) * dt;
+  const double invsurf = 1.0 / g.Surf[i];
+  const bool fast = g.fast_transport != 0;
+  for (int j = threadIdx.x; j < ns; j += RB) s_vtr[j] = fast ? s_vtr[j] - vmed : s_vtr[j];
+  __syncthreads();
+  // pass 1: RhoStar from rho0 with the residual velocity
+  for (int j = threadIdx.x; j < ns; j += RB) s_rs1[j] = th_star(s_r0, s_r0, j, ns, dxtheta, invdxtheta, s_vtr[j] * dt, true);
+  __syncthreads();
+  // rho after pass 1 (VanLeerTheta on Rho, RhoInt = rho0)
+  for (int j = threadIdx.x; j < ns; j += RB) {
+    const int jp = (j == ns - 1) ? 0 : j + 1;
+    const double qs = th_star(s_r0, s_r0, j, ns, dxtheta, invdxtheta, s_vtr[j] * dt, false);
+    const double qsp = th_star(s_r0, s_r0, jp, ns, dxtheta, invdxtheta, s_vtr[jp] * dt, false);
+    double varq = dxrad * qs * s_rs1[j] * s_vtr[j];
+    varq -= dxrad * qsp * s_rs1[jp] * s_vtr[jp];
+    s_r1[j] = s_r0[j] + varq * invsurf;
+  }
+  __syncthreads();
+  const double ksi2 = vtemp * dt;
+  if (fast) {
+    for (int j = threadIdx.x; j < ns; j += RB) s_rs2[j] = th_star(s_r1, s_r1, j, ns, dxtheta, invdxtheta, ksi2, true);
+    __syncthreads();
+    // rho after pass 2, into s_q, then shifted out
+    for (int j = threadIdx.x; j < ns; j += RB) {
+      const int jp = (j == ns - 1) ? 0 : j + 1;
+      const double qs = th_star(s_r1, s_r1, j, ns, dxtheta, invdxtheta, ksi2, false);
+      const double qsp = th_star(s_r1, s_r1, jp, ns, dxtheta, invdxtheta, ksi2, false);
+      double varq = dxrad * qs * s_rs2[j] * vtemp;
+      varq -= dxrad * qsp * s_rs2[jp] * vtemp;
+      s_q[j] = s_r1[j] + varq * invsurf;
+    }
+  } else {
+    for (int j = threadIdx.x; j < ns; j += RB) s_q[j] = s_r1[j];
+  }
+  __syncthreads();
+  const int sh_n = fast ? nshift : 0;
+  for (int j = threadIdx.x; j < ns; j += RB) {
+    int ji = j - sh_n;
+    while (ji < 0) ji += ns;
+    while (ji >= ns) ji -= ns;
+    orho[base + j] = s_q[ji];
+  }
+  const double *ins[5] = {xrp, xrm, xtp, xtm, xe};
+  double *outs[5] = {orp, orm, otp, otm, oe};
+  for (int which = 0; which < 5; which++) {
+    __syncthreads();
+    for (int j = threadIdx.x; j < ns; j += RB) s_q[j] = ins[which][base + j];
+    __syncthreads();
+    // pass 1 into registers (at most ns/RB values per thread), then back to LDS
+    double nv[16];
+    int c = 0;
+    for (int j = threadIdx.x; j < ns; j += RB, c++) {
+      const int jp = (j == ns - 1) ? 0 : j + 1;
+      const double qs = th_star(s_q, s_r0, j, ns, dxtheta, invdxtheta, s_vtr[j] * dt, false);
+      const double qsp = th_star(s_q, s_r0, jp, ns, dxtheta, invdxtheta, s_vtr[jp] * dt, false);
+      double varq = dxrad * qs * s_rs1[j] * s_vtr[j];
+      varq -= dxrad * qsp * s_rs1[jp] * s_vtr[jp];
+      nv[c] = s_q[j] + varq * invsurf;
+    }
+    __syncthreads();
+    c = 0;
+    for (int j = threadIdx.x; j < ns; j += RB, c++) s_q[j] = nv[c];
+    __syncthreads();
+    if (fast) {
+      c = 0;
+      for (int j = threadIdx.x; j < ns; j += RB, c++) {
+        const int jp = (j == ns - 1) ? 0 : j + 1;
+        const double qs = th_star(s_q, s_r1, j, ns, dxtheta, invdxtheta, ksi2, false);
+        const double qsp = th_star(s_q, s_r1, jp, ns, dxtheta, invdxtheta, ksi2, false);
+        double varq = dxrad * qs * s_rs2[j] * vtemp;
+        varq -= dxrad * qsp * s_rs2[jp] * vtemp;
+        nv[c] = s_q[j] + varq * invsurf;
+      }
+      __syncthreads();
+      c = 0;
+      for (int j = threadIdx.x; j < ns; j += RB, c++) s_q[j] = nv[c];
+      __syncthreads();
+    }
+    for (int j = threadIdx.x; j < ns; j += RB) {
+      int ji = j - sh_n;
+      while (ji < 0) ji += ns;
+      while (ji >= ns) ji -= ns;
+      outs[which][base + j] = s_q[ji];
+    }
+  }
+}
+
+// ComputeVelocities (TransportEuler.c:350-381)
+__global__ void __launch_bounds__(TB) k_velocities(Geo g, const double *__restrict__ rho,
+                                                   const double *__restrict__ rp, const double *__restrict__ rm,
+                                                   const double *__restrict__ tp, const double *__restrict__ tm,
+                                                   double *__restrict__ vr, double *__restrict__ vt) {
+  const int ns = g.ns, i = blockIdx.y, j = blockIdx.x * TB + threadIdx.x;
+  if (j >= ns) return;
+  const size_t l = (size_t)i * ns + j;
+  const size_t ljm = (j == 0) ? (size_t)i * ns + ns - 1 : l - 1;
+  if (i == 0) vr[l] = 0.0;
+  else {
+    const size_t lim = l - ns;
+    vr[l] = (rp[lim] + rm[l]) / (rho[l] + rho[lim] + 1e-20);
+  }
+  vt[l] = (tp[ljm] + tm[l]) / (rho[l] + rho[ljm] + 1e-15) / g.Rmed[i] - g.Rmed[i] * g.omega_frame;
+}
+
+// ring-ordered sum of an arbitrary field (for the host: torque log, DampingTW04)
+__global__ void __launch_bounds__(RB) k_ring_mean(Geo g, const double *__restrict__ f, double *__restrict__ out) {
+  extern __shared__ double sh[];
+  const int i = blockIdx.x, ns = g.ns;
+  for (int j = threadIdx.x; j < ns; j += RB) sh[j] = f[(size_t)i * ns + j];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = ordered_ring_sum(sh, ns);
+    s /= (double)ns;
+    out[i] = s;
+  }
+}
+
+// GasTotalMass partial sums: per ring, sequential in j (SideEuler.c:35-39)
+__global__ void __launch_bounds__(RB) k_ring_mass(Geo g, const double *__restrict__ rho, double *__restrict__ out) {
+  extern __shared__ double sh[];
+  const int i = blockIdx.x, ns = g.ns;
+  for (int j = threadIdx.x; j < ns; j += RB) sh[j] = g.Surf[i] * rho[(size_t)i * ns + j];
+  __syncthreads();
+  if (threadIdx.x == 0) out[i] = ordered_ring_sum(sh, ns);
+}
+
+__global__ void k_pressure(Geo g, const double *__restrict__ e, double *__restrict__ p, size_t n) {
+  size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) p[k] = (g.adiabind - 1.0) * e[k];
+}
+
+// ===========================================================================
+// host side
+
+static int need_dev(thorin_ctx *c) {
+  if (!c || !c->dev) {
+    thorin_set_error("this context has no device (created with thorin_ctx_create_host or failed)");
+    return -1;
+  }
+  return 0;
+}
+
+static inline dim3 cell_grid(const thorin_dev *d, int rings) {
+  return dim3((unsigned)((d->ns + TB - 1) / TB), (unsigned)rings);
+}
+
+extern "C" int thorin_dev_create(thorin_ctx *c) {
+  thorin_dev *d = (thorin_dev *)calloc(1, sizeof(thorin_dev));
+  if (!d) { thorin_set_error("out of host memory"); return -1; }
+  c->dev = d;
+  d->nr = c->nr;
+  d->ns = c->ns;
+  d->nf = c->nfield;
+  if ((size_t)6 * c->ns * sizeof(double) > 150 * 1024) {
+    thorin_set_error("NSEC too large for the LDS-resident azimuthal transport (max 3200)");
+    return -1;
+  }
+  if ((c->ns + TB - 1) / TB * TB / 2 < c->ns / 2) return -1;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+    thorin_set_error("no HIP device available: the FARGO_THORIN GPU path needs an MI355X (gfx950)");
+    return -1;
+  }
+  HCHECK(hipStreamCreateWithFlags(&d->s, hipStreamNonBlocking));
+  const size_t fb = (d->nf + 64) * sizeof(double);
+  for (int k = 0; k < NBUF; k++) {
+    HCHECK(hipMalloc(&d->b[k], fb));
+    HCHECK(hipMemsetAsync(d->b[k], 0, fb, d->s));
+  }
+  // 1D arrays: 27 slots of (nr+3) doubles
+  const int n1 = c->nr + 3;
+  HCHECK(hipMalloc(&d->d1base, sizeof(double) * (size_t)n1 * 32));
+  HCHECK(hipMemsetAsync(d->d1base, 0, sizeof(double) * (size_t)n1 * 32, d->s));
+  double *p1 = d->d1base;
+  const double *src[21] = {c->Rinf, c->Rsup, c->Rmed, c->Surf, c->InvRmed, c->InvSurf, c->InvDiffRmed,
+                           c->InvDiffRsup, c->InvRinf, c->OmegaInv, c->SigmaMed, c->SigmaInf, c->EnergyMed,
+                           c->VthetaMed, c->WaveKiller, c->pow_rmed15, c->pow_rinf15, c->cav_med, c->cav_inf,
+                           c->cfl_pow2, c->supp_torque};
+  for (int k = 0; k < 21; k++) HCHECK(hipMemcpyAsync(p1 + (size_t)k * n1, src[k], sizeof(double) * n1, hipMemcpyHostToDevice, d->s));
+  Geo &g = d->g;
+  memset(&g, 0, sizeof(g));
+  g.nr = c->nr;
+  g.ns = c->ns;
+  g.Rinf = p1; g.Rsup = p1 + n1; g.Rmed = p1 + 2 * n1; g.Surf = p1 + 3 * n1; g.InvRmed = p1 + 4 * n1;
+  g.InvSurf = p1 + 5 * n1; g.InvDiffRmed = p1 + 6 * n1; g.InvDiffRsup = p1 + 7 * n1; g.InvRinf = p1 + 8 * n1;
+  g.OmegaInv = p1 + 9 * n1; g.SigmaMed = p1 + 10 * n1; g.SigmaInf = p1 + 11 * n1; g.EnergyMed = p1 + 12 * n1;
+  g.VthetaMed = p1 + 13 * n1; g.WaveKiller = p1 + 14 * n1; g.pow_rmed15 = p1 + 15 * n1;
+  g.pow_rinf15 = p1 + 16 * n1; g.cav_med = p1 + 17 * n1; g.cav_inf = p1 + 18 * n1; g.cfl_pow2 = p1 + 19 * n1;
+  g.supp_torque = p1 + 20 * n1;
+  g.gcs = p1 + 21 * n1; g.gtemp = p1 + 22 * n1; g.Vmoy = p1 + 23 * n1; g.VMed = p1 + 24 * n1;
+  g.ringmin = p1 + 25 * n1;
+  d->ringtmp = p1 + 26 * n1;
+  int *ip;
+  HCHECK(hipMalloc(&ip, sizeof(int) * (size_t)n1 * 2));
+  g.Nshift = ip;
+  g.dampring = ip + n1;
+  {
+    std::vector<int> dr(n1, 0);
+    const thorin_params *p = &c->p;
+    if (p->damping_on) {
+      double DRin = p->rmin * p->dampingrminfrac, DRout = p->rmax * p->dampingrmaxfrac;
+      for (int i = 0; i < c->nr; i++) dr[i] = (c->Rmed[i] < DRin || c->Rmed[i] > DRout) ? 1 : 0;
+    }
+    HCHECK(hipMemcpy(ip + n1, dr.data(), sizeof(int) * n1, hipMemcpyHostToDevice));
+  }
+  const thorin_params *p = &c->p;
+  g.adiabind = p->adiabind;
+  g.sqrt_adiabind_inv = c->k.sqrt_adiabind_inv;
+  g.cv = c->k.cv;
+  g.viscosity = p->viscosity;
+  g.alphavisc = p->alphaviscosity;
+  g.alphamri = p->alphamri;
+  g.alphadead = p->alphadead;
+  g.tmri = p->tmri;
+  g.twidth = p->twidth;
+  g.t2si = c->k.t2si;
+  g.opa2cu = c->k.opa2cu;
+  g.rho2cgs = c->k.rho2cgs;
+  g.stefan = c->k.stefan;
+  g.sqrt2pi_inv = c->k.sqrt2pi_inv;
+  g.sqrt3 = c->k.sqrt3;
+  g.opacitydrop = p->opacitydrop;
+  g.parametricopacity = p->parametricopacity;
+  g.powerlawopacity = p->powerlawopacity;
+  g.gasflux = p->gasflux;
+  g.omega_frame = c->st.omega_frame;
+  g.lr01 = c->k.lr01;
+  g.lr12 = c->k.lr12;
+  g.thicksmooth = p->thicknesssmoothing;
+  g.visc_alpha = p->viscosity_alpha;
+  g.alpha_flock = p->alpha_flock_on;
+  g.fast_transport = p->fast_transport;
+  g.damp_init = p->damp_init;
+  g.damp_flux = p->damp_flux;
+  g.opacity_law = p->opacity_law;
+  g.stellar_irr = p->stellar_irradiation;
+  g.indirect_term = p->indirect_term;
+  g.exclude_hill = p->exclude_hill;
+  g.hillcut = p->hillcut;
+  // LP85 thresholds exactly as the reference's expressions (Opacity.c:205-220)
+  g.lp_t1 = pow(2.0e-4 / 2.0e16, -1.0 / 9.0);
+  g.lp_t2 = pow(2.0e16 / 5.0e-3, 1.0 / 8.0);
+  g.lp_t3a = pow(5.0e-3 / 2.0e34, -1.0 / 10.0);
+  g.lp_t4 = pow(2.0e34 / 2.0e-8, 1.0 / 12.0);
+  g.lp_t5a = pow(2.0e-8 / 1.0e-36, 1.0 / 7.0);
+  g.lp_t6a = pow(1.0e-36 / 1.5e20, -1.0 / 12.5);
+  HCHECK(hipMalloc(&d->ctl, sizeof(DevCtl)));
+  HCHECK(hipMemsetAsync(d->ctl, 0, sizeof(DevCtl), d->s));
+  HCHECK(hipHostMalloc(&d->hctl, sizeof(DevCtl), hipHostMallocDefault));
+  memset(d->hctl, 0, sizeof(DevCtl));
+  HCHECK(hipMalloc(&d->pk, sizeof(PlanetK) * MAXPL));
+  HCHECK(hipHostMalloc(&d->hpk, sizeof(PlanetK) * MAXPL, hipHostMallocDefault));
+  d->nblk_cells = ((c->ns + TB - 1) / TB) * c->nr;
+  HCHECK(hipMalloc(&d->red, sizeof(double) * (size_t)d->nblk_cells * (2 + 2 * 8)));
+  const int half = c->ns / 2;
+  d->nblk_sor = ((half + TB - 1) / TB) * (c->nr - 2);
+  HCHECK(hipMalloc(&d->sorp, sizeof(double) * (size_t)d->nblk_cells * 2));
+  HCHECK(hipMalloc(&d->sorc, sizeof(int) * (size_t)d->nblk_cells * 2));
+  // geometry fields
+  HCHECK(hipMemcpyAsync(d->b[B_ABS], c->h_abs, sizeof(double) * d->nf, hipMemcpyHostToDevice, d->s));
+  HCHECK(hipMemcpyAsync(d->b[B_ORD], c->h_ord, sizeof(double) * d->nf, hipMemcpyHostToDevice, d->s));
+  HCHECK(hipStreamSynchronize(d->s));
+  return 0;
+}
+
+extern "C" void thorin_dev_destroy(thorin_ctx *c) {
+  thorin_dev *d = c->dev;
+  if (!d) return;
+  if (d->s) (void)hipStreamSynchronize(d->s);
+  for (int k = 0; k < NBUF; k++) if (d->b[k]) (void)hipFree(d->b[k]);
+  if (d->d1base) (void)hipFree(d->d1base);
+  if (d->g.Nshift) (void)hipFree(d->g.Nshift);
+  if (d->ctl) (void)hipFree(d->ctl);
+  if (d->hctl) (void)hipHostFree(d->hctl);
+  if (d->pk) (void)hipFree(d->pk);
+  if (d->hpk) (void)hipHostFree(d->hpk);
+  if (d->red) (void)hipFree(d->red);
+  if (d->sorp) (void)hipFree(d->sorp);
+  if (d->sorc) (void)hipFree(d->sorc);
+  if (d->s) (void)hipStreamDestroy(d->s);
+  free(d);
+  c->dev = NULL;
+}
+
+static int upload_1d_profiles(thorin_ctx *c) {
+  thorin_dev *d = c->dev;
+  const int n1 = c->nr + 3;
+  double *p1 = d->d1base;
+  HCHECK(hipMemcpyAsync(p1 + 10 * n1, c->SigmaMed, sizeof(double) * n1, hipMemcpyHostToDevice, d->s));
+  HCHECK(hipMemcpyAsync(p1 + 11 * n1, c->SigmaInf, sizeof(double) * n1, hipMemcpyHostToDevice, d->s));
+  HCHECK(hipMemcpyAsync(p1 + 12 * n1, c->EnergyMed, sizeof(double) * n1, hipMemcpyHostToDevice, d->s));
+  HCHECK(hipMemcpyAsync(p1 + 13 * n1, c->VthetaMed, sizeof(double) * n1, hipMemcpyHostToDevice, d->s));
+  HCHECK(hipMemcpyAsync(d->g.gcs, c->globcsvec, sizeof(double) * n1, hipMemcpyHostToDevice, d->s));
+  HCHECK(hipMemcpyAsync(d->g.gtemp, c->globtempervec, sizeof(double) * n1, hipMemcpyHostToDevice, d->s));
+  return 0;
+}
+
+extern "C" int thorin_dev_upload_init(thorin_ctx *c) {
+  thorin_dev *d = c->dev;
+  if (upload_1d_profiles(c)) return -1;
+  const size_t nb = sizeof(double) * d->nf;
+  HCHECK(hipMemcpyAsync(d->b[B_DENS], c->h_dens, nb, hipMemcpyHostToDevice, d->s));
+  HCHECK(hipMemcpyAsync(d->b[B_VRAD], c->h_vrad, nb, hipMemcpyHostToDevice, d->s));
+  HCHECK(hipMemcpyAsync(d->b[B_VTHETA], c->h_vtheta, nb, hipMemcpyHostToDevice, d->s));
+  HCHECK(hipMemcpyAsync(d->b[B_ENERGY], c->h_energy, nb, hipMemcpyHostToDevice, d->s));
+  HCHECK(hipMemcpyAsync(d->b[B_LABEL], c->h_label, nb, hipMemcpyHostToDevice, d->s));
+  HCHECK(hipMemcpyAsync(d->b[B_CS], c->h_cs, nb, hipMemcpyHostToDevice, d->s));
+  HCHECK(hipMemcpyAsync(d->b[B_TEMPER], c->h_temper, nb, hipMemcpyHostToDevice, d->s));
+  HCHECK(hipMemcpyAsync(d->b[B_PRESSURE], c->h_press, nb, hipMemcpyHostToDevice, d->s));
+  HCHECK(hipStreamSynchronize(d->s));
+  return 0;
+}
+
+extern "C" int thorin_upload_field(thorin_ctx *c, int field, const double *host, size_t count) {
+  if (!c || !c->dev || field < 0 || field >= THORIN_NFIELDS || !host) { thorin_set_error("bad upload args"); return -1; }
+  thorin_dev *d = c->dev;
+  if (count > d->nf) count = d->nf;
+  HCHECK(hipMemcpyAsync(d->b[field], host, sizeof(double) * count, hipMemcpyHostToDevice, d->s));
+  HCHECK(hipStreamSynchronize(d->s));
+  return 0;
+}
+
+extern "C" int thorin_download_field(thorin_ctx *c, int field, double *host, size_t count) {
+  if (!c || !c->dev || field < 0 || field >= THORIN_NFIELDS || !host) { thorin_set_error("bad download args"); return -1; }
+  thorin_dev *d = c->dev;
+  if (count > d->nf) count = d->nf;
+  if (field == THORIN_F_PRESSURE) {
+    const size_t n = d->nf;
+    hipLaunchKernelGGL(k_pressure, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, d->s, d->g, d->b[B_ENERGY],
+                       d->b[B_PRESSURE], n);
+    HCHECK(hipGetLastError());
+  }
+  HCHECK(hipMemcpyAsync(host, d->b[field], sizeof(double) * count, hipMemcpyDeviceToHost, d->s));
+  HCHECK(hipStreamSynchronize(d->s));
+  return 0;
+}
+
+extern "C" int thorin_sync(thorin_ctx *c) {
+  if (need_dev(c)) return -1;
+  HCHECK(hipStreamSynchronize(c->dev->s));
+  return 0;
+}
+
+extern "C" int thorin_ring_means(thorin_ctx *c, int field, double *out) {
+  if (need_dev(c)) return -1;
+  thorin_dev *d = c->dev;
+  if (field < 0 || field >= THORIN_NFIELDS) return -1;
+  hipLaunchKernelGGL(k_ring_mean, dim3(d->nr), dim3(RB), sizeof(double) * d->ns, d->s, d->g, d->b[field], d->ringtmp);
+  HCHECK(hipGetLastError());
+  HCHECK(hipMemcpyAsync(out, d->ringtmp, sizeof(double) * d->nr, hipMemcpyDeviceToHost, d->s));
+  HCHECK(hipStreamSynchronize(d->s));
+  return 0;
+}
+
+extern "C" int thorin_gas_total_mass(thorin_ctx *c, double *out) {
+  if (need_dev(c)) return -1;
+  thorin_dev *d = c->dev;
+  std::vector<double> rings(d->nr);
+  hipLaunchKernelGGL(k_ring_mass, dim3(d->nr), dim3(RB), sizeof(double) * d->ns, d->s, d->g, d->b[B_DENS], d->ringtmp);
+  HCHECK(hipGetLastError());
+  HCHECK(hipMemcpyAsync(rings.data(), d->ringtmp, sizeof(double) * d->nr, hipMemcpyDeviceToHost, d->s));
+  HCHECK(hipStreamSynchronize(d->s));
+  double t = 0.0;   // the reference keeps one running sum over all cells; rings are summed in order here
+  for (int i = 0; i < d->nr; i++) t += rings[i];
+  *out = t;
+  return 0;
+}
+
+// --------------------------------------------------------------------------
+// SOR driver
+
+static int sor_solve(thorin_ctx *c, double omega, int *niter_out) {
+  thorin_dev *d = c->dev;
+  const Geo &g = d->g;
+  const int nr = d->nr, ns = d->ns;
+  dim3 gc = cell_grid(d, nr - 2);
+  const int nb0 = gc.x * gc.y;
+  hipLaunchKernelGGL(k_sor_resid0, gc, dim3(TB), 0, d->s, g, d->b[B_TEMPER], d->b[B_ML], d->b[B_MLIP],
+                     d->b[B_MLIM], d->b[B_MLJP], d->b[B_MLJM], d->b[B_RHS], d->sorp);
+  hipLaunchKernelGGL(k_sum_to, dim3(1), dim3(RB), 0, d->s, d->sorp, nb0, &d->ctl->sor_normres0);
+  HCHECK(hipMemsetAsync(&d->ctl->sor_done, 0, sizeof(int) * 3 + sizeof(unsigned int), d->s));
+  dim3 gs((unsigned)(((ns / 2) + TB - 1) / TB), (unsigned)(nr - 2));
+  int launched = 0;
+  int batch = c->st.sor_niter_last > 0 ? c->st.sor_niter_last + 1 : 8;
+  if (batch > 256) batch = 256;
+  for (;;) {
+    for (int b = 0; b < batch; b++) {
+      hipLaunchKernelGGL(k_sor_half, gs, dim3(TB), 0, d->s, g, d->b[B_TEMPER], d->b[B_ML], d->b[B_MLIP],
+                         d->b[B_MLIM], d->b[B_MLJP], d->b[B_MLJM], d->b[B_RHS], omega, 0, d->sorp, d->sorc, d->ctl,
+                         SORMAXITERS);
+      hipLaunchKernelGGL(k_sor_half, gs, dim3(TB), 0, d->s, g, d->b[B_TEMPER], d->b[B_ML], d->b[B_MLIP],
+                         d->b[B_MLIM], d->b[B_MLJP], d->b[B_MLJM], d->b[B_RHS], omega, 1, d->sorp, d->sorc, d->ctl,
+                         SORMAXITERS);
+    }
+    HCHECK(hipGetLastError());
+    launched += batch;
+    HCHECK(hipMemcpyAsync(d->hctl, d->ctl, sizeof(DevCtl), hipMemcpyDeviceToHost, d->s));
+    HCHECK(hipStreamSynchronize(d->s));
+    if (d->hctl->sor_done) { *niter_out = d->hctl->sor_niter; return 0; }
+    if (launched > SORMAXITERS + 2) { *niter_out = SORMAXITERS + 1; return 0; }
+    batch = batch < 64 ? 2 * batch : 128;
+  }
+}
+
+// IterateRelaxationParameter (EnergySources.c:340-403)
+static int iterate_relaxation(thorin_ctx *c) {
+  thorin_dev *d = c->dev;
+  double omegamin = 1.0, omega = 1.0, omegamax = OMEGAUPPER;
+  int nsplit = 9, Niter, count = 0;
+  const size_t nb = sizeof(double) * d->nf;
+  HCHECK(hipMemcpyAsync(d->b[B_TBACK], d->b[B_TEMPER], nb, hipMemcpyDeviceToDevice, d->s));
+  c->st.sor_domega = (omegamax - omegamin) / nsplit;
+  c->st.sor_niter_best = SORMAXITERS;
+  c->st.sor_omega_best = 1.0;
+  int saved_last = c->st.sor_niter_last;
+  for (;;) {
+    for (int n = 0; n <= nsplit; n++) {
+      c->st.sor_niter_last = 0;
+      if (sor_solve(c, omega, &Niter)) return -1;
+      if (Niter == c->st.sor_niter_best) count++;
+      if (Niter < c->st.sor_niter_best) { c->st.sor_niter_best = Niter; c->st.sor_omega_best = omega; }
+      omega += c->st.sor_domega;
+      HCHECK(hipMemcpyAsync(d->b[B_TEMPER], d->b[B_TBACK], nb, hipMemcpyDeviceToDevice, d->s));
+    }
+    if (c->st.sor_niter_best == SORMAXITERS) {
+      c->st.sor_niter_last = saved_last;
+      thorin_set_error("SOR did not converge when estimating the initial over-relaxation parameter");
+      return -1;
+    }
+    if (count == nsplit + 1) {
+      c->st.sor_domega = omegamax - omegamin;
+      return 0;
+    }
+    count = 0;
+    omegamin = c->st.sor_omega_best - c->st.sor_domega;
+    omegamax = c->st.sor_omega_best + c->st.sor_domega;
+    if (omegamin < 1.0) omegamin = 1.0;
+    if (omegamax > OMEGAUPPER) omegamax = OMEGAUPPER;
+    c->st.sor_domega = (omegamax - omegamin) / nsplit;
+    omega = omegamin;
+  }
+}
+
+// MinStepForRebound (ReboundInterface.c:360-385)
+static double minstep_rebound(const thorin_planets *sys) {
+  double dt = 1.e30;
+  for (int i = 0; i < sys->nb; i++)
+    for (int j = i + 1; j < sys->nb; j++) {
+      double r2 = (sys->x[i] - sys->x[j]) * (sys->x[i] - sys->x[j]) + (sys->y[i] - sys->y[j]) * (sys->y[i] - sys->y[j]) +
+                  (sys->z[i] - sys->z[j]) * (sys->z[i] - sys->z[j]);
+      double rv = (sys->x[i] - sys->x[j]) * (sys->vx[i] - sys->vx[j]) + (sys->y[i] - sys->y[j]) * (sys->vy[i] - sys->vy[j]) +
+                  (sys->z[i] - sys->z[j]) * (sys->vz[i] - sys->vz[j]);
+      double dtnew = fabs(r2 / rv);
+      if (dtnew < dt) dt = dtnew;
+    }
+  dt *= 0.25;
+  dt *= dt;
+  return 1.0 / dt;
+}
+
+static inline void swapb(thorin_dev *d, int a, int b) {
+  double *t = d->b[a];
+  d->b[a] = d->b[b];
+  d->b[b] = t;
+}
+
+#define LAUNCH(kern, grid, block, shmem, ...)                                    \
+  do {                                                                           \
+    hipLaunchKernelGGL(kern, grid, block, shmem, d->s, __VA_ARGS__);             \
+  } while (0)
+
+// AlgoGas (SourceEuler.c:210-317): one DT
+extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_fn nbody, void *nbody_user,
+                              thorin_step_log *log) {
+  if (need_dev(c)) return -1;
+  if (!sys) { thorin_set_error("thorin_algogas: null planetary system"); return -1; }
+  thorin_dev *d = c->dev;
+  const thorin_params *p = &c->p;
+  const int nr = d->nr, ns = d->ns;
+  const double DT = p->dt;
+  double dtemp = 0.0, dt = DT;
+  int rc = 0;
+  if (sys->nb > 8) { thorin_set_error("this build supports at most 8 planets in the force kernel"); return -1; }
+  if (log) memset(log, 0, sizeof(*log));
+  d->g.omega_frame = c->st.omega_frame;
+  const dim3 tb(TB), rb(RB);
+  const dim3 gall = cell_grid(d, nr);
+  while (dtemp < 0.999999999 * DT) {
+    double mt = c->st.physical_time / (p->masstaper * 2.0 * M_PI);
+    c->st.mass_taper = (mt > 1.0 ? 1.0 : pow(sin(mt * M_PI / 2.0), 2.0));
+    mt = c->st.mass_taper;
+    if (p->lambdadoubling != 0.0) {
+      thorin_cavity_factors(c);
+      const int n1 = nr + 3;
+      HCHECK(hipMemcpyAsync((double *)d->g.cav_med, c->cav_med, sizeof(double) * n1, hipMemcpyHostToDevice, d->s));
+      HCHECK(hipMemcpyAsync((double *)d->g.cav_inf, c->cav_inf, sizeof(double) * n1, hipMemcpyHostToDevice, d->s));
+    }
+    const double invdtreb_sq = minstep_rebound(sys);
+    // ---- ConditionCFL
+    LAUNCH(k_cfl_ring, dim3(nr), rb, sizeof(double) * (ns + RB), d->g, d->b[B_VRAD], d->b[B_VTHETA], d->b[B_CS],
+           invdtreb_sq);
+    LAUNCH(k_cfl_final, dim3(1), rb, 0, d->g, d->ctl);
+    HCHECK(hipGetLastError());
+    HCHECK(hipMemcpyAsync(d->hctl, d->ctl, sizeof(DevCtl), hipMemcpyDeviceToHost, d->s));
+    HCHECK(hipStreamSynchronize(d->s));
+    double newdt = d->hctl->newdt_raw, deltaT = DT - dtemp;
+    if (newdt > c->st.dt_stellar) {
+      double a = 1.0 * c->st.dt_stellar, b = p->minimumstep;
+      double mx = a > b ? a : b;
+      newdt = newdt < mx ? newdt : mx;
+    }
+    newdt = newdt < p->maximumstep ? newdt : p->maximumstep;
+    if (c->st.cfl_count == 0) {
+      if (p->firststep > 0.0) newdt = p->firststep;
+      c->st.cfl_count = 1;
+    }
+    if (deltaT < newdt) newdt = deltaT;
+    const int N = (int)(ceil(deltaT / newdt));
+    dt = (DT - dtemp) / (double)N;
+    dtemp += dt;
+    // ---- FillForcesArrays
+    double indir_px = 0.0, indir_py = 0.0;
+    {
+      double axs = 0.0, ays = 0.0;
+      for (int k = 0; k < sys->nb; k++) {
+        PlanetK &P = d->hpk[k];
+        const double xpl = sys->x[k], ypl = sys->y[k], zpl = sys->z[k];
+        double mpl = sys->mass[k];
+        const double dpl = sqrt(xpl * xpl + ypl * ypl + zpl * zpl);
+        P.x = xpl; P.y = ypl; P.z = zpl; P.m = mpl;
+        P.rhill = dpl * pow((1.0 / 3.0 * mpl), 1.0 / 3.0);
+        const double rst = 0.5 * P.rhill;
+        P.rst2 = rst * rst;
+        P.rst3 = P.rst2 * rst;
+        P.rst4 = P.rst3 * rst;
+        const double rsm = 0.05 * P.rhill;
+        P.rsm2 = rsm * rsm;
+        P.rmorevert2 = 2.0 * P.rhill;
+        P.rmorevert2 *= P.rmorevert2;
+        // ring of ThicknessSmoothing (Force.c:85-90)
+        const double r = sqrt(xpl * xpl + ypl * ypl);
+        P.iring = -1;
+        if (r >= c->Rinf[0] && r <= c->Rsup[nr - 1]) {
+          int i = 0;
+          while (c->Rsup[i] < r) i++;
+          P.iring = i;
+        }
+        if (p->indirect_term) {
+          const double invdpl3 = 1.0 / (dpl * dpl * dpl);
+          mpl *= mt;
+          axs += mpl * invdpl3 * xpl;
+          ays += mpl * invdpl3 * ypl;
+        }
+      }
+      indir_px = -axs;
+      indir_py = -ays;
+      if (sys->nb > 0)
+        HCHECK(hipMemcpyAsync(d->pk, d->hpk, sizeof(PlanetK) * sys->nb, hipMemcpyHostToDevice, d->s));
+    }
+    LAUNCH(k_forces, gall, tb, 0, d->g, d->b[B_DENS], d->b[B_CS], d->b[B_ABS], d->b[B_ORD], d->b[B_AGR],
+           d->b[B_AGT], d->pk, sys->nb, mt, d->red);
+    LAUNCH(k_forces_reduce, dim3(2 + 2 * sys->nb), rb, 0, d->red, d->nblk_cells, d->ctl);
+    LAUNCH(k_indirect, gall, tb, 0, d->g, d->b[B_AGR], d->b[B_AGT], d->b[B_ABS], d->b[B_ORD], d->ctl, indir_px,
+           indir_py);
+    HCHECK(hipGetLastError());
+    // ---- AdvanceSystemFromDisk (host; needs the reductions only when a planet feels the disk)
+    int anyfeel = 0;
+    for (int k = 0; k < sys->nb; k++) anyfeel |= sys->feeldisk[k];
+    const int want_tq = p->write_torque && log && log->nsub == 0 && sys->nb > 0;
+    if (anyfeel || want_tq) {
+      HCHECK(hipMemcpyAsync(d->hctl, d->ctl, sizeof(DevCtl), hipMemcpyDeviceToHost, d->s));
+      HCHECK(hipStreamSynchronize(d->s));
+      for (int k = 0; k < sys->nb; k++) { sys->ax[k] = d->hctl->axpl[k]; sys->ay[k] = d->hctl->aypl[k]; }
+    }
+    if (want_tq) {
+      // UpdateLog (Force.c:27-70) at the first substep of the DT
+      std::vector<double> rho1d(nr), cs1d(nr);
+      if (thorin_ring_means(c, THORIN_F_DENS, rho1d.data())) return -1;
+      HCHECK(hipMemcpy(cs1d.data(), d->g.gcs, sizeof(double) * nr, hipMemcpyDeviceToHost));
+      log->tq_rows = sys->nb < 8 ? sys->nb : 8;
+      for (int k = 0; k < log->tq_rows; k++) {
+        const double x = sys->x[k], y = sys->y[k], ax = sys->ax[k], ay = sys->ay[k], m = sys->mass[k];
+        const double r2 = x * x + y * y, r = sqrt(r2);
+        double ifrac;   // GetGlobalIFrac (LowTasks.c:21-31)
+        if (r < c->GlobalRmed[0]) ifrac = 0.0;
+        else if (r > c->GlobalRmed[nr - 1]) ifrac = (double)nr - 1.0;
+        else {
+          int i = 0;
+          while (c->GlobalRmed[i] <= r) i++;
+          ifrac = (double)i + (r - c->GlobalRmed[i - 1]) / (c->GlobalRmed[i] - c->GlobalRmed[i - 1]) - 1.0;
+        }
+        const double frac = ifrac - floor(ifrac);
+        const int ii = (int)ifrac;
+        const double rhopl = rho1d[ii] * (1.0 - frac) + (ii + 1 < nr ? rho1d[ii + 1] : 0.0) * frac;
+        const double cspl = cs1d[ii] * (1.0 - frac) + (ii + 1 < nr ? cs1d[ii + 1] : 0.0) * frac;
+        const double omega = pow(r, -1.5);
+        const double h = cspl / (omega * r) * c->k.sqrt_adiabind_inv;
+        double norm = m * omega * r2 / h;
+        norm *= norm;
+        norm *= rhopl;
+        const double tq = x * ay - y * ax;
+        const double row[7] = {c->st.physical_time, tq, tq * m * p->adiabind / norm, x, y, ax, ay};
+        for (int q = 0; q < 7; q++) log->tq[k][q] = row[q];
+      }
+    }
+    if (anyfeel) {
+      std::vector<double> rmean(nr), csmean(nr);
+      if (thorin_ring_means(c, THORIN_F_DENS, rmean.data())) return -1;
+      HCHECK(hipMemcpy(csmean.data(), d->g.gcs, sizeof(double) * nr, hipMemcpyDeviceToHost));
+      const double indx = -d->hctl->axstar, indy = -d->hctl->aystar;
+      for (int k = 0; k < sys->nb; k++) {
+        if (!sys->feeldisk[k]) continue;
+        // DampingTW04 (Pframeforce.c:571-603)
+        double damp = 0.0, x = sys->x[k], y = sys->y[k], z = sys->z[k], m = sys->mass[k], vz = sys->vz[k];
+        double r = sqrt(x * x + y * y);
+        if ((p->verticaldamping > 0.0) && (r >= c->Rinf[0] && r <= c->Rsup[nr - 1])) {
+          int i = 0;
+          while (c->Rsup[i] < r) i++;
+          double cs4inv = pow(csmean[i], -4.0), Om = pow(r, -1.5);
+          damp = p->verticaldamping * m * rmean[i] * Om * cs4inv * (-2.176 * vz - 0.871 * z * Om);
+        }
+        sys->vx[k] += dt * sys->ax[k];
+        sys->vy[k] += dt * sys->ay[k];
+        sys->vz[k] += dt * (sys->az[k] + damp);
+        sys->vx[k] += dt * indx;
+        sys->vy[k] += dt * indy;
+      }
+    }
+    // ---- host N-body (AdvanceSystemRebound, RotatePsys, SynchronizeFargoRebound)
+    if (nbody) {
+      int e = nbody(nbody_user, sys, c->st.physical_time, dt, c->st.omega_frame * dt);
+      if (e) { thorin_set_error("N-body step failed"); return -1; }
+    }
+    // ---- ApplyBoundaryCondition, DetectCrash, ComputePressureField, SubStep1
+    HCHECK(hipMemsetAsync(&d->ctl->crash_e, 0, 2 * sizeof(int), d->s));
+    if (p->damping_on)
+      LAUNCH(k_boundary, gall, tb, 0, d->g, d->b[B_VRAD], d->b[B_VTHETA], d->b[B_DENS], d->b[B_ENERGY], dt);
+    LAUNCH(k_substep1, gall, tb, 0, d->g, d->b[B_DENS], d->b[B_ENERGY], d->b[B_VRAD], d->b[B_VTHETA], d->b[B_AGR],
+           d->b[B_AGT], d->b[B_VRADINT], d->b[B_VTHETAINT], dt, d->ctl);
+    LAUNCH(k_divstress, gall, tb, 0, d->g, d->b[B_VRADINT], d->b[B_VTHETAINT], d->b[B_DENS], d->b[B_DIVV],
+           d->b[B_TAURR], d->b[B_TAURP], d->b[B_TAUPP]);
+    {
+      const int kepler = (!p->damping_on) || (p->damping_on && p->damp_vrad);
+      double vkin = 0.0, vkout = 0.0;
+      if (kepler) {
+        // ImposeKeplerianEdges (Viscosity.c:236-243), host libm
+        vkin = sqrt(1.0 * 1.0 / c->Rmed[0]) *
+               sqrt(1.0 - pow(thorin_aspectratio_host(c, c->Rmed[0]), 2.0) * pow(c->Rmed[0], 2.0 * p->flaringindex) *
+                              (1. + p->sigmaslope - 2.0 * p->flaringindex));
+        vkout = sqrt(1.0 * 1.0 / c->Rmed[nr - 1]) *
+                sqrt(1.0 - pow(thorin_aspectratio_host(c, c->Rmed[nr - 1]), 2.0) *
+                               pow(c->Rmed[nr - 1], 2.0 * p->flaringindex) * (1. + p->sigmaslope - 2.0 * p->flaringindex));
+      }
+      LAUNCH(k_visc_update, gall, tb, 0, d->g, d->b[B_VRADINT], d->b[B_VTHETAINT], d->b[B_DENS], d->b[B_TAURR],
+             d->b[B_TAURP], d->b[B_TAUPP], dt, kepler, vkin, vkout);
+    }
+    // ---- SubStep2, ActualiseGas, ApplyBoundaryCondition
+    LAUNCH(k_substep2, gall, tb, 0, d->g, d->b[B_DENS], d->b[B_VRADINT], d->b[B_VTHETAINT], d->b[B_ENERGY],
+           d->b[B_VRADNEW], d->b[B_VTHETANEW], d->b[B_EINT], dt);
+    swapb(d, B_VRAD, B_VRADNEW);
+    swapb(d, B_VTHETA, B_VTHETANEW);
+    if (p->damping_on)
+      LAUNCH(k_boundary, gall, tb, 0, d->g, d->b[B_VRAD], d->b[B_VTHETA], d->b[B_DENS], d->b[B_ENERGY], dt);
+    HCHECK(hipGetLastError());
+    // ---- SubStep3: Q+, implicit radiative diffusion
+    int niter = 0;
+    double omega_used = 0.0;
+    LAUNCH(k_divstress, gall, tb, 0, d->g, d->b[B_VRAD], d->b[B_VTHETA], d->b[B_DENS], d->b[B_DIVV],
+           d->b[B_TAURR], d->b[B_TAURP], d->b[B_TAUPP]);
+    LAUNCH(k_qplus, gall, tb, 0, d->g, d->b[B_DENS], d->b[B_TAURR], d->b[B_TAURP], d->b[B_TAUPP], d->b[B_DIVV],
+           d->b[B_QPLUS]);
+    LAUNCH(k_temper_cs, dim3(nr), rb, sizeof(double) * 2 * ns, d->g, d->b[B_DENS], d->b[B_EINT], d->b[B_TEMPER],
+           d->b[B_CS]);
+    LAUNCH(k_opacity_qminus, gall, tb, 0, d->g, d->b[B_DENS], d->b[B_CS], d->b[B_TEMPER], d->b[B_OPACITY],
+           d->b[B_QMINUS], d->opa_filled ? 0 : 1);
+    d->opa_filled = 1;
+    {
+      unsigned long long big = (unsigned long long)__builtin_bit_cast(unsigned long long, 1.0e38);
+      HCHECK(hipMemcpyAsync(&d->ctl->dtst_bits, &big, sizeof(big), hipMemcpyHostToDevice, d->s));
+    }
+    LAUNCH(k_diffcoef, gall, tb, 0, d->g, d->b[B_DENS], d->b[B_CS], d->b[B_TEMPER], d->b[B_OPACITY], d->b[B_DIFF],
+           d->ctl);
+    LAUNCH(k_matrix, cell_grid(d, nr - 2), tb, 0, d->g, d->b[B_DENS], d->b[B_CS], d->b[B_DIFF], d->b[B_DIVV],
+           d->b[B_QMINUS], d->b[B_QPLUS], d->b[B_TEMPER], d->b[B_ML], d->b[B_MLIP], d->b[B_MLIM], d->b[B_MLJP],
+           d->b[B_MLJM], d->b[B_RHS], dt);
+    HCHECK(hipGetLastError());
+    {
+      unsigned long long bits;
+      HCHECK(hipMemcpyAsync(&bits, &d->ctl->dtst_bits, sizeof(bits), hipMemcpyDeviceToHost, d->s));
+      HCHECK(hipStreamSynchronize(d->s));
+      c->st.dt_stellar = __builtin_bit_cast(double, bits);
+    }
+    if (c->st.sor_first) {
+      c->st.sor_first = 0;
+      if (iterate_relaxation(c)) return -1;
+      c->st.sor_omega = c->st.sor_omega_best;
+      c->st.sor_niter_last = c->st.sor_niter_best;
+    }
+    c->st.sor_omega += c->st.sor_domega;
+    if (c->st.sor_omega > OMEGAUPPER) { c->st.sor_omega = OMEGAUPPER; c->st.sor_domega = -c->st.sor_domega; }
+    if (c->st.sor_omega < 1.0) { c->st.sor_omega = 1.0; c->st.sor_domega = -c->st.sor_domega; }
+    omega_used = c->st.sor_omega;
+    if (sor_solve(c, c->st.sor_omega, &niter)) return -1;
+    if (niter > SORMAXITERS) rc = 2;
+    if (niter > c->st.sor_niter_last) c->st.sor_domega = -c->st.sor_domega;
+    c->st.sor_niter_last = niter;
+    LAUNCH(k_energy_new, gall, tb, 0, d->g, d->b[B_DENS], d->b[B_TEMPER], d->b[B_ENEW]);
+    swapb(d, B_ENERGY, B_ENEW);
+    // ---- Transport
+    {
+      RadQ q;
+      q.rho = d->b[B_DENS];
+      q.vr = d->b[B_VRAD];
+      q.vt = d->b[B_VTHETA];
+      q.e = d->b[B_ENERGY];
+      q.of = c->st.omega_frame;
+      LAUNCH(k_transport_rad, gall, tb, 0, d->g, q, dt, d->b[B_XRP], d->b[B_XRM], d->b[B_XTP], d->b[B_XTM],
+             d->b[B_XE], d->b[B_XRHO]);
+      LAUNCH(k_transport_theta, dim3(nr), rb, sizeof(double) * 6 * ns, d->g, d->b[B_VTHETA], d->b[B_XRP],
+             d->b[B_XRM], d->b[B_XTP], d->b[B_XTM], d->b[B_XE], d->b[B_XRHO], d->b[B_RP], d->b[B_RM], d->b[B_TP],
+             d->b[B_TM], d->b[B_ENERGY], d->b[B_DENS], dt);
+      LAUNCH(k_velocities, gall, tb, 0, d->g, d->b[B_DENS], d->b[B_RP], d->b[B_RM], d->b[B_TP], d->b[B_TM],
+             d->b[B_VRAD], d->b[B_VTHETA]);
+    }
+    if (p->damping_on)
+      LAUNCH(k_boundary, gall, tb, 0, d->g, d->b[B_VRAD], d->b[B_VTHETA], d->b[B_DENS], d->b[B_ENERGY], dt);
+    LAUNCH(k_temper_cs, dim3(nr), rb, sizeof(double) * 2 * ns, d->g, d->b[B_DENS], d->b[B_ENERGY], d->b[B_TEMPER],
+           d->b[B_CS]);
+    HCHECK(hipGetLastError());
+    c->st.physical_time += dt;
+    if (log && log->nsub < THORIN_MAXSUB) {
+      log->cfl_n[log->nsub] = N;
+      log->dt[log->nsub] = dt;
+      log->sor_niter[log->nsub] = niter;
+      log->sor_omega[log->nsub] = omega_used;
+      log->nsub++;
+    }
+    // crash flags of this substep (read back with the next CFL copy; fetched now for the log)
+    HCHECK(hipMemcpyAsync(d->hctl, d->ctl, sizeof(DevCtl), hipMemcpyDeviceToHost, d->s));
+    HCHECK(hipStreamSynchronize(d->s));
+    if (d->hctl->crash_e) {
+      c->st.already_crashed++;
+      if (log) { log->crashed[log->nsub - 1] = 1; log->crashed_any = 1; }
+      if (!rc) rc = 1;
+    }
+  }
+  return rc;
+}

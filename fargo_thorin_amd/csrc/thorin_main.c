@@ -1,0 +1,241 @@
+/*
+ * thorin_main.c -- host driver of the GPU hydro step, mirroring the
+ * reference's main loop (src_main/main.c:24-230): read in.par, set up the
+ * planets and the disk, then call AlgoGas once per DT and write the
+ * gas*<n>.dat outputs every NINTERM steps (Output.c:153-237).
+ *
+ *   thorin_amd [-v] [-0] [-s N] [-o outputdir] in.par
+ *
+ * Output files and formats follow the reference: raw native-endian float64
+ * gas<name><n>.dat (no header, ring-major), dims.dat, used_rad.dat,
+ * omegaframe.dat and tqwk<k>.dat.  The N-body state for restarts is written
+ * as nbody.state<n>.dat (text) because REBOUND's binary snapshot is not part
+ * of this build.
+ */
+#define _GNU_SOURCE
+#include "../../include/thorin_amd.h"
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <sys/stat.h>
+#include <sys/types.h>
+
+static void die(const char *what) {
+  fprintf(stderr, "thorin_amd: %s: %s\n", what, thorin_last_error());
+  exit(1);
+}
+
+static void mkdir_p(const char *dir) {
+  char tmp[THORIN_STRLEN];
+  size_t n;
+  char *q;
+  snprintf(tmp, sizeof tmp, "%s", dir);
+  n = strlen(tmp);
+  if (n && tmp[n - 1] == '/') tmp[n - 1] = 0;
+  for (q = tmp + 1; *q; q++)
+    if (*q == '/') { *q = 0; mkdir(tmp, 0755); *q = '/'; }
+  mkdir(tmp, 0755);
+}
+
+static void write_polar(const thorin_params *p, thorin_ctx *c, int field, const char *name, int n, double *buf) {
+  char fn[2 * THORIN_STRLEN];
+  FILE *f;
+  size_t cnt = (size_t)p->nrad * p->nsec;
+  if (thorin_download_field(c, field, buf, cnt)) die("download");
+  snprintf(fn, sizeof fn, "%sgas%s%d.dat", p->outputdir, name, n);
+  f = fopen(fn, "w");
+  if (!f) { fprintf(stderr, "cannot open %s\n", fn); exit(1); }
+  fwrite(buf, sizeof(double), cnt, f);
+  fclose(f);
+  printf("Writing 'gas%s%d.dat'...0/done\n", name, n);
+}
+
+/* SendOutput, Output.c:202-237 */
+static void send_output(const thorin_params *p, thorin_ctx *c, int n, double *buf, double *buf2) {
+  int ln = p->write_density ? n : 0;
+  printf("\n*** OUTPUT %d ***\n", n);
+  write_polar(p, c, THORIN_F_DENS, "dens", ln, buf);
+  ln = p->write_velocity ? n : 0;
+  write_polar(p, c, THORIN_F_VRAD, "vrad", ln, buf);
+  write_polar(p, c, THORIN_F_VTHETA, "vtheta", ln, buf);
+  if (p->write_energy) write_polar(p, c, THORIN_F_ENERGY, "energy", n, buf);
+  if (p->write_temperature) write_polar(p, c, THORIN_F_TEMPER, "temper", n, buf);
+  if (p->write_divergence) write_polar(p, c, THORIN_F_DIVV, "divv", n, buf);
+  if (p->write_qplus) write_polar(p, c, THORIN_F_QPLUS, "qplus", n, buf);
+  if (p->write_qbalance) {
+    char fn[2 * THORIN_STRLEN];
+    size_t k, cnt = (size_t)p->nrad * p->nsec;
+    FILE *f;
+    if (thorin_download_field(c, THORIN_F_QPLUS, buf, cnt) || thorin_download_field(c, THORIN_F_QMINUS, buf2, cnt)) die("download");
+    for (k = 0; k < cnt; k++) buf[k] = buf[k] - buf2[k];   /* ActualizeQbalance, Output.c:240-256 */
+    snprintf(fn, sizeof fn, "%sgasqbalance%d.dat", p->outputdir, n);
+    f = fopen(fn, "w");
+    if (f) { fwrite(buf, sizeof(double), cnt, f); fclose(f); }
+  }
+  if (p->write_opacity) write_polar(p, c, THORIN_F_OPACITY, "opacity", n, buf);
+}
+
+static int read_binary(const char *fn, double *dst, size_t cnt) {
+  FILE *f = fopen(fn, "r");
+  size_t got;
+  if (!f) { fprintf(stderr, "WARNING ! Can't read %s. Restarting with t=0 settings.\n", fn); return -1; }
+  got = fread(dst, sizeof(double), cnt, f);
+  fclose(f);
+  return got == cnt ? 0 : -1;
+}
+
+/* ReadfromAsciiFile, Init.c:148-199 */
+static void read_ascii(const char *fn, double *dst, size_t cnt) {
+  FILE *f = fopen(fn, "r");
+  size_t k;
+  double tmp;
+  if (!f) { fprintf(stderr, "\nError! Can't find the initialization file %s \n", fn); exit(1); }
+  for (k = 0; k < cnt; k++)
+    if (fscanf(f, "%lf", &dst[k]) == EOF) {
+      fprintf(stderr, "\nError! Number of values in the initialization file %s\nis smaller than the size of hydrodynamic arrays.\n", fn);
+      exit(1);
+    }
+  if (fscanf(f, "%lf", &tmp) != EOF) {
+    fprintf(stderr, "\nError! Number of values in the initialization file %s\nis larger than the size of hydrodynamic arrays.\n", fn);
+    exit(1);
+  }
+  fclose(f);
+}
+
+static void write_nbody_state(const thorin_params *p, const thorin_planets *s, double t, int n) {
+  char fn[2 * THORIN_STRLEN];
+  FILE *f;
+  int k;
+  snprintf(fn, sizeof fn, "%snbody.state%d.dat", p->outputdir, n);
+  f = fopen(fn, "w");
+  if (!f) return;
+  fprintf(f, "%d %.17g\n", s->nb, t);
+  for (k = 0; k < s->nb; k++)
+    fprintf(f, "%.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g\n", s->mass[k], s->x[k], s->y[k], s->z[k], s->vx[k],
+            s->vy[k], s->vz[k], s->gas[k], s->solid[k]);
+  fclose(f);
+}
+
+static int read_nbody_state(const thorin_params *p, thorin_planets *s, double *t, int n) {
+  char fn[2 * THORIN_STRLEN];
+  FILE *f;
+  int k, nb;
+  snprintf(fn, sizeof fn, "%snbody.state%d.dat", p->outputdir, n);
+  f = fopen(fn, "r");
+  if (!f) return -1;
+  if (fscanf(f, "%d %lf", &nb, t) != 2 || nb != s->nb) { fclose(f); return -1; }
+  for (k = 0; k < nb; k++)
+    if (fscanf(f, "%lf %lf %lf %lf %lf %lf %lf %lf %lf", &s->mass[k], &s->x[k], &s->y[k], &s->z[k], &s->vx[k], &s->vy[k],
+               &s->vz[k], &s->gas[k], &s->solid[k]) != 9) { fclose(f); return -1; }
+  fclose(f);
+  return 0;
+}
+
+int main(int argc, char **argv) {
+  thorin_params p;
+  thorin_planets sys;
+  thorin_ctx *c = NULL;
+  thorin_state st;
+  thorin_step_log *lg;
+  const char *parfile = NULL, *outdir = NULL;
+  int i, k, restart = 0, nrestart = 0, only_init = 0, verbose = 0, begin_i = 0;
+  size_t cnt;
+  double *buf, *buf2;
+  char fn[2 * THORIN_STRLEN];
+  FILE *f;
+  for (i = 1; i < argc; i++) {
+    if (argv[i][0] == '-') {
+      if (strchr(argv[i], 'v')) verbose = 1;
+      if (strchr(argv[i], '0')) only_init = 1;
+      if (strchr(argv[i], 's')) { restart = 1; if (++i < argc) nrestart = atoi(argv[i]); }
+      else if (strchr(argv[i], 'o')) { if (++i < argc) outdir = argv[i]; }
+    } else parfile = argv[i];
+  }
+  if (!parfile) { fprintf(stderr, "Usage: %s [-v] [-0] [-s N] [-o dir] in.par\n", argv[0]); return 1; }
+  if (thorin_params_read(parfile, &p)) die("parameters");
+  if (outdir) {
+    snprintf(p.outputdir, sizeof p.outputdir, "%s", outdir);
+    if (thorin_params_derive(&p)) die("parameters");
+  }
+  mkdir_p(p.outputdir);
+  if (thorin_planets_from_config(&p, &sys)) die("planets");
+  printf("%d planet(s) found.\n", sys.nb);
+  if (thorin_ctx_create(&p, &c)) die("context");
+  cnt = (size_t)p.nrad * p.nsec;
+  buf = (double *)malloc(sizeof(double) * (cnt + p.nsec));
+  buf2 = (double *)malloc(sizeof(double) * (cnt + p.nsec));
+  if (restart) {
+    double *d = malloc(sizeof(double) * cnt), *vr = malloc(sizeof(double) * cnt), *vt = malloc(sizeof(double) * cnt),
+           *t = malloc(sizeof(double) * cnt), tt = 0.0;
+    snprintf(fn, sizeof fn, "%sgasdens%d.dat", p.outputdir, nrestart); read_binary(fn, d, cnt);
+    snprintf(fn, sizeof fn, "%sgasvrad%d.dat", p.outputdir, nrestart); read_binary(fn, vr, cnt);
+    snprintf(fn, sizeof fn, "%sgasvtheta%d.dat", p.outputdir, nrestart); read_binary(fn, vt, cnt);
+    snprintf(fn, sizeof fn, "%sgastemper%d.dat", p.outputdir, nrestart); read_binary(fn, t, cnt);
+    if (thorin_init_state(c, 2, d, vr, vt, t)) die("restart");
+    if (read_nbody_state(&p, &sys, &tt, nrestart) == 0) {
+      thorin_get_state(c, &st);
+      st.physical_time = st.physical_time_initial = tt;
+      thorin_set_state(c, &st);
+    }
+    begin_i = nrestart * p.ninterm;
+    free(d); free(vr); free(vt); free(t);
+  } else if (p.init_from_file) {
+    double *d = malloc(sizeof(double) * cnt), *vr = malloc(sizeof(double) * cnt), *vt = malloc(sizeof(double) * cnt),
+           *t = malloc(sizeof(double) * cnt);
+    fprintf(stderr, "Initializing from files...\n");
+    read_ascii(p.densinfile, d, cnt);
+    read_ascii(p.vradinfile, vr, cnt);
+    read_ascii(p.vthetainfile, vt, cnt);
+    read_ascii(p.temperinfile, t, cnt);
+    if (thorin_init_state(c, 1, d, vr, vt, t)) die("init");
+    free(d); free(vr); free(vt); free(t);
+  } else {
+    if (thorin_init_state(c, 0, NULL, NULL, NULL, NULL)) die("init");
+  }
+  if (verbose) printf("NRAD %d NSEC %d DT %g NTOT %d NINTERM %d\n", p.nrad, p.nsec, p.dt, p.ntot, p.ninterm);
+  /* used_rad.dat (SourceEuler.c:119-129) and dims.dat (Output.c:191-200) */
+  {
+    thorin_geometry g;
+    thorin_get_geometry(c, &g);
+    snprintf(fn, sizeof fn, "%sused_rad.dat", p.outputdir);
+    f = fopen(fn, "w");
+    if (f) { for (i = 0; i <= p.nrad; i++) fprintf(f, "%.18g\n", g.radii[i]); fclose(f); }
+    snprintf(fn, sizeof fn, "%sdims.dat", p.outputdir);
+    f = fopen(fn, "w");
+    if (f) { fprintf(f, "%d\t%d\t\t%d\t%d\t%f\t%d\t%d\t%d\n", 0, 0, 0, 0, p.rmax, p.ntot / p.ninterm, p.nrad, p.nsec); fclose(f); }
+  }
+  lg = (thorin_step_log *)calloc(1, sizeof(thorin_step_log));
+  for (i = begin_i; i <= p.ntot; i++) {
+    int ts = i / p.ninterm;
+    if (p.ninterm * ts == i) {
+      thorin_get_state(c, &st);
+      send_output(&p, c, ts, buf, buf2);
+      write_nbody_state(&p, &sys, st.physical_time, ts);
+      snprintf(fn, sizeof fn, "%somegaframe.dat", p.outputdir);
+      f = fopen(fn, "a");
+      if (f) { fprintf(f, "%d\t%#.18g\n", ts, st.omega_frame); fclose(f); }
+      if (only_init) break;
+    }
+    k = thorin_algogas(c, &sys, thorin_nbody_step, NULL, lg);
+    if (k < 0) die("AlgoGas");
+    for (k = 0; k < lg->nsub; k++) printf("%c", lg->crashed[k] ? 'c' : '.');
+    printf("\n");
+    fflush(stdout);
+    if (p.write_torque)
+      for (k = 0; k < lg->tq_rows; k++) {
+        snprintf(fn, sizeof fn, "%stqwk%d.dat", p.outputdir, k);
+        f = fopen(fn, "a");
+        if (f) {
+          fprintf(f, "%#.18g\t%#.18g\t%#.18g\t%#.18g\t%#.18g\t%#.18g\t%#.18g\n", lg->tq[k][0], lg->tq[k][1], lg->tq[k][2],
+                  lg->tq[k][3], lg->tq[k][4], lg->tq[k][5], lg->tq[k][6]);
+          fclose(f);
+        }
+      }
+  }
+  thorin_ctx_destroy(c);
+  free(buf);
+  free(buf2);
+  free(lg);
+  return 0;
+}

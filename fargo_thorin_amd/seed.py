@@ -1,0 +1,85 @@
+"""Seeded synthetic disks for tests and benchmarks.
+
+A disk is given as the four arrays the reference's InitializeFromFile route
+reads (src_main/Init.c:86-105): surface density, radial and azimuthal
+velocity, temperature, each NRAD*NSEC doubles in ring-major order.
+
+Noise is drawn with SplitMix64: u(seed, k) = (splitmix64(seed ^ k) >> 11) * 2**-53
+with k = i*NSEC + j for cells.  The radial grid follows FillPolar1DArrays
+(SourceEuler.c:81-106) and is evaluated with Python's math module (the C
+library), so the same seed gives the same bits on every host.
+
+Profiles (code units, G = M_star = 1):
+  Sigma  = sigma0 * Rmed**(-slope) * (1 + noise * (2u - 1))
+  T      = h**2 / Rmed                      (so cs**2 = gamma*T, H/r = h)
+  vrad   = 0
+  vtheta = sqrt(1/Rmed) * sqrt(1 - T*Rmed)  (the pressure-corrected Keplerian
+           velocity of InitGasVelocity, Pframeforce.c:716, with cs**2 = gamma*T)
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+_M64 = np.uint64(0xFFFFFFFFFFFFFFFF)
+
+
+def splitmix64(x: np.ndarray) -> np.ndarray:
+    """SplitMix64 finaliser of uint64 inputs (vectorised, wraps mod 2**64)."""
+    z = (x.astype(np.uint64) + np.uint64(0x9E3779B97F4A7C15)) & _M64
+    z = ((z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)) & _M64
+    z = ((z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)) & _M64
+    return z ^ (z >> np.uint64(31))
+
+
+def uniform(seed: int, k: np.ndarray) -> np.ndarray:
+    """u(seed, k) in [0, 1) with 53 random bits."""
+    x = np.uint64(seed) ^ k.astype(np.uint64)
+    return (splitmix64(x) >> np.uint64(11)).astype(np.float64) * (2.0 ** -53)
+
+
+def radii(nrad: int, rmin: float, rmax: float, log_grid: bool = True) -> list[float]:
+    """Radii[0..nrad] exactly as FillPolar1DArrays computes them."""
+    if log_grid:
+        lg = math.log(rmax / rmin)
+        return [rmin * math.exp(float(i) / float(nrad) * lg) for i in range(nrad + 1)]
+    d = (rmax - rmin) / float(nrad)
+    return [rmin + d * float(i) for i in range(nrad + 1)]
+
+
+def rmed(nrad: int, rmin: float, rmax: float, log_grid: bool = True) -> np.ndarray:
+    r = radii(nrad, rmin, rmax, log_grid)
+    out = np.empty(nrad)
+    for i in range(nrad):
+        a, b = r[i], r[i + 1]
+        v = 2.0 / 3.0 * (b * b * b - a * a * a)
+        out[i] = v / (b * b - a * a)
+    return out
+
+
+def seeded_disk(nrad: int, nsec: int, *, rmin: float = 0.4, rmax: float = 2.5, log_grid: bool = True,
+                sigma0: float = 5e-4, slope: float = 0.5, h: float = 0.05, noise: float = 1e-3,
+                seed: int = 12345) -> dict[str, np.ndarray]:
+    """Returns dens, vrad, vtheta, temper as (nrad, nsec) float64 arrays."""
+    rm = rmed(nrad, rmin, rmax, log_grid)
+    sig_ring = np.array([sigma0 * math.pow(r, -slope) for r in rm])
+    t_ring = np.array([h * h / r for r in rm])
+    vt_ring = np.array([math.sqrt(1.0 / r) * math.sqrt(1.0 - (h * h / r) * r) for r in rm])
+    k = np.arange(nrad * nsec, dtype=np.uint64).reshape(nrad, nsec)
+    u = uniform(seed, k)
+    dens = sig_ring[:, None] * (1.0 + noise * (2.0 * u - 1.0))
+    temper = np.repeat(t_ring[:, None], nsec, axis=1)
+    vtheta = np.repeat(vt_ring[:, None], nsec, axis=1)
+    vrad = np.zeros((nrad, nsec))
+    return {"dens": np.ascontiguousarray(dens), "vrad": vrad, "vtheta": np.ascontiguousarray(vtheta),
+            "temper": np.ascontiguousarray(temper)}
+
+
+def write_cfg(path: str, arr: np.ndarray) -> None:
+    """ASCII file for InitializeFromFile: one value per line, ring-major,
+    shortest repr that round-trips through the reference's %lf."""
+    flat = np.ascontiguousarray(arr).ravel()
+    with open(path, "w") as f:
+        f.write("\n".join(repr(float(v)) for v in flat))
+        f.write("\n")

@@ -1,0 +1,237 @@
+/*
+ * thorin_amd.h -- C ABI of the MI355X-native FARGO_THORIN hydro step.
+ *
+ * The library (fargo_thorin_amd/libthorin_amd.so) keeps every PolarGrid field
+ * resident in HBM and advances it by one DT per thorin_algogas() call.  All
+ * signatures use plain C types: no torch, no HIP types.
+ *
+ * Reference interfaces replaced (paths relative to the reference tree):
+ *   thorin_algogas        <- AlgoGas()                 src_main/SourceEuler.c:210-317,
+ *                                                       prototype src_main/proto.h:74,
+ *                                                       called once per DT by main.c:201
+ *   thorin_ctx_create     <- InitEuler()+FillPolar1DArrays()+InitComputeAccel()
+ *                                                       SourceEuler.c:68-175, SideEuler.c:101-118
+ *   thorin_init_state     <- Initialization()          Init.c:72-145 (analytic, InitFromFile and
+ *                                                       restart routes)
+ *   thorin_upload_field /
+ *   thorin_download_field <- the host PolarGrid ->Field arrays (types.h:37-44) that
+ *                            WriteDiskPolar() (Output.c:153-189) and ReadfromFile()
+ *                            (Init.c:23-54) read and write
+ *   thorin_params_read    <- ReadVariables()/InitVariables()  Interpret.c:65-266, var.c:17-140
+ *
+ * Layout: a field is (nrad+1) rings of nsec doubles, cell l = j + i*nsec
+ * (SourceEuler.c:359); ring nrad is the ghost ring that vrad and the star
+ * values use (TransportEuler.c:271, Pframeforce.c:749).
+ *
+ * Error convention: every entry point returns 0 on success, a negative value
+ * on a HIP/allocation/argument error and a positive value for a physics
+ * warning (crash detected, SOR not converged); thorin_last_error() describes
+ * the last failure.  The library never calls exit().
+ */
+#ifndef THORIN_AMD_H
+#define THORIN_AMD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define THORIN_MAXPLANETS 120   /* types.h:67 MAXPLANETS */
+#define THORIN_STRLEN     512
+
+/* Opacity laws (Interpret.c:250-265, Opacity.c).  POWERLAW (kappa = k0*T^2 in
+ * cgs, k0 = PowerLawOpacity) has no counterpart in the reference. */
+enum { THORIN_OPA_BL94 = 0, THORIN_OPA_LP85 = 1, THORIN_OPA_ZHU12 = 2,
+       THORIN_OPA_CONST = 3, THORIN_OPA_POWERLAW = 4 };
+
+/* Every in.par key of var.c:19-139, plus the flags ReadVariables derives
+ * from them (Interpret.c:149-265). */
+typedef struct thorin_params {
+  /* var.c numeric keys */
+  double dt, sigma0; int ninterm, ntot;
+  double masstaper; int nrad, nsec; double rmin, rmax;
+  double thicknesssmoothing, rochesmoothing, aspectratio, viscosity, alphaviscosity;
+  double tmri, twidth, alphamri, alphadead, sigmaslope, releaseradius, releasedate;
+  double omegaframe, imposeddiskdrift, flaringindex, eccentricity;
+  double cavityradius, cavityratio, cavitywidth;
+  double transitionradius, transitionratio, transitionwidth, lambdadoubling;
+  double adiabind, coolingtime, opacitydrop, effectivetemperature, stellarradius;
+  double discalbedo, parametricopacity;
+  double dampingrminfrac, dampingrmaxfrac, dampingperiodfrac;
+  int noutelements; double planetarydensity; int targetnpl;
+  double ias15precission, ias15mindt;
+  double hillcut, verticaldamping, accretionrate;
+  double pebbleflux, pebblealpha, pebblecoagulation, pebblebulkdens, schmidtnumber;
+  int heatingdelay;
+  double parametricaccretion; int gettorqueforplanet;
+  double pebbletogasmax, fragmentfactor, fragmentthreshold, fragmentalpha, gasflux;
+  double evaporationtemperature, evaporationrate, firststep, minimumstep, maximumstep;
+  double powerlawopacity;          /* POWERLAW opacity coefficient (cgs); not in var.c */
+  /* var.c string keys (verbatim) */
+  char outputdir[THORIN_STRLEN], inputdir[THORIN_STRLEN];
+  char innerboundary[64], outerboundary[64], labeladvection[64], transport[64];
+  char planetconfig[THORIN_STRLEN], gridspacing[64], alphaflock[64], disk[64], frame[64];
+  char outersourcemass[64], writedensity[64], writevelocity[64], indirectterm[64];
+  char excludehill[64], energyequation[64], writetemperature[64], writeenergy[64];
+  char writedivv[64], writeqplus[64], writeqbalance[64], writeopacity[64];
+  char stellarirradiation[64], initializefromfile[64];
+  char densinfile[THORIN_STRLEN], vradinfile[THORIN_STRLEN];
+  char vthetainfile[THORIN_STRLEN], temperinfile[THORIN_STRLEN];
+  char damptowards[64], resolvecollisions[64], discardplanets[64], writetorquefiles[64];
+  char planetsfeeldisk[64], gasaccretionheating[64], pebbleaccretion[64], backreaction[64];
+  char accretionalheating[64], writeeta[64], particlediffusion[64], torquemapinfile[64];
+  char opacitytable[64], damping[64], aerodynamicdrag[64], elementswithdisk[64];
+  char pebblegravity[64];
+  /* derived flags (Interpret.c:149-265) */
+  int advect_label, outer_source_mass, fast_transport, open_inner, open_outer, non_reflecting;
+  int damping_on, damp_vrad, damp_init, damp_flux, log_grid, is_disk, corotating, guiding_center;
+  int write_velocity, write_density, indirect_term, exclude_hill, viscosity_alpha, alpha_flock_on;
+  int roche_smoothing, energy_eq, write_temperature, write_energy, write_divergence, write_qplus;
+  int write_qbalance, write_opacity, parametric_cooling, stellar_irradiation, init_from_file;
+  int write_torque, write_torque_map, collisions, monitor_npl, feel_disk, gas_accret_heating;
+  int pebbles, back_reaction, diffusive_particles, accret_heating, write_eta;
+  int prescribed_accretion, torque_density, discard_planets, aerodynamic_drag;
+  int elements_with_disk, pebble_gravity, opacity_law;
+} thorin_params;
+
+/* Planetary system as AlgoGas sees it (types.h:96-117), heliocentric. */
+typedef struct thorin_planets {
+  int nb;
+  double mass[THORIN_MAXPLANETS], gas[THORIN_MAXPLANETS], solid[THORIN_MAXPLANETS];
+  double x[THORIN_MAXPLANETS], y[THORIN_MAXPLANETS], z[THORIN_MAXPLANETS];
+  double vx[THORIN_MAXPLANETS], vy[THORIN_MAXPLANETS], vz[THORIN_MAXPLANETS];
+  double ax[THORIN_MAXPLANETS], ay[THORIN_MAXPLANETS], az[THORIN_MAXPLANETS];
+  double acc[THORIN_MAXPLANETS];
+  int feeldisk[THORIN_MAXPLANETS];
+} thorin_planets;
+
+/* Host N-body step that AlgoGas calls between the force evaluation and the
+ * boundary conditions: AdvanceSystemRebound + RotatePsys +
+ * SynchronizeFargoRebound (SourceEuler.c:260-269, ReboundInterface.c:262-355,
+ * Psys.c:336-355).  t is PhysicalTime at the start of the substep. */
+typedef int (*thorin_nbody_fn)(void *user, thorin_planets *sys, double t, double dt,
+                               double rotation_angle);
+
+/* Scalars AlgoGas keeps between calls (the reference's globals and
+ * function-local statics: global.h:14-20, SourceEuler.c:601,
+ * EnergySources.c:39-41,210-212). */
+typedef struct thorin_state {
+  double physical_time;      /* PhysicalTime */
+  double physical_time_initial;
+  double omega_frame;        /* OmegaFrame */
+  double mass_taper;         /* MassTaper */
+  double dt_stellar;         /* dt_stellar (EnergySources.c:855), 1e38 initially */
+  double sor_omega, sor_domega; /* omega, domega */
+  int sor_first, sor_niter_last, sor_niter_best; double sor_omega_best;
+  int cfl_count;             /* ConditionCFL's static count */
+  int already_crashed;       /* AlreadyCrashed */
+} thorin_state;
+
+/* What one thorin_algogas() call did; the log mirrors what the reference
+ * prints ('.'/'c' per substep, SourceEuler.c:285-287; SOR Niter under -d,
+ * EnergySources.c:314-315). */
+#define THORIN_MAXSUB 4096
+typedef struct thorin_step_log {
+  int nsub;                      /* substeps taken in this DT */
+  int crashed_any;               /* any substep reported a crash */
+  int cfl_n[THORIN_MAXSUB];      /* GasTimeStepsCFL of each substep */
+  double dt[THORIN_MAXSUB];      /* dt of each substep */
+  int sor_niter[THORIN_MAXSUB];  /* SOR iterations of each substep (0 without energy eq.) */
+  double sor_omega[THORIN_MAXSUB];
+  int crashed[THORIN_MAXSUB];
+  double lost_mass;              /* LostMass accumulated by VanLeerRadial (OpenInner) */
+  /* torque log of the first substep, one row per planet (UpdateLog,
+   * Force.c:27-70): PhysicalTime, tq, tq_norm, x, y, ax, ay */
+  int tq_rows;
+  double tq[8][7];
+} thorin_step_log;
+
+/* Field identifiers for upload/download.  Names are the reference's
+ * CreatePolarGrid names ("gas"+name is the output file prefix). */
+enum {
+  THORIN_F_DENS = 0, THORIN_F_VRAD, THORIN_F_VTHETA, THORIN_F_ENERGY, THORIN_F_LABEL,
+  THORIN_F_TEMPER, THORIN_F_PRESSURE, THORIN_F_SOUNDSPEED, THORIN_F_QPLUS, THORIN_F_QMINUS,
+  THORIN_F_OPACITY, THORIN_F_DIVV, THORIN_F_TAURR, THORIN_F_TAURP, THORIN_F_TAUPP,
+  THORIN_F_AGR, THORIN_F_AGT, THORIN_F_ENERGYINT, THORIN_F_ENERGYNEW,
+  THORIN_F_VRADINT, THORIN_F_VTHETAINT, THORIN_F_VRADNEW, THORIN_F_VTHETANEW,
+  THORIN_F_QBALANCE,
+  THORIN_F_PEBBLEDENS, THORIN_F_PEBBLEVRAD, THORIN_F_PEBBLEVTHETA, THORIN_F_PEBBLESIZE,
+  THORIN_F_PEBBLESTOKES, THORIN_F_ETA, THORIN_F_TORQUE,
+  THORIN_NFIELDS
+};
+
+/* 1D geometry (global.h:11-24) filled exactly as FillPolar1DArrays
+ * (SourceEuler.c:68-131) and Theo.c.  Arrays have nrad+1 entries. */
+typedef struct thorin_geometry {
+  int nrad, nsec;
+  const double *radii, *rinf, *rsup, *rmed, *surf, *inv_rmed, *inv_surf;
+  const double *inv_diff_rmed, *inv_diff_rsup, *inv_rinf, *omega_inv, *rmed2;
+  const double *sigma_med, *sigma_inf, *energy_med, *vtheta_med, *wave_killer;
+  const double *globcsvec, *globtempervec;
+} thorin_geometry;
+
+typedef struct thorin_ctx thorin_ctx;
+
+/* ---- parameters (Interpret.c:65-266) ---- */
+int  thorin_params_defaults(thorin_params *p);
+int  thorin_params_read(const char *path, thorin_params *p);
+int  thorin_params_derive(thorin_params *p);
+/* bindings: heap-allocated defaults, one "KEY value" in.par line at a time,
+ * read back a key, read a derived flag */
+thorin_params *thorin_params_new(void);
+void thorin_params_free(thorin_params *p);
+int  thorin_params_set(thorin_params *p, const char *key, const char *value);
+int  thorin_params_get(const thorin_params *p, const char *key, char *out, int outlen);
+int  thorin_params_flag(const thorin_params *p, const char *name);
+
+/* ---- context ---- */
+int  thorin_ctx_create(const thorin_params *p, thorin_ctx **out);
+/* host-only context (no device buffers): geometry and initial state only;
+ * thorin_host_field() returns the initial dens/vrad/vtheta/energy/label/
+ * temper/pressure/soundspeed as Initialization() leaves them */
+int  thorin_ctx_create_host(const thorin_params *p, thorin_ctx **out);
+int  thorin_host_field(thorin_ctx *ctx, int field, double *out, size_t count);
+int  thorin_ctx_destroy(thorin_ctx *ctx);
+int  thorin_get_geometry(thorin_ctx *ctx, thorin_geometry *g);
+int  thorin_get_state(thorin_ctx *ctx, thorin_state *s);
+int  thorin_set_state(thorin_ctx *ctx, const thorin_state *s);
+
+/* Initialization (Init.c:72-145).  mode 0: analytic InitEuler profiles;
+ * mode 1: the four host arrays (dens, vrad, vtheta, temper; nrad*nsec each,
+ * ring-major) replace the fields as the InitializeFromFile route does
+ * (energy = T*rho/(gamma-1), then RefillSigma/RefillEnergy/FillVtheta and
+ * cs, P, T); mode 2: restart, the same arrays are the gas*N.dat contents. */
+int  thorin_init_state(thorin_ctx *ctx, int mode, const double *dens, const double *vrad,
+                       const double *vtheta, const double *temper);
+
+/* ---- fields ---- */
+int  thorin_upload_field(thorin_ctx *ctx, int field, const double *host, size_t count);
+int  thorin_download_field(thorin_ctx *ctx, int field, double *host, size_t count);
+
+/* ---- the hot path: one DT of AlgoGas ---- */
+int  thorin_algogas(thorin_ctx *ctx, thorin_planets *sys, thorin_nbody_fn nbody,
+                    void *nbody_user, thorin_step_log *log);
+
+/* Disk quantities the host needs once per DT for the torque log
+ * (UpdateLog, Force.c:27-70): ring means of dens and soundspeed
+ * (nrad doubles each) and the last disk->planet accelerations in sys. */
+int  thorin_ring_means(thorin_ctx *ctx, int field, double *out);
+
+/* Total gas mass (GasTotalMass, SideEuler.c:26-51), reduced on the GPU. */
+int  thorin_gas_total_mass(thorin_ctx *ctx, double *out);
+
+/* Synchronise the device and return the library's last error text. */
+int  thorin_sync(thorin_ctx *ctx);
+const char *thorin_last_error(void);
+
+/* ---- host N-body (REBOUND stays host-side; this is the library's own
+ * integrator used when no REBOUND is linked) ---- */
+int  thorin_planets_from_config(const thorin_params *p, thorin_planets *sys);
+int  thorin_nbody_step(void *user, thorin_planets *sys, double t, double dt,
+                       double rotation_angle);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

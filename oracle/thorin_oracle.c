@@ -1,0 +1,1522 @@
+/*
+ * thorin_oracle.c -- CPU oracle for the FARGO_THORIN hydro step (AlgoGas).
+ *
+ * TEST INFRASTRUCTURE ONLY: loaded by tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py as the checker.  Never linked into the product.
+ *
+ * A sequential C99 restatement of the reference's per-DT hydro step for the
+ * in-scope options (single rank, so Zero_or_active=0, One_or_active=1,
+ * Max_or_active=NRAD, MaxMO_or_active=NRAD-1; split.c:45-48 with
+ * CPU_Number=1).  Every function cites the reference lines it follows.  The
+ * arithmetic keeps the reference's operation order, so that, compiled with
+ * gcc -std=c99 (no FMA contraction, as the reference's makefile:33 build),
+ * the oracle reproduces the reference bit for bit.
+ *
+ * Parity pins (tests/test_oracle_golden.py, tests/test_oracle_math.py):
+ *   - bitwise against outputs of the reference itself, built from its
+ *     sources by oracle/ref.mk and run on seeded disks (tests/golden/);
+ *   - closed forms / invariants: opacity laws vs independent restatements,
+ *     flux limiter, SOR solution vs a direct sparse solve, mass conservation
+ *     of transport, FARGO shift as an exact rotation.
+ * Functions whose branches no golden run exercises are listed as
+ * "parity unpinned" in DESIGN.md (e.g. stellar irradiation, OpenInner).
+ */
+#define _GNU_SOURCE
+#include "thorin_oracle.h"
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define PI 3.14159265358979323844          /* fondam.h:12 */
+#define G_ 1.0                              /* fondam.h:11 */
+#define GASCONST 1.0                        /* fondam.h:17 */
+#define MOLWEIGHT 1.0                       /* fondam.h:18 */
+#define CFLSECURITY 0.5                     /* SourceEuler.c:16 */
+#define CVNR 1.41                           /* SourceEuler.c:19 */
+#define SORMAXITERS 100000                  /* EnergySources.c:22 */
+#define SOREPS 1.0e-6                       /* EnergySources.c:23 */
+#define SORMIN 1.0e-15                      /* EnergySources.c:24 */
+#define SORPRECISION 1.0e-15                /* EnergySources.c:25 */
+#define OMEGAUPPER 1.9                      /* EnergySources.c:27 */
+
+/* unit constants, same expressions as fondam.h:28-39 (folded by gcc) */
+#define MSOL_SI 1.98855e30
+#define GM_SI 1.32712440018e20
+#define AU_SI 149597870700.0
+#define R_STANDARD 8.3144598
+#define MMW 2.4
+#define R_SI (R_STANDARD/(MMW*0.001))
+#define OPA2CU (MSOL_SI*1000.0/pow(AU_SI*100.0,2.0))
+#define T2SI (GM_SI/R_SI/AU_SI)
+#define TIME2SI (sqrt(pow(AU_SI,3.0)/GM_SI))
+#define RHO2CGS (MSOL_SI*1000.0/pow(AU_SI*100.0,3.0))
+#define STEFANBOLTZMANN (5.670367e-8*(1.0/MSOL_SI)*pow(1.0/TIME2SI,-3.0)*pow(1.0/T2SI,-4.0))
+#define SQRT2PI_INV (1.0/sqrt(2.0*PI))
+
+enum { F_DENS = THORIN_F_DENS };
+#define NF THORIN_NFIELDS
+/* private work fields beyond the public ids */
+enum { W_RHOSTAR = NF, W_RHOINT, W_TEMPERINT, W_DRR, W_DRP, W_DPP, W_RADMOMP, W_RADMOMM,
+       W_THETAMOMP, W_THETAMOMM, W_WORK, W_QRSTAR, W_VTHETARES, W_GRADTR, W_GRADTT,
+       W_GRADTM, W_DIFFC, W_DIFFR, W_DIFFT, W_VOLDENS, W_A, W_B, W_ML, W_MLIP, W_MLIM,
+       W_MLJP, W_MLJM, W_RHS, W_ABS, W_ORD, W_TEMPSHIFT, W_DQ, NW };
+
+struct oracle_ctx {
+  thorin_params p;
+  int nr, ns;
+  double *fld[NW];
+  double *Radii, *Rinf, *Rsup, *Rmed, *Surf, *InvRmed, *InvSurf, *InvDiffRmed, *InvDiffRsup;
+  double *InvRinf, *OmegaInv, *Rmed2, *GlobalRmed, *SigmaMed, *SigmaInf, *EnergyMed;
+  double *VthetaMed, *WaveKiller, *globcsvec, *globtempervec, *Vmoy, *Vresidual, *VMed;
+  int *Nshift;
+  double sqrt_adiabind_inv, CV, invdtreb_sq;
+  thorin_state st;
+  int opa_filled;
+  double indirect_x, indirect_y;
+};
+
+static double *fieldalloc(int nr, int ns) {
+  /* LowTasks.c:85: (Nr+3)*(Ns+1)+5 doubles, rings 0..Nr zeroed */
+  return (double *)calloc((size_t)(nr + 3) * (ns + 1) + 5, sizeof(double));
+}
+
+/* ---------------- geometry: FillPolar1DArrays (SourceEuler.c:68-131) ---------------- */
+static void o_fill_1d(oracle_ctx *o) {
+  int nr = o->nr, ns = o->ns, i;
+  double drrsep = (o->p.rmax - o->p.rmin) / (double)nr;
+  FILE *in;
+  char name[2 * THORIN_STRLEN];
+  snprintf(name, sizeof name, "%s%s", o->p.outputdir, "radii.dat");
+  in = fopen(name, "r");
+  if (in == NULL) {
+    if (o->p.log_grid)
+      for (i = 0; i <= nr; i++) o->Radii[i] = o->p.rmin * exp((double)i / (double)nr * log(o->p.rmax / o->p.rmin));
+    else
+      for (i = 0; i <= nr; i++) o->Radii[i] = o->p.rmin + drrsep * (double)(i);
+  } else {
+    float t;
+    for (i = 0; i <= nr; i++) { if (fscanf(in, "%f", &t) != 1) t = 0.f; o->Radii[i] = (double)t; }
+    fclose(in);
+  }
+  for (i = 0; i < nr; i++) {
+    o->GlobalRmed[i] = 2.0 / 3.0 * (o->Radii[i + 1] * o->Radii[i + 1] * o->Radii[i + 1] - o->Radii[i] * o->Radii[i] * o->Radii[i]);
+    o->GlobalRmed[i] = o->GlobalRmed[i] / (o->Radii[i + 1] * o->Radii[i + 1] - o->Radii[i] * o->Radii[i]);
+  }
+  for (i = 0; i < nr; i++) {
+    o->Rinf[i] = o->Radii[i];
+    o->Rsup[i] = o->Radii[i + 1];
+    o->Rmed[i] = 2.0 / 3.0 * (o->Rsup[i] * o->Rsup[i] * o->Rsup[i] - o->Rinf[i] * o->Rinf[i] * o->Rinf[i]);
+    o->Rmed[i] = o->Rmed[i] / (o->Rsup[i] * o->Rsup[i] - o->Rinf[i] * o->Rinf[i]);
+    o->Surf[i] = PI * (o->Rsup[i] * o->Rsup[i] - o->Rinf[i] * o->Rinf[i]) / (double)ns;
+    o->InvRmed[i] = 1.0 / o->Rmed[i];
+    o->InvSurf[i] = 1.0 / o->Surf[i];
+    o->InvDiffRsup[i] = 1.0 / (o->Rsup[i] - o->Rinf[i]);
+    o->InvRinf[i] = 1.0 / o->Rinf[i];
+    o->OmegaInv[i] = pow(o->Rmed[i], 1.5);
+    o->Rmed2[i] = o->Rmed[i] * o->Rmed[i];
+  }
+  o->Rinf[nr] = o->Radii[nr];
+  for (i = 1; i < nr; i++) o->InvDiffRmed[i] = 1.0 / (o->Rmed[i] - o->Rmed[i - 1]);
+}
+
+/* ---------------- 1D profiles (Theo.c, Viscosity.c) ---------------- */
+/* FViscosity, Viscosity.c:26-54 (single rank: GlobalRmed == Rmed) */
+static double o_fviscosity(oracle_ctx *o, double rad) {
+  const thorin_params *p = &o->p;
+  double viscosity, rmin, rmax, scale, alpha;
+  int i = 0;
+  viscosity = p->viscosity;
+  if (p->viscosity_alpha) {
+    while (o->GlobalRmed[i] < rad) i++;
+    viscosity = p->alphaviscosity * o->globcsvec[i] * o->globcsvec[i] * pow(rad, 1.5);
+  }
+  if (p->alpha_flock_on) {
+    while (o->GlobalRmed[i] < rad) i++;
+    alpha = (p->alphamri - p->alphadead) * (1.0 - tanh((p->tmri - o->globtempervec[i] * T2SI) / p->twidth)) / 2.0 + p->alphadead;
+    viscosity = alpha * o->globcsvec[i] * o->globcsvec[i] * pow(rad, 1.5);
+  }
+  rmin = p->cavityradius - p->cavitywidth * p->aspectratio;
+  rmax = p->cavityradius + p->cavitywidth * p->aspectratio;
+  scale = 1.0 + (o->st.physical_time - o->st.physical_time_initial) * p->lambdadoubling;
+  rmin *= scale;
+  rmax *= scale;
+  if (rad < rmin) viscosity *= p->cavityratio;
+  if ((rad >= rmin) && (rad <= rmax)) viscosity *= exp((rmax - rad) / (rmax - rmin) * log(p->cavityratio));
+  return viscosity;
+}
+
+/* AspectRatio, Viscosity.c:56-71 */
+static double o_aspectratio(oracle_ctx *o, double rad) {
+  const thorin_params *p = &o->p;
+  double ar = p->aspectratio, rmin, rmax, scale;
+  rmin = p->transitionradius - p->transitionwidth * p->aspectratio;
+  rmax = p->transitionradius + p->transitionwidth * p->aspectratio;
+  scale = 1.0 + (o->st.physical_time - o->st.physical_time_initial) * p->lambdadoubling;
+  rmin *= scale;
+  rmax *= scale;
+  if (rad < rmin) ar *= p->transitionratio;
+  if ((rad >= rmin) && (rad <= rmax)) ar *= exp((rmax - rad) / (rmax - rmin) * log(p->transitionratio));
+  return ar;
+}
+
+/* mpi_make1Dprofile single-rank path (mpiTasks.c:12-36): sequential ring mean */
+static void o_ring_mean(oracle_ctx *o, const double *f, double *out) {
+  int i, j;
+  for (i = 0; i < o->nr; i++) {
+    double s = 0.;
+    for (j = 0; j < o->ns; j++) s += f[i * o->ns + j];
+    s /= (double)o->ns;
+    out[i] = s;
+  }
+}
+
+/* ---------------- EOS fields (SourceEuler.c:715-792) ---------------- */
+static void o_sound_speed(oracle_ctx *o, const double *rho, const double *energy) {
+  int i, j, l, ns = o->ns;
+  double *cs = o->fld[THORIN_F_SOUNDSPEED];
+  double g = o->p.adiabind;
+  for (i = 0; i < o->nr; i++)
+    for (j = 0; j < ns; j++) {
+      l = j + i * ns;
+      if (!o->p.energy_eq)
+        cs[l] = o_aspectratio(o, o->Rmed[i]) * sqrt(G_ * 1.0 / o->Rmed[i]) * pow(o->Rmed[i], o->p.flaringindex);
+      else
+        cs[l] = sqrt(g * (g - 1.0) * fabs(energy[l]) / rho[l]);
+    }
+  o_ring_mean(o, cs, o->globcsvec);
+}
+
+static void o_pressure(oracle_ctx *o, const double *rho, const double *energy) {
+  int i, j, l, ns = o->ns;
+  double *pr = o->fld[THORIN_F_PRESSURE], *cs = o->fld[THORIN_F_SOUNDSPEED];
+  for (i = 0; i < o->nr; i++)
+    for (j = 0; j < ns; j++) {
+      l = j + i * ns;
+      if (!o->p.energy_eq) pr[l] = rho[l] * cs[l] * cs[l];
+      else pr[l] = (o->p.adiabind - 1.0) * energy[l];
+    }
+}
+
+static void o_temperature(oracle_ctx *o, const double *rho, const double *energy) {
+  int i, j, l, ns = o->ns;
+  double *t = o->fld[THORIN_F_TEMPER], *pr = o->fld[THORIN_F_PRESSURE];
+  double tmp1, tmp2;
+  for (i = 0; i < o->nr; i++)
+    for (j = 0; j < ns; j++) {
+      l = j + i * ns;
+      tmp2 = 1.0 / (rho[l] * GASCONST);
+      if (o->p.energy_eq) tmp1 = energy[l] * MOLWEIGHT * (o->p.adiabind - 1.0);
+      else tmp1 = MOLWEIGHT * pr[l];
+      t[l] = tmp1 * tmp2;
+    }
+  o_ring_mean(o, t, o->globtempervec);
+}
+
+/* ---------------- creation / initialisation ---------------- */
+int oracle_create(const thorin_params *p, oracle_ctx **out) {
+  oracle_ctx *o = (oracle_ctx *)calloc(1, sizeof(oracle_ctx));
+  int k, n;
+  if (!o) return -1;
+  o->p = *p;
+  o->nr = p->nrad;
+  o->ns = p->nsec;
+  if (o->nr < 4 || o->ns < 2) { free(o); return -2; }
+  for (k = 0; k < NW; k++) o->fld[k] = fieldalloc(o->nr, o->ns);
+  n = o->nr + 2;
+#define A1(x) o->x = (double *)calloc((size_t)n + 1, sizeof(double))
+  A1(Radii); A1(Rinf); A1(Rsup); A1(Rmed); A1(Surf); A1(InvRmed); A1(InvSurf); A1(InvDiffRmed);
+  A1(InvDiffRsup); A1(InvRinf); A1(OmegaInv); A1(Rmed2); A1(GlobalRmed); A1(SigmaMed); A1(SigmaInf);
+  A1(EnergyMed); A1(VthetaMed); A1(WaveKiller); A1(globcsvec); A1(globtempervec); A1(Vmoy);
+  A1(VMed);
+#undef A1
+  o->Vresidual = (double *)calloc((size_t)o->ns + 1, sizeof(double));
+  o->Nshift = (int *)calloc((size_t)n, sizeof(int));
+  o->sqrt_adiabind_inv = 1.0 / sqrt(p->adiabind);
+  o->CV = GASCONST / (MOLWEIGHT * (p->adiabind - 1.0));
+  o->st.dt_stellar = 1.0e38;
+  o->st.omega_frame = p->omegaframe;
+  o->st.sor_first = 1;
+  o->st.mass_taper = 0.0;
+  o_fill_1d(o);
+  /* InitComputeAccel, SideEuler.c:101-118 */
+  {
+    int i, j, l;
+    double *ab = o->fld[W_ABS], *od = o->fld[W_ORD];
+    for (i = 0; i < o->nr; i++)
+      for (j = 0; j < o->ns; j++) {
+        l = j + i * o->ns;
+        ab[l] = o->Rmed[i] * cos(2.0 * PI * (double)j / (double)o->ns);
+        od[l] = o->Rmed[i] * sin(2.0 * PI * (double)j / (double)o->ns);
+      }
+  }
+  /* SetWaveKillingZones, SideEuler.c:341-362 */
+  if (p->damping_on || p->pebbles) {
+    double DRin = p->rmin * p->dampingrminfrac, DRout = p->rmax * p->dampingrmaxfrac, damp;
+    double tauin = p->dampingperiodfrac * 2.0 * PI * pow(p->rmin, 1.5);
+    double tauout = p->dampingperiodfrac * 2.0 * PI * pow(DRout, 1.5);
+    int i;
+    for (i = 0; i < o->nr; i++) {
+      o->WaveKiller[i] = 0.0;
+      if (o->Rmed[i] < DRin) { damp = (DRin - o->Rmed[i]) / (DRin - p->rmin); o->WaveKiller[i] = damp * damp / tauin; }
+      if (o->Rmed[i] > DRout) { damp = (o->Rmed[i] - DRout) / (p->rmax - DRout); o->WaveKiller[i] = damp / tauout; }
+    }
+  }
+  /* InitLabel, Init.c:56-70 */
+  {
+    int i, j;
+    for (i = 0; i <= o->nr; i++)
+      for (j = 0; j < o->ns; j++)
+        o->fld[THORIN_F_LABEL][j + i * o->ns] = (o->Rmed[i] - p->rmin) / (p->rmax - p->rmin);
+  }
+  *out = o;
+  return 0;
+}
+
+void oracle_destroy(oracle_ctx *o) {
+  int k;
+  if (!o) return;
+  for (k = 0; k < NW; k++) free(o->fld[k]);
+  free(o->Radii); free(o->Rinf); free(o->Rsup); free(o->Rmed); free(o->Surf); free(o->InvRmed);
+  free(o->InvSurf); free(o->InvDiffRmed); free(o->InvDiffRsup); free(o->InvRinf); free(o->OmegaInv);
+  free(o->Rmed2); free(o->GlobalRmed); free(o->SigmaMed); free(o->SigmaInf); free(o->EnergyMed);
+  free(o->VthetaMed); free(o->WaveKiller); free(o->globcsvec); free(o->globtempervec); free(o->Vmoy);
+  free(o->VMed); free(o->Vresidual); free(o->Nshift);
+  free(o);
+}
+
+/* InitializeFromFile route: Init.c:86-108 then cs, P, T (Init.c:131-133).
+ * RefillSigma / RefillEnergy / FillVtheta follow Theo.c:33-57, 82-100, 132-150. */
+int oracle_init_from_arrays(oracle_ctx *o, const double *dens, const double *vrad,
+                            const double *vtheta, const double *temper) {
+  int nr = o->nr, ns = o->ns, i, j, l;
+  double *rho = o->fld[THORIN_F_DENS], *e = o->fld[THORIN_F_ENERGY];
+  memcpy(rho, dens, sizeof(double) * nr * ns);
+  memcpy(o->fld[THORIN_F_VRAD], vrad, sizeof(double) * nr * ns);
+  memcpy(o->fld[THORIN_F_VTHETA], vtheta, sizeof(double) * nr * ns);
+  if (o->p.energy_eq) {
+    memcpy(e, temper, sizeof(double) * nr * ns);
+    for (i = 0; i < nr; i++)
+      for (j = 0; j < ns; j++) { l = j + i * ns; e[l] = e[l] * rho[l] / (o->p.adiabind - 1.0); }
+  }
+  for (i = 0; i < nr; i++) {
+    double moy = 0.0;
+    for (j = 0; j < ns; j++) moy += rho[j + i * ns];
+    moy /= (double)ns;
+    o->SigmaMed[i] = moy;
+  }
+  o->SigmaInf[0] = o->SigmaMed[0];
+  for (i = 1; i < nr; i++)
+    o->SigmaInf[i] = (o->SigmaMed[i - 1] * (o->Rmed[i] - o->Rinf[i]) + o->SigmaMed[i] * (o->Rinf[i] - o->Rmed[i - 1])) / (o->Rmed[i] - o->Rmed[i - 1]);
+  if (o->p.energy_eq)
+    for (i = 0; i < nr; i++) {
+      double moy = 0.0;
+      for (j = 0; j < ns; j++) moy += e[j + i * ns];
+      moy /= (double)ns;
+      o->EnergyMed[i] = moy;
+    }
+  if (o->p.damp_init)
+    for (i = 0; i < nr; i++) {
+      double moy = 0.0, vtcorr = o->Rmed[i] * o->st.omega_frame;
+      for (j = 0; j < ns; j++) moy += o->fld[THORIN_F_VTHETA][j + i * ns] + vtcorr;
+      moy /= (double)ns;
+      o->VthetaMed[i] = moy;
+    }
+  o_sound_speed(o, rho, e);
+  o_pressure(o, rho, e);
+  o_temperature(o, rho, e);
+  /* main.c:168-170 recomputes the sound speed once more */
+  if (o->p.energy_eq) o_sound_speed(o, rho, e);
+  return 0;
+}
+
+/* ---------------- ConditionCFL (SourceEuler.c:589-712) ---------------- */
+static double o_min2(double a, double b) { return (b < a) ? b : a; }   /* SourceEuler.c:177-182 */
+static double o_max2(double a, double b) { return (b > a) ? b : a; }   /* SourceEuler.c:184-189 */
+
+static int o_condition_cfl_core(oracle_ctx *o, double deltaT, double *newdt_out) {
+  int i, j, l, ns = o->ns, nr = o->nr, lip, ljp;
+  double invdt1, invdt2, invdt3, invdt4, invdt5 = 1e-30, cs, newdt, dt;
+  double dxrad, dxtheta, dvr, dvt, visc = 0.;
+  const double *vt = o->fld[THORIN_F_VTHETA], *vr = o->fld[THORIN_F_VRAD];
+  const double *soundspeed = o->fld[THORIN_F_SOUNDSPEED];
+  newdt = 1e30;
+  for (i = 0; i < nr; i++) {
+    o->Vmoy[i] = 0.0;
+    for (j = 0; j < ns; j++) o->Vmoy[i] += vt[j + i * ns];
+    o->Vmoy[i] /= (double)ns;
+  }
+  for (i = 1; i < nr; i++) {
+    dxrad = o->Rsup[i] - o->Rinf[i];
+    dxtheta = o->Rmed[i] * 2.0 * PI / (double)ns;
+    for (j = 0; j < ns; j++) {
+      l = j + i * ns;
+      o->Vresidual[j] = o->p.fast_transport ? vt[l] - o->Vmoy[i] : vt[l];
+    }
+    o->Vresidual[ns] = o->Vresidual[0];
+    if (o->p.viscosity_alpha || (o->p.viscosity != 0.0)) visc = o_fviscosity(o, o->Rmed[i]);
+    for (j = 0; j < ns; j++) {
+      l = j + i * ns;
+      lip = l + ns;
+      ljp = l + 1;
+      if (j == ns - 1) ljp = i * ns;
+      cs = soundspeed[l];
+      invdt1 = cs / (o_min2(dxrad, dxtheta));
+      invdt2 = fabs(vr[l]) / dxrad;
+      invdt3 = fabs(o->Vresidual[j]) / dxtheta;
+      dvr = vr[lip] - vr[l];
+      dvt = vt[ljp] - vt[l];
+      if (dvr >= 0.0) dvr = 1e-10; else dvr = -dvr;
+      if (dvt >= 0.0) dvt = 1e-10; else dvt = -dvt;
+      invdt4 = o_max2(dvr / dxrad, dvt / dxtheta);
+      invdt4 *= 4.0 * CVNR * CVNR;
+      if (o->p.viscosity_alpha || (o->p.viscosity != 0.0))
+        invdt5 = visc * 4. / pow(o_min2(dxrad, dxtheta), 2);
+      dt = CFLSECURITY / sqrt(invdt1 * invdt1 + invdt2 * invdt2 + invdt3 * invdt3 + invdt4 * invdt4 + invdt5 * invdt5 + o->invdtreb_sq);
+      if (dt < newdt) newdt = dt;
+    }
+  }
+  for (i = 0; i < nr - 1; i++) {
+    dt = 2.0 * PI * CFLSECURITY / (double)ns / fabs(o->Vmoy[i] * o->InvRmed[i] - o->Vmoy[i + 1] * o->InvRmed[i + 1]);
+    if (dt < newdt) newdt = dt;
+  }
+  if (newdt > o->st.dt_stellar) {
+    double a = 1.0 * o->st.dt_stellar, b = o->p.minimumstep;
+    double mx = a > b ? a : b;
+    newdt = newdt < mx ? newdt : mx;
+  }
+  newdt = newdt < o->p.maximumstep ? newdt : o->p.maximumstep;
+  if (o->st.cfl_count == 0) {
+    if (o->p.firststep > 0.0) newdt = o->p.firststep;
+    o->st.cfl_count = 1;
+  }
+  if (deltaT < newdt) newdt = deltaT;
+  *newdt_out = newdt;
+  return (int)(ceil(deltaT / newdt));
+}
+
+int oracle_condition_cfl(oracle_ctx *o, double deltaT, double invdtreb_sq, double *newdt) {
+  o->invdtreb_sq = invdtreb_sq;
+  return o_condition_cfl_core(o, deltaT, newdt);
+}
+
+/* MinStepForRebound, ReboundInterface.c:360-385 (pairs of planets only) */
+static double o_minstep_rebound(const thorin_planets *sys) {
+  int i, j;
+  double dt = 1.e30, dtnew, r2, rv;
+  for (i = 0; i < sys->nb; i++)
+    for (j = i + 1; j < sys->nb; j++) {
+      r2 = (sys->x[i] - sys->x[j]) * (sys->x[i] - sys->x[j]) + (sys->y[i] - sys->y[j]) * (sys->y[i] - sys->y[j]) +
+           (sys->z[i] - sys->z[j]) * (sys->z[i] - sys->z[j]);
+      rv = (sys->x[i] - sys->x[j]) * (sys->vx[i] - sys->vx[j]) + (sys->y[i] - sys->y[j]) * (sys->vy[i] - sys->vy[j]) +
+           (sys->z[i] - sys->z[j]) * (sys->vz[i] - sys->vz[j]);
+      dtnew = fabs(r2 / rv);
+      if (dtnew < dt) dt = dtnew;
+    }
+  dt *= 0.25;
+  dt *= dt;
+  return 1.0 / dt;
+}
+
+/* ---------------- forces: ThicknessSmoothing + FillForcesArrays ---------------- */
+/* ThicknessSmoothing, Force.c:75-102 */
+static double o_thickness_smoothing(oracle_ctx *o, double x, double y) {
+  int i, j;
+  double r = sqrt(x * x + y * y), csavrg, H, sm = 0.0;
+  const double *cs = o->fld[THORIN_F_SOUNDSPEED];
+  if (r >= o->Rinf[0] && r <= o->Rsup[o->nr - 1]) {
+    i = 0;
+    while (o->Rsup[i] < r) i++;
+    csavrg = 0.0;
+    for (j = 0; j < o->ns; j++) csavrg += cs[j + i * o->ns];
+    csavrg /= (double)o->ns;
+    H = csavrg * o->OmegaInv[i] * o->sqrt_adiabind_inv;
+    sm = o->p.thicknesssmoothing * H;
+  }
+  return sm;   /* MPI_MAX over one rank */
+}
+
+/* FillForcesArrays, Pframeforce.c:43-510 (gas part; the pebble branches are
+ * added with the pebble fluid). */
+static void o_fill_forces(oracle_ctx *o, thorin_planets *sys) {
+  int i, j, l, nr = o->nr, ns = o->ns, k, npl = sys->nb, m, Nvert;
+  double *agr = o->fld[THORIN_F_AGR], *agt = o->fld[THORIN_F_AGT];
+  const double *rho = o->fld[THORIN_F_DENS], *cs = o->fld[THORIN_F_SOUNDSPEED];
+  const double *abs_ = o->fld[W_ABS], *ord = o->fld[W_ORD];
+  double xpl, ypl, zpl, mpl, dpl, invdpl3, smoothing;
+  double d2, d, H, H2, zmax, deltaz;
+  double rsm[THORIN_MAXPLANETS], rsm2[THORIN_MAXPLANETS], sigma, znode, znode2, denom, ax, ay, ar, at;
+  double rst, rst2[THORIN_MAXPLANETS], rst3[THORIN_MAXPLANETS], rst4[THORIN_MAXPLANETS];
+  double taper, omegainv, r2, Rsup_Rinf2, r2chk_1, r2chk_2, integstar, sum, tmp;
+  double cs_avr, integstar_avr, sigma_avr, H_avr, x, y, mcell, axindir, ayindir;
+  double indir_px, indir_py, axstar, aystar;
+  int IntegrateWithPlanet[THORIN_MAXPLANETS], IntegrateLocally;
+  double hillcut[THORIN_MAXPLANETS], rhill[THORIN_MAXPLANETS], smoothing2[THORIN_MAXPLANETS];
+  double Hpl[THORIN_MAXPLANETS], s2[THORIN_MAXPLANETS], integpl[THORIN_MAXPLANETS];
+  double rmorevert2[THORIN_MAXPLANETS], axpl[THORIN_MAXPLANETS], aypl[THORIN_MAXPLANETS];
+  const double mt = o->st.mass_taper;
+  for (i = 0; i < (nr + 1) * ns; i++) { agr[i] = 0.0; agt[i] = 0.0; }
+  axstar = 0.0;
+  aystar = 0.0;
+  for (k = 0; k < npl; k++) {
+    axpl[k] = 0.0; aypl[k] = 0.0; hillcut[k] = 1.0;
+    xpl = sys->x[k]; ypl = sys->y[k]; zpl = sys->z[k]; mpl = sys->mass[k];
+    dpl = sqrt(xpl * xpl + ypl * ypl + zpl * zpl);
+    rhill[k] = dpl * pow((1.0 / 3.0 * mpl), 1.0 / 3.0);
+    smoothing = o_thickness_smoothing(o, xpl, ypl);
+    Hpl[k] = smoothing / o->p.thicknesssmoothing;
+    smoothing2[k] = smoothing * smoothing;
+    rst = 0.5 * rhill[k];
+    rst2[k] = rst * rst;
+    rst3[k] = rst2[k] * rst;
+    rst4[k] = rst3[k] * rst;
+    rsm[k] = 0.05 * rhill[k];
+    rsm2[k] = rsm[k] * rsm[k];
+    rmorevert2[k] = 2.0 * rhill[k];
+    rmorevert2[k] *= rmorevert2[k];
+    if (o->p.indirect_term) {
+      invdpl3 = 1.0 / (dpl * dpl * dpl);
+      mpl *= mt;
+      axstar += mpl * invdpl3 * xpl;
+      aystar += mpl * invdpl3 * ypl;
+    }
+  }
+  indir_px = -axstar;
+  indir_py = -aystar;
+  axstar = 0.0;
+  aystar = 0.0;
+  for (i = 0; i < nr; i++) {
+    omegainv = o->OmegaInv[i];
+    r2 = o->Rmed2[i];
+    Rsup_Rinf2 = o->Rsup[i] - o->Rinf[i];
+    Rsup_Rinf2 *= Rsup_Rinf2;
+    Nvert = 10;
+    cs_avr = 0.0;
+    for (j = 0; j < ns; j++) cs_avr += cs[j + i * ns];
+    cs_avr /= (double)ns;
+    H_avr = cs_avr * omegainv * o->sqrt_adiabind_inv;
+    H2 = H_avr * H_avr;
+    zmax = 3.0 * H_avr;
+    deltaz = zmax / (double)Nvert;
+    sigma_avr = 0.0;
+    integstar_avr = 0.0;
+    for (m = 1; m <= Nvert; m++) {
+      znode = ((double)m - 0.5) * deltaz;
+      znode2 = znode * znode;
+      sum = exp(-(0.5 * znode2 / H2));
+      sigma_avr += sum;
+      d2 = r2 + znode2;
+      d = sqrt(d2);
+      denom = d * d2;
+      integstar_avr += sum / denom;
+    }
+    for (j = 0; j < ns; j++) {
+      IntegrateLocally = 0;
+      Nvert = 10;
+      ax = 0.0;
+      ay = 0.0;
+      l = j + i * ns;
+      x = abs_[l];
+      y = ord[l];
+      mcell = rho[l] * o->Surf[i];
+      H = cs[l] * omegainv * o->sqrt_adiabind_inv;
+      H2 = H * H;
+      if (fabs(H - H_avr) / H_avr > 0.05) IntegrateLocally = 1;
+      for (k = 0; k < npl; k++) {
+        xpl = sys->x[k]; ypl = sys->y[k]; zpl = sys->z[k];
+        s2[k] = (x - xpl) * (x - xpl) + (y - ypl) * (y - ypl);
+        r2chk_1 = 10.0 * Hpl[k];
+        r2chk_1 *= r2chk_1;
+        r2chk_2 = 5.0 * rhill[k];
+        r2chk_2 *= r2chk_2;
+        if (mt == 1.0 && (s2[k] < r2chk_1 || s2[k] < r2chk_2)) {
+          IntegrateLocally = 1;
+          IntegrateWithPlanet[k] = 1;
+          integpl[k] = 0.0;
+          d2 = s2[k] + zpl * zpl;
+          if (o->p.exclude_hill) hillcut[k] = 1.0 / (exp(-(sqrt(d2) / rhill[k] - o->p.hillcut) / o->p.hillcut * 10.0) + 1.0);
+        } else {
+          IntegrateWithPlanet[k] = 0;
+        }
+        if (s2[k] < rmorevert2[k]) Nvert = 30;
+      }
+      if (IntegrateLocally) {
+        zmax = 3.0 * H;
+        deltaz = zmax / (double)Nvert;
+        sigma = 0.0;
+        integstar = 0.0;
+        for (m = 1; m <= Nvert; m++) {
+          znode = ((double)m - 0.5) * deltaz;
+          znode2 = znode * znode;
+          sum = exp(-(0.5 * znode2 / H2));
+          sigma += sum;
+          d2 = r2 + znode2;
+          d = sqrt(d2);
+          denom = d * d2;
+          integstar += sum / denom;
+          for (k = 0; k < npl; k++) {
+            if (IntegrateWithPlanet[k]) {
+              zpl = sys->z[k];
+              d2 = s2[k] + (znode - zpl) * (znode - zpl);
+              if (d2 < rst2[k]) {
+                d = sqrt(d2);
+                taper = -3.0 * d2 * d2 / rst4[k] + 4.0 * d * d2 / rst3[k];
+              } else taper = 1.0;
+              d2 += (rsm2[k] > Rsup_Rinf2 ? rsm2[k] : Rsup_Rinf2);
+              d = sqrt(d2);
+              denom = d * d2;
+              integpl[k] += sum / denom * taper;
+              if (zpl < -0.05 * H || zpl > 0.05 * H) {
+                d2 = s2[k] + (-znode - zpl) * (-znode - zpl);
+                if (d2 < rst2[k]) {
+                  d = sqrt(d2);
+                  taper = -3.0 * d2 * d2 / rst4[k] + 4.0 * d * d2 / rst3[k];
+                } else taper = 1.0;
+                d2 += (rsm2[k] > Rsup_Rinf2 ? rsm2[k] : Rsup_Rinf2);
+                d = sqrt(d2);
+                denom = d * d2;
+                integpl[k] += sum / denom * taper;
+              }
+            }
+          }
+        }
+      } else {
+        sigma = sigma_avr;
+        integstar = integstar_avr;
+      }
+      tmp = integstar / sigma;
+      ar = -o->Rmed[i] * tmp;
+      agr[l] += ar;
+      axstar += mcell * tmp * x;
+      aystar += mcell * tmp * y;
+      for (k = 0; k < npl; k++) {
+        xpl = sys->x[k]; ypl = sys->y[k]; zpl = sys->z[k]; mpl = sys->mass[k];
+        if (IntegrateWithPlanet[k]) {
+          if (zpl < -0.05 * H || zpl > 0.05 * H) tmp = (x - xpl) * integpl[k] / (2.0 * sigma);
+          else tmp = (x - xpl) * integpl[k] / sigma;
+          ax += -mpl * tmp;
+          if (o->p.exclude_hill) tmp *= hillcut[k];
+          axpl[k] += mcell * tmp;
+          if (zpl < -0.05 * H || zpl > 0.05 * H) tmp = (y - ypl) * integpl[k] / (2.0 * sigma);
+          else tmp = (y - ypl) * integpl[k] / sigma;
+          ay += -mpl * tmp;
+          if (o->p.exclude_hill) tmp *= hillcut[k];
+          aypl[k] += mcell * tmp;
+        } else {
+          mpl *= mt;
+          d2 = s2[k] + zpl * zpl + smoothing2[k];
+          d = sqrt(d2);
+          denom = d * d2;
+          tmp = (x - xpl) / denom;
+          ax += -mpl * tmp;
+          if (o->p.exclude_hill) tmp *= hillcut[k];
+          axpl[k] += mcell * tmp;
+          tmp = (y - ypl) / denom;
+          ay += -mpl * tmp;
+          if (o->p.exclude_hill) tmp *= hillcut[k];
+          aypl[k] += mcell * tmp;
+        }
+      }
+      ar = (x * ax + y * ay) * o->InvRmed[i];
+      at = (x * ay - y * ax) * o->InvRmed[i];
+      agr[l] += ar;
+      agt[l] += at;
+    }
+  }
+  for (k = 0; k < npl; k++) { sys->ax[k] = axpl[k]; sys->ay[k] = aypl[k]; }
+  axindir = -axstar;
+  ayindir = -aystar;
+  if (o->p.indirect_term) { axindir += indir_px; ayindir += indir_py; }
+  for (i = 0; i < nr; i++)
+    for (j = 0; j < ns; j++) {
+      l = j + i * ns;
+      x = abs_[l];
+      y = ord[l];
+      ar = (x * axindir + y * ayindir) * o->InvRmed[i];
+      at = (x * ayindir - y * axindir) * o->InvRmed[i];
+      agr[l] += ar;
+      agt[l] += at;
+    }
+  /* IndirectTerm (Pframeforce.c:468-470), used by AdvanceSystemFromDisk */
+  o->indirect_x = -axstar;
+  o->indirect_y = -aystar;
+}
+
+/* DampingTW04, Pframeforce.c:571-603 */
+static double o_damping_tw04(oracle_ctx *o, double m, double x, double y, double z, double vz) {
+  double r, Omega, cs4inv, damp = 0.0, rhoavr = 0.0, csavr = 0.0;
+  const double *rho = o->fld[THORIN_F_DENS], *cs = o->fld[THORIN_F_SOUNDSPEED];
+  int i, j;
+  r = sqrt(x * x + y * y);
+  if ((o->p.verticaldamping > 0.0) && (r >= o->Rinf[0] && r <= o->Rsup[o->nr - 1])) {
+    i = 0;
+    while (o->Rsup[i] < r) i++;
+    for (j = 0; j < o->ns; j++) { rhoavr += rho[j + i * o->ns]; csavr += cs[j + i * o->ns]; }
+    csavr /= (double)o->ns;
+    rhoavr /= (double)o->ns;
+    cs4inv = pow(csavr, -4.0);
+    Omega = pow(r, -1.5);
+    damp = o->p.verticaldamping * m * rhoavr * Omega * cs4inv * (-2.176 * vz - 0.871 * z * Omega);
+  }
+  return damp;
+}
+
+/* AdvanceSystemFromDisk, Pframeforce.c:514-566 (torque log is host output) */
+static void o_advance_from_disk(oracle_ctx *o, thorin_planets *sys, double dt) {
+  int k;
+  for (k = 0; k < sys->nb; k++) {
+    if (sys->feeldisk[k]) {
+      double damp = o_damping_tw04(o, sys->mass[k], sys->x[k], sys->y[k], sys->z[k], sys->vz[k]);
+      sys->vx[k] += dt * sys->ax[k];
+      sys->vy[k] += dt * sys->ay[k];
+      sys->vz[k] += dt * (sys->az[k] + damp);
+      sys->vx[k] += dt * o->indirect_x;
+      sys->vy[k] += dt * o->indirect_y;
+    }
+  }
+}
+
+/* ---------------- boundaries (SideEuler.c:368-419, 490-503) ---------------- */
+static void o_boundary(oracle_ctx *o, double step) {
+  const thorin_params *p = &o->p;
+  int i, j, l, ns = o->ns;
+  double *vrad = o->fld[THORIN_F_VRAD], *vtheta = o->fld[THORIN_F_VTHETA];
+  double *rho = o->fld[THORIN_F_DENS], *energy = o->fld[THORIN_F_ENERGY];
+  double vrad0, vtheta0 = 0.0, rho0 = 0.0, energy0 = 0.0, DRin, DRout, lambda;
+  if (!p->damping_on) return;
+  DRin = p->rmin * p->dampingrminfrac;
+  DRout = p->rmax * p->dampingrmaxfrac;
+  for (i = 0; i < o->nr; i++) {
+    if (o->Rmed[i] < DRin || o->Rmed[i] > DRout) {
+      vrad0 = 0.0;
+      if (p->damp_init) {
+        rho0 = o->SigmaMed[i];
+        energy0 = o->EnergyMed[i];
+        vtheta0 = o->VthetaMed[i] - o->Rmed[i] * o->st.omega_frame;
+      }
+      if (p->damp_flux) {
+        vrad0 = -p->gasflux / (2.0 * PI * o->Rinf[i] * o->SigmaInf[i]);
+        rho0 = o->SigmaMed[i];
+      }
+      lambda = o->WaveKiller[i] * step;
+      for (j = 0; j < ns; j++) {
+        l = j + i * ns;
+        vrad[l] = (vrad[l] + lambda * vrad0) / (1.0 + lambda);
+        if (p->damp_init) {
+          rho[l] = (rho[l] + lambda * rho0) / (1.0 + lambda);
+          vtheta[l] = (vtheta[l] + lambda * vtheta0) / (1.0 + lambda);
+          if (p->energy_eq) energy[l] = (energy[l] + lambda * energy0) / (1.0 + lambda);
+        }
+        if (p->damp_flux) rho[l] = (rho[l] + lambda * rho0) / (1.0 + lambda);
+      }
+    }
+  }
+}
+
+static int o_detect_crash(oracle_ctx *o, const double *f) {   /* SourceEuler.c:48-66 */
+  int l, n = o->nr * o->ns, b = 0;
+  for (l = 0; l < n; l++) if (f[l] < 0.0) b = 1;
+  return b;
+}
+
+/* ---------------- viscosity (Viscosity.c:85-260) ---------------- */
+static void o_div_stress(oracle_ctx *o, const double *vr, const double *vt, const double *rho) {
+  int i, j, l, nr = o->nr, ns = o->ns, lip, ljp, ljm, lim, ljmim;
+  double *Drr = o->fld[W_DRR], *Drp = o->fld[W_DRP], *Dpp = o->fld[W_DPP];
+  double *divergence = o->fld[THORIN_F_DIVV];
+  double *Trr = o->fld[THORIN_F_TAURR], *Trp = o->fld[THORIN_F_TAURP], *Tpp = o->fld[THORIN_F_TAUPP];
+  double dphi = 2.0 * M_PI / (double)ns, invdphi = 1.0 / dphi, onethird = 1.0 / 3.0, viscosity;
+  for (i = 0; i < nr; i++)
+    for (j = 0; j < ns; j++) {
+      l = j + i * ns; lip = l + ns; ljp = l + 1;
+      if (j == ns - 1) ljp = i * ns;
+      Drr[l] = (vr[lip] - vr[l]) * o->InvDiffRsup[i];
+      Dpp[l] = (vt[ljp] - vt[l]) * invdphi * o->InvRmed[i] + 0.5 * (vr[lip] + vr[l]) * o->InvRmed[i];
+      divergence[l] = (vr[lip] * o->Rsup[i] - vr[l] * o->Rinf[i]) * o->InvDiffRsup[i] * o->InvRmed[i];
+      divergence[l] += (vt[ljp] - vt[l]) * invdphi * o->InvRmed[i];
+    }
+  for (i = 1; i < nr; i++)
+    for (j = 0; j < ns; j++) {
+      l = j + i * ns; ljm = l - 1;
+      if (j == 0) ljm = i * ns + ns - 1;
+      lim = l - ns;
+      Drp[l] = 0.5 * (o->Rinf[i] * (vt[l] * o->InvRmed[i] - vt[lim] * o->InvRmed[i - 1]) * o->InvDiffRmed[i] +
+                      (vr[l] - vr[ljm]) * invdphi * o->InvRinf[i]);
+    }
+  for (i = 0; i < nr; i++) {
+    viscosity = o_fviscosity(o, o->Rmed[i]);
+    for (j = 0; j < ns; j++) {
+      l = j + i * ns;
+      Trr[l] = 2.0 * rho[l] * viscosity * (Drr[l] - onethird * divergence[l]);
+      Tpp[l] = 2.0 * rho[l] * viscosity * (Dpp[l] - onethird * divergence[l]);
+    }
+  }
+  for (i = 1; i < nr; i++) {
+    viscosity = o_fviscosity(o, o->Rinf[i]);
+    for (j = 0; j < ns; j++) {
+      l = j + i * ns; lim = l - ns; ljm = l - 1;
+      if (j == 0) ljm = i * ns + ns - 1;
+      ljmim = ljm - ns;
+      Trp[l] = 2.0 * 0.25 * (rho[l] + rho[lim] + rho[ljm] + rho[ljmim]) * viscosity * Drp[l];
+    }
+  }
+}
+
+static void o_visc_update(oracle_ctx *o, double *vr, double *vt, const double *rho, double DeltaT) {
+  int i, j, l, nr = o->nr, ns = o->ns, lip, ljp, ljm, lim;
+  const double *Trr = o->fld[THORIN_F_TAURR], *Trp = o->fld[THORIN_F_TAURP], *Tpp = o->fld[THORIN_F_TAUPP];
+  double dphi = 2.0 * M_PI / (double)ns, invdphi = 1.0 / dphi;
+  for (i = 1; i < nr - 1; i++)
+    for (j = 0; j < ns; j++) {
+      l = j + i * ns; lip = l + ns; ljp = l + 1;
+      if (j == ns - 1) ljp = i * ns;
+      ljm = l - 1;
+      if (j == 0) ljm = i * ns + ns - 1;
+      vt[l] += DeltaT * o->InvRmed[i] * ((o->Rsup[i] * Trp[lip] - o->Rinf[i] * Trp[l]) * o->InvDiffRsup[i] +
+                                         (Tpp[l] - Tpp[ljm]) * invdphi + 0.5 * (Trp[l] + Trp[lip])) /
+               (0.5 * (rho[l] + rho[ljm]));
+    }
+  for (i = 1; i < nr; i++)
+    for (j = 0; j < ns; j++) {
+      l = j + i * ns; lip = l + ns; lim = l - ns; ljp = l + 1;
+      if (j == ns - 1) ljp = i * ns;
+      vr[l] += DeltaT * o->InvRinf[i] * ((o->Rmed[i] * Trr[l] - o->Rmed[i - 1] * Trr[lim]) * o->InvDiffRmed[i] +
+                                         (Trp[ljp] - Trp[l]) * invdphi - 0.5 * (Tpp[l] + Tpp[lim])) /
+               (0.5 * (rho[l] + rho[lim]));
+    }
+}
+
+static void o_keplerian_edges(oracle_ctx *o, double *vt) {   /* Viscosity.c:221-260 */
+  const thorin_params *p = &o->p;
+  int j, nr = o->nr, ns = o->ns;
+  double VKepIn, VKepOut;
+  VKepIn = sqrt(G_ * 1.0 / o->Rmed[0]) *
+           sqrt(1.0 - pow(o_aspectratio(o, o->Rmed[0]), 2.0) * pow(o->Rmed[0], 2.0 * p->flaringindex) *
+                          (1. + p->sigmaslope - 2.0 * p->flaringindex));
+  VKepOut = sqrt(G_ * 1.0 / o->Rmed[nr - 1]) *
+            sqrt(1.0 - pow(o_aspectratio(o, o->Rmed[nr - 1]), 2.0) * pow(o->Rmed[nr - 1], 2.0 * p->flaringindex) *
+                           (1. + p->sigmaslope - 2.0 * p->flaringindex));
+  for (j = 0; j < ns; j++) vt[j] = VKepIn - o->Rmed[0] * o->st.omega_frame;
+  for (j = 0; j < ns; j++) vt[j + (nr - 1) * ns] = VKepOut - o->Rmed[nr - 1] * o->st.omega_frame;
+}
+
+/* ---------------- SubStep1 / SubStep2 (SourceEuler.c:319-504) ---------------- */
+static void o_substep1(oracle_ctx *o, double dt) {
+  const thorin_params *p = &o->p;
+  int i, j, l, lim, ljm, ljp, nr = o->nr, ns = o->ns;
+  const double *rho = o->fld[THORIN_F_DENS], *vrad = o->fld[THORIN_F_VRAD], *vtheta = o->fld[THORIN_F_VTHETA];
+  double *vradint = o->fld[THORIN_F_VRADINT], *vthetaint = o->fld[THORIN_F_VTHETAINT];
+  const double *press = o->fld[THORIN_F_PRESSURE], *agr = o->fld[THORIN_F_AGR], *agt = o->fld[THORIN_F_AGT];
+  double gradp, vt2, dxtheta, tmp, invdxtheta, supp_torque;
+  for (i = 1; i < nr; i++)
+    for (j = 0; j < ns; j++) {
+      l = j + i * ns; lim = l - ns; ljp = l + 1;
+      if (j == ns - 1) ljp = i * ns;
+      gradp = (press[l] - press[lim]) * 2.0 / (rho[l] + rho[lim]) * o->InvDiffRmed[i];
+      vt2 = vtheta[l] + vtheta[ljp] + vtheta[lim] + vtheta[ljp - ns];
+      vt2 = vt2 / 4.0 + o->Rinf[i] * o->st.omega_frame;
+      vt2 = vt2 * vt2;
+      tmp = -gradp + 0.5 * (agr[l] + agr[lim]) + vt2 * o->InvRinf[i];
+      vradint[l] = vrad[l] + dt * tmp;
+    }
+  for (i = 0; i < nr; i++) {
+    supp_torque = p->imposeddiskdrift * .5 * pow(o->Rmed[i], -2.5 + p->sigmaslope);
+    dxtheta = 2.0 * PI / (double)ns * o->Rmed[i];
+    invdxtheta = 1.0 / dxtheta;
+    for (j = 0; j < ns; j++) {
+      l = j + i * ns; ljm = l - 1;
+      if (j == 0) ljm = i * ns + ns - 1;
+      gradp = (press[l] - press[ljm]) * 2.0 / (rho[l] + rho[ljm]) * invdxtheta;
+      tmp = -gradp + 0.5 * (agt[l] + agt[ljm]);
+      vthetaint[l] = vtheta[l] + dt * tmp;
+      vthetaint[l] += dt * supp_torque;
+    }
+  }
+  o_div_stress(o, vradint, vthetaint, rho);
+  o_visc_update(o, vradint, vthetaint, rho, dt);
+  if ((!p->damping_on) || (p->damping_on && p->damp_vrad)) o_keplerian_edges(o, vthetaint);
+}
+
+static void o_substep2(oracle_ctx *o, double dt) {
+  int i, j, l, lim, lip, ljm, ljp, nr = o->nr, ns = o->ns;
+  const double *rho = o->fld[THORIN_F_DENS], *vrad = o->fld[THORIN_F_VRADINT], *vtheta = o->fld[THORIN_F_VTHETAINT];
+  double *qr = o->fld[W_RHOINT], *qt = o->fld[W_TEMPERINT];
+  double *vradnew = o->fld[THORIN_F_VRADNEW], *vthetanew = o->fld[THORIN_F_VTHETANEW];
+  const double *energy = o->fld[THORIN_F_ENERGY];
+  double *energyint = o->fld[THORIN_F_ENERGYINT];
+  double dxtheta, invdxtheta, dv;
+  for (i = 0; i < nr; i++)
+    for (j = 0; j < ns; j++) {
+      l = j + i * ns; lip = l + ns; ljp = l + 1;
+      if (j == ns - 1) ljp = i * ns;
+      dv = vrad[lip] - vrad[l];
+      qr[l] = (dv < 0.0) ? CVNR * CVNR * rho[l] * dv * dv : 0.0;
+      dv = vtheta[ljp] - vtheta[l];
+      qt[l] = (dv < 0.0) ? CVNR * CVNR * rho[l] * dv * dv : 0.0;
+    }
+  for (i = 1; i < nr; i++)
+    for (j = 0; j < ns; j++) {
+      l = j + i * ns; lim = l - ns;
+      vradnew[l] = vrad[l] - dt * 2.0 / (rho[l] + rho[lim]) * (qr[l] - qr[lim]) * o->InvDiffRmed[i];
+    }
+  for (i = 0; i < nr; i++) {
+    dxtheta = 2.0 * PI / (double)ns * o->Rmed[i];
+    invdxtheta = 1.0 / dxtheta;
+    for (j = 0; j < ns; j++) {
+      l = j + i * ns; ljm = l - 1;
+      if (j == 0) ljm = i * ns + ns - 1;
+      vthetanew[l] = vtheta[l] - dt * 2.0 / (rho[l] + rho[ljm]) * (qt[l] - qt[ljm]) * invdxtheta;
+    }
+  }
+  if (o->p.energy_eq)
+    for (i = 0; i < nr; i++) {
+      dxtheta = 2.0 * PI / (double)ns * o->Rmed[i];
+      invdxtheta = 1.0 / dxtheta;
+      for (j = 0; j < ns; j++) {
+        l = j + i * ns; lip = l + ns; ljp = l + 1;
+        if (j == ns - 1) ljp = i * ns;
+        energyint[l] = energy[l] - dt * qr[l] * (vrad[lip] - vrad[l]) * o->InvDiffRsup[i] -
+                       dt * qt[l] * (vtheta[ljp] - vtheta[l]) * invdxtheta;
+      }
+    }
+}
+
+/* ActualiseGas, SourceEuler.c:192-208: rings 0..nr inclusive */
+static void o_actualise(oracle_ctx *o, double *dst, const double *src) {
+  memcpy(dst, src, sizeof(double) * (size_t)(o->nr + 1) * o->ns);
+}
+
+/* ---------------- opacity (Opacity.c) ---------------- */
+#define TEMP(i, j) pow(kappa[j] / kappa[i] * pow(rho, a[j] - a[i]), 1.0 / (b[i] - b[j]))
+static double o_opacity_BL94(double rho, double T) {   /* Opacity.c:31-75 */
+  static const double kappa[] = {2.0e-4, 2.0e16, 0.1, 2.0e81, 1.0e-8, 1.0e-36, 1.5e20, 0.348};
+  static const double a[] = {0.0, 0.0, 0.0, 1.0, 2.0 / 3.0, 1.0 / 3.0, 1.0, 0.0};
+  static const double b[] = {2.0, -7.0, 0.5, -24.0, 3.0, 10.0, -2.5, 0.0};
+  double k_, k1, k2, k3, t1, t2;
+  k_ = 1.0e-12;
+  if (T <= TEMP(2, 3)) {
+    k1 = kappa[0] * pow(T, b[0]);
+    k2 = kappa[1] * pow(T, b[1]);
+    k3 = kappa[2] * pow(T, b[2]);
+    t1 = 1.0 / (k1 * k1) + 1.0 / (k2 * k2);
+    t1 = 1.0 / (t1 * t1);
+    t2 = pow(k3 / (1.0 + 1.0e22 * pow(T, -10.0)), 4.0);
+    k_ = pow(t1 + t2, 0.25);
+  } else if ((T > TEMP(2, 3)) && (T <= TEMP(3, 4))) k_ = kappa[3] * pow(rho, a[3]) * pow(T, b[3]);
+  else if ((T > TEMP(3, 4)) && (T <= TEMP(4, 5))) k_ = kappa[4] * pow(rho, a[4]) * pow(T, b[4]);
+  else if ((T > TEMP(4, 5)) && (T <= TEMP(5, 6))) k_ = kappa[5] * pow(rho, a[5]) * pow(T, b[5]);
+  else if ((T > TEMP(5, 6)) && (T <= TEMP(6, 7))) k_ = kappa[6] * pow(rho, a[6]) * pow(T, b[6]);
+  else k_ = kappa[7];
+  return k_;
+}
+#undef TEMP
+
+static double o_opacity_LP85(double rho, double T) {   /* Opacity.c:201-227 */
+  double k;
+  if (T < pow(2.0e-4 / 2.0e16, -1.0 / 9.0)) k = 2.0e-4 * T * T;
+  else if (T < pow(2.0e16 / 5.0e-3, 1.0 / 8.0)) k = 2.0e16 * pow(T, -7.0);
+  else if (T < pow(5.0e-3 / 2.0e34, -1.0 / 10.0) * pow(rho, 1.0 / 15.0)) k = 5.0e-3 * T;
+  else if (T < pow(2.0e34 / 2.0e-8, 1.0 / 12.0)) k = 2.0e34 * pow(rho, 2.0 / 3.0) * pow(T, -9.0);
+  else if (T < pow(2.0e-8 / 1.0e-36, 1.0 / 7.0) * pow(rho, 1.0 / 21.0)) k = 2.0e-8 * pow(rho, 2.0 / 3.0) * T * T * T;
+  else if (T < pow(1.0e-36 / 1.5e20, -1.0 / 12.5) * pow(rho, 4.0 / 75.0)) k = 1.0e-36 * pow(rho, 1.0 / 3.0) * pow(T, 10.0);
+  else k = 1.5e20 * rho * pow(T, -5.0 / 2.0);
+  return k;
+}
+
+static double o_opacity_ZHU12(double rho, double T) {  /* Opacity.c:145-196 */
+  double xlop, xlp, xlt, pre;
+  pre = R_STANDARD * 1.0e7 / MMW * rho * T;
+  xlp = log10(pre);
+  xlt = log10(T);
+  if (xlt < 2.23567 + 0.01899 * (xlp - 5.)) xlop = 1.5 * (xlt - 1.16331) - 0.736364;
+  else if (xlt < 2.30713 + 0.01899 * (xlp - 5.)) xlop = -3.53154212 * xlt + 8.767726 - (7.24786 - 8.767726) * (xlp - 5.) / 16.;
+  else if (xlt < (17.7 - 0.62 + 1.5 * 2.30713) / (1.5 + 5.832)) xlop = 1.5 * (xlt - 2.30713) + 0.62;
+  else if (xlt < (5.9398 + 17.7) / (5.832 + 2.129)) xlop = -5.832 * xlt + 17.7;
+  else if (xlt < (129.88071 + 5.9398 + (142.996475 - 129.88071) * 0.1 * (xlp + 4.)) / (2.129 + 42.98075)) xlop = 2.129 * xlt - 5.9398;
+  else if (xlt < (129.88071 + 15.0125 + (142.996475 - 129.88071) * 0.1 * (xlp + 4.)) / (4.0625 + 42.98075))
+    xlop = 129.88071 - 42.98075 * xlt + (142.996475 - 129.88071) * 0.1 * (xlp + 4.0);
+  else if (xlt < 3.28 + xlp / 4. * 0.12) xlop = -15.0125 + 4.0625 * xlt;
+  else if (xlt < 3.41 + 0.03328 * xlp / 4.) xlop = 58.9294 - 18.4808 * xlt + (61.6346 - 58.9294) * xlp / 4.;
+  else if (xlt < 3.76 + (xlp - 4) / 2. * 0.03) xlop = -12.002 + 2.90477 * xlt + (xlp - 4) / 4. * (13.9953 - 12.002);
+  else if (xlt < 4.07 + (xlp - 4) / 2. * 0.08) xlop = -39.4077 + 10.1935 * xlt + (xlp - 4) / 2. * (40.1719 - 39.4077);
+  else if (xlt < 5.3715 + (xlp - 6) / 2. * 0.5594) xlop = 17.5935 - 3.3647 * xlt + (xlp - 6) / 2. * (17.5935 - 15.7376);
+  else xlop = -0.48;
+  if (xlop < 3.586 * xlt - 16.85 && xlt < 4.) xlop = 3.586 * xlt - 16.85;
+  return pow(10., xlop);
+}
+
+static double o_opacity(oracle_ctx *o, double rho, double T) {
+  switch (o->p.opacity_law) {
+    case THORIN_OPA_LP85: return o_opacity_LP85(rho, T);
+    case THORIN_OPA_ZHU12: return o_opacity_ZHU12(rho, T);
+    case THORIN_OPA_CONST: return o->p.parametricopacity;
+    case THORIN_OPA_POWERLAW: return o->p.powerlawopacity * T * T;
+    default: return o_opacity_BL94(rho, T);
+  }
+}
+
+/* exported for closed-form tests of the laws */
+double oracle_opacity_law(int law, double rho_cgs, double T_K, double k0) {
+  switch (law) {
+    case THORIN_OPA_LP85: return o_opacity_LP85(rho_cgs, T_K);
+    case THORIN_OPA_ZHU12: return o_opacity_ZHU12(rho_cgs, T_K);
+    case THORIN_OPA_CONST: return k0;
+    case THORIN_OPA_POWERLAW: return k0 * T_K * T_K;
+    default: return o_opacity_BL94(rho_cgs, T_K);
+  }
+}
+
+/* ---------------- implicit radiative diffusion (EnergySources.c) ---------------- */
+static double o_flux_limiter(double s) {   /* EnergySources.c:684-696 */
+  double tmp;
+  if (s <= 2.0) { tmp = 3.0 + sqrt(9.0 + 10.0 * s * s); return 2.0 / tmp; }
+  tmp = 10.0 * s + 9.0 + sqrt(81.0 + 180.0 * s);
+  return 10.0 / tmp;
+}
+double oracle_flux_limiter(double s) { return o_flux_limiter(s); }
+
+static double o_tau_eff(oracle_ctx *o, double tau) {   /* EnergySources.c:701-711 */
+  if (o->p.stellar_irradiation) return 0.375 * tau + 0.5 + 0.25 / tau;
+  return 0.375 * tau + 0.25 * sqrt(3.0) + 0.25 / tau;
+}
+
+/* SuccessiveOverrelaxation, EnergySources.c:406-511 (single rank) */
+static int o_sor(oracle_ctx *o, double omega, int errcheck) {
+  int n, i, j, l, nr = o->nr, ns = o->ns, ipass, jchess, lip, lim, ljp, ljm, changed;
+  double resid, normres0 = 0.0, normres, dtemper;
+  double *T = o->fld[THORIN_F_TEMPER];
+  const double *Ml = o->fld[W_ML], *Mlip = o->fld[W_MLIP], *Mlim = o->fld[W_MLIM];
+  const double *Mljp = o->fld[W_MLJP], *Mljm = o->fld[W_MLJM], *RHS = o->fld[W_RHS];
+  (void)errcheck;
+  for (i = 1; i < nr - 1; i++)
+    for (j = 0; j < ns; j++) {
+      l = j + i * ns; lip = l + ns; lim = l - ns; ljp = l + 1;
+      if (j == ns - 1) ljp = i * ns;
+      ljm = l - 1;
+      if (j == 0) ljm = i * ns + ns - 1;
+      resid = T[l] * Ml[l] + T[lip] * Mlip[l] + T[lim] * Mlim[l] + T[ljp] * Mljp[l] + T[ljm] * Mljm[l] - RHS[l];
+      normres0 += fabs(resid);
+    }
+  for (n = 1; n <= SORMAXITERS; n++) {
+    normres = 0.0;
+    changed = 0;
+    for (ipass = 1; ipass <= 2; ipass++) {
+      jchess = (ipass == 1) ? 0 : 1;
+      for (i = 1; i < nr - 1; i++) {
+        for (j = jchess; j < ns; j += 2) {
+          l = j + i * ns; lip = l + ns; lim = l - ns; ljp = l + 1;
+          if (j == ns - 1) ljp = i * ns;
+          ljm = l - 1;
+          if (j == 0) ljm = i * ns + ns - 1;
+          resid = T[l] * Ml[l] + T[lip] * Mlip[l] + T[lim] * Mlim[l] + T[ljp] * Mljp[l] + T[ljm] * Mljm[l] - RHS[l];
+          normres += fabs(resid);
+          dtemper = omega * resid / Ml[l];
+          T[l] -= dtemper;
+          if (dtemper > T[l] * SORPRECISION) changed++;
+        }
+        jchess = 1 - jchess;
+      }
+    }
+    if (normres < SOREPS * normres0) return n;
+    if (normres < SORMIN) return n;
+    if (changed == 0) return n;
+  }
+  return n;
+}
+
+/* IterateRelaxationParameter, EnergySources.c:340-403 */
+static int o_iterate_relaxation(oracle_ctx *o) {
+  double omegamin = 1.0, omega = 1.0, omegamax = OMEGAUPPER;
+  int nr = o->nr, ns = o->ns, n, nsplit = 9, Niter, count = 0;
+  double *T = o->fld[THORIN_F_TEMPER];
+  double *bck = (double *)malloc(sizeof(double) * (size_t)nr * ns);
+  memcpy(bck, T, sizeof(double) * (size_t)nr * ns);
+  o->st.sor_domega = (omegamax - omegamin) / nsplit;
+  o->st.sor_niter_best = SORMAXITERS;
+  o->st.sor_omega_best = 1.0;
+  for (;;) {
+    for (n = 0; n <= nsplit; n++) {
+      Niter = o_sor(o, omega, 0);
+      if (Niter == o->st.sor_niter_best) count++;
+      if (Niter < o->st.sor_niter_best) { o->st.sor_niter_best = Niter; o->st.sor_omega_best = omega; }
+      omega += o->st.sor_domega;
+      memcpy(T, bck, sizeof(double) * (size_t)nr * ns);
+    }
+    if (o->st.sor_niter_best == SORMAXITERS) { free(bck); return -1; }
+    if (count == nsplit + 1) { o->st.sor_domega = omegamax - omegamin; free(bck); return 0; }
+    count = 0;
+    omegamin = o->st.sor_omega_best - o->st.sor_domega;
+    omegamax = o->st.sor_omega_best + o->st.sor_domega;
+    if (omegamin < 1.0) omegamin = 1.0;
+    if (omegamax > OMEGAUPPER) omegamax = OMEGAUPPER;
+    o->st.sor_domega = (omegamax - omegamin) / nsplit;
+    omega = omegamin;
+  }
+}
+
+/* ImplicitRadiativeDiffusion, EnergySources.c:201-335, with the helpers it
+ * calls (MidplaneVolumeDensity 658-680, OpacityProfile Opacity.c:231-265,
+ * CalculateQminus 75-99, DiffusionCoefs 556-605, TemperatureGradient 609-653,
+ * DiffusionTimestep 833-856). */
+static int o_implicit_diffusion(oracle_ctx *o, double dt, double *omega_used) {
+  const thorin_params *p = &o->p;
+  int nr = o->nr, ns = o->ns, i, j, l, lip, lim, ljp, ljm, Niter;
+  double *rho = o->fld[THORIN_F_DENS], *eint = o->fld[THORIN_F_ENERGYINT], *enew = o->fld[THORIN_F_ENERGYNEW];
+  double *T = o->fld[THORIN_F_TEMPER], *cs = o->fld[THORIN_F_SOUNDSPEED];
+  double *voldens = o->fld[W_VOLDENS], *opa = o->fld[THORIN_F_OPACITY], *qminus = o->fld[THORIN_F_QMINUS];
+  double *gtr = o->fld[W_GRADTR], *gtt = o->fld[W_GRADTT], *gtm = o->fld[W_GRADTM];
+  double *D = o->fld[W_DIFFC], *Dr = o->fld[W_DIFFR], *Dt = o->fld[W_DIFFT];
+  double *A = o->fld[W_A], *B = o->fld[W_B], *Ml = o->fld[W_ML], *Mlip = o->fld[W_MLIP];
+  double *Mlim = o->fld[W_MLIM], *Mljp = o->fld[W_MLJP], *Mljm = o->fld[W_MLJM], *RHS = o->fld[W_RHS];
+  double *divv = o->fld[THORIN_F_DIVV], *qplus = o->fld[THORIN_F_QPLUS];
+  double H, dxtheta, invdxtheta, invdxtheta2, afac, bfac, r, t, gradt, fac, s, dtmin, deltar2, dttmp;
+  o_temperature(o, rho, eint);
+  o_sound_speed(o, rho, eint);
+  for (i = 0; i < nr; i++)
+    for (j = 0; j < ns; j++) {
+      l = j + i * ns;
+      H = cs[l] * o->OmegaInv[i] * o->sqrt_adiabind_inv;
+      voldens[l] = rho[l] / H * SQRT2PI_INV;
+    }
+  if (p->opacity_law == THORIN_OPA_CONST || p->parametricopacity > 0.0) {
+    if (!o->opa_filled) {
+      double tmp = p->parametricopacity * OPA2CU;
+      for (i = 0; i < nr; i++) for (j = 0; j < ns; j++) opa[j + i * ns] = tmp;
+    }
+  } else {
+    for (i = 0; i < nr; i++)
+      for (j = 0; j < ns; j++) {
+        l = j + i * ns;
+        opa[l] = o_opacity(o, voldens[l] * RHO2CGS, T[l] * T2SI) * OPA2CU;
+      }
+  }
+  o->opa_filled = 1;
+  for (i = 0; i < nr; i++)
+    for (j = 0; j < ns; j++) {
+      double tau, taueff;
+      l = j + i * ns;
+      tau = p->opacitydrop * 0.5 * opa[l] * rho[l];
+      taueff = o_tau_eff(o, tau);
+      qminus[l] = 2.0 * STEFANBOLTZMANN * pow(T[l], 3.0) / taueff;
+    }
+  /* TemperatureGradient */
+  for (i = 1; i < nr; i++) {
+    dxtheta = 2.0 * PI / (double)ns * o->Rmed[i];
+    invdxtheta = 1.0 / dxtheta;
+    for (j = 0; j < ns; j++) {
+      l = j + i * ns; lim = l - ns;
+      gtr[l] = (T[l] - T[lim]) * o->InvDiffRmed[i];
+      ljm = l - 1;
+      if (j == 0) ljm = i * ns + ns - 1;
+      gtt[l] = (T[l] - T[ljm]) * invdxtheta;
+    }
+  }
+  for (i = 1; i < nr - 1; i++)
+    for (j = 0; j < ns; j++) {
+      l = j + i * ns; lip = l + ns; ljp = l + 1;
+      if (j == ns - 1) ljp = i * ns;
+      r = 0.5 * (gtr[lip] + gtr[l]);
+      t = 0.5 * (gtt[ljp] + gtt[l]);
+      gtm[l] = sqrt(r * r + t * t);
+    }
+  /* DiffusionCoefs */
+  for (i = 0; i < nr; i++)
+    for (j = 0; j < ns; j++) {
+      l = j + i * ns;
+      if (i == 0) gradt = gtm[l + ns];
+      else if (i == nr - 1) gradt = gtm[l - ns];
+      else gradt = gtm[l];
+      fac = 1.0 / (voldens[l] * opa[l]);
+      s = 4.0 * gradt * fac / T[l];
+      D[l] = o_flux_limiter(s) * 16.0 * STEFANBOLTZMANN * pow(T[l], 3.0) * fac;
+    }
+  for (i = 1; i < nr; i++)
+    for (j = 0; j < ns; j++) {
+      l = j + i * ns; lim = l - ns; ljm = l - 1;
+      if (j == 0) ljm = i * ns + ns - 1;
+      Dr[l] = 0.5 * (D[l] + D[lim]);
+      Dt[l] = 0.5 * (D[l] + D[ljm]);
+    }
+  /* DiffusionTimestep */
+  dtmin = 1.0e38;
+  for (i = 0; i < nr; i++) {
+    deltar2 = o->Rsup[i] - o->Rinf[i];
+    deltar2 = deltar2 * deltar2 / 2.0;
+    for (j = 0; j < ns; j++) {
+      dttmp = deltar2 / D[j + i * ns];
+      if (dttmp < dtmin) dtmin = dttmp;
+    }
+  }
+  o->st.dt_stellar = dtmin;
+  /* matrix */
+  for (i = 1; i < nr; i++) {
+    dxtheta = 2.0 * PI / (double)ns * o->Rmed[i];
+    invdxtheta2 = 1.0 / dxtheta / dxtheta;
+    for (j = 0; j < ns; j++) {
+      l = j + i * ns;
+      A[l] = Dr[l] * o->Rinf[i] * o->InvDiffRmed[i];
+      H = cs[l] * o->OmegaInv[i] * o->sqrt_adiabind_inv;
+      B[l] = Dt[l] * invdxtheta2 * 2.0 * H;
+    }
+  }
+  for (i = 1; i < nr - 1; i++)
+    for (j = 0; j < ns; j++) {
+      l = j + i * ns; lip = l + ns; ljp = l + 1;
+      if (j == ns - 1) ljp = i * ns;
+      H = cs[l] * o->OmegaInv[i] * o->sqrt_adiabind_inv;
+      bfac = dt / (rho[l] * o->CV);
+      afac = bfac * o->InvDiffRsup[i] * 2.0 * H / o->Rmed[i];
+      Ml[l] = 1.0 + divv[l] * (p->adiabind - 1.0) * dt + 4.0 * bfac * qminus[l] + afac * (A[lip] + A[l]) + bfac * (B[ljp] + B[l]);
+      Mlip[l] = -afac * A[lip];
+      Mlim[l] = -afac * A[l];
+      Mljp[l] = -bfac * B[ljp];
+      Mljm[l] = -bfac * B[l];
+      RHS[l] = T[l] + bfac * (qplus[l] + 3.0 * qminus[l] * T[l]);
+    }
+  if (o->st.sor_first) {
+    o->st.sor_first = 0;
+    if (o_iterate_relaxation(o) < 0) return -1;
+    o->st.sor_omega = o->st.sor_omega_best;
+    o->st.sor_niter_last = o->st.sor_niter_best;
+  }
+  o->st.sor_omega += o->st.sor_domega;
+  if (o->st.sor_omega > OMEGAUPPER) { o->st.sor_omega = OMEGAUPPER; o->st.sor_domega = -o->st.sor_domega; }
+  if (o->st.sor_omega < 1.0) { o->st.sor_omega = 1.0; o->st.sor_domega = -o->st.sor_domega; }
+  *omega_used = o->st.sor_omega;
+  Niter = o_sor(o, o->st.sor_omega, 1);
+  if (Niter > o->st.sor_niter_last) o->st.sor_domega = -o->st.sor_domega;
+  o->st.sor_niter_last = Niter;
+  for (j = 0; j < ns; j++) T[j] = T[j + ns];
+  for (j = 0; j < ns; j++) T[j + (nr - 1) * ns] = T[j + (nr - 2) * ns];
+  for (i = 0; i < nr; i++)
+    for (j = 0; j < ns; j++) { l = j + i * ns; enew[l] = rho[l] * T[l] * o->CV; }
+  return Niter;
+}
+
+/* SubStep3, SourceEuler.c:508-586 (ParametricCooling branch included) */
+static int o_substep3(oracle_ctx *o, double dt, double *omega_used) {
+  const thorin_params *p = &o->p;
+  int nr = o->nr, ns = o->ns, i, j, l;
+  const double *rho = o->fld[THORIN_F_DENS], *energy = o->fld[THORIN_F_ENERGYINT];
+  double *enew = o->fld[THORIN_F_ENERGYNEW], *qplus = o->fld[THORIN_F_QPLUS];
+  const double *Trr = o->fld[THORIN_F_TAURR], *Trp = o->fld[THORIN_F_TAURP], *Tpp = o->fld[THORIN_F_TAUPP];
+  const double *divv = o->fld[THORIN_F_DIVV];
+  double viscosity, r0 = o->Rmed[0], r1 = o->Rmed[1], r2 = o->Rmed[2], q1, q2;
+  for (i = 1; i < nr; i++) {
+    viscosity = o_fviscosity(o, o->Rmed[i]);
+    for (j = 0; j < ns; j++) {
+      l = j + i * ns;
+      if (viscosity != 0.0) {
+        qplus[l] = 0.5 / viscosity / rho[l] * (Trr[l] * Trr[l] + 2.0 * Trp[l] * Trp[l] + Tpp[l] * Tpp[l]);
+        qplus[l] += (2.0 / 9.0) * viscosity * rho[l] * divv[l] * divv[l];
+      } else qplus[l] = 0.0;
+    }
+  }
+  viscosity = o_fviscosity(o, o->Rmed[0]);
+  for (j = 0; j < ns; j++) {
+    q1 = qplus[j + ns];
+    q2 = qplus[j + 2 * ns];
+    if (viscosity != 0.0) qplus[j] = q1 * exp(log(q1 / q2) * log(r0 / r1) / log(r1 / r2));
+    else qplus[j] = 0.0;
+  }
+  if (p->parametric_cooling) return -2;   /* deprecated route; not in scope */
+  return o_implicit_diffusion(o, dt, omega_used);
+}
+
+/* ---------------- transport (TransportEuler.c) ---------------- */
+static void o_divise(oracle_ctx *o, const double *num, const double *den, double *res) {
+  /* DivisePolarGrid, SideEuler.c:82-99 (with its Nrad/Nsec swap) */
+  int i, j, l, ns = o->nr, nr = o->ns;
+  for (i = 0; i <= nr; i++)
+    for (j = 0; j < ns; j++) { l = j + ns * i; res[l] = num[l] / (den[l] + 1e-20); }
+}
+
+static void o_star_rad(oracle_ctx *o, const double *qb, const double *vr, double *qs, double dt) {
+  /* ComputeStarRad, TransportEuler.c:232-273 */
+  int i, j, l, lq, lip, lim, nr = o->nr, ns = o->ns;
+  double *dq = o->fld[W_DQ], dqm, dqp;
+  for (j = 0; j < ns; j++) {
+    for (i = 0; i < nr; i++) {
+      l = j + i * ns; lq = i + j * nr; lip = l + ns; lim = l - ns;
+      if ((i == 0) || (i == nr - 1)) dq[lq] = 0.0;
+      else {
+        dqm = (qb[l] - qb[lim]) * o->InvDiffRmed[i];
+        dqp = (qb[lip] - qb[l]) * o->InvDiffRmed[i + 1];
+        if (dqp * dqm > 0.0) dq[lq] = 2.0 * dqp * dqm / (dqp + dqm);
+        else dq[lq] = 0.0;
+      }
+    }
+    for (i = 1; i < nr; i++) {
+      l = j + i * ns; lq = i + j * nr; lim = l - ns;
+      if (vr[l] > 0.0) qs[l] = qb[lim] + (o->Rmed[i] - o->Rmed[i - 1] - vr[l] * dt) * 0.5 * dq[lq - 1];
+      else qs[l] = qb[l] - (o->Rmed[i + 1] - o->Rmed[i] + vr[l] * dt) * 0.5 * dq[lq];
+    }
+    qs[j] = qs[j + ns * nr] = 0.0;
+  }
+}
+
+static void o_star_theta(oracle_ctx *o, const double *qb, const double *vt, double *qs, double dt) {
+  /* ComputeStarTheta, TransportEuler.c:275-316 */
+  int i, j, l, ljp, ljm, jm, nr = o->nr, ns = o->ns;
+  double *dq = o->fld[W_DQ], dqm, dqp, dxtheta, ksi, invdxtheta;
+  for (i = 0; i < nr; i++) {
+    dxtheta = 2.0 * PI / (double)ns * o->Rmed[i];
+    invdxtheta = 1.0 / dxtheta;
+    for (j = 0; j < ns; j++) {
+      l = j + i * ns; ljp = l + 1; ljm = l - 1;
+      if (j == 0) ljm = i * ns + ns - 1;
+      if (j == ns - 1) ljp = i * ns;
+      dqm = (qb[l] - qb[ljm]);
+      dqp = (qb[ljp] - qb[l]);
+      if ((dqp * dqm > 0.0) && (dqp + dqm != 0.0)) dq[l] = dqp * dqm / (dqp + dqm) * invdxtheta;
+      else dq[l] = 0.0;
+    }
+    for (j = 0; j < ns; j++) {
+      l = j + i * ns; jm = j - 1;
+      if (j == 0) jm = ns - 1;
+      ljm = jm + i * ns;
+      ksi = vt[l] * dt;
+      if (ksi > 0.0) qs[l] = qb[ljm] + (dxtheta - ksi) * dq[ljm];
+      else qs[l] = qb[l] - (dxtheta + ksi) * dq[l];
+    }
+  }
+}
+
+static void o_vanleer_rad(oracle_ctx *o, const double *vr, double *qb, double dt) {
+  /* VanLeerRadial, TransportEuler.c:384-415 */
+  int i, j, nr = o->nr, ns = o->ns, l, lip;
+  double *qrs = o->fld[W_QRSTAR], *rhos = o->fld[W_RHOSTAR], dtheta, varq;
+  o_divise(o, qb, o->fld[W_RHOINT], o->fld[W_WORK]);
+  o_star_rad(o, o->fld[W_WORK], vr, qrs, dt);
+  dtheta = 2.0 * PI / (double)ns;
+  for (i = 0; i < nr; i++)
+    for (j = 0; j < ns; j++) {
+      l = j + i * ns; lip = l + ns;
+      varq = dt * dtheta * o->Rinf[i] * qrs[l] * rhos[l] * vr[l];
+      varq -= dt * dtheta * o->Rsup[i] * qrs[lip] * rhos[lip] * vr[lip];
+      qb[l] += varq * o->InvSurf[i];
+    }
+}
+
+static void o_vanleer_theta(oracle_ctx *o, const double *vt, double *qb, double dt) {
+  /* VanLeerTheta, TransportEuler.c:417-445 */
+  int i, j, nr = o->nr, ns = o->ns, l, ljp;
+  double *qrs = o->fld[W_QRSTAR], *rhos = o->fld[W_RHOSTAR], dxrad, varq, invsurf;
+  o_divise(o, qb, o->fld[W_RHOINT], o->fld[W_WORK]);
+  o_star_theta(o, o->fld[W_WORK], vt, qrs, dt);
+  for (i = 0; i < nr; i++) {
+    dxrad = (o->Rsup[i] - o->Rinf[i]) * dt;
+    invsurf = 1.0 / o->Surf[i];
+    for (j = 0; j < ns; j++) {
+      l = j + i * ns; ljp = l + 1;
+      if (j == ns - 1) ljp = i * ns;
+      varq = dxrad * qrs[l] * rhos[l] * vt[l];
+      varq -= dxrad * qrs[ljp] * rhos[ljp] * vt[ljp];
+      qb[l] += varq * invsurf;
+    }
+  }
+}
+
+static void o_quantities_advection(oracle_ctx *o, const double *vt, double dt) {
+  /* QuantitiesAdvection, TransportEuler.c:159-175 */
+  double *rho = o->fld[THORIN_F_DENS];
+  o_star_theta(o, rho, vt, o->fld[W_RHOSTAR], dt);
+  o_actualise(o, o->fld[W_RHOINT], rho);
+  o_vanleer_theta(o, vt, o->fld[W_RADMOMP], dt);
+  o_vanleer_theta(o, vt, o->fld[W_RADMOMM], dt);
+  o_vanleer_theta(o, vt, o->fld[W_THETAMOMP], dt);
+  o_vanleer_theta(o, vt, o->fld[W_THETAMOMM], dt);
+  if (o->p.energy_eq) o_vanleer_theta(o, vt, o->fld[THORIN_F_ENERGY], dt);
+  o_vanleer_theta(o, vt, rho, dt);
+}
+
+static void o_advect_shift(oracle_ctx *o, double *val) {
+  /* AdvectSHIFT, TransportEuler.c:113-136 */
+  int i, j, ji, nr = o->nr, ns = o->ns;
+  double *tmp = o->fld[W_TEMPSHIFT];
+  for (i = 0; i < nr; i++) {
+    for (j = 0; j < ns; j++) {
+      ji = j - o->Nshift[i];
+      while (ji < 0) ji += ns;
+      while (ji >= ns) ji -= ns;
+      tmp[j + i * ns] = val[ji + i * ns];
+    }
+    for (j = 0; j < ns; j++) val[j + i * ns] = tmp[j + i * ns];
+  }
+}
+
+static void o_transport(oracle_ctx *o, double dt) {
+  /* Transport, TransportEuler.c:34-47, with ComputeLRMomenta 318-347,
+   * OneWindRad 49-65, OneWindTheta 138-156, ComputeResiduals 70-111,
+   * ComputeVelocities 350-381 */
+  int i, j, l, lip, ljp, lim, ljm, nr = o->nr, ns = o->ns;
+  double *rho = o->fld[THORIN_F_DENS], *vr = o->fld[THORIN_F_VRAD], *vt = o->fld[THORIN_F_VTHETA];
+  double *rp = o->fld[W_RADMOMP], *rm = o->fld[W_RADMOMM], *tp = o->fld[W_THETAMOMP], *tm = o->fld[W_THETAMOMM];
+  double *vtr = o->fld[W_VTHETARES];
+  double of = o->st.omega_frame;
+  for (i = 0; i < nr; i++)
+    for (j = 0; j < ns; j++) {
+      l = j + i * ns; lip = l + ns; ljp = l + 1;
+      if (j == ns - 1) ljp = i * ns;
+      rp[l] = rho[l] * vr[lip];
+      rm[l] = rho[l] * vr[l];
+      tp[l] = rho[l] * (vt[ljp] + o->Rmed[i] * of) * o->Rmed[i];
+      tm[l] = rho[l] * (vt[l] + o->Rmed[i] * of) * o->Rmed[i];
+    }
+  /* OneWindRad */
+  o_star_rad(o, rho, vr, o->fld[W_RHOSTAR], dt);
+  o_actualise(o, o->fld[W_RHOINT], rho);
+  o_vanleer_rad(o, vr, rp, dt);
+  o_vanleer_rad(o, vr, rm, dt);
+  o_vanleer_rad(o, vr, tp, dt);
+  o_vanleer_rad(o, vr, tm, dt);
+  if (o->p.energy_eq) o_vanleer_rad(o, vr, o->fld[THORIN_F_ENERGY], dt);
+  o_vanleer_rad(o, vr, rho, dt);
+  /* OneWindTheta: ComputeResiduals */
+  {
+    double invdt = 1.0 / dt, dpinvns = 2.0 * PI / (double)ns, moy, Ntilde, Nround, vtemp;
+    for (i = 0; i < nr; i++) {
+      moy = 0.0;
+      for (j = 0; j < ns; j++) moy += vt[j + i * ns];
+      o->VMed[i] = moy / (double)ns;
+      Ntilde = o->VMed[i] * o->InvRmed[i] * dt * (double)ns / 2.0 / PI;
+      Nround = floor(Ntilde + 0.5);
+      o->Nshift[i] = (int)(long)Nround;
+      if (o->p.fast_transport) {
+        vtemp = (Ntilde - Nround) * o->Rmed[i] * invdt * dpinvns;
+        for (j = 0; j < ns; j++) { l = j + i * ns; vtr[l] = vt[l] - o->VMed[i]; vt[l] = vtemp; }
+      } else {
+        for (j = 0; j < ns; j++) { l = j + i * ns; vtr[l] = vt[l]; vt[l] = 0.0; }
+      }
+    }
+  }
+  o_quantities_advection(o, vtr, dt);
+  if (o->p.fast_transport) {
+    o_quantities_advection(o, vt, dt);
+    o_advect_shift(o, rp);
+    o_advect_shift(o, rm);
+    o_advect_shift(o, tp);
+    o_advect_shift(o, tm);
+    if (o->p.energy_eq) o_advect_shift(o, o->fld[THORIN_F_ENERGY]);
+    o_advect_shift(o, rho);
+  }
+  /* ComputeVelocities */
+  for (i = 0; i < nr; i++)
+    for (j = 0; j < ns; j++) {
+      l = j + i * ns; lim = l - ns; ljm = l - 1;
+      if (j == 0) ljm = i * ns + ns - 1;
+      if (i == 0) vr[l] = 0.0;
+      else vr[l] = (rp[lim] + rm[l]) / (rho[l] + rho[lim] + 1e-20);
+      vt[l] = (tp[ljm] + tm[l]) / (rho[l] + rho[ljm] + 1e-15) / o->Rmed[i] - o->Rmed[i] * of;
+    }
+}
+
+/* ---------------- AlgoGas (SourceEuler.c:210-317) ---------------- */
+int oracle_algogas(oracle_ctx *o, thorin_planets *sys, thorin_nbody_fn nbody, void *user,
+                   thorin_step_log *log) {
+  const thorin_params *p = &o->p;
+  double dt, dtemp = 0.0, DT = p->dt, newdt, omega_used = 0.0;
+  int N, crashed, niter, rc = 0;
+  if (log) memset(log, 0, sizeof(*log));
+  dt = DT / 1.0;
+  while (dtemp < 0.999999999 * DT) {
+    double mt = o->st.physical_time / (p->masstaper * 2.0 * M_PI);
+    o->st.mass_taper = (mt > 1.0 ? 1.0 : pow(sin(mt * M_PI / 2.0), 2.0));
+    o->invdtreb_sq = o_minstep_rebound(sys);
+    N = o_condition_cfl_core(o, DT - dtemp, &newdt);
+    dt = (DT - dtemp) / (double)N;
+    dtemp += dt;
+    o_fill_forces(o, sys);
+    o_advance_from_disk(o, sys, dt);
+    if (nbody) nbody(user, sys, o->st.physical_time, dt, o->st.omega_frame * dt);
+    o_boundary(o, dt);
+    crashed = o_detect_crash(o, o->fld[THORIN_F_DENS]);
+    crashed = o_detect_crash(o, o->fld[THORIN_F_ENERGY]);
+    if (crashed) o->st.already_crashed++;
+    o_pressure(o, o->fld[THORIN_F_DENS], o->fld[THORIN_F_ENERGY]);
+    o_substep1(o, dt);
+    o_substep2(o, dt);
+    o_actualise(o, o->fld[THORIN_F_VRAD], o->fld[THORIN_F_VRADNEW]);
+    o_actualise(o, o->fld[THORIN_F_VTHETA], o->fld[THORIN_F_VTHETANEW]);
+    o_boundary(o, dt);
+    niter = 0;
+    if (p->energy_eq) {
+      o_div_stress(o, o->fld[THORIN_F_VRAD], o->fld[THORIN_F_VTHETA], o->fld[THORIN_F_DENS]);
+      niter = o_substep3(o, dt, &omega_used);
+      if (niter < 0) rc = -3;
+      o_actualise(o, o->fld[THORIN_F_ENERGY], o->fld[THORIN_F_ENERGYNEW]);
+    }
+    o_transport(o, dt);
+    o_boundary(o, dt);
+    o_temperature(o, o->fld[THORIN_F_DENS], o->fld[THORIN_F_ENERGY]);
+    if (p->energy_eq) o_sound_speed(o, o->fld[THORIN_F_DENS], o->fld[THORIN_F_ENERGY]);
+    o->st.physical_time += dt;
+    if (log && log->nsub < THORIN_MAXSUB) {
+      log->cfl_n[log->nsub] = N;
+      log->dt[log->nsub] = dt;
+      log->sor_niter[log->nsub] = niter;
+      log->sor_omega[log->nsub] = omega_used;
+      log->crashed[log->nsub] = crashed;
+      log->crashed_any |= crashed;
+      log->nsub++;
+    }
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+/* ---------------- accessors ---------------- */
+int oracle_get_field(oracle_ctx *o, int field, double *out, size_t count) {
+  size_t n = (size_t)(o->nr + 1) * o->ns;
+  if (field < 0 || field >= NW) return -1;
+  if (count > n) count = n;
+  memcpy(out, o->fld[field], count * sizeof(double));
+  return 0;
+}
+
+int oracle_set_field(oracle_ctx *o, int field, const double *in, size_t count) {
+  size_t n = (size_t)(o->nr + 1) * o->ns;
+  if (field < 0 || field >= NW) return -1;
+  if (count > n) count = n;
+  memcpy(o->fld[field], in, count * sizeof(double));
+  return 0;
+}
+
+int oracle_get_state(oracle_ctx *o, thorin_state *s) { *s = o->st; return 0; }
+int oracle_set_state(oracle_ctx *o, const thorin_state *s) { o->st = *s; return 0; }
+
+int oracle_get_1d(oracle_ctx *o, int which, double *out, size_t count) {
+  size_t k, n = (size_t)o->nr + 1;
+  const double *src = NULL;
+  if (count > n) count = n;
+  switch (which) {
+    case 0: src = o->Radii; break;
+    case 1: src = o->Rinf; break;
+    case 2: src = o->Rsup; break;
+    case 3: src = o->Rmed; break;
+    case 4: src = o->Surf; break;
+    case 5: src = o->InvDiffRmed; break;
+    case 6: src = o->SigmaMed; break;
+    case 7: src = o->SigmaInf; break;
+    case 8: src = o->EnergyMed; break;
+    case 9: src = o->WaveKiller; break;
+    case 10: src = o->globcsvec; break;
+    case 11: src = o->globtempervec; break;
+    case 12: src = o->VthetaMed; break;
+    case 13: for (k = 0; k < count; k++) out[k] = (double)o->Nshift[k]; return 0;
+    case 14: src = o->VMed; break;
+    default: return -1;
+  }
+  memcpy(out, src, count * sizeof(double));
+  return 0;
+}
+
+/* GasTotalMass, SideEuler.c:26-51 */
+double oracle_gas_total_mass(oracle_ctx *o) {
+  int i, j;
+  double total = 0.0;
+  const double *d = o->fld[THORIN_F_DENS];
+  for (i = 0; i < o->nr; i++)
+    for (j = 0; j < o->ns; j++) total += o->Surf[i] * d[j + i * o->ns];
+  return total;
+}

@@ -1,0 +1,57 @@
+"""Shared helpers of the test suite: golden fixtures, parameters, the planet
+trajectory replay used to pin the hydro step independently of the N-body."""
+from __future__ import annotations
+
+import os
+
+import numpy as np
+
+from fargo_thorin_amd import Params, make_planets
+from fargo_thorin_amd._capi import NBODY_FN, host_nbody_fn
+from tests.golden.make_golden import CASES, FIELDS
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def golden(name: str):
+    return np.load(os.path.join(GOLDEN, f"{name}.npz"))
+
+
+def params_for(cfg: dict, **extra) -> Params:
+    keys = dict(cfg)
+    keys.update({k: str(v) for k, v in extra.items()})
+    keys.setdefault("OutputDir", "/nonexistent-thorin-out/")
+    return Params(**keys)
+
+
+def planets_for(planets):
+    return make_planets([m for m, a in planets], [a for m, a in planets])
+
+
+class Replay:
+    """N-body callback that moves the planets along the reference's own
+    trajectory (tqwk0.dat rows: position at the start of each DT).  One
+    substep per DT is assumed, which the golden runs have."""
+
+    def __init__(self, tq):
+        self.tq = np.asarray(tq)
+        self.n = 0
+        self.cb = NBODY_FN(self._step)
+
+    def _step(self, user, sysp, t, dt, angle):
+        self.n += 1
+        if self.n < len(self.tq):
+            sysp.contents.x[0] = self.tq[self.n, 3]
+            sysp.contents.y[0] = self.tq[self.n, 4]
+        return 0
+
+
+def case(name):
+    cfg, planets, full = CASES[name]
+    return cfg, planets, full
+
+
+def rel_linf(a, b, floor=1e-300):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), floor))

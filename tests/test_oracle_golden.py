@@ -1,0 +1,87 @@
+"""The CPU oracle against the reference's own outputs (tests/golden/).
+
+The fixtures were written by tests/golden/make_golden.py, which runs the
+reference FARGO_THORIN (built from its sources by oracle/ref.mk) on seeded
+disks.  The oracle must reproduce every output field bit for bit, the SOR
+iteration count of every substep and PhysicalTime of every DT.  The planets
+follow the reference's own trajectory (Replay), so that the comparison pins
+the hydro step and not the N-body integrator.
+"""
+import hashlib
+
+import numpy as np
+import pytest
+
+from oracle import OracleDisk
+from tests.helpers import FIELDS, Replay, case, golden, params_for, planets_for
+
+FULL_CASES = ["wall_lp85_64x128", "damp_bl94_48x96", "dampinit_zhu12_40x64", "planet_q3e-5_48x96"]
+
+
+def run_oracle_case(name):
+    cfg, planets, _ = case(name)
+    g = golden(name)
+    o = OracleDisk(params_for(cfg))
+    o.init(g["init_dens"], g["init_vrad"], g["init_vtheta"], g["init_temper"])
+    sysp = planets_for(planets)
+    rep = Replay(g["tqwk0"])
+    ntot, ninterm = int(g["ntot"]), int(g["ninterm"])
+    snaps = {0: {f: o.field(f) for f in FIELDS}}
+    niters, times = [], [o.state().physical_time]
+    for i in range(ntot):
+        lg = o.algogas(sysp, rep.cb)
+        niters += [lg.sor_niter[k] for k in range(lg.nsub)]
+        times.append(o.state().physical_time)
+        if (i + 1) % ninterm == 0:
+            snaps[(i + 1) // ninterm] = {f: o.field(f) for f in FIELDS}
+    return g, snaps, niters, times
+
+
+@pytest.mark.parametrize("name", FULL_CASES)
+def test_oracle_bitwise_fields(name):
+    g, snaps, niters, times = run_oracle_case(name)
+    checked = 0
+    for n, fields in snaps.items():
+        for f, a in fields.items():
+            key = f"{f}{n}"
+            if key not in g:
+                continue
+            b = g[key]
+            assert np.array_equal(a, b), f"{key}: {np.count_nonzero(a != b)} cells differ"
+            checked += 1
+    assert checked >= 8
+
+
+@pytest.mark.parametrize("name", FULL_CASES)
+def test_oracle_sor_iterations_and_time(name):
+    g, snaps, niters, times = run_oracle_case(name)
+    ntot = int(g["ntot"])
+    ref_niter = list(g["sor_niter"][:len(niters)])
+    assert niters == ref_niter
+    # the reference logs PhysicalTime at the start of every DT (tqwk0.dat)
+    ref_t = g["tqwk0"][:ntot, 0]
+    assert np.array_equal(np.array(times[:ntot]), ref_t)
+
+
+def test_oracle_plain_128x256_200_steps_checksums():
+    """The 128x256 workload for its full 200 steps: every output field's
+    SHA-256 equals the reference's."""
+    name = "plain_128x256_200"
+    cfg, planets, _ = case(name)
+    g = golden(name)
+    o = OracleDisk(params_for(cfg))
+    o.init(g["init_dens"], g["init_vrad"], g["init_vtheta"], g["init_temper"])
+    sysp = planets_for(planets)
+    rep = Replay(g["tqwk0"])
+    ntot, ninterm = int(g["ntot"]), int(g["ninterm"])
+    for i in range(ntot + 1):
+        n = i // ninterm
+        if i % ninterm == 0:
+            for f in FIELDS:
+                key = f"sha_{f}{n}"
+                if key in g:
+                    h = np.frombuffer(hashlib.sha256(o.field(f).tobytes()).digest(), dtype=np.uint8)
+                    assert np.array_equal(h, g[key]), f"{f} at output {n}"
+        if i == ntot:
+            break
+        o.algogas(sysp, rep.cb)
