@@ -1,0 +1,167 @@
+#!/usr/bin/env python3
+"""Throughput of the GPU hydro step (AlgoGas) at 1024x1536 FP64 on one MI355X.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+
+A step is one DT of AlgoGas (CFL substeps, forces, source terms, viscosity,
+implicit radiative diffusion with SOR, FARGO transport) on the seeded
+1024x1536 disk with damping zones at both edges; the fields stay resident
+in HBM.  The first warmup step includes the reference's one-off search of
+the SOR relaxation parameter and is never timed.
+
+Printed: one JSON line with steps/s, the HIP-event roofline of the dominant
+kernel measured in the timed region, and the reference's own sequential CPU
+build (oracle/_ref/thorin) timed on this host on two DTs of the same
+workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "steps/s @1024×1536 fp64; SOR ms/step CNN-warm+FP32-refine vs plain @1e-10"
+HBM_PEAK_GBS = 8000.0
+
+# Compulsory FP64 traffic per cell and launch of each kernel (8 B per field
+# read or written once; the field lists are in DESIGN.md).  SOR: 64 B per
+# cell and iteration over the updated rings, split over its two half-sweep
+# launches.
+ALG_BYTES_PER_CELL = {
+    "k_sor_half": 32, "k_transport_theta": 104, "k_transport_rad": 80, "k_forces": 32,
+    "k_substep1": 64, "k_divstress": 56, "k_visc_update": 72, "k_substep2": 56, "k_qplus": 48,
+    "k_temper_cs": 32, "k_opacity_qminus": 40, "k_diffcoef": 40, "k_matrix": 104, "k_velocities": 56,
+    "k_energy_new": 24, "k_indirect": 48, "k_cfl_ring": 24, "k_sor_resid0": 56,
+}
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--nrad", type=int, default=1024)
+    ap.add_argument("--nsec", type=int, default=1536)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--roofline-kernel", default="auto")
+    return ap.parse_args()
+
+
+def cpu_baseline_reference(nrad, nsec):
+    """Wall-clock of the reference's sequential build on 2 DTs: runs of
+    Ntot = 1 and Ntot = 3 (AlgoGas runs Ntot+1 times), difference / 2."""
+    from fargo_thorin_amd import configs
+    from fargo_thorin_amd.seed import seeded_disk, write_cfg
+    ref = os.path.join(ROOT, "oracle", "_ref", "thorin")
+    if not os.path.exists(ref):
+        return None
+    tmp = tempfile.mkdtemp(prefix="thorin_cpu_")
+    cfgdir = os.path.join(tmp, "in")
+    os.makedirs(cfgdir)
+    disk = seeded_disk(nrad, nsec)
+    for k in ("dens", "vrad", "vtheta", "temper"):
+        write_cfg(os.path.join(cfgdir, f"gas{k}.cfg"), disk[k])
+    pf = os.path.join(tmp, "planet.cfg")
+    with open(pf, "w") as f:
+        f.write(configs.planet_text(configs.PLANET_ZERO_MASS))
+    walls = {}
+    for ntot in (1, 3):
+        out = os.path.join(tmp, f"out{ntot}")
+        os.makedirs(out)
+        cfg = configs.with_grid(configs.DAMPED_1024x1536, nrad, nsec, Ntot=ntot, Ninterm=ntot)
+        par = os.path.join(tmp, f"in{ntot}.par")
+        with open(par, "w") as f:
+            f.write(configs.inpar_text(cfg, out + "/", pf, cfgdir))
+        print(f"[bench] cpu baseline: reference run with Ntot={ntot}", file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
+        r = subprocess.run([ref, par], cwd=tmp, capture_output=True, text=True, timeout=900)
+        walls[ntot] = time.perf_counter() - t0
+        if r.returncode != 0:
+            return None
+    per_step = (walls[3] - walls[1]) / 2.0
+    return {"value": 1.0 / per_step, "unit": "steps/s", "cores": 1, "kind": "reference",
+            "sample": f"2 DTs of the {nrad}x{nsec} damped workload: wall(Ntot=3) - wall(Ntot=1) of the "
+                      f"reference's sequential build (oracle/_ref/thorin, 1 thread); "
+                      f"{walls[1]:.1f}s and {walls[3]:.1f}s"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU radial slabs are "
+                                   "described in DESIGN.md, not built)"}))
+        sys.exit(2)
+    import numpy as np
+
+    from fargo_thorin_amd import Disk, Params, configs, make_planets, seeded_disk
+
+    nrad, nsec = args.nrad, args.nsec
+    cfg = configs.with_grid(configs.DAMPED_1024x1536, nrad, nsec)
+    P = Params(**cfg, OutputDir="/nonexistent-thorin-out/")
+    disk = seeded_disk(nrad, nsec)
+    d = Disk(P)
+    d.init(disk["dens"], disk["vrad"], disk["vtheta"], disk["temper"])
+    sysp = make_planets([1e-13], [1.0])
+    print(f"[bench] {nrad}x{nsec}: warmup {args.warmup} (first DT includes the SOR omega search)",
+          file=sys.stderr, flush=True)
+    warm = max(args.warmup, 2)
+    d.algogas(sysp)
+    d.profile()                      # all kernels, to pick the dominant one
+    for _ in range(warm - 1):
+        d.algogas(sysp)
+    warm_prof = d.profile_read()
+    kname = args.roofline_kernel
+    if kname == "auto":
+        kname = max(warm_prof, key=lambda k: warm_prof[k][0])
+    d.profile(kernels=[kname])        # timed region: events around that kernel only
+    substeps, sor_iters = 0, 0
+    d.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        lg = d.algogas(sysp)
+        substeps += lg.nsub
+        sor_iters += sum(lg.sor_niter[k] for k in range(lg.nsub))
+    d.sync()
+    t1 = time.perf_counter()
+    prof = d.profile_read()
+    d.profile_off()
+    elapsed = t1 - t0
+    value = args.steps / elapsed
+    ms_k, n_k = prof.get(kname, (0.0, 0))
+    avg_s = (ms_k / max(n_k, 1)) * 1e-3
+    cells = (nrad - 2) * nsec if kname.startswith("k_sor") or kname == "k_matrix" else nrad * nsec
+    alg = ALG_BYTES_PER_CELL.get(kname, 0) * cells
+    achieved = alg / avg_s / 1e9 if avg_s > 0 else 0.0
+    out = {
+        "metric": METRIC, "value": value, "unit": "steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": warm, "ms_per_step": 1e3 * elapsed / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"damped_{nrad}x{nsec}", "nrad": nrad, "nsec": nsec, "dt": 1e-3,
+                   "substeps": substeps, "sor_iters_per_step": sor_iters / args.steps,
+                   "opacity": "LP85", "boundary": "DAMPING/ZEROVRAD", "planets": 1},
+        "roofline": {"kernel": kname, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "alg_bytes_per_launch": alg, "avg_launch_us": avg_s * 1e6, "launches": n_k,
+                     "kernel_share_of_step": (ms_k * 1e-3) / elapsed},
+        "kernel_ms_per_step_warmup": {k: v[0] / max(warm - 1, 1) for k, v in sorted(
+            warm_prof.items(), key=lambda kv: -kv[1][0])},
+    }
+    if not args.no_cpu_baseline:
+        try:
+            out["cpu_baseline"] = cpu_baseline_reference(nrad, nsec)
+        except Exception as e:  # the GPU figure stands without it
+            out["cpu_baseline"] = {"error": str(e)}
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

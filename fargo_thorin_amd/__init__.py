@@ -153,6 +153,12 @@ class Disk:
                "ring means")
         return out
 
+    def ring_profile(self, which: str) -> np.ndarray:
+        idx = {"globcsvec": 0, "globtempervec": 1, "vmoy": 2, "vmed": 3, "nshift": 4}[which]
+        out = np.empty(self.nrad)
+        _check(lib().thorin_ring_profile(self._h, idx, out.ctypes.data_as(C.POINTER(C.c_double))), "ring profile")
+        return out
+
     def total_mass(self) -> float:
         v = C.c_double()
         _check(lib().thorin_gas_total_mass(self._h, C.byref(v)), "total mass")
@@ -174,6 +180,37 @@ class Disk:
         for name, _ in _capi.Geometry._fields_[2:]:
             p = getattr(g, name)
             out[name] = np.ctypeslib.as_array(p, shape=(n,)).copy()
+        return out
+
+    # --- kernel timing (HIP events on the library's stream)
+    @staticmethod
+    def kernel_names() -> list:
+        out, k = [], 0
+        while True:
+            n = lib().thorin_prof_kernel_name(k)
+            if not n:
+                return out
+            out.append(n.decode())
+            k += 1
+
+    def profile(self, kernels=None) -> None:
+        names = self.kernel_names()
+        mask = 0
+        for k, n in enumerate(names):
+            if kernels is None or n in kernels:
+                mask |= 1 << k
+        _check(lib().thorin_prof_enable(self._h, 1, mask), "profile")
+
+    def profile_off(self) -> None:
+        _check(lib().thorin_prof_enable(self._h, 0, 0), "profile")
+
+    def profile_read(self) -> dict:
+        out = {}
+        for k, n in enumerate(self.kernel_names()):
+            ms, cnt = C.c_double(), C.c_longlong()
+            _check(lib().thorin_prof_read(self._h, k, C.byref(ms), C.byref(cnt)), "profile read")
+            if cnt.value:
+                out[n] = (ms.value, cnt.value)
         return out
 
     def sync(self) -> None:

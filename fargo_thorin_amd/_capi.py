@@ -85,8 +85,14 @@ def lib() -> C.CDLL:
         "thorin_download_field": (ip, [vp, ip, dp, sz]),
         "thorin_algogas": (ip, [vp, C.POINTER(Planets), NBODY_FN, vp, C.POINTER(StepLog)]),
         "thorin_ring_means": (ip, [vp, ip, dp]),
+        "thorin_ring_profile": (ip, [vp, ip, dp]),
+        "thorin_ctx_create_host": (ip, [vp, C.POINTER(vp)]),
+        "thorin_host_field": (ip, [vp, ip, dp, sz]),
         "thorin_gas_total_mass": (ip, [vp, dp]),
         "thorin_sync": (ip, [vp]),
+        "thorin_prof_enable": (ip, [vp, ip, C.c_ulonglong]),
+        "thorin_prof_read": (ip, [vp, ip, dp, C.POINTER(C.c_longlong)]),
+        "thorin_prof_kernel_name": (C.c_char_p, [ip]),
         "thorin_last_error": (C.c_char_p, []),
         "thorin_planets_from_config": (ip, [vp, C.POINTER(Planets)]),
         "thorin_nbody_step": (ip, [vp, C.POINTER(Planets), C.c_double, C.c_double, C.c_double]),

@@ -111,7 +111,27 @@ struct thorin_dev {
   double *ringtmp;              // per-ring scratch
   int nblk_cells, nblk_sor;
   int opa_filled;
+  // kernel timing with HIP events (thorin_prof_*)
+  int prof_on;
+  unsigned long long prof_mask;
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> *prof_pend;
+  std::vector<hipEvent_t> *prof_pool;
+  double prof_ms[64];
+  long long prof_cnt[64];
 };
+
+// kernels that the event timer knows, by launch name
+static const char *const KNAMES[] = {
+    "k_cfl_ring", "k_cfl_final", "k_forces", "k_forces_reduce", "k_indirect", "k_boundary", "k_substep1",
+    "k_divstress", "k_visc_update", "k_substep2", "k_qplus", "k_temper_cs", "k_opacity_qminus", "k_diffcoef",
+    "k_matrix", "k_sor_resid0", "k_sum_to", "k_sor_half", "k_energy_new", "k_transport_rad",
+    "k_transport_theta", "k_velocities", NULL};
+
+static int kid_of(const char *name) {
+  for (int i = 0; KNAMES[i]; i++)
+    if (!strcmp(KNAMES[i], name)) return i;
+  return -1;
+}
 
 // ---------------------------------------------------------------------------
 // small device helpers
@@ -1307,7 +1327,24 @@ extern "C" int thorin_dev_create(thorin_ctx *c) {
     thorin_set_error("no HIP device available: the FARGO_THORIN GPU path needs an MI355X (gfx950)");
     return -1;
   }
+  {
+    int dev = 0;
+    hipDeviceProp_t prop;
+    HCHECK(hipGetDevice(&dev));
+    HCHECK(hipGetDeviceProperties(&prop, dev));
+    const size_t need = (size_t)6 * c->ns * sizeof(double);
+    if (need > (size_t)prop.sharedMemPerBlock) {
+      char m[200];
+      snprintf(m, sizeof m, "NSEC %d needs %zu B of LDS per workgroup for the azimuthal transport; device allows %zu",
+               c->ns, need, (size_t)prop.sharedMemPerBlock);
+      thorin_set_error(m);
+      return -1;
+    }
+    HCHECK(hipFuncSetAttribute((const void *)k_transport_theta, hipFuncAttributeMaxDynamicSharedMemorySize, (int)need));
+  }
   HCHECK(hipStreamCreateWithFlags(&d->s, hipStreamNonBlocking));
+  d->prof_pend = new std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>>();
+  d->prof_pool = new std::vector<hipEvent_t>();
   const size_t fb = (d->nf + 64) * sizeof(double);
   for (int k = 0; k < NBUF; k++) {
     HCHECK(hipMalloc(&d->b[k], fb));
@@ -1347,7 +1384,8 @@ extern "C" int thorin_dev_create(thorin_ctx *c) {
       double DRin = p->rmin * p->dampingrminfrac, DRout = p->rmax * p->dampingrmaxfrac;
       for (int i = 0; i < c->nr; i++) dr[i] = (c->Rmed[i] < DRin || c->Rmed[i] > DRout) ? 1 : 0;
     }
-    HCHECK(hipMemcpy(ip + n1, dr.data(), sizeof(int) * n1, hipMemcpyHostToDevice));
+    HCHECK(hipMemcpyAsync(ip + n1, dr.data(), sizeof(int) * n1, hipMemcpyHostToDevice, d->s));
+    HCHECK(hipStreamSynchronize(d->s));
   }
   const thorin_params *p = &c->p;
   g.adiabind = p->adiabind;
@@ -1423,6 +1461,14 @@ extern "C" void thorin_dev_destroy(thorin_ctx *c) {
   if (d->red) (void)hipFree(d->red);
   if (d->sorp) (void)hipFree(d->sorp);
   if (d->sorc) (void)hipFree(d->sorc);
+  if (d->prof_pend) {
+    for (auto &r : *d->prof_pend) { (void)hipEventDestroy(r.second.first); (void)hipEventDestroy(r.second.second); }
+    delete d->prof_pend;
+  }
+  if (d->prof_pool) {
+    for (auto e : *d->prof_pool) (void)hipEventDestroy(e);
+    delete d->prof_pool;
+  }
   if (d->s) (void)hipStreamDestroy(d->s);
   free(d);
   c->dev = NULL;
@@ -1498,6 +1544,34 @@ extern "C" int thorin_ring_means(thorin_ctx *c, int field, double *out) {
   return 0;
 }
 
+__global__ void k_int_to_double(const int *__restrict__ a, double *__restrict__ b, int n) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) b[k] = (double)a[k];
+}
+
+// 1D device profiles: 0 globcsvec, 1 globtempervec, 2 Vmoy (ConditionCFL),
+// 3 VMed and 4 Nshift (ComputeResiduals) of the last substep
+extern "C" int thorin_ring_profile(thorin_ctx *c, int which, double *out) {
+  if (need_dev(c)) return -1;
+  thorin_dev *d = c->dev;
+  const double *src = NULL;
+  switch (which) {
+    case 0: src = d->g.gcs; break;
+    case 1: src = d->g.gtemp; break;
+    case 2: src = d->g.Vmoy; break;
+    case 3: src = d->g.VMed; break;
+    case 4:
+      hipLaunchKernelGGL(k_int_to_double, dim3((d->nr + 255) / 256), dim3(256), 0, d->s, d->g.Nshift, d->ringtmp, d->nr);
+      HCHECK(hipGetLastError());
+      src = d->ringtmp;
+      break;
+    default: thorin_set_error("unknown ring profile"); return -1;
+  }
+  HCHECK(hipMemcpyAsync(out, src, sizeof(double) * d->nr, hipMemcpyDeviceToHost, d->s));
+  HCHECK(hipStreamSynchronize(d->s));
+  return 0;
+}
+
 extern "C" int thorin_gas_total_mass(thorin_ctx *c, double *out) {
   if (need_dev(c)) return -1;
   thorin_dev *d = c->dev;
@@ -1512,6 +1586,75 @@ extern "C" int thorin_gas_total_mass(thorin_ctx *c, double *out) {
   return 0;
 }
 
+static hipEvent_t prof_event(thorin_dev *d) {
+  if (!d->prof_pool->empty()) {
+    hipEvent_t e = d->prof_pool->back();
+    d->prof_pool->pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+static int prof_flush(thorin_dev *d) {
+  if (!d->prof_pend || d->prof_pend->empty()) return 0;
+  HCHECK(hipStreamSynchronize(d->s));
+  for (auto &r : *d->prof_pend) {
+    float ms = 0.f;
+    HCHECK(hipEventElapsedTime(&ms, r.second.first, r.second.second));
+    d->prof_ms[r.first] += ms;
+    d->prof_cnt[r.first] += 1;
+    d->prof_pool->push_back(r.second.first);
+    d->prof_pool->push_back(r.second.second);
+  }
+  d->prof_pend->clear();
+  return 0;
+}
+
+// launch on the context's stream; with the timer on, bracket the kernels of
+// the selected ids with HIP events on that same stream
+#define LAUNCH(kern, grid, block, shmem, ...)                                              \
+  do {                                                                                     \
+    int kid_ = d->prof_on ? kid_of(#kern) : -1;                                            \
+    hipEvent_t ea_ = nullptr, eb_ = nullptr;                                               \
+    if (kid_ >= 0 && ((d->prof_mask >> kid_) & 1ull)) {                                    \
+      ea_ = prof_event(d);                                                                 \
+      eb_ = prof_event(d);                                                                 \
+      if (ea_ && eb_) (void)hipEventRecord(ea_, d->s);                                     \
+    }                                                                                      \
+    hipLaunchKernelGGL(kern, grid, block, shmem, d->s, __VA_ARGS__);                       \
+    if (ea_ && eb_) {                                                                      \
+      (void)hipEventRecord(eb_, d->s);                                                     \
+      d->prof_pend->push_back(std::make_pair(kid_, std::make_pair(ea_, eb_)));             \
+    }                                                                                      \
+  } while (0)
+
+extern "C" int thorin_prof_enable(thorin_ctx *c, int on, unsigned long long mask) {
+  if (need_dev(c)) return -1;
+  thorin_dev *d = c->dev;
+  if (prof_flush(d)) return -1;
+  d->prof_on = on;
+  d->prof_mask = mask;
+  for (int k = 0; k < 64; k++) { d->prof_ms[k] = 0.0; d->prof_cnt[k] = 0; }
+  return 0;
+}
+
+extern "C" int thorin_prof_read(thorin_ctx *c, int kid, double *total_ms, long long *count) {
+  if (need_dev(c)) return -1;
+  if (kid < 0 || kid >= 64) return -1;
+  if (prof_flush(c->dev)) return -1;
+  *total_ms = c->dev->prof_ms[kid];
+  *count = c->dev->prof_cnt[kid];
+  return 0;
+}
+
+extern "C" const char *thorin_prof_kernel_name(int kid) {
+  int n = 0;
+  while (KNAMES[n]) n++;
+  return (kid >= 0 && kid < n) ? KNAMES[kid] : NULL;
+}
+
 // --------------------------------------------------------------------------
 // SOR driver
 
@@ -1521,9 +1664,9 @@ static int sor_solve(thorin_ctx *c, double omega, int *niter_out) {
   const int nr = d->nr, ns = d->ns;
   dim3 gc = cell_grid(d, nr - 2);
   const int nb0 = gc.x * gc.y;
-  hipLaunchKernelGGL(k_sor_resid0, gc, dim3(TB), 0, d->s, g, d->b[B_TEMPER], d->b[B_ML], d->b[B_MLIP],
-                     d->b[B_MLIM], d->b[B_MLJP], d->b[B_MLJM], d->b[B_RHS], d->sorp);
-  hipLaunchKernelGGL(k_sum_to, dim3(1), dim3(RB), 0, d->s, d->sorp, nb0, &d->ctl->sor_normres0);
+  LAUNCH(k_sor_resid0, gc, dim3(TB), 0, g, d->b[B_TEMPER], d->b[B_ML], d->b[B_MLIP], d->b[B_MLIM], d->b[B_MLJP],
+         d->b[B_MLJM], d->b[B_RHS], d->sorp);
+  LAUNCH(k_sum_to, dim3(1), dim3(RB), 0, d->sorp, nb0, &d->ctl->sor_normres0);
   HCHECK(hipMemsetAsync(&d->ctl->sor_done, 0, sizeof(int) * 3 + sizeof(unsigned int), d->s));
   dim3 gs((unsigned)(((ns / 2) + TB - 1) / TB), (unsigned)(nr - 2));
   int launched = 0;
@@ -1531,17 +1674,16 @@ static int sor_solve(thorin_ctx *c, double omega, int *niter_out) {
   if (batch > 256) batch = 256;
   for (;;) {
     for (int b = 0; b < batch; b++) {
-      hipLaunchKernelGGL(k_sor_half, gs, dim3(TB), 0, d->s, g, d->b[B_TEMPER], d->b[B_ML], d->b[B_MLIP],
-                         d->b[B_MLIM], d->b[B_MLJP], d->b[B_MLJM], d->b[B_RHS], omega, 0, d->sorp, d->sorc, d->ctl,
-                         SORMAXITERS);
-      hipLaunchKernelGGL(k_sor_half, gs, dim3(TB), 0, d->s, g, d->b[B_TEMPER], d->b[B_ML], d->b[B_MLIP],
-                         d->b[B_MLIM], d->b[B_MLJP], d->b[B_MLJM], d->b[B_RHS], omega, 1, d->sorp, d->sorc, d->ctl,
-                         SORMAXITERS);
+      LAUNCH(k_sor_half, gs, dim3(TB), 0, g, d->b[B_TEMPER], d->b[B_ML], d->b[B_MLIP], d->b[B_MLIM],
+             d->b[B_MLJP], d->b[B_MLJM], d->b[B_RHS], omega, 0, d->sorp, d->sorc, d->ctl, SORMAXITERS);
+      LAUNCH(k_sor_half, gs, dim3(TB), 0, g, d->b[B_TEMPER], d->b[B_ML], d->b[B_MLIP], d->b[B_MLIM],
+             d->b[B_MLJP], d->b[B_MLJM], d->b[B_RHS], omega, 1, d->sorp, d->sorc, d->ctl, SORMAXITERS);
     }
     HCHECK(hipGetLastError());
     launched += batch;
     HCHECK(hipMemcpyAsync(d->hctl, d->ctl, sizeof(DevCtl), hipMemcpyDeviceToHost, d->s));
     HCHECK(hipStreamSynchronize(d->s));
+    if (prof_flush(d)) return -1;
     if (d->hctl->sor_done) { *niter_out = d->hctl->sor_niter; return 0; }
     if (launched > SORMAXITERS + 2) { *niter_out = SORMAXITERS + 1; return 0; }
     batch = batch < 64 ? 2 * batch : 128;
@@ -1610,10 +1752,6 @@ static inline void swapb(thorin_dev *d, int a, int b) {
   d->b[b] = t;
 }
 
-#define LAUNCH(kern, grid, block, shmem, ...)                                    \
-  do {                                                                           \
-    hipLaunchKernelGGL(kern, grid, block, shmem, d->s, __VA_ARGS__);             \
-  } while (0)
 
 // AlgoGas (SourceEuler.c:210-317): one DT
 extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_fn nbody, void *nbody_user,
@@ -1722,7 +1860,8 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
       // UpdateLog (Force.c:27-70) at the first substep of the DT
       std::vector<double> rho1d(nr), cs1d(nr);
       if (thorin_ring_means(c, THORIN_F_DENS, rho1d.data())) return -1;
-      HCHECK(hipMemcpy(cs1d.data(), d->g.gcs, sizeof(double) * nr, hipMemcpyDeviceToHost));
+      HCHECK(hipMemcpyAsync(cs1d.data(), d->g.gcs, sizeof(double) * nr, hipMemcpyDeviceToHost, d->s));
+      HCHECK(hipStreamSynchronize(d->s));
       log->tq_rows = sys->nb < 8 ? sys->nb : 8;
       for (int k = 0; k < log->tq_rows; k++) {
         const double x = sys->x[k], y = sys->y[k], ax = sys->ax[k], ay = sys->ay[k], m = sys->mass[k];
@@ -1752,7 +1891,8 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
     if (anyfeel) {
       std::vector<double> rmean(nr), csmean(nr);
       if (thorin_ring_means(c, THORIN_F_DENS, rmean.data())) return -1;
-      HCHECK(hipMemcpy(csmean.data(), d->g.gcs, sizeof(double) * nr, hipMemcpyDeviceToHost));
+      HCHECK(hipMemcpyAsync(csmean.data(), d->g.gcs, sizeof(double) * nr, hipMemcpyDeviceToHost, d->s));
+      HCHECK(hipStreamSynchronize(d->s));
       const double indx = -d->hctl->axstar, indy = -d->hctl->aystar;
       for (int k = 0; k < sys->nb; k++) {
         if (!sys->feeldisk[k]) continue;
@@ -1884,6 +2024,7 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
     // crash flags of this substep (read back with the next CFL copy; fetched now for the log)
     HCHECK(hipMemcpyAsync(d->hctl, d->ctl, sizeof(DevCtl), hipMemcpyDeviceToHost, d->s));
     HCHECK(hipStreamSynchronize(d->s));
+    if (prof_flush(d)) return -1;
     if (d->hctl->crash_e) {
       c->st.already_crashed++;
       if (log) { log->crashed[log->nsub - 1] = 1; log->crashed_any = 1; }

@@ -218,8 +218,20 @@ int  thorin_algogas(thorin_ctx *ctx, thorin_planets *sys, thorin_nbody_fn nbody,
  * (nrad doubles each) and the last disk->planet accelerations in sys. */
 int  thorin_ring_means(thorin_ctx *ctx, int field, double *out);
 
+/* Ring profiles of the last substep (nrad doubles): 0 globcsvec,
+ * 1 globtempervec (mpiTasks.c:12-62), 2 Vmoy (SourceEuler.c:609-618),
+ * 3 VMed and 4 Nshift (TransportEuler.c:85-95). */
+int  thorin_ring_profile(thorin_ctx *ctx, int which, double *out);
+
 /* Total gas mass (GasTotalMass, SideEuler.c:26-51), reduced on the GPU. */
 int  thorin_gas_total_mass(thorin_ctx *ctx, double *out);
+
+/* Kernel timing with HIP events recorded on the context's stream around the
+ * kernels whose id bit is set in mask (ids: thorin_prof_kernel_name).
+ * Enabling resets the totals; read returns total milliseconds and launches. */
+int  thorin_prof_enable(thorin_ctx *ctx, int on, unsigned long long mask);
+int  thorin_prof_read(thorin_ctx *ctx, int kernel_id, double *total_ms, long long *count);
+const char *thorin_prof_kernel_name(int kernel_id);
 
 /* Synchronise the device and return the library's last error text. */
 int  thorin_sync(thorin_ctx *ctx);
