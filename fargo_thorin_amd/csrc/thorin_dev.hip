@@ -55,17 +55,17 @@ enum {
   B_AGR = THORIN_F_AGR, B_AGT = THORIN_F_AGT, B_EINT = THORIN_F_ENERGYINT,
   B_ENEW = THORIN_F_ENERGYNEW, B_VRADINT = THORIN_F_VRADINT, B_VTHETAINT = THORIN_F_VTHETAINT,
   B_VRADNEW = THORIN_F_VRADNEW, B_VTHETANEW = THORIN_F_VTHETANEW,
-  B_DIFF = THORIN_NFIELDS, B_ML, B_MLIP, B_MLIM, B_MLJP, B_MLJM, B_RHS, B_TBACK,
+  B_DIFF = THORIN_NFIELDS, B_ML, B_MLIP, B_MLIM, B_MLJP, B_MLJM, B_RHS, B_TBACK, B_TEMPER2,
   B_RP, B_RM, B_TP, B_TM, B_XRP, B_XRM, B_XTP, B_XTM, B_XE, B_XRHO, B_ABS, B_ORD,
   NBUF
 };
 
 struct DevCtl {                 // device-side control block, mirrored to pinned host memory
   double newdt_raw;             // ConditionCFL minimum before the clamps
-  unsigned long long dtst_bits; // DiffusionTimestep minimum (positive double as bits)
+  double dt_stellar;            // DiffusionTimestep minimum
   int crash_e, crash_rho;
   int sor_done, sor_niter, sor_iter;
-  unsigned int sor_count;       // blocks finished in the current SOR iteration
+  int sor_cur;                  // T buffer holding the latest iterate (0: B_TEMPER, 1: B_TEMPER2)
   double sor_normres0, sor_normres;
   double axstar, aystar;
   double axpl[MAXPL], aypl[MAXPL];
@@ -84,6 +84,7 @@ struct Geo {                    // passed by value to every kernel
   const double *pow_rmed15, *pow_rinf15, *cav_med, *cav_inf, *cfl_pow2, *supp_torque;
   const int *dampring;
   double *gcs, *gtemp, *Vmoy, *VMed, *ringmin;
+  double *sig_avr, *ist_avr;    // ring-averaged vertical integrals of FillForcesArrays
   int *Nshift;
   double adiabind, sqrt_adiabind_inv, cv, viscosity, alphavisc, alphamri, alphadead, tmri, twidth;
   double t2si, opa2cu, rho2cgs, stefan, sqrt2pi_inv, sqrt3, opacitydrop, parametricopacity;
@@ -111,6 +112,9 @@ struct thorin_dev {
   double *ringtmp;              // per-ring scratch
   int nblk_cells, nblk_sor;
   int opa_filled;
+  int normres0_valid;           // ctl->sor_normres0 holds the residual of the current matrix
+  int cfl_ready;                // hctl->newdt_raw holds the CFL minimum of the current fields
+  double cfl_invdtreb;          // MinStepForRebound term it was formed with
   // kernel timing with HIP events (thorin_prof_*)
   int prof_on;
   unsigned long long prof_mask;
@@ -122,9 +126,9 @@ struct thorin_dev {
 
 // kernels that the event timer knows, by launch name
 static const char *const KNAMES[] = {
-    "k_cfl_ring", "k_cfl_final", "k_forces", "k_forces_reduce", "k_indirect", "k_boundary", "k_substep1",
+    "k_eos_cfl", "k_cfl_final", "k_forces", "k_forces_reduce", "k_indirect", "k_boundary", "k_substep1",
     "k_divstress", "k_visc_update", "k_substep2", "k_qplus", "k_temper_cs", "k_opacity_qminus", "k_diffcoef",
-    "k_matrix", "k_sor_resid0", "k_sum_to", "k_sor_half", "k_energy_new", "k_transport_rad",
+    "k_matrix", "k_sor_resid0", "k_sum_to", "k_min_to", "k_sor_iter", "k_sor_check", "k_energy_new", "k_transport_rad",
     "k_transport_theta", "k_velocities", NULL};
 
 static int kid_of(const char *name) {
@@ -136,26 +140,23 @@ static int kid_of(const char *name) {
 // ---------------------------------------------------------------------------
 // small device helpers
 
-__device__ __forceinline__ double fvisc_med(const Geo &g, int i) {
-  // FViscosity(Rmed[i]), Viscosity.c:26-54 (the radial search lands on ring i)
+// FViscosity (Viscosity.c:26-54) at Rmed[i] (pw = pow_rmed15, cav = cav_med)
+// or Rinf[i] (pow_rinf15, cav_inf); the radial search of the reference lands
+// on ring i, whose ring means are gcs_i and gtemp_i.
+__device__ __forceinline__ double fvisc_val(const Geo &g, double gcs_i, double gtemp_i, double pw, double cav) {
   double v = g.viscosity;
-  if (g.visc_alpha) v = g.alphavisc * g.gcs[i] * g.gcs[i] * g.pow_rmed15[i];
+  if (g.visc_alpha) v = g.alphavisc * gcs_i * gcs_i * pw;
   if (g.alpha_flock) {
-    double a = (g.alphamri - g.alphadead) * (1.0 - tanh((g.tmri - g.gtemp[i] * g.t2si) / g.twidth)) / 2.0 + g.alphadead;
-    v = a * g.gcs[i] * g.gcs[i] * g.pow_rmed15[i];
+    const double a = (g.alphamri - g.alphadead) * (1.0 - tanh((g.tmri - gtemp_i * g.t2si) / g.twidth)) / 2.0 + g.alphadead;
+    v = a * gcs_i * gcs_i * pw;
   }
-  return v * g.cav_med[i];
+  return v * cav;
 }
-
+__device__ __forceinline__ double fvisc_med(const Geo &g, int i) {
+  return fvisc_val(g, g.gcs[i], g.gtemp[i], g.pow_rmed15[i], g.cav_med[i]);
+}
 __device__ __forceinline__ double fvisc_inf(const Geo &g, int i) {
-  // FViscosity(Rinf[i])
-  double v = g.viscosity;
-  if (g.visc_alpha) v = g.alphavisc * g.gcs[i] * g.gcs[i] * g.pow_rinf15[i];
-  if (g.alpha_flock) {
-    double a = (g.alphamri - g.alphadead) * (1.0 - tanh((g.tmri - g.gtemp[i] * g.t2si) / g.twidth)) / 2.0 + g.alphadead;
-    v = a * g.gcs[i] * g.gcs[i] * g.pow_rinf15[i];
-  }
-  return v * g.cav_inf[i];
+  return fvisc_val(g, g.gcs[i], g.gtemp[i], g.pow_rinf15[i], g.cav_inf[i]);
 }
 
 // x^3 rounded once (double-double product): glibc's pow(x,3.0) is correctly
@@ -268,31 +269,104 @@ __device__ __forceinline__ double block_min(double v, double *sh) {
   return r;
 }
 
-// sequential ring sum in the reference's order: thread 0 walks the ring in LDS
+// sequential ring sum in the reference's order (one thread walks the ring);
+// the LDS reads of the next 16 values are issued before the 16 dependent
+// adds of the current batch, so the chain runs at the FP64 add latency.
 __device__ __forceinline__ double ordered_ring_sum(const double *sh, int ns) {
   double s = 0.0;
-  for (int j = 0; j < ns; j++) s += sh[j];
+  double a[16], b[16];
+  int j = 0;
+  if (ns >= 16) {
+#pragma unroll
+    for (int k = 0; k < 16; k++) a[k] = sh[k];
+    for (j = 16; j + 16 <= ns; j += 16) {
+#pragma unroll
+      for (int k = 0; k < 16; k++) b[k] = sh[j + k];
+#pragma unroll
+      for (int k = 0; k < 16; k++) s += a[k];
+#pragma unroll
+      for (int k = 0; k < 16; k++) a[k] = b[k];
+    }
+#pragma unroll
+    for (int k = 0; k < 16; k++) s += a[k];
+  }
+  for (; j < ns; j++) s += sh[j];
   return s;
 }
 
 // ---------------------------------------------------------------------------
-// ConditionCFL (SourceEuler.c:589-712): per-ring Vmoy (ordered) and per-cell
-// limits; the block minimum goes to ringmin[i].
-__global__ void __launch_bounds__(RB) k_cfl_ring(Geo g, const double *__restrict__ vr,
-                                                 const double *__restrict__ vt,
-                                                 const double *__restrict__ cs, double invdtreb_sq) {
-  extern __shared__ double sh[];       // ns + RB doubles
-  double *red = sh + g.ns;
-  const int i = blockIdx.x, ns = g.ns, nr = g.nr;
-  const double *vti = vt + (size_t)i * ns;
-  for (int j = threadIdx.x; j < ns; j += RB) sh[j] = vti[j];
+// End of a substep, one workgroup per ring:
+//  * ComputeTemperatureField + ComputeSoundSpeed (SourceEuler.c:765-792,
+//    715-737) from rho and the energy, with the ordered ring means
+//    globcsvec (and globtempervec when the Flock alpha needs it);
+//  * the per-cell part of the next substep's ConditionCFL
+//    (SourceEuler.c:589-672: ordered Vmoy, the five inverse time limits),
+//    whose block minimum goes to ringmin[i];
+//  * the ring-averaged vertical integrals of the next FillForcesArrays
+//    (Pframeforce.c:150-189), which only depend on globcsvec.
+// Three lanes in three different waves walk their ring sums concurrently.
+// With do_eos = 0 the temperature and sound speed already in memory are used.
+__global__ void __launch_bounds__(RB) k_eos_cfl(Geo g, const double *__restrict__ rho, const double *__restrict__ e,
+                                                double *__restrict__ T, double *__restrict__ cs,
+                                                const double *__restrict__ vr, const double *__restrict__ vt,
+                                                double invdtreb_sq, int do_eos) {
+  extern __shared__ double sh[];       // 3*ns + RB: vt, cs, T, reduction
+  const int i = blockIdx.x, ns = g.ns, nr = g.nr, tid = threadIdx.x;
+  double *s_vt = sh, *s_cs = sh + ns, *s_t = sh + 2 * ns, *red = sh + 3 * ns;
+  const double gam = g.adiabind;
+  const size_t base = (size_t)i * ns;
+  for (int j = tid; j < ns; j += RB) {
+    const size_t l = base + j;
+    s_vt[j] = vt[l];
+    if (do_eos) {
+      const double tmp2 = 1.0 / (rho[l] * 1.0);
+      const double tmp1 = e[l] * 1.0 * (gam - 1.0);
+      const double t = tmp1 * tmp2;
+      const double c = sqrt(gam * (gam - 1.0) * fabs(e[l]) / rho[l]);
+      T[l] = t;
+      cs[l] = c;
+      s_t[j] = t;
+      s_cs[j] = c;
+    } else {
+      s_cs[j] = cs[l];
+      s_t[j] = T[l];
+    }
+  }
   __syncthreads();
-  __shared__ double vmoy;
-  if (threadIdx.x == 0) {
-    double s = ordered_ring_sum(sh, ns);
-    s /= (double)ns;
-    vmoy = s;
-    g.Vmoy[i] = s;
+  __shared__ double vmoy, gcs_i, gtemp_i;
+  if (tid == 0) {
+    const double v = ordered_ring_sum(s_vt, ns) / (double)ns;
+    vmoy = v;
+    g.Vmoy[i] = v;
+  } else if (tid == 64) {
+    const double c = ordered_ring_sum(s_cs, ns) / (double)ns;
+    gcs_i = c;
+    g.gcs[i] = c;
+    // ring-averaged Gaussian vertical integrals of FillForcesArrays
+    const double H_avr = c * g.OmegaInv[i] * g.sqrt_adiabind_inv;
+    const double H2 = H_avr * H_avr;
+    const double r2 = g.Rmed[i] * g.Rmed[i];
+    const double deltaz = 3.0 * H_avr / (double)10;
+    double sig = 0.0, ist = 0.0;
+    for (int m = 1; m <= 10; m++) {
+      const double znode = ((double)m - 0.5) * deltaz;
+      const double znode2 = znode * znode;
+      const double sum = exp(-(0.5 * znode2 / H2));
+      sig += sum;
+      const double d2 = r2 + znode2;
+      const double dd = sqrt(d2);
+      const double denom = dd * d2;
+      ist += sum / denom;
+    }
+    g.sig_avr[i] = sig;
+    g.ist_avr[i] = ist;
+  } else if (tid == 128) {
+    double t = g.gtemp[i];
+    if (g.alpha_flock && do_eos) {
+      t = ordered_ring_sum(s_t, ns) / (double)ns;
+      g.gtemp[i] = t;
+    }
+    gtemp_i = t;
   }
   __syncthreads();
   double m = 1e30;
@@ -301,20 +375,20 @@ __global__ void __launch_bounds__(RB) k_cfl_ring(Geo g, const double *__restrict
     const double dxtheta = g.Rmed[i] * 2.0 * DPI / (double)ns;
     const double mn = (dxtheta < dxrad) ? dxtheta : dxrad;
     const int viscous = g.visc_alpha || (g.viscosity != 0.0);
-    const double visc = viscous ? fvisc_med(g, i) : 0.0;
+    const double visc = viscous ? fvisc_val(g, gcs_i, gtemp_i, g.pow_rmed15[i], g.cav_med[i]) : 0.0;
     const double invdt5 = viscous ? visc * 4. / g.cfl_pow2[i] : 1e-30;
-    for (int j = threadIdx.x; j < ns; j += RB) {
-      const size_t l = (size_t)i * ns + j;
+    for (int j = tid; j < ns; j += RB) {
+      const size_t l = base + j;
       const int jp = (j == ns - 1) ? 0 : j + 1;
-      const double vres = g.fast_transport ? sh[j] - vmoy : sh[j];
-      const double invdt1 = cs[l] / mn;
+      const double vres = g.fast_transport ? s_vt[j] - vmoy : s_vt[j];
+      const double invdt1 = s_cs[j] / mn;
       const double invdt2 = fabs(vr[l]) / dxrad;
       const double invdt3 = fabs(vres) / dxtheta;
       double dvr = vr[l + ns] - vr[l];
-      double dvt = sh[jp] - sh[j];
+      double dvt = s_vt[jp] - s_vt[j];
       if (dvr >= 0.0) dvr = 1e-10; else dvr = -dvr;
       if (dvt >= 0.0) dvt = 1e-10; else dvt = -dvt;
-      double a = dvr / dxrad, b = dvt / dxtheta;
+      const double a = dvr / dxrad, b = dvt / dxtheta;
       double invdt4 = (b > a) ? b : a;
       invdt4 *= 4.0 * CVNR * CVNR;
       const double dt = CFLSECURITY / sqrt(invdt1 * invdt1 + invdt2 * invdt2 + invdt3 * invdt3 + invdt4 * invdt4 +
@@ -323,7 +397,7 @@ __global__ void __launch_bounds__(RB) k_cfl_ring(Geo g, const double *__restrict
     }
   }
   m = block_min<RB>(m, red);
-  if (threadIdx.x == 0) g.ringmin[i] = m;
+  if (tid == 0) g.ringmin[i] = m;
 }
 
 // minimum over rings plus the shear limit between rings (SourceEuler.c:673-676)
@@ -363,22 +437,11 @@ __global__ void __launch_bounds__(TB) k_forces(Geo g, const double *__restrict__
     const double r2 = g.Rmed[i] * g.Rmed[i];
     double Rsup_Rinf2 = g.Rsup[i] - g.Rinf[i];
     Rsup_Rinf2 *= Rsup_Rinf2;
-    // ring average (cs_avr equals globcsvec[i]: same sequential sum)
+    // ring average (cs_avr equals globcsvec[i]: the same sequential sum); its
+    // vertical integrals were formed with the ring means
     const double H_avr = g.gcs[i] * omegainv * g.sqrt_adiabind_inv;
-    double H2 = H_avr * H_avr;
-    double zmax = 3.0 * H_avr;
-    double deltaz = zmax / (double)10;
-    double sigma_avr = 0.0, integstar_avr = 0.0;
-    for (int m = 1; m <= 10; m++) {
-      double znode = ((double)m - 0.5) * deltaz;
-      double znode2 = znode * znode;
-      double sum = exp(-(0.5 * znode2 / H2));
-      sigma_avr += sum;
-      double d2 = r2 + znode2;
-      double d = sqrt(d2);
-      double denom = d * d2;
-      integstar_avr += sum / denom;
-    }
+    double H2, zmax, deltaz;
+    const double sigma_avr = g.sig_avr[i], integstar_avr = g.ist_avr[i];
     int IntegrateLocally = 0, Nvert = 10;
     double ax = 0.0, ay = 0.0;
     const double x = abs_[l], y = ord[l];
@@ -744,35 +807,22 @@ __global__ void __launch_bounds__(TB) k_qplus(Geo g, const double *__restrict__ 
   }
 }
 
-// ComputeTemperatureField + ComputeSoundSpeed (SourceEuler.c:715-792) from an
-// energy field, with the ordered ring means globtempervec / globcsvec.
-__global__ void __launch_bounds__(RB) k_temper_cs(Geo g, const double *__restrict__ rho,
+// ComputeTemperatureField + ComputeSoundSpeed (SourceEuler.c:715-792) from
+// the intermediate energy, inside ImplicitRadiativeDiffusion
+// (EnergySources.c:234-235).  The ring means these calls also update are
+// overwritten at the end of the substep before anything reads them, so they
+// are not formed here.
+__global__ void __launch_bounds__(TB) k_temper_cs(Geo g, const double *__restrict__ rho,
                                                   const double *__restrict__ e, double *__restrict__ T,
                                                   double *__restrict__ cs) {
-  extern __shared__ double sh[];      // 2*ns
-  const int i = blockIdx.x, ns = g.ns;
+  const int ns = g.ns, i = blockIdx.y, j = blockIdx.x * TB + threadIdx.x;
+  if (j >= ns) return;
+  const size_t l = (size_t)i * ns + j;
   const double gam = g.adiabind;
-  for (int j = threadIdx.x; j < ns; j += RB) {
-    const size_t l = (size_t)i * ns + j;
-    const double tmp2 = 1.0 / (rho[l] * 1.0);
-    const double tmp1 = e[l] * 1.0 * (gam - 1.0);
-    const double t = tmp1 * tmp2;
-    const double c = sqrt(gam * (gam - 1.0) * fabs(e[l]) / rho[l]);
-    T[l] = t;
-    cs[l] = c;
-    sh[j] = t;
-    sh[ns + j] = c;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double s = ordered_ring_sum(sh, ns);
-    s /= (double)ns;
-    g.gtemp[i] = s;
-  } else if (threadIdx.x == 64) {
-    double s = ordered_ring_sum(sh + ns, ns);
-    s /= (double)ns;
-    g.gcs[i] = s;
-  }
+  const double tmp2 = 1.0 / (rho[l] * 1.0);
+  const double tmp1 = e[l] * 1.0 * (gam - 1.0);
+  T[l] = tmp1 * tmp2;
+  cs[l] = sqrt(gam * (gam - 1.0) * fabs(e[l]) / rho[l]);
 }
 
 // MidplaneVolumeDensity + OpacityProfile + CalculateQminus
@@ -829,7 +879,7 @@ __device__ __forceinline__ double flux_limiter(double s) {   // EnergySources.c:
 __global__ void __launch_bounds__(TB) k_diffcoef(Geo g, const double *__restrict__ rho,
                                                  const double *__restrict__ cs, const double *__restrict__ T,
                                                  const double *__restrict__ opa, double *__restrict__ D,
-                                                 DevCtl *ctl) {
+                                                 double *__restrict__ pmin) {
   __shared__ double red[TB];
   const int ns = g.ns, nr = g.nr, i = blockIdx.y, j = blockIdx.x * TB + threadIdx.x;
   double dtmin = 1.0e38;
@@ -849,8 +899,16 @@ __global__ void __launch_bounds__(TB) k_diffcoef(Geo g, const double *__restrict
     if (dttmp < dtmin) dtmin = dttmp;
   }
   dtmin = block_min<TB>(dtmin, red);
-  if (threadIdx.x == 0 && dtmin > 0.0)
-    atomicMin(&ctl->dtst_bits, (unsigned long long)__double_as_longlong(dtmin));
+  if (threadIdx.x == 0) pmin[blockIdx.y * gridDim.x + blockIdx.x] = dtmin;
+}
+
+// minimum of the per-block partial minima (order-independent)
+__global__ void __launch_bounds__(RB) k_min_to(const double *__restrict__ pmin, int nb, double *out) {
+  __shared__ double red[RB];
+  double m = 1.0e38;
+  for (int b = threadIdx.x; b < nb; b += RB) { const double v = pmin[b]; if (v < m) m = v; }
+  m = block_min<RB>(m, red);
+  if (threadIdx.x == 0) *out = m;
 }
 
 // implicit energy matrix (EnergySources.c:256-297) with A and B rebuilt on the fly
@@ -867,6 +925,9 @@ __device__ __forceinline__ double coefB(const Geo &g, const double *D, const dou
   return 0.5 * (D[l] + D[ljm]) * invdxtheta2 * 2.0 * H;
 }
 
+// The matrix kernel also returns the initial SOR residual sum
+// (EnergySources.c:435-449): block partials of |T.M - RHS| from the values
+// it has just computed.
 __global__ void __launch_bounds__(TB) k_matrix(Geo g, const double *__restrict__ rho,
                                                const double *__restrict__ cs, const double *__restrict__ D,
                                                const double *__restrict__ divv,
@@ -874,22 +935,35 @@ __global__ void __launch_bounds__(TB) k_matrix(Geo g, const double *__restrict__
                                                const double *__restrict__ qplus, const double *__restrict__ T,
                                                double *__restrict__ Ml, double *__restrict__ Mlip,
                                                double *__restrict__ Mlim, double *__restrict__ Mljp,
-                                               double *__restrict__ Mljm, double *__restrict__ RHS, double dt) {
-  const int ns = g.ns, i = blockIdx.y + 1, j = blockIdx.x * TB + threadIdx.x;
-  if (j >= ns) return;   // rings 1..nr-2 (grid.y = nr-2)
-  const size_t l = (size_t)i * ns + j, lip = l + ns;
-  const int jp = (j == ns - 1) ? 0 : j + 1;
-  const double Al = coefA(g, D, i, l), Alip = coefA(g, D, i + 1, lip);
-  const double Bl = coefB(g, D, cs, i, j), Bljp = coefB(g, D, cs, i, jp);
-  const double H = cs[l] * g.OmegaInv[i] * g.sqrt_adiabind_inv;
-  const double bfac = dt / (rho[l] * g.cv);
-  const double afac = bfac * g.InvDiffRsup[i] * 2.0 * H / g.Rmed[i];
-  Ml[l] = 1.0 + divv[l] * (g.adiabind - 1.0) * dt + 4.0 * bfac * qminus[l] + afac * (Alip + Al) + bfac * (Bljp + Bl);
-  Mlip[l] = -afac * Alip;
-  Mlim[l] = -afac * Al;
-  Mljp[l] = -bfac * Bljp;
-  Mljm[l] = -bfac * Bl;
-  RHS[l] = T[l] + bfac * (qplus[l] + 3.0 * qminus[l] * T[l]);
+                                               double *__restrict__ Mljm, double *__restrict__ RHS, double dt,
+                                               double *__restrict__ presid) {
+  __shared__ double red[TB];
+  const int ns = g.ns, i = blockIdx.y + 1, j = blockIdx.x * TB + threadIdx.x;   // rings 1..nr-2
+  double r0 = 0.0;
+  if (j < ns) {
+    const size_t l = (size_t)i * ns + j, lip = l + ns, lim = l - ns;
+    const int jp = (j == ns - 1) ? 0 : j + 1;
+    const size_t ljp = (size_t)i * ns + jp;
+    const size_t ljm = (j == 0) ? (size_t)i * ns + ns - 1 : l - 1;
+    const double Al = coefA(g, D, i, l), Alip = coefA(g, D, i + 1, lip);
+    const double Bl = coefB(g, D, cs, i, j), Bljp = coefB(g, D, cs, i, jp);
+    const double H = cs[l] * g.OmegaInv[i] * g.sqrt_adiabind_inv;
+    const double bfac = dt / (rho[l] * g.cv);
+    const double afac = bfac * g.InvDiffRsup[i] * 2.0 * H / g.Rmed[i];
+    const double ml = 1.0 + divv[l] * (g.adiabind - 1.0) * dt + 4.0 * bfac * qminus[l] + afac * (Alip + Al) + bfac * (Bljp + Bl);
+    const double mlip = -afac * Alip, mlim = -afac * Al, mljp = -bfac * Bljp, mljm = -bfac * Bl;
+    const double tl = T[l];
+    const double rhs = tl + bfac * (qplus[l] + 3.0 * qminus[l] * tl);
+    Ml[l] = ml;
+    Mlip[l] = mlip;
+    Mlim[l] = mlim;
+    Mljp[l] = mljp;
+    Mljm[l] = mljm;
+    RHS[l] = rhs;
+    r0 = fabs(tl * ml + T[lip] * mlip + T[lim] * mlim + T[ljp] * mljp + T[ljm] * mljm - rhs);
+  }
+  r0 = block_sum<TB>(r0, red);
+  if (threadIdx.x == 0) presid[blockIdx.y * gridDim.x + blockIdx.x] = r0;
 }
 
 // ---------------------------------------------------------------------------
@@ -930,64 +1004,119 @@ __global__ void __launch_bounds__(RB) k_sum_to(const double *__restrict__ partia
   if (threadIdx.x == 0) *out = s;
 }
 
-// one half-sweep; the block that finishes last in pass 1 evaluates the stop test
-__global__ void __launch_bounds__(TB) k_sor_half(Geo g, double *__restrict__ T, const double *__restrict__ Ml,
-                                                 const double *__restrict__ Mlip, const double *__restrict__ Mlim,
-                                                 const double *__restrict__ Mljp, const double *__restrict__ Mljm,
-                                                 const double *__restrict__ RHS, double omega, int pass,
-                                                 double *__restrict__ psum, int *__restrict__ pcnt,
-                                                 DevCtl *ctl, int maxiter) {
+// One SOR iteration (both half-sweeps) per launch.  A workgroup owns a tile
+// of SOR_TI rings x SOR_TJ sectors and stages T with a two-cell halo in LDS.
+// The red half-sweep is also applied to the halo cells next to the tile,
+// with the same inputs and arithmetic as their owner, so the black
+// half-sweep of the owned cells sees exactly the values a sequential sweep
+// gives.  T is double-buffered (all workgroups read the previous iterate),
+// and the buffer holding the latest iterate is ctl->sor_cur.
+#define SOR_TI 8
+#define SOR_TJ 128
+#define SOR_LW (SOR_TJ + 4)
+
+__device__ __forceinline__ void sor_cell(const Geo &g, double *t, int r, int c, int i, int jj,
+                                         const double *__restrict__ Ml, const double *__restrict__ Mlip,
+                                         const double *__restrict__ Mlim, const double *__restrict__ Mljp,
+                                         const double *__restrict__ Mljm, const double *__restrict__ RHS,
+                                         double omega, bool owned, double &acc, int &changed) {
+  const size_t l = (size_t)i * g.ns + jj;
+  const int k = r * SOR_LW + c;
+  const double ml = Ml[l];
+  const double resid = t[k] * ml + t[k + SOR_LW] * Mlip[l] + t[k - SOR_LW] * Mlim[l] + t[k + 1] * Mljp[l] +
+                       t[k - 1] * Mljm[l] - RHS[l];
+  const double dtemp = omega * resid / ml;
+  const double tn = t[k] - dtemp;
+  t[k] = tn;
+  if (owned) {
+    acc += fabs(resid);
+    if (dtemp > tn * SORPRECISION) changed++;
+  }
+}
+
+__global__ void __launch_bounds__(TB) k_sor_iter(Geo g, double *__restrict__ T0, double *__restrict__ T1,
+                                                 const double *__restrict__ Ml, const double *__restrict__ Mlip,
+                                                 const double *__restrict__ Mlim, const double *__restrict__ Mljp,
+                                                 const double *__restrict__ Mljm, const double *__restrict__ RHS,
+                                                 double omega, double *__restrict__ psum, int *__restrict__ pcnt,
+                                                 const DevCtl *__restrict__ ctl) {
+  __shared__ double t[(SOR_TI + 4) * SOR_LW];
   __shared__ double red[TB];
-  __shared__ int amlast;
   if (ctl->sor_done) return;
-  const int ns = g.ns, half = ns / 2, i = blockIdx.y + 1, t = blockIdx.x * TB + threadIdx.x;
-  const int nbx = gridDim.x, nb = gridDim.x * gridDim.y, bid = blockIdx.y * nbx + blockIdx.x;
-  double r = 0.0;
-  int changed = 0;
-  if (t < half) {
-    const int j0 = (pass == 0) ? ((i - 1) & 1) : (i & 1);
-    const int j = 2 * t + j0;
-    const size_t l = (size_t)i * ns + j;
-    const size_t ljp = (j == ns - 1) ? (size_t)i * ns : l + 1;
-    const size_t ljm = (j == 0) ? (size_t)i * ns + ns - 1 : l - 1;
-    const double resid = sor_resid(T, Ml, Mlip, Mlim, Mljp, Mljm, RHS, l, l + ns, l - ns, ljp, ljm);
-    r = fabs(resid);
-    const double dtemp = omega * resid / Ml[l];
-    const double tn = T[l] - dtemp;
-    T[l] = tn;
-    if (dtemp > tn * SORPRECISION) changed = 1;
+  const int cur = ctl->sor_cur;
+  const double *__restrict__ Tin = cur ? T1 : T0;
+  double *__restrict__ Tout = cur ? T0 : T1;
+  const int ns = g.ns, nr = g.nr;
+  const int i0 = 1 + blockIdx.y * SOR_TI;
+  const int i1 = min(i0 + SOR_TI, nr - 1);          // owned rings [i0, i1) within [1, nr-2]
+  const int j0 = blockIdx.x * SOR_TJ;
+  const int tw = min(SOR_TJ, ns - j0);               // owned sectors [j0, j0+tw)
+  const int ti = i1 - i0;
+  // stage rings i0-2 .. i1+1 and sectors j0-2 .. j0+tw+1 (periodic in phi)
+  for (int idx = threadIdx.x; idx < (ti + 4) * (tw + 4); idx += TB) {
+    const int r = idx / (tw + 4), c = idx - r * (tw + 4);
+    const int i = i0 - 2 + r;
+    if (i < 0 || i > nr - 1) continue;
+    int j = j0 - 2 + c;
+    j = (j % ns + ns) % ns;
+    t[r * SOR_LW + c] = Tin[(size_t)i * ns + j];
   }
-  r = block_sum<TB>(r, red);
-  double c = block_sum<TB>((double)changed, red);
-  if (threadIdx.x == 0) {
-    psum[pass * nb + bid] = r;
-    pcnt[pass * nb + bid] = (int)c;
-  }
-  if (pass == 0) return;
-  // pass 1: last block closes the iteration
-  __threadfence();
-  if (threadIdx.x == 0) amlast = (atomicAdd(&ctl->sor_count, 1u) == (unsigned)(nb - 1));
   __syncthreads();
-  if (!amlast) return;
-  __threadfence();
-  double s = 0.0;
-  long long cc = 0;
-  for (int b = threadIdx.x; b < nb; b += TB) {
+  double acc = 0.0;
+  int changed = 0;
+  // red half-sweep ((i+j) odd), owned cells and the one-cell halo
+  for (int idx = threadIdx.x; idx < (ti + 2) * (tw + 2); idx += TB) {
+    const int r = 1 + idx / (tw + 2), c = 1 + idx % (tw + 2);
+    const int i = i0 - 2 + r;
+    if (i < 1 || i > nr - 2) continue;
+    const int j = j0 - 2 + c;
+    if (((i + j) & 1) == 0) continue;
+    const int jj = (j % ns + ns) % ns;
+    const bool owned = (r >= 2 && r < ti + 2 && c >= 2 && c < tw + 2);
+    sor_cell(g, t, r, c, i, jj, Ml, Mlip, Mlim, Mljp, Mljm, RHS, omega, owned, acc, changed);
+  }
+  __syncthreads();
+  // black half-sweep ((i+j) even), owned cells
+  for (int idx = threadIdx.x; idx < ti * tw; idx += TB) {
+    const int r = 2 + idx / tw, c = 2 + idx % tw;
+    const int i = i0 - 2 + r;
+    const int j = j0 - 2 + c;
+    if (((i + j) & 1) != 0) continue;
+    sor_cell(g, t, r, c, i, j, Ml, Mlip, Mlim, Mljp, Mljm, RHS, omega, true, acc, changed);
+  }
+  __syncthreads();
+  for (int idx = threadIdx.x; idx < ti * tw; idx += TB) {
+    const int r = 2 + idx / tw, c = 2 + idx % tw;
+    Tout[(size_t)(i0 - 2 + r) * ns + (j0 - 2 + c)] = t[r * SOR_LW + c];
+  }
+  const int bid = blockIdx.y * gridDim.x + blockIdx.x;
+  acc = block_sum<TB>(acc, red);
+  const double cc = block_sum<TB>((double)changed, red);
+  if (threadIdx.x == 0) {
+    psum[bid] = acc;
+    pcnt[bid] = (int)cc;
+  }
+}
+
+// stop test of the iteration just done (EnergySources.c:487-493), partial
+// sums added in block order
+__global__ void __launch_bounds__(RB) k_sor_check(const double *__restrict__ psum, const int *__restrict__ pcnt,
+                                                  int nb, DevCtl *ctl, int maxiter) {
+  __shared__ double red[RB];
+  if (ctl->sor_done) return;
+  double s = 0.0, c = 0.0;
+  for (int b = threadIdx.x; b < nb; b += RB) {
     s += psum[b];
-    cc += pcnt[b];
+    c += (double)pcnt[b];
   }
-  for (int b = threadIdx.x; b < nb; b += TB) {
-    s += psum[nb + b];
-    cc += pcnt[nb + b];
-  }
-  s = block_sum<TB>(s, red);
-  double ccd = block_sum<TB>((double)cc, red);
+  s = block_sum<RB>(s, red);
+  c = block_sum<RB>(c, red);
   if (threadIdx.x == 0) {
     const int n = ctl->sor_iter + 1;
     ctl->sor_iter = n;
-    ctl->sor_count = 0;
+    ctl->sor_cur ^= 1;
     ctl->sor_normres = s;
-    if (s < SOREPS * ctl->sor_normres0 || s < SORMIN || ccd == 0.0) {
+    if (s < SOREPS * ctl->sor_normres0 || s < SORMIN || c == 0.0) {
       ctl->sor_done = 1;
       ctl->sor_niter = n;
     } else if (n >= maxiter) {
@@ -1013,146 +1142,213 @@ __global__ void __launch_bounds__(TB) k_energy_new(Geo g, const double *__restri
 // ---------------------------------------------------------------------------
 // Radial transport: ComputeLRMomenta (TransportEuler.c:318-347) + OneWindRad
 // (49-65) with ComputeStarRad (232-273), VanLeerRadial (384-415) and
-// DivisePolarGrid (SideEuler.c:82-99) for the six conserved quantities at
-// once.  Momenta and Work = Q/(RhoInt+1e-20) are rebuilt on the fly, which
-// gives the same values as the reference's stored arrays.
+// DivisePolarGrid (SideEuler.c:82-99) for the six conserved quantities.
+// One thread per sector j and RC_I rings: the thread walks rings i0-2 .. i1+1
+// of its column in registers (the radial stencil never leaves the column),
+// forming each work value W = Q/(RhoInt+1e-20), slope and interface flux
+// F(k) = dt*dtheta*Radii[k]*qs(k)*RhoStar(k)*vr(k) once.  The reference's
+// VanLeerRadial evaluates that product for the lower interface of ring i
+// with Rinf[i] and for the upper one with Rsup[i] = Radii[i+1]: both are F.
+// Momenta are rebuilt from rho, vr, vtheta exactly as ComputeLRMomenta
+// forms them.
+#define RC_I 8
+#define RC_R (RC_I + 4)         // rings i0-2 .. i1+1
+
 struct RadQ {
   const double *rho, *vr, *vt, *e;
   double of;
 };
 
-__device__ __forceinline__ double radq_val(const Geo &g, const RadQ &q, int which, int i, int j) {
-  const int ns = g.ns;
-  const size_t l = (size_t)i * ns + j;
-  const double r = q.rho[l];
-  switch (which) {
-    case 0: return r * q.vr[l + ns];                                         // RadMomP
-    case 1: return r * q.vr[l];                                              // RadMomM
-    case 2: {
-      const size_t ljp = (j == ns - 1) ? (size_t)i * ns : l + 1;
-      return r * (q.vt[ljp] + g.Rmed[i] * q.of) * g.Rmed[i];                // ThetaMomP
+// ComputeStarRad on the column: star values at interfaces k = i0 .. i0+RC_I
+// from work values w (rows r <-> ring i0-2+r)
+__device__ __forceinline__ void rad_stars(const Geo &g, int i0, const double *w, const double *vr, double dt,
+                                          double *out) {
+  const int nr = g.nr;
+  double dq[RC_R];
+#pragma unroll
+  for (int r = 0; r < RC_R; r++) dq[r] = 0.0;
+#pragma unroll
+  for (int r = 1; r < RC_R - 1; r++) {
+    const int k = i0 - 2 + r;
+    if (k > 0 && k < nr - 1) {
+      const double dqm = (w[r] - w[r - 1]) * g.InvDiffRmed[k];
+      const double dqp = (w[r + 1] - w[r]) * g.InvDiffRmed[k + 1];
+      dq[r] = (dqp * dqm > 0.0) ? 2.0 * dqp * dqm / (dqp + dqm) : 0.0;
     }
-    case 3: return r * (q.vt[l] + g.Rmed[i] * q.of) * g.Rmed[i];            // ThetaMomM
-    case 4: return q.e[l];                                                   // Energy
-    default: return r;                                                       // Rho
+  }
+#pragma unroll
+  for (int m = 0; m <= RC_I; m++) {
+    const int r = m + 2, k = i0 + m;
+    double st = 0.0;
+    if (k >= 1 && k <= nr - 1) {
+      const double vrk = vr[r];
+      if (vrk > 0.0) st = w[r - 1] + (g.Rmed[k] - g.Rmed[k - 1] - vrk * dt) * 0.5 * dq[r - 1];
+      else st = w[r] - (g.Rmed[k + 1] - g.Rmed[k] + vrk * dt) * 0.5 * dq[r];
+    }
+    out[m] = st;
   }
 }
 
-__device__ __forceinline__ double rad_work(const Geo &g, const RadQ &q, int which, int i, int j) {
-  if (which == 5) return q.rho[(size_t)i * g.ns + j];                       // RhoStar: Qbase = Rho
-  return radq_val(g, q, which, i, j) / (q.rho[(size_t)i * g.ns + j] + 1e-20);
-}
-__device__ __forceinline__ double rad_work_rho(const Geo &g, const RadQ &q, int i, int j) {
-  return q.rho[(size_t)i * g.ns + j] / (q.rho[(size_t)i * g.ns + j] + 1e-20);
-}
-
-// dq of ComputeStarRad at ring k for work values w(k-1), w(k), w(k+1)
-__device__ __forceinline__ double rad_dq(const Geo &g, int k, double wm, double w0, double wp) {
-  if (k == 0 || k == g.nr - 1) return 0.0;
-  const double dqm = (w0 - wm) * g.InvDiffRmed[k];
-  const double dqp = (wp - w0) * g.InvDiffRmed[k + 1];
-  return (dqp * dqm > 0.0) ? 2.0 * dqp * dqm / (dqp + dqm) : 0.0;
-}
-
-// star value at interface ring k (1..nr-1) given w(k-2..k+1)
-__device__ __forceinline__ double rad_star(const Geo &g, int k, double vrk, double dt, const double *w /* w[0..3] = k-2..k+1 */) {
-  if (vrk > 0.0) {
-    const double dq = rad_dq(g, k - 1, w[0], w[1], w[2]);
-    return w[1] + (g.Rmed[k] - g.Rmed[k - 1] - vrk * dt) * 0.5 * dq;
+__device__ __forceinline__ double rad_q(const Geo &g, const RadQ &q, int which, int k, int j, int jp, double rho,
+                                        const double *vr, int r) {
+  const size_t l = (size_t)k * g.ns + j;
+  switch (which) {
+    case 0: return rho * vr[r + 1];                                               // RadMomP
+    case 1: return rho * vr[r];                                                   // RadMomM
+    case 2: return rho * (q.vt[(size_t)k * g.ns + jp] + g.Rmed[k] * q.of) * g.Rmed[k];  // ThetaMomP
+    case 3: return rho * (q.vt[l] + g.Rmed[k] * q.of) * g.Rmed[k];              // ThetaMomM
+    case 4: return q.e[l];                                                        // Energy
+    default: return rho;                                                          // Rho
   }
-  const double dq = rad_dq(g, k, w[1], w[2], w[3]);
-  return w[2] - (g.Rmed[k + 1] - g.Rmed[k] + vrk * dt) * 0.5 * dq;
 }
 
 __global__ void __launch_bounds__(TB) k_transport_rad(Geo g, RadQ q, double dt, double *__restrict__ orp,
                                                       double *__restrict__ orm, double *__restrict__ otp,
                                                       double *__restrict__ otm, double *__restrict__ oe,
                                                       double *__restrict__ orho) {
-  const int ns = g.ns, nr = g.nr, i = blockIdx.y, j = blockIdx.x * TB + threadIdx.x;
+  const int ns = g.ns, nr = g.nr;
+  const int j = blockIdx.x * TB + threadIdx.x;
   if (j >= ns) return;
-  const size_t l = (size_t)i * ns + j, lip = l + ns;
-  const double vrl = q.vr[l], vrlip = q.vr[lip];
+  const int i0 = blockIdx.y * RC_I, i1 = min(i0 + RC_I, nr);
+  const int jp = (j == ns - 1) ? 0 : j + 1;
   const double dtheta = 2.0 * DPI / (double)ns;
-  // ring indices needed: i-2 .. i+2, clamped; values outside [0,nr-1] never used
-  int rk[5];
-  for (int k = 0; k < 5; k++) {
-    int r = i - 2 + k;
-    rk[k] = r < 0 ? 0 : (r > nr - 1 ? nr - 1 : r);
+  double rho[RC_R], vr[RC_R + 1];
+#pragma unroll
+  for (int r = 0; r < RC_R; r++) {
+    const int k = i0 - 2 + r;
+    rho[r] = (k >= 0 && k <= nr - 1) ? q.rho[(size_t)k * ns + j] : 0.0;
   }
-  // RhoStar at i and i+1 (ComputeStarRad(Rho): Work = Rho itself)
-  double wr[5];
-  for (int k = 0; k < 5; k++) wr[k] = q.rho[(size_t)rk[k] * ns + j];
-  const double rs_l = (i == 0) ? 0.0 : rad_star(g, i, vrl, dt, wr);
-  const double rs_lip = (i + 1 >= nr) ? 0.0 : rad_star(g, i + 1, vrlip, dt, wr + 1);
+#pragma unroll
+  for (int r = 0; r < RC_R + 1; r++) {
+    const int k = i0 - 2 + r;
+    vr[r] = (k >= 0 && k <= nr) ? q.vr[(size_t)k * ns + j] : 0.0;   // ring nr: ghost ring
+  }
+  double rs[RC_I + 1];
+  rad_stars(g, i0, rho, vr, dt, rs);            // RhoStar: ComputeStarRad(Rho)
   double *outs[6] = {orp, orm, otp, otm, oe, orho};
   for (int which = 0; which < 6; which++) {
-    double w[5];
-    for (int k = 0; k < 5; k++) {
-      const int r = rk[k];
-      w[k] = radq_val(g, q, which, r, j) / (q.rho[(size_t)r * ns + j] + 1e-20);
+    double w[RC_R];
+#pragma unroll
+    for (int r = 0; r < RC_R; r++) {
+      const int k = i0 - 2 + r;
+      w[r] = 0.0;
+      if (k >= 0 && k <= nr - 1) w[r] = rad_q(g, q, which, k, j, jp, rho[r], vr, r) / (rho[r] + 1e-20);
     }
-    const double qs_l = (i == 0) ? 0.0 : rad_star(g, i, vrl, dt, w);
-    const double qs_lip = (i + 1 >= nr) ? 0.0 : rad_star(g, i + 1, vrlip, dt, w + 1);
-    double varq = dt * dtheta * g.Rinf[i] * qs_l * rs_l * vrl;
-    varq -= dt * dtheta * g.Rsup[i] * qs_lip * rs_lip * vrlip;
-    const double qb = radq_val(g, q, which, i, j);
-    outs[which][l] = qb + varq * g.InvSurf[i];
+    double qs[RC_I + 1];
+    rad_stars(g, i0, w, vr, dt, qs);
+    double F[RC_I + 1];
+#pragma unroll
+    for (int m = 0; m <= RC_I; m++) {
+      const int k = i0 + m;
+      F[m] = (k >= 1 && k <= nr - 1) ? dt * dtheta * g.Rinf[k] * qs[m] * rs[m] * vr[m + 2] : 0.0;
+    }
+#pragma unroll
+    for (int m = 0; m < RC_I; m++) {
+      const int i = i0 + m;
+      if (i < i1) {
+        const double Q = rad_q(g, q, which, i, j, jp, rho[m + 2], vr, m + 2);
+        double varq = F[m];
+        varq -= F[m + 1];
+        outs[which][(size_t)i * ns + j] = Q + varq * g.InvSurf[i];
+      }
+    }
   }
 }
 
 // Azimuthal transport, ring-local: ComputeResiduals (TransportEuler.c:70-111),
-// QuantitiesAdvection with the residual and the uniform velocity (159-175,
-// ComputeStarTheta 275-316, VanLeerTheta 417-445) and AdvectSHIFT (113-136),
-// all in LDS, one workgroup per ring.  The rho history (before/after the
-// first pass) and both RhoStar arrays stay in LDS while the five other
-// quantities stream through.
-__device__ __forceinline__ double th_dq(const double *w, const double *rint, int j, int ns, double invdxtheta,
-                                        bool rhow) {
-  const int jm = (j == 0) ? ns - 1 : j - 1, jp = (j == ns - 1) ? 0 : j + 1;
-  double wm, w0, wp;
-  if (rhow) { wm = w[jm]; w0 = w[j]; wp = w[jp]; }
-  else {
-    wm = w[jm] / (rint[jm] + 1e-20);
-    w0 = w[j] / (rint[j] + 1e-20);
-    wp = w[jp] / (rint[jp] + 1e-20);
-  }
-  const double dqm = (w0 - wm);
-  const double dqp = (wp - w0);
+// QuantitiesAdvection with the residual and then the uniform velocity
+// (159-175; ComputeStarTheta 275-316, VanLeerTheta 417-445) and AdvectSHIFT
+// (113-136).  One workgroup per ring; a thread keeps its cells
+// j = tid + k*TT_T of every quantity, of rho before and after the first pass,
+// of both RhoStar arrays and of the residual velocity in registers; two LDS
+// arrays exchange work values and fluxes with the neighbouring cells.
+#define TT_T 512
+#define TT_NPT 6                // cells per thread: NSEC <= TT_T * TT_NPT
+
+// dq of ComputeStarTheta at x from the work values in w
+__device__ __forceinline__ double th_dq(const double *w, int x, int ns, double invdxtheta) {
+  const int xm = (x == 0) ? ns - 1 : x - 1, xp = (x == ns - 1) ? 0 : x + 1;
+  const double w0 = w[x];
+  const double dqm = (w0 - w[xm]);
+  const double dqp = (w[xp] - w0);
   return ((dqp * dqm > 0.0) && (dqp + dqm != 0.0)) ? dqp * dqm / (dqp + dqm) * invdxtheta : 0.0;
 }
 
-// star value at j; rhow: the base is w itself (RhoStar), else w/(rint+1e-20)
-__device__ __forceinline__ double th_star(const double *w, const double *rint, int j, int ns, double dxtheta,
-                                          double invdxtheta, double ksi, bool rhow) {
+// star value at j (ComputeStarTheta's second loop)
+__device__ __forceinline__ double th_star(const double *w, int j, int ns, double ksi, double dxtheta, double invdxtheta) {
   const int jm = (j == 0) ? ns - 1 : j - 1;
-  if (ksi > 0.0) {
-    const double wjm = rhow ? w[jm] : w[jm] / (rint[jm] + 1e-20);
-    return wjm + (dxtheta - ksi) * th_dq(w, rint, jm, ns, invdxtheta, rhow);
-  }
-  const double wj = rhow ? w[j] : w[j] / (rint[j] + 1e-20);
-  return wj - (dxtheta + ksi) * th_dq(w, rint, j, ns, invdxtheta, rhow);
+  if (ksi > 0.0) return w[jm] + (dxtheta - ksi) * th_dq(w, jm, ns, invdxtheta);
+  return w[j] - (dxtheta + ksi) * th_dq(w, j, ns, invdxtheta);
 }
 
-__global__ void __launch_bounds__(RB) k_transport_theta(Geo g, const double *__restrict__ vt,
-                                                        const double *__restrict__ xrp, const double *__restrict__ xrm,
-                                                        const double *__restrict__ xtp, const double *__restrict__ xtm,
-                                                        const double *__restrict__ xe, const double *__restrict__ xrho,
-                                                        double *__restrict__ orp, double *__restrict__ orm,
-                                                        double *__restrict__ otp, double *__restrict__ otm,
-                                                        double *__restrict__ oe, double *__restrict__ orho,
-                                                        double dt) {
-  extern __shared__ double sh[];       // 6*ns: vtr, rho0, rho1, rs1, rs2, q
-  const int i = blockIdx.x, ns = g.ns;
-  double *s_vtr = sh, *s_r0 = sh + ns, *s_r1 = sh + 2 * ns, *s_rs1 = sh + 3 * ns, *s_rs2 = sh + 4 * ns,
-         *s_q = sh + 5 * ns;
+struct ThGeom {
+  int ns;
+  double dt, dxtheta, invdxtheta, dxrad, invsurf;
+};
+
+// one VanLeerTheta (q held in registers): W = q/(rint+1e-20), star values,
+// fluxes dxrad*qs*RhoStar*v, update q += (F[j] - F[j+1]) * invsurf
+__device__ __forceinline__ void th_pass(const ThGeom &t, double *qv, const double *rint, const double *rs,
+                                        const double *vel, double vconst, bool uniform, double *s_w, double *s_f) {
+  const int tid = threadIdx.x, ns = t.ns;
+#pragma unroll
+  for (int k = 0; k < TT_NPT; k++) {
+    const int j = tid + k * TT_T;
+    if (j < ns) s_w[j] = qv[k] / (rint[k] + 1e-20);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < TT_NPT; k++) {
+    const int j = tid + k * TT_T;
+    if (j < ns) {
+      const double v = uniform ? vconst : vel[k];
+      const double qs = th_star(s_w, j, ns, v * t.dt, t.dxtheta, t.invdxtheta);
+      s_f[j] = t.dxrad * qs * rs[k] * v;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < TT_NPT; k++) {
+    const int j = tid + k * TT_T;
+    if (j < ns) {
+      const int jp = (j == ns - 1) ? 0 : j + 1;
+      double varq = s_f[j];
+      varq -= s_f[jp];
+      qv[k] = qv[k] + varq * t.invsurf;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(TT_T) k_transport_theta(Geo g, const double *__restrict__ vt,
+                                                          const double *__restrict__ xrp, const double *__restrict__ xrm,
+                                                          const double *__restrict__ xtp, const double *__restrict__ xtm,
+                                                          const double *__restrict__ xe, const double *__restrict__ xrho,
+                                                          double *__restrict__ orp, double *__restrict__ orm,
+                                                          double *__restrict__ otp, double *__restrict__ otm,
+                                                          double *__restrict__ oe, double *__restrict__ orho,
+                                                          double dt) {
+  extern __shared__ double sh[];       // 2*ns
+  const int i = blockIdx.x, ns = g.ns, tid = threadIdx.x;
+  double *s_w = sh, *s_f = sh + ns;
   const size_t base = (size_t)i * ns;
   __shared__ double vmed, vtemp;
   __shared__ int nshift;
-  for (int j = threadIdx.x; j < ns; j += RB) { s_vtr[j] = vt[base + j]; s_r0[j] = xrho[base + j]; }
+  double vtr[TT_NPT], r0[TT_NPT], r1[TT_NPT], rs1[TT_NPT], rs2[TT_NPT], qv[TT_NPT];
+#pragma unroll
+  for (int k = 0; k < TT_NPT; k++) {
+    const int j = tid + k * TT_T;
+    vtr[k] = 0.0;
+    r0[k] = 0.0;
+    if (j < ns) {
+      vtr[k] = vt[base + j];
+      r0[k] = xrho[base + j];
+      s_w[j] = vtr[k];
+      s_f[j] = r0[k];
+    }
+  }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    double moy = ordered_ring_sum(s_vtr, ns);
-    const double vm = moy / (double)ns;
+  if (tid == 0) {
+    const double vm = ordered_ring_sum(s_w, ns) / (double)ns;
     const double Ntilde = vm * g.InvRmed[i] * dt * (double)ns / 2.0 / DPI;
     const double Nround = floor(Ntilde + 0.5);
     vmed = vm;
@@ -1162,92 +1358,73 @@ __global__ void __launch_bounds__(RB) k_transport_theta(Geo g, const double *__r
     g.Nshift[i] = nshift;
   }
   __syncthreads();
-  const double dxtheta = 2.0 * DPI / (double)ns * g.Rmed[i];
-  const double invdxtheta = 1.0 / dxtheta;
-  const double dxrad = (g.Rsup[i] - g.Rinf[i]) * dt;
-  const double invsurf = 1.0 / g.Surf[i];
   const bool fast = g.fast_transport != 0;
-  for (int j = threadIdx.x; j < ns; j += RB) s_vtr[j] = fast ? s_vtr[j] - vmed : s_vtr[j];
-  __syncthreads();
-  // pass 1: RhoStar from rho0 with the residual velocity
-  for (int j = threadIdx.x; j < ns; j += RB) s_rs1[j] = th_star(s_r0, s_r0, j, ns, dxtheta, invdxtheta, s_vtr[j] * dt, true);
-  __syncthreads();
-  // rho after pass 1 (VanLeerTheta on Rho, RhoInt = rho0)
-  for (int j = threadIdx.x; j < ns; j += RB) {
-    const int jp = (j == ns - 1) ? 0 : j + 1;
-    const double qs = th_star(s_r0, s_r0, j, ns, dxtheta, invdxtheta, s_vtr[j] * dt, false);
-    const double qsp = th_star(s_r0, s_r0, jp, ns, dxtheta, invdxtheta, s_vtr[jp] * dt, false);
-    double varq = dxrad * qs * s_rs1[j] * s_vtr[j];
-    varq -= dxrad * qsp * s_rs1[jp] * s_vtr[jp];
-    s_r1[j] = s_r0[j] + varq * invsurf;
+  ThGeom t;
+  t.ns = ns;
+  t.dt = dt;
+  t.dxtheta = 2.0 * DPI / (double)ns * g.Rmed[i];
+  t.invdxtheta = 1.0 / t.dxtheta;
+  t.dxrad = (g.Rsup[i] - g.Rinf[i]) * dt;
+  t.invsurf = 1.0 / g.Surf[i];
+  const double vt2 = vtemp;
+  // residual velocity and RhoStar of the first pass (ComputeStarTheta(Rho): base rho itself, in s_f)
+#pragma unroll
+  for (int k = 0; k < TT_NPT; k++) {
+    const int j = tid + k * TT_T;
+    if (fast) vtr[k] = vtr[k] - vmed;
+    rs1[k] = (j < ns) ? th_star(s_f, j, ns, vtr[k] * dt, t.dxtheta, t.invdxtheta) : 0.0;
+    qv[k] = r0[k];
+  }
+  th_pass(t, qv, r0, rs1, vtr, 0.0, false, s_w, s_f);             // VanLeerTheta(Rho), residual velocity
+#pragma unroll
+  for (int k = 0; k < TT_NPT; k++) {
+    const int j = tid + k * TT_T;
+    r1[k] = qv[k];
+    if (j < ns) s_w[j] = qv[k];
   }
   __syncthreads();
-  const double ksi2 = vtemp * dt;
   if (fast) {
-    for (int j = threadIdx.x; j < ns; j += RB) s_rs2[j] = th_star(s_r1, s_r1, j, ns, dxtheta, invdxtheta, ksi2, true);
-    __syncthreads();
-    // rho after pass 2, into s_q, then shifted out
-    for (int j = threadIdx.x; j < ns; j += RB) {
-      const int jp = (j == ns - 1) ? 0 : j + 1;
-      const double qs = th_star(s_r1, s_r1, j, ns, dxtheta, invdxtheta, ksi2, false);
-      const double qsp = th_star(s_r1, s_r1, jp, ns, dxtheta, invdxtheta, ksi2, false);
-      double varq = dxrad * qs * s_rs2[j] * vtemp;
-      varq -= dxrad * qsp * s_rs2[jp] * vtemp;
-      s_q[j] = s_r1[j] + varq * invsurf;
+#pragma unroll
+    for (int k = 0; k < TT_NPT; k++) {
+      const int j = tid + k * TT_T;
+      rs2[k] = (j < ns) ? th_star(s_w, j, ns, vt2 * dt, t.dxtheta, t.invdxtheta) : 0.0;
     }
-  } else {
-    for (int j = threadIdx.x; j < ns; j += RB) s_q[j] = s_r1[j];
+    __syncthreads();
+    th_pass(t, qv, r1, rs2, nullptr, vt2, true, s_w, s_f);       // VanLeerTheta(Rho), uniform velocity
   }
-  __syncthreads();
   const int sh_n = fast ? nshift : 0;
-  for (int j = threadIdx.x; j < ns; j += RB) {
-    int ji = j - sh_n;
-    while (ji < 0) ji += ns;
-    while (ji >= ns) ji -= ns;
-    orho[base + j] = s_q[ji];
-  }
-  const double *ins[5] = {xrp, xrm, xtp, xtm, xe};
-  double *outs[5] = {orp, orm, otp, otm, oe};
-  for (int which = 0; which < 5; which++) {
+  // AdvectSHIFT through LDS: out[j] = q[j - Nshift]
+  auto store_shifted = [&](double *out) {
     __syncthreads();
-    for (int j = threadIdx.x; j < ns; j += RB) s_q[j] = ins[which][base + j];
-    __syncthreads();
-    // pass 1 into registers (at most ns/RB values per thread), then back to LDS
-    double nv[16];
-    int c = 0;
-    for (int j = threadIdx.x; j < ns; j += RB, c++) {
-      const int jp = (j == ns - 1) ? 0 : j + 1;
-      const double qs = th_star(s_q, s_r0, j, ns, dxtheta, invdxtheta, s_vtr[j] * dt, false);
-      const double qsp = th_star(s_q, s_r0, jp, ns, dxtheta, invdxtheta, s_vtr[jp] * dt, false);
-      double varq = dxrad * qs * s_rs1[j] * s_vtr[j];
-      varq -= dxrad * qsp * s_rs1[jp] * s_vtr[jp];
-      nv[c] = s_q[j] + varq * invsurf;
+#pragma unroll
+    for (int k = 0; k < TT_NPT; k++) {
+      const int j = tid + k * TT_T;
+      if (j < ns) s_w[j] = qv[k];
     }
     __syncthreads();
-    c = 0;
-    for (int j = threadIdx.x; j < ns; j += RB, c++) s_q[j] = nv[c];
-    __syncthreads();
-    if (fast) {
-      c = 0;
-      for (int j = threadIdx.x; j < ns; j += RB, c++) {
-        const int jp = (j == ns - 1) ? 0 : j + 1;
-        const double qs = th_star(s_q, s_r1, j, ns, dxtheta, invdxtheta, ksi2, false);
-        const double qsp = th_star(s_q, s_r1, jp, ns, dxtheta, invdxtheta, ksi2, false);
-        double varq = dxrad * qs * s_rs2[j] * vtemp;
-        varq -= dxrad * qsp * s_rs2[jp] * vtemp;
-        nv[c] = s_q[j] + varq * invsurf;
-      }
-      __syncthreads();
-      c = 0;
-      for (int j = threadIdx.x; j < ns; j += RB, c++) s_q[j] = nv[c];
-      __syncthreads();
-    }
-    for (int j = threadIdx.x; j < ns; j += RB) {
+    for (int j = tid; j < ns; j += TT_T) {
       int ji = j - sh_n;
       while (ji < 0) ji += ns;
       while (ji >= ns) ji -= ns;
-      outs[which][base + j] = s_q[ji];
+      out[base + j] = s_w[ji];
     }
+    __syncthreads();
+  };
+  store_shifted(orho);
+  const double *ins[5] = {xrp, xrm, xtp, xtm, xe};
+  double *outs[5] = {orp, orm, otp, otm, oe};
+  for (int which = 0; which < 5; which++) {
+#pragma unroll
+    for (int k = 0; k < TT_NPT; k++) {
+      const int j = tid + k * TT_T;
+      qv[k] = (j < ns) ? ins[which][base + j] : 0.0;
+    }
+    th_pass(t, qv, r0, rs1, vtr, 0.0, false, s_w, s_f);
+    if (fast) {
+      __syncthreads();
+      th_pass(t, qv, r1, rs2, nullptr, vt2, true, s_w, s_f);
+    }
+    store_shifted(outs[which]);
   }
 }
 
@@ -1298,6 +1475,12 @@ __global__ void k_pressure(Geo g, const double *__restrict__ e, double *__restri
 // ===========================================================================
 // host side
 
+static inline void swapb(thorin_dev *d, int a, int b) {
+  double *t = d->b[a];
+  d->b[a] = d->b[b];
+  d->b[b] = t;
+}
+
 static int need_dev(thorin_ctx *c) {
   if (!c || !c->dev) {
     thorin_set_error("this context has no device (created with thorin_ctx_create_host or failed)");
@@ -1317,8 +1500,8 @@ extern "C" int thorin_dev_create(thorin_ctx *c) {
   d->nr = c->nr;
   d->ns = c->ns;
   d->nf = c->nfield;
-  if ((size_t)6 * c->ns * sizeof(double) > 150 * 1024) {
-    thorin_set_error("NSEC too large for the LDS-resident azimuthal transport (max 3200)");
+  if (c->ns > TT_T * TT_NPT) {
+    thorin_set_error("NSEC too large for the azimuthal transport (max 3072)");
     return -1;
   }
   if ((c->ns + TB - 1) / TB * TB / 2 < c->ns / 2) return -1;
@@ -1332,7 +1515,7 @@ extern "C" int thorin_dev_create(thorin_ctx *c) {
     hipDeviceProp_t prop;
     HCHECK(hipGetDevice(&dev));
     HCHECK(hipGetDeviceProperties(&prop, dev));
-    const size_t need = (size_t)6 * c->ns * sizeof(double);
+    const size_t need = (size_t)2 * c->ns * sizeof(double);
     if (need > (size_t)prop.sharedMemPerBlock) {
       char m[200];
       snprintf(m, sizeof m, "NSEC %d needs %zu B of LDS per workgroup for the azimuthal transport; device allows %zu",
@@ -1373,6 +1556,8 @@ extern "C" int thorin_dev_create(thorin_ctx *c) {
   g.gcs = p1 + 21 * n1; g.gtemp = p1 + 22 * n1; g.Vmoy = p1 + 23 * n1; g.VMed = p1 + 24 * n1;
   g.ringmin = p1 + 25 * n1;
   d->ringtmp = p1 + 26 * n1;
+  g.sig_avr = p1 + 27 * n1;
+  g.ist_avr = p1 + 28 * n1;
   int *ip;
   HCHECK(hipMalloc(&ip, sizeof(int) * (size_t)n1 * 2));
   g.Nshift = ip;
@@ -1500,6 +1685,8 @@ extern "C" int thorin_dev_upload_init(thorin_ctx *c) {
   HCHECK(hipMemcpyAsync(d->b[B_TEMPER], c->h_temper, nb, hipMemcpyHostToDevice, d->s));
   HCHECK(hipMemcpyAsync(d->b[B_PRESSURE], c->h_press, nb, hipMemcpyHostToDevice, d->s));
   HCHECK(hipStreamSynchronize(d->s));
+  d->cfl_ready = 0;
+  d->opa_filled = 0;
   return 0;
 }
 
@@ -1509,6 +1696,7 @@ extern "C" int thorin_upload_field(thorin_ctx *c, int field, const double *host,
   if (count > d->nf) count = d->nf;
   HCHECK(hipMemcpyAsync(d->b[field], host, sizeof(double) * count, hipMemcpyHostToDevice, d->s));
   HCHECK(hipStreamSynchronize(d->s));
+  d->cfl_ready = 0;
   return 0;
 }
 
@@ -1662,29 +1850,40 @@ static int sor_solve(thorin_ctx *c, double omega, int *niter_out) {
   thorin_dev *d = c->dev;
   const Geo &g = d->g;
   const int nr = d->nr, ns = d->ns;
-  dim3 gc = cell_grid(d, nr - 2);
-  const int nb0 = gc.x * gc.y;
-  LAUNCH(k_sor_resid0, gc, dim3(TB), 0, g, d->b[B_TEMPER], d->b[B_ML], d->b[B_MLIP], d->b[B_MLIM], d->b[B_MLJP],
-         d->b[B_MLJM], d->b[B_RHS], d->sorp);
-  LAUNCH(k_sum_to, dim3(1), dim3(RB), 0, d->sorp, nb0, &d->ctl->sor_normres0);
-  HCHECK(hipMemsetAsync(&d->ctl->sor_done, 0, sizeof(int) * 3 + sizeof(unsigned int), d->s));
-  dim3 gs((unsigned)(((ns / 2) + TB - 1) / TB), (unsigned)(nr - 2));
+  const size_t nbytes = sizeof(double) * d->nf;
+  // initial residual (EnergySources.c:435-451); the matrix is unchanged and
+  // T is the same initial guess for every solve of a substep
+  if (!d->normres0_valid) {
+    dim3 gc = cell_grid(d, nr - 2);
+    const int nb0 = gc.x * gc.y;
+    LAUNCH(k_sor_resid0, gc, dim3(TB), 0, g, d->b[B_TEMPER], d->b[B_ML], d->b[B_MLIP], d->b[B_MLIM], d->b[B_MLJP],
+           d->b[B_MLJM], d->b[B_RHS], d->sorp);
+    LAUNCH(k_sum_to, dim3(1), dim3(RB), 0, d->sorp, nb0, &d->ctl->sor_normres0);
+    d->normres0_valid = 1;
+  }
+  HCHECK(hipMemcpyAsync(d->b[B_TEMPER2], d->b[B_TEMPER], nbytes, hipMemcpyDeviceToDevice, d->s));
+  HCHECK(hipMemsetAsync(&d->ctl->sor_done, 0, 4 * sizeof(int), d->s));
+  dim3 gs((unsigned)((ns + SOR_TJ - 1) / SOR_TJ), (unsigned)((nr - 2 + SOR_TI - 1) / SOR_TI));
+  const int nbs = gs.x * gs.y;
   int launched = 0;
   int batch = c->st.sor_niter_last > 0 ? c->st.sor_niter_last + 1 : 8;
   if (batch > 256) batch = 256;
   for (;;) {
     for (int b = 0; b < batch; b++) {
-      LAUNCH(k_sor_half, gs, dim3(TB), 0, g, d->b[B_TEMPER], d->b[B_ML], d->b[B_MLIP], d->b[B_MLIM],
-             d->b[B_MLJP], d->b[B_MLJM], d->b[B_RHS], omega, 0, d->sorp, d->sorc, d->ctl, SORMAXITERS);
-      LAUNCH(k_sor_half, gs, dim3(TB), 0, g, d->b[B_TEMPER], d->b[B_ML], d->b[B_MLIP], d->b[B_MLIM],
-             d->b[B_MLJP], d->b[B_MLJM], d->b[B_RHS], omega, 1, d->sorp, d->sorc, d->ctl, SORMAXITERS);
+      LAUNCH(k_sor_iter, gs, dim3(TB), 0, g, d->b[B_TEMPER], d->b[B_TEMPER2], d->b[B_ML], d->b[B_MLIP],
+             d->b[B_MLIM], d->b[B_MLJP], d->b[B_MLJM], d->b[B_RHS], omega, d->sorp, d->sorc, d->ctl);
+      LAUNCH(k_sor_check, dim3(1), dim3(RB), 0, d->sorp, d->sorc, nbs, d->ctl, SORMAXITERS);
     }
     HCHECK(hipGetLastError());
     launched += batch;
     HCHECK(hipMemcpyAsync(d->hctl, d->ctl, sizeof(DevCtl), hipMemcpyDeviceToHost, d->s));
     HCHECK(hipStreamSynchronize(d->s));
     if (prof_flush(d)) return -1;
-    if (d->hctl->sor_done) { *niter_out = d->hctl->sor_niter; return 0; }
+    if (d->hctl->sor_done) {
+      *niter_out = d->hctl->sor_niter;
+      if (d->hctl->sor_cur) swapb(d, B_TEMPER, B_TEMPER2);
+      return 0;
+    }
     if (launched > SORMAXITERS + 2) { *niter_out = SORMAXITERS + 1; return 0; }
     batch = batch < 64 ? 2 * batch : 128;
   }
@@ -1746,11 +1945,6 @@ static double minstep_rebound(const thorin_planets *sys) {
   return 1.0 / dt;
 }
 
-static inline void swapb(thorin_dev *d, int a, int b) {
-  double *t = d->b[a];
-  d->b[a] = d->b[b];
-  d->b[b] = t;
-}
 
 
 // AlgoGas (SourceEuler.c:210-317): one DT
@@ -1773,20 +1967,21 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
     double mt = c->st.physical_time / (p->masstaper * 2.0 * M_PI);
     c->st.mass_taper = (mt > 1.0 ? 1.0 : pow(sin(mt * M_PI / 2.0), 2.0));
     mt = c->st.mass_taper;
-    if (p->lambdadoubling != 0.0) {
-      thorin_cavity_factors(c);
-      const int n1 = nr + 3;
-      HCHECK(hipMemcpyAsync((double *)d->g.cav_med, c->cav_med, sizeof(double) * n1, hipMemcpyHostToDevice, d->s));
-      HCHECK(hipMemcpyAsync((double *)d->g.cav_inf, c->cav_inf, sizeof(double) * n1, hipMemcpyHostToDevice, d->s));
-    }
+    // ---- ConditionCFL: the per-cell minima of this substep were formed at
+    // the end of the previous one (k_eos_cfl) and read back with its sync;
+    // after an (re)initialisation or a change of the planets they are
+    // formed here.
     const double invdtreb_sq = minstep_rebound(sys);
-    // ---- ConditionCFL
-    LAUNCH(k_cfl_ring, dim3(nr), rb, sizeof(double) * (ns + RB), d->g, d->b[B_VRAD], d->b[B_VTHETA], d->b[B_CS],
-           invdtreb_sq);
-    LAUNCH(k_cfl_final, dim3(1), rb, 0, d->g, d->ctl);
-    HCHECK(hipGetLastError());
-    HCHECK(hipMemcpyAsync(d->hctl, d->ctl, sizeof(DevCtl), hipMemcpyDeviceToHost, d->s));
-    HCHECK(hipStreamSynchronize(d->s));
+    if (!d->cfl_ready || invdtreb_sq != d->cfl_invdtreb) {
+      LAUNCH(k_eos_cfl, dim3(nr), rb, sizeof(double) * (3 * ns + RB), d->g, d->b[B_DENS], d->b[B_ENERGY],
+             d->b[B_TEMPER], d->b[B_CS], d->b[B_VRAD], d->b[B_VTHETA], invdtreb_sq, 0);
+      LAUNCH(k_cfl_final, dim3(1), rb, 0, d->g, d->ctl);
+      HCHECK(hipGetLastError());
+      HCHECK(hipMemcpyAsync(d->hctl, d->ctl, sizeof(DevCtl), hipMemcpyDeviceToHost, d->s));
+      HCHECK(hipStreamSynchronize(d->s));
+      d->cfl_ready = 1;
+      d->cfl_invdtreb = invdtreb_sq;
+    }
     double newdt = d->hctl->newdt_raw, deltaT = DT - dtemp;
     if (newdt > c->st.dt_stellar) {
       double a = 1.0 * c->st.dt_stellar, b = p->minimumstep;
@@ -1955,27 +2150,22 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
            d->b[B_TAURR], d->b[B_TAURP], d->b[B_TAUPP]);
     LAUNCH(k_qplus, gall, tb, 0, d->g, d->b[B_DENS], d->b[B_TAURR], d->b[B_TAURP], d->b[B_TAUPP], d->b[B_DIVV],
            d->b[B_QPLUS]);
-    LAUNCH(k_temper_cs, dim3(nr), rb, sizeof(double) * 2 * ns, d->g, d->b[B_DENS], d->b[B_EINT], d->b[B_TEMPER],
-           d->b[B_CS]);
+    LAUNCH(k_temper_cs, gall, tb, 0, d->g, d->b[B_DENS], d->b[B_EINT], d->b[B_TEMPER], d->b[B_CS]);
     LAUNCH(k_opacity_qminus, gall, tb, 0, d->g, d->b[B_DENS], d->b[B_CS], d->b[B_TEMPER], d->b[B_OPACITY],
            d->b[B_QMINUS], d->opa_filled ? 0 : 1);
     d->opa_filled = 1;
-    {
-      unsigned long long big = (unsigned long long)__builtin_bit_cast(unsigned long long, 1.0e38);
-      HCHECK(hipMemcpyAsync(&d->ctl->dtst_bits, &big, sizeof(big), hipMemcpyHostToDevice, d->s));
-    }
     LAUNCH(k_diffcoef, gall, tb, 0, d->g, d->b[B_DENS], d->b[B_CS], d->b[B_TEMPER], d->b[B_OPACITY], d->b[B_DIFF],
-           d->ctl);
-    LAUNCH(k_matrix, cell_grid(d, nr - 2), tb, 0, d->g, d->b[B_DENS], d->b[B_CS], d->b[B_DIFF], d->b[B_DIVV],
-           d->b[B_QMINUS], d->b[B_QPLUS], d->b[B_TEMPER], d->b[B_ML], d->b[B_MLIP], d->b[B_MLIM], d->b[B_MLJP],
-           d->b[B_MLJM], d->b[B_RHS], dt);
-    HCHECK(hipGetLastError());
+           d->red);
+    LAUNCH(k_min_to, dim3(1), rb, 0, d->red, d->nblk_cells, &d->ctl->dt_stellar);
     {
-      unsigned long long bits;
-      HCHECK(hipMemcpyAsync(&bits, &d->ctl->dtst_bits, sizeof(bits), hipMemcpyDeviceToHost, d->s));
-      HCHECK(hipStreamSynchronize(d->s));
-      c->st.dt_stellar = __builtin_bit_cast(double, bits);
+      const dim3 gm = cell_grid(d, nr - 2);
+      LAUNCH(k_matrix, gm, tb, 0, d->g, d->b[B_DENS], d->b[B_CS], d->b[B_DIFF], d->b[B_DIVV], d->b[B_QMINUS],
+             d->b[B_QPLUS], d->b[B_TEMPER], d->b[B_ML], d->b[B_MLIP], d->b[B_MLIM], d->b[B_MLJP], d->b[B_MLJM],
+             d->b[B_RHS], dt, d->sorp);
+      LAUNCH(k_sum_to, dim3(1), rb, 0, d->sorp, (int)(gm.x * gm.y), &d->ctl->sor_normres0);
+      d->normres0_valid = 1;
     }
+    HCHECK(hipGetLastError());
     if (c->st.sor_first) {
       c->st.sor_first = 0;
       if (iterate_relaxation(c)) return -1;
@@ -2000,9 +2190,9 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
       q.vt = d->b[B_VTHETA];
       q.e = d->b[B_ENERGY];
       q.of = c->st.omega_frame;
-      LAUNCH(k_transport_rad, gall, tb, 0, d->g, q, dt, d->b[B_XRP], d->b[B_XRM], d->b[B_XTP], d->b[B_XTM],
-             d->b[B_XE], d->b[B_XRHO]);
-      LAUNCH(k_transport_theta, dim3(nr), rb, sizeof(double) * 6 * ns, d->g, d->b[B_VTHETA], d->b[B_XRP],
+      LAUNCH(k_transport_rad, dim3((unsigned)((ns + TB - 1) / TB), (unsigned)((nr + RC_I - 1) / RC_I)), tb, 0,
+             d->g, q, dt, d->b[B_XRP], d->b[B_XRM], d->b[B_XTP], d->b[B_XTM], d->b[B_XE], d->b[B_XRHO]);
+      LAUNCH(k_transport_theta, dim3(nr), dim3(TT_T), sizeof(double) * 2 * ns, d->g, d->b[B_VTHETA], d->b[B_XRP],
              d->b[B_XRM], d->b[B_XTP], d->b[B_XTM], d->b[B_XE], d->b[B_XRHO], d->b[B_RP], d->b[B_RM], d->b[B_TP],
              d->b[B_TM], d->b[B_ENERGY], d->b[B_DENS], dt);
       LAUNCH(k_velocities, gall, tb, 0, d->g, d->b[B_DENS], d->b[B_RP], d->b[B_RM], d->b[B_TP], d->b[B_TM],
@@ -2010,10 +2200,20 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
     }
     if (p->damping_on)
       LAUNCH(k_boundary, gall, tb, 0, d->g, d->b[B_VRAD], d->b[B_VTHETA], d->b[B_DENS], d->b[B_ENERGY], dt);
-    LAUNCH(k_temper_cs, dim3(nr), rb, sizeof(double) * 2 * ns, d->g, d->b[B_DENS], d->b[B_ENERGY], d->b[B_TEMPER],
-           d->b[B_CS]);
-    HCHECK(hipGetLastError());
     c->st.physical_time += dt;
+    // ---- end of the substep: T, cs and their ring means, and the CFL
+    // minima of the next substep (its planets are those of now)
+    if (p->lambdadoubling != 0.0) {
+      thorin_cavity_factors(c);
+      const int n1 = nr + 3;
+      HCHECK(hipMemcpyAsync((double *)d->g.cav_med, c->cav_med, sizeof(double) * n1, hipMemcpyHostToDevice, d->s));
+      HCHECK(hipMemcpyAsync((double *)d->g.cav_inf, c->cav_inf, sizeof(double) * n1, hipMemcpyHostToDevice, d->s));
+    }
+    d->cfl_invdtreb = minstep_rebound(sys);
+    LAUNCH(k_eos_cfl, dim3(nr), rb, sizeof(double) * (3 * ns + RB), d->g, d->b[B_DENS], d->b[B_ENERGY],
+           d->b[B_TEMPER], d->b[B_CS], d->b[B_VRAD], d->b[B_VTHETA], d->cfl_invdtreb, 1);
+    LAUNCH(k_cfl_final, dim3(1), rb, 0, d->g, d->ctl);
+    HCHECK(hipGetLastError());
     if (log && log->nsub < THORIN_MAXSUB) {
       log->cfl_n[log->nsub] = N;
       log->dt[log->nsub] = dt;
@@ -2021,9 +2221,11 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
       log->sor_omega[log->nsub] = omega_used;
       log->nsub++;
     }
-    // crash flags of this substep (read back with the next CFL copy; fetched now for the log)
+    // one sync per substep: crash flags, dt_stellar and the next CFL minimum
     HCHECK(hipMemcpyAsync(d->hctl, d->ctl, sizeof(DevCtl), hipMemcpyDeviceToHost, d->s));
     HCHECK(hipStreamSynchronize(d->s));
+    d->cfl_ready = 1;
+    c->st.dt_stellar = d->hctl->dt_stellar;
     if (prof_flush(d)) return -1;
     if (d->hctl->crash_e) {
       c->st.already_crashed++;
