@@ -131,9 +131,11 @@ static const char *const KNAMES[] = {
     "k_matrix", "k_sor_resid0", "k_sum_to", "k_min_to", "k_sor_iter", "k_sor_check", "k_energy_new", "k_transport_rad",
     "k_transport_theta", "k_velocities", NULL};
 
+// a template instance ("k_name<3>") counts as its kernel
 static int kid_of(const char *name) {
+  const size_t n = strcspn(name, "<");
   for (int i = 0; KNAMES[i]; i++)
-    if (!strcmp(KNAMES[i], name)) return i;
+    if (strlen(KNAMES[i]) == n && !strncmp(KNAMES[i], name, n)) return i;
   return -1;
 }
 
@@ -1143,127 +1145,111 @@ __global__ void __launch_bounds__(TB) k_energy_new(Geo g, const double *__restri
 // Radial transport: ComputeLRMomenta (TransportEuler.c:318-347) + OneWindRad
 // (49-65) with ComputeStarRad (232-273), VanLeerRadial (384-415) and
 // DivisePolarGrid (SideEuler.c:82-99) for the six conserved quantities.
-// One thread per sector j and RC_I rings: the thread walks rings i0-2 .. i1+1
-// of its column in registers (the radial stencil never leaves the column),
-// forming each work value W = Q/(RhoInt+1e-20), slope and interface flux
-// F(k) = dt*dtheta*Radii[k]*qs(k)*RhoStar(k)*vr(k) once.  The reference's
-// VanLeerRadial evaluates that product for the lower interface of ring i
-// with Rinf[i] and for the upper one with Rsup[i] = Radii[i+1]: both are F.
-// Momenta are rebuilt from rho, vr, vtheta exactly as ComputeLRMomenta
-// forms them.
-#define RC_I 8
-#define RC_R (RC_I + 4)         // rings i0-2 .. i1+1
+// A workgroup handles one quantity on a tile of TB sectors x RC_I rings; each
+// thread walks its column outward with a rolling window: at ring i it loads
+// ring i+2, forms W(i+2) = Q/(RhoInt+1e-20), the slopes dq(i+1) of W and of
+// rho, the star values at interface i+1 and the flux
+// F(i+1) = dt*dtheta*Radii[i+1]*qs*RhoStar*vr, and writes
+// Q(i) + (F(i) - F(i+1))*InvSurf[i].  The reference evaluates that product
+// for the lower interface of ring i with Rinf[i] and for the upper one with
+// Rsup[i] = Radii[i+1]: both are F.  Momenta are rebuilt from rho, vr,
+// vtheta exactly as ComputeLRMomenta forms them.  The window starts four
+// rings below the tile (rings outside the grid read as 0, as the reference's
+// dq = 0 and qs = 0 there make them).  The six workgroups of one tile are
+// numbered so that they land on the same XCD (blocks go round-robin over the
+// 8 XCDs), sharing its L2 for the common rho and vr.
+#define RC_I 16
+#define RC_NQ 6
 
 struct RadQ {
   const double *rho, *vr, *vt, *e;
+  double *out[RC_NQ];          // RadMomP, RadMomM, ThetaMomP, ThetaMomM, Energy, Rho
   double of;
+  int ntile_j, ntiles;
 };
 
-// ComputeStarRad on the column: star values at interfaces k = i0 .. i0+RC_I
-// from work values w (rows r <-> ring i0-2+r)
-__device__ __forceinline__ void rad_stars(const Geo &g, int i0, const double *w, const double *vr, double dt,
-                                          double *out) {
-  const int nr = g.nr;
-  double dq[RC_R];
-#pragma unroll
-  for (int r = 0; r < RC_R; r++) dq[r] = 0.0;
-#pragma unroll
-  for (int r = 1; r < RC_R - 1; r++) {
-    const int k = i0 - 2 + r;
-    if (k > 0 && k < nr - 1) {
-      const double dqm = (w[r] - w[r - 1]) * g.InvDiffRmed[k];
-      const double dqp = (w[r + 1] - w[r]) * g.InvDiffRmed[k + 1];
-      dq[r] = (dqp * dqm > 0.0) ? 2.0 * dqp * dqm / (dqp + dqm) : 0.0;
-    }
-  }
-#pragma unroll
-  for (int m = 0; m <= RC_I; m++) {
-    const int r = m + 2, k = i0 + m;
-    double st = 0.0;
-    if (k >= 1 && k <= nr - 1) {
-      const double vrk = vr[r];
-      if (vrk > 0.0) st = w[r - 1] + (g.Rmed[k] - g.Rmed[k - 1] - vrk * dt) * 0.5 * dq[r - 1];
-      else st = w[r] - (g.Rmed[k + 1] - g.Rmed[k] + vrk * dt) * 0.5 * dq[r];
-    }
-    out[m] = st;
-  }
+// dq of ComputeStarRad at ring k from W at k-1, k, k+1
+__device__ __forceinline__ double rad_dq(const Geo &g, int k, double wm, double w0, double wp) {
+  if (k <= 0 || k >= g.nr - 1) return 0.0;
+  const double dqm = (w0 - wm) * g.InvDiffRmed[k];
+  const double dqp = (wp - w0) * g.InvDiffRmed[k + 1];
+  return (dqp * dqm > 0.0) ? 2.0 * dqp * dqm / (dqp + dqm) : 0.0;
 }
 
-__device__ __forceinline__ double rad_q(const Geo &g, const RadQ &q, int which, int k, int j, int jp, double rho,
-                                        const double *vr, int r) {
-  const size_t l = (size_t)k * g.ns + j;
-  switch (which) {
-    case 0: return rho * vr[r + 1];                                               // RadMomP
-    case 1: return rho * vr[r];                                                   // RadMomM
-    case 2: return rho * (q.vt[(size_t)k * g.ns + jp] + g.Rmed[k] * q.of) * g.Rmed[k];  // ThetaMomP
-    case 3: return rho * (q.vt[l] + g.Rmed[k] * q.of) * g.Rmed[k];              // ThetaMomM
-    case 4: return q.e[l];                                                        // Energy
-    default: return rho;                                                          // Rho
-  }
+// star value of ComputeStarRad at interface k
+__device__ __forceinline__ double rad_star(const Geo &g, int k, double wm, double w0, double dqm, double dq0,
+                                           double vrk, double dt) {
+  if (k < 1 || k > g.nr - 1) return 0.0;
+  if (vrk > 0.0) return wm + (g.Rmed[k] - g.Rmed[k - 1] - vrk * dt) * 0.5 * dqm;
+  return w0 - (g.Rmed[k + 1] - g.Rmed[k] + vrk * dt) * 0.5 * dq0;
 }
 
-__global__ void __launch_bounds__(TB) k_transport_rad(Geo g, RadQ q, double dt, double *__restrict__ orp,
-                                                      double *__restrict__ orm, double *__restrict__ otp,
-                                                      double *__restrict__ otm, double *__restrict__ oe,
-                                                      double *__restrict__ orho) {
+__global__ void __launch_bounds__(TB) k_transport_rad(Geo g, RadQ q, double dt) {
   const int ns = g.ns, nr = g.nr;
-  const int j = blockIdx.x * TB + threadIdx.x;
+  const int b = blockIdx.x, slot = b & 7, rest = b >> 3;
+  const int which = rest % RC_NQ;
+  const int tile = (rest / RC_NQ) * 8 + slot;
+  if (tile >= q.ntiles) return;
+  const int tj = tile % q.ntile_j, ti = tile / q.ntile_j;
+  const int j = tj * TB + threadIdx.x;
   if (j >= ns) return;
-  const int i0 = blockIdx.y * RC_I, i1 = min(i0 + RC_I, nr);
+  const int i0 = ti * RC_I, i1 = min(i0 + RC_I, nr);
   const int jp = (j == ns - 1) ? 0 : j + 1;
   const double dtheta = 2.0 * DPI / (double)ns;
-  double rho[RC_R], vr[RC_R + 1];
-#pragma unroll
-  for (int r = 0; r < RC_R; r++) {
-    const int k = i0 - 2 + r;
-    rho[r] = (k >= 0 && k <= nr - 1) ? q.rho[(size_t)k * ns + j] : 0.0;
-  }
-#pragma unroll
-  for (int r = 0; r < RC_R + 1; r++) {
-    const int k = i0 - 2 + r;
-    vr[r] = (k >= 0 && k <= nr) ? q.vr[(size_t)k * ns + j] : 0.0;   // ring nr: ghost ring
-  }
-  double rs[RC_I + 1];
-  rad_stars(g, i0, rho, vr, dt, rs);            // RhoStar: ComputeStarRad(Rho)
-  double *outs[6] = {orp, orm, otp, otm, oe, orho};
-  for (int which = 0; which < 6; which++) {
-    double w[RC_R];
-#pragma unroll
-    for (int r = 0; r < RC_R; r++) {
-      const int k = i0 - 2 + r;
-      w[r] = 0.0;
-      if (k >= 0 && k <= nr - 1) w[r] = rad_q(g, q, which, k, j, jp, rho[r], vr, r) / (rho[r] + 1e-20);
-    }
-    double qs[RC_I + 1];
-    rad_stars(g, i0, w, vr, dt, qs);
-    double F[RC_I + 1];
-#pragma unroll
-    for (int m = 0; m <= RC_I; m++) {
-      const int k = i0 + m;
-      F[m] = (k >= 1 && k <= nr - 1) ? dt * dtheta * g.Rinf[k] * qs[m] * rs[m] * vr[m + 2] : 0.0;
-    }
-#pragma unroll
-    for (int m = 0; m < RC_I; m++) {
-      const int i = i0 + m;
-      if (i < i1) {
-        const double Q = rad_q(g, q, which, i, j, jp, rho[m + 2], vr, m + 2);
-        double varq = F[m];
-        varq -= F[m + 1];
-        outs[which][(size_t)i * ns + j] = Q + varq * g.InvSurf[i];
+  double *__restrict__ out = q.out[which];
+  // rolling state for output ring i: rho and W at i, i+1; Q at i, i+1;
+  // slopes at i; flux at i; vr at i+1, i+2
+  double rho0 = 0.0, rho1 = 0.0, w0 = 0.0, w1 = 0.0, Q0 = 0.0, Q1 = 0.0;
+  double drho0 = 0.0, dq0 = 0.0, F0 = 0.0;
+  double vr1 = (i0 - 3 >= 0) ? q.vr[(size_t)(i0 - 3) * ns + j] : 0.0;
+  double vr2 = (i0 - 2 >= 0) ? q.vr[(size_t)(i0 - 2) * ns + j] : 0.0;
+#pragma unroll 2
+  for (int i = i0 - 4; i < i1; i++) {
+    const int k = i + 2;                       // ring loaded now
+    double rho2 = 0.0, Q2 = 0.0, vr3 = 0.0;
+    if (k + 1 >= 0 && k + 1 <= nr) vr3 = q.vr[(size_t)(k + 1) * ns + j];   // ring nr: ghost ring
+    if (k >= 0 && k < nr) {
+      const size_t l = (size_t)k * ns + j;
+      rho2 = q.rho[l];
+      switch (which) {
+        case 0: Q2 = rho2 * vr3; break;
+        case 1: Q2 = rho2 * vr2; break;
+        case 2: Q2 = rho2 * (q.vt[(size_t)k * ns + jp] + g.Rmed[k] * q.of) * g.Rmed[k]; break;
+        case 3: Q2 = rho2 * (q.vt[l] + g.Rmed[k] * q.of) * g.Rmed[k]; break;
+        case 4: Q2 = q.e[l]; break;
+        default: Q2 = rho2; break;
       }
     }
+    const double w2 = Q2 / (rho2 + 1e-20);
+    const double drho1 = rad_dq(g, i + 1, rho0, rho1, rho2);
+    const double dq1 = rad_dq(g, i + 1, w0, w1, w2);
+    const double rs = rad_star(g, i + 1, rho0, rho1, drho0, drho1, vr1, dt);
+    const double qs = rad_star(g, i + 1, w0, w1, dq0, dq1, vr1, dt);
+    const double F1 = (i + 1 >= 1 && i + 1 <= nr - 1) ? dt * dtheta * g.Rinf[i + 1] * qs * rs * vr1 : 0.0;
+    if (i >= i0) {
+      double varq = F0;
+      varq -= F1;
+      out[(size_t)i * ns + j] = Q0 + varq * g.InvSurf[i];
+    }
+    rho0 = rho1; rho1 = rho2;
+    w0 = w1; w1 = w2;
+    Q0 = Q1; Q1 = Q2;
+    drho0 = drho1; dq0 = dq1; F0 = F1;
+    vr1 = vr2; vr2 = vr3;
   }
 }
 
 // Azimuthal transport, ring-local: ComputeResiduals (TransportEuler.c:70-111),
 // QuantitiesAdvection with the residual and then the uniform velocity
 // (159-175; ComputeStarTheta 275-316, VanLeerTheta 417-445) and AdvectSHIFT
-// (113-136).  One workgroup per ring; a thread keeps its cells
+// (113-136).  One workgroup per ring; a thread keeps its NPT cells
 // j = tid + k*TT_T of every quantity, of rho before and after the first pass,
 // of both RhoStar arrays and of the residual velocity in registers; two LDS
-// arrays exchange work values and fluxes with the neighbouring cells.
+// arrays exchange work values and fluxes with the neighbouring cells.  NPT is
+// the smallest count with NSEC <= TT_T*NPT, so that no register holds a cell
+// outside the ring.
 #define TT_T 512
-#define TT_NPT 6                // cells per thread: NSEC <= TT_T * TT_NPT
+#define TT_NPT 6                // largest instance: NSEC <= 3072
 
 // dq of ComputeStarTheta at x from the work values in w
 __device__ __forceinline__ double th_dq(const double *w, int x, int ns, double invdxtheta) {
@@ -1288,17 +1274,18 @@ struct ThGeom {
 
 // one VanLeerTheta (q held in registers): W = q/(rint+1e-20), star values,
 // fluxes dxrad*qs*RhoStar*v, update q += (F[j] - F[j+1]) * invsurf
+template <int NPT>
 __device__ __forceinline__ void th_pass(const ThGeom &t, double *qv, const double *rint, const double *rs,
                                         const double *vel, double vconst, bool uniform, double *s_w, double *s_f) {
   const int tid = threadIdx.x, ns = t.ns;
 #pragma unroll
-  for (int k = 0; k < TT_NPT; k++) {
+  for (int k = 0; k < NPT; k++) {
     const int j = tid + k * TT_T;
     if (j < ns) s_w[j] = qv[k] / (rint[k] + 1e-20);
   }
   __syncthreads();
 #pragma unroll
-  for (int k = 0; k < TT_NPT; k++) {
+  for (int k = 0; k < NPT; k++) {
     const int j = tid + k * TT_T;
     if (j < ns) {
       const double v = uniform ? vconst : vel[k];
@@ -1308,7 +1295,7 @@ __device__ __forceinline__ void th_pass(const ThGeom &t, double *qv, const doubl
   }
   __syncthreads();
 #pragma unroll
-  for (int k = 0; k < TT_NPT; k++) {
+  for (int k = 0; k < NPT; k++) {
     const int j = tid + k * TT_T;
     if (j < ns) {
       const int jp = (j == ns - 1) ? 0 : j + 1;
@@ -1319,6 +1306,7 @@ __device__ __forceinline__ void th_pass(const ThGeom &t, double *qv, const doubl
   }
 }
 
+template <int NPT>
 __global__ void __launch_bounds__(TT_T) k_transport_theta(Geo g, const double *__restrict__ vt,
                                                           const double *__restrict__ xrp, const double *__restrict__ xrm,
                                                           const double *__restrict__ xtp, const double *__restrict__ xtm,
@@ -1333,9 +1321,9 @@ __global__ void __launch_bounds__(TT_T) k_transport_theta(Geo g, const double *_
   const size_t base = (size_t)i * ns;
   __shared__ double vmed, vtemp;
   __shared__ int nshift;
-  double vtr[TT_NPT], r0[TT_NPT], r1[TT_NPT], rs1[TT_NPT], rs2[TT_NPT], qv[TT_NPT];
+  double vtr[NPT], r0[NPT], r1[NPT], rs1[NPT], rs2[NPT], qv[NPT];
 #pragma unroll
-  for (int k = 0; k < TT_NPT; k++) {
+  for (int k = 0; k < NPT; k++) {
     const int j = tid + k * TT_T;
     vtr[k] = 0.0;
     r0[k] = 0.0;
@@ -1369,15 +1357,15 @@ __global__ void __launch_bounds__(TT_T) k_transport_theta(Geo g, const double *_
   const double vt2 = vtemp;
   // residual velocity and RhoStar of the first pass (ComputeStarTheta(Rho): base rho itself, in s_f)
 #pragma unroll
-  for (int k = 0; k < TT_NPT; k++) {
+  for (int k = 0; k < NPT; k++) {
     const int j = tid + k * TT_T;
     if (fast) vtr[k] = vtr[k] - vmed;
     rs1[k] = (j < ns) ? th_star(s_f, j, ns, vtr[k] * dt, t.dxtheta, t.invdxtheta) : 0.0;
     qv[k] = r0[k];
   }
-  th_pass(t, qv, r0, rs1, vtr, 0.0, false, s_w, s_f);             // VanLeerTheta(Rho), residual velocity
+  th_pass<NPT>(t, qv, r0, rs1, vtr, 0.0, false, s_w, s_f);             // VanLeerTheta(Rho), residual velocity
 #pragma unroll
-  for (int k = 0; k < TT_NPT; k++) {
+  for (int k = 0; k < NPT; k++) {
     const int j = tid + k * TT_T;
     r1[k] = qv[k];
     if (j < ns) s_w[j] = qv[k];
@@ -1385,19 +1373,19 @@ __global__ void __launch_bounds__(TT_T) k_transport_theta(Geo g, const double *_
   __syncthreads();
   if (fast) {
 #pragma unroll
-    for (int k = 0; k < TT_NPT; k++) {
+    for (int k = 0; k < NPT; k++) {
       const int j = tid + k * TT_T;
       rs2[k] = (j < ns) ? th_star(s_w, j, ns, vt2 * dt, t.dxtheta, t.invdxtheta) : 0.0;
     }
     __syncthreads();
-    th_pass(t, qv, r1, rs2, nullptr, vt2, true, s_w, s_f);       // VanLeerTheta(Rho), uniform velocity
+    th_pass<NPT>(t, qv, r1, rs2, nullptr, vt2, true, s_w, s_f);       // VanLeerTheta(Rho), uniform velocity
   }
   const int sh_n = fast ? nshift : 0;
   // AdvectSHIFT through LDS: out[j] = q[j - Nshift]
   auto store_shifted = [&](double *out) {
     __syncthreads();
 #pragma unroll
-    for (int k = 0; k < TT_NPT; k++) {
+    for (int k = 0; k < NPT; k++) {
       const int j = tid + k * TT_T;
       if (j < ns) s_w[j] = qv[k];
     }
@@ -1415,14 +1403,14 @@ __global__ void __launch_bounds__(TT_T) k_transport_theta(Geo g, const double *_
   double *outs[5] = {orp, orm, otp, otm, oe};
   for (int which = 0; which < 5; which++) {
 #pragma unroll
-    for (int k = 0; k < TT_NPT; k++) {
+    for (int k = 0; k < NPT; k++) {
       const int j = tid + k * TT_T;
       qv[k] = (j < ns) ? ins[which][base + j] : 0.0;
     }
-    th_pass(t, qv, r0, rs1, vtr, 0.0, false, s_w, s_f);
+    th_pass<NPT>(t, qv, r0, rs1, vtr, 0.0, false, s_w, s_f);
     if (fast) {
       __syncthreads();
-      th_pass(t, qv, r1, rs2, nullptr, vt2, true, s_w, s_f);
+      th_pass<NPT>(t, qv, r1, rs2, nullptr, vt2, true, s_w, s_f);
     }
     store_shifted(outs[which]);
   }
@@ -1523,7 +1511,11 @@ extern "C" int thorin_dev_create(thorin_ctx *c) {
       thorin_set_error(m);
       return -1;
     }
-    HCHECK(hipFuncSetAttribute((const void *)k_transport_theta, hipFuncAttributeMaxDynamicSharedMemorySize, (int)need));
+    const void *tt[TT_NPT] = {(const void *)k_transport_theta<1>, (const void *)k_transport_theta<2>,
+                              (const void *)k_transport_theta<3>, (const void *)k_transport_theta<4>,
+                              (const void *)k_transport_theta<5>, (const void *)k_transport_theta<6>};
+    for (int k = 0; k < TT_NPT; k++)
+      HCHECK(hipFuncSetAttribute(tt[k], hipFuncAttributeMaxDynamicSharedMemorySize, (int)need));
   }
   HCHECK(hipStreamCreateWithFlags(&d->s, hipStreamNonBlocking));
   d->prof_pend = new std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>>();
@@ -2189,12 +2181,25 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
       q.vr = d->b[B_VRAD];
       q.vt = d->b[B_VTHETA];
       q.e = d->b[B_ENERGY];
+      const int outs[RC_NQ] = {B_XRP, B_XRM, B_XTP, B_XTM, B_XE, B_XRHO};
+      for (int k = 0; k < RC_NQ; k++) q.out[k] = d->b[outs[k]];
       q.of = c->st.omega_frame;
-      LAUNCH(k_transport_rad, dim3((unsigned)((ns + TB - 1) / TB), (unsigned)((nr + RC_I - 1) / RC_I)), tb, 0,
-             d->g, q, dt, d->b[B_XRP], d->b[B_XRM], d->b[B_XTP], d->b[B_XTM], d->b[B_XE], d->b[B_XRHO]);
-      LAUNCH(k_transport_theta, dim3(nr), dim3(TT_T), sizeof(double) * 2 * ns, d->g, d->b[B_VTHETA], d->b[B_XRP],
-             d->b[B_XRM], d->b[B_XTP], d->b[B_XTM], d->b[B_XE], d->b[B_XRHO], d->b[B_RP], d->b[B_RM], d->b[B_TP],
-             d->b[B_TM], d->b[B_ENERGY], d->b[B_DENS], dt);
+      q.ntile_j = (ns + TB - 1) / TB;
+      q.ntiles = q.ntile_j * ((nr + RC_I - 1) / RC_I);
+      LAUNCH(k_transport_rad, dim3((unsigned)((q.ntiles + 7) / 8 * 8 * RC_NQ)), tb, 0, d->g, q, dt);
+#define TT_LAUNCH(NPT)                                                                                         \
+  LAUNCH(k_transport_theta<NPT>, dim3(nr), dim3(TT_T), sizeof(double) * 2 * ns, d->g, d->b[B_VTHETA], d->b[B_XRP],  \
+         d->b[B_XRM], d->b[B_XTP], d->b[B_XTM], d->b[B_XE], d->b[B_XRHO], d->b[B_RP], d->b[B_RM], d->b[B_TP],     \
+         d->b[B_TM], d->b[B_ENERGY], d->b[B_DENS], dt)
+      switch ((ns + TT_T - 1) / TT_T) {
+        case 1: TT_LAUNCH(1); break;
+        case 2: TT_LAUNCH(2); break;
+        case 3: TT_LAUNCH(3); break;
+        case 4: TT_LAUNCH(4); break;
+        case 5: TT_LAUNCH(5); break;
+        default: TT_LAUNCH(6); break;
+      }
+#undef TT_LAUNCH
       LAUNCH(k_velocities, gall, tb, 0, d->g, d->b[B_DENS], d->b[B_RP], d->b[B_RM], d->b[B_TP], d->b[B_TM],
              d->b[B_VRAD], d->b[B_VTHETA]);
     }

@@ -137,24 +137,27 @@ def test_gpu_axisymmetric_disk_without_planet():
         assert np.max(np.abs(a - a[:, :1])) / scale < 1e-8, f
 
 
-def test_gpu_larger_grid_parity_256x384():
-    """A grid with several workgroups per ring and ring-local LDS transport
-    of 384 cells: 5 DTs against the oracle."""
-    _print_case("256x384")
+@pytest.mark.parametrize("nrad,nsec,steps", [(256, 384, 5), (24, 1536, 3), (21, 1100, 3), (12, 2900, 2)])
+def test_gpu_larger_grid_parity(nrad, nsec, steps):
+    """Grids with several workgroups per ring, radial tiles that do not divide
+    NRAD, and azimuthal rings of 1 to 6 cells per thread (1536 is the
+    1024x1536 workload's ring; 1100 and 2900 leave threads without a cell):
+    DTs against the oracle."""
+    _print_case(f"{nrad}x{nsec}")
     from fargo_thorin_amd import configs, seeded_disk
-    cfg = configs.with_grid(configs.DAMPED_1024x1536, 256, 384)
-    disk = seeded_disk(256, 384, seed=777)
+    cfg = configs.with_grid(configs.DAMPED_1024x1536, nrad, nsec)
+    disk = seeded_disk(nrad, nsec, seed=777)
     P = params_for(cfg)
     d, o = Disk(P), OracleDisk(P)
     for x in (d, o):
         x.init(disk["dens"], disk["vrad"], disk["vtheta"], disk["temper"])
     sd, so = planets_for([(1e-13, 1.0)]), planets_for([(1e-13, 1.0)])
     nb = host_nbody_fn()
-    for step in range(5):
+    for step in range(steps):
         lg = d.algogas(sd, nb)
         lo = o.algogas(so, nb)
         assert [lg.sor_niter[k] for k in range(lg.nsub)] == [lo.sor_niter[k] for k in range(lo.nsub)]
-        assert np.array_equal(d.ring_profile("nshift"), o.one_d("nshift")[:256])
+        assert np.array_equal(d.ring_profile("nshift"), o.one_d("nshift")[:nrad])
     for f in STATE_FIELDS:
         err = rel_linf(d.field(f), o.field(f))
         assert err <= TOL, f"{f}: {err:.3e}"
