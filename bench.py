@@ -35,7 +35,7 @@ HBM_PEAK_GBS = 8000.0
 # cell and iteration over the updated rings, split over its two half-sweep
 # launches.
 ALG_BYTES_PER_CELL = {
-    "k_sor_half": 32, "k_transport_theta": 104, "k_transport_rad": 80, "k_forces": 32,
+    "k_sor_iter": 64, "k_transport_theta": 104, "k_transport_rad": 80, "k_forces": 32,
     "k_substep1": 64, "k_divstress": 56, "k_visc_update": 72, "k_substep2": 56, "k_qplus": 48,
     "k_temper_cs": 32, "k_opacity_qminus": 40, "k_diffcoef": 40, "k_matrix": 104, "k_velocities": 56,
     "k_energy_new": 24, "k_indirect": 48, "k_cfl_ring": 24, "k_sor_resid0": 56,

@@ -69,6 +69,7 @@ struct DevCtl {                 // device-side control block, mirrored to pinned
   double sor_normres0, sor_normres;
   double axstar, aystar;
   double axpl[MAXPL], aypl[MAXPL];
+  double tq_rho[16], tq_cs[16]; // torque log: ring means next to the logged planets
 };
 
 struct PlanetK {                // per-planet constants of FillForcesArrays (Pframeforce.c:93-121)
@@ -109,6 +110,9 @@ struct thorin_dev {
   double *red;                  // block partial sums (forces)
   double *sorp;                 // SOR partial sums (2 passes)
   int *sorc;                    // SOR partial counts
+  double *pstellar;             // DiffusionTimestep block minima
+  double *mresp;                // block partials of the initial SOR residual
+  int nb_mresp;
   double *ringtmp;              // per-ring scratch
   int nblk_cells, nblk_sor;
   int opa_filled;
@@ -129,7 +133,7 @@ static const char *const KNAMES[] = {
     "k_eos_cfl", "k_cfl_final", "k_forces", "k_forces_reduce", "k_indirect", "k_boundary", "k_substep1",
     "k_divstress", "k_visc_update", "k_substep2", "k_qplus", "k_temper_cs", "k_opacity_qminus", "k_diffcoef",
     "k_matrix", "k_sor_resid0", "k_sum_to", "k_min_to", "k_sor_iter", "k_sor_check", "k_energy_new", "k_transport_rad",
-    "k_transport_theta", "k_velocities", NULL};
+    "k_transport_theta", "k_velocities", "k_tq_rings", NULL};
 
 // a template instance ("k_name<3>") counts as its kernel
 static int kid_of(const char *name) {
@@ -271,6 +275,38 @@ __device__ __forceinline__ double block_min(double v, double *sh) {
   return r;
 }
 
+// Per-thread part of a block reduction over p[tid], p[tid+NT], ... (that
+// order): eight loads are issued before their adds, so the loop is not one
+// memory latency per element.
+template <int NT>
+__device__ __forceinline__ double strided_sum(const double *__restrict__ p, int n) {
+  double s = 0.0;
+  int b = threadIdx.x;
+  for (; b + 7 * NT < n; b += 8 * NT) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) v[u] = p[b + u * NT];
+#pragma unroll
+    for (int u = 0; u < 8; u++) s += v[u];
+  }
+  for (; b < n; b += NT) s += p[b];
+  return s;
+}
+
+template <int NT>
+__device__ __forceinline__ double strided_min(const double *__restrict__ p, int n, double m) {
+  int b = threadIdx.x;
+  for (; b + 7 * NT < n; b += 8 * NT) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) v[u] = p[b + u * NT];
+#pragma unroll
+    for (int u = 0; u < 8; u++) m = v[u] < m ? v[u] : m;
+  }
+  for (; b < n; b += NT) { const double v = p[b]; m = v < m ? v : m; }
+  return m;
+}
+
 // sequential ring sum in the reference's order (one thread walks the ring);
 // the LDS reads of the next 16 values are issued before the 16 dependent
 // adds of the current batch, so the chain runs at the FP64 add latency.
@@ -403,10 +439,19 @@ __global__ void __launch_bounds__(RB) k_eos_cfl(Geo g, const double *__restrict_
 }
 
 // minimum over rings plus the shear limit between rings (SourceEuler.c:673-676)
-__global__ void __launch_bounds__(RB) k_cfl_final(Geo g, DevCtl *ctl) {
+// The minimum of the DiffusionTimestep partials of this substep's k_diffcoef
+// (nstellar of them; 0: none formed) goes to ctl->dt_stellar.
+__global__ void __launch_bounds__(RB) k_cfl_final(Geo g, DevCtl *ctl, const double *__restrict__ pstellar,
+                                                  int nstellar) {
   __shared__ double red[RB];
+  if (nstellar > 0) {
+    double ms = 1.0e38;
+    ms = strided_min<RB>(pstellar, nstellar, ms);
+    ms = block_min<RB>(ms, red);
+    if (threadIdx.x == 0) ctl->dt_stellar = ms;
+  }
   double m = 1e30;
-  for (int i = 1 + threadIdx.x; i < g.nr; i += RB) { double v = g.ringmin[i]; if (v < m) m = v; }
+  m = strided_min<RB>(g.ringmin + 1, g.nr - 1, m);
   for (int i = threadIdx.x; i < g.nr - 1; i += RB) {
     double dt = 2.0 * DPI * CFLSECURITY / (double)g.ns /
                 fabs(g.Vmoy[i] * g.InvRmed[i] - g.Vmoy[i + 1] * g.InvRmed[i + 1]);
@@ -581,9 +626,10 @@ __global__ void __launch_bounds__(RB) k_forces_reduce(const double *__restrict__
   __shared__ double red[RB];
   const int q = blockIdx.x;
   double s = 0.0;
-  for (int b = threadIdx.x; b < nb; b += RB) s += partial[(size_t)q * nb + b];
+  s = strided_sum<RB>(partial + (size_t)q * nb, nb);
   s = block_sum<RB>(s, red);
   if (threadIdx.x == 0) {
+    if (q == 0) { ctl->crash_e = 0; ctl->crash_rho = 0; }   // DetectCrash flags of this substep
     if (q == 0) ctl->axstar = s;
     else if (q == 1) ctl->aystar = s;
     else if ((q & 1) == 0) ctl->axpl[(q - 2) / 2] = s;
@@ -908,7 +954,7 @@ __global__ void __launch_bounds__(TB) k_diffcoef(Geo g, const double *__restrict
 __global__ void __launch_bounds__(RB) k_min_to(const double *__restrict__ pmin, int nb, double *out) {
   __shared__ double red[RB];
   double m = 1.0e38;
-  for (int b = threadIdx.x; b < nb; b += RB) { const double v = pmin[b]; if (v < m) m = v; }
+  m = strided_min<RB>(pmin, nb, m);
   m = block_min<RB>(m, red);
   if (threadIdx.x == 0) *out = m;
 }
@@ -1001,7 +1047,7 @@ __global__ void __launch_bounds__(TB) k_sor_resid0(Geo g, const double *__restri
 __global__ void __launch_bounds__(RB) k_sum_to(const double *__restrict__ partial, int nb, double *out) {
   __shared__ double red[RB];
   double s = 0.0;
-  for (int b = threadIdx.x; b < nb; b += RB) s += partial[b];
+  s = strided_sum<RB>(partial, nb);
   s = block_sum<RB>(s, red);
   if (threadIdx.x == 0) *out = s;
 }
@@ -1017,100 +1063,182 @@ __global__ void __launch_bounds__(RB) k_sum_to(const double *__restrict__ partia
 #define SOR_TJ 128
 #define SOR_LW (SOR_TJ + 4)
 
-__device__ __forceinline__ void sor_cell(const Geo &g, double *t, int r, int c, int i, int jj,
-                                         const double *__restrict__ Ml, const double *__restrict__ Mlip,
-                                         const double *__restrict__ Mlim, const double *__restrict__ Mljp,
-                                         const double *__restrict__ Mljm, const double *__restrict__ RHS,
-                                         double omega, bool owned, double &acc, int &changed) {
-  const size_t l = (size_t)i * g.ns + jj;
-  const int k = r * SOR_LW + c;
-  const double ml = Ml[l];
-  const double resid = t[k] * ml + t[k + SOR_LW] * Mlip[l] + t[k - SOR_LW] * Mlim[l] + t[k + 1] * Mljp[l] +
-                       t[k - 1] * Mljm[l] - RHS[l];
-  const double dtemp = omega * resid / ml;
+// one cell of a half-sweep: its matrix row is loaded before the tile's T is
+// staged, so that all global loads of the workgroup are in flight together
+struct SorCell {
+  double ml, mip, mim, mjp, mjm, rhs;
+  int k;                        // LDS index of the cell, -1: no cell
+  bool own;
+};
+
+__device__ __forceinline__ void sor_load(SorCell &x, size_t l, int k, bool own, const double *__restrict__ Ml,
+                                         const double *__restrict__ Mlip, const double *__restrict__ Mlim,
+                                         const double *__restrict__ Mljp, const double *__restrict__ Mljm,
+                                         const double *__restrict__ RHS) {
+  x.ml = Ml[l];
+  x.mip = Mlip[l];
+  x.mim = Mlim[l];
+  x.mjp = Mljp[l];
+  x.mjm = Mljm[l];
+  x.rhs = RHS[l];
+  x.k = k;
+  x.own = own;
+}
+
+__device__ __forceinline__ void sor_update(const SorCell &x, double *t, double omega, double &acc, int &changed) {
+  const int k = x.k;
+  const double resid = t[k] * x.ml + t[k + SOR_LW] * x.mip + t[k - SOR_LW] * x.mim + t[k + 1] * x.mjp +
+                       t[k - 1] * x.mjm - x.rhs;
+  const double dtemp = omega * resid / x.ml;
   const double tn = t[k] - dtemp;
   t[k] = tn;
-  if (owned) {
+  if (x.own) {
     acc += fabs(resid);
     if (dtemp > tn * SORPRECISION) changed++;
   }
 }
+
+// red cells of the tile and its one-cell halo: per staged row r = 1..ti+2 the
+// columns c = cf, cf+2, ... <= tw+2 with (i+j) odd; black cells: owned rows
+// and columns with (i+j) even
+#define SOR_NRED (((SOR_TI + 2) * ((SOR_TJ + 3) / 2) + TB - 1) / TB)
+#define SOR_NBLK ((SOR_TI * ((SOR_TJ + 1) / 2) + TB - 1) / TB)
 
 __global__ void __launch_bounds__(TB) k_sor_iter(Geo g, double *__restrict__ T0, double *__restrict__ T1,
                                                  const double *__restrict__ Ml, const double *__restrict__ Mlip,
                                                  const double *__restrict__ Mlim, const double *__restrict__ Mljp,
                                                  const double *__restrict__ Mljm, const double *__restrict__ RHS,
                                                  double omega, double *__restrict__ psum, int *__restrict__ pcnt,
-                                                 const DevCtl *__restrict__ ctl) {
+                                                 DevCtl *__restrict__ ctl, int it) {
   __shared__ double t[(SOR_TI + 4) * SOR_LW];
   __shared__ double red[TB];
-  if (ctl->sor_done) return;
-  const int cur = ctl->sor_cur;
-  const double *__restrict__ Tin = cur ? T1 : T0;
-  double *__restrict__ Tout = cur ? T0 : T1;
   const int ns = g.ns, nr = g.nr;
   const int i0 = 1 + blockIdx.y * SOR_TI;
   const int i1 = min(i0 + SOR_TI, nr - 1);          // owned rings [i0, i1) within [1, nr-2]
   const int j0 = blockIdx.x * SOR_TJ;
   const int tw = min(SOR_TJ, ns - j0);               // owned sectors [j0, j0+tw)
   const int ti = i1 - i0;
+  const int tid = threadIdx.x;
+  int cur = 0;
+  if (it == 0) {
+    // first iteration of a solve: reset the control words, and give T1 the
+    // rings no sweep writes (0, NRAD-1 and the ghost ring)
+    if (blockIdx.x == 0 && blockIdx.y == 0 && tid == 0) {
+      ctl->sor_done = 0;
+      ctl->sor_niter = 0;
+      ctl->sor_iter = 0;
+      ctl->sor_cur = 0;
+    }
+    if (blockIdx.y == 0)
+      for (int j = j0 + tid; j < j0 + tw; j += TB) T1[j] = T0[j];
+    if (i1 == nr - 1)
+      for (int j = j0 + tid; j < j0 + tw; j += TB) {
+        T1[(size_t)(nr - 1) * ns + j] = T0[(size_t)(nr - 1) * ns + j];
+        T1[(size_t)nr * ns + j] = T0[(size_t)nr * ns + j];
+      }
+  } else {
+    if (ctl->sor_done) return;
+    cur = ctl->sor_cur;
+  }
+  const double *__restrict__ Tin = cur ? T1 : T0;
+  double *__restrict__ Tout = cur ? T0 : T1;
+  SorCell rc[SOR_NRED], bc[SOR_NBLK];
+  const int rw = (tw + 3) / 2, bw = (tw + 1) / 2;
+#pragma unroll
+  for (int q = 0; q < SOR_NRED; q++) {
+    rc[q].k = -1;
+    const int sl = tid + q * TB;
+    if (sl < (ti + 2) * rw) {
+      const int r = 1 + sl / rw, m = sl - (r - 1) * rw;
+      const int i = i0 - 2 + r;
+      const int c = (((i + j0 - 1) & 1) ? 1 : 2) + 2 * m;
+      if (c <= tw + 2 && i >= 1 && i <= nr - 2) {
+        int j = j0 - 2 + c;
+        if (j < 0) j += ns;
+        if (j >= ns) j -= ns;
+        const bool own = (r >= 2 && r < ti + 2 && c >= 2 && c < tw + 2);
+        sor_load(rc[q], (size_t)i * ns + j, r * SOR_LW + c, own, Ml, Mlip, Mlim, Mljp, Mljm, RHS);
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < SOR_NBLK; q++) {
+    bc[q].k = -1;
+    const int sl = tid + q * TB;
+    if (sl < ti * bw) {
+      const int r = 2 + sl / bw, m = sl - (r - 2) * bw;
+      const int i = i0 - 2 + r;
+      const int c = (((i + j0) & 1) ? 3 : 2) + 2 * m;
+      if (c <= tw + 1)
+        sor_load(bc[q], (size_t)i * ns + (j0 - 2 + c), r * SOR_LW + c, true, Ml, Mlip, Mlim, Mljp, Mljm, RHS);
+    }
+  }
   // stage rings i0-2 .. i1+1 and sectors j0-2 .. j0+tw+1 (periodic in phi)
-  for (int idx = threadIdx.x; idx < (ti + 4) * (tw + 4); idx += TB) {
+  for (int idx = tid; idx < (ti + 4) * (tw + 4); idx += TB) {
     const int r = idx / (tw + 4), c = idx - r * (tw + 4);
     const int i = i0 - 2 + r;
     if (i < 0 || i > nr - 1) continue;
     int j = j0 - 2 + c;
-    j = (j % ns + ns) % ns;
+    if (j < 0) j += ns;
+    if (j >= ns) j -= ns;
     t[r * SOR_LW + c] = Tin[(size_t)i * ns + j];
   }
   __syncthreads();
   double acc = 0.0;
   int changed = 0;
-  // red half-sweep ((i+j) odd), owned cells and the one-cell halo
-  for (int idx = threadIdx.x; idx < (ti + 2) * (tw + 2); idx += TB) {
-    const int r = 1 + idx / (tw + 2), c = 1 + idx % (tw + 2);
-    const int i = i0 - 2 + r;
-    if (i < 1 || i > nr - 2) continue;
-    const int j = j0 - 2 + c;
-    if (((i + j) & 1) == 0) continue;
-    const int jj = (j % ns + ns) % ns;
-    const bool owned = (r >= 2 && r < ti + 2 && c >= 2 && c < tw + 2);
-    sor_cell(g, t, r, c, i, jj, Ml, Mlip, Mlim, Mljp, Mljm, RHS, omega, owned, acc, changed);
-  }
+  // red half-sweep ((i+j) odd): owned cells and the one-cell halo
+#pragma unroll
+  for (int q = 0; q < SOR_NRED; q++)
+    if (rc[q].k >= 0) sor_update(rc[q], t, omega, acc, changed);
   __syncthreads();
-  // black half-sweep ((i+j) even), owned cells
-  for (int idx = threadIdx.x; idx < ti * tw; idx += TB) {
-    const int r = 2 + idx / tw, c = 2 + idx % tw;
-    const int i = i0 - 2 + r;
-    const int j = j0 - 2 + c;
-    if (((i + j) & 1) != 0) continue;
-    sor_cell(g, t, r, c, i, j, Ml, Mlip, Mlim, Mljp, Mljm, RHS, omega, true, acc, changed);
-  }
+  // black half-sweep ((i+j) even): owned cells
+#pragma unroll
+  for (int q = 0; q < SOR_NBLK; q++)
+    if (bc[q].k >= 0) sor_update(bc[q], t, omega, acc, changed);
   __syncthreads();
-  for (int idx = threadIdx.x; idx < ti * tw; idx += TB) {
+  for (int idx = tid; idx < ti * tw; idx += TB) {
     const int r = 2 + idx / tw, c = 2 + idx % tw;
     Tout[(size_t)(i0 - 2 + r) * ns + (j0 - 2 + c)] = t[r * SOR_LW + c];
   }
   const int bid = blockIdx.y * gridDim.x + blockIdx.x;
   acc = block_sum<TB>(acc, red);
   const double cc = block_sum<TB>((double)changed, red);
-  if (threadIdx.x == 0) {
+  if (tid == 0) {
     psum[bid] = acc;
     pcnt[bid] = (int)cc;
   }
 }
 
 // stop test of the iteration just done (EnergySources.c:487-493), partial
-// sums added in block order
+// sums added in block order.  After the first iteration of a solve it also
+// forms the initial residual normres0 (EnergySources.c:435-451) from the
+// matrix kernel's block partials.
 __global__ void __launch_bounds__(RB) k_sor_check(const double *__restrict__ psum, const int *__restrict__ pcnt,
-                                                  int nb, DevCtl *ctl, int maxiter) {
+                                                  int nb, DevCtl *ctl, int maxiter, const double *__restrict__ mresp,
+                                                  int nbm, int it) {
   __shared__ double red[RB];
-  if (ctl->sor_done) return;
-  double s = 0.0, c = 0.0;
-  for (int b = threadIdx.x; b < nb; b += RB) {
-    s += psum[b];
-    c += (double)pcnt[b];
+  if (it == 0) {
+    double r = 0.0;
+    r = strided_sum<RB>(mresp, nbm);
+    r = block_sum<RB>(r, red);
+    if (threadIdx.x == 0) ctl->sor_normres0 = r;
+    __syncthreads();
   }
+  if (ctl->sor_done) return;
+  double s = strided_sum<RB>(psum, nb);
+  int ci = 0;
+  {
+    int b = threadIdx.x;
+    for (; b + 7 * RB < nb; b += 8 * RB) {
+      int v[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) v[u] = pcnt[b + u * RB];
+#pragma unroll
+      for (int u = 0; u < 8; u++) ci += v[u];
+    }
+    for (; b < nb; b += RB) ci += pcnt[b];
+  }
+  double c = (double)ci;
   s = block_sum<RB>(s, red);
   c = block_sum<RB>(c, red);
   if (threadIdx.x == 0) {
@@ -1168,74 +1296,127 @@ struct RadQ {
   int ntile_j, ntiles;
 };
 
-// dq of ComputeStarRad at ring k from W at k-1, k, k+1
-__device__ __forceinline__ double rad_dq(const Geo &g, int k, double wm, double w0, double wp) {
-  if (k <= 0 || k >= g.nr - 1) return 0.0;
-  const double dqm = (w0 - wm) * g.InvDiffRmed[k];
-  const double dqp = (wp - w0) * g.InvDiffRmed[k + 1];
+// ring geometry of one radial tile, staged in LDS: entry r is ring i0-4+r
+// (0 outside 0..NRAD); every value the rolling window needs is ring-uniform
+#define RC_G (RC_I + 6)
+struct RadGeo {
+  double rmed[RC_G], idr[RC_G], rinf[RC_G], isurf[RC_G];
+};
+
+// slope of ComputeStarRad at ring k (dq = 0 on the first and last ring)
+__device__ __forceinline__ double rad_dq_s(int k, int nr, double idr_k, double idr_k1, double wm, double w0,
+                                           double wp) {
+  if (k <= 0 || k >= nr - 1) return 0.0;
+  const double dqm = (w0 - wm) * idr_k;
+  const double dqp = (wp - w0) * idr_k1;
   return (dqp * dqm > 0.0) ? 2.0 * dqp * dqm / (dqp + dqm) : 0.0;
 }
 
-// star value of ComputeStarRad at interface k
-__device__ __forceinline__ double rad_star(const Geo &g, int k, double wm, double w0, double dqm, double dq0,
-                                           double vrk, double dt) {
-  if (k < 1 || k > g.nr - 1) return 0.0;
-  if (vrk > 0.0) return wm + (g.Rmed[k] - g.Rmed[k - 1] - vrk * dt) * 0.5 * dqm;
-  return w0 - (g.Rmed[k + 1] - g.Rmed[k] + vrk * dt) * 0.5 * dq0;
+// star value of ComputeStarRad at interface k (0 on ring 0 and the ghost ring)
+__device__ __forceinline__ double rad_star_s(int k, int nr, double rm_km, double rm_k, double rm_kp, double wm,
+                                             double w0, double dqm, double dq0, double vrk, double dt) {
+  if (k < 1 || k > nr - 1) return 0.0;
+  if (vrk > 0.0) return wm + (rm_k - rm_km - vrk * dt) * 0.5 * dqm;
+  return w0 - (rm_kp - rm_k + vrk * dt) * 0.5 * dq0;
 }
 
-__global__ void __launch_bounds__(TB) k_transport_rad(Geo g, RadQ q, double dt) {
+// one column of quantity W (0 RadMomP, 1 RadMomM, 2 ThetaMomP, 3 ThetaMomM,
+// 4 Energy, 5 Rho) over rings i0 .. i1-1.  The loads of ring i+2 are issued
+// two iterations ahead (clamped into the grid, replaced by 0 outside it).
+template <int W>
+__device__ __forceinline__ void rad_column(const Geo &g, const RadQ &q, const RadGeo &sg, double dt, int j, int i0,
+                                           int i1) {
   const int ns = g.ns, nr = g.nr;
-  const int b = blockIdx.x, slot = b & 7, rest = b >> 3;
-  const int which = rest % RC_NQ;
-  const int tile = (rest / RC_NQ) * 8 + slot;
-  if (tile >= q.ntiles) return;
-  const int tj = tile % q.ntile_j, ti = tile / q.ntile_j;
-  const int j = tj * TB + threadIdx.x;
-  if (j >= ns) return;
-  const int i0 = ti * RC_I, i1 = min(i0 + RC_I, nr);
   const int jp = (j == ns - 1) ? 0 : j + 1;
+  const int gb = i0 - 4;                        // ring of sg entry 0
   const double dtheta = 2.0 * DPI / (double)ns;
-  double *__restrict__ out = q.out[which];
+  double *__restrict__ out = q.out[W];
+  // raw loads of the ring k = i+2 and of vr at k+1
+  auto load = [&](int k, double &rho_l, double &vr_l, double &x_l) {
+    const int kc = min(max(k, 0), nr - 1);
+    const int kv = min(max(k + 1, 0), nr);     // ring nr: ghost ring of vr
+    const size_t l = (size_t)kc * ns + j;
+    rho_l = q.rho[l];
+    vr_l = q.vr[(size_t)kv * ns + j];
+    x_l = 0.0;
+    if (W == 2) x_l = q.vt[(size_t)kc * ns + jp];
+    if (W == 3) x_l = q.vt[l];
+    if (W == 4) x_l = q.e[l];
+  };
   // rolling state for output ring i: rho and W at i, i+1; Q at i, i+1;
   // slopes at i; flux at i; vr at i+1, i+2
   double rho0 = 0.0, rho1 = 0.0, w0 = 0.0, w1 = 0.0, Q0 = 0.0, Q1 = 0.0;
   double drho0 = 0.0, dq0 = 0.0, F0 = 0.0;
   double vr1 = (i0 - 3 >= 0) ? q.vr[(size_t)(i0 - 3) * ns + j] : 0.0;
   double vr2 = (i0 - 2 >= 0) ? q.vr[(size_t)(i0 - 2) * ns + j] : 0.0;
-#pragma unroll 2
+  double pa_rho, pa_vr, pa_x, pb_rho, pb_vr, pb_x;    // rings i+2 and i+3
+  load(i0 - 2, pa_rho, pa_vr, pa_x);
+  load(i0 - 1, pb_rho, pb_vr, pb_x);
   for (int i = i0 - 4; i < i1; i++) {
-    const int k = i + 2;                       // ring loaded now
-    double rho2 = 0.0, Q2 = 0.0, vr3 = 0.0;
-    if (k + 1 >= 0 && k + 1 <= nr) vr3 = q.vr[(size_t)(k + 1) * ns + j];   // ring nr: ghost ring
-    if (k >= 0 && k < nr) {
-      const size_t l = (size_t)k * ns + j;
-      rho2 = q.rho[l];
-      switch (which) {
-        case 0: Q2 = rho2 * vr3; break;
-        case 1: Q2 = rho2 * vr2; break;
-        case 2: Q2 = rho2 * (q.vt[(size_t)k * ns + jp] + g.Rmed[k] * q.of) * g.Rmed[k]; break;
-        case 3: Q2 = rho2 * (q.vt[l] + g.Rmed[k] * q.of) * g.Rmed[k]; break;
-        case 4: Q2 = q.e[l]; break;
-        default: Q2 = rho2; break;
-      }
+    const int k = i + 2;                       // ring used now
+    const double rho_l = pa_rho, vr_l = pa_vr, x_l = pa_x;
+    pa_rho = pb_rho; pa_vr = pb_vr; pa_x = pb_x;
+    if (i + 2 < i1) load(k + 2, pb_rho, pb_vr, pb_x);
+    const bool in = (k >= 0 && k < nr);
+    const bool inv = (k + 1 >= 0 && k + 1 <= nr);
+    const double vr3 = inv ? vr_l : 0.0;
+    const double rho2 = in ? rho_l : 0.0;
+    const int r = i + 1 - gb;                  // sg entry of ring i+1
+    double Q2 = 0.0;
+    if (in) {
+      if (W == 0) Q2 = rho2 * vr3;
+      if (W == 1) Q2 = rho2 * vr2;
+      if (W == 2 || W == 3) Q2 = rho2 * (x_l + sg.rmed[r + 1] * q.of) * sg.rmed[r + 1];
+      if (W == 4) Q2 = x_l;
+      if (W == 5) Q2 = rho2;
     }
     const double w2 = Q2 / (rho2 + 1e-20);
-    const double drho1 = rad_dq(g, i + 1, rho0, rho1, rho2);
-    const double dq1 = rad_dq(g, i + 1, w0, w1, w2);
-    const double rs = rad_star(g, i + 1, rho0, rho1, drho0, drho1, vr1, dt);
-    const double qs = rad_star(g, i + 1, w0, w1, dq0, dq1, vr1, dt);
-    const double F1 = (i + 1 >= 1 && i + 1 <= nr - 1) ? dt * dtheta * g.Rinf[i + 1] * qs * rs * vr1 : 0.0;
+    const double idr_k = sg.idr[r], idr_k1 = sg.idr[r + 1];
+    const double rm_km = sg.rmed[r - 1], rm_k = sg.rmed[r], rm_kp = sg.rmed[r + 1];
+    const double drho1 = rad_dq_s(i + 1, nr, idr_k, idr_k1, rho0, rho1, rho2);
+    const double dq1 = rad_dq_s(i + 1, nr, idr_k, idr_k1, w0, w1, w2);
+    const double rs = rad_star_s(i + 1, nr, rm_km, rm_k, rm_kp, rho0, rho1, drho0, drho1, vr1, dt);
+    const double qs = rad_star_s(i + 1, nr, rm_km, rm_k, rm_kp, w0, w1, dq0, dq1, vr1, dt);
+    const double F1 = (i + 1 >= 1 && i + 1 <= nr - 1) ? dt * dtheta * sg.rinf[r] * qs * rs * vr1 : 0.0;
     if (i >= i0) {
       double varq = F0;
       varq -= F1;
-      out[(size_t)i * ns + j] = Q0 + varq * g.InvSurf[i];
+      out[(size_t)i * ns + j] = Q0 + varq * sg.isurf[r - 1];
     }
     rho0 = rho1; rho1 = rho2;
     w0 = w1; w1 = w2;
     Q0 = Q1; Q1 = Q2;
     drho0 = drho1; dq0 = dq1; F0 = F1;
     vr1 = vr2; vr2 = vr3;
+  }
+}
+
+__global__ void __launch_bounds__(TB) k_transport_rad(Geo g, RadQ q, double dt) {
+  __shared__ RadGeo sg;
+  const int b = blockIdx.x, slot = b & 7, rest = b >> 3;
+  const int which = rest % RC_NQ;
+  const int tile = (rest / RC_NQ) * 8 + slot;
+  if (tile >= q.ntiles) return;                // whole workgroup
+  const int tj = tile % q.ntile_j, ti = tile / q.ntile_j;
+  const int i0 = ti * RC_I, i1 = min(i0 + RC_I, g.nr);
+  for (int t = threadIdx.x; t < RC_G; t += TB) {
+    const int k = i0 - 4 + t;
+    const bool ok = (k >= 0 && k <= g.nr);
+    sg.rmed[t] = ok ? g.Rmed[k] : 0.0;
+    sg.idr[t] = ok ? g.InvDiffRmed[k] : 0.0;
+    sg.rinf[t] = ok ? g.Rinf[k] : 0.0;
+    sg.isurf[t] = ok ? g.InvSurf[k] : 0.0;
+  }
+  __syncthreads();
+  const int j = tj * TB + threadIdx.x;
+  if (j >= g.ns) return;
+  switch (which) {
+    case 0: rad_column<0>(g, q, sg, dt, j, i0, i1); break;
+    case 1: rad_column<1>(g, q, sg, dt, j, i0, i1); break;
+    case 2: rad_column<2>(g, q, sg, dt, j, i0, i1); break;
+    case 3: rad_column<3>(g, q, sg, dt, j, i0, i1); break;
+    case 4: rad_column<4>(g, q, sg, dt, j, i0, i1); break;
+    default: rad_column<5>(g, q, sg, dt, j, i0, i1); break;
   }
 }
 
@@ -1260,11 +1441,15 @@ __device__ __forceinline__ double th_dq(const double *w, int x, int ns, double i
   return ((dqp * dqm > 0.0) && (dqp + dqm != 0.0)) ? dqp * dqm / (dqp + dqm) * invdxtheta : 0.0;
 }
 
-// star value at j (ComputeStarTheta's second loop)
+// star value at j (ComputeStarTheta's second loop); only the upwind cell's
+// slope is formed
 __device__ __forceinline__ double th_star(const double *w, int j, int ns, double ksi, double dxtheta, double invdxtheta) {
   const int jm = (j == 0) ? ns - 1 : j - 1;
-  if (ksi > 0.0) return w[jm] + (dxtheta - ksi) * th_dq(w, jm, ns, invdxtheta);
-  return w[j] - (dxtheta + ksi) * th_dq(w, j, ns, invdxtheta);
+  const bool up = ksi > 0.0;
+  const int x = up ? jm : j;
+  const double d = th_dq(w, x, ns, invdxtheta);
+  const double wx = w[x];
+  return up ? wx + (dxtheta - ksi) * d : wx - (dxtheta + ksi) * d;
 }
 
 struct ThGeom {
@@ -1401,11 +1586,22 @@ __global__ void __launch_bounds__(TT_T) k_transport_theta(Geo g, const double *_
   store_shifted(orho);
   const double *ins[5] = {xrp, xrm, xtp, xtm, xe};
   double *outs[5] = {orp, orm, otp, otm, oe};
+  double nxt[NPT];             // the next quantity, loaded one quantity ahead
+#pragma unroll
+  for (int k = 0; k < NPT; k++) {
+    const int j = tid + k * TT_T;
+    nxt[k] = (j < ns) ? xrp[base + j] : 0.0;
+  }
   for (int which = 0; which < 5; which++) {
 #pragma unroll
-    for (int k = 0; k < NPT; k++) {
-      const int j = tid + k * TT_T;
-      qv[k] = (j < ns) ? ins[which][base + j] : 0.0;
+    for (int k = 0; k < NPT; k++) qv[k] = nxt[k];
+    if (which < 4) {
+      const double *src = ins[which + 1];
+#pragma unroll
+      for (int k = 0; k < NPT; k++) {
+        const int j = tid + k * TT_T;
+        nxt[k] = (j < ns) ? src[base + j] : 0.0;
+      }
     }
     th_pass<NPT>(t, qv, r0, rs1, vtr, 0.0, false, s_w, s_f);
     if (fast) {
@@ -1443,6 +1639,31 @@ __global__ void __launch_bounds__(RB) k_ring_mean(Geo g, const double *__restric
     double s = ordered_ring_sum(sh, ns);
     s /= (double)ns;
     out[i] = s;
+  }
+}
+
+// UpdateLog (Force.c:41-42, 51-52) needs the 1D profiles of rho and cs only
+// on the two rings next to each logged planet: block b averages rho over
+// ring r[b] in the order of mpi_make1Dprofile and copies the sound-speed
+// ring mean of the same ring (globcsvec, the profile of the current cs);
+// ring -1 (beyond the outer edge) gives 0.
+struct TqRings {
+  int r[16];
+};
+__global__ void __launch_bounds__(RB) k_tq_rings(Geo g, const double *__restrict__ rho, TqRings rr, DevCtl *ctl) {
+  extern __shared__ double sh[];
+  const int b = blockIdx.x, i = rr.r[b], ns = g.ns;
+  if (i < 0) {
+    if (threadIdx.x == 0) { ctl->tq_rho[b] = 0.0; ctl->tq_cs[b] = 0.0; }
+    return;
+  }
+  for (int j = threadIdx.x; j < ns; j += RB) sh[j] = rho[(size_t)i * ns + j];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = ordered_ring_sum(sh, ns);
+    m /= (double)ns;
+    ctl->tq_rho[b] = m;
+    ctl->tq_cs[b] = g.gcs[i];
   }
 }
 
@@ -1488,6 +1709,11 @@ extern "C" int thorin_dev_create(thorin_ctx *c) {
   d->nr = c->nr;
   d->ns = c->ns;
   d->nf = c->nfield;
+  if (c->ns % 2 != 0) {
+    thorin_set_error("NSEC must be even: the red-black SOR sweep of the energy equation alternates colours "
+                     "around each ring");
+    return -1;
+  }
   if (c->ns > TT_T * TT_NPT) {
     thorin_set_error("NSEC too large for the azimuthal transport (max 3072)");
     return -1;
@@ -1617,6 +1843,8 @@ extern "C" int thorin_dev_create(thorin_ctx *c) {
   d->nblk_sor = ((half + TB - 1) / TB) * (c->nr - 2);
   HCHECK(hipMalloc(&d->sorp, sizeof(double) * (size_t)d->nblk_cells * 2));
   HCHECK(hipMalloc(&d->sorc, sizeof(int) * (size_t)d->nblk_cells * 2));
+  HCHECK(hipMalloc(&d->pstellar, sizeof(double) * (size_t)d->nblk_cells));
+  HCHECK(hipMalloc(&d->mresp, sizeof(double) * (size_t)d->nblk_cells));
   // geometry fields
   HCHECK(hipMemcpyAsync(d->b[B_ABS], c->h_abs, sizeof(double) * d->nf, hipMemcpyHostToDevice, d->s));
   HCHECK(hipMemcpyAsync(d->b[B_ORD], c->h_ord, sizeof(double) * d->nf, hipMemcpyHostToDevice, d->s));
@@ -1638,6 +1866,8 @@ extern "C" void thorin_dev_destroy(thorin_ctx *c) {
   if (d->red) (void)hipFree(d->red);
   if (d->sorp) (void)hipFree(d->sorp);
   if (d->sorc) (void)hipFree(d->sorc);
+  if (d->pstellar) (void)hipFree(d->pstellar);
+  if (d->mresp) (void)hipFree(d->mresp);
   if (d->prof_pend) {
     for (auto &r : *d->prof_pend) { (void)hipEventDestroy(r.second.first); (void)hipEventDestroy(r.second.second); }
     delete d->prof_pend;
@@ -1842,19 +2072,15 @@ static int sor_solve(thorin_ctx *c, double omega, int *niter_out) {
   thorin_dev *d = c->dev;
   const Geo &g = d->g;
   const int nr = d->nr, ns = d->ns;
-  const size_t nbytes = sizeof(double) * d->nf;
   // initial residual (EnergySources.c:435-451); the matrix is unchanged and
   // T is the same initial guess for every solve of a substep
   if (!d->normres0_valid) {
     dim3 gc = cell_grid(d, nr - 2);
-    const int nb0 = gc.x * gc.y;
     LAUNCH(k_sor_resid0, gc, dim3(TB), 0, g, d->b[B_TEMPER], d->b[B_ML], d->b[B_MLIP], d->b[B_MLIM], d->b[B_MLJP],
-           d->b[B_MLJM], d->b[B_RHS], d->sorp);
-    LAUNCH(k_sum_to, dim3(1), dim3(RB), 0, d->sorp, nb0, &d->ctl->sor_normres0);
+           d->b[B_MLJM], d->b[B_RHS], d->mresp);
+    d->nb_mresp = (int)(gc.x * gc.y);
     d->normres0_valid = 1;
   }
-  HCHECK(hipMemcpyAsync(d->b[B_TEMPER2], d->b[B_TEMPER], nbytes, hipMemcpyDeviceToDevice, d->s));
-  HCHECK(hipMemsetAsync(&d->ctl->sor_done, 0, 4 * sizeof(int), d->s));
   dim3 gs((unsigned)((ns + SOR_TJ - 1) / SOR_TJ), (unsigned)((nr - 2 + SOR_TI - 1) / SOR_TI));
   const int nbs = gs.x * gs.y;
   int launched = 0;
@@ -1862,9 +2088,11 @@ static int sor_solve(thorin_ctx *c, double omega, int *niter_out) {
   if (batch > 256) batch = 256;
   for (;;) {
     for (int b = 0; b < batch; b++) {
+      const int it = launched + b;
       LAUNCH(k_sor_iter, gs, dim3(TB), 0, g, d->b[B_TEMPER], d->b[B_TEMPER2], d->b[B_ML], d->b[B_MLIP],
-             d->b[B_MLIM], d->b[B_MLJP], d->b[B_MLJM], d->b[B_RHS], omega, d->sorp, d->sorc, d->ctl);
-      LAUNCH(k_sor_check, dim3(1), dim3(RB), 0, d->sorp, d->sorc, nbs, d->ctl, SORMAXITERS);
+             d->b[B_MLIM], d->b[B_MLJP], d->b[B_MLJM], d->b[B_RHS], omega, d->sorp, d->sorc, d->ctl, it);
+      LAUNCH(k_sor_check, dim3(1), dim3(RB), 0, d->sorp, d->sorc, nbs, d->ctl, SORMAXITERS, d->mresp, d->nb_mresp,
+             it);
     }
     HCHECK(hipGetLastError());
     launched += batch;
@@ -1967,7 +2195,7 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
     if (!d->cfl_ready || invdtreb_sq != d->cfl_invdtreb) {
       LAUNCH(k_eos_cfl, dim3(nr), rb, sizeof(double) * (3 * ns + RB), d->g, d->b[B_DENS], d->b[B_ENERGY],
              d->b[B_TEMPER], d->b[B_CS], d->b[B_VRAD], d->b[B_VTHETA], invdtreb_sq, 0);
-      LAUNCH(k_cfl_final, dim3(1), rb, 0, d->g, d->ctl);
+      LAUNCH(k_cfl_final, dim3(1), rb, 0, d->g, d->ctl, (const double *)nullptr, 0);
       HCHECK(hipGetLastError());
       HCHECK(hipMemcpyAsync(d->hctl, d->ctl, sizeof(DevCtl), hipMemcpyDeviceToHost, d->s));
       HCHECK(hipStreamSynchronize(d->s));
@@ -2038,21 +2266,22 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
     int anyfeel = 0;
     for (int k = 0; k < sys->nb; k++) anyfeel |= sys->feeldisk[k];
     const int want_tq = p->write_torque && log && log->nsub == 0 && sys->nb > 0;
-    if (anyfeel || want_tq) {
+    if (anyfeel) {
       HCHECK(hipMemcpyAsync(d->hctl, d->ctl, sizeof(DevCtl), hipMemcpyDeviceToHost, d->s));
       HCHECK(hipStreamSynchronize(d->s));
       for (int k = 0; k < sys->nb; k++) { sys->ax[k] = d->hctl->axpl[k]; sys->ay[k] = d->hctl->aypl[k]; }
     }
+    // UpdateLog (Force.c:27-70) at the first substep of the DT: the planet
+    // side is taken now, the two ring means per planet are formed on the
+    // device and the rows are completed after the end-of-substep sync
+    struct { double t, x, y, m, frac; } tqp[8];
+    int ntq = 0;
     if (want_tq) {
-      // UpdateLog (Force.c:27-70) at the first substep of the DT
-      std::vector<double> rho1d(nr), cs1d(nr);
-      if (thorin_ring_means(c, THORIN_F_DENS, rho1d.data())) return -1;
-      HCHECK(hipMemcpyAsync(cs1d.data(), d->g.gcs, sizeof(double) * nr, hipMemcpyDeviceToHost, d->s));
-      HCHECK(hipStreamSynchronize(d->s));
-      log->tq_rows = sys->nb < 8 ? sys->nb : 8;
-      for (int k = 0; k < log->tq_rows; k++) {
-        const double x = sys->x[k], y = sys->y[k], ax = sys->ax[k], ay = sys->ay[k], m = sys->mass[k];
-        const double r2 = x * x + y * y, r = sqrt(r2);
+      TqRings rr;
+      ntq = sys->nb < 8 ? sys->nb : 8;
+      for (int k = 0; k < ntq; k++) {
+        const double x = sys->x[k], y = sys->y[k];
+        const double r = sqrt(x * x + y * y);
         double ifrac;   // GetGlobalIFrac (LowTasks.c:21-31)
         if (r < c->GlobalRmed[0]) ifrac = 0.0;
         else if (r > c->GlobalRmed[nr - 1]) ifrac = (double)nr - 1.0;
@@ -2061,19 +2290,16 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
           while (c->GlobalRmed[i] <= r) i++;
           ifrac = (double)i + (r - c->GlobalRmed[i - 1]) / (c->GlobalRmed[i] - c->GlobalRmed[i - 1]) - 1.0;
         }
-        const double frac = ifrac - floor(ifrac);
         const int ii = (int)ifrac;
-        const double rhopl = rho1d[ii] * (1.0 - frac) + (ii + 1 < nr ? rho1d[ii + 1] : 0.0) * frac;
-        const double cspl = cs1d[ii] * (1.0 - frac) + (ii + 1 < nr ? cs1d[ii + 1] : 0.0) * frac;
-        const double omega = pow(r, -1.5);
-        const double h = cspl / (omega * r) * c->k.sqrt_adiabind_inv;
-        double norm = m * omega * r2 / h;
-        norm *= norm;
-        norm *= rhopl;
-        const double tq = x * ay - y * ax;
-        const double row[7] = {c->st.physical_time, tq, tq * m * p->adiabind / norm, x, y, ax, ay};
-        for (int q = 0; q < 7; q++) log->tq[k][q] = row[q];
+        tqp[k].t = c->st.physical_time;
+        tqp[k].x = x;
+        tqp[k].y = y;
+        tqp[k].m = sys->mass[k];
+        tqp[k].frac = ifrac - floor(ifrac);
+        rr.r[2 * k] = ii;
+        rr.r[2 * k + 1] = ii + 1 < nr ? ii + 1 : -1;
       }
+      LAUNCH(k_tq_rings, dim3((unsigned)(2 * ntq)), rb, sizeof(double) * ns, d->g, d->b[B_DENS], rr, d->ctl);
     }
     if (anyfeel) {
       std::vector<double> rmean(nr), csmean(nr);
@@ -2105,7 +2331,6 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
       if (e) { thorin_set_error("N-body step failed"); return -1; }
     }
     // ---- ApplyBoundaryCondition, DetectCrash, ComputePressureField, SubStep1
-    HCHECK(hipMemsetAsync(&d->ctl->crash_e, 0, 2 * sizeof(int), d->s));
     if (p->damping_on)
       LAUNCH(k_boundary, gall, tb, 0, d->g, d->b[B_VRAD], d->b[B_VTHETA], d->b[B_DENS], d->b[B_ENERGY], dt);
     LAUNCH(k_substep1, gall, tb, 0, d->g, d->b[B_DENS], d->b[B_ENERGY], d->b[B_VRAD], d->b[B_VTHETA], d->b[B_AGR],
@@ -2147,14 +2372,13 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
            d->b[B_QMINUS], d->opa_filled ? 0 : 1);
     d->opa_filled = 1;
     LAUNCH(k_diffcoef, gall, tb, 0, d->g, d->b[B_DENS], d->b[B_CS], d->b[B_TEMPER], d->b[B_OPACITY], d->b[B_DIFF],
-           d->red);
-    LAUNCH(k_min_to, dim3(1), rb, 0, d->red, d->nblk_cells, &d->ctl->dt_stellar);
+           d->pstellar);
     {
       const dim3 gm = cell_grid(d, nr - 2);
       LAUNCH(k_matrix, gm, tb, 0, d->g, d->b[B_DENS], d->b[B_CS], d->b[B_DIFF], d->b[B_DIVV], d->b[B_QMINUS],
              d->b[B_QPLUS], d->b[B_TEMPER], d->b[B_ML], d->b[B_MLIP], d->b[B_MLIM], d->b[B_MLJP], d->b[B_MLJM],
-             d->b[B_RHS], dt, d->sorp);
-      LAUNCH(k_sum_to, dim3(1), rb, 0, d->sorp, (int)(gm.x * gm.y), &d->ctl->sor_normres0);
+             d->b[B_RHS], dt, d->mresp);
+      d->nb_mresp = (int)(gm.x * gm.y);
       d->normres0_valid = 1;
     }
     HCHECK(hipGetLastError());
@@ -2217,7 +2441,7 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
     d->cfl_invdtreb = minstep_rebound(sys);
     LAUNCH(k_eos_cfl, dim3(nr), rb, sizeof(double) * (3 * ns + RB), d->g, d->b[B_DENS], d->b[B_ENERGY],
            d->b[B_TEMPER], d->b[B_CS], d->b[B_VRAD], d->b[B_VTHETA], d->cfl_invdtreb, 1);
-    LAUNCH(k_cfl_final, dim3(1), rb, 0, d->g, d->ctl);
+    LAUNCH(k_cfl_final, dim3(1), rb, 0, d->g, d->ctl, (const double *)d->pstellar, d->nblk_cells);
     HCHECK(hipGetLastError());
     if (log && log->nsub < THORIN_MAXSUB) {
       log->cfl_n[log->nsub] = N;
@@ -2231,6 +2455,25 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
     HCHECK(hipStreamSynchronize(d->s));
     d->cfl_ready = 1;
     c->st.dt_stellar = d->hctl->dt_stellar;
+    for (int k = 0; k < sys->nb; k++) { sys->ax[k] = d->hctl->axpl[k]; sys->ay[k] = d->hctl->aypl[k]; }
+    if (ntq) {
+      log->tq_rows = ntq;
+      for (int k = 0; k < ntq; k++) {
+        const double x = tqp[k].x, y = tqp[k].y, m = tqp[k].m, frac = tqp[k].frac;
+        const double ax = d->hctl->axpl[k], ay = d->hctl->aypl[k];
+        const double r2 = x * x + y * y, r = sqrt(r2);
+        const double rhopl = d->hctl->tq_rho[2 * k] * (1.0 - frac) + d->hctl->tq_rho[2 * k + 1] * frac;
+        const double cspl = d->hctl->tq_cs[2 * k] * (1.0 - frac) + d->hctl->tq_cs[2 * k + 1] * frac;
+        const double omega = pow(r, -1.5);
+        const double h = cspl / (omega * r) * c->k.sqrt_adiabind_inv;
+        double norm = m * omega * r2 / h;
+        norm *= norm;
+        norm *= rhopl;
+        const double tq = x * ay - y * ax;
+        const double row[7] = {tqp[k].t, tq, tq * m * p->adiabind / norm, x, y, ax, ay};
+        for (int q = 0; q < 7; q++) log->tq[k][q] = row[q];
+      }
+    }
     if (prof_flush(d)) return -1;
     if (d->hctl->crash_e) {
       c->st.already_crashed++;

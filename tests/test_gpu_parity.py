@@ -45,10 +45,22 @@ def test_gpu_vs_reference_golden(name):
     for f in STATE_FIELDS:
         if f"{f}0" in g:
             assert np.array_equal(d.field(f), g[f"{f}0"]), f"initial {f}"
+    tq = g["tqwk0"]
     for i in range(ntot):
         lg = d.algogas(sysp, rep.cb)
         niters += [lg.sor_niter[k] for k in range(lg.nsub)]
         times.append(d.state().physical_time)
+        # torque log row of this DT (UpdateLog): time and position exact;
+        # accelerations (block-ordered reductions) and torques to 1e-10 of
+        # the acceleration scale
+        assert lg.tq_rows >= 1
+        row = [lg.tq[0][q] for q in range(7)]
+        assert row[0] == tq[i, 0] and row[3] == tq[i, 3] and row[4] == tq[i, 4], f"DT {i}: torque row time/position"
+        scale = abs(tq[i, 5]) + abs(tq[i, 6])
+        for q in (1, 5, 6):
+            assert abs(row[q] - tq[i, q]) <= 1e-10 * scale, f"DT {i}: torque column {q}"
+        if tq[i, 1] != 0.0:
+            assert abs(row[2] - tq[i, 2]) <= 1e-10 * scale * abs(tq[i, 2] / tq[i, 1]), f"DT {i}: normalised torque"
         if (i + 1) % ninterm == 0:
             n = (i + 1) // ninterm
             for f in STATE_FIELDS:
