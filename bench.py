@@ -30,16 +30,37 @@ sys.path.insert(0, ROOT)
 METRIC = "steps/s @1024×1536 fp64; SOR ms/step CNN-warm+FP32-refine vs plain @1e-10"
 HBM_PEAK_GBS = 8000.0
 
-# Compulsory FP64 traffic per cell and launch of each kernel (8 B per field
-# read or written once; the field lists are in DESIGN.md).  SOR: 64 B per
-# cell and iteration over the updated rings, split over its two half-sweep
-# launches.
+# Compulsory FP64 traffic per cell and launch of each kernel: 8 B per full
+# field the kernel must read or write once.  SOR: 6 matrix coefficients + T
+# read, T written = 64 B per cell and iteration over rings 1..NRAD-2.
 ALG_BYTES_PER_CELL = {
-    "k_sor_iter": 64, "k_transport_theta": 104, "k_transport_rad": 80, "k_forces": 32,
-    "k_substep1": 64, "k_divstress": 56, "k_visc_update": 72, "k_substep2": 56, "k_qplus": 48,
+    "k_sor_iter": 64, "k_transport_theta": 104, "k_transport_rad": 80, "k_forces": 48,
+    "k_substep1": 64, "k_divstress": 56, "k_visc_update": 64, "k_substep2": 56, "k_qplus": 48,
     "k_temper_cs": 32, "k_opacity_qminus": 40, "k_diffcoef": 40, "k_matrix": 104, "k_velocities": 56,
-    "k_energy_new": 24, "k_indirect": 48, "k_cfl_ring": 24, "k_sor_resid0": 56,
+    "k_energy_new": 24, "k_indirect": 48, "k_eos_cfl": 48, "k_sor_resid0": 56,
 }
+# Whole step: the fused-dataflow minimum of the hydro step, 82 field passes
+# per substep (656 B/cell) plus 64 B/cell per SOR iteration.
+STEP_BYTES_PER_CELL = 656
+SOR_BYTES_PER_CELL_ITER = 64
+# HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
+# (KiB per dispatch), scaled by what the counters read for a 1 GiB device copy
+# (FETCH_SIZE reads half the bytes moved, WRITE_SIZE all of them).
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_damped_1024x1536.txt")
+FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
+
+
+def measured_traffic(kname):
+    """Bytes per launch of kname from the committed counter summary, or None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) >= 3 and parts[0].split("<")[0] == kname:
+                    return FETCH_SCALE * float(parts[1]) * 1024 + WRITE_SCALE * float(parts[2]) * 1024
+    except OSError:
+        return None
+    return None
 
 
 def parse():
@@ -51,7 +72,21 @@ def parse():
     ap.add_argument("--nsec", type=int, default=1536)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--roofline-kernel", default="auto")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 1 GiB device copy (3 times), the reference for scaling "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes")
     return ap.parse_args()
+
+
+def copy_calibration():
+    import torch
+    n = 1 << 27                       # 2^27 doubles = 1 GiB
+    x = torch.ones(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": n * 8, "copies": 3}))
 
 
 def cpu_baseline_reference(nrad, nsec):
@@ -94,6 +129,9 @@ def cpu_baseline_reference(nrad, nsec):
 
 def main():
     args = parse()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU radial slabs are "
                                    "described in DESIGN.md, not built)"}))
@@ -139,6 +177,9 @@ def main():
     cells = (nrad - 2) * nsec if kname.startswith("k_sor") or kname == "k_matrix" else nrad * nsec
     alg = ALG_BYTES_PER_CELL.get(kname, 0) * cells
     achieved = alg / avg_s / 1e9 if avg_s > 0 else 0.0
+    traffic = measured_traffic(kname) if (nrad, nsec) == (1024, 1536) else None
+    step_bytes = (STEP_BYTES_PER_CELL * nrad * nsec +
+                  SOR_BYTES_PER_CELL_ITER * (nrad - 2) * nsec * sor_iters / args.steps)
     out = {
         "metric": METRIC, "value": value, "unit": "steps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": warm, "ms_per_step": 1e3 * elapsed / args.steps, "higher_is_better": True,
@@ -147,9 +188,12 @@ def main():
                    "substeps": substeps, "sor_iters_per_step": sor_iters / args.steps,
                    "opacity": "LP85", "boundary": "DAMPING/ZEROVRAD", "planets": 1},
         "roofline": {"kernel": kname, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "alg_bytes_per_launch": alg, "avg_launch_us": avg_s * 1e6, "launches": n_k,
                      "kernel_share_of_step": (ms_k * 1e-3) / elapsed},
+        "step_roofline": {"alg_bytes_per_step": step_bytes, "achieved": step_bytes / (elapsed / args.steps) / 1e9,
+                          "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                          "frac": step_bytes / (elapsed / args.steps) / 1e9 / HBM_PEAK_GBS},
         "kernel_ms_per_step_warmup": {k: v[0] / max(warm - 1, 1) for k, v in sorted(
             warm_prof.items(), key=lambda kv: -kv[1][0])},
     }
