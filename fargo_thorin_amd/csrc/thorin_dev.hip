@@ -80,6 +80,7 @@ struct PlanetK {                // per-planet constants of FillForcesArrays (Pfr
 
 struct Geo {                    // passed by value to every kernel
   int nr, ns;
+  int ncb;                      // TB-wide column blocks per ring
   const double *Rinf, *Rsup, *Rmed, *Surf, *InvRmed, *InvSurf, *InvDiffRmed, *InvDiffRsup, *InvRinf;
   const double *OmegaInv, *SigmaMed, *SigmaInf, *EnergyMed, *VthetaMed, *WaveKiller;
   const double *pow_rmed15, *pow_rinf15, *cav_med, *cav_inf, *cfl_pow2, *supp_torque;
@@ -131,9 +132,9 @@ struct thorin_dev {
 // kernels that the event timer knows, by launch name
 static const char *const KNAMES[] = {
     "k_eos_cfl", "k_cfl_final", "k_forces", "k_forces_reduce", "k_indirect", "k_boundary", "k_substep1",
-    "k_divstress", "k_visc_update", "k_substep2", "k_qplus", "k_temper_cs", "k_opacity_qminus", "k_diffcoef",
-    "k_matrix", "k_sor_resid0", "k_sum_to", "k_min_to", "k_sor_iter", "k_sor_check", "k_energy_new", "k_transport_rad",
-    "k_transport_theta", "k_velocities", "k_tq_rings", NULL};
+    "k_divstress", "k_visc_update", "k_substep2",
+    "k_matrix", "k_sor_resid0", "k_sor_iter", "k_sor_check", "k_energy_new", "k_transport_rad",
+    "k_transport_theta", "k_velocities", "k_tq_rings", "k_substep3", NULL};
 
 // a template instance ("k_name<3>") counts as its kernel
 static int kid_of(const char *name) {
@@ -273,6 +274,34 @@ __device__ __forceinline__ double block_min(double v, double *sh) {
   double r = sh[0];
   __syncthreads();
   return r;
+}
+
+// Cell kernels run on a 1D grid of TB-wide column blocks over rings
+// i0 .. i0+nring-1.  Workgroups are dispatched round-robin over the 8 XCDs
+// (block b to XCD b % 8), each with its own L2; numbering the logical blocks
+// so that XCD x gets the contiguous band x*per .. x*per+per-1 keeps the
+// neighbouring rings of a radial stencil in the same L2.  L is the logical
+// block (ring-major); false for the padding blocks of the last band.
+__device__ __forceinline__ bool xcd_cell(const Geo &g, int i0, int nring, int &i, int &j, int &L) {
+  const int total = g.ncb * nring, per = (total + 7) >> 3;
+  const int b = blockIdx.x;
+  L = (b & 7) * per + (b >> 3);
+  if (L >= total) return false;
+  const int q = L / g.ncb;
+  i = i0 + q;
+  j = (L - q * g.ncb) * TB + threadIdx.x;
+  return true;
+}
+
+// the same banding for a grid of ntx x nty tiles (tile x fastest)
+__device__ __forceinline__ bool xcd_tile(int ntx, int nty, int &tx, int &ty, int &L) {
+  const int total = ntx * nty, per = (total + 7) >> 3;
+  const int b = blockIdx.x;
+  L = (b & 7) * per + (b >> 3);
+  if (L >= total) return false;
+  ty = L / ntx;
+  tx = L - ty * ntx;
+  return true;
 }
 
 // Per-thread part of a block reduction over p[tid], p[tid+NT], ... (that
@@ -439,7 +468,7 @@ __global__ void __launch_bounds__(RB) k_eos_cfl(Geo g, const double *__restrict_
 }
 
 // minimum over rings plus the shear limit between rings (SourceEuler.c:673-676)
-// The minimum of the DiffusionTimestep partials of this substep's k_diffcoef
+// The minimum of the DiffusionTimestep partials of this substep's k_substep3
 // (nstellar of them; 0: none formed) goes to ctl->dt_stellar.
 __global__ void __launch_bounds__(RB) k_cfl_final(Geo g, DevCtl *ctl, const double *__restrict__ pstellar,
                                                   int nstellar) {
@@ -472,8 +501,9 @@ __global__ void __launch_bounds__(TB) k_forces(Geo g, const double *__restrict__
                                                const PlanetK *__restrict__ pk, int npl, double mt,
                                                double *__restrict__ partial) {
   __shared__ double red[TB];
-  const int ns = g.ns, i = blockIdx.y;
-  const int j = blockIdx.x * TB + threadIdx.x;
+  const int ns = g.ns;
+  int i, j, bid;
+  if (!xcd_cell(g, 0, g.nr, i, j, bid)) return;
   const bool act = j < ns;
   double axs = 0.0, ays = 0.0;
   double axpl_loc[8], aypl_loc[8];
@@ -608,7 +638,7 @@ __global__ void __launch_bounds__(TB) k_forces(Geo g, const double *__restrict__
     agt[l] = 0.0 + at;
   }
   // partial sums of this block: [axstar, aystar, axpl[k], aypl[k]]
-  const int nb = gridDim.x * gridDim.y, bid = blockIdx.y * gridDim.x + blockIdx.x;
+  const int nb = g.ncb * g.nr;
   double v = block_sum<TB>(axs, red);
   if (threadIdx.x == 0) partial[bid] = v;
   v = block_sum<TB>(ays, red);
@@ -641,8 +671,9 @@ __global__ void __launch_bounds__(RB) k_forces_reduce(const double *__restrict__
 __global__ void __launch_bounds__(TB) k_indirect(Geo g, double *__restrict__ agr, double *__restrict__ agt,
                                                  const double *__restrict__ abs_, const double *__restrict__ ord,
                                                  const DevCtl *__restrict__ ctl, double indir_px, double indir_py) {
-  const int ns = g.ns, i = blockIdx.y, j = blockIdx.x * TB + threadIdx.x;
-  if (j >= ns) return;
+  const int ns = g.ns;
+  int i, j, L_;
+  if (!xcd_cell(g, 0, g.nr, i, j, L_) || j >= ns) return;
   double axindir = -ctl->axstar, ayindir = -ctl->aystar;
   if (g.indirect_term) { axindir += indir_px; ayindir += indir_py; }
   const size_t l = (size_t)i * ns + j;
@@ -657,8 +688,9 @@ __global__ void __launch_bounds__(TB) k_indirect(Geo g, double *__restrict__ agr
 // DampingBoundary (SideEuler.c:368-419) on the rings flagged by the host
 __global__ void __launch_bounds__(TB) k_boundary(Geo g, double *__restrict__ vrad, double *__restrict__ vtheta,
                                                  double *__restrict__ rho, double *__restrict__ energy, double step) {
-  const int ns = g.ns, i = blockIdx.y, j = blockIdx.x * TB + threadIdx.x;
-  if (j >= ns || !g.dampring[i]) return;
+  const int ns = g.ns;
+  int i, j, L_;
+  if (!xcd_cell(g, 0, g.nr, i, j, L_) || j >= ns || !g.dampring[i]) return;
   double vrad0 = 0.0, rho0 = 0.0, energy0 = 0.0, vtheta0 = 0.0;
   if (g.damp_init) {
     rho0 = g.SigmaMed[i];
@@ -692,8 +724,9 @@ __global__ void __launch_bounds__(TB) k_substep1(Geo g, const double *__restrict
                                                  const double *__restrict__ agt,
                                                  double *__restrict__ vradint, double *__restrict__ vthetaint,
                                                  double dt, DevCtl *ctl) {
-  const int ns = g.ns, i = blockIdx.y, j = blockIdx.x * TB + threadIdx.x;
-  if (j >= ns) return;
+  const int ns = g.ns;
+  int i, j, L_;
+  if (!xcd_cell(g, 0, g.nr, i, j, L_) || j >= ns) return;
   const double gm1 = g.adiabind - 1.0;
   const size_t l = (size_t)i * ns + j;
   const size_t ljm = (j == 0) ? (size_t)i * ns + ns - 1 : l - 1;
@@ -726,13 +759,17 @@ __global__ void __launch_bounds__(TB) k_substep1(Geo g, const double *__restrict
 
 // UpdateDivVelocAndStressTensor (Viscosity.c:85-162); Drr, Dpp, Drp stay in
 // registers.
-__global__ void __launch_bounds__(TB) k_divstress(Geo g, const double *__restrict__ vr,
-                                                  const double *__restrict__ vt,
-                                                  const double *__restrict__ rho,
-                                                  double *__restrict__ divv, double *__restrict__ Trr,
-                                                  double *__restrict__ Trp, double *__restrict__ Tpp) {
-  const int ns = g.ns, i = blockIdx.y, j = blockIdx.x * TB + threadIdx.x;
-  if (j >= ns) return;
+// divergence and stress tensor of one cell (Viscosity.c:114-160); trp is
+// formed only for i >= 1 (has_trp)
+struct Stress {
+  double dv, trr, tpp, trp;
+  bool has_trp;
+};
+
+__device__ __forceinline__ Stress stress_cell(const Geo &g, const double *__restrict__ vr,
+                                              const double *__restrict__ vt, const double *__restrict__ rho, int i,
+                                              int j) {
+  const int ns = g.ns;
   const size_t l = (size_t)i * ns + j, lip = l + ns;
   const size_t ljp = (j == ns - 1) ? (size_t)i * ns : l + 1;
   const size_t ljm = (j == 0) ? (size_t)i * ns + ns - 1 : l - 1;
@@ -741,17 +778,37 @@ __global__ void __launch_bounds__(TB) k_divstress(Geo g, const double *__restric
   const double Dpp = (vt[ljp] - vt[l]) * invdphi * g.InvRmed[i] + 0.5 * (vr[lip] + vr[l]) * g.InvRmed[i];
   double dv = (vr[lip] * g.Rsup[i] - vr[l] * g.Rinf[i]) * g.InvDiffRsup[i] * g.InvRmed[i];
   dv += (vt[ljp] - vt[l]) * invdphi * g.InvRmed[i];
-  divv[l] = dv;
+  Stress st;
+  st.dv = dv;
   const double visc = fvisc_med(g, i);
-  Trr[l] = 2.0 * rho[l] * visc * (Drr - onethird * dv);
-  Tpp[l] = 2.0 * rho[l] * visc * (Dpp - onethird * dv);
+  st.trr = 2.0 * rho[l] * visc * (Drr - onethird * dv);
+  st.tpp = 2.0 * rho[l] * visc * (Dpp - onethird * dv);
+  st.has_trp = i >= 1;
+  st.trp = 0.0;
   if (i >= 1) {
     const size_t lim = l - ns, ljmim = ljm - ns;
     const double Drp = 0.5 * (g.Rinf[i] * (vt[l] * g.InvRmed[i] - vt[lim] * g.InvRmed[i - 1]) * g.InvDiffRmed[i] +
                               (vr[l] - vr[ljm]) * invdphi * g.InvRinf[i]);
     const double vinf = fvisc_inf(g, i);
-    Trp[l] = 2.0 * 0.25 * (rho[l] + rho[lim] + rho[ljm] + rho[ljmim]) * vinf * Drp;
+    st.trp = 2.0 * 0.25 * (rho[l] + rho[lim] + rho[ljm] + rho[ljmim]) * vinf * Drp;
   }
+  return st;
+}
+
+__global__ void __launch_bounds__(TB) k_divstress(Geo g, const double *__restrict__ vr,
+                                                  const double *__restrict__ vt,
+                                                  const double *__restrict__ rho,
+                                                  double *__restrict__ divv, double *__restrict__ Trr,
+                                                  double *__restrict__ Trp, double *__restrict__ Tpp) {
+  const int ns = g.ns;
+  int i, j, L_;
+  if (!xcd_cell(g, 0, g.nr, i, j, L_) || j >= ns) return;
+  const size_t l = (size_t)i * ns + j;
+  const Stress st = stress_cell(g, vr, vt, rho, i, j);
+  divv[l] = st.dv;
+  Trr[l] = st.trr;
+  Tpp[l] = st.tpp;
+  if (st.has_trp) Trp[l] = st.trp;
 }
 
 // UpdateVelocityWithViscousTerms (Viscosity.c:165-218) + ImposeKeplerianEdges
@@ -762,8 +819,9 @@ __global__ void __launch_bounds__(TB) k_visc_update(Geo g, double *__restrict__ 
                                                     const double *__restrict__ Trp,
                                                     const double *__restrict__ Tpp, double DeltaT,
                                                     int kepler, double vkin, double vkout) {
-  const int ns = g.ns, nr = g.nr, i = blockIdx.y, j = blockIdx.x * TB + threadIdx.x;
-  if (j >= ns) return;
+  const int ns = g.ns, nr = g.nr;
+  int i, j, L_;
+  if (!xcd_cell(g, 0, g.nr, i, j, L_) || j >= ns) return;
   const size_t l = (size_t)i * ns + j, lip = l + ns;
   const size_t ljp = (j == ns - 1) ? (size_t)i * ns : l + 1;
   const size_t ljm = (j == 0) ? (size_t)i * ns + ns - 1 : l - 1;
@@ -797,8 +855,9 @@ __global__ void __launch_bounds__(TB) k_substep2(Geo g, const double *__restrict
                                                  const double *__restrict__ energy,
                                                  double *__restrict__ vradnew, double *__restrict__ vthetanew,
                                                  double *__restrict__ energyint, double dt) {
-  const int ns = g.ns, nr = g.nr, i = blockIdx.y, j = blockIdx.x * TB + threadIdx.x;
-  if (j >= ns) return;
+  const int ns = g.ns, nr = g.nr;
+  int i, j, L_;
+  if (!xcd_cell(g, 0, g.nr, i, j, L_) || j >= ns) return;
   const size_t l = (size_t)i * ns + j, lip = l + ns;
   const size_t ljp = (j == ns - 1) ? (size_t)i * ns : l + 1;
   const size_t ljm = (j == 0) ? (size_t)i * ns + ns - 1 : l - 1;
@@ -824,99 +883,6 @@ __global__ void __launch_bounds__(TB) k_substep2(Geo g, const double *__restrict
                  dt * qt * (vtheta[ljp] - vtheta[l]) * invdxtheta;
 }
 
-// ---------------------------------------------------------------------------
-// SubStep3 viscous heating Q+ (SourceEuler.c:541-567)
-__device__ __forceinline__ double qplus_cell(const Geo &g, int i, size_t l, const double *rho,
-                                             const double *Trr, const double *Trp, const double *Tpp,
-                                             const double *divv) {
-  const double visc = fvisc_med(g, i);
-  if (visc != 0.0) {
-    double q = 0.5 / visc / rho[l] * (Trr[l] * Trr[l] + 2.0 * Trp[l] * Trp[l] + Tpp[l] * Tpp[l]);
-    q += (2.0 / 9.0) * visc * rho[l] * divv[l] * divv[l];
-    return q;
-  }
-  return 0.0;
-}
-
-__global__ void __launch_bounds__(TB) k_qplus(Geo g, const double *__restrict__ rho,
-                                              const double *__restrict__ Trr, const double *__restrict__ Trp,
-                                              const double *__restrict__ Tpp, const double *__restrict__ divv,
-                                              double *__restrict__ qplus) {
-  const int ns = g.ns, i = blockIdx.y, j = blockIdx.x * TB + threadIdx.x;
-  if (j >= ns) return;
-  const size_t l = (size_t)i * ns + j;
-  if (i >= 1) {
-    qplus[l] = qplus_cell(g, i, l, rho, Trr, Trp, Tpp, divv);
-  } else {
-    const double v0 = fvisc_med(g, 0);
-    const double q1 = qplus_cell(g, 1, l + ns, rho, Trr, Trp, Tpp, divv);
-    const double q2 = qplus_cell(g, 2, l + 2 * ns, rho, Trr, Trp, Tpp, divv);
-    qplus[l] = (v0 != 0.0) ? q1 * exp(log(q1 / q2) * g.lr01 / g.lr12) : 0.0;
-  }
-}
-
-// ComputeTemperatureField + ComputeSoundSpeed (SourceEuler.c:715-792) from
-// the intermediate energy, inside ImplicitRadiativeDiffusion
-// (EnergySources.c:234-235).  The ring means these calls also update are
-// overwritten at the end of the substep before anything reads them, so they
-// are not formed here.
-__global__ void __launch_bounds__(TB) k_temper_cs(Geo g, const double *__restrict__ rho,
-                                                  const double *__restrict__ e, double *__restrict__ T,
-                                                  double *__restrict__ cs) {
-  const int ns = g.ns, i = blockIdx.y, j = blockIdx.x * TB + threadIdx.x;
-  if (j >= ns) return;
-  const size_t l = (size_t)i * ns + j;
-  const double gam = g.adiabind;
-  const double tmp2 = 1.0 / (rho[l] * 1.0);
-  const double tmp1 = e[l] * 1.0 * (gam - 1.0);
-  T[l] = tmp1 * tmp2;
-  cs[l] = sqrt(gam * (gam - 1.0) * fabs(e[l]) / rho[l]);
-}
-
-// MidplaneVolumeDensity + OpacityProfile + CalculateQminus
-// (EnergySources.c:658-680, Opacity.c:231-265, EnergySources.c:75-99)
-__global__ void __launch_bounds__(TB) k_opacity_qminus(Geo g, const double *__restrict__ rho,
-                                                       const double *__restrict__ cs,
-                                                       const double *__restrict__ T,
-                                                       double *__restrict__ opa, double *__restrict__ qminus,
-                                                       int fill_opacity) {
-  const int ns = g.ns, i = blockIdx.y, j = blockIdx.x * TB + threadIdx.x;
-  if (j >= ns) return;
-  const size_t l = (size_t)i * ns + j;
-  double o;
-  if (g.opacity_law == THORIN_OPA_CONST) {
-    o = fill_opacity ? g.parametricopacity * g.opa2cu : opa[l];
-  } else {
-    const double H = cs[l] * g.OmegaInv[i] * g.sqrt_adiabind_inv;
-    const double voldens = rho[l] / H * g.sqrt2pi_inv;
-    o = opacity_law(g, voldens * g.rho2cgs, T[l] * g.t2si) * g.opa2cu;
-  }
-  opa[l] = o;
-  const double tau = g.opacitydrop * 0.5 * o * rho[l];
-  const double taueff = 0.375 * tau + 0.25 * g.sqrt3 + 0.25 / tau;
-  qminus[l] = 2.0 * g.stefan * cube_cr(T[l]) / taueff;
-}
-
-// TemperatureGradient + DiffusionCoefs + DiffusionTimestep
-// (EnergySources.c:556-653, 833-856); |grad T| is rebuilt from T on the fly.
-__device__ __forceinline__ double gradtmag(const Geo &g, const double *T, int i, int j) {
-  const int ns = g.ns;
-  const size_t l = (size_t)i * ns + j, lip = l + ns, lim = l - ns, lipm = lip - ns;
-  const int jp = (j == ns - 1) ? 0 : j + 1;
-  const int jm = (j == 0) ? ns - 1 : j - 1;
-  const size_t ljp = (size_t)i * ns + jp;
-  const size_t ljm = (size_t)i * ns + jm;
-  const size_t ljpm = (size_t)i * ns + ((jp == 0) ? ns - 1 : jp - 1);
-  const double invdxtheta = 1.0 / (2.0 * DPI / (double)ns * g.Rmed[i]);
-  const double gtr_l = (T[l] - T[lim]) * g.InvDiffRmed[i];
-  const double gtr_lip = (T[lip] - T[lipm]) * g.InvDiffRmed[i + 1];
-  const double gtt_l = (T[l] - T[ljm]) * invdxtheta;
-  const double gtt_ljp = (T[ljp] - T[ljpm]) * invdxtheta;
-  const double r = 0.5 * (gtr_lip + gtr_l);
-  const double t = 0.5 * (gtt_ljp + gtt_l);
-  return sqrt(r * r + t * t);
-}
-
 __device__ __forceinline__ double flux_limiter(double s) {   // EnergySources.c:684-696
   double tmp;
   if (s <= 2.0) { tmp = 3.0 + sqrt(9.0 + 10.0 * s * s); return 2.0 / tmp; }
@@ -924,39 +890,114 @@ __device__ __forceinline__ double flux_limiter(double s) {   // EnergySources.c:
   return 10.0 / tmp;
 }
 
-__global__ void __launch_bounds__(TB) k_diffcoef(Geo g, const double *__restrict__ rho,
-                                                 const double *__restrict__ cs, const double *__restrict__ T,
-                                                 const double *__restrict__ opa, double *__restrict__ D,
-                                                 double *__restrict__ pmin) {
+// SubStep3 up to the matrix, fused (one pass over the grid): the second
+// UpdateDivVelocAndStressTensor (SourceEuler.c:302), viscous heating Q+ with
+// the ring-0 extrapolation (SourceEuler.c:541-567; ring-0 cells rebuild the
+// stress of rings 1 and 2 of their sector), T and cs from EnergyInt
+// (EnergySources.c:234-235), MidplaneVolumeDensity + OpacityProfile +
+// CalculateQminus, and TemperatureGradient + DiffusionCoefs +
+// DiffusionTimestep partials (EnergySources.c:556-653, 833-856) with the
+// neighbours' T rebuilt from EnergyInt and rho.  Every value is formed by the
+// same expressions as the separate reference steps.
+__device__ __forceinline__ double temper_of(const Geo &g, const double *__restrict__ e,
+                                            const double *__restrict__ rho, size_t l) {
+  const double tmp2 = 1.0 / (rho[l] * 1.0);
+  const double tmp1 = e[l] * 1.0 * (g.adiabind - 1.0);
+  return tmp1 * tmp2;
+}
+
+__device__ __forceinline__ double qplus_of(const Geo &g, int i, double rho, const Stress &st) {
+  const double visc = fvisc_med(g, i);
+  if (visc != 0.0) {
+    double q = 0.5 / visc / rho * (st.trr * st.trr + 2.0 * st.trp * st.trp + st.tpp * st.tpp);
+    q += (2.0 / 9.0) * visc * rho * st.dv * st.dv;
+    return q;
+  }
+  return 0.0;
+}
+
+__global__ void __launch_bounds__(TB) k_substep3(Geo g, const double *__restrict__ vr, const double *__restrict__ vt,
+                                                 const double *__restrict__ rho, const double *__restrict__ eint,
+                                                 double *__restrict__ divv, double *__restrict__ Trr,
+                                                 double *__restrict__ Trp, double *__restrict__ Tpp,
+                                                 double *__restrict__ qplus, double *__restrict__ T,
+                                                 double *__restrict__ cs, double *__restrict__ opa,
+                                                 double *__restrict__ qminus, double *__restrict__ D,
+                                                 double *__restrict__ pmin, int fill_opacity) {
   __shared__ double red[TB];
-  const int ns = g.ns, nr = g.nr, i = blockIdx.y, j = blockIdx.x * TB + threadIdx.x;
+  const int ns = g.ns, nr = g.nr;
+  int i, j, L;
+  if (!xcd_cell(g, 0, g.nr, i, j, L)) return;
   double dtmin = 1.0e38;
   if (j < ns) {
     const size_t l = (size_t)i * ns + j;
+    const double rl = rho[l];
+    // stress tensor and Q+
+    const Stress st = stress_cell(g, vr, vt, rho, i, j);
+    divv[l] = st.dv;
+    Trr[l] = st.trr;
+    Tpp[l] = st.tpp;
+    if (st.has_trp) Trp[l] = st.trp;
+    double qp;
+    if (i >= 1) {
+      qp = qplus_of(g, i, rl, st);
+    } else {
+      const double v0 = fvisc_med(g, 0);
+      const Stress s1 = stress_cell(g, vr, vt, rho, 1, j), s2 = stress_cell(g, vr, vt, rho, 2, j);
+      const double q1 = qplus_of(g, 1, rho[l + ns], s1);
+      const double q2 = qplus_of(g, 2, rho[l + 2 * ns], s2);
+      qp = (v0 != 0.0) ? q1 * exp(log(q1 / q2) * g.lr01 / g.lr12) : 0.0;
+    }
+    qplus[l] = qp;
+    // T, cs
+    const double gam = g.adiabind;
+    const double tl = temper_of(g, eint, rho, l);
+    const double cl = sqrt(gam * (gam - 1.0) * fabs(eint[l]) / rl);
+    T[l] = tl;
+    cs[l] = cl;
+    // opacity and Q-
+    double o;
+    if (g.opacity_law == THORIN_OPA_CONST) {
+      o = fill_opacity ? g.parametricopacity * g.opa2cu : opa[l];
+    } else {
+      const double H = cl * g.OmegaInv[i] * g.sqrt_adiabind_inv;
+      const double voldens = rl / H * g.sqrt2pi_inv;
+      o = opacity_law(g, voldens * g.rho2cgs, tl * g.t2si) * g.opa2cu;
+    }
+    opa[l] = o;
+    const double tau = g.opacitydrop * 0.5 * o * rl;
+    const double taueff = 0.375 * tau + 0.25 * g.sqrt3 + 0.25 / tau;
+    qminus[l] = 2.0 * g.stefan * cube_cr(tl) / taueff;
+    // |grad T| at ring ig (clamped to 1..nr-2) from T rebuilt at the four neighbours
     const int ig = (i == 0) ? 1 : ((i == nr - 1) ? nr - 2 : i);
-    const double gradt = gradtmag(g, T, ig, j);
-    const double H = cs[l] * g.OmegaInv[i] * g.sqrt_adiabind_inv;
-    const double voldens = rho[l] / H * g.sqrt2pi_inv;
-    const double fac = 1.0 / (voldens * opa[l]);
-    const double s = 4.0 * gradt * fac / T[l];
-    const double d = flux_limiter(s) * 16.0 * g.stefan * cube_cr(T[l]) * fac;
-    D[l] = d;
+    const size_t lg = (size_t)ig * ns + j;
+    const int jp = (j == ns - 1) ? 0 : j + 1;
+    const int jm = (j == 0) ? ns - 1 : j - 1;
+    const double tg = (ig == i) ? tl : temper_of(g, eint, rho, lg);
+    const double t_im = temper_of(g, eint, rho, lg - ns), t_ip = temper_of(g, eint, rho, lg + ns);
+    const double t_jm = temper_of(g, eint, rho, (size_t)ig * ns + jm);
+    const double t_jp = temper_of(g, eint, rho, (size_t)ig * ns + jp);
+    const double invdxtheta = 1.0 / (2.0 * DPI / (double)ns * g.Rmed[ig]);
+    const double gtr_l = (tg - t_im) * g.InvDiffRmed[ig];
+    const double gtr_lip = (t_ip - tg) * g.InvDiffRmed[ig + 1];
+    const double gtt_l = (tg - t_jm) * invdxtheta;
+    const double gtt_ljp = (t_jp - tg) * invdxtheta;
+    const double rr = 0.5 * (gtr_lip + gtr_l);
+    const double tt = 0.5 * (gtt_ljp + gtt_l);
+    const double gradt = sqrt(rr * rr + tt * tt);
+    const double H = cl * g.OmegaInv[i] * g.sqrt_adiabind_inv;
+    const double voldens = rl / H * g.sqrt2pi_inv;
+    const double fac = 1.0 / (voldens * o);
+    const double sfl = 4.0 * gradt * fac / tl;
+    const double dcoef = flux_limiter(sfl) * 16.0 * g.stefan * cube_cr(tl) * fac;
+    D[l] = dcoef;
     double deltar2 = g.Rsup[i] - g.Rinf[i];
     deltar2 = deltar2 * deltar2 / 2.0;
-    const double dttmp = deltar2 / d;
+    const double dttmp = deltar2 / dcoef;
     if (dttmp < dtmin) dtmin = dttmp;
   }
   dtmin = block_min<TB>(dtmin, red);
-  if (threadIdx.x == 0) pmin[blockIdx.y * gridDim.x + blockIdx.x] = dtmin;
-}
-
-// minimum of the per-block partial minima (order-independent)
-__global__ void __launch_bounds__(RB) k_min_to(const double *__restrict__ pmin, int nb, double *out) {
-  __shared__ double red[RB];
-  double m = 1.0e38;
-  m = strided_min<RB>(pmin, nb, m);
-  m = block_min<RB>(m, red);
-  if (threadIdx.x == 0) *out = m;
+  if (threadIdx.x == 0) pmin[L] = dtmin;
 }
 
 // implicit energy matrix (EnergySources.c:256-297) with A and B rebuilt on the fly
@@ -986,7 +1027,9 @@ __global__ void __launch_bounds__(TB) k_matrix(Geo g, const double *__restrict__
                                                double *__restrict__ Mljm, double *__restrict__ RHS, double dt,
                                                double *__restrict__ presid) {
   __shared__ double red[TB];
-  const int ns = g.ns, i = blockIdx.y + 1, j = blockIdx.x * TB + threadIdx.x;   // rings 1..nr-2
+  const int ns = g.ns;
+  int i, j, L;                                      // rings 1..nr-2
+  if (!xcd_cell(g, 1, g.nr - 2, i, j, L)) return;
   double r0 = 0.0;
   if (j < ns) {
     const size_t l = (size_t)i * ns + j, lip = l + ns, lim = l - ns;
@@ -1011,7 +1054,7 @@ __global__ void __launch_bounds__(TB) k_matrix(Geo g, const double *__restrict__
     r0 = fabs(tl * ml + T[lip] * mlip + T[lim] * mlim + T[ljp] * mljp + T[ljm] * mljm - rhs);
   }
   r0 = block_sum<TB>(r0, red);
-  if (threadIdx.x == 0) presid[blockIdx.y * gridDim.x + blockIdx.x] = r0;
+  if (threadIdx.x == 0) presid[L] = r0;
 }
 
 // ---------------------------------------------------------------------------
@@ -1032,7 +1075,9 @@ __global__ void __launch_bounds__(TB) k_sor_resid0(Geo g, const double *__restri
                                                    const double *__restrict__ Mljm, const double *__restrict__ RHS,
                                                    double *__restrict__ partial) {
   __shared__ double red[TB];
-  const int ns = g.ns, i = blockIdx.y + 1, j = blockIdx.x * TB + threadIdx.x;
+  const int ns = g.ns;
+  int i, j, L;
+  if (!xcd_cell(g, 1, g.nr - 2, i, j, L)) return;
   double r = 0.0;
   if (j < ns) {
     const size_t l = (size_t)i * ns + j;
@@ -1041,16 +1086,9 @@ __global__ void __launch_bounds__(TB) k_sor_resid0(Geo g, const double *__restri
     r = fabs(sor_resid(T, Ml, Mlip, Mlim, Mljp, Mljm, RHS, l, l + ns, l - ns, ljp, ljm));
   }
   r = block_sum<TB>(r, red);
-  if (threadIdx.x == 0) partial[blockIdx.y * gridDim.x + blockIdx.x] = r;
+  if (threadIdx.x == 0) partial[L] = r;
 }
 
-__global__ void __launch_bounds__(RB) k_sum_to(const double *__restrict__ partial, int nb, double *out) {
-  __shared__ double red[RB];
-  double s = 0.0;
-  s = strided_sum<RB>(partial, nb);
-  s = block_sum<RB>(s, red);
-  if (threadIdx.x == 0) *out = s;
-}
 
 // One SOR iteration (both half-sweeps) per launch.  A workgroup owns a tile
 // of SOR_TI rings x SOR_TJ sectors and stages T with a two-cell halo in LDS.
@@ -1113,9 +1151,11 @@ __global__ void __launch_bounds__(TB) k_sor_iter(Geo g, double *__restrict__ T0,
   __shared__ double t[(SOR_TI + 4) * SOR_LW];
   __shared__ double red[TB];
   const int ns = g.ns, nr = g.nr;
-  const int i0 = 1 + blockIdx.y * SOR_TI;
+  int tx, ty, bid;
+  if (!xcd_tile((ns + SOR_TJ - 1) / SOR_TJ, (nr - 2 + SOR_TI - 1) / SOR_TI, tx, ty, bid)) return;
+  const int i0 = 1 + ty * SOR_TI;
   const int i1 = min(i0 + SOR_TI, nr - 1);          // owned rings [i0, i1) within [1, nr-2]
-  const int j0 = blockIdx.x * SOR_TJ;
+  const int j0 = tx * SOR_TJ;
   const int tw = min(SOR_TJ, ns - j0);               // owned sectors [j0, j0+tw)
   const int ti = i1 - i0;
   const int tid = threadIdx.x;
@@ -1123,13 +1163,13 @@ __global__ void __launch_bounds__(TB) k_sor_iter(Geo g, double *__restrict__ T0,
   if (it == 0) {
     // first iteration of a solve: reset the control words, and give T1 the
     // rings no sweep writes (0, NRAD-1 and the ghost ring)
-    if (blockIdx.x == 0 && blockIdx.y == 0 && tid == 0) {
+    if (bid == 0 && tid == 0) {
       ctl->sor_done = 0;
       ctl->sor_niter = 0;
       ctl->sor_iter = 0;
       ctl->sor_cur = 0;
     }
-    if (blockIdx.y == 0)
+    if (ty == 0)
       for (int j = j0 + tid; j < j0 + tw; j += TB) T1[j] = T0[j];
     if (i1 == nr - 1)
       for (int j = j0 + tid; j < j0 + tw; j += TB) {
@@ -1200,7 +1240,6 @@ __global__ void __launch_bounds__(TB) k_sor_iter(Geo g, double *__restrict__ T0,
     const int r = 2 + idx / tw, c = 2 + idx % tw;
     Tout[(size_t)(i0 - 2 + r) * ns + (j0 - 2 + c)] = t[r * SOR_LW + c];
   }
-  const int bid = blockIdx.y * gridDim.x + blockIdx.x;
   acc = block_sum<TB>(acc, red);
   const double cc = block_sum<TB>((double)changed, red);
   if (tid == 0) {
@@ -1259,8 +1298,9 @@ __global__ void __launch_bounds__(RB) k_sor_check(const double *__restrict__ psu
 // edges of T and EnergyNew = rho*T*CV (EnergySources.c:319-334)
 __global__ void __launch_bounds__(TB) k_energy_new(Geo g, const double *__restrict__ rho, double *__restrict__ T,
                                                    double *__restrict__ enew) {
-  const int ns = g.ns, nr = g.nr, i = blockIdx.y, j = blockIdx.x * TB + threadIdx.x;
-  if (j >= ns) return;
+  const int ns = g.ns, nr = g.nr;
+  int i, j, L_;
+  if (!xcd_cell(g, 0, g.nr, i, j, L_) || j >= ns) return;
   const size_t l = (size_t)i * ns + j;
   double t = T[l];
   if (i == 0) { t = T[l + ns]; T[l] = t; }
@@ -1393,9 +1433,12 @@ __device__ __forceinline__ void rad_column(const Geo &g, const RadQ &q, const Ra
 
 __global__ void __launch_bounds__(TB) k_transport_rad(Geo g, RadQ q, double dt) {
   __shared__ RadGeo sg;
+  // XCD x (= b % 8) gets the contiguous band of tiles x*per .. x*per+per-1,
+  // all six quantities of each
   const int b = blockIdx.x, slot = b & 7, rest = b >> 3;
   const int which = rest % RC_NQ;
-  const int tile = (rest / RC_NQ) * 8 + slot;
+  const int per = (q.ntiles + 7) >> 3;
+  const int tile = slot * per + rest / RC_NQ;
   if (tile >= q.ntiles) return;                // whole workgroup
   const int tj = tile % q.ntile_j, ti = tile / q.ntile_j;
   const int i0 = ti * RC_I, i1 = min(i0 + RC_I, g.nr);
@@ -1617,8 +1660,9 @@ __global__ void __launch_bounds__(TB) k_velocities(Geo g, const double *__restri
                                                    const double *__restrict__ rp, const double *__restrict__ rm,
                                                    const double *__restrict__ tp, const double *__restrict__ tm,
                                                    double *__restrict__ vr, double *__restrict__ vt) {
-  const int ns = g.ns, i = blockIdx.y, j = blockIdx.x * TB + threadIdx.x;
-  if (j >= ns) return;
+  const int ns = g.ns;
+  int i, j, L_;
+  if (!xcd_cell(g, 0, g.nr, i, j, L_) || j >= ns) return;
   const size_t l = (size_t)i * ns + j;
   const size_t ljm = (j == 0) ? (size_t)i * ns + ns - 1 : l - 1;
   if (i == 0) vr[l] = 0.0;
@@ -1698,8 +1742,11 @@ static int need_dev(thorin_ctx *c) {
   return 0;
 }
 
+// 1D grid of the XCD-banded cell blocks over `rings` rings (xcd_cell), and
+// the number of logical blocks (= block partials a reduction kernel writes)
+static inline int cell_count(const thorin_dev *d, int rings) { return d->g.ncb * rings; }
 static inline dim3 cell_grid(const thorin_dev *d, int rings) {
-  return dim3((unsigned)((d->ns + TB - 1) / TB), (unsigned)rings);
+  return dim3((unsigned)((cell_count(d, rings) + 7) / 8 * 8));
 }
 
 extern "C" int thorin_dev_create(thorin_ctx *c) {
@@ -1765,6 +1812,7 @@ extern "C" int thorin_dev_create(thorin_ctx *c) {
   memset(&g, 0, sizeof(g));
   g.nr = c->nr;
   g.ns = c->ns;
+  g.ncb = (c->ns + TB - 1) / TB;
   g.Rinf = p1; g.Rsup = p1 + n1; g.Rmed = p1 + 2 * n1; g.Surf = p1 + 3 * n1; g.InvRmed = p1 + 4 * n1;
   g.InvSurf = p1 + 5 * n1; g.InvDiffRmed = p1 + 6 * n1; g.InvDiffRsup = p1 + 7 * n1; g.InvRinf = p1 + 8 * n1;
   g.OmegaInv = p1 + 9 * n1; g.SigmaMed = p1 + 10 * n1; g.SigmaInf = p1 + 11 * n1; g.EnergyMed = p1 + 12 * n1;
@@ -2078,11 +2126,11 @@ static int sor_solve(thorin_ctx *c, double omega, int *niter_out) {
     dim3 gc = cell_grid(d, nr - 2);
     LAUNCH(k_sor_resid0, gc, dim3(TB), 0, g, d->b[B_TEMPER], d->b[B_ML], d->b[B_MLIP], d->b[B_MLIM], d->b[B_MLJP],
            d->b[B_MLJM], d->b[B_RHS], d->mresp);
-    d->nb_mresp = (int)(gc.x * gc.y);
+    d->nb_mresp = cell_count(d, nr - 2);
     d->normres0_valid = 1;
   }
-  dim3 gs((unsigned)((ns + SOR_TJ - 1) / SOR_TJ), (unsigned)((nr - 2 + SOR_TI - 1) / SOR_TI));
-  const int nbs = gs.x * gs.y;
+  const int nbs = ((ns + SOR_TJ - 1) / SOR_TJ) * ((nr - 2 + SOR_TI - 1) / SOR_TI);
+  const dim3 gs((unsigned)((nbs + 7) / 8 * 8));
   int launched = 0;
   int batch = c->st.sor_niter_last > 0 ? c->st.sor_niter_last + 1 : 8;
   if (batch > 256) batch = 256;
@@ -2363,22 +2411,16 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
     // ---- SubStep3: Q+, implicit radiative diffusion
     int niter = 0;
     double omega_used = 0.0;
-    LAUNCH(k_divstress, gall, tb, 0, d->g, d->b[B_VRAD], d->b[B_VTHETA], d->b[B_DENS], d->b[B_DIVV],
-           d->b[B_TAURR], d->b[B_TAURP], d->b[B_TAUPP]);
-    LAUNCH(k_qplus, gall, tb, 0, d->g, d->b[B_DENS], d->b[B_TAURR], d->b[B_TAURP], d->b[B_TAUPP], d->b[B_DIVV],
-           d->b[B_QPLUS]);
-    LAUNCH(k_temper_cs, gall, tb, 0, d->g, d->b[B_DENS], d->b[B_EINT], d->b[B_TEMPER], d->b[B_CS]);
-    LAUNCH(k_opacity_qminus, gall, tb, 0, d->g, d->b[B_DENS], d->b[B_CS], d->b[B_TEMPER], d->b[B_OPACITY],
-           d->b[B_QMINUS], d->opa_filled ? 0 : 1);
+    LAUNCH(k_substep3, gall, tb, 0, d->g, d->b[B_VRAD], d->b[B_VTHETA], d->b[B_DENS], d->b[B_EINT], d->b[B_DIVV],
+           d->b[B_TAURR], d->b[B_TAURP], d->b[B_TAUPP], d->b[B_QPLUS], d->b[B_TEMPER], d->b[B_CS],
+           d->b[B_OPACITY], d->b[B_QMINUS], d->b[B_DIFF], d->pstellar, d->opa_filled ? 0 : 1);
     d->opa_filled = 1;
-    LAUNCH(k_diffcoef, gall, tb, 0, d->g, d->b[B_DENS], d->b[B_CS], d->b[B_TEMPER], d->b[B_OPACITY], d->b[B_DIFF],
-           d->pstellar);
     {
       const dim3 gm = cell_grid(d, nr - 2);
       LAUNCH(k_matrix, gm, tb, 0, d->g, d->b[B_DENS], d->b[B_CS], d->b[B_DIFF], d->b[B_DIVV], d->b[B_QMINUS],
              d->b[B_QPLUS], d->b[B_TEMPER], d->b[B_ML], d->b[B_MLIP], d->b[B_MLIM], d->b[B_MLJP], d->b[B_MLJM],
              d->b[B_RHS], dt, d->mresp);
-      d->nb_mresp = (int)(gm.x * gm.y);
+      d->nb_mresp = cell_count(d, nr - 2);
       d->normres0_valid = 1;
     }
     HCHECK(hipGetLastError());
