@@ -119,6 +119,7 @@ struct thorin_dev {
   int opa_filled;
   int normres0_valid;           // ctl->sor_normres0 holds the residual of the current matrix
   int cfl_ready;                // hctl->newdt_raw holds the CFL minimum of the current fields
+  int niter_hist[4], nsolve;    // SOR iteration counts of the last four solves
   double cfl_invdtreb;          // MinStepForRebound term it was formed with
   // kernel timing with HIP events (thorin_prof_*)
   int prof_on;
@@ -293,6 +294,18 @@ __device__ __forceinline__ bool xcd_cell(const Geo &g, int i0, int nring, int &i
   return true;
 }
 
+// plain round-robin numbering (logical block = launch block) for kernels
+// whose cost varies strongly from ring to ring (the force kernel near a
+// planet), so that the expensive rings spread over all XCDs
+__device__ __forceinline__ bool rr_cell(const Geo &g, int i0, int nring, int &i, int &j, int &L) {
+  L = blockIdx.x;
+  if (L >= g.ncb * nring) return false;
+  const int q = L / g.ncb;
+  i = i0 + q;
+  j = (L - q * g.ncb) * TB + threadIdx.x;
+  return true;
+}
+
 // the same banding for a grid of ntx x nty tiles (tile x fastest)
 __device__ __forceinline__ bool xcd_tile(int ntx, int nty, int &tx, int &ty, int &L) {
   const int total = ntx * nty, per = (total + 7) >> 3;
@@ -376,8 +389,9 @@ __device__ __forceinline__ double ordered_ring_sum(const double *sh, int ns) {
 __global__ void __launch_bounds__(RB) k_eos_cfl(Geo g, const double *__restrict__ rho, const double *__restrict__ e,
                                                 double *__restrict__ T, double *__restrict__ cs,
                                                 const double *__restrict__ vr, const double *__restrict__ vt,
-                                                double invdtreb_sq, int do_eos) {
+                                                double invdtreb_sq, int do_eos, const int *__restrict__ guard) {
   extern __shared__ double sh[];       // 3*ns + RB: vt, cs, T, reduction
+  if (guard && *guard == 0) return;
   const int i = blockIdx.x, ns = g.ns, nr = g.nr, tid = threadIdx.x;
   double *s_vt = sh, *s_cs = sh + ns, *s_t = sh + 2 * ns, *red = sh + 3 * ns;
   const double gam = g.adiabind;
@@ -471,8 +485,9 @@ __global__ void __launch_bounds__(RB) k_eos_cfl(Geo g, const double *__restrict_
 // The minimum of the DiffusionTimestep partials of this substep's k_substep3
 // (nstellar of them; 0: none formed) goes to ctl->dt_stellar.
 __global__ void __launch_bounds__(RB) k_cfl_final(Geo g, DevCtl *ctl, const double *__restrict__ pstellar,
-                                                  int nstellar) {
+                                                  int nstellar, const int *__restrict__ guard) {
   __shared__ double red[RB];
+  if (guard && *guard == 0) return;
   if (nstellar > 0) {
     double ms = 1.0e38;
     ms = strided_min<RB>(pstellar, nstellar, ms);
@@ -503,7 +518,7 @@ __global__ void __launch_bounds__(TB) k_forces(Geo g, const double *__restrict__
   __shared__ double red[TB];
   const int ns = g.ns;
   int i, j, bid;
-  if (!xcd_cell(g, 0, g.nr, i, j, bid)) return;
+  if (!rr_cell(g, 0, g.nr, i, j, bid)) return;
   const bool act = j < ns;
   double axs = 0.0, ays = 0.0;
   double axpl_loc[8], aypl_loc[8];
@@ -687,9 +702,11 @@ __global__ void __launch_bounds__(TB) k_indirect(Geo g, double *__restrict__ agr
 // ---------------------------------------------------------------------------
 // DampingBoundary (SideEuler.c:368-419) on the rings flagged by the host
 __global__ void __launch_bounds__(TB) k_boundary(Geo g, double *__restrict__ vrad, double *__restrict__ vtheta,
-                                                 double *__restrict__ rho, double *__restrict__ energy, double step) {
+                                                 double *__restrict__ rho, double *__restrict__ energy, double step,
+                                                 const int *__restrict__ guard) {
   const int ns = g.ns;
   int i, j, L_;
+  if (guard && *guard == 0) return;
   if (!xcd_cell(g, 0, g.nr, i, j, L_) || j >= ns || !g.dampring[i]) return;
   double vrad0 = 0.0, rho0 = 0.0, energy0 = 0.0, vtheta0 = 0.0;
   if (g.damp_init) {
@@ -1147,12 +1164,14 @@ __global__ void __launch_bounds__(TB) k_sor_iter(Geo g, double *__restrict__ T0,
                                                  const double *__restrict__ Mlim, const double *__restrict__ Mljp,
                                                  const double *__restrict__ Mljm, const double *__restrict__ RHS,
                                                  double omega, double *__restrict__ psum, int *__restrict__ pcnt,
-                                                 DevCtl *__restrict__ ctl, int it) {
+                                                 DevCtl *__restrict__ ctl, int it,
+                                                 const double *__restrict__ mresp, int nbm) {
   __shared__ double t[(SOR_TI + 4) * SOR_LW];
   __shared__ double red[TB];
   const int ns = g.ns, nr = g.nr;
+  const int ntx = (ns + SOR_TJ - 1) / SOR_TJ;
   int tx, ty, bid;
-  if (!xcd_tile((ns + SOR_TJ - 1) / SOR_TJ, (nr - 2 + SOR_TI - 1) / SOR_TI, tx, ty, bid)) return;
+  if (!xcd_tile(ntx, (nr - 2 + SOR_TI - 1) / SOR_TI, tx, ty, bid)) return;
   const int i0 = 1 + ty * SOR_TI;
   const int i1 = min(i0 + SOR_TI, nr - 1);          // owned rings [i0, i1) within [1, nr-2]
   const int j0 = tx * SOR_TJ;
@@ -1163,11 +1182,18 @@ __global__ void __launch_bounds__(TB) k_sor_iter(Geo g, double *__restrict__ T0,
   if (it == 0) {
     // first iteration of a solve: reset the control words, and give T1 the
     // rings no sweep writes (0, NRAD-1 and the ghost ring)
-    if (bid == 0 && tid == 0) {
-      ctl->sor_done = 0;
-      ctl->sor_niter = 0;
-      ctl->sor_iter = 0;
-      ctl->sor_cur = 0;
+    if (bid == 0) {
+      // initial residual normres0 (EnergySources.c:435-451) from the matrix
+      // kernel's block partials
+      double r = strided_sum<TB>(mresp, nbm);
+      r = block_sum<TB>(r, red);
+      if (tid == 0) {
+        ctl->sor_normres0 = r;
+        ctl->sor_done = 0;
+        ctl->sor_niter = 0;
+        ctl->sor_iter = 0;
+        ctl->sor_cur = 0;
+      }
     }
     if (ty == 0)
       for (int j = j0 + tid; j < j0 + tw; j += TB) T1[j] = T0[j];
@@ -1178,7 +1204,7 @@ __global__ void __launch_bounds__(TB) k_sor_iter(Geo g, double *__restrict__ T0,
       }
   } else {
     if (ctl->sor_done) return;
-    cur = ctl->sor_cur;
+    cur = it & 1;
   }
   const double *__restrict__ Tin = cur ? T1 : T0;
   double *__restrict__ Tout = cur ? T0 : T1;
@@ -1241,28 +1267,18 @@ __global__ void __launch_bounds__(TB) k_sor_iter(Geo g, double *__restrict__ T0,
     Tout[(size_t)(i0 - 2 + r) * ns + (j0 - 2 + c)] = t[r * SOR_LW + c];
   }
   acc = block_sum<TB>(acc, red);
-  const double cc = block_sum<TB>((double)changed, red);
+  const double nch = block_sum<TB>((double)changed, red);
   if (tid == 0) {
     psum[bid] = acc;
-    pcnt[bid] = (int)cc;
+    pcnt[bid] = (int)nch;
   }
 }
 
-// stop test of the iteration just done (EnergySources.c:487-493), partial
-// sums added in block order.  After the first iteration of a solve it also
-// forms the initial residual normres0 (EnergySources.c:435-451) from the
-// matrix kernel's block partials.
+// stop test of iteration it (EnergySources.c:487-493), partial sums added in
+// block order; the last iterate is in T1 after an odd number of iterations
 __global__ void __launch_bounds__(RB) k_sor_check(const double *__restrict__ psum, const int *__restrict__ pcnt,
-                                                  int nb, DevCtl *ctl, int maxiter, const double *__restrict__ mresp,
-                                                  int nbm, int it) {
+                                                  int nb, DevCtl *ctl, int maxiter, int it) {
   __shared__ double red[RB];
-  if (it == 0) {
-    double r = 0.0;
-    r = strided_sum<RB>(mresp, nbm);
-    r = block_sum<RB>(r, red);
-    if (threadIdx.x == 0) ctl->sor_normres0 = r;
-    __syncthreads();
-  }
   if (ctl->sor_done) return;
   double s = strided_sum<RB>(psum, nb);
   int ci = 0;
@@ -1277,13 +1293,12 @@ __global__ void __launch_bounds__(RB) k_sor_check(const double *__restrict__ psu
     }
     for (; b < nb; b += RB) ci += pcnt[b];
   }
-  double c = (double)ci;
   s = block_sum<RB>(s, red);
-  c = block_sum<RB>(c, red);
+  const double c = block_sum<RB>((double)ci, red);
   if (threadIdx.x == 0) {
-    const int n = ctl->sor_iter + 1;
+    const int n = it + 1;
     ctl->sor_iter = n;
-    ctl->sor_cur ^= 1;
+    ctl->sor_cur = n & 1;
     ctl->sor_normres = s;
     if (s < SOREPS * ctl->sor_normres0 || s < SORMIN || c == 0.0) {
       ctl->sor_done = 1;
@@ -1295,16 +1310,24 @@ __global__ void __launch_bounds__(RB) k_sor_check(const double *__restrict__ psu
   }
 }
 
-// edges of T and EnergyNew = rho*T*CV (EnergySources.c:319-334)
-__global__ void __launch_bounds__(TB) k_energy_new(Geo g, const double *__restrict__ rho, double *__restrict__ T,
-                                                   double *__restrict__ enew) {
+// edges of T and EnergyNew = rho*T*CV (EnergySources.c:319-334).  The SOR's
+// last iterate is in T0 or T1 (ctl->sor_cur); T0 receives it, so that T0
+// holds the solved temperature afterwards.  Nothing is done until the SOR
+// has converged (ctl->sor_done).
+__global__ void __launch_bounds__(TB) k_energy_new(Geo g, const double *__restrict__ rho, double *__restrict__ T0,
+                                                   const double *__restrict__ T1, double *__restrict__ enew,
+                                                   const DevCtl *__restrict__ ctl) {
   const int ns = g.ns, nr = g.nr;
   int i, j, L_;
+  if (ctl->sor_done == 0) return;
   if (!xcd_cell(g, 0, g.nr, i, j, L_) || j >= ns) return;
+  const int cur = ctl->sor_cur;
+  const double *__restrict__ Ts = cur ? T1 : T0;
   const size_t l = (size_t)i * ns + j;
-  double t = T[l];
-  if (i == 0) { t = T[l + ns]; T[l] = t; }
-  if (i == nr - 1) { t = T[l - ns]; T[l] = t; }
+  double t = Ts[l];
+  if (i == 0) t = Ts[l + ns];
+  if (i == nr - 1) t = Ts[l - ns];
+  if (cur || i == 0 || i == nr - 1) T0[l] = t;
   enew[l] = rho[l] * t * g.cv;
   if (i == 0) enew[(size_t)nr * ns + j] = 0.0;
 }
@@ -1332,6 +1355,7 @@ __global__ void __launch_bounds__(TB) k_energy_new(Geo g, const double *__restri
 struct RadQ {
   const double *rho, *vr, *vt, *e;
   double *out[RC_NQ];          // RadMomP, RadMomM, ThetaMomP, ThetaMomM, Energy, Rho
+  const int *guard;             // skip the launch while *guard == 0
   double of;
   int ntile_j, ntiles;
 };
@@ -1433,6 +1457,7 @@ __device__ __forceinline__ void rad_column(const Geo &g, const RadQ &q, const Ra
 
 __global__ void __launch_bounds__(TB) k_transport_rad(Geo g, RadQ q, double dt) {
   __shared__ RadGeo sg;
+  if (q.guard && *q.guard == 0) return;
   // XCD x (= b % 8) gets the contiguous band of tiles x*per .. x*per+per-1,
   // all six quantities of each
   const int b = blockIdx.x, slot = b & 7, rest = b >> 3;
@@ -1542,8 +1567,9 @@ __global__ void __launch_bounds__(TT_T) k_transport_theta(Geo g, const double *_
                                                           double *__restrict__ orp, double *__restrict__ orm,
                                                           double *__restrict__ otp, double *__restrict__ otm,
                                                           double *__restrict__ oe, double *__restrict__ orho,
-                                                          double dt) {
+                                                          double dt, const int *__restrict__ guard) {
   extern __shared__ double sh[];       // 2*ns
+  if (guard && *guard == 0) return;
   const int i = blockIdx.x, ns = g.ns, tid = threadIdx.x;
   double *s_w = sh, *s_f = sh + ns;
   const size_t base = (size_t)i * ns;
@@ -1659,9 +1685,11 @@ __global__ void __launch_bounds__(TT_T) k_transport_theta(Geo g, const double *_
 __global__ void __launch_bounds__(TB) k_velocities(Geo g, const double *__restrict__ rho,
                                                    const double *__restrict__ rp, const double *__restrict__ rm,
                                                    const double *__restrict__ tp, const double *__restrict__ tm,
-                                                   double *__restrict__ vr, double *__restrict__ vt) {
+                                                   double *__restrict__ vr, double *__restrict__ vt,
+                                                   const int *__restrict__ guard) {
   const int ns = g.ns;
   int i, j, L_;
+  if (guard && *guard == 0) return;
   if (!xcd_cell(g, 0, g.nr, i, j, L_) || j >= ns) return;
   const size_t l = (size_t)i * ns + j;
   const size_t ljm = (j == 0) ? (size_t)i * ns + ns - 1 : l - 1;
@@ -2116,13 +2144,15 @@ extern "C" const char *thorin_prof_kernel_name(int kid) {
 // --------------------------------------------------------------------------
 // SOR driver
 
-static int sor_solve(thorin_ctx *c, double omega, int *niter_out) {
+// enqueue iterations it0 .. it0+count-1 (iteration + stop test each); the
+// first one also forms the initial residual when the matrix kernel has not
+static int sor_launch(thorin_ctx *c, double omega, int it0, int count) {
   thorin_dev *d = c->dev;
   const Geo &g = d->g;
   const int nr = d->nr, ns = d->ns;
   // initial residual (EnergySources.c:435-451); the matrix is unchanged and
   // T is the same initial guess for every solve of a substep
-  if (!d->normres0_valid) {
+  if (it0 == 0 && !d->normres0_valid) {
     dim3 gc = cell_grid(d, nr - 2);
     LAUNCH(k_sor_resid0, gc, dim3(TB), 0, g, d->b[B_TEMPER], d->b[B_ML], d->b[B_MLIP], d->b[B_MLIM], d->b[B_MLJP],
            d->b[B_MLJM], d->b[B_RHS], d->mresp);
@@ -2131,31 +2161,43 @@ static int sor_solve(thorin_ctx *c, double omega, int *niter_out) {
   }
   const int nbs = ((ns + SOR_TJ - 1) / SOR_TJ) * ((nr - 2 + SOR_TI - 1) / SOR_TI);
   const dim3 gs((unsigned)((nbs + 7) / 8 * 8));
-  int launched = 0;
+  for (int it = it0; it < it0 + count; it++) {
+    LAUNCH(k_sor_iter, gs, dim3(TB), 0, g, d->b[B_TEMPER], d->b[B_TEMPER2], d->b[B_ML], d->b[B_MLIP],
+           d->b[B_MLIM], d->b[B_MLJP], d->b[B_MLJM], d->b[B_RHS], omega, d->sorp, d->sorc, d->ctl, it,
+           (const double *)d->mresp, d->nb_mresp);
+    LAUNCH(k_sor_check, dim3(1), dim3(RB), 0, d->sorp, d->sorc, nbs, d->ctl, SORMAXITERS, it);
+  }
+  HCHECK(hipGetLastError());
+  return 0;
+}
+
+// Run the solve to its end with host checks between batches, the first
+// `launched` iterations being already enqueued.  The last iterate stays in
+// B_TEMPER or B_TEMPER2 as ctl->sor_cur says (k_energy_new picks it up).
+static int sor_finish(thorin_ctx *c, double omega, int launched, int *niter_out) {
+  thorin_dev *d = c->dev;
   int batch = c->st.sor_niter_last > 0 ? c->st.sor_niter_last + 1 : 8;
   if (batch > 256) batch = 256;
+  if (launched == 0) {
+    if (sor_launch(c, omega, 0, batch)) return -1;
+    launched = batch;
+  }
   for (;;) {
-    for (int b = 0; b < batch; b++) {
-      const int it = launched + b;
-      LAUNCH(k_sor_iter, gs, dim3(TB), 0, g, d->b[B_TEMPER], d->b[B_TEMPER2], d->b[B_ML], d->b[B_MLIP],
-             d->b[B_MLIM], d->b[B_MLJP], d->b[B_MLJM], d->b[B_RHS], omega, d->sorp, d->sorc, d->ctl, it);
-      LAUNCH(k_sor_check, dim3(1), dim3(RB), 0, d->sorp, d->sorc, nbs, d->ctl, SORMAXITERS, d->mresp, d->nb_mresp,
-             it);
-    }
-    HCHECK(hipGetLastError());
-    launched += batch;
     HCHECK(hipMemcpyAsync(d->hctl, d->ctl, sizeof(DevCtl), hipMemcpyDeviceToHost, d->s));
     HCHECK(hipStreamSynchronize(d->s));
     if (prof_flush(d)) return -1;
     if (d->hctl->sor_done) {
       *niter_out = d->hctl->sor_niter;
-      if (d->hctl->sor_cur) swapb(d, B_TEMPER, B_TEMPER2);
       return 0;
     }
     if (launched > SORMAXITERS + 2) { *niter_out = SORMAXITERS + 1; return 0; }
     batch = batch < 64 ? 2 * batch : 128;
+    if (sor_launch(c, omega, launched, batch)) return -1;
+    launched += batch;
   }
 }
+
+static int sor_solve(thorin_ctx *c, double omega, int *niter_out) { return sor_finish(c, omega, 0, niter_out); }
 
 // IterateRelaxationParameter (EnergySources.c:340-403)
 static int iterate_relaxation(thorin_ctx *c) {
@@ -2242,8 +2284,8 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
     const double invdtreb_sq = minstep_rebound(sys);
     if (!d->cfl_ready || invdtreb_sq != d->cfl_invdtreb) {
       LAUNCH(k_eos_cfl, dim3(nr), rb, sizeof(double) * (3 * ns + RB), d->g, d->b[B_DENS], d->b[B_ENERGY],
-             d->b[B_TEMPER], d->b[B_CS], d->b[B_VRAD], d->b[B_VTHETA], invdtreb_sq, 0);
-      LAUNCH(k_cfl_final, dim3(1), rb, 0, d->g, d->ctl, (const double *)nullptr, 0);
+             d->b[B_TEMPER], d->b[B_CS], d->b[B_VRAD], d->b[B_VTHETA], invdtreb_sq, 0, (const int *)nullptr);
+      LAUNCH(k_cfl_final, dim3(1), rb, 0, d->g, d->ctl, (const double *)nullptr, 0, (const int *)nullptr);
       HCHECK(hipGetLastError());
       HCHECK(hipMemcpyAsync(d->hctl, d->ctl, sizeof(DevCtl), hipMemcpyDeviceToHost, d->s));
       HCHECK(hipStreamSynchronize(d->s));
@@ -2380,7 +2422,8 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
     }
     // ---- ApplyBoundaryCondition, DetectCrash, ComputePressureField, SubStep1
     if (p->damping_on)
-      LAUNCH(k_boundary, gall, tb, 0, d->g, d->b[B_VRAD], d->b[B_VTHETA], d->b[B_DENS], d->b[B_ENERGY], dt);
+      LAUNCH(k_boundary, gall, tb, 0, d->g, d->b[B_VRAD], d->b[B_VTHETA], d->b[B_DENS], d->b[B_ENERGY], dt,
+             (const int *)nullptr);
     LAUNCH(k_substep1, gall, tb, 0, d->g, d->b[B_DENS], d->b[B_ENERGY], d->b[B_VRAD], d->b[B_VTHETA], d->b[B_AGR],
            d->b[B_AGT], d->b[B_VRADINT], d->b[B_VTHETAINT], dt, d->ctl);
     LAUNCH(k_divstress, gall, tb, 0, d->g, d->b[B_VRADINT], d->b[B_VTHETAINT], d->b[B_DENS], d->b[B_DIVV],
@@ -2406,7 +2449,8 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
     swapb(d, B_VRAD, B_VRADNEW);
     swapb(d, B_VTHETA, B_VTHETANEW);
     if (p->damping_on)
-      LAUNCH(k_boundary, gall, tb, 0, d->g, d->b[B_VRAD], d->b[B_VTHETA], d->b[B_DENS], d->b[B_ENERGY], dt);
+      LAUNCH(k_boundary, gall, tb, 0, d->g, d->b[B_VRAD], d->b[B_VTHETA], d->b[B_DENS], d->b[B_ENERGY], dt,
+             (const int *)nullptr);
     HCHECK(hipGetLastError());
     // ---- SubStep3: Q+, implicit radiative diffusion
     int niter = 0;
@@ -2434,46 +2478,20 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
     if (c->st.sor_omega > OMEGAUPPER) { c->st.sor_omega = OMEGAUPPER; c->st.sor_domega = -c->st.sor_domega; }
     if (c->st.sor_omega < 1.0) { c->st.sor_omega = 1.0; c->st.sor_domega = -c->st.sor_domega; }
     omega_used = c->st.sor_omega;
-    if (sor_solve(c, c->st.sor_omega, &niter)) return -1;
-    if (niter > SORMAXITERS) rc = 2;
-    if (niter > c->st.sor_niter_last) c->st.sor_domega = -c->st.sor_domega;
-    c->st.sor_niter_last = niter;
-    LAUNCH(k_energy_new, gall, tb, 0, d->g, d->b[B_DENS], d->b[B_TEMPER], d->b[B_ENEW]);
-    swapb(d, B_ENERGY, B_ENEW);
-    // ---- Transport
-    {
-      RadQ q;
-      q.rho = d->b[B_DENS];
-      q.vr = d->b[B_VRAD];
-      q.vt = d->b[B_VTHETA];
-      q.e = d->b[B_ENERGY];
-      const int outs[RC_NQ] = {B_XRP, B_XRM, B_XTP, B_XTM, B_XE, B_XRHO};
-      for (int k = 0; k < RC_NQ; k++) q.out[k] = d->b[outs[k]];
-      q.of = c->st.omega_frame;
-      q.ntile_j = (ns + TB - 1) / TB;
-      q.ntiles = q.ntile_j * ((nr + RC_I - 1) / RC_I);
-      LAUNCH(k_transport_rad, dim3((unsigned)((q.ntiles + 7) / 8 * 8 * RC_NQ)), tb, 0, d->g, q, dt);
-#define TT_LAUNCH(NPT)                                                                                         \
-  LAUNCH(k_transport_theta<NPT>, dim3(nr), dim3(TT_T), sizeof(double) * 2 * ns, d->g, d->b[B_VTHETA], d->b[B_XRP],  \
-         d->b[B_XRM], d->b[B_XTP], d->b[B_XTM], d->b[B_XE], d->b[B_XRHO], d->b[B_RP], d->b[B_RM], d->b[B_TP],     \
-         d->b[B_TM], d->b[B_ENERGY], d->b[B_DENS], dt)
-      switch ((ns + TT_T - 1) / TT_T) {
-        case 1: TT_LAUNCH(1); break;
-        case 2: TT_LAUNCH(2); break;
-        case 3: TT_LAUNCH(3); break;
-        case 4: TT_LAUNCH(4); break;
-        case 5: TT_LAUNCH(5); break;
-        default: TT_LAUNCH(6); break;
-      }
-#undef TT_LAUNCH
-      LAUNCH(k_velocities, gall, tb, 0, d->g, d->b[B_DENS], d->b[B_RP], d->b[B_RM], d->b[B_TP], d->b[B_TM],
-             d->b[B_VRAD], d->b[B_VTHETA]);
-    }
-    if (p->damping_on)
-      LAUNCH(k_boundary, gall, tb, 0, d->g, d->b[B_VRAD], d->b[B_VTHETA], d->b[B_DENS], d->b[B_ENERGY], dt);
+    // The iterations are enqueued without a host round trip: the previous
+    // solve's count + 2 of them, each ending in the device-side stop test
+    // (iterations after convergence do nothing).  Everything after the solve
+    // is enqueued behind them and does nothing unless the solve converged
+    // (guard = ctl->sor_done); in the rare case it needed more iterations the
+    // solve is finished with host checks and the rest of the substep is
+    // enqueued again from the same buffers.
+    int sor_k = c->st.sor_niter_last;
+    for (int k = 0; k < 4; k++) sor_k = d->niter_hist[k] > sor_k ? d->niter_hist[k] : sor_k;
+    sor_k += 1;
+    if (sor_k < 2) sor_k = 2;
+    if (sor_k > SORMAXITERS + 2) sor_k = SORMAXITERS + 2;
+    if (sor_launch(c, c->st.sor_omega, 0, sor_k)) return -1;
     c->st.physical_time += dt;
-    // ---- end of the substep: T, cs and their ring means, and the CFL
-    // minima of the next substep (its planets are those of now)
     if (p->lambdadoubling != 0.0) {
       thorin_cavity_factors(c);
       const int n1 = nr + 3;
@@ -2481,10 +2499,68 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
       HCHECK(hipMemcpyAsync((double *)d->g.cav_inf, c->cav_inf, sizeof(double) * n1, hipMemcpyHostToDevice, d->s));
     }
     d->cfl_invdtreb = minstep_rebound(sys);
-    LAUNCH(k_eos_cfl, dim3(nr), rb, sizeof(double) * (3 * ns + RB), d->g, d->b[B_DENS], d->b[B_ENERGY],
-           d->b[B_TEMPER], d->b[B_CS], d->b[B_VRAD], d->b[B_VTHETA], d->cfl_invdtreb, 1);
-    LAUNCH(k_cfl_final, dim3(1), rb, 0, d->g, d->ctl, (const double *)d->pstellar, d->nblk_cells);
-    HCHECK(hipGetLastError());
+    const int *guard = &d->ctl->sor_done;
+    double *bsave[NBUF];
+    memcpy(bsave, d->b, sizeof bsave);
+    auto after_sor = [&]() -> int {
+      LAUNCH(k_energy_new, gall, tb, 0, d->g, d->b[B_DENS], d->b[B_TEMPER], d->b[B_TEMPER2], d->b[B_ENEW], d->ctl);
+      swapb(d, B_ENERGY, B_ENEW);
+      // ---- Transport
+      {
+        RadQ q;
+        q.rho = d->b[B_DENS];
+        q.vr = d->b[B_VRAD];
+        q.vt = d->b[B_VTHETA];
+        q.e = d->b[B_ENERGY];
+        const int outs[RC_NQ] = {B_XRP, B_XRM, B_XTP, B_XTM, B_XE, B_XRHO};
+        for (int k = 0; k < RC_NQ; k++) q.out[k] = d->b[outs[k]];
+        q.guard = guard;
+        q.of = c->st.omega_frame;
+        q.ntile_j = (ns + TB - 1) / TB;
+        q.ntiles = q.ntile_j * ((nr + RC_I - 1) / RC_I);
+        LAUNCH(k_transport_rad, dim3((unsigned)((q.ntiles + 7) / 8 * 8 * RC_NQ)), tb, 0, d->g, q, dt);
+#define TT_LAUNCH(NPT)                                                                                         \
+  LAUNCH(k_transport_theta<NPT>, dim3(nr), dim3(TT_T), sizeof(double) * 2 * ns, d->g, d->b[B_VTHETA], d->b[B_XRP],  \
+         d->b[B_XRM], d->b[B_XTP], d->b[B_XTM], d->b[B_XE], d->b[B_XRHO], d->b[B_RP], d->b[B_RM], d->b[B_TP],     \
+         d->b[B_TM], d->b[B_ENERGY], d->b[B_DENS], dt, guard)
+        switch ((ns + TT_T - 1) / TT_T) {
+          case 1: TT_LAUNCH(1); break;
+          case 2: TT_LAUNCH(2); break;
+          case 3: TT_LAUNCH(3); break;
+          case 4: TT_LAUNCH(4); break;
+          case 5: TT_LAUNCH(5); break;
+          default: TT_LAUNCH(6); break;
+        }
+#undef TT_LAUNCH
+        LAUNCH(k_velocities, gall, tb, 0, d->g, d->b[B_DENS], d->b[B_RP], d->b[B_RM], d->b[B_TP], d->b[B_TM],
+               d->b[B_VRAD], d->b[B_VTHETA], guard);
+      }
+      if (p->damping_on)
+        LAUNCH(k_boundary, gall, tb, 0, d->g, d->b[B_VRAD], d->b[B_VTHETA], d->b[B_DENS], d->b[B_ENERGY], dt, guard);
+      // ---- end of the substep: T, cs and their ring means, and the CFL
+      // minima of the next substep (its planets are those of now)
+      LAUNCH(k_eos_cfl, dim3(nr), rb, sizeof(double) * (3 * ns + RB), d->g, d->b[B_DENS], d->b[B_ENERGY],
+             d->b[B_TEMPER], d->b[B_CS], d->b[B_VRAD], d->b[B_VTHETA], d->cfl_invdtreb, 1, guard);
+      LAUNCH(k_cfl_final, dim3(1), rb, 0, d->g, d->ctl, (const double *)d->pstellar, d->nblk_cells, guard);
+      HCHECK(hipGetLastError());
+      // one sync per substep: SOR result, crash flags, dt_stellar, the next
+      // CFL minimum, the planet accelerations
+      HCHECK(hipMemcpyAsync(d->hctl, d->ctl, sizeof(DevCtl), hipMemcpyDeviceToHost, d->s));
+      HCHECK(hipStreamSynchronize(d->s));
+      return 0;
+    };
+    if (after_sor()) return -1;
+    if (d->hctl->sor_done) {
+      niter = d->hctl->sor_niter;
+    } else {
+      if (sor_finish(c, c->st.sor_omega, sor_k, &niter)) return -1;
+      memcpy(d->b, bsave, sizeof bsave);
+      if (after_sor()) return -1;
+    }
+    if (niter > SORMAXITERS) rc = 2;
+    if (niter > c->st.sor_niter_last) c->st.sor_domega = -c->st.sor_domega;
+    c->st.sor_niter_last = niter;
+    d->niter_hist[d->nsolve++ & 3] = niter;
     if (log && log->nsub < THORIN_MAXSUB) {
       log->cfl_n[log->nsub] = N;
       log->dt[log->nsub] = dt;
@@ -2492,9 +2568,6 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
       log->sor_omega[log->nsub] = omega_used;
       log->nsub++;
     }
-    // one sync per substep: crash flags, dt_stellar and the next CFL minimum
-    HCHECK(hipMemcpyAsync(d->hctl, d->ctl, sizeof(DevCtl), hipMemcpyDeviceToHost, d->s));
-    HCHECK(hipStreamSynchronize(d->s));
     d->cfl_ready = 1;
     c->st.dt_stellar = d->hctl->dt_stellar;
     for (int k = 0; k < sys->nb; k++) { sys->ax[k] = d->hctl->axpl[k]; sys->ay[k] = d->hctl->aypl[k]; }
