@@ -72,6 +72,10 @@ struct DevCtl {                 // device-side control block, mirrored to pinned
   double tq_rho[16], tq_cs[16]; // torque log: ring means next to the logged planets
 };
 
+struct TqRings {                // rings of the torque log's ring means (-1: none)
+  int r[16];
+};
+
 struct PlanetK {                // per-planet constants of FillForcesArrays (Pframeforce.c:93-121)
   double x, y, z, m, rhill, rst2, rst3, rst4, rsm2, rmorevert2;
   int iring;                    // ring of ThicknessSmoothing, -1 outside the grid
@@ -120,6 +124,7 @@ struct thorin_dev {
   int normres0_valid;           // ctl->sor_normres0 holds the residual of the current matrix
   int cfl_ready;                // hctl->newdt_raw holds the CFL minimum of the current fields
   int niter_hist[4], nsolve;    // SOR iteration counts of the last four solves
+  int rs_rows;                  // rings per workgroup of k_ring_means (LDS holds them whole)
   double cfl_invdtreb;          // MinStepForRebound term it was formed with
   // kernel timing with HIP events (thorin_prof_*)
   int prof_on;
@@ -135,7 +140,7 @@ static const char *const KNAMES[] = {
     "k_eos_cfl", "k_cfl_final", "k_forces", "k_forces_reduce", "k_indirect", "k_boundary", "k_substep1",
     "k_divstress", "k_visc_update", "k_substep2",
     "k_matrix", "k_sor_resid0", "k_sor_iter", "k_sor_check", "k_energy_new", "k_transport_rad",
-    "k_transport_theta", "k_velocities", "k_tq_rings", "k_substep3", NULL};
+    "k_transport_theta", "k_velocities", "k_substep3", NULL};
 
 // a template instance ("k_name<3>") counts as its kernel
 static int kid_of(const char *name) {
@@ -247,32 +252,39 @@ __device__ __forceinline__ double opacity_law(const Geo &g, double rho, double T
 }
 
 // fixed-order block reduction (sum), blockDim.x == n threads, power of two
+// Block reductions of N threads (a multiple of the 64-lane wave): a
+// butterfly inside each wave (every lane ends with the same value: IEEE
+// addition is commutative), then the N/64 wave results in wave order.  sh
+// needs N/64 doubles; every thread gets the result.
 template <int N>
 __device__ __forceinline__ double block_sum(double v, double *sh) {
-  int t = threadIdx.x;
-  sh[t] = v;
-  __syncthreads();
 #pragma unroll
-  for (int s = N / 2; s > 0; s >>= 1) {
-    if (t < s) sh[t] = sh[t] + sh[t + s];
-    __syncthreads();
-  }
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  if (N == 64) return v;
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) sh[w] = v;
+  __syncthreads();
   double r = sh[0];
+#pragma unroll
+  for (int k = 1; k < N / 64; k++) r += sh[k];
   __syncthreads();
   return r;
 }
 
 template <int N>
 __device__ __forceinline__ double block_min(double v, double *sh) {
-  int t = threadIdx.x;
-  sh[t] = v;
-  __syncthreads();
 #pragma unroll
-  for (int s = N / 2; s > 0; s >>= 1) {
-    if (t < s) { double o = sh[t + s]; if (o < sh[t]) sh[t] = o; }
-    __syncthreads();
+  for (int o = 32; o > 0; o >>= 1) {
+    const double u = __shfl_xor(v, o, 64);
+    v = u < v ? u : v;
   }
+  if (N == 64) return v;
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) sh[w] = v;
+  __syncthreads();
   double r = sh[0];
+#pragma unroll
+  for (int k = 1; k < N / 64; k++) r = sh[k] < r ? sh[k] : r;
   __syncthreads();
   return r;
 }
@@ -283,9 +295,9 @@ __device__ __forceinline__ double block_min(double v, double *sh) {
 // so that XCD x gets the contiguous band x*per .. x*per+per-1 keeps the
 // neighbouring rings of a radial stencil in the same L2.  L is the logical
 // block (ring-major); false for the padding blocks of the last band.
-__device__ __forceinline__ bool xcd_cell(const Geo &g, int i0, int nring, int &i, int &j, int &L) {
+__device__ __forceinline__ bool xcd_cell(const Geo &g, int i0, int nring, int &i, int &j, int &L, int boff = 0) {
   const int total = g.ncb * nring, per = (total + 7) >> 3;
-  const int b = blockIdx.x;
+  const int b = blockIdx.x - boff;           // boff: leading blocks with another job (a multiple of 8)
   L = (b & 7) * per + (b >> 3);
   if (L >= total) return false;
   const int q = L / g.ncb;
@@ -294,11 +306,36 @@ __device__ __forceinline__ bool xcd_cell(const Geo &g, int i0, int nring, int &i
   return true;
 }
 
+// Ordered sum of one ring read by one lane straight from global memory (the
+// association of ordered_ring_sum): the next 16 values are loaded before the
+// 16 dependent adds of the current ones.
+__device__ __forceinline__ double ordered_row_sum_global(const double *__restrict__ row, int ns) {
+  double s = 0.0;
+  double a[16], b[16];
+  int j = 0;
+  if (ns >= 16) {
+#pragma unroll
+    for (int k = 0; k < 16; k++) a[k] = row[k];
+    for (j = 16; j + 16 <= ns; j += 16) {
+#pragma unroll
+      for (int k = 0; k < 16; k++) b[k] = row[j + k];
+#pragma unroll
+      for (int k = 0; k < 16; k++) s += a[k];
+#pragma unroll
+      for (int k = 0; k < 16; k++) a[k] = b[k];
+    }
+#pragma unroll
+    for (int k = 0; k < 16; k++) s += a[k];
+  }
+  for (; j < ns; j++) s += row[j];
+  return s;
+}
+
 // plain round-robin numbering (logical block = launch block) for kernels
 // whose cost varies strongly from ring to ring (the force kernel near a
 // planet), so that the expensive rings spread over all XCDs
-__device__ __forceinline__ bool rr_cell(const Geo &g, int i0, int nring, int &i, int &j, int &L) {
-  L = blockIdx.x;
+__device__ __forceinline__ bool rr_cell(const Geo &g, int i0, int nring, int &i, int &j, int &L, int boff = 0) {
+  L = blockIdx.x - boff;
   if (L >= g.ncb * nring) return false;
   const int q = L / g.ncb;
   i = i0 + q;
@@ -384,97 +421,120 @@ __device__ __forceinline__ double ordered_ring_sum(const double *sh, int ns) {
 //    whose block minimum goes to ringmin[i];
 //  * the ring-averaged vertical integrals of the next FillForcesArrays
 //    (Pframeforce.c:150-189), which only depend on globcsvec.
-// Three lanes in three different waves walk their ring sums concurrently.
+// The ring sums are sequential in j; lanes 0, 1, 2 of the first wave walk
+// them side by side while the other waves wait.  The CFL terms that need no
+// ring mean are formed before the sums (EC_MAX cells per thread) and the sum
+// under the square root is completed afterwards in the reference's order.
 // With do_eos = 0 the temperature and sound speed already in memory are used.
+#define EC_MAX 12              // cells per thread: NSEC <= RB * EC_MAX
 __global__ void __launch_bounds__(RB) k_eos_cfl(Geo g, const double *__restrict__ rho, const double *__restrict__ e,
                                                 double *__restrict__ T, double *__restrict__ cs,
                                                 const double *__restrict__ vr, const double *__restrict__ vt,
                                                 double invdtreb_sq, int do_eos, const int *__restrict__ guard) {
-  extern __shared__ double sh[];       // 3*ns + RB: vt, cs, T, reduction
+  extern __shared__ double sh[];       // (2 or 3)*ns: vt, cs [, T]
+  __shared__ double red[RB / 64];
+  __shared__ double vmoy, gcs_i, gtemp_i;
   if (guard && *guard == 0) return;
   const int i = blockIdx.x, ns = g.ns, nr = g.nr, tid = threadIdx.x;
-  double *s_vt = sh, *s_cs = sh + ns, *s_t = sh + 2 * ns, *red = sh + 3 * ns;
+  const bool want_t = g.alpha_flock && do_eos;
+  double *s_vt = sh, *s_cs = sh + ns, *s_t = sh + 2 * ns;
   const double gam = g.adiabind;
   const size_t base = (size_t)i * ns;
-  for (int j = tid; j < ns; j += RB) {
-    const size_t l = base + j;
-    s_vt[j] = vt[l];
-    if (do_eos) {
-      const double tmp2 = 1.0 / (rho[l] * 1.0);
-      const double tmp1 = e[l] * 1.0 * (gam - 1.0);
-      const double t = tmp1 * tmp2;
-      const double c = sqrt(gam * (gam - 1.0) * fabs(e[l]) / rho[l]);
-      T[l] = t;
-      cs[l] = c;
-      s_t[j] = t;
+  const bool cfl = (i >= 1 && i < nr);
+  const double dxrad = g.Rsup[i] - g.Rinf[i];
+  const double dxtheta = g.Rmed[i] * 2.0 * DPI / (double)ns;
+  const double mn = (dxtheta < dxrad) ? dxtheta : dxrad;
+  double pre[EC_MAX], q4[EC_MAX];
+#pragma unroll
+  for (int k = 0; k < EC_MAX; k++) {
+    const int j = tid + k * RB;
+    pre[k] = 0.0;
+    q4[k] = 0.0;
+    if (j < ns) {
+      const size_t l = base + j;
+      const double vtl = vt[l];
+      s_vt[j] = vtl;
+      double c;
+      if (do_eos) {
+        const double tmp2 = 1.0 / (rho[l] * 1.0);
+        const double tmp1 = e[l] * 1.0 * (gam - 1.0);
+        const double t = tmp1 * tmp2;
+        c = sqrt(gam * (gam - 1.0) * fabs(e[l]) / rho[l]);
+        T[l] = t;
+        cs[l] = c;
+        if (want_t) s_t[j] = t;
+      } else {
+        c = cs[l];
+      }
       s_cs[j] = c;
-    } else {
-      s_cs[j] = cs[l];
-      s_t[j] = T[l];
+      if (cfl) {
+        const int jp = (j == ns - 1) ? 0 : j + 1;
+        const double invdt1 = c / mn;
+        const double invdt2 = fabs(vr[l]) / dxrad;
+        double dvr = vr[l + ns] - vr[l];
+        double dvt = vt[base + jp] - vtl;
+        if (dvr >= 0.0) dvr = 1e-10; else dvr = -dvr;
+        if (dvt >= 0.0) dvt = 1e-10; else dvt = -dvt;
+        const double a = dvr / dxrad, b = dvt / dxtheta;
+        double invdt4 = (b > a) ? b : a;
+        invdt4 *= 4.0 * CVNR * CVNR;
+        pre[k] = invdt1 * invdt1 + invdt2 * invdt2;
+        q4[k] = invdt4 * invdt4;
+      }
     }
   }
   __syncthreads();
-  __shared__ double vmoy, gcs_i, gtemp_i;
-  if (tid == 0) {
-    const double v = ordered_ring_sum(s_vt, ns) / (double)ns;
-    vmoy = v;
-    g.Vmoy[i] = v;
-  } else if (tid == 64) {
-    const double c = ordered_ring_sum(s_cs, ns) / (double)ns;
-    gcs_i = c;
-    g.gcs[i] = c;
-    // ring-averaged Gaussian vertical integrals of FillForcesArrays
-    const double H_avr = c * g.OmegaInv[i] * g.sqrt_adiabind_inv;
-    const double H2 = H_avr * H_avr;
-    const double r2 = g.Rmed[i] * g.Rmed[i];
-    const double deltaz = 3.0 * H_avr / (double)10;
-    double sig = 0.0, ist = 0.0;
-    for (int m = 1; m <= 10; m++) {
-      const double znode = ((double)m - 0.5) * deltaz;
-      const double znode2 = znode * znode;
-      const double sum = exp(-(0.5 * znode2 / H2));
-      sig += sum;
-      const double d2 = r2 + znode2;
-      const double dd = sqrt(d2);
-      const double denom = dd * d2;
-      ist += sum / denom;
+  if (tid < 3) {
+    const double *src = (tid == 0) ? s_vt : ((tid == 1) ? s_cs : s_t);
+    if (tid < 2 || want_t) {
+      const double m = ordered_ring_sum(src, ns) / (double)ns;
+      if (tid == 0) {
+        vmoy = m;
+        g.Vmoy[i] = m;
+      } else if (tid == 1) {
+        gcs_i = m;
+        g.gcs[i] = m;
+        // ring-averaged Gaussian vertical integrals of FillForcesArrays
+        const double H_avr = m * g.OmegaInv[i] * g.sqrt_adiabind_inv;
+        const double H2 = H_avr * H_avr;
+        const double r2 = g.Rmed[i] * g.Rmed[i];
+        const double deltaz = 3.0 * H_avr / (double)10;
+        double sig = 0.0, ist = 0.0;
+        for (int q = 1; q <= 10; q++) {
+          const double znode = ((double)q - 0.5) * deltaz;
+          const double znode2 = znode * znode;
+          const double sum = exp(-(0.5 * znode2 / H2));
+          sig += sum;
+          const double d2 = r2 + znode2;
+          const double dd = sqrt(d2);
+          const double denom = dd * d2;
+          ist += sum / denom;
+        }
+        g.sig_avr[i] = sig;
+        g.ist_avr[i] = ist;
+      } else {
+        gtemp_i = m;
+        g.gtemp[i] = m;
+      }
     }
-    g.sig_avr[i] = sig;
-    g.ist_avr[i] = ist;
-  } else if (tid == 128) {
-    double t = g.gtemp[i];
-    if (g.alpha_flock && do_eos) {
-      t = ordered_ring_sum(s_t, ns) / (double)ns;
-      g.gtemp[i] = t;
-    }
-    gtemp_i = t;
+    if (tid == 2 && !want_t) gtemp_i = g.gtemp[i];
   }
   __syncthreads();
   double m = 1e30;
-  if (i >= 1 && i < nr) {
-    const double dxrad = g.Rsup[i] - g.Rinf[i];
-    const double dxtheta = g.Rmed[i] * 2.0 * DPI / (double)ns;
-    const double mn = (dxtheta < dxrad) ? dxtheta : dxrad;
+  if (cfl) {
     const int viscous = g.visc_alpha || (g.viscosity != 0.0);
     const double visc = viscous ? fvisc_val(g, gcs_i, gtemp_i, g.pow_rmed15[i], g.cav_med[i]) : 0.0;
     const double invdt5 = viscous ? visc * 4. / g.cfl_pow2[i] : 1e-30;
-    for (int j = tid; j < ns; j += RB) {
-      const size_t l = base + j;
-      const int jp = (j == ns - 1) ? 0 : j + 1;
-      const double vres = g.fast_transport ? s_vt[j] - vmoy : s_vt[j];
-      const double invdt1 = s_cs[j] / mn;
-      const double invdt2 = fabs(vr[l]) / dxrad;
-      const double invdt3 = fabs(vres) / dxtheta;
-      double dvr = vr[l + ns] - vr[l];
-      double dvt = s_vt[jp] - s_vt[j];
-      if (dvr >= 0.0) dvr = 1e-10; else dvr = -dvr;
-      if (dvt >= 0.0) dvt = 1e-10; else dvt = -dvt;
-      const double a = dvr / dxrad, b = dvt / dxtheta;
-      double invdt4 = (b > a) ? b : a;
-      invdt4 *= 4.0 * CVNR * CVNR;
-      const double dt = CFLSECURITY / sqrt(invdt1 * invdt1 + invdt2 * invdt2 + invdt3 * invdt3 + invdt4 * invdt4 +
-                                           invdt5 * invdt5 + invdtreb_sq);
-      if (dt < m) m = dt;
+    const double vm = vmoy;
+#pragma unroll
+    for (int k = 0; k < EC_MAX; k++) {
+      const int j = tid + k * RB;
+      if (j < ns) {
+        const double vres = g.fast_transport ? s_vt[j] - vm : s_vt[j];
+        const double invdt3 = fabs(vres) / dxtheta;
+        const double dt = CFLSECURITY / sqrt(pre[k] + invdt3 * invdt3 + q4[k] + invdt5 * invdt5 + invdtreb_sq);
+        if (dt < m) m = dt;
+      }
     }
   }
   m = block_min<RB>(m, red);
@@ -514,11 +574,30 @@ __global__ void __launch_bounds__(TB) k_forces(Geo g, const double *__restrict__
                                                const double *__restrict__ ord,
                                                double *__restrict__ agr, double *__restrict__ agt,
                                                const PlanetK *__restrict__ pk, int npl, double mt,
-                                               double *__restrict__ partial) {
+                                               double *__restrict__ partial, TqRings tq, int ntq_rings,
+                                               DevCtl *__restrict__ ctl) {
   __shared__ double red[TB];
   const int ns = g.ns;
+  const int lead = ntq_rings > 0 ? 1 : 0;
+  if ((int)blockIdx.x < lead) {
+    // leading block: the torque log's ring means of rho next to the logged
+    // planets (UpdateLog, Force.c:41-52), lane b walking ring tq.r[b], and the
+    // sound-speed ring means of the same rings (globcsvec); ring -1 gives 0
+    const int b = threadIdx.x;
+    if (b < ntq_rings) {
+      const int r = tq.r[b];
+      double m = 0.0, c = 0.0;
+      if (r >= 0) {
+        m = ordered_row_sum_global(rho + (size_t)r * ns, ns) / (double)ns;
+        c = g.gcs[r];
+      }
+      ctl->tq_rho[b] = m;
+      ctl->tq_cs[b] = c;
+    }
+    return;
+  }
   int i, j, bid;
-  if (!rr_cell(g, 0, g.nr, i, j, bid)) return;
+  if (!rr_cell(g, 0, g.nr, i, j, bid, lead)) return;
   const bool act = j < ns;
   double axs = 0.0, ays = 0.0;
   double axpl_loc[8], aypl_loc[8];
@@ -940,11 +1019,19 @@ __global__ void __launch_bounds__(TB) k_substep3(Geo g, const double *__restrict
                                                  double *__restrict__ qplus, double *__restrict__ T,
                                                  double *__restrict__ cs, double *__restrict__ opa,
                                                  double *__restrict__ qminus, double *__restrict__ D,
-                                                 double *__restrict__ pmin, int fill_opacity) {
+                                                 double *__restrict__ pmin, int fill_opacity, int nsum) {
   __shared__ double red[TB];
   const int ns = g.ns, nr = g.nr;
+  if ((int)blockIdx.x < nsum) {
+    // leading blocks: VMed of ComputeResiduals (TransportEuler.c:86-91) for
+    // the coming transport, 64 rings per block, lane r walking ring r; vt is
+    // final for the transport from here on
+    const int r = blockIdx.x * 64 + threadIdx.x;
+    if (threadIdx.x < 64 && r < nr) g.VMed[r] = ordered_row_sum_global(vt + (size_t)r * ns, ns) / (double)ns;
+    return;
+  }
   int i, j, L;
-  if (!xcd_cell(g, 0, g.nr, i, j, L)) return;
+  if (!xcd_cell(g, 0, g.nr, i, j, L, nsum)) return;
   double dtmin = 1.0e38;
   if (j < ns) {
     const size_t l = (size_t)i * ns + j;
@@ -1488,6 +1575,27 @@ __global__ void __launch_bounds__(TB) k_transport_rad(Geo g, RadQ q, double dt) 
   }
 }
 
+// Ordered ring means out[r] = (sum over j in order of f[r][j]) / NSEC
+// (mpi_make1Dprofile order, mpiTasks.c:26-32) for `rows` rings per
+// workgroup: the workgroup stages its rings whole in LDS (one coalesced pass,
+// all loads in flight together), then lane r of the first wave walks ring r.
+// The chain of NSEC dependent additions per ring is the cost; a workgroup's
+// rings share it.
+__global__ void __launch_bounds__(256) k_ring_means(const double *__restrict__ f, int nr, int ns, int rows,
+                                                     double *__restrict__ out) {
+  extern __shared__ double sh[];       // rows * ns
+  const int r0 = blockIdx.x * rows;
+  const int nrow = min(rows, nr - r0);
+  const int n = nrow * ns;
+  const double *__restrict__ src = f + (size_t)r0 * ns;
+  for (int k = threadIdx.x; k < n; k += 256) sh[k] = src[k];
+  __syncthreads();
+  if (threadIdx.x < nrow) {
+    const double m = ordered_ring_sum(sh + (size_t)threadIdx.x * ns, ns) / (double)ns;
+    out[r0 + threadIdx.x] = m;
+  }
+}
+
 // Azimuthal transport, ring-local: ComputeResiduals (TransportEuler.c:70-111),
 // QuantitiesAdvection with the residual and then the uniform velocity
 // (159-175; ComputeStarTheta 275-316, VanLeerTheta 417-445) and AdvectSHIFT
@@ -1590,13 +1698,12 @@ __global__ void __launch_bounds__(TT_T) k_transport_theta(Geo g, const double *_
   }
   __syncthreads();
   if (tid == 0) {
-    const double vm = ordered_ring_sum(s_w, ns) / (double)ns;
+    const double vm = g.VMed[i];         // ordered ring mean of vtheta (k_ring_means)
     const double Ntilde = vm * g.InvRmed[i] * dt * (double)ns / 2.0 / DPI;
     const double Nround = floor(Ntilde + 0.5);
     vmed = vm;
     nshift = (int)(long)Nround;
     vtemp = g.fast_transport ? (Ntilde - Nround) * g.Rmed[i] * (1.0 / dt) * (2.0 * DPI / (double)ns) : 0.0;
-    g.VMed[i] = vm;
     g.Nshift[i] = nshift;
   }
   __syncthreads();
@@ -1701,44 +1808,6 @@ __global__ void __launch_bounds__(TB) k_velocities(Geo g, const double *__restri
   vt[l] = (tp[ljm] + tm[l]) / (rho[l] + rho[ljm] + 1e-15) / g.Rmed[i] - g.Rmed[i] * g.omega_frame;
 }
 
-// ring-ordered sum of an arbitrary field (for the host: torque log, DampingTW04)
-__global__ void __launch_bounds__(RB) k_ring_mean(Geo g, const double *__restrict__ f, double *__restrict__ out) {
-  extern __shared__ double sh[];
-  const int i = blockIdx.x, ns = g.ns;
-  for (int j = threadIdx.x; j < ns; j += RB) sh[j] = f[(size_t)i * ns + j];
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double s = ordered_ring_sum(sh, ns);
-    s /= (double)ns;
-    out[i] = s;
-  }
-}
-
-// UpdateLog (Force.c:41-42, 51-52) needs the 1D profiles of rho and cs only
-// on the two rings next to each logged planet: block b averages rho over
-// ring r[b] in the order of mpi_make1Dprofile and copies the sound-speed
-// ring mean of the same ring (globcsvec, the profile of the current cs);
-// ring -1 (beyond the outer edge) gives 0.
-struct TqRings {
-  int r[16];
-};
-__global__ void __launch_bounds__(RB) k_tq_rings(Geo g, const double *__restrict__ rho, TqRings rr, DevCtl *ctl) {
-  extern __shared__ double sh[];
-  const int b = blockIdx.x, i = rr.r[b], ns = g.ns;
-  if (i < 0) {
-    if (threadIdx.x == 0) { ctl->tq_rho[b] = 0.0; ctl->tq_cs[b] = 0.0; }
-    return;
-  }
-  for (int j = threadIdx.x; j < ns; j += RB) sh[j] = rho[(size_t)i * ns + j];
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double m = ordered_ring_sum(sh, ns);
-    m /= (double)ns;
-    ctl->tq_rho[b] = m;
-    ctl->tq_cs[b] = g.gcs[i];
-  }
-}
-
 // GasTotalMass partial sums: per ring, sequential in j (SideEuler.c:35-39)
 __global__ void __launch_bounds__(RB) k_ring_mass(Geo g, const double *__restrict__ rho, double *__restrict__ out) {
   extern __shared__ double sh[];
@@ -1768,6 +1837,12 @@ static int need_dev(thorin_ctx *c) {
     return -1;
   }
   return 0;
+}
+
+// dynamic LDS of k_eos_cfl: the vtheta and cs rings, and T when the Flock
+// alpha needs its ring mean
+static inline size_t eos_lds(const thorin_dev *d, int do_eos) {
+  return sizeof(double) * (size_t)d->ns * ((d->g.alpha_flock && do_eos) ? 3 : 2);
 }
 
 // 1D grid of the XCD-banded cell blocks over `rings` rings (xcd_cell), and
@@ -1817,6 +1892,16 @@ extern "C" int thorin_dev_create(thorin_ctx *c) {
                               (const void *)k_transport_theta<5>, (const void *)k_transport_theta<6>};
     for (int k = 0; k < TT_NPT; k++)
       HCHECK(hipFuncSetAttribute(tt[k], hipFuncAttributeMaxDynamicSharedMemorySize, (int)need));
+    // k_ring_means: as many whole rings as fit in 96 KB of LDS, at most 8
+    d->rs_rows = (int)((96 * 1024) / ((size_t)c->ns * sizeof(double)));
+    if (d->rs_rows > 8) d->rs_rows = 8;
+    if (d->rs_rows < 1) d->rs_rows = 1;
+    const size_t rs_lds = sizeof(double) * (size_t)d->rs_rows * c->ns;
+    if (rs_lds > (size_t)prop.sharedMemPerBlock) {
+      thorin_set_error("NSEC too large for the ring-mean kernel's LDS");
+      return -1;
+    }
+    HCHECK(hipFuncSetAttribute((const void *)k_ring_means, hipFuncAttributeMaxDynamicSharedMemorySize, (int)rs_lds));
   }
   HCHECK(hipStreamCreateWithFlags(&d->s, hipStreamNonBlocking));
   d->prof_pend = new std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>>();
@@ -2023,7 +2108,9 @@ extern "C" int thorin_ring_means(thorin_ctx *c, int field, double *out) {
   if (need_dev(c)) return -1;
   thorin_dev *d = c->dev;
   if (field < 0 || field >= THORIN_NFIELDS) return -1;
-  hipLaunchKernelGGL(k_ring_mean, dim3(d->nr), dim3(RB), sizeof(double) * d->ns, d->s, d->g, d->b[field], d->ringtmp);
+  hipLaunchKernelGGL(k_ring_means, dim3((unsigned)((d->nr + d->rs_rows - 1) / d->rs_rows)), dim3(256),
+                     sizeof(double) * (size_t)d->rs_rows * d->ns, d->s, (const double *)d->b[field], d->nr, d->ns,
+                     d->rs_rows, d->ringtmp);
   HCHECK(hipGetLastError());
   HCHECK(hipMemcpyAsync(out, d->ringtmp, sizeof(double) * d->nr, hipMemcpyDeviceToHost, d->s));
   HCHECK(hipStreamSynchronize(d->s));
@@ -2283,7 +2370,7 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
     // formed here.
     const double invdtreb_sq = minstep_rebound(sys);
     if (!d->cfl_ready || invdtreb_sq != d->cfl_invdtreb) {
-      LAUNCH(k_eos_cfl, dim3(nr), rb, sizeof(double) * (3 * ns + RB), d->g, d->b[B_DENS], d->b[B_ENERGY],
+      LAUNCH(k_eos_cfl, dim3(nr), rb, eos_lds(d, 0), d->g, d->b[B_DENS], d->b[B_ENERGY],
              d->b[B_TEMPER], d->b[B_CS], d->b[B_VRAD], d->b[B_VTHETA], invdtreb_sq, 0, (const int *)nullptr);
       LAUNCH(k_cfl_final, dim3(1), rb, 0, d->g, d->ctl, (const double *)nullptr, 0, (const int *)nullptr);
       HCHECK(hipGetLastError());
@@ -2346,28 +2433,15 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
       if (sys->nb > 0)
         HCHECK(hipMemcpyAsync(d->pk, d->hpk, sizeof(PlanetK) * sys->nb, hipMemcpyHostToDevice, d->s));
     }
-    LAUNCH(k_forces, gall, tb, 0, d->g, d->b[B_DENS], d->b[B_CS], d->b[B_ABS], d->b[B_ORD], d->b[B_AGR],
-           d->b[B_AGT], d->pk, sys->nb, mt, d->red);
-    LAUNCH(k_forces_reduce, dim3(2 + 2 * sys->nb), rb, 0, d->red, d->nblk_cells, d->ctl);
-    LAUNCH(k_indirect, gall, tb, 0, d->g, d->b[B_AGR], d->b[B_AGT], d->b[B_ABS], d->b[B_ORD], d->ctl, indir_px,
-           indir_py);
-    HCHECK(hipGetLastError());
-    // ---- AdvanceSystemFromDisk (host; needs the reductions only when a planet feels the disk)
-    int anyfeel = 0;
-    for (int k = 0; k < sys->nb; k++) anyfeel |= sys->feeldisk[k];
     const int want_tq = p->write_torque && log && log->nsub == 0 && sys->nb > 0;
-    if (anyfeel) {
-      HCHECK(hipMemcpyAsync(d->hctl, d->ctl, sizeof(DevCtl), hipMemcpyDeviceToHost, d->s));
-      HCHECK(hipStreamSynchronize(d->s));
-      for (int k = 0; k < sys->nb; k++) { sys->ax[k] = d->hctl->axpl[k]; sys->ay[k] = d->hctl->aypl[k]; }
-    }
-    // UpdateLog (Force.c:27-70) at the first substep of the DT: the planet
-    // side is taken now, the two ring means per planet are formed on the
-    // device and the rows are completed after the end-of-substep sync
+    // UpdateLog (Force.c:27-70, called from AdvanceSystemFromDisk) at the first
+    // substep of the DT: the planet side is taken now, the two ring means per
+    // planet are formed by the force kernel's leading block and the rows are
+    // completed after the end-of-substep sync
     struct { double t, x, y, m, frac; } tqp[8];
     int ntq = 0;
+    TqRings rr;
     if (want_tq) {
-      TqRings rr;
       ntq = sys->nb < 8 ? sys->nb : 8;
       for (int k = 0; k < ntq; k++) {
         const double x = sys->x[k], y = sys->y[k];
@@ -2389,7 +2463,20 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
         rr.r[2 * k] = ii;
         rr.r[2 * k + 1] = ii + 1 < nr ? ii + 1 : -1;
       }
-      LAUNCH(k_tq_rings, dim3((unsigned)(2 * ntq)), rb, sizeof(double) * ns, d->g, d->b[B_DENS], rr, d->ctl);
+    }
+    LAUNCH(k_forces, dim3(gall.x + (ntq ? 1u : 0u)), tb, 0, d->g, d->b[B_DENS], d->b[B_CS], d->b[B_ABS],
+           d->b[B_ORD], d->b[B_AGR], d->b[B_AGT], d->pk, sys->nb, mt, d->red, rr, 2 * ntq, d->ctl);
+    LAUNCH(k_forces_reduce, dim3(2 + 2 * sys->nb), rb, 0, d->red, d->nblk_cells, d->ctl);
+    LAUNCH(k_indirect, gall, tb, 0, d->g, d->b[B_AGR], d->b[B_AGT], d->b[B_ABS], d->b[B_ORD], d->ctl, indir_px,
+           indir_py);
+    HCHECK(hipGetLastError());
+    // ---- AdvanceSystemFromDisk (host; needs the reductions only when a planet feels the disk)
+    int anyfeel = 0;
+    for (int k = 0; k < sys->nb; k++) anyfeel |= sys->feeldisk[k];
+    if (anyfeel) {
+      HCHECK(hipMemcpyAsync(d->hctl, d->ctl, sizeof(DevCtl), hipMemcpyDeviceToHost, d->s));
+      HCHECK(hipStreamSynchronize(d->s));
+      for (int k = 0; k < sys->nb; k++) { sys->ax[k] = d->hctl->axpl[k]; sys->ay[k] = d->hctl->aypl[k]; }
     }
     if (anyfeel) {
       std::vector<double> rmean(nr), csmean(nr);
@@ -2455,9 +2542,14 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
     // ---- SubStep3: Q+, implicit radiative diffusion
     int niter = 0;
     double omega_used = 0.0;
-    LAUNCH(k_substep3, gall, tb, 0, d->g, d->b[B_VRAD], d->b[B_VTHETA], d->b[B_DENS], d->b[B_EINT], d->b[B_DIVV],
-           d->b[B_TAURR], d->b[B_TAURP], d->b[B_TAUPP], d->b[B_QPLUS], d->b[B_TEMPER], d->b[B_CS],
-           d->b[B_OPACITY], d->b[B_QMINUS], d->b[B_DIFF], d->pstellar, d->opa_filled ? 0 : 1);
+    {
+      const int nsum = (nr + 63) / 64;                   // ring-mean blocks, 64 rings each
+      const int nsum8 = (nsum + 7) / 8 * 8;
+      LAUNCH(k_substep3, dim3(gall.x + (unsigned)nsum8), tb, 0, d->g, d->b[B_VRAD], d->b[B_VTHETA], d->b[B_DENS],
+             d->b[B_EINT], d->b[B_DIVV], d->b[B_TAURR], d->b[B_TAURP], d->b[B_TAUPP], d->b[B_QPLUS],
+             d->b[B_TEMPER], d->b[B_CS], d->b[B_OPACITY], d->b[B_QMINUS], d->b[B_DIFF], d->pstellar,
+             d->opa_filled ? 0 : 1, nsum8);
+    }
     d->opa_filled = 1;
     {
       const dim3 gm = cell_grid(d, nr - 2);
@@ -2539,7 +2631,7 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
         LAUNCH(k_boundary, gall, tb, 0, d->g, d->b[B_VRAD], d->b[B_VTHETA], d->b[B_DENS], d->b[B_ENERGY], dt, guard);
       // ---- end of the substep: T, cs and their ring means, and the CFL
       // minima of the next substep (its planets are those of now)
-      LAUNCH(k_eos_cfl, dim3(nr), rb, sizeof(double) * (3 * ns + RB), d->g, d->b[B_DENS], d->b[B_ENERGY],
+      LAUNCH(k_eos_cfl, dim3(nr), rb, eos_lds(d, 1), d->g, d->b[B_DENS], d->b[B_ENERGY],
              d->b[B_TEMPER], d->b[B_CS], d->b[B_VRAD], d->b[B_VTHETA], d->cfl_invdtreb, 1, guard);
       LAUNCH(k_cfl_final, dim3(1), rb, 0, d->g, d->ctl, (const double *)d->pstellar, d->nblk_cells, guard);
       HCHECK(hipGetLastError());
