@@ -213,6 +213,26 @@ class Disk:
                 out[n] = (ms.value, cnt.value)
         return out
 
+    # --- the energy equation's implicit solve on its own
+    def rad_system(self, dt: float = 0.0) -> float:
+        """Assemble SubStep3's radiative-diffusion system of the current fields
+        for a step dt (dt <= 0: the hydro CFL step); its T is kept as the
+        initial guess of rad_solve.  Returns the dt used."""
+        used = C.c_double()
+        _check(lib().thorin_rad_system(self._h, C.c_double(dt), C.byref(used)), "thorin_rad_system")
+        return used.value
+
+    def rad_solve(self, method: int = 0, tol: float = 1e-10, omega: float = 1.8, inner_sweeps: int = 8,
+                  max_iters: int = 100000) -> dict:
+        """Solve the assembled system from its initial guess: method 0 = FP64
+        red-black SOR, 1 = FP32 SOR sweeps in FP64 iterative refinement.
+        T is left in field "temper"."""
+        o = _capi.SolveOpts(method=method, inner_sweeps=inner_sweeps, max_iters=max_iters, tol=tol, omega=omega)
+        r = _capi.SolveResult()
+        _check(lib().thorin_rad_solve(self._h, C.byref(o), C.byref(r)), "thorin_rad_solve")
+        return {"sweeps": r.sweeps, "rounds": r.rounds, "converged": bool(r.converged), "ms": r.ms,
+                "resid0": r.resid0, "resid": r.resid}
+
     def sync(self) -> None:
         _check(lib().thorin_sync(self._h), "sync")
 

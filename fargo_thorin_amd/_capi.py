@@ -49,6 +49,16 @@ class Geometry(C.Structure):
         "globcsvec", "globtempervec")]
 
 
+class SolveOpts(C.Structure):
+    _fields_ = [("method", C.c_int), ("inner_sweeps", C.c_int), ("max_iters", C.c_int),
+                ("tol", C.c_double), ("omega", C.c_double)]
+
+
+class SolveResult(C.Structure):
+    _fields_ = [("sweeps", C.c_int), ("rounds", C.c_int), ("converged", C.c_int),
+                ("ms", C.c_double), ("resid0", C.c_double), ("resid", C.c_double)]
+
+
 NBODY_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(Planets), C.c_double, C.c_double, C.c_double)
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -96,6 +106,8 @@ def lib() -> C.CDLL:
         "thorin_last_error": (C.c_char_p, []),
         "thorin_planets_from_config": (ip, [vp, C.POINTER(Planets)]),
         "thorin_nbody_step": (ip, [vp, C.POINTER(Planets), C.c_double, C.c_double, C.c_double]),
+        "thorin_rad_system": (ip, [vp, C.c_double, dp]),
+        "thorin_rad_solve": (ip, [vp, C.POINTER(SolveOpts), C.POINTER(SolveResult)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

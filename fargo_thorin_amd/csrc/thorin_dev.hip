@@ -72,6 +72,11 @@ struct DevCtl {                 // device-side control block, mirrored to pinned
   double tq_rho[16], tq_cs[16]; // torque log: ring means next to the logged planets
 };
 
+struct Mat32 {                  // FP32 matrix diagonals, residual and correction double buffer
+  float *ml, *mip, *mim, *mjp, *mjm, *r;
+  float *e0, *e1;
+};
+
 struct TqRings {                // rings of the torque log's ring means (-1: none)
   int r[16];
 };
@@ -125,6 +130,8 @@ struct thorin_dev {
   int cfl_ready;                // hctl->newdt_raw holds the CFL minimum of the current fields
   int niter_hist[4], nsolve;    // SOR iteration counts of the last four solves
   int rs_rows;                  // rings per workgroup of k_ring_means (LDS holds them whole)
+  int rad_ready;                // thorin_rad_system has assembled the energy system
+  struct Mat32 m32;             // FP32 copies for the mixed-precision solve (allocated on first use)
   double cfl_invdtreb;          // MinStepForRebound term it was formed with
   // kernel timing with HIP events (thorin_prof_*)
   int prof_on;
@@ -1364,7 +1371,7 @@ __global__ void __launch_bounds__(TB) k_sor_iter(Geo g, double *__restrict__ T0,
 // stop test of iteration it (EnergySources.c:487-493), partial sums added in
 // block order; the last iterate is in T1 after an odd number of iterations
 __global__ void __launch_bounds__(RB) k_sor_check(const double *__restrict__ psum, const int *__restrict__ pcnt,
-                                                  int nb, DevCtl *ctl, int maxiter, int it) {
+                                                  int nb, DevCtl *ctl, int maxiter, int it, double eps) {
   __shared__ double red[RB];
   if (ctl->sor_done) return;
   double s = strided_sum<RB>(psum, nb);
@@ -1387,7 +1394,7 @@ __global__ void __launch_bounds__(RB) k_sor_check(const double *__restrict__ psu
     ctl->sor_iter = n;
     ctl->sor_cur = n & 1;
     ctl->sor_normres = s;
-    if (s < SOREPS * ctl->sor_normres0 || s < SORMIN || c == 0.0) {
+    if (s < eps * ctl->sor_normres0 || s < SORMIN || c == 0.0) {
       ctl->sor_done = 1;
       ctl->sor_niter = n;
     } else if (n >= maxiter) {
@@ -2029,6 +2036,11 @@ extern "C" void thorin_dev_destroy(thorin_ctx *c) {
   if (d->sorc) (void)hipFree(d->sorc);
   if (d->pstellar) (void)hipFree(d->pstellar);
   if (d->mresp) (void)hipFree(d->mresp);
+  {
+    float *f[8] = {d->m32.ml, d->m32.mip, d->m32.mim, d->m32.mjp, d->m32.mjm, d->m32.r, d->m32.e0, d->m32.e1};
+    for (int k = 0; k < 8; k++)
+      if (f[k]) (void)hipFree(f[k]);
+  }
   if (d->prof_pend) {
     for (auto &r : *d->prof_pend) { (void)hipEventDestroy(r.second.first); (void)hipEventDestroy(r.second.second); }
     delete d->prof_pend;
@@ -2233,7 +2245,8 @@ extern "C" const char *thorin_prof_kernel_name(int kid) {
 
 // enqueue iterations it0 .. it0+count-1 (iteration + stop test each); the
 // first one also forms the initial residual when the matrix kernel has not
-static int sor_launch(thorin_ctx *c, double omega, int it0, int count) {
+static int sor_launch(thorin_ctx *c, double omega, int it0, int count, double eps = SOREPS,
+                      int maxiter = SORMAXITERS) {
   thorin_dev *d = c->dev;
   const Geo &g = d->g;
   const int nr = d->nr, ns = d->ns;
@@ -2252,7 +2265,7 @@ static int sor_launch(thorin_ctx *c, double omega, int it0, int count) {
     LAUNCH(k_sor_iter, gs, dim3(TB), 0, g, d->b[B_TEMPER], d->b[B_TEMPER2], d->b[B_ML], d->b[B_MLIP],
            d->b[B_MLIM], d->b[B_MLJP], d->b[B_MLJM], d->b[B_RHS], omega, d->sorp, d->sorc, d->ctl, it,
            (const double *)d->mresp, d->nb_mresp);
-    LAUNCH(k_sor_check, dim3(1), dim3(RB), 0, d->sorp, d->sorc, nbs, d->ctl, SORMAXITERS, it);
+    LAUNCH(k_sor_check, dim3(1), dim3(RB), 0, d->sorp, d->sorc, nbs, d->ctl, maxiter, it, eps);
   }
   HCHECK(hipGetLastError());
   return 0;
@@ -2261,12 +2274,13 @@ static int sor_launch(thorin_ctx *c, double omega, int it0, int count) {
 // Run the solve to its end with host checks between batches, the first
 // `launched` iterations being already enqueued.  The last iterate stays in
 // B_TEMPER or B_TEMPER2 as ctl->sor_cur says (k_energy_new picks it up).
-static int sor_finish(thorin_ctx *c, double omega, int launched, int *niter_out) {
+static int sor_finish(thorin_ctx *c, double omega, int launched, int *niter_out, double eps = SOREPS,
+                      int maxiter = SORMAXITERS, int first_batch = 0) {
   thorin_dev *d = c->dev;
-  int batch = c->st.sor_niter_last > 0 ? c->st.sor_niter_last + 1 : 8;
+  int batch = first_batch > 0 ? first_batch : (c->st.sor_niter_last > 0 ? c->st.sor_niter_last + 1 : 8);
   if (batch > 256) batch = 256;
   if (launched == 0) {
-    if (sor_launch(c, omega, 0, batch)) return -1;
+    if (sor_launch(c, omega, 0, batch, eps, maxiter)) return -1;
     launched = batch;
   }
   for (;;) {
@@ -2277,9 +2291,9 @@ static int sor_finish(thorin_ctx *c, double omega, int launched, int *niter_out)
       *niter_out = d->hctl->sor_niter;
       return 0;
     }
-    if (launched > SORMAXITERS + 2) { *niter_out = SORMAXITERS + 1; return 0; }
+    if (launched > maxiter + 2) { *niter_out = maxiter + 1; return 0; }
     batch = batch < 64 ? 2 * batch : 128;
-    if (sor_launch(c, omega, launched, batch)) return -1;
+    if (sor_launch(c, omega, launched, batch, eps, maxiter)) return -1;
     launched += batch;
   }
 }
@@ -2689,4 +2703,279 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
     }
   }
   return rc;
+}
+
+// ===========================================================================
+// The energy equation's implicit solve on its own (EnergySources.c:200-335),
+// for solver studies: the plain FP64 red-black SOR of the reference, and FP32
+// red-black SOR correction sweeps inside FP64 iterative refinement.
+//
+// Refinement round: r = A.T - RHS in FP64 (rings 1..NRAD-2, T of rings 0 and
+// NRAD-1 fixed), then A.e = -r solved approximately by `inner` FP32 red-black
+// sweeps from e = 0 (e = 0 on the fixed rings), then T += e in FP64.  The
+// rounds stop when sum|r| <= tol * sum|r0|.
+
+// FP64 residual r = A.T - RHS of rings 1..nr-2 as FP32, with block partials of
+// sum|r| (FP64)
+__global__ void __launch_bounds__(TB) k_mp_resid(Geo g, const double *__restrict__ T,
+                                                 const double *__restrict__ Ml, const double *__restrict__ Mlip,
+                                                 const double *__restrict__ Mlim, const double *__restrict__ Mljp,
+                                                 const double *__restrict__ Mljm, const double *__restrict__ RHS,
+                                                 float *__restrict__ r32, double *__restrict__ partial) {
+  __shared__ double red[TB / 64];
+  const int ns = g.ns;
+  int i, j, L;
+  if (!xcd_cell(g, 1, g.nr - 2, i, j, L)) return;
+  double a = 0.0;
+  if (j < ns) {
+    const size_t l = (size_t)i * ns + j;
+    const size_t ljp = (j == ns - 1) ? (size_t)i * ns : l + 1;
+    const size_t ljm = (j == 0) ? (size_t)i * ns + ns - 1 : l - 1;
+    const double r = sor_resid(T, Ml, Mlip, Mlim, Mljp, Mljm, RHS, l, l + ns, l - ns, ljp, ljm);
+    r32[l] = (float)r;
+    a = fabs(r);
+  }
+  a = block_sum<TB>(a, red);
+  if (threadIdx.x == 0) partial[L] = a;
+}
+
+// FP32 copies of the five matrix diagonals (rings 1..nr-2)
+__global__ void __launch_bounds__(TB) k_mp_convert(Geo g, const double *__restrict__ Ml,
+                                                   const double *__restrict__ Mlip, const double *__restrict__ Mlim,
+                                                   const double *__restrict__ Mljp, const double *__restrict__ Mljm,
+                                                   Mat32 m) {
+  const int ns = g.ns;
+  int i, j, L;
+  if (!xcd_cell(g, 1, g.nr - 2, i, j, L) || j >= ns) return;
+  const size_t l = (size_t)i * ns + j;
+  m.ml[l] = (float)Ml[l];
+  m.mip[l] = (float)Mlip[l];
+  m.mim[l] = (float)Mlim[l];
+  m.mjp[l] = (float)Mljp[l];
+  m.mjm[l] = (float)Mljm[l];
+}
+
+// T += e on rings 1..nr-2
+__global__ void __launch_bounds__(TB) k_mp_update(Geo g, double *__restrict__ T, const float *__restrict__ e) {
+  const int ns = g.ns;
+  int i, j, L;
+  if (!xcd_cell(g, 1, g.nr - 2, i, j, L) || j >= ns) return;
+  const size_t l = (size_t)i * ns + j;
+  T[l] += (double)e[l];
+}
+
+// one FP32 red-black sweep of A.e = -r, tiled like k_sor_iter (red
+// half-sweep also on the one-cell halo); ein -> eout
+struct SorCell32 {
+  float ml, mip, mim, mjp, mjm, r;
+  int k;
+};
+
+__global__ void __launch_bounds__(TB) k_mp_sweep(Geo g, const float *__restrict__ ein, float *__restrict__ eout,
+                                                 Mat32 m, float omega) {
+  __shared__ float t[(SOR_TI + 4) * SOR_LW];
+  const int ns = g.ns, nr = g.nr;
+  const int ntx = (ns + SOR_TJ - 1) / SOR_TJ;
+  int tx, ty, bid;
+  if (!xcd_tile(ntx, (nr - 2 + SOR_TI - 1) / SOR_TI, tx, ty, bid)) return;
+  const int i0 = 1 + ty * SOR_TI, i1 = min(i0 + SOR_TI, nr - 1);
+  const int j0 = tx * SOR_TJ, tw = min(SOR_TJ, ns - j0), ti = i1 - i0, tid = threadIdx.x;
+  SorCell32 rc[SOR_NRED], bc[SOR_NBLK];
+  const int rw = (tw + 3) / 2, bw = (tw + 1) / 2;
+  auto load = [&](SorCell32 &x, size_t l, int k) {
+    x.ml = m.ml[l]; x.mip = m.mip[l]; x.mim = m.mim[l]; x.mjp = m.mjp[l]; x.mjm = m.mjm[l]; x.r = m.r[l];
+    x.k = k;
+  };
+#pragma unroll
+  for (int q = 0; q < SOR_NRED; q++) {
+    rc[q].k = -1;
+    const int sl = tid + q * TB;
+    if (sl < (ti + 2) * rw) {
+      const int r = 1 + sl / rw, mm = sl - (r - 1) * rw;
+      const int i = i0 - 2 + r;
+      const int c = (((i + j0 - 1) & 1) ? 1 : 2) + 2 * mm;
+      if (c <= tw + 2 && i >= 1 && i <= nr - 2) {
+        int j = j0 - 2 + c;
+        if (j < 0) j += ns;
+        if (j >= ns) j -= ns;
+        load(rc[q], (size_t)i * ns + j, r * SOR_LW + c);
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < SOR_NBLK; q++) {
+    bc[q].k = -1;
+    const int sl = tid + q * TB;
+    if (sl < ti * bw) {
+      const int r = 2 + sl / bw, mm = sl - (r - 2) * bw;
+      const int i = i0 - 2 + r;
+      const int c = (((i + j0) & 1) ? 3 : 2) + 2 * mm;
+      if (c <= tw + 1) load(bc[q], (size_t)i * ns + (j0 - 2 + c), r * SOR_LW + c);
+    }
+  }
+  for (int idx = tid; idx < (ti + 4) * (tw + 4); idx += TB) {
+    const int r = idx / (tw + 4), c = idx - r * (tw + 4);
+    const int i = i0 - 2 + r;
+    float v = 0.0f;                       // rings 0 and nr-1: e = 0
+    if (i >= 1 && i <= nr - 2) {
+      int j = j0 - 2 + c;
+      if (j < 0) j += ns;
+      if (j >= ns) j -= ns;
+      v = ein[(size_t)i * ns + j];
+    }
+    t[r * SOR_LW + c] = v;
+  }
+  __syncthreads();
+  auto upd = [&](const SorCell32 &x) {
+    const int k = x.k;
+    const float res = t[k] * x.ml + t[k + SOR_LW] * x.mip + t[k - SOR_LW] * x.mim + t[k + 1] * x.mjp +
+                      t[k - 1] * x.mjm + x.r;
+    t[k] = t[k] - omega * res / x.ml;
+  };
+#pragma unroll
+  for (int q = 0; q < SOR_NRED; q++)
+    if (rc[q].k >= 0) upd(rc[q]);
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < SOR_NBLK; q++)
+    if (bc[q].k >= 0) upd(bc[q]);
+  __syncthreads();
+  for (int idx = tid; idx < ti * tw; idx += TB) {
+    const int r = 2 + idx / tw, c = 2 + idx % tw;
+    eout[(size_t)(i0 - 2 + r) * ns + (j0 - 2 + c)] = t[r * SOR_LW + c];
+  }
+}
+
+// sum|A.T - RHS| over rings 1..nr-2 (block partials summed in order on the host)
+static int mp_residual(thorin_ctx *c, float *r32, double *out) {
+  thorin_dev *d = c->dev;
+  const int nr = d->nr;
+  const dim3 gc = cell_grid(d, nr - 2);
+  const int nb = cell_count(d, nr - 2);
+  hipLaunchKernelGGL(k_mp_resid, gc, dim3(TB), 0, d->s, d->g, (const double *)d->b[B_TEMPER],
+                     (const double *)d->b[B_ML], (const double *)d->b[B_MLIP], (const double *)d->b[B_MLIM],
+                     (const double *)d->b[B_MLJP], (const double *)d->b[B_MLJM], (const double *)d->b[B_RHS], r32,
+                     d->mresp);
+  HCHECK(hipGetLastError());
+  std::vector<double> part(nb);
+  HCHECK(hipMemcpyAsync(part.data(), d->mresp, sizeof(double) * nb, hipMemcpyDeviceToHost, d->s));
+  HCHECK(hipStreamSynchronize(d->s));
+  double t = 0.0;
+  for (int k = 0; k < nb; k++) t += part[k];
+  *out = t;
+  return 0;
+}
+
+extern "C" int thorin_rad_system(thorin_ctx *c, double dt, double *dt_used) {
+  if (need_dev(c)) return -1;
+  thorin_dev *d = c->dev;
+  const int nr = d->nr, ns = d->ns;
+  const size_t nb = sizeof(double) * d->nf;
+  const dim3 gall = cell_grid(d, nr), tb(TB), rb(RB);
+  if (dt <= 0.0) {
+    // the hydro CFL step of the current fields (ConditionCFL without planets)
+    LAUNCH(k_eos_cfl, dim3(nr), rb, eos_lds(d, 0), d->g, d->b[B_DENS], d->b[B_ENERGY], d->b[B_TEMPER],
+           d->b[B_CS], d->b[B_VRAD], d->b[B_VTHETA], 0.0, 0, (const int *)nullptr);
+    LAUNCH(k_cfl_final, dim3(1), rb, 0, d->g, d->ctl, (const double *)nullptr, 0, (const int *)nullptr);
+    HCHECK(hipMemcpyAsync(d->hctl, d->ctl, sizeof(DevCtl), hipMemcpyDeviceToHost, d->s));
+    HCHECK(hipStreamSynchronize(d->s));
+    dt = d->hctl->newdt_raw;
+    (void)ns;
+  }
+  if (dt_used) *dt_used = dt;
+  HCHECK(hipMemcpyAsync(d->b[B_EINT], d->b[B_ENERGY], nb, hipMemcpyDeviceToDevice, d->s));
+  LAUNCH(k_substep3, gall, tb, 0, d->g, d->b[B_VRAD], d->b[B_VTHETA], d->b[B_DENS], d->b[B_EINT], d->b[B_DIVV],
+         d->b[B_TAURR], d->b[B_TAURP], d->b[B_TAUPP], d->b[B_QPLUS], d->b[B_TEMPER], d->b[B_CS], d->b[B_OPACITY],
+         d->b[B_QMINUS], d->b[B_DIFF], d->pstellar, d->opa_filled ? 0 : 1, 0);
+  d->opa_filled = 1;
+  const dim3 gm = cell_grid(d, nr - 2);
+  LAUNCH(k_matrix, gm, tb, 0, d->g, d->b[B_DENS], d->b[B_CS], d->b[B_DIFF], d->b[B_DIVV], d->b[B_QMINUS],
+         d->b[B_QPLUS], d->b[B_TEMPER], d->b[B_ML], d->b[B_MLIP], d->b[B_MLIM], d->b[B_MLJP], d->b[B_MLJM],
+         d->b[B_RHS], dt, d->mresp);
+  d->nb_mresp = cell_count(d, nr - 2);
+  d->normres0_valid = 1;
+  HCHECK(hipMemcpyAsync(d->b[B_TBACK], d->b[B_TEMPER], nb, hipMemcpyDeviceToDevice, d->s));
+  HCHECK(hipGetLastError());
+  HCHECK(hipStreamSynchronize(d->s));
+  d->rad_ready = 1;
+  return 0;
+}
+
+extern "C" int thorin_rad_solve(thorin_ctx *c, const thorin_solve_opts *o, thorin_solve_result *res) {
+  if (need_dev(c)) return -1;
+  thorin_dev *d = c->dev;
+  if (!d->rad_ready) { thorin_set_error("thorin_rad_solve: no system assembled (thorin_rad_system)"); return -1; }
+  if (!o || !res || o->tol <= 0.0 || o->omega <= 0.0 || o->omega >= 2.0) {
+    thorin_set_error("thorin_rad_solve: need tol > 0 and 0 < omega < 2");
+    return -1;
+  }
+  memset(res, 0, sizeof *res);
+  const int nr = d->nr, ns = d->ns;
+  const size_t nb = sizeof(double) * d->nf;
+  const int maxit = o->max_iters > 0 ? o->max_iters : 100000;
+  HCHECK(hipMemcpyAsync(d->b[B_TEMPER], d->b[B_TBACK], nb, hipMemcpyDeviceToDevice, d->s));
+  if (!d->m32.ml) {
+    float **f[8] = {&d->m32.ml, &d->m32.mip, &d->m32.mim, &d->m32.mjp, &d->m32.mjm, &d->m32.r, &d->m32.e0,
+                    &d->m32.e1};
+    for (int k = 0; k < 8; k++) {
+      HCHECK(hipMalloc(f[k], sizeof(float) * d->nf));
+      HCHECK(hipMemsetAsync(*f[k], 0, sizeof(float) * d->nf, d->s));
+    }
+  }
+  double r0 = 0.0;
+  if (mp_residual(c, d->m32.r, &r0)) return -1;
+  res->resid0 = r0;
+  hipEvent_t ea, eb;
+  HCHECK(hipEventCreate(&ea));
+  HCHECK(hipEventCreate(&eb));
+  HCHECK(hipEventRecord(ea, d->s));
+  double rcur = r0;
+  if (o->method == 0) {
+    // the reference's SOR with stop test sum|r| (in-sweep) < tol * normres0
+    d->normres0_valid = 0;
+    int niter = 0;
+    if (sor_finish(c, o->omega, 0, &niter, o->tol, maxit, 16)) return -1;
+    res->sweeps = niter > maxit ? maxit : niter;
+    // the last iterate into B_TEMPER (T1 after an odd count)
+    if (d->hctl->sor_cur) HCHECK(hipMemcpyAsync(d->b[B_TEMPER], d->b[B_TEMPER2], nb, hipMemcpyDeviceToDevice, d->s));
+    res->rounds = 1;
+  } else if (o->method == 1) {
+    const dim3 gm = cell_grid(d, nr - 2), tb(TB);
+    const int nbs = ((ns + SOR_TJ - 1) / SOR_TJ) * ((nr - 2 + SOR_TI - 1) / SOR_TI);
+    const dim3 gs((unsigned)((nbs + 7) / 8 * 8));
+    const int inner = o->inner_sweeps > 0 ? o->inner_sweeps : 8;
+    hipLaunchKernelGGL(k_mp_convert, gm, tb, 0, d->s, d->g, (const double *)d->b[B_ML],
+                       (const double *)d->b[B_MLIP], (const double *)d->b[B_MLIM], (const double *)d->b[B_MLJP],
+                       (const double *)d->b[B_MLJM], d->m32);
+    while (rcur > o->tol * r0 && res->sweeps < maxit) {
+      // r32 holds A.T - RHS of the current T; solve A.e = -r from e = 0
+      HCHECK(hipMemsetAsync(d->m32.e0, 0, sizeof(float) * d->nf, d->s));
+      HCHECK(hipMemsetAsync(d->m32.e1, 0, sizeof(float) * d->nf, d->s));
+      float *ein = d->m32.e0, *eout = d->m32.e1;
+      for (int k = 0; k < inner; k++) {
+        hipLaunchKernelGGL(k_mp_sweep, gs, tb, 0, d->s, d->g, (const float *)ein, eout, d->m32, (float)o->omega);
+        float *t = ein; ein = eout; eout = t;
+      }
+      hipLaunchKernelGGL(k_mp_update, gm, tb, 0, d->s, d->g, d->b[B_TEMPER], (const float *)ein);
+      HCHECK(hipGetLastError());
+      res->sweeps += inner;
+      res->rounds++;
+      if (mp_residual(c, d->m32.r, &rcur)) return -1;
+    }
+  } else {
+    thorin_set_error("thorin_rad_solve: unknown method");
+    return -1;
+  }
+  HCHECK(hipEventRecord(eb, d->s));
+  HCHECK(hipEventSynchronize(eb));
+  float ms = 0.f;
+  HCHECK(hipEventElapsedTime(&ms, ea, eb));
+  (void)hipEventDestroy(ea);
+  (void)hipEventDestroy(eb);
+  res->ms = ms;
+  double rfin = 0.0;
+  if (mp_residual(c, d->m32.r, &rfin)) return -1;
+  res->resid = rfin;
+  res->converged = rfin <= o->tol * r0;
+  return 0;
 }

@@ -83,3 +83,39 @@ def write_cfg(path: str, arr: np.ndarray) -> None:
     with open(path, "w") as f:
         f.write("\n".join(repr(float(v)) for v in flat))
         f.write("\n")
+
+
+# T2SI of fondam.h: code temperature unit in K (G Msun / (R/mu) / AU)
+T2SI = 1.32712440018e20 / (8.3144598 / (2.4 * 0.001)) / 149597870700.0
+
+
+def sor_disk(nrad: int = 2048, nsec: int = 3072, *, base_nrad: int = 128, base_nsec: int = 256,
+             rmin: float = 0.4, rmax: float = 2.5, seed: int = 12345) -> dict[str, np.ndarray]:
+    """The mixed-precision SOR workload's disk: the 128x256 seeded surface
+    density interpolated bilinearly in (ln r, phi) (periodic in phi) onto the
+    target log grid, T = 2000 K (r/0.4)^-2.513 (20 K at r = 2.5) in code
+    units, vr = 0 and the pressure-corrected Keplerian vtheta of that T."""
+    base = seeded_disk(base_nrad, base_nsec, rmin=rmin, rmax=rmax, seed=seed)
+    lr_b = np.log(rmed(base_nrad, rmin, rmax, True))
+    rm = rmed(nrad, rmin, rmax, True)
+    lr = np.log(rm)
+    # radial weights (clamped at the edges)
+    fi = np.interp(lr, lr_b, np.arange(base_nrad, dtype=float))
+    i0 = np.clip(np.floor(fi).astype(int), 0, base_nrad - 2)
+    wi = (fi - i0)[:, None]
+    # azimuthal weights on cell centres, periodic
+    phi = (np.arange(nsec) + 0.5) / nsec
+    fj = phi * base_nsec - 0.5
+    j0 = np.floor(fj).astype(int)
+    wj = (fj - j0)[None, :]
+    j0m = j0 % base_nsec
+    j1m = (j0 + 1) % base_nsec
+    d = base["dens"]
+    lo = d[i0][:, j0m] * (1 - wj) + d[i0][:, j1m] * wj
+    hi = d[i0 + 1][:, j0m] * (1 - wj) + d[i0 + 1][:, j1m] * wj
+    dens = lo * (1 - wi) + hi * wi
+    t_ring = 2000.0 * (rm / 0.4) ** (-2.513) / T2SI
+    vt_ring = np.sqrt(1.0 / rm) * np.sqrt(1.0 - t_ring * rm)
+    return {"dens": np.ascontiguousarray(dens), "vrad": np.zeros((nrad, nsec)),
+            "vtheta": np.ascontiguousarray(np.repeat(vt_ring[:, None], nsec, axis=1)),
+            "temper": np.ascontiguousarray(np.repeat(t_ring[:, None], nsec, axis=1))}

@@ -233,6 +233,30 @@ int  thorin_prof_enable(thorin_ctx *ctx, int on, unsigned long long mask);
 int  thorin_prof_read(thorin_ctx *ctx, int kernel_id, double *total_ms, long long *count);
 const char *thorin_prof_kernel_name(int kernel_id);
 
+/* ---- the energy equation's implicit solve on its own (EnergySources.c:
+ * 200-335), for solver studies.  No counterpart entry point in the reference
+ * (there the solve is internal to ImplicitRadiativeDiffusion).
+ * thorin_rad_system assembles the system of the current fields for a step dt
+ * (SubStep3's T, cs, opacity, Q+, Q-, diffusion coefficients and the
+ * five-point matrix) and keeps T as the initial guess; thorin_rad_solve
+ * solves it from that guess, leaving T in THORIN_F_TEMPER.
+ *   method 0: FP64 red-black SOR of the reference (stop: the in-sweep
+ *             sum|A.T - RHS| below tol times its initial value);
+ *   method 1: FP32 red-black SOR correction sweeps (inner_sweeps per round,
+ *             from e = 0) inside FP64 iterative refinement; stop when the
+ *             FP64 sum|A.T - RHS| is below tol times its initial value. */
+typedef struct thorin_solve_opts {
+  int method, inner_sweeps, max_iters;
+  double tol, omega;
+} thorin_solve_opts;
+typedef struct thorin_solve_result {
+  int sweeps, rounds, converged;
+  double ms;                 /* GPU time of the solve (HIP events on the stream) */
+  double resid0, resid;      /* FP64 sum|A.T - RHS| before and after */
+} thorin_solve_result;
+int  thorin_rad_system(thorin_ctx *ctx, double dt, double *dt_used);   /* dt <= 0: the hydro CFL step */
+int  thorin_rad_solve(thorin_ctx *ctx, const thorin_solve_opts *o, thorin_solve_result *r);
+
 /* Synchronise the device and return the library's last error text. */
 int  thorin_sync(thorin_ctx *ctx);
 const char *thorin_last_error(void);
