@@ -71,6 +71,7 @@ def parse():
     ap.add_argument("--nrad", type=int, default=1024)
     ap.add_argument("--nsec", type=int, default=1536)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-sor-study", action="store_true")
     ap.add_argument("--roofline-kernel", default="auto")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (3 times), the reference for scaling "
@@ -125,6 +126,43 @@ def cpu_baseline_reference(nrad, nsec):
             "sample": f"2 DTs of the {nrad}x{nsec} damped workload: wall(Ntot=3) - wall(Ntot=1) of the "
                       f"reference's sequential build (oracle/_ref/thorin, 1 thread); "
                       f"{walls[1]:.1f}s and {walls[3]:.1f}s"}
+
+
+def sor_study(nrad=2048, nsec=3072):
+    """The energy equation's implicit solve at nrad x nsec (BL94 opacity,
+    T 20-2000 K, dt = the hydro CFL step): plain FP64 red-black SOR against
+    FP32 red-black correction sweeps in FP64 iterative refinement, both to a
+    1e-10 relative FP64 residual at omega = 1.8.  The number of FP32 sweeps per
+    refinement round is picked by timing 16, 32 and 64 once."""
+    import numpy as np
+
+    from fargo_thorin_amd import Disk, Params, configs
+    from fargo_thorin_amd.seed import sor_disk
+    cfg = configs.with_grid(configs.SOR_2048x3072, nrad, nsec)
+    P = Params(**cfg, OutputDir="/nonexistent-thorin-out/")
+    disk = sor_disk(nrad, nsec)
+    d = Disk(P)
+    d.init(disk["dens"], disk["vrad"], disk["vtheta"], disk["temper"])
+    dt = d.rad_system(0.0)
+    tol, omega = 1e-10, 1.8
+    tune = {}
+    for inner in (16, 32, 64):
+        tune[inner] = d.rad_solve(1, tol=tol, omega=omega, inner_sweeps=inner)["ms"]
+    inner = min(tune, key=tune.get)
+    plain = min((d.rad_solve(0, tol=tol, omega=omega) for _ in range(2)), key=lambda r: r["ms"])
+    t_plain = d.field("temper")
+    mixed = min((d.rad_solve(1, tol=tol, omega=omega, inner_sweeps=inner) for _ in range(2)),
+                key=lambda r: r["ms"])
+    t_mixed = d.field("temper")
+    err = float(np.max(np.abs(t_mixed - t_plain)) / np.max(np.abs(t_plain)))
+    d.close()
+    return {"workload": f"sor_{nrad}x{nsec}", "dt": dt, "tol": tol, "omega": omega,
+            "plain_fp64": {"ms": plain["ms"], "sweeps": plain["sweeps"],
+                           "rel_resid": plain["resid"] / plain["resid0"]},
+            "fp32_refine": {"ms": mixed["ms"], "sweeps": mixed["sweeps"], "rounds": mixed["rounds"],
+                            "inner_sweeps": inner, "rel_resid": mixed["resid"] / mixed["resid0"],
+                            "rel_err_vs_fp64_sor": err},
+            "speedup": plain["ms"] / mixed["ms"], "inner_sweep_timing_ms": tune}
 
 
 def main():
@@ -197,6 +235,10 @@ def main():
         "kernel_ms_per_step_warmup": {k: v[0] / max(warm - 1, 1) for k, v in sorted(
             warm_prof.items(), key=lambda kv: -kv[1][0])},
     }
+    d.close()
+    if not args.no_sor_study:
+        print("[bench] energy-solve study at 2048x3072", file=sys.stderr, flush=True)
+        out["sor_study"] = sor_study()
     if not args.no_cpu_baseline:
         try:
             out["cpu_baseline"] = cpu_baseline_reference(nrad, nsec)

@@ -18,6 +18,9 @@ FIELDS = ["dens", "vrad", "vtheta", "energy", "label", "temper", "pressure", "so
           "vradint", "vthetaint", "vradnew", "vthetanew", "qbalance", "pebbledens", "pebblevrad",
           "pebblevtheta", "pebblesize", "pebblestokes", "eta", "torque"]
 FIELD_ID = {n: i for i, n in enumerate(FIELDS)}
+# read-only diagnostics: the energy equation's five-point system (THORIN_X_*)
+FIELD_ID.update({"sys_ml": 64, "sys_mlip": 65, "sys_mlim": 66, "sys_mljp": 67, "sys_mljm": 68, "sys_rhs": 69,
+                 "sys_diff": 70})
 OPACITY = {"BL94": 0, "LP85": 1, "ZHU12": 2, "CONST": 3, "POWERLAW": 4}
 
 

@@ -2096,6 +2096,15 @@ extern "C" int thorin_upload_field(thorin_ctx *c, int field, const double *host,
 }
 
 extern "C" int thorin_download_field(thorin_ctx *c, int field, double *host, size_t count) {
+  if (c && c->dev && host && field >= THORIN_X_ML && field <= THORIN_X_DIFF) {
+    static const int xb[] = {B_ML, B_MLIP, B_MLIM, B_MLJP, B_MLJM, B_RHS, B_DIFF};
+    thorin_dev *d = c->dev;
+    if (count > d->nf) count = d->nf;
+    HCHECK(hipMemcpyAsync(host, d->b[xb[field - THORIN_X_ML]], sizeof(double) * count, hipMemcpyDeviceToHost,
+                          d->s));
+    HCHECK(hipStreamSynchronize(d->s));
+    return 0;
+  }
   if (!c || !c->dev || field < 0 || field >= THORIN_NFIELDS || !host) { thorin_set_error("bad download args"); return -1; }
   thorin_dev *d = c->dev;
   if (count > d->nf) count = d->nf;

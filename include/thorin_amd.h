@@ -161,6 +161,12 @@ enum {
   THORIN_NFIELDS
 };
 
+/* Read-only diagnostics for thorin_download_field: the energy equation's
+ * five-point system of the last assembly (EnergySources.c:256-297, rings
+ * 1..nrad-2) and the diffusion coefficient it was built from. */
+enum { THORIN_X_ML = 64, THORIN_X_MLIP, THORIN_X_MLIM, THORIN_X_MLJP, THORIN_X_MLJM, THORIN_X_RHS,
+       THORIN_X_DIFF };
+
 /* 1D geometry (global.h:11-24) filled exactly as FillPolar1DArrays
  * (SourceEuler.c:68-131) and Theo.c.  Arrays have nrad+1 entries. */
 typedef struct thorin_geometry {

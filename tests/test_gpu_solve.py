@@ -1,0 +1,80 @@
+"""The energy equation's implicit solve on its own (thorin_rad_system /
+thorin_rad_solve): the FP64 red-black SOR and the FP32-sweep iterative
+refinement both reach the requested FP64 residual, agree with each other, and
+agree with a direct sparse solve (scipy) of the same five-point system, which
+pins the solution independently of any sweep order."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+from fargo_thorin_amd import Disk, Params, configs
+from fargo_thorin_amd.seed import sor_disk
+
+pytestmark = pytest.mark.gpu
+
+
+def _disk(nr, ns):
+    print(f"[gpu-solve] {nr}x{ns}", flush=True)
+    cfg = configs.with_grid(configs.SOR_2048x3072, nr, ns)
+    P = Params(**cfg, OutputDir="/nonexistent-thorin-out/")
+    disk = sor_disk(nr, ns)
+    d = Disk(P)
+    d.init(disk["dens"], disk["vrad"], disk["vtheta"], disk["temper"])
+    return d
+
+
+def _direct(d, T0):
+    """Solve rings 1..nr-2 of A.T = RHS with T of rings 0 and nr-1 fixed."""
+    nr, ns = d.nrad, d.nsec
+    M = {k: d.field(k) for k in ("sys_ml", "sys_mlip", "sys_mlim", "sys_mljp", "sys_mljm", "sys_rhs")}
+    n = (nr - 2) * ns
+    idx = lambda i, j: (i - 1) * ns + (j % ns)
+    rows, cols, vals = [], [], []
+    b = np.empty(n)
+    for i in range(1, nr - 1):
+        for j in range(ns):
+            k = idx(i, j)
+            rhs = M["sys_rhs"][i, j]
+            for coef, ii, jj in ((M["sys_ml"][i, j], i, j), (M["sys_mlip"][i, j], i + 1, j),
+                                 (M["sys_mlim"][i, j], i - 1, j), (M["sys_mljp"][i, j], i, j + 1),
+                                 (M["sys_mljm"][i, j], i, j - 1)):
+                if 1 <= ii <= nr - 2:
+                    rows.append(k); cols.append(idx(ii, jj)); vals.append(coef)
+                else:
+                    rhs -= coef * T0[ii, jj % ns]
+            b[k] = rhs
+    A = sp.csr_matrix((vals, (rows, cols)), shape=(n, n))
+    return spla.spsolve(A, b).reshape(nr - 2, ns)
+
+
+@pytest.mark.parametrize("nr,ns", [(48, 96), (130, 256)])
+def test_gpu_solvers_agree_with_direct_solve(nr, ns):
+    d = _disk(nr, ns)
+    dt = d.rad_system(0.0)
+    assert dt > 0
+    T0 = d.field("temper")
+    direct = _direct(d, T0)
+    scale = np.max(np.abs(direct))
+    for method, kw in ((0, {}), (1, {"inner_sweeps": 16})):
+        r = d.rad_solve(method, tol=1e-12, omega=1.8, **kw)
+        # method 0 stops on the reference's in-sweep residual, method 1 on the
+        # true FP64 residual
+        assert r["resid"] <= (1e-11 if method == 0 else 1e-12) * r["resid0"], (method, r)
+        T = d.field("temper")
+        assert np.array_equal(T[0], T0[0]) and np.array_equal(T[-1], T0[-1])
+        err = np.max(np.abs(T[1:-1] - direct)) / scale
+        assert err < 1e-10, (method, err)
+
+
+def test_gpu_mixed_precision_matches_fp64_sor_at_1e10():
+    d = _disk(256, 384)
+    d.rad_system(0.0)
+    p = d.rad_solve(0, tol=1e-10, omega=1.8)
+    Tp = d.field("temper")
+    m = d.rad_solve(1, tol=1e-10, omega=1.8, inner_sweeps=16)
+    Tm = d.field("temper")
+    assert m["converged"] and m["resid"] <= 1e-10 * m["resid0"]
+    assert p["resid"] <= 1e-9 * p["resid0"]
+    # both are within the solve tolerance of the exact solution
+    assert np.max(np.abs(Tm - Tp)) / np.max(np.abs(Tp)) < 1e-9
