@@ -21,7 +21,8 @@ _lib = None
 
 ONE_D = {"radii": 0, "rinf": 1, "rsup": 2, "rmed": 3, "surf": 4, "inv_diff_rmed": 5, "sigma_med": 6,
          "sigma_inf": 7, "energy_med": 8, "wave_killer": 9, "globcsvec": 10, "globtempervec": 11,
-         "vtheta_med": 12, "nshift": 13, "vmed": 14}
+         "vtheta_med": 12, "nshift": 13, "vmed": 14, "peb_dens_init": 15, "peb_vrad_init": 16,
+         "peb_vtheta_init": 17}
 
 
 def lib() -> C.CDLL:
@@ -36,6 +37,7 @@ def lib() -> C.CDLL:
         ("oracle_create", ip, [vp, C.POINTER(vp)]),
         ("oracle_destroy", None, [vp]),
         ("oracle_init_from_arrays", ip, [vp, dp, dp, dp, dp]),
+        ("oracle_init_pebbles", ip, [vp, ip, dp, dp, dp, dp]),
         ("oracle_algogas", ip, [vp, C.POINTER(Planets), NBODY_FN, vp, C.POINTER(StepLog)]),
         ("oracle_get_field", ip, [vp, ip, dp, sz]),
         ("oracle_set_field", ip, [vp, ip, dp, sz]),
@@ -71,6 +73,17 @@ class OracleDisk:
         ptrs = [a.ctypes.data_as(C.POINTER(C.c_double)) for a in arrs]
         if lib().oracle_init_from_arrays(self._h, *ptrs) != 0:
             raise RuntimeError("oracle init failed")
+
+    def init_pebbles(self, mode: int = 0, pdens=None, pvrad=None, pvtheta=None, psize=None):
+        """mode 0: equilibrium pebble disk from the gas; mode 1: restart from arrays."""
+        if mode == 0:
+            ptrs = [None] * 4
+        else:
+            arrs = [np.ascontiguousarray(a, dtype=np.float64).ravel() for a in (pdens, pvrad, pvtheta, psize)]
+            self._peb_keep = arrs
+            ptrs = [a.ctypes.data_as(C.POINTER(C.c_double)) for a in arrs]
+        if lib().oracle_init_pebbles(self._h, mode, *ptrs) != 0:
+            raise RuntimeError("oracle pebble init failed")
 
     def algogas(self, planets: Planets, nbody) -> StepLog:
         rc = lib().oracle_algogas(self._h, C.byref(planets), nbody, None, C.byref(self.log))

@@ -60,7 +60,9 @@ enum { F_DENS = THORIN_F_DENS };
 enum { W_RHOSTAR = NF, W_RHOINT, W_TEMPERINT, W_DRR, W_DRP, W_DPP, W_RADMOMP, W_RADMOMM,
        W_THETAMOMP, W_THETAMOMM, W_WORK, W_QRSTAR, W_VTHETARES, W_GRADTR, W_GRADTT,
        W_GRADTM, W_DIFFC, W_DIFFR, W_DIFFT, W_VOLDENS, W_A, W_B, W_ML, W_MLIP, W_MLIM,
-       W_MLJP, W_MLJM, W_RHS, W_ABS, W_ORD, W_TEMPSHIFT, W_DQ, NW };
+       W_MLJP, W_MLJM, W_RHS, W_ABS, W_ORD, W_TEMPSHIFT, W_DQ,
+       /* pebble fluid (Pebbles.c:37-56, SourceEuler.c:155-158) */
+       W_PAGR, W_PAGT, W_PACCR, W_PACCT, W_GACCR, W_GACCT, W_DRAGR, W_DRAGT, W_MACC, NW };
 
 struct oracle_ctx {
   thorin_params p;
@@ -74,6 +76,9 @@ struct oracle_ctx {
   thorin_state st;
   int opa_filled;
   double indirect_x, indirect_y;
+  /* pebbles */
+  double *PebDensInit, *PebVradInit, *PebVthetaInit, *vt1D;
+  double pebbulkdens, invdtpeb_sq;
 };
 
 static double *fieldalloc(int nr, int ns) {
@@ -228,8 +233,9 @@ int oracle_create(const thorin_params *p, oracle_ctx **out) {
   A1(Radii); A1(Rinf); A1(Rsup); A1(Rmed); A1(Surf); A1(InvRmed); A1(InvSurf); A1(InvDiffRmed);
   A1(InvDiffRsup); A1(InvRinf); A1(OmegaInv); A1(Rmed2); A1(GlobalRmed); A1(SigmaMed); A1(SigmaInf);
   A1(EnergyMed); A1(VthetaMed); A1(WaveKiller); A1(globcsvec); A1(globtempervec); A1(Vmoy);
-  A1(VMed);
+  A1(VMed); A1(PebDensInit); A1(PebVradInit); A1(PebVthetaInit); A1(vt1D);
 #undef A1
+  o->pebbulkdens = p->pebblebulkdens / RHO2CGS;   /* Pebbles.c:55 */
   o->Vresidual = (double *)calloc((size_t)o->ns + 1, sizeof(double));
   o->Nshift = (int *)calloc((size_t)n, sizeof(int));
   o->sqrt_adiabind_inv = 1.0 / sqrt(p->adiabind);
@@ -282,6 +288,7 @@ void oracle_destroy(oracle_ctx *o) {
   free(o->Rmed2); free(o->GlobalRmed); free(o->SigmaMed); free(o->SigmaInf); free(o->EnergyMed);
   free(o->VthetaMed); free(o->WaveKiller); free(o->globcsvec); free(o->globtempervec); free(o->Vmoy);
   free(o->VMed); free(o->Vresidual); free(o->Nshift);
+  free(o->PebDensInit); free(o->PebVradInit); free(o->PebVthetaInit); free(o->vt1D);
   free(o);
 }
 
@@ -372,7 +379,11 @@ static int o_condition_cfl_core(oracle_ctx *o, double deltaT, double *newdt_out)
       invdt4 *= 4.0 * CVNR * CVNR;
       if (o->p.viscosity_alpha || (o->p.viscosity != 0.0))
         invdt5 = visc * 4. / pow(o_min2(dxrad, dxtheta), 2);
-      dt = CFLSECURITY / sqrt(invdt1 * invdt1 + invdt2 * invdt2 + invdt3 * invdt3 + invdt4 * invdt4 + invdt5 * invdt5 + o->invdtreb_sq);
+      if (!o->p.pebbles)
+        dt = CFLSECURITY / sqrt(invdt1 * invdt1 + invdt2 * invdt2 + invdt3 * invdt3 + invdt4 * invdt4 + invdt5 * invdt5 + o->invdtreb_sq);
+      else
+        dt = CFLSECURITY / sqrt(invdt1 * invdt1 + invdt2 * invdt2 + invdt3 * invdt3 + invdt4 * invdt4 + invdt5 * invdt5 +
+                                o->invdtpeb_sq + o->invdtreb_sq);
       if (dt < newdt) newdt = dt;
     }
   }
@@ -455,7 +466,17 @@ static void o_fill_forces(oracle_ctx *o, thorin_planets *sys) {
   double Hpl[THORIN_MAXPLANETS], s2[THORIN_MAXPLANETS], integpl[THORIN_MAXPLANETS];
   double rmorevert2[THORIN_MAXPLANETS], axpl[THORIN_MAXPLANETS], aypl[THORIN_MAXPLANETS];
   const double mt = o->st.mass_taper;
-  for (i = 0; i < (nr + 1) * ns; i++) { agr[i] = 0.0; agt[i] = 0.0; }
+  /* pebbles (Pframeforce.c:56-57, 73-78, 150-214, 250-330, 342-441, 451-455, 504-508) */
+  const int peb = o->p.pebbles, pgrav = o->p.pebble_gravity;
+  double *pagr = o->fld[W_PAGR], *pagt = o->fld[W_PAGT];
+  const double *prho = o->fld[THORIN_F_PEBBLEDENS], *tau = o->fld[THORIN_F_PEBBLESTOKES];
+  const double palpha = o->p.pebblealpha;
+  double tau_avr = 0.0, p_H_avr = 0.0, p_integstar_avr = 0.0, p_integstar = 0.0, pax = 0.0, pay = 0.0;
+  double pH = 0.0, Hcorr = 0.0, pmcell = 0.0, p_integpl[THORIN_MAXPLANETS];
+  for (i = 0; i < (nr + 1) * ns; i++) {
+    agr[i] = 0.0; agt[i] = 0.0;
+    if (peb) { pagr[i] = 0.0; pagt[i] = 0.0; }
+  }
   axstar = 0.0;
   aystar = 0.0;
   for (k = 0; k < npl; k++) {
@@ -492,9 +513,18 @@ static void o_fill_forces(oracle_ctx *o, thorin_planets *sys) {
     Rsup_Rinf2 *= Rsup_Rinf2;
     Nvert = 10;
     cs_avr = 0.0;
-    for (j = 0; j < ns; j++) cs_avr += cs[j + i * ns];
+    if (peb) tau_avr = 0.0;
+    for (j = 0; j < ns; j++) {
+      cs_avr += cs[j + i * ns];
+      if (peb) tau_avr += tau[j + i * ns];
+    }
     cs_avr /= (double)ns;
     H_avr = cs_avr * omegainv * o->sqrt_adiabind_inv;
+    if (peb) {
+      tau_avr /= (double)ns;
+      p_H_avr = H_avr * sqrt(palpha / tau_avr);
+      p_integstar_avr = 0.0;
+    }
     H2 = H_avr * H_avr;
     zmax = 3.0 * H_avr;
     deltaz = zmax / (double)Nvert;
@@ -509,12 +539,20 @@ static void o_fill_forces(oracle_ctx *o, thorin_planets *sys) {
       d = sqrt(d2);
       denom = d * d2;
       integstar_avr += sum / denom;
+      if (peb) {
+        znode2 *= palpha / tau_avr;
+        d2 = r2 + znode2;
+        d = sqrt(d2);
+        denom = d * d2;
+        p_integstar_avr += sum / denom;
+      }
     }
     for (j = 0; j < ns; j++) {
       IntegrateLocally = 0;
       Nvert = 10;
       ax = 0.0;
       ay = 0.0;
+      if (peb) { pax = 0.0; pay = 0.0; }
       l = j + i * ns;
       x = abs_[l];
       y = ord[l];
@@ -522,6 +560,12 @@ static void o_fill_forces(oracle_ctx *o, thorin_planets *sys) {
       H = cs[l] * omegainv * o->sqrt_adiabind_inv;
       H2 = H * H;
       if (fabs(H - H_avr) / H_avr > 0.05) IntegrateLocally = 1;
+      if (peb) {
+        pmcell = prho[l] * o->Surf[i];
+        Hcorr = sqrt(palpha / tau[l]);
+        pH = H * Hcorr;
+        if (fabs(pH - p_H_avr) / p_H_avr > 0.05) IntegrateLocally = 1;
+      }
       for (k = 0; k < npl; k++) {
         xpl = sys->x[k]; ypl = sys->y[k]; zpl = sys->z[k];
         s2[k] = (x - xpl) * (x - xpl) + (y - ypl) * (y - ypl);
@@ -533,6 +577,7 @@ static void o_fill_forces(oracle_ctx *o, thorin_planets *sys) {
           IntegrateLocally = 1;
           IntegrateWithPlanet[k] = 1;
           integpl[k] = 0.0;
+          if (peb) p_integpl[k] = 0.0;
           d2 = s2[k] + zpl * zpl;
           if (o->p.exclude_hill) hillcut[k] = 1.0 / (exp(-(sqrt(d2) / rhill[k] - o->p.hillcut) / o->p.hillcut * 10.0) + 1.0);
         } else {
@@ -545,6 +590,7 @@ static void o_fill_forces(oracle_ctx *o, thorin_planets *sys) {
         deltaz = zmax / (double)Nvert;
         sigma = 0.0;
         integstar = 0.0;
+        if (peb) p_integstar = 0.0;
         for (m = 1; m <= Nvert; m++) {
           znode = ((double)m - 0.5) * deltaz;
           znode2 = znode * znode;
@@ -554,6 +600,12 @@ static void o_fill_forces(oracle_ctx *o, thorin_planets *sys) {
           d = sqrt(d2);
           denom = d * d2;
           integstar += sum / denom;
+          if (peb) {
+            d2 = r2 + znode2 * Hcorr * Hcorr;
+            d = sqrt(d2);
+            denom = d * d2;
+            p_integstar += sum / denom;
+          }
           for (k = 0; k < npl; k++) {
             if (IntegrateWithPlanet[k]) {
               zpl = sys->z[k];
@@ -577,18 +629,46 @@ static void o_fill_forces(oracle_ctx *o, thorin_planets *sys) {
                 denom = d * d2;
                 integpl[k] += sum / denom * taper;
               }
+              if (peb) {
+                d2 = s2[k] + (znode * Hcorr - zpl) * (znode * Hcorr - zpl);
+                if (d2 < rst2[k]) {
+                  d = sqrt(d2);
+                  taper = -3.0 * d2 * d2 / rst4[k] + 4.0 * d * d2 / rst3[k];
+                } else taper = 1.0;
+                d2 += (rsm2[k] > Rsup_Rinf2 ? rsm2[k] : Rsup_Rinf2);
+                d = sqrt(d2);
+                denom = d * d2;
+                p_integpl[k] += sum / denom * taper;
+                if (zpl < -0.05 * pH || zpl > 0.05 * pH) {
+                  d2 = s2[k] + (-znode * Hcorr - zpl) * (-znode * Hcorr - zpl);
+                  if (d2 < rst2[k]) {
+                    d = sqrt(d2);
+                    taper = -3.0 * d2 * d2 / rst4[k] + 4.0 * d * d2 / rst3[k];
+                  } else taper = 1.0;
+                  d2 += (rsm2[k] > Rsup_Rinf2 ? rsm2[k] : Rsup_Rinf2);
+                  d = sqrt(d2);
+                  denom = d * d2;
+                  p_integpl[k] += sum / denom * taper;
+                }
+              }
             }
           }
         }
       } else {
         sigma = sigma_avr;
         integstar = integstar_avr;
+        if (peb) p_integstar = p_integstar_avr;
       }
       tmp = integstar / sigma;
       ar = -o->Rmed[i] * tmp;
       agr[l] += ar;
       axstar += mcell * tmp * x;
       aystar += mcell * tmp * y;
+      if (peb) {
+        tmp = p_integstar / sigma;
+        ar = -o->Rmed[i] * tmp;
+        pagr[l] += ar;
+      }
       for (k = 0; k < npl; k++) {
         xpl = sys->x[k]; ypl = sys->y[k]; zpl = sys->z[k]; mpl = sys->mass[k];
         if (IntegrateWithPlanet[k]) {
@@ -602,6 +682,22 @@ static void o_fill_forces(oracle_ctx *o, thorin_planets *sys) {
           ay += -mpl * tmp;
           if (o->p.exclude_hill) tmp *= hillcut[k];
           aypl[k] += mcell * tmp;
+          if (peb) {
+            if (zpl < -0.05 * pH || zpl > 0.05 * pH) tmp = (x - xpl) * p_integpl[k] / (2.0 * sigma);
+            else tmp = (x - xpl) * p_integpl[k] / sigma;
+            pax += -mpl * tmp;
+            if (pgrav) {
+              if (o->p.exclude_hill) tmp *= hillcut[k];
+              axpl[k] += pmcell * tmp;
+            }
+            if (zpl < -0.05 * pH || zpl > 0.05 * pH) tmp = (y - ypl) * p_integpl[k] / (2.0 * sigma);
+            else tmp = (y - ypl) * p_integpl[k] / sigma;
+            pay += -mpl * tmp;
+            if (pgrav) {
+              if (o->p.exclude_hill) tmp *= hillcut[k];
+              aypl[k] += pmcell * tmp;
+            }
+          }
         } else {
           mpl *= mt;
           d2 = s2[k] + zpl * zpl + smoothing2[k];
@@ -615,12 +711,35 @@ static void o_fill_forces(oracle_ctx *o, thorin_planets *sys) {
           ay += -mpl * tmp;
           if (o->p.exclude_hill) tmp *= hillcut[k];
           aypl[k] += mcell * tmp;
+          if (peb) {
+            d2 += -smoothing2[k] + smoothing2[k] * Hcorr * Hcorr;
+            d = sqrt(d2);
+            denom = d * d2;
+            tmp = (x - xpl) / denom;
+            pax += -mpl * tmp;
+            if (pgrav) {
+              if (o->p.exclude_hill) tmp *= hillcut[k];
+              axpl[k] += pmcell * tmp;
+            }
+            tmp = (y - ypl) / denom;
+            pay += -mpl * tmp;
+            if (pgrav) {
+              if (o->p.exclude_hill) tmp *= hillcut[k];
+              aypl[k] += pmcell * tmp;
+            }
+          }
         }
       }
       ar = (x * ax + y * ay) * o->InvRmed[i];
       at = (x * ay - y * ax) * o->InvRmed[i];
       agr[l] += ar;
       agt[l] += at;
+      if (peb) {
+        ar = (x * pax + y * pay) * o->InvRmed[i];
+        at = (x * pay - y * pax) * o->InvRmed[i];
+        pagr[l] += ar;
+        pagt[l] += at;
+      }
     }
   }
   for (k = 0; k < npl; k++) { sys->ax[k] = axpl[k]; sys->ay[k] = aypl[k]; }
@@ -636,6 +755,7 @@ static void o_fill_forces(oracle_ctx *o, thorin_planets *sys) {
       at = (x * ayindir - y * axindir) * o->InvRmed[i];
       agr[l] += ar;
       agt[l] += at;
+      if (peb) { pagr[l] += ar; pagt[l] += at; }
     }
   /* IndirectTerm (Pframeforce.c:468-470), used by AdvanceSystemFromDisk */
   o->indirect_x = -axstar;
@@ -807,6 +927,8 @@ static void o_substep1(oracle_ctx *o, double dt) {
   const double *rho = o->fld[THORIN_F_DENS], *vrad = o->fld[THORIN_F_VRAD], *vtheta = o->fld[THORIN_F_VTHETA];
   double *vradint = o->fld[THORIN_F_VRADINT], *vthetaint = o->fld[THORIN_F_VTHETAINT];
   const double *press = o->fld[THORIN_F_PRESSURE], *agr = o->fld[THORIN_F_AGR], *agt = o->fld[THORIN_F_AGT];
+  double *accr = o->fld[W_GACCR], *acct = o->fld[W_GACCT];
+  const double *dragr = o->fld[W_DRAGR], *dragt = o->fld[W_DRAGT];
   double gradp, vt2, dxtheta, tmp, invdxtheta, supp_torque;
   for (i = 1; i < nr; i++)
     for (j = 0; j < ns; j++) {
@@ -818,6 +940,14 @@ static void o_substep1(oracle_ctx *o, double dt) {
       vt2 = vt2 * vt2;
       tmp = -gradp + 0.5 * (agr[l] + agr[lim]) + vt2 * o->InvRinf[i];
       vradint[l] = vrad[l] + dt * tmp;
+      if (p->pebbles) {   /* SourceEuler.c:371-379 */
+        accr[l] = tmp;
+        if (p->back_reaction) {
+          tmp = dragr[l] * 2.0 / (rho[l] + rho[lim]);
+          vradint[l] += dt * tmp;
+          accr[l] += tmp;
+        }
+      }
     }
   for (i = 0; i < nr; i++) {
     supp_torque = p->imposeddiskdrift * .5 * pow(o->Rmed[i], -2.5 + p->sigmaslope);
@@ -830,6 +960,14 @@ static void o_substep1(oracle_ctx *o, double dt) {
       tmp = -gradp + 0.5 * (agt[l] + agt[ljm]);
       vthetaint[l] = vtheta[l] + dt * tmp;
       vthetaint[l] += dt * supp_torque;
+      if (p->pebbles) {   /* SourceEuler.c:397-405 */
+        acct[l] = tmp + supp_torque;
+        if (p->back_reaction) {
+          tmp = dragt[l] * 2.0 / (rho[l] + rho[ljm]);
+          vthetaint[l] += dt * tmp;
+          acct[l] += tmp;
+        }
+      }
     }
   }
   o_div_stress(o, vradint, vthetaint, rho);
@@ -1317,16 +1455,16 @@ static void o_vanleer_theta(oracle_ctx *o, const double *vt, double *qb, double 
   }
 }
 
-static void o_quantities_advection(oracle_ctx *o, const double *vt, double dt) {
-  /* QuantitiesAdvection, TransportEuler.c:159-175 */
-  double *rho = o->fld[THORIN_F_DENS];
+static void o_quantities_advection(oracle_ctx *o, double *rho, double *energy, const double *vt, double dt) {
+  /* QuantitiesAdvection, TransportEuler.c:159-175 (energy == NULL:
+   * QuantitiesAdvectionPebbles, 491-505) */
   o_star_theta(o, rho, vt, o->fld[W_RHOSTAR], dt);
   o_actualise(o, o->fld[W_RHOINT], rho);
   o_vanleer_theta(o, vt, o->fld[W_RADMOMP], dt);
   o_vanleer_theta(o, vt, o->fld[W_RADMOMM], dt);
   o_vanleer_theta(o, vt, o->fld[W_THETAMOMP], dt);
   o_vanleer_theta(o, vt, o->fld[W_THETAMOMM], dt);
-  if (o->p.energy_eq) o_vanleer_theta(o, vt, o->fld[THORIN_F_ENERGY], dt);
+  if (energy) o_vanleer_theta(o, vt, energy, dt);
   o_vanleer_theta(o, vt, rho, dt);
 }
 
@@ -1345,12 +1483,12 @@ static void o_advect_shift(oracle_ctx *o, double *val) {
   }
 }
 
-static void o_transport(oracle_ctx *o, double dt) {
+static void o_transport(oracle_ctx *o, double *rho, double *vr, double *vt, double *energy, double dt) {
   /* Transport, TransportEuler.c:34-47, with ComputeLRMomenta 318-347,
    * OneWindRad 49-65, OneWindTheta 138-156, ComputeResiduals 70-111,
-   * ComputeVelocities 350-381 */
+   * ComputeVelocities 350-381.  energy == NULL: TransportPebbles (449-505)
+   * for the pebble fluid (no energy, same momenta and shift) */
   int i, j, l, lip, ljp, lim, ljm, nr = o->nr, ns = o->ns;
-  double *rho = o->fld[THORIN_F_DENS], *vr = o->fld[THORIN_F_VRAD], *vt = o->fld[THORIN_F_VTHETA];
   double *rp = o->fld[W_RADMOMP], *rm = o->fld[W_RADMOMM], *tp = o->fld[W_THETAMOMP], *tm = o->fld[W_THETAMOMM];
   double *vtr = o->fld[W_VTHETARES];
   double of = o->st.omega_frame;
@@ -1370,7 +1508,7 @@ static void o_transport(oracle_ctx *o, double dt) {
   o_vanleer_rad(o, vr, rm, dt);
   o_vanleer_rad(o, vr, tp, dt);
   o_vanleer_rad(o, vr, tm, dt);
-  if (o->p.energy_eq) o_vanleer_rad(o, vr, o->fld[THORIN_F_ENERGY], dt);
+  if (energy) o_vanleer_rad(o, vr, energy, dt);
   o_vanleer_rad(o, vr, rho, dt);
   /* OneWindTheta: ComputeResiduals */
   {
@@ -1390,14 +1528,14 @@ static void o_transport(oracle_ctx *o, double dt) {
       }
     }
   }
-  o_quantities_advection(o, vtr, dt);
+  o_quantities_advection(o, rho, energy, vtr, dt);
   if (o->p.fast_transport) {
-    o_quantities_advection(o, vt, dt);
+    o_quantities_advection(o, rho, energy, vt, dt);
     o_advect_shift(o, rp);
     o_advect_shift(o, rm);
     o_advect_shift(o, tp);
     o_advect_shift(o, tm);
-    if (o->p.energy_eq) o_advect_shift(o, o->fld[THORIN_F_ENERGY]);
+    if (energy) o_advect_shift(o, energy);
     o_advect_shift(o, rho);
   }
   /* ComputeVelocities */
@@ -1411,6 +1549,479 @@ static void o_transport(oracle_ctx *o, double dt) {
     }
 }
 
+/* ---------------- pebble fluid (Pebbles.c) ---------------- */
+#define TRAPEZMAX 16           /* Pebbles.c:21 */
+#define TRAPEZEPS 1.0e-4       /* Pebbles.c:22 */
+#define EVAPOREDSIGMA 1.0e-8   /* Pebbles.c:835 */
+#define FLUX2CU (0.000003 / (2.0 * PI))   /* fondam.h:54 */
+
+/* PebbleStokesNumbers, Pebbles.c:272-293 (Epstein drag from the pebble size) */
+static void o_peb_stokes(oracle_ctx *o) {
+  int i, j, l, nr = o->nr, ns = o->ns;
+  const double *rho = o->fld[THORIN_F_DENS], *psize = o->fld[THORIN_F_PEBBLESIZE];
+  double *tau = o->fld[THORIN_F_PEBBLESTOKES];
+  for (i = 0; i < nr; i++)
+    for (j = 0; j < ns; j++) {
+      l = j + i * ns;
+      tau[l] = 2.0 * o->pebbulkdens * psize[l] / rho[l] * o->sqrt_adiabind_inv;
+    }
+}
+
+/* CriticalCharTime, Pebbles.c:735-770 */
+static void o_peb_cfl(oracle_ctx *o) {
+  int i, j, l, ns = o->ns;
+  const double *pvr = o->fld[THORIN_F_PEBBLEVRAD], *pvt = o->fld[THORIN_F_PEBBLEVTHETA];
+  const double *vr = o->fld[THORIN_F_VRAD], *vt = o->fld[THORIN_F_VTHETA];
+  double dxrad, dxtheta, pvtmed = 0.0, invdt1, invdt2, invdt3, invdt4, vres, temp;
+  o->invdtpeb_sq = -1.e30;
+  for (i = 1; i < o->nr; i++) {
+    dxrad = o->Rsup[i] - o->Rinf[i];
+    dxtheta = o->Rmed[i] * 2.0 * PI / (double)ns;
+    if (o->p.fast_transport) {
+      pvtmed = 0.0;
+      for (j = 0; j < ns; j++) pvtmed += pvt[j + i * ns];
+      pvtmed /= (double)ns;
+    }
+    for (j = 0; j < ns; j++) {
+      l = j + i * ns;
+      invdt1 = fabs(pvr[l]) / dxrad;
+      vres = o->p.fast_transport ? pvt[l] - pvtmed : pvt[l];
+      invdt2 = fabs(vres) / dxtheta;
+      invdt3 = fabs(pvr[l] - vr[l]) / dxrad;
+      invdt4 = fabs(pvt[l] - vt[l]) / dxtheta;
+      temp = invdt1 * invdt1 + invdt2 * invdt2 + invdt3 * invdt3 + invdt4 * invdt4;
+      o->invdtpeb_sq = o_max2(o->invdtpeb_sq, temp);
+    }
+  }
+}
+
+/* Trapzd, Pebbles.c:295-320: refinement n of the trapezoidal rule for
+ * exp(-z^2/(2 H2)) on [a,b]; *integral carries the previous refinement */
+static double o_trapzd(int n, double a, double b, double H2, double *integral) {
+  double iter, spacing, x, sum;
+  int j, it;
+  if (n == 1) {
+    *integral = 0.5 * (b - a) * (exp(-0.5 * a * a / H2) + exp(-0.5 * b * b / H2));
+    return *integral;
+  }
+  for (it = 1, j = 1; j < n - 1; j++) it <<= 1;
+  iter = it;
+  spacing = (b - a) / iter;
+  x = a + 0.5 * spacing;
+  sum = 0.0;
+  for (j = 1; j <= it; j++) {
+    sum += exp(-0.5 * x * x / H2);
+    x += spacing;
+  }
+  *integral = 0.5 * (*integral + (b - a) * sum / iter);
+  return *integral;
+}
+
+/* IntegrateColumnMass, Pebbles.c:325-342; returns -1 and sets *ok = 0 when
+ * the refinement does not converge (the reference exits) */
+static double o_column_mass(double a, double b, double Hpeb, int *ok) {
+  int n;
+  double integral, iold = 0.0, H2 = Hpeb * Hpeb, state = 0.0;
+  for (n = 1; n <= TRAPEZMAX; n++) {
+    integral = o_trapzd(n, a, b, H2, &state);
+    if (n > 5)
+      if (fabs(integral - iold) < TRAPEZEPS * fabs(iold) || (integral == 0.0 && iold == 0.0)) return integral;
+    iold = integral;
+  }
+  *ok = 0;
+  return 0.0;
+}
+
+/* GetGlobalIFrac, LowTasks.c:21-31 */
+static double o_global_ifrac(oracle_ctx *o, double r) {
+  int i = 0, nr = o->nr;
+  if (r < o->GlobalRmed[0]) return 0.0;
+  if (r > o->GlobalRmed[nr - 1]) return (double)nr - 1.0;
+  while (o->GlobalRmed[i] <= r) i++;
+  return (double)i + (r - o->GlobalRmed[i - 1]) / (o->GlobalRmed[i] - o->GlobalRmed[i - 1]) - 1.0;
+}
+
+/* AccretePebblesOntoPlanets, Pebbles.c:346-538 (single rank; the
+ * MPI_Allreduce sums are the local sums).  Returns -1 if a column
+ * integration does not converge. */
+static int o_peb_accrete(oracle_ctx *o, thorin_planets *sys, double dt) {
+  int nr = o->nr, ns = o->ns, k, i, j, l, imin, imax, ncell, ok = 1;
+  double *pdens = o->fld[THORIN_F_PEBBLEDENS], *macc = o->fld[W_MACC];
+  const double *abs_ = o->fld[W_ABS], *ord = o->fld[W_ORD], *pvr = o->fld[THORIN_F_PEBBLEVRAD];
+  const double *pvt = o->fld[THORIN_F_PEBBLEVTHETA], *tau = o->fld[THORIN_F_PEBBLESTOKES];
+  const double *cs = o->fld[THORIN_F_SOUNDSPEED];
+  const double of = o->st.omega_frame;
+  double dMplanet, dPXplanet, dPYplanet, Xplanet, Yplanet, Zplanet, VXplanet, VYplanet, VZplanet;
+  double Mplanet, Msolid, PXplanet, PYplanet, Rplanet, Rplanet3D, Omegaplanet, Rhill;
+  double vtcorr, xcell, ycell, vrcell, vtcell, vxcell, vycell, flow, flowx, flowy, vrel;
+  double ifrac, frac, Mt, Rbondi, tbondi, fac, Reff = 0.0, Reff2, R2, H, Hpeb, zmin, zmax, dM, mcell;
+  double voldenspeb, tau_mean, dM_upper, Hpeb_mean, pdens_mean, veloc, dM_model, limiter;
+  o_ring_mean(o, o->fld[THORIN_F_VTHETA], o->vt1D);
+  for (k = 0; k < sys->nb; k++) {
+    dMplanet = dPXplanet = dPYplanet = 0.0;
+    Xplanet = sys->x[k]; Yplanet = sys->y[k]; Zplanet = sys->z[k];
+    VXplanet = sys->vx[k]; VYplanet = sys->vy[k]; VZplanet = sys->vz[k];
+    Mplanet = sys->mass[k];
+    Msolid = sys->solid[k];
+    PXplanet = VXplanet * Mplanet;
+    PYplanet = VYplanet * Mplanet;
+    Rplanet = Xplanet * Xplanet + Yplanet * Yplanet;
+    Rplanet3D = sqrt(Rplanet + Zplanet * Zplanet);
+    Rplanet = sqrt(Rplanet);
+    Rhill = Rplanet3D * pow(1.0 / 3.0 * Mplanet, 1.0 / 3.0);
+    Omegaplanet = pow(Rplanet, -1.5);
+    ifrac = o_global_ifrac(o, Rplanet);
+    frac = ifrac - floor(ifrac);
+    i = (int)ifrac;
+    flow = o->vt1D[i] * (1.0 - frac) + o->vt1D[i + 1] * frac + Rplanet * of;
+    flowx = -flow * Yplanet / Rplanet;
+    flowy = flow * Xplanet / Rplanet;
+    vrel = sqrt((VXplanet - flowx) * (VXplanet - flowx) + (VYplanet - flowy) * (VYplanet - flowy) + VZplanet * VZplanet);
+    Mt = sqrt(1.0 / 3.0) * pow(vrel, 3.0) / Omegaplanet;
+    imax = nr - 1;
+    imin = 0;
+    while (o->Rinf[imax] > (Rplanet + Rhill) && imax >= imin) imax--;
+    while (o->Rsup[imin] < (Rplanet - Rhill) && imin <= imax) imin++;
+    /* step 1: mean Stokes number over the annulus of the Hill sphere */
+    ncell = 0;
+    tau_mean = 0.0;
+    for (i = imin; i <= imax; i++)
+      for (j = 0; j < ns; j++) { tau_mean += tau[j + i * ns]; ncell++; }
+    tau_mean /= (double)ncell;
+    /* step 2: effective accretion radius */
+    if (Mplanet < Mt) {
+      Rbondi = Mplanet / vrel / vrel;
+      tbondi = Rbondi / vrel;
+      fac = sqrt(tau_mean * pow(Rplanet, 1.5) / tbondi);
+      Reff = Rbondi * fac;
+    } else {
+      fac = pow(tau_mean / 0.1, 1.0 / 3.0);
+      Reff = Rhill * fac;
+    }
+    if (Reff > Rhill) Reff = Rhill;
+    Reff2 = Reff * Reff;
+    /* step 3: upper limit of the accreted mass, cell by cell */
+    dM_upper = 0.0;
+    Hpeb_mean = 0.0;
+    pdens_mean = 0.0;
+    ncell = 0;
+    for (i = imin; i <= imax; i++) {
+      vtcorr = o->Rmed[i] * of;
+      (void)vtcorr;
+      for (j = 0; j < ns; j++) {
+        l = j + i * ns;
+        xcell = abs_[l];
+        ycell = ord[l];
+        R2 = (xcell - Xplanet) * (xcell - Xplanet) + (ycell - Yplanet) * (ycell - Yplanet);
+        if (R2 <= Reff2) {
+          H = cs[l] * o->OmegaInv[i] * o->sqrt_adiabind_inv;
+          Hpeb = H * sqrt(o->p.pebblealpha / tau[l]);
+          zmin = Zplanet - sqrt(Reff2 - R2);
+          zmax = Zplanet + sqrt(Reff2 - R2);
+          mcell = o->Surf[i] * pdens[l];
+          voldenspeb = pdens[l] / Hpeb * SQRT2PI_INV;
+          dM = voldenspeb * o->Surf[i] * o_column_mass(zmin, zmax, Hpeb, &ok);
+          if (!ok) return -1;
+          if (dM > mcell) dM = 0.99999999 * mcell;
+          macc[l] = dM;
+          dM_upper += dM;
+          Hpeb_mean += Hpeb;
+          pdens_mean += pdens[l];
+          ncell++;
+        }
+      }
+    }
+    if (ncell > 0) {
+      Hpeb_mean /= (double)ncell;
+      pdens_mean /= (double)ncell;
+    }
+    /* step 4: model accretion rate (2D/3D, Bondi/Hill) limits the upper bound */
+    if (Mplanet < Mt) veloc = vrel;
+    else veloc = Reff * Omegaplanet;
+    if (Hpeb_mean < Reff) dM_model = 2.0 * Reff * veloc * pdens_mean;
+    else dM_model = PI * Reff * Reff * veloc * pdens_mean / Hpeb_mean * SQRT2PI_INV;
+    dM_model *= dt;
+    if (dM_upper > dM_model) limiter = dM_model / dM_upper;
+    else limiter = 1.0;
+    /* step 5: remove the mass from the cells, collect mass and momentum */
+    for (i = imin; i <= imax; i++) {
+      vtcorr = o->Rmed[i] * of;
+      for (j = 0; j < ns; j++) {
+        l = j + i * ns;
+        if (macc[l] > 0.0) {
+          xcell = abs_[l];
+          ycell = ord[l];
+          vrcell = pvr[l];
+          vtcell = pvt[l] + vtcorr;
+          vxcell = (vrcell * xcell - vtcell * ycell) * o->InvRmed[i];
+          vycell = (vrcell * ycell + vtcell * xcell) * o->InvRmed[i];
+          dM = macc[l] * limiter;
+          macc[l] = 0.0;
+          mcell = o->Surf[i] * pdens[l];
+          pdens[l] = (mcell - dM) / o->Surf[i];
+          if (pdens[l] < 0.0) pdens[l] = 1.e-20;
+          dMplanet += dM;
+          dPXplanet += dM * vxcell;
+          dPYplanet += dM * vycell;
+        }
+      }
+    }
+    /* step 7: update the planet */
+    PXplanet += dPXplanet;
+    PYplanet += dPYplanet;
+    Mplanet += dMplanet;
+    Msolid += dMplanet;
+    sys->mass[k] = Mplanet;
+    sys->solid[k] = Msolid;
+    sys->vx[k] = PXplanet / Mplanet;
+    sys->vy[k] = PYplanet / Mplanet;
+  }
+  return 0;
+}
+
+/* EvaporatePebbles, Pebbles.c:837-866 */
+static void o_peb_evaporate(oracle_ctx *o, double dt) {
+  int i, j, l;
+  double facc, evaptemp, *dens = o->fld[THORIN_F_PEBBLEDENS];
+  const double *temp = o->fld[THORIN_F_TEMPER];
+  if (o->p.evaporationrate <= 0.0) return;
+  facc = o->p.evaporationrate * dt;
+  if (facc > 1.0) facc = 1.0;
+  evaptemp = o->p.evaporationtemperature / T2SI;
+  for (i = 0; i < o->nr; i++)
+    for (j = 0; j < o->ns; j++) {
+      l = j + i * o->ns;
+      if (temp[l] > evaptemp) {
+        dens[l] *= (1.0 - facc);
+        if (dens[l] < EVAPOREDSIGMA) dens[l] = EVAPOREDSIGMA;
+      }
+    }
+}
+
+/* DampPebbles, SideEuler.c:423-454 */
+static void o_peb_damp(oracle_ctx *o, double dt) {
+  int i, j, l, ns = o->ns;
+  double DRin = o->p.rmin * o->p.dampingrminfrac, DRout = o->p.rmax * o->p.dampingrmaxfrac, lambda;
+  double pdens0, pvr0, pvt0;
+  double *pdens = o->fld[THORIN_F_PEBBLEDENS], *pvr = o->fld[THORIN_F_PEBBLEVRAD], *pvt = o->fld[THORIN_F_PEBBLEVTHETA];
+  for (i = 0; i < o->nr; i++) {
+    if (o->Rmed[i] < DRin || o->Rmed[i] > DRout) {
+      lambda = o->WaveKiller[i] * dt;
+      pdens0 = o->PebDensInit[i];
+      pvr0 = o->PebVradInit[i];
+      pvt0 = o->PebVthetaInit[i] - o->Rmed[i] * o->st.omega_frame;
+      for (j = 0; j < ns; j++) {
+        l = j + i * ns;
+        pdens[l] = (pdens[l] + lambda * pdens0) / (1.0 + lambda);
+        pvr[l] = (pvr[l] + lambda * pvr0) / (1.0 + lambda);
+        pvt[l] = (pvt[l] + lambda * pvt0) / (1.0 + lambda);
+      }
+    }
+  }
+}
+
+/* SourceTermsPebbles, Pebbles.c:549-612 */
+static void o_peb_source_terms(oracle_ctx *o, double dt) {
+  int i, j, l, lim, ljm, ljp, nr = o->nr, ns = o->ns;
+  const double *pvr = o->fld[THORIN_F_PEBBLEVRAD], *pvt = o->fld[THORIN_F_PEBBLEVTHETA];
+  const double *pdens = o->fld[THORIN_F_PEBBLEDENS], *tau = o->fld[THORIN_F_PEBBLESTOKES];
+  const double *vr = o->fld[THORIN_F_VRAD], *vt = o->fld[THORIN_F_VTHETA];
+  const double *pagr = o->fld[W_PAGR], *pagt = o->fld[W_PAGT];
+  double *paccr = o->fld[W_PACCR], *pacct = o->fld[W_PACCT], *dragr = o->fld[W_DRAGR], *dragt = o->fld[W_DRAGT];
+  double Omega, vt2, drag;
+  o_peb_damp(o, dt);
+  for (i = 1; i < nr; i++) {
+    Omega = 1.0 / o->OmegaInv[i];
+    for (j = 0; j < ns; j++) {
+      l = j + i * ns; lim = l - ns; ljp = l + 1;
+      if (j == ns - 1) ljp = i * ns;
+      vt2 = pvt[l] + pvt[ljp] + pvt[lim] + pvt[ljp - ns];
+      vt2 = vt2 / 4.0 + o->Rinf[i] * o->st.omega_frame;
+      vt2 = vt2 * vt2;
+      paccr[l] = vt2 * o->InvRinf[i] + 0.5 * (pagr[l] + pagr[lim]);
+      if (o->p.back_reaction) {
+        drag = Omega * (pvr[l] - vr[l]) / tau[l];
+        dragr[l] = 0.5 * (pdens[l] + pdens[lim]) * drag;
+      }
+    }
+  }
+  for (i = 0; i < nr; i++) {
+    Omega = 1.0 / o->OmegaInv[i];
+    for (j = 0; j < ns; j++) {
+      l = j + i * ns; ljm = l - 1;
+      if (j == 0) ljm = i * ns + ns - 1;
+      pacct[l] = 0.5 * (pagt[l] + pagt[ljm]);
+      if (o->p.back_reaction) {
+        drag = Omega * (pvt[l] - vt[l]) / tau[l];
+        dragt[l] = 0.5 * (pdens[l] + pdens[ljm]) * drag;
+      }
+    }
+  }
+}
+
+/* SubStep1Pebbles, Pebbles.c:616-650 (semi-implicit drag step; vr, vt are
+ * the gas velocities before SubStep1's update) */
+static void o_peb_substep1(oracle_ctx *o, double dt) {
+  int i, j, l, nr = o->nr, ns = o->ns;
+  double *pvr = o->fld[THORIN_F_PEBBLEVRAD], *pvt = o->fld[THORIN_F_PEBBLEVTHETA];
+  const double *tau = o->fld[THORIN_F_PEBBLESTOKES], *paccr = o->fld[W_PACCR], *pacct = o->fld[W_PACCT];
+  const double *accr = o->fld[W_GACCR], *acct = o->fld[W_GACCT];
+  const double *vr = o->fld[THORIN_F_VRAD], *vt = o->fld[THORIN_F_VTHETA];
+  double Omega, efac;
+  for (i = 0; i < nr; i++) {
+    Omega = 1.0 / o->OmegaInv[i];
+    for (j = 0; j < ns; j++) {
+      l = j + i * ns;
+      efac = exp(-dt * Omega / tau[l]);
+      if (i > 0)
+        pvr[l] = pvr[l] * efac + accr[l] * dt + (vr[l] + (paccr[l] - accr[l]) * tau[l] / Omega) * (1.0 - efac);
+      pvt[l] = pvt[l] * efac + acct[l] * dt + (vt[l] + (pacct[l] - acct[l]) * tau[l] / Omega) * (1.0 - efac);
+    }
+  }
+}
+
+/* ParticleDiffusion, Pebbles.c:654-695 */
+static void o_peb_diffusion(oracle_ctx *o) {
+  int i, j, l, lim, ljm, nr = o->nr, ns = o->ns;
+  double corr, dxtheta, invdxtheta;
+  const double *rho = o->fld[THORIN_F_DENS], *pdens = o->fld[THORIN_F_PEBBLEDENS];
+  double *pvr = o->fld[THORIN_F_PEBBLEVRAD], *pvt = o->fld[THORIN_F_PEBBLEVTHETA];
+  for (i = 1; i < nr; i++)
+    for (j = 0; j < ns; j++) {
+      l = j + i * ns; lim = l - ns;
+      corr = -o_fviscosity(o, o->Rinf[i]) / o->p.schmidtnumber;
+      corr *= (rho[l] + rho[lim]) / (pdens[l] + pdens[lim]);
+      corr *= (pdens[l] / rho[l] - pdens[lim] / rho[lim]) * o->InvDiffRmed[i];
+      pvr[l] += corr;
+    }
+  for (i = 0; i < nr; i++) {
+    dxtheta = 2.0 * PI / (double)ns * o->Rmed[i];
+    invdxtheta = 1.0 / dxtheta;
+    for (j = 0; j < ns; j++) {
+      l = j + i * ns; ljm = l - 1;
+      if (j == 0) ljm = i * ns + ns - 1;
+      corr = -o_fviscosity(o, o->Rmed[i]) / o->p.schmidtnumber;
+      corr *= (rho[l] + rho[ljm]) / (pdens[l] + pdens[ljm]);
+      corr *= (pdens[l] / rho[l] - pdens[ljm] / rho[ljm]) * invdxtheta;
+      pvt[l] += corr;
+    }
+  }
+}
+
+/* EvolvePebbleDisk, Pebbles.c:699-708 (TransportPebbles: TransportEuler.c:449-505) */
+static void o_peb_evolve(oracle_ctx *o, double dt) {
+  o_peb_damp(o, dt);
+  o_transport(o, o->fld[THORIN_F_PEBBLEDENS], o->fld[THORIN_F_PEBBLEVRAD], o->fld[THORIN_F_PEBBLEVTHETA], NULL, dt);
+  o_peb_damp(o, dt);
+}
+
+/* EtaPressureSupport (Pebbles.c:236-270) and InitPebblesViaFlux
+ * (Pebbles.c:79-202): the coagulation-drift equilibrium pebble disk of a
+ * fresh run (EquilPebbleDisk, Pebbles.c:59-64) */
+static void o_peb_equilibrium(oracle_ctx *o) {
+  const thorin_params *p = &o->p;
+  int i, j, l, lim, ljp, nr = o->nr, ns = o->ns;
+  double *etafc = o->fld[W_WORK], *etacc = o->fld[THORIN_F_ETA];
+  const double *vt = o->fld[THORIN_F_VTHETA], *rho = o->fld[THORIN_F_DENS], *vr = o->fld[THORIN_F_VRAD];
+  const double *cs = o->fld[THORIN_F_SOUNDSPEED];
+  double *pdens = o->fld[THORIN_F_PEBBLEDENS], *tau = o->fld[THORIN_F_PEBBLESTOKES];
+  double *psize = o->fld[THORIN_F_PEBBLESIZE], *pvr = o->fld[THORIN_F_PEBBLEVRAD];
+  double *pvt = o->fld[THORIN_F_PEBBLEVTHETA];
+  const double of = o->st.omega_frame, pebbleflux = p->pebbleflux * FLUX2CU, Ndf = 0.5;
+  double vtcorr, vkinv, vtcc;
+  for (i = 0; i < nr; i++) {
+    vtcorr = o->Rmed[i] * of;
+    vkinv = sqrt(o->Rmed[i]);
+    for (j = 0; j < ns; j++) {
+      l = j + i * ns; ljp = l + 1;
+      if (j == ns - 1) ljp = i * ns;
+      vtcc = 0.5 * (vt[l] + vt[ljp]);
+      etacc[l] = 1.0 - (vtcc + vtcorr) * vkinv;
+    }
+  }
+  for (i = 1; i < nr; i++)
+    for (j = 0; j < ns; j++) {
+      l = j + i * ns; lim = l - ns;
+      etafc[l] = 0.5 * (etacc[l] + etacc[lim]);
+    }
+  for (j = 0; j < ns; j++) etafc[j] = etafc[j + ns];
+  for (i = 0; i < nr; i++) {
+    double rho1 = 0.0, vr1 = 0.0, etafc1 = 0.0, etacc1 = 0.0, cs1 = 0.0;
+    double tau1, psize1, taudf, adf, taufrag, afrag, vkepl, fac, etaterm, pvr1, pvrtemp, pvt1, pdens1;
+    for (j = 0; j < ns; j++) {
+      l = j + i * ns;
+      rho1 += rho[l]; vr1 += vr[l]; etafc1 += etafc[l]; etacc1 += etacc[l]; cs1 += cs[l];
+    }
+    rho1 /= (double)ns; vr1 /= (double)ns; etafc1 /= (double)ns; etacc1 /= (double)ns; cs1 /= (double)ns;
+    tau1 = sqrt(3.0) * p->pebblecoagulation * pebbleflux / (32.0 * PI * pow(o->Rmed[i], -1.5) * rho1);
+    tau1 = sqrt(tau1) / (o->Rmed[i] * etacc1);
+    psize1 = fabs(tau1) * sqrt(p->adiabind) * rho1 / (2.0 * o->pebbulkdens);
+    taudf = p->fragmentthreshold * sqrt(o->Rmed[i]) / (2.0 * etacc1 * (1.0 - Ndf));
+    adf = fabs(taudf) * sqrt(p->adiabind) * rho1 / (2.0 * o->pebbulkdens);
+    taufrag = p->fragmentfactor / (3.0 * p->fragmentalpha) * pow(p->fragmentthreshold / cs1, 2);
+    afrag = fabs(taufrag) * sqrt(p->adiabind) * rho1 / (2.0 * o->pebbulkdens);
+    if ((psize1 < adf) && (psize1 < afrag)) {
+    } else if ((taudf > 0.0) && (adf < psize1) && (adf < afrag)) {
+      tau1 = taudf; psize1 = adf;
+    } else {
+      tau1 = taufrag; psize1 = afrag;
+    }
+    vtcorr = o->Rmed[i] * of;
+    vkepl = pow(o->Rmed[i], -0.5);
+    fac = 1.0 / (tau1 * tau1 + 1.0);
+    etaterm = etafc1 * vkepl;
+    pvr1 = -2.0 * tau1 * fac * (etaterm - 0.5 / tau1 * vr1);
+    etaterm = etacc1 * vkepl;
+    pvrtemp = -2.0 * tau1 * fac * (etaterm - 0.5 / tau1 * vr1);
+    pvt1 = -fac * (etaterm - 0.5 * tau1 * vr1);
+    pvt1 += (vkepl - vtcorr);
+    pdens1 = pebbleflux / (2.0 * PI * o->Rmed[i] * fabs(pvrtemp));
+    if (pdens1 / rho1 > p->pebbletogasmax) pdens1 = rho1 * p->pebbletogasmax;
+    for (j = 0; j < ns; j++) {
+      l = j + i * ns;
+      psize[l] = psize1; tau[l] = tau1; pdens[l] = pdens1; pvr[l] = pvr1; pvt[l] = pvt1;
+    }
+    o->PebDensInit[i] = pdens1;
+    o->PebVradInit[i] = pvr1;
+    o->PebVthetaInit[i] = pvt1 + o->Rmed[i] * of;
+  }
+}
+
+/* BckpFieldsForBC, Pebbles.c:206-232 (damping targets of a restarted run) */
+static void o_peb_backup(oracle_ctx *o) {
+  int i, j, l, ns = o->ns;
+  const double *pdens = o->fld[THORIN_F_PEBBLEDENS], *pvr = o->fld[THORIN_F_PEBBLEVRAD];
+  const double *pvt = o->fld[THORIN_F_PEBBLEVTHETA];
+  for (i = 0; i < o->nr; i++) {
+    o->PebDensInit[i] = 0.0; o->PebVradInit[i] = 0.0; o->PebVthetaInit[i] = 0.0;
+    for (j = 0; j < ns; j++) {
+      l = j + i * ns;
+      o->PebDensInit[i] += pdens[l]; o->PebVradInit[i] += pvr[l]; o->PebVthetaInit[i] += pvt[l];
+    }
+    o->PebDensInit[i] /= (double)ns;
+    o->PebVradInit[i] /= (double)ns;
+    o->PebVthetaInit[i] /= (double)ns;
+    o->PebVthetaInit[i] += o->Rmed[i] * o->st.omega_frame;
+  }
+}
+
+/* Pebble disk set-up (Init.c:138-144): mode 0 = equilibrium disk of a fresh
+ * run from the current gas state; mode 1 = restart from the given pebble
+ * density, velocities and size (RestartPebbleDisk, Pebbles.c:67-75). */
+int oracle_init_pebbles(oracle_ctx *o, int mode, const double *pdens, const double *pvrad,
+                        const double *pvtheta, const double *psize) {
+  size_t n = (size_t)o->nr * o->ns;
+  if (!o->p.pebbles) return -1;
+  if (mode == 0) { o_peb_equilibrium(o); return 0; }
+  if (mode != 1 || !pdens || !pvrad || !pvtheta || !psize) return -1;
+  memcpy(o->fld[THORIN_F_PEBBLEDENS], pdens, n * sizeof(double));
+  memcpy(o->fld[THORIN_F_PEBBLEVRAD], pvrad, n * sizeof(double));
+  memcpy(o->fld[THORIN_F_PEBBLEVTHETA], pvtheta, n * sizeof(double));
+  memcpy(o->fld[THORIN_F_PEBBLESIZE], psize, n * sizeof(double));
+  o_peb_backup(o);
+  return 0;
+}
+
 /* ---------------- AlgoGas (SourceEuler.c:210-317) ---------------- */
 int oracle_algogas(oracle_ctx *o, thorin_planets *sys, thorin_nbody_fn nbody, void *user,
                    thorin_step_log *log) {
@@ -1422,9 +2033,15 @@ int oracle_algogas(oracle_ctx *o, thorin_planets *sys, thorin_nbody_fn nbody, vo
   while (dtemp < 0.999999999 * DT) {
     double mt = o->st.physical_time / (p->masstaper * 2.0 * M_PI);
     o->st.mass_taper = (mt > 1.0 ? 1.0 : pow(sin(mt * M_PI / 2.0), 2.0));
+    if (p->pebbles) o_peb_stokes(o);
     o->invdtreb_sq = o_minstep_rebound(sys);
+    if (p->pebbles) o_peb_cfl(o);
     N = o_condition_cfl_core(o, DT - dtemp, &newdt);
     dt = (DT - dtemp) / (double)N;
+    if (p->pebbles) {
+      if (o_peb_accrete(o, sys, dt)) return -4;
+      o_peb_evaporate(o, dt);
+    }
     dtemp += dt;
     o_fill_forces(o, sys);
     o_advance_from_disk(o, sys, dt);
@@ -1432,9 +2049,15 @@ int oracle_algogas(oracle_ctx *o, thorin_planets *sys, thorin_nbody_fn nbody, vo
     o_boundary(o, dt);
     crashed = o_detect_crash(o, o->fld[THORIN_F_DENS]);
     crashed = o_detect_crash(o, o->fld[THORIN_F_ENERGY]);
+    if (p->pebbles) crashed = o_detect_crash(o, o->fld[THORIN_F_PEBBLEDENS]);
     if (crashed) o->st.already_crashed++;
     o_pressure(o, o->fld[THORIN_F_DENS], o->fld[THORIN_F_ENERGY]);
+    if (p->pebbles) o_peb_source_terms(o, dt);
     o_substep1(o, dt);
+    if (p->pebbles) {
+      o_peb_substep1(o, dt);
+      if (p->diffusive_particles) o_peb_diffusion(o);
+    }
     o_substep2(o, dt);
     o_actualise(o, o->fld[THORIN_F_VRAD], o->fld[THORIN_F_VRADNEW]);
     o_actualise(o, o->fld[THORIN_F_VTHETA], o->fld[THORIN_F_VTHETANEW]);
@@ -1446,10 +2069,12 @@ int oracle_algogas(oracle_ctx *o, thorin_planets *sys, thorin_nbody_fn nbody, vo
       if (niter < 0) rc = -3;
       o_actualise(o, o->fld[THORIN_F_ENERGY], o->fld[THORIN_F_ENERGYNEW]);
     }
-    o_transport(o, dt);
+    o_transport(o, o->fld[THORIN_F_DENS], o->fld[THORIN_F_VRAD], o->fld[THORIN_F_VTHETA],
+                p->energy_eq ? o->fld[THORIN_F_ENERGY] : NULL, dt);
     o_boundary(o, dt);
     o_temperature(o, o->fld[THORIN_F_DENS], o->fld[THORIN_F_ENERGY]);
     if (p->energy_eq) o_sound_speed(o, o->fld[THORIN_F_DENS], o->fld[THORIN_F_ENERGY]);
+    if (p->pebbles) o_peb_evolve(o, dt);
     o->st.physical_time += dt;
     if (log && log->nsub < THORIN_MAXSUB) {
       log->cfl_n[log->nsub] = N;
@@ -1505,6 +2130,9 @@ int oracle_get_1d(oracle_ctx *o, int which, double *out, size_t count) {
     case 12: src = o->VthetaMed; break;
     case 13: for (k = 0; k < count; k++) out[k] = (double)o->Nshift[k]; return 0;
     case 14: src = o->VMed; break;
+    case 15: src = o->PebDensInit; break;
+    case 16: src = o->PebVradInit; break;
+    case 17: src = o->PebVthetaInit; break;
     default: return -1;
   }
   memcpy(out, src, count * sizeof(double));

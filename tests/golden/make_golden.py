@@ -30,7 +30,8 @@ from fargo_thorin_amd.seed import seeded_disk, write_cfg  # noqa: E402
 REF = os.path.join(ROOT, "oracle", "_ref", "thorin")
 OUT = os.path.join(ROOT, "tests", "golden")
 
-FIELDS = ["dens", "vrad", "vtheta", "temper", "energy", "divv", "qplus", "opacity"]
+FIELDS = ["dens", "vrad", "vtheta", "temper", "energy", "divv", "qplus", "opacity",
+          "pebbledens", "pebblevrad", "pebblevtheta", "pebblesize", "pebblestokes"]
 
 # name: (config keys, planets, keep full fields?)
 CASES = {
@@ -50,6 +51,13 @@ CASES = {
     # a 10 Earth-mass planet (q = 3e-5) that does not feel the disk
     "planet_q3e-5_48x96": (configs.with_grid(configs.PLAIN_128x256, 48, 96, Ntot=8, Ninterm=4, WriteEnergy="YES"),
                            [(3e-5, 1.0)], True),
+    # two-fluid disk: equilibrium pebble disk of a fresh run (coagulation-drift
+    # model), drag back-reaction, particle diffusion, a q = 3e-5 planet that
+    # feels the disk and accretes pebbles, evaporation above 1000 K
+    "pebbles_q3e-5_64x128": (configs.with_grid(configs.DAMPED_1024x1536, 64, 128, Ntot=10, Ninterm=5,
+                                               PebbleAccretion="YES", BackReaction="YES",
+                                               PlanetsFeelDisk="YES", WriteEnergy="YES"),
+                             [(3e-5, 1.0)], True),
     # the 128x256 / 200-step workload itself: checksums and ring profiles only
     "plain_128x256_200": (dict(configs.PLAIN_128x256, Ninterm="100", WriteEnergy="YES"),
                           configs.PLANET_ZERO_MASS, False),

@@ -24,21 +24,30 @@ def params_for(cfg: dict, **extra) -> Params:
     return Params(**keys)
 
 
-def planets_for(planets):
-    return make_planets([m for m, a in planets], [a for m, a in planets])
+def planets_for(planets, cfg=None):
+    feel = bool(cfg) and str(cfg.get("PlanetsFeelDisk", "NO"))[:1] in "yY"
+    return make_planets([m for m, a in planets], [a for m, a in planets], feel_disk=feel)
 
 
 class Replay:
     """N-body callback that moves the planets along the reference's own
     trajectory (tqwk0.dat rows: position at the start of each DT).  One
-    substep per DT is assumed, which the golden runs have."""
+    substep per DT is assumed, which the golden runs have.  With integrate,
+    the library's own N-body step advances the planets first, so that the
+    velocities (which the log does not hold) follow the orbit to round-off;
+    the positions are then replaced by the logged ones."""
 
-    def __init__(self, tq):
+    def __init__(self, tq, integrate: bool = False):
         self.tq = np.asarray(tq)
         self.n = 0
         self.cb = NBODY_FN(self._step)
+        self.nb = host_nbody_fn() if integrate else None
 
     def _step(self, user, sysp, t, dt, angle):
+        if self.nb is not None:
+            rc = self.nb(user, sysp, t, dt, angle)
+            if rc:
+                return rc
         self.n += 1
         if self.n < len(self.tq):
             sysp.contents.x[0] = self.tq[self.n, 3]

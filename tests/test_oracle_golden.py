@@ -85,3 +85,44 @@ def test_oracle_plain_128x256_200_steps_checksums():
         if i == ntot:
             break
         o.algogas(sysp, rep.cb)
+
+
+PEB_CASE = "pebbles_q3e-5_64x128"
+PEB_FIELDS = ["pebbledens", "pebblevrad", "pebblevtheta"]
+
+
+def test_oracle_pebbles_bitwise():
+    """Two-fluid run with a q = 3e-5 planet that feels the disk and accretes
+    pebbles: the equilibrium pebble disk of a fresh run (sizes, Stokes
+    numbers, density, velocities), then every gas and pebble output field,
+    the SOR iterations and PhysicalTime equal the reference's bit for bit."""
+    cfg, planets, _ = case(PEB_CASE)
+    g = golden(PEB_CASE)
+    o = OracleDisk(params_for(cfg))
+    o.init(g["init_dens"], g["init_vrad"], g["init_vtheta"], g["init_temper"])
+    o.init_pebbles(0)
+    for f in PEB_FIELDS + ["pebblesize", "pebblestokes"]:
+        assert np.array_equal(o.field(f), g[f"{f}0"]), f"initial {f}"
+    sysp = planets_for(planets, cfg)
+    m0 = sysp.mass[0]
+    rep = Replay(g["tqwk0"], integrate=True)
+    ntot, ninterm = int(g["ntot"]), int(g["ninterm"])
+    niters, times = [], [o.state().physical_time]
+    checked = 0
+    for i in range(ntot):
+        lg = o.algogas(sysp, rep.cb)
+        niters += [lg.sor_niter[k] for k in range(lg.nsub)]
+        times.append(o.state().physical_time)
+        if (i + 1) % ninterm == 0:
+            n = (i + 1) // ninterm
+            for f in FIELDS + PEB_FIELDS:
+                key = f"{f}{n}"
+                if key in g:
+                    a = o.field(f)
+                    assert np.array_equal(a, g[key]), f"{key}: {np.count_nonzero(a != g[key])} cells differ"
+                    checked += 1
+    assert checked >= 8
+    assert niters == list(g["sor_niter"][:len(niters)])
+    assert np.array_equal(np.array(times[:ntot]), g["tqwk0"][:ntot, 0])
+    # the planet accreted pebbles
+    assert sysp.mass[0] > m0 and sysp.solid[0] > 0.0
