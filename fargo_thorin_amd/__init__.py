@@ -21,7 +21,7 @@ import numpy as np
 
 from . import _capi
 from ._capi import FIELD_ID, FIELDS, OPACITY, Planets, State, StepLog, last_error, lib
-from .seed import seeded_disk, write_cfg
+from .seed import seeded_disk, seeded_pebbles, write_cfg
 
 __all__ = ["Params", "Disk", "Planets", "State", "StepLog", "seeded_disk", "write_cfg", "FIELDS",
            "OPACITY", "ThorinError", "make_planets"]
@@ -124,6 +124,21 @@ class Disk:
                 raise ValueError(f"initial arrays must hold NRAD*NSEC = {n} values")
         ptrs = [a.ctypes.data_as(C.POINTER(C.c_double)) for a in arrs]
         _check(lib().thorin_init_state(self._h, 1 if mode is None else mode, *ptrs), "init")
+
+    def init_pebbles(self, mode: int = 0, pdens=None, pvrad=None, pvtheta=None, psize=None) -> None:
+        """Pebble disk (Init.c:138-144): mode 0 = coagulation-drift equilibrium
+        from the initial gas (EquilPebbleDisk); mode 1 = the given pebble
+        density, velocities and sizes (RestartPebbleDisk)."""
+        if mode == 0:
+            _check(lib().thorin_init_pebbles(self._h, 0, None, None, None, None), "init_pebbles")
+            return
+        arrs = [np.ascontiguousarray(a, dtype=np.float64).ravel() for a in (pdens, pvrad, pvtheta, psize)]
+        n = self.nrad * self.nsec
+        for a in arrs:
+            if a.size != n:
+                raise ValueError(f"pebble arrays must hold NRAD*NSEC = {n} values")
+        ptrs = [a.ctypes.data_as(C.POINTER(C.c_double)) for a in arrs]
+        _check(lib().thorin_init_pebbles(self._h, mode, *ptrs), "init_pebbles")
 
     # --- the hot path
     def algogas(self, planets: Planets, nbody=None) -> StepLog:

@@ -94,6 +94,7 @@ def lib() -> C.CDLL:
         "thorin_get_state": (ip, [vp, C.POINTER(State)]),
         "thorin_set_state": (ip, [vp, C.POINTER(State)]),
         "thorin_init_state": (ip, [vp, ip, dp, dp, dp, dp]),
+        "thorin_init_pebbles": (ip, [vp, ip, dp, dp, dp, dp]),
         "thorin_upload_field": (ip, [vp, ip, dp, sz]),
         "thorin_download_field": (ip, [vp, ip, dp, sz]),
         "thorin_algogas": (ip, [vp, C.POINTER(Planets), NBODY_FN, vp, C.POINTER(StepLog)]),

@@ -10,6 +10,10 @@ POWERLAWOPACITY, the build-only kappa = k0*T^2 option.
   damped_1024x1536 : the same physics at 1024x1536 with damping zones at both
                      radial edges (ZEROVRAD), 100 steps -- the hydro-step
                      benchmark workload
+  pebbles_1024x1536: the damped disk plus a pebble fluid (Sigma_p = 0.01
+                     Sigma, Stokes numbers 0.01-0.1 per ring, seed.seeded_pebbles),
+                     drag back-reaction, particle diffusion, a q = 3e-5 planet
+                     at r = 1 that feels the disk and accretes pebbles, 500 steps
   sor_2048x3072    : the energy equation's implicit solve at 2048x3072 with
                      BL94 opacity on the interpolated disk of seed.sor_disk
                      (T 20-2000 K), dt = the hydro CFL step -- the solver
@@ -30,6 +34,10 @@ PLAIN_128x256 = dict(BASE, Nrad="128", Nsec="256", InnerBoundary="WALL", Ntot="2
 
 DAMPED_1024x1536 = dict(BASE, Nrad="1024", Nsec="1536", InnerBoundary="DAMPING", DampTowards="ZEROVRAD",
                         Ntot="100", Ninterm="100")
+
+PEBBLES_1024x1536 = dict(DAMPED_1024x1536, PebbleAccretion="YES", BackReaction="YES", PlanetsFeelDisk="YES",
+                         Ntot="500", Ninterm="500")
+PLANET_Q3E5 = [(3e-5, 1.0)]
 
 SOR_2048x3072 = dict(BASE, Nrad="2048", Nsec="3072", InnerBoundary="DAMPING", DampTowards="ZEROVRAD",
                      OpacityTable="BL94", Ntot="1", Ninterm="1")

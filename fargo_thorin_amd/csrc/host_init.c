@@ -298,6 +298,8 @@ int thorin_ctx_destroy(thorin_ctx *c) {
   free(c->InvSurf); free(c->InvDiffRmed); free(c->InvDiffRsup); free(c->InvRinf); free(c->OmegaInv);
   free(c->Rmed2); free(c->GlobalRmed); free(c->SigmaMed); free(c->SigmaInf); free(c->EnergyMed);
   free(c->VthetaMed); free(c->WaveKiller); free(c->globcsvec); free(c->globtempervec);
+  free(c->h_pdens); free(c->h_pvr); free(c->h_pvt); free(c->h_psize); free(c->h_tau); free(c->h_eta);
+  free(c->PebDensInit); free(c->PebVradInit); free(c->PebVthetaInit);
   free(c->pow_rmed15); free(c->pow_rinf15); free(c->cav_med); free(c->cav_inf); free(c->cfl_pow2);
   free(c->supp_torque);
   free(c->h_dens); free(c->h_vrad); free(c->h_vtheta); free(c->h_energy); free(c->h_label);
@@ -423,6 +425,128 @@ int thorin_init_state(thorin_ctx *c, int mode, const double *dens, const double 
   return 0;
 }
 
+/* ---------------- pebble fluid set-up (Init.c:138-144) ---------------- */
+#define TH_FLUX2CU (0.000003 / (2.0 * TH_PI))   /* fondam.h:54 */
+
+/* EtaPressureSupport (Pebbles.c:236-270) + InitPebblesViaFlux
+ * (Pebbles.c:79-202): coagulation-drift equilibrium of a fresh run, from the
+ * initial gas state */
+static void peb_equilibrium(thorin_ctx *c) {
+  const thorin_params *p = &c->p;
+  int i, j, l, lim, ljp, nr = c->nr, ns = c->ns;
+  double *etacc = c->h_eta, *etafc = (double *)calloc(c->nfield, sizeof(double));
+  const double *vt = c->h_vtheta, *rho = c->h_dens, *vr = c->h_vrad, *cs = c->h_cs;
+  const double of = c->st.omega_frame, pebbleflux = p->pebbleflux * TH_FLUX2CU, Ndf = 0.5;
+  const double pebbulkdens = p->pebblebulkdens / TH_RHO2CGS;
+  for (i = 0; i < nr; i++) {
+    double vtcorr = c->Rmed[i] * of, vkinv = sqrt(c->Rmed[i]);
+    for (j = 0; j < ns; j++) {
+      l = j + i * ns;
+      ljp = (j == ns - 1) ? i * ns : l + 1;
+      etacc[l] = 1.0 - (0.5 * (vt[l] + vt[ljp]) + vtcorr) * vkinv;
+    }
+  }
+  for (i = 1; i < nr; i++)
+    for (j = 0; j < ns; j++) {
+      l = j + i * ns;
+      lim = l - ns;
+      etafc[l] = 0.5 * (etacc[l] + etacc[lim]);
+    }
+  for (j = 0; j < ns; j++) etafc[j] = etafc[j + ns];
+  for (i = 0; i < nr; i++) {
+    double rho1 = 0.0, vr1 = 0.0, etafc1 = 0.0, etacc1 = 0.0, cs1 = 0.0;
+    double tau1, psize1, taudf, adf, taufrag, afrag, vkepl, fac, etaterm, pvr1, pvrtemp, pvt1, pdens1;
+    for (j = 0; j < ns; j++) {
+      l = j + i * ns;
+      rho1 += rho[l]; vr1 += vr[l]; etafc1 += etafc[l]; etacc1 += etacc[l]; cs1 += cs[l];
+    }
+    rho1 /= (double)ns; vr1 /= (double)ns; etafc1 /= (double)ns; etacc1 /= (double)ns; cs1 /= (double)ns;
+    /* drift-limited, differential-drift and fragmentation-limited sizes */
+    tau1 = sqrt(3.0) * p->pebblecoagulation * pebbleflux / (32.0 * TH_PI * pow(c->Rmed[i], -1.5) * rho1);
+    tau1 = sqrt(tau1) / (c->Rmed[i] * etacc1);
+    psize1 = fabs(tau1) * sqrt(p->adiabind) * rho1 / (2.0 * pebbulkdens);
+    taudf = p->fragmentthreshold * sqrt(c->Rmed[i]) / (2.0 * etacc1 * (1.0 - Ndf));
+    adf = fabs(taudf) * sqrt(p->adiabind) * rho1 / (2.0 * pebbulkdens);
+    taufrag = p->fragmentfactor / (3.0 * p->fragmentalpha) * pow(p->fragmentthreshold / cs1, 2);
+    afrag = fabs(taufrag) * sqrt(p->adiabind) * rho1 / (2.0 * pebbulkdens);
+    if (!((psize1 < adf) && (psize1 < afrag))) {
+      if ((taudf > 0.0) && (adf < psize1) && (adf < afrag)) { tau1 = taudf; psize1 = adf; }
+      else { tau1 = taufrag; psize1 = afrag; }
+    }
+    /* steady drift velocities; the density from the imposed mass flux */
+    vkepl = pow(c->Rmed[i], -0.5);
+    fac = 1.0 / (tau1 * tau1 + 1.0);
+    etaterm = etafc1 * vkepl;
+    pvr1 = -2.0 * tau1 * fac * (etaterm - 0.5 / tau1 * vr1);
+    etaterm = etacc1 * vkepl;
+    pvrtemp = -2.0 * tau1 * fac * (etaterm - 0.5 / tau1 * vr1);
+    pvt1 = -fac * (etaterm - 0.5 * tau1 * vr1);
+    pvt1 += (vkepl - c->Rmed[i] * of);
+    pdens1 = pebbleflux / (2.0 * TH_PI * c->Rmed[i] * fabs(pvrtemp));
+    if (pdens1 / rho1 > p->pebbletogasmax) pdens1 = rho1 * p->pebbletogasmax;
+    for (j = 0; j < ns; j++) {
+      l = j + i * ns;
+      c->h_psize[l] = psize1; c->h_tau[l] = tau1; c->h_pdens[l] = pdens1; c->h_pvr[l] = pvr1; c->h_pvt[l] = pvt1;
+    }
+    c->PebDensInit[i] = pdens1;
+    c->PebVradInit[i] = pvr1;
+    c->PebVthetaInit[i] = pvt1 + c->Rmed[i] * of;
+  }
+  free(etafc);
+}
+
+/* BckpFieldsForBC (Pebbles.c:206-232): ordered ring means as damping targets */
+static void peb_backup(thorin_ctx *c) {
+  int i, j, l, ns = c->ns;
+  for (i = 0; i < c->nr; i++) {
+    c->PebDensInit[i] = 0.0; c->PebVradInit[i] = 0.0; c->PebVthetaInit[i] = 0.0;
+    for (j = 0; j < ns; j++) {
+      l = j + i * ns;
+      c->PebDensInit[i] += c->h_pdens[l]; c->PebVradInit[i] += c->h_pvr[l]; c->PebVthetaInit[i] += c->h_pvt[l];
+    }
+    c->PebDensInit[i] /= (double)ns;
+    c->PebVradInit[i] /= (double)ns;
+    c->PebVthetaInit[i] /= (double)ns;
+    c->PebVthetaInit[i] += c->Rmed[i] * c->st.omega_frame;
+  }
+}
+
+int thorin_init_pebbles(thorin_ctx *c, int mode, const double *pdens, const double *pvrad,
+                        const double *pvtheta, const double *psize) {
+  size_t ncell;
+  if (!c) { thorin_set_error("null context"); return -1; }
+  if (!c->p.pebbles) { thorin_set_error("thorin_init_pebbles: PebbleAccretion is NO in the parameters"); return -1; }
+  if (mode != 0 && mode != 1) { thorin_set_error("thorin_init_pebbles: mode must be 0 or 1"); return -1; }
+  if (mode == 1 && (!pdens || !pvrad || !pvtheta || !psize)) {
+    thorin_set_error("thorin_init_pebbles: mode 1 needs pdens, pvrad, pvtheta and psize arrays");
+    return -1;
+  }
+  ncell = (size_t)c->nr * c->ns;
+  if (!c->h_pdens) {
+    c->h_pdens = alloc1((int)c->nfield); c->h_pvr = alloc1((int)c->nfield); c->h_pvt = alloc1((int)c->nfield);
+    c->h_psize = alloc1((int)c->nfield); c->h_tau = alloc1((int)c->nfield); c->h_eta = alloc1((int)c->nfield);
+    c->PebDensInit = alloc1(c->nr + 3); c->PebVradInit = alloc1(c->nr + 3); c->PebVthetaInit = alloc1(c->nr + 3);
+    if (!c->h_pdens || !c->h_pvr || !c->h_pvt || !c->h_psize || !c->h_tau || !c->h_eta || !c->PebDensInit ||
+        !c->PebVradInit || !c->PebVthetaInit) {
+      thorin_set_error("out of host memory");
+      return -1;
+    }
+  }
+  if (mode == 0) {
+    peb_equilibrium(c);
+  } else {
+    memcpy(c->h_pdens, pdens, sizeof(double) * ncell);
+    memcpy(c->h_pvr, pvrad, sizeof(double) * ncell);
+    memcpy(c->h_pvt, pvtheta, sizeof(double) * ncell);
+    memcpy(c->h_psize, psize, sizeof(double) * ncell);
+    memset(c->h_tau, 0, sizeof(double) * c->nfield);
+    peb_backup(c);
+  }
+  c->peb_ready = 1;
+  if (c->dev && thorin_dev_upload_pebbles(c) != 0) return -1;
+  return 0;
+}
+
 /* initial host copy of a field (valid after thorin_init_state) */
 int thorin_host_field(thorin_ctx *c, int field, double *out, size_t count) {
   const double *src = NULL;
@@ -436,8 +560,15 @@ int thorin_host_field(thorin_ctx *c, int field, double *out, size_t count) {
     case THORIN_F_TEMPER: src = c->h_temper; break;
     case THORIN_F_PRESSURE: src = c->h_press; break;
     case THORIN_F_SOUNDSPEED: src = c->h_cs; break;
+    case THORIN_F_PEBBLEDENS: src = c->h_pdens; break;
+    case THORIN_F_PEBBLEVRAD: src = c->h_pvr; break;
+    case THORIN_F_PEBBLEVTHETA: src = c->h_pvt; break;
+    case THORIN_F_PEBBLESIZE: src = c->h_psize; break;
+    case THORIN_F_PEBBLESTOKES: src = c->h_tau; break;
+    case THORIN_F_ETA: src = c->h_eta; break;
     default: thorin_set_error("no host copy of this field"); return -1;
   }
+  if (!src) { thorin_set_error("no host copy of this field (pebbles not set up)"); return -1; }
   memcpy(out, src, sizeof(double) * count);
   return 0;
 }

@@ -57,8 +57,15 @@ enum {
   B_VRADNEW = THORIN_F_VRADNEW, B_VTHETANEW = THORIN_F_VTHETANEW,
   B_DIFF = THORIN_NFIELDS, B_ML, B_MLIP, B_MLIM, B_MLJP, B_MLJM, B_RHS, B_TBACK, B_TEMPER2,
   B_RP, B_RM, B_TP, B_TM, B_XRP, B_XRM, B_XTP, B_XTM, B_XE, B_XRHO, B_ABS, B_ORD,
+  // pebble fluid work fields (Pebbles.c:37-56, SourceEuler.c:155-158)
+  B_PAGR, B_PAGT, B_PACCR, B_PACCT, B_GACCR, B_GACCT, B_DRAGR, B_DRAGT, B_MACC,
   NBUF
 };
+enum {
+  B_PDENS = THORIN_F_PEBBLEDENS, B_PVR = THORIN_F_PEBBLEVRAD, B_PVT = THORIN_F_PEBBLEVTHETA,
+  B_PSIZE = THORIN_F_PEBBLESIZE, B_TAU = THORIN_F_PEBBLESTOKES
+};
+#define PEB_MAXPL 8             // planets that accrete pebbles (as the force kernel)
 
 struct DevCtl {                 // device-side control block, mirrored to pinned host memory
   double newdt_raw;             // ConditionCFL minimum before the clamps
@@ -70,6 +77,9 @@ struct DevCtl {                 // device-side control block, mirrored to pinned
   double axstar, aystar;
   double axpl[MAXPL], aypl[MAXPL];
   double tq_rho[16], tq_cs[16]; // torque log: ring means next to the logged planets
+  int crash_peb;                // DetectCrashPebbles of this substep
+  int acc_fail;                 // IntegrateColumnMass did not converge
+  double pacc[PEB_MAXPL][3];    // pebble accretion per planet: dM, dPX, dPY
 };
 
 struct Mat32 {                  // FP32 matrix diagonals, residual and correction double buffer
@@ -94,7 +104,7 @@ struct Geo {                    // passed by value to every kernel
   const double *OmegaInv, *SigmaMed, *SigmaInf, *EnergyMed, *VthetaMed, *WaveKiller;
   const double *pow_rmed15, *pow_rinf15, *cav_med, *cav_inf, *cfl_pow2, *supp_torque;
   const int *dampring;
-  double *gcs, *gtemp, *Vmoy, *VMed, *ringmin;
+  double *gcs, *gtemp, *Vmoy, *VMed, *ringmax;
   double *sig_avr, *ist_avr;    // ring-averaged vertical integrals of FillForcesArrays
   int *Nshift;
   double adiabind, sqrt_adiabind_inv, cv, viscosity, alphavisc, alphamri, alphadead, tmri, twidth;
@@ -105,6 +115,14 @@ struct Geo {                    // passed by value to every kernel
   double hillcut;
   // LP85 thresholds folded on the host (Opacity.c:205-220)
   double lp_t1, lp_t2, lp_t3a, lp_t4, lp_t5a, lp_t6a;
+  // pebble fluid
+  int pebbles, back_reaction, pebble_gravity;
+  double palpha, pbulk, schmidt;
+  const double *peb_dens0, *peb_vr0, *peb_vt0;   // PebDensInit, PebVradInit, PebVthetaInit
+  const int *pdampring;                          // rings DampPebbles acts on
+  double *tau_avr, *phavr, *pist_avr;            // ring means of FillForcesArrays' pebble part
+  double *pebmax;                                // CriticalCharTime ring maxima
+  double *pvmed;                                 // ordered ring mean of the pebble vtheta
 };
 
 struct thorin_dev {
@@ -133,6 +151,8 @@ struct thorin_dev {
   int rad_ready;                // thorin_rad_system has assembled the energy system
   struct Mat32 m32;             // FP32 copies for the mixed-precision solve (allocated on first use)
   double cfl_invdtreb;          // MinStepForRebound term it was formed with
+  double *acc_stage;            // pinned staging of the pebble accretion's host part
+  size_t acc_stage_n;
   // kernel timing with HIP events (thorin_prof_*)
   int prof_on;
   unsigned long long prof_mask;
@@ -147,7 +167,8 @@ static const char *const KNAMES[] = {
     "k_eos_cfl", "k_cfl_final", "k_forces", "k_forces_reduce", "k_indirect", "k_boundary", "k_substep1",
     "k_divstress", "k_visc_update", "k_substep2",
     "k_matrix", "k_sor_resid0", "k_sor_iter", "k_sor_check", "k_energy_new", "k_transport_rad",
-    "k_transport_theta", "k_velocities", "k_substep3", NULL};
+    "k_transport_theta", "k_velocities", "k_substep3", "k_peb_cfl", "k_peb_damp", "k_peb_source",
+    "k_peb_substep1", "k_peb_accrete", "k_peb_evaporate", "k_ring_means", NULL};
 
 // a template instance ("k_name<3>") counts as its kernel
 static int kid_of(const char *name) {
@@ -296,6 +317,24 @@ __device__ __forceinline__ double block_min(double v, double *sh) {
   return r;
 }
 
+template <int N>
+__device__ __forceinline__ double block_max(double v, double *sh) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const double u = __shfl_xor(v, o, 64);
+    v = u > v ? u : v;
+  }
+  if (N == 64) return v;
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) sh[w] = v;
+  __syncthreads();
+  double r = sh[0];
+#pragma unroll
+  for (int k = 1; k < N / 64; k++) r = sh[k] > r ? sh[k] : r;
+  __syncthreads();
+  return r;
+}
+
 // Cell kernels run on a 1D grid of TB-wide column blocks over rings
 // i0 .. i0+nring-1.  Workgroups are dispatched round-robin over the 8 XCDs
 // (block b to XCD b % 8), each with its own L2; numbering the logical blocks
@@ -424,10 +463,16 @@ __device__ __forceinline__ double ordered_ring_sum(const double *sh, int ns) {
 //    715-737) from rho and the energy, with the ordered ring means
 //    globcsvec (and globtempervec when the Flock alpha needs it);
 //  * the per-cell part of the next substep's ConditionCFL
-//    (SourceEuler.c:589-672: ordered Vmoy, the five inverse time limits),
-//    whose block minimum goes to ringmin[i];
+//    (SourceEuler.c:589-672: ordered Vmoy, the five inverse time limits).
+//    The reference takes the minimum over cells of
+//    CFLSECURITY/sqrt(S + invdtpeb_sq + invdtreb_sq) with S the sum of the
+//    five squares; every step of that expression is monotone in S (IEEE
+//    rounding of +, sqrt and / is), so the minimum is the expression at the
+//    largest S.  The block maximum of S goes to ringmax[i] and k_cfl_final
+//    applies the rest once;
 //  * the ring-averaged vertical integrals of the next FillForcesArrays
-//    (Pframeforce.c:150-189), which only depend on globcsvec.
+//    (Pframeforce.c:150-189), which only depend on globcsvec, and their
+//    pebble counterparts from the Stokes-number ring means of k_peb_cfl.
 // The ring sums are sequential in j; lanes 0, 1, 2 of the first wave walk
 // them side by side while the other waves wait.  The CFL terms that need no
 // ring mean are formed before the sums (EC_MAX cells per thread) and the sum
@@ -437,7 +482,7 @@ __device__ __forceinline__ double ordered_ring_sum(const double *sh, int ns) {
 __global__ void __launch_bounds__(RB) k_eos_cfl(Geo g, const double *__restrict__ rho, const double *__restrict__ e,
                                                 double *__restrict__ T, double *__restrict__ cs,
                                                 const double *__restrict__ vr, const double *__restrict__ vt,
-                                                double invdtreb_sq, int do_eos, const int *__restrict__ guard) {
+                                                int do_eos, const int *__restrict__ guard) {
   extern __shared__ double sh[];       // (2 or 3)*ns: vt, cs [, T]
   __shared__ double red[RB / 64];
   __shared__ double vmoy, gcs_i, gtemp_i;
@@ -506,7 +551,10 @@ __global__ void __launch_bounds__(RB) k_eos_cfl(Geo g, const double *__restrict_
         const double H2 = H_avr * H_avr;
         const double r2 = g.Rmed[i] * g.Rmed[i];
         const double deltaz = 3.0 * H_avr / (double)10;
-        double sig = 0.0, ist = 0.0;
+        // pebbles: H_peb = H sqrt(alpha/tau) with the ring-mean Stokes number
+        // (Pframeforce.c:160-164, 180-186)
+        const double ta = g.pebbles ? g.tau_avr[i] : 1.0;
+        double sig = 0.0, ist = 0.0, pist = 0.0;
         for (int q = 1; q <= 10; q++) {
           const double znode = ((double)q - 0.5) * deltaz;
           const double znode2 = znode * znode;
@@ -516,9 +564,19 @@ __global__ void __launch_bounds__(RB) k_eos_cfl(Geo g, const double *__restrict_
           const double dd = sqrt(d2);
           const double denom = dd * d2;
           ist += sum / denom;
+          if (g.pebbles) {
+            const double pz2 = znode2 * (g.palpha / ta);
+            const double pd2 = r2 + pz2;
+            const double pd = sqrt(pd2);
+            pist += sum / (pd * pd2);
+          }
         }
         g.sig_avr[i] = sig;
         g.ist_avr[i] = ist;
+        if (g.pebbles) {
+          g.phavr[i] = H_avr * sqrt(g.palpha / ta);
+          g.pist_avr[i] = pist;
+        }
       } else {
         gtemp_i = m;
         g.gtemp[i] = m;
@@ -527,7 +585,7 @@ __global__ void __launch_bounds__(RB) k_eos_cfl(Geo g, const double *__restrict_
     if (tid == 2 && !want_t) gtemp_i = g.gtemp[i];
   }
   __syncthreads();
-  double m = 1e30;
+  double m = -1.0;
   if (cfl) {
     const int viscous = g.visc_alpha || (g.viscosity != 0.0);
     const double visc = viscous ? fvisc_val(g, gcs_i, gtemp_i, g.pow_rmed15[i], g.cav_med[i]) : 0.0;
@@ -539,20 +597,24 @@ __global__ void __launch_bounds__(RB) k_eos_cfl(Geo g, const double *__restrict_
       if (j < ns) {
         const double vres = g.fast_transport ? s_vt[j] - vm : s_vt[j];
         const double invdt3 = fabs(vres) / dxtheta;
-        const double dt = CFLSECURITY / sqrt(pre[k] + invdt3 * invdt3 + q4[k] + invdt5 * invdt5 + invdtreb_sq);
-        if (dt < m) m = dt;
+        const double S = pre[k] + invdt3 * invdt3 + q4[k] + invdt5 * invdt5;
+        if (S > m) m = S;
       }
     }
   }
-  m = block_min<RB>(m, red);
-  if (tid == 0) g.ringmin[i] = m;
+  m = block_max<RB>(m, red);
+  if (tid == 0) g.ringmax[i] = m;
 }
 
-// minimum over rings plus the shear limit between rings (SourceEuler.c:673-676)
-// The minimum of the DiffusionTimestep partials of this substep's k_substep3
-// (nstellar of them; 0: none formed) goes to ctl->dt_stellar.
+// ConditionCFL's minimum (SourceEuler.c:620-676): the cell term at the
+// largest S of rings 1..NRAD-1, with the pebble limit invdtpeb_sq
+// (CriticalCharTime, the maximum of k_peb_cfl's ring maxima) and the N-body
+// limit invdtreb_sq added in the reference's order, then the shear limit
+// between rings.  The minimum of the DiffusionTimestep partials of this
+// substep's k_substep3 (nstellar of them; 0: none formed) goes to
+// ctl->dt_stellar.
 __global__ void __launch_bounds__(RB) k_cfl_final(Geo g, DevCtl *ctl, const double *__restrict__ pstellar,
-                                                  int nstellar, const int *__restrict__ guard) {
+                                                  int nstellar, double invdtreb_sq, const int *__restrict__ guard) {
   __shared__ double red[RB];
   if (guard && *guard == 0) return;
   if (nstellar > 0) {
@@ -561,8 +623,22 @@ __global__ void __launch_bounds__(RB) k_cfl_final(Geo g, DevCtl *ctl, const doub
     ms = block_min<RB>(ms, red);
     if (threadIdx.x == 0) ctl->dt_stellar = ms;
   }
+  double S = -1.0, P = -1.e30;
+  for (int i = 1 + threadIdx.x; i < g.nr; i += RB) {
+    const double v = g.ringmax[i];
+    S = v > S ? v : S;
+    if (g.pebbles) { const double u = g.pebmax[i]; P = u > P ? u : P; }
+  }
+  S = block_max<RB>(S, red);
   double m = 1e30;
-  m = strided_min<RB>(g.ringmin + 1, g.nr - 1, m);
+  if (g.pebbles) {
+    P = block_max<RB>(P, red);
+    const double dt = CFLSECURITY / sqrt(S + P + invdtreb_sq);
+    if (dt < m) m = dt;
+  } else {
+    const double dt = CFLSECURITY / sqrt(S + invdtreb_sq);
+    if (dt < m) m = dt;
+  }
   for (int i = threadIdx.x; i < g.nr - 1; i += RB) {
     double dt = 2.0 * DPI * CFLSECURITY / (double)g.ns /
                 fabs(g.Vmoy[i] * g.InvRmed[i] - g.Vmoy[i + 1] * g.InvRmed[i + 1]);
@@ -573,8 +649,18 @@ __global__ void __launch_bounds__(RB) k_cfl_final(Geo g, DevCtl *ctl, const doub
 }
 
 // ---------------------------------------------------------------------------
-// FillForcesArrays (Pframeforce.c:43-510), gas part.  One thread per cell;
-// per-block partial sums of the disk->star and disk->planet accelerations.
+// FillForcesArrays (Pframeforce.c:43-510).  One thread per cell; per-block
+// partial sums of the disk->star and disk->planet accelerations.  PEB adds
+// the pebble fluid's accelerations pagr, pagt (vertical extent
+// H_peb = H sqrt(alpha/tau)) and, with PebbleGravity, the pebbles' pull on
+// the planets; the pebble scale height also decides where the integration
+// is local for both fluids (Pframeforce.c:206-214).
+struct PebF {
+  const double *prho, *tau;
+  double *pagr, *pagt;
+};
+
+template <bool PEB>
 __global__ void __launch_bounds__(TB) k_forces(Geo g, const double *__restrict__ rho,
                                                const double *__restrict__ cs,
                                                const double *__restrict__ abs_,
@@ -582,7 +668,7 @@ __global__ void __launch_bounds__(TB) k_forces(Geo g, const double *__restrict__
                                                double *__restrict__ agr, double *__restrict__ agt,
                                                const PlanetK *__restrict__ pk, int npl, double mt,
                                                double *__restrict__ partial, TqRings tq, int ntq_rings,
-                                               DevCtl *__restrict__ ctl) {
+                                               DevCtl *__restrict__ ctl, PebF pf) {
   __shared__ double red[TB];
   const int ns = g.ns;
   const int lead = ntq_rings > 0 ? 1 : 0;
@@ -627,6 +713,14 @@ __global__ void __launch_bounds__(TB) k_forces(Geo g, const double *__restrict__
     const double H = cs[l] * omegainv * g.sqrt_adiabind_inv;
     H2 = H * H;
     if (fabs(H - H_avr) / H_avr > 0.05) IntegrateLocally = 1;
+    double pmcell = 0.0, Hcorr = 0.0, pH = 0.0;
+    if (PEB) {
+      pmcell = pf.prho[l] * g.Surf[i];
+      Hcorr = sqrt(g.palpha / pf.tau[l]);
+      pH = H * Hcorr;
+      const double p_H_avr = g.phavr[i];
+      if (fabs(pH - p_H_avr) / p_H_avr > 0.05) IntegrateLocally = 1;
+    }
     unsigned int withpl = 0;    // bit k: IntegrateWithPlanet[k] (npl <= 8 on this path)
     double s2[8], hillcut[8];
     for (int k = 0; k < npl; k++) {
@@ -648,8 +742,8 @@ __global__ void __launch_bounds__(TB) k_forces(Geo g, const double *__restrict__
       }
       if (s2[k] < P.rmorevert2) Nvert = 30;
     }
-    double sigma, integstar, integpl[8];
-    for (int k = 0; k < 8; k++) integpl[k] = 0.0;
+    double sigma, integstar, integpl[8], p_integstar = 0.0, p_integpl[8];
+    for (int k = 0; k < 8; k++) { integpl[k] = 0.0; p_integpl[k] = 0.0; }
     if (IntegrateLocally) {
       zmax = 3.0 * H;
       deltaz = zmax / (double)Nvert;
@@ -664,6 +758,12 @@ __global__ void __launch_bounds__(TB) k_forces(Geo g, const double *__restrict__
         double d = sqrt(d2);
         double denom = d * d2;
         integstar += sum / denom;
+        if (PEB) {
+          d2 = r2 + znode2 * Hcorr * Hcorr;
+          d = sqrt(d2);
+          denom = d * d2;
+          p_integstar += sum / denom;
+        }
         for (int k = 0; k < npl; k++) {
           if (withpl & (1u << k)) {
             const PlanetK P = pk[k];
@@ -688,18 +788,48 @@ __global__ void __launch_bounds__(TB) k_forces(Geo g, const double *__restrict__
               denom = d * d2;
               integpl[k] += sum / denom * taper;
             }
+            if (PEB) {
+              const double zp = znode * Hcorr;
+              d2 = s2[k] + (zp - zpl) * (zp - zpl);
+              if (d2 < P.rst2) {
+                d = sqrt(d2);
+                taper = -3.0 * d2 * d2 / P.rst4 + 4.0 * d * d2 / P.rst3;
+              } else taper = 1.0;
+              d2 += (P.rsm2 > Rsup_Rinf2 ? P.rsm2 : Rsup_Rinf2);
+              d = sqrt(d2);
+              denom = d * d2;
+              p_integpl[k] += sum / denom * taper;
+              if (zpl < -0.05 * pH || zpl > 0.05 * pH) {
+                const double zm = -znode * Hcorr;
+                d2 = s2[k] + (zm - zpl) * (zm - zpl);
+                if (d2 < P.rst2) {
+                  d = sqrt(d2);
+                  taper = -3.0 * d2 * d2 / P.rst4 + 4.0 * d * d2 / P.rst3;
+                } else taper = 1.0;
+                d2 += (P.rsm2 > Rsup_Rinf2 ? P.rsm2 : Rsup_Rinf2);
+                d = sqrt(d2);
+                denom = d * d2;
+                p_integpl[k] += sum / denom * taper;
+              }
+            }
           }
         }
       }
     } else {
       sigma = sigma_avr;
       integstar = integstar_avr;
+      if (PEB) p_integstar = g.pist_avr[i];
     }
     double tmp = integstar / sigma;
     double ar = -g.Rmed[i] * tmp;
     double agr_l = 0.0 + ar;
     axs = mcell * tmp * x;
     ays = mcell * tmp * y;
+    double pagr_l = 0.0, pax = 0.0, pay = 0.0;
+    if (PEB) {
+      tmp = p_integstar / sigma;
+      pagr_l = 0.0 + (-g.Rmed[i] * tmp);
+    }
     for (int k = 0; k < npl; k++) {
       const PlanetK P = pk[k];
       double mpl = P.m;
@@ -714,6 +844,22 @@ __global__ void __launch_bounds__(TB) k_forces(Geo g, const double *__restrict__
         ay += -mpl * tmp;
         if (g.exclude_hill) tmp *= hillcut[k];
         aypl_loc[k] = mcell * tmp;
+        if (PEB) {
+          if (P.z < -0.05 * pH || P.z > 0.05 * pH) tmp = (x - P.x) * p_integpl[k] / (2.0 * sigma);
+          else tmp = (x - P.x) * p_integpl[k] / sigma;
+          pax += -mpl * tmp;
+          if (g.pebble_gravity) {
+            if (g.exclude_hill) tmp *= hillcut[k];
+            axpl_loc[k] += pmcell * tmp;
+          }
+          if (P.z < -0.05 * pH || P.z > 0.05 * pH) tmp = (y - P.y) * p_integpl[k] / (2.0 * sigma);
+          else tmp = (y - P.y) * p_integpl[k] / sigma;
+          pay += -mpl * tmp;
+          if (g.pebble_gravity) {
+            if (g.exclude_hill) tmp *= hillcut[k];
+            aypl_loc[k] += pmcell * tmp;
+          }
+        }
       } else {
         double smoothing = 0.0;
         if (P.iring >= 0) smoothing = g.thicksmooth * (g.gcs[P.iring] * g.OmegaInv[P.iring] * g.sqrt_adiabind_inv);
@@ -730,6 +876,23 @@ __global__ void __launch_bounds__(TB) k_forces(Geo g, const double *__restrict__
         ay += -mpl * tmp;
         if (g.exclude_hill) tmp *= hillcut[k];
         aypl_loc[k] = mcell * tmp;
+        if (PEB) {
+          d2 += -smoothing2 + smoothing2 * Hcorr * Hcorr;
+          d = sqrt(d2);
+          denom = d * d2;
+          tmp = (x - P.x) / denom;
+          pax += -mpl * tmp;
+          if (g.pebble_gravity) {
+            if (g.exclude_hill) tmp *= hillcut[k];
+            axpl_loc[k] += pmcell * tmp;
+          }
+          tmp = (y - P.y) / denom;
+          pay += -mpl * tmp;
+          if (g.pebble_gravity) {
+            if (g.exclude_hill) tmp *= hillcut[k];
+            aypl_loc[k] += pmcell * tmp;
+          }
+        }
       }
     }
     ar = (x * ax + y * ay) * g.InvRmed[i];
@@ -737,6 +900,12 @@ __global__ void __launch_bounds__(TB) k_forces(Geo g, const double *__restrict__
     agr_l += ar;
     agr[l] = agr_l;
     agt[l] = 0.0 + at;
+    if (PEB) {
+      ar = (x * pax + y * pay) * g.InvRmed[i];
+      at = (x * pay - y * pax) * g.InvRmed[i];
+      pf.pagr[l] = pagr_l + ar;
+      pf.pagt[l] = 0.0 + at;
+    }
   }
   // partial sums of this block: [axstar, aystar, axpl[k], aypl[k]]
   const int nb = g.ncb * g.nr;
@@ -760,7 +929,7 @@ __global__ void __launch_bounds__(RB) k_forces_reduce(const double *__restrict__
   s = strided_sum<RB>(partial + (size_t)q * nb, nb);
   s = block_sum<RB>(s, red);
   if (threadIdx.x == 0) {
-    if (q == 0) { ctl->crash_e = 0; ctl->crash_rho = 0; }   // DetectCrash flags of this substep
+    if (q == 0) { ctl->crash_e = 0; ctl->crash_rho = 0; ctl->crash_peb = 0; }   // DetectCrash flags of this substep
     if (q == 0) ctl->axstar = s;
     else if (q == 1) ctl->aystar = s;
     else if ((q & 1) == 0) ctl->axpl[(q - 2) / 2] = s;
@@ -771,7 +940,8 @@ __global__ void __launch_bounds__(RB) k_forces_reduce(const double *__restrict__
 // indirect term (Pframeforce.c:488-509)
 __global__ void __launch_bounds__(TB) k_indirect(Geo g, double *__restrict__ agr, double *__restrict__ agt,
                                                  const double *__restrict__ abs_, const double *__restrict__ ord,
-                                                 const DevCtl *__restrict__ ctl, double indir_px, double indir_py) {
+                                                 const DevCtl *__restrict__ ctl, double indir_px, double indir_py,
+                                                 double *__restrict__ pagr, double *__restrict__ pagt) {
   const int ns = g.ns;
   int i, j, L_;
   if (!xcd_cell(g, 0, g.nr, i, j, L_) || j >= ns) return;
@@ -783,6 +953,322 @@ __global__ void __launch_bounds__(TB) k_indirect(Geo g, double *__restrict__ agr
   const double at = (x * ayindir - y * axindir) * g.InvRmed[i];
   agr[l] += ar;
   agt[l] += at;
+  if (pagr) {
+    pagr[l] += ar;
+    pagt[l] += at;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Pebble fluid (Pebbles.c), the second fluid of the two-fluid runs.
+
+// Start of a substep, one workgroup per ring: PebbleStokesNumbers
+// (Pebbles.c:272-293), the ordered ring mean of the Stokes numbers that
+// FillForcesArrays uses (Pframeforce.c:151-163) and the per-cell part of
+// CriticalCharTime (Pebbles.c:735-770) with its ordered ring mean of the
+// pebble vtheta; lanes 0 and 1 of the first wave walk the two ring sums side
+// by side.  The ring maximum of the four squared inverse times goes to
+// pebmax[i] (ring 0 is not part of CriticalCharTime).
+__global__ void __launch_bounds__(RB) k_peb_cfl(Geo g, const double *__restrict__ rho,
+                                                const double *__restrict__ psize, double *__restrict__ tau,
+                                                const double *__restrict__ pvr, const double *__restrict__ pvt,
+                                                const double *__restrict__ vr, const double *__restrict__ vt,
+                                                const int *__restrict__ guard) {
+  extern __shared__ double sh[];       // 2*ns: tau, pebble vtheta
+  __shared__ double red[RB / 64];
+  __shared__ double pvtmed;
+  if (guard && *guard == 0) return;
+  const int i = blockIdx.x, ns = g.ns, tid = threadIdx.x;
+  double *s_tau = sh, *s_pvt = sh + ns;
+  const size_t base = (size_t)i * ns;
+  for (int j = tid; j < ns; j += RB) {
+    const size_t l = base + j;
+    const double t = 2.0 * g.pbulk * psize[l] / rho[l] * g.sqrt_adiabind_inv;
+    tau[l] = t;
+    s_tau[j] = t;
+    s_pvt[j] = pvt[l];
+  }
+  __syncthreads();
+  if (tid < 2) {
+    const double m = ordered_ring_sum(tid == 0 ? s_tau : s_pvt, ns) / (double)ns;
+    if (tid == 0) g.tau_avr[i] = m;
+    else pvtmed = m;
+  }
+  __syncthreads();
+  double m = -1.0;
+  if (i >= 1) {
+    const double dxrad = g.Rsup[i] - g.Rinf[i];
+    const double dxtheta = g.Rmed[i] * 2.0 * DPI / (double)ns;
+    const double pm = pvtmed;
+    for (int j = tid; j < ns; j += RB) {
+      const size_t l = base + j;
+      const double pr = pvr[l], pt = s_pvt[j];
+      const double invdt1 = fabs(pr) / dxrad;
+      const double vres = g.fast_transport ? pt - pm : pt;
+      const double invdt2 = fabs(vres) / dxtheta;
+      const double invdt3 = fabs(pr - vr[l]) / dxrad;
+      const double invdt4 = fabs(pt - vt[l]) / dxtheta;
+      const double S = invdt1 * invdt1 + invdt2 * invdt2 + invdt3 * invdt3 + invdt4 * invdt4;
+      if (S > m) m = S;
+    }
+  }
+  m = block_max<RB>(m, red);
+  if (tid == 0) g.pebmax[i] = m;
+}
+
+// DampPebbles (SideEuler.c:423-454) on the rings of the damping zones; with
+// detect, DetectCrashPebbles (SourceEuler.c:274, 48-66) first looks at every
+// cell's density before the damping.
+__global__ void __launch_bounds__(TB) k_peb_damp(Geo g, double *__restrict__ pdens, double *__restrict__ pvr,
+                                                 double *__restrict__ pvt, double dt, int detect, DevCtl *ctl,
+                                                 const int *__restrict__ guard) {
+  const int ns = g.ns;
+  int i, j, L_;
+  if (guard && *guard == 0) return;
+  if (!xcd_cell(g, 0, g.nr, i, j, L_) || j >= ns) return;
+  if (!detect && !g.pdampring[i]) return;
+  const size_t l = (size_t)i * ns + j;
+  const double pd = pdens[l];
+  if (detect && pd < 0.0) ctl->crash_peb = 1;
+  if (!g.pdampring[i]) return;
+  const double lambda = g.WaveKiller[i] * dt;
+  const double pdens0 = g.peb_dens0[i], pvr0 = g.peb_vr0[i];
+  const double pvt0 = g.peb_vt0[i] - g.Rmed[i] * g.omega_frame;
+  pdens[l] = (pd + lambda * pdens0) / (1.0 + lambda);
+  pvr[l] = (pvr[l] + lambda * pvr0) / (1.0 + lambda);
+  pvt[l] = (pvt[l] + lambda * pvt0) / (1.0 + lambda);
+}
+
+// SourceTermsPebbles after its DampPebbles (Pebbles.c:549-612): pebble
+// accelerations paccr (rings >= 1) and pacct and, with BackReaction, the
+// drag on the gas dragr (rings >= 1) and dragt.
+__global__ void __launch_bounds__(TB) k_peb_source(Geo g, const double *__restrict__ pdens,
+                                                   const double *__restrict__ pvr, const double *__restrict__ pvt,
+                                                   const double *__restrict__ tau, const double *__restrict__ vr,
+                                                   const double *__restrict__ vt, const double *__restrict__ pagr,
+                                                   const double *__restrict__ pagt, double *__restrict__ paccr,
+                                                   double *__restrict__ pacct, double *__restrict__ dragr,
+                                                   double *__restrict__ dragt) {
+  const int ns = g.ns;
+  int i, j, L_;
+  if (!xcd_cell(g, 0, g.nr, i, j, L_) || j >= ns) return;
+  const size_t l = (size_t)i * ns + j;
+  const size_t ljm = (j == 0) ? (size_t)i * ns + ns - 1 : l - 1;
+  const size_t ljp = (j == ns - 1) ? (size_t)i * ns : l + 1;
+  const double Omega = 1.0 / g.OmegaInv[i];
+  if (i >= 1) {
+    const size_t lim = l - ns;
+    double vt2 = pvt[l] + pvt[ljp] + pvt[lim] + pvt[ljp - ns];
+    vt2 = vt2 / 4.0 + g.Rinf[i] * g.omega_frame;
+    vt2 = vt2 * vt2;
+    paccr[l] = vt2 * g.InvRinf[i] + 0.5 * (pagr[l] + pagr[lim]);
+    if (g.back_reaction) {
+      const double drag = Omega * (pvr[l] - vr[l]) / tau[l];
+      dragr[l] = 0.5 * (pdens[l] + pdens[lim]) * drag;
+    }
+  }
+  pacct[l] = 0.5 * (pagt[l] + pagt[ljm]);
+  if (g.back_reaction) {
+    const double drag = Omega * (pvt[l] - vt[l]) / tau[l];
+    dragt[l] = 0.5 * (pdens[l] + pdens[ljm]) * drag;
+  }
+}
+
+// SubStep1Pebbles (Pebbles.c:616-650), the semi-implicit drag step with the
+// gas velocities from before SubStep1, then ParticleDiffusion
+// (Pebbles.c:654-695), whose corrections depend on the two densities only
+// (so one pass over the cells does both).
+__global__ void __launch_bounds__(TB) k_peb_substep1(Geo g, double *__restrict__ pvr, double *__restrict__ pvt,
+                                                     const double *__restrict__ tau, const double *__restrict__ paccr,
+                                                     const double *__restrict__ pacct, const double *__restrict__ accr,
+                                                     const double *__restrict__ acct, const double *__restrict__ vr,
+                                                     const double *__restrict__ vt, const double *__restrict__ rho,
+                                                     const double *__restrict__ pdens, double dt, int diffuse) {
+  const int ns = g.ns;
+  int i, j, L_;
+  if (!xcd_cell(g, 0, g.nr, i, j, L_) || j >= ns) return;
+  const size_t l = (size_t)i * ns + j;
+  const size_t ljm = (j == 0) ? (size_t)i * ns + ns - 1 : l - 1;
+  const double Omega = 1.0 / g.OmegaInv[i];
+  const double t = tau[l];
+  const double efac = exp(-dt * Omega / t);
+  double nvr = pvr[l];
+  if (i > 0) nvr = nvr * efac + accr[l] * dt + (vr[l] + (paccr[l] - accr[l]) * t / Omega) * (1.0 - efac);
+  double nvt = pvt[l] * efac + acct[l] * dt + (vt[l] + (pacct[l] - acct[l]) * t / Omega) * (1.0 - efac);
+  if (diffuse) {
+    if (i >= 1) {
+      const size_t lim = l - ns;
+      double corr = -fvisc_inf(g, i) / g.schmidt;
+      corr *= (rho[l] + rho[lim]) / (pdens[l] + pdens[lim]);
+      corr *= (pdens[l] / rho[l] - pdens[lim] / rho[lim]) * g.InvDiffRmed[i];
+      nvr += corr;
+    }
+    const double dxtheta = 2.0 * DPI / (double)ns * g.Rmed[i];
+    const double invdxtheta = 1.0 / dxtheta;
+    double corr = -fvisc_med(g, i) / g.schmidt;
+    corr *= (rho[l] + rho[ljm]) / (pdens[l] + pdens[ljm]);
+    corr *= (pdens[l] / rho[l] - pdens[ljm] / rho[ljm]) * invdxtheta;
+    nvt += corr;
+  }
+  pvr[l] = nvr;
+  pvt[l] = nvt;
+}
+
+// EvaporatePebbles (Pebbles.c:837-866)
+__global__ void __launch_bounds__(TB) k_peb_evaporate(Geo g, double *__restrict__ pdens, const double *__restrict__ T,
+                                                      double facc, double evaptemp) {
+  const int ns = g.ns;
+  int i, j, L_;
+  if (!xcd_cell(g, 0, g.nr, i, j, L_) || j >= ns) return;
+  const size_t l = (size_t)i * ns + j;
+  if (T[l] > evaptemp) {
+    double d = pdens[l] * (1.0 - facc);
+    if (d < 1.0e-8) d = 1.0e-8;          // EVAPOREDSIGMA, Pebbles.c:835
+    pdens[l] = d;
+  }
+}
+
+// IntegrateColumnMass (Pebbles.c:325-342) over the refinements of Trapzd
+// (Pebbles.c:295-320); ok = 0 if it does not converge in TRAPEZMAX steps
+#define TRAPEZMAX 16
+#define TRAPEZEPS 1.0e-4
+__device__ double column_mass(double a, double b, double Hpeb, int &ok) {
+  const double H2 = Hpeb * Hpeb;
+  double integral = 0.0, iold = 0.0;
+  for (int n = 1; n <= TRAPEZMAX; n++) {
+    if (n == 1) {
+      integral = 0.5 * (b - a) * (exp(-0.5 * a * a / H2) + exp(-0.5 * b * b / H2));
+    } else {
+      int it = 1;
+      for (int j = 1; j < n - 1; j++) it <<= 1;
+      const double iter = it;
+      const double spacing = (b - a) / iter;
+      double x = a + 0.5 * spacing, sum = 0.0;
+      for (int j = 1; j <= it; j++) {
+        sum += exp(-0.5 * x * x / H2);
+        x += spacing;
+      }
+      integral = 0.5 * (integral + (b - a) * sum / iter);
+    }
+    if (n > 5)
+      if (fabs(integral - iold) < TRAPEZEPS * fabs(iold) || (integral == 0.0 && iold == 0.0)) return integral;
+    iold = integral;
+  }
+  ok = 0;
+  return 0.0;
+}
+
+// planet-side part of AccretePebblesOntoPlanets formed on the host with the
+// reference's expressions and summation order (Pebbles.c:374-441: headwind,
+// transition mass, the annulus' mean Stokes number and the effective
+// accretion radius Reff; cells at the edge of Reff make the accreted mass
+// depend on Reff through sqrt(Reff^2 - R^2), so Reff is kept bit-exact)
+struct PebAccK {
+  int np, pad;
+  double x[PEB_MAXPL], y[PEB_MAXPL], z[PEB_MAXPL], m[PEB_MAXPL], om[PEB_MAXPL];
+  double reff[PEB_MAXPL], vrel[PEB_MAXPL], mt[PEB_MAXPL];
+  int imin[PEB_MAXPL], imax[PEB_MAXPL];
+};
+
+// AccretePebblesOntoPlanets (Pebbles.c:400-538), STEPS 3-5: one workgroup
+// that takes the planets in order (each sees the pebble density the previous
+// one left).  The sums over the cells inside Reff are tree sums.  The mass
+// and the x- and y-momentum each planet takes from the disk go to
+// ctl->pacc; the host updates the planet as STEP 7 does.
+#define ACC_T 1024
+__global__ void __launch_bounds__(ACC_T) k_peb_accrete(Geo g, PebAccK P, double *__restrict__ pdens,
+                                                       const double *__restrict__ pvr, const double *__restrict__ pvt,
+                                                       const double *__restrict__ tau, const double *__restrict__ cs,
+                                                       const double *__restrict__ abs_, const double *__restrict__ ord,
+                                                       double *__restrict__ macc, double dt, DevCtl *ctl) {
+  __shared__ double red[ACC_T / 64];
+  __shared__ double s_lim;
+  const int tid = threadIdx.x, ns = g.ns;
+  if (tid == 0) ctl->acc_fail = 0;
+  for (int k = 0; k < P.np; k++) {
+    const int i0 = P.imin[k], i1 = P.imax[k];
+    const int nann = (i1 >= i0) ? (i1 - i0 + 1) * ns : 0;
+    const size_t b0 = (size_t)i0 * ns;
+    const double X = P.x[k], Y = P.y[k], Z = P.z[k], M = P.m[k];
+    // STEP 3: upper limit of the mass each cell below Reff can give
+    const double Reff = P.reff[k], Reff2 = Reff * Reff;
+    double du = 0.0, hp = 0.0, pdm = 0.0, nc = 0.0;
+    for (int q = tid; q < nann; q += ACC_T) {
+      const int i = i0 + q / ns;
+      const size_t l = b0 + q;
+      const double xc = abs_[l], yc = ord[l];
+      const double R2 = (xc - X) * (xc - X) + (yc - Y) * (yc - Y);
+      if (R2 <= Reff2) {
+        const double H = cs[l] * g.OmegaInv[i] * g.sqrt_adiabind_inv;
+        const double Hpeb = H * sqrt(g.palpha / tau[l]);
+        const double zmin = Z - sqrt(Reff2 - R2), zmax = Z + sqrt(Reff2 - R2);
+        const double pd = pdens[l];
+        const double mcell = g.Surf[i] * pd;
+        const double vol = pd / Hpeb * g.sqrt2pi_inv;
+        int ok = 1;
+        const double cm = column_mass(zmin, zmax, Hpeb, ok);
+        if (!ok) ctl->acc_fail = 1;
+        double dM = vol * g.Surf[i] * cm;
+        if (dM > mcell) dM = 0.99999999 * mcell;
+        macc[l] = dM;
+        du += dM;
+        hp += Hpeb;
+        pdm += pd;
+        nc += 1.0;
+      }
+    }
+    du = block_sum<ACC_T>(du, red);
+    hp = block_sum<ACC_T>(hp, red);
+    pdm = block_sum<ACC_T>(pdm, red);
+    nc = block_sum<ACC_T>(nc, red);
+    // STEP 4: the model accretion rate limits the upper bound
+    if (tid == 0) {
+      const int ncell = (int)nc;
+      if (ncell > 0) {
+        hp /= (double)ncell;
+        pdm /= (double)ncell;
+      }
+      const double veloc = (M < P.mt[k]) ? P.vrel[k] : Reff * P.om[k];
+      double dM_model;
+      if (hp < Reff) dM_model = 2.0 * Reff * veloc * pdm;
+      else dM_model = DPI * Reff * Reff * veloc * pdm / hp * g.sqrt2pi_inv;
+      dM_model *= dt;
+      s_lim = (du > dM_model) ? dM_model / du : 1.0;
+    }
+    __syncthreads();
+    // STEP 5: take the mass from the cells, collect mass and momentum
+    const double limiter = s_lim;
+    double dm = 0.0, dpx = 0.0, dpy = 0.0;
+    for (int q = tid; q < nann; q += ACC_T) {
+      const size_t l = b0 + q;
+      const double mc = macc[l];
+      if (mc > 0.0) {
+        const int i = i0 + q / ns;
+        const double xc = abs_[l], yc = ord[l];
+        const double vrc = pvr[l], vtc = pvt[l] + g.Rmed[i] * g.omega_frame;
+        const double vxc = (vrc * xc - vtc * yc) * g.InvRmed[i];
+        const double vyc = (vrc * yc + vtc * xc) * g.InvRmed[i];
+        const double dM = mc * limiter;
+        macc[l] = 0.0;
+        const double mcell = g.Surf[i] * pdens[l];
+        double pn = (mcell - dM) / g.Surf[i];
+        if (pn < 0.0) pn = 1.e-20;
+        pdens[l] = pn;
+        dm += dM;
+        dpx += dM * vxc;
+        dpy += dM * vyc;
+      }
+    }
+    dm = block_sum<ACC_T>(dm, red);
+    dpx = block_sum<ACC_T>(dpx, red);
+    dpy = block_sum<ACC_T>(dpy, red);
+    if (tid == 0) {
+      ctl->pacc[k][0] = dm;
+      ctl->pacc[k][1] = dpx;
+      ctl->pacc[k][2] = dpy;
+    }
+    __syncthreads();
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -826,7 +1312,11 @@ __global__ void __launch_bounds__(TB) k_substep1(Geo g, const double *__restrict
                                                  const double *__restrict__ agr,
                                                  const double *__restrict__ agt,
                                                  double *__restrict__ vradint, double *__restrict__ vthetaint,
-                                                 double dt, DevCtl *ctl) {
+                                                 double dt, DevCtl *ctl, double *__restrict__ accr,
+                                                 double *__restrict__ acct, const double *__restrict__ dragr,
+                                                 const double *__restrict__ dragt) {
+  // accr/acct (pebbles: the gas acceleration, SourceEuler.c:371-405) and
+  // dragr/dragt (BackReaction) are null when unused
   const int ns = g.ns;
   int i, j, L_;
   if (!xcd_cell(g, 0, g.nr, i, j, L_) || j >= ns) return;
@@ -845,7 +1335,17 @@ __global__ void __launch_bounds__(TB) k_substep1(Geo g, const double *__restrict
     vt2 = vt2 / 4.0 + g.Rinf[i] * g.omega_frame;
     vt2 = vt2 * vt2;
     double tmp = -gradp + 0.5 * (agr[l] + agr[lim]) + vt2 * g.InvRinf[i];
-    vradint[l] = vrad[l] + dt * tmp;
+    double v = vrad[l] + dt * tmp;
+    if (accr) {
+      double a = tmp;
+      if (dragr) {
+        const double t2 = dragr[l] * 2.0 / (rho[l] + rho[lim]);
+        v += dt * t2;
+        a += t2;
+      }
+      accr[l] = a;
+    }
+    vradint[l] = v;
   }
   {
     const double supp_torque = g.supp_torque[i];
@@ -856,6 +1356,15 @@ __global__ void __launch_bounds__(TB) k_substep1(Geo g, const double *__restrict
     double tmp = -gradp + 0.5 * (agt[l] + agt[ljm]);
     double v = vtheta[l] + dt * tmp;
     v += dt * supp_torque;
+    if (acct) {
+      double a = tmp + supp_torque;
+      if (dragt) {
+        const double t2 = dragt[l] * 2.0 / (rho[l] + rho[ljm]);
+        v += dt * t2;
+        a += t2;
+      }
+      acct[l] = a;
+    }
     vthetaint[l] = v;
   }
 }
@@ -1452,6 +1961,7 @@ struct RadQ {
   const int *guard;             // skip the launch while *guard == 0
   double of;
   int ntile_j, ntiles;
+  int nq;                       // 6; 5 for the pebble fluid (no energy: quantity 4 skipped)
 };
 
 // ring geometry of one radial tile, staged in LDS: entry r is ring i0-4+r
@@ -1555,9 +2065,10 @@ __global__ void __launch_bounds__(TB) k_transport_rad(Geo g, RadQ q, double dt) 
   // XCD x (= b % 8) gets the contiguous band of tiles x*per .. x*per+per-1,
   // all six quantities of each
   const int b = blockIdx.x, slot = b & 7, rest = b >> 3;
-  const int which = rest % RC_NQ;
+  int which = rest % q.nq;
+  if (q.nq == RC_NQ - 1 && which == 4) which = 5;
   const int per = (q.ntiles + 7) >> 3;
-  const int tile = slot * per + rest / RC_NQ;
+  const int tile = slot * per + rest / q.nq;
   if (tile >= q.ntiles) return;                // whole workgroup
   const int tj = tile % q.ntile_j, ti = tile / q.ntile_j;
   const int i0 = ti * RC_I, i1 = min(i0 + RC_I, g.nr);
@@ -1589,8 +2100,9 @@ __global__ void __launch_bounds__(TB) k_transport_rad(Geo g, RadQ q, double dt) 
 // The chain of NSEC dependent additions per ring is the cost; a workgroup's
 // rings share it.
 __global__ void __launch_bounds__(256) k_ring_means(const double *__restrict__ f, int nr, int ns, int rows,
-                                                     double *__restrict__ out) {
+                                                     double *__restrict__ out, const int *__restrict__ guard) {
   extern __shared__ double sh[];       // rows * ns
+  if (guard && *guard == 0) return;
   const int r0 = blockIdx.x * rows;
   const int nrow = min(rows, nr - r0);
   const int n = nrow * ns;
@@ -1682,7 +2194,8 @@ __global__ void __launch_bounds__(TT_T) k_transport_theta(Geo g, const double *_
                                                           double *__restrict__ orp, double *__restrict__ orm,
                                                           double *__restrict__ otp, double *__restrict__ otm,
                                                           double *__restrict__ oe, double *__restrict__ orho,
-                                                          double dt, const int *__restrict__ guard) {
+                                                          double dt, const int *__restrict__ guard, int nq) {
+  // nq quantities after rho: 5 (momenta and energy), 4 for the pebble fluid
   extern __shared__ double sh[];       // 2*ns
   if (guard && *guard == 0) return;
   const int i = blockIdx.x, ns = g.ns, tid = threadIdx.x;
@@ -1775,10 +2288,10 @@ __global__ void __launch_bounds__(TT_T) k_transport_theta(Geo g, const double *_
     const int j = tid + k * TT_T;
     nxt[k] = (j < ns) ? xrp[base + j] : 0.0;
   }
-  for (int which = 0; which < 5; which++) {
+  for (int which = 0; which < nq; which++) {
 #pragma unroll
     for (int k = 0; k < NPT; k++) qv[k] = nxt[k];
-    if (which < 4) {
+    if (which < nq - 1) {
       const double *src = ins[which + 1];
 #pragma unroll
       for (int k = 0; k < NPT; k++) {
@@ -1909,6 +2422,7 @@ extern "C" int thorin_dev_create(thorin_ctx *c) {
       return -1;
     }
     HCHECK(hipFuncSetAttribute((const void *)k_ring_means, hipFuncAttributeMaxDynamicSharedMemorySize, (int)rs_lds));
+    HCHECK(hipFuncSetAttribute((const void *)k_peb_cfl, hipFuncAttributeMaxDynamicSharedMemorySize, (int)need));
   }
   HCHECK(hipStreamCreateWithFlags(&d->s, hipStreamNonBlocking));
   d->prof_pend = new std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>>();
@@ -1920,8 +2434,8 @@ extern "C" int thorin_dev_create(thorin_ctx *c) {
   }
   // 1D arrays: 27 slots of (nr+3) doubles
   const int n1 = c->nr + 3;
-  HCHECK(hipMalloc(&d->d1base, sizeof(double) * (size_t)n1 * 32));
-  HCHECK(hipMemsetAsync(d->d1base, 0, sizeof(double) * (size_t)n1 * 32, d->s));
+  HCHECK(hipMalloc(&d->d1base, sizeof(double) * (size_t)n1 * 40));
+  HCHECK(hipMemsetAsync(d->d1base, 0, sizeof(double) * (size_t)n1 * 40, d->s));
   double *p1 = d->d1base;
   const double *src[21] = {c->Rinf, c->Rsup, c->Rmed, c->Surf, c->InvRmed, c->InvSurf, c->InvDiffRmed,
                            c->InvDiffRsup, c->InvRinf, c->OmegaInv, c->SigmaMed, c->SigmaInf, c->EnergyMed,
@@ -1940,22 +2454,34 @@ extern "C" int thorin_dev_create(thorin_ctx *c) {
   g.pow_rinf15 = p1 + 16 * n1; g.cav_med = p1 + 17 * n1; g.cav_inf = p1 + 18 * n1; g.cfl_pow2 = p1 + 19 * n1;
   g.supp_torque = p1 + 20 * n1;
   g.gcs = p1 + 21 * n1; g.gtemp = p1 + 22 * n1; g.Vmoy = p1 + 23 * n1; g.VMed = p1 + 24 * n1;
-  g.ringmin = p1 + 25 * n1;
+  g.ringmax = p1 + 25 * n1;
   d->ringtmp = p1 + 26 * n1;
   g.sig_avr = p1 + 27 * n1;
   g.ist_avr = p1 + 28 * n1;
+  g.peb_dens0 = p1 + 29 * n1;
+  g.peb_vr0 = p1 + 30 * n1;
+  g.peb_vt0 = p1 + 31 * n1;
+  g.tau_avr = p1 + 32 * n1;
+  g.phavr = p1 + 33 * n1;
+  g.pist_avr = p1 + 34 * n1;
+  g.pebmax = p1 + 35 * n1;
+  g.pvmed = p1 + 36 * n1;
   int *ip;
-  HCHECK(hipMalloc(&ip, sizeof(int) * (size_t)n1 * 2));
+  HCHECK(hipMalloc(&ip, sizeof(int) * (size_t)n1 * 3));
   g.Nshift = ip;
   g.dampring = ip + n1;
+  g.pdampring = ip + 2 * n1;
   {
-    std::vector<int> dr(n1, 0);
+    // rings of DampingBoundary (Damping YES) and of DampPebbles (always)
+    std::vector<int> dr(2 * n1, 0);
     const thorin_params *p = &c->p;
-    if (p->damping_on) {
-      double DRin = p->rmin * p->dampingrminfrac, DRout = p->rmax * p->dampingrmaxfrac;
-      for (int i = 0; i < c->nr; i++) dr[i] = (c->Rmed[i] < DRin || c->Rmed[i] > DRout) ? 1 : 0;
+    double DRin = p->rmin * p->dampingrminfrac, DRout = p->rmax * p->dampingrmaxfrac;
+    for (int i = 0; i < c->nr; i++) {
+      const int z = (c->Rmed[i] < DRin || c->Rmed[i] > DRout) ? 1 : 0;
+      dr[i] = p->damping_on ? z : 0;
+      dr[n1 + i] = z;
     }
-    HCHECK(hipMemcpyAsync(ip + n1, dr.data(), sizeof(int) * n1, hipMemcpyHostToDevice, d->s));
+    HCHECK(hipMemcpyAsync(ip + n1, dr.data(), sizeof(int) * 2 * n1, hipMemcpyHostToDevice, d->s));
     HCHECK(hipStreamSynchronize(d->s));
   }
   const thorin_params *p = &c->p;
@@ -1992,6 +2518,12 @@ extern "C" int thorin_dev_create(thorin_ctx *c) {
   g.indirect_term = p->indirect_term;
   g.exclude_hill = p->exclude_hill;
   g.hillcut = p->hillcut;
+  g.pebbles = p->pebbles;
+  g.back_reaction = p->back_reaction;
+  g.pebble_gravity = p->pebble_gravity;
+  g.palpha = p->pebblealpha;
+  g.pbulk = p->pebblebulkdens / TH_RHO2CGS;
+  g.schmidt = p->schmidtnumber;
   // LP85 thresholds exactly as the reference's expressions (Opacity.c:205-220)
   g.lp_t1 = pow(2.0e-4 / 2.0e16, -1.0 / 9.0);
   g.lp_t2 = pow(2.0e16 / 5.0e-3, 1.0 / 8.0);
@@ -2036,6 +2568,7 @@ extern "C" void thorin_dev_destroy(thorin_ctx *c) {
   if (d->sorc) (void)hipFree(d->sorc);
   if (d->pstellar) (void)hipFree(d->pstellar);
   if (d->mresp) (void)hipFree(d->mresp);
+  if (d->acc_stage) (void)hipHostFree(d->acc_stage);
   {
     float *f[8] = {d->m32.ml, d->m32.mip, d->m32.mim, d->m32.mjp, d->m32.mjm, d->m32.r, d->m32.e0, d->m32.e1};
     for (int k = 0; k < 8; k++)
@@ -2085,6 +2618,25 @@ extern "C" int thorin_dev_upload_init(thorin_ctx *c) {
   return 0;
 }
 
+// pebble fields and damping targets of thorin_init_pebbles
+extern "C" int thorin_dev_upload_pebbles(thorin_ctx *c) {
+  thorin_dev *d = c->dev;
+  const size_t nb = sizeof(double) * (size_t)c->nr * c->ns;
+  const int n1 = c->nr + 3;
+  HCHECK(hipMemcpyAsync(d->b[B_PDENS], c->h_pdens, nb, hipMemcpyHostToDevice, d->s));
+  HCHECK(hipMemcpyAsync(d->b[B_PVR], c->h_pvr, nb, hipMemcpyHostToDevice, d->s));
+  HCHECK(hipMemcpyAsync(d->b[B_PVT], c->h_pvt, nb, hipMemcpyHostToDevice, d->s));
+  HCHECK(hipMemcpyAsync(d->b[B_PSIZE], c->h_psize, nb, hipMemcpyHostToDevice, d->s));
+  HCHECK(hipMemcpyAsync(d->b[B_TAU], c->h_tau, nb, hipMemcpyHostToDevice, d->s));
+  HCHECK(hipMemcpyAsync(d->b[THORIN_F_ETA], c->h_eta, nb, hipMemcpyHostToDevice, d->s));
+  HCHECK(hipMemcpyAsync((double *)d->g.peb_dens0, c->PebDensInit, sizeof(double) * n1, hipMemcpyHostToDevice, d->s));
+  HCHECK(hipMemcpyAsync((double *)d->g.peb_vr0, c->PebVradInit, sizeof(double) * n1, hipMemcpyHostToDevice, d->s));
+  HCHECK(hipMemcpyAsync((double *)d->g.peb_vt0, c->PebVthetaInit, sizeof(double) * n1, hipMemcpyHostToDevice, d->s));
+  HCHECK(hipStreamSynchronize(d->s));
+  d->cfl_ready = 0;
+  return 0;
+}
+
 extern "C" int thorin_upload_field(thorin_ctx *c, int field, const double *host, size_t count) {
   if (!c || !c->dev || field < 0 || field >= THORIN_NFIELDS || !host) { thorin_set_error("bad upload args"); return -1; }
   thorin_dev *d = c->dev;
@@ -2131,7 +2683,7 @@ extern "C" int thorin_ring_means(thorin_ctx *c, int field, double *out) {
   if (field < 0 || field >= THORIN_NFIELDS) return -1;
   hipLaunchKernelGGL(k_ring_means, dim3((unsigned)((d->nr + d->rs_rows - 1) / d->rs_rows)), dim3(256),
                      sizeof(double) * (size_t)d->rs_rows * d->ns, d->s, (const double *)d->b[field], d->nr, d->ns,
-                     d->rs_rows, d->ringtmp);
+                     d->rs_rows, d->ringtmp, (const int *)nullptr);
   HCHECK(hipGetLastError());
   HCHECK(hipMemcpyAsync(out, d->ringtmp, sizeof(double) * d->nr, hipMemcpyDeviceToHost, d->s));
   HCHECK(hipStreamSynchronize(d->s));
@@ -2368,6 +2920,136 @@ static double minstep_rebound(const thorin_planets *sys) {
 
 
 // AlgoGas (SourceEuler.c:210-317): one DT
+// AccretePebblesOntoPlanets STEPS 0-2 on the host (Pebbles.c:374-441): the
+// planet's scalars, vt1D next to it (ordered ring means of the gas vtheta,
+// mpi_make1Dprofile order), the annulus of its Hill sphere, the annulus'
+// mean Stokes number in the reference's sequential order and Reff.  The two
+// vtheta rings and the annulus' Stokes numbers come from the device (the
+// values the substep starts with).
+static int peb_accrete_host(thorin_ctx *c, const thorin_planets *sys, PebAccK *P) {
+  thorin_dev *d = c->dev;
+  const int nr = d->nr, ns = d->ns;
+  struct Pl { double Rp, Rhill, frac; int iv, imin, imax; } pl[PEB_MAXPL];
+  int lo = nr, hi = -1;
+  P->np = sys->nb;
+  for (int k = 0; k < sys->nb; k++) {
+    const double X = sys->x[k], Y = sys->y[k], Z = sys->z[k], M = sys->mass[k];
+    double Rp = X * X + Y * Y;
+    const double Rp3D = sqrt(Rp + Z * Z);
+    Rp = sqrt(Rp);
+    pl[k].Rp = Rp;
+    pl[k].Rhill = Rp3D * pow(1.0 / 3.0 * M, 1.0 / 3.0);
+    P->x[k] = X; P->y[k] = Y; P->z[k] = Z; P->m[k] = M;
+    P->om[k] = pow(Rp, -1.5);
+    double ifrac;   // GetGlobalIFrac (LowTasks.c:21-31)
+    if (Rp < c->GlobalRmed[0]) ifrac = 0.0;
+    else if (Rp > c->GlobalRmed[nr - 1]) ifrac = (double)nr - 1.0;
+    else {
+      int i = 0;
+      while (c->GlobalRmed[i] <= Rp) i++;
+      ifrac = (double)i + (Rp - c->GlobalRmed[i - 1]) / (c->GlobalRmed[i] - c->GlobalRmed[i - 1]) - 1.0;
+    }
+    pl[k].frac = ifrac - floor(ifrac);
+    pl[k].iv = (int)ifrac;
+    int imax = nr - 1, imin = 0;
+    while (imax >= imin && c->Rinf[imax] > (Rp + pl[k].Rhill)) imax--;
+    while (imin <= imax && c->Rsup[imin] < (Rp - pl[k].Rhill)) imin++;
+    pl[k].imin = imin;
+    pl[k].imax = imax;
+    P->imin[k] = imin;
+    P->imax[k] = imax;
+    if (imin <= imax) { lo = imin < lo ? imin : lo; hi = imax > hi ? imax : hi; }
+  }
+  // staging: vtheta rings iv, iv+1 of each planet, then the Stokes rings lo..hi
+  const size_t nv = (size_t)2 * sys->nb * ns, nt = hi >= lo ? (size_t)(hi - lo + 1) * ns : 0;
+  if (nv + nt > d->acc_stage_n) {
+    if (d->acc_stage) HCHECK(hipHostFree(d->acc_stage));
+    d->acc_stage = nullptr;
+    HCHECK(hipHostMalloc(&d->acc_stage, sizeof(double) * (nv + nt), hipHostMallocDefault));
+    d->acc_stage_n = nv + nt;
+  }
+  double *sv = d->acc_stage, *st = d->acc_stage + nv;
+  for (int k = 0; k < sys->nb; k++)
+    for (int q = 0; q < 2; q++) {
+      const int r = pl[k].iv + q;
+      if (r < nr)
+        HCHECK(hipMemcpyAsync(sv + (size_t)(2 * k + q) * ns, d->b[B_VTHETA] + (size_t)r * ns, sizeof(double) * ns,
+                              hipMemcpyDeviceToHost, d->s));
+    }
+  if (nt)
+    HCHECK(hipMemcpyAsync(st, d->b[B_TAU] + (size_t)lo * ns, sizeof(double) * nt, hipMemcpyDeviceToHost, d->s));
+  HCHECK(hipStreamSynchronize(d->s));
+  for (int k = 0; k < sys->nb; k++) {
+    double vt1[2];
+    for (int q = 0; q < 2; q++) {
+      vt1[q] = 0.0;   // vt1D[nr] stays 0 (mpi_make1Dprofile fills rings 0..nr-1)
+      if (pl[k].iv + q < nr) {
+        const double *row = sv + (size_t)(2 * k + q) * ns;
+        double s = 0.0;
+        for (int j = 0; j < ns; j++) s += row[j];
+        vt1[q] = s / (double)ns;
+      }
+    }
+    const double X = P->x[k], Y = P->y[k], M = P->m[k], Rp = pl[k].Rp;
+    const double VX = sys->vx[k], VY = sys->vy[k], VZ = sys->vz[k];
+    const double flow = vt1[0] * (1.0 - pl[k].frac) + vt1[1] * pl[k].frac + Rp * c->st.omega_frame;
+    const double flowx = -flow * Y / Rp, flowy = flow * X / Rp;
+    const double vrel = sqrt((VX - flowx) * (VX - flowx) + (VY - flowy) * (VY - flowy) + VZ * VZ);
+    const double Mt = sqrt(1.0 / 3.0) * pow(vrel, 3.0) / P->om[k];
+    int ncell = 0;
+    double tau_mean = 0.0;
+    for (int i = pl[k].imin; i <= pl[k].imax; i++) {
+      const double *row = st + (size_t)(i - lo) * ns;
+      for (int j = 0; j < ns; j++) { tau_mean += row[j]; ncell++; }
+    }
+    tau_mean /= (double)ncell;
+    double Reff, fac;
+    if (M < Mt) {
+      const double Rbondi = M / vrel / vrel;
+      const double tbondi = Rbondi / vrel;
+      fac = sqrt(tau_mean * pow(Rp, 1.5) / tbondi);
+      Reff = Rbondi * fac;
+    } else {
+      fac = pow(tau_mean / 0.1, 1.0 / 3.0);
+      Reff = pl[k].Rhill * fac;
+    }
+    if (Reff > pl[k].Rhill) Reff = pl[k].Rhill;
+    P->reff[k] = Reff;
+    P->vrel[k] = vrel;
+    P->mt[k] = Mt;
+  }
+  return 0;
+}
+
+// Transport (TransportEuler.c:34-47) of one fluid: radial van Leer of the
+// momenta, [energy,] density into the X buffers, azimuthal pass with the
+// FARGO shift back into the fields and the momenta, ComputeVelocities.  be < 0:
+// no energy (TransportPebbles).  g.VMed is the fluid's vtheta ring mean.
+static int transport(thorin_dev *d, const Geo &g, const RadQ &q, int brho, int bvr, int bvt, int be, double dt,
+                     const int *guard) {
+  const int nr = d->nr, ns = d->ns;
+  const dim3 gall = cell_grid(d, nr), tb(TB);
+  LAUNCH(k_transport_rad, dim3((unsigned)((q.ntiles + 7) / 8 * 8 * q.nq)), tb, 0, g, q, dt);
+  double *xe = be >= 0 ? d->b[B_XE] : nullptr, *oe = be >= 0 ? d->b[be] : nullptr;
+  const int nq = be >= 0 ? 5 : 4;
+#define TT_LAUNCH(NPT)                                                                                        \
+  LAUNCH(k_transport_theta<NPT>, dim3(nr), dim3(TT_T), sizeof(double) * 2 * ns, g, d->b[bvt], d->b[B_XRP],     \
+         d->b[B_XRM], d->b[B_XTP], d->b[B_XTM], xe, d->b[B_XRHO], d->b[B_RP], d->b[B_RM], d->b[B_TP], d->b[B_TM], \
+         oe, d->b[brho], dt, guard, nq)
+  switch ((ns + TT_T - 1) / TT_T) {
+    case 1: TT_LAUNCH(1); break;
+    case 2: TT_LAUNCH(2); break;
+    case 3: TT_LAUNCH(3); break;
+    case 4: TT_LAUNCH(4); break;
+    case 5: TT_LAUNCH(5); break;
+    default: TT_LAUNCH(6); break;
+  }
+#undef TT_LAUNCH
+  LAUNCH(k_velocities, gall, tb, 0, g, d->b[brho], d->b[B_RP], d->b[B_RM], d->b[B_TP], d->b[B_TM], d->b[bvr],
+         d->b[bvt], guard);
+  return 0;
+}
+
 extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_fn nbody, void *nbody_user,
                               thorin_step_log *log) {
   if (need_dev(c)) return -1;
@@ -2379,6 +3061,11 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
   double dtemp = 0.0, dt = DT;
   int rc = 0;
   if (sys->nb > 8) { thorin_set_error("this build supports at most 8 planets in the force kernel"); return -1; }
+  const int peb = p->pebbles;
+  if (peb && !c->peb_ready) {
+    thorin_set_error("PebbleAccretion YES: the pebble disk must be set up with thorin_init_pebbles first");
+    return -1;
+  }
   if (log) memset(log, 0, sizeof(*log));
   d->g.omega_frame = c->st.omega_frame;
   const dim3 tb(TB), rb(RB);
@@ -2393,9 +3080,13 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
     // formed here.
     const double invdtreb_sq = minstep_rebound(sys);
     if (!d->cfl_ready || invdtreb_sq != d->cfl_invdtreb) {
+      if (peb)   // PebbleStokesNumbers + CriticalCharTime (SourceEuler.c:236, 241)
+        LAUNCH(k_peb_cfl, dim3(nr), rb, sizeof(double) * 2 * ns, d->g, d->b[B_DENS], d->b[B_PSIZE], d->b[B_TAU],
+               d->b[B_PVR], d->b[B_PVT], d->b[B_VRAD], d->b[B_VTHETA], (const int *)nullptr);
       LAUNCH(k_eos_cfl, dim3(nr), rb, eos_lds(d, 0), d->g, d->b[B_DENS], d->b[B_ENERGY],
-             d->b[B_TEMPER], d->b[B_CS], d->b[B_VRAD], d->b[B_VTHETA], invdtreb_sq, 0, (const int *)nullptr);
-      LAUNCH(k_cfl_final, dim3(1), rb, 0, d->g, d->ctl, (const double *)nullptr, 0, (const int *)nullptr);
+             d->b[B_TEMPER], d->b[B_CS], d->b[B_VRAD], d->b[B_VTHETA], 0, (const int *)nullptr);
+      LAUNCH(k_cfl_final, dim3(1), rb, 0, d->g, d->ctl, (const double *)nullptr, 0, invdtreb_sq,
+             (const int *)nullptr);
       HCHECK(hipGetLastError());
       HCHECK(hipMemcpyAsync(d->hctl, d->ctl, sizeof(DevCtl), hipMemcpyDeviceToHost, d->s));
       HCHECK(hipStreamSynchronize(d->s));
@@ -2416,6 +3107,40 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
     if (deltaT < newdt) newdt = deltaT;
     const int N = (int)(ceil(deltaT / newdt));
     dt = (DT - dtemp) / (double)N;
+    // ---- AccretePebblesOntoPlanets, EvaporatePebbles (SourceEuler.c:249-250)
+    if (peb && sys->nb > 0) {
+      PebAccK P;
+      memset(&P, 0, sizeof P);
+      if (peb_accrete_host(c, sys, &P)) return -1;
+      LAUNCH(k_peb_accrete, dim3(1), dim3(ACC_T), 0, d->g, P, d->b[B_PDENS], d->b[B_PVR], d->b[B_PVT], d->b[B_TAU],
+             d->b[B_CS], d->b[B_ABS], d->b[B_ORD], d->b[B_MACC], dt, d->ctl);
+      HCHECK(hipGetLastError());
+      HCHECK(hipMemcpyAsync(d->hctl, d->ctl, sizeof(DevCtl), hipMemcpyDeviceToHost, d->s));
+      HCHECK(hipStreamSynchronize(d->s));
+      if (d->hctl->acc_fail) {
+        thorin_set_error("pebble accretion: integration of the column mass did not converge");
+        return -1;
+      }
+      // STEP 7 (Pebbles.c:529-537)
+      for (int k = 0; k < sys->nb; k++) {
+        double M = sys->mass[k];
+        double PX = sys->vx[k] * M, PY = sys->vy[k] * M;
+        const double dM = d->hctl->pacc[k][0];
+        PX += d->hctl->pacc[k][1];
+        PY += d->hctl->pacc[k][2];
+        M += dM;
+        sys->mass[k] = M;
+        sys->solid[k] = sys->solid[k] + dM;
+        sys->vx[k] = PX / M;
+        sys->vy[k] = PY / M;
+      }
+    }
+    if (peb && p->evaporationrate > 0.0) {
+      double facc = p->evaporationrate * dt;
+      if (facc > 1.0) facc = 1.0;
+      LAUNCH(k_peb_evaporate, gall, tb, 0, d->g, d->b[B_PDENS], d->b[B_TEMPER], facc,
+             p->evaporationtemperature / c->k.t2si);
+    }
     dtemp += dt;
     // ---- FillForcesArrays
     double indir_px = 0.0, indir_py = 0.0;
@@ -2487,11 +3212,20 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
         rr.r[2 * k + 1] = ii + 1 < nr ? ii + 1 : -1;
       }
     }
-    LAUNCH(k_forces, dim3(gall.x + (ntq ? 1u : 0u)), tb, 0, d->g, d->b[B_DENS], d->b[B_CS], d->b[B_ABS],
-           d->b[B_ORD], d->b[B_AGR], d->b[B_AGT], d->pk, sys->nb, mt, d->red, rr, 2 * ntq, d->ctl);
+    PebF pf;
+    pf.prho = d->b[B_PDENS];
+    pf.tau = d->b[B_TAU];
+    pf.pagr = d->b[B_PAGR];
+    pf.pagt = d->b[B_PAGT];
+    if (peb)
+      LAUNCH(k_forces<true>, dim3(gall.x + (ntq ? 1u : 0u)), tb, 0, d->g, d->b[B_DENS], d->b[B_CS], d->b[B_ABS],
+             d->b[B_ORD], d->b[B_AGR], d->b[B_AGT], d->pk, sys->nb, mt, d->red, rr, 2 * ntq, d->ctl, pf);
+    else
+      LAUNCH(k_forces<false>, dim3(gall.x + (ntq ? 1u : 0u)), tb, 0, d->g, d->b[B_DENS], d->b[B_CS], d->b[B_ABS],
+             d->b[B_ORD], d->b[B_AGR], d->b[B_AGT], d->pk, sys->nb, mt, d->red, rr, 2 * ntq, d->ctl, pf);
     LAUNCH(k_forces_reduce, dim3(2 + 2 * sys->nb), rb, 0, d->red, d->nblk_cells, d->ctl);
     LAUNCH(k_indirect, gall, tb, 0, d->g, d->b[B_AGR], d->b[B_AGT], d->b[B_ABS], d->b[B_ORD], d->ctl, indir_px,
-           indir_py);
+           indir_py, peb ? d->b[B_PAGR] : (double *)nullptr, peb ? d->b[B_PAGT] : (double *)nullptr);
     HCHECK(hipGetLastError());
     // ---- AdvanceSystemFromDisk (host; needs the reductions only when a planet feels the disk)
     int anyfeel = 0;
@@ -2534,8 +3268,22 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
     if (p->damping_on)
       LAUNCH(k_boundary, gall, tb, 0, d->g, d->b[B_VRAD], d->b[B_VTHETA], d->b[B_DENS], d->b[B_ENERGY], dt,
              (const int *)nullptr);
+    if (peb) {
+      // DetectCrashPebbles, then SourceTermsPebbles (SourceEuler.c:274, 291)
+      LAUNCH(k_peb_damp, gall, tb, 0, d->g, d->b[B_PDENS], d->b[B_PVR], d->b[B_PVT], dt, 1, d->ctl,
+             (const int *)nullptr);
+      LAUNCH(k_peb_source, gall, tb, 0, d->g, d->b[B_PDENS], d->b[B_PVR], d->b[B_PVT], d->b[B_TAU], d->b[B_VRAD],
+             d->b[B_VTHETA], d->b[B_PAGR], d->b[B_PAGT], d->b[B_PACCR], d->b[B_PACCT], d->b[B_DRAGR], d->b[B_DRAGT]);
+    }
+    const int br = peb && p->back_reaction;
     LAUNCH(k_substep1, gall, tb, 0, d->g, d->b[B_DENS], d->b[B_ENERGY], d->b[B_VRAD], d->b[B_VTHETA], d->b[B_AGR],
-           d->b[B_AGT], d->b[B_VRADINT], d->b[B_VTHETAINT], dt, d->ctl);
+           d->b[B_AGT], d->b[B_VRADINT], d->b[B_VTHETAINT], dt, d->ctl, peb ? d->b[B_GACCR] : (double *)nullptr,
+           peb ? d->b[B_GACCT] : (double *)nullptr, br ? d->b[B_DRAGR] : (const double *)nullptr,
+           br ? d->b[B_DRAGT] : (const double *)nullptr);
+    if (peb)   // SubStep1Pebbles + ParticleDiffusion (SourceEuler.c:293-296)
+      LAUNCH(k_peb_substep1, gall, tb, 0, d->g, d->b[B_PVR], d->b[B_PVT], d->b[B_TAU], d->b[B_PACCR], d->b[B_PACCT],
+             d->b[B_GACCR], d->b[B_GACCT], d->b[B_VRAD], d->b[B_VTHETA], d->b[B_DENS], d->b[B_PDENS], dt,
+             p->diffusive_particles);
     LAUNCH(k_divstress, gall, tb, 0, d->g, d->b[B_VRADINT], d->b[B_VTHETAINT], d->b[B_DENS], d->b[B_DIVV],
            d->b[B_TAURR], d->b[B_TAURP], d->b[B_TAUPP]);
     {
@@ -2633,30 +3381,46 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
         q.of = c->st.omega_frame;
         q.ntile_j = (ns + TB - 1) / TB;
         q.ntiles = q.ntile_j * ((nr + RC_I - 1) / RC_I);
-        LAUNCH(k_transport_rad, dim3((unsigned)((q.ntiles + 7) / 8 * 8 * RC_NQ)), tb, 0, d->g, q, dt);
-#define TT_LAUNCH(NPT)                                                                                         \
-  LAUNCH(k_transport_theta<NPT>, dim3(nr), dim3(TT_T), sizeof(double) * 2 * ns, d->g, d->b[B_VTHETA], d->b[B_XRP],  \
-         d->b[B_XRM], d->b[B_XTP], d->b[B_XTM], d->b[B_XE], d->b[B_XRHO], d->b[B_RP], d->b[B_RM], d->b[B_TP],     \
-         d->b[B_TM], d->b[B_ENERGY], d->b[B_DENS], dt, guard)
-        switch ((ns + TT_T - 1) / TT_T) {
-          case 1: TT_LAUNCH(1); break;
-          case 2: TT_LAUNCH(2); break;
-          case 3: TT_LAUNCH(3); break;
-          case 4: TT_LAUNCH(4); break;
-          case 5: TT_LAUNCH(5); break;
-          default: TT_LAUNCH(6); break;
-        }
-#undef TT_LAUNCH
-        LAUNCH(k_velocities, gall, tb, 0, d->g, d->b[B_DENS], d->b[B_RP], d->b[B_RM], d->b[B_TP], d->b[B_TM],
-               d->b[B_VRAD], d->b[B_VTHETA], guard);
+        q.nq = RC_NQ;
+        if (transport(d, d->g, q, B_DENS, B_VRAD, B_VTHETA, B_ENERGY, dt, guard)) return -1;
       }
       if (p->damping_on)
         LAUNCH(k_boundary, gall, tb, 0, d->g, d->b[B_VRAD], d->b[B_VTHETA], d->b[B_DENS], d->b[B_ENERGY], dt, guard);
+      if (peb) {
+        // ---- EvolvePebbleDisk (SourceEuler.c:312, Pebbles.c:699-708): damping,
+        // TransportPebbles (TransportEuler.c:449-505: the gas scheme without
+        // energy; ComputeResiduals with the pebble vtheta's ring mean, whose
+        // Nshift overwrites the gas one as in the reference), damping; then
+        // the next substep's Stokes numbers and pebble CFL limit
+        LAUNCH(k_peb_damp, gall, tb, 0, d->g, d->b[B_PDENS], d->b[B_PVR], d->b[B_PVT], dt, 0, d->ctl, guard);
+        hipLaunchKernelGGL(k_ring_means, dim3((unsigned)((nr + d->rs_rows - 1) / d->rs_rows)), dim3(256),
+                           sizeof(double) * (size_t)d->rs_rows * ns, d->s, (const double *)d->b[B_PVT], nr, ns,
+                           d->rs_rows, d->g.pvmed, guard);
+        Geo gp = d->g;
+        gp.VMed = d->g.pvmed;
+        RadQ q;
+        q.rho = d->b[B_PDENS];
+        q.vr = d->b[B_PVR];
+        q.vt = d->b[B_PVT];
+        q.e = nullptr;
+        const int outs[RC_NQ] = {B_XRP, B_XRM, B_XTP, B_XTM, B_XE, B_XRHO};
+        for (int k = 0; k < RC_NQ; k++) q.out[k] = d->b[outs[k]];
+        q.guard = guard;
+        q.of = c->st.omega_frame;
+        q.ntile_j = (ns + TB - 1) / TB;
+        q.ntiles = q.ntile_j * ((nr + RC_I - 1) / RC_I);
+        q.nq = RC_NQ - 1;
+        if (transport(d, gp, q, B_PDENS, B_PVR, B_PVT, -1, dt, guard)) return -1;
+        LAUNCH(k_peb_damp, gall, tb, 0, d->g, d->b[B_PDENS], d->b[B_PVR], d->b[B_PVT], dt, 0, d->ctl, guard);
+        LAUNCH(k_peb_cfl, dim3(nr), rb, sizeof(double) * 2 * ns, d->g, d->b[B_DENS], d->b[B_PSIZE], d->b[B_TAU],
+               d->b[B_PVR], d->b[B_PVT], d->b[B_VRAD], d->b[B_VTHETA], guard);
+      }
       // ---- end of the substep: T, cs and their ring means, and the CFL
       // minima of the next substep (its planets are those of now)
       LAUNCH(k_eos_cfl, dim3(nr), rb, eos_lds(d, 1), d->g, d->b[B_DENS], d->b[B_ENERGY],
-             d->b[B_TEMPER], d->b[B_CS], d->b[B_VRAD], d->b[B_VTHETA], d->cfl_invdtreb, 1, guard);
-      LAUNCH(k_cfl_final, dim3(1), rb, 0, d->g, d->ctl, (const double *)d->pstellar, d->nblk_cells, guard);
+             d->b[B_TEMPER], d->b[B_CS], d->b[B_VRAD], d->b[B_VTHETA], 1, guard);
+      LAUNCH(k_cfl_final, dim3(1), rb, 0, d->g, d->ctl, (const double *)d->pstellar, d->nblk_cells,
+             d->cfl_invdtreb, guard);
       HCHECK(hipGetLastError());
       // one sync per substep: SOR result, crash flags, dt_stellar, the next
       // CFL minimum, the planet accelerations
@@ -2705,7 +3469,8 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
       }
     }
     if (prof_flush(d)) return -1;
-    if (d->hctl->crash_e) {
+    // DetectCrash (SourceEuler.c:272-274): the last test of the substep decides
+    if (peb ? d->hctl->crash_peb : d->hctl->crash_e) {
       c->st.already_crashed++;
       if (log) { log->crashed[log->nsub - 1] = 1; log->crashed_any = 1; }
       if (!rc) rc = 1;
@@ -2883,9 +3648,11 @@ extern "C" int thorin_rad_system(thorin_ctx *c, double dt, double *dt_used) {
   const dim3 gall = cell_grid(d, nr), tb(TB), rb(RB);
   if (dt <= 0.0) {
     // the hydro CFL step of the current fields (ConditionCFL without planets)
-    LAUNCH(k_eos_cfl, dim3(nr), rb, eos_lds(d, 0), d->g, d->b[B_DENS], d->b[B_ENERGY], d->b[B_TEMPER],
-           d->b[B_CS], d->b[B_VRAD], d->b[B_VTHETA], 0.0, 0, (const int *)nullptr);
-    LAUNCH(k_cfl_final, dim3(1), rb, 0, d->g, d->ctl, (const double *)nullptr, 0, (const int *)nullptr);
+    Geo gg = d->g;
+    gg.pebbles = 0;
+    LAUNCH(k_eos_cfl, dim3(nr), rb, eos_lds(d, 0), gg, d->b[B_DENS], d->b[B_ENERGY], d->b[B_TEMPER],
+           d->b[B_CS], d->b[B_VRAD], d->b[B_VTHETA], 0, (const int *)nullptr);
+    LAUNCH(k_cfl_final, dim3(1), rb, 0, gg, d->ctl, (const double *)nullptr, 0, 0.0, (const int *)nullptr);
     HCHECK(hipMemcpyAsync(d->hctl, d->ctl, sizeof(DevCtl), hipMemcpyDeviceToHost, d->s));
     HCHECK(hipStreamSynchronize(d->s));
     dt = d->hctl->newdt_raw;

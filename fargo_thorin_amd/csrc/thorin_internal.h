@@ -55,6 +55,12 @@ struct thorin_ctx {
   /* host staging copies used during initialisation (nfield doubles each) */
   double *h_dens, *h_vrad, *h_vtheta, *h_energy, *h_label, *h_cs, *h_press, *h_temper;
   double *h_abs, *h_ord;
+  /* pebble fluid (allocated by thorin_init_pebbles): host copies of the
+   * initial pebble fields and the damping targets PebDensInit, PebVradInit,
+   * PebVthetaInit (global.h, Pebbles.c:190-193, 206-232) */
+  double *h_pdens, *h_pvr, *h_pvt, *h_psize, *h_tau, *h_eta;
+  double *PebDensInit, *PebVradInit, *PebVthetaInit;
+  int peb_ready;
   struct thorin_dev *dev;
 };
 
@@ -67,6 +73,7 @@ void thorin_cavity_factors(thorin_ctx *c);
 int thorin_dev_create(thorin_ctx *c);
 void thorin_dev_destroy(thorin_ctx *c);
 int thorin_dev_upload_init(thorin_ctx *c);
+int thorin_dev_upload_pebbles(thorin_ctx *c);
 
 #ifdef __cplusplus
 }

@@ -119,3 +119,25 @@ def sor_disk(nrad: int = 2048, nsec: int = 3072, *, base_nrad: int = 128, base_n
     return {"dens": np.ascontiguousarray(dens), "vrad": np.zeros((nrad, nsec)),
             "vtheta": np.ascontiguousarray(np.repeat(vt_ring[:, None], nsec, axis=1)),
             "temper": np.ascontiguousarray(np.repeat(t_ring[:, None], nsec, axis=1))}
+
+
+# RHO2CGS of fondam.h (Msun*1000/(AU*100)^3): code density unit in g/cm^3
+RHO2CGS = 1.98855e30 * 1000.0 / (149597870700.0 * 100.0) ** 3
+
+
+def seeded_pebbles(disk: dict, *, ratio: float = 0.01, st_min: float = 0.01, st_max: float = 0.1,
+                   bulkdens: float = 1.0, adiabind: float = 1.4, seed: int = 4242) -> dict[str, np.ndarray]:
+    """The two-fluid workload's pebble disk on a gas disk of seeded_disk:
+    surface density ratio * Sigma_gas, Stokes numbers log-uniform in
+    [st_min, st_max] per ring (seeded), the grain sizes that give them with
+    the Epstein law of PebbleStokesNumbers (tau = 2 rho_bulk s / Sigma /
+    sqrt(gamma)), vr = 0 and the gas vtheta."""
+    dens = disk["dens"]
+    nrad, nsec = dens.shape
+    u = uniform(seed, np.arange(nrad, dtype=np.uint64))
+    st = np.exp(np.log(st_min) + u * (np.log(st_max) - np.log(st_min)))
+    pbulk = bulkdens / RHO2CGS
+    psize = st[:, None] * dens * math.sqrt(adiabind) / (2.0 * pbulk)
+    return {"pdens": np.ascontiguousarray(ratio * dens), "pvrad": np.zeros((nrad, nsec)),
+            "pvtheta": np.ascontiguousarray(disk["vtheta"].copy()), "psize": np.ascontiguousarray(psize),
+            "stokes": st}

@@ -13,6 +13,8 @@
  *                                                       SourceEuler.c:68-175, SideEuler.c:101-118
  *   thorin_init_state     <- Initialization()          Init.c:72-145 (analytic, InitFromFile and
  *                                                       restart routes)
+ *   thorin_init_pebbles   <- EquilPebbleDisk()/RestartPebbleDisk()  Init.c:138-144,
+ *                                                       Pebbles.c:59-75, 79-232
  *   thorin_upload_field /
  *   thorin_download_field <- the host PolarGrid ->Field arrays (types.h:37-44) that
  *                            WriteDiskPolar() (Output.c:153-189) and ReadfromFile()
@@ -210,6 +212,17 @@ int  thorin_set_state(thorin_ctx *ctx, const thorin_state *s);
  * cs, P, T); mode 2: restart, the same arrays are the gas*N.dat contents. */
 int  thorin_init_state(thorin_ctx *ctx, int mode, const double *dens, const double *vrad,
                        const double *vtheta, const double *temper);
+
+/* Pebble disk (Init.c:138-144), after thorin_init_state when PebbleAccretion
+ * is YES.  mode 0: the coagulation-drift equilibrium of a fresh run from the
+ * initial gas state (EquilPebbleDisk: EtaPressureSupport + InitPebblesViaFlux,
+ * Pebbles.c:59-64, 79-202, 236-270), which also fills PEBBLESIZE,
+ * PEBBLESTOKES and ETA; mode 1: restart, the four host arrays (nrad*nsec
+ * each: pebble density, vrad, vtheta, size) replace the pebble fields
+ * (RestartPebbleDisk, Pebbles.c:67-75) and their ordered ring means become
+ * the damping targets (BckpFieldsForBC, Pebbles.c:206-232). */
+int  thorin_init_pebbles(thorin_ctx *ctx, int mode, const double *pdens, const double *pvrad,
+                         const double *pvtheta, const double *psize);
 
 /* ---- fields ---- */
 int  thorin_upload_field(thorin_ctx *ctx, int field, const double *host, size_t count);

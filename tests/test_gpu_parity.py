@@ -173,3 +173,83 @@ def test_gpu_larger_grid_parity(nrad, nsec, steps):
     for f in STATE_FIELDS:
         err = rel_linf(d.field(f), o.field(f))
         assert err <= TOL, f"{f}: {err:.3e}"
+
+
+PEB_CASE = "pebbles_q3e-5_64x128"
+PEB_FIELDS = ["pebbledens", "pebblevrad", "pebblevtheta"]
+
+
+def test_gpu_pebbles_vs_reference_golden():
+    """Two-fluid run of the reference (equilibrium pebble disk of a fresh run,
+    back-reaction, particle diffusion, evaporation, a q = 3e-5 planet that
+    feels the disk and accretes pebbles): the set-up is bit-exact, the SOR
+    iterations and PhysicalTime equal, gas and pebble fields within 1e-10."""
+    _print_case("pebbles golden")
+    cfg, planets, _ = case(PEB_CASE)
+    g = golden(PEB_CASE)
+    d = Disk(params_for(cfg))
+    d.init(g["init_dens"], g["init_vrad"], g["init_vtheta"], g["init_temper"])
+    d.init_pebbles(0)
+    for f in PEB_FIELDS + ["pebblesize", "pebblestokes"]:
+        assert np.array_equal(d.field(f), g[f"{f}0"]), f"initial {f}"
+    sysp = planets_for(planets, cfg)
+    m0 = sysp.mass[0]
+    rep = Replay(g["tqwk0"], integrate=True)
+    ntot, ninterm = int(g["ntot"]), int(g["ninterm"])
+    niters, times = [], [d.state().physical_time]
+    for i in range(ntot):
+        lg = d.algogas(sysp, rep.cb)
+        niters += [lg.sor_niter[k] for k in range(lg.nsub)]
+        times.append(d.state().physical_time)
+        if (i + 1) % ninterm == 0:
+            n = (i + 1) // ninterm
+            for f in STATE_FIELDS + PEB_FIELDS:
+                key = f"{f}{n}"
+                if key in g:
+                    err = rel_linf(d.field(f), g[key])
+                    assert err <= TOL, f"{key}: rel Linf {err:.3e}"
+    assert niters == list(g["sor_niter"][:len(niters)])
+    assert np.array_equal(np.array(times[:ntot]), g["tqwk0"][:ntot, 0])
+    assert sysp.mass[0] > m0
+
+
+@pytest.mark.parametrize("nrad,nsec,steps", [(64, 128, 6), (256, 384, 4), (24, 1536, 3)])
+def test_gpu_pebbles_vs_oracle_stepwise(nrad, nsec, steps):
+    """The two-fluid workload's seeded pebble disk (Sigma_p = 0.01 Sigma,
+    Stokes numbers 0.01-0.1 per ring) set up as a restart, same host N-body on
+    both sides: CFL substeps, dt, SOR iterations and Nshift exact; gas and
+    pebble fields and Stokes numbers within 1e-10; the planet's mass, solid
+    mass and velocity after accretion within 1e-12 (the sums over the cells
+    inside the accretion radius are tree sums on the GPU)."""
+    _print_case(f"pebbles stepwise {nrad}x{nsec}")
+    from fargo_thorin_amd import configs, seeded_disk, seeded_pebbles
+    cfg = configs.with_grid(configs.PEBBLES_1024x1536, nrad, nsec)
+    disk = seeded_disk(nrad, nsec, seed=777)
+    pb = seeded_pebbles(disk)
+    P = params_for(cfg)
+    d, o = Disk(P), OracleDisk(P)
+    for x in (d, o):
+        x.init(disk["dens"], disk["vrad"], disk["vtheta"], disk["temper"])
+        x.init_pebbles(1, pb["pdens"], pb["pvrad"], pb["pvtheta"], pb["psize"])
+    sd, so = planets_for(configs.PLANET_Q3E5, cfg), planets_for(configs.PLANET_Q3E5, cfg)
+    nb = host_nbody_fn()
+    for step in range(steps):
+        lg = d.algogas(sd, nb)
+        lo = o.algogas(so, nb)
+        assert [lg.cfl_n[k] for k in range(lg.nsub)] == [lo.cfl_n[k] for k in range(lo.nsub)], f"step {step}: N"
+        assert [lg.dt[k] for k in range(lg.nsub)] == [lo.dt[k] for k in range(lo.nsub)], f"step {step}: dt"
+        assert [lg.sor_niter[k] for k in range(lg.nsub)] == [lo.sor_niter[k] for k in range(lo.nsub)]
+        assert np.array_equal(d.ring_profile("nshift"), o.one_d("nshift")[:nrad]), f"step {step}: Nshift"
+        for q in ("mass", "solid", "vx", "vy"):
+            a, b = getattr(sd, q)[0], getattr(so, q)[0]
+            assert abs(a - b) <= 1e-12 * abs(b), f"step {step}: planet {q} {a!r} vs {b!r}"
+    for f in STATE_FIELDS + PEB_FIELDS:
+        err = rel_linf(d.field(f), o.field(f))
+        assert err <= TOL, f"{f}: {err:.3e}"
+    # the library forms the Stokes numbers of the next substep at the end of
+    # a substep (PebbleStokesNumbers, Pebbles.c:287-290, from the new density)
+    from fargo_thorin_amd.seed import RHO2CGS
+    tau = 2.0 * (1.0 / RHO2CGS) * o.field("pebblesize") / o.field("dens") * (1.0 / np.sqrt(1.4))
+    assert rel_linf(d.field("pebblestokes"), tau) <= TOL
+    # on the coarse 24-ring grid no cell centre lies inside Reff
+    assert (sd.solid[0] > 0.0) == (so.solid[0] > 0.0) == (nrad > 24)
