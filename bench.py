@@ -72,6 +72,7 @@ def parse():
     ap.add_argument("--nsec", type=int, default=1536)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-sor-study", action="store_true")
+    ap.add_argument("--no-two-fluid", action="store_true")
     ap.add_argument("--roofline-kernel", default="auto")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (3 times), the reference for scaling "
@@ -165,6 +166,43 @@ def sor_study(nrad=2048, nsec=3072):
             "speedup": plain["ms"] / mixed["ms"], "inner_sweep_timing_ms": tune}
 
 
+def two_fluid_study(nrad=1024, nsec=1536, steps=20, warmup=3):
+    """The two-fluid workload: the damped disk plus the seeded pebble disk
+    (Sigma_p = 0.01 Sigma, Stokes numbers 0.01-0.1 per ring), back-reaction,
+    particle diffusion, evaporation and a q = 3e-5 planet that feels the disk
+    and accretes pebbles (one host sync per substep for the planet update).
+    Steps/s over `steps` DTs after `warmup`, and the per-kernel split."""
+    from fargo_thorin_amd import Disk, Params, configs, seeded_disk, seeded_pebbles
+    from fargo_thorin_amd import make_planets
+    cfg = configs.with_grid(configs.PEBBLES_1024x1536, nrad, nsec)
+    P = Params(**cfg, OutputDir="/nonexistent-thorin-out/")
+    disk = seeded_disk(nrad, nsec)
+    pb = seeded_pebbles(disk)
+    d = Disk(P)
+    d.init(disk["dens"], disk["vrad"], disk["vtheta"], disk["temper"])
+    d.init_pebbles(1, pb["pdens"], pb["pvrad"], pb["pvtheta"], pb["psize"])
+    sysp = make_planets([3e-5], [1.0], feel_disk=True)
+    for _ in range(warmup):
+        d.algogas(sysp)
+    d.sync()
+    t0 = time.perf_counter()
+    substeps = 0
+    for _ in range(steps):
+        substeps += d.algogas(sysp).nsub
+    d.sync()
+    el = time.perf_counter() - t0
+    d.profile()
+    for _ in range(3):
+        d.algogas(sysp)
+    prof = d.profile_read()
+    d.profile_off()
+    m_acc = sysp.solid[0]
+    d.close()
+    return {"workload": f"pebbles_{nrad}x{nsec}", "steps": steps, "substeps": substeps,
+            "steps_per_s": steps / el, "ms_per_step": 1e3 * el / steps, "accreted_solid_mass": m_acc,
+            "kernel_ms_per_step": {k: v[0] / 3 for k, v in sorted(prof.items(), key=lambda kv: -kv[1][0])}}
+
+
 def main():
     args = parse()
     if args.copy_calibration:
@@ -239,6 +277,9 @@ def main():
     if not args.no_sor_study:
         print("[bench] energy-solve study at 2048x3072", file=sys.stderr, flush=True)
         out["sor_study"] = sor_study()
+    if not args.no_two_fluid:
+        print("[bench] two-fluid workload at 1024x1536", file=sys.stderr, flush=True)
+        out["two_fluid"] = two_fluid_study(nrad, nsec)
     if not args.no_cpu_baseline:
         try:
             out["cpu_baseline"] = cpu_baseline_reference(nrad, nsec)

@@ -1168,6 +1168,7 @@ struct PebAccK {
   double x[PEB_MAXPL], y[PEB_MAXPL], z[PEB_MAXPL], m[PEB_MAXPL], om[PEB_MAXPL];
   double reff[PEB_MAXPL], vrel[PEB_MAXPL], mt[PEB_MAXPL];
   int imin[PEB_MAXPL], imax[PEB_MAXPL];
+  int jlo[PEB_MAXPL], jn[PEB_MAXPL];   // azimuthal window jlo .. jlo+jn-1 (mod NSEC) holding every cell within Reff
 };
 
 // AccretePebblesOntoPlanets (Pebbles.c:400-538), STEPS 3-5: one workgroup
@@ -1186,16 +1187,22 @@ __global__ void __launch_bounds__(ACC_T) k_peb_accrete(Geo g, PebAccK P, double 
   const int tid = threadIdx.x, ns = g.ns;
   if (tid == 0) ctl->acc_fail = 0;
   for (int k = 0; k < P.np; k++) {
-    const int i0 = P.imin[k], i1 = P.imax[k];
-    const int nann = (i1 >= i0) ? (i1 - i0 + 1) * ns : 0;
-    const size_t b0 = (size_t)i0 * ns;
+    const int i0 = P.imin[k], i1 = P.imax[k], jn = P.jn[k], jlo = P.jlo[k];
+    const int nann = (i1 >= i0) ? (i1 - i0 + 1) * jn : 0;
     const double X = P.x[k], Y = P.y[k], Z = P.z[k], M = P.m[k];
+    // cell q of the window: ring i0 + q / jn, sector jlo + q % jn (mod NSEC)
+    auto cell = [&](int q, int &i) -> size_t {
+      i = i0 + q / jn;
+      int j = jlo + q % jn;
+      if (j >= ns) j -= ns;
+      return (size_t)i * ns + j;
+    };
     // STEP 3: upper limit of the mass each cell below Reff can give
     const double Reff = P.reff[k], Reff2 = Reff * Reff;
     double du = 0.0, hp = 0.0, pdm = 0.0, nc = 0.0;
     for (int q = tid; q < nann; q += ACC_T) {
-      const int i = i0 + q / ns;
-      const size_t l = b0 + q;
+      int i;
+      const size_t l = cell(q, i);
       const double xc = abs_[l], yc = ord[l];
       const double R2 = (xc - X) * (xc - X) + (yc - Y) * (yc - Y);
       if (R2 <= Reff2) {
@@ -1240,10 +1247,10 @@ __global__ void __launch_bounds__(ACC_T) k_peb_accrete(Geo g, PebAccK P, double 
     const double limiter = s_lim;
     double dm = 0.0, dpx = 0.0, dpy = 0.0;
     for (int q = tid; q < nann; q += ACC_T) {
-      const size_t l = b0 + q;
+      int i;
+      const size_t l = cell(q, i);
       const double mc = macc[l];
       if (mc > 0.0) {
-        const int i = i0 + q / ns;
         const double xc = abs_[l], yc = ord[l];
         const double vrc = pvr[l], vtc = pvt[l] + g.Rmed[i] * g.omega_frame;
         const double vxc = (vrc * xc - vtc * yc) * g.InvRmed[i];
@@ -3015,6 +3022,23 @@ static int peb_accrete_host(thorin_ctx *c, const thorin_planets *sys, PebAccK *P
     }
     if (Reff > pl[k].Rhill) Reff = pl[k].Rhill;
     P->reff[k] = Reff;
+    // Sectors that can hold a cell centre within Reff of the planet: a cell
+    // at radius r and angle dphi from it lies at least r |sin dphi| away (and
+    // at least r away for |dphi| >= pi/2), so with Reff < r_min every such
+    // cell has |dphi| <= asin(Reff / r_min); two sectors of margin.
+    P->jlo[k] = 0;
+    P->jn[k] = ns;
+    if (pl[k].imin <= pl[k].imax && Reff < c->Rmed[pl[k].imin]) {
+      const double dphi = 2.0 * TH_PI / (double)ns;
+      const int w = (int)(asin(Reff / c->Rmed[pl[k].imin]) / dphi) + 2;
+      if (2 * w + 1 < ns) {
+        double ph = atan2(Y, X);
+        if (ph < 0.0) ph += 2.0 * TH_PI;
+        const int jc = (int)floor(ph / dphi + 0.5) % ns;
+        P->jlo[k] = ((jc - w) % ns + ns) % ns;
+        P->jn[k] = 2 * w + 1;
+      }
+    }
     P->vrel[k] = vrel;
     P->mt[k] = Mt;
   }
