@@ -24,7 +24,7 @@ from ._capi import FIELD_ID, FIELDS, OPACITY, Planets, State, StepLog, last_erro
 from .seed import seeded_disk, seeded_pebbles, write_cfg
 
 __all__ = ["Params", "Disk", "Planets", "State", "StepLog", "seeded_disk", "write_cfg", "FIELDS",
-           "OPACITY", "ThorinError", "make_planets"]
+           "OPACITY", "ThorinError", "make_planets", "WarmStart", "seeded_pebbles"]
 
 
 class ThorinError(RuntimeError):
@@ -238,15 +238,18 @@ class Disk:
         return used.value
 
     def rad_solve(self, method: int = 0, tol: float = 1e-10, omega: float = 1.8, inner_sweeps: int = 8,
-                  max_iters: int = 100000) -> dict:
+                  max_iters: int = 100000, warm: "Optional[WarmStart]" = None, sweep_block: int = 0) -> dict:
         """Solve the assembled system from its initial guess: method 0 = FP64
-        red-black SOR, 1 = FP32 SOR sweeps in FP64 iterative refinement.
+        red-black SOR, 1 = FP32 SOR sweeps in FP64 iterative refinement; with
+        warm (method 1), from the network's guess when it lowers the residual.
         T is left in field "temper"."""
-        o = _capi.SolveOpts(method=method, inner_sweeps=inner_sweeps, max_iters=max_iters, tol=tol, omega=omega)
+        o = _capi.SolveOpts(method=method, inner_sweeps=inner_sweeps, max_iters=max_iters, tol=tol, omega=omega,
+                            warm=warm._h if warm is not None else None, sweep_block=sweep_block)
         r = _capi.SolveResult()
         _check(lib().thorin_rad_solve(self._h, C.byref(o), C.byref(r)), "thorin_rad_solve")
         return {"sweeps": r.sweeps, "rounds": r.rounds, "converged": bool(r.converged), "ms": r.ms,
-                "resid0": r.resid0, "resid": r.resid}
+                "resid0": r.resid0, "resid": r.resid, "warm_used": bool(r.warm_used),
+                "resid_warm": r.resid_warm}
 
     def sync(self) -> None:
         _check(lib().thorin_sync(self._h), "sync")
@@ -254,6 +257,63 @@ class Disk:
     def close(self) -> None:
         if getattr(self, "_h", None):
             lib().thorin_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class WarmStart:
+    """The learned warm start of the energy solve (thorin_cnn_*): a 4-level,
+    32-channel U-Net trained on the GPU on pairs the solver generates."""
+
+    def __init__(self, seed: int = 1):
+        h = C.c_void_p()
+        _check(lib().thorin_cnn_create(seed, C.byref(h)), "thorin_cnn_create")
+        self._h = h
+
+    def n_params(self) -> int:
+        return int(lib().thorin_cnn_param_count(self._h))
+
+    def params(self) -> np.ndarray:
+        out = np.empty(self.n_params(), dtype=np.float32)
+        _check(lib().thorin_cnn_get_params(self._h, out.ctypes.data_as(C.POINTER(C.c_float)), out.size), "params")
+        return out
+
+    def set_params(self, p) -> None:
+        a = np.ascontiguousarray(p, dtype=np.float32)
+        _check(lib().thorin_cnn_set_params(self._h, a.ctypes.data_as(C.POINTER(C.c_float)), a.size), "set params")
+
+    def grad(self, x, t):
+        """forward + backward on host arrays: (loss, gradient, prediction)"""
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        H, W = t.shape
+        g = np.empty(self.n_params(), dtype=np.float32)
+        y = np.empty((H, W), dtype=np.float32)
+        L = C.c_double()
+        fp = C.POINTER(C.c_float)
+        _check(lib().thorin_cnn_grad(self._h, H, W, x.ctypes.data_as(fp), t.ctypes.data_as(fp), C.byref(L),
+                                     g.ctypes.data_as(fp), y.ctypes.data_as(fp)), "thorin_cnn_grad")
+        return L.value, g, y
+
+    def add_pair(self, disk: Disk) -> None:
+        _check(lib().thorin_cnn_add_pair(self._h, disk._h), "thorin_cnn_add_pair")
+
+    def n_pairs(self) -> int:
+        return int(lib().thorin_cnn_num_pairs(self._h))
+
+    def train(self, iters: int, lr: float = 1e-3):
+        a, b = C.c_double(), C.c_double()
+        _check(lib().thorin_cnn_train(self._h, iters, lr, C.byref(a), C.byref(b)), "thorin_cnn_train")
+        return a.value, b.value
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().thorin_cnn_destroy(self._h)
             self._h = None
 
     def __del__(self):

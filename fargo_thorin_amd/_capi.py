@@ -54,12 +54,13 @@ class Geometry(C.Structure):
 
 class SolveOpts(C.Structure):
     _fields_ = [("method", C.c_int), ("inner_sweeps", C.c_int), ("max_iters", C.c_int),
-                ("tol", C.c_double), ("omega", C.c_double)]
+                ("tol", C.c_double), ("omega", C.c_double), ("warm", C.c_void_p), ("sweep_block", C.c_int)]
 
 
 class SolveResult(C.Structure):
     _fields_ = [("sweeps", C.c_int), ("rounds", C.c_int), ("converged", C.c_int),
-                ("ms", C.c_double), ("resid0", C.c_double), ("resid", C.c_double)]
+                ("ms", C.c_double), ("resid0", C.c_double), ("resid", C.c_double),
+                ("warm_used", C.c_int), ("resid_warm", C.c_double)]
 
 
 NBODY_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(Planets), C.c_double, C.c_double, C.c_double)
@@ -112,6 +113,16 @@ def lib() -> C.CDLL:
         "thorin_nbody_step": (ip, [vp, C.POINTER(Planets), C.c_double, C.c_double, C.c_double]),
         "thorin_rad_system": (ip, [vp, C.c_double, dp]),
         "thorin_rad_solve": (ip, [vp, C.POINTER(SolveOpts), C.POINTER(SolveResult)]),
+        "thorin_cnn_create": (ip, [C.c_ulonglong, C.POINTER(vp)]),
+        "thorin_cnn_destroy": (ip, [vp]),
+        "thorin_cnn_param_count": (sz, [vp]),
+        "thorin_cnn_get_params": (ip, [vp, C.POINTER(C.c_float), sz]),
+        "thorin_cnn_set_params": (ip, [vp, C.POINTER(C.c_float), sz]),
+        "thorin_cnn_add_pair": (ip, [vp, vp]),
+        "thorin_cnn_num_pairs": (ip, [vp]),
+        "thorin_cnn_train": (ip, [vp, ip, C.c_float, dp, dp]),
+        "thorin_cnn_grad": (ip, [vp, ip, ip, C.POINTER(C.c_float), C.POINTER(C.c_float), dp,
+                                 C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

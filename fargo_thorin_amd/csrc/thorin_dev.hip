@@ -3644,6 +3644,77 @@ __global__ void __launch_bounds__(TB) k_mp_sweep(Geo g, const float *__restrict_
   }
 }
 
+// Several FP32 red-black sweeps per launch (temporal blocking).  A
+// workgroup owns an interior tile of MB_TI rings x MB_TJ sectors and stages
+// e of the tile plus a halo of MB_H = 2*MB_K cells in LDS; each half-sweep
+// leaves the outermost not-yet-updated layer stale, so after up to MB_K
+// sweeps (2*MB_K half-sweeps) the interior is exactly what MB_K sweeps of
+// k_mp_sweep give.  Thread t keeps the coefficients of its 16 cells in
+// registers: rows (t >> 6) + 4s, s = 0..7, columns 2(t & 63) and +1 (one cell
+// of each colour per row).  Colour order as k_mp_sweep: (i + j) odd first.
+#define MB_K 4
+#define MB_H (2 * MB_K)
+#define MB_ER 32
+#define MB_EW 128
+#define MB_LW (MB_EW + 4)
+#define MB_TI (MB_ER - 2 * MB_H)
+#define MB_TJ (MB_EW - 2 * MB_H)
+__global__ void __launch_bounds__(TB) k_mp_sweeps(Geo g, const float *__restrict__ ein, float *__restrict__ eout,
+                                                  Mat32 m, float omega, int nsw) {
+  __shared__ float t[MB_ER * MB_LW];
+  const int ns = g.ns, nr = g.nr;
+  const int ntx = (ns + MB_TJ - 1) / MB_TJ, nty = (nr - 2 + MB_TI - 1) / MB_TI;
+  int tx, ty, bid;
+  if (!xcd_tile(ntx, nty, tx, ty, bid)) return;
+  const int i0 = 1 + ty * MB_TI, j0 = tx * MB_TJ;
+  const int ti = min(MB_TI, nr - 1 - i0), tw = min(MB_TJ, ns - j0);
+  const int tid = threadIdx.x, cp = 2 * (tid & 63), rb = tid >> 6;
+  for (int idx = tid; idx < MB_ER * MB_EW; idx += TB) {
+    const int r = idx / MB_EW, c = idx % MB_EW;
+    const int i = i0 - MB_H + r;
+    const int j = ((j0 - MB_H + c) % ns + ns) % ns;
+    float v = 0.0f;                       // rings 0 and nr-1 (and outside): e = 0
+    if (i >= 1 && i <= nr - 2) v = ein[(size_t)i * ns + j];
+    t[r * MB_LW + c] = v;
+  }
+  float ml[16], mip[16], mim[16], mjp[16], mjm[16], rr[16];
+  unsigned act = 0u, odd = 0u;
+#pragma unroll
+  for (int q = 0; q < 16; q++) {
+    const int r = rb + 4 * (q >> 1), c = cp + (q & 1);
+    const int i = i0 - MB_H + r;
+    const int j = ((j0 - MB_H + c) % ns + ns) % ns;
+    ml[q] = 1.0f; mip[q] = mim[q] = mjp[q] = mjm[q] = rr[q] = 0.0f;
+    if (i >= 1 && i <= nr - 2 && r >= 1 && r <= MB_ER - 2 && c >= 1 && c <= MB_EW - 2) {
+      const size_t l = (size_t)i * ns + j;
+      ml[q] = m.ml[l]; mip[q] = m.mip[l]; mim[q] = m.mim[l]; mjp[q] = m.mjp[l]; mjm[q] = m.mjm[l]; rr[q] = m.r[l];
+      act |= 1u << q;
+      if ((i + j) & 1) odd |= 1u << q;
+    }
+  }
+  __syncthreads();
+  for (int sw = 0; sw < nsw; sw++) {
+#pragma unroll
+    for (int colour = 0; colour < 2; colour++) {
+      const unsigned sel = act & (colour == 0 ? odd : ~odd);
+#pragma unroll
+      for (int q = 0; q < 16; q++) {
+        if (sel & (1u << q)) {
+          const int k = (rb + 4 * (q >> 1)) * MB_LW + cp + (q & 1);
+          const float res = t[k] * ml[q] + t[k + MB_LW] * mip[q] + t[k - MB_LW] * mim[q] + t[k + 1] * mjp[q] +
+                            t[k - 1] * mjm[q] + rr[q];
+          t[k] = t[k] - omega * res / ml[q];
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (int idx = tid; idx < ti * tw; idx += TB) {
+    const int r = idx / tw, c = idx - r * tw;
+    eout[(size_t)(i0 + r) * ns + (j0 + c)] = t[(MB_H + r) * MB_LW + MB_H + c];
+  }
+}
+
 // sum|A.T - RHS| over rings 1..nr-2 (block partials summed in order on the host)
 static int mp_residual(thorin_ctx *c, float *r32, double *out) {
   thorin_dev *d = c->dev;
@@ -3701,6 +3772,20 @@ extern "C" int thorin_rad_system(thorin_ctx *c, double dt, double *dt_used) {
   return 0;
 }
 
+extern "C" int thorin_dev_system(thorin_ctx *c, thorin_sysptrs *p) {
+  if (need_dev(c)) return -1;
+  thorin_dev *d = c->dev;
+  if (!d->rad_ready) { thorin_set_error("no energy system assembled (thorin_rad_system)"); return -1; }
+  p->nr = d->nr;
+  p->ns = d->ns;
+  p->ml = d->b[B_ML]; p->mlip = d->b[B_MLIP]; p->mlim = d->b[B_MLIM]; p->mljp = d->b[B_MLJP];
+  p->mljm = d->b[B_MLJM]; p->rhs = d->b[B_RHS];
+  p->T = d->b[B_TEMPER];
+  p->T0 = d->b[B_TBACK];
+  p->stream = (void *)d->s;
+  return 0;
+}
+
 extern "C" int thorin_rad_solve(thorin_ctx *c, const thorin_solve_opts *o, thorin_solve_result *res) {
   if (need_dev(c)) return -1;
   thorin_dev *d = c->dev;
@@ -3725,11 +3810,31 @@ extern "C" int thorin_rad_solve(thorin_ctx *c, const thorin_solve_opts *o, thori
   double r0 = 0.0;
   if (mp_residual(c, d->m32.r, &r0)) return -1;
   res->resid0 = r0;
+  if (o->warm && o->method != 1) {
+    thorin_set_error("thorin_rad_solve: the learned warm start is used with method 1");
+    return -1;
+  }
   hipEvent_t ea, eb;
   HCHECK(hipEventCreate(&ea));
   HCHECK(hipEventCreate(&eb));
   HCHECK(hipEventRecord(ea, d->s));
   double rcur = r0;
+  if (o->warm) {
+    // the network's guess replaces T0 only if it lowers the FP64 residual;
+    // the target stays tol * r0 of T0
+    if (thorin_cnn_predict_dev(o->warm, c, d->b[B_TEMPER2])) return -1;
+    HCHECK(hipMemcpyAsync(d->b[B_TEMPER], d->b[B_TEMPER2], nb, hipMemcpyDeviceToDevice, d->s));
+    double rg = 0.0;
+    if (mp_residual(c, d->m32.r, &rg)) return -1;
+    res->resid_warm = rg;
+    if (rg < r0) {
+      res->warm_used = 1;
+      rcur = rg;
+    } else {
+      HCHECK(hipMemcpyAsync(d->b[B_TEMPER], d->b[B_TBACK], nb, hipMemcpyDeviceToDevice, d->s));
+      if (mp_residual(c, d->m32.r, &rcur)) return -1;
+    }
+  }
   if (o->method == 0) {
     // the reference's SOR with stop test sum|r| (in-sweep) < tol * normres0
     d->normres0_valid = 0;
@@ -3744,6 +3849,10 @@ extern "C" int thorin_rad_solve(thorin_ctx *c, const thorin_solve_opts *o, thori
     const int nbs = ((ns + SOR_TJ - 1) / SOR_TJ) * ((nr - 2 + SOR_TI - 1) / SOR_TI);
     const dim3 gs((unsigned)((nbs + 7) / 8 * 8));
     const int inner = o->inner_sweeps > 0 ? o->inner_sweeps : 8;
+    // sweeps per launch: 1 = k_mp_sweep; 2..MB_K = k_mp_sweeps (LDS-blocked)
+    const int blk = o->sweep_block > 0 ? (o->sweep_block < MB_K ? o->sweep_block : MB_K) : MB_K;
+    const int ntb = ((ns + MB_TJ - 1) / MB_TJ) * ((nr - 2 + MB_TI - 1) / MB_TI);
+    const dim3 gb((unsigned)((ntb + 7) / 8 * 8));
     hipLaunchKernelGGL(k_mp_convert, gm, tb, 0, d->s, d->g, (const double *)d->b[B_ML],
                        (const double *)d->b[B_MLIP], (const double *)d->b[B_MLIM], (const double *)d->b[B_MLJP],
                        (const double *)d->b[B_MLJM], d->m32);
@@ -3752,9 +3861,15 @@ extern "C" int thorin_rad_solve(thorin_ctx *c, const thorin_solve_opts *o, thori
       HCHECK(hipMemsetAsync(d->m32.e0, 0, sizeof(float) * d->nf, d->s));
       HCHECK(hipMemsetAsync(d->m32.e1, 0, sizeof(float) * d->nf, d->s));
       float *ein = d->m32.e0, *eout = d->m32.e1;
-      for (int k = 0; k < inner; k++) {
-        hipLaunchKernelGGL(k_mp_sweep, gs, tb, 0, d->s, d->g, (const float *)ein, eout, d->m32, (float)o->omega);
+      for (int k = 0; k < inner;) {
+        const int nsw = blk > 1 ? (inner - k < blk ? inner - k : blk) : 1;
+        if (nsw > 1)
+          hipLaunchKernelGGL(k_mp_sweeps, gb, tb, 0, d->s, d->g, (const float *)ein, eout, d->m32, (float)o->omega,
+                             nsw);
+        else
+          hipLaunchKernelGGL(k_mp_sweep, gs, tb, 0, d->s, d->g, (const float *)ein, eout, d->m32, (float)o->omega);
         float *t = ein; ein = eout; eout = t;
+        k += nsw;
       }
       hipLaunchKernelGGL(k_mp_update, gm, tb, 0, d->s, d->g, d->b[B_TEMPER], (const float *)ein);
       HCHECK(hipGetLastError());

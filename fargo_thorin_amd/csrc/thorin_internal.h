@@ -75,6 +75,18 @@ void thorin_dev_destroy(thorin_ctx *c);
 int thorin_dev_upload_init(thorin_ctx *c);
 int thorin_dev_upload_pebbles(thorin_ctx *c);
 
+/* device pointers of the energy system that thorin_rad_system assembled
+ * (rings 1..nr-2 of nr x ns doubles), the current T and the T0 it started
+ * from, and the context's HIP stream */
+typedef struct thorin_sysptrs {
+  int nr, ns;
+  const double *ml, *mlip, *mlim, *mljp, *mljm, *rhs;
+  const double *T, *T0;
+  void *stream;
+} thorin_sysptrs;
+int thorin_dev_system(thorin_ctx *c, thorin_sysptrs *p);
+int thorin_cnn_predict_dev(thorin_cnn *n, thorin_ctx *c, double *Tg);
+
 #ifdef __cplusplus
 }
 #endif

@@ -264,17 +264,49 @@ const char *thorin_prof_kernel_name(int kernel_id);
  *   method 1: FP32 red-black SOR correction sweeps (inner_sweeps per round,
  *             from e = 0) inside FP64 iterative refinement; stop when the
  *             FP64 sum|A.T - RHS| is below tol times its initial value. */
+typedef struct thorin_cnn thorin_cnn;
 typedef struct thorin_solve_opts {
   int method, inner_sweeps, max_iters;
   double tol, omega;
+  thorin_cnn *warm;          /* learned warm start (method 1 only), NULL: none */
+  int sweep_block;           /* method 1: FP32 sweeps per launch, 1..4 (0: 4); more than one
+                                runs them on an LDS tile with a halo (same iterates) */
 } thorin_solve_opts;
 typedef struct thorin_solve_result {
   int sweeps, rounds, converged;
   double ms;                 /* GPU time of the solve (HIP events on the stream) */
-  double resid0, resid;      /* FP64 sum|A.T - RHS| before and after */
+  double resid0, resid;      /* FP64 sum|A.T - RHS| of T0 and after */
+  int warm_used;             /* the warm start lowered the residual and was kept */
+  double resid_warm;         /* FP64 sum|A.T - RHS| of the network's guess */
 } thorin_solve_result;
 int  thorin_rad_system(thorin_ctx *ctx, double dt, double *dt_used);   /* dt <= 0: the hydro CFL step */
 int  thorin_rad_solve(thorin_ctx *ctx, const thorin_solve_opts *o, thorin_solve_result *r);
+
+/* ---- learned warm start of the energy solve.  No counterpart in the
+ * reference.  A 4-level, 32-channel U-Net (BF16 weights and activations,
+ * FP32 accumulation on MFMA, FP32 master weights and Adam) maps the features
+ * of an assembled system (the Jacobi correction of T0 scaled by its RMS s,
+ * the off-diagonals over the diagonal, log of the diagonal) to
+ * (T* - T0)/(T0 s); thorin_rad_solve with opts.warm starts from
+ * T0 + prediction when that lowers the residual.  Grids must have NRAD and
+ * NSEC divisible by 8.  Parameters: per convolution W[9][cin][cout] then
+ * b[cout], FP32; convolutions 16->32, 7 x 32->32, 32->16. */
+int    thorin_cnn_create(unsigned long long seed, thorin_cnn **out);
+int    thorin_cnn_destroy(thorin_cnn *n);
+size_t thorin_cnn_param_count(const thorin_cnn *n);
+int    thorin_cnn_get_params(thorin_cnn *n, float *out, size_t count);
+int    thorin_cnn_set_params(thorin_cnn *n, const float *in, size_t count);
+/* a training pair: features of the system thorin_rad_system assembled and
+ * the target from the T that the last thorin_rad_solve left */
+int    thorin_cnn_add_pair(thorin_cnn *n, thorin_ctx *ctx);
+int    thorin_cnn_num_pairs(const thorin_cnn *n);
+/* iters Adam steps over the stored pairs in turn (on the GPU); mean loss of
+ * the first and the last pass over the pairs */
+int    thorin_cnn_train(thorin_cnn *n, int iters, float lr, double *loss_first, double *loss_last);
+/* forward + backward on host arrays (x: H*W*16 features, t: H*W target):
+ * loss (mean square over rings 1..H-2), parameter gradient, prediction */
+int    thorin_cnn_grad(thorin_cnn *n, int H, int W, const float *x, const float *t, double *loss, float *grad,
+                       float *y);
 
 /* Synchronise the device and return the library's last error text. */
 int  thorin_sync(thorin_ctx *ctx);

@@ -1,0 +1,55 @@
+"""The CPU restatement of the warm-start network (oracle/cnn_oracle.py) is
+itself pinned here: its float64 backward pass against central finite
+differences of its forward pass, its BF16 rounding against exact values, and
+the convolution's boundary handling against a direct sum."""
+import numpy as np
+
+from oracle import cnn_oracle as C
+
+
+def test_bf16_rounding_is_nearest_even():
+    ulp = 2.0 ** -7
+    cases = [(1.0, 1.0), (1.0 + ulp / 2, 1.0), (1.0 + 1.5 * ulp, 1.0 + 2 * ulp), (1.0 + 0.75 * ulp, 1.0 + ulp),
+             (-3.0, -3.0), (0.0, 0.0), (1.0 + 0.25 * ulp, 1.0)]
+    for x, want in cases:
+        assert C.to_bf16(np.array([x]))[0] == want, x
+
+
+def test_conv_boundaries_match_a_direct_sum():
+    rng = np.random.default_rng(3)
+    H, W = 4, 6
+    x = rng.standard_normal((H, W, 16))
+    w = rng.standard_normal((9, 16, 32))
+    y = C.conv(x, w)
+    for h in range(H):
+        for j in range(W):
+            s = np.zeros(32)
+            for t in range(9):
+                hh, jj = h + t // 3 - 1, (j + t % 3 - 1) % W     # zero across rings, periodic in sectors
+                if 0 <= hh < H:
+                    s += x[hh, jj] @ w[t]
+            assert np.allclose(y[h, j], s, rtol=1e-12, atol=1e-12)
+
+
+def test_backward_matches_finite_differences():
+    rng = np.random.default_rng(7)
+    H, W = 8, 16
+    p = rng.standard_normal(C.n_params()) * 0.2
+    x = rng.standard_normal((H, W, 16))
+    t = rng.standard_normal((H, W))
+    loss, g, _, _ = C.forward_backward(p, x, t, bf16=False)
+    assert loss > 0
+    idx = rng.choice(C.n_params(), 80, replace=False)
+    # include every bias of the first and last layers (boundary cases of the layout)
+    idx = np.concatenate([idx, np.arange(9 * 16 * 32, 9 * 16 * 32 + 32), np.arange(C.n_params() - 16, C.n_params())])
+    bad = []
+    for k in idx:
+        h = 1e-6 * max(1.0, abs(p[k]))
+        pp, pm = p.copy(), p.copy()
+        pp[k] += h
+        pm[k] -= h
+        fd = (C.loss_only(pp, x, t) - C.loss_only(pm, x, t)) / (2 * h)
+        if abs(fd - g[k]) > 1e-6 * max(1.0, abs(fd)) + 1e-9:
+            bad.append((int(k), fd, g[k]))
+    # output channels 1..15 do not reach the loss: their gradient is exactly 0
+    assert not bad, bad[:5]

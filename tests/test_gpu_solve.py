@@ -78,3 +78,21 @@ def test_gpu_mixed_precision_matches_fp64_sor_at_1e10():
     assert p["resid"] <= 1e-9 * p["resid0"]
     # both are within the solve tolerance of the exact solution
     assert np.max(np.abs(Tm - Tp)) / np.max(np.abs(Tp)) < 1e-9
+
+
+@pytest.mark.parametrize("nr,ns,inner", [(48, 96, 8), (130, 256, 12), (250, 390, 7)])
+def test_gpu_blocked_fp32_sweeps_equal_single_sweeps(nr, ns, inner):
+    """Several FP32 red-black sweeps per launch on an LDS tile with a halo
+    (k_mp_sweeps) give bit for bit the iterates of one sweep per launch
+    (k_mp_sweep): grids narrower than one tile (96 sectors: the tile wraps
+    around the ring), tiles that do not divide the grid, and inner counts
+    that leave a partial block (12 = 4 + 4 + 4, 7 = 4 + 3)."""
+    d = _disk(nr, ns)
+    d.rad_system(0.0)
+    ref = d.rad_solve(1, tol=1e-10, omega=1.8, inner_sweeps=inner, sweep_block=1)
+    t1 = d.field("temper")
+    for blk in (4, 3, 2):
+        r = d.rad_solve(1, tol=1e-10, omega=1.8, inner_sweeps=inner, sweep_block=blk)
+        assert r["sweeps"] == ref["sweeps"] and r["rounds"] == ref["rounds"]
+        assert np.array_equal(d.field("temper"), t1), f"sweep_block {blk}"
+    assert ref["converged"]
