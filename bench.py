@@ -129,41 +129,83 @@ def cpu_baseline_reference(nrad, nsec):
                       f"{walls[1]:.1f}s and {walls[3]:.1f}s"}
 
 
-def sor_study(nrad=2048, nsec=3072):
+def sor_study(nrad=2048, nsec=3072, train_s=25.0):
     """The energy equation's implicit solve at nrad x nsec (BL94 opacity,
-    T 20-2000 K, dt = the hydro CFL step): plain FP64 red-black SOR against
-    FP32 red-black correction sweeps in FP64 iterative refinement, both to a
-    1e-10 relative FP64 residual at omega = 1.8.  The number of FP32 sweeps per
-    refinement round is picked by timing 16, 32 and 64 once."""
+    T 20-2000 K, dt = the hydro CFL step), all to a 1e-10 relative FP64
+    residual at omega = 1.8:
+      * plain FP64 red-black SOR (the reference's solver);
+      * FP32 red-black correction sweeps in FP64 iterative refinement, four
+        sweeps per launch on LDS tiles; sweeps per round picked by timing 32
+        and 64 once;
+      * the same from the learned warm start: the U-Net trained here on the
+        GPU (Adam, residual loss) for about train_s seconds on 12 pairs the
+        solver generates from other seeds and steps (0.5, 1, 2 x CFL dt);
+        the study disk itself is not among them.
+    Errors are stated against the FP64 SOR result."""
     import numpy as np
 
-    from fargo_thorin_amd import Disk, Params, configs
+    from fargo_thorin_amd import Disk, Params, WarmStart, configs
     from fargo_thorin_amd.seed import sor_disk
     cfg = configs.with_grid(configs.SOR_2048x3072, nrad, nsec)
     P = Params(**cfg, OutputDir="/nonexistent-thorin-out/")
-    disk = sor_disk(nrad, nsec)
     d = Disk(P)
-    d.init(disk["dens"], disk["vrad"], disk["vtheta"], disk["temper"])
-    dt = d.rad_system(0.0)
+
+    def assemble(seed, mult=1.0):
+        disk = sor_disk(nrad, nsec, seed=seed)
+        d.init(disk["dens"], disk["vrad"], disk["vtheta"], disk["temper"])
+        dt0 = d.rad_system(0.0)
+        return d.rad_system(dt0 * mult) if mult != 1.0 else dt0
+
     tol, omega = 1e-10, 1.8
+    # training pairs (other disks) and the network
+    t0 = time.perf_counter()
+    net = WarmStart(seed=3)
+    net.set_loss(1)
+    for seed in range(4):
+        for mult in (0.5, 1.0, 2.0):
+            assemble(200 + seed, mult)
+            d.rad_solve(1, tol=1e-11, omega=omega, inner_sweeps=64)
+            net.add_pair(d)
+    t_pairs = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    iters, loss = 0, (0.0, 0.0)
+    while time.perf_counter() - t1 < train_s:
+        loss = net.train(48, 3e-4)
+        iters += 48
+    t_train = time.perf_counter() - t1
+    # the study disk
+    dt = assemble(12345)
     tune = {}
-    for inner in (16, 32, 64):
+    for inner in (32, 64):
         tune[inner] = d.rad_solve(1, tol=tol, omega=omega, inner_sweeps=inner)["ms"]
     inner = min(tune, key=tune.get)
     plain = min((d.rad_solve(0, tol=tol, omega=omega) for _ in range(2)), key=lambda r: r["ms"])
     t_plain = d.field("temper")
-    mixed = min((d.rad_solve(1, tol=tol, omega=omega, inner_sweeps=inner) for _ in range(2)),
-                key=lambda r: r["ms"])
+    mixed = min((d.rad_solve(1, tol=tol, omega=omega, inner_sweeps=inner) for _ in range(2)), key=lambda r: r["ms"])
     t_mixed = d.field("temper")
-    err = float(np.max(np.abs(t_mixed - t_plain)) / np.max(np.abs(t_plain)))
+    warm = min((d.rad_solve(1, tol=tol, omega=omega, inner_sweeps=inner, warm=net) for _ in range(2)),
+               key=lambda r: r["ms"])
+    t_warm = d.field("temper")
+    err = lambda t: float(np.max(np.abs(t - t_plain)) / np.max(np.abs(t_plain)))
+    out = {"workload": f"sor_{nrad}x{nsec}", "dt": dt, "tol": tol, "omega": omega,
+           "plain_fp64": {"ms": plain["ms"], "sweeps": plain["sweeps"],
+                          "rel_resid": plain["resid"] / plain["resid0"]},
+           "fp32_refine": {"ms": mixed["ms"], "sweeps": mixed["sweeps"], "rounds": mixed["rounds"],
+                           "inner_sweeps": inner, "sweeps_per_launch": 4,
+                           "rel_resid": mixed["resid"] / mixed["resid0"], "rel_err_vs_fp64_sor": err(t_mixed)},
+           "cnn_warm_fp32_refine": {"ms": warm["ms"], "sweeps": warm["sweeps"], "rounds": warm["rounds"],
+                                    "warm_used": warm["warm_used"],
+                                    "guess_rel_resid": warm["resid_warm"] / warm["resid0"],
+                                    "rel_resid": warm["resid"] / warm["resid0"], "rel_err_vs_fp64_sor": err(t_warm),
+                                    "training": {"pairs": net.n_pairs(), "iters": iters, "seconds": t_train,
+                                                 "pair_generation_s": t_pairs, "loss_first": loss[0],
+                                                 "loss_last": loss[1], "loss": "residual of the guess"}},
+           "speedup_fp32_refine": plain["ms"] / mixed["ms"],
+           "speedup_cnn_warm_fp32_refine": plain["ms"] / warm["ms"],
+           "inner_sweep_timing_ms": tune}
+    net.close()
     d.close()
-    return {"workload": f"sor_{nrad}x{nsec}", "dt": dt, "tol": tol, "omega": omega,
-            "plain_fp64": {"ms": plain["ms"], "sweeps": plain["sweeps"],
-                           "rel_resid": plain["resid"] / plain["resid0"]},
-            "fp32_refine": {"ms": mixed["ms"], "sweeps": mixed["sweeps"], "rounds": mixed["rounds"],
-                            "inner_sweeps": inner, "rel_resid": mixed["resid"] / mixed["resid0"],
-                            "rel_err_vs_fp64_sor": err},
-            "speedup": plain["ms"] / mixed["ms"], "inner_sweep_timing_ms": tune}
+    return out
 
 
 def two_fluid_study(nrad=1024, nsec=1536, steps=20, warmup=3):

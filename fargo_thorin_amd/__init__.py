@@ -287,18 +287,29 @@ class WarmStart:
         a = np.ascontiguousarray(p, dtype=np.float32)
         _check(lib().thorin_cnn_set_params(self._h, a.ctypes.data_as(C.POINTER(C.c_float)), a.size), "set params")
 
-    def grad(self, x, t):
-        """forward + backward on host arrays: (loss, gradient, prediction)"""
+    def grad(self, x, t=None, aux=None):
+        """forward + backward on host arrays: (loss, gradient, prediction);
+        with aux (5 x H x W) the residual loss"""
         x = np.ascontiguousarray(x, dtype=np.float32)
-        t = np.ascontiguousarray(t, dtype=np.float32)
-        H, W = t.shape
+        H, W = x.shape[:2]
+        fp = C.POINTER(C.c_float)
+        tp = ap = None
+        if t is not None:
+            t = np.ascontiguousarray(t, dtype=np.float32)
+            tp = t.ctypes.data_as(fp)
+        if aux is not None:
+            aux = np.ascontiguousarray(aux, dtype=np.float32)
+            ap = aux.ctypes.data_as(fp)
         g = np.empty(self.n_params(), dtype=np.float32)
         y = np.empty((H, W), dtype=np.float32)
         L = C.c_double()
-        fp = C.POINTER(C.c_float)
-        _check(lib().thorin_cnn_grad(self._h, H, W, x.ctypes.data_as(fp), t.ctypes.data_as(fp), C.byref(L),
-                                     g.ctypes.data_as(fp), y.ctypes.data_as(fp)), "thorin_cnn_grad")
+        _check(lib().thorin_cnn_grad2(self._h, H, W, x.ctypes.data_as(fp), tp, ap, C.byref(L),
+                                      g.ctypes.data_as(fp), y.ctypes.data_as(fp)), "thorin_cnn_grad")
         return L.value, g, y
+
+    def set_loss(self, mode: int) -> None:
+        """0: error of the normalised correction; 1: residual of the guess"""
+        _check(lib().thorin_cnn_set_loss(self._h, mode), "set_loss")
 
     def add_pair(self, disk: Disk) -> None:
         _check(lib().thorin_cnn_add_pair(self._h, disk._h), "thorin_cnn_add_pair")

@@ -123,6 +123,9 @@ def lib() -> C.CDLL:
         "thorin_cnn_train": (ip, [vp, ip, C.c_float, dp, dp]),
         "thorin_cnn_grad": (ip, [vp, ip, ip, C.POINTER(C.c_float), C.POINTER(C.c_float), dp,
                                  C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+        "thorin_cnn_grad2": (ip, [vp, ip, ip, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float), dp,
+                                  C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+        "thorin_cnn_set_loss": (ip, [vp, ip]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

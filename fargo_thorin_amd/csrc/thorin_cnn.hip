@@ -76,67 +76,173 @@ static bf16_t h_f2bf(float f) {
 }
 
 // ---------------------------------------------------------------------------
-// 3x3 convolution as an implicit GEMM.  One wave computes 16 consecutive
-// pixels of a row (the MFMA M dimension) for all COUT channels (COUT/16 MFMA
-// tiles); K runs over the 9 taps x CIN channels, 16 at a time.  MFMA operand
-// layouts (16x16x16): lane l holds A[l%16][4(l/16)+i], B[4(l/16)+i][l%16] and
-// D[4(l/16)+i][l%16], i = 0..3.  wpk holds the weights in B order:
-// wpk[((t*CB + cb)*NB + nb)*64 + lane] = 4 BF16 of W[t][cb*16+4(lane/16)+i][nb*16+lane%16].
+// 3x3 convolution as an implicit GEMM, computed transposed so that a lane
+// ends with four consecutive channels of one pixel: D^T[co][px] =
+// sum_k W^T[co][k] X^T[k][px] with K running over the 9 taps x CIN channels,
+// 16 at a time.  MFMA operand layouts (16x16x16): lane l holds A[l%16][4(l/16)+i],
+// B[4(l/16)+i][l%16] and D[4(l/16)+i][l%16], i = 0..3; so the A fragment is
+// W[t][cb*16+4(l/16)+i][nb*16+l%16] (packed by k_pack as wpk[((t*CB+cb)*NB+nb)*64+l])
+// and the B fragment is 4 consecutive input channels of pixel l%16 (one
+// 8-byte load), D gives output channels nb*16+4(l/16)+i of pixel l%16 (one
+// 8-byte BF16 or 16-byte FP32 store).  A wave computes MT tiles of 16
+// consecutive pixels of a row and reuses each weight fragment for all of them.
 // mode: 1 relu; 2 FP32 output; 4 zero the output where mask <= 0 (the relu
 // of the layer whose input gradient this is).
-template <int CIN, int COUT>
+template <int CIN, int COUT, int MT>
 __global__ void __launch_bounds__(256) k_conv(const bf16_t *__restrict__ x, const bf16_t *__restrict__ wpk,
                                               const float *__restrict__ bias, void *__restrict__ out, int H, int W,
                                               int mode, const bf16_t *__restrict__ mask) {
   constexpr int NB = COUT / 16, CB = CIN / 16;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int tilesW = (W + 15) >> 4;
+  const int tilesW = (W + 16 * MT - 1) / (16 * MT);
   const long tile = (long)blockIdx.x * 4 + wv;
   if (tile >= (long)H * tilesW) return;
-  const int h = (int)(tile / tilesW), w0 = (int)(tile % tilesW) * 16;
+  const int h = (int)(tile / tilesW), w0 = (int)(tile % tilesW) * 16 * MT;
   const int m = lane & 15, kq = lane >> 4;
-  f4 acc[NB];
+  f4 acc[MT][NB];
 #pragma unroll
-  for (int nb = 0; nb < NB; nb++) acc[nb] = (f4){0.f, 0.f, 0.f, 0.f};
+  for (int u = 0; u < MT; u++)
+#pragma unroll
+    for (int nb = 0; nb < NB; nb++) acc[u][nb] = (f4){0.f, 0.f, 0.f, 0.f};
   const s4 *__restrict__ wp = (const s4 *)wpk;
-  const bool inrow = (w0 + m) < W;
 #pragma unroll
   for (int t = 0; t < 9; t++) {
     const int hh = h + t / 3 - 1;
-    int ww = w0 + m + (t % 3) - 1;
-    ww = ww < 0 ? ww + W : (ww >= W ? ww - W : ww);
-    const bool ok = inrow && hh >= 0 && hh < H;
-    const bf16_t *src = x + ((size_t)(ok ? hh : 0) * W + (ok ? ww : 0)) * CIN + kq * 4;
+    const bool rok = hh >= 0 && hh < H;
 #pragma unroll
     for (int cb = 0; cb < CB; cb++) {
-      s4 a = (s4){0, 0, 0, 0};
-      if (ok) a = *(const s4 *)(src + cb * 16);
+      s4 wf[NB];
 #pragma unroll
-      for (int nb = 0; nb < NB; nb++) {
-        const s4 b = wp[((t * CB + cb) * NB + nb) * 64 + lane];
-        acc[nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[nb], 0, 0, 0);
+      for (int nb = 0; nb < NB; nb++) wf[nb] = wp[((t * CB + cb) * NB + nb) * 64 + lane];
+#pragma unroll
+      for (int u = 0; u < MT; u++) {
+        const int px = w0 + 16 * u + m;
+        int ww = px + (t % 3) - 1;
+        ww = ww < 0 ? ww + W : (ww >= W ? ww - W : ww);
+        s4 xf = (s4){0, 0, 0, 0};
+        if (rok && px < W) xf = *(const s4 *)(x + ((size_t)hh * W + ww) * CIN + cb * 16 + kq * 4);
+#pragma unroll
+        for (int nb = 0; nb < NB; nb++) acc[u][nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wf[nb], xf, acc[u][nb], 0, 0, 0);
       }
     }
   }
 #pragma unroll
-  for (int nb = 0; nb < NB; nb++) {
-    const int co = nb * 16 + m;
-    const float bv = bias ? bias[co] : 0.f;
+  for (int u = 0; u < MT; u++) {
+    const int px = w0 + 16 * u + m;
+    if (px >= W) continue;
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
-      const int px = w0 + 4 * kq + i;
-      if (px >= W) continue;
+    for (int nb = 0; nb < NB; nb++) {
+      const int co = nb * 16 + 4 * kq;
       const size_t o = ((size_t)h * W + px) * COUT + co;
-      float v = acc[nb][i] + bv;
-      if (mode & 1) v = v > 0.f ? v : 0.f;
-      if ((mode & 4) && !(bf2f(mask[o]) > 0.f)) v = 0.f;
-      if (mode & 2) ((float *)out)[o] = v;
-      else ((bf16_t *)out)[o] = f2bf(v);
+      float v[4];
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        v[i] = acc[u][nb][i] + (bias ? bias[co + i] : 0.f);
+        if (mode & 1) v[i] = v[i] > 0.f ? v[i] : 0.f;
+      }
+      if (mode & 4) {
+        const s4 mk = *(const s4 *)(mask + o);
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+          if (!(bf2f(mk[i]) > 0.f)) v[i] = 0.f;
+      }
+      if (mode & 2) {
+        *(f4 *)((float *)out + o) = (f4){v[0], v[1], v[2], v[3]};
+      } else {
+        s4 r;
+#pragma unroll
+        for (int i = 0; i < 4; i++) r[i] = (short)f2bf(v[i]);
+        *(s4 *)((bf16_t *)out + o) = r;
+      }
     }
   }
 }
 
-// weights W[t][ci][co] (FP32 master) -> B-order BF16 for the forward
+// The same convolution with the input staged in LDS: a workgroup (4 waves)
+// covers CV_SEG consecutive pixels of one row; rows h-1..h+1 of the segment
+// plus one halo pixel on each side (periodic) are loaded once (8-byte
+// vectors) and every tap reads them from LDS.  The pixel stride is padded
+// (36 or 20 BF16) so that the 16 pixels of a fragment hit distinct banks.
+#define CV_SEG 256
+template <int CIN, int COUT>
+__global__ void __launch_bounds__(256) k_conv_lds(const bf16_t *__restrict__ x, const bf16_t *__restrict__ wpk,
+                                                  const float *__restrict__ bias, void *__restrict__ out, int H, int W,
+                                                  int mode, const bf16_t *__restrict__ mask) {
+  constexpr int NB = COUT / 16, CB = CIN / 16, PS = CIN + (CIN == 32 ? 4 : 4), MT = CV_SEG / 64;
+  constexpr int NP = CV_SEG + 2;
+  __shared__ bf16_t sx[3 * NP * PS];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int segs = (W + CV_SEG - 1) / CV_SEG;
+  const int h = blockIdx.x / segs, w0 = (blockIdx.x % segs) * CV_SEG;
+  if (h >= H) return;
+  for (int e = threadIdx.x; e < 3 * NP * (CIN / 4); e += 256) {
+    const int c4 = e % (CIN / 4), rest = e / (CIN / 4);
+    const int p = rest % NP, r = rest / NP;
+    const int hh = h + r - 1;
+    const int ww = ((w0 + p - 1) % W + W) % W;
+    s4 v = (s4){0, 0, 0, 0};
+    if (hh >= 0 && hh < H) v = *(const s4 *)(x + ((size_t)hh * W + ww) * CIN + c4 * 4);
+    *(s4 *)&sx[(r * NP + p) * PS + c4 * 4] = v;
+  }
+  __syncthreads();
+  const int m = lane & 15, kq = lane >> 4;
+  f4 acc[MT][NB];
+#pragma unroll
+  for (int u = 0; u < MT; u++)
+#pragma unroll
+    for (int nb = 0; nb < NB; nb++) acc[u][nb] = (f4){0.f, 0.f, 0.f, 0.f};
+  const s4 *__restrict__ wp = (const s4 *)wpk;
+  const int pbase = wv * 64 + m;          // pixel of this lane in tile u: pbase + 16u
+#pragma unroll
+  for (int t = 0; t < 9; t++) {
+    const int r = t / 3, dw = t % 3;      // sx pixel of output pixel p for tap dw: p + dw
+#pragma unroll
+    for (int cb = 0; cb < CB; cb++) {
+      s4 wf[NB];
+#pragma unroll
+      for (int nb = 0; nb < NB; nb++) wf[nb] = wp[((t * CB + cb) * NB + nb) * 64 + lane];
+#pragma unroll
+      for (int u = 0; u < MT; u++) {
+        const s4 xf = *(const s4 *)&sx[(r * NP + pbase + 16 * u + dw) * PS + cb * 16 + kq * 4];
+#pragma unroll
+        for (int nb = 0; nb < NB; nb++)
+          acc[u][nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wf[nb], xf, acc[u][nb], 0, 0, 0);
+      }
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < MT; u++) {
+    const int px = w0 + pbase + 16 * u;
+    if (px >= W) continue;
+#pragma unroll
+    for (int nb = 0; nb < NB; nb++) {
+      const int co = nb * 16 + 4 * kq;
+      const size_t o = ((size_t)h * W + px) * COUT + co;
+      float v[4];
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        v[i] = acc[u][nb][i] + (bias ? bias[co + i] : 0.f);
+        if (mode & 1) v[i] = v[i] > 0.f ? v[i] : 0.f;
+      }
+      if (mode & 4) {
+        const s4 mk = *(const s4 *)(mask + o);
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+          if (!(bf2f(mk[i]) > 0.f)) v[i] = 0.f;
+      }
+      if (mode & 2) {
+        *(f4 *)((float *)out + o) = (f4){v[0], v[1], v[2], v[3]};
+      } else {
+        s4 rr;
+#pragma unroll
+        for (int i = 0; i < 4; i++) rr[i] = (short)f2bf(v[i]);
+        *(s4 *)((bf16_t *)out + o) = rr;
+      }
+    }
+  }
+}
+
+// weights W[t][ci][co] (FP32 master) -> MFMA-fragment order BF16 for the forward
 // convolution, or for the input-gradient convolution (taps mirrored, ci and
 // co swapped: dX[p][ci] = sum_t sum_co dZ[p - off_t][co] W[t][ci][co])
 __global__ void k_pack(const float *__restrict__ w, int cin, int cout, int transposed, bf16_t *__restrict__ wpk) {
@@ -251,12 +357,20 @@ __global__ void __launch_bounds__(256) k_bgrad(const bf16_t *__restrict__ dz, lo
     part[blockIdx.x * C + threadIdx.x] = t;
   }
 }
-__global__ void k_bgrad_reduce(const float *__restrict__ part, int nblk, int C, float *__restrict__ db) {
-  const int c = threadIdx.x;
-  if (c >= C) return;
+// one workgroup of 256: thread (c, k) sums blocks k, k + 256/C, ... of channel c
+__global__ void __launch_bounds__(256) k_bgrad_reduce(const float *__restrict__ part, int nblk, int C,
+                                                      float *__restrict__ db) {
+  __shared__ float red[256];
+  const int c = threadIdx.x % C, k0 = threadIdx.x / C, lanes = 256 / C;
   float s = 0.f;
-  for (int b = 0; b < nblk; b++) s += part[b * C + c];
-  db[c] = s;
+  for (int b = k0; b < nblk; b += lanes) s += part[b * C + c];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  if (threadIdx.x < C) {
+    float t = 0.f;
+    for (int k = 0; k < lanes; k++) t += red[k * C + threadIdx.x];
+    db[threadIdx.x] = t;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -416,12 +530,21 @@ __global__ void __launch_bounds__(256) k_feat1(const double *__restrict__ T, con
   }
   if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
-__global__ void k_scale(const double *__restrict__ part, int nb, long nvalid, double *__restrict__ sc) {
-  if (threadIdx.x != 0) return;
+__global__ void __launch_bounds__(256) k_scale(const double *__restrict__ part, int nb, long nvalid,
+                                               double *__restrict__ sc) {
+  __shared__ double red[256];
   double s = 0.0;
-  for (int b = 0; b < nb; b++) s += part[b];
-  s = sqrt(s / (double)nvalid);
-  sc[0] = s > 1e-300 ? s : 1e-300;
+  for (int b = threadIdx.x; b < nb; b += 256) s += part[b];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double v = sqrt(red[0] / (double)nvalid);
+    sc[0] = v > 1e-300 ? v : 1e-300;
+  }
 }
 __global__ void k_feat2(const float *__restrict__ jac, const double *__restrict__ Ml, const double *__restrict__ Mlip,
                         const double *__restrict__ Mlim, const double *__restrict__ Mljp,
@@ -459,6 +582,83 @@ __global__ void k_target(const double *__restrict__ Tsol, const double *__restri
     t[l] = (i >= 1 && i <= H - 2) ? (float)((Tsol[l] - T0[l]) / (T0[l] * s)) : 0.f;
   }
 }
+// Residual loss: with the guess T0 + e, e = y tau T0 s on rings 1..H-2 (0 on
+// rings 0 and H-1), the normalised residual of cell l is
+//   rho_l = -f0_l + tau (y_l + wip_l y_{l+W} + wim_l y_{l-W} + wjp_l y_{l+1} + wjm_l y_{l-1})
+// with f0 = -(A.T0 - RHS)/(Ml T0 s) and w_nb = (M_nb/Ml)(T0_nb/T0_l) (aux
+// planes 0..4); loss = sum rho^2 / n over rings 1..H-2.
+__global__ void __launch_bounds__(256) k_rloss(const float *__restrict__ y, const float *__restrict__ aux, float tau,
+                                               int H, int W, float *__restrict__ rho, double *__restrict__ part) {
+  __shared__ double red[256];
+  const long n = (long)H * W;
+  const double inv = 1.0 / ((double)(H - 2) * W);
+  double s = 0.0;
+  for (long l = (long)blockIdx.x * 256 + threadIdx.x; l < n; l += (long)gridDim.x * 256) {
+    const int i = (int)(l / W), j = (int)(l % W);
+    float r = 0.f;
+    if (i >= 1 && i <= H - 2) {
+      const long ljp = (j == W - 1) ? l - j : l + 1, ljm = (j == 0) ? l + W - 1 : l - 1;
+      const float yip = (i + 1 <= H - 2) ? y[(l + W) * COUTL] : 0.f;
+      const float yim = (i - 1 >= 1) ? y[(l - W) * COUTL] : 0.f;
+      const float a = y[l * COUTL] + aux[n + l] * yip + aux[2 * n + l] * yim + aux[3 * n + l] * y[ljp * COUTL] +
+                      aux[4 * n + l] * y[ljm * COUTL];
+      r = -aux[l] + tau * a;
+      s += (double)r * r;
+    }
+    rho[l] = r;
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0] * inv;
+}
+// its gradient: dy_m = 2 tau / n (rho_m + wip_{m-W} rho_{m-W} + wim_{m+W} rho_{m+W}
+//                                 + wjp_{m-1} rho_{m-1} + wjm_{m+1} rho_{m+1}) on rings 1..H-2
+__global__ void k_rgrad(const float *__restrict__ rho, const float *__restrict__ aux, float tau, int H, int W,
+                        bf16_t *__restrict__ dy) {
+  const long n = (long)H * W;
+  const float c = (float)(2.0 / ((double)(H - 2) * W)) * tau;
+  for (long m = (long)blockIdx.x * blockDim.x + threadIdx.x; m < n; m += (long)gridDim.x * blockDim.x) {
+    const int i = (int)(m / W), j = (int)(m % W);
+    float g = 0.f;
+    if (i >= 1 && i <= H - 2) {
+      const long mjm = (j == 0) ? m + W - 1 : m - 1, mjp = (j == W - 1) ? m - j : m + 1;
+      g = rho[m] + aux[n + m - W] * rho[m - W] + aux[2 * n + m + W] * rho[m + W] + aux[3 * n + mjm] * rho[mjm] +
+          aux[4 * n + mjp] * rho[mjp];
+      g *= c;
+    }
+    *(s4 *)(dy + m * COUTL) = (s4){(short)f2bf(g), 0, 0, 0};
+    *(s4 *)(dy + m * COUTL + 4) = (s4){0, 0, 0, 0};
+    *(s4 *)(dy + m * COUTL + 8) = (s4){0, 0, 0, 0};
+    *(s4 *)(dy + m * COUTL + 12) = (s4){0, 0, 0, 0};
+  }
+}
+// aux planes of a training pair from the system, T0, the Jacobi feature and s
+__global__ void k_aux(const float *__restrict__ jac, const double *__restrict__ T0, const double *__restrict__ Ml,
+                      const double *__restrict__ Mlip, const double *__restrict__ Mlim, const double *__restrict__ Mljp,
+                      const double *__restrict__ Mljm, int H, int W, const double *__restrict__ sc,
+                      float *__restrict__ aux) {
+  const long n = (long)H * W;
+  const double is = 1.0 / sc[0];
+  for (long l = (long)blockIdx.x * blockDim.x + threadIdx.x; l < n; l += (long)gridDim.x * blockDim.x) {
+    const int i = (int)(l / W), j = (int)(l % W);
+    float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f, a4 = 0.f;
+    if (i >= 1 && i <= H - 2) {
+      const long ljp = (j == W - 1) ? l - j : l + 1, ljm = (j == 0) ? l + W - 1 : l - 1;
+      const double ml = Ml[l], t0 = T0[l];
+      a0 = (float)((double)jac[l] * is);
+      a1 = (float)(Mlip[l] / ml * (T0[l + W] / t0));
+      a2 = (float)(Mlim[l] / ml * (T0[l - W] / t0));
+      a3 = (float)(Mljp[l] / ml * (T0[ljp] / t0));
+      a4 = (float)(Mljm[l] / ml * (T0[ljm] / t0));
+    }
+    aux[l] = a0; aux[n + l] = a1; aux[2 * n + l] = a2; aux[3 * n + l] = a3; aux[4 * n + l] = a4;
+  }
+}
+
 // sum of squares (block partials)
 __global__ void __launch_bounds__(256) k_sumsq(const float *__restrict__ t, long n, double *__restrict__ part) {
   __shared__ double red[256];
@@ -496,6 +696,7 @@ struct Sample {
   int H, W;
   bf16_t *x;
   float *t;
+  float *aux;                 // residual-loss planes (f0, wip, wim, wjp, wjm)
 };
 
 struct thorin_cnn {
@@ -510,7 +711,8 @@ struct thorin_cnn {
   float tinv;                 // 1/tscale during training, 1 for thorin_cnn_grad
   int H, W;                   // activation grid (0: none)
   bf16_t *x, *a[9], *p1, *p2, *p3, *u0, *u1, *u2, *dy, *dz[9], *du0, *du1, *du2, *dp1, *dp2, *dp3;
-  float *y, *t, *jac, *wpart, *bpart;
+  float *y, *t, *jac, *wpart, *bpart, *aux, *rho;
+  int lossmode;               // 0: mean square of the correction error; 1: of the guess' residual
   double *lpart, *fpart, *sc;
   int nwblk;
   std::vector<Sample> samples;
@@ -534,7 +736,7 @@ static void free_acts(thorin_cnn *n) {
     if (n->a[k]) { (void)hipFree(n->a[k]); n->a[k] = nullptr; }
     if (n->dz[k]) { (void)hipFree(n->dz[k]); n->dz[k] = nullptr; }
   }
-  float **fs[] = {&n->y, &n->t, &n->jac, &n->wpart, &n->bpart};
+  float **fs[] = {&n->y, &n->t, &n->jac, &n->wpart, &n->bpart, &n->aux, &n->rho};
   for (auto p : fs)
     if (*p) { (void)hipFree(*p); *p = nullptr; }
   double **ds[] = {&n->lpart, &n->fpart};
@@ -564,6 +766,8 @@ static int alloc_acts(thorin_cnn *n, int H, int W) {
   HCK(hipMalloc(&n->y, px0 * COUTL * sizeof(float)));
   HCK(hipMalloc(&n->t, px0 * sizeof(float)));
   HCK(hipMalloc(&n->jac, px0 * sizeof(float)));
+  HCK(hipMalloc(&n->aux, 5 * px0 * sizeof(float)));
+  HCK(hipMalloc(&n->rho, px0 * sizeof(float)));
   n->nwblk = 512;
   HCK(hipMalloc(&n->wpart, (size_t)n->nwblk * 9 * 4 * 256 * sizeof(float)));
   HCK(hipMalloc(&n->bpart, (size_t)8192 * WID * sizeof(float)));
@@ -586,16 +790,33 @@ static int pack_all(thorin_cnn *n) {
 }
 
 // one convolution launch of the right instance
+static const int CONV_MT = 4;      // 16-pixel tiles per wave
 static int conv(thorin_cnn *n, int cin, int cout, const bf16_t *x, const bf16_t *wpk, const float *bias, void *out,
                 int H, int W, int mode, const bf16_t *mask) {
-  const long tiles = (long)H * ((W + 15) / 16);
+  if (W >= CV_SEG / 2) {
+    // rows wide enough to fill most of a 256-pixel segment: LDS-staged input
+    const dim3 g((unsigned)(H * ((W + CV_SEG - 1) / CV_SEG))), b(256);
+    if (cin == 16 && cout == 32)
+      hipLaunchKernelGGL((k_conv_lds<16, 32>), g, b, 0, n->s, x, wpk, bias, out, H, W, mode, mask);
+    else if (cin == 32 && cout == 32)
+      hipLaunchKernelGGL((k_conv_lds<32, 32>), g, b, 0, n->s, x, wpk, bias, out, H, W, mode, mask);
+    else if (cin == 32 && cout == 16)
+      hipLaunchKernelGGL((k_conv_lds<32, 16>), g, b, 0, n->s, x, wpk, bias, out, H, W, mode, mask);
+    else {
+      thorin_set_error("no convolution instance for these channels");
+      return -1;
+    }
+    HCK(hipGetLastError());
+    return 0;
+  }
+  const long tiles = (long)H * ((W + 16 * CONV_MT - 1) / (16 * CONV_MT));
   const dim3 g((unsigned)((tiles + 3) / 4)), b(256);
   if (cin == 16 && cout == 32)
-    hipLaunchKernelGGL((k_conv<16, 32>), g, b, 0, n->s, x, wpk, bias, out, H, W, mode, mask);
+    hipLaunchKernelGGL((k_conv<16, 32, CONV_MT>), g, b, 0, n->s, x, wpk, bias, out, H, W, mode, mask);
   else if (cin == 32 && cout == 32)
-    hipLaunchKernelGGL((k_conv<32, 32>), g, b, 0, n->s, x, wpk, bias, out, H, W, mode, mask);
+    hipLaunchKernelGGL((k_conv<32, 32, CONV_MT>), g, b, 0, n->s, x, wpk, bias, out, H, W, mode, mask);
   else if (cin == 32 && cout == 16)
-    hipLaunchKernelGGL((k_conv<32, 16>), g, b, 0, n->s, x, wpk, bias, out, H, W, mode, mask);
+    hipLaunchKernelGGL((k_conv<32, 16, CONV_MT>), g, b, 0, n->s, x, wpk, bias, out, H, W, mode, mask);
   else {
     thorin_set_error("no convolution instance for these channels");
     return -1;
@@ -621,7 +842,7 @@ static int wgrad(thorin_cnn *n, int k, const bf16_t *x, const bf16_t *dz, int H,
   const int lanes = 256 / Lk.cout;
   const int bb = grid_for(npx, lanes) < 4096 ? grid_for(npx, lanes) : 4096;
   hipLaunchKernelGGL(k_bgrad, dim3(bb), dim3(256), 0, n->s, dz, npx, Lk.cout, n->bpart);
-  hipLaunchKernelGGL(k_bgrad_reduce, dim3(1), dim3(64), 0, n->s, (const float *)n->bpart, bb, Lk.cout,
+  hipLaunchKernelGGL(k_bgrad_reduce, dim3(1), dim3(256), 0, n->s, (const float *)n->bpart, bb, Lk.cout,
                      dw + (size_t)9 * Lk.cin * Lk.cout);
   HCK(hipGetLastError());
   return 0;
@@ -663,8 +884,16 @@ static int backward(thorin_cnn *n, double *loss) {
   auto Hl = [&](int l) { return H >> l; };
   auto Wl = [&](int l) { return W >> l; };
   const int lb = grid_for((long)H * W) < 8192 ? grid_for((long)H * W) : 8192;
-  hipLaunchKernelGGL(k_loss, dim3(lb), dim3(256), 0, n->s, (const float *)n->y, (const float *)n->t, n->tinv, H, W,
-                     n->dy, n->lpart);
+  if (n->lossmode == 0) {
+    hipLaunchKernelGGL(k_loss, dim3(lb), dim3(256), 0, n->s, (const float *)n->y, (const float *)n->t, n->tinv, H, W,
+                       n->dy, n->lpart);
+  } else {
+    const float tau = 1.0f / n->tinv;
+    hipLaunchKernelGGL(k_rloss, dim3(lb), dim3(256), 0, n->s, (const float *)n->y, (const float *)n->aux, tau, H, W,
+                       n->rho, n->lpart);
+    hipLaunchKernelGGL(k_rgrad, dim3(grid_for((long)H * W)), dim3(256), 0, n->s, (const float *)n->rho,
+                       (const float *)n->aux, tau, H, W, n->dy);
+  }
   // layer 9 (32 -> 16): weights from (a8, dy); input gradient masked by a8 > 0
   if (wgrad(n, 8, n->a[8], n->dy, H, W)) return -1;
   if (conv(n, 16, 32, n->dy, n->wpkT[8], nullptr, n->dz[8], H, W, 4, n->a[8])) return -1;
@@ -723,7 +952,7 @@ static int features(thorin_cnn *n, const thorin_sysptrs &sp) {
   const int nb = grid_for(npx) < 8192 ? grid_for(npx) : 8192;
   hipLaunchKernelGGL(k_feat1, dim3(nb), dim3(256), 0, n->s, sp.T, sp.ml, sp.mlip, sp.mlim, sp.mljp, sp.mljm, sp.rhs,
                      H, W, n->jac, n->fpart);
-  hipLaunchKernelGGL(k_scale, dim3(1), dim3(64), 0, n->s, (const double *)n->fpart, nb, (long)(H - 2) * W, n->sc);
+  hipLaunchKernelGGL(k_scale, dim3(1), dim3(256), 0, n->s, (const double *)n->fpart, nb, (long)(H - 2) * W, n->sc);
   hipLaunchKernelGGL(k_feat2, dim3(grid_for(npx)), dim3(256), 0, n->s, (const float *)n->jac, sp.ml, sp.mlip, sp.mlim,
                      sp.mljp, sp.mljm, H, W, (const double *)n->sc, n->x);
   HCK(hipGetLastError());
@@ -740,7 +969,8 @@ extern "C" int thorin_cnn_create(unsigned long long seed, thorin_cnn **out) {
   n->x = n->p1 = n->p2 = n->p3 = n->u0 = n->u1 = n->u2 = n->dy = nullptr;
   n->du0 = n->du1 = n->du2 = n->dp1 = n->dp2 = n->dp3 = nullptr;
   for (int k = 0; k < 9; k++) { n->a[k] = nullptr; n->dz[k] = nullptr; }
-  n->y = n->t = n->jac = n->wpart = n->bpart = nullptr;
+  n->y = n->t = n->jac = n->wpart = n->bpart = n->aux = n->rho = nullptr;
+  n->lossmode = 0;
   n->lpart = n->fpart = nullptr;
   n->H = n->W = 0;
   n->adam_t = 0;
@@ -809,7 +1039,7 @@ extern "C" int thorin_cnn_destroy(thorin_cnn *n) {
   if (!n) return 0;
   (void)hipStreamSynchronize(n->s);
   free_acts(n);
-  for (auto &s : n->samples) { (void)hipFree(s.x); (void)hipFree(s.t); }
+  for (auto &s : n->samples) { (void)hipFree(s.x); (void)hipFree(s.t); (void)hipFree(s.aux); }
   float *fs[] = {n->w, n->g, n->m, n->v};
   for (auto p : fs)
     if (p) (void)hipFree(p);
@@ -845,16 +1075,28 @@ extern "C" int thorin_cnn_set_params(thorin_cnn *n, const float *in, size_t coun
 // channel 0).  For checks of the kernels against a CPU restatement.
 extern "C" int thorin_cnn_grad(thorin_cnn *n, int H, int W, const float *x, const float *t, double *loss, float *grad,
                                float *y) {
-  if (!n || !x || !t) { thorin_set_error("thorin_cnn_grad: null argument"); return -1; }
+  return thorin_cnn_grad2(n, H, W, x, t, nullptr, loss, grad, y);
+}
+
+// the same with aux != NULL (5 planes of H*W: f0, wip, wim, wjp, wjm): the
+// residual loss with tau = 1 instead of the correction loss (t unused)
+extern "C" int thorin_cnn_grad2(thorin_cnn *n, int H, int W, const float *x, const float *t, const float *aux,
+                                double *loss, float *grad, float *y) {
+  if (!n || !x || (!t && !aux)) { thorin_set_error("thorin_cnn_grad: null argument"); return -1; }
   if (alloc_acts(n, H, W)) return -1;
   const size_t npx = (size_t)H * W;
   std::vector<bf16_t> xb(npx * CIN0);
   for (size_t e = 0; e < xb.size(); e++) xb[e] = h_f2bf(x[e]);
   HCK(hipMemcpyAsync(n->x, xb.data(), xb.size() * sizeof(bf16_t), hipMemcpyHostToDevice, n->s));
-  HCK(hipMemcpyAsync(n->t, t, npx * sizeof(float), hipMemcpyHostToDevice, n->s));
+  if (t) HCK(hipMemcpyAsync(n->t, t, npx * sizeof(float), hipMemcpyHostToDevice, n->s));
+  if (aux) HCK(hipMemcpyAsync(n->aux, aux, 5 * npx * sizeof(float), hipMemcpyHostToDevice, n->s));
   n->tinv = 1.0f;
-  if (forward(n)) return -1;
-  if (backward(n, loss)) return -1;
+  const int keep = n->lossmode;
+  n->lossmode = aux ? 1 : 0;
+  int rc = forward(n);
+  if (!rc) rc = backward(n, loss);
+  n->lossmode = keep;
+  if (rc) return -1;
   if (grad) HCK(hipMemcpyAsync(grad, n->g, n->nparam * sizeof(float), hipMemcpyDeviceToHost, n->s));
   if (y) {
     std::vector<float> yy(npx * COUTL);
@@ -886,6 +1128,9 @@ extern "C" int thorin_cnn_add_pair(thorin_cnn *n, thorin_ctx *c) {
   const size_t npx = (size_t)sp.nr * sp.ns;
   HCK(hipMalloc(&sm.x, npx * CIN0 * sizeof(bf16_t)));
   HCK(hipMalloc(&sm.t, npx * sizeof(float)));
+  HCK(hipMalloc(&sm.aux, 5 * npx * sizeof(float)));
+  hipLaunchKernelGGL(k_aux, dim3(grid_for((long)npx)), dim3(256), 0, n->s, (const float *)n->jac, sp.T0, sp.ml, sp.mlip,
+                     sp.mlim, sp.mljp, sp.mljm, sp.nr, sp.ns, (const double *)n->sc, sm.aux);
   HCK(hipMemcpyAsync(sm.x, n->x, npx * CIN0 * sizeof(bf16_t), hipMemcpyDeviceToDevice, n->s));
   hipLaunchKernelGGL(k_target, dim3(grid_for((long)npx)), dim3(256), 0, n->s, sp.T, sp.T0, sp.nr, sp.ns,
                      (const double *)n->sc, sm.t);
@@ -922,6 +1167,7 @@ extern "C" int thorin_cnn_train(thorin_cnn *n, int iters, float lr, double *loss
     const Sample &sm = n->samples[it % ns];
     HCK(hipMemcpyAsync(n->x, sm.x, npx * CIN0 * sizeof(bf16_t), hipMemcpyDeviceToDevice, n->s));
     HCK(hipMemcpyAsync(n->t, sm.t, npx * sizeof(float), hipMemcpyDeviceToDevice, n->s));
+    if (n->lossmode) HCK(hipMemcpyAsync(n->aux, sm.aux, 5 * npx * sizeof(float), hipMemcpyDeviceToDevice, n->s));
     if (forward(n)) return -1;
     const bool want = it < ns || it >= iters - ns;
     double L = 0.0;
@@ -958,4 +1204,12 @@ extern "C" int thorin_cnn_predict_dev(thorin_cnn *n, thorin_ctx *c, double *Tg) 
   }
   n->s = keep;
   return rc;
+}
+
+// training objective: 0 = mean square of (prediction - normalised correction),
+// 1 = mean square of the normalised residual of the guess T0 + e
+extern "C" int thorin_cnn_set_loss(thorin_cnn *n, int mode) {
+  if (!n || mode < 0 || mode > 1) { thorin_set_error("thorin_cnn_set_loss: mode 0 or 1"); return -1; }
+  n->lossmode = mode;
+  return 0;
 }

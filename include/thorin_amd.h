@@ -307,6 +307,15 @@ int    thorin_cnn_train(thorin_cnn *n, int iters, float lr, double *loss_first, 
  * loss (mean square over rings 1..H-2), parameter gradient, prediction */
 int    thorin_cnn_grad(thorin_cnn *n, int H, int W, const float *x, const float *t, double *loss, float *grad,
                        float *y);
+/* the same with the residual loss when aux != NULL: aux holds 5 planes of
+ * H*W floats (f0, wip, wim, wjp, wjm) and the loss is the mean square over
+ * rings 1..H-2 of rho = -f0 + y + wip y[i+1] + wim y[i-1] + wjp y[j+1] + wjm y[j-1]
+ * (y = 0 on rings 0 and H-1); t may be NULL */
+int    thorin_cnn_grad2(thorin_cnn *n, int H, int W, const float *x, const float *t, const float *aux,
+                        double *loss, float *grad, float *y);
+/* training objective: 0 = mean square error of the normalised correction
+ * (default); 1 = mean square of the normalised residual of the guess */
+int    thorin_cnn_set_loss(thorin_cnn *n, int mode);
 
 /* Synchronise the device and return the library's last error text. */
 int  thorin_sync(thorin_ctx *ctx);

@@ -100,8 +100,35 @@ def up_grad(du):
     return ((du[0::2, 0::2] + du[0::2, 1::2]) + du[1::2, 0::2]) + du[1::2, 1::2]
 
 
-def forward_backward(params, x, t, bf16=True):
-    """loss, flat gradient, prediction y (H x W) and the activations"""
+def residual(y, aux):
+    """rho = -f0 + y + wip y[i+1] + wim y[i-1] + wjp y[j+1] + wjm y[j-1] on rings
+    1..H-2 (y = 0 on rings 0 and H-1), aux = (f0, wip, wim, wjp, wjm)"""
+    H = y.shape[0]
+    yv = np.zeros_like(y)
+    yv[1:H - 1] = y[1:H - 1]
+    f0, wip, wim, wjp, wjm = aux
+    rho = np.zeros_like(y)
+    a = (yv + wip * np.roll(yv, -1, axis=0) + wim * np.roll(yv, 1, axis=0) + wjp * np.roll(yv, -1, axis=1)
+         + wjm * np.roll(yv, 1, axis=1))
+    rho[1:H - 1] = (-f0 + a)[1:H - 1]
+    return rho
+
+
+def residual_grad(rho, aux):
+    """d(sum rho^2)/dy / 2 (the transposed stencil), zero on rings 0 and H-1"""
+    H = rho.shape[0]
+    f0, wip, wim, wjp, wjm = aux
+    g = (rho + np.roll(wip * rho, 1, axis=0) + np.roll(wim * rho, -1, axis=0) + np.roll(wjp * rho, 1, axis=1)
+         + np.roll(wjm * rho, -1, axis=1))
+    g[0] = 0.0
+    g[H - 1] = 0.0
+    return g
+
+
+def forward_backward(params, x, t, bf16=True, aux=None):
+    """loss, flat gradient, prediction y (H x W) and the activations; with
+    aux the residual loss mean(rho^2) over rings 1..H-2 replaces the
+    correction loss"""
     r = to_bf16 if bf16 else (lambda a: np.asarray(a, dtype=np.float64))
     relu = lambda a: np.maximum(a, 0.0)
     L = unpack(params)
@@ -126,11 +153,16 @@ def forward_backward(params, x, t, bf16=True):
     a[8] = r(relu(conv(u0, Wq[7], B[7])))
     y = conv(a[8], Wq[8], B[8])[..., 0]
     n = (H - 2) * W
-    d = np.zeros((H, W))
-    d[1:H - 1] = y[1:H - 1] - t[1:H - 1]
-    loss = float(np.sum(d * d) / n)
     dy = np.zeros((H, W, 16))
-    dy[..., 0] = r(2.0 * d / n)
+    if aux is None:
+        d = np.zeros((H, W))
+        d[1:H - 1] = y[1:H - 1] - t[1:H - 1]
+        loss = float(np.sum(d * d) / n)
+        dy[..., 0] = r(2.0 * d / n)
+    else:
+        rho = residual(y, aux)
+        loss = float(np.sum(rho * rho) / n)
+        dy[..., 0] = r(2.0 * residual_grad(rho, aux) / n)
     g = [None] * 9
     g[8] = conv_weight_grad(a[8], dy)
     dz8 = r(conv_input_grad(dy, Wq[8]) * (a[8] > 0))
@@ -159,6 +191,6 @@ def forward_backward(params, x, t, bf16=True):
     return loss, flat, y, a
 
 
-def loss_only(params, x, t):
+def loss_only(params, x, t, aux=None):
     """float64 loss without rounding (for finite differences)"""
-    return forward_backward(params, x, t, bf16=False)[0]
+    return forward_backward(params, x, t, bf16=False, aux=aux)[0]

@@ -53,3 +53,29 @@ def test_backward_matches_finite_differences():
             bad.append((int(k), fd, g[k]))
     # output channels 1..15 do not reach the loss: their gradient is exactly 0
     assert not bad, bad[:5]
+
+
+def test_residual_loss_gradient_matches_finite_differences():
+    rng = np.random.default_rng(9)
+    H, W = 8, 16
+    p = rng.standard_normal(C.n_params()) * 0.2
+    x = rng.standard_normal((H, W, 16))
+    aux = [rng.standard_normal((H, W)), -0.3 * rng.random((H, W)), -0.3 * rng.random((H, W)),
+           -0.2 * rng.random((H, W)), -0.2 * rng.random((H, W))]
+    loss, g, y, _ = C.forward_backward(p, x, None, bf16=False, aux=aux)
+    # the residual itself against a direct loop
+    rho = C.residual(y, aux)
+    for i in range(1, H - 1):
+        for j in range(W):
+            yv = lambda ii, jj: y[ii, jj % W] if 1 <= ii <= H - 2 else 0.0
+            want = -aux[0][i, j] + yv(i, j) + aux[1][i, j] * yv(i + 1, j) + aux[2][i, j] * yv(i - 1, j) + \
+                aux[3][i, j] * yv(i, j + 1) + aux[4][i, j] * yv(i, j - 1)
+            assert abs(rho[i, j] - want) < 1e-12
+    idx = rng.choice(C.n_params(), 60, replace=False)
+    for k in idx:
+        h = 1e-6 * max(1.0, abs(p[k]))
+        pp, pm = p.copy(), p.copy()
+        pp[k] += h
+        pm[k] -= h
+        fd = (C.loss_only(pp, x, None, aux) - C.loss_only(pm, x, None, aux)) / (2 * h)
+        assert abs(fd - g[k]) <= 1e-5 * max(1.0, abs(fd)) + 1e-9, (k, fd, g[k])

@@ -23,7 +23,7 @@ def rel(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
 
-@pytest.mark.parametrize("H,W", [(16, 32), (24, 48)])
+@pytest.mark.parametrize("H,W", [(16, 32), (24, 48), (16, 320)])
 def test_gpu_cnn_gradients_match_the_oracle(H, W):
     print(f"[gpu-cnn] gradients {H}x{W}", flush=True)
     net = WarmStart(seed=11)
@@ -50,6 +50,31 @@ def test_gpu_cnn_gradients_match_the_oracle(H, W):
     # output channels 1..15 do not reach the loss
     w9 = g[off - n:off - 16].reshape(9, 32, 16)
     assert np.all(w9[:, :, 1:] == 0.0)
+
+
+@pytest.mark.parametrize("H,W", [(16, 32), (24, 48), (16, 320)])
+def test_gpu_cnn_residual_loss_gradients_match_the_oracle(H, W):
+    print(f"[gpu-cnn] residual-loss gradients {H}x{W}", flush=True)
+    net = WarmStart(seed=13)
+    rng = np.random.default_rng(100 + H)
+    p = net.params()
+    x = rng.standard_normal((H, W, 16)).astype(np.float32)
+    x[..., 6:] = 0.0
+    aux = np.stack([rng.standard_normal((H, W)), -0.3 * rng.random((H, W)), -0.3 * rng.random((H, W)),
+                    -0.2 * rng.random((H, W)), -0.2 * rng.random((H, W))]).astype(np.float32)
+    L, g, y = net.grad(x, aux=aux)
+    Lo, go, yo, _ = CO.forward_backward(p.astype(np.float64), x, None, bf16=True, aux=list(aux.astype(np.float64)))
+    assert abs(L - Lo) <= 1e-3 * Lo
+    assert rel(y, yo) <= 1e-3
+    off = 0
+    for k, (ci, co) in enumerate(CO.LAYERS):
+        n = 9 * ci * co + co
+        a, b = g[off:off + n].astype(float), go[off:off + n]
+        err = rel(a, b)
+        cos = float(a @ b / (np.linalg.norm(a) * np.linalg.norm(b)))
+        # the first layer collects the most BF16 rounding flips of the pass
+        assert err <= 3e-2 and cos >= 0.999, f"layer {k}: gradient rel L2 {err:.3e}, cosine {cos:.6f}"
+        off += n
 
 
 def test_gpu_warm_start_lowers_residual_and_sweeps():
