@@ -73,6 +73,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-sor-study", action="store_true")
     ap.add_argument("--no-two-fluid", action="store_true")
+    ap.add_argument("--study", choices=["sor", "two-fluid"], default=None,
+                    help="run only this study and print its JSON (for profiling)")
     ap.add_argument("--roofline-kernel", default="auto")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (3 times), the reference for scaling "
@@ -249,6 +251,12 @@ def main():
     args = parse()
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.study == "sor":
+        print(json.dumps(sor_study()))
+        return
+    if args.study == "two-fluid":
+        print(json.dumps(two_fluid_study()))
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU radial slabs are "

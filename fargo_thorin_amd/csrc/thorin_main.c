@@ -74,6 +74,12 @@ static void send_output(const thorin_params *p, thorin_ctx *c, int n, double *bu
     if (f) { fwrite(buf, sizeof(double), cnt, f); fclose(f); }
   }
   if (p->write_opacity) write_polar(p, c, THORIN_F_OPACITY, "opacity", n, buf);
+  if (p->pebbles) {   /* WritePebbles, Pebbles.c:772-778 */
+    write_polar(p, c, THORIN_F_PEBBLEDENS, "pebbledens", n, buf);
+    write_polar(p, c, THORIN_F_PEBBLEVRAD, "pebblevrad", n, buf);
+    write_polar(p, c, THORIN_F_PEBBLEVTHETA, "pebblevtheta", n, buf);
+    if (p->write_eta) write_polar(p, c, THORIN_F_ETA, "eta", n, buf);
+  }
 }
 
 static int read_binary(const char *fn, double *dst, size_t cnt) {
@@ -192,6 +198,24 @@ int main(int argc, char **argv) {
     free(d); free(vr); free(vt); free(t);
   } else {
     if (thorin_init_state(c, 0, NULL, NULL, NULL, NULL)) die("init");
+  }
+  /* pebble disk (Init.c:138-144): RestartPebbleDisk or EquilPebbleDisk */
+  if (p.pebbles) {
+    if (restart) {
+      double *pd = malloc(sizeof(double) * cnt), *pvr = malloc(sizeof(double) * cnt),
+             *pvt = malloc(sizeof(double) * cnt), *ps = malloc(sizeof(double) * cnt);
+      snprintf(fn, sizeof fn, "%sgaspebbledens%d.dat", p.outputdir, nrestart); read_binary(fn, pd, cnt);
+      snprintf(fn, sizeof fn, "%sgaspebblesize%d.dat", p.outputdir, 0); read_binary(fn, ps, cnt);
+      snprintf(fn, sizeof fn, "%sgaspebblevrad%d.dat", p.outputdir, nrestart); read_binary(fn, pvr, cnt);
+      snprintf(fn, sizeof fn, "%sgaspebblevtheta%d.dat", p.outputdir, nrestart); read_binary(fn, pvt, cnt);
+      if (thorin_init_pebbles(c, 1, pd, pvr, pvt, ps)) die("pebble restart");
+      free(pd); free(pvr); free(pvt); free(ps);
+    } else {
+      if (thorin_init_pebbles(c, 0, NULL, NULL, NULL, NULL)) die("pebble disk");
+      /* InitPebblesViaFlux writes the Stokes numbers and sizes of output 0 */
+      write_polar(&p, c, THORIN_F_PEBBLESTOKES, "pebblestokes", 0, buf);
+      write_polar(&p, c, THORIN_F_PEBBLESIZE, "pebblesize", 0, buf);
+    }
   }
   if (verbose) printf("NRAD %d NSEC %d DT %g NTOT %d NINTERM %d\n", p.nrad, p.nsec, p.dt, p.ntot, p.ninterm);
   /* used_rad.dat (SourceEuler.c:119-129) and dims.dat (Output.c:191-200) */

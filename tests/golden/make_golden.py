@@ -101,6 +101,17 @@ def run_case(name, cfg, planets, full):
                     data[f"{fld}{n}"] = a
                 else:
                     data[f"rows_{fld}{n}"] = a[:: max(1, nrad // 8)].copy()
+        if name.startswith("pebbles"):
+            # restart from output 1 (-s 1): the pebble damping targets become
+            # the ring means of the restart files (BckpFieldsForBC); keep the
+            # output 2 the restarted run writes
+            r2 = subprocess.run([REF, "-s", "1", par], cwd=tmp, capture_output=True, text=True, timeout=3600)
+            if r2.returncode != 0:
+                raise RuntimeError(f"reference restart failed on {name}:\n{r2.stdout[-2000:]}\n{r2.stderr[-2000:]}")
+            for fld in FIELDS:
+                fn = os.path.join(outdir, f"gas{fld}2.dat")
+                if os.path.exists(fn):
+                    data[f"restart_{fld}2"] = np.fromfile(fn, dtype=np.float64).reshape(nrad, nsec)
         tq = np.loadtxt(os.path.join(outdir, "tqwk0.dat"), ndmin=2)
         data["tqwk0"] = tq
         data["used_rad"] = np.loadtxt(os.path.join(outdir, "used_rad.dat"))
