@@ -73,7 +73,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-sor-study", action="store_true")
     ap.add_argument("--no-two-fluid", action="store_true")
-    ap.add_argument("--study", choices=["sor", "two-fluid"], default=None,
+    ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--study", choices=["sor", "two-fluid", "ensemble"], default=None,
                     help="run only this study and print its JSON (for profiling)")
     ap.add_argument("--roofline-kernel", default="auto")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -265,6 +266,80 @@ def two_fluid_study(nrad=1024, nsec=1536, steps=20, warmup=3):
             "kernel_ms_per_step": {k: v[0] / 3 for k, v in sorted(prof.items(), key=lambda kv: -kv[1][0])}}
 
 
+def ensemble_study(n=16, steps=10, warmup=2):
+    """The ensemble workload: 16 damped disks of uneven grids (512x768 to
+    1024x1536) with seeded alpha, Sigma0 and h, advanced one DT at a time
+    (a) disk after disk with thorin_algogas and (b) together with
+    thorin_ensemble_algogas, where each kernel of a substep is one launch over
+    all disks.  Disk-steps/s of both over `steps` DTs after `warmup` (the
+    first DT holds each disk's SOR over-relaxation scan), launches per DT, and
+    whether both runs end with bit-identical fields."""
+    import numpy as np
+
+    from fargo_thorin_amd import Disk, Ensemble, Params, configs, make_planets, seeded_disk
+    mem = []
+    for c in configs.ensemble_configs(n):
+        c = dict(c)
+        seed = c.pop("_seed")
+        nr, ns = int(c["Nrad"]), int(c["Nsec"])
+        mem.append((c, seeded_disk(nr, ns, sigma0=float(c["Sigma0"]), h=float(c["AspectRatio"]), seed=seed)))
+
+    def make():
+        ds = []
+        for c, dk in mem:
+            d = Disk(Params(**c, OutputDir="/nonexistent-thorin-out/"))
+            d.init(dk["dens"], dk["vrad"], dk["vtheta"], dk["temper"])
+            ds.append(d)
+        return ds
+
+    cells = sum(int(c["Nrad"]) * int(c["Nsec"]) for c, _ in mem)
+    out = {"workload": f"ensemble_{n}", "disks": n, "cells": cells,
+           "grids": [f"{c['Nrad']}x{c['Nsec']}" for c, _ in mem], "steps": steps, "warmup": warmup}
+    # (a) one disk after the other
+    A = make()
+    pa = [make_planets([1e-13], [1.0]) for _ in mem]
+    for _ in range(warmup):
+        for d, p in zip(A, pa):
+            d.algogas(p)
+    for d in A:
+        d.sync()
+    t0 = time.perf_counter()
+    subs = 0
+    for _ in range(steps):
+        for d, p in zip(A, pa):
+            subs += d.algogas(p).nsub
+    for d in A:
+        d.sync()
+    el_a = time.perf_counter() - t0
+    out["one_by_one"] = {"disk_steps_per_s": n * steps / el_a, "ms_per_dt": 1e3 * el_a / steps,
+                         "substeps": subs}
+    # (b) the ensemble
+    B = make()
+    pb = [make_planets([1e-13], [1.0]) for _ in mem]
+    E = Ensemble(B)
+    for _ in range(warmup):
+        E.algogas(pb)
+    l0, o0 = E.stats()
+    t0 = time.perf_counter()
+    subs = 0
+    for _ in range(steps):
+        subs += sum(lg.nsub for lg in E.algogas(pb))
+    el_b = time.perf_counter() - t0
+    l1, o1 = E.stats()
+    out["ensemble"] = {"disk_steps_per_s": n * steps / el_b, "ms_per_dt": 1e3 * el_b / steps, "substeps": subs,
+                       "launches_per_dt": (l1 - l0) / steps, "disk_kernel_ops_per_dt": (o1 - o0) / steps}
+    out["speedup"] = el_a / el_b
+    E.close()
+    same = True
+    for a, b in zip(A, B):
+        for f in ("dens", "vrad", "vtheta", "energy"):
+            same &= bool(np.array_equal(a.field(f), b.field(f)))
+    out["bit_identical"] = same
+    for d in A + B:
+        d.close()
+    return out
+
+
 def main():
     args = parse()
     if args.copy_calibration:
@@ -275,6 +350,9 @@ def main():
         return
     if args.study == "two-fluid":
         print(json.dumps(two_fluid_study()))
+        return
+    if args.study == "ensemble":
+        print(json.dumps(ensemble_study()))
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU radial slabs are "
@@ -355,6 +433,9 @@ def main():
         out["roofline"]["frac_of_copy"] = out["roofline"]["achieved"] / out["hbm_copy"]["GBps"]
     except Exception as e:  # torch absent: the figure is optional
         out["hbm_copy"] = {"error": str(e)}
+    if not args.no_ensemble:
+        print("[bench] ensemble of 16 disks", file=sys.stderr, flush=True)
+        out["ensemble"] = ensemble_study()
     if not args.no_two_fluid:
         print("[bench] two-fluid workload at 1024x1536", file=sys.stderr, flush=True)
         out["two_fluid"] = two_fluid_study(nrad, nsec)

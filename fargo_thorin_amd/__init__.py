@@ -24,7 +24,7 @@ from ._capi import FIELD_ID, FIELDS, OPACITY, Planets, State, StepLog, last_erro
 from .seed import seeded_disk, seeded_pebbles, write_cfg
 
 __all__ = ["Params", "Disk", "Planets", "State", "StepLog", "seeded_disk", "write_cfg", "FIELDS",
-           "OPACITY", "ThorinError", "make_planets", "WarmStart", "seeded_pebbles"]
+           "OPACITY", "ThorinError", "make_planets", "WarmStart", "seeded_pebbles", "Ensemble"]
 
 
 class ThorinError(RuntimeError):
@@ -257,6 +257,53 @@ class Disk:
     def close(self) -> None:
         if getattr(self, "_h", None):
             lib().thorin_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Ensemble:
+    """Disks advanced together (thorin_ensemble_*): every kernel of a substep
+    is one launch over all disks' grids; each disk's results are those of its
+    own Disk.algogas, bit for bit."""
+
+    def __init__(self, disks):
+        self.disks = list(disks)
+        n = len(self.disks)
+        arr = (C.c_void_p * n)(*[d._h for d in self.disks])
+        h = C.c_void_p()
+        _check(lib().thorin_ensemble_create(arr, n, C.byref(h)), "thorin_ensemble_create")
+        self._h = h
+        self.logs = (StepLog * n)()
+        self.rcs = (C.c_int * n)()
+        self._nbody = _capi.host_nbody_fn()
+
+    def algogas(self, planets, nbody=None):
+        """One DT of every disk; planets: one Planets per disk.  Returns the
+        per-disk step logs."""
+        n = len(self.disks)
+        if len(planets) != n:
+            raise ValueError(f"one Planets per disk is needed ({n})")
+        ptrs = (C.POINTER(Planets) * n)(*[C.pointer(p) for p in planets])
+        cb = self._nbody if nbody is None else nbody
+        rc = lib().thorin_ensemble_algogas(self._h, ptrs, cb, None, self.logs, self.rcs)
+        _check(rc, "thorin_ensemble_algogas")
+        self.last_rc = rc
+        return list(self.logs)
+
+    def stats(self):
+        """(launches issued, per-disk kernel operations they carried)"""
+        a, b = C.c_longlong(), C.c_longlong()
+        _check(lib().thorin_ensemble_stats(self._h, C.byref(a), C.byref(b)), "thorin_ensemble_stats")
+        return a.value, b.value
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().thorin_ensemble_destroy(self._h)
             self._h = None
 
     def __del__(self):

@@ -100,6 +100,11 @@ def lib() -> C.CDLL:
         "thorin_download_field": (ip, [vp, ip, dp, sz]),
         "thorin_algogas": (ip, [vp, C.POINTER(Planets), NBODY_FN, vp, C.POINTER(StepLog)]),
         "thorin_ring_means": (ip, [vp, ip, dp]),
+        "thorin_ensemble_create": (ip, [C.POINTER(vp), ip, C.POINTER(vp)]),
+        "thorin_ensemble_algogas": (ip, [vp, C.POINTER(C.POINTER(Planets)), NBODY_FN, C.POINTER(vp),
+                                         C.POINTER(StepLog), C.POINTER(ip)]),
+        "thorin_ensemble_stats": (ip, [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
+        "thorin_ensemble_destroy": (None, [vp]),
         "thorin_ring_profile": (ip, [vp, ip, dp]),
         "thorin_ctx_create_host": (ip, [vp, C.POINTER(vp)]),
         "thorin_host_field": (ip, [vp, ip, dp, sz]),

@@ -14,6 +14,11 @@ POWERLAWOPACITY, the build-only kappa = k0*T^2 option.
                      Sigma, Stokes numbers 0.01-0.1 per ring, seed.seeded_pebbles),
                      drag back-reaction, particle diffusion, a q = 3e-5 planet
                      at r = 1 that feels the disk and accretes pebbles, 500 steps
+  ensemble_16      : 16 damped disks of uneven grids advanced together
+                     (ensemble_configs): disk d draws u(1000+d, k), k = 0..3:
+                     Nrad from {512, 640, 768, 896, 1024} (k=0), Nsec = 1.5 Nrad,
+                     alpha log-uniform in [1e-4, 1e-2] (k=1), Sigma0 = 5e-4 x
+                     10^(log10(0.3) + u) (k=2), h uniform in [0.03, 0.07] (k=3)
   sor_2048x3072    : the energy equation's implicit solve at 2048x3072 with
                      BL94 opacity on the interpolated disk of seed.sor_disk
                      (T 20-2000 K), dt = the hydro CFL step -- the solver
@@ -65,3 +70,32 @@ def planet_text(planets) -> str:
     (ReboundInterface.c:157-166)."""
     return "".join(f"Planet{k + 1}\t{m!r}\t0.0\t0.0\t{a!r}\t0.0\t0.0\t0.0\t0.0\t0.0\n"
                    for k, (m, a) in enumerate(planets))
+
+
+ENSEMBLE_SIZES = [512, 640, 768, 896, 1024]
+
+
+def ensemble_configs(n: int = 16, seed0: int = 1000, sizes=None) -> list[dict]:
+    """The ensemble's members: in.par keys plus the seed of each disk (key
+    "_seed"; it is not an in.par key).  sizes replaces the Nrad choices (for
+    small test ensembles); Nsec = 1.5 Nrad."""
+    import math
+
+    import numpy as np
+
+    from .seed import uniform
+    sizes = ENSEMBLE_SIZES if sizes is None else list(sizes)
+    out = []
+    for d in range(n):
+        seed = seed0 + d
+        u = uniform(seed, np.arange(4, dtype=np.uint64))
+        nrad = sizes[min(int(u[0] * len(sizes)), len(sizes) - 1)]
+        nsec = nrad * 3 // 2
+        alpha = 10.0 ** (-4.0 + 2.0 * u[1])
+        sigma0 = 5e-4 * 10.0 ** (math.log10(0.3) + u[2])
+        h = 0.03 + 0.04 * u[3]
+        cfg = with_grid(DAMPED_1024x1536, nrad, nsec, AlphaViscosity=repr(float(alpha)),
+                        Sigma0=repr(float(sigma0)), AspectRatio=repr(float(h)))
+        cfg["_seed"] = seed
+        out.append(cfg)
+    return out

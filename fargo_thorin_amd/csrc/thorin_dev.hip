@@ -15,6 +15,15 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+
+#include <condition_variable>
+#include <functional>
+#include <mutex>
+#include <new>
+#include <thread>
+#include <tuple>
+#include <unordered_map>
+#include <utility>
 #include <vector>
 
 #include "thorin_internal.h"
@@ -97,8 +106,13 @@ struct PlanetK {                // per-planet constants of FillForcesArrays (Pfr
   int pad;
 };
 
+struct BOff {                   // first workgroup of this disk in a batched launch (0 otherwise),
+  int blk0;                     // for the kernels without a Geo argument
+};
+
 struct Geo {                    // passed by value to every kernel
   int nr, ns;
+  int blk0;                     // first workgroup of this disk in a batched launch (0 otherwise)
   int ncb;                      // TB-wide column blocks per ring
   const double *Rinf, *Rsup, *Rmed, *Surf, *InvRmed, *InvSurf, *InvDiffRmed, *InvDiffRsup, *InvRinf;
   const double *OmegaInv, *SigmaMed, *SigmaInf, *EnergyMed, *VthetaMed, *WaveKiller;
@@ -160,6 +174,8 @@ struct thorin_dev {
   std::vector<hipEvent_t> *prof_pool;
   double prof_ms[64];
   long long prof_cnt[64];
+  struct Rec *rec;              // ensemble mode: operations are recorded, not issued (thorin_ensemble_*)
+  int tt_npt_min;               // smallest register count per thread of k_transport_theta (ensemble)
 };
 
 // kernels that the event timer knows, by launch name
@@ -343,7 +359,7 @@ __device__ __forceinline__ double block_max(double v, double *sh) {
 // block (ring-major); false for the padding blocks of the last band.
 __device__ __forceinline__ bool xcd_cell(const Geo &g, int i0, int nring, int &i, int &j, int &L, int boff = 0) {
   const int total = g.ncb * nring, per = (total + 7) >> 3;
-  const int b = blockIdx.x - boff;           // boff: leading blocks with another job (a multiple of 8)
+  const int b = (int)blockIdx.x - g.blk0 - boff;   // boff: leading blocks with another job (a multiple of 8)
   L = (b & 7) * per + (b >> 3);
   if (L >= total) return false;
   const int q = L / g.ncb;
@@ -381,7 +397,7 @@ __device__ __forceinline__ double ordered_row_sum_global(const double *__restric
 // whose cost varies strongly from ring to ring (the force kernel near a
 // planet), so that the expensive rings spread over all XCDs
 __device__ __forceinline__ bool rr_cell(const Geo &g, int i0, int nring, int &i, int &j, int &L, int boff = 0) {
-  L = blockIdx.x - boff;
+  L = (int)blockIdx.x - g.blk0 - boff;
   if (L >= g.ncb * nring) return false;
   const int q = L / g.ncb;
   i = i0 + q;
@@ -390,9 +406,9 @@ __device__ __forceinline__ bool rr_cell(const Geo &g, int i0, int nring, int &i,
 }
 
 // the same banding for a grid of ntx x nty tiles (tile x fastest)
-__device__ __forceinline__ bool xcd_tile(int ntx, int nty, int &tx, int &ty, int &L) {
+__device__ __forceinline__ bool xcd_tile(int ntx, int nty, int &tx, int &ty, int &L, int blk0 = 0) {
   const int total = ntx * nty, per = (total + 7) >> 3;
-  const int b = blockIdx.x;
+  const int b = (int)blockIdx.x - blk0;
   L = (b & 7) * per + (b >> 3);
   if (L >= total) return false;
   ty = L / ntx;
@@ -479,7 +495,7 @@ __device__ __forceinline__ double ordered_ring_sum(const double *sh, int ns) {
 // under the square root is completed afterwards in the reference's order.
 // With do_eos = 0 the temperature and sound speed already in memory are used.
 #define EC_MAX 12              // cells per thread: NSEC <= RB * EC_MAX
-__global__ void __launch_bounds__(RB) k_eos_cfl(Geo g, const double *__restrict__ rho, const double *__restrict__ e,
+__device__ __forceinline__ void b_eos_cfl(Geo g, const double *__restrict__ rho, const double *__restrict__ e,
                                                 double *__restrict__ T, double *__restrict__ cs,
                                                 const double *__restrict__ vr, const double *__restrict__ vt,
                                                 int do_eos, const int *__restrict__ guard) {
@@ -487,7 +503,7 @@ __global__ void __launch_bounds__(RB) k_eos_cfl(Geo g, const double *__restrict_
   __shared__ double red[RB / 64];
   __shared__ double vmoy, gcs_i, gtemp_i;
   if (guard && *guard == 0) return;
-  const int i = blockIdx.x, ns = g.ns, nr = g.nr, tid = threadIdx.x;
+  const int i = (int)blockIdx.x - g.blk0, ns = g.ns, nr = g.nr, tid = threadIdx.x;
   const bool want_t = g.alpha_flock && do_eos;
   double *s_vt = sh, *s_cs = sh + ns, *s_t = sh + 2 * ns;
   const double gam = g.adiabind;
@@ -605,6 +621,13 @@ __global__ void __launch_bounds__(RB) k_eos_cfl(Geo g, const double *__restrict_
   m = block_max<RB>(m, red);
   if (tid == 0) g.ringmax[i] = m;
 }
+__global__ void __launch_bounds__(RB) k_eos_cfl(Geo g, const double *__restrict__ rho, const double *__restrict__ e,
+                                                double *__restrict__ T, double *__restrict__ cs,
+                                                const double *__restrict__ vr, const double *__restrict__ vt,
+                                                int do_eos, const int *__restrict__ guard) {
+  b_eos_cfl(g, rho, e, T, cs, vr, vt, do_eos, guard);
+}
+
 
 // ConditionCFL's minimum (SourceEuler.c:620-676): the cell term at the
 // largest S of rings 1..NRAD-1, with the pebble limit invdtpeb_sq
@@ -613,7 +636,7 @@ __global__ void __launch_bounds__(RB) k_eos_cfl(Geo g, const double *__restrict_
 // between rings.  The minimum of the DiffusionTimestep partials of this
 // substep's k_substep3 (nstellar of them; 0: none formed) goes to
 // ctl->dt_stellar.
-__global__ void __launch_bounds__(RB) k_cfl_final(Geo g, DevCtl *ctl, const double *__restrict__ pstellar,
+__device__ __forceinline__ void b_cfl_final(Geo g, DevCtl *ctl, const double *__restrict__ pstellar,
                                                   int nstellar, double invdtreb_sq, const int *__restrict__ guard) {
   __shared__ double red[RB];
   if (guard && *guard == 0) return;
@@ -647,6 +670,11 @@ __global__ void __launch_bounds__(RB) k_cfl_final(Geo g, DevCtl *ctl, const doub
   m = block_min<RB>(m, red);
   if (threadIdx.x == 0) ctl->newdt_raw = m;
 }
+__global__ void __launch_bounds__(RB) k_cfl_final(Geo g, DevCtl *ctl, const double *__restrict__ pstellar,
+                                                  int nstellar, double invdtreb_sq, const int *__restrict__ guard) {
+  b_cfl_final(g, ctl, pstellar, nstellar, invdtreb_sq, guard);
+}
+
 
 // ---------------------------------------------------------------------------
 // FillForcesArrays (Pframeforce.c:43-510).  One thread per cell; per-block
@@ -661,7 +689,7 @@ struct PebF {
 };
 
 template <bool PEB>
-__global__ void __launch_bounds__(TB) k_forces(Geo g, const double *__restrict__ rho,
+__device__ __forceinline__ void b_forces(Geo g, const double *__restrict__ rho,
                                                const double *__restrict__ cs,
                                                const double *__restrict__ abs_,
                                                const double *__restrict__ ord,
@@ -672,7 +700,7 @@ __global__ void __launch_bounds__(TB) k_forces(Geo g, const double *__restrict__
   __shared__ double red[TB];
   const int ns = g.ns;
   const int lead = ntq_rings > 0 ? 1 : 0;
-  if ((int)blockIdx.x < lead) {
+  if ((int)blockIdx.x - g.blk0 < lead) {
     // leading block: the torque log's ring means of rho next to the logged
     // planets (UpdateLog, Force.c:41-52), lane b walking ring tq.r[b], and the
     // sound-speed ring means of the same rings (globcsvec); ring -1 gives 0
@@ -920,11 +948,23 @@ __global__ void __launch_bounds__(TB) k_forces(Geo g, const double *__restrict__
     if (threadIdx.x == 0) partial[(3 + 2 * k) * nb + bid] = v;
   }
 }
+template <bool PEB>
+__global__ void __launch_bounds__(TB) k_forces(Geo g, const double *__restrict__ rho,
+                                               const double *__restrict__ cs,
+                                               const double *__restrict__ abs_,
+                                               const double *__restrict__ ord,
+                                               double *__restrict__ agr, double *__restrict__ agt,
+                                               const PlanetK *__restrict__ pk, int npl, double mt,
+                                               double *__restrict__ partial, TqRings tq, int ntq_rings,
+                                               DevCtl *__restrict__ ctl, PebF pf) {
+  b_forces<PEB>(g, rho, cs, abs_, ord, agr, agt, pk, npl, mt, partial, tq, ntq_rings, ctl, pf);
+}
+
 
 // reduce the partial sums in block order; one block per quantity
-__global__ void __launch_bounds__(RB) k_forces_reduce(const double *__restrict__ partial, int nb, DevCtl *ctl) {
+__device__ __forceinline__ void b_forces_reduce(const double *__restrict__ partial, int nb, DevCtl *ctl, BOff bo) {
   __shared__ double red[RB];
-  const int q = blockIdx.x;
+  const int q = (int)blockIdx.x - bo.blk0;
   double s = 0.0;
   s = strided_sum<RB>(partial + (size_t)q * nb, nb);
   s = block_sum<RB>(s, red);
@@ -936,9 +976,13 @@ __global__ void __launch_bounds__(RB) k_forces_reduce(const double *__restrict__
     else ctl->aypl[(q - 3) / 2] = s;
   }
 }
+__global__ void __launch_bounds__(RB) k_forces_reduce(const double *__restrict__ partial, int nb, DevCtl *ctl, BOff bo) {
+  b_forces_reduce(partial, nb, ctl, bo);
+}
+
 
 // indirect term (Pframeforce.c:488-509)
-__global__ void __launch_bounds__(TB) k_indirect(Geo g, double *__restrict__ agr, double *__restrict__ agt,
+__device__ __forceinline__ void b_indirect(Geo g, double *__restrict__ agr, double *__restrict__ agt,
                                                  const double *__restrict__ abs_, const double *__restrict__ ord,
                                                  const DevCtl *__restrict__ ctl, double indir_px, double indir_py,
                                                  double *__restrict__ pagr, double *__restrict__ pagt) {
@@ -958,6 +1002,13 @@ __global__ void __launch_bounds__(TB) k_indirect(Geo g, double *__restrict__ agr
     pagt[l] += at;
   }
 }
+__global__ void __launch_bounds__(TB) k_indirect(Geo g, double *__restrict__ agr, double *__restrict__ agt,
+                                                 const double *__restrict__ abs_, const double *__restrict__ ord,
+                                                 const DevCtl *__restrict__ ctl, double indir_px, double indir_py,
+                                                 double *__restrict__ pagr, double *__restrict__ pagt) {
+  b_indirect(g, agr, agt, abs_, ord, ctl, indir_px, indir_py, pagr, pagt);
+}
+
 
 // ---------------------------------------------------------------------------
 // Pebble fluid (Pebbles.c), the second fluid of the two-fluid runs.
@@ -969,7 +1020,7 @@ __global__ void __launch_bounds__(TB) k_indirect(Geo g, double *__restrict__ agr
 // pebble vtheta; lanes 0 and 1 of the first wave walk the two ring sums side
 // by side.  The ring maximum of the four squared inverse times goes to
 // pebmax[i] (ring 0 is not part of CriticalCharTime).
-__global__ void __launch_bounds__(RB) k_peb_cfl(Geo g, const double *__restrict__ rho,
+__device__ __forceinline__ void b_peb_cfl(Geo g, const double *__restrict__ rho,
                                                 const double *__restrict__ psize, double *__restrict__ tau,
                                                 const double *__restrict__ pvr, const double *__restrict__ pvt,
                                                 const double *__restrict__ vr, const double *__restrict__ vt,
@@ -978,7 +1029,7 @@ __global__ void __launch_bounds__(RB) k_peb_cfl(Geo g, const double *__restrict_
   __shared__ double red[RB / 64];
   __shared__ double pvtmed;
   if (guard && *guard == 0) return;
-  const int i = blockIdx.x, ns = g.ns, tid = threadIdx.x;
+  const int i = (int)blockIdx.x - g.blk0, ns = g.ns, tid = threadIdx.x;
   double *s_tau = sh, *s_pvt = sh + ns;
   const size_t base = (size_t)i * ns;
   for (int j = tid; j < ns; j += RB) {
@@ -1015,11 +1066,19 @@ __global__ void __launch_bounds__(RB) k_peb_cfl(Geo g, const double *__restrict_
   m = block_max<RB>(m, red);
   if (tid == 0) g.pebmax[i] = m;
 }
+__global__ void __launch_bounds__(RB) k_peb_cfl(Geo g, const double *__restrict__ rho,
+                                                const double *__restrict__ psize, double *__restrict__ tau,
+                                                const double *__restrict__ pvr, const double *__restrict__ pvt,
+                                                const double *__restrict__ vr, const double *__restrict__ vt,
+                                                const int *__restrict__ guard) {
+  b_peb_cfl(g, rho, psize, tau, pvr, pvt, vr, vt, guard);
+}
+
 
 // DampPebbles (SideEuler.c:423-454) on the rings of the damping zones; with
 // detect, DetectCrashPebbles (SourceEuler.c:274, 48-66) first looks at every
 // cell's density before the damping.
-__global__ void __launch_bounds__(TB) k_peb_damp(Geo g, double *__restrict__ pdens, double *__restrict__ pvr,
+__device__ __forceinline__ void b_peb_damp(Geo g, double *__restrict__ pdens, double *__restrict__ pvr,
                                                  double *__restrict__ pvt, double dt, int detect, DevCtl *ctl,
                                                  const int *__restrict__ guard) {
   const int ns = g.ns;
@@ -1038,11 +1097,17 @@ __global__ void __launch_bounds__(TB) k_peb_damp(Geo g, double *__restrict__ pde
   pvr[l] = (pvr[l] + lambda * pvr0) / (1.0 + lambda);
   pvt[l] = (pvt[l] + lambda * pvt0) / (1.0 + lambda);
 }
+__global__ void __launch_bounds__(TB) k_peb_damp(Geo g, double *__restrict__ pdens, double *__restrict__ pvr,
+                                                 double *__restrict__ pvt, double dt, int detect, DevCtl *ctl,
+                                                 const int *__restrict__ guard) {
+  b_peb_damp(g, pdens, pvr, pvt, dt, detect, ctl, guard);
+}
+
 
 // SourceTermsPebbles after its DampPebbles (Pebbles.c:549-612): pebble
 // accelerations paccr (rings >= 1) and pacct and, with BackReaction, the
 // drag on the gas dragr (rings >= 1) and dragt.
-__global__ void __launch_bounds__(TB) k_peb_source(Geo g, const double *__restrict__ pdens,
+__device__ __forceinline__ void b_peb_source(Geo g, const double *__restrict__ pdens,
                                                    const double *__restrict__ pvr, const double *__restrict__ pvt,
                                                    const double *__restrict__ tau, const double *__restrict__ vr,
                                                    const double *__restrict__ vt, const double *__restrict__ pagr,
@@ -1073,12 +1138,22 @@ __global__ void __launch_bounds__(TB) k_peb_source(Geo g, const double *__restri
     dragt[l] = 0.5 * (pdens[l] + pdens[ljm]) * drag;
   }
 }
+__global__ void __launch_bounds__(TB) k_peb_source(Geo g, const double *__restrict__ pdens,
+                                                   const double *__restrict__ pvr, const double *__restrict__ pvt,
+                                                   const double *__restrict__ tau, const double *__restrict__ vr,
+                                                   const double *__restrict__ vt, const double *__restrict__ pagr,
+                                                   const double *__restrict__ pagt, double *__restrict__ paccr,
+                                                   double *__restrict__ pacct, double *__restrict__ dragr,
+                                                   double *__restrict__ dragt) {
+  b_peb_source(g, pdens, pvr, pvt, tau, vr, vt, pagr, pagt, paccr, pacct, dragr, dragt);
+}
+
 
 // SubStep1Pebbles (Pebbles.c:616-650), the semi-implicit drag step with the
 // gas velocities from before SubStep1, then ParticleDiffusion
 // (Pebbles.c:654-695), whose corrections depend on the two densities only
 // (so one pass over the cells does both).
-__global__ void __launch_bounds__(TB) k_peb_substep1(Geo g, double *__restrict__ pvr, double *__restrict__ pvt,
+__device__ __forceinline__ void b_peb_substep1(Geo g, double *__restrict__ pvr, double *__restrict__ pvt,
                                                      const double *__restrict__ tau, const double *__restrict__ paccr,
                                                      const double *__restrict__ pacct, const double *__restrict__ accr,
                                                      const double *__restrict__ acct, const double *__restrict__ vr,
@@ -1113,9 +1188,18 @@ __global__ void __launch_bounds__(TB) k_peb_substep1(Geo g, double *__restrict__
   pvr[l] = nvr;
   pvt[l] = nvt;
 }
+__global__ void __launch_bounds__(TB) k_peb_substep1(Geo g, double *__restrict__ pvr, double *__restrict__ pvt,
+                                                     const double *__restrict__ tau, const double *__restrict__ paccr,
+                                                     const double *__restrict__ pacct, const double *__restrict__ accr,
+                                                     const double *__restrict__ acct, const double *__restrict__ vr,
+                                                     const double *__restrict__ vt, const double *__restrict__ rho,
+                                                     const double *__restrict__ pdens, double dt, int diffuse) {
+  b_peb_substep1(g, pvr, pvt, tau, paccr, pacct, accr, acct, vr, vt, rho, pdens, dt, diffuse);
+}
+
 
 // EvaporatePebbles (Pebbles.c:837-866)
-__global__ void __launch_bounds__(TB) k_peb_evaporate(Geo g, double *__restrict__ pdens, const double *__restrict__ T,
+__device__ __forceinline__ void b_peb_evaporate(Geo g, double *__restrict__ pdens, const double *__restrict__ T,
                                                       double facc, double evaptemp) {
   const int ns = g.ns;
   int i, j, L_;
@@ -1127,6 +1211,11 @@ __global__ void __launch_bounds__(TB) k_peb_evaporate(Geo g, double *__restrict_
     pdens[l] = d;
   }
 }
+__global__ void __launch_bounds__(TB) k_peb_evaporate(Geo g, double *__restrict__ pdens, const double *__restrict__ T,
+                                                      double facc, double evaptemp) {
+  b_peb_evaporate(g, pdens, T, facc, evaptemp);
+}
+
 
 // IntegrateColumnMass (Pebbles.c:325-342) over the refinements of Trapzd
 // (Pebbles.c:295-320); ok = 0 if it does not converge in TRAPEZMAX steps
@@ -1177,7 +1266,7 @@ struct PebAccK {
 // and the x- and y-momentum each planet takes from the disk go to
 // ctl->pacc; the host updates the planet as STEP 7 does.
 #define ACC_T 1024
-__global__ void __launch_bounds__(ACC_T) k_peb_accrete(Geo g, PebAccK P, double *__restrict__ pdens,
+__device__ __forceinline__ void b_peb_accrete(Geo g, PebAccK P, double *__restrict__ pdens,
                                                        const double *__restrict__ pvr, const double *__restrict__ pvt,
                                                        const double *__restrict__ tau, const double *__restrict__ cs,
                                                        const double *__restrict__ abs_, const double *__restrict__ ord,
@@ -1277,10 +1366,18 @@ __global__ void __launch_bounds__(ACC_T) k_peb_accrete(Geo g, PebAccK P, double 
     __syncthreads();
   }
 }
+__global__ void __launch_bounds__(ACC_T) k_peb_accrete(Geo g, PebAccK P, double *__restrict__ pdens,
+                                                       const double *__restrict__ pvr, const double *__restrict__ pvt,
+                                                       const double *__restrict__ tau, const double *__restrict__ cs,
+                                                       const double *__restrict__ abs_, const double *__restrict__ ord,
+                                                       double *__restrict__ macc, double dt, DevCtl *ctl) {
+  b_peb_accrete(g, P, pdens, pvr, pvt, tau, cs, abs_, ord, macc, dt, ctl);
+}
+
 
 // ---------------------------------------------------------------------------
 // DampingBoundary (SideEuler.c:368-419) on the rings flagged by the host
-__global__ void __launch_bounds__(TB) k_boundary(Geo g, double *__restrict__ vrad, double *__restrict__ vtheta,
+__device__ __forceinline__ void b_boundary(Geo g, double *__restrict__ vrad, double *__restrict__ vtheta,
                                                  double *__restrict__ rho, double *__restrict__ energy, double step,
                                                  const int *__restrict__ guard) {
   const int ns = g.ns;
@@ -1307,12 +1404,18 @@ __global__ void __launch_bounds__(TB) k_boundary(Geo g, double *__restrict__ vra
   }
   if (g.damp_flux) rho[l] = (rho[l] + lambda * rho0) / (1.0 + lambda);
 }
+__global__ void __launch_bounds__(TB) k_boundary(Geo g, double *__restrict__ vrad, double *__restrict__ vtheta,
+                                                 double *__restrict__ rho, double *__restrict__ energy, double step,
+                                                 const int *__restrict__ guard) {
+  b_boundary(g, vrad, vtheta, rho, energy, step, guard);
+}
+
 
 // ---------------------------------------------------------------------------
 // SubStep1 source terms (SourceEuler.c:319-406) with P = (gamma-1) e on the
 // fly (ComputePressureField, SourceEuler.c:740-762) and DetectCrash
 // (SourceEuler.c:48-66, 272-273).
-__global__ void __launch_bounds__(TB) k_substep1(Geo g, const double *__restrict__ rho,
+__device__ __forceinline__ void b_substep1(Geo g, const double *__restrict__ rho,
                                                  const double *__restrict__ energy,
                                                  const double *__restrict__ vrad,
                                                  const double *__restrict__ vtheta,
@@ -1375,6 +1478,19 @@ __global__ void __launch_bounds__(TB) k_substep1(Geo g, const double *__restrict
     vthetaint[l] = v;
   }
 }
+__global__ void __launch_bounds__(TB) k_substep1(Geo g, const double *__restrict__ rho,
+                                                 const double *__restrict__ energy,
+                                                 const double *__restrict__ vrad,
+                                                 const double *__restrict__ vtheta,
+                                                 const double *__restrict__ agr,
+                                                 const double *__restrict__ agt,
+                                                 double *__restrict__ vradint, double *__restrict__ vthetaint,
+                                                 double dt, DevCtl *ctl, double *__restrict__ accr,
+                                                 double *__restrict__ acct, const double *__restrict__ dragr,
+                                                 const double *__restrict__ dragt) {
+  b_substep1(g, rho, energy, vrad, vtheta, agr, agt, vradint, vthetaint, dt, ctl, accr, acct, dragr, dragt);
+}
+
 
 // UpdateDivVelocAndStressTensor (Viscosity.c:85-162); Drr, Dpp, Drp stay in
 // registers.
@@ -1414,7 +1530,7 @@ __device__ __forceinline__ Stress stress_cell(const Geo &g, const double *__rest
   return st;
 }
 
-__global__ void __launch_bounds__(TB) k_divstress(Geo g, const double *__restrict__ vr,
+__device__ __forceinline__ void b_divstress(Geo g, const double *__restrict__ vr,
                                                   const double *__restrict__ vt,
                                                   const double *__restrict__ rho,
                                                   double *__restrict__ divv, double *__restrict__ Trr,
@@ -1429,10 +1545,18 @@ __global__ void __launch_bounds__(TB) k_divstress(Geo g, const double *__restric
   Tpp[l] = st.tpp;
   if (st.has_trp) Trp[l] = st.trp;
 }
+__global__ void __launch_bounds__(TB) k_divstress(Geo g, const double *__restrict__ vr,
+                                                  const double *__restrict__ vt,
+                                                  const double *__restrict__ rho,
+                                                  double *__restrict__ divv, double *__restrict__ Trr,
+                                                  double *__restrict__ Trp, double *__restrict__ Tpp) {
+  b_divstress(g, vr, vt, rho, divv, Trr, Trp, Tpp);
+}
+
 
 // UpdateVelocityWithViscousTerms (Viscosity.c:165-218) + ImposeKeplerianEdges
 // (Viscosity.c:221-260, VKepIn/VKepOut from the host)
-__global__ void __launch_bounds__(TB) k_visc_update(Geo g, double *__restrict__ vr, double *__restrict__ vt,
+__device__ __forceinline__ void b_visc_update(Geo g, double *__restrict__ vr, double *__restrict__ vt,
                                                     const double *__restrict__ rho,
                                                     const double *__restrict__ Trr,
                                                     const double *__restrict__ Trp,
@@ -1462,13 +1586,22 @@ __global__ void __launch_bounds__(TB) k_visc_update(Geo g, double *__restrict__ 
     if (i == nr - 1) vt[l] = vkout - g.Rmed[nr - 1] * g.omega_frame;
   }
 }
+__global__ void __launch_bounds__(TB) k_visc_update(Geo g, double *__restrict__ vr, double *__restrict__ vt,
+                                                    const double *__restrict__ rho,
+                                                    const double *__restrict__ Trr,
+                                                    const double *__restrict__ Trp,
+                                                    const double *__restrict__ Tpp, double DeltaT,
+                                                    int kepler, double vkin, double vkout) {
+  b_visc_update(g, vr, vt, rho, Trr, Trp, Tpp, DeltaT, kepler, vkin, vkout);
+}
+
 
 // SubStep2 (SourceEuler.c:417-504): artificial viscosity q computed on the fly
 __device__ __forceinline__ double qvnr(double rho, double dv) {
   return (dv < 0.0) ? CVNR * CVNR * rho * dv * dv : 0.0;
 }
 
-__global__ void __launch_bounds__(TB) k_substep2(Geo g, const double *__restrict__ rho,
+__device__ __forceinline__ void b_substep2(Geo g, const double *__restrict__ rho,
                                                  const double *__restrict__ vrad,
                                                  const double *__restrict__ vtheta,
                                                  const double *__restrict__ energy,
@@ -1501,6 +1634,15 @@ __global__ void __launch_bounds__(TB) k_substep2(Geo g, const double *__restrict
   energyint[l] = energy[l] - dt * qr * (vrad[lip] - vrad[l]) * g.InvDiffRsup[i] -
                  dt * qt * (vtheta[ljp] - vtheta[l]) * invdxtheta;
 }
+__global__ void __launch_bounds__(TB) k_substep2(Geo g, const double *__restrict__ rho,
+                                                 const double *__restrict__ vrad,
+                                                 const double *__restrict__ vtheta,
+                                                 const double *__restrict__ energy,
+                                                 double *__restrict__ vradnew, double *__restrict__ vthetanew,
+                                                 double *__restrict__ energyint, double dt) {
+  b_substep2(g, rho, vrad, vtheta, energy, vradnew, vthetanew, energyint, dt);
+}
+
 
 __device__ __forceinline__ double flux_limiter(double s) {   // EnergySources.c:684-696
   double tmp;
@@ -1535,7 +1677,7 @@ __device__ __forceinline__ double qplus_of(const Geo &g, int i, double rho, cons
   return 0.0;
 }
 
-__global__ void __launch_bounds__(TB) k_substep3(Geo g, const double *__restrict__ vr, const double *__restrict__ vt,
+__device__ __forceinline__ void b_substep3(Geo g, const double *__restrict__ vr, const double *__restrict__ vt,
                                                  const double *__restrict__ rho, const double *__restrict__ eint,
                                                  double *__restrict__ divv, double *__restrict__ Trr,
                                                  double *__restrict__ Trp, double *__restrict__ Tpp,
@@ -1545,11 +1687,11 @@ __global__ void __launch_bounds__(TB) k_substep3(Geo g, const double *__restrict
                                                  double *__restrict__ pmin, int fill_opacity, int nsum) {
   __shared__ double red[TB];
   const int ns = g.ns, nr = g.nr;
-  if ((int)blockIdx.x < nsum) {
+  if ((int)blockIdx.x - g.blk0 < nsum) {
     // leading blocks: VMed of ComputeResiduals (TransportEuler.c:86-91) for
     // the coming transport, 64 rings per block, lane r walking ring r; vt is
     // final for the transport from here on
-    const int r = blockIdx.x * 64 + threadIdx.x;
+    const int r = ((int)blockIdx.x - g.blk0) * 64 + threadIdx.x;
     if (threadIdx.x < 64 && r < nr) g.VMed[r] = ordered_row_sum_global(vt + (size_t)r * ns, ns) / (double)ns;
     return;
   }
@@ -1626,6 +1768,17 @@ __global__ void __launch_bounds__(TB) k_substep3(Geo g, const double *__restrict
   dtmin = block_min<TB>(dtmin, red);
   if (threadIdx.x == 0) pmin[L] = dtmin;
 }
+__global__ void __launch_bounds__(TB) k_substep3(Geo g, const double *__restrict__ vr, const double *__restrict__ vt,
+                                                 const double *__restrict__ rho, const double *__restrict__ eint,
+                                                 double *__restrict__ divv, double *__restrict__ Trr,
+                                                 double *__restrict__ Trp, double *__restrict__ Tpp,
+                                                 double *__restrict__ qplus, double *__restrict__ T,
+                                                 double *__restrict__ cs, double *__restrict__ opa,
+                                                 double *__restrict__ qminus, double *__restrict__ D,
+                                                 double *__restrict__ pmin, int fill_opacity, int nsum) {
+  b_substep3(g, vr, vt, rho, eint, divv, Trr, Trp, Tpp, qplus, T, cs, opa, qminus, D, pmin, fill_opacity, nsum);
+}
+
 
 // implicit energy matrix (EnergySources.c:256-297) with A and B rebuilt on the fly
 __device__ __forceinline__ double coefA(const Geo &g, const double *D, int i, size_t l) {
@@ -1644,7 +1797,7 @@ __device__ __forceinline__ double coefB(const Geo &g, const double *D, const dou
 // The matrix kernel also returns the initial SOR residual sum
 // (EnergySources.c:435-449): block partials of |T.M - RHS| from the values
 // it has just computed.
-__global__ void __launch_bounds__(TB) k_matrix(Geo g, const double *__restrict__ rho,
+__device__ __forceinline__ void b_matrix(Geo g, const double *__restrict__ rho,
                                                const double *__restrict__ cs, const double *__restrict__ D,
                                                const double *__restrict__ divv,
                                                const double *__restrict__ qminus,
@@ -1683,6 +1836,18 @@ __global__ void __launch_bounds__(TB) k_matrix(Geo g, const double *__restrict__
   r0 = block_sum<TB>(r0, red);
   if (threadIdx.x == 0) presid[L] = r0;
 }
+__global__ void __launch_bounds__(TB) k_matrix(Geo g, const double *__restrict__ rho,
+                                               const double *__restrict__ cs, const double *__restrict__ D,
+                                               const double *__restrict__ divv,
+                                               const double *__restrict__ qminus,
+                                               const double *__restrict__ qplus, const double *__restrict__ T,
+                                               double *__restrict__ Ml, double *__restrict__ Mlip,
+                                               double *__restrict__ Mlim, double *__restrict__ Mljp,
+                                               double *__restrict__ Mljm, double *__restrict__ RHS, double dt,
+                                               double *__restrict__ presid) {
+  b_matrix(g, rho, cs, D, divv, qminus, qplus, T, Ml, Mlip, Mlim, Mljp, Mljm, RHS, dt, presid);
+}
+
 
 // ---------------------------------------------------------------------------
 // SOR (EnergySources.c:406-511).  Red-black in place: within one pass every
@@ -1696,7 +1861,7 @@ __device__ __forceinline__ double sor_resid(const double *T, const double *Ml, c
   return T[l] * Ml[l] + T[lip] * Mlip[l] + T[lim] * Mlim[l] + T[ljp] * Mljp[l] + T[ljm] * Mljm[l] - RHS[l];
 }
 
-__global__ void __launch_bounds__(TB) k_sor_resid0(Geo g, const double *__restrict__ T,
+__device__ __forceinline__ void b_sor_resid0(Geo g, const double *__restrict__ T,
                                                    const double *__restrict__ Ml, const double *__restrict__ Mlip,
                                                    const double *__restrict__ Mlim, const double *__restrict__ Mljp,
                                                    const double *__restrict__ Mljm, const double *__restrict__ RHS,
@@ -1715,6 +1880,14 @@ __global__ void __launch_bounds__(TB) k_sor_resid0(Geo g, const double *__restri
   r = block_sum<TB>(r, red);
   if (threadIdx.x == 0) partial[L] = r;
 }
+__global__ void __launch_bounds__(TB) k_sor_resid0(Geo g, const double *__restrict__ T,
+                                                   const double *__restrict__ Ml, const double *__restrict__ Mlip,
+                                                   const double *__restrict__ Mlim, const double *__restrict__ Mljp,
+                                                   const double *__restrict__ Mljm, const double *__restrict__ RHS,
+                                                   double *__restrict__ partial) {
+  b_sor_resid0(g, T, Ml, Mlip, Mlim, Mljp, Mljm, RHS, partial);
+}
+
 
 
 // One SOR iteration (both half-sweeps) per launch.  A workgroup owns a tile
@@ -1769,7 +1942,7 @@ __device__ __forceinline__ void sor_update(const SorCell &x, double *t, double o
 #define SOR_NRED (((SOR_TI + 2) * ((SOR_TJ + 3) / 2) + TB - 1) / TB)
 #define SOR_NBLK ((SOR_TI * ((SOR_TJ + 1) / 2) + TB - 1) / TB)
 
-__global__ void __launch_bounds__(TB) k_sor_iter(Geo g, double *__restrict__ T0, double *__restrict__ T1,
+__device__ __forceinline__ void b_sor_iter(Geo g, double *__restrict__ T0, double *__restrict__ T1,
                                                  const double *__restrict__ Ml, const double *__restrict__ Mlip,
                                                  const double *__restrict__ Mlim, const double *__restrict__ Mljp,
                                                  const double *__restrict__ Mljm, const double *__restrict__ RHS,
@@ -1781,7 +1954,7 @@ __global__ void __launch_bounds__(TB) k_sor_iter(Geo g, double *__restrict__ T0,
   const int ns = g.ns, nr = g.nr;
   const int ntx = (ns + SOR_TJ - 1) / SOR_TJ;
   int tx, ty, bid;
-  if (!xcd_tile(ntx, (nr - 2 + SOR_TI - 1) / SOR_TI, tx, ty, bid)) return;
+  if (!xcd_tile(ntx, (nr - 2 + SOR_TI - 1) / SOR_TI, tx, ty, bid, g.blk0)) return;
   const int i0 = 1 + ty * SOR_TI;
   const int i1 = min(i0 + SOR_TI, nr - 1);          // owned rings [i0, i1) within [1, nr-2]
   const int j0 = tx * SOR_TJ;
@@ -1883,10 +2056,20 @@ __global__ void __launch_bounds__(TB) k_sor_iter(Geo g, double *__restrict__ T0,
     pcnt[bid] = (int)nch;
   }
 }
+__global__ void __launch_bounds__(TB) k_sor_iter(Geo g, double *__restrict__ T0, double *__restrict__ T1,
+                                                 const double *__restrict__ Ml, const double *__restrict__ Mlip,
+                                                 const double *__restrict__ Mlim, const double *__restrict__ Mljp,
+                                                 const double *__restrict__ Mljm, const double *__restrict__ RHS,
+                                                 double omega, double *__restrict__ psum, int *__restrict__ pcnt,
+                                                 DevCtl *__restrict__ ctl, int it,
+                                                 const double *__restrict__ mresp, int nbm) {
+  b_sor_iter(g, T0, T1, Ml, Mlip, Mlim, Mljp, Mljm, RHS, omega, psum, pcnt, ctl, it, mresp, nbm);
+}
+
 
 // stop test of iteration it (EnergySources.c:487-493), partial sums added in
 // block order; the last iterate is in T1 after an odd number of iterations
-__global__ void __launch_bounds__(RB) k_sor_check(const double *__restrict__ psum, const int *__restrict__ pcnt,
+__device__ __forceinline__ void b_sor_check(const double *__restrict__ psum, const int *__restrict__ pcnt,
                                                   int nb, DevCtl *ctl, int maxiter, int it, double eps) {
   __shared__ double red[RB];
   if (ctl->sor_done) return;
@@ -1919,12 +2102,17 @@ __global__ void __launch_bounds__(RB) k_sor_check(const double *__restrict__ psu
     }
   }
 }
+__global__ void __launch_bounds__(RB) k_sor_check(const double *__restrict__ psum, const int *__restrict__ pcnt,
+                                                  int nb, DevCtl *ctl, int maxiter, int it, double eps) {
+  b_sor_check(psum, pcnt, nb, ctl, maxiter, it, eps);
+}
+
 
 // edges of T and EnergyNew = rho*T*CV (EnergySources.c:319-334).  The SOR's
 // last iterate is in T0 or T1 (ctl->sor_cur); T0 receives it, so that T0
 // holds the solved temperature afterwards.  Nothing is done until the SOR
 // has converged (ctl->sor_done).
-__global__ void __launch_bounds__(TB) k_energy_new(Geo g, const double *__restrict__ rho, double *__restrict__ T0,
+__device__ __forceinline__ void b_energy_new(Geo g, const double *__restrict__ rho, double *__restrict__ T0,
                                                    const double *__restrict__ T1, double *__restrict__ enew,
                                                    const DevCtl *__restrict__ ctl) {
   const int ns = g.ns, nr = g.nr;
@@ -1941,6 +2129,12 @@ __global__ void __launch_bounds__(TB) k_energy_new(Geo g, const double *__restri
   enew[l] = rho[l] * t * g.cv;
   if (i == 0) enew[(size_t)nr * ns + j] = 0.0;
 }
+__global__ void __launch_bounds__(TB) k_energy_new(Geo g, const double *__restrict__ rho, double *__restrict__ T0,
+                                                   const double *__restrict__ T1, double *__restrict__ enew,
+                                                   const DevCtl *__restrict__ ctl) {
+  b_energy_new(g, rho, T0, T1, enew, ctl);
+}
+
 
 // ---------------------------------------------------------------------------
 // Radial transport: ComputeLRMomenta (TransportEuler.c:318-347) + OneWindRad
@@ -2066,12 +2260,12 @@ __device__ __forceinline__ void rad_column(const Geo &g, const RadQ &q, const Ra
   }
 }
 
-__global__ void __launch_bounds__(TB) k_transport_rad(Geo g, RadQ q, double dt) {
+__device__ __forceinline__ void b_transport_rad(Geo g, RadQ q, double dt) {
   __shared__ RadGeo sg;
   if (q.guard && *q.guard == 0) return;
   // XCD x (= b % 8) gets the contiguous band of tiles x*per .. x*per+per-1,
   // all six quantities of each
-  const int b = blockIdx.x, slot = b & 7, rest = b >> 3;
+  const int b = (int)blockIdx.x - g.blk0, slot = b & 7, rest = b >> 3;
   int which = rest % q.nq;
   if (q.nq == RC_NQ - 1 && which == 4) which = 5;
   const int per = (q.ntiles + 7) >> 3;
@@ -2099,6 +2293,10 @@ __global__ void __launch_bounds__(TB) k_transport_rad(Geo g, RadQ q, double dt) 
     default: rad_column<5>(g, q, sg, dt, j, i0, i1); break;
   }
 }
+__global__ void __launch_bounds__(TB) k_transport_rad(Geo g, RadQ q, double dt) {
+  b_transport_rad(g, q, dt);
+}
+
 
 // Ordered ring means out[r] = (sum over j in order of f[r][j]) / NSEC
 // (mpi_make1Dprofile order, mpiTasks.c:26-32) for `rows` rings per
@@ -2106,11 +2304,11 @@ __global__ void __launch_bounds__(TB) k_transport_rad(Geo g, RadQ q, double dt) 
 // all loads in flight together), then lane r of the first wave walks ring r.
 // The chain of NSEC dependent additions per ring is the cost; a workgroup's
 // rings share it.
-__global__ void __launch_bounds__(256) k_ring_means(const double *__restrict__ f, int nr, int ns, int rows,
-                                                     double *__restrict__ out, const int *__restrict__ guard) {
+__device__ __forceinline__ void b_ring_means(const double *__restrict__ f, int nr, int ns, int rows,
+                                             double *__restrict__ out, const int *__restrict__ guard, BOff bo) {
   extern __shared__ double sh[];       // rows * ns
   if (guard && *guard == 0) return;
-  const int r0 = blockIdx.x * rows;
+  const int r0 = ((int)blockIdx.x - bo.blk0) * rows;
   const int nrow = min(rows, nr - r0);
   const int n = nrow * ns;
   const double *__restrict__ src = f + (size_t)r0 * ns;
@@ -2121,6 +2319,12 @@ __global__ void __launch_bounds__(256) k_ring_means(const double *__restrict__ f
     out[r0 + threadIdx.x] = m;
   }
 }
+__global__ void __launch_bounds__(256) k_ring_means(const double *__restrict__ f, int nr, int ns, int rows,
+                                                     double *__restrict__ out, const int *__restrict__ guard,
+                                                     BOff bo) {
+  b_ring_means(f, nr, ns, rows, out, guard, bo);
+}
+
 
 // Azimuthal transport, ring-local: ComputeResiduals (TransportEuler.c:70-111),
 // QuantitiesAdvection with the residual and then the uniform velocity
@@ -2194,7 +2398,7 @@ __device__ __forceinline__ void th_pass(const ThGeom &t, double *qv, const doubl
 }
 
 template <int NPT>
-__global__ void __launch_bounds__(TT_T) k_transport_theta(Geo g, const double *__restrict__ vt,
+__device__ __forceinline__ void b_transport_theta(Geo g, const double *__restrict__ vt,
                                                           const double *__restrict__ xrp, const double *__restrict__ xrm,
                                                           const double *__restrict__ xtp, const double *__restrict__ xtm,
                                                           const double *__restrict__ xe, const double *__restrict__ xrho,
@@ -2205,7 +2409,7 @@ __global__ void __launch_bounds__(TT_T) k_transport_theta(Geo g, const double *_
   // nq quantities after rho: 5 (momenta and energy), 4 for the pebble fluid
   extern __shared__ double sh[];       // 2*ns
   if (guard && *guard == 0) return;
-  const int i = blockIdx.x, ns = g.ns, tid = threadIdx.x;
+  const int i = (int)blockIdx.x - g.blk0, ns = g.ns, tid = threadIdx.x;
   double *s_w = sh, *s_f = sh + ns;
   const size_t base = (size_t)i * ns;
   __shared__ double vmed, vtemp;
@@ -2314,9 +2518,21 @@ __global__ void __launch_bounds__(TT_T) k_transport_theta(Geo g, const double *_
     store_shifted(outs[which]);
   }
 }
+template <int NPT>
+__global__ void __launch_bounds__(TT_T) k_transport_theta(Geo g, const double *__restrict__ vt,
+                                                          const double *__restrict__ xrp, const double *__restrict__ xrm,
+                                                          const double *__restrict__ xtp, const double *__restrict__ xtm,
+                                                          const double *__restrict__ xe, const double *__restrict__ xrho,
+                                                          double *__restrict__ orp, double *__restrict__ orm,
+                                                          double *__restrict__ otp, double *__restrict__ otm,
+                                                          double *__restrict__ oe, double *__restrict__ orho,
+                                                          double dt, const int *__restrict__ guard, int nq) {
+  b_transport_theta<NPT>(g, vt, xrp, xrm, xtp, xtm, xe, xrho, orp, orm, otp, otm, oe, orho, dt, guard, nq);
+}
+
 
 // ComputeVelocities (TransportEuler.c:350-381)
-__global__ void __launch_bounds__(TB) k_velocities(Geo g, const double *__restrict__ rho,
+__device__ __forceinline__ void b_velocities(Geo g, const double *__restrict__ rho,
                                                    const double *__restrict__ rp, const double *__restrict__ rm,
                                                    const double *__restrict__ tp, const double *__restrict__ tm,
                                                    double *__restrict__ vr, double *__restrict__ vt,
@@ -2334,6 +2550,14 @@ __global__ void __launch_bounds__(TB) k_velocities(Geo g, const double *__restri
   }
   vt[l] = (tp[ljm] + tm[l]) / (rho[l] + rho[ljm] + 1e-15) / g.Rmed[i] - g.Rmed[i] * g.omega_frame;
 }
+__global__ void __launch_bounds__(TB) k_velocities(Geo g, const double *__restrict__ rho,
+                                                   const double *__restrict__ rp, const double *__restrict__ rm,
+                                                   const double *__restrict__ tp, const double *__restrict__ tm,
+                                                   double *__restrict__ vr, double *__restrict__ vt,
+                                                   const int *__restrict__ guard) {
+  b_velocities(g, rho, rp, rm, tp, tm, vr, vt, guard);
+}
+
 
 // GasTotalMass partial sums: per ring, sequential in j (SideEuler.c:35-39)
 __global__ void __launch_bounds__(RB) k_ring_mass(Geo g, const double *__restrict__ rho, double *__restrict__ out) {
@@ -2684,18 +2908,6 @@ extern "C" int thorin_sync(thorin_ctx *c) {
   return 0;
 }
 
-extern "C" int thorin_ring_means(thorin_ctx *c, int field, double *out) {
-  if (need_dev(c)) return -1;
-  thorin_dev *d = c->dev;
-  if (field < 0 || field >= THORIN_NFIELDS) return -1;
-  hipLaunchKernelGGL(k_ring_means, dim3((unsigned)((d->nr + d->rs_rows - 1) / d->rs_rows)), dim3(256),
-                     sizeof(double) * (size_t)d->rs_rows * d->ns, d->s, (const double *)d->b[field], d->nr, d->ns,
-                     d->rs_rows, d->ringtmp, (const int *)nullptr);
-  HCHECK(hipGetLastError());
-  HCHECK(hipMemcpyAsync(out, d->ringtmp, sizeof(double) * d->nr, hipMemcpyDeviceToHost, d->s));
-  HCHECK(hipStreamSynchronize(d->s));
-  return 0;
-}
 
 __global__ void k_int_to_double(const int *__restrict__ a, double *__restrict__ b, int n) {
   int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -2767,21 +2979,285 @@ static int prof_flush(thorin_dev *d) {
 
 // launch on the context's stream; with the timer on, bracket the kernels of
 // the selected ids with HIP events on that same stream
-#define LAUNCH(kern, grid, block, shmem, ...)                                              \
-  do {                                                                                     \
-    int kid_ = d->prof_on ? kid_of(#kern) : -1;                                            \
-    hipEvent_t ea_ = nullptr, eb_ = nullptr;                                               \
-    if (kid_ >= 0 && ((d->prof_mask >> kid_) & 1ull)) {                                    \
-      ea_ = prof_event(d);                                                                 \
-      eb_ = prof_event(d);                                                                 \
-      if (ea_ && eb_) (void)hipEventRecord(ea_, d->s);                                     \
-    }                                                                                      \
-    hipLaunchKernelGGL(kern, grid, block, shmem, d->s, __VA_ARGS__);                       \
-    if (ea_ && eb_) {                                                                      \
-      (void)hipEventRecord(eb_, d->s);                                                     \
-      d->prof_pend->push_back(std::make_pair(kid_, std::make_pair(ea_, eb_)));             \
-    }                                                                                      \
+// ---------------------------------------------------------------------------
+// Ensemble execution (thorin_ensemble_*).
+//
+// Each disk of an ensemble runs its own AlgoGas host logic on a host thread
+// of its own, but issues nothing to the GPU: kernel launches, copies and
+// stream syncs go through dev_launch / dev_copy / dev_sync, which in ensemble
+// mode append to the disk's queue (Rec) and, for a sync, block the thread.
+// When every disk thread waits or has finished, the ensemble merges the
+// queues into one stream: the head operations that launch the same kernel
+// become ONE launch over the disks' grids laid end to end (k_batch: a
+// workgroup finds its disk from the block offsets and runs the kernel body
+// with that disk's arguments; the disk's Geo / BOff tell the body its first
+// workgroup), copies run as recorded.  Every disk's own order of operations
+// is kept, and each workgroup runs exactly the code and arguments of the
+// plain launch, so every disk computes what its own AlgoGas computes, bit for
+// bit.
+
+template <typename... P>
+std::tuple<P...> params_of(void (*)(P...));
+
+template <int NT, auto Body, typename... P>
+__global__ void __launch_bounds__(NT) k_batch(const std::tuple<P...> *__restrict__ tab,
+                                              const int *__restrict__ off, int n) {
+  int d = 0;
+  while (d + 1 < n && (int)blockIdx.x >= off[d + 1]) d++;
+  std::apply(Body, tab[d]);
+}
+
+template <int NT, auto Body, typename T>
+struct BatchKernel;
+template <int NT, auto Body, typename... P>
+struct BatchKernel<NT, Body, std::tuple<P...>> {
+  static constexpr auto fn = &k_batch<NT, Body, P...>;
+};
+
+// kernels whose body can run in a batched launch
+template <auto K>
+struct Batch {
+  static constexpr bool ok = false;
+};
+#define BATCH_BODY(KERN, BODY, NT)             \
+  template <>                                  \
+  struct Batch<&KERN> {                        \
+    static constexpr bool ok = true;           \
+    static constexpr auto body = &BODY;        \
+    static constexpr int nt = NT;              \
+  };
+BATCH_BODY(k_eos_cfl, b_eos_cfl, RB)
+BATCH_BODY(k_cfl_final, b_cfl_final, RB)
+BATCH_BODY(k_forces<false>, b_forces<false>, TB)
+BATCH_BODY(k_forces<true>, b_forces<true>, TB)
+BATCH_BODY(k_forces_reduce, b_forces_reduce, RB)
+BATCH_BODY(k_indirect, b_indirect, TB)
+BATCH_BODY(k_peb_cfl, b_peb_cfl, RB)
+BATCH_BODY(k_peb_damp, b_peb_damp, TB)
+BATCH_BODY(k_peb_source, b_peb_source, TB)
+BATCH_BODY(k_peb_substep1, b_peb_substep1, TB)
+BATCH_BODY(k_peb_evaporate, b_peb_evaporate, TB)
+BATCH_BODY(k_peb_accrete, b_peb_accrete, ACC_T)
+BATCH_BODY(k_boundary, b_boundary, TB)
+BATCH_BODY(k_substep1, b_substep1, TB)
+BATCH_BODY(k_divstress, b_divstress, TB)
+BATCH_BODY(k_visc_update, b_visc_update, TB)
+BATCH_BODY(k_substep2, b_substep2, TB)
+BATCH_BODY(k_substep3, b_substep3, TB)
+BATCH_BODY(k_matrix, b_matrix, TB)
+BATCH_BODY(k_sor_resid0, b_sor_resid0, TB)
+BATCH_BODY(k_sor_iter, b_sor_iter, TB)
+BATCH_BODY(k_sor_check, b_sor_check, RB)
+BATCH_BODY(k_energy_new, b_energy_new, TB)
+BATCH_BODY(k_transport_rad, b_transport_rad, TB)
+BATCH_BODY(k_ring_means, b_ring_means, 256)
+BATCH_BODY(k_transport_theta<1>, b_transport_theta<1>, TT_T)
+BATCH_BODY(k_transport_theta<2>, b_transport_theta<2>, TT_T)
+BATCH_BODY(k_transport_theta<3>, b_transport_theta<3>, TT_T)
+BATCH_BODY(k_transport_theta<4>, b_transport_theta<4>, TT_T)
+BATCH_BODY(k_transport_theta<5>, b_transport_theta<5>, TT_T)
+BATCH_BODY(k_transport_theta<6>, b_transport_theta<6>, TT_T)
+BATCH_BODY(k_velocities, b_velocities, TB)
+#undef BATCH_BODY
+
+static inline void set_blk0(Geo &g, int o) { g.blk0 = o; }
+static inline void set_blk0(BOff &b, int o) { b.blk0 = o; }
+template <typename T>
+static inline void set_blk0(T &, int) {}
+
+// staging of the batched launches' argument tables: pinned host memory and
+// its device copy, one host-to-device copy per flush
+struct Stage {
+  char *h = nullptr;
+  char *dv = nullptr;
+  size_t cap = 0, used = 0;
+  size_t take(size_t n) {
+    const size_t o = used;
+    used += (n + 255) & ~(size_t)255;
+    return o;
+  }
+};
+
+struct RecOp;
+typedef std::function<int(hipStream_t, const char *)> IssueFn;
+// plan: appends the tables of a group of operations of one kernel to the
+// host image of the stage and returns what issues them (given the device
+// address of the stage)
+typedef IssueFn (*PlanFn)(RecOp **ops, int n, std::vector<char> &tables);
+
+struct RecOp {
+  int type;                     // 0 kernel, 1 copy
+  const void *key;              // kernel identity: the batching group
+  PlanFn plan;
+  std::vector<char> args;       // the kernel's argument tuple
+  dim3 grid, block;
+  size_t shmem;
+  void *dst;
+  const void *src;
+  size_t bytes;
+  hipMemcpyKind kind;
+};
+
+struct thorin_ensemble;
+struct Rec {                    // one disk's queue
+  std::vector<RecOp> q;
+  thorin_ensemble *e;
+  int idx;
+};
+
+struct thorin_ensemble {
+  int n;
+  thorin_ctx **c;
+  hipStream_t s;
+  std::mutex m;
+  std::condition_variable cv_coord, cv_disk;
+  int nwait, nfinished;
+  unsigned long gen;
+  int err;
+  std::vector<Rec> recs;
+  Stage st;
+  long long launches, batched_ops;   // statistics: launches issued, kernel operations they carried
+};
+
+// the table of a group: n argument tuples, then n+1 block offsets; the host
+// image is built in `tables`, the device address is the stage's base + offset
+template <auto K>
+static IssueFn plan_batch(RecOp **ops, int n, std::vector<char> &tables) {
+  using Tup = decltype(params_of(K));
+  const size_t o = tables.size();
+  const size_t tb = (sizeof(Tup) * (size_t)n + 15) & ~(size_t)15;
+  const size_t need = (tb + sizeof(int) * (size_t)(n + 1) + 255) & ~(size_t)255;
+  tables.resize(o + need);
+  Tup *t = reinterpret_cast<Tup *>(tables.data() + o);
+  int *off = reinterpret_cast<int *>(tables.data() + o + tb);
+  int b = 0;
+  size_t shm = 0;
+  for (int k = 0; k < n; k++) {
+    new (&t[k]) Tup(*reinterpret_cast<const Tup *>(ops[k]->args.data()));
+    std::apply([&](auto &...a) { (set_blk0(a, b), ...); }, t[k]);
+    off[k] = b;
+    b += (int)ops[k]->grid.x;
+    shm = ops[k]->shmem > shm ? ops[k]->shmem : shm;
+  }
+  off[n] = b;
+  const dim3 block = ops[0]->block;
+  const int nb = b;
+  return [=](hipStream_t s, const char *dev) -> int {
+    constexpr auto fn = BatchKernel<Batch<K>::nt, Batch<K>::body, Tup>::fn;
+    if (shm > 65536) {
+      static bool set = false;
+      if (!set) {
+        HCHECK(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+        set = true;
+      }
+    }
+    hipLaunchKernelGGL(fn, dim3((unsigned)nb), block, shm, s, reinterpret_cast<const Tup *>(dev + o),
+                       reinterpret_cast<const int *>(dev + o + tb), n);
+    HCHECK(hipGetLastError());
+    return 0;
+  };
+}
+
+// a kernel without a batched body: each disk's launch as recorded
+template <auto K>
+static IssueFn plan_plain(RecOp **ops, int n, std::vector<char> &) {
+  using Tup = decltype(params_of(K));
+  std::vector<Tup> args;
+  std::vector<std::pair<dim3, std::pair<dim3, size_t>>> cfg;
+  for (int k = 0; k < n; k++) {
+    args.push_back(*reinterpret_cast<const Tup *>(ops[k]->args.data()));
+    cfg.push_back({ops[k]->grid, {ops[k]->block, ops[k]->shmem}});
+  }
+  return [=](hipStream_t s, const char *) -> int {
+    for (size_t k = 0; k < args.size(); k++)
+      std::apply([&](auto... a) { hipLaunchKernelGGL(K, cfg[k].first, cfg[k].second.first, cfg[k].second.second, s, a...); },
+                 args[k]);
+    HCHECK(hipGetLastError());
+    return 0;
+  };
+}
+
+static int rec_wait(Rec *r);
+
+template <auto K, typename... A>
+static int dev_launch(thorin_dev *d, const char *name, dim3 grid, dim3 block, size_t shmem, A &&...a) {
+  using Tup = decltype(params_of(K));
+  if (d->rec) {
+    RecOp op;
+    op.type = 0;
+    op.key = (const void *)K;
+    op.grid = grid;
+    op.block = block;
+    op.shmem = shmem;
+    op.args.resize(sizeof(Tup));
+    new (op.args.data()) Tup(std::forward<A>(a)...);
+    if constexpr (Batch<K>::ok) op.plan = &plan_batch<K>;
+    else op.plan = &plan_plain<K>;
+    d->rec->q.push_back(std::move(op));
+    return 0;
+  }
+  int kid = d->prof_on ? kid_of(name) : -1;
+  hipEvent_t ea = nullptr, eb = nullptr;
+  if (kid >= 0 && ((d->prof_mask >> kid) & 1ull)) {
+    ea = prof_event(d);
+    eb = prof_event(d);
+    if (ea && eb) (void)hipEventRecord(ea, d->s);
+  }
+  hipLaunchKernelGGL(K, grid, block, shmem, d->s, std::forward<A>(a)...);
+  if (ea && eb) {
+    (void)hipEventRecord(eb, d->s);
+    d->prof_pend->push_back(std::make_pair(kid, std::make_pair(ea, eb)));
+  }
+  return 0;
+}
+
+static int dev_copy(thorin_dev *d, void *dst, const void *src, size_t bytes, hipMemcpyKind kind) {
+  if (d->rec) {
+    RecOp op;
+    op.type = 1;
+    op.key = nullptr;
+    op.plan = nullptr;
+    op.dst = dst;
+    op.src = src;
+    op.bytes = bytes;
+    op.kind = kind;
+    d->rec->q.push_back(std::move(op));
+    return 0;
+  }
+  HCHECK(hipMemcpyAsync(dst, src, bytes, kind, d->s));
+  return 0;
+}
+
+static int dev_sync(thorin_dev *d) {
+  if (d->rec) return rec_wait(d->rec);
+  HCHECK(hipStreamSynchronize(d->s));
+  return 0;
+}
+
+#define LAUNCH(kern, grid, block, shmem, ...)                                       \
+  do {                                                                              \
+    if (dev_launch<&kern>(d, #kern, grid, block, shmem, __VA_ARGS__)) return -1;    \
   } while (0)
+#define DCOPY(dst, src, bytes, kind)                                 \
+  do {                                                               \
+    if (dev_copy(d, dst, src, bytes, kind)) return -1;               \
+  } while (0)
+#define DSYNC()                  \
+  do {                           \
+    if (dev_sync(d)) return -1;  \
+  } while (0)
+
+extern "C" int thorin_ring_means(thorin_ctx *c, int field, double *out) {
+  if (need_dev(c)) return -1;
+  thorin_dev *d = c->dev;
+  if (field < 0 || field >= THORIN_NFIELDS) return -1;
+  LAUNCH(k_ring_means, dim3((unsigned)((d->nr + d->rs_rows - 1) / d->rs_rows)), dim3(256),
+         sizeof(double) * (size_t)d->rs_rows * d->ns, (const double *)d->b[field], d->nr, d->ns, d->rs_rows,
+         d->ringtmp, (const int *)nullptr, BOff{0});
+  DCOPY(out, d->ringtmp, sizeof(double) * d->nr, hipMemcpyDeviceToHost);
+  DSYNC();
+  return 0;
+}
 
 extern "C" int thorin_prof_enable(thorin_ctx *c, int on, unsigned long long mask) {
   if (need_dev(c)) return -1;
@@ -2852,8 +3328,8 @@ static int sor_finish(thorin_ctx *c, double omega, int launched, int *niter_out,
     launched = batch;
   }
   for (;;) {
-    HCHECK(hipMemcpyAsync(d->hctl, d->ctl, sizeof(DevCtl), hipMemcpyDeviceToHost, d->s));
-    HCHECK(hipStreamSynchronize(d->s));
+    DCOPY(d->hctl, d->ctl, sizeof(DevCtl), hipMemcpyDeviceToHost);
+    DSYNC();
     if (prof_flush(d)) return -1;
     if (d->hctl->sor_done) {
       *niter_out = d->hctl->sor_niter;
@@ -2874,7 +3350,7 @@ static int iterate_relaxation(thorin_ctx *c) {
   double omegamin = 1.0, omega = 1.0, omegamax = OMEGAUPPER;
   int nsplit = 9, Niter, count = 0;
   const size_t nb = sizeof(double) * d->nf;
-  HCHECK(hipMemcpyAsync(d->b[B_TBACK], d->b[B_TEMPER], nb, hipMemcpyDeviceToDevice, d->s));
+  DCOPY(d->b[B_TBACK], d->b[B_TEMPER], nb, hipMemcpyDeviceToDevice);
   c->st.sor_domega = (omegamax - omegamin) / nsplit;
   c->st.sor_niter_best = SORMAXITERS;
   c->st.sor_omega_best = 1.0;
@@ -2886,7 +3362,7 @@ static int iterate_relaxation(thorin_ctx *c) {
       if (Niter == c->st.sor_niter_best) count++;
       if (Niter < c->st.sor_niter_best) { c->st.sor_niter_best = Niter; c->st.sor_omega_best = omega; }
       omega += c->st.sor_domega;
-      HCHECK(hipMemcpyAsync(d->b[B_TEMPER], d->b[B_TBACK], nb, hipMemcpyDeviceToDevice, d->s));
+      DCOPY(d->b[B_TEMPER], d->b[B_TBACK], nb, hipMemcpyDeviceToDevice);
     }
     if (c->st.sor_niter_best == SORMAXITERS) {
       c->st.sor_niter_last = saved_last;
@@ -2980,12 +3456,12 @@ static int peb_accrete_host(thorin_ctx *c, const thorin_planets *sys, PebAccK *P
     for (int q = 0; q < 2; q++) {
       const int r = pl[k].iv + q;
       if (r < nr)
-        HCHECK(hipMemcpyAsync(sv + (size_t)(2 * k + q) * ns, d->b[B_VTHETA] + (size_t)r * ns, sizeof(double) * ns,
-                              hipMemcpyDeviceToHost, d->s));
+        DCOPY(sv + (size_t)(2 * k + q) * ns, d->b[B_VTHETA] + (size_t)r * ns, sizeof(double) * ns,
+                              hipMemcpyDeviceToHost);
     }
   if (nt)
-    HCHECK(hipMemcpyAsync(st, d->b[B_TAU] + (size_t)lo * ns, sizeof(double) * nt, hipMemcpyDeviceToHost, d->s));
-  HCHECK(hipStreamSynchronize(d->s));
+    DCOPY(st, d->b[B_TAU] + (size_t)lo * ns, sizeof(double) * nt, hipMemcpyDeviceToHost);
+  DSYNC();
   for (int k = 0; k < sys->nb; k++) {
     double vt1[2];
     for (int q = 0; q < 2; q++) {
@@ -3060,7 +3536,9 @@ static int transport(thorin_dev *d, const Geo &g, const RadQ &q, int brho, int b
   LAUNCH(k_transport_theta<NPT>, dim3(nr), dim3(TT_T), sizeof(double) * 2 * ns, g, d->b[bvt], d->b[B_XRP],     \
          d->b[B_XRM], d->b[B_XTP], d->b[B_XTM], xe, d->b[B_XRHO], d->b[B_RP], d->b[B_RM], d->b[B_TP], d->b[B_TM], \
          oe, d->b[brho], dt, guard, nq)
-  switch ((ns + TT_T - 1) / TT_T) {
+  int npt = (ns + TT_T - 1) / TT_T;
+  if (npt < d->tt_npt_min) npt = d->tt_npt_min;   // an ensemble runs its disks with one instance
+  switch (npt) {
     case 1: TT_LAUNCH(1); break;
     case 2: TT_LAUNCH(2); break;
     case 3: TT_LAUNCH(3); break;
@@ -3112,8 +3590,8 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
       LAUNCH(k_cfl_final, dim3(1), rb, 0, d->g, d->ctl, (const double *)nullptr, 0, invdtreb_sq,
              (const int *)nullptr);
       HCHECK(hipGetLastError());
-      HCHECK(hipMemcpyAsync(d->hctl, d->ctl, sizeof(DevCtl), hipMemcpyDeviceToHost, d->s));
-      HCHECK(hipStreamSynchronize(d->s));
+      DCOPY(d->hctl, d->ctl, sizeof(DevCtl), hipMemcpyDeviceToHost);
+      DSYNC();
       d->cfl_ready = 1;
       d->cfl_invdtreb = invdtreb_sq;
     }
@@ -3139,8 +3617,8 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
       LAUNCH(k_peb_accrete, dim3(1), dim3(ACC_T), 0, d->g, P, d->b[B_PDENS], d->b[B_PVR], d->b[B_PVT], d->b[B_TAU],
              d->b[B_CS], d->b[B_ABS], d->b[B_ORD], d->b[B_MACC], dt, d->ctl);
       HCHECK(hipGetLastError());
-      HCHECK(hipMemcpyAsync(d->hctl, d->ctl, sizeof(DevCtl), hipMemcpyDeviceToHost, d->s));
-      HCHECK(hipStreamSynchronize(d->s));
+      DCOPY(d->hctl, d->ctl, sizeof(DevCtl), hipMemcpyDeviceToHost);
+      DSYNC();
       if (d->hctl->acc_fail) {
         thorin_set_error("pebble accretion: integration of the column mass did not converge");
         return -1;
@@ -3203,7 +3681,7 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
       indir_px = -axs;
       indir_py = -ays;
       if (sys->nb > 0)
-        HCHECK(hipMemcpyAsync(d->pk, d->hpk, sizeof(PlanetK) * sys->nb, hipMemcpyHostToDevice, d->s));
+        DCOPY(d->pk, d->hpk, sizeof(PlanetK) * sys->nb, hipMemcpyHostToDevice);
     }
     const int want_tq = p->write_torque && log && log->nsub == 0 && sys->nb > 0;
     // UpdateLog (Force.c:27-70, called from AdvanceSystemFromDisk) at the first
@@ -3247,7 +3725,7 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
     else
       LAUNCH(k_forces<false>, dim3(gall.x + (ntq ? 1u : 0u)), tb, 0, d->g, d->b[B_DENS], d->b[B_CS], d->b[B_ABS],
              d->b[B_ORD], d->b[B_AGR], d->b[B_AGT], d->pk, sys->nb, mt, d->red, rr, 2 * ntq, d->ctl, pf);
-    LAUNCH(k_forces_reduce, dim3(2 + 2 * sys->nb), rb, 0, d->red, d->nblk_cells, d->ctl);
+    LAUNCH(k_forces_reduce, dim3(2 + 2 * sys->nb), rb, 0, d->red, d->nblk_cells, d->ctl, BOff{0});
     LAUNCH(k_indirect, gall, tb, 0, d->g, d->b[B_AGR], d->b[B_AGT], d->b[B_ABS], d->b[B_ORD], d->ctl, indir_px,
            indir_py, peb ? d->b[B_PAGR] : (double *)nullptr, peb ? d->b[B_PAGT] : (double *)nullptr);
     HCHECK(hipGetLastError());
@@ -3255,15 +3733,15 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
     int anyfeel = 0;
     for (int k = 0; k < sys->nb; k++) anyfeel |= sys->feeldisk[k];
     if (anyfeel) {
-      HCHECK(hipMemcpyAsync(d->hctl, d->ctl, sizeof(DevCtl), hipMemcpyDeviceToHost, d->s));
-      HCHECK(hipStreamSynchronize(d->s));
+      DCOPY(d->hctl, d->ctl, sizeof(DevCtl), hipMemcpyDeviceToHost);
+      DSYNC();
       for (int k = 0; k < sys->nb; k++) { sys->ax[k] = d->hctl->axpl[k]; sys->ay[k] = d->hctl->aypl[k]; }
     }
     if (anyfeel) {
       std::vector<double> rmean(nr), csmean(nr);
       if (thorin_ring_means(c, THORIN_F_DENS, rmean.data())) return -1;
-      HCHECK(hipMemcpyAsync(csmean.data(), d->g.gcs, sizeof(double) * nr, hipMemcpyDeviceToHost, d->s));
-      HCHECK(hipStreamSynchronize(d->s));
+      DCOPY(csmean.data(), d->g.gcs, sizeof(double) * nr, hipMemcpyDeviceToHost);
+      DSYNC();
       const double indx = -d->hctl->axstar, indy = -d->hctl->aystar;
       for (int k = 0; k < sys->nb; k++) {
         if (!sys->feeldisk[k]) continue;
@@ -3382,8 +3860,8 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
     if (p->lambdadoubling != 0.0) {
       thorin_cavity_factors(c);
       const int n1 = nr + 3;
-      HCHECK(hipMemcpyAsync((double *)d->g.cav_med, c->cav_med, sizeof(double) * n1, hipMemcpyHostToDevice, d->s));
-      HCHECK(hipMemcpyAsync((double *)d->g.cav_inf, c->cav_inf, sizeof(double) * n1, hipMemcpyHostToDevice, d->s));
+      DCOPY((double *)d->g.cav_med, c->cav_med, sizeof(double) * n1, hipMemcpyHostToDevice);
+      DCOPY((double *)d->g.cav_inf, c->cav_inf, sizeof(double) * n1, hipMemcpyHostToDevice);
     }
     d->cfl_invdtreb = minstep_rebound(sys);
     const int *guard = &d->ctl->sor_done;
@@ -3417,9 +3895,9 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
         // Nshift overwrites the gas one as in the reference), damping; then
         // the next substep's Stokes numbers and pebble CFL limit
         LAUNCH(k_peb_damp, gall, tb, 0, d->g, d->b[B_PDENS], d->b[B_PVR], d->b[B_PVT], dt, 0, d->ctl, guard);
-        hipLaunchKernelGGL(k_ring_means, dim3((unsigned)((nr + d->rs_rows - 1) / d->rs_rows)), dim3(256),
-                           sizeof(double) * (size_t)d->rs_rows * ns, d->s, (const double *)d->b[B_PVT], nr, ns,
-                           d->rs_rows, d->g.pvmed, guard);
+        LAUNCH(k_ring_means, dim3((unsigned)((nr + d->rs_rows - 1) / d->rs_rows)), dim3(256),
+               sizeof(double) * (size_t)d->rs_rows * ns, (const double *)d->b[B_PVT], nr, ns, d->rs_rows,
+               d->g.pvmed, guard, BOff{0});
         Geo gp = d->g;
         gp.VMed = d->g.pvmed;
         RadQ q;
@@ -3448,8 +3926,8 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
       HCHECK(hipGetLastError());
       // one sync per substep: SOR result, crash flags, dt_stellar, the next
       // CFL minimum, the planet accelerations
-      HCHECK(hipMemcpyAsync(d->hctl, d->ctl, sizeof(DevCtl), hipMemcpyDeviceToHost, d->s));
-      HCHECK(hipStreamSynchronize(d->s));
+      DCOPY(d->hctl, d->ctl, sizeof(DevCtl), hipMemcpyDeviceToHost);
+      DSYNC();
       return 0;
     };
     if (after_sor()) return -1;
@@ -3501,6 +3979,228 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
     }
   }
   return rc;
+}
+
+// ===========================================================================
+// Ensemble of disks advanced together (see the recorder above).
+
+static int rec_wait(Rec *r) {
+  thorin_ensemble *e = r->e;
+  std::unique_lock<std::mutex> lk(e->m);
+  const unsigned long g = e->gen;
+  e->nwait++;
+  e->cv_coord.notify_one();
+  e->cv_disk.wait(lk, [&] { return e->gen != g; });
+  return e->err ? -1 : 0;
+}
+
+// Merge the recorded queues into one stream.  At every step the copies at the
+// heads run first; then, among the kernels at the heads, one that no other
+// disk still has further down its queue is launched for every disk whose head
+// it is (the largest such group); if there is none, the largest group.  With
+// the disks' sequences alike this launches each kernel once for all of them,
+// and a disk whose solve needs more SOR iterations runs those alone before
+// the rest of the substep is batched again.
+static int ens_flush(thorin_ensemble *e) {
+  const int n = e->n;
+  std::vector<size_t> pos((size_t)n, 0);
+  std::vector<std::unordered_map<const void *, int>> left((size_t)n);
+  for (int k = 0; k < n; k++)
+    for (auto &op : e->recs[k].q)
+      if (op.type == 0) left[k][op.key]++;
+  std::vector<char> tables;
+  std::vector<IssueFn> issue;
+  std::vector<RecOp *> grp;
+  long long nops = 0;
+  for (;;) {
+    for (int k = 0; k < n; k++) {
+      auto &q = e->recs[k].q;
+      while (pos[k] < q.size() && q[pos[k]].type == 1) {
+        const RecOp &op = q[pos[k]++];
+        void *dst = op.dst;
+        const void *src = op.src;
+        const size_t bytes = op.bytes;
+        const hipMemcpyKind kind = op.kind;
+        issue.push_back([=](hipStream_t s, const char *) -> int {
+          HCHECK(hipMemcpyAsync(dst, src, bytes, kind, s));
+          return 0;
+        });
+      }
+    }
+    const void *best = nullptr;
+    int best_n = 0, best_ready = 0;
+    for (int k = 0; k < n; k++) {
+      auto &q = e->recs[k].q;
+      if (pos[k] >= q.size()) continue;
+      const void *key = q[pos[k]].key;
+      int cnt = 0, ready = 1;
+      for (int m = 0; m < n; m++) {
+        auto &qm = e->recs[m].q;
+        if (pos[m] < qm.size() && qm[pos[m]].key == key) cnt++;
+        else {
+          auto it = left[m].find(key);
+          if (it != left[m].end() && it->second > 0) ready = 0;
+        }
+      }
+      if ((ready > best_ready) || (ready == best_ready && cnt > best_n)) {
+        best = key;
+        best_n = cnt;
+        best_ready = ready;
+      }
+    }
+    if (!best) break;
+    grp.clear();
+    for (int k = 0; k < n; k++) {
+      auto &q = e->recs[k].q;
+      if (pos[k] < q.size() && q[pos[k]].key == best) {
+        grp.push_back(&q[pos[k]++]);
+        left[k][best]--;
+      }
+    }
+    issue.push_back(grp[0]->plan(grp.data(), (int)grp.size(), tables));
+    e->launches++;
+    nops += (long long)grp.size();
+  }
+  e->batched_ops += nops;
+  if (tables.size() > e->st.cap) {
+    if (e->st.h) HCHECK(hipHostFree(e->st.h));
+    if (e->st.dv) HCHECK(hipFree(e->st.dv));
+    e->st.h = nullptr;
+    e->st.dv = nullptr;
+    size_t cap = 1 << 20;
+    while (cap < tables.size()) cap *= 2;
+    HCHECK(hipHostMalloc((void **)&e->st.h, cap, hipHostMallocDefault));
+    HCHECK(hipMalloc((void **)&e->st.dv, cap));
+    e->st.cap = cap;
+  }
+  if (!tables.empty()) {
+    memcpy(e->st.h, tables.data(), tables.size());
+    HCHECK(hipMemcpyAsync(e->st.dv, e->st.h, tables.size(), hipMemcpyHostToDevice, e->s));
+  }
+  for (auto &f : issue)
+    if (f(e->s, e->st.dv)) return -1;
+  HCHECK(hipStreamSynchronize(e->s));
+  for (int k = 0; k < n; k++) e->recs[k].q.clear();
+  return 0;
+}
+
+extern "C" int thorin_ensemble_create(thorin_ctx *const *disks, int n, thorin_ensemble **out) {
+  if (!out) { thorin_set_error("thorin_ensemble_create: null output"); return -1; }
+  *out = nullptr;
+  if (!disks || n < 1 || n > THORIN_MAXDISKS) {
+    thorin_set_error("thorin_ensemble_create: between 1 and THORIN_MAXDISKS contexts are needed");
+    return -1;
+  }
+  for (int k = 0; k < n; k++) {
+    if (need_dev(disks[k])) return -1;
+    for (int m = 0; m < k; m++)
+      if (disks[m] == disks[k]) { thorin_set_error("thorin_ensemble_create: a context appears twice"); return -1; }
+    if (disks[k]->dev->rec) { thorin_set_error("thorin_ensemble_create: a context belongs to a running ensemble"); return -1; }
+  }
+  thorin_ensemble *e = new (std::nothrow) thorin_ensemble();
+  if (!e) { thorin_set_error("out of host memory"); return -1; }
+  e->n = n;
+  e->c = new thorin_ctx *[n];
+  for (int k = 0; k < n; k++) e->c[k] = disks[k];
+  e->recs.resize((size_t)n);
+  for (int k = 0; k < n; k++) { e->recs[k].e = e; e->recs[k].idx = k; }
+  // one instance of the azimuthal transport for all disks
+  int npt = 1;
+  for (int k = 0; k < n; k++) {
+    const int v = (disks[k]->dev->ns + TT_T - 1) / TT_T;
+    npt = v > npt ? v : npt;
+  }
+  for (int k = 0; k < n; k++) disks[k]->dev->tt_npt_min = npt;
+  if (hipStreamCreateWithFlags(&e->s, hipStreamNonBlocking) != hipSuccess) {
+    thorin_set_error("thorin_ensemble_create: stream creation failed");
+    delete[] e->c;
+    delete e;
+    return -1;
+  }
+  // work queued on the disks' own streams comes first
+  for (int k = 0; k < n; k++)
+    if (hipStreamSynchronize(disks[k]->dev->s) != hipSuccess) {
+      thorin_set_error("thorin_ensemble_create: a disk's stream failed");
+      (void)hipStreamDestroy(e->s);
+      delete[] e->c;
+      delete e;
+      return -1;
+    }
+  *out = e;
+  return 0;
+}
+
+extern "C" void thorin_ensemble_destroy(thorin_ensemble *e) {
+  if (!e) return;
+  (void)hipStreamSynchronize(e->s);
+  for (int k = 0; k < e->n; k++)
+    if (e->c[k] && e->c[k]->dev) e->c[k]->dev->tt_npt_min = 0;
+  if (e->st.h) (void)hipHostFree(e->st.h);
+  if (e->st.dv) (void)hipFree(e->st.dv);
+  (void)hipStreamDestroy(e->s);
+  delete[] e->c;
+  delete e;
+}
+
+extern "C" int thorin_ensemble_algogas(thorin_ensemble *e, thorin_planets *const *sys, thorin_nbody_fn nbody,
+                                       void *const *nbody_user, thorin_step_log *logs, int *rc_out) {
+  if (!e || !sys) { thorin_set_error("thorin_ensemble_algogas: null argument"); return -1; }
+  const int n = e->n;
+  for (int k = 0; k < n; k++)
+    if (!sys[k]) { thorin_set_error("thorin_ensemble_algogas: null planetary system"); return -1; }
+  e->nwait = 0;
+  e->nfinished = 0;
+  e->err = 0;
+  for (int k = 0; k < n; k++) {
+    e->recs[k].q.clear();
+    e->c[k]->dev->rec = &e->recs[k];
+  }
+  std::vector<int> rc((size_t)n, 0);
+  std::vector<std::thread> th;
+  th.reserve((size_t)n);
+  for (int k = 0; k < n; k++)
+    th.emplace_back([&, k] {
+      rc[k] = thorin_algogas(e->c[k], sys[k], nbody, nbody_user ? nbody_user[k] : nullptr, logs ? &logs[k] : nullptr);
+      std::lock_guard<std::mutex> lk(e->m);
+      e->nfinished++;
+      e->cv_coord.notify_one();
+    });
+  int status = 0;
+  for (;;) {
+    std::unique_lock<std::mutex> lk(e->m);
+    e->cv_coord.wait(lk, [&] { return e->nwait + e->nfinished == n; });
+    bool pending = false;
+    for (int k = 0; k < n; k++) pending |= !e->recs[k].q.empty();
+    if (e->nfinished == n && !pending) break;
+    // every disk thread waits or has finished: the queues are the coordinator's
+    lk.unlock();
+    const int f = ens_flush(e);
+    lk.lock();
+    if (f) {
+      e->err = 1;
+      status = -1;
+      for (int k = 0; k < n; k++) e->recs[k].q.clear();
+    }
+    e->nwait = 0;
+    e->gen++;
+    e->cv_disk.notify_all();
+  }
+  for (auto &t : th) t.join();
+  for (int k = 0; k < n; k++) e->c[k]->dev->rec = nullptr;
+  int worst = 0;
+  for (int k = 0; k < n; k++) {
+    if (rc_out) rc_out[k] = rc[k];
+    if (rc[k] < 0) status = -1;
+    else if (rc[k] > worst) worst = rc[k];
+  }
+  return status ? -1 : worst;
+}
+
+extern "C" int thorin_ensemble_stats(const thorin_ensemble *e, long long *launches, long long *kernel_ops) {
+  if (!e) return -1;
+  if (launches) *launches = e->launches;
+  if (kernel_ops) *kernel_ops = e->batched_ops;
+  return 0;
 }
 
 // ===========================================================================

@@ -232,6 +232,27 @@ int  thorin_download_field(thorin_ctx *ctx, int field, double *host, size_t coun
 int  thorin_algogas(thorin_ctx *ctx, thorin_planets *sys, thorin_nbody_fn nbody,
                     void *nbody_user, thorin_step_log *log);
 
+/* ---- an ensemble of disks advanced together.  No counterpart entry point in
+ * the reference, which runs one disk per process; each disk follows
+ * exactly the AlgoGas of thorin_algogas (SourceEuler.c:210-317).
+ * The disks' host logic runs on one thread each; the GPU work of all disks
+ * is merged so that each kernel of a substep is ONE launch over every disk's
+ * grid (the disks may differ in size, parameters and substep count).  Each
+ * disk's results are bit-identical to thorin_algogas on that disk alone.
+ * The contexts stay usable on their own between ensemble calls.
+ * thorin_ensemble_algogas advances every disk by one DT: sys, nbody_user,
+ * logs and rc_out hold one entry per disk (nbody_user, logs and rc_out may
+ * be NULL); nbody is called from the disk's thread.  Returns -1 if any
+ * disk failed, else the largest per-disk return code. */
+#define THORIN_MAXDISKS 64
+typedef struct thorin_ensemble thorin_ensemble;
+int  thorin_ensemble_create(thorin_ctx *const *disks, int n, thorin_ensemble **out);
+int  thorin_ensemble_algogas(thorin_ensemble *e, thorin_planets *const *sys, thorin_nbody_fn nbody,
+                             void *const *nbody_user, thorin_step_log *logs, int *rc_out);
+/* launches issued so far and the per-disk kernel operations they carried */
+int  thorin_ensemble_stats(const thorin_ensemble *e, long long *launches, long long *kernel_ops);
+void thorin_ensemble_destroy(thorin_ensemble *e);
+
 /* Disk quantities the host needs once per DT for the torque log
  * (UpdateLog, Force.c:27-70): ring means of dens and soundspeed
  * (nrad doubles each) and the last disk->planet accelerations in sys. */

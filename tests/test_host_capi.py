@@ -141,3 +141,36 @@ def test_host_pebble_restart_targets_equal_the_oracle():
         assert np.array_equal(host_field(h, _capi.FIELD_ID["pebbledens"], 32, 48), pb["pdens"])
     finally:
         lib().thorin_ctx_destroy(h)
+
+
+def test_ensemble_refuses_contexts_without_a_device():
+    """thorin_ensemble_create checks its contexts before touching the GPU."""
+    P = params_for(configs.with_grid(configs.PLAIN_128x256, 16, 32))
+    h = host_ctx(P)
+    try:
+        e = C.c_void_p()
+        arr = (C.c_void_p * 1)(h)
+        assert lib().thorin_ensemble_create(arr, 1, C.byref(e)) == -1
+        assert "no device" in last_error()
+        assert lib().thorin_ensemble_create(arr, 0, C.byref(e)) == -1
+        assert lib().thorin_ensemble_create(None, 1, C.byref(e)) == -1
+    finally:
+        lib().thorin_ctx_destroy(h)
+
+
+def test_ensemble_members_follow_the_workload_draws():
+    """16 members: Nrad from the five sizes, Nsec = 1.5 Nrad, alpha in
+    [1e-4, 1e-2] log-uniform, Sigma0 in 5e-4 x [0.3, 3], h in [0.03, 0.07];
+    seeds 1000+d; every member is a valid parameter set."""
+    cfgs = configs.ensemble_configs()
+    assert len(cfgs) == 16 and [c["_seed"] for c in cfgs] == list(range(1000, 1016))
+    for c in cfgs:
+        nr, ns = int(c["Nrad"]), int(c["Nsec"])
+        assert nr in configs.ENSEMBLE_SIZES and ns * 2 == 3 * nr
+        assert 1e-4 <= float(c["AlphaViscosity"]) <= 1e-2
+        assert 0.3 * 5e-4 <= float(c["Sigma0"]) <= 3.0 * 5e-4
+        assert 0.03 <= float(c["AspectRatio"]) <= 0.07
+        P = params_for({k: v for k, v in c.items() if k != "_seed"})
+        assert P.num("NRAD") == nr
+    # the draws differ between members
+    assert len({c["AlphaViscosity"] for c in cfgs}) == 16
