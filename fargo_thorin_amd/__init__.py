@@ -98,6 +98,11 @@ def make_planets(masses, radii_au, feel_disk: bool = False) -> Planets:
     return s
 
 
+def _tuning_dict(t) -> dict:
+    return {"sor_iters_per_launch": t.sor_iters_per_launch, "rad_rows": t.rad_rows,
+            "sor_us_per_iter": list(t.sor_us_per_iter), "rad_us": list(t.rad_us)}
+
+
 class Disk:
     """All PolarGrid fields of one disk, resident on the GPU."""
 
@@ -148,6 +153,22 @@ class Disk:
         _check(rc, "thorin_algogas")
         self.last_rc = rc
         return self.log
+
+    # --- launch shapes (results do not depend on them)
+    def autotune(self) -> dict:
+        """Time the launch shapes on this grid and keep the fastest."""
+        t = _capi.Tuning()
+        _check(lib().thorin_autotune(self._h, C.byref(t)), "thorin_autotune")
+        return _tuning_dict(t)
+
+    def set_tuning(self, sor_iters_per_launch: int = 1, rad_rows: int = 16) -> None:
+        t = _capi.Tuning(sor_iters_per_launch=sor_iters_per_launch, rad_rows=rad_rows)
+        _check(lib().thorin_set_tuning(self._h, C.byref(t)), "thorin_set_tuning")
+
+    def tuning(self) -> dict:
+        t = _capi.Tuning()
+        _check(lib().thorin_get_tuning(self._h, C.byref(t)), "thorin_get_tuning")
+        return _tuning_dict(t)
 
     # --- fields
     def field(self, name: str, ghost: bool = False) -> np.ndarray:
@@ -296,10 +317,21 @@ class Ensemble:
         return list(self.logs)
 
     def stats(self):
-        """(launches issued, per-disk kernel operations they carried)"""
-        a, b = C.c_longlong(), C.c_longlong()
-        _check(lib().thorin_ensemble_stats(self._h, C.byref(a), C.byref(b)), "thorin_ensemble_stats")
-        return a.value, b.value
+        """(launches issued, per-disk kernel operations they carried, graphs
+        captured, graph replays)"""
+        v = [C.c_longlong() for _ in range(4)]
+        _check(lib().thorin_ensemble_stats(self._h, *[C.byref(x) for x in v]), "thorin_ensemble_stats")
+        return tuple(x.value for x in v)
+
+    def autotune(self) -> dict:
+        """One launch shape for all disks: the least time summed over them."""
+        t = _capi.Tuning()
+        _check(lib().thorin_ensemble_autotune(self._h, C.byref(t)), "thorin_ensemble_autotune")
+        return _tuning_dict(t)
+
+    def set_graphs(self, on: bool) -> None:
+        """Replay each repeating merged substep as a HIP graph (default on)."""
+        _check(lib().thorin_ensemble_set_graphs(self._h, int(bool(on))), "thorin_ensemble_set_graphs")
 
     def close(self) -> None:
         if getattr(self, "_h", None):

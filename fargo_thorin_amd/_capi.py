@@ -63,6 +63,11 @@ class SolveResult(C.Structure):
                 ("warm_used", C.c_int), ("resid_warm", C.c_double)]
 
 
+class Tuning(C.Structure):
+    _fields_ = [("sor_iters_per_launch", C.c_int), ("rad_rows", C.c_int),
+                ("sor_us_per_iter", C.c_double * 2), ("rad_us", C.c_double * 3)]
+
+
 NBODY_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(Planets), C.c_double, C.c_double, C.c_double)
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -103,7 +108,13 @@ def lib() -> C.CDLL:
         "thorin_ensemble_create": (ip, [C.POINTER(vp), ip, C.POINTER(vp)]),
         "thorin_ensemble_algogas": (ip, [vp, C.POINTER(C.POINTER(Planets)), NBODY_FN, C.POINTER(vp),
                                          C.POINTER(StepLog), C.POINTER(ip)]),
-        "thorin_ensemble_stats": (ip, [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
+        "thorin_ensemble_stats": (ip, [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong),
+                                       C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
+        "thorin_ensemble_set_graphs": (ip, [vp, ip]),
+        "thorin_autotune": (ip, [vp, C.POINTER(Tuning)]),
+        "thorin_set_tuning": (ip, [vp, C.POINTER(Tuning)]),
+        "thorin_get_tuning": (ip, [vp, C.POINTER(Tuning)]),
+        "thorin_ensemble_autotune": (ip, [vp, C.POINTER(Tuning)]),
         "thorin_ensemble_destroy": (None, [vp]),
         "thorin_ring_profile": (ip, [vp, ip, dp]),
         "thorin_ctx_create_host": (ip, [vp, C.POINTER(vp)]),

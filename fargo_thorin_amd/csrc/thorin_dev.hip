@@ -16,6 +16,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <condition_variable>
 #include <functional>
 #include <mutex>
@@ -64,7 +65,7 @@ enum {
   B_AGR = THORIN_F_AGR, B_AGT = THORIN_F_AGT, B_EINT = THORIN_F_ENERGYINT,
   B_ENEW = THORIN_F_ENERGYNEW, B_VRADINT = THORIN_F_VRADINT, B_VTHETAINT = THORIN_F_VTHETAINT,
   B_VRADNEW = THORIN_F_VRADNEW, B_VTHETANEW = THORIN_F_VTHETANEW,
-  B_DIFF = THORIN_NFIELDS, B_ML, B_MLIP, B_MLIM, B_MLJP, B_MLJM, B_RHS, B_TBACK, B_TEMPER2,
+  B_DIFF = THORIN_NFIELDS, B_ML, B_MLIP, B_MLIM, B_MLJP, B_MLJM, B_RHS, B_TBACK, B_TEMPER2, B_TEMPER3,
   B_RP, B_RM, B_TP, B_TM, B_XRP, B_XRM, B_XTP, B_XTM, B_XE, B_XRHO, B_ABS, B_ORD,
   // pebble fluid work fields (Pebbles.c:37-56, SourceEuler.c:155-158)
   B_PAGR, B_PAGT, B_PACCR, B_PACCT, B_GACCR, B_GACCT, B_DRAGR, B_DRAGT, B_MACC,
@@ -162,6 +163,9 @@ struct thorin_dev {
   int cfl_ready;                // hctl->newdt_raw holds the CFL minimum of the current fields
   int niter_hist[4], nsolve;    // SOR iteration counts of the last four solves
   int rs_rows;                  // rings per workgroup of k_ring_means (LDS holds them whole)
+  int sor_nit;                  // SOR iterations per launch (1 or 2; thorin_tune)
+  int sor_queue_min;            // ensemble: SOR iterations to queue at least
+  int rad_rows;                 // rings per tile of the radial transport (8, 16, 32; thorin_tune)
   int rad_ready;                // thorin_rad_system has assembled the energy system
   struct Mat32 m32;             // FP32 copies for the mixed-precision solve (allocated on first use)
   double cfl_invdtreb;          // MinStepForRebound term it was formed with
@@ -1890,26 +1894,37 @@ __global__ void __launch_bounds__(TB) k_sor_resid0(Geo g, const double *__restri
 
 
 
-// One SOR iteration (both half-sweeps) per launch.  A workgroup owns a tile
-// of SOR_TI rings x SOR_TJ sectors and stages T with a two-cell halo in LDS.
-// The red half-sweep is also applied to the halo cells next to the tile,
-// with the same inputs and arithmetic as their owner, so the black
-// half-sweep of the owned cells sees exactly the values a sequential sweep
-// gives.  T is double-buffered (all workgroups read the previous iterate),
-// and the buffer holding the latest iterate is ctl->sor_cur.
+// NIT red-black SOR iterations (EnergySources.c:455-493) per launch.  A
+// workgroup owns a tile of SOR_TI rings x SOR_TJ sectors and stages T with a
+// halo of 2*NIT cells in LDS.  Half-sweep s (red for even s, black for odd s)
+// is applied to the cells of its colour within 2*NIT-1-s cells of the tile:
+// a halo cell is updated with the same inputs and arithmetic as its owner
+// (the cells it reads were updated one half-sweep earlier one cell further
+// out), so after the last half-sweep the owned cells hold exactly the
+// iterates of a sequential sweep.  Iterate n lives in X[n % 3] (X = T0, T1,
+// T2; the initial T in T0): a launch from iteration it reads X[it % 3] and
+// writes X[(it+k+1) % 3] after its k-th iteration, so that a solve that stops
+// inside the launch has its iterate (ctl->sor_cur names the buffer).  The
+// stop-test partials of each iteration are summed over the owned cells in a
+// fixed order (through LDS), the same for every NIT, so the iteration count
+// does not depend on how many iterations a launch runs.
 #define SOR_TI 8
 #define SOR_TJ 128
 #define SOR_LW (SOR_TJ + 4)
+// cells per thread of the one-sweep tiles (FP32 sweeps): red with a one-cell halo, black owned
+#define SOR_NRED (((SOR_TI + 2) * ((SOR_TJ + 3) / 2) + TB - 1) / TB)
+#define SOR_NBLK ((SOR_TI * ((SOR_TJ + 1) / 2) + TB - 1) / TB)
 
 // one cell of a half-sweep: its matrix row is loaded before the tile's T is
 // staged, so that all global loads of the workgroup are in flight together
 struct SorCell {
   double ml, mip, mim, mjp, mjm, rhs;
   int k;                        // LDS index of the cell, -1: no cell
-  bool own;
+  int d;                        // distance from the owned tile (0: owned)
+  int o;                        // index among the owned cells (owned only)
 };
 
-__device__ __forceinline__ void sor_load(SorCell &x, size_t l, int k, bool own, const double *__restrict__ Ml,
+__device__ __forceinline__ void sor_load(SorCell &x, size_t l, const double *__restrict__ Ml,
                                          const double *__restrict__ Mlip, const double *__restrict__ Mlim,
                                          const double *__restrict__ Mljp, const double *__restrict__ Mljm,
                                          const double *__restrict__ RHS) {
@@ -1919,37 +1934,58 @@ __device__ __forceinline__ void sor_load(SorCell &x, size_t l, int k, bool own, 
   x.mjp = Mljp[l];
   x.mjm = Mljm[l];
   x.rhs = RHS[l];
-  x.k = k;
-  x.own = own;
 }
 
-__device__ __forceinline__ void sor_update(const SorCell &x, double *t, double omega, double &acc, int &changed) {
+template <int LW>
+__device__ __forceinline__ void sor_update(const SorCell &x, double *t, double omega, double *sacc, int *schg) {
   const int k = x.k;
-  const double resid = t[k] * x.ml + t[k + SOR_LW] * x.mip + t[k - SOR_LW] * x.mim + t[k + 1] * x.mjp +
+  const double resid = t[k] * x.ml + t[k + LW] * x.mip + t[k - LW] * x.mim + t[k + 1] * x.mjp +
                        t[k - 1] * x.mjm - x.rhs;
   const double dtemp = omega * resid / x.ml;
   const double tn = t[k] - dtemp;
   t[k] = tn;
-  if (x.own) {
-    acc += fabs(resid);
-    if (dtemp > tn * SORPRECISION) changed++;
+  if (x.d == 0) {
+    sacc[x.o] = fabs(resid);
+    schg[x.o] = (dtemp > tn * SORPRECISION) ? 1 : 0;
   }
 }
 
-// red cells of the tile and its one-cell halo: per staged row r = 1..ti+2 the
-// columns c = cf, cf+2, ... <= tw+2 with (i+j) odd; black cells: owned rows
-// and columns with (i+j) even
-#define SOR_NRED (((SOR_TI + 2) * ((SOR_TJ + 3) / 2) + TB - 1) / TB)
-#define SOR_NBLK ((SOR_TI * ((SOR_TJ + 1) / 2) + TB - 1) / TB)
+template <int NIT>
+struct SorShape {
+  static constexpr int H = 2 * NIT;                        // staged halo
+  static constexpr int LW = SOR_TJ + 2 * H;                // LDS row
+  static constexpr int LR = SOR_TI + 2 * H;                // LDS rows
+  static constexpr int HR = 2 * NIT - 1, HB = 2 * NIT - 2; // halo of the red / black cells
+  static constexpr int NRED = ((SOR_TI + 2 * HR) * ((SOR_TJ + 2 * HR + 1) / 2) + TB - 1) / TB;
+  static constexpr int NBLK = ((SOR_TI + 2 * HB) * ((SOR_TJ + 2 * HB + 1) / 2) + TB - 1) / TB;
+};
 
+// slot sl of the cells of one colour (odd: (i+j) odd, red) within halo h of
+// the tile: row rho in [-h, ti+h), columns kappa in [-h, tw+h) of that colour
+__device__ __forceinline__ bool sor_cell(int sl, int h, bool odd, int i0, int j0, int ti, int tw, int &rho,
+                                         int &kappa) {
+  const int rw = (tw + 2 * h + 1) / 2;
+  if (sl >= (ti + 2 * h) * rw) return false;
+  const int r = sl / rw, m = sl - r * rw;
+  rho = r - h;
+  const int i = i0 + rho;
+  const int par = (i + j0 - h) & 1;           // parity of the first column
+  kappa = -h + ((par == (odd ? 1 : 0)) ? 0 : 1) + 2 * m;
+  return kappa < tw + h;
+}
+
+template <int NIT>
 __device__ __forceinline__ void b_sor_iter(Geo g, double *__restrict__ T0, double *__restrict__ T1,
-                                                 const double *__restrict__ Ml, const double *__restrict__ Mlip,
-                                                 const double *__restrict__ Mlim, const double *__restrict__ Mljp,
-                                                 const double *__restrict__ Mljm, const double *__restrict__ RHS,
-                                                 double omega, double *__restrict__ psum, int *__restrict__ pcnt,
-                                                 DevCtl *__restrict__ ctl, int it,
-                                                 const double *__restrict__ mresp, int nbm) {
-  __shared__ double t[(SOR_TI + 4) * SOR_LW];
+                                           double *__restrict__ T2, const double *__restrict__ Ml,
+                                           const double *__restrict__ Mlip, const double *__restrict__ Mlim,
+                                           const double *__restrict__ Mljp, const double *__restrict__ Mljm,
+                                           const double *__restrict__ RHS, double omega, double *__restrict__ psum,
+                                           int *__restrict__ pcnt, DevCtl *__restrict__ ctl, int it,
+                                           const double *__restrict__ mresp, int nbm, int nbs) {
+  typedef SorShape<NIT> S;
+  __shared__ double t[S::LR * S::LW];
+  __shared__ double sacc[SOR_TI * SOR_TJ];
+  __shared__ int schg[SOR_TI * SOR_TJ];
   __shared__ double red[TB];
   const int ns = g.ns, nr = g.nr;
   const int ntx = (ns + SOR_TJ - 1) / SOR_TJ;
@@ -1961,10 +1997,9 @@ __device__ __forceinline__ void b_sor_iter(Geo g, double *__restrict__ T0, doubl
   const int tw = min(SOR_TJ, ns - j0);               // owned sectors [j0, j0+tw)
   const int ti = i1 - i0;
   const int tid = threadIdx.x;
-  int cur = 0;
   if (it == 0) {
-    // first iteration of a solve: reset the control words, and give T1 the
-    // rings no sweep writes (0, NRAD-1 and the ghost ring)
+    // first iteration of a solve: reset the control words, and give T1 and
+    // T2 the rings no sweep writes (0, NRAD-1 and the ghost ring)
     if (bid == 0) {
       // initial residual normres0 (EnergySources.c:435-451) from the matrix
       // kernel's block partials
@@ -1979,148 +2014,165 @@ __device__ __forceinline__ void b_sor_iter(Geo g, double *__restrict__ T0, doubl
       }
     }
     if (ty == 0)
-      for (int j = j0 + tid; j < j0 + tw; j += TB) T1[j] = T0[j];
-    if (i1 == nr - 1)
       for (int j = j0 + tid; j < j0 + tw; j += TB) {
-        T1[(size_t)(nr - 1) * ns + j] = T0[(size_t)(nr - 1) * ns + j];
-        T1[(size_t)nr * ns + j] = T0[(size_t)nr * ns + j];
+        T1[j] = T0[j];
+        T2[j] = T0[j];
       }
-  } else {
-    if (ctl->sor_done) return;
-    cur = it & 1;
+    if (i1 == nr - 1)
+      for (int j = j0 + tid; j < j0 + tw; j += TB)
+        for (int r = nr - 1; r <= nr; r++) {
+          T1[(size_t)r * ns + j] = T0[(size_t)r * ns + j];
+          T2[(size_t)r * ns + j] = T0[(size_t)r * ns + j];
+        }
+  } else if (ctl->sor_done) {
+    return;
   }
-  const double *__restrict__ Tin = cur ? T1 : T0;
-  double *__restrict__ Tout = cur ? T0 : T1;
-  SorCell rc[SOR_NRED], bc[SOR_NBLK];
-  const int rw = (tw + 3) / 2, bw = (tw + 1) / 2;
+  double *X[3] = {T0, T1, T2};
+  const double *__restrict__ Tin = X[it % 3];
+  SorCell rc[S::NRED], bc[S::NBLK];
+  auto setup = [&](SorCell *cs, int ncell, int h, bool odd) {
 #pragma unroll
-  for (int q = 0; q < SOR_NRED; q++) {
-    rc[q].k = -1;
-    const int sl = tid + q * TB;
-    if (sl < (ti + 2) * rw) {
-      const int r = 1 + sl / rw, m = sl - (r - 1) * rw;
-      const int i = i0 - 2 + r;
-      const int c = (((i + j0 - 1) & 1) ? 1 : 2) + 2 * m;
-      if (c <= tw + 2 && i >= 1 && i <= nr - 2) {
-        int j = j0 - 2 + c;
-        if (j < 0) j += ns;
-        if (j >= ns) j -= ns;
-        const bool own = (r >= 2 && r < ti + 2 && c >= 2 && c < tw + 2);
-        sor_load(rc[q], (size_t)i * ns + j, r * SOR_LW + c, own, Ml, Mlip, Mlim, Mljp, Mljm, RHS);
-      }
+    for (int q = 0; q < ncell; q++) {
+      cs[q].k = -1;
+      int rho, kappa;
+      if (!sor_cell(tid + q * TB, h, odd, i0, j0, ti, tw, rho, kappa)) continue;
+      const int i = i0 + rho;
+      if (i < 1 || i > nr - 2) continue;
+      int j = j0 + kappa;
+      while (j < 0) j += ns;
+      while (j >= ns) j -= ns;
+      const int dr = rho < 0 ? -rho : (rho >= ti ? rho - ti + 1 : 0);
+      const int dc = kappa < 0 ? -kappa : (kappa >= tw ? kappa - tw + 1 : 0);
+      cs[q].k = (rho + S::H) * S::LW + (kappa + S::H);
+      cs[q].d = dr > dc ? dr : dc;
+      cs[q].o = cs[q].d == 0 ? rho * tw + kappa : -1;
+      sor_load(cs[q], (size_t)i * ns + j, Ml, Mlip, Mlim, Mljp, Mljm, RHS);
     }
-  }
-#pragma unroll
-  for (int q = 0; q < SOR_NBLK; q++) {
-    bc[q].k = -1;
-    const int sl = tid + q * TB;
-    if (sl < ti * bw) {
-      const int r = 2 + sl / bw, m = sl - (r - 2) * bw;
-      const int i = i0 - 2 + r;
-      const int c = (((i + j0) & 1) ? 3 : 2) + 2 * m;
-      if (c <= tw + 1)
-        sor_load(bc[q], (size_t)i * ns + (j0 - 2 + c), r * SOR_LW + c, true, Ml, Mlip, Mlim, Mljp, Mljm, RHS);
-    }
-  }
-  // stage rings i0-2 .. i1+1 and sectors j0-2 .. j0+tw+1 (periodic in phi)
-  for (int idx = tid; idx < (ti + 4) * (tw + 4); idx += TB) {
-    const int r = idx / (tw + 4), c = idx - r * (tw + 4);
-    const int i = i0 - 2 + r;
+  };
+  setup(rc, S::NRED, S::HR, true);
+  setup(bc, S::NBLK, S::HB, false);
+  // stage rings i0-H .. i1+H-1 and sectors j0-H .. j0+tw+H-1 (periodic in phi)
+  const int sw = tw + 2 * S::H;
+  for (int idx = tid; idx < (ti + 2 * S::H) * sw; idx += TB) {
+    const int r = idx / sw, c = idx - r * sw;
+    const int i = i0 - S::H + r;
     if (i < 0 || i > nr - 1) continue;
-    int j = j0 - 2 + c;
-    if (j < 0) j += ns;
-    if (j >= ns) j -= ns;
-    t[r * SOR_LW + c] = Tin[(size_t)i * ns + j];
+    int j = j0 - S::H + c;
+    while (j < 0) j += ns;
+    while (j >= ns) j -= ns;
+    t[r * S::LW + c] = Tin[(size_t)i * ns + j];
   }
   __syncthreads();
-  double acc = 0.0;
-  int changed = 0;
-  // red half-sweep ((i+j) odd): owned cells and the one-cell halo
+  const int nown = ti * tw;
 #pragma unroll
-  for (int q = 0; q < SOR_NRED; q++)
-    if (rc[q].k >= 0) sor_update(rc[q], t, omega, acc, changed);
-  __syncthreads();
-  // black half-sweep ((i+j) even): owned cells
+  for (int k = 0; k < NIT; k++) {
+    const int hr = 2 * NIT - 1 - 2 * k, hb = hr - 1;
 #pragma unroll
-  for (int q = 0; q < SOR_NBLK; q++)
-    if (bc[q].k >= 0) sor_update(bc[q], t, omega, acc, changed);
-  __syncthreads();
-  for (int idx = tid; idx < ti * tw; idx += TB) {
-    const int r = 2 + idx / tw, c = 2 + idx % tw;
-    Tout[(size_t)(i0 - 2 + r) * ns + (j0 - 2 + c)] = t[r * SOR_LW + c];
-  }
-  acc = block_sum<TB>(acc, red);
-  const double nch = block_sum<TB>((double)changed, red);
-  if (tid == 0) {
-    psum[bid] = acc;
-    pcnt[bid] = (int)nch;
+    for (int q = 0; q < S::NRED; q++)
+      if (rc[q].k >= 0 && rc[q].d <= hr) sor_update<S::LW>(rc[q], t, omega, sacc, schg);
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < S::NBLK; q++)
+      if (bc[q].k >= 0 && bc[q].d <= hb) sor_update<S::LW>(bc[q], t, omega, sacc, schg);
+    __syncthreads();
+    // iterate it+k+1 of the owned cells, and its stop-test partials
+    double *__restrict__ Tout = X[(it + k + 1) % 3];
+    for (int idx = tid; idx < nown; idx += TB) {
+      const int r = idx / tw, c = idx - r * tw;
+      Tout[(size_t)(i0 + r) * ns + (j0 + c)] = t[(r + S::H) * S::LW + (c + S::H)];
+    }
+    double acc = 0.0;
+    int chg = 0;
+    for (int idx = tid; idx < nown; idx += TB) {
+      acc += sacc[idx];
+      chg += schg[idx];
+    }
+    acc = block_sum<TB>(acc, red);
+    const double nch = block_sum<TB>((double)chg, red);
+    if (tid == 0) {
+      psum[(size_t)k * nbs + bid] = acc;
+      pcnt[(size_t)k * nbs + bid] = (int)nch;
+    }
   }
 }
+
+template <int NIT>
 __global__ void __launch_bounds__(TB) k_sor_iter(Geo g, double *__restrict__ T0, double *__restrict__ T1,
-                                                 const double *__restrict__ Ml, const double *__restrict__ Mlip,
-                                                 const double *__restrict__ Mlim, const double *__restrict__ Mljp,
-                                                 const double *__restrict__ Mljm, const double *__restrict__ RHS,
-                                                 double omega, double *__restrict__ psum, int *__restrict__ pcnt,
-                                                 DevCtl *__restrict__ ctl, int it,
-                                                 const double *__restrict__ mresp, int nbm) {
-  b_sor_iter(g, T0, T1, Ml, Mlip, Mlim, Mljp, Mljm, RHS, omega, psum, pcnt, ctl, it, mresp, nbm);
+                                                 double *__restrict__ T2, const double *__restrict__ Ml,
+                                                 const double *__restrict__ Mlip, const double *__restrict__ Mlim,
+                                                 const double *__restrict__ Mljp, const double *__restrict__ Mljm,
+                                                 const double *__restrict__ RHS, double omega, double *__restrict__ psum,
+                                                 int *__restrict__ pcnt, DevCtl *__restrict__ ctl, int it,
+                                                 const double *__restrict__ mresp, int nbm, int nbs) {
+  b_sor_iter<NIT>(g, T0, T1, T2, Ml, Mlip, Mlim, Mljp, Mljm, RHS, omega, psum, pcnt, ctl, it, mresp, nbm, nbs);
 }
 
-
-// stop test of iteration it (EnergySources.c:487-493), partial sums added in
-// block order; the last iterate is in T1 after an odd number of iterations
-__device__ __forceinline__ void b_sor_check(const double *__restrict__ psum, const int *__restrict__ pcnt,
-                                                  int nb, DevCtl *ctl, int maxiter, int it, double eps) {
+// stop test of iterations it+1 .. it+NIT (EnergySources.c:487-493), partial
+// sums added in block order; the solve ends at the first iteration that
+// meets it
+template <int NIT>
+__device__ __forceinline__ void b_sor_check(const double *__restrict__ psum, const int *__restrict__ pcnt, int nb,
+                                            DevCtl *ctl, int maxiter, int it, double eps) {
   __shared__ double red[RB];
+  __shared__ int stop;
   if (ctl->sor_done) return;
-  double s = strided_sum<RB>(psum, nb);
-  int ci = 0;
-  {
-    int b = threadIdx.x;
-    for (; b + 7 * RB < nb; b += 8 * RB) {
-      int v[8];
+  for (int k = 0; k < NIT; k++) {
+    double s = strided_sum<RB>(psum + (size_t)k * nb, nb);
+    int ci = 0;
+    {
+      const int *__restrict__ pc = pcnt + (size_t)k * nb;
+      int b = threadIdx.x;
+      for (; b + 7 * RB < nb; b += 8 * RB) {
+        int v[8];
 #pragma unroll
-      for (int u = 0; u < 8; u++) v[u] = pcnt[b + u * RB];
+        for (int u = 0; u < 8; u++) v[u] = pc[b + u * RB];
 #pragma unroll
-      for (int u = 0; u < 8; u++) ci += v[u];
+        for (int u = 0; u < 8; u++) ci += v[u];
+      }
+      for (; b < nb; b += RB) ci += pc[b];
     }
-    for (; b < nb; b += RB) ci += pcnt[b];
-  }
-  s = block_sum<RB>(s, red);
-  const double c = block_sum<RB>((double)ci, red);
-  if (threadIdx.x == 0) {
-    const int n = it + 1;
-    ctl->sor_iter = n;
-    ctl->sor_cur = n & 1;
-    ctl->sor_normres = s;
-    if (s < eps * ctl->sor_normres0 || s < SORMIN || c == 0.0) {
-      ctl->sor_done = 1;
-      ctl->sor_niter = n;
-    } else if (n >= maxiter) {
-      ctl->sor_done = 1;
-      ctl->sor_niter = maxiter + 1;
+    s = block_sum<RB>(s, red);
+    const double c = block_sum<RB>((double)ci, red);
+    if (threadIdx.x == 0) {
+      const int n = it + k + 1;
+      stop = 0;
+      ctl->sor_iter = n;
+      ctl->sor_cur = n % 3;
+      ctl->sor_normres = s;
+      if (s < eps * ctl->sor_normres0 || s < SORMIN || c == 0.0) {
+        ctl->sor_done = 1;
+        ctl->sor_niter = n;
+        stop = 1;
+      } else if (n >= maxiter) {
+        ctl->sor_done = 1;
+        ctl->sor_niter = maxiter + 1;
+        stop = 1;
+      }
     }
+    __syncthreads();
+    if (stop) return;
   }
 }
+
+template <int NIT>
 __global__ void __launch_bounds__(RB) k_sor_check(const double *__restrict__ psum, const int *__restrict__ pcnt,
                                                   int nb, DevCtl *ctl, int maxiter, int it, double eps) {
-  b_sor_check(psum, pcnt, nb, ctl, maxiter, it, eps);
+  b_sor_check<NIT>(psum, pcnt, nb, ctl, maxiter, it, eps);
 }
 
-
 // edges of T and EnergyNew = rho*T*CV (EnergySources.c:319-334).  The SOR's
-// last iterate is in T0 or T1 (ctl->sor_cur); T0 receives it, so that T0
+// last iterate is in T0, T1 or T2 (ctl->sor_cur); T0 receives it, so that T0
 // holds the solved temperature afterwards.  Nothing is done until the SOR
 // has converged (ctl->sor_done).
 __device__ __forceinline__ void b_energy_new(Geo g, const double *__restrict__ rho, double *__restrict__ T0,
-                                                   const double *__restrict__ T1, double *__restrict__ enew,
-                                                   const DevCtl *__restrict__ ctl) {
+                                             const double *__restrict__ T1, const double *__restrict__ T2,
+                                             double *__restrict__ enew, const DevCtl *__restrict__ ctl) {
   const int ns = g.ns, nr = g.nr;
   int i, j, L_;
   if (ctl->sor_done == 0) return;
   if (!xcd_cell(g, 0, g.nr, i, j, L_) || j >= ns) return;
   const int cur = ctl->sor_cur;
-  const double *__restrict__ Ts = cur ? T1 : T0;
+  const double *__restrict__ Ts = cur == 0 ? T0 : (cur == 1 ? T1 : T2);
   const size_t l = (size_t)i * ns + j;
   double t = Ts[l];
   if (i == 0) t = Ts[l + ns];
@@ -2130,9 +2182,9 @@ __device__ __forceinline__ void b_energy_new(Geo g, const double *__restrict__ r
   if (i == 0) enew[(size_t)nr * ns + j] = 0.0;
 }
 __global__ void __launch_bounds__(TB) k_energy_new(Geo g, const double *__restrict__ rho, double *__restrict__ T0,
-                                                   const double *__restrict__ T1, double *__restrict__ enew,
-                                                   const DevCtl *__restrict__ ctl) {
-  b_energy_new(g, rho, T0, T1, enew, ctl);
+                                                   const double *__restrict__ T1, const double *__restrict__ T2,
+                                                   double *__restrict__ enew, const DevCtl *__restrict__ ctl) {
+  b_energy_new(g, rho, T0, T1, T2, enew, ctl);
 }
 
 
@@ -2140,7 +2192,9 @@ __global__ void __launch_bounds__(TB) k_energy_new(Geo g, const double *__restri
 // Radial transport: ComputeLRMomenta (TransportEuler.c:318-347) + OneWindRad
 // (49-65) with ComputeStarRad (232-273), VanLeerRadial (384-415) and
 // DivisePolarGrid (SideEuler.c:82-99) for the six conserved quantities.
-// A workgroup handles one quantity on a tile of TB sectors x RC_I rings; each
+// A workgroup handles one quantity on a tile of TB sectors x RI rings (8, 16
+// or 32; a cell's result does not depend on RI: every tile restarts its window
+// four rings below the tile); each
 // thread walks its column outward with a rolling window: at ring i it loads
 // ring i+2, forms W(i+2) = Q/(RhoInt+1e-20), the slopes dq(i+1) of W and of
 // rho, the star values at interface i+1 and the flux
@@ -2153,7 +2207,6 @@ __global__ void __launch_bounds__(TB) k_energy_new(Geo g, const double *__restri
 // dq = 0 and qs = 0 there make them).  The six workgroups of one tile are
 // numbered so that they land on the same XCD (blocks go round-robin over the
 // 8 XCDs), sharing its L2 for the common rho and vr.
-#define RC_I 16
 #define RC_NQ 6
 
 struct RadQ {
@@ -2163,13 +2216,15 @@ struct RadQ {
   double of;
   int ntile_j, ntiles;
   int nq;                       // 6; 5 for the pebble fluid (no energy: quantity 4 skipped)
+  int rows;                     // rings per tile (8, 16 or 32)
 };
 
 // ring geometry of one radial tile, staged in LDS: entry r is ring i0-4+r
 // (0 outside 0..NRAD); every value the rolling window needs is ring-uniform
-#define RC_G (RC_I + 6)
+template <int RI>
 struct RadGeo {
-  double rmed[RC_G], idr[RC_G], rinf[RC_G], isurf[RC_G];
+  static constexpr int G = RI + 6;
+  double rmed[G], idr[G], rinf[G], isurf[G];
 };
 
 // slope of ComputeStarRad at ring k (dq = 0 on the first and last ring)
@@ -2192,8 +2247,8 @@ __device__ __forceinline__ double rad_star_s(int k, int nr, double rm_km, double
 // one column of quantity W (0 RadMomP, 1 RadMomM, 2 ThetaMomP, 3 ThetaMomM,
 // 4 Energy, 5 Rho) over rings i0 .. i1-1.  The loads of ring i+2 are issued
 // two iterations ahead (clamped into the grid, replaced by 0 outside it).
-template <int W>
-__device__ __forceinline__ void rad_column(const Geo &g, const RadQ &q, const RadGeo &sg, double dt, int j, int i0,
+template <int W, typename SG>
+__device__ __forceinline__ void rad_column(const Geo &g, const RadQ &q, const SG &sg, double dt, int j, int i0,
                                            int i1) {
   const int ns = g.ns, nr = g.nr;
   const int jp = (j == ns - 1) ? 0 : j + 1;
@@ -2260,8 +2315,9 @@ __device__ __forceinline__ void rad_column(const Geo &g, const RadQ &q, const Ra
   }
 }
 
+template <int RI>
 __device__ __forceinline__ void b_transport_rad(Geo g, RadQ q, double dt) {
-  __shared__ RadGeo sg;
+  __shared__ RadGeo<RI> sg;
   if (q.guard && *q.guard == 0) return;
   // XCD x (= b % 8) gets the contiguous band of tiles x*per .. x*per+per-1,
   // all six quantities of each
@@ -2272,8 +2328,8 @@ __device__ __forceinline__ void b_transport_rad(Geo g, RadQ q, double dt) {
   const int tile = slot * per + rest / q.nq;
   if (tile >= q.ntiles) return;                // whole workgroup
   const int tj = tile % q.ntile_j, ti = tile / q.ntile_j;
-  const int i0 = ti * RC_I, i1 = min(i0 + RC_I, g.nr);
-  for (int t = threadIdx.x; t < RC_G; t += TB) {
+  const int i0 = ti * RI, i1 = min(i0 + RI, g.nr);
+  for (int t = threadIdx.x; t < RadGeo<RI>::G; t += TB) {
     const int k = i0 - 4 + t;
     const bool ok = (k >= 0 && k <= g.nr);
     sg.rmed[t] = ok ? g.Rmed[k] : 0.0;
@@ -2293,8 +2349,9 @@ __device__ __forceinline__ void b_transport_rad(Geo g, RadQ q, double dt) {
     default: rad_column<5>(g, q, sg, dt, j, i0, i1); break;
   }
 }
+template <int RI>
 __global__ void __launch_bounds__(TB) k_transport_rad(Geo g, RadQ q, double dt) {
-  b_transport_rad(g, q, dt);
+  b_transport_rad<RI>(g, q, dt);
 }
 
 
@@ -2647,6 +2704,8 @@ extern "C" int thorin_dev_create(thorin_ctx *c) {
     d->rs_rows = (int)((96 * 1024) / ((size_t)c->ns * sizeof(double)));
     if (d->rs_rows > 8) d->rs_rows = 8;
     if (d->rs_rows < 1) d->rs_rows = 1;
+    d->sor_nit = 1;
+    d->rad_rows = 16;
     const size_t rs_lds = sizeof(double) * (size_t)d->rs_rows * c->ns;
     if (rs_lds > (size_t)prop.sharedMemPerBlock) {
       thorin_set_error("NSEC too large for the ring-mean kernel's LDS");
@@ -3046,10 +3105,14 @@ BATCH_BODY(k_substep2, b_substep2, TB)
 BATCH_BODY(k_substep3, b_substep3, TB)
 BATCH_BODY(k_matrix, b_matrix, TB)
 BATCH_BODY(k_sor_resid0, b_sor_resid0, TB)
-BATCH_BODY(k_sor_iter, b_sor_iter, TB)
-BATCH_BODY(k_sor_check, b_sor_check, RB)
+BATCH_BODY(k_sor_iter<1>, b_sor_iter<1>, TB)
+BATCH_BODY(k_sor_iter<2>, b_sor_iter<2>, TB)
+BATCH_BODY(k_sor_check<1>, b_sor_check<1>, RB)
+BATCH_BODY(k_sor_check<2>, b_sor_check<2>, RB)
 BATCH_BODY(k_energy_new, b_energy_new, TB)
-BATCH_BODY(k_transport_rad, b_transport_rad, TB)
+BATCH_BODY(k_transport_rad<8>, b_transport_rad<8>, TB)
+BATCH_BODY(k_transport_rad<16>, b_transport_rad<16>, TB)
+BATCH_BODY(k_transport_rad<32>, b_transport_rad<32>, TB)
 BATCH_BODY(k_ring_means, b_ring_means, 256)
 BATCH_BODY(k_transport_theta<1>, b_transport_theta<1>, TT_T)
 BATCH_BODY(k_transport_theta<2>, b_transport_theta<2>, TT_T)
@@ -3083,7 +3146,10 @@ typedef std::function<int(hipStream_t, const char *)> IssueFn;
 // plan: appends the tables of a group of operations of one kernel to the
 // host image of the stage and returns what issues them (given the device
 // address of the stage)
-typedef IssueFn (*PlanFn)(RecOp **ops, int n, std::vector<char> &tables);
+// The words appended to sig describe everything the issued work depends on
+// besides the table contents (kernel, grid, offsets, copy addresses): two
+// flushes with equal signatures issue the same graph.
+typedef IssueFn (*PlanFn)(RecOp **ops, int n, std::vector<char> &tables, std::vector<unsigned long long> &sig);
 
 struct RecOp {
   int type;                     // 0 kernel, 1 copy
@@ -3096,6 +3162,13 @@ struct RecOp {
   const void *src;
   size_t bytes;
   hipMemcpyKind kind;
+};
+
+struct EnsGraph {               // a captured flush
+  std::vector<unsigned long long> sig;
+  hipGraph_t g;
+  hipGraphExec_t x;
+  unsigned long long used;
 };
 
 struct thorin_ensemble;
@@ -3117,12 +3190,17 @@ struct thorin_ensemble {
   std::vector<Rec> recs;
   Stage st;
   long long launches, batched_ops;   // statistics: launches issued, kernel operations they carried
+  int use_graphs;                    // replay flushes as HIP graphs
+  std::vector<struct EnsGraph> graphs;
+  std::vector<std::vector<unsigned long long>> seen;   // signatures met once
+  unsigned long long tick;
+  long long captures, replays;
 };
 
 // the table of a group: n argument tuples, then n+1 block offsets; the host
 // image is built in `tables`, the device address is the stage's base + offset
 template <auto K>
-static IssueFn plan_batch(RecOp **ops, int n, std::vector<char> &tables) {
+static IssueFn plan_batch(RecOp **ops, int n, std::vector<char> &tables, std::vector<unsigned long long> &sig) {
   using Tup = decltype(params_of(K));
   const size_t o = tables.size();
   const size_t tb = (sizeof(Tup) * (size_t)n + 15) & ~(size_t)15;
@@ -3142,15 +3220,17 @@ static IssueFn plan_batch(RecOp **ops, int n, std::vector<char> &tables) {
   off[n] = b;
   const dim3 block = ops[0]->block;
   const int nb = b;
+  constexpr auto fn = BatchKernel<Batch<K>::nt, Batch<K>::body, Tup>::fn;
+  if (shm > 65536) {
+    static size_t set = 0;
+    if (shm > set &&
+        hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm) == hipSuccess)
+      set = shm;
+  }
+  const unsigned long long w[] = {(unsigned long long)(uintptr_t)K, (unsigned long long)n, (unsigned long long)nb,
+                                  (unsigned long long)block.x, (unsigned long long)shm, (unsigned long long)o};
+  sig.insert(sig.end(), w, w + 6);
   return [=](hipStream_t s, const char *dev) -> int {
-    constexpr auto fn = BatchKernel<Batch<K>::nt, Batch<K>::body, Tup>::fn;
-    if (shm > 65536) {
-      static bool set = false;
-      if (!set) {
-        HCHECK(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
-        set = true;
-      }
-    }
     hipLaunchKernelGGL(fn, dim3((unsigned)nb), block, shm, s, reinterpret_cast<const Tup *>(dev + o),
                        reinterpret_cast<const int *>(dev + o + tb), n);
     HCHECK(hipGetLastError());
@@ -3160,7 +3240,7 @@ static IssueFn plan_batch(RecOp **ops, int n, std::vector<char> &tables) {
 
 // a kernel without a batched body: each disk's launch as recorded
 template <auto K>
-static IssueFn plan_plain(RecOp **ops, int n, std::vector<char> &) {
+static IssueFn plan_plain(RecOp **ops, int n, std::vector<char> &, std::vector<unsigned long long> &sig) {
   using Tup = decltype(params_of(K));
   std::vector<Tup> args;
   std::vector<std::pair<dim3, std::pair<dim3, size_t>>> cfg;
@@ -3168,6 +3248,8 @@ static IssueFn plan_plain(RecOp **ops, int n, std::vector<char> &) {
     args.push_back(*reinterpret_cast<const Tup *>(ops[k]->args.data()));
     cfg.push_back({ops[k]->grid, {ops[k]->block, ops[k]->shmem}});
   }
+  // the arguments are baked into the launches: such a flush is never replayed
+  sig.push_back(~0ull);
   return [=](hipStream_t s, const char *) -> int {
     for (size_t k = 0; k < args.size(); k++)
       std::apply([&](auto... a) { hipLaunchKernelGGL(K, cfg[k].first, cfg[k].second.first, cfg[k].second.second, s, a...); },
@@ -3289,7 +3371,8 @@ extern "C" const char *thorin_prof_kernel_name(int kid) {
 
 // enqueue iterations it0 .. it0+count-1 (iteration + stop test each); the
 // first one also forms the initial residual when the matrix kernel has not
-static int sor_launch(thorin_ctx *c, double omega, int it0, int count, double eps = SOREPS,
+// count is rounded up to whole launches of sor_nit iterations
+static int sor_launch(thorin_ctx *c, double omega, int it0, int &count, double eps = SOREPS,
                       int maxiter = SORMAXITERS) {
   thorin_dev *d = c->dev;
   const Geo &g = d->g;
@@ -3305,11 +3388,20 @@ static int sor_launch(thorin_ctx *c, double omega, int it0, int count, double ep
   }
   const int nbs = ((ns + SOR_TJ - 1) / SOR_TJ) * ((nr - 2 + SOR_TI - 1) / SOR_TI);
   const dim3 gs((unsigned)((nbs + 7) / 8 * 8));
-  for (int it = it0; it < it0 + count; it++) {
-    LAUNCH(k_sor_iter, gs, dim3(TB), 0, g, d->b[B_TEMPER], d->b[B_TEMPER2], d->b[B_ML], d->b[B_MLIP],
-           d->b[B_MLIM], d->b[B_MLJP], d->b[B_MLJM], d->b[B_RHS], omega, d->sorp, d->sorc, d->ctl, it,
-           (const double *)d->mresp, d->nb_mresp);
-    LAUNCH(k_sor_check, dim3(1), dim3(RB), 0, d->sorp, d->sorc, nbs, d->ctl, maxiter, it, eps);
+  const int nit = d->sor_nit == 2 ? 2 : 1;
+  count = (count + nit - 1) / nit * nit;
+  for (int it = it0; it < it0 + count; it += nit) {
+    if (nit == 2) {
+      LAUNCH(k_sor_iter<2>, gs, dim3(TB), 0, g, d->b[B_TEMPER], d->b[B_TEMPER2], d->b[B_TEMPER3], d->b[B_ML],
+             d->b[B_MLIP], d->b[B_MLIM], d->b[B_MLJP], d->b[B_MLJM], d->b[B_RHS], omega, d->sorp, d->sorc, d->ctl,
+             it, (const double *)d->mresp, d->nb_mresp, nbs);
+      LAUNCH(k_sor_check<2>, dim3(1), dim3(RB), 0, d->sorp, d->sorc, nbs, d->ctl, maxiter, it, eps);
+    } else {
+      LAUNCH(k_sor_iter<1>, gs, dim3(TB), 0, g, d->b[B_TEMPER], d->b[B_TEMPER2], d->b[B_TEMPER3], d->b[B_ML],
+             d->b[B_MLIP], d->b[B_MLIM], d->b[B_MLJP], d->b[B_MLJM], d->b[B_RHS], omega, d->sorp, d->sorc, d->ctl,
+             it, (const double *)d->mresp, d->nb_mresp, nbs);
+      LAUNCH(k_sor_check<1>, dim3(1), dim3(RB), 0, d->sorp, d->sorc, nbs, d->ctl, maxiter, it, eps);
+    }
   }
   HCHECK(hipGetLastError());
   return 0;
@@ -3317,7 +3409,7 @@ static int sor_launch(thorin_ctx *c, double omega, int it0, int count, double ep
 
 // Run the solve to its end with host checks between batches, the first
 // `launched` iterations being already enqueued.  The last iterate stays in
-// B_TEMPER or B_TEMPER2 as ctl->sor_cur says (k_energy_new picks it up).
+// B_TEMPER, B_TEMPER2 or B_TEMPER3 as ctl->sor_cur says (k_energy_new picks it up).
 static int sor_finish(thorin_ctx *c, double omega, int launched, int *niter_out, double eps = SOREPS,
                       int maxiter = SORMAXITERS, int first_batch = 0) {
   thorin_dev *d = c->dev;
@@ -3529,7 +3621,10 @@ static int transport(thorin_dev *d, const Geo &g, const RadQ &q, int brho, int b
                      const int *guard) {
   const int nr = d->nr, ns = d->ns;
   const dim3 gall = cell_grid(d, nr), tb(TB);
-  LAUNCH(k_transport_rad, dim3((unsigned)((q.ntiles + 7) / 8 * 8 * q.nq)), tb, 0, g, q, dt);
+  const dim3 grad((unsigned)((q.ntiles + 7) / 8 * 8 * q.nq));
+  if (q.rows == 8) LAUNCH(k_transport_rad<8>, grad, tb, 0, g, q, dt);
+  else if (q.rows == 32) LAUNCH(k_transport_rad<32>, grad, tb, 0, g, q, dt);
+  else LAUNCH(k_transport_rad<16>, grad, tb, 0, g, q, dt);
   double *xe = be >= 0 ? d->b[B_XE] : nullptr, *oe = be >= 0 ? d->b[be] : nullptr;
   const int nq = be >= 0 ? 5 : 4;
 #define TT_LAUNCH(NPT)                                                                                        \
@@ -3854,6 +3949,7 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
     for (int k = 0; k < 4; k++) sor_k = d->niter_hist[k] > sor_k ? d->niter_hist[k] : sor_k;
     sor_k += 1;
     if (sor_k < 2) sor_k = 2;
+    if (sor_k < d->sor_queue_min) sor_k = d->sor_queue_min;   // an ensemble queues alike for all disks
     if (sor_k > SORMAXITERS + 2) sor_k = SORMAXITERS + 2;
     if (sor_launch(c, c->st.sor_omega, 0, sor_k)) return -1;
     c->st.physical_time += dt;
@@ -3868,7 +3964,8 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
     double *bsave[NBUF];
     memcpy(bsave, d->b, sizeof bsave);
     auto after_sor = [&]() -> int {
-      LAUNCH(k_energy_new, gall, tb, 0, d->g, d->b[B_DENS], d->b[B_TEMPER], d->b[B_TEMPER2], d->b[B_ENEW], d->ctl);
+      LAUNCH(k_energy_new, gall, tb, 0, d->g, d->b[B_DENS], d->b[B_TEMPER], d->b[B_TEMPER2], d->b[B_TEMPER3],
+             d->b[B_ENEW], d->ctl);
       swapb(d, B_ENERGY, B_ENEW);
       // ---- Transport
       {
@@ -3882,7 +3979,8 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
         q.guard = guard;
         q.of = c->st.omega_frame;
         q.ntile_j = (ns + TB - 1) / TB;
-        q.ntiles = q.ntile_j * ((nr + RC_I - 1) / RC_I);
+        q.rows = d->rad_rows;
+        q.ntiles = q.ntile_j * ((nr + q.rows - 1) / q.rows);
         q.nq = RC_NQ;
         if (transport(d, d->g, q, B_DENS, B_VRAD, B_VTHETA, B_ENERGY, dt, guard)) return -1;
       }
@@ -3910,7 +4008,8 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
         q.guard = guard;
         q.of = c->st.omega_frame;
         q.ntile_j = (ns + TB - 1) / TB;
-        q.ntiles = q.ntile_j * ((nr + RC_I - 1) / RC_I);
+        q.rows = d->rad_rows;
+        q.ntiles = q.ntile_j * ((nr + q.rows - 1) / q.rows);
         q.nq = RC_NQ - 1;
         if (transport(d, gp, q, B_PDENS, B_PVR, B_PVT, -1, dt, guard)) return -1;
         LAUNCH(k_peb_damp, gall, tb, 0, d->g, d->b[B_PDENS], d->b[B_PVR], d->b[B_PVT], dt, 0, d->ctl, guard);
@@ -4010,6 +4109,7 @@ static int ens_flush(thorin_ensemble *e) {
       if (op.type == 0) left[k][op.key]++;
   std::vector<char> tables;
   std::vector<IssueFn> issue;
+  std::vector<unsigned long long> sig;
   std::vector<RecOp *> grp;
   long long nops = 0;
   for (;;) {
@@ -4021,6 +4121,9 @@ static int ens_flush(thorin_ensemble *e) {
         const void *src = op.src;
         const size_t bytes = op.bytes;
         const hipMemcpyKind kind = op.kind;
+        const unsigned long long w[] = {1ull, (unsigned long long)(uintptr_t)dst, (unsigned long long)(uintptr_t)src,
+                                        (unsigned long long)bytes, (unsigned long long)kind};
+        sig.insert(sig.end(), w, w + 5);
         issue.push_back([=](hipStream_t s, const char *) -> int {
           HCHECK(hipMemcpyAsync(dst, src, bytes, kind, s));
           return 0;
@@ -4057,7 +4160,7 @@ static int ens_flush(thorin_ensemble *e) {
         left[k][best]--;
       }
     }
-    issue.push_back(grp[0]->plan(grp.data(), (int)grp.size(), tables));
+    issue.push_back(grp[0]->plan(grp.data(), (int)grp.size(), tables, sig));
     e->launches++;
     nops += (long long)grp.size();
   }
@@ -4073,12 +4176,71 @@ static int ens_flush(thorin_ensemble *e) {
     HCHECK(hipMalloc((void **)&e->st.dv, cap));
     e->st.cap = cap;
   }
-  if (!tables.empty()) {
-    memcpy(e->st.h, tables.data(), tables.size());
-    HCHECK(hipMemcpyAsync(e->st.dv, e->st.h, tables.size(), hipMemcpyHostToDevice, e->s));
+  if (!tables.empty()) memcpy(e->st.h, tables.data(), tables.size());
+  auto enqueue = [&]() -> int {
+    if (!tables.empty()) HCHECK(hipMemcpyAsync(e->st.dv, e->st.h, tables.size(), hipMemcpyHostToDevice, e->s));
+    for (auto &f : issue)
+      if (f(e->s, e->st.dv)) return -1;
+    return 0;
+  };
+  const bool replayable = e->use_graphs && !issue.empty() &&
+                          std::find(sig.begin(), sig.end(), ~0ull) == sig.end();
+  if (!replayable) {
+    if (enqueue()) return -1;
+  } else {
+    // the flush as a graph: the tables' upload, the merged launches and the
+    // copies; replayed while the signature repeats (the usual substep)
+    sig.push_back((unsigned long long)tables.size());
+    sig.push_back((unsigned long long)(uintptr_t)e->st.dv);
+    sig.push_back((unsigned long long)(uintptr_t)e->st.h);
+    EnsGraph *G = nullptr;
+    for (auto &x : e->graphs)
+      if (x.sig == sig) G = &x;
+    // a flush is captured the second time its signature comes: one-off
+    // flushes (the first DT's over-relaxation scan) run as they are
+    bool seen = G != nullptr;
+    for (size_t k = 0; !seen && k < e->seen.size(); k++) seen = e->seen[k] == sig;
+    if (!seen) {
+      if (e->seen.size() >= 16) e->seen.erase(e->seen.begin());
+      e->seen.push_back(sig);
+      if (enqueue()) return -1;
+    } else if (G) {
+      e->replays++;
+    } else {
+      if (e->graphs.size() >= 8) {    // evict the least recently used
+        size_t v = 0;
+        for (size_t k = 1; k < e->graphs.size(); k++)
+          if (e->graphs[k].used < e->graphs[v].used) v = k;
+        (void)hipGraphExecDestroy(e->graphs[v].x);
+        (void)hipGraphDestroy(e->graphs[v].g);
+        e->graphs.erase(e->graphs.begin() + (long)v);
+      }
+      EnsGraph x;
+      x.sig = sig;
+      HCHECK(hipStreamBeginCapture(e->s, hipStreamCaptureModeThreadLocal));
+      const int r = enqueue();
+      hipGraph_t g = nullptr;
+      const hipError_t ce = hipStreamEndCapture(e->s, &g);
+      if (r) {
+        if (g) (void)hipGraphDestroy(g);
+        return -1;
+      }
+      HCHECK(ce);
+      x.g = g;
+      const hipError_t ie = hipGraphInstantiate(&x.x, g, nullptr, nullptr, 0);
+      if (ie != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        HCHECK(ie);
+      }
+      e->graphs.push_back(std::move(x));
+      G = &e->graphs.back();
+      e->captures++;
+    }
+    if (G) {
+      G->used = ++e->tick;
+      HCHECK(hipGraphLaunch(G->x, e->s));
+    }
   }
-  for (auto &f : issue)
-    if (f(e->s, e->st.dv)) return -1;
   HCHECK(hipStreamSynchronize(e->s));
   for (int k = 0; k < n; k++) e->recs[k].q.clear();
   return 0;
@@ -4104,6 +4266,7 @@ extern "C" int thorin_ensemble_create(thorin_ctx *const *disks, int n, thorin_en
   for (int k = 0; k < n; k++) e->c[k] = disks[k];
   e->recs.resize((size_t)n);
   for (int k = 0; k < n; k++) { e->recs[k].e = e; e->recs[k].idx = k; }
+  e->use_graphs = 1;
   // one instance of the azimuthal transport for all disks
   int npt = 1;
   for (int k = 0; k < n; k++) {
@@ -4135,6 +4298,10 @@ extern "C" void thorin_ensemble_destroy(thorin_ensemble *e) {
   (void)hipStreamSynchronize(e->s);
   for (int k = 0; k < e->n; k++)
     if (e->c[k] && e->c[k]->dev) e->c[k]->dev->tt_npt_min = 0;
+  for (auto &x : e->graphs) {
+    (void)hipGraphExecDestroy(x.x);
+    (void)hipGraphDestroy(x.g);
+  }
   if (e->st.h) (void)hipHostFree(e->st.h);
   if (e->st.dv) (void)hipFree(e->st.dv);
   (void)hipStreamDestroy(e->s);
@@ -4151,9 +4318,22 @@ extern "C" int thorin_ensemble_algogas(thorin_ensemble *e, thorin_planets *const
   e->nwait = 0;
   e->nfinished = 0;
   e->err = 0;
+  // every disk queues as many SOR iterations as the disk that expects the
+  // most (those after its convergence do nothing): the merged substeps
+  // then repeat from DT to DT and replay as graphs
+  int qmin = 0;
+  for (int k = 0; k < n; k++) {
+    const thorin_dev *d = e->c[k]->dev;
+    int q = e->c[k]->st.sor_niter_last;
+    for (int m = 0; m < 4; m++) q = d->niter_hist[m] > q ? d->niter_hist[m] : q;
+    q += 1;
+    qmin = q > qmin ? q : qmin;
+  }
+  if (qmin > 256) qmin = 256;
   for (int k = 0; k < n; k++) {
     e->recs[k].q.clear();
     e->c[k]->dev->rec = &e->recs[k];
+    e->c[k]->dev->sor_queue_min = qmin;
   }
   std::vector<int> rc((size_t)n, 0);
   std::vector<std::thread> th;
@@ -4186,7 +4366,10 @@ extern "C" int thorin_ensemble_algogas(thorin_ensemble *e, thorin_planets *const
     e->cv_disk.notify_all();
   }
   for (auto &t : th) t.join();
-  for (int k = 0; k < n; k++) e->c[k]->dev->rec = nullptr;
+  for (int k = 0; k < n; k++) {
+    e->c[k]->dev->rec = nullptr;
+    e->c[k]->dev->sor_queue_min = 0;
+  }
   int worst = 0;
   for (int k = 0; k < n; k++) {
     if (rc_out) rc_out[k] = rc[k];
@@ -4196,10 +4379,187 @@ extern "C" int thorin_ensemble_algogas(thorin_ensemble *e, thorin_planets *const
   return status ? -1 : worst;
 }
 
-extern "C" int thorin_ensemble_stats(const thorin_ensemble *e, long long *launches, long long *kernel_ops) {
+extern "C" int thorin_ensemble_stats(const thorin_ensemble *e, long long *launches, long long *kernel_ops,
+                                     long long *graph_captures, long long *graph_replays) {
   if (!e) return -1;
   if (launches) *launches = e->launches;
   if (kernel_ops) *kernel_ops = e->batched_ops;
+  if (graph_captures) *graph_captures = e->captures;
+  if (graph_replays) *graph_replays = e->replays;
+  return 0;
+}
+
+extern "C" int thorin_ensemble_set_graphs(thorin_ensemble *e, int on) {
+  if (!e) return -1;
+  e->use_graphs = on ? 1 : 0;
+  return 0;
+}
+
+// ===========================================================================
+// Autotuner: the launch shapes that leave every result unchanged -- SOR
+// iterations per launch (1 or 2, the stop-test sums do not depend on it) and
+// rings per tile of the radial transport (8, 16, 32) -- timed on this disk's
+// grid with HIP events, the fastest kept.
+
+// A diagonally dominant five-point system for timing the SOR launches whose
+// solution (-5000) lies far below the temperature: Gauss-Seidel sweeps
+// (omega = 1) approach it monotonically from above, so every cell keeps
+// changing and the stop test never ends the timed iterations early.
+__global__ void __launch_bounds__(TB) k_tune_system(double *__restrict__ ml, double *__restrict__ mip,
+                                                    double *__restrict__ mim, double *__restrict__ mjp,
+                                                    double *__restrict__ mjm, double *__restrict__ rhs,
+                                                    const double *__restrict__ T, size_t n) {
+  const size_t l = (size_t)blockIdx.x * TB + threadIdx.x;
+  if (l >= n) return;
+  ml[l] = 1.0;
+  mip[l] = -0.2;
+  mim[l] = -0.2;
+  mjp[l] = -0.2;
+  mjm[l] = -0.2;
+  rhs[l] = -1000.0;
+  (void)T;
+}
+
+static int tune_time(thorin_dev *d, const std::function<int()> &work, int reps, double *ms_out) {
+  hipEvent_t a, b;
+  HCHECK(hipEventCreate(&a));
+  HCHECK(hipEventCreate(&b));
+  double best = 1e30;
+  for (int r = 0; r < reps; r++) {
+    HCHECK(hipEventRecord(a, d->s));
+    if (work()) return -1;
+    HCHECK(hipEventRecord(b, d->s));
+    HCHECK(hipEventSynchronize(b));
+    float ms = 0.f;
+    HCHECK(hipEventElapsedTime(&ms, a, b));
+    best = ms < best ? ms : best;
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  *ms_out = best;
+  return 0;
+}
+
+extern "C" int thorin_autotune(thorin_ctx *c, thorin_tuning *out) {
+  if (need_dev(c)) return -1;
+  thorin_dev *d = c->dev;
+  if (d->rec) { thorin_set_error("thorin_autotune: the context is inside an ensemble step"); return -1; }
+  const int nr = d->nr, ns = d->ns;
+  const size_t nb = sizeof(double) * d->nf;
+  thorin_tuning t;
+  memset(&t, 0, sizeof t);
+  HCHECK(hipStreamSynchronize(d->s));
+  // the SOR launches overwrite the temperature buffers and the matrix: the
+  // temperature is kept in B_TBACK and restored, the matrix is assembled
+  // again by the next substep (or thorin_rad_system)
+  HCHECK(hipMemcpyAsync(d->b[B_TBACK], d->b[B_TEMPER], nb, hipMemcpyDeviceToDevice, d->s));
+  hipLaunchKernelGGL(k_tune_system, dim3((unsigned)((d->nf + TB - 1) / TB)), dim3(TB), 0, d->s, d->b[B_ML],
+                     d->b[B_MLIP], d->b[B_MLIM], d->b[B_MLJP], d->b[B_MLJM], d->b[B_RHS],
+                     (const double *)d->b[B_TEMPER], d->nf);
+  HCHECK(hipGetLastError());
+  d->normres0_valid = 0;
+  d->rad_ready = 0;
+  const int saved_nit = d->sor_nit, saved_rows = d->rad_rows;
+  const int iters = 8;
+  for (int v = 0; v < 2; v++) {
+    d->sor_nit = v + 1;
+    double ms = 0.0;
+    auto work = [&]() -> int {
+      HCHECK(hipMemcpyAsync(d->b[B_TEMPER], d->b[B_TBACK], nb, hipMemcpyDeviceToDevice, d->s));
+      int cnt = iters;
+      return sor_launch(c, 1.0, 0, cnt, 0.0, 1 << 30);
+    };
+    if (work()) return -1;      // warm-up
+    if (tune_time(d, work, 3, &ms)) return -1;
+    // the copy of T (one pass) is timed too; it is the same for both
+    t.sor_us_per_iter[v] = 1e3 * ms / iters;
+  }
+  RadQ q;
+  q.rho = d->b[B_DENS];
+  q.vr = d->b[B_VRAD];
+  q.vt = d->b[B_VTHETA];
+  q.e = d->b[B_ENERGY];
+  const int outs[RC_NQ] = {B_XRP, B_XRM, B_XTP, B_XTM, B_XE, B_XRHO};
+  for (int k = 0; k < RC_NQ; k++) q.out[k] = d->b[outs[k]];
+  q.guard = nullptr;
+  q.of = c->st.omega_frame;
+  q.ntile_j = (ns + TB - 1) / TB;
+  q.nq = RC_NQ;
+  const int rows[3] = {8, 16, 32};
+  for (int v = 0; v < 3; v++) {
+    q.rows = rows[v];
+    q.ntiles = q.ntile_j * ((nr + q.rows - 1) / q.rows);
+    const dim3 grad((unsigned)((q.ntiles + 7) / 8 * 8 * q.nq)), tb(TB);
+    const double dt = c->p.dt;
+    auto work = [&]() -> int {
+      if (q.rows == 8) LAUNCH(k_transport_rad<8>, grad, tb, 0, d->g, q, dt);
+      else if (q.rows == 32) LAUNCH(k_transport_rad<32>, grad, tb, 0, d->g, q, dt);
+      else LAUNCH(k_transport_rad<16>, grad, tb, 0, d->g, q, dt);
+      HCHECK(hipGetLastError());
+      return 0;
+    };
+    double ms = 0.0;
+    if (work()) return -1;
+    if (tune_time(d, work, 3, &ms)) return -1;
+    t.rad_us[v] = 1e3 * ms;
+  }
+  HCHECK(hipMemcpyAsync(d->b[B_TEMPER], d->b[B_TBACK], nb, hipMemcpyDeviceToDevice, d->s));
+  HCHECK(hipStreamSynchronize(d->s));
+  d->normres0_valid = 0;
+  (void)saved_nit;
+  (void)saved_rows;
+  t.sor_iters_per_launch = t.sor_us_per_iter[1] < t.sor_us_per_iter[0] ? 2 : 1;
+  int bi = 0;
+  for (int v = 1; v < 3; v++)
+    if (t.rad_us[v] < t.rad_us[bi]) bi = v;
+  t.rad_rows = rows[bi];
+  d->sor_nit = t.sor_iters_per_launch;
+  d->rad_rows = t.rad_rows;
+  if (out) *out = t;
+  return 0;
+}
+
+extern "C" int thorin_set_tuning(thorin_ctx *c, const thorin_tuning *t) {
+  if (need_dev(c)) return -1;
+  if (!t || (t->sor_iters_per_launch != 1 && t->sor_iters_per_launch != 2) ||
+      (t->rad_rows != 8 && t->rad_rows != 16 && t->rad_rows != 32)) {
+    thorin_set_error("thorin_set_tuning: sor_iters_per_launch must be 1 or 2, rad_rows 8, 16 or 32");
+    return -1;
+  }
+  c->dev->sor_nit = t->sor_iters_per_launch;
+  c->dev->rad_rows = t->rad_rows;
+  return 0;
+}
+
+extern "C" int thorin_get_tuning(thorin_ctx *c, thorin_tuning *t) {
+  if (need_dev(c) || !t) return -1;
+  memset(t, 0, sizeof *t);
+  t->sor_iters_per_launch = c->dev->sor_nit;
+  t->rad_rows = c->dev->rad_rows;
+  return 0;
+}
+
+// one choice for all disks (a merged launch runs one kernel): the one with
+// the least time summed over the disks
+extern "C" int thorin_ensemble_autotune(thorin_ensemble *e, thorin_tuning *out) {
+  if (!e) return -1;
+  thorin_tuning sum;
+  memset(&sum, 0, sizeof sum);
+  for (int k = 0; k < e->n; k++) {
+    thorin_tuning t;
+    if (thorin_autotune(e->c[k], &t)) return -1;
+    for (int v = 0; v < 2; v++) sum.sor_us_per_iter[v] += t.sor_us_per_iter[v];
+    for (int v = 0; v < 3; v++) sum.rad_us[v] += t.rad_us[v];
+  }
+  const int rows[3] = {8, 16, 32};
+  sum.sor_iters_per_launch = sum.sor_us_per_iter[1] < sum.sor_us_per_iter[0] ? 2 : 1;
+  int bi = 0;
+  for (int v = 1; v < 3; v++)
+    if (sum.rad_us[v] < sum.rad_us[bi]) bi = v;
+  sum.rad_rows = rows[bi];
+  for (int k = 0; k < e->n; k++)
+    if (thorin_set_tuning(e->c[k], &sum)) return -1;
+  if (out) *out = sum;
   return 0;
 }
 
@@ -4541,8 +4901,10 @@ extern "C" int thorin_rad_solve(thorin_ctx *c, const thorin_solve_opts *o, thori
     int niter = 0;
     if (sor_finish(c, o->omega, 0, &niter, o->tol, maxit, 16)) return -1;
     res->sweeps = niter > maxit ? maxit : niter;
-    // the last iterate into B_TEMPER (T1 after an odd count)
-    if (d->hctl->sor_cur) HCHECK(hipMemcpyAsync(d->b[B_TEMPER], d->b[B_TEMPER2], nb, hipMemcpyDeviceToDevice, d->s));
+    // the last iterate into B_TEMPER (it is in T0, T1 or T2 by sor_cur)
+    if (d->hctl->sor_cur)
+      HCHECK(hipMemcpyAsync(d->b[B_TEMPER], d->b[d->hctl->sor_cur == 1 ? B_TEMPER2 : B_TEMPER3], nb,
+                            hipMemcpyDeviceToDevice, d->s));
     res->rounds = 1;
   } else if (o->method == 1) {
     const dim3 gm = cell_grid(d, nr - 2), tb(TB);

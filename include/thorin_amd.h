@@ -249,9 +249,33 @@ typedef struct thorin_ensemble thorin_ensemble;
 int  thorin_ensemble_create(thorin_ctx *const *disks, int n, thorin_ensemble **out);
 int  thorin_ensemble_algogas(thorin_ensemble *e, thorin_planets *const *sys, thorin_nbody_fn nbody,
                              void *const *nbody_user, thorin_step_log *logs, int *rc_out);
-/* launches issued so far and the per-disk kernel operations they carried */
-int  thorin_ensemble_stats(const thorin_ensemble *e, long long *launches, long long *kernel_ops);
+/* Each merged substep (the launches between two host syncs) is captured as
+ * a HIP graph and replayed while it repeats (on by default). */
+int  thorin_ensemble_set_graphs(thorin_ensemble *e, int on);
+/* launches issued so far, the per-disk kernel operations they carried, and
+ * the graphs captured and replayed (any pointer may be NULL) */
+int  thorin_ensemble_stats(const thorin_ensemble *e, long long *launches, long long *kernel_ops,
+                           long long *graph_captures, long long *graph_replays);
 void thorin_ensemble_destroy(thorin_ensemble *e);
+
+/* ---- autotuner.  No counterpart in the reference.  Launch shapes that
+ * leave every result bit-identical: SOR iterations per launch (1, or 2 with
+ * a four-cell halo in LDS and three temperature buffers) and rings per tile
+ * of the radial transport (8, 16, 32).  thorin_autotune times each on the
+ * context's grid with HIP events (on a synthetic system; the temperature is
+ * restored, the energy system is assembled again by the next substep) and
+ * keeps the fastest; thorin_ensemble_autotune picks one shape for all disks
+ * (least time summed over them).  Call them between steps. */
+typedef struct thorin_tuning {
+  int sor_iters_per_launch;       /* 1 or 2 (default 1) */
+  int rad_rows;                   /* 8, 16 or 32 (default 16) */
+  double sor_us_per_iter[2];      /* measured: per SOR iteration, 1 / 2 per launch */
+  double rad_us[3];               /* measured: one radial transport launch, 8 / 16 / 32 rings */
+} thorin_tuning;
+int  thorin_autotune(thorin_ctx *ctx, thorin_tuning *out);
+int  thorin_set_tuning(thorin_ctx *ctx, const thorin_tuning *t);
+int  thorin_get_tuning(thorin_ctx *ctx, thorin_tuning *out);
+int  thorin_ensemble_autotune(thorin_ensemble *e, thorin_tuning *out);
 
 /* Disk quantities the host needs once per DT for the torque log
  * (UpdateLog, Force.c:27-70): ring means of dens and soundspeed

@@ -52,37 +52,45 @@ def _compare(a, b, fields):
     assert np.array_equal(a.ring_profile("nshift"), b.ring_profile("nshift"))
 
 
-@pytest.mark.parametrize("sizes,n,steps", [([32, 48, 64], 4, 4), ([512, 768], 3, 2)])
-def test_ensemble_matches_single_disks(sizes, n, steps):
+@pytest.mark.parametrize("sizes,n,steps,graphs", [([32, 48, 64], 4, 6, True), ([32, 48, 64], 4, 3, False),
+                                                  ([512, 768], 3, 3, True)])
+def test_ensemble_matches_single_disks(sizes, n, steps, graphs):
     """Disks of uneven grids and parameters (alpha, Sigma0, h drawn as the
     ensemble workload draws them), from the first DT on (the SOR
-    over-relaxation scan runs batched too)."""
-    _print_case(f"sizes {sizes} n {n} steps {steps}")
+    over-relaxation scan runs batched too); with graphs, the repeating
+    substeps are replayed graphs, each replay checked against the disks
+    stepped alone."""
+    _print_case(f"sizes {sizes} n {n} steps {steps} graphs {graphs}")
     mem = _members(sizes, n)
     solo = [_make(c, dk) for c, dk in mem]
     ens_disks = [_make(c, dk) for c, dk in mem]
     ens = Ensemble(ens_disks)
+    ens.set_graphs(graphs)
     nb = host_nbody_fn()
     ps = [planets_for(configs.PLANET_ZERO_MASS) for _ in mem]
     pe = [planets_for(configs.PLANET_ZERO_MASS) for _ in mem]
     for step in range(steps):
         if step == 1:
-            l0, o0 = ens.stats()
+            l0, o0, _, _ = ens.stats()
         logs_s = [_log(d.algogas(p, nb)) for d, p in zip(solo, ps)]
         logs_e = [_log(lg) for lg in ens.algogas(pe, nb)]
         assert logs_s == logs_e, f"step {step}"
         for a, b in zip(ps, pe):
             assert (a.x[0], a.y[0], a.vx[0], a.vy[0]) == (b.x[0], b.y[0], b.vx[0], b.vy[0])
-    for a, b in zip(solo, ens_disks):
-        _compare(a, b, FIELDS)
+        for a, b in zip(solo, ens_disks):
+            _compare(a, b, FIELDS)
     # after the first DT (each disk's over-relaxation scan: solves of
     # different lengths): the ~20 kernels of a substep outside the SOR
     # iterations are launched once for all disks (each shared launch saves
     # n-1 launches); SOR iterations that only some disks need run for fewer
-    l1, o1 = ens.stats()
+    l1, o1, cap, rep = ens.stats()
     saved = (o1 - o0) - (l1 - l0)
-    _print_case(f"launches {l1 - l0}, disk kernel operations {o1 - o0}")
+    _print_case(f"launches {l1 - l0}, disk kernel operations {o1 - o0}, graphs captured {cap}, replays {rep}")
     assert saved >= (steps - 1) * (n - 1) * 15, (l1 - l0, o1 - o0)
+    if graphs:
+        assert rep > 0
+    else:
+        assert cap == 0 and rep == 0
     ens.close()
     # the contexts stay usable on their own
     for d, p in zip(ens_disks, pe):
@@ -112,3 +120,34 @@ def test_ensemble_two_fluid_and_gas_disks():
     for a, b in zip(solo[1:], ens_disks[1:]):
         _compare(a, b, FIELDS)
     ens.close()
+
+
+def test_launch_shapes_leave_results_unchanged():
+    """Every launch shape the autotuner chooses from (SOR iterations per
+    launch 1 or 2, radial transport tiles of 8, 16 or 32 rings) gives the
+    same bits, and the autotuner restores the state it times on."""
+    _print_case("launch shapes")
+    cfg, dk = _members([64], 1)[0]
+    ref = _make(cfg, dk)
+    nb = host_nbody_fn()
+    pr = planets_for(configs.PLANET_ZERO_MASS)
+    logs_r = [_log(ref.algogas(pr, nb)) for _ in range(4)]
+    for nit in (1, 2):
+        for rows in (8, 16, 32):
+            d = _make(cfg, dk)
+            d.set_tuning(nit, rows)
+            p = planets_for(configs.PLANET_ZERO_MASS)
+            logs = [_log(d.algogas(p, nb)) for _ in range(4)]
+            assert logs == logs_r, (nit, rows)
+            _compare(ref, d, FIELDS)
+            d.close()
+    d = _make(cfg, dk)
+    p = planets_for(configs.PLANET_ZERO_MASS)
+    logs = [_log(d.algogas(p, nb))]
+    t = d.autotune()                     # between steps
+    _print_case(f"tuning {t}")
+    assert t["sor_iters_per_launch"] in (1, 2) and t["rad_rows"] in (8, 16, 32)
+    assert min(t["sor_us_per_iter"]) > 0 and min(t["rad_us"]) > 0
+    logs += [_log(d.algogas(p, nb)) for _ in range(3)]
+    assert logs == logs_r
+    _compare(ref, d, FIELDS)
