@@ -268,7 +268,7 @@ def two_fluid_study(nrad=1024, nsec=1536, steps=20, warmup=3):
             "kernel_ms_per_step": {k: v[0] / 3 for k, v in sorted(prof.items(), key=lambda kv: -kv[1][0])}}
 
 
-def ensemble_study(n=16, steps=10, warmup=2):
+def ensemble_study(n=16, steps=10, warmup=2, cnn=True):
     """The ensemble workload: 16 damped disks of uneven grids (512x768 to
     1024x1536) with seeded alpha, Sigma0 and h, advanced one DT at a time
     (a) disk after disk with thorin_algogas and (b) together with
@@ -336,10 +336,12 @@ def ensemble_study(n=16, steps=10, warmup=2):
         out[key] = {"disk_steps_per_s": n * steps / el_b, "ms_per_dt": 1e3 * el_b / steps, "substeps": subs,
                     "launches_per_dt": (l1 - l0) / steps, "disk_kernel_ops_per_dt": (o1 - o0) / steps,
                     "graph_captures": c1 - c0, "graph_replays": r1 - r0}
-        E.close()
         for a, b in zip(A, B):
             for f in ("dens", "vrad", "vtheta", "energy"):
                 same &= bool(np.array_equal(a.field(f), b.field(f)))
+        if graphs and cnn:
+            out["cnn_warm_start"] = ensemble_cnn_study(E, B, pb)
+        E.close()
         for d in B:
             d.close()
     out["speedup"] = out["one_by_one"]["ms_per_dt"] / out["ensemble"]["ms_per_dt"]
@@ -347,6 +349,70 @@ def ensemble_study(n=16, steps=10, warmup=2):
     for d in A:
         d.close()
     return out
+
+
+def ensemble_cnn_study(E, disks, planets, train_s=20.0, tol=1e-10, omega=1.8, inner=16):
+    """The learned warm start on the ensemble: the network is trained on the
+    GPU on pairs that the ensemble's own disks give (the first half of the
+    disks, at two times of the run: each pair is the system of the disk's
+    substep and its solution to 1e-11), then the energy solve of every disk
+    of the other half is timed to a 1e-10 relative residual three ways: the
+    reference's FP64 SOR, FP32 sweeps in FP64 refinement, and the same from
+    the network's guess.  The solves use the disk's own substep dt."""
+    from fargo_thorin_amd import WarmStart
+    n = len(disks)
+    train, evald = list(range(n // 2)), list(range(n // 2, n))
+
+    def sub_dt(k):
+        lg = E.logs[k]
+        return lg.dt[0] if lg.nsub else 1e-3
+    net = WarmStart(seed=5)
+    net.set_loss(1)
+    t0 = time.perf_counter()
+    for rnd in range(2):
+        if rnd:
+            for _ in range(5):
+                E.algogas(planets)
+        for k in train:
+            d = disks[k]
+            d.rad_system(sub_dt(k))
+            d.rad_solve(1, tol=1e-11, omega=omega, inner_sweeps=64)
+            net.add_pair(d)
+    t_pairs = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    iters, loss = 0, (0.0, 0.0)
+    while time.perf_counter() - t1 < train_s:
+        loss = net.train(32, 3e-4)
+        iters += 32
+    t_train = time.perf_counter() - t1
+    rows = []
+    for k in evald:
+        d = disks[k]
+        dt = sub_dt(k)
+        res = {}
+        for key, kw in (("plain_fp64", dict(method=0)), ("fp32_refine", dict(method=1, inner_sweeps=inner)),
+                        ("cnn_warm_fp32_refine", dict(method=1, inner_sweeps=inner, warm=net))):
+            best = None
+            for _ in range(2):
+                d.rad_system(dt)
+                r = d.rad_solve(tol=tol, omega=omega, **kw)
+                best = r if best is None or r["ms"] < best["ms"] else best
+            res[key] = best
+        rows.append({"grid": f"{d.nrad}x{d.nsec}", "dt": dt,
+                     "ms": {k: v["ms"] for k, v in res.items()},
+                     "sweeps": {k: v["sweeps"] for k, v in res.items()},
+                     "warm_used": res["cnn_warm_fp32_refine"]["warm_used"],
+                     "guess_rel_resid": res["cnn_warm_fp32_refine"]["resid_warm"] /
+                     res["cnn_warm_fp32_refine"]["resid0"]})
+    tot = {k: sum(r["ms"][k] for r in rows) for k in rows[0]["ms"]}
+    net_pairs = net.n_pairs()
+    net.close()
+    return {"train_disks": len(train), "eval_disks": len(evald), "pairs": net_pairs, "pair_generation_s": t_pairs,
+            "train_iters": iters, "train_s": t_train, "loss_first": loss[0], "loss_last": loss[1],
+            "tol": tol, "omega": omega, "inner_sweeps": inner,
+            "ms_per_disk_step": {k: v / len(rows) for k, v in tot.items()},
+            "speedup_vs_plain": {k: tot["plain_fp64"] / v for k, v in tot.items()},
+            "disks": rows}
 
 
 def main():

@@ -709,7 +709,8 @@ struct thorin_cnn {
   float tscale;               // RMS of the training targets (standardisation), 1 before any pair
   double tss;                 // running sum of the pairs' mean squared targets
   float tinv;                 // 1/tscale during training, 1 for thorin_cnn_grad
-  int H, W;                   // activation grid (0: none)
+  int H, W;                   // activation grid of the current pass (0: none)
+  size_t cap;                 // pixels the activation buffers hold (grids up to that size)
   bf16_t *x, *a[9], *p1, *p2, *p3, *u0, *u1, *u2, *dy, *dz[9], *du0, *du1, *du2, *dp1, *dp2, *dp3;
   float *y, *t, *jac, *wpart, *bpart, *aux, *rho;
   int lossmode;               // 0: mean square of the correction error; 1: of the guess' residual
@@ -743,16 +744,23 @@ static void free_acts(thorin_cnn *n) {
   for (auto p : ds)
     if (*p) { (void)hipFree(*p); *p = nullptr; }
   n->H = n->W = 0;
+  n->cap = 0;
 }
 
+// activation buffers for an H x W grid; they grow to the largest grid met
+// (the pairs of an ensemble have several sizes) and are reused for smaller
 static int alloc_acts(thorin_cnn *n, int H, int W) {
-  if (n->H == H && n->W == W) return 0;
   if (H % 8 || W % 8 || H < 16 || W < 16) {
     thorin_set_error("the warm-start network needs NRAD and NSEC divisible by 8 (three 2x2 poolings), >= 16");
     return -1;
   }
-  free_acts(n);
   const size_t px0 = (size_t)H * W;
+  if (px0 <= n->cap) {
+    n->H = H;
+    n->W = W;
+    return 0;
+  }
+  free_acts(n);
   auto px = [&](int l) { return px0 >> (2 * l); };
   auto A = [&](bf16_t **p, size_t cnt) -> int { HCK(hipMalloc(p, cnt * sizeof(bf16_t))); return 0; };
   if (A(&n->x, px0 * CIN0)) return -1;
@@ -775,6 +783,7 @@ static int alloc_acts(thorin_cnn *n, int H, int W) {
   HCK(hipMalloc(&n->fpart, 8192 * sizeof(double)));
   n->H = H;
   n->W = W;
+  n->cap = px0;
   return 0;
 }
 
@@ -973,6 +982,7 @@ extern "C" int thorin_cnn_create(unsigned long long seed, thorin_cnn **out) {
   n->lossmode = 0;
   n->lpart = n->fpart = nullptr;
   n->H = n->W = 0;
+  n->cap = 0;
   n->adam_t = 0;
   n->tscale = 1.0f;
   n->tss = 0.0;
@@ -1115,10 +1125,6 @@ extern "C" int thorin_cnn_add_pair(thorin_cnn *n, thorin_ctx *c) {
   thorin_sysptrs sp;
   if (!n || thorin_dev_system(c, &sp)) { if (!n) thorin_set_error("null network"); return -1; }
   HCK(hipStreamSynchronize((hipStream_t)sp.stream));   // the solve that left T
-  if (!n->samples.empty() && (n->samples[0].H != sp.nr || n->samples[0].W != sp.ns)) {
-    thorin_set_error("thorin_cnn_add_pair: all training pairs must have one grid size");
-    return -1;
-  }
   thorin_sysptrs s0 = sp;
   s0.T = sp.T0;               // features of the initial temperature
   if (features(n, s0)) return -1;
@@ -1157,14 +1163,20 @@ extern "C" int thorin_cnn_num_pairs(const thorin_cnn *n) { return n ? (int)n->sa
 extern "C" int thorin_cnn_train(thorin_cnn *n, int iters, float lr, double *loss_first, double *loss_last) {
   if (!n || n->samples.empty() || iters < 1) { thorin_set_error("thorin_cnn_train: no pairs"); return -1; }
   const int ns = (int)n->samples.size();
-  if (alloc_acts(n, n->samples[0].H, n->samples[0].W)) return -1;
-  const size_t npx = (size_t)n->H * n->W;
+  {
+    int big = 0;     // room for the largest pair first
+    for (int k = 1; k < ns; k++)
+      if ((size_t)n->samples[k].H * n->samples[k].W > (size_t)n->samples[big].H * n->samples[big].W) big = k;
+    if (alloc_acts(n, n->samples[big].H, n->samples[big].W)) return -1;
+  }
   double lf = 0.0, ll = 0.0;
   int cf = 0, cl = 0;
   const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
   n->tinv = 1.0f / n->tscale;
   for (int it = 0; it < iters; it++) {
     const Sample &sm = n->samples[it % ns];
+    if (alloc_acts(n, sm.H, sm.W)) return -1;
+    const size_t npx = (size_t)n->H * n->W;
     HCK(hipMemcpyAsync(n->x, sm.x, npx * CIN0 * sizeof(bf16_t), hipMemcpyDeviceToDevice, n->s));
     HCK(hipMemcpyAsync(n->t, sm.t, npx * sizeof(float), hipMemcpyDeviceToDevice, n->s));
     if (n->lossmode) HCK(hipMemcpyAsync(n->aux, sm.aux, 5 * npx * sizeof(float), hipMemcpyDeviceToDevice, n->s));
