@@ -175,14 +175,34 @@ __global__ void __launch_bounds__(256) k_conv_lds(const bf16_t *__restrict__ x, 
   const int segs = (W + CV_SEG - 1) / CV_SEG;
   const int h = blockIdx.x / segs, w0 = (blockIdx.x % segs) * CV_SEG;
   if (h >= H) return;
-  for (int e = threadIdx.x; e < 3 * NP * (CIN / 4); e += 256) {
-    const int c4 = e % (CIN / 4), rest = e / (CIN / 4);
-    const int p = rest % NP, r = rest / NP;
-    const int hh = h + r - 1;
-    const int ww = ((w0 + p - 1) % W + W) % W;
-    s4 v = (s4){0, 0, 0, 0};
-    if (hh >= 0 && hh < H) v = *(const s4 *)(x + ((size_t)hh * W + ww) * CIN + c4 * 4);
-    *(s4 *)&sx[(r * NP + p) * PS + c4 * 4] = v;
+  {
+    // the strip's loads are all issued before their LDS stores, so that they
+    // are in flight together (one memory latency per strip, not per element)
+    constexpr int NE = 3 * NP * (CIN / 4), NL = (NE + 255) / 256;
+    s4 v[NL];
+#pragma unroll
+    for (int k = 0; k < NL; k++) {
+      const int e = threadIdx.x + k * 256;
+      v[k] = (s4){0, 0, 0, 0};
+      if (e < NE) {
+        const int c4 = e % (CIN / 4), rest = e / (CIN / 4);
+        const int p = rest % NP, r = rest / NP;
+        const int hh = h + r - 1;
+        int ww = w0 + p - 1;                   // periodic in the sectors
+        while (ww < 0) ww += W;
+        while (ww >= W) ww -= W;
+        if (hh >= 0 && hh < H) v[k] = *(const s4 *)(x + ((size_t)hh * W + ww) * CIN + c4 * 4);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < NL; k++) {
+      const int e = threadIdx.x + k * 256;
+      if (e < NE) {
+        const int c4 = e % (CIN / 4), rest = e / (CIN / 4);
+        const int p = rest % NP, r = rest / NP;
+        *(s4 *)&sx[(r * NP + p) * PS + c4 * 4] = v[k];
+      }
+    }
   }
   __syncthreads();
   const int m = lane & 15, kq = lane >> 4;
@@ -300,32 +320,35 @@ __global__ void __launch_bounds__(256) k_wgrad(const bf16_t *__restrict__ x, con
       *(s4 *)&sdz[px][c4 * 4] = v;
     }
     __syncthreads();
+    // (no early exit from the unrolled loops: acc must stay in registers)
 #pragma unroll
     for (int q = 0; q < TPW; q++) {
       const int tile = wv + 4 * q;
-      if (tile >= NT) break;
-      const int nb = tile % NB, cb = (tile / NB) % CB, t = tile / (NB * CB);
-      const int r = t / 3, dw = t % 3;          // sx column of pixel p for tap dw: p + dw
+      if (tile < NT) {
+        const int nb = tile % NB, cb = (tile / NB) % CB, t = tile / (NB * CB);
+        const int r = t / 3, dw = t % 3;          // sx column of pixel p for tap dw: p + dw
 #pragma unroll
-      for (int kc = 0; kc < WG_SEG / 16; kc++) {
-        s4 a, b;
+        for (int kc = 0; kc < WG_SEG / 16; kc++) {
+          s4 a, b;
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
-          const int p = kc * 16 + 4 * (lane >> 4) + i;
-          a[i] = (short)sx[r][p + dw][cb * 16 + (lane & 15)];
-          b[i] = (short)sdz[p][nb * 16 + (lane & 15)];
+          for (int i = 0; i < 4; i++) {
+            const int p = kc * 16 + 4 * (lane >> 4) + i;
+            a[i] = (short)sx[r][p + dw][cb * 16 + (lane & 15)];
+            b[i] = (short)sdz[p][nb * 16 + (lane & 15)];
+          }
+          acc[q] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[q], 0, 0, 0);
         }
-        acc[q] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[q], 0, 0, 0);
       }
     }
   }
 #pragma unroll
   for (int q = 0; q < TPW; q++) {
     const int tile = wv + 4 * q;
-    if (tile >= NT) break;
-    float *dst = part + ((size_t)blockIdx.x * NT + tile) * 256;
+    if (tile < NT) {
+      float *dst = part + ((size_t)blockIdx.x * NT + tile) * 256;
 #pragma unroll
-    for (int i = 0; i < 4; i++) dst[(4 * (lane >> 4) + i) * 16 + (lane & 15)] = acc[q][i];
+      for (int i = 0; i < 4; i++) dst[(4 * (lane >> 4) + i) * 16 + (lane & 15)] = acc[q][i];
+    }
   }
 }
 
