@@ -92,6 +92,8 @@ struct DevCtl {                 // device-side control block, mirrored to pinned
   double pacc[PEB_MAXPL][3];    // pebble accretion per planet: dM, dPX, dPY
 };
 
+static_assert(sizeof(DevCtl) % 8 == 0, "k_cfl_final mirrors DevCtl in 8-byte words");
+
 struct Mat32 {                  // FP32 matrix diagonals, residual and correction double buffer
   float *ml, *mip, *mim, *mjp, *mjm, *r;
   float *e0, *e1;
@@ -109,6 +111,10 @@ struct PlanetK {                // per-planet constants of FillForcesArrays (Pfr
 
 struct BOff {                   // first workgroup of this disk in a batched launch (0 otherwise),
   int blk0;                     // for the kernels without a Geo argument
+};
+
+struct PlanetsK {               // the planets of the force kernel, passed by value (no copy)
+  PlanetK p[8];
 };
 
 struct Geo {                    // passed by value to every kernel
@@ -149,7 +155,7 @@ struct thorin_dev {
   Geo g;
   DevCtl *ctl;                  // device
   DevCtl *hctl;                 // pinned host mirror
-  PlanetK *pk, *hpk;            // device / pinned host planet constants
+
   double *red;                  // block partial sums (forces)
   double *sorp;                 // SOR partial sums (2 passes)
   int *sorc;                    // SOR partial counts
@@ -641,7 +647,8 @@ __global__ void __launch_bounds__(RB) k_eos_cfl(Geo g, const double *__restrict_
 // substep's k_substep3 (nstellar of them; 0: none formed) goes to
 // ctl->dt_stellar.
 __device__ __forceinline__ void b_cfl_final(Geo g, DevCtl *ctl, const double *__restrict__ pstellar,
-                                                  int nstellar, double invdtreb_sq, const int *__restrict__ guard) {
+                                                  int nstellar, double invdtreb_sq, const int *__restrict__ guard,
+                                             DevCtl *__restrict__ hout) {
   __shared__ double red[RB];
   if (guard && *guard == 0) return;
   if (nstellar > 0) {
@@ -673,10 +680,19 @@ __device__ __forceinline__ void b_cfl_final(Geo g, DevCtl *ctl, const double *__
   }
   m = block_min<RB>(m, red);
   if (threadIdx.x == 0) ctl->newdt_raw = m;
+  if (hout) {
+    // the control block straight into the host's pinned mirror: the end of
+    // the substep needs no copy of its own before the host's sync
+    __syncthreads();
+    const unsigned long long *src = (const unsigned long long *)ctl;
+    unsigned long long *dst = (unsigned long long *)hout;
+    for (int w = threadIdx.x; w < (int)(sizeof(DevCtl) / 8); w += RB) dst[w] = src[w];
+  }
 }
 __global__ void __launch_bounds__(RB) k_cfl_final(Geo g, DevCtl *ctl, const double *__restrict__ pstellar,
-                                                  int nstellar, double invdtreb_sq, const int *__restrict__ guard) {
-  b_cfl_final(g, ctl, pstellar, nstellar, invdtreb_sq, guard);
+                                                  int nstellar, double invdtreb_sq, const int *__restrict__ guard,
+                                                  DevCtl *__restrict__ hout) {
+  b_cfl_final(g, ctl, pstellar, nstellar, invdtreb_sq, guard, hout);
 }
 
 
@@ -698,7 +714,7 @@ __device__ __forceinline__ void b_forces(Geo g, const double *__restrict__ rho,
                                                const double *__restrict__ abs_,
                                                const double *__restrict__ ord,
                                                double *__restrict__ agr, double *__restrict__ agt,
-                                               const PlanetK *__restrict__ pk, int npl, double mt,
+                                               const PlanetsK &pk, int npl, double mt,
                                                double *__restrict__ partial, TqRings tq, int ntq_rings,
                                                DevCtl *__restrict__ ctl, PebF pf) {
   __shared__ double red[TB];
@@ -756,7 +772,7 @@ __device__ __forceinline__ void b_forces(Geo g, const double *__restrict__ rho,
     unsigned int withpl = 0;    // bit k: IntegrateWithPlanet[k] (npl <= 8 on this path)
     double s2[8], hillcut[8];
     for (int k = 0; k < npl; k++) {
-      const PlanetK P = pk[k];
+      const PlanetK P = pk.p[k];
       double smoothing = 0.0;
       if (P.iring >= 0) smoothing = g.thicksmooth * (g.gcs[P.iring] * g.OmegaInv[P.iring] * g.sqrt_adiabind_inv);
       const double Hpl = smoothing / g.thicksmooth;
@@ -798,7 +814,7 @@ __device__ __forceinline__ void b_forces(Geo g, const double *__restrict__ rho,
         }
         for (int k = 0; k < npl; k++) {
           if (withpl & (1u << k)) {
-            const PlanetK P = pk[k];
+            const PlanetK P = pk.p[k];
             double zpl = P.z, taper;
             d2 = s2[k] + (znode - zpl) * (znode - zpl);
             if (d2 < P.rst2) {
@@ -863,7 +879,7 @@ __device__ __forceinline__ void b_forces(Geo g, const double *__restrict__ rho,
       pagr_l = 0.0 + (-g.Rmed[i] * tmp);
     }
     for (int k = 0; k < npl; k++) {
-      const PlanetK P = pk[k];
+      const PlanetK P = pk.p[k];
       double mpl = P.m;
       if (withpl & (1u << k)) {
         if (P.z < -0.05 * H || P.z > 0.05 * H) tmp = (x - P.x) * integpl[k] / (2.0 * sigma);
@@ -958,7 +974,7 @@ __global__ void __launch_bounds__(TB) k_forces(Geo g, const double *__restrict__
                                                const double *__restrict__ abs_,
                                                const double *__restrict__ ord,
                                                double *__restrict__ agr, double *__restrict__ agt,
-                                               const PlanetK *__restrict__ pk, int npl, double mt,
+                                               const PlanetsK pk, int npl, double mt,
                                                double *__restrict__ partial, TqRings tq, int ntq_rings,
                                                DevCtl *__restrict__ ctl, PebF pf) {
   b_forces<PEB>(g, rho, cs, abs_, ord, agr, agt, pk, npl, mt, partial, tq, ntq_rings, ctl, pf);
@@ -1270,7 +1286,7 @@ struct PebAccK {
 // and the x- and y-momentum each planet takes from the disk go to
 // ctl->pacc; the host updates the planet as STEP 7 does.
 #define ACC_T 1024
-__device__ __forceinline__ void b_peb_accrete(Geo g, PebAccK P, double *__restrict__ pdens,
+__device__ __forceinline__ void b_peb_accrete(Geo g, const PebAccK &P, double *__restrict__ pdens,
                                                        const double *__restrict__ pvr, const double *__restrict__ pvt,
                                                        const double *__restrict__ tau, const double *__restrict__ cs,
                                                        const double *__restrict__ abs_, const double *__restrict__ ord,
@@ -2825,8 +2841,6 @@ extern "C" int thorin_dev_create(thorin_ctx *c) {
   HCHECK(hipMemsetAsync(d->ctl, 0, sizeof(DevCtl), d->s));
   HCHECK(hipHostMalloc(&d->hctl, sizeof(DevCtl), hipHostMallocDefault));
   memset(d->hctl, 0, sizeof(DevCtl));
-  HCHECK(hipMalloc(&d->pk, sizeof(PlanetK) * MAXPL));
-  HCHECK(hipHostMalloc(&d->hpk, sizeof(PlanetK) * MAXPL, hipHostMallocDefault));
   d->nblk_cells = ((c->ns + TB - 1) / TB) * c->nr;
   HCHECK(hipMalloc(&d->red, sizeof(double) * (size_t)d->nblk_cells * (2 + 2 * 8)));
   const int half = c->ns / 2;
@@ -2851,8 +2865,6 @@ extern "C" void thorin_dev_destroy(thorin_ctx *c) {
   if (d->g.Nshift) (void)hipFree(d->g.Nshift);
   if (d->ctl) (void)hipFree(d->ctl);
   if (d->hctl) (void)hipHostFree(d->hctl);
-  if (d->pk) (void)hipFree(d->pk);
-  if (d->hpk) (void)hipHostFree(d->hpk);
   if (d->red) (void)hipFree(d->red);
   if (d->sorp) (void)hipFree(d->sorp);
   if (d->sorc) (void)hipFree(d->sorc);
@@ -3683,7 +3695,7 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
       LAUNCH(k_eos_cfl, dim3(nr), rb, eos_lds(d, 0), d->g, d->b[B_DENS], d->b[B_ENERGY],
              d->b[B_TEMPER], d->b[B_CS], d->b[B_VRAD], d->b[B_VTHETA], 0, (const int *)nullptr);
       LAUNCH(k_cfl_final, dim3(1), rb, 0, d->g, d->ctl, (const double *)nullptr, 0, invdtreb_sq,
-             (const int *)nullptr);
+             (const int *)nullptr, (DevCtl *)nullptr);
       HCHECK(hipGetLastError());
       DCOPY(d->hctl, d->ctl, sizeof(DevCtl), hipMemcpyDeviceToHost);
       DSYNC();
@@ -3739,12 +3751,14 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
              p->evaporationtemperature / c->k.t2si);
     }
     dtemp += dt;
-    // ---- FillForcesArrays
+    // ---- FillForcesArrays (the planets' constants go to the kernel by value)
     double indir_px = 0.0, indir_py = 0.0;
+    PlanetsK pks;
+    memset(&pks, 0, sizeof pks);
     {
       double axs = 0.0, ays = 0.0;
       for (int k = 0; k < sys->nb; k++) {
-        PlanetK &P = d->hpk[k];
+        PlanetK &P = pks.p[k];
         const double xpl = sys->x[k], ypl = sys->y[k], zpl = sys->z[k];
         double mpl = sys->mass[k];
         const double dpl = sqrt(xpl * xpl + ypl * ypl + zpl * zpl);
@@ -3775,8 +3789,6 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
       }
       indir_px = -axs;
       indir_py = -ays;
-      if (sys->nb > 0)
-        DCOPY(d->pk, d->hpk, sizeof(PlanetK) * sys->nb, hipMemcpyHostToDevice);
     }
     const int want_tq = p->write_torque && log && log->nsub == 0 && sys->nb > 0;
     // UpdateLog (Force.c:27-70, called from AdvanceSystemFromDisk) at the first
@@ -3816,10 +3828,10 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
     pf.pagt = d->b[B_PAGT];
     if (peb)
       LAUNCH(k_forces<true>, dim3(gall.x + (ntq ? 1u : 0u)), tb, 0, d->g, d->b[B_DENS], d->b[B_CS], d->b[B_ABS],
-             d->b[B_ORD], d->b[B_AGR], d->b[B_AGT], d->pk, sys->nb, mt, d->red, rr, 2 * ntq, d->ctl, pf);
+             d->b[B_ORD], d->b[B_AGR], d->b[B_AGT], pks, sys->nb, mt, d->red, rr, 2 * ntq, d->ctl, pf);
     else
       LAUNCH(k_forces<false>, dim3(gall.x + (ntq ? 1u : 0u)), tb, 0, d->g, d->b[B_DENS], d->b[B_CS], d->b[B_ABS],
-             d->b[B_ORD], d->b[B_AGR], d->b[B_AGT], d->pk, sys->nb, mt, d->red, rr, 2 * ntq, d->ctl, pf);
+             d->b[B_ORD], d->b[B_AGR], d->b[B_AGT], pks, sys->nb, mt, d->red, rr, 2 * ntq, d->ctl, pf);
     LAUNCH(k_forces_reduce, dim3(2 + 2 * sys->nb), rb, 0, d->red, d->nblk_cells, d->ctl, BOff{0});
     LAUNCH(k_indirect, gall, tb, 0, d->g, d->b[B_AGR], d->b[B_AGT], d->b[B_ABS], d->b[B_ORD], d->ctl, indir_px,
            indir_py, peb ? d->b[B_PAGR] : (double *)nullptr, peb ? d->b[B_PAGT] : (double *)nullptr);
@@ -4020,12 +4032,15 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
       // minima of the next substep (its planets are those of now)
       LAUNCH(k_eos_cfl, dim3(nr), rb, eos_lds(d, 1), d->g, d->b[B_DENS], d->b[B_ENERGY],
              d->b[B_TEMPER], d->b[B_CS], d->b[B_VRAD], d->b[B_VTHETA], 1, guard);
-      LAUNCH(k_cfl_final, dim3(1), rb, 0, d->g, d->ctl, (const double *)d->pstellar, d->nblk_cells,
-             d->cfl_invdtreb, guard);
-      HCHECK(hipGetLastError());
       // one sync per substep: SOR result, crash flags, dt_stellar, the next
-      // CFL minimum, the planet accelerations
-      DCOPY(d->hctl, d->ctl, sizeof(DevCtl), hipMemcpyDeviceToHost);
+      // CFL minimum, the planet accelerations, written to the pinned mirror
+      // by k_cfl_final.  It does nothing while the SOR has not converged:
+      // the mirror then reads sor_done = 0 and the solve is finished from
+      // the host.
+      d->hctl->sor_done = 0;
+      LAUNCH(k_cfl_final, dim3(1), rb, 0, d->g, d->ctl, (const double *)d->pstellar, d->nblk_cells,
+             d->cfl_invdtreb, guard, d->hctl);
+      HCHECK(hipGetLastError());
       DSYNC();
       return 0;
     };
@@ -4807,7 +4822,8 @@ extern "C" int thorin_rad_system(thorin_ctx *c, double dt, double *dt_used) {
     gg.pebbles = 0;
     LAUNCH(k_eos_cfl, dim3(nr), rb, eos_lds(d, 0), gg, d->b[B_DENS], d->b[B_ENERGY], d->b[B_TEMPER],
            d->b[B_CS], d->b[B_VRAD], d->b[B_VTHETA], 0, (const int *)nullptr);
-    LAUNCH(k_cfl_final, dim3(1), rb, 0, gg, d->ctl, (const double *)nullptr, 0, 0.0, (const int *)nullptr);
+    LAUNCH(k_cfl_final, dim3(1), rb, 0, gg, d->ctl, (const double *)nullptr, 0, 0.0, (const int *)nullptr,
+           (DevCtl *)nullptr);
     HCHECK(hipMemcpyAsync(d->hctl, d->ctl, sizeof(DevCtl), hipMemcpyDeviceToHost, d->s));
     HCHECK(hipStreamSynchronize(d->s));
     dt = d->hctl->newdt_raw;
