@@ -241,6 +241,13 @@ static int ctx_create(const thorin_params *p, thorin_ctx **out, int with_device)
   c->h_energy = alloc1((int)c->nfield); c->h_label = alloc1((int)c->nfield); c->h_cs = alloc1((int)c->nfield);
   c->h_press = alloc1((int)c->nfield); c->h_temper = alloc1((int)c->nfield);
   c->h_abs = alloc1((int)c->nfield); c->h_ord = alloc1((int)c->nfield);
+  c->h_trig = alloc1(2 * c->ns);
+  if (!c->h_dens || !c->h_vrad || !c->h_vtheta || !c->h_energy || !c->h_label || !c->h_cs || !c->h_press ||
+      !c->h_temper || !c->h_abs || !c->h_ord || !c->h_trig) {
+    thorin_set_error("out of host memory");
+    thorin_ctx_destroy(c);
+    return -1;
+  }
   c->st.dt_stellar = 1.0e38;
   c->st.omega_frame = p->omegaframe;
   c->st.sor_first = 1;
@@ -269,7 +276,12 @@ static int ctx_create(const thorin_params *p, thorin_ctx **out, int with_device)
       return -1;
     }
   }
-  /* InitComputeAccel, SideEuler.c:101-118 */
+  /* InitComputeAccel, SideEuler.c:101-118; the device forms Rmed[i] times
+     these per-sector values, the same products */
+  for (j = 0; j < c->ns; j++) {
+    c->h_trig[j] = cos(2.0 * TH_PI * (double)j / (double)c->ns);
+    c->h_trig[c->ns + j] = sin(2.0 * TH_PI * (double)j / (double)c->ns);
+  }
   for (i = 0; i < c->nr; i++)
     for (j = 0; j < c->ns; j++) {
       c->h_abs[j + i * c->ns] = c->Rmed[i] * cos(2.0 * TH_PI * (double)j / (double)c->ns);
@@ -303,7 +315,7 @@ int thorin_ctx_destroy(thorin_ctx *c) {
   free(c->pow_rmed15); free(c->pow_rinf15); free(c->cav_med); free(c->cav_inf); free(c->cfl_pow2);
   free(c->supp_torque);
   free(c->h_dens); free(c->h_vrad); free(c->h_vtheta); free(c->h_energy); free(c->h_label);
-  free(c->h_cs); free(c->h_press); free(c->h_temper); free(c->h_abs); free(c->h_ord);
+  free(c->h_cs); free(c->h_press); free(c->h_temper); free(c->h_abs); free(c->h_ord); free(c->h_trig);
   free(c);
   return 0;
 }

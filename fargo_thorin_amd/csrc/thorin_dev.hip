@@ -66,7 +66,7 @@ enum {
   B_ENEW = THORIN_F_ENERGYNEW, B_VRADINT = THORIN_F_VRADINT, B_VTHETAINT = THORIN_F_VTHETAINT,
   B_VRADNEW = THORIN_F_VRADNEW, B_VTHETANEW = THORIN_F_VTHETANEW,
   B_DIFF = THORIN_NFIELDS, B_ML, B_MLIP, B_MLIM, B_MLJP, B_MLJM, B_RHS, B_TBACK, B_TEMPER2, B_TEMPER3,
-  B_RP, B_RM, B_TP, B_TM, B_XRP, B_XRM, B_XTP, B_XTM, B_XE, B_XRHO, B_ABS, B_ORD,
+  B_RP, B_RM, B_TP, B_TM, B_XRP, B_XRM, B_XTP, B_XTM, B_XE, B_XRHO,
   // pebble fluid work fields (Pebbles.c:37-56, SourceEuler.c:155-158)
   B_PAGR, B_PAGT, B_PACCR, B_PACCT, B_GACCR, B_GACCT, B_DRAGR, B_DRAGT, B_MACC,
   NBUF
@@ -125,6 +125,10 @@ struct Geo {                    // passed by value to every kernel
   const double *OmegaInv, *SigmaMed, *SigmaInf, *EnergyMed, *VthetaMed, *WaveKiller;
   const double *pow_rmed15, *pow_rinf15, *cav_med, *cav_inf, *cfl_pow2, *supp_torque;
   const int *dampring;
+  // cos and sin of the sector angles 2 pi j / NSEC (host libm): a cell's
+  // x = Rmed[i] cos_j and y = Rmed[i] sin_j are the reference's
+  // CellAbscissa / CellOrdinate (SideEuler.c:101-118) bit for bit
+  const double *cosj, *sinj;
   double *gcs, *gtemp, *Vmoy, *VMed, *ringmax;
   double *sig_avr, *ist_avr;    // ring-averaged vertical integrals of FillForcesArrays
   int *Nshift;
@@ -152,6 +156,7 @@ struct thorin_dev {
   size_t nf;
   double *b[NBUF];
   double *d1base;
+  double *trig;                 // cos, sin of the sector angles (2*NSEC)
   Geo g;
   DevCtl *ctl;                  // device
   DevCtl *hctl;                 // pinned host mirror
@@ -711,8 +716,6 @@ struct PebF {
 template <bool PEB>
 __device__ __forceinline__ void b_forces(Geo g, const double *__restrict__ rho,
                                                const double *__restrict__ cs,
-                                               const double *__restrict__ abs_,
-                                               const double *__restrict__ ord,
                                                double *__restrict__ agr, double *__restrict__ agt,
                                                const PlanetsK &pk, int npl, double mt,
                                                double *__restrict__ partial, TqRings tq, int ntq_rings,
@@ -756,7 +759,7 @@ __device__ __forceinline__ void b_forces(Geo g, const double *__restrict__ rho,
     const double sigma_avr = g.sig_avr[i], integstar_avr = g.ist_avr[i];
     int IntegrateLocally = 0, Nvert = 10;
     double ax = 0.0, ay = 0.0;
-    const double x = abs_[l], y = ord[l];
+    const double x = g.Rmed[i] * g.cosj[j], y = g.Rmed[i] * g.sinj[j];
     const double mcell = rho[l] * g.Surf[i];
     const double H = cs[l] * omegainv * g.sqrt_adiabind_inv;
     H2 = H * H;
@@ -971,13 +974,11 @@ __device__ __forceinline__ void b_forces(Geo g, const double *__restrict__ rho,
 template <bool PEB>
 __global__ void __launch_bounds__(TB) k_forces(Geo g, const double *__restrict__ rho,
                                                const double *__restrict__ cs,
-                                               const double *__restrict__ abs_,
-                                               const double *__restrict__ ord,
                                                double *__restrict__ agr, double *__restrict__ agt,
                                                const PlanetsK pk, int npl, double mt,
                                                double *__restrict__ partial, TqRings tq, int ntq_rings,
                                                DevCtl *__restrict__ ctl, PebF pf) {
-  b_forces<PEB>(g, rho, cs, abs_, ord, agr, agt, pk, npl, mt, partial, tq, ntq_rings, ctl, pf);
+  b_forces<PEB>(g, rho, cs, agr, agt, pk, npl, mt, partial, tq, ntq_rings, ctl, pf);
 }
 
 
@@ -1003,8 +1004,7 @@ __global__ void __launch_bounds__(RB) k_forces_reduce(const double *__restrict__
 
 // indirect term (Pframeforce.c:488-509)
 __device__ __forceinline__ void b_indirect(Geo g, double *__restrict__ agr, double *__restrict__ agt,
-                                                 const double *__restrict__ abs_, const double *__restrict__ ord,
-                                                 const DevCtl *__restrict__ ctl, double indir_px, double indir_py,
+                                                                                                  const DevCtl *__restrict__ ctl, double indir_px, double indir_py,
                                                  double *__restrict__ pagr, double *__restrict__ pagt) {
   const int ns = g.ns;
   int i, j, L_;
@@ -1012,7 +1012,7 @@ __device__ __forceinline__ void b_indirect(Geo g, double *__restrict__ agr, doub
   double axindir = -ctl->axstar, ayindir = -ctl->aystar;
   if (g.indirect_term) { axindir += indir_px; ayindir += indir_py; }
   const size_t l = (size_t)i * ns + j;
-  const double x = abs_[l], y = ord[l];
+  const double x = g.Rmed[i] * g.cosj[j], y = g.Rmed[i] * g.sinj[j];
   const double ar = (x * axindir + y * ayindir) * g.InvRmed[i];
   const double at = (x * ayindir - y * axindir) * g.InvRmed[i];
   agr[l] += ar;
@@ -1023,10 +1023,9 @@ __device__ __forceinline__ void b_indirect(Geo g, double *__restrict__ agr, doub
   }
 }
 __global__ void __launch_bounds__(TB) k_indirect(Geo g, double *__restrict__ agr, double *__restrict__ agt,
-                                                 const double *__restrict__ abs_, const double *__restrict__ ord,
-                                                 const DevCtl *__restrict__ ctl, double indir_px, double indir_py,
+                                                                                                  const DevCtl *__restrict__ ctl, double indir_px, double indir_py,
                                                  double *__restrict__ pagr, double *__restrict__ pagt) {
-  b_indirect(g, agr, agt, abs_, ord, ctl, indir_px, indir_py, pagr, pagt);
+  b_indirect(g, agr, agt, ctl, indir_px, indir_py, pagr, pagt);
 }
 
 
@@ -1289,8 +1288,7 @@ struct PebAccK {
 __device__ __forceinline__ void b_peb_accrete(Geo g, const PebAccK &P, double *__restrict__ pdens,
                                                        const double *__restrict__ pvr, const double *__restrict__ pvt,
                                                        const double *__restrict__ tau, const double *__restrict__ cs,
-                                                       const double *__restrict__ abs_, const double *__restrict__ ord,
-                                                       double *__restrict__ macc, double dt, DevCtl *ctl) {
+                                                                                                              double *__restrict__ macc, double dt, DevCtl *ctl) {
   __shared__ double red[ACC_T / 64];
   __shared__ double s_lim;
   const int tid = threadIdx.x, ns = g.ns;
@@ -1312,7 +1310,8 @@ __device__ __forceinline__ void b_peb_accrete(Geo g, const PebAccK &P, double *_
     for (int q = tid; q < nann; q += ACC_T) {
       int i;
       const size_t l = cell(q, i);
-      const double xc = abs_[l], yc = ord[l];
+      const int jc = (int)(l - (size_t)i * ns);
+      const double xc = g.Rmed[i] * g.cosj[jc], yc = g.Rmed[i] * g.sinj[jc];
       const double R2 = (xc - X) * (xc - X) + (yc - Y) * (yc - Y);
       if (R2 <= Reff2) {
         const double H = cs[l] * g.OmegaInv[i] * g.sqrt_adiabind_inv;
@@ -1360,7 +1359,8 @@ __device__ __forceinline__ void b_peb_accrete(Geo g, const PebAccK &P, double *_
       const size_t l = cell(q, i);
       const double mc = macc[l];
       if (mc > 0.0) {
-        const double xc = abs_[l], yc = ord[l];
+        const int jc = (int)(l - (size_t)i * ns);
+        const double xc = g.Rmed[i] * g.cosj[jc], yc = g.Rmed[i] * g.sinj[jc];
         const double vrc = pvr[l], vtc = pvt[l] + g.Rmed[i] * g.omega_frame;
         const double vxc = (vrc * xc - vtc * yc) * g.InvRmed[i];
         const double vyc = (vrc * yc + vtc * xc) * g.InvRmed[i];
@@ -1389,9 +1389,8 @@ __device__ __forceinline__ void b_peb_accrete(Geo g, const PebAccK &P, double *_
 __global__ void __launch_bounds__(ACC_T) k_peb_accrete(Geo g, PebAccK P, double *__restrict__ pdens,
                                                        const double *__restrict__ pvr, const double *__restrict__ pvt,
                                                        const double *__restrict__ tau, const double *__restrict__ cs,
-                                                       const double *__restrict__ abs_, const double *__restrict__ ord,
-                                                       double *__restrict__ macc, double dt, DevCtl *ctl) {
-  b_peb_accrete(g, P, pdens, pvr, pvt, tau, cs, abs_, ord, macc, dt, ctl);
+                                                                                                              double *__restrict__ macc, double dt, DevCtl *ctl) {
+  b_peb_accrete(g, P, pdens, pvr, pvt, tau, cs, macc, dt, ctl);
 }
 
 
@@ -2849,9 +2848,11 @@ extern "C" int thorin_dev_create(thorin_ctx *c) {
   HCHECK(hipMalloc(&d->sorc, sizeof(int) * (size_t)d->nblk_cells * 2));
   HCHECK(hipMalloc(&d->pstellar, sizeof(double) * (size_t)d->nblk_cells));
   HCHECK(hipMalloc(&d->mresp, sizeof(double) * (size_t)d->nblk_cells));
-  // geometry fields
-  HCHECK(hipMemcpyAsync(d->b[B_ABS], c->h_abs, sizeof(double) * d->nf, hipMemcpyHostToDevice, d->s));
-  HCHECK(hipMemcpyAsync(d->b[B_ORD], c->h_ord, sizeof(double) * d->nf, hipMemcpyHostToDevice, d->s));
+  // cos / sin of the sector angles
+  HCHECK(hipMalloc(&d->trig, sizeof(double) * 2 * (size_t)d->ns));
+  HCHECK(hipMemcpyAsync(d->trig, c->h_trig, sizeof(double) * 2 * (size_t)d->ns, hipMemcpyHostToDevice, d->s));
+  d->g.cosj = d->trig;
+  d->g.sinj = d->trig + d->ns;
   HCHECK(hipStreamSynchronize(d->s));
   return 0;
 }
@@ -2867,6 +2868,7 @@ extern "C" void thorin_dev_destroy(thorin_ctx *c) {
   if (d->hctl) (void)hipHostFree(d->hctl);
   if (d->red) (void)hipFree(d->red);
   if (d->sorp) (void)hipFree(d->sorp);
+  if (d->trig) (void)hipFree(d->trig);
   if (d->sorc) (void)hipFree(d->sorc);
   if (d->pstellar) (void)hipFree(d->pstellar);
   if (d->mresp) (void)hipFree(d->mresp);
@@ -3722,7 +3724,7 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
       memset(&P, 0, sizeof P);
       if (peb_accrete_host(c, sys, &P)) return -1;
       LAUNCH(k_peb_accrete, dim3(1), dim3(ACC_T), 0, d->g, P, d->b[B_PDENS], d->b[B_PVR], d->b[B_PVT], d->b[B_TAU],
-             d->b[B_CS], d->b[B_ABS], d->b[B_ORD], d->b[B_MACC], dt, d->ctl);
+             d->b[B_CS], d->b[B_MACC], dt, d->ctl);
       HCHECK(hipGetLastError());
       DCOPY(d->hctl, d->ctl, sizeof(DevCtl), hipMemcpyDeviceToHost);
       DSYNC();
@@ -3827,13 +3829,13 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
     pf.pagr = d->b[B_PAGR];
     pf.pagt = d->b[B_PAGT];
     if (peb)
-      LAUNCH(k_forces<true>, dim3(gall.x + (ntq ? 1u : 0u)), tb, 0, d->g, d->b[B_DENS], d->b[B_CS], d->b[B_ABS],
-             d->b[B_ORD], d->b[B_AGR], d->b[B_AGT], pks, sys->nb, mt, d->red, rr, 2 * ntq, d->ctl, pf);
+      LAUNCH(k_forces<true>, dim3(gall.x + (ntq ? 1u : 0u)), tb, 0, d->g, d->b[B_DENS], d->b[B_CS],
+             d->b[B_AGR], d->b[B_AGT], pks, sys->nb, mt, d->red, rr, 2 * ntq, d->ctl, pf);
     else
-      LAUNCH(k_forces<false>, dim3(gall.x + (ntq ? 1u : 0u)), tb, 0, d->g, d->b[B_DENS], d->b[B_CS], d->b[B_ABS],
-             d->b[B_ORD], d->b[B_AGR], d->b[B_AGT], pks, sys->nb, mt, d->red, rr, 2 * ntq, d->ctl, pf);
+      LAUNCH(k_forces<false>, dim3(gall.x + (ntq ? 1u : 0u)), tb, 0, d->g, d->b[B_DENS], d->b[B_CS],
+             d->b[B_AGR], d->b[B_AGT], pks, sys->nb, mt, d->red, rr, 2 * ntq, d->ctl, pf);
     LAUNCH(k_forces_reduce, dim3(2 + 2 * sys->nb), rb, 0, d->red, d->nblk_cells, d->ctl, BOff{0});
-    LAUNCH(k_indirect, gall, tb, 0, d->g, d->b[B_AGR], d->b[B_AGT], d->b[B_ABS], d->b[B_ORD], d->ctl, indir_px,
+    LAUNCH(k_indirect, gall, tb, 0, d->g, d->b[B_AGR], d->b[B_AGT], d->ctl, indir_px,
            indir_py, peb ? d->b[B_PAGR] : (double *)nullptr, peb ? d->b[B_PAGT] : (double *)nullptr);
     HCHECK(hipGetLastError());
     // ---- AdvanceSystemFromDisk (host; needs the reductions only when a planet feels the disk)

@@ -55,6 +55,7 @@ struct thorin_ctx {
   /* host staging copies used during initialisation (nfield doubles each) */
   double *h_dens, *h_vrad, *h_vtheta, *h_energy, *h_label, *h_cs, *h_press, *h_temper;
   double *h_abs, *h_ord;
+  double *h_trig;               /* cos(2 pi j/NSEC), then sin (InitComputeAccel's angles) */
   /* pebble fluid (allocated by thorin_init_pebbles): host copies of the
    * initial pebble fields and the damping targets PebDensInit, PebVradInit,
    * PebVthetaInit (global.h, Pebbles.c:190-193, 206-232) */
