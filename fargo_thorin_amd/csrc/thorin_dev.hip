@@ -1723,9 +1723,13 @@ __device__ __forceinline__ void b_substep3(Geo g, const double *__restrict__ vr,
     // stress tensor and Q+
     const Stress st = stress_cell(g, vr, vt, rho, i, j);
     divv[l] = st.dv;
-    Trr[l] = st.trr;
-    Tpp[l] = st.tpp;
-    if (st.has_trp) Trp[l] = st.trp;
+    // the tensor itself is not read again before the next substep's
+    // ViscousTerms forms it anew: stored only when asked (Trr non-null)
+    if (Trr) {
+      Trr[l] = st.trr;
+      Tpp[l] = st.tpp;
+      if (st.has_trp) Trp[l] = st.trp;
+    }
     double qp;
     if (i >= 1) {
       qp = qplus_of(g, i, rl, st);
@@ -3928,7 +3932,7 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
       const int nsum = (nr + 63) / 64;                   // ring-mean blocks, 64 rings each
       const int nsum8 = (nsum + 7) / 8 * 8;
       LAUNCH(k_substep3, dim3(gall.x + (unsigned)nsum8), tb, 0, d->g, d->b[B_VRAD], d->b[B_VTHETA], d->b[B_DENS],
-             d->b[B_EINT], d->b[B_DIVV], d->b[B_TAURR], d->b[B_TAURP], d->b[B_TAUPP], d->b[B_QPLUS],
+             d->b[B_EINT], d->b[B_DIVV], (double *)nullptr, (double *)nullptr, (double *)nullptr, d->b[B_QPLUS],
              d->b[B_TEMPER], d->b[B_CS], d->b[B_OPACITY], d->b[B_QMINUS], d->b[B_DIFF], d->pstellar,
              d->opa_filled ? 0 : 1, nsum8);
     }
