@@ -1559,7 +1559,7 @@ __device__ __forceinline__ void b_divstress(Geo g, const double *__restrict__ vr
   if (!xcd_cell(g, 0, g.nr, i, j, L_) || j >= ns) return;
   const size_t l = (size_t)i * ns + j;
   const Stress st = stress_cell(g, vr, vt, rho, i, j);
-  divv[l] = st.dv;
+  if (divv) divv[l] = st.dv;    // SubStep3 forms the divergence again before it is read
   Trr[l] = st.trr;
   Tpp[l] = st.tpp;
   if (st.has_trp) Trp[l] = st.trp;
@@ -3899,7 +3899,7 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
       LAUNCH(k_peb_substep1, gall, tb, 0, d->g, d->b[B_PVR], d->b[B_PVT], d->b[B_TAU], d->b[B_PACCR], d->b[B_PACCT],
              d->b[B_GACCR], d->b[B_GACCT], d->b[B_VRAD], d->b[B_VTHETA], d->b[B_DENS], d->b[B_PDENS], dt,
              p->diffusive_particles);
-    LAUNCH(k_divstress, gall, tb, 0, d->g, d->b[B_VRADINT], d->b[B_VTHETAINT], d->b[B_DENS], d->b[B_DIVV],
+    LAUNCH(k_divstress, gall, tb, 0, d->g, d->b[B_VRADINT], d->b[B_VTHETAINT], d->b[B_DENS], (double *)nullptr,
            d->b[B_TAURR], d->b[B_TAURP], d->b[B_TAUPP]);
     {
       const int kepler = (!p->damping_on) || (p->damping_on && p->damp_vrad);
