@@ -86,7 +86,8 @@ struct DevCtl {                 // device-side control block, mirrored to pinned
   double sor_normres0, sor_normres;
   double axstar, aystar;
   double axpl[MAXPL], aypl[MAXPL];
-  double tq_rho[16], tq_cs[16]; // torque log: ring means next to the logged planets
+  double tq_rho[24], tq_cs[24]; // ring means of rho and cs: the torque log's rings next to the logged
+                                // planets (2 each), then the ring of each planet that DampingTW04 damps
   int crash_peb;                // DetectCrashPebbles of this substep
   int acc_fail;                 // IntegrateColumnMass did not converge
   double pacc[PEB_MAXPL][3];    // pebble accretion per planet: dM, dPX, dPY
@@ -99,8 +100,8 @@ struct Mat32 {                  // FP32 matrix diagonals, residual and correctio
   float *e0, *e1;
 };
 
-struct TqRings {                // rings of the torque log's ring means (-1: none)
-  int r[16];
+struct TqRings {                // rings of the leading block's ring means (-1: none)
+  int r[24];
 };
 
 struct PlanetK {                // per-planet constants of FillForcesArrays (Pframeforce.c:93-121)
@@ -2719,10 +2720,9 @@ extern "C" int thorin_dev_create(thorin_ctx *c) {
                               (const void *)k_transport_theta<5>, (const void *)k_transport_theta<6>};
     for (int k = 0; k < TT_NPT; k++)
       HCHECK(hipFuncSetAttribute(tt[k], hipFuncAttributeMaxDynamicSharedMemorySize, (int)need));
-    // k_ring_means: as many whole rings as fit in 96 KB of LDS, at most 8
-    d->rs_rows = (int)((96 * 1024) / ((size_t)c->ns * sizeof(double)));
-    if (d->rs_rows > 8) d->rs_rows = 8;
-    if (d->rs_rows < 1) d->rs_rows = 1;
+    // k_ring_means: one ring per workgroup (the most workgroups staging and
+    // walking at once), two for the narrow rings of small grids
+    d->rs_rows = c->ns < 512 ? 2 : 1;
     d->sor_nit = 1;
     d->rad_rows = 16;
     const size_t rs_lds = sizeof(double) * (size_t)d->rs_rows * c->ns;
@@ -3827,17 +3827,33 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
         rr.r[2 * k + 1] = ii + 1 < nr ? ii + 1 : -1;
       }
     }
+    // DampingTW04 (Pframeforce.c:571-603) needs the ring means of rho and cs
+    // at the ring of each planet that feels the disk: formed by the same
+    // leading block, after the torque log's rings
+    int nrings = 2 * ntq, dslot[8];
+    for (int k = 0; k < 8; k++) dslot[k] = -1;
+    for (int k = 0; k < sys->nb && k < 8; k++) {
+      if (!sys->feeldisk[k] || !(p->verticaldamping > 0.0)) continue;
+      const double x = sys->x[k], y = sys->y[k];
+      const double r = sqrt(x * x + y * y);
+      if (r >= c->Rinf[0] && r <= c->Rsup[nr - 1]) {
+        int i = 0;
+        while (c->Rsup[i] < r) i++;
+        dslot[k] = nrings;
+        rr.r[nrings++] = i;
+      }
+    }
     PebF pf;
     pf.prho = d->b[B_PDENS];
     pf.tau = d->b[B_TAU];
     pf.pagr = d->b[B_PAGR];
     pf.pagt = d->b[B_PAGT];
     if (peb)
-      LAUNCH(k_forces<true>, dim3(gall.x + (ntq ? 1u : 0u)), tb, 0, d->g, d->b[B_DENS], d->b[B_CS],
-             d->b[B_AGR], d->b[B_AGT], pks, sys->nb, mt, d->red, rr, 2 * ntq, d->ctl, pf);
+      LAUNCH(k_forces<true>, dim3(gall.x + (nrings ? 1u : 0u)), tb, 0, d->g, d->b[B_DENS], d->b[B_CS],
+             d->b[B_AGR], d->b[B_AGT], pks, sys->nb, mt, d->red, rr, nrings, d->ctl, pf);
     else
-      LAUNCH(k_forces<false>, dim3(gall.x + (ntq ? 1u : 0u)), tb, 0, d->g, d->b[B_DENS], d->b[B_CS],
-             d->b[B_AGR], d->b[B_AGT], pks, sys->nb, mt, d->red, rr, 2 * ntq, d->ctl, pf);
+      LAUNCH(k_forces<false>, dim3(gall.x + (nrings ? 1u : 0u)), tb, 0, d->g, d->b[B_DENS], d->b[B_CS],
+             d->b[B_AGR], d->b[B_AGT], pks, sys->nb, mt, d->red, rr, nrings, d->ctl, pf);
     LAUNCH(k_forces_reduce, dim3(2 + 2 * sys->nb), rb, 0, d->red, d->nblk_cells, d->ctl, BOff{0});
     LAUNCH(k_indirect, gall, tb, 0, d->g, d->b[B_AGR], d->b[B_AGT], d->ctl, indir_px,
            indir_py, peb ? d->b[B_PAGR] : (double *)nullptr, peb ? d->b[B_PAGT] : (double *)nullptr);
@@ -3851,21 +3867,17 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
       for (int k = 0; k < sys->nb; k++) { sys->ax[k] = d->hctl->axpl[k]; sys->ay[k] = d->hctl->aypl[k]; }
     }
     if (anyfeel) {
-      std::vector<double> rmean(nr), csmean(nr);
-      if (thorin_ring_means(c, THORIN_F_DENS, rmean.data())) return -1;
-      DCOPY(csmean.data(), d->g.gcs, sizeof(double) * nr, hipMemcpyDeviceToHost);
-      DSYNC();
       const double indx = -d->hctl->axstar, indy = -d->hctl->aystar;
       for (int k = 0; k < sys->nb; k++) {
         if (!sys->feeldisk[k]) continue;
-        // DampingTW04 (Pframeforce.c:571-603)
+        // DampingTW04 (Pframeforce.c:571-603), with the ring means of the
+        // planet's ring from the force kernel's leading block
         double damp = 0.0, x = sys->x[k], y = sys->y[k], z = sys->z[k], m = sys->mass[k], vz = sys->vz[k];
         double r = sqrt(x * x + y * y);
-        if ((p->verticaldamping > 0.0) && (r >= c->Rinf[0] && r <= c->Rsup[nr - 1])) {
-          int i = 0;
-          while (c->Rsup[i] < r) i++;
-          double cs4inv = pow(csmean[i], -4.0), Om = pow(r, -1.5);
-          damp = p->verticaldamping * m * rmean[i] * Om * cs4inv * (-2.176 * vz - 0.871 * z * Om);
+        if (k < 8 && dslot[k] >= 0) {
+          const double rmean = d->hctl->tq_rho[dslot[k]], csmean = d->hctl->tq_cs[dslot[k]];
+          double cs4inv = pow(csmean, -4.0), Om = pow(r, -1.5);
+          damp = p->verticaldamping * m * rmean * Om * cs4inv * (-2.176 * vz - 0.871 * z * Om);
         }
         sys->vx[k] += dt * sys->ax[k];
         sys->vy[k] += dt * sys->ay[k];
