@@ -359,8 +359,19 @@ __global__ void k_wgrad_reduce(const float *__restrict__ part, int nblk, int cin
   for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
     const int tile = e >> 8, mn = e & 255, mi = mn >> 4, ni = mn & 15;
     const int nb = tile % NB, cb = (tile / NB) % CB, t = tile / (NB * CB);
+    // blocks added in order; eight loads in flight before their adds
+    const float *__restrict__ src = part + (size_t)tile * 256 + mn;
+    const size_t stride = (size_t)NT * 256;
     float s = 0.f;
-    for (int b = 0; b < nblk; b++) s += part[((size_t)b * NT + tile) * 256 + mn];
+    int b = 0;
+    for (; b + 8 <= nblk; b += 8) {
+      float v[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) v[u] = src[(size_t)(b + u) * stride];
+#pragma unroll
+      for (int u = 0; u < 8; u++) s += v[u];
+    }
+    for (; b < nblk; b++) s += src[(size_t)b * stride];
     dw[((size_t)t * cin + cb * 16 + mi) * cout + nb * 16 + ni] = s;
   }
 }
@@ -386,7 +397,15 @@ __global__ void __launch_bounds__(256) k_bgrad_reduce(const float *__restrict__ 
   __shared__ float red[256];
   const int c = threadIdx.x % C, k0 = threadIdx.x / C, lanes = 256 / C;
   float s = 0.f;
-  for (int b = k0; b < nblk; b += lanes) s += part[b * C + c];
+  int b = k0;
+  for (; b + 7 * lanes < nblk; b += 8 * lanes) {   // eight loads in flight, added in order
+    float v[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) v[u] = part[(b + u * lanes) * C + c];
+#pragma unroll
+    for (int u = 0; u < 8; u++) s += v[u];
+  }
+  for (; b < nblk; b += lanes) s += part[b * C + c];
   red[threadIdx.x] = s;
   __syncthreads();
   if (threadIdx.x < C) {
