@@ -158,27 +158,26 @@ __global__ void __launch_bounds__(256) k_conv(const bf16_t *__restrict__ x, cons
   }
 }
 
-// The same convolution with the input staged in LDS: a workgroup (4 waves)
-// covers CV_SEG consecutive pixels of one row; rows h-1..h+1 of the segment
-// plus one halo pixel on each side (periodic) are loaded once (8-byte
-// vectors) and every tap reads them from LDS.  The pixel stride is padded
-// (36 or 20 BF16) so that the 16 pixels of a fragment hit distinct banks.
-#define CV_SEG 256
+// The same convolution with the input staged in LDS, four output rows per
+// workgroup (one per wave): rows h0-1 .. h0+4 of a CV_SEG4-pixel segment plus
+// one halo pixel on each side (periodic) are loaded once (8-byte vectors, all
+// in flight before their LDS stores), so an input row is read 1.5 times, and
+// every tap reads them from LDS.  The pixel stride is padded (CIN + 4 BF16) so
+// that the 16 pixels of a fragment hit distinct banks.
+#define CV_SEG4 128
 template <int CIN, int COUT>
-__global__ void __launch_bounds__(256) k_conv_lds(const bf16_t *__restrict__ x, const bf16_t *__restrict__ wpk,
-                                                  const float *__restrict__ bias, void *__restrict__ out, int H, int W,
-                                                  int mode, const bf16_t *__restrict__ mask) {
-  constexpr int NB = COUT / 16, CB = CIN / 16, PS = CIN + (CIN == 32 ? 4 : 4), MT = CV_SEG / 64;
-  constexpr int NP = CV_SEG + 2;
-  __shared__ bf16_t sx[3 * NP * PS];
+__global__ void __launch_bounds__(256) k_conv_lds4(const bf16_t *__restrict__ x, const bf16_t *__restrict__ wpk,
+                                                   const float *__restrict__ bias, void *__restrict__ out, int H,
+                                                   int W, int mode, const bf16_t *__restrict__ mask) {
+  constexpr int NB = COUT / 16, CB = CIN / 16, PS = CIN + 4, MT = CV_SEG4 / 16;
+  constexpr int NP = CV_SEG4 + 2, NR = 6;
+  __shared__ bf16_t sx[NR * NP * PS];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int segs = (W + CV_SEG - 1) / CV_SEG;
-  const int h = blockIdx.x / segs, w0 = (blockIdx.x % segs) * CV_SEG;
-  if (h >= H) return;
+  const int segs = (W + CV_SEG4 - 1) / CV_SEG4;
+  const int h0 = (blockIdx.x / segs) * 4, w0 = (blockIdx.x % segs) * CV_SEG4;
+  if (h0 >= H) return;
   {
-    // the strip's loads are all issued before their LDS stores, so that they
-    // are in flight together (one memory latency per strip, not per element)
-    constexpr int NE = 3 * NP * (CIN / 4), NL = (NE + 255) / 256;
+    constexpr int NE = NR * NP * (CIN / 4), NL = (NE + 255) / 256;
     s4 v[NL];
 #pragma unroll
     for (int k = 0; k < NL; k++) {
@@ -187,8 +186,8 @@ __global__ void __launch_bounds__(256) k_conv_lds(const bf16_t *__restrict__ x, 
       if (e < NE) {
         const int c4 = e % (CIN / 4), rest = e / (CIN / 4);
         const int p = rest % NP, r = rest / NP;
-        const int hh = h + r - 1;
-        int ww = w0 + p - 1;                   // periodic in the sectors
+        const int hh = h0 + r - 1;
+        int ww = w0 + p - 1;
         while (ww < 0) ww += W;
         while (ww >= W) ww -= W;
         if (hh >= 0 && hh < H) v[k] = *(const s4 *)(x + ((size_t)hh * W + ww) * CIN + c4 * 4);
@@ -205,6 +204,8 @@ __global__ void __launch_bounds__(256) k_conv_lds(const bf16_t *__restrict__ x, 
     }
   }
   __syncthreads();
+  const int h = h0 + wv;
+  if (h >= H) return;
   const int m = lane & 15, kq = lane >> 4;
   f4 acc[MT][NB];
 #pragma unroll
@@ -212,10 +213,9 @@ __global__ void __launch_bounds__(256) k_conv_lds(const bf16_t *__restrict__ x, 
 #pragma unroll
     for (int nb = 0; nb < NB; nb++) acc[u][nb] = (f4){0.f, 0.f, 0.f, 0.f};
   const s4 *__restrict__ wp = (const s4 *)wpk;
-  const int pbase = wv * 64 + m;          // pixel of this lane in tile u: pbase + 16u
 #pragma unroll
   for (int t = 0; t < 9; t++) {
-    const int r = t / 3, dw = t % 3;      // sx pixel of output pixel p for tap dw: p + dw
+    const int r = wv + t / 3, dw = t % 3;   // staged row of the wave's tap row; sx pixel p + dw
 #pragma unroll
     for (int cb = 0; cb < CB; cb++) {
       s4 wf[NB];
@@ -223,7 +223,7 @@ __global__ void __launch_bounds__(256) k_conv_lds(const bf16_t *__restrict__ x, 
       for (int nb = 0; nb < NB; nb++) wf[nb] = wp[((t * CB + cb) * NB + nb) * 64 + lane];
 #pragma unroll
       for (int u = 0; u < MT; u++) {
-        const s4 xf = *(const s4 *)&sx[(r * NP + pbase + 16 * u + dw) * PS + cb * 16 + kq * 4];
+        const s4 xf = *(const s4 *)&sx[(r * NP + m + 16 * u + dw) * PS + cb * 16 + kq * 4];
 #pragma unroll
         for (int nb = 0; nb < NB; nb++)
           acc[u][nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wf[nb], xf, acc[u][nb], 0, 0, 0);
@@ -232,30 +232,30 @@ __global__ void __launch_bounds__(256) k_conv_lds(const bf16_t *__restrict__ x, 
   }
 #pragma unroll
   for (int u = 0; u < MT; u++) {
-    const int px = w0 + pbase + 16 * u;
+    const int px = w0 + m + 16 * u;
     if (px >= W) continue;
 #pragma unroll
     for (int nb = 0; nb < NB; nb++) {
       const int co = nb * 16 + 4 * kq;
       const size_t o = ((size_t)h * W + px) * COUT + co;
-      float v[4];
+      float vv[4];
 #pragma unroll
       for (int i = 0; i < 4; i++) {
-        v[i] = acc[u][nb][i] + (bias ? bias[co + i] : 0.f);
-        if (mode & 1) v[i] = v[i] > 0.f ? v[i] : 0.f;
+        vv[i] = acc[u][nb][i] + (bias ? bias[co + i] : 0.f);
+        if (mode & 1) vv[i] = vv[i] > 0.f ? vv[i] : 0.f;
       }
       if (mode & 4) {
         const s4 mk = *(const s4 *)(mask + o);
 #pragma unroll
         for (int i = 0; i < 4; i++)
-          if (!(bf2f(mk[i]) > 0.f)) v[i] = 0.f;
+          if (!(bf2f(mk[i]) > 0.f)) vv[i] = 0.f;
       }
       if (mode & 2) {
-        *(f4 *)((float *)out + o) = (f4){v[0], v[1], v[2], v[3]};
+        *(f4 *)((float *)out + o) = (f4){vv[0], vv[1], vv[2], vv[3]};
       } else {
         s4 rr;
 #pragma unroll
-        for (int i = 0; i < 4; i++) rr[i] = (short)f2bf(v[i]);
+        for (int i = 0; i < 4; i++) rr[i] = (short)f2bf(vv[i]);
         *(s4 *)((bf16_t *)out + o) = rr;
       }
     }
@@ -844,15 +844,15 @@ static int pack_all(thorin_cnn *n) {
 static const int CONV_MT = 4;      // 16-pixel tiles per wave
 static int conv(thorin_cnn *n, int cin, int cout, const bf16_t *x, const bf16_t *wpk, const float *bias, void *out,
                 int H, int W, int mode, const bf16_t *mask) {
-  if (W >= CV_SEG / 2) {
-    // rows wide enough to fill most of a 256-pixel segment: LDS-staged input
-    const dim3 g((unsigned)(H * ((W + CV_SEG - 1) / CV_SEG))), b(256);
+  if (W >= CV_SEG4) {
+    // rows of at least one 128-pixel segment: LDS-staged input, four rows per workgroup
+    const dim3 g((unsigned)(((H + 3) / 4) * ((W + CV_SEG4 - 1) / CV_SEG4))), b(256);
     if (cin == 16 && cout == 32)
-      hipLaunchKernelGGL((k_conv_lds<16, 32>), g, b, 0, n->s, x, wpk, bias, out, H, W, mode, mask);
+      hipLaunchKernelGGL((k_conv_lds4<16, 32>), g, b, 0, n->s, x, wpk, bias, out, H, W, mode, mask);
     else if (cin == 32 && cout == 32)
-      hipLaunchKernelGGL((k_conv_lds<32, 32>), g, b, 0, n->s, x, wpk, bias, out, H, W, mode, mask);
+      hipLaunchKernelGGL((k_conv_lds4<32, 32>), g, b, 0, n->s, x, wpk, bias, out, H, W, mode, mask);
     else if (cin == 32 && cout == 16)
-      hipLaunchKernelGGL((k_conv_lds<32, 16>), g, b, 0, n->s, x, wpk, bias, out, H, W, mode, mask);
+      hipLaunchKernelGGL((k_conv_lds4<32, 16>), g, b, 0, n->s, x, wpk, bias, out, H, W, mode, mask);
     else {
       thorin_set_error("no convolution instance for these channels");
       return -1;

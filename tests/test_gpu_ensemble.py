@@ -151,3 +151,28 @@ def test_launch_shapes_leave_results_unchanged():
     logs += [_log(d.algogas(p, nb)) for _ in range(3)]
     assert logs == logs_r
     _compare(ref, d, FIELDS)
+
+
+def test_ensemble_disks_with_different_substep_counts():
+    """A disk whose DT needs several CFL substeps next to disks that need one:
+    the merged queues drift apart and rejoin (a disk that has finished its DT
+    drops out of the merged launches), and every disk stays bit-identical."""
+    _print_case("different substep counts")
+    mem = _members([48, 64], 3)
+    mem[1][0]["DT"] = "3.0"                        # several substeps per DT
+    solo = [_make(c, dk) for c, dk in mem]
+    ens_disks = [_make(c, dk) for c, dk in mem]
+    ens = Ensemble(ens_disks)
+    nb = host_nbody_fn()
+    ps = [planets_for(configs.PLANET_ZERO_MASS) for _ in mem]
+    pe = [planets_for(configs.PLANET_ZERO_MASS) for _ in mem]
+    nsubs = set()
+    for step in range(3):
+        logs_s = [_log(d.algogas(p, nb)) for d, p in zip(solo, ps)]
+        logs_e = [_log(lg) for lg in ens.algogas(pe, nb)]
+        assert logs_s == logs_e, f"step {step}"
+        nsubs |= {lg[0] for lg in logs_e}
+    assert len(nsubs) > 1, nsubs
+    for a, b in zip(solo, ens_disks):
+        _compare(a, b, FIELDS)
+    ens.close()
