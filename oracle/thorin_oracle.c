@@ -18,8 +18,8 @@
  *   - closed forms / invariants: opacity laws vs independent restatements,
  *     flux limiter, SOR solution vs a direct sparse solve, mass conservation
  *     of transport, FARGO shift as an exact rotation.
- * Functions whose branches no golden run exercises are listed as
- * "parity unpinned" in DESIGN.md (e.g. stellar irradiation, OpenInner).
+ * Branches that no golden run exercises (stellar irradiation, OpenInner and
+ * the other options the library refuses) are "parity unpinned".
  */
 #define _GNU_SOURCE
 #include "thorin_oracle.h"

@@ -8,7 +8,7 @@
  * product (fargo_thorin_amd/libthorin_amd.so) never links or calls it.
  *
  * Parity status of each function is stated in oracle/thorin_oracle.c next to
- * the function and summarised in DESIGN.md.
+ * the function.
  */
 #ifndef THORIN_ORACLE_H
 #define THORIN_ORACLE_H
