@@ -151,6 +151,23 @@ struct Geo {                    // passed by value to every kernel
   double *pvmed;                                 // ordered ring mean of the pebble vtheta
 };
 
+// FillForcesArrays' launch: what it depends on besides the fields (the
+// planets, MassTaper, the torque log's rings) and what the host needs of it
+// afterwards.  A gas-only substep enqueues the next substep's force kernels
+// behind its own end (the fields they read are final there), so that they
+// run while the host makes its part of the next substep; the next substep
+// uses them if its inputs are those (same kernels, same inputs: same bits).
+struct ForceSet {
+  int valid;
+  int nb, want_tq;
+  double mt, t;
+  double x[8], y[8], z[8], m[8];
+  int feel[8];
+  int ntq, dslot[8];
+  double tq_t[8], tq_x[8], tq_y[8], tq_m[8], tq_frac[8];
+  double indir_px, indir_py;
+};
+
 struct thorin_dev {
   hipStream_t s;
   int nr, ns;
@@ -177,6 +194,8 @@ struct thorin_dev {
   int rs_rows;                  // rings per workgroup of k_ring_means (LDS holds them whole)
   int sor_nit;                  // SOR iterations per launch (1 or 2; thorin_tune)
   int sor_queue_min;            // ensemble: SOR iterations to queue at least
+  ForceSet spec;                // force kernels enqueued ahead for the next substep
+  hipEvent_t ev_end;            // end of a substep's own work (the host waits for it)
   int rad_rows;                 // rings per tile of the radial transport (8, 16, 32; thorin_tune)
   int rad_ready;                // thorin_rad_system has assembled the energy system
   struct Mat32 m32;             // FP32 copies for the mixed-precision solve (allocated on first use)
@@ -720,8 +739,10 @@ __device__ __forceinline__ void b_forces(Geo g, const double *__restrict__ rho,
                                                double *__restrict__ agr, double *__restrict__ agt,
                                                const PlanetsK &pk, int npl, double mt,
                                                double *__restrict__ partial, TqRings tq, int ntq_rings,
-                                               DevCtl *__restrict__ ctl, PebF pf) {
+                                               DevCtl *__restrict__ ctl, PebF pf,
+                                              const int *__restrict__ guard) {
   __shared__ double red[TB];
+  if (guard && *guard == 0) return;
   const int ns = g.ns;
   const int lead = ntq_rings > 0 ? 1 : 0;
   if ((int)blockIdx.x - g.blk0 < lead) {
@@ -978,14 +999,17 @@ __global__ void __launch_bounds__(TB) k_forces(Geo g, const double *__restrict__
                                                double *__restrict__ agr, double *__restrict__ agt,
                                                const PlanetsK pk, int npl, double mt,
                                                double *__restrict__ partial, TqRings tq, int ntq_rings,
-                                               DevCtl *__restrict__ ctl, PebF pf) {
-  b_forces<PEB>(g, rho, cs, agr, agt, pk, npl, mt, partial, tq, ntq_rings, ctl, pf);
+                                               DevCtl *__restrict__ ctl, PebF pf,
+                                               const int *__restrict__ guard) {
+  b_forces<PEB>(g, rho, cs, agr, agt, pk, npl, mt, partial, tq, ntq_rings, ctl, pf, guard);
 }
 
 
 // reduce the partial sums in block order; one block per quantity
-__device__ __forceinline__ void b_forces_reduce(const double *__restrict__ partial, int nb, DevCtl *ctl, BOff bo) {
+__device__ __forceinline__ void b_forces_reduce(const double *__restrict__ partial, int nb, DevCtl *ctl, BOff bo,
+                                                const int *__restrict__ guard) {
   __shared__ double red[RB];
+  if (guard && *guard == 0) return;
   const int q = (int)blockIdx.x - bo.blk0;
   double s = 0.0;
   s = strided_sum<RB>(partial + (size_t)q * nb, nb);
@@ -998,17 +1022,20 @@ __device__ __forceinline__ void b_forces_reduce(const double *__restrict__ parti
     else ctl->aypl[(q - 3) / 2] = s;
   }
 }
-__global__ void __launch_bounds__(RB) k_forces_reduce(const double *__restrict__ partial, int nb, DevCtl *ctl, BOff bo) {
-  b_forces_reduce(partial, nb, ctl, bo);
+__global__ void __launch_bounds__(RB) k_forces_reduce(const double *__restrict__ partial, int nb, DevCtl *ctl, BOff bo,
+                                                      const int *__restrict__ guard) {
+  b_forces_reduce(partial, nb, ctl, bo, guard);
 }
 
 
 // indirect term (Pframeforce.c:488-509)
 __device__ __forceinline__ void b_indirect(Geo g, double *__restrict__ agr, double *__restrict__ agt,
                                                                                                   const DevCtl *__restrict__ ctl, double indir_px, double indir_py,
-                                                 double *__restrict__ pagr, double *__restrict__ pagt) {
+                                                 double *__restrict__ pagr, double *__restrict__ pagt,
+                                              const int *__restrict__ guard) {
   const int ns = g.ns;
   int i, j, L_;
+  if (guard && *guard == 0) return;
   if (!xcd_cell(g, 0, g.nr, i, j, L_) || j >= ns) return;
   double axindir = -ctl->axstar, ayindir = -ctl->aystar;
   if (g.indirect_term) { axindir += indir_px; ayindir += indir_py; }
@@ -1025,8 +1052,9 @@ __device__ __forceinline__ void b_indirect(Geo g, double *__restrict__ agr, doub
 }
 __global__ void __launch_bounds__(TB) k_indirect(Geo g, double *__restrict__ agr, double *__restrict__ agt,
                                                                                                   const DevCtl *__restrict__ ctl, double indir_px, double indir_py,
-                                                 double *__restrict__ pagr, double *__restrict__ pagt) {
-  b_indirect(g, agr, agt, ctl, indir_px, indir_py, pagr, pagt);
+                                                 double *__restrict__ pagr, double *__restrict__ pagt,
+                                                 const int *__restrict__ guard) {
+  b_indirect(g, agr, agt, ctl, indir_px, indir_py, pagr, pagt, guard);
 }
 
 
@@ -2734,6 +2762,7 @@ extern "C" int thorin_dev_create(thorin_ctx *c) {
     HCHECK(hipFuncSetAttribute((const void *)k_peb_cfl, hipFuncAttributeMaxDynamicSharedMemorySize, (int)need));
   }
   HCHECK(hipStreamCreateWithFlags(&d->s, hipStreamNonBlocking));
+  HCHECK(hipEventCreateWithFlags(&d->ev_end, hipEventDisableTiming));
   d->prof_pend = new std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>>();
   d->prof_pool = new std::vector<hipEvent_t>();
   const size_t fb = (d->nf + 64) * sizeof(double);
@@ -2890,6 +2919,7 @@ extern "C" void thorin_dev_destroy(thorin_ctx *c) {
     for (auto e : *d->prof_pool) (void)hipEventDestroy(e);
     delete d->prof_pool;
   }
+  if (d->ev_end) (void)hipEventDestroy(d->ev_end);
   if (d->s) (void)hipStreamDestroy(d->s);
   free(d);
   c->dev = NULL;
@@ -2922,6 +2952,7 @@ extern "C" int thorin_dev_upload_init(thorin_ctx *c) {
   HCHECK(hipMemcpyAsync(d->b[B_PRESSURE], c->h_press, nb, hipMemcpyHostToDevice, d->s));
   HCHECK(hipStreamSynchronize(d->s));
   d->cfl_ready = 0;
+  d->spec.valid = 0;
   d->opa_filled = 0;
   return 0;
 }
@@ -2942,6 +2973,7 @@ extern "C" int thorin_dev_upload_pebbles(thorin_ctx *c) {
   HCHECK(hipMemcpyAsync((double *)d->g.peb_vt0, c->PebVthetaInit, sizeof(double) * n1, hipMemcpyHostToDevice, d->s));
   HCHECK(hipStreamSynchronize(d->s));
   d->cfl_ready = 0;
+  d->spec.valid = 0;
   return 0;
 }
 
@@ -2952,6 +2984,7 @@ extern "C" int thorin_upload_field(thorin_ctx *c, int field, const double *host,
   HCHECK(hipMemcpyAsync(d->b[field], host, sizeof(double) * count, hipMemcpyHostToDevice, d->s));
   HCHECK(hipStreamSynchronize(d->s));
   d->cfl_ready = 0;
+  d->spec.valid = 0;
   return 0;
 }
 
@@ -3665,6 +3698,146 @@ static int transport(thorin_dev *d, const Geo &g, const RadQ &q, int brho, int b
   return 0;
 }
 
+
+// FillForcesArrays (Pframeforce.c:43-510) for the planets sys at MassTaper mt:
+// planet constants, the torque log's rings (want_tq: UpdateLog, Force.c:27-70,
+// at the first substep of a DT) and DampingTW04's rings, then k_forces,
+// k_forces_reduce, k_indirect.  guard: as the post-SOR kernels.
+static bool force_set_matches(const ForceSet &f, const thorin_planets *sys, double mt, int want_tq, double t) {
+  if (f.nb != sys->nb || f.want_tq != want_tq || f.mt != mt || f.t != t || sys->nb > 8) return false;
+  for (int k = 0; k < sys->nb; k++)
+    if (f.x[k] != sys->x[k] || f.y[k] != sys->y[k] || f.z[k] != sys->z[k] || f.m[k] != sys->mass[k] ||
+        f.feel[k] != sys->feeldisk[k])
+      return false;
+  return true;
+}
+
+static int launch_forces(thorin_ctx *c, const thorin_planets *sys, double mt, int want_tq, const int *guard,
+                         ForceSet *fs) {
+  thorin_dev *d = c->dev;
+  const thorin_params *p = &c->p;
+  const int nr = d->nr;
+  const int peb = p->pebbles;
+  const dim3 tb(TB), rb(RB);
+  const dim3 gall = cell_grid(d, nr);
+  memset(fs, 0, sizeof *fs);
+  fs->nb = sys->nb;
+  fs->want_tq = want_tq;
+  fs->mt = mt;
+  fs->t = c->st.physical_time;
+  for (int k = 0; k < sys->nb && k < 8; k++) {
+    fs->x[k] = sys->x[k];
+    fs->y[k] = sys->y[k];
+    fs->z[k] = sys->z[k];
+    fs->m[k] = sys->mass[k];
+    fs->feel[k] = sys->feeldisk[k];
+  }
+  double indir_px = 0.0, indir_py = 0.0;
+  PlanetsK pks;
+  memset(&pks, 0, sizeof pks);
+  {
+    double axs = 0.0, ays = 0.0;
+    for (int k = 0; k < sys->nb; k++) {
+      PlanetK &P = pks.p[k];
+      const double xpl = sys->x[k], ypl = sys->y[k], zpl = sys->z[k];
+      double mpl = sys->mass[k];
+      const double dpl = sqrt(xpl * xpl + ypl * ypl + zpl * zpl);
+      P.x = xpl; P.y = ypl; P.z = zpl; P.m = mpl;
+      P.rhill = dpl * pow((1.0 / 3.0 * mpl), 1.0 / 3.0);
+      const double rst = 0.5 * P.rhill;
+      P.rst2 = rst * rst;
+      P.rst3 = P.rst2 * rst;
+      P.rst4 = P.rst3 * rst;
+      const double rsm = 0.05 * P.rhill;
+      P.rsm2 = rsm * rsm;
+      P.rmorevert2 = 2.0 * P.rhill;
+      P.rmorevert2 *= P.rmorevert2;
+      // ring of ThicknessSmoothing (Force.c:85-90)
+      const double r = sqrt(xpl * xpl + ypl * ypl);
+      P.iring = -1;
+      if (r >= c->Rinf[0] && r <= c->Rsup[nr - 1]) {
+        int i = 0;
+        while (c->Rsup[i] < r) i++;
+        P.iring = i;
+      }
+      if (p->indirect_term) {
+        const double invdpl3 = 1.0 / (dpl * dpl * dpl);
+        mpl *= mt;
+        axs += mpl * invdpl3 * xpl;
+        ays += mpl * invdpl3 * ypl;
+      }
+    }
+    indir_px = -axs;
+    indir_py = -ays;
+  }
+  // UpdateLog (Force.c:27-70, called from AdvanceSystemFromDisk) at the first
+  // substep of the DT: the planet side is taken now, the two ring means per
+  // planet are formed by the force kernel's leading block and the rows are
+  // completed after the end-of-substep sync
+  int ntq = 0;
+  TqRings rr;
+  if (want_tq) {
+    ntq = sys->nb < 8 ? sys->nb : 8;
+    for (int k = 0; k < ntq; k++) {
+      const double x = sys->x[k], y = sys->y[k];
+      const double r = sqrt(x * x + y * y);
+      double ifrac;   // GetGlobalIFrac (LowTasks.c:21-31)
+      if (r < c->GlobalRmed[0]) ifrac = 0.0;
+      else if (r > c->GlobalRmed[nr - 1]) ifrac = (double)nr - 1.0;
+      else {
+        int i = 0;
+        while (c->GlobalRmed[i] <= r) i++;
+        ifrac = (double)i + (r - c->GlobalRmed[i - 1]) / (c->GlobalRmed[i] - c->GlobalRmed[i - 1]) - 1.0;
+      }
+      const int ii = (int)ifrac;
+      fs->tq_t[k] = c->st.physical_time;
+      fs->tq_x[k] = x;
+      fs->tq_y[k] = y;
+      fs->tq_m[k] = sys->mass[k];
+      fs->tq_frac[k] = ifrac - floor(ifrac);
+      rr.r[2 * k] = ii;
+      rr.r[2 * k + 1] = ii + 1 < nr ? ii + 1 : -1;
+    }
+  }
+  // DampingTW04 (Pframeforce.c:571-603) needs the ring means of rho and cs
+  // at the ring of each planet that feels the disk: formed by the same
+  // leading block, after the torque log's rings
+  int nrings = 2 * ntq, dslot[8];
+  for (int k = 0; k < 8; k++) dslot[k] = -1;
+  for (int k = 0; k < sys->nb && k < 8; k++) {
+    if (!sys->feeldisk[k] || !(p->verticaldamping > 0.0)) continue;
+    const double x = sys->x[k], y = sys->y[k];
+    const double r = sqrt(x * x + y * y);
+    if (r >= c->Rinf[0] && r <= c->Rsup[nr - 1]) {
+      int i = 0;
+      while (c->Rsup[i] < r) i++;
+      dslot[k] = nrings;
+      rr.r[nrings++] = i;
+    }
+  }
+  PebF pf;
+  pf.prho = d->b[B_PDENS];
+  pf.tau = d->b[B_TAU];
+  pf.pagr = d->b[B_PAGR];
+  pf.pagt = d->b[B_PAGT];
+  if (peb)
+    LAUNCH(k_forces<true>, dim3(gall.x + (nrings ? 1u : 0u)), tb, 0, d->g, d->b[B_DENS], d->b[B_CS],
+           d->b[B_AGR], d->b[B_AGT], pks, sys->nb, mt, d->red, rr, nrings, d->ctl, pf, guard);
+  else
+    LAUNCH(k_forces<false>, dim3(gall.x + (nrings ? 1u : 0u)), tb, 0, d->g, d->b[B_DENS], d->b[B_CS],
+           d->b[B_AGR], d->b[B_AGT], pks, sys->nb, mt, d->red, rr, nrings, d->ctl, pf, guard);
+  LAUNCH(k_forces_reduce, dim3(2 + 2 * sys->nb), rb, 0, d->red, d->nblk_cells, d->ctl, BOff{0}, guard);
+  LAUNCH(k_indirect, gall, tb, 0, d->g, d->b[B_AGR], d->b[B_AGT], d->ctl, indir_px,
+         indir_py, peb ? d->b[B_PAGR] : (double *)nullptr, peb ? d->b[B_PAGT] : (double *)nullptr, guard);
+  HCHECK(hipGetLastError());
+  fs->ntq = ntq;
+  for (int k = 0; k < 8; k++) fs->dslot[k] = dslot[k];
+  fs->indir_px = indir_px;
+  fs->indir_py = indir_py;
+  fs->valid = 1;
+  return 0;
+}
+
 extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_fn nbody, void *nbody_user,
                               thorin_step_log *log) {
   if (need_dev(c)) return -1;
@@ -3758,106 +3931,16 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
     }
     dtemp += dt;
     // ---- FillForcesArrays (the planets' constants go to the kernel by value)
-    double indir_px = 0.0, indir_py = 0.0;
-    PlanetsK pks;
-    memset(&pks, 0, sizeof pks);
-    {
-      double axs = 0.0, ays = 0.0;
-      for (int k = 0; k < sys->nb; k++) {
-        PlanetK &P = pks.p[k];
-        const double xpl = sys->x[k], ypl = sys->y[k], zpl = sys->z[k];
-        double mpl = sys->mass[k];
-        const double dpl = sqrt(xpl * xpl + ypl * ypl + zpl * zpl);
-        P.x = xpl; P.y = ypl; P.z = zpl; P.m = mpl;
-        P.rhill = dpl * pow((1.0 / 3.0 * mpl), 1.0 / 3.0);
-        const double rst = 0.5 * P.rhill;
-        P.rst2 = rst * rst;
-        P.rst3 = P.rst2 * rst;
-        P.rst4 = P.rst3 * rst;
-        const double rsm = 0.05 * P.rhill;
-        P.rsm2 = rsm * rsm;
-        P.rmorevert2 = 2.0 * P.rhill;
-        P.rmorevert2 *= P.rmorevert2;
-        // ring of ThicknessSmoothing (Force.c:85-90)
-        const double r = sqrt(xpl * xpl + ypl * ypl);
-        P.iring = -1;
-        if (r >= c->Rinf[0] && r <= c->Rsup[nr - 1]) {
-          int i = 0;
-          while (c->Rsup[i] < r) i++;
-          P.iring = i;
-        }
-        if (p->indirect_term) {
-          const double invdpl3 = 1.0 / (dpl * dpl * dpl);
-          mpl *= mt;
-          axs += mpl * invdpl3 * xpl;
-          ays += mpl * invdpl3 * ypl;
-        }
-      }
-      indir_px = -axs;
-      indir_py = -ays;
-    }
     const int want_tq = p->write_torque && log && log->nsub == 0 && sys->nb > 0;
-    // UpdateLog (Force.c:27-70, called from AdvanceSystemFromDisk) at the first
-    // substep of the DT: the planet side is taken now, the two ring means per
-    // planet are formed by the force kernel's leading block and the rows are
-    // completed after the end-of-substep sync
-    struct { double t, x, y, m, frac; } tqp[8];
-    int ntq = 0;
-    TqRings rr;
-    if (want_tq) {
-      ntq = sys->nb < 8 ? sys->nb : 8;
-      for (int k = 0; k < ntq; k++) {
-        const double x = sys->x[k], y = sys->y[k];
-        const double r = sqrt(x * x + y * y);
-        double ifrac;   // GetGlobalIFrac (LowTasks.c:21-31)
-        if (r < c->GlobalRmed[0]) ifrac = 0.0;
-        else if (r > c->GlobalRmed[nr - 1]) ifrac = (double)nr - 1.0;
-        else {
-          int i = 0;
-          while (c->GlobalRmed[i] <= r) i++;
-          ifrac = (double)i + (r - c->GlobalRmed[i - 1]) / (c->GlobalRmed[i] - c->GlobalRmed[i - 1]) - 1.0;
-        }
-        const int ii = (int)ifrac;
-        tqp[k].t = c->st.physical_time;
-        tqp[k].x = x;
-        tqp[k].y = y;
-        tqp[k].m = sys->mass[k];
-        tqp[k].frac = ifrac - floor(ifrac);
-        rr.r[2 * k] = ii;
-        rr.r[2 * k + 1] = ii + 1 < nr ? ii + 1 : -1;
-      }
+    ForceSet fset;
+    if (!peb && !d->rec && d->spec.valid && force_set_matches(d->spec, sys, mt, want_tq, c->st.physical_time)) {
+      fset = d->spec;           // enqueued at the end of the previous substep
+    } else {
+      if (launch_forces(c, sys, mt, want_tq, nullptr, &fset)) return -1;
     }
-    // DampingTW04 (Pframeforce.c:571-603) needs the ring means of rho and cs
-    // at the ring of each planet that feels the disk: formed by the same
-    // leading block, after the torque log's rings
-    int nrings = 2 * ntq, dslot[8];
-    for (int k = 0; k < 8; k++) dslot[k] = -1;
-    for (int k = 0; k < sys->nb && k < 8; k++) {
-      if (!sys->feeldisk[k] || !(p->verticaldamping > 0.0)) continue;
-      const double x = sys->x[k], y = sys->y[k];
-      const double r = sqrt(x * x + y * y);
-      if (r >= c->Rinf[0] && r <= c->Rsup[nr - 1]) {
-        int i = 0;
-        while (c->Rsup[i] < r) i++;
-        dslot[k] = nrings;
-        rr.r[nrings++] = i;
-      }
-    }
-    PebF pf;
-    pf.prho = d->b[B_PDENS];
-    pf.tau = d->b[B_TAU];
-    pf.pagr = d->b[B_PAGR];
-    pf.pagt = d->b[B_PAGT];
-    if (peb)
-      LAUNCH(k_forces<true>, dim3(gall.x + (nrings ? 1u : 0u)), tb, 0, d->g, d->b[B_DENS], d->b[B_CS],
-             d->b[B_AGR], d->b[B_AGT], pks, sys->nb, mt, d->red, rr, nrings, d->ctl, pf);
-    else
-      LAUNCH(k_forces<false>, dim3(gall.x + (nrings ? 1u : 0u)), tb, 0, d->g, d->b[B_DENS], d->b[B_CS],
-             d->b[B_AGR], d->b[B_AGT], pks, sys->nb, mt, d->red, rr, nrings, d->ctl, pf);
-    LAUNCH(k_forces_reduce, dim3(2 + 2 * sys->nb), rb, 0, d->red, d->nblk_cells, d->ctl, BOff{0});
-    LAUNCH(k_indirect, gall, tb, 0, d->g, d->b[B_AGR], d->b[B_AGT], d->ctl, indir_px,
-           indir_py, peb ? d->b[B_PAGR] : (double *)nullptr, peb ? d->b[B_PAGT] : (double *)nullptr);
-    HCHECK(hipGetLastError());
+    d->spec.valid = 0;
+    const ForceSet *fs = &fset;
+    const int ntq = fs->ntq;
     // ---- AdvanceSystemFromDisk (host; needs the reductions only when a planet feels the disk)
     int anyfeel = 0;
     for (int k = 0; k < sys->nb; k++) anyfeel |= sys->feeldisk[k];
@@ -3874,8 +3957,8 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
         // planet's ring from the force kernel's leading block
         double damp = 0.0, x = sys->x[k], y = sys->y[k], z = sys->z[k], m = sys->mass[k], vz = sys->vz[k];
         double r = sqrt(x * x + y * y);
-        if (k < 8 && dslot[k] >= 0) {
-          const double rmean = d->hctl->tq_rho[dslot[k]], csmean = d->hctl->tq_cs[dslot[k]];
+        if (k < 8 && fs->dslot[k] >= 0) {
+          const double rmean = d->hctl->tq_rho[fs->dslot[k]], csmean = d->hctl->tq_cs[fs->dslot[k]];
           double cs4inv = pow(csmean, -4.0), Om = pow(r, -1.5);
           damp = p->verticaldamping * m * rmean * Om * cs4inv * (-2.176 * vz - 0.871 * z * Om);
         }
@@ -4059,7 +4142,26 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
       LAUNCH(k_cfl_final, dim3(1), rb, 0, d->g, d->ctl, (const double *)d->pstellar, d->nblk_cells,
              d->cfl_invdtreb, guard, d->hctl);
       HCHECK(hipGetLastError());
-      DSYNC();
+      d->spec.valid = 0;
+      if (peb || d->rec) {
+        DSYNC();
+        return 0;
+      }
+      // The next substep's FillForcesArrays is enqueued behind the sync point
+      // and runs while the host reads the mirror and steps the planets: its
+      // inputs (the gas density and sound speed of now, the planets as they
+      // are now, MassTaper at the new time) are those the next substep would
+      // use unless the host changes the planets, which the next substep
+      // detects by comparing the planet set before it uses these forces.
+      HCHECK(hipEventRecord(d->ev_end, d->s));
+      {
+        double mtn = c->st.physical_time / (p->masstaper * 2.0 * M_PI);
+        mtn = (mtn > 1.0 ? 1.0 : pow(sin(mtn * M_PI / 2.0), 2.0));
+        const int tqn = !(dtemp < 0.999999999 * DT) && p->write_torque && sys->nb > 0;
+        if (launch_forces(c, sys, mtn, tqn, guard, &d->spec)) return -1;
+      }
+      HCHECK(hipEventSynchronize(d->ev_end));
+      if (!d->hctl->sor_done) d->spec.valid = 0;   // the guarded launches did nothing
       return 0;
     };
     if (after_sor()) return -1;
@@ -4087,7 +4189,7 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
     if (ntq) {
       log->tq_rows = ntq;
       for (int k = 0; k < ntq; k++) {
-        const double x = tqp[k].x, y = tqp[k].y, m = tqp[k].m, frac = tqp[k].frac;
+        const double x = fs->tq_x[k], y = fs->tq_y[k], m = fs->tq_m[k], frac = fs->tq_frac[k];
         const double ax = d->hctl->axpl[k], ay = d->hctl->aypl[k];
         const double r2 = x * x + y * y, r = sqrt(r2);
         const double rhopl = d->hctl->tq_rho[2 * k] * (1.0 - frac) + d->hctl->tq_rho[2 * k + 1] * frac;
@@ -4098,7 +4200,7 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
         norm *= norm;
         norm *= rhopl;
         const double tq = x * ay - y * ax;
-        const double row[7] = {tqp[k].t, tq, tq * m * p->adiabind / norm, x, y, ax, ay};
+        const double row[7] = {fs->tq_t[k], tq, tq * m * p->adiabind / norm, x, y, ax, ay};
         for (int q = 0; q < 7; q++) log->tq[k][q] = row[q];
       }
     }
@@ -4476,6 +4578,7 @@ static int tune_time(thorin_dev *d, const std::function<int()> &work, int reps, 
 extern "C" int thorin_autotune(thorin_ctx *c, thorin_tuning *out) {
   if (need_dev(c)) return -1;
   thorin_dev *d = c->dev;
+  d->spec.valid = 0;                     // forces enqueued for a next substep are redone
   if (d->rec) { thorin_set_error("thorin_autotune: the context is inside an ensemble step"); return -1; }
   const int nr = d->nr, ns = d->ns;
   const size_t nb = sizeof(double) * d->nf;
@@ -4831,6 +4934,7 @@ static int mp_residual(thorin_ctx *c, float *r32, double *out) {
 extern "C" int thorin_rad_system(thorin_ctx *c, double dt, double *dt_used) {
   if (need_dev(c)) return -1;
   thorin_dev *d = c->dev;
+  d->spec.valid = 0;                     // forces enqueued for a next substep are redone
   const int nr = d->nr, ns = d->ns;
   const size_t nb = sizeof(double) * d->nf;
   const dim3 gall = cell_grid(d, nr), tb(TB), rb(RB);
@@ -4869,6 +4973,7 @@ extern "C" int thorin_rad_system(thorin_ctx *c, double dt, double *dt_used) {
 extern "C" int thorin_dev_system(thorin_ctx *c, thorin_sysptrs *p) {
   if (need_dev(c)) return -1;
   thorin_dev *d = c->dev;
+  d->spec.valid = 0;                     // forces enqueued for a next substep are redone
   if (!d->rad_ready) { thorin_set_error("no energy system assembled (thorin_rad_system)"); return -1; }
   p->nr = d->nr;
   p->ns = d->ns;
@@ -4883,6 +4988,7 @@ extern "C" int thorin_dev_system(thorin_ctx *c, thorin_sysptrs *p) {
 extern "C" int thorin_rad_solve(thorin_ctx *c, const thorin_solve_opts *o, thorin_solve_result *res) {
   if (need_dev(c)) return -1;
   thorin_dev *d = c->dev;
+  d->spec.valid = 0;                     // forces enqueued for a next substep are redone
   if (!d->rad_ready) { thorin_set_error("thorin_rad_solve: no system assembled (thorin_rad_system)"); return -1; }
   if (!o || !res || o->tol <= 0.0 || o->omega <= 0.0 || o->omega >= 2.0) {
     thorin_set_error("thorin_rad_solve: need tol > 0 and 0 < omega < 2");

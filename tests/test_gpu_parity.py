@@ -106,6 +106,45 @@ def test_gpu_vs_oracle_stepwise(name, steps):
     assert d.state().physical_time == o.state().physical_time
 
 
+def _tq_row(lg):
+    return lg.tq_rows, [lg.tq[0][q] for q in range(7)]
+
+
+@pytest.mark.parametrize("feel", [False, True])
+def test_gpu_forces_enqueued_ahead_match_forces_enqueued_in_place(feel):
+    """The next substep's planet forces are enqueued behind each substep's
+    sync point, before the host has stepped the planets on, and used when
+    the planet set then matches.  A disk whose own density is re-uploaded
+    between DTs discards them (its forces are always enqueued in place); a
+    disk whose planet is nudged between DTs no longer matches them.  Equal
+    inputs give equal bits, step logs and torque rows either way."""
+    name = "planet_q3e-5_48x96"
+    _print_case("forces ahead", name, "feel", feel)
+    cfg, planets, _ = case(name)
+    g = golden(name)
+    nb = host_nbody_fn()
+    disks, systems = [], []
+    for _ in range(4):
+        d = Disk(params_for(cfg))
+        d.init(g["init_dens"], g["init_vrad"], g["init_vtheta"], g["init_temper"])
+        disks.append(d)
+        systems.append(planets_for(planets, {"PlanetsFeelDisk": "YES"} if feel else None))
+    for step in range(8):
+        logs = [d.algogas(s, nb) for d, s in zip(disks, systems)]
+        rows = [(_tq_row(lg), [lg.sor_niter[k] for k in range(lg.nsub)]) for lg in logs]
+        assert rows[0] == rows[1] and rows[2] == rows[3], f"step {step}"
+        disks[1].upload("dens", disks[1].field("dens"))
+        disks[3].upload("dens", disks[3].field("dens"))
+        for s in systems[2:]:
+            s.x[0] *= 1.0 + 1e-7
+        for a, b in ((disks[0], disks[1]), (disks[2], disks[3])):
+            for f in STATE_FIELDS:
+                assert np.array_equal(a.field(f), b.field(f)), f"step {step} {f}"
+        for a, b in ((systems[0], systems[1]), (systems[2], systems[3])):
+            assert (a.x[0], a.y[0], a.vx[0], a.vy[0]) == (b.x[0], b.y[0], b.vx[0], b.vy[0])
+    assert not np.array_equal(disks[0].field("dens"), disks[2].field("dens"))
+
+
 def test_gpu_mass_conservation_walls():
     """WALL boundaries and no damping: transport conserves the total mass."""
     _print_case("mass")
