@@ -428,6 +428,64 @@ __device__ __forceinline__ double ordered_row_sum_global(const double *__restric
   return s;
 }
 
+// The same sequential sum formed by one wave: chunks of RS_CHUNK values are
+// loaded coalesced (the next chunk's loads in flight while the current one
+// is added) and staged through the wave's LDS slice st (RS_CHUNK doubles),
+// from which lane 0 adds them in index order.  Every lane of the wave calls
+// it; the result is valid in lane 0.  One lane walking the ring through
+// global memory waits a memory latency per 16 values; here the wait is one
+// latency per chunk, hidden behind the additions.
+static const int RS_CH = 8;
+static const int RS_CHUNK = 64 * RS_CH;
+__device__ __forceinline__ double wave_ordered_row_sum(const double *__restrict__ row, int ns, double *st) {
+  const int lane = threadIdx.x & 63;
+  double v[RS_CH];
+#pragma unroll
+  for (int k = 0; k < RS_CH; k++) {
+    const int j = k * 64 + lane;
+    v[k] = j < ns ? row[j] : 0.0;
+  }
+  double s = 0.0;
+  for (int c0 = 0; c0 < ns; c0 += RS_CHUNK) {
+#pragma unroll
+    for (int k = 0; k < RS_CH; k++) st[k * 64 + lane] = v[k];
+#pragma unroll
+    for (int k = 0; k < RS_CH; k++) {
+      const int j = c0 + RS_CHUNK + k * 64 + lane;
+      v[k] = j < ns ? row[j] : 0.0;
+    }
+    // LDS operations of one wave complete in order: lane 0 reads what the
+    // other lanes wrote above
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (lane == 0) {
+      if (ns - c0 >= RS_CHUNK) {
+        // a full chunk: the next 16 values are read while 16 are added
+        double a[16], b[16];
+#pragma unroll
+        for (int k = 0; k < 16; k++) a[k] = st[k];
+        for (int q = 16; q < RS_CHUNK; q += 16) {
+#pragma unroll
+          for (int k = 0; k < 16; k++) b[k] = st[q + k];
+#pragma unroll
+          for (int k = 0; k < 16; k++) s += a[k];
+#pragma unroll
+          for (int k = 0; k < 16; k++) a[k] = b[k];
+        }
+#pragma unroll
+        for (int k = 0; k < 16; k++) s += a[k];
+      } else {
+        for (int q = 0; q < ns - c0; q++) s += st[q];
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+  return s;
+}
+
 // plain round-robin numbering (logical block = launch block) for kernels
 // whose cost varies strongly from ring to ring (the force kernel near a
 // planet), so that the expensive rings spread over all XCDs
@@ -747,18 +805,22 @@ __device__ __forceinline__ void b_forces(Geo g, const double *__restrict__ rho,
   const int lead = ntq_rings > 0 ? 1 : 0;
   if ((int)blockIdx.x - g.blk0 < lead) {
     // leading block: the torque log's ring means of rho next to the logged
-    // planets (UpdateLog, Force.c:41-52), lane b walking ring tq.r[b], and the
-    // sound-speed ring means of the same rings (globcsvec); ring -1 gives 0
-    const int b = threadIdx.x;
-    if (b < ntq_rings) {
+    // planets (UpdateLog, Force.c:41-52), wave w summing rings tq.r[w],
+    // tq.r[w + 4], ..., and the sound-speed ring means of the same rings
+    // (globcsvec); ring -1 gives 0
+    __shared__ double st[TB / 64][RS_CHUNK];
+    const int w = threadIdx.x >> 6;
+    for (int b = w; b < ntq_rings; b += TB / 64) {
       const int r = tq.r[b];
       double m = 0.0, c = 0.0;
       if (r >= 0) {
-        m = ordered_row_sum_global(rho + (size_t)r * ns, ns) / (double)ns;
+        m = wave_ordered_row_sum(rho + (size_t)r * ns, ns, st[w]) / (double)ns;
         c = g.gcs[r];
       }
-      ctl->tq_rho[b] = m;
-      ctl->tq_cs[b] = c;
+      if ((threadIdx.x & 63) == 0) {
+        ctl->tq_rho[b] = m;
+        ctl->tq_cs[b] = c;
+      }
     }
     return;
   }
@@ -1737,10 +1799,15 @@ __device__ __forceinline__ void b_substep3(Geo g, const double *__restrict__ vr,
   const int ns = g.ns, nr = g.nr;
   if ((int)blockIdx.x - g.blk0 < nsum) {
     // leading blocks: VMed of ComputeResiduals (TransportEuler.c:86-91) for
-    // the coming transport, 64 rings per block, lane r walking ring r; vt is
-    // final for the transport from here on
-    const int r = ((int)blockIdx.x - g.blk0) * 64 + threadIdx.x;
-    if (threadIdx.x < 64 && r < nr) g.VMed[r] = ordered_row_sum_global(vt + (size_t)r * ns, ns) / (double)ns;
+    // the coming transport, one ring per wave; vt is final for the transport
+    // from here on
+    __shared__ double st[TB / 64][RS_CHUNK];
+    const int w = threadIdx.x >> 6;
+    const int r = ((int)blockIdx.x - g.blk0) * (TB / 64) + w;
+    if (r < nr) {
+      const double m = wave_ordered_row_sum(vt + (size_t)r * ns, ns, st[w]) / (double)ns;
+      if ((threadIdx.x & 63) == 0) g.VMed[r] = m;
+    }
     return;
   }
   int i, j, L;
@@ -4024,7 +4091,7 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
     int niter = 0;
     double omega_used = 0.0;
     {
-      const int nsum = (nr + 63) / 64;                   // ring-mean blocks, 64 rings each
+      const int nsum = (nr + TB / 64 - 1) / (TB / 64);   // ring-mean blocks, one ring per wave
       const int nsum8 = (nsum + 7) / 8 * 8;
       LAUNCH(k_substep3, dim3(gall.x + (unsigned)nsum8), tb, 0, d->g, d->b[B_VRAD], d->b[B_VTHETA], d->b[B_DENS],
              d->b[B_EINT], d->b[B_DIVV], (double *)nullptr, (double *)nullptr, (double *)nullptr, d->b[B_QPLUS],
