@@ -223,6 +223,7 @@ static const char *const KNAMES[] = {
 
 // a template instance ("k_name<3>") counts as its kernel
 static int kid_of(const char *name) {
+  if (name[0] == '(') name++;                 // "(k_name<a, b>)": a macro argument with a comma
   const size_t n = strcspn(name, "<");
   for (int i = 0; KNAMES[i]; i++)
     if (strlen(KNAMES[i]) == n && !strncmp(KNAMES[i], name, n)) return i;
@@ -461,20 +462,20 @@ __device__ __forceinline__ double wave_ordered_row_sum(const double *__restrict_
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     if (lane == 0) {
       if (ns - c0 >= RS_CHUNK) {
-        // a full chunk: the next 16 values are read while 16 are added
-        double a[16], b[16];
+        // a full chunk: the next 8 values are read while 8 are added
+        double a[8], b[8];
 #pragma unroll
-        for (int k = 0; k < 16; k++) a[k] = st[k];
-        for (int q = 16; q < RS_CHUNK; q += 16) {
+        for (int k = 0; k < 8; k++) a[k] = st[k];
+        for (int q = 8; q < RS_CHUNK; q += 8) {
 #pragma unroll
-          for (int k = 0; k < 16; k++) b[k] = st[q + k];
+          for (int k = 0; k < 8; k++) b[k] = st[q + k];
 #pragma unroll
-          for (int k = 0; k < 16; k++) s += a[k];
+          for (int k = 0; k < 8; k++) s += a[k];
 #pragma unroll
-          for (int k = 0; k < 16; k++) a[k] = b[k];
+          for (int k = 0; k < 8; k++) a[k] = b[k];
         }
 #pragma unroll
-        for (int k = 0; k < 16; k++) s += a[k];
+        for (int k = 0; k < 8; k++) s += a[k];
       } else {
         for (int q = 0; q < ns - c0; q++) s += st[q];
       }
@@ -791,7 +792,7 @@ struct PebF {
   double *pagr, *pagt;
 };
 
-template <bool PEB>
+template <bool PEB, int NP>
 __device__ __forceinline__ void b_forces(Geo g, const double *__restrict__ rho,
                                                const double *__restrict__ cs,
                                                double *__restrict__ agr, double *__restrict__ agt,
@@ -828,8 +829,10 @@ __device__ __forceinline__ void b_forces(Geo g, const double *__restrict__ rho,
   if (!rr_cell(g, 0, g.nr, i, j, bid, lead)) return;
   const bool act = j < ns;
   double axs = 0.0, ays = 0.0;
-  double axpl_loc[8], aypl_loc[8];
-  for (int k = 0; k < 8; k++) { axpl_loc[k] = 0.0; aypl_loc[k] = 0.0; }
+  // NP: size of the per-planet arrays (1 for one planet: fewer registers)
+  const int np = npl < NP ? npl : NP;
+  double axpl_loc[NP], aypl_loc[NP];
+  for (int k = 0; k < NP; k++) { axpl_loc[k] = 0.0; aypl_loc[k] = 0.0; }
   if (act) {
     const size_t l = (size_t)i * ns + j;
     const double omegainv = g.OmegaInv[i];
@@ -857,8 +860,8 @@ __device__ __forceinline__ void b_forces(Geo g, const double *__restrict__ rho,
       if (fabs(pH - p_H_avr) / p_H_avr > 0.05) IntegrateLocally = 1;
     }
     unsigned int withpl = 0;    // bit k: IntegrateWithPlanet[k] (npl <= 8 on this path)
-    double s2[8], hillcut[8];
-    for (int k = 0; k < npl; k++) {
+    double s2[NP], hillcut[NP];
+    for (int k = 0; k < np; k++) {
       const PlanetK P = pk.p[k];
       double smoothing = 0.0;
       if (P.iring >= 0) smoothing = g.thicksmooth * (g.gcs[P.iring] * g.OmegaInv[P.iring] * g.sqrt_adiabind_inv);
@@ -877,8 +880,8 @@ __device__ __forceinline__ void b_forces(Geo g, const double *__restrict__ rho,
       }
       if (s2[k] < P.rmorevert2) Nvert = 30;
     }
-    double sigma, integstar, integpl[8], p_integstar = 0.0, p_integpl[8];
-    for (int k = 0; k < 8; k++) { integpl[k] = 0.0; p_integpl[k] = 0.0; }
+    double sigma, integstar, integpl[NP], p_integstar = 0.0, p_integpl[NP];
+    for (int k = 0; k < NP; k++) { integpl[k] = 0.0; p_integpl[k] = 0.0; }
     if (IntegrateLocally) {
       zmax = 3.0 * H;
       deltaz = zmax / (double)Nvert;
@@ -899,7 +902,7 @@ __device__ __forceinline__ void b_forces(Geo g, const double *__restrict__ rho,
           denom = d * d2;
           p_integstar += sum / denom;
         }
-        for (int k = 0; k < npl; k++) {
+        for (int k = 0; k < np; k++) {
           if (withpl & (1u << k)) {
             const PlanetK P = pk.p[k];
             double zpl = P.z, taper;
@@ -965,7 +968,7 @@ __device__ __forceinline__ void b_forces(Geo g, const double *__restrict__ rho,
       tmp = p_integstar / sigma;
       pagr_l = 0.0 + (-g.Rmed[i] * tmp);
     }
-    for (int k = 0; k < npl; k++) {
+    for (int k = 0; k < np; k++) {
       const PlanetK P = pk.p[k];
       double mpl = P.m;
       if (withpl & (1u << k)) {
@@ -1048,14 +1051,14 @@ __device__ __forceinline__ void b_forces(Geo g, const double *__restrict__ rho,
   if (threadIdx.x == 0) partial[bid] = v;
   v = block_sum<TB>(ays, red);
   if (threadIdx.x == 0) partial[nb + bid] = v;
-  for (int k = 0; k < npl; k++) {
+  for (int k = 0; k < np; k++) {
     v = block_sum<TB>(axpl_loc[k], red);
     if (threadIdx.x == 0) partial[(2 + 2 * k) * nb + bid] = v;
     v = block_sum<TB>(aypl_loc[k], red);
     if (threadIdx.x == 0) partial[(3 + 2 * k) * nb + bid] = v;
   }
 }
-template <bool PEB>
+template <bool PEB, int NP>
 __global__ void __launch_bounds__(TB) k_forces(Geo g, const double *__restrict__ rho,
                                                const double *__restrict__ cs,
                                                double *__restrict__ agr, double *__restrict__ agt,
@@ -1063,7 +1066,7 @@ __global__ void __launch_bounds__(TB) k_forces(Geo g, const double *__restrict__
                                                double *__restrict__ partial, TqRings tq, int ntq_rings,
                                                DevCtl *__restrict__ ctl, PebF pf,
                                                const int *__restrict__ guard) {
-  b_forces<PEB>(g, rho, cs, agr, agt, pk, npl, mt, partial, tq, ntq_rings, ctl, pf, guard);
+  b_forces<PEB, NP>(g, rho, cs, agr, agt, pk, npl, mt, partial, tq, ntq_rings, ctl, pf, guard);
 }
 
 
@@ -1534,12 +1537,29 @@ __device__ __forceinline__ void b_substep1(Geo g, const double *__restrict__ rho
                                                  double *__restrict__ vradint, double *__restrict__ vthetaint,
                                                  double dt, DevCtl *ctl, double *__restrict__ accr,
                                                  double *__restrict__ acct, const double *__restrict__ dragr,
-                                                 const double *__restrict__ dragt) {
+                                                 const double *__restrict__ dragt, int fold_ind, double ind_px,
+                                                 double ind_py) {
   // accr/acct (pebbles: the gas acceleration, SourceEuler.c:371-405) and
-  // dragr/dragt (BackReaction) are null when unused
+  // dragr/dragt (BackReaction) are null when unused.  fold_ind: agr/agt do
+  // not hold the indirect term yet (no k_indirect pass); it is added here to
+  // each value read, with the same expression k_indirect adds
   const int ns = g.ns;
   int i, j, L_;
   if (!xcd_cell(g, 0, g.nr, i, j, L_) || j >= ns) return;
+  double axindir = 0.0, ayindir = 0.0;
+  if (fold_ind) {
+    axindir = -ctl->axstar;
+    ayindir = -ctl->aystar;
+    if (g.indirect_term) { axindir += ind_px; ayindir += ind_py; }
+  }
+  auto ind_r = [&](int ii, int jj) {
+    const double x = g.Rmed[ii] * g.cosj[jj], y = g.Rmed[ii] * g.sinj[jj];
+    return (x * axindir + y * ayindir) * g.InvRmed[ii];
+  };
+  auto ind_t = [&](int ii, int jj) {
+    const double x = g.Rmed[ii] * g.cosj[jj], y = g.Rmed[ii] * g.sinj[jj];
+    return (x * ayindir - y * axindir) * g.InvRmed[ii];
+  };
   const double gm1 = g.adiabind - 1.0;
   const size_t l = (size_t)i * ns + j;
   const size_t ljm = (j == 0) ? (size_t)i * ns + ns - 1 : l - 1;
@@ -1554,7 +1574,12 @@ __device__ __forceinline__ void b_substep1(Geo g, const double *__restrict__ rho
     double vt2 = vtheta[l] + vtheta[ljp] + vtheta[lim] + vtheta[ljp - ns];
     vt2 = vt2 / 4.0 + g.Rinf[i] * g.omega_frame;
     vt2 = vt2 * vt2;
-    double tmp = -gradp + 0.5 * (agr[l] + agr[lim]) + vt2 * g.InvRinf[i];
+    double a_l = agr[l], a_lim = agr[lim];
+    if (fold_ind) {
+      a_l += ind_r(i, j);
+      a_lim += ind_r(i - 1, j);
+    }
+    double tmp = -gradp + 0.5 * (a_l + a_lim) + vt2 * g.InvRinf[i];
     double v = vrad[l] + dt * tmp;
     if (accr) {
       double a = tmp;
@@ -1573,7 +1598,12 @@ __device__ __forceinline__ void b_substep1(Geo g, const double *__restrict__ rho
     const double invdxtheta = 1.0 / dxtheta;
     const double pljm = gm1 * energy[ljm];
     double gradp = (pl - pljm) * 2.0 / (rho[l] + rho[ljm]) * invdxtheta;
-    double tmp = -gradp + 0.5 * (agt[l] + agt[ljm]);
+    double t_l = agt[l], t_jm = agt[ljm];
+    if (fold_ind) {
+      t_l += ind_t(i, j);
+      t_jm += ind_t(i, j == 0 ? ns - 1 : j - 1);
+    }
+    double tmp = -gradp + 0.5 * (t_l + t_jm);
     double v = vtheta[l] + dt * tmp;
     v += dt * supp_torque;
     if (acct) {
@@ -1597,8 +1627,10 @@ __global__ void __launch_bounds__(TB) k_substep1(Geo g, const double *__restrict
                                                  double *__restrict__ vradint, double *__restrict__ vthetaint,
                                                  double dt, DevCtl *ctl, double *__restrict__ accr,
                                                  double *__restrict__ acct, const double *__restrict__ dragr,
-                                                 const double *__restrict__ dragt) {
-  b_substep1(g, rho, energy, vrad, vtheta, agr, agt, vradint, vthetaint, dt, ctl, accr, acct, dragr, dragt);
+                                                 const double *__restrict__ dragt, int fold_ind, double ind_px,
+                                                 double ind_py) {
+  b_substep1(g, rho, energy, vrad, vtheta, agr, agt, vradint, vthetaint, dt, ctl, accr, acct, dragr, dragt, fold_ind,
+             ind_px, ind_py);
 }
 
 
@@ -3205,8 +3237,10 @@ struct Batch {
   };
 BATCH_BODY(k_eos_cfl, b_eos_cfl, RB)
 BATCH_BODY(k_cfl_final, b_cfl_final, RB)
-BATCH_BODY(k_forces<false>, b_forces<false>, TB)
-BATCH_BODY(k_forces<true>, b_forces<true>, TB)
+BATCH_BODY((k_forces<false, 1>), (b_forces<false, 1>), TB)
+BATCH_BODY((k_forces<false, 8>), (b_forces<false, 8>), TB)
+BATCH_BODY((k_forces<true, 1>), (b_forces<true, 1>), TB)
+BATCH_BODY((k_forces<true, 8>), (b_forces<true, 8>), TB)
 BATCH_BODY(k_forces_reduce, b_forces_reduce, RB)
 BATCH_BODY(k_indirect, b_indirect, TB)
 BATCH_BODY(k_peb_cfl, b_peb_cfl, RB)
@@ -3887,15 +3921,23 @@ static int launch_forces(thorin_ctx *c, const thorin_planets *sys, double mt, in
   pf.tau = d->b[B_TAU];
   pf.pagr = d->b[B_PAGR];
   pf.pagt = d->b[B_PAGT];
-  if (peb)
-    LAUNCH(k_forces<true>, dim3(gall.x + (nrings ? 1u : 0u)), tb, 0, d->g, d->b[B_DENS], d->b[B_CS],
-           d->b[B_AGR], d->b[B_AGT], pks, sys->nb, mt, d->red, rr, nrings, d->ctl, pf, guard);
-  else
-    LAUNCH(k_forces<false>, dim3(gall.x + (nrings ? 1u : 0u)), tb, 0, d->g, d->b[B_DENS], d->b[B_CS],
-           d->b[B_AGR], d->b[B_AGT], pks, sys->nb, mt, d->red, rr, nrings, d->ctl, pf, guard);
+  const dim3 gf(gall.x + (nrings ? 1u : 0u));
+#define FORCES_LAUNCH(K)                                                                                  \
+  LAUNCH(K, gf, tb, 0, d->g, d->b[B_DENS], d->b[B_CS], d->b[B_AGR], d->b[B_AGT], pks, sys->nb, mt, d->red, rr, \
+         nrings, d->ctl, pf, guard)
+  // one planet (or none): the per-planet arrays of the kernel shrink to one
+  // entry, which halves its registers
+  if (peb && sys->nb <= 1) FORCES_LAUNCH((k_forces<true, 1>));
+  else if (peb) FORCES_LAUNCH((k_forces<true, 8>));
+  else if (sys->nb <= 1) FORCES_LAUNCH((k_forces<false, 1>));
+  else FORCES_LAUNCH((k_forces<false, 8>));
+#undef FORCES_LAUNCH
   LAUNCH(k_forces_reduce, dim3(2 + 2 * sys->nb), rb, 0, d->red, d->nblk_cells, d->ctl, BOff{0}, guard);
-  LAUNCH(k_indirect, gall, tb, 0, d->g, d->b[B_AGR], d->b[B_AGT], d->ctl, indir_px,
-         indir_py, peb ? d->b[B_PAGR] : (double *)nullptr, peb ? d->b[B_PAGT] : (double *)nullptr, guard);
+  // the gas-only path adds the indirect term in k_substep1 (the only reader
+  // of agr/agt); the pebble fluid's accelerations need it stored
+  if (peb)
+    LAUNCH(k_indirect, gall, tb, 0, d->g, d->b[B_AGR], d->b[B_AGT], d->ctl, indir_px, indir_py, d->b[B_PAGR],
+           d->b[B_PAGT], guard);
   HCHECK(hipGetLastError());
   fs->ntq = ntq;
   for (int k = 0; k < 8; k++) fs->dslot[k] = dslot[k];
@@ -4056,7 +4098,7 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
     LAUNCH(k_substep1, gall, tb, 0, d->g, d->b[B_DENS], d->b[B_ENERGY], d->b[B_VRAD], d->b[B_VTHETA], d->b[B_AGR],
            d->b[B_AGT], d->b[B_VRADINT], d->b[B_VTHETAINT], dt, d->ctl, peb ? d->b[B_GACCR] : (double *)nullptr,
            peb ? d->b[B_GACCT] : (double *)nullptr, br ? d->b[B_DRAGR] : (const double *)nullptr,
-           br ? d->b[B_DRAGT] : (const double *)nullptr);
+           br ? d->b[B_DRAGT] : (const double *)nullptr, peb ? 0 : 1, fs->indir_px, fs->indir_py);
     if (peb)   // SubStep1Pebbles + ParticleDiffusion (SourceEuler.c:293-296)
       LAUNCH(k_peb_substep1, gall, tb, 0, d->g, d->b[B_PVR], d->b[B_PVT], d->b[B_TAU], d->b[B_PACCR], d->b[B_PACCT],
              d->b[B_GACCR], d->b[B_GACCT], d->b[B_VRAD], d->b[B_VTHETA], d->b[B_DENS], d->b[B_PDENS], dt,
