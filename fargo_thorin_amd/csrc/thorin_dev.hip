@@ -589,7 +589,7 @@ __device__ __forceinline__ double ordered_ring_sum(const double *sh, int ns) {
 // under the square root is completed afterwards in the reference's order.
 // With do_eos = 0 the temperature and sound speed already in memory are used.
 #define EC_MAX 12              // cells per thread: NSEC <= RB * EC_MAX
-__device__ __forceinline__ void b_eos_cfl(Geo g, const double *__restrict__ rho, const double *__restrict__ e,
+__device__ __forceinline__ void b_eos_cfl(const Geo &g, const double *__restrict__ rho, const double *__restrict__ e,
                                                 double *__restrict__ T, double *__restrict__ cs,
                                                 const double *__restrict__ vr, const double *__restrict__ vt,
                                                 int do_eos, const int *__restrict__ guard) {
@@ -730,7 +730,7 @@ __global__ void __launch_bounds__(RB) k_eos_cfl(Geo g, const double *__restrict_
 // between rings.  The minimum of the DiffusionTimestep partials of this
 // substep's k_substep3 (nstellar of them; 0: none formed) goes to
 // ctl->dt_stellar.
-__device__ __forceinline__ void b_cfl_final(Geo g, DevCtl *ctl, const double *__restrict__ pstellar,
+__device__ __forceinline__ void b_cfl_final(const Geo &g, DevCtl *ctl, const double *__restrict__ pstellar,
                                                   int nstellar, double invdtreb_sq, const int *__restrict__ guard,
                                              DevCtl *__restrict__ hout) {
   __shared__ double red[RB];
@@ -793,11 +793,11 @@ struct PebF {
 };
 
 template <bool PEB, int NP>
-__device__ __forceinline__ void b_forces(Geo g, const double *__restrict__ rho,
+__device__ __forceinline__ void b_forces(const Geo &g, const double *__restrict__ rho,
                                                const double *__restrict__ cs,
                                                double *__restrict__ agr, double *__restrict__ agt,
                                                const PlanetsK &pk, int npl, double mt,
-                                               double *__restrict__ partial, TqRings tq, int ntq_rings,
+                                               double *__restrict__ partial, const TqRings &tq, int ntq_rings,
                                                DevCtl *__restrict__ ctl, PebF pf,
                                               const int *__restrict__ guard) {
   __shared__ double red[TB];
@@ -1094,7 +1094,7 @@ __global__ void __launch_bounds__(RB) k_forces_reduce(const double *__restrict__
 
 
 // indirect term (Pframeforce.c:488-509)
-__device__ __forceinline__ void b_indirect(Geo g, double *__restrict__ agr, double *__restrict__ agt,
+__device__ __forceinline__ void b_indirect(const Geo &g, double *__restrict__ agr, double *__restrict__ agt,
                                                                                                   const DevCtl *__restrict__ ctl, double indir_px, double indir_py,
                                                  double *__restrict__ pagr, double *__restrict__ pagt,
                                               const int *__restrict__ guard) {
@@ -1133,7 +1133,7 @@ __global__ void __launch_bounds__(TB) k_indirect(Geo g, double *__restrict__ agr
 // pebble vtheta; lanes 0 and 1 of the first wave walk the two ring sums side
 // by side.  The ring maximum of the four squared inverse times goes to
 // pebmax[i] (ring 0 is not part of CriticalCharTime).
-__device__ __forceinline__ void b_peb_cfl(Geo g, const double *__restrict__ rho,
+__device__ __forceinline__ void b_peb_cfl(const Geo &g, const double *__restrict__ rho,
                                                 const double *__restrict__ psize, double *__restrict__ tau,
                                                 const double *__restrict__ pvr, const double *__restrict__ pvt,
                                                 const double *__restrict__ vr, const double *__restrict__ vt,
@@ -1191,7 +1191,7 @@ __global__ void __launch_bounds__(RB) k_peb_cfl(Geo g, const double *__restrict_
 // DampPebbles (SideEuler.c:423-454) on the rings of the damping zones; with
 // detect, DetectCrashPebbles (SourceEuler.c:274, 48-66) first looks at every
 // cell's density before the damping.
-__device__ __forceinline__ void b_peb_damp(Geo g, double *__restrict__ pdens, double *__restrict__ pvr,
+__device__ __forceinline__ void b_peb_damp(const Geo &g, double *__restrict__ pdens, double *__restrict__ pvr,
                                                  double *__restrict__ pvt, double dt, int detect, DevCtl *ctl,
                                                  const int *__restrict__ guard) {
   const int ns = g.ns;
@@ -1220,7 +1220,7 @@ __global__ void __launch_bounds__(TB) k_peb_damp(Geo g, double *__restrict__ pde
 // SourceTermsPebbles after its DampPebbles (Pebbles.c:549-612): pebble
 // accelerations paccr (rings >= 1) and pacct and, with BackReaction, the
 // drag on the gas dragr (rings >= 1) and dragt.
-__device__ __forceinline__ void b_peb_source(Geo g, const double *__restrict__ pdens,
+__device__ __forceinline__ void b_peb_source(const Geo &g, const double *__restrict__ pdens,
                                                    const double *__restrict__ pvr, const double *__restrict__ pvt,
                                                    const double *__restrict__ tau, const double *__restrict__ vr,
                                                    const double *__restrict__ vt, const double *__restrict__ pagr,
@@ -1266,7 +1266,7 @@ __global__ void __launch_bounds__(TB) k_peb_source(Geo g, const double *__restri
 // gas velocities from before SubStep1, then ParticleDiffusion
 // (Pebbles.c:654-695), whose corrections depend on the two densities only
 // (so one pass over the cells does both).
-__device__ __forceinline__ void b_peb_substep1(Geo g, double *__restrict__ pvr, double *__restrict__ pvt,
+__device__ __forceinline__ void b_peb_substep1(const Geo &g, double *__restrict__ pvr, double *__restrict__ pvt,
                                                      const double *__restrict__ tau, const double *__restrict__ paccr,
                                                      const double *__restrict__ pacct, const double *__restrict__ accr,
                                                      const double *__restrict__ acct, const double *__restrict__ vr,
@@ -1312,7 +1312,7 @@ __global__ void __launch_bounds__(TB) k_peb_substep1(Geo g, double *__restrict__
 
 
 // EvaporatePebbles (Pebbles.c:837-866)
-__device__ __forceinline__ void b_peb_evaporate(Geo g, double *__restrict__ pdens, const double *__restrict__ T,
+__device__ __forceinline__ void b_peb_evaporate(const Geo &g, double *__restrict__ pdens, const double *__restrict__ T,
                                                       double facc, double evaptemp) {
   const int ns = g.ns;
   int i, j, L_;
@@ -1379,7 +1379,7 @@ struct PebAccK {
 // and the x- and y-momentum each planet takes from the disk go to
 // ctl->pacc; the host updates the planet as STEP 7 does.
 #define ACC_T 1024
-__device__ __forceinline__ void b_peb_accrete(Geo g, const PebAccK &P, double *__restrict__ pdens,
+__device__ __forceinline__ void b_peb_accrete(const Geo &g, const PebAccK &P, double *__restrict__ pdens,
                                                        const double *__restrict__ pvr, const double *__restrict__ pvt,
                                                        const double *__restrict__ tau, const double *__restrict__ cs,
                                                                                                               double *__restrict__ macc, double dt, DevCtl *ctl) {
@@ -1490,7 +1490,7 @@ __global__ void __launch_bounds__(ACC_T) k_peb_accrete(Geo g, PebAccK P, double 
 
 // ---------------------------------------------------------------------------
 // DampingBoundary (SideEuler.c:368-419) on the rings flagged by the host
-__device__ __forceinline__ void b_boundary(Geo g, double *__restrict__ vrad, double *__restrict__ vtheta,
+__device__ __forceinline__ void b_boundary(const Geo &g, double *__restrict__ vrad, double *__restrict__ vtheta,
                                                  double *__restrict__ rho, double *__restrict__ energy, double step,
                                                  const int *__restrict__ guard) {
   const int ns = g.ns;
@@ -1528,7 +1528,7 @@ __global__ void __launch_bounds__(TB) k_boundary(Geo g, double *__restrict__ vra
 // SubStep1 source terms (SourceEuler.c:319-406) with P = (gamma-1) e on the
 // fly (ComputePressureField, SourceEuler.c:740-762) and DetectCrash
 // (SourceEuler.c:48-66, 272-273).
-__device__ __forceinline__ void b_substep1(Geo g, const double *__restrict__ rho,
+__device__ __forceinline__ void b_substep1(const Geo &g, const double *__restrict__ rho,
                                                  const double *__restrict__ energy,
                                                  const double *__restrict__ vrad,
                                                  const double *__restrict__ vtheta,
@@ -1672,7 +1672,7 @@ __device__ __forceinline__ Stress stress_cell(const Geo &g, const double *__rest
   return st;
 }
 
-__device__ __forceinline__ void b_divstress(Geo g, const double *__restrict__ vr,
+__device__ __forceinline__ void b_divstress(const Geo &g, const double *__restrict__ vr,
                                                   const double *__restrict__ vt,
                                                   const double *__restrict__ rho,
                                                   double *__restrict__ divv, double *__restrict__ Trr,
@@ -1698,7 +1698,7 @@ __global__ void __launch_bounds__(TB) k_divstress(Geo g, const double *__restric
 
 // UpdateVelocityWithViscousTerms (Viscosity.c:165-218) + ImposeKeplerianEdges
 // (Viscosity.c:221-260, VKepIn/VKepOut from the host)
-__device__ __forceinline__ void b_visc_update(Geo g, double *__restrict__ vr, double *__restrict__ vt,
+__device__ __forceinline__ void b_visc_update(const Geo &g, double *__restrict__ vr, double *__restrict__ vt,
                                                     const double *__restrict__ rho,
                                                     const double *__restrict__ Trr,
                                                     const double *__restrict__ Trp,
@@ -1743,7 +1743,7 @@ __device__ __forceinline__ double qvnr(double rho, double dv) {
   return (dv < 0.0) ? CVNR * CVNR * rho * dv * dv : 0.0;
 }
 
-__device__ __forceinline__ void b_substep2(Geo g, const double *__restrict__ rho,
+__device__ __forceinline__ void b_substep2(const Geo &g, const double *__restrict__ rho,
                                                  const double *__restrict__ vrad,
                                                  const double *__restrict__ vtheta,
                                                  const double *__restrict__ energy,
@@ -1819,7 +1819,7 @@ __device__ __forceinline__ double qplus_of(const Geo &g, int i, double rho, cons
   return 0.0;
 }
 
-__device__ __forceinline__ void b_substep3(Geo g, const double *__restrict__ vr, const double *__restrict__ vt,
+__device__ __forceinline__ void b_substep3(const Geo &g, const double *__restrict__ vr, const double *__restrict__ vt,
                                                  const double *__restrict__ rho, const double *__restrict__ eint,
                                                  double *__restrict__ divv, double *__restrict__ Trr,
                                                  double *__restrict__ Trp, double *__restrict__ Tpp,
@@ -1948,7 +1948,7 @@ __device__ __forceinline__ double coefB(const Geo &g, const double *D, const dou
 // The matrix kernel also returns the initial SOR residual sum
 // (EnergySources.c:435-449): block partials of |T.M - RHS| from the values
 // it has just computed.
-__device__ __forceinline__ void b_matrix(Geo g, const double *__restrict__ rho,
+__device__ __forceinline__ void b_matrix(const Geo &g, const double *__restrict__ rho,
                                                const double *__restrict__ cs, const double *__restrict__ D,
                                                const double *__restrict__ divv,
                                                const double *__restrict__ qminus,
@@ -2012,7 +2012,7 @@ __device__ __forceinline__ double sor_resid(const double *T, const double *Ml, c
   return T[l] * Ml[l] + T[lip] * Mlip[l] + T[lim] * Mlim[l] + T[ljp] * Mljp[l] + T[ljm] * Mljm[l] - RHS[l];
 }
 
-__device__ __forceinline__ void b_sor_resid0(Geo g, const double *__restrict__ T,
+__device__ __forceinline__ void b_sor_resid0(const Geo &g, const double *__restrict__ T,
                                                    const double *__restrict__ Ml, const double *__restrict__ Mlip,
                                                    const double *__restrict__ Mlim, const double *__restrict__ Mljp,
                                                    const double *__restrict__ Mljm, const double *__restrict__ RHS,
@@ -2122,7 +2122,7 @@ __device__ __forceinline__ bool sor_cell(int sl, int h, bool odd, int i0, int j0
 }
 
 template <int NIT>
-__device__ __forceinline__ void b_sor_iter(Geo g, double *__restrict__ T0, double *__restrict__ T1,
+__device__ __forceinline__ void b_sor_iter(const Geo &g, double *__restrict__ T0, double *__restrict__ T1,
                                            double *__restrict__ T2, const double *__restrict__ Ml,
                                            const double *__restrict__ Mlip, const double *__restrict__ Mlim,
                                            const double *__restrict__ Mljp, const double *__restrict__ Mljm,
@@ -2311,7 +2311,7 @@ __global__ void __launch_bounds__(RB) k_sor_check(const double *__restrict__ psu
 // last iterate is in T0, T1 or T2 (ctl->sor_cur); T0 receives it, so that T0
 // holds the solved temperature afterwards.  Nothing is done until the SOR
 // has converged (ctl->sor_done).
-__device__ __forceinline__ void b_energy_new(Geo g, const double *__restrict__ rho, double *__restrict__ T0,
+__device__ __forceinline__ void b_energy_new(const Geo &g, const double *__restrict__ rho, double *__restrict__ T0,
                                              const double *__restrict__ T1, const double *__restrict__ T2,
                                              double *__restrict__ enew, const DevCtl *__restrict__ ctl) {
   const int ns = g.ns, nr = g.nr;
@@ -2463,7 +2463,7 @@ __device__ __forceinline__ void rad_column(const Geo &g, const RadQ &q, const SG
 }
 
 template <int RI>
-__device__ __forceinline__ void b_transport_rad(Geo g, RadQ q, double dt) {
+__device__ __forceinline__ void b_transport_rad(const Geo &g, RadQ q, double dt) {
   __shared__ RadGeo<RI> sg;
   if (q.guard && *q.guard == 0) return;
   // XCD x (= b % 8) gets the contiguous band of tiles x*per .. x*per+per-1,
@@ -2602,7 +2602,7 @@ __device__ __forceinline__ void th_pass(const ThGeom &t, double *qv, const doubl
 }
 
 template <int NPT>
-__device__ __forceinline__ void b_transport_theta(Geo g, const double *__restrict__ vt,
+__device__ __forceinline__ void b_transport_theta(const Geo &g, const double *__restrict__ vt,
                                                           const double *__restrict__ xrp, const double *__restrict__ xrm,
                                                           const double *__restrict__ xtp, const double *__restrict__ xtm,
                                                           const double *__restrict__ xe, const double *__restrict__ xrho,
@@ -2736,7 +2736,7 @@ __global__ void __launch_bounds__(TT_T) k_transport_theta(Geo g, const double *_
 
 
 // ComputeVelocities (TransportEuler.c:350-381)
-__device__ __forceinline__ void b_velocities(Geo g, const double *__restrict__ rho,
+__device__ __forceinline__ void b_velocities(const Geo &g, const double *__restrict__ rho,
                                                    const double *__restrict__ rp, const double *__restrict__ rm,
                                                    const double *__restrict__ tp, const double *__restrict__ tm,
                                                    double *__restrict__ vr, double *__restrict__ vt,
