@@ -3211,9 +3211,15 @@ std::tuple<P...> params_of(void (*)(P...));
 template <int NT, auto Body, typename... P>
 __global__ void __launch_bounds__(NT) k_batch(const std::tuple<P...> *__restrict__ tab,
                                               const int *__restrict__ off, int n) {
-  int d = 0;
-  while (d + 1 < n && (int)blockIdx.x >= off[d + 1]) d++;
-  std::apply(Body, tab[d]);
+  // the disk is the last one whose first block is at or before this block
+  // (off ascending, off[0] = 0): a binary search on scalar loads
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (off[mid] <= (int)blockIdx.x) lo = mid;
+    else hi = mid - 1;
+  }
+  std::apply(Body, tab[lo]);
 }
 
 template <int NT, auto Body, typename T>
