@@ -589,6 +589,7 @@ __device__ __forceinline__ double ordered_ring_sum(const double *sh, int ns) {
 // under the square root is completed afterwards in the reference's order.
 // With do_eos = 0 the temperature and sound speed already in memory are used.
 #define EC_MAX 12              // cells per thread: NSEC <= RB * EC_MAX
+#define EC_PAD 4
 __device__ __forceinline__ void b_eos_cfl(const Geo &g, const double *__restrict__ rho, const double *__restrict__ e,
                                                 double *__restrict__ T, double *__restrict__ cs,
                                                 const double *__restrict__ vr, const double *__restrict__ vt,
@@ -599,7 +600,8 @@ __device__ __forceinline__ void b_eos_cfl(const Geo &g, const double *__restrict
   if (guard && *guard == 0) return;
   const int i = (int)blockIdx.x - g.blk0, ns = g.ns, nr = g.nr, tid = threadIdx.x;
   const bool want_t = g.alpha_flock && do_eos;
-  double *s_vt = sh, *s_cs = sh + ns, *s_t = sh + 2 * ns;
+  // rows EC_PAD doubles longer than NSEC: lanes 0, 1, 2 read distinct banks
+  double *s_vt = sh, *s_cs = sh + (ns + EC_PAD), *s_t = sh + 2 * (ns + EC_PAD);
   const double gam = g.adiabind;
   const size_t base = (size_t)i * ns;
   const bool cfl = (i >= 1 && i < nr);
@@ -646,53 +648,69 @@ __device__ __forceinline__ void b_eos_cfl(const Geo &g, const double *__restrict
     }
   }
   __syncthreads();
+  double msum = 0.0;
   if (tid < 3) {
     const double *src = (tid == 0) ? s_vt : ((tid == 1) ? s_cs : s_t);
     if (tid < 2 || want_t) {
       const double m = ordered_ring_sum(src, ns) / (double)ns;
+      msum = m;
       if (tid == 0) {
         vmoy = m;
         g.Vmoy[i] = m;
       } else if (tid == 1) {
         gcs_i = m;
         g.gcs[i] = m;
-        // ring-averaged Gaussian vertical integrals of FillForcesArrays
-        const double H_avr = m * g.OmegaInv[i] * g.sqrt_adiabind_inv;
-        const double H2 = H_avr * H_avr;
-        const double r2 = g.Rmed[i] * g.Rmed[i];
-        const double deltaz = 3.0 * H_avr / (double)10;
-        // pebbles: H_peb = H sqrt(alpha/tau) with the ring-mean Stokes number
-        // (Pframeforce.c:160-164, 180-186)
-        const double ta = g.pebbles ? g.tau_avr[i] : 1.0;
-        double sig = 0.0, ist = 0.0, pist = 0.0;
-        for (int q = 1; q <= 10; q++) {
-          const double znode = ((double)q - 0.5) * deltaz;
-          const double znode2 = znode * znode;
-          const double sum = exp(-(0.5 * znode2 / H2));
-          sig += sum;
-          const double d2 = r2 + znode2;
-          const double dd = sqrt(d2);
-          const double denom = dd * d2;
-          ist += sum / denom;
-          if (g.pebbles) {
-            const double pz2 = znode2 * (g.palpha / ta);
-            const double pd2 = r2 + pz2;
-            const double pd = sqrt(pd2);
-            pist += sum / (pd * pd2);
-          }
-        }
-        g.sig_avr[i] = sig;
-        g.ist_avr[i] = ist;
-        if (g.pebbles) {
-          g.phavr[i] = H_avr * sqrt(g.palpha / ta);
-          g.pist_avr[i] = pist;
-        }
       } else {
         gtemp_i = m;
         g.gtemp[i] = m;
       }
     }
     if (tid == 2 && !want_t) gtemp_i = g.gtemp[i];
+  }
+  if (tid < 64) {
+    // ring-averaged Gaussian vertical integrals of FillForcesArrays from
+    // globcsvec: node q on lane q - 1, the terms then added in q order
+    const double m = __shfl(msum, 1, 64);
+    const double H_avr = m * g.OmegaInv[i] * g.sqrt_adiabind_inv;
+    const double H2 = H_avr * H_avr;
+    const double r2 = g.Rmed[i] * g.Rmed[i];
+    const double deltaz = 3.0 * H_avr / (double)10;
+    // pebbles: H_peb = H sqrt(alpha/tau) with the ring-mean Stokes number
+    // (Pframeforce.c:160-164, 180-186)
+    const double ta = g.pebbles ? g.tau_avr[i] : 1.0;
+    double t_sig = 0.0, t_ist = 0.0, t_pist = 0.0;
+    if (tid < 10) {
+      const int q = tid + 1;
+      const double znode = ((double)q - 0.5) * deltaz;
+      const double znode2 = znode * znode;
+      const double sum = exp(-(0.5 * znode2 / H2));
+      t_sig = sum;
+      const double d2 = r2 + znode2;
+      const double dd = sqrt(d2);
+      const double denom = dd * d2;
+      t_ist = sum / denom;
+      if (g.pebbles) {
+        const double pz2 = znode2 * (g.palpha / ta);
+        const double pd2 = r2 + pz2;
+        const double pd = sqrt(pd2);
+        t_pist = sum / (pd * pd2);
+      }
+    }
+    double sig = 0.0, ist = 0.0, pist = 0.0;
+#pragma unroll
+    for (int q = 0; q < 10; q++) {
+      sig += __shfl(t_sig, q, 64);
+      ist += __shfl(t_ist, q, 64);
+      pist += __shfl(t_pist, q, 64);
+    }
+    if (tid == 1) {
+      g.sig_avr[i] = sig;
+      g.ist_avr[i] = ist;
+      if (g.pebbles) {
+        g.phavr[i] = H_avr * sqrt(g.palpha / ta);
+        g.pist_avr[i] = pist;
+      }
+    }
   }
   __syncthreads();
   double m = -1.0;
@@ -2797,7 +2815,7 @@ static int need_dev(thorin_ctx *c) {
 // dynamic LDS of k_eos_cfl: the vtheta and cs rings, and T when the Flock
 // alpha needs its ring mean
 static inline size_t eos_lds(const thorin_dev *d, int do_eos) {
-  return sizeof(double) * (size_t)d->ns * ((d->g.alpha_flock && do_eos) ? 3 : 2);
+  return sizeof(double) * (size_t)(d->ns + EC_PAD) * ((d->g.alpha_flock && do_eos) ? 3 : 2);
 }
 
 // 1D grid of the XCD-banded cell blocks over `rings` rings (xcd_cell), and
@@ -2859,6 +2877,13 @@ extern "C" int thorin_dev_create(thorin_ctx *c) {
     }
     HCHECK(hipFuncSetAttribute((const void *)k_ring_means, hipFuncAttributeMaxDynamicSharedMemorySize, (int)rs_lds));
     HCHECK(hipFuncSetAttribute((const void *)k_peb_cfl, hipFuncAttributeMaxDynamicSharedMemorySize, (int)need));
+    // k_eos_cfl: up to three rows of NSEC + EC_PAD
+    const size_t eos_need = sizeof(double) * 3 * (size_t)(c->ns + EC_PAD);
+    if (eos_need > (size_t)prop.sharedMemPerBlock) {
+      thorin_set_error("NSEC too large for the end-of-substep kernel's LDS");
+      return -1;
+    }
+    HCHECK(hipFuncSetAttribute((const void *)k_eos_cfl, hipFuncAttributeMaxDynamicSharedMemorySize, (int)eos_need));
   }
   HCHECK(hipStreamCreateWithFlags(&d->s, hipStreamNonBlocking));
   HCHECK(hipEventCreateWithFlags(&d->ev_end, hipEventDisableTiming));
