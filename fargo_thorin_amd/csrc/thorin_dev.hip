@@ -2279,43 +2279,60 @@ template <int NIT>
 __device__ __forceinline__ void b_sor_check(const double *__restrict__ psum, const int *__restrict__ pcnt, int nb,
                                             DevCtl *ctl, int maxiter, int it, double eps) {
   __shared__ double red[RB];
-  __shared__ int stop;
   if (ctl->sor_done) return;
-  for (int k = 0; k < NIT; k++) {
-    double s = strided_sum<RB>(psum + (size_t)k * nb, nb);
-    int ci = 0;
-    {
-      const int *__restrict__ pc = pcnt + (size_t)k * nb;
-      int b = threadIdx.x;
-      for (; b + 7 * RB < nb; b += 8 * RB) {
-        int v[8];
+  // the partial sums of all NIT iterations are read together (eight loads
+  // of each array in flight per thread); each thread adds its partials in
+  // block order as a strided sum does
+  double s[NIT], c[NIT];
+  int ci[NIT];
 #pragma unroll
-        for (int u = 0; u < 8; u++) v[u] = pc[b + u * RB];
+  for (int k = 0; k < NIT; k++) { s[k] = 0.0; ci[k] = 0; }
+  int b = threadIdx.x;
+  for (; b + 7 * RB < nb; b += 8 * RB) {
+    double v[NIT][8];
+    int w[NIT][8];
 #pragma unroll
-        for (int u = 0; u < 8; u++) ci += v[u];
+    for (int k = 0; k < NIT; k++)
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        v[k][u] = psum[(size_t)k * nb + b + u * RB];
+        w[k][u] = pcnt[(size_t)k * nb + b + u * RB];
       }
-      for (; b < nb; b += RB) ci += pc[b];
+#pragma unroll
+    for (int k = 0; k < NIT; k++)
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        s[k] += v[k][u];
+        ci[k] += w[k][u];
+      }
+  }
+  for (; b < nb; b += RB)
+#pragma unroll
+    for (int k = 0; k < NIT; k++) {
+      s[k] += psum[(size_t)k * nb + b];
+      ci[k] += pcnt[(size_t)k * nb + b];
     }
-    s = block_sum<RB>(s, red);
-    const double c = block_sum<RB>((double)ci, red);
-    if (threadIdx.x == 0) {
+#pragma unroll
+  for (int k = 0; k < NIT; k++) {
+    s[k] = block_sum<RB>(s[k], red);
+    c[k] = block_sum<RB>((double)ci[k], red);
+  }
+  if (threadIdx.x == 0) {
+    for (int k = 0; k < NIT; k++) {
       const int n = it + k + 1;
-      stop = 0;
       ctl->sor_iter = n;
       ctl->sor_cur = n % 3;
-      ctl->sor_normres = s;
-      if (s < eps * ctl->sor_normres0 || s < SORMIN || c == 0.0) {
+      ctl->sor_normres = s[k];
+      if (s[k] < eps * ctl->sor_normres0 || s[k] < SORMIN || c[k] == 0.0) {
         ctl->sor_done = 1;
         ctl->sor_niter = n;
-        stop = 1;
+        break;
       } else if (n >= maxiter) {
         ctl->sor_done = 1;
         ctl->sor_niter = maxiter + 1;
-        stop = 1;
+        break;
       }
     }
-    __syncthreads();
-    if (stop) return;
   }
 }
 
