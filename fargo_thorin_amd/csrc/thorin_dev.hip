@@ -751,44 +751,74 @@ __global__ void __launch_bounds__(RB) k_eos_cfl(Geo g, const double *__restrict_
 __device__ __forceinline__ void b_cfl_final(const Geo &g, DevCtl *ctl, const double *__restrict__ pstellar,
                                                   int nstellar, double invdtreb_sq, const int *__restrict__ guard,
                                              DevCtl *__restrict__ hout) {
-  __shared__ double red[RB];
+  // the four extrema (largest S and pebble term, smallest stellar partial and
+  // shear limit) are independent: each thread forms its share of all four,
+  // then one reduction; min and max are exact, so the order does not matter
+  __shared__ double red[4][RB / 64];
   if (guard && *guard == 0) return;
-  if (nstellar > 0) {
-    double ms = 1.0e38;
-    ms = strided_min<RB>(pstellar, nstellar, ms);
-    ms = block_min<RB>(ms, red);
-    if (threadIdx.x == 0) ctl->dt_stellar = ms;
-  }
-  double S = -1.0, P = -1.e30;
+  double ms = 1.0e38;
+  if (nstellar > 0) ms = strided_min<RB>(pstellar, nstellar, ms);
+  double S = -1.0, P = -1.e30, msh = 1e30;
   for (int i = 1 + threadIdx.x; i < g.nr; i += RB) {
     const double v = g.ringmax[i];
     S = v > S ? v : S;
     if (g.pebbles) { const double u = g.pebmax[i]; P = u > P ? u : P; }
   }
-  S = block_max<RB>(S, red);
-  double m = 1e30;
-  if (g.pebbles) {
-    P = block_max<RB>(P, red);
-    const double dt = CFLSECURITY / sqrt(S + P + invdtreb_sq);
-    if (dt < m) m = dt;
-  } else {
-    const double dt = CFLSECURITY / sqrt(S + invdtreb_sq);
-    if (dt < m) m = dt;
-  }
   for (int i = threadIdx.x; i < g.nr - 1; i += RB) {
     double dt = 2.0 * DPI * CFLSECURITY / (double)g.ns /
                 fabs(g.Vmoy[i] * g.InvRmed[i] - g.Vmoy[i + 1] * g.InvRmed[i + 1]);
-    if (dt < m) m = dt;
+    if (dt < msh) msh = dt;
   }
-  m = block_min<RB>(m, red);
-  if (threadIdx.x == 0) ctl->newdt_raw = m;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    double u = __shfl_xor(S, o, 64);
+    S = u > S ? u : S;
+    u = __shfl_xor(P, o, 64);
+    P = u > P ? u : P;
+    u = __shfl_xor(ms, o, 64);
+    ms = u < ms ? u : ms;
+    u = __shfl_xor(msh, o, 64);
+    msh = u < msh ? u : msh;
+  }
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    red[0][w] = S;
+    red[1][w] = P;
+    red[2][w] = ms;
+    red[3][w] = msh;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    S = red[0][0];
+    P = red[1][0];
+    ms = red[2][0];
+    msh = red[3][0];
+#pragma unroll
+    for (int k = 1; k < RB / 64; k++) {
+      S = red[0][k] > S ? red[0][k] : S;
+      P = red[1][k] > P ? red[1][k] : P;
+      ms = red[2][k] < ms ? red[2][k] : ms;
+      msh = red[3][k] < msh ? red[3][k] : msh;
+    }
+    if (nstellar > 0) ctl->dt_stellar = ms;
+    double m = 1e30;
+    if (g.pebbles) {
+      const double dt = CFLSECURITY / sqrt(S + P + invdtreb_sq);
+      if (dt < m) m = dt;
+    } else {
+      const double dt = CFLSECURITY / sqrt(S + invdtreb_sq);
+      if (dt < m) m = dt;
+    }
+    if (msh < m) m = msh;
+    ctl->newdt_raw = m;
+  }
   if (hout) {
     // the control block straight into the host's pinned mirror: the end of
     // the substep needs no copy of its own before the host's sync
     __syncthreads();
     const unsigned long long *src = (const unsigned long long *)ctl;
     unsigned long long *dst = (unsigned long long *)hout;
-    for (int w = threadIdx.x; w < (int)(sizeof(DevCtl) / 8); w += RB) dst[w] = src[w];
+    for (int q = threadIdx.x; q < (int)(sizeof(DevCtl) / 8); q += RB) dst[q] = src[q];
   }
 }
 __global__ void __launch_bounds__(RB) k_cfl_final(Geo g, DevCtl *ctl, const double *__restrict__ pstellar,
