@@ -1093,18 +1093,35 @@ __device__ __forceinline__ void b_forces(const Geo &g, const double *__restrict_
       pf.pagt[l] = 0.0 + at;
     }
   }
-  // partial sums of this block: [axstar, aystar, axpl[k], aypl[k]]
+  // partial sums of this block: [axstar, aystar, axpl[k], aypl[k]], reduced
+  // together in one pass, each by block_sum's tree (lanes by xor shuffles,
+  // then the waves in order)
   const int nb = g.ncb * g.nr;
-  double v = block_sum<TB>(axs, red);
-  if (threadIdx.x == 0) partial[bid] = v;
-  v = block_sum<TB>(ays, red);
-  if (threadIdx.x == 0) partial[nb + bid] = v;
-  for (int k = 0; k < np; k++) {
-    v = block_sum<TB>(axpl_loc[k], red);
-    if (threadIdx.x == 0) partial[(2 + 2 * k) * nb + bid] = v;
-    v = block_sum<TB>(aypl_loc[k], red);
-    if (threadIdx.x == 0) partial[(3 + 2 * k) * nb + bid] = v;
+  constexpr int NV = 2 + 2 * NP, NW = TB / 64;
+  static_assert(NV * NW <= TB, "the reduction rows fit the block's LDS buffer");
+  double v[NV];
+  v[0] = axs;
+  v[1] = ays;
+#pragma unroll
+  for (int k = 0; k < NP; k++) {
+    v[2 + 2 * k] = axpl_loc[k];
+    v[3 + 2 * k] = aypl_loc[k];
   }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+    for (int q = 0; q < NV; q++) v[q] += __shfl_xor(v[q], o, 64);
+  if ((threadIdx.x & 63) == 0)
+#pragma unroll
+    for (int q = 0; q < NV; q++) red[q * NW + (threadIdx.x >> 6)] = v[q];
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (int q = 0; q < 2 + 2 * np; q++) {
+      double r = red[q * NW];
+#pragma unroll
+      for (int w = 1; w < NW; w++) r += red[q * NW + w];
+      partial[(size_t)q * nb + bid] = r;
+    }
 }
 template <bool PEB, int NP>
 __global__ void __launch_bounds__(TB) k_forces(Geo g, const double *__restrict__ rho,
