@@ -53,7 +53,7 @@ def _compare(a, b, fields):
 
 
 @pytest.mark.parametrize("sizes,n,steps,graphs", [([32, 48, 64], 4, 6, True), ([32, 48, 64], 4, 3, False),
-                                                  ([512, 768], 3, 3, True)])
+                                                  ([512, 768], 3, 3, True), ([32, 48], 11, 3, True)])
 def test_ensemble_matches_single_disks(sizes, n, steps, graphs):
     """Disks of uneven grids and parameters (alpha, Sigma0, h drawn as the
     ensemble workload draws them), from the first DT on (the SOR
