@@ -215,7 +215,6 @@ static int ctx_create(const thorin_params *p, thorin_ctx **out, int with_device)
   }
   if (p->corotating) { thorin_set_error("Frame COROTATING/GUIDING-CENTER is not supported by this build"); return -1; }
   if (p->parametric_cooling) { thorin_set_error("CoolingTime > 0 (parametric cooling) is not supported by this build"); return -1; }
-  if (p->stellar_irradiation) { thorin_set_error("StellarIrradiation is not supported by this build"); return -1; }
   if (p->advect_label) { thorin_set_error("LabelAdvection is not supported by this build"); return -1; }
   if (!p->energy_eq) { thorin_set_error("EnergyEquation NO (isothermal) is not supported by this build"); return -1; }
   if (p->prescribed_accretion || p->accretionrate > 1e-10 || p->aerodynamic_drag || p->gas_accret_heating ||

@@ -137,6 +137,7 @@ struct Geo {                    // passed by value to every kernel
   double t2si, opa2cu, rho2cgs, stefan, sqrt2pi_inv, sqrt3, opacitydrop, parametricopacity;
   double powerlawopacity, gasflux, omega_frame, lr01, lr12, thicksmooth;
   int visc_alpha, alpha_flock, fast_transport, damp_init, damp_flux, opacity_law, stellar_irr;
+  double rstar, tirr4fac;       // stellar irradiation: R_star in AU and (1-A) Teff^4 R_star^2 in code units
   int indirect_term, exclude_hill;
   double hillcut;
   // LP85 thresholds folded on the host (Opacity.c:205-220)
@@ -1951,7 +1952,8 @@ __device__ __forceinline__ void b_substep3(const Geo &g, const double *__restric
     }
     opa[l] = o;
     const double tau = g.opacitydrop * 0.5 * o * rl;
-    const double taueff = 0.375 * tau + 0.25 * g.sqrt3 + 0.25 / tau;
+    // EffectiveOpticalDepth (EnergySources.c:701-711)
+    const double taueff = 0.375 * tau + (g.stellar_irr ? 0.5 : 0.25 * g.sqrt3) + 0.25 / tau;
     qminus[l] = 2.0 * g.stefan * cube_cr(tl) / taueff;
     // |grad T| at ring ig (clamped to 1..nr-2) from T rebuilt at the four neighbours
     const int ig = (i == 0) ? 1 : ((i == nr - 1) ? nr - 2 : i);
@@ -2013,6 +2015,30 @@ __device__ __forceinline__ double coefB(const Geo &g, const double *D, const dou
 // The matrix kernel also returns the initial SOR residual sum
 // (EnergySources.c:435-449): block partials of |T.M - RHS| from the values
 // it has just computed.
+// Stellar irradiation of cell l of ring i, 1 <= i <= nr-2: CalculateFlaring
+// (EnergySources.c:143-165: the sine of the grazing angle from the radial
+// derivative of the pressure scale height, 0 for surfaces tilted away; the
+// edge rings' values feed no matrix row) and CalculateQirr
+// (EnergySources.c:103-135).  pow(Rsup[i], 1.5) is pow_rinf15[i + 1].
+__device__ __forceinline__ double qirr_cell(const Geo &g, int i, size_t l, const double *__restrict__ cs,
+                                            const double *__restrict__ opa, const double *__restrict__ rho) {
+  const size_t lip = l + g.ns, lim = l - g.ns;
+  const double csin = 0.5 * (cs[l] + cs[lim]);
+  const double csout = 0.5 * (cs[l] + cs[lip]);
+  const double Hin = csin * g.pow_rinf15[i] * g.sqrt_adiabind_inv;
+  const double Hout = csout * g.pow_rinf15[i + 1] * g.sqrt_adiabind_inv;
+  const double dHdr = (Hout - Hin) * g.InvDiffRsup[i];
+  const double H = cs[l] * g.OmegaInv[i] * g.sqrt_adiabind_inv;
+  double fl = atan(dHdr) - atan((H - 0.4 * g.rstar) * g.InvRmed[i]);
+  fl = (fl >= 0.0) ? sin(fl) : 0.0;
+  if (!(fl > 0.0)) return 0.0;
+  const double r2inv = 1.0 / (g.Rmed[i] * g.Rmed[i]);
+  const double tau = g.opacitydrop * 0.5 * opa[l] * rho[l];
+  const double taueff = 0.375 * tau + 0.5 + 0.25 / tau;
+  const double Tirr4 = g.tirr4fac * r2inv * fl;
+  return 2.0 * g.stefan * Tirr4 / taueff;
+}
+
 __device__ __forceinline__ void b_matrix(const Geo &g, const double *__restrict__ rho,
                                                const double *__restrict__ cs, const double *__restrict__ D,
                                                const double *__restrict__ divv,
@@ -2021,7 +2047,7 @@ __device__ __forceinline__ void b_matrix(const Geo &g, const double *__restrict_
                                                double *__restrict__ Ml, double *__restrict__ Mlip,
                                                double *__restrict__ Mlim, double *__restrict__ Mljp,
                                                double *__restrict__ Mljm, double *__restrict__ RHS, double dt,
-                                               double *__restrict__ presid) {
+                                               double *__restrict__ presid, const double *__restrict__ opa) {
   __shared__ double red[TB];
   const int ns = g.ns;
   int i, j, L;                                      // rings 1..nr-2
@@ -2040,7 +2066,8 @@ __device__ __forceinline__ void b_matrix(const Geo &g, const double *__restrict_
     const double ml = 1.0 + divv[l] * (g.adiabind - 1.0) * dt + 4.0 * bfac * qminus[l] + afac * (Alip + Al) + bfac * (Bljp + Bl);
     const double mlip = -afac * Alip, mlim = -afac * Al, mljp = -bfac * Bljp, mljm = -bfac * Bl;
     const double tl = T[l];
-    const double rhs = tl + bfac * (qplus[l] + 3.0 * qminus[l] * tl);
+    double rhs = tl + bfac * (qplus[l] + 3.0 * qminus[l] * tl);
+    if (g.stellar_irr) rhs += bfac * qirr_cell(g, i, l, cs, opa, rho);   // EnergySources.c:290
     Ml[l] = ml;
     Mlip[l] = mlip;
     Mlim[l] = mlim;
@@ -2060,8 +2087,8 @@ __global__ void __launch_bounds__(TB) k_matrix(Geo g, const double *__restrict__
                                                double *__restrict__ Ml, double *__restrict__ Mlip,
                                                double *__restrict__ Mlim, double *__restrict__ Mljp,
                                                double *__restrict__ Mljm, double *__restrict__ RHS, double dt,
-                                               double *__restrict__ presid) {
-  b_matrix(g, rho, cs, D, divv, qminus, qplus, T, Ml, Mlip, Mlim, Mljp, Mljm, RHS, dt, presid);
+                                               double *__restrict__ presid, const double *__restrict__ opa) {
+  b_matrix(g, rho, cs, D, divv, qminus, qplus, T, Ml, Mlip, Mlim, Mljp, Mljm, RHS, dt, presid, opa);
 }
 
 
@@ -3041,6 +3068,15 @@ extern "C" int thorin_dev_create(thorin_ctx *c) {
   g.damp_flux = p->damp_flux;
   g.opacity_law = p->opacity_law;
   g.stellar_irr = p->stellar_irradiation;
+  {
+    // CalculateFlaring's R_star and CalculateQirr's factor (EnergySources.c:111-117, 152)
+    g.rstar = p->stellarradius * 6.957e8 / TH_AU_SI;
+    double Teff4 = p->effectivetemperature / TH_T2SI;
+    Teff4 = pow(Teff4, 4.0);
+    double Rstar2 = 6.957e8 / TH_AU_SI;
+    Rstar2 = pow(p->stellarradius * Rstar2, 2.0);
+    g.tirr4fac = (1.0 - p->discalbedo) * Teff4 * Rstar2;
+  }
   g.indirect_term = p->indirect_term;
   g.exclude_hill = p->exclude_hill;
   g.hillcut = p->hillcut;
@@ -4240,7 +4276,7 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
       const dim3 gm = cell_grid(d, nr - 2);
       LAUNCH(k_matrix, gm, tb, 0, d->g, d->b[B_DENS], d->b[B_CS], d->b[B_DIFF], d->b[B_DIVV], d->b[B_QMINUS],
              d->b[B_QPLUS], d->b[B_TEMPER], d->b[B_ML], d->b[B_MLIP], d->b[B_MLIM], d->b[B_MLJP], d->b[B_MLJM],
-             d->b[B_RHS], dt, d->mresp);
+             d->b[B_RHS], dt, d->mresp, (const double *)d->b[B_OPACITY]);
       d->nb_mresp = cell_count(d, nr - 2);
       d->normres0_valid = 1;
     }
@@ -5164,7 +5200,7 @@ extern "C" int thorin_rad_system(thorin_ctx *c, double dt, double *dt_used) {
   const dim3 gm = cell_grid(d, nr - 2);
   LAUNCH(k_matrix, gm, tb, 0, d->g, d->b[B_DENS], d->b[B_CS], d->b[B_DIFF], d->b[B_DIVV], d->b[B_QMINUS],
          d->b[B_QPLUS], d->b[B_TEMPER], d->b[B_ML], d->b[B_MLIP], d->b[B_MLIM], d->b[B_MLJP], d->b[B_MLJM],
-         d->b[B_RHS], dt, d->mresp);
+         d->b[B_RHS], dt, d->mresp, (const double *)d->b[B_OPACITY]);
   d->nb_mresp = cell_count(d, nr - 2);
   d->normres0_valid = 1;
   HCHECK(hipMemcpyAsync(d->b[B_TBACK], d->b[B_TEMPER], nb, hipMemcpyDeviceToDevice, d->s));

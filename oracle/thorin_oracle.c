@@ -18,8 +18,9 @@
  *   - closed forms / invariants: opacity laws vs independent restatements,
  *     flux limiter, SOR solution vs a direct sparse solve, mass conservation
  *     of transport, FARGO shift as an exact rotation.
- * Branches that no golden run exercises (stellar irradiation, OpenInner and
- * the other options the library refuses) are "parity unpinned".
+ * Branches that no golden run exercises (OpenInner and the other options the
+ * library refuses) are "parity unpinned"; stellar irradiation is pinned by
+ * the golden run irr_lp85_48x96.
  */
 #define _GNU_SOURCE
 #include "thorin_oracle.h"
@@ -1118,6 +1119,39 @@ static double o_tau_eff(oracle_ctx *o, double tau) {   /* EnergySources.c:701-71
   return 0.375 * tau + 0.25 * sqrt(3.0) + 0.25 / tau;
 }
 
+/* Stellar irradiation of cell (i, j), 1 <= i <= nr-2: CalculateFlaring
+ * (EnergySources.c:143-165: the sine of the grazing angle from the radial
+ * derivative of the pressure scale height, 0 for surfaces tilted away) and
+ * CalculateQirr (EnergySources.c:103-135: the heating it gives through the
+ * effective optical depth).  The edge rings' flaring values of the reference
+ * (EnergySources.c:166-183) feed no matrix row and are not formed. */
+static double o_qirr(oracle_ctx *o, int i, int j, const double *cs, const double *opa, const double *rho) {
+  const thorin_params *p = &o->p;
+  const int ns = o->ns;
+  const int l = j + i * ns, lip = l + ns, lim = l - ns;
+  double Rstar, Teff4, Rstar2, Tirr4fac, csin, csout, Hin, Hout, dHdr, H, fl, tau, taueff, Tirr4, r2inv;
+  Rstar = p->stellarradius * 6.957e8 / AU_SI;
+  Teff4 = p->effectivetemperature / T2SI;
+  Teff4 = pow(Teff4, 4.0);
+  Rstar2 = 6.957e8 / AU_SI;
+  Rstar2 = pow(p->stellarradius * Rstar2, 2.0);
+  Tirr4fac = (1.0 - p->discalbedo) * Teff4 * Rstar2;
+  csin = 0.5 * (cs[l] + cs[lim]);
+  csout = 0.5 * (cs[l] + cs[lip]);
+  Hin = csin * pow(o->Rinf[i], 1.5) * o->sqrt_adiabind_inv;
+  Hout = csout * pow(o->Rsup[i], 1.5) * o->sqrt_adiabind_inv;
+  dHdr = (Hout - Hin) * o->InvDiffRsup[i];
+  H = cs[l] * o->OmegaInv[i] * o->sqrt_adiabind_inv;
+  fl = atan(dHdr) - atan((H - 0.4 * Rstar) * o->InvRmed[i]);
+  fl = (fl >= 0.0) ? sin(fl) : 0.0;
+  if (!(fl > 0.0)) return 0.0;
+  r2inv = 1.0 / (o->Rmed[i] * o->Rmed[i]);
+  tau = p->opacitydrop * 0.5 * opa[l] * rho[l];
+  taueff = o_tau_eff(o, tau);
+  Tirr4 = Tirr4fac * r2inv * fl;
+  return 2.0 * STEFANBOLTZMANN * Tirr4 / taueff;
+}
+
 /* SuccessiveOverrelaxation, EnergySources.c:406-511 (single rank) */
 static int o_sor(oracle_ctx *o, double omega, int errcheck) {
   int n, i, j, l, nr = o->nr, ns = o->ns, ipass, jchess, lip, lim, ljp, ljm, changed;
@@ -1310,6 +1344,7 @@ static int o_implicit_diffusion(oracle_ctx *o, double dt, double *omega_used) {
       Mljp[l] = -bfac * B[ljp];
       Mljm[l] = -bfac * B[l];
       RHS[l] = T[l] + bfac * (qplus[l] + 3.0 * qminus[l] * T[l]);
+      if (p->stellar_irradiation) RHS[l] += bfac * o_qirr(o, i, j, cs, opa, rho);   /* EnergySources.c:290 */
     }
   if (o->st.sor_first) {
     o->st.sor_first = 0;

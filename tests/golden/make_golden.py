@@ -58,6 +58,10 @@ CASES = {
                                                PebbleAccretion="YES", BackReaction="YES",
                                                PlanetsFeelDisk="YES", WriteEnergy="YES"),
                              [(3e-5, 1.0)], True),
+    # stellar irradiation (flaring surface, Qirr in the energy equation's RHS)
+    "irr_lp85_48x96": (configs.with_grid(configs.PLAIN_128x256, 48, 96, Ntot=8, Ninterm=4, StellarIrradiation="YES",
+                                         WriteEnergy="YES", WriteDivV="YES", WriteQplus="YES", WriteOpacity="YES"),
+                       configs.PLANET_ZERO_MASS, True),
     # the 128x256 / 200-step workload itself: checksums and ring profiles only
     "plain_128x256_200": (dict(configs.PLAIN_128x256, Ninterm="100", WriteEnergy="YES"),
                           configs.PLANET_ZERO_MASS, False),
