@@ -297,6 +297,41 @@ static int ctx_create(const thorin_params *p, thorin_ctx **out, int with_device)
       if (c->Rmed[i] > DRout) { damp = (c->Rmed[i] - DRout) / (p->rmax - DRout); c->WaveKiller[i] = damp / tauout; }
     }
   }
+  /* InitOpacities (Opacity2.c:252-300): the gas table of OpacityTable SM03,
+   * INPUTDIR/data_gasopacity_malygin.txt; only its Rosseland part is used */
+  if (p->opacity_law == THORIN_OPA_SM03) {
+    char name[2 * THORIN_STRLEN];
+    FILE *in;
+    int ok, k;
+    double v;
+    snprintf(name, sizeof name, "%s%s", p->inputdir, "data_gasopacity_malygin.txt");
+    in = fopen(name, "r");
+    if (!in) {
+      char m[2 * THORIN_STRLEN + 64];
+      snprintf(m, sizeof m, "OpacityTable SM03: opacity file '%s' not found", name);
+      thorin_set_error(m);
+      thorin_ctx_destroy(c);
+      return -1;
+    }
+    ok = fscanf(in, "%i", &c->sm_rows) == 1 && fscanf(in, "%i", &c->sm_cols) == 1 && c->sm_rows > 1 &&
+         c->sm_cols > 2 && c->sm_rows <= 4096 && c->sm_cols <= 4096;
+    if (ok) {
+      c->sm_pgrid = alloc1(c->sm_rows);
+      c->sm_tgrid = alloc1(c->sm_cols);
+      c->sm_ross = alloc1(c->sm_rows * c->sm_cols);
+      ok = c->sm_pgrid && c->sm_tgrid && c->sm_ross;
+      for (k = 0; ok && k < c->sm_rows; k++) ok = fscanf(in, "%lf", &c->sm_pgrid[k]) == 1;
+      for (k = 0; ok && k < c->sm_cols; k++) ok = fscanf(in, "%lf", &c->sm_tgrid[k]) == 1;
+      for (k = 0; ok && k < c->sm_rows * c->sm_cols; k++) ok = fscanf(in, "%lf", &v) == 1;   /* Planck */
+      for (k = 0; ok && k < c->sm_rows * c->sm_cols; k++) ok = fscanf(in, "%lf", &c->sm_ross[k]) == 1;
+    }
+    fclose(in);
+    if (!ok) {
+      thorin_set_error("OpacityTable SM03: the opacity file is short or malformed");
+      thorin_ctx_destroy(c);
+      return -1;
+    }
+  }
   if (with_device && thorin_dev_create(c) != 0) { thorin_ctx_destroy(c); return -1; }
   *out = c;
   return 0;
@@ -315,6 +350,7 @@ int thorin_ctx_destroy(thorin_ctx *c) {
   free(c->supp_torque);
   free(c->h_dens); free(c->h_vrad); free(c->h_vtheta); free(c->h_energy); free(c->h_label);
   free(c->h_cs); free(c->h_press); free(c->h_temper); free(c->h_abs); free(c->h_ord); free(c->h_trig);
+  free(c->sm_pgrid); free(c->sm_tgrid); free(c->sm_ross);
   free(c);
   return 0;
 }

@@ -186,10 +186,8 @@ int thorin_params_derive(thorin_params *p) {
   else if (p->powerlawopacity > 0.0) p->opacity_law = THORIN_OPA_POWERLAW;
   else if (p->opacitytable[0] == 'Z') p->opacity_law = THORIN_OPA_ZHU12;
   else if (p->opacitytable[0] == 'L') p->opacity_law = THORIN_OPA_LP85;
-  else if (p->opacitytable[0] == 'S') {
-    thorin_set_error("OpacityTable SM03 (Opacity2.c) is not supported by this build.");
-    return -1;
-  } else p->opacity_law = THORIN_OPA_BL94;
+  else if (p->opacitytable[0] == 'S') p->opacity_law = THORIN_OPA_SM03;
+  else p->opacity_law = THORIN_OPA_BL94;
   return 0;
 }
 

@@ -138,6 +138,10 @@ struct Geo {                    // passed by value to every kernel
   double powerlawopacity, gasflux, omega_frame, lr01, lr12, thicksmooth;
   int visc_alpha, alpha_flock, fast_transport, damp_init, damp_flux, opacity_law, stellar_irr;
   double rstar, tirr4fac;       // stellar irradiation: R_star in AU and (1-A) Teff^4 R_star^2 in code units
+  // OpacityTable SM03: the gas table (rows pressures x cols temperatures)
+  const double *sm_pg, *sm_tg, *sm_ross;
+  int sm_rows, sm_cols;
+  double sm_fac;                // rows / log10(p_max / p_min), as the reference forms it per call
   int indirect_term, exclude_hill;
   double hillcut;
   // LP85 thresholds folded on the host (Opacity.c:205-220)
@@ -176,6 +180,7 @@ struct thorin_dev {
   double *b[NBUF];
   double *d1base;
   double *trig;                 // cos, sin of the sector angles (2*NSEC)
+  double *sm;                   // OpacityTable SM03: pressure grid, temperature grid, Rosseland table
   Geo g;
   DevCtl *ctl;                  // device
   DevCtl *hctl;                 // pinned host mirror
@@ -322,8 +327,114 @@ __device__ __forceinline__ double opacity_zhu12(double rho, double T) {
   return pow(10., xlop);
 }
 
+// OpacityTable SM03 (Opacity2.c:24-245): the Rosseland mean of Semenov et
+// al. (2003) dust and Malygin et al. (2014) gas.  The dust coefficients and
+// evaporation temperatures are the published tables the reference holds
+// (Opacity2.c:107-137).
+__device__ __forceinline__ double sm_interp(const double *x, const double *y, int n, double x0) {   // Opacity2.c:28-40
+  if (n <= 1) return y[0];
+  if (x0 < x[0]) return y[0];
+  if (x0 > x[n - 1]) return y[n - 1];
+  int i = 1;
+  while ((x[i] < x0) && (i < n - 1)) i++;
+  return y[i - 1] + (y[i] - y[i - 1]) * (x0 - x[i - 1]) / (x[i] - x[i - 1]);
+}
+
+// opacity_malygin (Opacity2.c:44-95), Rosseland table: bilinear in the
+// pressure-temperature grid.  The temperature cell is the last i <= cols-3
+// with T >= tgrid[i] (the reference's scan; a binary search on the ascending
+// grid finds the same i); the pressure cell comes from the log-uniform
+// index estimate, clamped
+__device__ __forceinline__ double sm_gas(const Geo &g, double rho, double T_gas) {
+  const int rows = g.sm_rows, cols = g.sm_cols;
+  const double *pg = g.sm_pg, *tg = g.sm_tg, *a = g.sm_ross;
+  double tP = 8.3144598 * 1.0e7 / 2.4 * rho * T_gas;
+  double tT = T_gas;
+  if (tT < tg[0]) return 0.0;
+  else if (tT > tg[cols - 1]) tT = tg[cols - 1];
+  if (tP < pg[0]) tP = pg[0];
+  else if (tP > pg[rows - 1]) tP = pg[rows - 1];
+  int lo = 0, hi = cols - 3;               // last i in [0, cols-3] with tg[i] <= tT (tg[0] <= tT holds)
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (tT >= tg[mid]) lo = mid;
+    else hi = mid - 1;
+  }
+  const int Tlow = lo, Thigh = lo + 1;
+  int Plow = (int)(g.sm_fac * log10(tP / pg[0]));
+  Plow = Plow < 0 ? 0 : (Plow > rows - 2 ? rows - 2 : Plow);
+  const int Phigh = Plow + 1;
+  const double denom = 1.0 / ((tg[Thigh] - tg[Tlow]) * (pg[Phigh] - pg[Plow]));
+  const double mul1 = (tg[Thigh] - tT) * (pg[Phigh] - tP);
+  const double mul2 = (tT - tg[Tlow]) * (pg[Phigh] - tP);
+  const double mul3 = (tg[Thigh] - tT) * (tP - pg[Plow]);
+  const double mul4 = (tT - tg[Tlow]) * (tP - pg[Plow]);
+  return denom * (mul1 * a[Tlow + Plow * cols] + mul2 * a[Thigh + Plow * cols] + mul3 * a[Tlow + Phigh * cols] +
+                  mul4 * a[Thigh + Phigh * cols]);
+}
+
+__device__ __forceinline__ double opacity_sm03(const Geo &g, double rho, double T_dust) {   // Opacity2.c:99-245
+  static const double eD[36] = {
+    0.244153190819473e-03, -0.191588103585833e-03, 0.229132294521843e-03, 0.454088516995579e-06, -0.389280273285906e-08, 0.319006401785413e-11,
+    0.292517586013911e+01, -0.801305197566973e-01, 0.923001367150898e-03, -0.349287851393541e-05, 0.587151269508960e-08, -0.371333580736933e-11,
+    -0.125893536782728e+02, 0.160197849829283e+00, -0.582601311634824e-03, 0.110792133181219e-05, -0.105506373600323e-08, 0.404997080931974e-12,
+    -0.192550086994197e+01, 0.354301116460647e-01, -0.127355043519226e-03, 0.233773506650911e-06, -0.207587683535083e-09, 0.728005983960845e-13,
+    0.165244978116957e+01, -0.260479348963077e-02, 0.590717655258634e-05, -0.300202906476749e-08, 0.803836688553263e-12, -0.916709776405312e-16,
+    0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  static const double logtt[8][4] = {
+    {2.03742650, 2.92168648, 2.95520654, 2.96801571}, {2.07188201, 2.95808585, 2.99122608, 2.99869516},
+    {2.11058971, 2.99738638, 3.02077549, 3.03181227}, {2.15533604, 3.04139269, 3.05269394, 3.06744284},
+    {2.20139712, 3.08990511, 3.08707121, 3.10619090}, {2.25527251, 3.14457421, 3.12417806, 3.14860265},
+    {2.31597035, 3.20736504, 3.16494737, 3.19589965}, {2.38738983, 3.28057837, 3.20978301, 3.24895362}};
+  static const double logrr[8] = {-18.0, -16.0, -14.0, -12.0, -10.0, -8.0, -6.0, -4.0};
+  if (T_dust <= 0.9) return 0.9;
+  const double logrho = log10(rho);
+  double T_ev[4], temp[8];
+  for (int i = 0; i < 4; i++) {
+    for (int j = 0; j < 8; j++) temp[j] = logtt[j][i];
+    T_ev[i] = pow(10.0, sm_interp(logrr, temp, 8, logrho));
+  }
+  double T[5], dT[5];
+  T[0] = T_ev[0]; T[1] = 275.0; T[2] = 425.0; T[3] = 680.0;
+  const double tmax1 = T_ev[1] > T_ev[2] ? T_ev[1] : T_ev[2];
+  const double tmax2 = T_ev[2] > T_ev[3] ? T_ev[2] : T_ev[3];
+  T[4] = tmax1 < tmax2 ? tmax1 : tmax2;
+  dT[0] = 5.0; dT[1] = 5.0; dT[2] = 15.0; dT[3] = 5.0; dT[4] = 100.0;
+  int KK = 5;
+  if (T_dust <= (T[0] + dT[0])) KK = 0;
+  for (int i = 1; i < 5; i++)
+    if ((T_dust > (T[i - 1] + dT[i - 1])) && (T_dust <= (T[i] + dT[i]))) KK = i;
+  double aKext = 0.0;
+  if (KK != 5) {
+    int smooth = 0;
+    for (int i = 0; i < 5; i++)       // the reference passes the double difference to the int abs()
+      if (abs((int)(T_dust - T[i])) < dT[i]) smooth = 1;
+    if (smooth == 1) {
+      const double T1 = T[KK] - dT[KK], T2 = T[KK] + dT[KK], TD = T_dust - T[KK];
+      int j = (KK + 1) * 6;
+      const double aKrL = ((((eD[j - 1] * T1 + eD[j - 2]) * T1 + eD[j - 3]) * T1 + eD[j - 4]) * T1 + eD[j - 5]) * T1 +
+                          eD[j - 6];
+      double aKrR = 0.0;
+      if (KK != 4) {
+        j = (KK + 2) * 6;
+        aKrR = ((((eD[j - 1] * T2 + eD[j - 2]) * T2 + eD[j - 3]) * T2 + eD[j - 4]) * T2 + eD[j - 5]) * T2 + eD[j - 6];
+      }
+      const double AA = 0.5e0 * (aKrL - aKrR), BB = 0.5e0 * (aKrL + aKrR);
+      const double FF = DPI / 2.0 / dT[KK];
+      aKext = BB - AA * sin(FF * TD);
+    } else {
+      const int j = (KK + 1) * 6;
+      aKext = ((((eD[j - 1] * T_dust + eD[j - 2]) * T_dust + eD[j - 3]) * T_dust + eD[j - 4]) * T_dust + eD[j - 5]) *
+                  T_dust + eD[j - 6];
+    }
+  }
+  const double kappa_gas = sm_gas(g, rho, T_dust);
+  return aKext > kappa_gas ? aKext : kappa_gas;
+}
+
 __device__ __forceinline__ double opacity_law(const Geo &g, double rho, double T) {
   switch (g.opacity_law) {
+    case THORIN_OPA_SM03: return opacity_sm03(g, rho, T);
     case THORIN_OPA_LP85: return opacity_lp85(g, rho, T);
     case THORIN_OPA_ZHU12: return opacity_zhu12(rho, T);
     case THORIN_OPA_CONST: return g.parametricopacity;
@@ -3110,6 +3221,21 @@ extern "C" int thorin_dev_create(thorin_ctx *c) {
   HCHECK(hipMemcpyAsync(d->trig, c->h_trig, sizeof(double) * 2 * (size_t)d->ns, hipMemcpyHostToDevice, d->s));
   d->g.cosj = d->trig;
   d->g.sinj = d->trig + d->ns;
+  if (c->p.opacity_law == THORIN_OPA_SM03) {
+    const int rows = c->sm_rows, cols = c->sm_cols;
+    const size_t n = (size_t)rows + cols + (size_t)rows * cols;
+    HCHECK(hipMalloc(&d->sm, sizeof(double) * n));
+    HCHECK(hipMemcpyAsync(d->sm, c->sm_pgrid, sizeof(double) * rows, hipMemcpyHostToDevice, d->s));
+    HCHECK(hipMemcpyAsync(d->sm + rows, c->sm_tgrid, sizeof(double) * cols, hipMemcpyHostToDevice, d->s));
+    HCHECK(hipMemcpyAsync(d->sm + rows + cols, c->sm_ross, sizeof(double) * rows * cols, hipMemcpyHostToDevice,
+                          d->s));
+    d->g.sm_pg = d->sm;
+    d->g.sm_tg = d->sm + rows;
+    d->g.sm_ross = d->sm + rows + cols;
+    d->g.sm_rows = rows;
+    d->g.sm_cols = cols;
+    d->g.sm_fac = ((double)rows) / log10(c->sm_pgrid[rows - 1] / c->sm_pgrid[0]);
+  }
   HCHECK(hipStreamSynchronize(d->s));
   return 0;
 }
@@ -3126,6 +3252,7 @@ extern "C" void thorin_dev_destroy(thorin_ctx *c) {
   if (d->red) (void)hipFree(d->red);
   if (d->sorp) (void)hipFree(d->sorp);
   if (d->trig) (void)hipFree(d->trig);
+  if (d->sm) (void)hipFree(d->sm);
   if (d->sorc) (void)hipFree(d->sorc);
   if (d->pstellar) (void)hipFree(d->pstellar);
   if (d->mresp) (void)hipFree(d->mresp);

@@ -56,6 +56,8 @@ struct thorin_ctx {
   double *h_dens, *h_vrad, *h_vtheta, *h_energy, *h_label, *h_cs, *h_press, *h_temper;
   double *h_abs, *h_ord;
   double *h_trig;               /* cos(2 pi j/NSEC), then sin (InitComputeAccel's angles) */
+  int sm_rows, sm_cols;          /* OpacityTable SM03: the gas table's pressure and temperature grids */
+  double *sm_pgrid, *sm_tgrid, *sm_ross;   /* and its Rosseland values (InitOpacities, Opacity2.c:252-300) */
   /* pebble fluid (allocated by thorin_init_pebbles): host copies of the
    * initial pebble fields and the damping targets PebDensInit, PebVradInit,
    * PebVthetaInit (global.h, Pebbles.c:190-193, 206-232) */

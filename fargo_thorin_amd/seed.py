@@ -85,6 +85,18 @@ def write_cfg(path: str, arr: np.ndarray) -> None:
         f.write("\n")
 
 
+def write_opacity_table(path: str, rows: int, cols: int, pgrid, tgrid, planck, rosseland) -> None:
+    """The gas opacity table of OpacityTable SM03 in the layout its reader
+    takes (InitOpacities, Opacity2.c:252-300): rows, cols, then the pressure
+    grid, the temperature grid, the Planck and the Rosseland values, one
+    number per line, each written so that %lf reads back the same double."""
+    with open(path, "w") as f:
+        f.write(f"{int(rows)}\n{int(cols)}\n")
+        for a in (pgrid, tgrid, planck, rosseland):
+            f.write("\n".join(repr(float(v)) for v in np.ravel(a)))
+            f.write("\n")
+
+
 # T2SI of fondam.h: code temperature unit in K (G Msun / (R/mu) / AU)
 T2SI = 1.32712440018e20 / (8.3144598 / (2.4 * 0.001)) / 149597870700.0
 

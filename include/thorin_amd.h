@@ -45,7 +45,7 @@ extern "C" {
 /* Opacity laws (Interpret.c:250-265, Opacity.c).  POWERLAW (kappa = k0*T^2 in
  * cgs, k0 = PowerLawOpacity) has no counterpart in the reference. */
 enum { THORIN_OPA_BL94 = 0, THORIN_OPA_LP85 = 1, THORIN_OPA_ZHU12 = 2,
-       THORIN_OPA_CONST = 3, THORIN_OPA_POWERLAW = 4 };
+       THORIN_OPA_CONST = 3, THORIN_OPA_POWERLAW = 4, THORIN_OPA_SM03 = 5 };
 
 /* Every in.par key of var.c:19-139, plus the flags ReadVariables derives
  * from them (Interpret.c:149-265). */

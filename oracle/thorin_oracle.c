@@ -68,6 +68,8 @@ enum { W_RHOSTAR = NF, W_RHOINT, W_TEMPERINT, W_DRR, W_DRP, W_DPP, W_RADMOMP, W_
 struct oracle_ctx {
   thorin_params p;
   int nr, ns;
+  int sm_rows, sm_cols;                 /* OpacityTable SM03: the gas table (Opacity2.c:22-23) */
+  double *sm_pgrid, *sm_tgrid, *sm_ross;
   double *fld[NW];
   double *Radii, *Rinf, *Rsup, *Rmed, *Surf, *InvRmed, *InvSurf, *InvDiffRmed, *InvDiffRsup;
   double *InvRinf, *OmegaInv, *Rmed2, *GlobalRmed, *SigmaMed, *SigmaInf, *EnergyMed;
@@ -269,6 +271,30 @@ int oracle_create(const thorin_params *p, oracle_ctx **out) {
       if (o->Rmed[i] > DRout) { damp = (o->Rmed[i] - DRout) / (p->rmax - DRout); o->WaveKiller[i] = damp / tauout; }
     }
   }
+  /* InitOpacities (Opacity2.c:252-300): the gas table, Rosseland part */
+  if (p->opacity_law == THORIN_OPA_SM03) {
+    char name[2 * THORIN_STRLEN];
+    FILE *in;
+    int i, ok = 1;
+    double *planck;
+    snprintf(name, sizeof name, "%s%s", p->inputdir, "data_gasopacity_malygin.txt");
+    in = fopen(name, "r");
+    if (!in) { oracle_destroy(o); return -3; }
+    ok = fscanf(in, "%i", &o->sm_rows) == 1 && fscanf(in, "%i", &o->sm_cols) == 1 && o->sm_rows > 1 && o->sm_cols > 2;
+    if (ok) {
+      o->sm_pgrid = (double *)malloc(sizeof(double) * o->sm_rows);
+      o->sm_tgrid = (double *)malloc(sizeof(double) * o->sm_cols);
+      o->sm_ross = (double *)malloc(sizeof(double) * o->sm_rows * o->sm_cols);
+      planck = (double *)malloc(sizeof(double) * o->sm_rows * o->sm_cols);
+      for (i = 0; ok && i < o->sm_rows; i++) ok = fscanf(in, "%lf", &o->sm_pgrid[i]) == 1;
+      for (i = 0; ok && i < o->sm_cols; i++) ok = fscanf(in, "%lf", &o->sm_tgrid[i]) == 1;
+      for (i = 0; ok && i < o->sm_rows * o->sm_cols; i++) ok = fscanf(in, "%lf", &planck[i]) == 1;
+      for (i = 0; ok && i < o->sm_rows * o->sm_cols; i++) ok = fscanf(in, "%lf", &o->sm_ross[i]) == 1;
+      free(planck);
+    }
+    fclose(in);
+    if (!ok) { oracle_destroy(o); return -3; }
+  }
   /* InitLabel, Init.c:56-70 */
   {
     int i, j;
@@ -284,6 +310,7 @@ void oracle_destroy(oracle_ctx *o) {
   int k;
   if (!o) return;
   for (k = 0; k < NW; k++) free(o->fld[k]);
+  free(o->sm_pgrid); free(o->sm_tgrid); free(o->sm_ross);
   free(o->Radii); free(o->Rinf); free(o->Rsup); free(o->Rmed); free(o->Surf); free(o->InvRmed);
   free(o->InvSurf); free(o->InvDiffRmed); free(o->InvDiffRsup); free(o->InvRinf); free(o->OmegaInv);
   free(o->Rmed2); free(o->GlobalRmed); free(o->SigmaMed); free(o->SigmaInf); free(o->EnergyMed);
@@ -1084,8 +1111,109 @@ static double o_opacity_ZHU12(double rho, double T) {  /* Opacity.c:145-196 */
   return pow(10., xlop);
 }
 
+/* OpacityTable SM03 (Opacity2.c:24-245): the Rosseland mean of Semenov et al.
+ * (2003) dust and Malygin et al. (2014) gas.  The coefficient tables are the
+ * published ones the reference holds (Opacity2.c:107-137). */
+static double o_interp(const double *x, const double *y, int n, double x0) {   /* Opacity2.c:28-40 */
+  int i = 1;
+  if (n <= 1) return y[0];
+  if (x0 < x[0]) return y[0];
+  if (x0 > x[n - 1]) return y[n - 1];
+  while ((x[i] < x0) && (i < n - 1)) i++;
+  return y[i - 1] + (y[i] - y[i - 1]) * (x0 - x[i - 1]) / (x[i] - x[i - 1]);
+}
+
+static double o_opacity_malygin(const oracle_ctx *o, double rho, double T_gas) {   /* Opacity2.c:44-95, Rosseland */
+  const int rows = o->sm_rows, cols = o->sm_cols;
+  const double *pg = o->sm_pgrid, *tg = o->sm_tgrid, *a = o->sm_ross;
+  double tP = 8.3144598 * 1.0e7 / 2.4 * rho * T_gas, tT = T_gas, fac, denom, mul1, mul2, mul3, mul4;
+  int i, Tlow = 0, Thigh = 1, Plow, Phigh;
+  if (tT < tg[0]) return 0.0;
+  else if (tT > tg[cols - 1]) tT = tg[cols - 1];
+  if (tP < pg[0]) tP = pg[0];
+  else if (tP > pg[rows - 1]) tP = pg[rows - 1];
+  for (i = 0; i < cols - 2; i++)
+    if (tT >= tg[i]) { Tlow = i; Thigh = i + 1; }
+  fac = ((double)rows) / log10(pg[rows - 1] / pg[0]);
+  Plow = (int)(fac * log10(tP / pg[0]));
+  if (Plow < 0) Plow = 0;
+  if (Plow > rows - 2) Plow = rows - 2;
+  Phigh = Plow + 1;
+  denom = 1.0 / ((tg[Thigh] - tg[Tlow]) * (pg[Phigh] - pg[Plow]));
+  mul1 = (tg[Thigh] - tT) * (pg[Phigh] - tP);
+  mul2 = (tT - tg[Tlow]) * (pg[Phigh] - tP);
+  mul3 = (tg[Thigh] - tT) * (tP - pg[Plow]);
+  mul4 = (tT - tg[Tlow]) * (tP - pg[Plow]);
+  return denom * (mul1 * a[Tlow + Plow * cols] + mul2 * a[Thigh + Plow * cols] + mul3 * a[Tlow + Phigh * cols] +
+                  mul4 * a[Thigh + Phigh * cols]);
+}
+
+static double o_opacity_SM03(const oracle_ctx *o, double rho, double T_dust) {   /* Opacity2.c:99-245 */
+  static const double eD[36] = {
+    0.244153190819473e-03, -0.191588103585833e-03, 0.229132294521843e-03, 0.454088516995579e-06, -0.389280273285906e-08, 0.319006401785413e-11,
+    0.292517586013911e+01, -0.801305197566973e-01, 0.923001367150898e-03, -0.349287851393541e-05, 0.587151269508960e-08, -0.371333580736933e-11,
+    -0.125893536782728e+02, 0.160197849829283e+00, -0.582601311634824e-03, 0.110792133181219e-05, -0.105506373600323e-08, 0.404997080931974e-12,
+    -0.192550086994197e+01, 0.354301116460647e-01, -0.127355043519226e-03, 0.233773506650911e-06, -0.207587683535083e-09, 0.728005983960845e-13,
+    0.165244978116957e+01, -0.260479348963077e-02, 0.590717655258634e-05, -0.300202906476749e-08, 0.803836688553263e-12, -0.916709776405312e-16,
+    0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  static const double logtt[8][4] = {
+    {2.03742650, 2.92168648, 2.95520654, 2.96801571}, {2.07188201, 2.95808585, 2.99122608, 2.99869516},
+    {2.11058971, 2.99738638, 3.02077549, 3.03181227}, {2.15533604, 3.04139269, 3.05269394, 3.06744284},
+    {2.20139712, 3.08990511, 3.08707121, 3.10619090}, {2.25527251, 3.14457421, 3.12417806, 3.14860265},
+    {2.31597035, 3.20736504, 3.16494737, 3.19589965}, {2.38738983, 3.28057837, 3.20978301, 3.24895362}};
+  static const double logrr[8] = {-18.0, -16.0, -14.0, -12.0, -10.0, -8.0, -6.0, -4.0};
+  double T_ev[4], temp[8], dT[5], T[5], tmax1, tmax2, tmin, T1, T2, TD, aKrL, aKrR, AA, BB, FF, aKext = 0.0;
+  double logrho, kappa_gas;
+  int i, j, KK, smooth;
+  if (T_dust <= 0.9) return 0.9;
+  logrho = log10(rho);
+  for (i = 0; i < 4; i++) {
+    for (j = 0; j < 8; j++) temp[j] = logtt[j][i];
+    T_ev[i] = pow(10.0, o_interp(logrr, temp, 8, logrho));
+  }
+  T[0] = T_ev[0]; T[1] = 275.0; T[2] = 425.0; T[3] = 680.0;
+  tmax1 = T_ev[1] > T_ev[2] ? T_ev[1] : T_ev[2];
+  tmax2 = T_ev[2] > T_ev[3] ? T_ev[2] : T_ev[3];
+  tmin = tmax1 < tmax2 ? tmax1 : tmax2;
+  T[4] = tmin;
+  dT[0] = 5.0; dT[1] = 5.0; dT[2] = 15.0; dT[3] = 5.0; dT[4] = 100.0;
+  KK = 5;
+  if (T_dust <= (T[0] + dT[0])) KK = 0;
+  for (i = 1; i < 5; i++)
+    if ((T_dust > (T[i - 1] + dT[i - 1])) && (T_dust <= (T[i] + dT[i]))) KK = i;
+  if (KK != 5) {
+    smooth = 0;
+    /* the reference passes the double difference to the int abs() */
+    for (i = 0; i < 5; i++)
+      if (abs((int)(T_dust - T[i])) < dT[i]) smooth = 1;
+    if (smooth == 1) {
+      T1 = T[KK] - dT[KK];
+      T2 = T[KK] + dT[KK];
+      TD = T_dust - T[KK];
+      j = (KK + 1) * 6;
+      aKrL = ((((eD[j - 1] * T1 + eD[j - 2]) * T1 + eD[j - 3]) * T1 + eD[j - 4]) * T1 + eD[j - 5]) * T1 + eD[j - 6];
+      if (KK == 4) aKrR = 0.0;
+      else {
+        j = (KK + 2) * 6;
+        aKrR = ((((eD[j - 1] * T2 + eD[j - 2]) * T2 + eD[j - 3]) * T2 + eD[j - 4]) * T2 + eD[j - 5]) * T2 + eD[j - 6];
+      }
+      AA = 0.5e0 * (aKrL - aKrR);
+      BB = 0.5e0 * (aKrL + aKrR);
+      FF = PI / 2.0 / dT[KK];
+      aKext = BB - AA * sin(FF * TD);
+    } else {
+      j = (KK + 1) * 6;
+      aKext = ((((eD[j - 1] * T_dust + eD[j - 2]) * T_dust + eD[j - 3]) * T_dust + eD[j - 4]) * T_dust + eD[j - 5]) *
+                  T_dust + eD[j - 6];
+    }
+  }
+  kappa_gas = o_opacity_malygin(o, rho, T_dust);
+  return aKext > kappa_gas ? aKext : kappa_gas;
+}
+
 static double o_opacity(oracle_ctx *o, double rho, double T) {
   switch (o->p.opacity_law) {
+    case THORIN_OPA_SM03: return o_opacity_SM03(o, rho, T);
     case THORIN_OPA_LP85: return o_opacity_LP85(rho, T);
     case THORIN_OPA_ZHU12: return o_opacity_ZHU12(rho, T);
     case THORIN_OPA_CONST: return o->p.parametricopacity;

@@ -25,10 +25,25 @@ ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__)
 sys.path.insert(0, ROOT)
 
 from fargo_thorin_amd import configs  # noqa: E402
-from fargo_thorin_amd.seed import seeded_disk, write_cfg  # noqa: E402
+from fargo_thorin_amd.seed import seeded_disk, write_cfg, write_opacity_table  # noqa: E402
 
 REF = os.path.join(ROOT, "oracle", "_ref", "thorin")
 OUT = os.path.join(ROOT, "tests", "golden")
+# the reference's gas opacity table of OpacityTable SM03 (data), kept as the
+# fixture sm03_gas_table.npz
+MALYGIN = "/root/reference/in/data_gasopacity_malygin.txt"
+TABLE = os.path.join(OUT, "sm03_gas_table.npz")
+
+
+def gas_table():
+    if not os.path.exists(TABLE):
+        tok = open(MALYGIN).read().split()
+        rows, cols = int(tok[0]), int(tok[1])
+        v = np.array([float(x) for x in tok[2:2 + rows + cols + 2 * rows * cols]])
+        np.savez_compressed(TABLE, rows=rows, cols=cols, pgrid=v[:rows], tgrid=v[rows:rows + cols],
+                            planck=v[rows + cols:rows + cols + rows * cols],
+                            rosseland=v[rows + cols + rows * cols:])
+    return np.load(TABLE)
 
 FIELDS = ["dens", "vrad", "vtheta", "temper", "energy", "divv", "qplus", "opacity",
           "pebbledens", "pebblevrad", "pebblevtheta", "pebblesize", "pebblestokes"]
@@ -62,6 +77,11 @@ CASES = {
     "irr_lp85_48x96": (configs.with_grid(configs.PLAIN_128x256, 48, 96, Ntot=8, Ninterm=4, StellarIrradiation="YES",
                                          WriteEnergy="YES", WriteDivV="YES", WriteQplus="YES", WriteOpacity="YES"),
                        configs.PLANET_ZERO_MASS, True),
+    # Semenov et al. 2003 dust + Malygin et al. 2014 gas opacity (OpacityTable SM03),
+    # damping zones, T 20-2000 K so that the dust regimes and the gas table both act
+    "damp_sm03_48x96": (configs.with_grid(configs.DAMPED_1024x1536, 48, 96, Ntot=8, Ninterm=4, OpacityTable="SM03",
+                                          WriteEnergy="YES", WriteDivV="YES", WriteQplus="YES",
+                                          WriteOpacity="YES"), configs.PLANET_ZERO_MASS, True),
     # the 128x256 / 200-step workload itself: checksums and ring profiles only
     "plain_128x256_200": (dict(configs.PLAIN_128x256, Ninterm="100", WriteEnergy="YES"),
                           configs.PLANET_ZERO_MASS, False),
@@ -78,6 +98,10 @@ def run_case(name, cfg, planets, full):
         disk = seeded_disk(nrad, nsec, log_grid=cfg.get("RadialSpacing", "LOG")[0] in "lL")
         for k in ("dens", "vrad", "vtheta", "temper"):
             write_cfg(os.path.join(cfgdir, f"gas{k}.cfg"), disk[k])
+        if cfg.get("OpacityTable", "")[:1] in "sS":
+            t = gas_table()
+            write_opacity_table(os.path.join(cfgdir, "data_gasopacity_malygin.txt"), t["rows"], t["cols"],
+                                t["pgrid"], t["tgrid"], t["planck"], t["rosseland"])
         pf = os.path.join(tmp, "planet.cfg")
         with open(pf, "w") as f:
             f.write(configs.planet_text(planets))

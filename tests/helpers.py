@@ -17,10 +17,31 @@ def golden(name: str):
     return np.load(os.path.join(GOLDEN, f"{name}.npz"))
 
 
+_TABLE_DIR = None
+
+
+def opacity_table_dir() -> str:
+    """A directory holding the SM03 gas opacity table (the reference's
+    data_gasopacity_malygin.txt, from the fixture sm03_gas_table.npz) in the
+    layout its reader takes; made once per test session."""
+    global _TABLE_DIR
+    if _TABLE_DIR is None:
+        import tempfile
+
+        from fargo_thorin_amd.seed import write_opacity_table
+        t = np.load(os.path.join(GOLDEN, "sm03_gas_table.npz"))
+        _TABLE_DIR = tempfile.mkdtemp(prefix="thorin_sm03_")
+        write_opacity_table(os.path.join(_TABLE_DIR, "data_gasopacity_malygin.txt"), t["rows"], t["cols"],
+                            t["pgrid"], t["tgrid"], t["planck"], t["rosseland"])
+    return _TABLE_DIR + "/"
+
+
 def params_for(cfg: dict, **extra) -> Params:
     keys = dict(cfg)
     keys.update({k: str(v) for k, v in extra.items()})
     keys.setdefault("OutputDir", "/nonexistent-thorin-out/")
+    if str(keys.get("OpacityTable", ""))[:1] in "sS":
+        keys.setdefault("InputDir", opacity_table_dir())
     return Params(**keys)
 
 

@@ -21,7 +21,8 @@ from fargo_thorin_amd._capi import host_nbody_fn
 pytestmark = pytest.mark.gpu
 
 TOL = 1e-10
-FULL_CASES = ["wall_lp85_64x128", "damp_bl94_48x96", "dampinit_zhu12_40x64", "planet_q3e-5_48x96", "irr_lp85_48x96"]
+FULL_CASES = ["wall_lp85_64x128", "damp_bl94_48x96", "dampinit_zhu12_40x64", "planet_q3e-5_48x96", "irr_lp85_48x96",
+              "damp_sm03_48x96"]
 # fields whose values are differences of nearly equal terms: compared
 # relative to the field's own scale
 STATE_FIELDS = ["dens", "vrad", "vtheta", "temper", "energy", "divv", "qplus", "opacity"]
@@ -72,7 +73,8 @@ def test_gpu_vs_reference_golden(name):
     assert np.array_equal(np.array(times[:ntot]), g["tqwk0"][:ntot, 0])
 
 
-@pytest.mark.parametrize("name,steps", [("wall_lp85_64x128", 25), ("damp_bl94_48x96", 25), ("irr_lp85_48x96", 25)])
+@pytest.mark.parametrize("name,steps", [("wall_lp85_64x128", 25), ("damp_bl94_48x96", 25), ("irr_lp85_48x96", 25),
+                                        ("damp_sm03_48x96", 25)])
 def test_gpu_vs_oracle_stepwise(name, steps):
     """Same inputs and the same host N-body for both: per DT the CFL substep
     counts, dt, SOR iterations and Nshift agree exactly and the fields to 1e-10."""

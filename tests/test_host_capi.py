@@ -60,6 +60,23 @@ def test_out_of_scope_options_are_refused(keys, msg):
     assert rc != 0 and msg.lower() in last_error().lower()
 
 
+def test_sm03_opacity_needs_its_table(tmp_path):
+    """OpacityTable SM03 reads INPUTDIR/data_gasopacity_malygin.txt at set-up
+    (InitOpacities, Opacity2.c:252-300): a missing file is an error that names
+    it; with the table the context is made."""
+    cfg = configs.with_grid(configs.DAMPED_1024x1536, 16, 32, OpacityTable="SM03")
+    h = C.c_void_p()
+    P = params_for(cfg, InputDir=str(tmp_path) + "/")
+    assert lib().thorin_ctx_create_host(P.ptr, C.byref(h)) != 0
+    assert "data_gasopacity_malygin.txt" in last_error() and "not found" in last_error()
+    (tmp_path / "data_gasopacity_malygin.txt").write_text("126\n94\n1.0\n")
+    assert lib().thorin_ctx_create_host(P.ptr, C.byref(h)) != 0 and "malformed" in last_error()
+    P = params_for(cfg)                    # the fixture table
+    h = host_ctx(P)
+    assert P.get("OPACITYTABLE")[:1] == "S"
+    lib().thorin_ctx_destroy(h)
+
+
 def host_ctx(P):
     h = C.c_void_p()
     assert lib().thorin_ctx_create_host(P.ptr, C.byref(h)) == 0, last_error()

@@ -15,7 +15,8 @@ import pytest
 from oracle import OracleDisk
 from tests.helpers import FIELDS, Replay, case, golden, params_for, planets_for
 
-FULL_CASES = ["wall_lp85_64x128", "damp_bl94_48x96", "dampinit_zhu12_40x64", "planet_q3e-5_48x96", "irr_lp85_48x96"]
+FULL_CASES = ["wall_lp85_64x128", "damp_bl94_48x96", "dampinit_zhu12_40x64", "planet_q3e-5_48x96", "irr_lp85_48x96",
+              "damp_sm03_48x96"]
 
 
 def run_oracle_case(name):
