@@ -91,6 +91,7 @@ struct DevCtl {                 // device-side control block, mirrored to pinned
   int crash_peb;                // DetectCrashPebbles of this substep
   int acc_fail;                 // IntegrateColumnMass did not converge
   double pacc[PEB_MAXPL][3];    // pebble accretion per planet: dM, dPX, dPY
+  double gacc[8][3];            // gas accretion per accreting planet, in order: dM, dPX, dPY
 };
 
 static_assert(sizeof(DevCtl) % 8 == 0, "k_cfl_final mirrors DevCtl in 8-byte words");
@@ -1535,6 +1536,98 @@ __device__ double column_mass(double a, double b, double Hpeb, int &ok) {
   }
   ok = 0;
   return 0.0;
+}
+
+// AccreteOntoPlanets (Planet.c:33-163): gas accretion onto the planets with
+// an accretion rate, taken in order (each sees the density the previous one
+// left).  The window, Roche radius and Kley's fractions come from the host
+// with the reference's expressions; every cell of the window is one thread's.
+// qtrap/trapzd (Planet.c:176-214) integrate exp(-z^2) to QTRAPEPS = 1e-4.
+struct GasAccK {
+  int np, pad;
+  double x[8], y[8], z[8], rroche[8], facc1[8], facc2[8];
+  int imin[8], imax[8], jmin[8], jmax[8];
+};
+#define GACC_T 256
+__device__ double gas_qtrap(double a, double b) {
+  double s = 0.0, olds = 0.0;
+  for (int n = 1; n <= 16; n++) {
+    if (n == 1) {
+      s = 0.5 * (b - a) * (exp(-a * a) + exp(-b * b));
+    } else {
+      int it = 1;
+      for (int j = 1; j < n - 1; j++) it <<= 1;
+      const double tnm = it, del = (b - a) / tnm;
+      double x = a + 0.5 * del, sum = 0.0;
+      for (int j = 1; j <= it; j++, x += del) sum += exp(-x * x);
+      s = 0.5 * (s + (b - a) * sum / tnm);
+    }
+    if (n > 5)
+      if (fabs(s - olds) < 1.e-4 * fabs(olds) || (s == 0.0 && olds == 0.0)) return s;
+    olds = s;
+  }
+  return s;
+}
+__global__ void __launch_bounds__(GACC_T) k_gas_accrete(Geo g, GasAccK P, double *__restrict__ dens,
+                                                        const double *__restrict__ vrad,
+                                                        const double *__restrict__ vtheta,
+                                                        const double *__restrict__ cs, DevCtl *ctl) {
+  __shared__ double red[GACC_T / 64];
+  const int ns = g.ns, tid = threadIdx.x;
+  const double frac1 = 0.75, frac2 = 0.45;
+  for (int q = 0; q < P.np; q++) {
+    const int i0 = P.imin[q], i1 = P.imax[q], j0 = P.jmin[q], nj = P.jmax[q] - P.jmin[q] + 1;
+    const int n = (i1 >= i0 && nj > 0) ? (i1 - i0 + 1) * nj : 0;
+    const double X = P.x[q], Y = P.y[q], Z = P.z[q], RRoche = P.rroche[q], facc1 = P.facc1[q], facc2 = P.facc2[q];
+    double dm = 0.0, dpx = 0.0, dpy = 0.0;
+    for (int c = tid; c < n; c += GACC_T) {
+      const int i = i0 + c / nj;
+      int jf = j0 + c % nj;
+      while (jf < 0) jf += ns;
+      while (jf >= ns) jf -= ns;
+      const size_t l = (size_t)i * ns + jf, lip = l + ns;
+      const size_t ljp = (jf == ns - 1) ? (size_t)i * ns : l + 1;
+      const double xc = g.Rmed[i] * g.cosj[jf], yc = g.Rmed[i] * g.sinj[jf];
+      const double dx = X - xc, dy = Y - yc;
+      const double distance = sqrt(dx * dx + dy * dy);
+      const double vtcell = 0.5 * (vtheta[l] + vtheta[ljp]) + g.Rmed[i] * g.omega_frame;
+      const double vrcell = 0.5 * (vrad[l] + vrad[lip]);
+      const double vxcell = (vrcell * xc - vtcell * yc) / g.Rmed[i];
+      const double vycell = (vrcell * yc + vtcell * xc) / g.Rmed[i];
+      const double H = cs[l] * g.OmegaInv[i] * g.sqrt_adiabind_inv;
+      if (distance < frac1 * RRoche) {
+        const double dz = sqrt(pow(frac1 * RRoche, 2) - pow(distance, 2));
+        const double z1 = Z - dz, z2 = Z + dz;
+        double factor = gas_qtrap(z1 / (sqrt(2.0) * H), z2 / (sqrt(2.0) * H));
+        factor *= 1.0 / (sqrt(2.0 * DPI) * H);
+        const double deltaM = facc1 * factor * dens[l] * g.Surf[i];
+        dens[l] *= (1.0 - facc1 * factor);
+        dpx += deltaM * vxcell;
+        dpy += deltaM * vycell;
+        dm += deltaM;
+      }
+      if (distance < frac2 * RRoche) {
+        const double dz = sqrt(pow(frac2 * RRoche, 2) - pow(distance, 2));
+        const double z1 = Z - dz, z2 = Z + dz;
+        double factor = gas_qtrap(z1 / (sqrt(2.0) * H), z2 / (sqrt(2.0) * H));
+        factor *= 1.0 / (sqrt(2.0 * DPI) * H);
+        const double deltaM = facc2 * factor * dens[l] * g.Surf[i];
+        dens[l] *= (1.0 - facc2);                // the reference removes facc2, not facc2*factor
+        dpx += deltaM * vxcell;
+        dpy += deltaM * vycell;
+        dm += deltaM;
+      }
+    }
+    dm = block_sum<GACC_T>(dm, red);
+    dpx = block_sum<GACC_T>(dpx, red);
+    dpy = block_sum<GACC_T>(dpy, red);
+    if (tid == 0) {
+      ctl->gacc[q][0] = dm;
+      ctl->gacc[q][1] = dpx;
+      ctl->gacc[q][2] = dpy;
+    }
+    __syncthreads();
+  }
 }
 
 // planet-side part of AccretePebblesOntoPlanets formed on the host with the
@@ -4225,6 +4318,8 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
   d->g.omega_frame = c->st.omega_frame;
   const dim3 tb(TB), rb(RB);
   const dim3 gall = cell_grid(d, nr);
+  int gacc = 0;                          // a planet accretes gas (AccreteOntoPlanets)
+  for (int k = 0; k < sys->nb; k++) gacc |= sys->acc[k] > 1e-10;
   while (dtemp < 0.999999999 * DT) {
     double mt = c->st.physical_time / (p->masstaper * 2.0 * M_PI);
     c->st.mass_taper = (mt > 1.0 ? 1.0 : pow(sin(mt * M_PI / 2.0), 2.0));
@@ -4262,6 +4357,60 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
     if (deltaT < newdt) newdt = deltaT;
     const int N = (int)(ceil(deltaT / newdt));
     dt = (DT - dtemp) / (double)N;
+    // ---- AccreteOntoPlanets (SourceEuler.c:247, Planet.c:33-163): the
+    // planet side with the reference's expressions, the cells on the device
+    if (gacc) {
+      GasAccK P;
+      memset(&P, 0, sizeof P);
+      int idx[8];
+      for (int k = 0; k < sys->nb; k++) {
+        if (!(sys->acc[k] > 1e-10)) continue;
+        const int q = P.np++;
+        idx[q] = k;
+        const double facc = fabs(dt) * (sys->acc[k]);
+        double facc1 = 1.0 / 3.0 * facc, facc2 = 2.0 / 3.0 * facc;
+        facc1 = facc1 < 1.0 ? facc1 : 1.0;
+        facc2 = facc2 < 1.0 ? facc2 : 1.0;
+        const double X = sys->x[k], Y = sys->y[k], Z = sys->z[k], M = sys->mass[k];
+        const double Rperp = sqrt(X * X + Y * Y);
+        const double Rplanet = sqrt(X * X + Y * Y + Z * Z);
+        const double RRoche = pow((1.0 / 3.0 * M * c->st.mass_taper), (1.0 / 3.0)) * Rplanet;
+        int i_min = 0, i_max = nr - 1;
+        while (i_min < nr && c->Rsup[i_min] < Rperp - RRoche) i_min++;
+        while ((c->Rinf[i_max] > Rperp + RRoche) && (i_max > 0)) i_max--;
+        const double angle = atan2(Y, X);
+        P.jmin[q] = (int)((double)ns / 2.0 / TH_PI * (angle - 2.0 * RRoche / Rperp));
+        P.jmax[q] = (int)((double)ns / 2.0 / TH_PI * (angle + 2.0 * RRoche / Rperp));
+        if (P.jmax[q] - P.jmin[q] + 1 > ns) {
+          thorin_set_error("gas accretion: the planet's Roche window spans more than a ring");
+          return -1;
+        }
+        P.imin[q] = i_min;
+        P.imax[q] = i_max < nr - 1 ? i_max : nr - 1;
+        P.x[q] = X; P.y[q] = Y; P.z[q] = Z;
+        P.rroche[q] = RRoche;
+        P.facc1[q] = facc1;
+        P.facc2[q] = facc2;
+      }
+      LAUNCH(k_gas_accrete, dim3(1), dim3(GACC_T), 0, d->g, P, d->b[B_DENS], (const double *)d->b[B_VRAD],
+             (const double *)d->b[B_VTHETA], (const double *)d->b[B_CS], d->ctl);
+      HCHECK(hipGetLastError());
+      DCOPY(d->hctl, d->ctl, sizeof(DevCtl), hipMemcpyDeviceToHost);
+      DSYNC();
+      for (int q = 0; q < P.np; q++) {
+        const int k = idx[q];
+        double M = sys->mass[k];
+        double PX = M * sys->vx[k], PY = M * sys->vy[k];
+        PX += d->hctl->gacc[q][1];
+        PY += d->hctl->gacc[q][2];
+        M += d->hctl->gacc[q][0];
+        if (sys->feeldisk[k]) {
+          sys->vx[k] = PX / M;
+          sys->vy[k] = PY / M;
+        }
+        sys->mass[k] = M;
+      }
+    }
     // ---- AccretePebblesOntoPlanets, EvaporatePebbles (SourceEuler.c:249-250)
     if (peb && sys->nb > 0) {
       PebAccK P;
@@ -4300,7 +4449,8 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
     // ---- FillForcesArrays (the planets' constants go to the kernel by value)
     const int want_tq = p->write_torque && log && log->nsub == 0 && sys->nb > 0;
     ForceSet fset;
-    if (!peb && !d->rec && d->spec.valid && force_set_matches(d->spec, sys, mt, want_tq, c->st.physical_time)) {
+    if (!peb && !gacc && !d->rec && d->spec.valid &&
+        force_set_matches(d->spec, sys, mt, want_tq, c->st.physical_time)) {
       fset = d->spec;           // enqueued at the end of the previous substep
     } else {
       if (launch_forces(c, sys, mt, want_tq, nullptr, &fset)) return -1;
@@ -4510,7 +4660,7 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
              d->cfl_invdtreb, guard, d->hctl);
       HCHECK(hipGetLastError());
       d->spec.valid = 0;
-      if (peb || d->rec) {
+      if (peb || gacc || d->rec) {   // (the gas accretion changes the density before the forces)
         DSYNC();
         return 0;
       }

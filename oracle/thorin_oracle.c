@@ -1804,6 +1804,120 @@ static double o_global_ifrac(oracle_ctx *o, double r) {
   return (double)i + (r - o->GlobalRmed[i - 1]) / (o->GlobalRmed[i] - o->GlobalRmed[i - 1]) - 1.0;
 }
 
+/* qtrap/trapzd of exp(-z^2) (Planet.c:166-214, QTRAPEPS 1e-4, QTRAPMAX 16) */
+static double o_gas_qtrap(double a, double b) {
+  double s = 0.0, olds = 0.0, x, tnm, sum, del;
+  int n, it, j;
+  for (n = 1; n <= 16; n++) {
+    if (n == 1) {
+      s = 0.5 * (b - a) * (exp(-a * a) + exp(-b * b));
+    } else {
+      for (it = 1, j = 1; j < n - 1; j++) it <<= 1;
+      tnm = it;
+      del = (b - a) / tnm;
+      x = a + 0.5 * del;
+      for (sum = 0.0, j = 1; j <= it; j++, x += del) sum += exp(-x * x);
+      s = 0.5 * (s + (b - a) * sum / tnm);
+    }
+    if (n > 5)
+      if (fabs(s - olds) < 1.e-4 * fabs(olds) || (s == 0.0 && olds == 0.0)) return s;
+    olds = s;
+  }
+  return s;
+}
+
+/* AccreteOntoPlanets, Planet.c:33-163 (single rank, sequential sums in the
+ * reference's loop order; GasAccretHeating is a refused option) */
+static void o_gas_accrete(oracle_ctx *o, thorin_planets *sys, double dt) {
+  const thorin_params *p = &o->p;
+  const int nr = o->nr, ns = o->ns;
+  double *dens = o->fld[THORIN_F_DENS], *vrad = o->fld[THORIN_F_VRAD], *vtheta = o->fld[THORIN_F_VTHETA];
+  const double *cs = o->fld[THORIN_F_SOUNDSPEED];
+  const double *abs_ = o->fld[W_ABS], *ord = o->fld[W_ORD];
+  int k, i, j, jf, l, lip, ljp, i_min, i_max, j_min, j_max;
+  for (k = 0; k < sys->nb; k++) {
+    double dMplanet, dPxPlanet, dPyPlanet, facc, facc1, facc2, frac1, frac2, Xplanet, Yplanet, Zplanet;
+    double VXplanet, VYplanet, Mplanet, Rperp, Rplanet, RRoche, angle, PxPlanet, PyPlanet;
+    if (!(sys->acc[k] > 1e-10)) continue;
+    dMplanet = dPxPlanet = dPyPlanet = 0.0;
+    facc = fabs(dt) * (sys->acc[k]);
+    facc1 = 1.0 / 3.0 * facc;
+    facc2 = 2.0 / 3.0 * facc;
+    frac1 = 0.75;
+    frac2 = 0.45;
+    facc1 = facc1 < 1.0 ? facc1 : 1.0;
+    facc2 = facc2 < 1.0 ? facc2 : 1.0;
+    Xplanet = sys->x[k]; Yplanet = sys->y[k]; Zplanet = sys->z[k];
+    VXplanet = sys->vx[k]; VYplanet = sys->vy[k]; Mplanet = sys->mass[k];
+    Rperp = sqrt(Xplanet * Xplanet + Yplanet * Yplanet);
+    Rplanet = sqrt(Xplanet * Xplanet + Yplanet * Yplanet + Zplanet * Zplanet);
+    RRoche = pow((1.0 / 3.0 * Mplanet * o->st.mass_taper), (1.0 / 3.0)) * Rplanet;
+    i_min = 0;
+    i_max = nr - 1;
+    while (i_min < nr && o->Rsup[i_min] < Rperp - RRoche) i_min++;
+    while ((o->Rinf[i_max] > Rperp + RRoche) && (i_max > 0)) i_max--;
+    angle = atan2(Yplanet, Xplanet);
+    j_min = (int)((double)ns / 2.0 / PI * (angle - 2.0 * RRoche / Rperp));
+    j_max = (int)((double)ns / 2.0 / PI * (angle + 2.0 * RRoche / Rperp));
+    PxPlanet = Mplanet * VXplanet;
+    PyPlanet = Mplanet * VYplanet;
+    for (i = i_min; i <= i_max; i++)
+      for (j = j_min; j <= j_max; j++) {
+        double xc, yc, dx, dy, distance, vtcell, vrcell, vxcell, vycell, H, dz, z1, z2, factor, deltaM;
+        jf = j;
+        while (jf < 0) jf += ns;
+        while (jf >= ns) jf -= ns;
+        l = jf + i * ns;
+        lip = l + ns;
+        ljp = l + 1;
+        if (jf == ns - 1) ljp = i * ns;
+        xc = abs_[l];
+        yc = ord[l];
+        dx = Xplanet - xc;
+        dy = Yplanet - yc;
+        distance = sqrt(dx * dx + dy * dy);
+        vtcell = 0.5 * (vtheta[l] + vtheta[ljp]) + o->Rmed[i] * o->st.omega_frame;
+        vrcell = 0.5 * (vrad[l] + vrad[lip]);
+        vxcell = (vrcell * xc - vtcell * yc) / o->Rmed[i];
+        vycell = (vrcell * yc + vtcell * xc) / o->Rmed[i];
+        H = cs[l] * o->OmegaInv[i] * o->sqrt_adiabind_inv;
+        if (distance < frac1 * RRoche) {
+          dz = sqrt(pow(frac1 * RRoche, 2) - pow(distance, 2));
+          z1 = Zplanet - dz;
+          z2 = Zplanet + dz;
+          factor = o_gas_qtrap(z1 / (sqrt(2.0) * H), z2 / (sqrt(2.0) * H));
+          factor *= 1.0 / (sqrt(2.0 * PI) * H);
+          deltaM = facc1 * factor * dens[l] * o->Surf[i];
+          dens[l] *= (1.0 - facc1 * factor);
+          dPxPlanet += deltaM * vxcell;
+          dPyPlanet += deltaM * vycell;
+          dMplanet += deltaM;
+        }
+        if (distance < frac2 * RRoche) {
+          dz = sqrt(pow(frac2 * RRoche, 2) - pow(distance, 2));
+          z1 = Zplanet - dz;
+          z2 = Zplanet + dz;
+          factor = o_gas_qtrap(z1 / (sqrt(2.0) * H), z2 / (sqrt(2.0) * H));
+          factor *= 1.0 / (sqrt(2.0 * PI) * H);
+          deltaM = facc2 * factor * dens[l] * o->Surf[i];
+          dens[l] *= (1.0 - facc2);
+          dPxPlanet += deltaM * vxcell;
+          dPyPlanet += deltaM * vycell;
+          dMplanet += deltaM;
+        }
+      }
+    PxPlanet += dPxPlanet;
+    PyPlanet += dPyPlanet;
+    Mplanet += dMplanet;
+    if (sys->feeldisk[k]) {
+      sys->vx[k] = PxPlanet / Mplanet;
+      sys->vy[k] = PyPlanet / Mplanet;
+    }
+    sys->mass[k] = Mplanet;
+  }
+  (void)p;
+}
+
 /* AccretePebblesOntoPlanets, Pebbles.c:346-538 (single rank; the
  * MPI_Allreduce sums are the local sums).  Returns -1 if a column
  * integration does not converge. */
@@ -2201,6 +2315,7 @@ int oracle_algogas(oracle_ctx *o, thorin_planets *sys, thorin_nbody_fn nbody, vo
     if (p->pebbles) o_peb_cfl(o);
     N = o_condition_cfl_core(o, DT - dtemp, &newdt);
     dt = (DT - dtemp) / (double)N;
+    o_gas_accrete(o, sys, dt);
     if (p->pebbles) {
       if (o_peb_accrete(o, sys, dt)) return -4;
       o_peb_evaporate(o, dt);

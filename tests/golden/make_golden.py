@@ -82,6 +82,10 @@ CASES = {
     "damp_sm03_48x96": (configs.with_grid(configs.DAMPED_1024x1536, 48, 96, Ntot=8, Ninterm=4, OpacityTable="SM03",
                                           WriteEnergy="YES", WriteDivV="YES", WriteQplus="YES",
                                           WriteOpacity="YES"), configs.PLANET_ZERO_MASS, True),
+    # gas accretion onto a Jupiter-mass planet (Kley's scheme, AccretionRate 5)
+    "gasacc_q1e-3_64x128": (configs.with_grid(configs.PLAIN_128x256, 64, 128, Ntot=8, Ninterm=4, AccretionRate="5.0",
+                                              WriteEnergy="YES", WriteDivV="YES", WriteQplus="YES",
+                                              WriteOpacity="YES"), [(1e-3, 1.0)], True),
     # the 128x256 / 200-step workload itself: checksums and ring profiles only
     "plain_128x256_200": (dict(configs.PLAIN_128x256, Ninterm="100", WriteEnergy="YES"),
                           configs.PLANET_ZERO_MASS, False),

@@ -46,8 +46,14 @@ def params_for(cfg: dict, **extra) -> Params:
 
 
 def planets_for(planets, cfg=None):
+    """The planets of a case; with its config, the flags and the accretion
+    rate every planet takes from it (ReboundInterface.c:191)."""
     feel = bool(cfg) and str(cfg.get("PlanetsFeelDisk", "NO"))[:1] in "yY"
-    return make_planets([m for m, a in planets], [a for m, a in planets], feel_disk=feel)
+    s = make_planets([m for m, a in planets], [a for m, a in planets], feel_disk=feel)
+    acc = float(cfg.get("AccretionRate", 0.0)) if cfg else 0.0
+    for k in range(s.nb):
+        s.acc[k] = acc
+    return s
 
 
 class Replay:

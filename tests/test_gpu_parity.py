@@ -22,7 +22,7 @@ pytestmark = pytest.mark.gpu
 
 TOL = 1e-10
 FULL_CASES = ["wall_lp85_64x128", "damp_bl94_48x96", "dampinit_zhu12_40x64", "planet_q3e-5_48x96", "irr_lp85_48x96",
-              "damp_sm03_48x96"]
+              "damp_sm03_48x96", "gasacc_q1e-3_64x128"]
 # fields whose values are differences of nearly equal terms: compared
 # relative to the field's own scale
 STATE_FIELDS = ["dens", "vrad", "vtheta", "temper", "energy", "divv", "qplus", "opacity"]
@@ -39,7 +39,7 @@ def test_gpu_vs_reference_golden(name):
     g = golden(name)
     d = Disk(params_for(cfg))
     d.init(g["init_dens"], g["init_vrad"], g["init_vtheta"], g["init_temper"])
-    sysp = planets_for(planets)
+    sysp = planets_for(planets, cfg)
     rep = Replay(g["tqwk0"])
     ntot, ninterm = int(g["ntot"]), int(g["ninterm"])
     niters, times = [], [d.state().physical_time]
@@ -74,7 +74,7 @@ def test_gpu_vs_reference_golden(name):
 
 
 @pytest.mark.parametrize("name,steps", [("wall_lp85_64x128", 25), ("damp_bl94_48x96", 25), ("irr_lp85_48x96", 25),
-                                        ("damp_sm03_48x96", 25)])
+                                        ("damp_sm03_48x96", 25), ("gasacc_q1e-3_64x128", 25)])
 def test_gpu_vs_oracle_stepwise(name, steps):
     """Same inputs and the same host N-body for both: per DT the CFL substep
     counts, dt, SOR iterations and Nshift agree exactly and the fields to 1e-10."""
@@ -86,7 +86,7 @@ def test_gpu_vs_oracle_stepwise(name, steps):
     o = OracleDisk(P)
     for x in (d, o):
         x.init(g["init_dens"], g["init_vrad"], g["init_vtheta"], g["init_temper"])
-    sd, so = planets_for(planets), planets_for(planets)
+    sd, so = planets_for(planets, cfg), planets_for(planets, cfg)
     nb = host_nbody_fn()
     for step in range(steps):
         lg = d.algogas(sd, nb)
@@ -105,6 +105,9 @@ def test_gpu_vs_oracle_stepwise(name, steps):
         for f in STATE_FIELDS:
             err = rel_linf(d.field(f), o.field(f))
             assert err <= TOL, f"step {step} {f}: {err:.3e}"
+        # planet masses (gas accretion: block-ordered sums on the GPU)
+        for k in range(sd.nb):
+            assert abs(sd.mass[k] - so.mass[k]) <= 1e-12 * abs(so.mass[k]), f"step {step}: planet {k} mass"
     assert d.state().physical_time == o.state().physical_time
 
 

@@ -16,7 +16,7 @@ from oracle import OracleDisk
 from tests.helpers import FIELDS, Replay, case, golden, params_for, planets_for
 
 FULL_CASES = ["wall_lp85_64x128", "damp_bl94_48x96", "dampinit_zhu12_40x64", "planet_q3e-5_48x96", "irr_lp85_48x96",
-              "damp_sm03_48x96"]
+              "damp_sm03_48x96", "gasacc_q1e-3_64x128"]
 
 
 def run_oracle_case(name):
@@ -24,7 +24,7 @@ def run_oracle_case(name):
     g = golden(name)
     o = OracleDisk(params_for(cfg))
     o.init(g["init_dens"], g["init_vrad"], g["init_vtheta"], g["init_temper"])
-    sysp = planets_for(planets)
+    sysp = planets_for(planets, cfg)
     rep = Replay(g["tqwk0"])
     ntot, ninterm = int(g["ntot"]), int(g["ninterm"])
     snaps = {0: {f: o.field(f) for f in FIELDS}}
