@@ -217,9 +217,8 @@ static int ctx_create(const thorin_params *p, thorin_ctx **out, int with_device)
   if (p->parametric_cooling) { thorin_set_error("CoolingTime > 0 (parametric cooling) is not supported by this build"); return -1; }
   if (p->advect_label) { thorin_set_error("LabelAdvection is not supported by this build"); return -1; }
   if (!p->energy_eq) { thorin_set_error("EnergyEquation NO (isothermal) is not supported by this build"); return -1; }
-  if (p->prescribed_accretion || p->aerodynamic_drag || p->gas_accret_heating ||
-      p->accret_heating || p->torque_density) {
-    thorin_set_error("prescribed accretion / accretion heating / aerodynamic drag / torque density are not supported by this build");
+  if (p->prescribed_accretion || p->aerodynamic_drag || p->torque_density) {
+    thorin_set_error("prescribed accretion / aerodynamic drag / torque density are not supported by this build");
     return -1;
   }
   c = (thorin_ctx *)calloc(1, sizeof(thorin_ctx));

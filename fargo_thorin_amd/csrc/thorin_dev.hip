@@ -202,6 +202,7 @@ struct thorin_dev {
   int sor_nit;                  // SOR iterations per launch (1 or 2; thorin_tune)
   int sor_queue_min;            // ensemble: SOR iterations to queue at least
   ForceSet spec;                // force kernels enqueued ahead for the next substep
+  int heat_max;                 // heatsrc_max: set by the pebble accretion when AccretionalHeating is on
   hipEvent_t ev_end;            // end of a substep's own work (the host waits for it)
   int rad_rows;                 // rings per tile of the radial transport (8, 16, 32; thorin_tune)
   int rad_ready;                // thorin_rad_system has assembled the energy system
@@ -1543,6 +1544,15 @@ __device__ double column_mass(double a, double b, double Hpeb, int &ok) {
 // left).  The window, Roche radius and Kley's fractions come from the host
 // with the reference's expressions; every cell of the window is one thread's.
 // qtrap/trapzd (Planet.c:176-214) integrate exp(-z^2) to QTRAPEPS = 1e-4.
+// accretion heating sources of one substep (heatsrc, heatsrc_index,
+// heatsrc_max of global.h:21-28): k_matrix adds bfac*q[k] to the RHS of cell
+// idx[k] for k < n, in k order (EnergySources.c:291-294)
+struct HeatK {
+  int n, pad;
+  int idx[8];
+  double q[8];
+};
+
 struct GasAccK {
   int np, pad;
   double x[8], y[8], z[8], rroche[8], facc1[8], facc2[8];
@@ -2251,7 +2261,8 @@ __device__ __forceinline__ void b_matrix(const Geo &g, const double *__restrict_
                                                double *__restrict__ Ml, double *__restrict__ Mlip,
                                                double *__restrict__ Mlim, double *__restrict__ Mljp,
                                                double *__restrict__ Mljm, double *__restrict__ RHS, double dt,
-                                               double *__restrict__ presid, const double *__restrict__ opa) {
+                                               double *__restrict__ presid, const double *__restrict__ opa,
+                                               const HeatK &hk) {
   __shared__ double red[TB];
   const int ns = g.ns;
   int i, j, L;                                      // rings 1..nr-2
@@ -2272,6 +2283,8 @@ __device__ __forceinline__ void b_matrix(const Geo &g, const double *__restrict_
     const double tl = T[l];
     double rhs = tl + bfac * (qplus[l] + 3.0 * qminus[l] * tl);
     if (g.stellar_irr) rhs += bfac * qirr_cell(g, i, l, cs, opa, rho);   // EnergySources.c:290
+    for (int k = 0; k < hk.n; k++)                                          // EnergySources.c:291-294
+      if ((int)l == hk.idx[k]) rhs += bfac * hk.q[k];
     Ml[l] = ml;
     Mlip[l] = mlip;
     Mlim[l] = mlim;
@@ -2291,8 +2304,9 @@ __global__ void __launch_bounds__(TB) k_matrix(Geo g, const double *__restrict__
                                                double *__restrict__ Ml, double *__restrict__ Mlip,
                                                double *__restrict__ Mlim, double *__restrict__ Mljp,
                                                double *__restrict__ Mljm, double *__restrict__ RHS, double dt,
-                                               double *__restrict__ presid, const double *__restrict__ opa) {
-  b_matrix(g, rho, cs, D, divv, qminus, qplus, T, Ml, Mlip, Mlim, Mljp, Mljm, RHS, dt, presid, opa);
+                                               double *__restrict__ presid, const double *__restrict__ opa,
+                                               HeatK hk) {
+  b_matrix(g, rho, cs, D, divv, qminus, qplus, T, Ml, Mlip, Mlim, Mljp, Mljm, RHS, dt, presid, opa, hk);
 }
 
 
@@ -4298,6 +4312,31 @@ static int launch_forces(thorin_ctx *c, const thorin_planets *sys, double mt, in
   return 0;
 }
 
+// Heating1 (Planet.c:222-246): the accretion luminosity of planet k spread
+// over the cell that holds it, tapered in over 4 HEATINGDELAY DTs
+static void heating1(const thorin_ctx *c, HeatK *h, double Mplanet, double dMplanet, double Xplanet, double Yplanet,
+                     double dt, int k) {
+  const thorin_params *p = &c->p;
+  const int nr = c->nr, ns = c->ns;
+  const double Rperp = sqrt(Xplanet * Xplanet + Yplanet * Yplanet);
+  if (Rperp < c->Rinf[0] || Rperp > c->Rsup[nr - 1]) return;
+  const double tdelay = 4.0 * p->dt * (double)p->heatingdelay;
+  double taper = c->st.physical_time / tdelay;
+  if (taper > 1.0) taper = 1.0;
+  const double plrho = p->planetarydensity / TH_RHO2CGS;
+  const double plradius = pow(3.0 * Mplanet / (4.0 * TH_PI * plrho), 1.0 / 3.0);
+  const double L = Mplanet * dMplanet / dt / plradius;
+  double ang = atan2(Yplanet, Xplanet);
+  if (ang < 0.0) ang += 2.0 * TH_PI;
+  int i = 0;
+  while (c->Rsup[i] < Rperp) i++;
+  int j = (int)floor((double)ns / 2.0 / TH_PI * ang + 0.5);
+  if (j == ns) j = 0;
+  h->idx[k] = j + i * ns;
+  const double dE = L * taper / c->Surf[i];
+  h->q[k] += dE;
+}
+
 extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_fn nbody, void *nbody_user,
                               thorin_step_log *log) {
   if (need_dev(c)) return -1;
@@ -4357,6 +4396,10 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
     if (deltaT < newdt) newdt = deltaT;
     const int N = (int)(ceil(deltaT / newdt));
     dt = (DT - dtemp) / (double)N;
+    // ---- InitHeating (SourceEuler.c:246, Planet.c:251-258)
+    HeatK heat;
+    memset(&heat, 0, sizeof heat);
+    for (int k = 0; k < 8; k++) heat.idx[k] = -1;
     // ---- AccreteOntoPlanets (SourceEuler.c:247, Planet.c:33-163): the
     // planet side with the reference's expressions, the cells on the device
     if (gacc) {
@@ -4400,6 +4443,8 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
       for (int q = 0; q < P.np; q++) {
         const int k = idx[q];
         double M = sys->mass[k];
+        if (p->gas_accret_heating && k < 8)
+          heating1(c, &heat, M, d->hctl->gacc[q][0], sys->x[k], sys->y[k], dt, k);
         double PX = M * sys->vx[k], PY = M * sys->vy[k];
         PX += d->hctl->gacc[q][1];
         PY += d->hctl->gacc[q][2];
@@ -4412,6 +4457,7 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
       }
     }
     // ---- AccretePebblesOntoPlanets, EvaporatePebbles (SourceEuler.c:249-250)
+    if (peb && p->accret_heating) d->heat_max = sys->nb;   // Pebbles.c:379
     if (peb && sys->nb > 0) {
       PebAccK P;
       memset(&P, 0, sizeof P);
@@ -4430,6 +4476,8 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
         double M = sys->mass[k];
         double PX = sys->vx[k] * M, PY = sys->vy[k] * M;
         const double dM = d->hctl->pacc[k][0];
+        // OPTIONAL STEP 6 (Pebbles.c:525-528)
+        if (p->accret_heating && k < 8) heating1(c, &heat, M, dM, sys->x[k], sys->y[k], dt, k);
         PX += d->hctl->pacc[k][1];
         PY += d->hctl->pacc[k][2];
         M += dM;
@@ -4551,9 +4599,10 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
     d->opa_filled = 1;
     {
       const dim3 gm = cell_grid(d, nr - 2);
+      heat.n = d->heat_max < 8 ? d->heat_max : 8;       // heatsrc_max as the matrix reads it
       LAUNCH(k_matrix, gm, tb, 0, d->g, d->b[B_DENS], d->b[B_CS], d->b[B_DIFF], d->b[B_DIVV], d->b[B_QMINUS],
              d->b[B_QPLUS], d->b[B_TEMPER], d->b[B_ML], d->b[B_MLIP], d->b[B_MLIM], d->b[B_MLJP], d->b[B_MLJM],
-             d->b[B_RHS], dt, d->mresp, (const double *)d->b[B_OPACITY]);
+             d->b[B_RHS], dt, d->mresp, (const double *)d->b[B_OPACITY], heat);
       d->nb_mresp = cell_count(d, nr - 2);
       d->normres0_valid = 1;
     }
@@ -5477,7 +5526,7 @@ extern "C" int thorin_rad_system(thorin_ctx *c, double dt, double *dt_used) {
   const dim3 gm = cell_grid(d, nr - 2);
   LAUNCH(k_matrix, gm, tb, 0, d->g, d->b[B_DENS], d->b[B_CS], d->b[B_DIFF], d->b[B_DIVV], d->b[B_QMINUS],
          d->b[B_QPLUS], d->b[B_TEMPER], d->b[B_ML], d->b[B_MLIP], d->b[B_MLIM], d->b[B_MLJP], d->b[B_MLJM],
-         d->b[B_RHS], dt, d->mresp, (const double *)d->b[B_OPACITY]);
+         d->b[B_RHS], dt, d->mresp, (const double *)d->b[B_OPACITY], HeatK{});
   d->nb_mresp = cell_count(d, nr - 2);
   d->normres0_valid = 1;
   HCHECK(hipMemcpyAsync(d->b[B_TBACK], d->b[B_TEMPER], nb, hipMemcpyDeviceToDevice, d->s));

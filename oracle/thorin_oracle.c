@@ -69,6 +69,8 @@ struct oracle_ctx {
   thorin_params p;
   int nr, ns;
   int sm_rows, sm_cols;                 /* OpacityTable SM03: the gas table (Opacity2.c:22-23) */
+  int heatsrc_max, heatsrc_index[8];    /* accretion heating (global.h:21-28) */
+  double heatsrc[8];
   double *sm_pgrid, *sm_tgrid, *sm_ross;
   double *fld[NW];
   double *Radii, *Rinf, *Rsup, *Rmed, *Surf, *InvRmed, *InvSurf, *InvDiffRmed, *InvDiffRsup;
@@ -1473,6 +1475,11 @@ static int o_implicit_diffusion(oracle_ctx *o, double dt, double *omega_used) {
       Mljm[l] = -bfac * B[l];
       RHS[l] = T[l] + bfac * (qplus[l] + 3.0 * qminus[l] * T[l]);
       if (p->stellar_irradiation) RHS[l] += bfac * o_qirr(o, i, j, cs, opa, rho);   /* EnergySources.c:290 */
+      if (p->accret_heating || p->gas_accret_heating) {                        /* EnergySources.c:291-294 */
+        int k;
+        for (k = 0; k < o->heatsrc_max && k < 8; k++)
+          if (l == o->heatsrc_index[k]) RHS[l] += bfac * o->heatsrc[k];
+      }
     }
   if (o->st.sor_first) {
     o->st.sor_first = 0;
@@ -1804,6 +1811,32 @@ static double o_global_ifrac(oracle_ctx *o, double r) {
   return (double)i + (r - o->GlobalRmed[i - 1]) / (o->GlobalRmed[i] - o->GlobalRmed[i - 1]) - 1.0;
 }
 
+/* Heating1, Planet.c:222-246 */
+static void o_heating1(oracle_ctx *o, double Mplanet, double dMplanet, double Xplanet, double Yplanet, double dt,
+                       int k) {
+  const thorin_params *p = &o->p;
+  int i, j, l;
+  double Rperp, tdelay, taper, plrho, plradius, L, ang, dE;
+  Rperp = sqrt(Xplanet * Xplanet + Yplanet * Yplanet);
+  if (Rperp < o->Rinf[0] || Rperp > o->Rsup[o->nr - 1]) return;
+  tdelay = 4.0 * p->dt * (double)p->heatingdelay;
+  taper = o->st.physical_time / tdelay;
+  if (taper > 1.0) taper = 1.0;
+  plrho = p->planetarydensity / RHO2CGS;
+  plradius = pow(3.0 * Mplanet / (4.0 * PI * plrho), 1.0 / 3.0);
+  L = Mplanet * dMplanet / dt / plradius;
+  ang = atan2(Yplanet, Xplanet);
+  if (ang < 0.0) ang += 2.0 * PI;
+  i = 0;
+  while (o->Rsup[i] < Rperp) i++;
+  j = floor((double)o->ns / 2.0 / PI * ang + 0.5);
+  if (j == o->ns) j = 0;
+  l = j + i * o->ns;
+  o->heatsrc_index[k] = l;
+  dE = L * taper / o->Surf[i];
+  o->heatsrc[k] += dE;
+}
+
 /* qtrap/trapzd of exp(-z^2) (Planet.c:166-214, QTRAPEPS 1e-4, QTRAPMAX 16) */
 static double o_gas_qtrap(double a, double b) {
   double s = 0.0, olds = 0.0, x, tnm, sum, del;
@@ -1906,6 +1939,7 @@ static void o_gas_accrete(oracle_ctx *o, thorin_planets *sys, double dt) {
           dMplanet += deltaM;
         }
       }
+    if (p->gas_accret_heating && k < 8) o_heating1(o, Mplanet, dMplanet, Xplanet, Yplanet, dt, k);
     PxPlanet += dPxPlanet;
     PyPlanet += dPyPlanet;
     Mplanet += dMplanet;
@@ -1933,6 +1967,7 @@ static int o_peb_accrete(oracle_ctx *o, thorin_planets *sys, double dt) {
   double vtcorr, xcell, ycell, vrcell, vtcell, vxcell, vycell, flow, flowx, flowy, vrel;
   double ifrac, frac, Mt, Rbondi, tbondi, fac, Reff = 0.0, Reff2, R2, H, Hpeb, zmin, zmax, dM, mcell;
   double voldenspeb, tau_mean, dM_upper, Hpeb_mean, pdens_mean, veloc, dM_model, limiter;
+  if (o->p.accret_heating) o->heatsrc_max = sys->nb;   /* Pebbles.c:379 */
   o_ring_mean(o, o->fld[THORIN_F_VTHETA], o->vt1D);
   for (k = 0; k < sys->nb; k++) {
     dMplanet = dPXplanet = dPYplanet = 0.0;
@@ -2043,6 +2078,8 @@ static int o_peb_accrete(oracle_ctx *o, thorin_planets *sys, double dt) {
         }
       }
     }
+    /* optional step 6: accretion heating (Pebbles.c:525-528) */
+    if (o->p.accret_heating && k < 8) o_heating1(o, Mplanet, dMplanet, Xplanet, Yplanet, dt, k);
     /* step 7: update the planet */
     PXplanet += dPXplanet;
     PYplanet += dPYplanet;
@@ -2315,6 +2352,10 @@ int oracle_algogas(oracle_ctx *o, thorin_planets *sys, thorin_nbody_fn nbody, vo
     if (p->pebbles) o_peb_cfl(o);
     N = o_condition_cfl_core(o, DT - dtemp, &newdt);
     dt = (DT - dtemp) / (double)N;
+    {   /* InitHeating, Planet.c:251-258 */
+      int k;
+      for (k = 0; k < 8; k++) { o->heatsrc[k] = 0.0; o->heatsrc_index[k] = -1; }
+    }
     o_gas_accrete(o, sys, dt);
     if (p->pebbles) {
       if (o_peb_accrete(o, sys, dt)) return -4;

@@ -82,6 +82,13 @@ CASES = {
     "damp_sm03_48x96": (configs.with_grid(configs.DAMPED_1024x1536, 48, 96, Ntot=8, Ninterm=4, OpacityTable="SM03",
                                           WriteEnergy="YES", WriteDivV="YES", WriteQplus="YES",
                                           WriteOpacity="YES"), configs.PLANET_ZERO_MASS, True),
+    # the two-fluid run with accretion heating from the pebble and the gas
+    # accretion (AccretionalHeating, GasAccretionHeating; AccretionRate 5)
+    "pebheat_q3e-5_64x128": (configs.with_grid(configs.DAMPED_1024x1536, 64, 128, Ntot=10, Ninterm=5,
+                                               PebbleAccretion="YES", BackReaction="YES", PlanetsFeelDisk="YES",
+                                               AccretionalHeating="YES", GasAccretionHeating="YES",
+                                               AccretionRate="5.0", HeatingDelay="1", WriteEnergy="YES"),
+                             [(3e-5, 1.0)], True),
     # gas accretion onto a Jupiter-mass planet (Kley's scheme, AccretionRate 5)
     "gasacc_q1e-3_64x128": (configs.with_grid(configs.PLAIN_128x256, 64, 128, Ntot=8, Ninterm=4, AccretionRate="5.0",
                                               WriteEnergy="YES", WriteDivV="YES", WriteQplus="YES",

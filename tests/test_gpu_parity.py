@@ -223,14 +223,15 @@ PEB_CASE = "pebbles_q3e-5_64x128"
 PEB_FIELDS = ["pebbledens", "pebblevrad", "pebblevtheta"]
 
 
-def test_gpu_pebbles_vs_reference_golden():
+@pytest.mark.parametrize("name", [PEB_CASE, "pebheat_q3e-5_64x128"])
+def test_gpu_pebbles_vs_reference_golden(name):
     """Two-fluid run of the reference (equilibrium pebble disk of a fresh run,
     back-reaction, particle diffusion, evaporation, a q = 3e-5 planet that
     feels the disk and accretes pebbles): the set-up is bit-exact, the SOR
     iterations and PhysicalTime equal, gas and pebble fields within 1e-10."""
-    _print_case("pebbles golden")
-    cfg, planets, _ = case(PEB_CASE)
-    g = golden(PEB_CASE)
+    _print_case("pebbles golden", name)
+    cfg, planets, _ = case(name)
+    g = golden(name)
     d = Disk(params_for(cfg))
     d.init(g["init_dens"], g["init_vrad"], g["init_vtheta"], g["init_temper"])
     d.init_pebbles(0)

@@ -92,13 +92,14 @@ PEB_CASE = "pebbles_q3e-5_64x128"
 PEB_FIELDS = ["pebbledens", "pebblevrad", "pebblevtheta"]
 
 
-def test_oracle_pebbles_bitwise():
+@pytest.mark.parametrize("name", [PEB_CASE, "pebheat_q3e-5_64x128"])
+def test_oracle_pebbles_bitwise(name):
     """Two-fluid run with a q = 3e-5 planet that feels the disk and accretes
     pebbles: the equilibrium pebble disk of a fresh run (sizes, Stokes
     numbers, density, velocities), then every gas and pebble output field,
     the SOR iterations and PhysicalTime equal the reference's bit for bit."""
-    cfg, planets, _ = case(PEB_CASE)
-    g = golden(PEB_CASE)
+    cfg, planets, _ = case(name)
+    g = golden(name)
     o = OracleDisk(params_for(cfg))
     o.init(g["init_dens"], g["init_vrad"], g["init_vtheta"], g["init_temper"])
     o.init_pebbles(0)
