@@ -19,8 +19,10 @@
  *     flux limiter, SOR solution vs a direct sparse solve, mass conservation
  *     of transport, FARGO shift as an exact rotation.
  * Branches that no golden run exercises (OpenInner and the other options the
- * library refuses) are "parity unpinned"; stellar irradiation is pinned by
- * the golden run irr_lp85_48x96.
+ * library refuses) are "parity unpinned".  Stellar irradiation, the SM03
+ * opacity, gas accretion and accretion heating are pinned by the golden runs
+ * irr_lp85_48x96, damp_sm03_48x96, gasacc_q1e-3_64x128 and
+ * pebheat_q3e-5_64x128.
  */
 #define _GNU_SOURCE
 #include "thorin_oracle.h"
