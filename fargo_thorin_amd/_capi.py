@@ -20,7 +20,9 @@ FIELDS = ["dens", "vrad", "vtheta", "energy", "label", "temper", "pressure", "so
 FIELD_ID = {n: i for i, n in enumerate(FIELDS)}
 # read-only diagnostics: the energy equation's five-point system (THORIN_X_*)
 FIELD_ID.update({"sys_ml": 64, "sys_mlip": 65, "sys_mlim": 66, "sys_mljp": 67, "sys_mljm": 68, "sys_rhs": 69,
-                 "sys_diff": 70})
+                 "sys_diff": 70,
+                 # the first crash's fields (the gas*999.dat dumps)
+                 "crash_dens": 71, "crash_vrad": 72, "crash_vtheta": 73, "crash_temper": 74})
 OPACITY = {"BL94": 0, "LP85": 1, "ZHU12": 2, "CONST": 3, "POWERLAW": 4}
 
 
@@ -42,6 +44,7 @@ class StepLog(C.Structure):
     _fields_ = [("nsub", C.c_int), ("crashed_any", C.c_int), ("cfl_n", C.c_int * MAXSUB),
                 ("dt", C.c_double * MAXSUB), ("sor_niter", C.c_int * MAXSUB),
                 ("sor_omega", C.c_double * MAXSUB), ("crashed", C.c_int * MAXSUB), ("lost_mass", C.c_double),
+                ("crash_dumped", C.c_int), ("crash_time", C.c_double),
                 ("tq_rows", C.c_int), ("tq", (C.c_double * 7) * 8)]
 
 

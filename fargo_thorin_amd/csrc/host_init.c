@@ -12,14 +12,23 @@
 #define _GNU_SOURCE
 #include "thorin_internal.h"
 #include <math.h>
+#include <stdarg.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
-static char g_err[512];
+/* one error text per thread: the ensemble steps each disk on a thread of
+ * its own, and copies a failing disk's text to the caller's thread */
+static __thread char g_err[512];
 void thorin_set_error(const char *msg) {
   strncpy(g_err, msg, sizeof(g_err) - 1);
   g_err[sizeof(g_err) - 1] = 0;
+}
+void thorin_set_errorf(const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof g_err, fmt, ap);
+  va_end(ap);
 }
 const char *thorin_last_error(void) { return g_err; }
 
@@ -418,14 +427,47 @@ static void fill_vtheta_med(thorin_ctx *c) {
   }
 }
 
-/* Initialization, Init.c:72-145 */
+/* Initialization, Init.c:72-145.  Modes 0 and 1 make the initial state
+ * (InitEuler, or the InitializeFromFile arrays with RefillSigma, RefillEnergy
+ * and FillVtheta).  Mode 2 is the restart route: the reference forms the
+ * damping profiles from that initial state first and only then reads the
+ * restart files into the fields (Init.c:106-126), so mode 2 keeps the
+ * profiles of the last mode 0/1 state (making the analytic one first when
+ * there is none; an analytic state is refilled as the reference refills
+ * it) and replaces the fields. */
+static void init_fields_from(thorin_ctx *c, const double *dens, const double *vrad, const double *vtheta,
+                             const double *temper) {
+  int i, j, l, nr = c->nr, ns = c->ns;
+  size_t ncell = (size_t)nr * ns;
+  memcpy(c->h_dens, dens, sizeof(double) * ncell);
+  memcpy(c->h_vrad, vrad, sizeof(double) * ncell);
+  memcpy(c->h_vtheta, vtheta, sizeof(double) * ncell);
+  memcpy(c->h_energy, temper, sizeof(double) * ncell);
+  for (i = 0; i < nr; i++)
+    for (j = 0; j < ns; j++) {
+      l = j + i * ns;
+      c->h_energy[l] = c->h_energy[l] * c->h_dens[l] / (c->p.adiabind - 1.0);
+    }
+}
+
 int thorin_init_state(thorin_ctx *c, int mode, const double *dens, const double *vrad,
                       const double *vtheta, const double *temper) {
   int i, j, l, nr = c->nr, ns = c->ns;
-  size_t ncell = (size_t)nr * ns;
+  if (mode < 0 || mode > 2) { thorin_set_error("thorin_init_state: mode must be 0, 1 or 2"); return -1; }
   if (mode != 0 && (!dens || !vrad || !vtheta || !temper)) {
     thorin_set_error("thorin_init_state: mode 1/2 needs dens, vrad, vtheta and temper arrays");
     return -1;
+  }
+  if (mode == 2) {
+    if (c->t0_mode != 2) {   /* RefillSigma/RefillEnergy/FillVtheta of the analytic state */
+      if (thorin_init_state(c, 0, NULL, NULL, NULL, NULL)) return -1;
+      refill_profiles(c);
+      if (c->p.damp_init) fill_vtheta_med(c);
+    }
+    init_fields_from(c, dens, vrad, vtheta, temper);
+    host_eos(c);
+    if (c->dev && thorin_dev_upload_init(c) != 0) return -1;
+    return 0;
   }
   memset(c->h_dens, 0, sizeof(double) * c->nfield);
   memset(c->h_vrad, 0, sizeof(double) * c->nfield);
@@ -452,20 +494,13 @@ int thorin_init_state(thorin_ctx *c, int mode, const double *dens, const double 
   /* InitLabel, Init.c:56-70 */
   for (i = 0; i <= nr; i++)
     for (j = 0; j < ns; j++) c->h_label[j + i * ns] = (c->Rmed[i] - c->p.rmin) / (c->p.rmax - c->p.rmin);
-  if (mode != 0) {
-    memcpy(c->h_dens, dens, sizeof(double) * ncell);
-    memcpy(c->h_vrad, vrad, sizeof(double) * ncell);
-    memcpy(c->h_vtheta, vtheta, sizeof(double) * ncell);
-    memcpy(c->h_energy, temper, sizeof(double) * ncell);
-    for (i = 0; i < nr; i++)
-      for (j = 0; j < ns; j++) {
-        l = j + i * ns;
-        c->h_energy[l] = c->h_energy[l] * c->h_dens[l] / (c->p.adiabind - 1.0);
-      }
+  if (mode == 1) {
+    init_fields_from(c, dens, vrad, vtheta, temper);
     refill_profiles(c);
     if (c->p.damp_init) fill_vtheta_med(c);
     host_eos(c);
   }
+  c->t0_mode = mode + 1;
   /* main.c:168-170: cs once more after InitComputeAccel (same values) */
   if (c->dev && thorin_dev_upload_init(c) != 0) return -1;
   return 0;

@@ -171,9 +171,11 @@ static int bs_step(int n, const double *m, double *q, double H) {
 
 /* AdvanceSystemRebound + RotatePsys + SynchronizeFargoRebound
  * (SourceEuler.c:260-269).  The substep is split until each Bulirsch-Stoer
- * step converges (close encounters). */
+ * step converges (close encounters); if 2^11 pieces still do not converge,
+ * the planets are left as they were and 1 is returned (thorin_algogas then
+ * fails the DT with "N-body step failed"). */
 int thorin_nbody_step(void *user, thorin_planets *sys, double t, double dt, double angle) {
-  int n = sys->nb, i, pieces = 1, tries;
+  int n = sys->nb, i, pieces = 1, tries, bad = 1;
   double q[6 * THORIN_MAXPLANETS], q0[6 * THORIN_MAXPLANETS], sint, cost;
   (void)user;
   (void)t;
@@ -182,11 +184,15 @@ int thorin_nbody_step(void *user, thorin_planets *sys, double t, double dt, doub
     q0[6 * i] = sys->x[i]; q0[6 * i + 1] = sys->y[i]; q0[6 * i + 2] = sys->z[i];
     q0[6 * i + 3] = sys->vx[i]; q0[6 * i + 4] = sys->vy[i]; q0[6 * i + 5] = sys->vz[i];
   }
-  for (tries = 0; tries < 12; tries++, pieces *= 2) {
-    int k, bad = 0;
+  for (tries = 0; tries < 12 && bad; tries++, pieces *= 2) {
+    int k;
+    bad = 0;
     memcpy(q, q0, sizeof(double) * 6 * n);
     for (k = 0; k < pieces; k++) bad |= bs_step(n, sys->mass, q, dt / pieces);
-    if (!bad) break;
+  }
+  if (bad) {
+    thorin_set_error("thorin_nbody_step: the Bulirsch-Stoer step did not converge in 2048 pieces of the substep");
+    return 1;
   }
   sint = sin(angle);
   cost = cos(angle);

@@ -21,6 +21,7 @@
 #include <functional>
 #include <mutex>
 #include <new>
+#include <string>
 #include <thread>
 #include <tuple>
 #include <unordered_map>
@@ -69,6 +70,8 @@ enum {
   B_RP, B_RM, B_TP, B_TM, B_XRP, B_XRM, B_XTP, B_XTM, B_XE, B_XRHO,
   // pebble fluid work fields (Pebbles.c:37-56, SourceEuler.c:155-158)
   B_PAGR, B_PAGT, B_PACCR, B_PACCT, B_GACCR, B_GACCT, B_DRAGR, B_DRAGT, B_MACC,
+  // the first crash's dens, vrad, vtheta, temperature (SourceEuler.c:279-282)
+  B_CRASH_DENS, B_CRASH_VRAD, B_CRASH_VTHETA, B_CRASH_TEMPER,
   NBUF
 };
 enum {
@@ -1952,15 +1955,33 @@ __device__ __forceinline__ Stress stress_cell(const Geo &g, const double *__rest
   return st;
 }
 
+// The first crash's snapshot (SourceEuler.c:275-284): the reference writes
+// Rho, Vrad, Vtheta and Temperature as DetectCrash sees them, after the
+// boundary conditions and before SubStep1.  SubStep1 (which sets the flags)
+// writes only VradInt/VthetaInt, so the next kernel over all cells still
+// reads those values and copies them when the flag that decides (the pebble
+// test with pebbles, else the energy test) is set.  ctl is null once the run
+// has crashed.
+struct CrashSnap {
+  const DevCtl *ctl;
+  int peb;
+  const double *src[4];
+  double *dst[4];
+};
+
 __device__ __forceinline__ void b_divstress(const Geo &g, const double *__restrict__ vr,
                                                   const double *__restrict__ vt,
                                                   const double *__restrict__ rho,
                                                   double *__restrict__ divv, double *__restrict__ Trr,
-                                                  double *__restrict__ Trp, double *__restrict__ Tpp) {
+                                                  double *__restrict__ Trp, double *__restrict__ Tpp,
+                                                  const CrashSnap &sn) {
   const int ns = g.ns;
   int i, j, L_;
   if (!xcd_cell(g, 0, g.nr, i, j, L_) || j >= ns) return;
   const size_t l = (size_t)i * ns + j;
+  if (sn.ctl && (sn.peb ? sn.ctl->crash_peb : sn.ctl->crash_e)) {
+    for (int k = 0; k < 4; k++) sn.dst[k][l] = sn.src[k][l];
+  }
   const Stress st = stress_cell(g, vr, vt, rho, i, j);
   if (divv) divv[l] = st.dv;    // SubStep3 forms the divergence again before it is read
   Trr[l] = st.trr;
@@ -1971,8 +1992,9 @@ __global__ void __launch_bounds__(TB) k_divstress(Geo g, const double *__restric
                                                   const double *__restrict__ vt,
                                                   const double *__restrict__ rho,
                                                   double *__restrict__ divv, double *__restrict__ Trr,
-                                                  double *__restrict__ Trp, double *__restrict__ Tpp) {
-  b_divstress(g, vr, vt, rho, divv, Trr, Trp, Tpp);
+                                                  double *__restrict__ Trp, double *__restrict__ Tpp,
+                                                  CrashSnap sn) {
+  b_divstress(g, vr, vt, rho, divv, Trr, Trp, Tpp, sn);
 }
 
 
@@ -3447,8 +3469,9 @@ extern "C" int thorin_upload_field(thorin_ctx *c, int field, const double *host,
 }
 
 extern "C" int thorin_download_field(thorin_ctx *c, int field, double *host, size_t count) {
-  if (c && c->dev && host && field >= THORIN_X_ML && field <= THORIN_X_DIFF) {
-    static const int xb[] = {B_ML, B_MLIP, B_MLIM, B_MLJP, B_MLJM, B_RHS, B_DIFF};
+  if (c && c->dev && host && field >= THORIN_X_ML && field <= THORIN_X_CRASH_TEMPER) {
+    static const int xb[] = {B_ML, B_MLIP, B_MLIM, B_MLJP, B_MLJM, B_RHS, B_DIFF,
+                             B_CRASH_DENS, B_CRASH_VRAD, B_CRASH_VTHETA, B_CRASH_TEMPER};
     thorin_dev *d = c->dev;
     if (count > d->nf) count = d->nf;
     HCHECK(hipMemcpyAsync(host, d->b[xb[field - THORIN_X_ML]], sizeof(double) * count, hipMemcpyDeviceToHost,
@@ -4394,7 +4417,13 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
       c->st.cfl_count = 1;
     }
     if (deltaT < newdt) newdt = deltaT;
+    if (!(newdt > 0.0) || !(deltaT / newdt <= (double)THORIN_MAX_CFL_N)) {
+      thorin_set_errorf("ConditionCFL gave dt = %g for a remaining step of %g: the step count is not finite or "
+                       "exceeds %d; the substep is refused", newdt, deltaT, THORIN_MAX_CFL_N);
+      return -1;
+    }
     const int N = (int)(ceil(deltaT / newdt));
+    const double t_sub = c->st.physical_time;
     dt = (DT - dtemp) / (double)N;
     // ---- InitHeating (SourceEuler.c:246, Planet.c:251-258)
     HeatK heat;
@@ -4537,7 +4566,12 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
     // ---- host N-body (AdvanceSystemRebound, RotatePsys, SynchronizeFargoRebound)
     if (nbody) {
       int e = nbody(nbody_user, sys, c->st.physical_time, dt, c->st.omega_frame * dt);
-      if (e) { thorin_set_error("N-body step failed"); return -1; }
+      if (e) {
+        char m[600];
+        snprintf(m, sizeof m, "N-body step failed (%d): %s", e, thorin_last_error());
+        thorin_set_error(m);
+        return -1;
+      }
     }
     // ---- ApplyBoundaryCondition, DetectCrash, ComputePressureField, SubStep1
     if (p->damping_on)
@@ -4559,8 +4593,17 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
       LAUNCH(k_peb_substep1, gall, tb, 0, d->g, d->b[B_PVR], d->b[B_PVT], d->b[B_TAU], d->b[B_PACCR], d->b[B_PACCT],
              d->b[B_GACCR], d->b[B_GACCT], d->b[B_VRAD], d->b[B_VTHETA], d->b[B_DENS], d->b[B_PDENS], dt,
              p->diffusive_particles);
+    CrashSnap snap;
+    memset(&snap, 0, sizeof snap);
+    if (c->st.already_crashed == 0) {
+      snap.ctl = d->ctl;
+      snap.peb = peb;
+      const int sb[4] = {B_DENS, B_VRAD, B_VTHETA, B_TEMPER}, db[4] = {B_CRASH_DENS, B_CRASH_VRAD, B_CRASH_VTHETA,
+                                                                       B_CRASH_TEMPER};
+      for (int k = 0; k < 4; k++) { snap.src[k] = d->b[sb[k]]; snap.dst[k] = d->b[db[k]]; }
+    }
     LAUNCH(k_divstress, gall, tb, 0, d->g, d->b[B_VRADINT], d->b[B_VTHETAINT], d->b[B_DENS], (double *)nullptr,
-           d->b[B_TAURR], d->b[B_TAURP], d->b[B_TAUPP]);
+           d->b[B_TAURR], d->b[B_TAURP], d->b[B_TAUPP], snap);
     {
       const int kepler = (!p->damping_on) || (p->damping_on && p->damp_vrad);
       double vkin = 0.0, vkout = 0.0;
@@ -4773,6 +4816,10 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
     if (prof_flush(d)) return -1;
     // DetectCrash (SourceEuler.c:272-274): the last test of the substep decides
     if (peb ? d->hctl->crash_peb : d->hctl->crash_e) {
+      if (c->st.already_crashed == 0 && log) {   // k_divstress took the snapshot
+        log->crash_dumped = 1;
+        log->crash_time = t_sub;
+      }
       c->st.already_crashed++;
       if (log) { log->crashed[log->nsub - 1] = 1; log->crashed_any = 1; }
       if (!rc) rc = 1;
@@ -5038,11 +5085,13 @@ extern "C" int thorin_ensemble_algogas(thorin_ensemble *e, thorin_planets *const
   }
   std::vector<int> rc((size_t)n, 0);
   std::vector<std::thread> th;
+  std::string disk_err;          // the first failing disk's error text (set on its thread)
   th.reserve((size_t)n);
   for (int k = 0; k < n; k++)
     th.emplace_back([&, k] {
       rc[k] = thorin_algogas(e->c[k], sys[k], nbody, nbody_user ? nbody_user[k] : nullptr, logs ? &logs[k] : nullptr);
       std::lock_guard<std::mutex> lk(e->m);
+      if (rc[k] < 0 && disk_err.empty()) disk_err = "disk " + std::to_string(k) + ": " + thorin_last_error();
       e->nfinished++;
       e->cv_coord.notify_one();
     });
@@ -5077,6 +5126,7 @@ extern "C" int thorin_ensemble_algogas(thorin_ensemble *e, thorin_planets *const
     if (rc[k] < 0) status = -1;
     else if (rc[k] > worst) worst = rc[k];
   }
+  if (!disk_err.empty()) thorin_set_error(disk_err.c_str());
   return status ? -1 : worst;
 }
 

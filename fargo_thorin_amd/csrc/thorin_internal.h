@@ -64,10 +64,12 @@ struct thorin_ctx {
   double *h_pdens, *h_pvr, *h_pvt, *h_psize, *h_tau, *h_eta;
   double *PebDensInit, *PebVradInit, *PebVthetaInit;
   int peb_ready;
+  int t0_mode;                  /* the last initial state (thorin_init_state mode 0 or 1) + 1; 0: none */
   struct thorin_dev *dev;
 };
 
 void thorin_set_error(const char *msg);
+void thorin_set_errorf(const char *fmt, ...);
 double thorin_fviscosity_host(const thorin_ctx *c, double rad);
 double thorin_aspectratio_host(const thorin_ctx *c, double rad);
 void thorin_cavity_factors(thorin_ctx *c);

@@ -82,13 +82,36 @@ static void send_output(const thorin_params *p, thorin_ctx *c, int n, double *bu
   }
 }
 
+/* ReadfromFile, Init.c:23-54: dst holds the t=0 field; a missing file
+ * leaves it (with the reference's warning), a short one replaces only what
+ * it holds. */
 static int read_binary(const char *fn, double *dst, size_t cnt) {
   FILE *f = fopen(fn, "r");
   size_t got;
   if (!f) { fprintf(stderr, "WARNING ! Can't read %s. Restarting with t=0 settings.\n", fn); return -1; }
   got = fread(dst, sizeof(double), cnt, f);
   fclose(f);
+  if (got != cnt) fprintf(stderr, "WARNING ! %s holds %zu of %zu values.\n", fn, got, cnt);
   return got == cnt ? 0 : -1;
+}
+
+/* the first crash's fields (SourceEuler.c:276-282: WriteDiskPolar(..., 999)) */
+static void write_crash_dump(const thorin_params *p, thorin_ctx *c, double *buf) {
+  static const int ids[4] = {THORIN_X_CRASH_DENS, THORIN_X_CRASH_VRAD, THORIN_X_CRASH_VTHETA, THORIN_X_CRASH_TEMPER};
+  static const char *names[4] = {"dens", "vrad", "vtheta", "temper"};
+  char fn[2 * THORIN_STRLEN];
+  size_t cnt = (size_t)p->nrad * p->nsec;
+  int k;
+  FILE *f;
+  for (k = 0; k < 4; k++) {
+    if (thorin_download_field(c, ids[k], buf, cnt)) die("download");
+    snprintf(fn, sizeof fn, "%sgas%s999.dat", p->outputdir, names[k]);
+    f = fopen(fn, "w");
+    if (!f) { fprintf(stderr, "cannot open %s\n", fn); exit(1); }
+    fwrite(buf, sizeof(double), cnt, f);
+    fclose(f);
+    printf("Writing 'gas%s999.dat'...0/done\n", names[k]);
+  }
 }
 
 /* ReadfromAsciiFile, Init.c:148-199 */
@@ -172,8 +195,22 @@ int main(int argc, char **argv) {
   buf = (double *)malloc(sizeof(double) * (cnt + p.nsec));
   buf2 = (double *)malloc(sizeof(double) * (cnt + p.nsec));
   if (restart) {
+    /* Init.c:81-126: the initial state first (its ring profiles are the
+     * damping targets), then the restart files over its fields.  A missing
+     * gastemper file keeps the t=0 temperature (the reference would take
+     * its t=0 energy for a temperature there). */
     double *d = malloc(sizeof(double) * cnt), *vr = malloc(sizeof(double) * cnt), *vt = malloc(sizeof(double) * cnt),
            *t = malloc(sizeof(double) * cnt), tt = 0.0;
+    if (p.init_from_file) {
+      read_ascii(p.densinfile, d, cnt);
+      read_ascii(p.vradinfile, vr, cnt);
+      read_ascii(p.vthetainfile, vt, cnt);
+      read_ascii(p.temperinfile, t, cnt);
+      if (thorin_init_state(c, 1, d, vr, vt, t)) die("init");
+    } else if (thorin_init_state(c, 0, NULL, NULL, NULL, NULL)) die("init");
+    if (thorin_download_field(c, THORIN_F_DENS, d, cnt) || thorin_download_field(c, THORIN_F_VRAD, vr, cnt) ||
+        thorin_download_field(c, THORIN_F_VTHETA, vt, cnt) || thorin_download_field(c, THORIN_F_TEMPER, t, cnt))
+      die("download");
     snprintf(fn, sizeof fn, "%sgasdens%d.dat", p.outputdir, nrestart); read_binary(fn, d, cnt);
     snprintf(fn, sizeof fn, "%sgasvrad%d.dat", p.outputdir, nrestart); read_binary(fn, vr, cnt);
     snprintf(fn, sizeof fn, "%sgasvtheta%d.dat", p.outputdir, nrestart); read_binary(fn, vt, cnt);
@@ -202,8 +239,9 @@ int main(int argc, char **argv) {
   /* pebble disk (Init.c:138-144): RestartPebbleDisk or EquilPebbleDisk */
   if (p.pebbles) {
     if (restart) {
-      double *pd = malloc(sizeof(double) * cnt), *pvr = malloc(sizeof(double) * cnt),
-             *pvt = malloc(sizeof(double) * cnt), *ps = malloc(sizeof(double) * cnt);
+      /* RestartPebbleDisk (Pebbles.c:67-75) reads into the zeroed pebble grids */
+      double *pd = calloc(cnt, sizeof(double)), *pvr = calloc(cnt, sizeof(double)),
+             *pvt = calloc(cnt, sizeof(double)), *ps = calloc(cnt, sizeof(double));
       snprintf(fn, sizeof fn, "%sgaspebbledens%d.dat", p.outputdir, nrestart); read_binary(fn, pd, cnt);
       snprintf(fn, sizeof fn, "%sgaspebblesize%d.dat", p.outputdir, 0); read_binary(fn, ps, cnt);
       snprintf(fn, sizeof fn, "%sgaspebblevrad%d.dat", p.outputdir, nrestart); read_binary(fn, pvr, cnt);
@@ -243,6 +281,10 @@ int main(int argc, char **argv) {
     }
     k = thorin_algogas(c, &sys, thorin_nbody_step, NULL, lg);
     if (k < 0) die("AlgoGas");
+    if (lg->crash_dumped) {   /* SourceEuler.c:275-284 */
+      printf("\nCrash! at time %.12g\n", lg->crash_time);
+      write_crash_dump(&p, c, buf);
+    }
     for (k = 0; k < lg->nsub; k++) printf("%c", lg->crashed[k] ? 'c' : '.');
     printf("\n");
     fflush(stdout);

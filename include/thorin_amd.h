@@ -143,6 +143,12 @@ typedef struct thorin_step_log {
   double sor_omega[THORIN_MAXSUB];
   int crashed[THORIN_MAXSUB];
   double lost_mass;              /* LostMass accumulated by VanLeerRadial (OpenInner) */
+  /* the first crash of the run (AlreadyCrashed == 0, SourceEuler.c:275-284)
+   * happened in this call: crash_time is PhysicalTime at the start of that
+   * substep ("Crash! at time"), and THORIN_X_CRASH_* hold dens, vrad, vtheta
+   * and the temperature as DetectCrash saw them (the gas*999.dat dumps) */
+  int crash_dumped;
+  double crash_time;
   /* torque log of the first substep, one row per planet (UpdateLog,
    * Force.c:27-70): PhysicalTime, tq, tq_norm, x, y, ax, ay */
   int tq_rows;
@@ -167,7 +173,16 @@ enum {
  * five-point system of the last assembly (EnergySources.c:256-297, rings
  * 1..nrad-2) and the diffusion coefficient it was built from. */
 enum { THORIN_X_ML = 64, THORIN_X_MLIP, THORIN_X_MLIM, THORIN_X_MLJP, THORIN_X_MLJM, THORIN_X_RHS,
-       THORIN_X_DIFF };
+       THORIN_X_DIFF,
+       /* the fields of the run's first crash, taken where DetectCrash tests
+        * them (after the boundary conditions, before SubStep1), as
+        * WriteDiskPolar(..., 999) writes them (SourceEuler.c:279-282) */
+       THORIN_X_CRASH_DENS, THORIN_X_CRASH_VRAD, THORIN_X_CRASH_VTHETA, THORIN_X_CRASH_TEMPER };
+
+/* thorin_algogas refuses a substep whose CFL step count N is not finite or
+ * exceeds this (the reference would loop on it); it returns < 0 with the
+ * state of the substep's start. */
+#define THORIN_MAX_CFL_N 10000000
 
 /* 1D geometry (global.h:11-24) filled exactly as FillPolar1DArrays
  * (SourceEuler.c:68-131) and Theo.c.  Arrays have nrad+1 entries. */

@@ -79,6 +79,7 @@ struct oracle_ctx {
   double *InvRinf, *OmegaInv, *Rmed2, *GlobalRmed, *SigmaMed, *SigmaInf, *EnergyMed;
   double *VthetaMed, *WaveKiller, *globcsvec, *globtempervec, *Vmoy, *Vresidual, *VMed;
   int *Nshift;
+  double *crash[4];                     /* the first crash's Rho, Vrad, Vtheta, Temperature (SourceEuler.c:276-282) */
   double sqrt_adiabind_inv, CV, invdtreb_sq;
   thorin_state st;
   int opa_filled;
@@ -314,6 +315,7 @@ void oracle_destroy(oracle_ctx *o) {
   int k;
   if (!o) return;
   for (k = 0; k < NW; k++) free(o->fld[k]);
+  for (k = 0; k < 4; k++) free(o->crash[k]);
   free(o->sm_pgrid); free(o->sm_tgrid); free(o->sm_ross);
   free(o->Radii); free(o->Rinf); free(o->Rsup); free(o->Rmed); free(o->Surf); free(o->InvRmed);
   free(o->InvSurf); free(o->InvDiffRmed); free(o->InvDiffRsup); free(o->InvRinf); free(o->OmegaInv);
@@ -435,6 +437,8 @@ static int o_condition_cfl_core(oracle_ctx *o, double deltaT, double *newdt_out)
   }
   if (deltaT < newdt) newdt = deltaT;
   *newdt_out = newdt;
+  /* a non-finite or absurd step count is refused (the reference would loop) */
+  if (!(newdt > 0.0) || !(deltaT / newdt <= (double)THORIN_MAX_CFL_N)) return -1;
   return (int)(ceil(deltaT / newdt));
 }
 
@@ -2353,6 +2357,7 @@ int oracle_algogas(oracle_ctx *o, thorin_planets *sys, thorin_nbody_fn nbody, vo
     o->invdtreb_sq = o_minstep_rebound(sys);
     if (p->pebbles) o_peb_cfl(o);
     N = o_condition_cfl_core(o, DT - dtemp, &newdt);
+    if (N < 0) return -5;
     dt = (DT - dtemp) / (double)N;
     {   /* InitHeating, Planet.c:251-258 */
       int k;
@@ -2371,7 +2376,19 @@ int oracle_algogas(oracle_ctx *o, thorin_planets *sys, thorin_nbody_fn nbody, vo
     crashed = o_detect_crash(o, o->fld[THORIN_F_DENS]);
     crashed = o_detect_crash(o, o->fld[THORIN_F_ENERGY]);
     if (p->pebbles) crashed = o_detect_crash(o, o->fld[THORIN_F_PEBBLEDENS]);
-    if (crashed) o->st.already_crashed++;
+    if (crashed) {
+      if (o->st.already_crashed == 0) {   /* WriteDiskPolar (..., 999), SourceEuler.c:276-282 */
+        const int src[4] = {THORIN_F_DENS, THORIN_F_VRAD, THORIN_F_VTHETA, THORIN_F_TEMPER};
+        size_t n = (size_t)(o->nr + 1) * o->ns;
+        int k;
+        for (k = 0; k < 4; k++) {
+          if (!o->crash[k]) o->crash[k] = (double *)malloc(n * sizeof(double));
+          memcpy(o->crash[k], o->fld[src[k]], n * sizeof(double));
+        }
+        if (log) { log->crash_dumped = 1; log->crash_time = o->st.physical_time; }
+      }
+      o->st.already_crashed++;
+    }
     o_pressure(o, o->fld[THORIN_F_DENS], o->fld[THORIN_F_ENERGY]);
     if (p->pebbles) o_peb_source_terms(o, dt);
     o_substep1(o, dt);
@@ -2414,6 +2431,12 @@ int oracle_algogas(oracle_ctx *o, thorin_planets *sys, thorin_nbody_fn nbody, vo
 /* ---------------- accessors ---------------- */
 int oracle_get_field(oracle_ctx *o, int field, double *out, size_t count) {
   size_t n = (size_t)(o->nr + 1) * o->ns;
+  if (field >= THORIN_X_CRASH_DENS && field <= THORIN_X_CRASH_TEMPER) {
+    if (!o->crash[field - THORIN_X_CRASH_DENS]) return -1;
+    if (count > n) count = n;
+    memcpy(out, o->crash[field - THORIN_X_CRASH_DENS], count * sizeof(double));
+    return 0;
+  }
   if (field < 0 || field >= NW) return -1;
   if (count > n) count = n;
   memcpy(out, o->fld[field], count * sizeof(double));

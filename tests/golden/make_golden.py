@@ -48,7 +48,13 @@ def gas_table():
 FIELDS = ["dens", "vrad", "vtheta", "temper", "energy", "divv", "qplus", "opacity",
           "pebbledens", "pebblevrad", "pebblevtheta", "pebblesize", "pebblestokes"]
 
-# name: (config keys, planets, keep full fields?)
+# name: (config keys, planets, stored fields).  Stored fields: True keeps every
+# output field whole; False keeps every (nrad//8)-th ring; an int k keeps every
+# k-th ring.  Ring means and SHA-256 digests are kept for every field.  A config
+# key "_seeded_pebbles" (not an in.par key) runs the two-fluid workload's seeded
+# pebble disk through the reference's restart route: a fresh run writes output
+# 0 (gas fields and the N-body archive), the seeded pebble fields replace its
+# gaspebble*0.dat files, and the run restarts from output 0 with -s 0.
 CASES = {
     # config-1 physics on a small grid, WALL boundary, LP85 opacity
     "wall_lp85_64x128": (configs.with_grid(configs.PLAIN_128x256, 64, 128, Ntot=12, Ninterm=6,
@@ -96,17 +102,65 @@ CASES = {
     # the 128x256 / 200-step workload itself: checksums and ring profiles only
     "plain_128x256_200": (dict(configs.PLAIN_128x256, Ninterm="100", WriteEnergy="YES"),
                           configs.PLANET_ZERO_MASS, False),
+    # the 1024x1536 damped workload for its 100 steps (outputs after 50 and 100)
+    "damped_1024x1536_100": (dict(configs.DAMPED_1024x1536, Ninterm="50", WriteEnergy="YES"),
+                             configs.PLANET_ZERO_MASS, 64),
+    # the 1024x1536 two-fluid workload (seeded pebble disk, Sigma_p = 0.01 Sigma,
+    # Stokes numbers 0.01-0.1 per ring), its first 100 of 500 steps
+    "pebbles_1024x1536_100": (dict(configs.PEBBLES_1024x1536, Ntot="100", Ninterm="50", WriteEnergy="YES",
+                                   _seeded_pebbles="1"), configs.PLANET_Q3E5, 64),
+    # Flock et al. (2016) alpha(T) (the MRI transition at 900 K lies inside the
+    # disk) and the viscosity cavity with its time-dependent LambdaDoubling scale
+    "flock_cavity_48x96": (configs.with_grid(configs.PLAIN_128x256, 48, 96, Ntot=8, Ninterm=4, AlphaFlock="YES",
+                                             TMRI="900.0", TWidth="100.0", AlphaMRI="1e-2", AlphaDead="1e-4",
+                                             CavityRadius="0.8", CavityRatio="0.1", CavityWidth="2.0",
+                                             LambdaDoubling="50.0", WriteEnergy="YES", WriteDivV="YES",
+                                             WriteQplus="YES", WriteOpacity="YES"), configs.PLANET_ZERO_MASS, True),
+    # damping zones towards a prescribed radial mass flux (DampTowards G, GasFlux)
+    "dampflux_48x96": (configs.with_grid(configs.DAMPED_1024x1536, 48, 96, Ntot=8, Ninterm=4, DampTowards="G",
+                                         GasFlux="1e-6", WriteEnergy="YES", WriteDivV="YES", WriteQplus="YES",
+                                         WriteOpacity="YES"), configs.PLANET_ZERO_MASS, True),
+    # LP85 on a disk colder than the ice line (166.8 K) everywhere: only the
+    # ice-grain branch kappa = 2e-4 T^2 acts, which POWERLAW with
+    # PowerLawOpacity 2e-4 reproduces
+    "cold_lp85_48x96": (configs.with_grid(configs.PLAIN_128x256, 48, 96, Ntot=8, Ninterm=4, Rmin="1.2",
+                                          AspectRatio="0.025", WriteEnergy="YES", WriteDivV="YES", WriteQplus="YES",
+                                          WriteOpacity="YES"), configs.PLANET_ZERO_MASS, True),
+    # the cold disk with one cell at -0.01 of its temperature (negative
+    # energy, positive density): DetectCrash fires in DTs 0 and 1, the fields
+    # stay finite and the disk recovers in DT 2
+    "crash_cold_48x96": (configs.with_grid(configs.PLAIN_128x256, 48, 96, Ntot=3, Ninterm=3, Rmin="1.2",
+                                           AspectRatio="0.025", WriteEnergy="YES", _temper_cell="20,40,-0.01"),
+                         configs.PLANET_ZERO_MASS, True),
+    # an imposed supplementary torque that drives the disk inwards (ImposedDiskDrift)
+    "drift_48x96": (configs.with_grid(configs.DAMPED_1024x1536, 48, 96, Ntot=8, Ninterm=4,
+                                      ImposedDiskDrift="-1e-4", WriteEnergy="YES", WriteDivV="YES",
+                                      WriteQplus="YES", WriteOpacity="YES"), configs.PLANET_ZERO_MASS, True),
 }
 
 
+def _write_raw(path, arr):
+    np.ascontiguousarray(arr, dtype=np.float64).tofile(path)
+
+
 def run_case(name, cfg, planets, full):
+    cfg = dict(cfg)
+    seeded_pebbles = bool(cfg.pop("_seeded_pebbles", ""))
+    # "_temper_cell": "i,j,factor" scales one cell of the initial temperature
+    temper_cell = cfg.pop("_temper_cell", "")
     nrad, nsec = int(cfg["Nrad"]), int(cfg["Nsec"])
     with tempfile.TemporaryDirectory() as tmp:
         cfgdir = os.path.join(tmp, "in")
         outdir = os.path.join(tmp, "out")
         os.makedirs(cfgdir)
         os.makedirs(outdir)
-        disk = seeded_disk(nrad, nsec, log_grid=cfg.get("RadialSpacing", "LOG")[0] in "lL")
+        disk = seeded_disk(nrad, nsec, log_grid=cfg.get("RadialSpacing", "LOG")[0] in "lL",
+                           rmin=float(cfg["Rmin"]), rmax=float(cfg["Rmax"]), h=float(cfg["AspectRatio"]),
+                           sigma0=float(cfg["Sigma0"]), slope=float(cfg["SigmaSlope"]))
+        if temper_cell:
+            ci, cj, cf = temper_cell.split(",")
+            disk["temper"] = disk["temper"].copy()
+            disk["temper"][int(ci), int(cj)] *= float(cf)
         for k in ("dens", "vrad", "vtheta", "temper"):
             write_cfg(os.path.join(cfgdir, f"gas{k}.cfg"), disk[k])
         if cfg.get("OpacityTable", "")[:1] in "sS":
@@ -117,17 +171,40 @@ def run_case(name, cfg, planets, full):
         with open(pf, "w") as f:
             f.write(configs.planet_text(planets))
         par = os.path.join(tmp, "in.par")
+        restart0 = {}
+        if seeded_pebbles:
+            # fresh run that writes output 0 only (and one DT that is discarded)
+            with open(par, "w") as f:
+                f.write(configs.inpar_text(dict(cfg, Ntot="0"), outdir + "/", pf, cfgdir))
+            r0 = subprocess.run([REF, par], cwd=tmp, capture_output=True, text=True, timeout=3600)
+            if r0.returncode != 0:
+                raise RuntimeError(f"reference set-up run failed on {name}:\n{r0.stdout[-2000:]}")
+            for k in ("dens", "vrad", "vtheta", "temper"):
+                a = np.fromfile(os.path.join(outdir, f"gas{k}0.dat"), dtype=np.float64)
+                restart0[f"fresh_sha_{k}0"] = np.frombuffer(hashlib.sha256(a.tobytes()).digest(), dtype=np.uint8)
+            pb = seeded_pebbles_of(disk, cfg)
+            for k, src in (("dens", "pdens"), ("vrad", "pvrad"), ("vtheta", "pvtheta"), ("size", "psize")):
+                _write_raw(os.path.join(outdir, f"gaspebble{k}0.dat"), pb[src])
+            for fn in os.listdir(outdir):
+                if fn.startswith("tqwk"):
+                    os.remove(os.path.join(outdir, fn))
         with open(par, "w") as f:
             f.write(configs.inpar_text(cfg, outdir + "/", pf, cfgdir))
         # -d: the reference prints its CFL block and "SOR: Niter = n" for every substep
-        r = subprocess.run([REF, "-d", par], cwd=tmp, capture_output=True, text=True, timeout=3600)
+        cmd = [REF, "-d"] + (["-s", "0"] if seeded_pebbles else []) + [par]
+        r = subprocess.run(cmd, cwd=tmp, capture_output=True, text=True, timeout=7200)
         if r.returncode != 0:
             raise RuntimeError(f"reference failed on {name}:\n{r.stdout[-2000:]}\n{r.stderr[-2000:]}")
         ntot, ninterm = int(cfg["Ntot"]), int(cfg["Ninterm"])
         outs = list(range(0, ntot // ninterm + 1))
         data = {"nrad": nrad, "nsec": nsec, "ntot": ntot, "ninterm": ninterm, "outputs": np.array(outs)}
-        for k in ("dens", "vrad", "vtheta", "temper"):
-            data["init_" + k] = disk[k]
+        data.update(restart0)
+        if not isinstance(full, int) or isinstance(full, bool):
+            # small grids keep the seeded initial arrays; the large ones are
+            # made again from the seed (fargo_thorin_amd.seed.seeded_disk)
+            for k in ("dens", "vrad", "vtheta", "temper"):
+                data["init_" + k] = disk[k]
+        stride = None if full is True else (max(1, nrad // 8) if full is False else int(full))
         for n in outs:
             for fld in FIELDS:
                 fn = os.path.join(outdir, f"gas{fld}{n}.dat")
@@ -136,11 +213,11 @@ def run_case(name, cfg, planets, full):
                 a = np.fromfile(fn, dtype=np.float64).reshape(nrad, nsec)
                 data[f"sha_{fld}{n}"] = np.frombuffer(hashlib.sha256(a.tobytes()).digest(), dtype=np.uint8)
                 data[f"ringmean_{fld}{n}"] = a.mean(axis=1)
-                if full:
+                if stride is None:
                     data[f"{fld}{n}"] = a
-                else:
-                    data[f"rows_{fld}{n}"] = a[:: max(1, nrad // 8)].copy()
-        if name.startswith("pebbles"):
+                elif n > 0 or full is False:
+                    data[f"rows_{fld}{n}"] = a[::stride].copy()
+        if name == "pebbles_q3e-5_64x128":
             # restart from output 1 (-s 1): the pebble damping targets become
             # the ring means of the restart files (BckpFieldsForBC); keep the
             # output 2 the restarted run writes
@@ -151,9 +228,29 @@ def run_case(name, cfg, planets, full):
                 fn = os.path.join(outdir, f"gas{fld}2.dat")
                 if os.path.exists(fn):
                     data[f"restart_{fld}2"] = np.fromfile(fn, dtype=np.float64).reshape(nrad, nsec)
+        # the first crash's dump (SourceEuler.c:276-284) and the crash marks
+        for fld in ("dens", "vrad", "vtheta", "temper"):
+            fn = os.path.join(outdir, f"gas{fld}999.dat")
+            if os.path.exists(fn):
+                data[f"{fld}999"] = np.fromfile(fn, dtype=np.float64).reshape(nrad, nsec)
+        crash = re.findall(r"Crash! at time (\S+)", r.stdout)
+        if crash:
+            data["crash_time"] = np.array([float(x) for x in crash])
+            # the per-substep marks ('c' crashed, '.' not; SourceEuler.c:285-287)
+            # of a run without -d, where each DT's marks start a line
+            rm = subprocess.run([REF, par], cwd=tmp, capture_output=True, text=True, timeout=3600)
+            marks = "".join(m.group(1) for m in re.finditer(r"^([c.]+)(?:\t|$)", rm.stdout, re.M))
+            data["crash_marks"] = np.frombuffer(marks.encode(), dtype=np.uint8)
+        if stride is not None:
+            data["row_stride"] = np.array(stride)
         tq = np.loadtxt(os.path.join(outdir, "tqwk0.dat"), ndmin=2)
         data["tqwk0"] = tq
         data["used_rad"] = np.loadtxt(os.path.join(outdir, "used_rad.dat"))
+        # OutputElements rows (hash, time, a, e, i, Omega, omega, f, m, gas,
+        # solid, r, x, y, z, vx, vy, vz; %.12g) of every NOUTELEMENTS-th DT
+        orb = os.path.join(outdir, "nbody.orbits.dat")
+        if os.path.exists(orb) and os.path.getsize(orb) > 0:
+            data["nbody_orbits"] = np.loadtxt(orb, ndmin=2)
         # one ConditionCFL block per substep; SOR iterations per substep
         data["n_substeps"] = np.array(r.stdout.count("Timestep control information"))
         data["sor_niter"] = np.array([int(m) for m in re.findall(r"SOR: Niter = (\d+)", r.stdout)])
@@ -162,6 +259,13 @@ def run_case(name, cfg, planets, full):
         np.savez_compressed(os.path.join(OUT, f"{name}.npz"), **data)
         print(f"{name}: {int(data['n_substeps'])} substeps, SOR niter {data['sor_niter'][:6]}..., "
               f"scan {len(data['omega_scan'])} solves")
+
+
+def seeded_pebbles_of(disk, cfg):
+    """The two-fluid workload's seeded pebble disk for a case's gas disk."""
+    from fargo_thorin_amd.seed import seeded_pebbles
+    return seeded_pebbles(disk, bulkdens=float(cfg.get("PebbleBulkDens", 1.0)),
+                          adiabind=float(cfg.get("Adiabind", 1.4)))
 
 
 def main():

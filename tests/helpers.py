@@ -37,7 +37,7 @@ def opacity_table_dir() -> str:
 
 
 def params_for(cfg: dict, **extra) -> Params:
-    keys = dict(cfg)
+    keys = {k: v for k, v in cfg.items() if not k.startswith("_")}
     keys.update({k: str(v) for k, v in extra.items()})
     keys.setdefault("OutputDir", "/nonexistent-thorin-out/")
     if str(keys.get("OpacityTable", ""))[:1] in "sS":
@@ -91,3 +91,21 @@ def rel_linf(a, b, floor=1e-300):
     a = np.asarray(a)
     b = np.asarray(b)
     return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), floor))
+
+
+def initial_disk(name):
+    """The seeded initial disk of a golden case: the fixture's arrays where it
+    keeps them, else made again from the seed with the case's disk keys."""
+    from fargo_thorin_amd import seeded_disk
+    g = golden(name)
+    if "init_dens" in g:
+        return {k: g["init_" + k] for k in ("dens", "vrad", "vtheta", "temper")}
+    cfg, _, _ = case(name)
+    return seeded_disk(int(cfg["Nrad"]), int(cfg["Nsec"]), log_grid=cfg.get("RadialSpacing", "LOG")[0] in "lL",
+                       rmin=float(cfg["Rmin"]), rmax=float(cfg["Rmax"]), h=float(cfg["AspectRatio"]),
+                       sigma0=float(cfg["Sigma0"]), slope=float(cfg["SigmaSlope"]))
+
+
+def sha(a) -> np.ndarray:
+    import hashlib
+    return np.frombuffer(hashlib.sha256(np.ascontiguousarray(a).tobytes()).digest(), dtype=np.uint8)

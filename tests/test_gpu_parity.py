@@ -22,7 +22,8 @@ pytestmark = pytest.mark.gpu
 
 TOL = 1e-10
 FULL_CASES = ["wall_lp85_64x128", "damp_bl94_48x96", "dampinit_zhu12_40x64", "planet_q3e-5_48x96", "irr_lp85_48x96",
-              "damp_sm03_48x96", "gasacc_q1e-3_64x128"]
+              "damp_sm03_48x96", "gasacc_q1e-3_64x128", "flock_cavity_48x96", "dampflux_48x96", "drift_48x96",
+              "cold_lp85_48x96"]
 # fields whose values are differences of nearly equal terms: compared
 # relative to the field's own scale
 STATE_FIELDS = ["dens", "vrad", "vtheta", "temper", "energy", "divv", "qplus", "opacity"]
@@ -74,7 +75,8 @@ def test_gpu_vs_reference_golden(name):
 
 
 @pytest.mark.parametrize("name,steps", [("wall_lp85_64x128", 25), ("damp_bl94_48x96", 25), ("irr_lp85_48x96", 25),
-                                        ("damp_sm03_48x96", 25), ("gasacc_q1e-3_64x128", 25)])
+                                        ("damp_sm03_48x96", 25), ("gasacc_q1e-3_64x128", 25),
+                                        ("flock_cavity_48x96", 25), ("dampflux_48x96", 25), ("drift_48x96", 25)])
 def test_gpu_vs_oracle_stepwise(name, steps):
     """Same inputs and the same host N-body for both: per DT the CFL substep
     counts, dt, SOR iterations and Nshift agree exactly and the fields to 1e-10."""
@@ -298,3 +300,68 @@ def test_gpu_pebbles_vs_oracle_stepwise(nrad, nsec, steps):
     assert rel_linf(d.field("pebblestokes"), tau) <= TOL
     # on the coarse 24-ring grid no cell centre lies inside Reff
     assert (sd.solid[0] > 0.0) == (so.solid[0] > 0.0) == (nrad > 24)
+
+
+def test_gpu_powerlaw_opacity_vs_reference_lp85_on_cold_disk():
+    """OpacityTable POWERLAW (kappa = k0 T^2) with k0 = 2e-4 on the cold disk
+    (64-132 K) against the reference's LP85 run, whose ice-grain branch is
+    2e-4 T^2 there: SOR iterations and time exact, fields to 1e-10."""
+    name = "cold_lp85_48x96"
+    _print_case("powerlaw", name)
+    cfg, planets, _ = case(name)
+    g = golden(name)
+    d = Disk(params_for(cfg, OpacityTable="POWERLAW", PowerLawOpacity="2e-4"))
+    d.init(g["init_dens"], g["init_vrad"], g["init_vtheta"], g["init_temper"])
+    sysp = planets_for(planets, cfg)
+    rep = Replay(g["tqwk0"])
+    niters = []
+    ntot, ninterm = int(g["ntot"]), int(g["ninterm"])
+    for i in range(ntot):
+        lg = d.algogas(sysp, rep.cb)
+        niters += [lg.sor_niter[k] for k in range(lg.nsub)]
+        if (i + 1) % ninterm == 0:
+            n = (i + 1) // ninterm
+            for f in STATE_FIELDS:
+                if f"{f}{n}" in g:
+                    err = rel_linf(d.field(f), g[f"{f}{n}"])
+                    assert err <= TOL, f"{f}{n}: {err:.3e}"
+    assert niters == list(g["sor_niter"][:len(niters)])
+
+
+def test_gpu_crash_run_and_dump_vs_reference():
+    """The cold disk with one cell at -0.01 of its temperature: the crash
+    marks of every substep ('cc..'), the first crash's time and its dump
+    (gas{dens,vrad,vtheta,temper}999.dat) and the fields after three DTs
+    against the reference; N and the marks against the oracle per DT."""
+    name = "crash_cold_48x96"
+    _print_case("crash", name)
+    cfg, planets, _ = case(name)
+    g = golden(name)
+    P = params_for(cfg)
+    d, o = Disk(P), OracleDisk(P)
+    for x in (d, o):
+        x.init(g["init_dens"], g["init_vrad"], g["init_vtheta"], g["init_temper"])
+    sd, so = planets_for(planets, cfg), planets_for(planets, cfg)
+    rep_d, rep_o = Replay(g["tqwk0"]), Replay(g["tqwk0"])
+    marks, dumps, rcs = "", [], []
+    for i in range(int(g["ntot"])):
+        lg = d.algogas(sd, rep_d.cb)
+        rcs.append(d.last_rc)
+        gm = "".join("c" if lg.crashed[k] else "." for k in range(lg.nsub))
+        gn = [lg.cfl_n[k] for k in range(lg.nsub)]
+        if lg.crash_dumped:
+            dumps.append(lg.crash_time)
+        lo = o.algogas(so, rep_o.cb)
+        assert gn == [lo.cfl_n[k] for k in range(lo.nsub)], f"DT {i}: N"
+        assert gm == "".join("c" if lo.crashed[k] else "." for k in range(lo.nsub)), f"DT {i}: marks"
+        marks += gm
+    assert marks == bytes(g["crash_marks"]).decode()[: len(marks)]
+    assert rcs[:2] == [1, 1] and rcs[2] == 0
+    assert dumps == list(g["crash_time"])
+    assert d.state().already_crashed == o.state().already_crashed == 2
+    for f in ("dens", "vrad", "vtheta", "temper"):
+        err = rel_linf(d.field("crash_" + f), g[f"{f}999"])
+        assert err <= TOL, f"{f}999: {err:.3e}"
+    for f in ("dens", "vrad", "vtheta", "temper", "energy"):
+        err = rel_linf(d.field(f), g[f"{f}1"])
+        assert err <= TOL, f"{f}1: {err:.3e}"

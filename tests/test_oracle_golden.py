@@ -16,7 +16,8 @@ from oracle import OracleDisk
 from tests.helpers import FIELDS, Replay, case, golden, params_for, planets_for
 
 FULL_CASES = ["wall_lp85_64x128", "damp_bl94_48x96", "dampinit_zhu12_40x64", "planet_q3e-5_48x96", "irr_lp85_48x96",
-              "damp_sm03_48x96", "gasacc_q1e-3_64x128"]
+              "damp_sm03_48x96", "gasacc_q1e-3_64x128", "flock_cavity_48x96", "dampflux_48x96", "drift_48x96",
+              "cold_lp85_48x96"]
 
 
 def run_oracle_case(name):
@@ -128,3 +129,29 @@ def test_oracle_pebbles_bitwise(name):
     assert np.array_equal(np.array(times[:ntot]), g["tqwk0"][:ntot, 0])
     # the planet accreted pebbles
     assert sysp.mass[0] > m0 and sysp.solid[0] > 0.0
+
+
+def test_oracle_crash_run_and_dump():
+    """A cold disk with one cell at -0.01 of its temperature: DetectCrash
+    fires in the first two DTs (the reference's marks 'cc..'), the first
+    crash's dump (gas{dens,vrad,vtheta,temper}999.dat) and its time, then
+    every field after three DTs, all bit for bit."""
+    name = "crash_cold_48x96"
+    cfg, planets, _ = case(name)
+    g = golden(name)
+    o = OracleDisk(params_for(cfg))
+    o.init(g["init_dens"], g["init_vrad"], g["init_vtheta"], g["init_temper"])
+    sysp = planets_for(planets, cfg)
+    rep = Replay(g["tqwk0"])
+    marks, dumps = "", []
+    for i in range(int(g["ntot"])):
+        lg = o.algogas(sysp, rep.cb)
+        marks += "".join("c" if lg.crashed[k] else "." for k in range(lg.nsub))
+        if lg.crash_dumped:
+            dumps.append(lg.crash_time)
+    assert marks == bytes(g["crash_marks"]).decode()[: len(marks)]
+    assert dumps == list(g["crash_time"])
+    for f in ("dens", "vrad", "vtheta", "temper"):
+        assert np.array_equal(o.field("crash_" + f), g[f"{f}999"]), f"{f}999"
+    for f in ("dens", "vrad", "vtheta", "temper", "energy"):
+        assert np.array_equal(o.field(f), g[f"{f}1"]), f"{f}1"
