@@ -259,15 +259,18 @@ class Disk:
         return used.value
 
     def rad_solve(self, method: int = 0, tol: float = 1e-10, omega: float = 1.8, inner_sweeps: int = 8,
-                  max_iters: int = 100000, warm: "Optional[WarmStart]" = None, sweep_block: int = 0) -> dict:
+                  max_iters: int = 100000, warm: "Optional[WarmStart]" = None, sweep_block: int = 0,
+                  noalloc: bool = False) -> dict:
         """Solve the assembled system from its initial guess: method 0 = FP64
         red-black SOR, 1 = FP32 SOR sweeps in FP64 iterative refinement; with
         warm (method 1), from the network's guess when it lowers the residual.
+        noalloc: the allocation-free entry point (thorin_rad_solve_noalloc).
         T is left in field "temper"."""
         o = _capi.SolveOpts(method=method, inner_sweeps=inner_sweeps, max_iters=max_iters, tol=tol, omega=omega,
                             warm=warm._h if warm is not None else None, sweep_block=sweep_block)
         r = _capi.SolveResult()
-        _check(lib().thorin_rad_solve(self._h, C.byref(o), C.byref(r)), "thorin_rad_solve")
+        fn = lib().thorin_rad_solve_noalloc if noalloc else lib().thorin_rad_solve
+        _check(fn(self._h, C.byref(o), C.byref(r)), "thorin_rad_solve_noalloc" if noalloc else "thorin_rad_solve")
         return {"sweeps": r.sweeps, "rounds": r.rounds, "converged": bool(r.converged), "ms": r.ms,
                 "resid0": r.resid0, "resid": r.resid, "warm_used": bool(r.warm_used),
                 "resid_warm": r.resid_warm}

@@ -132,6 +132,7 @@ def lib() -> C.CDLL:
         "thorin_nbody_step": (ip, [vp, C.POINTER(Planets), C.c_double, C.c_double, C.c_double]),
         "thorin_rad_system": (ip, [vp, C.c_double, dp]),
         "thorin_rad_solve": (ip, [vp, C.POINTER(SolveOpts), C.POINTER(SolveResult)]),
+        "thorin_rad_solve_noalloc": (ip, [vp, C.POINTER(SolveOpts), C.POINTER(SolveResult)]),
         "thorin_cnn_create": (ip, [C.c_ulonglong, C.POINTER(vp)]),
         "thorin_cnn_destroy": (ip, [vp]),
         "thorin_cnn_param_count": (sz, [vp]),

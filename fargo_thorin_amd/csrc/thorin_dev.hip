@@ -207,6 +207,9 @@ struct thorin_dev {
   ForceSet spec;                // force kernels enqueued ahead for the next substep
   int heat_max;                 // heatsrc_max: set by the pebble accretion when AccretionalHeating is on
   hipEvent_t ev_end;            // end of a substep's own work (the host waits for it)
+  hipEvent_t ev_t0, ev_t1;      // the energy solve's timing (thorin_rad_solve*)
+  double *h_part;               // pinned: block partials of the FP64 residual (nblk_cells)
+  int device;                   // the HIP device of the context's stream
   int rad_rows;                 // rings per tile of the radial transport (8, 16, 32; thorin_tune)
   int rad_ready;                // thorin_rad_system has assembled the energy system
   struct Mat32 m32;             // FP32 copies for the mixed-precision solve (allocated on first use)
@@ -3345,6 +3348,10 @@ extern "C" int thorin_dev_create(thorin_ctx *c) {
   HCHECK(hipMalloc(&d->sorc, sizeof(int) * (size_t)d->nblk_cells * 2));
   HCHECK(hipMalloc(&d->pstellar, sizeof(double) * (size_t)d->nblk_cells));
   HCHECK(hipMalloc(&d->mresp, sizeof(double) * (size_t)d->nblk_cells));
+  HCHECK(hipHostMalloc(&d->h_part, sizeof(double) * (size_t)d->nblk_cells, hipHostMallocDefault));
+  HCHECK(hipEventCreate(&d->ev_t0));
+  HCHECK(hipEventCreate(&d->ev_t1));
+  HCHECK(hipGetDevice(&d->device));
   // cos / sin of the sector angles
   HCHECK(hipMalloc(&d->trig, sizeof(double) * 2 * (size_t)d->ns));
   HCHECK(hipMemcpyAsync(d->trig, c->h_trig, sizeof(double) * 2 * (size_t)d->ns, hipMemcpyHostToDevice, d->s));
@@ -3400,6 +3407,9 @@ extern "C" void thorin_dev_destroy(thorin_ctx *c) {
     delete d->prof_pool;
   }
   if (d->ev_end) (void)hipEventDestroy(d->ev_end);
+  if (d->ev_t0) (void)hipEventDestroy(d->ev_t0);
+  if (d->ev_t1) (void)hipEventDestroy(d->ev_t1);
+  if (d->h_part) (void)hipHostFree(d->h_part);
   if (d->s) (void)hipStreamDestroy(d->s);
   free(d);
   c->dev = NULL;
@@ -5538,11 +5548,10 @@ static int mp_residual(thorin_ctx *c, float *r32, double *out) {
                      (const double *)d->b[B_MLJP], (const double *)d->b[B_MLJM], (const double *)d->b[B_RHS], r32,
                      d->mresp);
   HCHECK(hipGetLastError());
-  std::vector<double> part(nb);
-  HCHECK(hipMemcpyAsync(part.data(), d->mresp, sizeof(double) * nb, hipMemcpyDeviceToHost, d->s));
+  HCHECK(hipMemcpyAsync(d->h_part, d->mresp, sizeof(double) * nb, hipMemcpyDeviceToHost, d->s));
   HCHECK(hipStreamSynchronize(d->s));
   double t = 0.0;
-  for (int k = 0; k < nb; k++) t += part[k];
+  for (int k = 0; k < nb; k++) t += d->h_part[k];
   *out = t;
   return 0;
 }
@@ -5601,39 +5610,35 @@ extern "C" int thorin_dev_system(thorin_ctx *c, thorin_sysptrs *p) {
   return 0;
 }
 
-extern "C" int thorin_rad_solve(thorin_ctx *c, const thorin_solve_opts *o, thorin_solve_result *res) {
-  if (need_dev(c)) return -1;
+// zero fill of FP32 planes (a kernel, so that no memset node is recorded
+// where a caller captures the stream)
+__global__ void __launch_bounds__(256) k_zero_f32(float *__restrict__ p, size_t n) {
+  for (size_t k = (size_t)blockIdx.x * 256 + threadIdx.x; k < n; k += (size_t)gridDim.x * 256) p[k] = 0.0f;
+}
+
+static void zero_f32(thorin_dev *d, float *p, size_t n) {
+  size_t nb = (n + 255) / 256;
+  const unsigned g = (unsigned)(nb < 4096 ? (nb > 0 ? nb : 1) : 4096);
+  hipLaunchKernelGGL(k_zero_f32, dim3(g), dim3(256), 0, d->s, p, n);
+}
+
+// The solve of thorin_rad_solve and thorin_rad_solve_noalloc on the FP32
+// planes m (matrix diagonals, residual, correction double buffer).
+static int rad_solve_on(thorin_ctx *c, const thorin_solve_opts *o, thorin_solve_result *res, Mat32 m) {
   thorin_dev *d = c->dev;
-  d->spec.valid = 0;                     // forces enqueued for a next substep are redone
-  if (!d->rad_ready) { thorin_set_error("thorin_rad_solve: no system assembled (thorin_rad_system)"); return -1; }
-  if (!o || !res || o->tol <= 0.0 || o->omega <= 0.0 || o->omega >= 2.0) {
-    thorin_set_error("thorin_rad_solve: need tol > 0 and 0 < omega < 2");
-    return -1;
-  }
   memset(res, 0, sizeof *res);
   const int nr = d->nr, ns = d->ns;
   const size_t nb = sizeof(double) * d->nf;
   const int maxit = o->max_iters > 0 ? o->max_iters : 100000;
   HCHECK(hipMemcpyAsync(d->b[B_TEMPER], d->b[B_TBACK], nb, hipMemcpyDeviceToDevice, d->s));
-  if (!d->m32.ml) {
-    float **f[8] = {&d->m32.ml, &d->m32.mip, &d->m32.mim, &d->m32.mjp, &d->m32.mjm, &d->m32.r, &d->m32.e0,
-                    &d->m32.e1};
-    for (int k = 0; k < 8; k++) {
-      HCHECK(hipMalloc(f[k], sizeof(float) * d->nf));
-      HCHECK(hipMemsetAsync(*f[k], 0, sizeof(float) * d->nf, d->s));
-    }
-  }
   double r0 = 0.0;
-  if (mp_residual(c, d->m32.r, &r0)) return -1;
+  if (mp_residual(c, m.r, &r0)) return -1;
   res->resid0 = r0;
   if (o->warm && o->method != 1) {
     thorin_set_error("thorin_rad_solve: the learned warm start is used with method 1");
     return -1;
   }
-  hipEvent_t ea, eb;
-  HCHECK(hipEventCreate(&ea));
-  HCHECK(hipEventCreate(&eb));
-  HCHECK(hipEventRecord(ea, d->s));
+  HCHECK(hipEventRecord(d->ev_t0, d->s));
   double rcur = r0;
   if (o->warm) {
     // the network's guess replaces T0 only if it lowers the FP64 residual;
@@ -5641,14 +5646,14 @@ extern "C" int thorin_rad_solve(thorin_ctx *c, const thorin_solve_opts *o, thori
     if (thorin_cnn_predict_dev(o->warm, c, d->b[B_TEMPER2])) return -1;
     HCHECK(hipMemcpyAsync(d->b[B_TEMPER], d->b[B_TEMPER2], nb, hipMemcpyDeviceToDevice, d->s));
     double rg = 0.0;
-    if (mp_residual(c, d->m32.r, &rg)) return -1;
+    if (mp_residual(c, m.r, &rg)) return -1;
     res->resid_warm = rg;
     if (rg < r0) {
       res->warm_used = 1;
       rcur = rg;
     } else {
       HCHECK(hipMemcpyAsync(d->b[B_TEMPER], d->b[B_TBACK], nb, hipMemcpyDeviceToDevice, d->s));
-      if (mp_residual(c, d->m32.r, &rcur)) return -1;
+      if (mp_residual(c, m.r, &rcur)) return -1;
     }
   }
   if (o->method == 0) {
@@ -5673,19 +5678,18 @@ extern "C" int thorin_rad_solve(thorin_ctx *c, const thorin_solve_opts *o, thori
     const dim3 gb((unsigned)((ntb + 7) / 8 * 8));
     hipLaunchKernelGGL(k_mp_convert, gm, tb, 0, d->s, d->g, (const double *)d->b[B_ML],
                        (const double *)d->b[B_MLIP], (const double *)d->b[B_MLIM], (const double *)d->b[B_MLJP],
-                       (const double *)d->b[B_MLJM], d->m32);
+                       (const double *)d->b[B_MLJM], m);
     while (rcur > o->tol * r0 && res->sweeps < maxit) {
       // r32 holds A.T - RHS of the current T; solve A.e = -r from e = 0
-      HCHECK(hipMemsetAsync(d->m32.e0, 0, sizeof(float) * d->nf, d->s));
-      HCHECK(hipMemsetAsync(d->m32.e1, 0, sizeof(float) * d->nf, d->s));
-      float *ein = d->m32.e0, *eout = d->m32.e1;
+      zero_f32(d, m.e0, d->nf);
+      zero_f32(d, m.e1, d->nf);
+      float *ein = m.e0, *eout = m.e1;
       for (int k = 0; k < inner;) {
         const int nsw = blk > 1 ? (inner - k < blk ? inner - k : blk) : 1;
         if (nsw > 1)
-          hipLaunchKernelGGL(k_mp_sweeps, gb, tb, 0, d->s, d->g, (const float *)ein, eout, d->m32, (float)o->omega,
-                             nsw);
+          hipLaunchKernelGGL(k_mp_sweeps, gb, tb, 0, d->s, d->g, (const float *)ein, eout, m, (float)o->omega, nsw);
         else
-          hipLaunchKernelGGL(k_mp_sweep, gs, tb, 0, d->s, d->g, (const float *)ein, eout, d->m32, (float)o->omega);
+          hipLaunchKernelGGL(k_mp_sweep, gs, tb, 0, d->s, d->g, (const float *)ein, eout, m, (float)o->omega);
         float *t = ein; ein = eout; eout = t;
         k += nsw;
       }
@@ -5693,22 +5697,112 @@ extern "C" int thorin_rad_solve(thorin_ctx *c, const thorin_solve_opts *o, thori
       HCHECK(hipGetLastError());
       res->sweeps += inner;
       res->rounds++;
-      if (mp_residual(c, d->m32.r, &rcur)) return -1;
+      if (mp_residual(c, m.r, &rcur)) return -1;
     }
   } else {
     thorin_set_error("thorin_rad_solve: unknown method");
     return -1;
   }
-  HCHECK(hipEventRecord(eb, d->s));
-  HCHECK(hipEventSynchronize(eb));
+  HCHECK(hipEventRecord(d->ev_t1, d->s));
+  HCHECK(hipEventSynchronize(d->ev_t1));
   float ms = 0.f;
-  HCHECK(hipEventElapsedTime(&ms, ea, eb));
-  (void)hipEventDestroy(ea);
-  (void)hipEventDestroy(eb);
+  HCHECK(hipEventElapsedTime(&ms, d->ev_t0, d->ev_t1));
   res->ms = ms;
   double rfin = 0.0;
-  if (mp_residual(c, d->m32.r, &rfin)) return -1;
+  if (mp_residual(c, m.r, &rfin)) return -1;
   res->resid = rfin;
   res->converged = rfin <= o->tol * r0;
   return 0;
+}
+
+static int rad_solve_check(thorin_ctx *c, const thorin_solve_opts *o, thorin_solve_result *res, const char *who) {
+  if (need_dev(c)) return -1;
+  if (!c->dev->rad_ready) {
+    thorin_set_errorf("%s: no system assembled (thorin_rad_system)", who);
+    return -1;
+  }
+  if (!o || !res || o->tol <= 0.0 || o->omega <= 0.0 || o->omega >= 2.0) {
+    thorin_set_errorf("%s: need tol > 0 and 0 < omega < 2", who);
+    return -1;
+  }
+  return 0;
+}
+
+extern "C" int thorin_rad_solve(thorin_ctx *c, const thorin_solve_opts *o, thorin_solve_result *res) {
+  if (rad_solve_check(c, o, res, "thorin_rad_solve")) return -1;
+  thorin_dev *d = c->dev;
+  d->spec.valid = 0;                     // forces enqueued for a next substep are redone
+  if (!d->m32.ml) {
+    float **f[8] = {&d->m32.ml, &d->m32.mip, &d->m32.mim, &d->m32.mjp, &d->m32.mjm, &d->m32.r, &d->m32.e0,
+                    &d->m32.e1};
+    for (int k = 0; k < 8; k++) {
+      HCHECK(hipMalloc(f[k], sizeof(float) * d->nf));
+      zero_f32(d, *f[k], d->nf);
+    }
+  }
+  return rad_solve_on(c, o, res, d->m32);
+}
+
+// ---------------------------------------------------------------------------
+// The energy solve without allocation (thorin_rad_solve_noalloc).  Its FP32
+// planes (matrix diagonals, residual, correction double buffer) are slices of
+// one file-scope device array; each device holds its own copy, whose address
+// is looked up once per device with that device current.
+#define NOALLOC_PLANE_ALIGN 64   // floats: 256-byte aligned planes
+#define NOALLOC_PLANE ((size_t)THORIN_NOALLOC_MAX_CELLS + NOALLOC_PLANE_ALIGN)
+static __device__ __align__(256) float scratch[8 * NOALLOC_PLANE];
+#define NOALLOC_MAXDEV 64
+static std::mutex g_scratch_mu;
+static float *g_scratch_dev[NOALLOC_MAXDEV];   // device address of scratch, per device
+
+static int scratch_for_device(int dev, float **out) {
+  if (dev < 0 || dev >= NOALLOC_MAXDEV) {
+    thorin_set_errorf("thorin_rad_solve_noalloc: device %d beyond the %d the library keeps", dev, NOALLOC_MAXDEV);
+    return -1;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    int cur = 0;
+    HCHECK(hipGetDevice(&cur));
+    if (cur != dev) HCHECK(hipSetDevice(dev));
+    float *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress((void **)&p, HIP_SYMBOL(scratch));
+    if (cur != dev) (void)hipSetDevice(cur);
+    if (e != hipSuccess || !p) {
+      thorin_set_errorf("thorin_rad_solve_noalloc: hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+      return -1;
+    }
+    g_scratch_dev[dev] = p;
+  }
+  *out = g_scratch_dev[dev];
+  return 0;
+}
+
+extern "C" int thorin_rad_solve_noalloc(thorin_ctx *c, const thorin_solve_opts *o, thorin_solve_result *res) {
+  if (rad_solve_check(c, o, res, "thorin_rad_solve_noalloc")) return -1;
+  thorin_dev *d = c->dev;
+  if (o->warm) {
+    thorin_set_error("thorin_rad_solve_noalloc: no warm start (the network allocates its activations)");
+    return -1;
+  }
+  if (d->nf > (size_t)THORIN_NOALLOC_MAX_CELLS) {
+    thorin_set_errorf("thorin_rad_solve_noalloc: (NRAD+1)*NSEC = %zu exceeds THORIN_NOALLOC_MAX_CELLS = %d", d->nf,
+                      THORIN_NOALLOC_MAX_CELLS);
+    return -1;
+  }
+  if (d->prof_on) {
+    thorin_set_error("thorin_rad_solve_noalloc: kernel timing (thorin_prof_enable) creates events; switch it off");
+    return -1;
+  }
+  float *base = nullptr;
+  if (scratch_for_device(d->device, &base)) return -1;
+  d->spec.valid = 0;
+  const size_t stride = (d->nf + NOALLOC_PLANE_ALIGN - 1) / NOALLOC_PLANE_ALIGN * NOALLOC_PLANE_ALIGN;
+  Mat32 m;
+  float **f[8] = {&m.ml, &m.mip, &m.mim, &m.mjp, &m.mjm, &m.r, &m.e0, &m.e1};
+  for (int k = 0; k < 8; k++) *f[k] = base + (size_t)k * stride;
+  // the array keeps the last call's contents (of any grid): the planes start
+  // at zero, as the allocated ones of thorin_rad_solve do
+  zero_f32(d, base, 8 * stride);
+  return rad_solve_on(c, o, res, m);
 }

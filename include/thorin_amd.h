@@ -342,6 +342,22 @@ typedef struct thorin_solve_result {
 int  thorin_rad_system(thorin_ctx *ctx, double dt, double *dt_used);   /* dt <= 0: the hydro CFL step */
 int  thorin_rad_solve(thorin_ctx *ctx, const thorin_solve_opts *o, thorin_solve_result *r);
 
+/* ---- the same solve with no allocation.  No counterpart in the reference;
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * methods, stop tests and bits as thorin_rad_solve, without a warm start (the
+ * network allocates its activations) and with kernel timing off.  Where
+ * thorin_rad_solve allocates its FP32 planes (matrix diagonals, residual,
+ * correction double buffer: 8 floats per cell) on first use, this one takes
+ * them from one library-wide device array of 8 * (THORIN_NOALLOC_MAX_CELLS + 64)
+ * floats (51 MB), which each GPU the library runs on holds until the process
+ * ends; grids with (NRAD+1)*NSEC above THORIN_NOALLOC_MAX_CELLS (1024x1536
+ * fits) are refused.  Every call on a GPU shares that GPU's array and clears
+ * it first: calls that overlap on one GPU (two host threads, two contexts)
+ * race. */
+#define THORIN_NOALLOC_MAX_CELLS 1600000
+int  thorin_rad_solve_noalloc(thorin_ctx *ctx, const thorin_solve_opts *o, thorin_solve_result *r);
+
 /* ---- learned warm start of the energy solve.  No counterpart in the
  * reference.  A 4-level, 32-channel U-Net (BF16 weights and activations,
  * FP32 accumulation on MFMA, FP32 master weights and Adam) maps the features
