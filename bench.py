@@ -19,6 +19,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import shutil
 import subprocess
 import sys
 import tempfile
@@ -114,42 +115,85 @@ def copy_bandwidth():
     return {"bytes_per_copy": 2 * n * 8, "ms": ms, "GBps": 2 * n * 8 / (ms * 1e-3) / 1e9}
 
 
+CPU_CORES = 16      # the host CPU share of one GPU on the benchmark box
+
+
+def _ref_wall(cmd, cfgdir, pf, tmp, cfg, env, tag):
+    out = os.path.join(tmp, tag)
+    os.makedirs(out)
+    par = os.path.join(tmp, f"{tag}.par")
+    with open(par, "w") as f:
+        from fargo_thorin_amd import configs
+        f.write(configs.inpar_text(cfg, out + "/", pf, cfgdir))
+    t0 = time.perf_counter()
+    r = subprocess.run(cmd + [par], cwd=tmp, capture_output=True, text=True, timeout=900, env=env)
+    if r.returncode != 0:
+        raise RuntimeError(f"{tag}: {(r.stderr or r.stdout)[-300:]}")
+    return time.perf_counter() - t0
+
+
 def cpu_baseline_reference(nrad, nsec):
-    """Wall-clock of the reference's sequential build on 2 DTs: runs of
-    Ntot = 1 and Ntot = 3 (AlgoGas runs Ntot+1 times), difference / 2."""
+    """The reference's own builds (oracle/_ref, compiled from its sources by
+    oracle/ref.mk) timed on this host: wall(Ntot = 1 + n) - wall(Ntot = 1)
+    over n DTs (AlgoGas runs Ntot + 1 times; the set-up and the first DT's
+    over-relaxation scan cancel):
+      * sequential (1 core, mpi_dummy), n = 5;
+      * OpenMP (make_omp.sh: -fopenmp, CPU_CORES threads), n = 10;
+      * MPI (BUILD=parallel, mpiexec -n CPU_CORES), n = 10.
+    The value reported is the fastest parallel build; the 128x256 workload's
+    200-DT rate of the sequential and OpenMP builds is added."""
     from fargo_thorin_amd import configs
     from fargo_thorin_amd.seed import seeded_disk, write_cfg
-    ref = os.path.join(ROOT, "oracle", "_ref", "thorin")
-    if not os.path.exists(ref):
+    refdir = os.path.join(ROOT, "oracle", "_ref")
+    seq = os.path.join(refdir, "thorin")
+    if not os.path.exists(seq):
         return None
     tmp = tempfile.mkdtemp(prefix="thorin_cpu_")
-    cfgdir = os.path.join(tmp, "in")
-    os.makedirs(cfgdir)
-    disk = seeded_disk(nrad, nsec)
-    for k in ("dens", "vrad", "vtheta", "temper"):
-        write_cfg(os.path.join(cfgdir, f"gas{k}.cfg"), disk[k])
     pf = os.path.join(tmp, "planet.cfg")
     with open(pf, "w") as f:
         f.write(configs.planet_text(configs.PLANET_ZERO_MASS))
-    walls = {}
-    for ntot in (1, 3):
-        out = os.path.join(tmp, f"out{ntot}")
-        os.makedirs(out)
-        cfg = configs.with_grid(configs.DAMPED_1024x1536, nrad, nsec, Ntot=ntot, Ninterm=ntot)
-        par = os.path.join(tmp, f"in{ntot}.par")
-        with open(par, "w") as f:
-            f.write(configs.inpar_text(cfg, out + "/", pf, cfgdir))
-        print(f"[bench] cpu baseline: reference run with Ntot={ntot}", file=sys.stderr, flush=True)
-        t0 = time.perf_counter()
-        r = subprocess.run([ref, par], cwd=tmp, capture_output=True, text=True, timeout=900)
-        walls[ntot] = time.perf_counter() - t0
-        if r.returncode != 0:
-            return None
-    per_step = (walls[3] - walls[1]) / 2.0
-    return {"value": 1.0 / per_step, "unit": "steps/s", "cores": 1, "kind": "reference",
-            "sample": f"2 DTs of the {nrad}x{nsec} damped workload: wall(Ntot=3) - wall(Ntot=1) of the "
-                      f"reference's sequential build (oracle/_ref/thorin, 1 thread); "
-                      f"{walls[1]:.1f}s and {walls[3]:.1f}s"}
+    env1 = dict(os.environ, OMP_NUM_THREADS="1")
+    envn = dict(os.environ, OMP_NUM_THREADS=str(CPU_CORES))
+    builds = {"sequential": ([seq], env1, 1, 5)}
+    if os.path.exists(os.path.join(refdir, "thorin_omp")):
+        builds["openmp"] = ([os.path.join(refdir, "thorin_omp")], envn, CPU_CORES, 10)
+    mpiexec = shutil.which("mpiexec") or ("/opt/conda/bin/mpiexec" if os.path.exists("/opt/conda/bin/mpiexec") else None)
+    if mpiexec and os.path.exists(os.path.join(refdir, "thorin_mpi")):
+        builds["mpi"] = ([mpiexec, "-n", str(CPU_CORES), os.path.join(refdir, "thorin_mpi")], env1, CPU_CORES, 10)
+    res = {}
+    for grid, cases in (((nrad, nsec), builds),
+                        ((128, 256), {k: v for k, v in builds.items() if k in ("sequential", "openmp")})):
+        gr, gs = grid
+        cfgdir = os.path.join(tmp, f"in{gr}x{gs}")
+        os.makedirs(cfgdir)
+        disk = seeded_disk(gr, gs)
+        for k in ("dens", "vrad", "vtheta", "temper"):
+            write_cfg(os.path.join(cfgdir, f"gas{k}.cfg"), disk[k])
+        base = configs.DAMPED_1024x1536 if grid == (nrad, nsec) else configs.PLAIN_128x256
+        for name, (cmd, env, cores, n) in cases.items():
+            if grid == (128, 256):
+                n = 200
+            print(f"[bench] cpu baseline: reference {name} build, {gr}x{gs}, {n} DTs", file=sys.stderr, flush=True)
+            try:
+                w = {}
+                for ntot in (1, 1 + n):
+                    cfg = configs.with_grid(base, gr, gs, Ntot=ntot, Ninterm=ntot)
+                    w[ntot] = _ref_wall(cmd, cfgdir, pf, tmp, cfg, env, f"{name}_{gr}x{gs}_{ntot}")
+                per = (w[1 + n] - w[1]) / n
+                res[f"{name}_{gr}x{gs}"] = {"steps_per_s": 1.0 / per, "cores": cores, "dts": n,
+                                            "wall_s": [w[1], w[1 + n]]}
+            except Exception as e:  # one build failing leaves the others
+                res[f"{name}_{gr}x{gs}"] = {"error": str(e)}
+    shutil.rmtree(tmp, ignore_errors=True)
+    par_keys = [k for k in (f"openmp_{nrad}x{nsec}", f"mpi_{nrad}x{nsec}") if "steps_per_s" in res.get(k, {})]
+    best = max(par_keys, key=lambda k: res[k]["steps_per_s"]) if par_keys else f"sequential_{nrad}x{nsec}"
+    if "steps_per_s" not in res.get(best, {}):
+        return {"error": "no reference build ran", "builds": res}
+    b = res[best]
+    return {"value": b["steps_per_s"], "unit": "steps/s", "cores": b["cores"], "kind": "reference",
+            "sample": f"{b['dts']} DTs of the {nrad}x{nsec} damped workload, reference {best.split('_')[0]} build "
+                      f"(oracle/_ref), {b['cores']} cores: wall(Ntot={1 + b['dts']}) - wall(Ntot=1)",
+            "builds": res}
 
 
 def sor_study(nrad=2048, nsec=3072, train_s=25.0):
@@ -415,6 +459,34 @@ def ensemble_cnn_study(E, disks, planets, train_s=20.0, tol=1e-10, omega=1.8, in
             "disks": rows}
 
 
+def c_driver_timing(nrad=1024, nsec=1536, ntot=60):
+    """The same workload stepped by the library's C driver (thorin_amd -t):
+    the AlgoGas calls timed in C, with no Python between them.  The driver
+    starts from the analytic InitEuler disk (InitializeFromFile NO), writes
+    outputs only at the first and last DT, and no torque files."""
+    import subprocess
+    import tempfile
+
+    from fargo_thorin_amd import configs
+    exe = os.path.join(ROOT, "fargo_thorin_amd", "thorin_amd")
+    cfg = configs.with_grid(configs.DAMPED_1024x1536, nrad, nsec, Ntot=ntot, Ninterm=ntot,
+                            InitializeFromFile="NO", WriteTorqueFiles="NO")
+    with tempfile.TemporaryDirectory() as tmp:
+        pf = os.path.join(tmp, "planet.cfg")
+        with open(pf, "w") as f:
+            f.write(configs.planet_text(configs.PLANET_ZERO_MASS))
+        par = os.path.join(tmp, "in.par")
+        with open(par, "w") as f:
+            f.write(configs.inpar_text(cfg, tmp + "/out/", pf, tmp))
+        r = subprocess.run([exe, "-t", par], capture_output=True, text=True, timeout=600)
+    line = [x for x in r.stdout.splitlines() if x.startswith("timing:")]
+    if r.returncode != 0 or not line:
+        return {"error": (r.stderr or r.stdout)[-400:]}
+    w = line[-1].split()
+    return {"calls": int(w[1]), "ms_per_step": float(w[6]), "steps_per_s": 1e3 / float(w[6]),
+            "disk": "analytic InitEuler profiles (no noise)"}
+
+
 def main():
     args = parse()
     if args.copy_calibration:
@@ -512,6 +584,8 @@ def main():
             warm_prof.items(), key=lambda kv: -kv[1][0])},
     }
     d.close()
+    print("[bench] the same workload stepped by the C driver", file=sys.stderr, flush=True)
+    out["c_driver"] = c_driver_timing(nrad, nsec)
     if not args.no_sor_study:
         print("[bench] energy-solve study at 2048x3072", file=sys.stderr, flush=True)
         out["sor_study"] = sor_study()

@@ -3563,10 +3563,22 @@ static hipEvent_t prof_event(thorin_dev *d) {
   return e;
 }
 
-static int prof_flush(thorin_dev *d) {
+// Adds the timed launches whose end event has completed to the totals.
+// drain: wait for all of them first (when the totals are read); otherwise
+// the stream is not synchronised (the launches enqueued ahead of the host,
+// such as the next substep's forces, keep running) and the pending ones,
+// in stream order, stay for a later flush.
+static int prof_flush(thorin_dev *d, int drain = 0) {
   if (!d->prof_pend || d->prof_pend->empty()) return 0;
-  HCHECK(hipStreamSynchronize(d->s));
-  for (auto &r : *d->prof_pend) {
+  if (drain) HCHECK(hipStreamSynchronize(d->s));
+  size_t k = 0;
+  for (; k < d->prof_pend->size(); k++) {
+    auto &r = (*d->prof_pend)[k];
+    if (!drain) {
+      const hipError_t q = hipEventQuery(r.second.second);
+      if (q == hipErrorNotReady) break;
+      HCHECK(q);
+    }
     float ms = 0.f;
     HCHECK(hipEventElapsedTime(&ms, r.second.first, r.second.second));
     d->prof_ms[r.first] += ms;
@@ -3574,7 +3586,7 @@ static int prof_flush(thorin_dev *d) {
     d->prof_pool->push_back(r.second.first);
     d->prof_pool->push_back(r.second.second);
   }
-  d->prof_pend->clear();
+  d->prof_pend->erase(d->prof_pend->begin(), d->prof_pend->begin() + (long)k);
   return 0;
 }
 
@@ -3894,7 +3906,7 @@ extern "C" int thorin_ring_means(thorin_ctx *c, int field, double *out) {
 extern "C" int thorin_prof_enable(thorin_ctx *c, int on, unsigned long long mask) {
   if (need_dev(c)) return -1;
   thorin_dev *d = c->dev;
-  if (prof_flush(d)) return -1;
+  if (prof_flush(d, 1)) return -1;
   d->prof_on = on;
   d->prof_mask = mask;
   for (int k = 0; k < 64; k++) { d->prof_ms[k] = 0.0; d->prof_cnt[k] = 0; }
@@ -3904,7 +3916,7 @@ extern "C" int thorin_prof_enable(thorin_ctx *c, int on, unsigned long long mask
 extern "C" int thorin_prof_read(thorin_ctx *c, int kid, double *total_ms, long long *count) {
   if (need_dev(c)) return -1;
   if (kid < 0 || kid >= 64) return -1;
-  if (prof_flush(c->dev)) return -1;
+  if (prof_flush(c->dev, 1)) return -1;
   *total_ms = c->dev->prof_ms[kid];
   *count = c->dev->prof_cnt[kid];
   return 0;
@@ -4386,7 +4398,14 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
     thorin_set_error("PebbleAccretion YES: the pebble disk must be set up with thorin_init_pebbles first");
     return -1;
   }
-  if (log) memset(log, 0, sizeof(*log));
+  if (log) {   // the per-substep arrays are written as substeps are added (nsub says how many)
+    log->nsub = 0;
+    log->crashed_any = 0;
+    log->lost_mass = 0.0;
+    log->crash_dumped = 0;
+    log->crash_time = 0.0;
+    log->tq_rows = 0;
+  }
   d->g.omega_frame = c->st.omega_frame;
   const dim3 tb(TB), rb(RB);
   const dim3 gall = cell_grid(d, nr);
@@ -4573,16 +4592,27 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
         sys->vy[k] += dt * indy;
       }
     }
-    // ---- host N-body (AdvanceSystemRebound, RotatePsys, SynchronizeFargoRebound)
-    if (nbody) {
-      int e = nbody(nbody_user, sys, c->st.physical_time, dt, c->st.omega_frame * dt);
+    // ---- host N-body (AdvanceSystemRebound, RotatePsys, SynchronizeFargoRebound,
+    // SourceEuler.c:260-269) comes next in the reference.  No kernel from
+    // here to the end of the substep reads the planets (the forces were
+    // formed above; in the fixed frame nothing else is), so the substep's
+    // kernels are enqueued first and the host steps the planets while they
+    // run; k_cfl_final and the next substep's forces, which use the new
+    // planets, are enqueued after it.
+    const double t_nbody = c->st.physical_time;
+    int nbody_done = 0;
+    auto step_planets = [&]() -> int {
+      if (nbody_done || !nbody) return 0;
+      nbody_done = 1;
+      int e = nbody(nbody_user, sys, t_nbody, dt, c->st.omega_frame * dt);
       if (e) {
         char m[600];
         snprintf(m, sizeof m, "N-body step failed (%d): %s", e, thorin_last_error());
         thorin_set_error(m);
         return -1;
       }
-    }
+      return 0;
+    };
     // ---- ApplyBoundaryCondition, DetectCrash, ComputePressureField, SubStep1
     if (p->damping_on)
       LAUNCH(k_boundary, gall, tb, 0, d->g, d->b[B_VRAD], d->b[B_VTHETA], d->b[B_DENS], d->b[B_ENERGY], dt,
@@ -4691,7 +4721,6 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
       DCOPY((double *)d->g.cav_med, c->cav_med, sizeof(double) * n1, hipMemcpyHostToDevice);
       DCOPY((double *)d->g.cav_inf, c->cav_inf, sizeof(double) * n1, hipMemcpyHostToDevice);
     }
-    d->cfl_invdtreb = minstep_rebound(sys);
     const int *guard = &d->ctl->sor_done;
     double *bsave[NBUF];
     memcpy(bsave, d->b, sizeof bsave);
@@ -4752,6 +4781,9 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
       // minima of the next substep (its planets are those of now)
       LAUNCH(k_eos_cfl, dim3(nr), rb, eos_lds(d, 1), d->g, d->b[B_DENS], d->b[B_ENERGY],
              d->b[B_TEMPER], d->b[B_CS], d->b[B_VRAD], d->b[B_VTHETA], 1, guard);
+      // the planets' step, overlapped with the kernels enqueued so far
+      if (step_planets()) return -1;
+      d->cfl_invdtreb = minstep_rebound(sys);
       // one sync per substep: SOR result, crash flags, dt_stellar, the next
       // CFL minimum, the planet accelerations, written to the pinned mirror
       // by k_cfl_final.  It does nothing while the SOR has not converged:
@@ -4800,6 +4832,7 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
       log->dt[log->nsub] = dt;
       log->sor_niter[log->nsub] = niter;
       log->sor_omega[log->nsub] = omega_used;
+      log->crashed[log->nsub] = 0;
       log->nsub++;
     }
     d->cfl_ready = 1;

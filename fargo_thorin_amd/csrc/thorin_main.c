@@ -4,7 +4,10 @@
  * planets and the disk, then call AlgoGas once per DT and write the
  * gas*<n>.dat outputs every NINTERM steps (Output.c:153-237).
  *
- *   thorin_amd [-v] [-0] [-s N] [-o outputdir] in.par
+ *   thorin_amd [-v] [-0] [-t] [-s N] [-o outputdir] in.par
+ *
+ * -t prints the wall time of the AlgoGas calls (CLOCK_MONOTONIC around each
+ * call, the first three excluded) when the run ends.
  *
  * Output files and formats follow the reference: raw native-endian float64
  * gas<name><n>.dat (no header, ring-major), dims.dat, used_rad.dat,
@@ -20,6 +23,13 @@
 #include <string.h>
 #include <sys/stat.h>
 #include <sys/types.h>
+#include <time.h>
+
+static double now_s(void) {
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
 
 static void die(const char *what) {
   fprintf(stderr, "thorin_amd: %s: %s\n", what, thorin_last_error());
@@ -168,7 +178,8 @@ int main(int argc, char **argv) {
   thorin_state st;
   thorin_step_log *lg;
   const char *parfile = NULL, *outdir = NULL;
-  int i, k, restart = 0, nrestart = 0, only_init = 0, verbose = 0, begin_i = 0;
+  int i, k, restart = 0, nrestart = 0, only_init = 0, verbose = 0, begin_i = 0, timing = 0, ntimed = 0, ncalls = 0;
+  double t_calls = 0.0;
   size_t cnt;
   double *buf, *buf2;
   char fn[2 * THORIN_STRLEN];
@@ -176,6 +187,7 @@ int main(int argc, char **argv) {
   for (i = 1; i < argc; i++) {
     if (argv[i][0] == '-') {
       if (strchr(argv[i], 'v')) verbose = 1;
+      if (strchr(argv[i], 't')) timing = 1;
       if (strchr(argv[i], '0')) only_init = 1;
       if (strchr(argv[i], 's')) { restart = 1; if (++i < argc) nrestart = atoi(argv[i]); }
       else if (strchr(argv[i], 'o')) { if (++i < argc) outdir = argv[i]; }
@@ -279,7 +291,11 @@ int main(int argc, char **argv) {
       if (f) { fprintf(f, "%d\t%#.18g\n", ts, st.omega_frame); fclose(f); }
       if (only_init) break;
     }
-    k = thorin_algogas(c, &sys, thorin_nbody_step, NULL, lg);
+    {
+      const double t0 = now_s();
+      k = thorin_algogas(c, &sys, thorin_nbody_step, NULL, lg);
+      if (++ncalls > 3) { t_calls += now_s() - t0; ntimed++; }
+    }
     if (k < 0) die("AlgoGas");
     if (lg->crash_dumped) {   /* SourceEuler.c:275-284 */
       printf("\nCrash! at time %.12g\n", lg->crash_time);
@@ -299,6 +315,9 @@ int main(int argc, char **argv) {
         }
       }
   }
+  if (timing && ntimed > 0)
+    printf("timing: %d AlgoGas calls after 3: %.4f ms per call (%.1f calls/s)\n", ntimed, 1e3 * t_calls / ntimed,
+           ntimed / t_calls);
   thorin_ctx_destroy(c);
   free(buf);
   free(buf2);

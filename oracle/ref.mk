@@ -23,7 +23,21 @@ REB_SRC  := rebound tree particle gravity integrator integrator_whfast integrato
 MAIN_OBJ := $(MAIN_SRC:%=$(OUT)/main_%.o)
 REB_OBJ  := $(REB_SRC:%=$(OUT)/reb_%.o)
 
-all: $(OUT)/thorin
+# The reference's two parallel builds, for CPU timing only (not parity
+# oracles: the OpenMP red-black SOR colours race, EnergySources.c:459-484, and
+# the MPI build splits the ring sums over ranks):
+#   thorin_omp : FARGO_ARCH=THORIN_OPENMP, BUILD=sequential (make_omp.sh,
+#                makefile:39-46): -fopenmp -DOPENMP, REBOUND with OPENMP=1
+#   thorin_mpi : FARGO_ARCH=THORIN, BUILD=parallel (makefile:32-37, 109-113):
+#                mpicc -D_PARALLEL without mpi_dummy (MPICH wrapper, host gcc);
+#                built only where an
+#                mpicc exists (MPICC, else mpicc on PATH, else /opt/conda/bin)
+MPICC ?= $(shell command -v mpicc 2>/dev/null || (test -x /opt/conda/bin/mpicc && echo /opt/conda/bin/mpicc))
+OMP_OBJ  := $(MAIN_SRC:%=$(OUT)/omp_main_%.o) $(REB_SRC:%=$(OUT)/omp_reb_%.o)
+MPI_MAIN := $(filter-out mpi_dummy,$(MAIN_SRC))
+MPI_OBJ  := $(MPI_MAIN:%=$(OUT)/mpi_main_%.o) $(REB_OBJ)
+
+all: $(OUT)/thorin $(OUT)/thorin_omp $(if $(MPICC),$(OUT)/thorin_mpi)
 
 $(OUT):
 	mkdir -p $(OUT)
@@ -36,6 +50,21 @@ $(OUT)/reb_%.o: $(REF)/src_reb/%.c | $(OUT)
 
 $(OUT)/thorin: $(MAIN_OBJ) $(REB_OBJ)
 	$(CC) -O3 -o $@ $(MAIN_OBJ) $(REB_OBJ) -lm -lrt
+
+$(OUT)/omp_main_%.o: $(REF)/src_main/%.c | $(OUT)
+	$(CC) -c $(MAIN_CFLAGS) -fopenmp -DOPENMP -o $@ $<
+
+$(OUT)/omp_reb_%.o: $(REF)/src_reb/%.c | $(OUT)
+	$(CC) -c $(REB_CFLAGS) -fopenmp -DOPENMP -o $@ $<
+
+$(OUT)/thorin_omp: $(OMP_OBJ)
+	$(CC) -O3 -fopenmp -o $@ $(OMP_OBJ) -lm -lrt
+
+$(OUT)/mpi_main_%.o: $(REF)/src_main/%.c | $(OUT)
+	$(MPICC) -cc=$(CC) -c $(MAIN_CFLAGS) -D_PARALLEL -o $@ $<
+
+$(OUT)/thorin_mpi: $(MPI_OBJ)
+	$(MPICC) -cc=$(CC) -O3 -o $@ $(MPI_OBJ) -lm -lrt
 
 clean:
 	rm -rf $(OUT)
