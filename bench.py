@@ -36,9 +36,9 @@ HBM_PEAK_GBS = 8000.0
 # read, T written = 64 B per cell and iteration over rings 1..NRAD-2.
 ALG_BYTES_PER_CELL = {
     "k_sor_iter": 64, "k_transport_theta": 104, "k_transport_rad": 80, "k_forces": 48,
-    "k_substep1": 64, "k_divstress": 56, "k_visc_update": 64, "k_substep2": 56, "k_qplus": 48,
-    "k_temper_cs": 32, "k_opacity_qminus": 40, "k_diffcoef": 40, "k_matrix": 104, "k_velocities": 56,
-    "k_energy_new": 24, "k_indirect": 48, "k_eos_cfl": 48, "k_sor_resid0": 56,
+    "k_substep1": 64, "k_divstress": 56, "k_visc_update": 64, "k_substep2": 56, "k_matrix": 104, "k_velocities": 56,
+    "k_energy_new": 24, "k_indirect": 48, "k_eos_cfl": 48, "k_eos_cfl_vel": 80, "k_sor_resid0": 56,
+    "k_substep3": 96,
 }
 # Whole step: the fused-dataflow minimum of the hydro step, 82 field passes
 # per substep (656 B/cell) plus 64 B/cell per SOR iteration.

@@ -66,7 +66,7 @@ enum {
   B_AGR = THORIN_F_AGR, B_AGT = THORIN_F_AGT, B_EINT = THORIN_F_ENERGYINT,
   B_ENEW = THORIN_F_ENERGYNEW, B_VRADINT = THORIN_F_VRADINT, B_VTHETAINT = THORIN_F_VTHETAINT,
   B_VRADNEW = THORIN_F_VRADNEW, B_VTHETANEW = THORIN_F_VTHETANEW,
-  B_DIFF = THORIN_NFIELDS, B_ML, B_MLIP, B_MLIM, B_MLJP, B_MLJM, B_RHS, B_TBACK, B_TEMPER2, B_TEMPER3,
+  B_DIFF = THORIN_NFIELDS, B_ML, B_MLIP, B_MLIM, B_MLJP, B_MLJM, B_RHS, B_TBACK, B_TEMPER2, B_TEMPER3, B_TEMPER4,
   B_RP, B_RM, B_TP, B_TM, B_XRP, B_XRM, B_XTP, B_XTM, B_XE, B_XRHO,
   // pebble fluid work fields (Pebbles.c:37-56, SourceEuler.c:155-158)
   B_PAGR, B_PAGT, B_PACCR, B_PACCT, B_GACCR, B_GACCT, B_DRAGR, B_DRAGT, B_MACC,
@@ -85,7 +85,8 @@ struct DevCtl {                 // device-side control block, mirrored to pinned
   double dt_stellar;            // DiffusionTimestep minimum
   int crash_e, crash_rho;
   int sor_done, sor_niter, sor_iter;
-  int sor_cur;                  // T buffer holding the latest iterate (0: B_TEMPER, 1: B_TEMPER2)
+  int sor_cur;                  // T buffer holding the latest iterate (0: B_TEMPER, 1: B_TEMPER2, 2: B_TEMPER3,
+                                // 3: B_TEMPER4)
   double sor_normres0, sor_normres;
   double axstar, aystar;
   double axpl[MAXPL], aypl[MAXPL];
@@ -202,7 +203,7 @@ struct thorin_dev {
   int cfl_ready;                // hctl->newdt_raw holds the CFL minimum of the current fields
   int niter_hist[4], nsolve;    // SOR iteration counts of the last four solves
   int rs_rows;                  // rings per workgroup of k_ring_means (LDS holds them whole)
-  int sor_nit;                  // SOR iterations per launch (1 or 2; thorin_tune)
+  int sor_nit;                  // SOR iterations per launch (1, 2 or 3; thorin_tune)
   int sor_queue_min;            // ensemble: SOR iterations to queue at least
   ForceSet spec;                // force kernels enqueued ahead for the next substep
   int heat_max;                 // heatsrc_max: set by the pebble accretion when AccretionalHeating is on
@@ -210,6 +211,7 @@ struct thorin_dev {
   hipEvent_t ev_t0, ev_t1;      // the energy solve's timing (thorin_rad_solve*)
   double *h_part;               // pinned: block partials of the FP64 residual (nblk_cells)
   int device;                   // the HIP device of the context's stream
+  int damp_nlo, damp_hi0;       // k_boundary's ring bands: 0 .. damp_nlo-1 and damp_hi0 .. NRAD-1
   int rad_rows;                 // rings per tile of the radial transport (8, 16, 32; thorin_tune)
   int rad_ready;                // thorin_rad_system has assembled the energy system
   struct Mat32 m32;             // FP32 copies for the mixed-precision solve (allocated on first use)
@@ -233,7 +235,7 @@ static const char *const KNAMES[] = {
     "k_divstress", "k_visc_update", "k_substep2",
     "k_matrix", "k_sor_resid0", "k_sor_iter", "k_sor_check", "k_energy_new", "k_transport_rad",
     "k_transport_theta", "k_velocities", "k_substep3", "k_peb_cfl", "k_peb_damp", "k_peb_source",
-    "k_peb_substep1", "k_peb_accrete", "k_peb_evaporate", "k_ring_means", NULL};
+    "k_peb_substep1", "k_peb_accrete", "k_peb_evaporate", "k_ring_means", "k_eos_cfl_vel", NULL};
 
 // a template instance ("k_name<3>") counts as its kernel
 static int kid_of(const char *name) {
@@ -708,12 +710,27 @@ __device__ __forceinline__ double ordered_ring_sum(const double *sh, int ns) {
 // ring mean are formed before the sums (EC_MAX cells per thread) and the sum
 // under the square root is completed afterwards in the reference's order.
 // With do_eos = 0 the temperature and sound speed already in memory are used.
+//
+// VEL (the end of a gas-only substep): ComputeVelocities
+// (TransportEuler.c:350-381) from the transport's momenta and the damping of
+// ApplyBoundaryCondition (SideEuler.c:380-414, ZEROVRAD or no damping: vrad
+// only, pointwise, with the ring's lambda) are done here too, and vr, vtheta
+// written: the workgroup of ring i forms vtheta of ring i (and of its next
+// sector) and vr of rings i and i+1 with the same expressions as the owner
+// of each cell, so every value equals what the separate kernels give.
 #define EC_MAX 12              // cells per thread: NSEC <= RB * EC_MAX
 #define EC_PAD 4
-__device__ __forceinline__ void b_eos_cfl(const Geo &g, const double *__restrict__ rho, const double *__restrict__ e,
-                                                double *__restrict__ T, double *__restrict__ cs,
-                                                const double *__restrict__ vr, const double *__restrict__ vt,
-                                                int do_eos, const int *__restrict__ guard) {
+struct VelSrc {                 // VEL: the momenta of the transport and the damping
+  const double *rp, *rm, *tp, *tm;
+  double step;                  // dt (lambda = WaveKiller * step)
+  int damp;                     // ZEROVRAD damping on
+};
+
+template <bool VEL>
+__device__ __forceinline__ void b_eos_cfl_t(const Geo &g, const double *__restrict__ rho, const double *__restrict__ e,
+                                            double *__restrict__ T, double *__restrict__ cs, double *__restrict__ vr,
+                                            double *__restrict__ vt, int do_eos, const int *__restrict__ guard,
+                                            const VelSrc &vs) {
   extern __shared__ double sh[];       // (2 or 3)*ns: vt, cs [, T]
   __shared__ double red[RB / 64];
   __shared__ double vmoy, gcs_i, gtemp_i;
@@ -736,7 +753,32 @@ __device__ __forceinline__ void b_eos_cfl(const Geo &g, const double *__restrict
     q4[k] = 0.0;
     if (j < ns) {
       const size_t l = base + j;
-      const double vtl = vt[l];
+      double vtl, vrl = 0.0, vrp = 0.0, vtp = 0.0;
+      if (VEL) {
+        const double rl = rho[l];
+        const size_t ljm = (j == 0) ? base + ns - 1 : l - 1;
+        const size_t ljp = (j == ns - 1) ? base : l + 1;
+        vtl = (vs.tp[ljm] + vs.tm[l]) / (rl + rho[ljm] + 1e-15) / g.Rmed[i] - g.Rmed[i] * g.omega_frame;
+        vtp = (vs.tp[l] + vs.tm[ljp]) / (rho[ljp] + rl + 1e-15) / g.Rmed[i] - g.Rmed[i] * g.omega_frame;
+        if (i > 0) vrl = (vs.rp[l - ns] + vs.rm[l]) / (rl + rho[l - ns] + 1e-20);
+        if (vs.damp && g.dampring[i]) {
+          const double lam = g.WaveKiller[i] * vs.step;
+          vrl = (vrl + lam * 0.0) / (1.0 + lam);
+        }
+        if (i + 1 < nr) {
+          vrp = (vs.rp[l] + vs.rm[l + ns]) / (rho[l + ns] + rl + 1e-20);
+          if (vs.damp && g.dampring[i + 1]) {
+            const double lam = g.WaveKiller[i + 1] * vs.step;
+            vrp = (vrp + lam * 0.0) / (1.0 + lam);
+          }
+        } else {
+          vrp = vr[l + ns];        // the ghost ring (no kernel of the substep writes it)
+        }
+        vt[l] = vtl;
+        vr[l] = vrl;
+      } else {
+        vtl = vt[l];
+      }
       s_vt[j] = vtl;
       double c;
       if (do_eos) {
@@ -754,9 +796,14 @@ __device__ __forceinline__ void b_eos_cfl(const Geo &g, const double *__restrict
       if (cfl) {
         const int jp = (j == ns - 1) ? 0 : j + 1;
         const double invdt1 = c / mn;
-        const double invdt2 = fabs(vr[l]) / dxrad;
-        double dvr = vr[l + ns] - vr[l];
-        double dvt = vt[base + jp] - vtl;
+        if (!VEL) {
+          vrl = vr[l];
+          vrp = vr[l + ns];
+          vtp = vt[base + jp];
+        }
+        const double invdt2 = fabs(vrl) / dxrad;
+        double dvr = vrp - vrl;
+        double dvt = vtp - vtl;
         if (dvr >= 0.0) dvr = 1e-10; else dvr = -dvr;
         if (dvt >= 0.0) dvt = 1e-10; else dvt = -dvt;
         const double a = dvr / dxrad, b = dvt / dxtheta;
@@ -853,11 +900,30 @@ __device__ __forceinline__ void b_eos_cfl(const Geo &g, const double *__restrict
   m = block_max<RB>(m, red);
   if (tid == 0) g.ringmax[i] = m;
 }
+__device__ __forceinline__ void b_eos_cfl(const Geo &g, const double *__restrict__ rho, const double *__restrict__ e,
+                                          double *__restrict__ T, double *__restrict__ cs, const double *__restrict__ vr,
+                                          const double *__restrict__ vt, int do_eos, const int *__restrict__ guard) {
+  const VelSrc none = {nullptr, nullptr, nullptr, nullptr, 0.0, 0};
+  b_eos_cfl_t<false>(g, rho, e, T, cs, const_cast<double *>(vr), const_cast<double *>(vt), do_eos, guard, none);
+}
 __global__ void __launch_bounds__(RB) k_eos_cfl(Geo g, const double *__restrict__ rho, const double *__restrict__ e,
                                                 double *__restrict__ T, double *__restrict__ cs,
                                                 const double *__restrict__ vr, const double *__restrict__ vt,
                                                 int do_eos, const int *__restrict__ guard) {
   b_eos_cfl(g, rho, e, T, cs, vr, vt, do_eos, guard);
+}
+__device__ __forceinline__ void b_eos_cfl_vel(const Geo &g, const double *__restrict__ rho,
+                                              const double *__restrict__ e, double *__restrict__ T,
+                                              double *__restrict__ cs, double *__restrict__ vr, double *__restrict__ vt,
+                                              const int *__restrict__ guard, const VelSrc &vs) {
+  b_eos_cfl_t<true>(g, rho, e, T, cs, vr, vt, 1, guard, vs);
+}
+__global__ void __launch_bounds__(RB) k_eos_cfl_vel(Geo g, const double *__restrict__ rho,
+                                                    const double *__restrict__ e, double *__restrict__ T,
+                                                    double *__restrict__ cs, double *__restrict__ vr,
+                                                    double *__restrict__ vt, const int *__restrict__ guard,
+                                                    VelSrc vs) {
+  b_eos_cfl_vel(g, rho, e, T, cs, vr, vt, guard, vs);
 }
 
 
@@ -1776,13 +1842,18 @@ __global__ void __launch_bounds__(ACC_T) k_peb_accrete(Geo g, PebAccK P, double 
 
 // ---------------------------------------------------------------------------
 // DampingBoundary (SideEuler.c:368-419) on the rings flagged by the host
+// The launch covers only the two bands of damped rings: rings 0 .. nlo-1 and
+// hi0 .. NRAD-1 (DRin and DRout bound contiguous bands), one TB-wide column
+// block per workgroup.
 __device__ __forceinline__ void b_boundary(const Geo &g, double *__restrict__ vrad, double *__restrict__ vtheta,
                                                  double *__restrict__ rho, double *__restrict__ energy, double step,
-                                                 const int *__restrict__ guard) {
+                                                 const int *__restrict__ guard, int nlo, int hi0) {
   const int ns = g.ns;
-  int i, j, L_;
   if (guard && *guard == 0) return;
-  if (!xcd_cell(g, 0, g.nr, i, j, L_) || j >= ns || !g.dampring[i]) return;
+  const int b = (int)blockIdx.x - g.blk0, q = b / g.ncb;
+  const int i = q < nlo ? q : hi0 + (q - nlo);
+  const int j = (b - q * g.ncb) * TB + (int)threadIdx.x;
+  if (i >= g.nr || j >= ns || !g.dampring[i]) return;
   double vrad0 = 0.0, rho0 = 0.0, energy0 = 0.0, vtheta0 = 0.0;
   if (g.damp_init) {
     rho0 = g.SigmaMed[i];
@@ -1805,8 +1876,8 @@ __device__ __forceinline__ void b_boundary(const Geo &g, double *__restrict__ vr
 }
 __global__ void __launch_bounds__(TB) k_boundary(Geo g, double *__restrict__ vrad, double *__restrict__ vtheta,
                                                  double *__restrict__ rho, double *__restrict__ energy, double step,
-                                                 const int *__restrict__ guard) {
-  b_boundary(g, vrad, vtheta, rho, energy, step, guard);
+                                                 const int *__restrict__ guard, int nlo, int hi0) {
+  b_boundary(g, vrad, vtheta, rho, energy, step, guard, nlo, hi0);
 }
 
 
@@ -2383,9 +2454,10 @@ __global__ void __launch_bounds__(TB) k_sor_resid0(Geo g, const double *__restri
 // a halo cell is updated with the same inputs and arithmetic as its owner
 // (the cells it reads were updated one half-sweep earlier one cell further
 // out), so after the last half-sweep the owned cells hold exactly the
-// iterates of a sequential sweep.  Iterate n lives in X[n % 3] (X = T0, T1,
-// T2; the initial T in T0): a launch from iteration it reads X[it % 3] and
-// writes X[(it+k+1) % 3] after its k-th iteration, so that a solve that stops
+// iterates of a sequential sweep.  Iterate n lives in X[n % 4] (X = T0 .. T3;
+// the initial T in T0): a launch from iteration it reads X[it % 4] and writes
+// X[(it+k+1) % 4] after its k-th iteration (k < NIT <= 3, so no launch writes
+// the buffer other workgroups stage their halos from), and a solve that stops
 // inside the launch has its iterate (ctl->sor_cur names the buffer).  The
 // stop-test partials of each iteration are summed over the owned cells in a
 // fixed order (through LDS), the same for every NIT, so the iteration count
@@ -2458,12 +2530,14 @@ __device__ __forceinline__ bool sor_cell(int sl, int h, bool odd, int i0, int j0
 
 template <int NIT>
 __device__ __forceinline__ void b_sor_iter(const Geo &g, double *__restrict__ T0, double *__restrict__ T1,
-                                           double *__restrict__ T2, const double *__restrict__ Ml,
+                                           double *__restrict__ T2, double *__restrict__ T3,
+                                           const double *__restrict__ Ml,
                                            const double *__restrict__ Mlip, const double *__restrict__ Mlim,
                                            const double *__restrict__ Mljp, const double *__restrict__ Mljm,
                                            const double *__restrict__ RHS, double omega, double *__restrict__ psum,
                                            int *__restrict__ pcnt, DevCtl *__restrict__ ctl, int it,
                                            const double *__restrict__ mresp, int nbm, int nbs) {
+  static_assert(NIT >= 1 && NIT <= 3, "four iterate buffers hold at most three iterations per launch");
   typedef SorShape<NIT> S;
   __shared__ double t[S::LR * S::LW];
   __shared__ double sacc[SOR_TI * SOR_TJ];
@@ -2499,18 +2573,20 @@ __device__ __forceinline__ void b_sor_iter(const Geo &g, double *__restrict__ T0
       for (int j = j0 + tid; j < j0 + tw; j += TB) {
         T1[j] = T0[j];
         T2[j] = T0[j];
+        T3[j] = T0[j];
       }
     if (i1 == nr - 1)
       for (int j = j0 + tid; j < j0 + tw; j += TB)
         for (int r = nr - 1; r <= nr; r++) {
           T1[(size_t)r * ns + j] = T0[(size_t)r * ns + j];
           T2[(size_t)r * ns + j] = T0[(size_t)r * ns + j];
+          T3[(size_t)r * ns + j] = T0[(size_t)r * ns + j];
         }
   } else if (ctl->sor_done) {
     return;
   }
-  double *X[3] = {T0, T1, T2};
-  const double *__restrict__ Tin = X[it % 3];
+  double *X[4] = {T0, T1, T2, T3};
+  const double *__restrict__ Tin = X[it % 4];
   SorCell rc[S::NRED], bc[S::NBLK];
   auto setup = [&](SorCell *cs, int ncell, int h, bool odd) {
 #pragma unroll
@@ -2558,7 +2634,7 @@ __device__ __forceinline__ void b_sor_iter(const Geo &g, double *__restrict__ T0
       if (bc[q].k >= 0 && bc[q].d <= hb) sor_update<S::LW>(bc[q], t, omega, sacc, schg);
     __syncthreads();
     // iterate it+k+1 of the owned cells, and its stop-test partials
-    double *__restrict__ Tout = X[(it + k + 1) % 3];
+    double *__restrict__ Tout = X[(it + k + 1) % 4];
     for (int idx = tid; idx < nown; idx += TB) {
       const int r = idx / tw, c = idx - r * tw;
       Tout[(size_t)(i0 + r) * ns + (j0 + c)] = t[(r + S::H) * S::LW + (c + S::H)];
@@ -2580,13 +2656,14 @@ __device__ __forceinline__ void b_sor_iter(const Geo &g, double *__restrict__ T0
 
 template <int NIT>
 __global__ void __launch_bounds__(TB) k_sor_iter(Geo g, double *__restrict__ T0, double *__restrict__ T1,
-                                                 double *__restrict__ T2, const double *__restrict__ Ml,
+                                                 double *__restrict__ T2, double *__restrict__ T3,
+                                                 const double *__restrict__ Ml,
                                                  const double *__restrict__ Mlip, const double *__restrict__ Mlim,
                                                  const double *__restrict__ Mljp, const double *__restrict__ Mljm,
                                                  const double *__restrict__ RHS, double omega, double *__restrict__ psum,
                                                  int *__restrict__ pcnt, DevCtl *__restrict__ ctl, int it,
                                                  const double *__restrict__ mresp, int nbm, int nbs) {
-  b_sor_iter<NIT>(g, T0, T1, T2, Ml, Mlip, Mlim, Mljp, Mljm, RHS, omega, psum, pcnt, ctl, it, mresp, nbm, nbs);
+  b_sor_iter<NIT>(g, T0, T1, T2, T3, Ml, Mlip, Mlim, Mljp, Mljm, RHS, omega, psum, pcnt, ctl, it, mresp, nbm, nbs);
 }
 
 // stop test of iterations it+1 .. it+NIT (EnergySources.c:487-493), partial
@@ -2638,7 +2715,7 @@ __device__ __forceinline__ void b_sor_check(const double *__restrict__ psum, con
     for (int k = 0; k < NIT; k++) {
       const int n = it + k + 1;
       ctl->sor_iter = n;
-      ctl->sor_cur = n % 3;
+      ctl->sor_cur = n % 4;
       ctl->sor_normres = s[k];
       if (s[k] < eps * ctl->sor_normres0 || s[k] < SORMIN || c[k] == 0.0) {
         ctl->sor_done = 1;
@@ -2660,18 +2737,19 @@ __global__ void __launch_bounds__(RB) k_sor_check(const double *__restrict__ psu
 }
 
 // edges of T and EnergyNew = rho*T*CV (EnergySources.c:319-334).  The SOR's
-// last iterate is in T0, T1 or T2 (ctl->sor_cur); T0 receives it, so that T0
+// last iterate is in T0 .. T3 (ctl->sor_cur); T0 receives it, so that T0
 // holds the solved temperature afterwards.  Nothing is done until the SOR
 // has converged (ctl->sor_done).
 __device__ __forceinline__ void b_energy_new(const Geo &g, const double *__restrict__ rho, double *__restrict__ T0,
                                              const double *__restrict__ T1, const double *__restrict__ T2,
-                                             double *__restrict__ enew, const DevCtl *__restrict__ ctl) {
+                                             const double *__restrict__ T3, double *__restrict__ enew,
+                                             const DevCtl *__restrict__ ctl) {
   const int ns = g.ns, nr = g.nr;
   int i, j, L_;
   if (ctl->sor_done == 0) return;
   if (!xcd_cell(g, 0, g.nr, i, j, L_) || j >= ns) return;
   const int cur = ctl->sor_cur;
-  const double *__restrict__ Ts = cur == 0 ? T0 : (cur == 1 ? T1 : T2);
+  const double *__restrict__ Ts = cur == 0 ? T0 : (cur == 1 ? T1 : (cur == 2 ? T2 : T3));
   const size_t l = (size_t)i * ns + j;
   double t = Ts[l];
   if (i == 0) t = Ts[l + ns];
@@ -2682,8 +2760,9 @@ __device__ __forceinline__ void b_energy_new(const Geo &g, const double *__restr
 }
 __global__ void __launch_bounds__(TB) k_energy_new(Geo g, const double *__restrict__ rho, double *__restrict__ T0,
                                                    const double *__restrict__ T1, const double *__restrict__ T2,
-                                                   double *__restrict__ enew, const DevCtl *__restrict__ ctl) {
-  b_energy_new(g, rho, T0, T1, T2, enew, ctl);
+                                                   const double *__restrict__ T3, double *__restrict__ enew,
+                                                   const DevCtl *__restrict__ ctl) {
+  b_energy_new(g, rho, T0, T1, T2, T3, enew, ctl);
 }
 
 
@@ -3159,6 +3238,13 @@ static inline dim3 cell_grid(const thorin_dev *d, int rings) {
   return dim3((unsigned)((cell_count(d, rings) + 7) / 8 * 8));
 }
 
+// k_boundary: the two bands of damped rings (at least one block)
+static inline dim3 bnd_grid(const thorin_dev *d) {
+  const int rings = d->damp_nlo + (d->nr - d->damp_hi0);
+  const int nb = rings * ((d->ns + TB - 1) / TB);
+  return dim3((unsigned)(nb > 0 ? nb : 1));
+}
+
 extern "C" int thorin_dev_create(thorin_ctx *c) {
   thorin_dev *d = (thorin_dev *)calloc(1, sizeof(thorin_dev));
   if (!d) { thorin_set_error("out of host memory"); return -1; }
@@ -3218,6 +3304,8 @@ extern "C" int thorin_dev_create(thorin_ctx *c) {
       return -1;
     }
     HCHECK(hipFuncSetAttribute((const void *)k_eos_cfl, hipFuncAttributeMaxDynamicSharedMemorySize, (int)eos_need));
+    HCHECK(hipFuncSetAttribute((const void *)k_eos_cfl_vel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)eos_need));
   }
   HCHECK(hipStreamCreateWithFlags(&d->s, hipStreamNonBlocking));
   HCHECK(hipEventCreateWithFlags(&d->ev_end, hipEventDisableTiming));
@@ -3277,6 +3365,17 @@ extern "C" int thorin_dev_create(thorin_ctx *c) {
       dr[i] = p->damping_on ? z : 0;
       dr[n1 + i] = z;
     }
+    // the bands k_boundary is launched over: rings below the first undamped
+    // ring and from the last undamped ring on (all rings when none is undamped)
+    int lo = 0, hi = c->nr;
+    while (lo < c->nr && dr[lo]) lo++;
+    if (lo < c->nr) {
+      while (hi > lo && dr[hi - 1]) hi--;
+      for (int i = lo; i < hi; i++)
+        if (dr[i]) { lo = 0; hi = 0; break; }   // damped rings inside: cover the whole grid
+    }
+    d->damp_nlo = lo;
+    d->damp_hi0 = hi;
     HCHECK(hipMemcpyAsync(ip + n1, dr.data(), sizeof(int) * 2 * n1, hipMemcpyHostToDevice, d->s));
     HCHECK(hipStreamSynchronize(d->s));
   }
@@ -3344,8 +3443,8 @@ extern "C" int thorin_dev_create(thorin_ctx *c) {
   HCHECK(hipMalloc(&d->red, sizeof(double) * (size_t)d->nblk_cells * (2 + 2 * 8)));
   const int half = c->ns / 2;
   d->nblk_sor = ((half + TB - 1) / TB) * (c->nr - 2);
-  HCHECK(hipMalloc(&d->sorp, sizeof(double) * (size_t)d->nblk_cells * 2));
-  HCHECK(hipMalloc(&d->sorc, sizeof(int) * (size_t)d->nblk_cells * 2));
+  HCHECK(hipMalloc(&d->sorp, sizeof(double) * (size_t)d->nblk_cells * 3));   // up to 3 iterations per launch
+  HCHECK(hipMalloc(&d->sorc, sizeof(int) * (size_t)d->nblk_cells * 3));
   HCHECK(hipMalloc(&d->pstellar, sizeof(double) * (size_t)d->nblk_cells));
   HCHECK(hipMalloc(&d->mresp, sizeof(double) * (size_t)d->nblk_cells));
   HCHECK(hipHostMalloc(&d->h_part, sizeof(double) * (size_t)d->nblk_cells, hipHostMallocDefault));
@@ -3646,6 +3745,7 @@ struct Batch {
     static constexpr int nt = NT;              \
   };
 BATCH_BODY(k_eos_cfl, b_eos_cfl, RB)
+BATCH_BODY(k_eos_cfl_vel, b_eos_cfl_vel, RB)
 BATCH_BODY(k_cfl_final, b_cfl_final, RB)
 BATCH_BODY((k_forces<false, 1>), (b_forces<false, 1>), TB)
 BATCH_BODY((k_forces<false, 8>), (b_forces<false, 8>), TB)
@@ -3669,8 +3769,10 @@ BATCH_BODY(k_matrix, b_matrix, TB)
 BATCH_BODY(k_sor_resid0, b_sor_resid0, TB)
 BATCH_BODY(k_sor_iter<1>, b_sor_iter<1>, TB)
 BATCH_BODY(k_sor_iter<2>, b_sor_iter<2>, TB)
+BATCH_BODY(k_sor_iter<3>, b_sor_iter<3>, TB)
 BATCH_BODY(k_sor_check<1>, b_sor_check<1>, RB)
 BATCH_BODY(k_sor_check<2>, b_sor_check<2>, RB)
+BATCH_BODY(k_sor_check<3>, b_sor_check<3>, RB)
 BATCH_BODY(k_energy_new, b_energy_new, TB)
 BATCH_BODY(k_transport_rad<8>, b_transport_rad<8>, TB)
 BATCH_BODY(k_transport_rad<16>, b_transport_rad<16>, TB)
@@ -3933,7 +4035,7 @@ extern "C" const char *thorin_prof_kernel_name(int kid) {
 
 // enqueue iterations it0 .. it0+count-1 (iteration + stop test each); the
 // first one also forms the initial residual when the matrix kernel has not
-// count is rounded up to whole launches of sor_nit iterations
+// launches of sor_nit iterations while that many remain, then single ones
 static int sor_launch(thorin_ctx *c, double omega, int it0, int &count, double eps = SOREPS,
                       int maxiter = SORMAXITERS) {
   thorin_dev *d = c->dev;
@@ -3950,21 +4052,24 @@ static int sor_launch(thorin_ctx *c, double omega, int it0, int &count, double e
   }
   const int nbs = ((ns + SOR_TJ - 1) / SOR_TJ) * ((nr - 2 + SOR_TI - 1) / SOR_TI);
   const dim3 gs((unsigned)((nbs + 7) / 8 * 8));
-  const int nit = d->sor_nit == 2 ? 2 : 1;
-  count = (count + nit - 1) / nit * nit;
-  for (int it = it0; it < it0 + count; it += nit) {
-    if (nit == 2) {
-      LAUNCH(k_sor_iter<2>, gs, dim3(TB), 0, g, d->b[B_TEMPER], d->b[B_TEMPER2], d->b[B_TEMPER3], d->b[B_ML],
-             d->b[B_MLIP], d->b[B_MLIM], d->b[B_MLJP], d->b[B_MLJM], d->b[B_RHS], omega, d->sorp, d->sorc, d->ctl,
-             it, (const double *)d->mresp, d->nb_mresp, nbs);
-      LAUNCH(k_sor_check<2>, dim3(1), dim3(RB), 0, d->sorp, d->sorc, nbs, d->ctl, maxiter, it, eps);
-    } else {
-      LAUNCH(k_sor_iter<1>, gs, dim3(TB), 0, g, d->b[B_TEMPER], d->b[B_TEMPER2], d->b[B_TEMPER3], d->b[B_ML],
-             d->b[B_MLIP], d->b[B_MLIM], d->b[B_MLJP], d->b[B_MLJM], d->b[B_RHS], omega, d->sorp, d->sorc, d->ctl,
-             it, (const double *)d->mresp, d->nb_mresp, nbs);
-      LAUNCH(k_sor_check<1>, dim3(1), dim3(RB), 0, d->sorp, d->sorc, nbs, d->ctl, maxiter, it, eps);
-    }
+  // launches of sor_nit iterations while that many remain, then single ones
+  // (the iterations queued beyond the expected count are a margin)
+  const int nmax = (d->sor_nit >= 1 && d->sor_nit <= 3) ? d->sor_nit : 1;
+#define SOR_LAUNCH(N)                                                                                             \
+  do {                                                                                                            \
+    LAUNCH(k_sor_iter<N>, gs, dim3(TB), 0, g, d->b[B_TEMPER], d->b[B_TEMPER2], d->b[B_TEMPER3], d->b[B_TEMPER4],  \
+           d->b[B_ML], d->b[B_MLIP], d->b[B_MLIM], d->b[B_MLJP], d->b[B_MLJM], d->b[B_RHS], omega, d->sorp,        \
+           d->sorc, d->ctl, it, (const double *)d->mresp, d->nb_mresp, nbs);                                     \
+    LAUNCH(k_sor_check<N>, dim3(1), dim3(RB), 0, d->sorp, d->sorc, nbs, d->ctl, maxiter, it, eps);               \
+  } while (0)
+  for (int it = it0; it < it0 + count;) {
+    const int nit = (it0 + count - it >= nmax) ? nmax : 1;
+    if (nit == 3) SOR_LAUNCH(3);
+    else if (nit == 2) SOR_LAUNCH(2);
+    else SOR_LAUNCH(1);
+    it += nit;
   }
+#undef SOR_LAUNCH
   HCHECK(hipGetLastError());
   return 0;
 }
@@ -4179,8 +4284,9 @@ static int peb_accrete_host(thorin_ctx *c, const thorin_planets *sys, PebAccK *P
 // momenta, [energy,] density into the X buffers, azimuthal pass with the
 // FARGO shift back into the fields and the momenta, ComputeVelocities.  be < 0:
 // no energy (TransportPebbles).  g.VMed is the fluid's vtheta ring mean.
+// with_vel = 0: the velocities are left to the end-of-substep kernel (k_eos_cfl_vel)
 static int transport(thorin_dev *d, const Geo &g, const RadQ &q, int brho, int bvr, int bvt, int be, double dt,
-                     const int *guard) {
+                     const int *guard, int with_vel = 1) {
   const int nr = d->nr, ns = d->ns;
   const dim3 gall = cell_grid(d, nr), tb(TB);
   const dim3 grad((unsigned)((q.ntiles + 7) / 8 * 8 * q.nq));
@@ -4204,8 +4310,9 @@ static int transport(thorin_dev *d, const Geo &g, const RadQ &q, int brho, int b
     default: TT_LAUNCH(6); break;
   }
 #undef TT_LAUNCH
-  LAUNCH(k_velocities, gall, tb, 0, g, d->b[brho], d->b[B_RP], d->b[B_RM], d->b[B_TP], d->b[B_TM], d->b[bvr],
-         d->b[bvt], guard);
+  if (with_vel)
+    LAUNCH(k_velocities, gall, tb, 0, g, d->b[brho], d->b[B_RP], d->b[B_RM], d->b[B_TP], d->b[B_TM], d->b[bvr],
+           d->b[bvt], guard);
   return 0;
 }
 
@@ -4615,8 +4722,8 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
     };
     // ---- ApplyBoundaryCondition, DetectCrash, ComputePressureField, SubStep1
     if (p->damping_on)
-      LAUNCH(k_boundary, gall, tb, 0, d->g, d->b[B_VRAD], d->b[B_VTHETA], d->b[B_DENS], d->b[B_ENERGY], dt,
-             (const int *)nullptr);
+      LAUNCH(k_boundary, bnd_grid(d), tb, 0, d->g, d->b[B_VRAD], d->b[B_VTHETA], d->b[B_DENS], d->b[B_ENERGY], dt,
+             (const int *)nullptr, d->damp_nlo, d->damp_hi0);
     if (peb) {
       // DetectCrashPebbles, then SourceTermsPebbles (SourceEuler.c:274, 291)
       LAUNCH(k_peb_damp, gall, tb, 0, d->g, d->b[B_PDENS], d->b[B_PVR], d->b[B_PVT], dt, 1, d->ctl,
@@ -4665,8 +4772,8 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
     swapb(d, B_VRAD, B_VRADNEW);
     swapb(d, B_VTHETA, B_VTHETANEW);
     if (p->damping_on)
-      LAUNCH(k_boundary, gall, tb, 0, d->g, d->b[B_VRAD], d->b[B_VTHETA], d->b[B_DENS], d->b[B_ENERGY], dt,
-             (const int *)nullptr);
+      LAUNCH(k_boundary, bnd_grid(d), tb, 0, d->g, d->b[B_VRAD], d->b[B_VTHETA], d->b[B_DENS], d->b[B_ENERGY], dt,
+             (const int *)nullptr, d->damp_nlo, d->damp_hi0);
     HCHECK(hipGetLastError());
     // ---- SubStep3: Q+, implicit radiative diffusion
     int niter = 0;
@@ -4722,11 +4829,14 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
       DCOPY((double *)d->g.cav_inf, c->cav_inf, sizeof(double) * n1, hipMemcpyHostToDevice);
     }
     const int *guard = &d->ctl->sor_done;
+    // a gas-only substep with no damping or ZEROVRAD damping forms the
+    // velocities and their damping in its end-of-substep kernel
+    const bool fuse_vel = !peb && (!p->damping_on || (p->damp_vrad && !p->damp_init && !p->damp_flux));
     double *bsave[NBUF];
     memcpy(bsave, d->b, sizeof bsave);
     auto after_sor = [&]() -> int {
       LAUNCH(k_energy_new, gall, tb, 0, d->g, d->b[B_DENS], d->b[B_TEMPER], d->b[B_TEMPER2], d->b[B_TEMPER3],
-             d->b[B_ENEW], d->ctl);
+             d->b[B_TEMPER4], d->b[B_ENEW], d->ctl);
       swapb(d, B_ENERGY, B_ENEW);
       // ---- Transport
       {
@@ -4743,10 +4853,11 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
         q.rows = d->rad_rows;
         q.ntiles = q.ntile_j * ((nr + q.rows - 1) / q.rows);
         q.nq = RC_NQ;
-        if (transport(d, d->g, q, B_DENS, B_VRAD, B_VTHETA, B_ENERGY, dt, guard)) return -1;
+        if (transport(d, d->g, q, B_DENS, B_VRAD, B_VTHETA, B_ENERGY, dt, guard, !fuse_vel)) return -1;
       }
-      if (p->damping_on)
-        LAUNCH(k_boundary, gall, tb, 0, d->g, d->b[B_VRAD], d->b[B_VTHETA], d->b[B_DENS], d->b[B_ENERGY], dt, guard);
+      if (p->damping_on && !fuse_vel)
+        LAUNCH(k_boundary, bnd_grid(d), tb, 0, d->g, d->b[B_VRAD], d->b[B_VTHETA], d->b[B_DENS], d->b[B_ENERGY], dt,
+               guard, d->damp_nlo, d->damp_hi0);
       if (peb) {
         // ---- EvolvePebbleDisk (SourceEuler.c:312, Pebbles.c:699-708): damping,
         // TransportPebbles (TransportEuler.c:449-505: the gas scheme without
@@ -4779,8 +4890,14 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
       }
       // ---- end of the substep: T, cs and their ring means, and the CFL
       // minima of the next substep (its planets are those of now)
-      LAUNCH(k_eos_cfl, dim3(nr), rb, eos_lds(d, 1), d->g, d->b[B_DENS], d->b[B_ENERGY],
-             d->b[B_TEMPER], d->b[B_CS], d->b[B_VRAD], d->b[B_VTHETA], 1, guard);
+      if (fuse_vel) {
+        const VelSrc vs = {d->b[B_RP], d->b[B_RM], d->b[B_TP], d->b[B_TM], dt, p->damping_on};
+        LAUNCH(k_eos_cfl_vel, dim3(nr), rb, eos_lds(d, 1), d->g, d->b[B_DENS], d->b[B_ENERGY], d->b[B_TEMPER],
+               d->b[B_CS], d->b[B_VRAD], d->b[B_VTHETA], guard, vs);
+      } else {
+        LAUNCH(k_eos_cfl, dim3(nr), rb, eos_lds(d, 1), d->g, d->b[B_DENS], d->b[B_ENERGY],
+               d->b[B_TEMPER], d->b[B_CS], d->b[B_VRAD], d->b[B_VTHETA], 1, guard);
+      }
       // the planets' step, overlapped with the kernels enqueued so far
       if (step_planets()) return -1;
       d->cfl_invdtreb = minstep_rebound(sys);
@@ -5234,6 +5351,27 @@ static int tune_time(thorin_dev *d, const std::function<int()> &work, int reps, 
   return 0;
 }
 
+// The SOR iterations a solve is expected to take: the most of the last
+// solves (as the queue of a substep counts them), 3 before any solve.
+static int sor_expected(const thorin_ctx *c) {
+  int e = c->st.sor_niter_last;
+  for (int k = 0; k < 4; k++) e = std::max(e, c->dev->niter_hist[k]);
+  return e > 0 ? e : 3;
+}
+
+// The launch shape that finishes a solve of `expect` iterations first: the
+// queue runs expect / n launches of n iterations and the rest one at a time
+// (launches queued after the stop return at once).
+static int sor_pick(const double *us_per_iter, int expect) {
+  int best = 1;
+  double tbest = 0.0;
+  for (int n = 1; n <= 3; n++) {
+    const double t = us_per_iter[n - 1] * (double)((expect / n) * n) + us_per_iter[0] * (double)(expect % n);
+    if (n == 1 || t < tbest) { best = n; tbest = t; }
+  }
+  return best;
+}
+
 extern "C" int thorin_autotune(thorin_ctx *c, thorin_tuning *out) {
   if (need_dev(c)) return -1;
   thorin_dev *d = c->dev;
@@ -5255,8 +5393,8 @@ extern "C" int thorin_autotune(thorin_ctx *c, thorin_tuning *out) {
   d->normres0_valid = 0;
   d->rad_ready = 0;
   const int saved_nit = d->sor_nit, saved_rows = d->rad_rows;
-  const int iters = 8;
-  for (int v = 0; v < 2; v++) {
+  const int iters = 6;          // whole launches of 1, 2 and 3 iterations
+  for (int v = 0; v < 3; v++) {
     d->sor_nit = v + 1;
     double ms = 0.0;
     auto work = [&]() -> int {
@@ -5266,7 +5404,7 @@ extern "C" int thorin_autotune(thorin_ctx *c, thorin_tuning *out) {
     };
     if (work()) return -1;      // warm-up
     if (tune_time(d, work, 3, &ms)) return -1;
-    // the copy of T (one pass) is timed too; it is the same for both
+    // the copy of T (one pass) is timed too; it is the same for all
     t.sor_us_per_iter[v] = 1e3 * ms / iters;
   }
   RadQ q;
@@ -5303,7 +5441,7 @@ extern "C" int thorin_autotune(thorin_ctx *c, thorin_tuning *out) {
   d->normres0_valid = 0;
   (void)saved_nit;
   (void)saved_rows;
-  t.sor_iters_per_launch = t.sor_us_per_iter[1] < t.sor_us_per_iter[0] ? 2 : 1;
+  t.sor_iters_per_launch = sor_pick(t.sor_us_per_iter, sor_expected(c));
   int bi = 0;
   for (int v = 1; v < 3; v++)
     if (t.rad_us[v] < t.rad_us[bi]) bi = v;
@@ -5316,9 +5454,9 @@ extern "C" int thorin_autotune(thorin_ctx *c, thorin_tuning *out) {
 
 extern "C" int thorin_set_tuning(thorin_ctx *c, const thorin_tuning *t) {
   if (need_dev(c)) return -1;
-  if (!t || (t->sor_iters_per_launch != 1 && t->sor_iters_per_launch != 2) ||
+  if (!t || t->sor_iters_per_launch < 1 || t->sor_iters_per_launch > 3 ||
       (t->rad_rows != 8 && t->rad_rows != 16 && t->rad_rows != 32)) {
-    thorin_set_error("thorin_set_tuning: sor_iters_per_launch must be 1 or 2, rad_rows 8, 16 or 32");
+    thorin_set_error("thorin_set_tuning: sor_iters_per_launch must be 1, 2 or 3, rad_rows 8, 16 or 32");
     return -1;
   }
   c->dev->sor_nit = t->sor_iters_per_launch;
@@ -5343,11 +5481,13 @@ extern "C" int thorin_ensemble_autotune(thorin_ensemble *e, thorin_tuning *out) 
   for (int k = 0; k < e->n; k++) {
     thorin_tuning t;
     if (thorin_autotune(e->c[k], &t)) return -1;
-    for (int v = 0; v < 2; v++) sum.sor_us_per_iter[v] += t.sor_us_per_iter[v];
+    for (int v = 0; v < 3; v++) sum.sor_us_per_iter[v] += t.sor_us_per_iter[v];
     for (int v = 0; v < 3; v++) sum.rad_us[v] += t.rad_us[v];
   }
   const int rows[3] = {8, 16, 32};
-  sum.sor_iters_per_launch = sum.sor_us_per_iter[1] < sum.sor_us_per_iter[0] ? 2 : 1;
+  int expect = 0;
+  for (int k = 0; k < e->n; k++) expect = std::max(expect, sor_expected(e->c[k]));
+  sum.sor_iters_per_launch = sor_pick(sum.sor_us_per_iter, expect);
   int bi = 0;
   for (int v = 1; v < 3; v++)
     if (sum.rad_us[v] < sum.rad_us[bi]) bi = v;
@@ -5695,9 +5835,9 @@ static int rad_solve_on(thorin_ctx *c, const thorin_solve_opts *o, thorin_solve_
     int niter = 0;
     if (sor_finish(c, o->omega, 0, &niter, o->tol, maxit, 16)) return -1;
     res->sweeps = niter > maxit ? maxit : niter;
-    // the last iterate into B_TEMPER (it is in T0, T1 or T2 by sor_cur)
+    // the last iterate into B_TEMPER (it is in T0 .. T3 by sor_cur)
     if (d->hctl->sor_cur)
-      HCHECK(hipMemcpyAsync(d->b[B_TEMPER], d->b[d->hctl->sor_cur == 1 ? B_TEMPER2 : B_TEMPER3], nb,
+      HCHECK(hipMemcpyAsync(d->b[B_TEMPER], d->b[d->hctl->sor_cur == 1 ? B_TEMPER2 : (d->hctl->sor_cur == 2 ? B_TEMPER3 : B_TEMPER4)], nb,
                             hipMemcpyDeviceToDevice, d->s));
     res->rounds = 1;
   } else if (o->method == 1) {

@@ -274,17 +274,20 @@ int  thorin_ensemble_stats(const thorin_ensemble *e, long long *launches, long l
 void thorin_ensemble_destroy(thorin_ensemble *e);
 
 /* ---- autotuner.  No counterpart in the reference.  Launch shapes that
- * leave every result bit-identical: SOR iterations per launch (1, or 2 with
- * a four-cell halo in LDS and three temperature buffers) and rings per tile
+ * leave every result bit-identical: SOR iterations per launch (1, 2 or 3,
+ * with a halo of twice that many cells in LDS and four temperature buffers)
+ * and rings per tile
  * of the radial transport (8, 16, 32).  thorin_autotune times each on the
  * context's grid with HIP events (on a synthetic system; the temperature is
  * restored, the energy system is assembled again by the next substep) and
- * keeps the fastest; thorin_ensemble_autotune picks one shape for all disks
- * (least time summed over them).  Call them between steps. */
+ * keeps the fastest: for the SOR, the shape that finishes a solve of the
+ * expected length first (the most iterations of the last solves, 3 before
+ * any; a solve runs whole launches); thorin_ensemble_autotune picks one shape
+ * for all disks (least time summed over them).  Call them between steps. */
 typedef struct thorin_tuning {
-  int sor_iters_per_launch;       /* 1 or 2 (default 1) */
+  int sor_iters_per_launch;       /* 1, 2 or 3 (default 1) */
   int rad_rows;                   /* 8, 16 or 32 (default 16) */
-  double sor_us_per_iter[2];      /* measured: per SOR iteration, 1 / 2 per launch */
+  double sor_us_per_iter[3];      /* measured: per SOR iteration, 1 / 2 / 3 per launch */
   double rad_us[3];               /* measured: one radial transport launch, 8 / 16 / 32 rings */
 } thorin_tuning;
 int  thorin_autotune(thorin_ctx *ctx, thorin_tuning *out);

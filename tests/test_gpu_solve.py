@@ -96,3 +96,23 @@ def test_gpu_blocked_fp32_sweeps_equal_single_sweeps(nr, ns, inner):
         assert r["sweeps"] == ref["sweeps"] and r["rounds"] == ref["rounds"]
         assert np.array_equal(d.field("temper"), t1), f"sweep_block {blk}"
     assert ref["converged"]
+
+
+def test_gpu_fp64_sor_iterations_per_launch_give_the_same_bits():
+    """The FP64 SOR with 1, 2 and 3 iterations per launch (LDS halos of 2, 4
+    and 6 cells, four iterate buffers): the same sweep count and the same
+    temperature bit for bit, on a solve long enough that the stop falls
+    inside a launch."""
+    d = _disk(130, 256)
+    d.rad_system(0.0)
+    ref = None
+    for nit in (1, 2, 3):
+        d.set_tuning(nit, 16)
+        r = d.rad_solve(0, tol=1e-12, omega=1.7)
+        t = d.field("temper")
+        if ref is None:
+            ref = (r["sweeps"], t)
+            assert r["sweeps"] % 3 != 0 or r["sweeps"] % 2 != 0
+        else:
+            assert r["sweeps"] == ref[0], nit
+            assert np.array_equal(t, ref[1]), nit
