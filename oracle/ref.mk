@@ -37,7 +37,16 @@ OMP_OBJ  := $(MAIN_SRC:%=$(OUT)/omp_main_%.o) $(REB_SRC:%=$(OUT)/omp_reb_%.o)
 MPI_MAIN := $(filter-out mpi_dummy,$(MAIN_SRC))
 MPI_OBJ  := $(MPI_MAIN:%=$(OUT)/mpi_main_%.o) $(REB_OBJ)
 
-all: $(OUT)/thorin $(OUT)/thorin_omp $(if $(MPICC),$(OUT)/thorin_mpi)
+# The reference's own driver with this build's AlgoGas (the drop-in, for the
+# tests): its objects as above, SourceEuler.o's AlgoGas made weak so that the
+# shim fargo_thorin_amd/shim/algogas_shim.c (compiled with the reference's
+# headers) replaces it, linked with fargo_thorin_amd/libthorin_amd.so.
+SHIM    := fargo_thorin_amd/shim/algogas_shim.c
+LIBGPU  := fargo_thorin_amd/libthorin_amd.so
+GPU_OBJ := $(filter-out $(OUT)/main_SourceEuler.o,$(MAIN_OBJ)) $(OUT)/main_SourceEuler_weak.o \
+           $(OUT)/shim_algogas.o $(REB_OBJ)
+
+all: $(OUT)/thorin $(OUT)/thorin_omp $(if $(MPICC),$(OUT)/thorin_mpi) $(if $(wildcard $(LIBGPU)),$(OUT)/thorin_gpu)
 
 $(OUT):
 	mkdir -p $(OUT)
@@ -70,3 +79,12 @@ clean:
 	rm -rf $(OUT)
 
 .PHONY: all clean
+
+$(OUT)/main_SourceEuler_weak.o: $(OUT)/main_SourceEuler.o
+	objcopy --weaken-symbol=AlgoGas $< $@
+
+$(OUT)/shim_algogas.o: $(SHIM) include/thorin_amd.h | $(OUT)
+	$(CC) -c $(MAIN_CFLAGS) -Iinclude -o $@ $<
+
+$(OUT)/thorin_gpu: $(GPU_OBJ) $(LIBGPU)
+	$(CC) -O3 -o $@ $(GPU_OBJ) -Lfargo_thorin_amd -lthorin_amd -Wl,-rpath,'$$ORIGIN/../../fargo_thorin_amd' -lm -lrt

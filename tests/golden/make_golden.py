@@ -143,55 +143,71 @@ def _write_raw(path, arr):
     np.ascontiguousarray(arr, dtype=np.float64).tofile(path)
 
 
-def run_case(name, cfg, planets, full):
+def prepare_case(name, cfg, planets, tmp, binary=REF):
+    """Writes a case's inputs under tmp (seeded InitializeFromFile arrays, the
+    opacity table, planet.cfg, in.par) and returns (par, outdir, disk, extra).
+    A "_seeded_pebbles" case first runs `binary` fresh for output 0 and puts
+    the seeded pebble fields in its place (extra: the fresh set-up's SHA-256
+    digests); the run then restarts from output 0 (-s 0)."""
     cfg = dict(cfg)
     seeded_pebbles = bool(cfg.pop("_seeded_pebbles", ""))
     # "_temper_cell": "i,j,factor" scales one cell of the initial temperature
     temper_cell = cfg.pop("_temper_cell", "")
     nrad, nsec = int(cfg["Nrad"]), int(cfg["Nsec"])
-    with tempfile.TemporaryDirectory() as tmp:
-        cfgdir = os.path.join(tmp, "in")
-        outdir = os.path.join(tmp, "out")
-        os.makedirs(cfgdir)
-        os.makedirs(outdir)
-        disk = seeded_disk(nrad, nsec, log_grid=cfg.get("RadialSpacing", "LOG")[0] in "lL",
-                           rmin=float(cfg["Rmin"]), rmax=float(cfg["Rmax"]), h=float(cfg["AspectRatio"]),
-                           sigma0=float(cfg["Sigma0"]), slope=float(cfg["SigmaSlope"]))
-        if temper_cell:
-            ci, cj, cf = temper_cell.split(",")
-            disk["temper"] = disk["temper"].copy()
-            disk["temper"][int(ci), int(cj)] *= float(cf)
-        for k in ("dens", "vrad", "vtheta", "temper"):
-            write_cfg(os.path.join(cfgdir, f"gas{k}.cfg"), disk[k])
-        if cfg.get("OpacityTable", "")[:1] in "sS":
-            t = gas_table()
-            write_opacity_table(os.path.join(cfgdir, "data_gasopacity_malygin.txt"), t["rows"], t["cols"],
-                                t["pgrid"], t["tgrid"], t["planck"], t["rosseland"])
-        pf = os.path.join(tmp, "planet.cfg")
-        with open(pf, "w") as f:
-            f.write(configs.planet_text(planets))
-        par = os.path.join(tmp, "in.par")
-        restart0 = {}
-        if seeded_pebbles:
-            # fresh run that writes output 0 only (and one DT that is discarded)
-            with open(par, "w") as f:
-                f.write(configs.inpar_text(dict(cfg, Ntot="0"), outdir + "/", pf, cfgdir))
-            r0 = subprocess.run([REF, par], cwd=tmp, capture_output=True, text=True, timeout=3600)
-            if r0.returncode != 0:
-                raise RuntimeError(f"reference set-up run failed on {name}:\n{r0.stdout[-2000:]}")
-            for k in ("dens", "vrad", "vtheta", "temper"):
-                a = np.fromfile(os.path.join(outdir, f"gas{k}0.dat"), dtype=np.float64)
-                restart0[f"fresh_sha_{k}0"] = np.frombuffer(hashlib.sha256(a.tobytes()).digest(), dtype=np.uint8)
-            pb = seeded_pebbles_of(disk, cfg)
-            for k, src in (("dens", "pdens"), ("vrad", "pvrad"), ("vtheta", "pvtheta"), ("size", "psize")):
-                _write_raw(os.path.join(outdir, f"gaspebble{k}0.dat"), pb[src])
-            for fn in os.listdir(outdir):
-                if fn.startswith("tqwk"):
-                    os.remove(os.path.join(outdir, fn))
+    cfgdir = os.path.join(tmp, "in")
+    outdir = os.path.join(tmp, "out")
+    os.makedirs(cfgdir)
+    os.makedirs(outdir)
+    disk = seeded_disk(nrad, nsec, log_grid=cfg.get("RadialSpacing", "LOG")[0] in "lL",
+                       rmin=float(cfg["Rmin"]), rmax=float(cfg["Rmax"]), h=float(cfg["AspectRatio"]),
+                       sigma0=float(cfg["Sigma0"]), slope=float(cfg["SigmaSlope"]))
+    if temper_cell:
+        ci, cj, cf = temper_cell.split(",")
+        disk["temper"] = disk["temper"].copy()
+        disk["temper"][int(ci), int(cj)] *= float(cf)
+    for k in ("dens", "vrad", "vtheta", "temper"):
+        write_cfg(os.path.join(cfgdir, f"gas{k}.cfg"), disk[k])
+    if cfg.get("OpacityTable", "")[:1] in "sS":
+        t = gas_table()
+        write_opacity_table(os.path.join(cfgdir, "data_gasopacity_malygin.txt"), t["rows"], t["cols"],
+                            t["pgrid"], t["tgrid"], t["planck"], t["rosseland"])
+    pf = os.path.join(tmp, "planet.cfg")
+    with open(pf, "w") as f:
+        f.write(configs.planet_text(planets))
+    par = os.path.join(tmp, "in.par")
+    extra = {}
+    if seeded_pebbles:
+        # fresh run that writes output 0 only (and one DT that is discarded)
         with open(par, "w") as f:
-            f.write(configs.inpar_text(cfg, outdir + "/", pf, cfgdir))
+            f.write(configs.inpar_text(dict(cfg, Ntot="0"), outdir + "/", pf, cfgdir))
+        r0 = subprocess.run([binary, par], cwd=tmp, capture_output=True, text=True, timeout=3600)
+        if r0.returncode != 0:
+            raise RuntimeError(f"set-up run failed on {name}:\n{r0.stdout[-2000:]}\n{r0.stderr[-2000:]}")
+        for k in ("dens", "vrad", "vtheta", "temper"):
+            a = np.fromfile(os.path.join(outdir, f"gas{k}0.dat"), dtype=np.float64)
+            extra[f"fresh_sha_{k}0"] = np.frombuffer(hashlib.sha256(a.tobytes()).digest(), dtype=np.uint8)
+        pb = seeded_pebbles_of(disk, cfg)
+        for k, src in (("dens", "pdens"), ("vrad", "pvrad"), ("vtheta", "pvtheta"), ("size", "psize")):
+            _write_raw(os.path.join(outdir, f"gaspebble{k}0.dat"), pb[src])
+        for fn in os.listdir(outdir):
+            if fn.startswith("tqwk"):
+                os.remove(os.path.join(outdir, fn))
+    with open(par, "w") as f:
+        f.write(configs.inpar_text(cfg, outdir + "/", pf, cfgdir))
+    return par, outdir, disk, extra
+
+
+def run_args(cfg, par, debug=True):
+    """The command-line arguments of a case's main run."""
+    return (["-d"] if debug else []) + (["-s", "0"] if cfg.get("_seeded_pebbles") else []) + [par]
+
+
+def run_case(name, cfg, planets, full):
+    nrad, nsec = int(cfg["Nrad"]), int(cfg["Nsec"])
+    with tempfile.TemporaryDirectory() as tmp:
+        par, outdir, disk, restart0 = prepare_case(name, cfg, planets, tmp)
         # -d: the reference prints its CFL block and "SOR: Niter = n" for every substep
-        cmd = [REF, "-d"] + (["-s", "0"] if seeded_pebbles else []) + [par]
+        cmd = [REF] + run_args(cfg, par)
         r = subprocess.run(cmd, cwd=tmp, capture_output=True, text=True, timeout=7200)
         if r.returncode != 0:
             raise RuntimeError(f"reference failed on {name}:\n{r.stdout[-2000:]}\n{r.stderr[-2000:]}")

@@ -2,7 +2,9 @@
 trajectory replay used to pin the hydro step independently of the N-body."""
 from __future__ import annotations
 
+import contextlib
 import os
+import tempfile
 
 import numpy as np
 
@@ -109,3 +111,12 @@ def initial_disk(name):
 def sha(a) -> np.ndarray:
     import hashlib
     return np.frombuffer(hashlib.sha256(np.ascontiguousarray(a).tobytes()).digest(), dtype=np.uint8)
+
+
+@contextlib.contextmanager
+def short_tmpdir():
+    """A working directory for runs of the reference's driver, under /tmp:
+    WriteDiskPolar forms its file names in an 80-byte buffer (Output.c:159),
+    so the output directory's path has to stay short."""
+    with tempfile.TemporaryDirectory(prefix="t", dir="/tmp") as d:
+        yield d
