@@ -3608,6 +3608,71 @@ extern "C" int thorin_sync(thorin_ctx *c) {
   return 0;
 }
 
+// The device math library as the kernels call it (same translation unit,
+// same flags), one value per thread.
+__global__ void k_math_eval(int fn, const double *__restrict__ x, const double *__restrict__ y,
+                            double *__restrict__ out, int n) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const double a = x[k];
+  double r;
+  switch (fn) {
+    case THORIN_MATH_POW: r = pow(a, y[k]); break;
+    case THORIN_MATH_EXP: r = exp(a); break;
+    case THORIN_MATH_LOG: r = log(a); break;
+    case THORIN_MATH_LOG10: r = log10(a); break;
+    case THORIN_MATH_TANH: r = tanh(a); break;
+    case THORIN_MATH_SIN: r = sin(a); break;
+    case THORIN_MATH_ATAN2: r = atan2(a, y[k]); break;
+    case THORIN_MATH_ASIN: r = asin(a); break;
+    case THORIN_MATH_ATAN: r = atan(a); break;
+    default: r = sqrt(a); break;
+  }
+  out[k] = r;
+}
+
+extern "C" int thorin_math_eval(int fn, const double *x, const double *y, double *out, size_t n) {
+  if (fn < 0 || fn >= THORIN_MATH_N) {
+    thorin_set_error("thorin_math_eval: unknown function");
+    return -1;
+  }
+  const int two = (fn == THORIN_MATH_POW || fn == THORIN_MATH_ATAN2);
+  if (!x || !out || (two && !y) || n > (size_t)1 << 26) {
+    thorin_set_error("thorin_math_eval: needs x, out (and y for pow, atan2) and at most 2^26 values");
+    return -1;
+  }
+  if (n == 0) return 0;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+    thorin_set_error("no HIP device available: the FARGO_THORIN GPU path needs an MI355X (gfx950)");
+    return -1;
+  }
+  const size_t nb = n * sizeof(double);
+  double *dx = nullptr, *dy = nullptr, *dout = nullptr;
+  int rc = -1;
+  hipError_t e = hipMalloc((void **)&dx, nb);
+  if (e == hipSuccess && two) e = hipMalloc((void **)&dy, nb);
+  if (e == hipSuccess) e = hipMalloc((void **)&dout, nb);
+  if (e == hipSuccess) e = hipMemcpy(dx, x, nb, hipMemcpyHostToDevice);
+  if (e == hipSuccess && two) e = hipMemcpy(dy, y, nb, hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_math_eval, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0, fn, dx, dy, dout, (int)n);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpy(out, dout, nb, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) {
+    rc = 0;
+  } else {
+    char m[200];
+    snprintf(m, sizeof m, "thorin_math_eval: HIP error %s", hipGetErrorString(e));
+    thorin_set_error(m);
+  }
+  if (dx) (void)hipFree(dx);
+  if (dy) (void)hipFree(dy);
+  if (dout) (void)hipFree(dout);
+  return rc;
+}
+
 
 __global__ void k_int_to_double(const int *__restrict__ a, double *__restrict__ b, int n) {
   int k = blockIdx.x * blockDim.x + threadIdx.x;

@@ -396,6 +396,16 @@ int    thorin_cnn_grad2(thorin_cnn *n, int H, int W, const float *x, const float
  * (default); 1 = mean square of the normalised residual of the guess */
 int    thorin_cnn_set_loss(thorin_cnn *n, int mode);
 
+/* ---- the device math library, for checking it.  No counterpart in the
+ * reference (whose host builds call glibc's).  out[i] = f(x[i]), or
+ * f(x[i], y[i]) for pow and atan2, evaluated on the GPU by the device
+ * functions the kernels call, compiled with the kernels' flags; host arrays
+ * of n <= 2^26 values. */
+enum { THORIN_MATH_POW = 0, THORIN_MATH_EXP, THORIN_MATH_LOG, THORIN_MATH_LOG10, THORIN_MATH_TANH,
+       THORIN_MATH_SIN, THORIN_MATH_ATAN2, THORIN_MATH_ASIN, THORIN_MATH_ATAN, THORIN_MATH_SQRT,
+       THORIN_MATH_N };
+int  thorin_math_eval(int fn, const double *x, const double *y, double *out, size_t n);
+
 /* Synchronise the device and return the library's last error text. */
 int  thorin_sync(thorin_ctx *ctx);
 const char *thorin_last_error(void);

@@ -12,17 +12,24 @@
  * gcc -std=c99 (no FMA contraction, as the reference's makefile:33 build),
  * the oracle reproduces the reference bit for bit.
  *
- * Parity pins (tests/test_oracle_golden.py, tests/test_oracle_math.py):
- *   - bitwise against outputs of the reference itself, built from its
- *     sources by oracle/ref.mk and run on seeded disks (tests/golden/);
- *   - closed forms / invariants: opacity laws vs independent restatements,
- *     flux limiter, SOR solution vs a direct sparse solve, mass conservation
- *     of transport, FARGO shift as an exact rotation.
- * Branches that no golden run exercises (OpenInner and the other options the
- * library refuses) are "parity unpinned".  Stellar irradiation, the SM03
- * opacity, gas accretion and accretion heating are pinned by the golden runs
- * irr_lp85_48x96, damp_sm03_48x96, gasacc_q1e-3_64x128 and
- * pebheat_q3e-5_64x128.
+ * Parity pins:
+ *   - bitwise against the reference itself, built from its sources by
+ *     oracle/ref.mk and run on seeded disks (fixtures under tests/golden/,
+ *     made by tests/golden/make_golden.py): every output field, the SOR
+ *     iteration counts, PhysicalTime of every DT, the crash marks and the
+ *     first crash's dump (tests/test_oracle_golden.py).  The golden runs
+ *     cover WALL and damped boundaries (ZEROVRAD, INIT, G), the LP85, BL94,
+ *     ZHU12 and SM03 opacities, stellar irradiation, Flock alpha with the
+ *     viscosity cavity, ImposedDiskDrift, gas accretion, the two-fluid disk
+ *     with pebble accretion, back-reaction, diffusion and accretion heating;
+ *   - closed forms, for what the golden runs leave open: the opacity laws'
+ *     published regime exponents and coefficients and their continuity at the
+ *     regime borders, POWERLAW's scaling and its equality with LP85's ice
+ *     branch (bitwise, and as a whole run against the reference's LP85 run on
+ *     a disk colder than the ice line), the flux limiter's limits
+ *     (tests/test_oracle_math.py).
+ * The options the library refuses (OpenInner and the others that
+ * thorin_ctx_create rejects) are not restated here.
  */
 #define _GNU_SOURCE
 #include "thorin_oracle.h"

@@ -132,6 +132,17 @@ CASES = {
     "crash_cold_48x96": (configs.with_grid(configs.PLAIN_128x256, 48, 96, Ntot=3, Ninterm=3, Rmin="1.2",
                                            AspectRatio="0.025", WriteEnergy="YES", _temper_cell="20,40,-0.01"),
                          configs.PLANET_ZERO_MASS, True),
+    # the standard (non-FARGO) azimuthal transport, a constant kinematic
+    # viscosity instead of alpha, and no indirect term
+    "std_nu_noind_48x96": (configs.with_grid(configs.PLAIN_128x256, 48, 96, Ntot=8, Ninterm=4, Transport="STANDARD",
+                                             AlphaViscosity="0.0", Viscosity="1e-5", IndirectTerm="NO",
+                                             WriteEnergy="YES", WriteDivV="YES", WriteQplus="YES",
+                                             WriteOpacity="YES"), configs.PLANET_ZERO_MASS, True),
+    # a q = 1e-4 planet that feels the disk with its Hill sphere excluded
+    # from the disk force (ExcludeHill, HillCut 0.8)
+    "hill_q1e-4_48x96": (configs.with_grid(configs.PLAIN_128x256, 48, 96, Ntot=8, Ninterm=4, ExcludeHill="YES",
+                                           PlanetsFeelDisk="YES", WriteEnergy="YES", WriteDivV="YES",
+                                           WriteQplus="YES", WriteOpacity="YES"), [(1e-4, 1.0)], True),
     # an imposed supplementary torque that drives the disk inwards (ImposedDiskDrift)
     "drift_48x96": (configs.with_grid(configs.DAMPED_1024x1536, 48, 96, Ntot=8, Ninterm=4,
                                       ImposedDiskDrift="-1e-4", WriteEnergy="YES", WriteDivV="YES",

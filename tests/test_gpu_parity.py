@@ -23,7 +23,7 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-10
 FULL_CASES = ["wall_lp85_64x128", "damp_bl94_48x96", "dampinit_zhu12_40x64", "planet_q3e-5_48x96", "irr_lp85_48x96",
               "damp_sm03_48x96", "gasacc_q1e-3_64x128", "flock_cavity_48x96", "dampflux_48x96", "drift_48x96",
-              "cold_lp85_48x96"]
+              "cold_lp85_48x96", "std_nu_noind_48x96", "hill_q1e-4_48x96"]
 # fields whose values are differences of nearly equal terms: compared
 # relative to the field's own scale
 STATE_FIELDS = ["dens", "vrad", "vtheta", "temper", "energy", "divv", "qplus", "opacity"]

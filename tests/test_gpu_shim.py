@@ -125,7 +125,8 @@ def _check_orbits(g, outdir):
 
 
 @needs_bin
-@pytest.mark.parametrize("name", ["wall_lp85_64x128", "damp_bl94_48x96", "flock_cavity_48x96"])
+@pytest.mark.parametrize("name", ["wall_lp85_64x128", "damp_bl94_48x96", "flock_cavity_48x96", "std_nu_noind_48x96",
+                                  "hill_q1e-4_48x96"])
 def test_shim_runs_the_reference_driver(name, run_dir):
     cfg, planets, _ = CASES[name]
     g = golden(name)

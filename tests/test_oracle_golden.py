@@ -17,7 +17,7 @@ from tests.helpers import FIELDS, Replay, case, golden, params_for, planets_for
 
 FULL_CASES = ["wall_lp85_64x128", "damp_bl94_48x96", "dampinit_zhu12_40x64", "planet_q3e-5_48x96", "irr_lp85_48x96",
               "damp_sm03_48x96", "gasacc_q1e-3_64x128", "flock_cavity_48x96", "dampflux_48x96", "drift_48x96",
-              "cold_lp85_48x96"]
+              "cold_lp85_48x96", "std_nu_noind_48x96", "hill_q1e-4_48x96"]
 
 
 def run_oracle_case(name):
