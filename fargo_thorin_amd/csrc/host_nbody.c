@@ -39,6 +39,66 @@ static void elements_to_state(double mu, double a, double e, double inc, double 
   *vz = v0 * ((e + cf) * co * si - sf * si * so);
 }
 
+/* the angle in [0, pi] whose cosine is num/den, negated when side < 0 (an
+ * angle measured from the x axis takes the sign of its y component); 0 or
+ * pi where rounding puts the cosine outside [-1, 1], and 0 for den = 0 */
+static double angle_of(double num, double den, double side) {
+  const double c = num / den;
+  if (!(c > -1.0 && c < 1.0)) return c <= -1.0 ? TH_PI : 0.0;
+  return side < 0.0 ? -acos(c) : acos(c);
+}
+
+/* heliocentric state -> osculating elements about a primary of mass mp (G =
+ * 1), in the conventions of the elements OutputElements writes
+ * (ReboundInterface.c:405-441, through the vendored REBOUND's orbit routine):
+ * a from the vis-viva energy, e from the Laplace-Runge-Lenz vector, the
+ * angles from the x axis.  Within 1e-8 rad of the reference plane the node
+ * is undefined: Omega = 0 and omega, f come from the longitudes of the
+ * pericentre and of the planet. */
+int thorin_orbit_elements(const thorin_planets *sys, int k, double mp, double el[6]) {
+  if (!sys || !el || k < 0 || k >= sys->nb || !(mp > 0.0)) {
+    thorin_set_error("thorin_orbit_elements: bad planet index or primary mass");
+    return -1;
+  }
+  const double mu = sys->mass[k] + mp;
+  const double x = sys->x[k], y = sys->y[k], z = sys->z[k];
+  const double vx = sys->vx[k], vy = sys->vy[k], vz = sys->vz[k];
+  const double r = sqrt(x * x + y * y + z * z);
+  if (!(r > 0.0)) {
+    thorin_set_error("thorin_orbit_elements: planet at the primary");
+    return -1;
+  }
+  const double v2 = vx * vx + vy * vy + vz * vz;
+  const double a = -mu / (v2 - 2.0 * mu / r);
+  const double hx = y * vz - z * vy, hy = z * vx - x * vz, hz = x * vy - y * vx;
+  const double h = sqrt(hx * hx + hy * hy + hz * hz);
+  const double rdot = (x * vx + y * vy + z * vz) / r;
+  /* e vector = ((v^2 - mu/r) r_vec - (r rdot) v_vec) / mu */
+  const double w = v2 - mu / r, rr = r * rdot;
+  const double ex = (w * x - rr * vx) / mu, ey = (w * y - rr * vy) / mu, ez = (w * z - rr * vz) / mu;
+  const double e = sqrt(ex * ex + ey * ey + ez * ez);
+  const double inc = angle_of(hz, h, 1.0);
+  const double nx = -hy, ny = hx, nn = sqrt(nx * nx + ny * ny);   /* z x h: the ascending node */
+  const double Om = angle_of(nx, nn, ny);
+  double om, f;
+  if (inc < 1e-8 || inc > TH_PI - 1e-8) {
+    const double varpi = angle_of(ex, e, ey), lam = angle_of(x, r, y);
+    if (inc < TH_PI / 2.0) {
+      om = varpi - Om;
+      f = lam - varpi;
+    } else {
+      om = Om - varpi;
+      f = varpi - lam;
+    }
+  } else {
+    const double u = angle_of(nx * x + ny * y, nn * r, z);        /* argument of latitude */
+    om = angle_of(nx * ex + ny * ey, nn * e, ez);
+    f = u - om;
+  }
+  el[0] = a; el[1] = e; el[2] = inc; el[3] = Om; el[4] = om; el[5] = f;
+  return 0;
+}
+
 int thorin_planets_from_config(const thorin_params *p, thorin_planets *sys) {
   FILE *in = fopen(p->planetconfig, "r");
   char s[512], nm[512];

@@ -11,9 +11,9 @@
  *
  * Output files and formats follow the reference: raw native-endian float64
  * gas<name><n>.dat (no header, ring-major), dims.dat, used_rad.dat,
- * omegaframe.dat and tqwk<k>.dat.  The N-body state for restarts is written
- * as nbody.state<n>.dat (text) because REBOUND's binary snapshot is not part
- * of this build.
+ * omegaframe.dat, tqwk<k>.dat and nbody.orbits.dat.  The N-body state for
+ * restarts is written as nbody.state<n>.dat (text) because REBOUND's binary
+ * snapshot is not part of this build.
  */
 #define _GNU_SOURCE
 #include "../../include/thorin_amd.h"
@@ -156,6 +156,42 @@ static void write_nbody_state(const thorin_params *p, const thorin_planets *s, d
   fclose(f);
 }
 
+/* nbody.orbits.dat (ReboundInterface.c:197-200, 244-247, 405-441): per
+ * planet its index, the time, a, e, inc, Omega, omega, f about the primary
+ * (of mass 1, plus the initial disk's mass inside the planet's distance from
+ * the star's axis with ElementsWithDisk), mass, gas and solid mass times
+ * MassTaper, the planet's radius at PlanetaryDensity, x, y, z, vx, vy, vz.
+ * sigma_med is NULL for the rows written before the disk exists. */
+#define ORB_RHO2CGS (1.98855e30 * 1000.0 / pow(149597870700.0 * 100.0, 3.0))   /* fondam.h:27,30,38 */
+static void write_orbits(FILE *f, const thorin_params *p, const thorin_geometry *g, const double *sigma_med,
+                         const thorin_planets *s, double t, double taper) {
+  const double rdenominv = 1.0 / (4.0 * M_PI * (p->planetarydensity / ORB_RHO2CGS));
+  int k, i;
+  if (!f) return;
+  for (k = 0; k < s->nb; k++) {
+    double el[6], mp = 1.0;
+    if (p->elements_with_disk && sigma_med) {
+      /* GetInteriorDiskMass (ReboundInterface.c:387-402): whole rings below
+       * the cylindrical radius, the ring that holds it in proportion */
+      const double rp = sqrt(s->x[k] * s->x[k] + s->y[k] * s->y[k]);
+      double md = 0.0;
+      for (i = 0; i < p->nrad && rp >= g->rsup[i]; i++) md += p->nsec * g->surf[i] * sigma_med[i];
+      if (i < p->nrad && rp > g->rinf[i] && rp < g->rsup[i])
+        md += (rp - g->rinf[i]) / (g->rsup[i] - g->rinf[i]) * p->nsec * g->surf[i] * sigma_med[i];
+      mp += md;
+    }
+    if (thorin_orbit_elements(s, k, mp, el)) {
+      for (i = 0; i < 6; i++) el[i] = NAN;
+    }
+    fprintf(f, "%d\t%.12g\t%.12g\t%.12g\t%.12g\t%.12g\t%.12g\t%.12g\t%.12g\t%.12g\t%.12g\t%.12g\t%.12g\t%.12g\t%.12g\t%.12g"
+               "\t%.12g\t%.12g\n",
+            k + 1, t, el[0], el[1], el[2], el[3], el[4], el[5], s->mass[k] * taper, s->gas[k] * taper,
+            s->solid[k] * taper, pow(3.0 * s->mass[k] * rdenominv, 1.0 / 3.0), s->x[k], s->y[k], s->z[k], s->vx[k],
+            s->vy[k], s->vz[k]);
+  }
+  fflush(f);
+}
+
 static int read_nbody_state(const thorin_params *p, thorin_planets *s, double *t, int n) {
   char fn[2 * THORIN_STRLEN];
   FILE *f;
@@ -179,6 +215,9 @@ int main(int argc, char **argv) {
   thorin_step_log *lg;
   const char *parfile = NULL, *outdir = NULL;
   int i, k, restart = 0, nrestart = 0, only_init = 0, verbose = 0, begin_i = 0, timing = 0, ntimed = 0, ncalls = 0;
+  FILE *orbits;
+  thorin_geometry geo;
+  double t_orbits = 0.0;
   double t_calls = 0.0;
   size_t cnt;
   double *buf, *buf2;
@@ -233,6 +272,7 @@ int main(int argc, char **argv) {
       st.physical_time = st.physical_time_initial = tt;
       thorin_set_state(c, &st);
     }
+    t_orbits = tt;
     begin_i = nrestart * p.ninterm;
     free(d); free(vr); free(vt); free(t);
   } else if (p.init_from_file) {
@@ -279,6 +319,12 @@ int main(int argc, char **argv) {
     f = fopen(fn, "w");
     if (f) { fprintf(f, "%d\t%d\t\t%d\t%d\t%f\t%d\t%d\t%d\n", 0, 0, 0, 0, p.rmax, p.ntot / p.ninterm, p.nrad, p.nsec); fclose(f); }
   }
+  /* the row of the set-up, MassTaper taken as 1 (ReboundInterface.c:199, 246) */
+  snprintf(fn, sizeof fn, "%snbody.orbits.dat", p.outputdir);
+  orbits = fopen(fn, restart ? "a" : "w");
+  if (!orbits) fprintf(stderr, "WARNING ! Can't write %s\n", fn);
+  thorin_get_geometry(c, &geo);
+  write_orbits(orbits, &p, &geo, NULL, &sys, t_orbits, 1.0);
   lg = (thorin_step_log *)calloc(1, sizeof(thorin_step_log));
   for (i = begin_i; i <= p.ntot; i++) {
     int ts = i / p.ninterm;
@@ -304,6 +350,10 @@ int main(int argc, char **argv) {
     for (k = 0; k < lg->nsub; k++) printf("%c", lg->crashed[k] ? 'c' : '.');
     printf("\n");
     fflush(stdout);
+    if (p.noutelements > 0 && (i + 1) % p.noutelements == 0) {   /* main.c:202-207 */
+      thorin_get_state(c, &st);
+      write_orbits(orbits, &p, &geo, geo.sigma_med, &sys, st.physical_time, st.mass_taper);
+    }
     if (p.write_torque)
       for (k = 0; k < lg->tq_rows; k++) {
         snprintf(fn, sizeof fn, "%stqwk%d.dat", p.outputdir, k);
@@ -318,6 +368,7 @@ int main(int argc, char **argv) {
   if (timing && ntimed > 0)
     printf("timing: %d AlgoGas calls after 3: %.4f ms per call (%.1f calls/s)\n", ntimed, 1e3 * t_calls / ntimed,
            ntimed / t_calls);
+  if (orbits) fclose(orbits);
   thorin_ctx_destroy(c);
   free(buf);
   free(buf2);

@@ -415,6 +415,11 @@ const char *thorin_last_error(void);
 int  thorin_planets_from_config(const thorin_params *p, thorin_planets *sys);
 int  thorin_nbody_step(void *user, thorin_planets *sys, double t, double dt,
                        double rotation_angle);
+/* Osculating elements of planet k about a primary of mass primary_mass (1,
+ * or 1 plus the disk mass inside the planet's orbit with ElementsWithDisk):
+ * el = a, e, inc, Omega, omega, f (radians), as OutputElements writes them
+ * to nbody.orbits.dat (ReboundInterface.c:405-441). */
+int  thorin_orbit_elements(const thorin_planets *sys, int k, double primary_mass, double el[6]);
 
 #ifdef __cplusplus
 }

@@ -54,6 +54,10 @@ def test_gpu_driver_two_fluid_run_and_restart(tmp_path):
     assert np.array_equal(used, g["used_rad"])
     tq = np.loadtxt(outdir / "tqwk0.dat", ndmin=2)
     assert np.array_equal(tq[:, 0], g["tqwk0"][:len(tq), 0])
+    # the planet's position at the start of every DT: the library's N-body
+    # step (with the disk's kicks and the pebble accretion) against REBOUND's
+    err = np.max(np.abs(tq[:, 3:5] - g["tqwk0"][:len(tq), 3:5]))
+    assert err <= 1e-10, f"planet trajectory: {err:.3e}"
     # restart from output 1 of this run; the reference's own restart (-s 1,
     # damping targets = ring means of the restart files) is the comparison
     print("[gpu-driver] restart -s 1", flush=True)
@@ -64,3 +68,16 @@ def test_gpu_driver_two_fluid_run_and_restart(tmp_path):
     for f in ("dens", "vrad", "vtheta", "temper", "pebbledens", "pebblevrad", "pebblevtheta"):
         err = rel_linf(_read(outdir, f, 2, (nr, ns)), g[f"restart_{f}2"])
         assert err <= 1e-10, f"restart {f}2: {err:.3e}"
+    # nbody.orbits.dat of both runs (the set-up row, one row per DT, the
+    # restart's row) against the reference's (ReboundInterface.c:405-441)
+    orb = np.loadtxt(outdir / "nbody.orbits.dat", ndmin=2)
+    ref = g["nbody_orbits"]
+    assert orb.shape == ref.shape, f"{orb.shape} vs {ref.shape}"
+    assert np.array_equal(orb[:, :2], ref[:, :2]), "planet index and time"
+    for c in (2, 8, 9, 10, 11):           # a, mass, gas, solid, radius
+        scale = max(np.max(np.abs(ref[:, c])), 1e-300)
+        assert np.max(np.abs(orb[:, c] - ref[:, c])) <= 1e-10 * scale, f"column {c}"
+    for c in (3, 4, 5) + tuple(range(12, 18)):   # e, inc, Omega, x .. vz
+        assert np.max(np.abs(orb[:, c] - ref[:, c])) <= 1e-10, f"column {c}"
+    lon = orb[:, 5] + orb[:, 6] + orb[:, 7] - ref[:, 5] - ref[:, 6] - ref[:, 7]
+    assert np.max(np.abs((lon + np.pi) % (2 * np.pi) - np.pi)) <= 1e-9, "longitude"
