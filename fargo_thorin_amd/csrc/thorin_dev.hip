@@ -2770,39 +2770,44 @@ __global__ void __launch_bounds__(TB) k_energy_new(Geo g, const double *__restri
 // Radial transport: ComputeLRMomenta (TransportEuler.c:318-347) + OneWindRad
 // (49-65) with ComputeStarRad (232-273), VanLeerRadial (384-415) and
 // DivisePolarGrid (SideEuler.c:82-99) for the six conserved quantities.
-// A workgroup handles one quantity on a tile of TB sectors x RI rings (8, 16
-// or 32; a cell's result does not depend on RI: every tile restarts its window
-// four rings below the tile); each
-// thread walks its column outward with a rolling window: at ring i it loads
-// ring i+2, forms W(i+2) = Q/(RhoInt+1e-20), the slopes dq(i+1) of W and of
-// rho, the star values at interface i+1 and the flux
-// F(i+1) = dt*dtheta*Radii[i+1]*qs*RhoStar*vr, and writes
-// Q(i) + (F(i) - F(i+1))*InvSurf[i].  The reference evaluates that product
-// for the lower interface of ring i with Rinf[i] and for the upper one with
-// Rsup[i] = Radii[i+1]: both are F.  Momenta are rebuilt from rho, vr,
-// vtheta exactly as ComputeLRMomenta forms them.  The window starts four
-// rings below the tile (rings outside the grid read as 0, as the reference's
-// dq = 0 and qs = 0 there make them).  The six workgroups of one tile are
-// numbered so that they land on the same XCD (blocks go round-robin over the
-// 8 XCDs), sharing its L2 for the common rho and vr.
+// A workgroup handles a tile of RT_J sectors x RI rings (8, 16 or 32) for
+// all quantities: one wave per quantity.  First the whole workgroup stages
+// rho (rings i0-2 .. i0+RI+1) and vr (i0-3 .. i0+RI+2) of the tile in LDS and
+// forms from them the density's slopes drho (rings i0-1 .. i0+RI) and star
+// values RhoStar (interfaces i0 .. i0+RI), which every quantity's flux
+// uses; then each wave walks its columns outward: at ring i it forms
+// W(i+2) = Q/(RhoInt+1e-20), the slope dq(i+1), the star value qs at
+// interface i+1 and the flux F(i+1) = dt*dtheta*Radii[i+1]*qs*RhoStar*vr,
+// and writes Q(i) + (F(i) - F(i+1))*InvSurf[i].  The reference evaluates that
+// product for the lower interface of ring i with Rinf[i] and for the upper
+// one with Rsup[i] = Radii[i+1]: both are F.  Momenta are rebuilt from rho,
+// vr, vtheta exactly as ComputeLRMomenta forms them.  Rings outside the grid
+// read as 0 (as the reference's dq = 0 and qs = 0 there make them); a cell's
+// result does not depend on RI.  Tiles are numbered so that XCD x (= block %
+// 8) gets a contiguous band of them.
 #define RC_NQ 6
+#define RT_J 64                  // sectors per tile: one wave
+#define RT_T (RT_J * RC_NQ)      // threads per workgroup: one wave per quantity
 
 struct RadQ {
   const double *rho, *vr, *vt, *e;
   double *out[RC_NQ];          // RadMomP, RadMomM, ThetaMomP, ThetaMomM, Energy, Rho
   const int *guard;             // skip the launch while *guard == 0
   double of;
-  int ntile_j, ntiles;
+  int ntile_j, ntiles;          // tiles of RT_J sectors x rows rings
   int nq;                       // 6; 5 for the pebble fluid (no energy: quantity 4 skipped)
   int rows;                     // rings per tile (8, 16 or 32)
 };
 
-// ring geometry of one radial tile, staged in LDS: entry r is ring i0-4+r
-// (0 outside 0..NRAD); every value the rolling window needs is ring-uniform
+// ring geometry of one radial tile and the shared density values, in LDS.
+// Geometry entry r is ring i0-4+r (0 outside 0..NRAD); rho entry a is ring
+// i0-2+a, vr entry a ring i0-3+a, drho entry a ring i0-1+a, rs entry a
+// interface i0+a (values outside the grid are 0).
 template <int RI>
 struct RadGeo {
   static constexpr int G = RI + 6;
   double rmed[G], idr[G], rinf[G], isurf[G];
+  double rho[RI + 4][RT_J], vr[RI + 6][RT_J], drho[RI + 2][RT_J], rs[RI + 1][RT_J];
 };
 
 // slope of ComputeStarRad at ring k (dq = 0 on the first and last ring)
@@ -2822,74 +2827,61 @@ __device__ __forceinline__ double rad_star_s(int k, int nr, double rm_km, double
   return w0 - (rm_kp - rm_k + vrk * dt) * 0.5 * dq0;
 }
 
-// one column of quantity W (0 RadMomP, 1 RadMomM, 2 ThetaMomP, 3 ThetaMomM,
-// 4 Energy, 5 Rho) over rings i0 .. i1-1.  The loads of ring i+2 are issued
-// two iterations ahead (clamped into the grid, replaced by 0 outside it).
-template <int W, typename SG>
-__device__ __forceinline__ void rad_column(const Geo &g, const RadQ &q, const SG &sg, double dt, int j, int i0,
-                                           int i1) {
+// one column (tile column c, sector j) of quantity W (0 RadMomP, 1 RadMomM,
+// 2 ThetaMomP, 3 ThetaMomM, 4 Energy, 5 Rho) over rings i0 .. i1-1
+// (i1 - i0 <= RI).  The walk has the fixed length RI + 4 and is unrolled:
+// the loads of later rings go out ahead of the arithmetic and the rolling
+// values stay in named registers; rings past i1 (a partial last tile) are
+// walked on clamped data and store nothing.
+template <int W, int RI>
+__device__ __forceinline__ void rad_column(const Geo &g, const RadQ &q, const RadGeo<RI> &sg, double dt, int c,
+                                           int j, int i0, int i1) {
   const int ns = g.ns, nr = g.nr;
   const int jp = (j == ns - 1) ? 0 : j + 1;
-  const int gb = i0 - 4;                        // ring of sg entry 0
+  const int gb = i0 - 4;                        // ring of geometry entry 0
   const double dtheta = 2.0 * DPI / (double)ns;
+  const double *__restrict__ pvt = q.vt;
+  const double *__restrict__ pe = q.e;
   double *__restrict__ out = q.out[W];
-  // raw loads of the ring k = i+2 and of vr at k+1
-  auto load = [&](int k, double &rho_l, double &vr_l, double &x_l) {
+  // rolling state for output ring i: W at i, i+1; Q at i, i+1; dq at i;
+  // flux at i
+  double w0 = 0.0, w1 = 0.0, Q0 = 0.0, Q1 = 0.0, dq0 = 0.0, F0 = 0.0;
+#pragma unroll
+  for (int t = 0; t < RI + 4; t++) {
+    const int i = gb + t;
+    const int k = i + 2;                       // ring entering the window
     const int kc = min(max(k, 0), nr - 1);
-    const int kv = min(max(k + 1, 0), nr);     // ring nr: ghost ring of vr
-    const size_t l = (size_t)kc * ns + j;
-    rho_l = q.rho[l];
-    vr_l = q.vr[(size_t)kv * ns + j];
-    x_l = 0.0;
-    if (W == 2) x_l = q.vt[(size_t)kc * ns + jp];
-    if (W == 3) x_l = q.vt[l];
-    if (W == 4) x_l = q.e[l];
-  };
-  // rolling state for output ring i: rho and W at i, i+1; Q at i, i+1;
-  // slopes at i; flux at i; vr at i+1, i+2
-  double rho0 = 0.0, rho1 = 0.0, w0 = 0.0, w1 = 0.0, Q0 = 0.0, Q1 = 0.0;
-  double drho0 = 0.0, dq0 = 0.0, F0 = 0.0;
-  double vr1 = (i0 - 3 >= 0) ? q.vr[(size_t)(i0 - 3) * ns + j] : 0.0;
-  double vr2 = (i0 - 2 >= 0) ? q.vr[(size_t)(i0 - 2) * ns + j] : 0.0;
-  double pa_rho, pa_vr, pa_x, pb_rho, pb_vr, pb_x;    // rings i+2 and i+3
-  load(i0 - 2, pa_rho, pa_vr, pa_x);
-  load(i0 - 1, pb_rho, pb_vr, pb_x);
-  for (int i = i0 - 4; i < i1; i++) {
-    const int k = i + 2;                       // ring used now
-    const double rho_l = pa_rho, vr_l = pa_vr, x_l = pa_x;
-    pa_rho = pb_rho; pa_vr = pb_vr; pa_x = pb_x;
-    if (i + 2 < i1) load(k + 2, pb_rho, pb_vr, pb_x);
-    const bool in = (k >= 0 && k < nr);
-    const bool inv = (k + 1 >= 0 && k + 1 <= nr);
-    const double vr3 = inv ? vr_l : 0.0;
-    const double rho2 = in ? rho_l : 0.0;
-    const int r = i + 1 - gb;                  // sg entry of ring i+1
+    const double rho2 = sg.rho[t][c];          // 0 outside the grid
+    double x_l = 0.0;
+    if (W == 2) x_l = pvt[kc * ns + jp];
+    if (W == 3) x_l = pvt[kc * ns + j];
+    if (W == 4) x_l = pe[kc * ns + j];
+    const int r = t + 1;                       // geometry entry of ring i+1
     double Q2 = 0.0;
-    if (in) {
-      if (W == 0) Q2 = rho2 * vr3;
-      if (W == 1) Q2 = rho2 * vr2;
+    if (k >= 0 && k < nr) {
+      if (W == 0) Q2 = rho2 * sg.vr[t + 2][c];
+      if (W == 1) Q2 = rho2 * sg.vr[t + 1][c];
       if (W == 2 || W == 3) Q2 = rho2 * (x_l + sg.rmed[r + 1] * q.of) * sg.rmed[r + 1];
       if (W == 4) Q2 = x_l;
       if (W == 5) Q2 = rho2;
     }
     const double w2 = Q2 / (rho2 + 1e-20);
-    const double idr_k = sg.idr[r], idr_k1 = sg.idr[r + 1];
-    const double rm_km = sg.rmed[r - 1], rm_k = sg.rmed[r], rm_kp = sg.rmed[r + 1];
-    const double drho1 = rad_dq_s(i + 1, nr, idr_k, idr_k1, rho0, rho1, rho2);
-    const double dq1 = rad_dq_s(i + 1, nr, idr_k, idr_k1, w0, w1, w2);
-    const double rs = rad_star_s(i + 1, nr, rm_km, rm_k, rm_kp, rho0, rho1, drho0, drho1, vr1, dt);
-    const double qs = rad_star_s(i + 1, nr, rm_km, rm_k, rm_kp, w0, w1, dq0, dq1, vr1, dt);
-    const double F1 = (i + 1 >= 1 && i + 1 <= nr - 1) ? dt * dtheta * sg.rinf[r] * qs * rs * vr1 : 0.0;
-    if (i >= i0) {
+    // dq at ring i+1 is used from t = 2 on, the flux at interface i+1 from t = 3
+    double dq1 = 0.0, F1 = 0.0;
+    if (t >= 2) dq1 = rad_dq_s(i + 1, nr, sg.idr[r], sg.idr[r + 1], w0, w1, w2);
+    if (t >= 3) {
+      const double vr1 = sg.vr[t][c];
+      const double qs = rad_star_s(i + 1, nr, sg.rmed[r - 1], sg.rmed[r], sg.rmed[r + 1], w0, w1, dq0, dq1, vr1, dt);
+      F1 = (i + 1 >= 1 && i + 1 <= nr - 1) ? dt * dtheta * sg.rinf[r] * qs * sg.rs[t - 3][c] * vr1 : 0.0;
+    }
+    if (t >= 4 && i < i1) {
       double varq = F0;
       varq -= F1;
-      out[(size_t)i * ns + j] = Q0 + varq * sg.isurf[r - 1];
+      out[i * ns + j] = Q0 + varq * sg.isurf[r - 1];
     }
-    rho0 = rho1; rho1 = rho2;
     w0 = w1; w1 = w2;
     Q0 = Q1; Q1 = Q2;
-    drho0 = drho1; dq0 = dq1; F0 = F1;
-    vr1 = vr2; vr2 = vr3;
+    dq0 = dq1; F0 = F1;
   }
 }
 
@@ -2897,38 +2889,74 @@ template <int RI>
 __device__ __forceinline__ void b_transport_rad(const Geo &g, RadQ q, double dt) {
   __shared__ RadGeo<RI> sg;
   if (q.guard && *q.guard == 0) return;
-  // XCD x (= b % 8) gets the contiguous band of tiles x*per .. x*per+per-1,
-  // all six quantities of each
-  const int b = (int)blockIdx.x - g.blk0, slot = b & 7, rest = b >> 3;
-  int which = rest % q.nq;
-  if (q.nq == RC_NQ - 1 && which == 4) which = 5;
+  const int b = (int)blockIdx.x - g.blk0, slot = b & 7;
   const int per = (q.ntiles + 7) >> 3;
-  const int tile = slot * per + rest / q.nq;
+  const int tile = slot * per + (b >> 3);
   if (tile >= q.ntiles) return;                // whole workgroup
   const int tj = tile % q.ntile_j, ti = tile / q.ntile_j;
   const int i0 = ti * RI, i1 = min(i0 + RI, g.nr);
-  for (int t = threadIdx.x; t < RadGeo<RI>::G; t += TB) {
+  const int nr = g.nr, ns = g.ns, j0 = tj * RT_J;
+  for (int t = threadIdx.x; t < RadGeo<RI>::G; t += RT_T) {
     const int k = i0 - 4 + t;
-    const bool ok = (k >= 0 && k <= g.nr);
+    const bool ok = (k >= 0 && k <= nr);
     sg.rmed[t] = ok ? g.Rmed[k] : 0.0;
     sg.idr[t] = ok ? g.InvDiffRmed[k] : 0.0;
     sg.rinf[t] = ok ? g.Rinf[k] : 0.0;
     sg.isurf[t] = ok ? g.InvSurf[k] : 0.0;
   }
+  // rho of rings 0..NRAD-1 and vr of rings 0..NRAD (the ghost ring); 0
+  // elsewhere and on sectors past NSEC.  Every thread issues all its loads
+  // (at clamped, in-bounds addresses) before the first LDS write.
+  {
+    constexpr int NE = ((RI + 6) * RT_J + RT_T - 1) / RT_T;
+    double rv[NE], vv[NE];
+#pragma unroll
+    for (int u = 0; u < NE; u++) {
+      const int e = threadIdx.x + u * RT_T;
+      const int a = e / RT_J, j = j0 + e % RT_J;
+      const int jc = min(j, ns - 1);
+      const int kr = min(max(i0 - 2 + a, 0), nr - 1), kv = min(max(i0 - 3 + a, 0), nr);
+      rv[u] = q.rho[kr * ns + jc];
+      vv[u] = q.vr[kv * ns + jc];
+    }
+#pragma unroll
+    for (int u = 0; u < NE; u++) {
+      const int e = threadIdx.x + u * RT_T;
+      const int a = e / RT_J, c = e % RT_J, j = j0 + c;
+      const int kr = i0 - 2 + a, kv = i0 - 3 + a;
+      if (a < RI + 4) sg.rho[a][c] = (j < ns && kr >= 0 && kr < nr) ? rv[u] : 0.0;
+      if (a < RI + 6) sg.vr[a][c] = (j < ns && kv >= 0 && kv <= nr) ? vv[u] : 0.0;
+    }
+  }
   __syncthreads();
-  const int j = tj * TB + threadIdx.x;
-  if (j >= g.ns) return;
+  // the density's slopes (ComputeStarRad on rho) ...
+  for (int e = threadIdx.x; e < (RI + 2) * RT_J; e += RT_T) {
+    const int a = e / RT_J, c = e % RT_J, m = i0 - 1 + a, r = a + 3;   // geometry entry of ring m
+    sg.drho[a][c] = rad_dq_s(m, nr, sg.idr[r], sg.idr[r + 1], sg.rho[a][c], sg.rho[a + 1][c], sg.rho[a + 2][c]);
+  }
+  __syncthreads();
+  // ... and its star values RhoStar at interfaces i0 .. i0+RI
+  for (int e = threadIdx.x; e < (RI + 1) * RT_J; e += RT_T) {
+    const int a = e / RT_J, c = e % RT_J, m = i0 + a, r = a + 4;
+    sg.rs[a][c] = rad_star_s(m, nr, sg.rmed[r - 1], sg.rmed[r], sg.rmed[r + 1], sg.rho[a + 1][c], sg.rho[a + 2][c],
+                             sg.drho[a][c], sg.drho[a + 1][c], sg.vr[a + 3][c], dt);
+  }
+  __syncthreads();
+  const int c = threadIdx.x % RT_J, j = j0 + c;
+  const int which = threadIdx.x / RT_J;
+  if (q.nq == RC_NQ - 1 && which == 4) return;   // the pebble fluid has no energy
+  if (j >= ns) return;
   switch (which) {
-    case 0: rad_column<0>(g, q, sg, dt, j, i0, i1); break;
-    case 1: rad_column<1>(g, q, sg, dt, j, i0, i1); break;
-    case 2: rad_column<2>(g, q, sg, dt, j, i0, i1); break;
-    case 3: rad_column<3>(g, q, sg, dt, j, i0, i1); break;
-    case 4: rad_column<4>(g, q, sg, dt, j, i0, i1); break;
-    default: rad_column<5>(g, q, sg, dt, j, i0, i1); break;
+    case 0: rad_column<0, RI>(g, q, sg, dt, c, j, i0, i1); break;
+    case 1: rad_column<1, RI>(g, q, sg, dt, c, j, i0, i1); break;
+    case 2: rad_column<2, RI>(g, q, sg, dt, c, j, i0, i1); break;
+    case 3: rad_column<3, RI>(g, q, sg, dt, c, j, i0, i1); break;
+    case 4: rad_column<4, RI>(g, q, sg, dt, c, j, i0, i1); break;
+    default: rad_column<5, RI>(g, q, sg, dt, c, j, i0, i1); break;
   }
 }
 template <int RI>
-__global__ void __launch_bounds__(TB) k_transport_rad(Geo g, RadQ q, double dt) {
+__global__ void __launch_bounds__(RT_T) k_transport_rad(Geo g, RadQ q, double dt) {
   b_transport_rad<RI>(g, q, dt);
 }
 
@@ -3839,9 +3867,9 @@ BATCH_BODY(k_sor_check<1>, b_sor_check<1>, RB)
 BATCH_BODY(k_sor_check<2>, b_sor_check<2>, RB)
 BATCH_BODY(k_sor_check<3>, b_sor_check<3>, RB)
 BATCH_BODY(k_energy_new, b_energy_new, TB)
-BATCH_BODY(k_transport_rad<8>, b_transport_rad<8>, TB)
-BATCH_BODY(k_transport_rad<16>, b_transport_rad<16>, TB)
-BATCH_BODY(k_transport_rad<32>, b_transport_rad<32>, TB)
+BATCH_BODY(k_transport_rad<8>, b_transport_rad<8>, RT_T)
+BATCH_BODY(k_transport_rad<16>, b_transport_rad<16>, RT_T)
+BATCH_BODY(k_transport_rad<32>, b_transport_rad<32>, RT_T)
 BATCH_BODY(k_ring_means, b_ring_means, 256)
 BATCH_BODY(k_transport_theta<1>, b_transport_theta<1>, TT_T)
 BATCH_BODY(k_transport_theta<2>, b_transport_theta<2>, TT_T)
@@ -4354,10 +4382,10 @@ static int transport(thorin_dev *d, const Geo &g, const RadQ &q, int brho, int b
                      const int *guard, int with_vel = 1) {
   const int nr = d->nr, ns = d->ns;
   const dim3 gall = cell_grid(d, nr), tb(TB);
-  const dim3 grad((unsigned)((q.ntiles + 7) / 8 * 8 * q.nq));
-  if (q.rows == 8) LAUNCH(k_transport_rad<8>, grad, tb, 0, g, q, dt);
-  else if (q.rows == 32) LAUNCH(k_transport_rad<32>, grad, tb, 0, g, q, dt);
-  else LAUNCH(k_transport_rad<16>, grad, tb, 0, g, q, dt);
+  const dim3 grad((unsigned)((q.ntiles + 7) / 8 * 8)), trad(RT_T);
+  if (q.rows == 8) LAUNCH(k_transport_rad<8>, grad, trad, 0, g, q, dt);
+  else if (q.rows == 32) LAUNCH(k_transport_rad<32>, grad, trad, 0, g, q, dt);
+  else LAUNCH(k_transport_rad<16>, grad, trad, 0, g, q, dt);
   double *xe = be >= 0 ? d->b[B_XE] : nullptr, *oe = be >= 0 ? d->b[be] : nullptr;
   const int nq = be >= 0 ? 5 : 4;
 #define TT_LAUNCH(NPT)                                                                                        \
@@ -4914,7 +4942,7 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
         for (int k = 0; k < RC_NQ; k++) q.out[k] = d->b[outs[k]];
         q.guard = guard;
         q.of = c->st.omega_frame;
-        q.ntile_j = (ns + TB - 1) / TB;
+        q.ntile_j = (ns + RT_J - 1) / RT_J;
         q.rows = d->rad_rows;
         q.ntiles = q.ntile_j * ((nr + q.rows - 1) / q.rows);
         q.nq = RC_NQ;
@@ -4944,7 +4972,7 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
         for (int k = 0; k < RC_NQ; k++) q.out[k] = d->b[outs[k]];
         q.guard = guard;
         q.of = c->st.omega_frame;
-        q.ntile_j = (ns + TB - 1) / TB;
+        q.ntile_j = (ns + RT_J - 1) / RT_J;
         q.rows = d->rad_rows;
         q.ntiles = q.ntile_j * ((nr + q.rows - 1) / q.rows);
         q.nq = RC_NQ - 1;
@@ -5481,18 +5509,18 @@ extern "C" int thorin_autotune(thorin_ctx *c, thorin_tuning *out) {
   for (int k = 0; k < RC_NQ; k++) q.out[k] = d->b[outs[k]];
   q.guard = nullptr;
   q.of = c->st.omega_frame;
-  q.ntile_j = (ns + TB - 1) / TB;
+  q.ntile_j = (ns + RT_J - 1) / RT_J;
   q.nq = RC_NQ;
   const int rows[3] = {8, 16, 32};
   for (int v = 0; v < 3; v++) {
     q.rows = rows[v];
     q.ntiles = q.ntile_j * ((nr + q.rows - 1) / q.rows);
-    const dim3 grad((unsigned)((q.ntiles + 7) / 8 * 8 * q.nq)), tb(TB);
+    const dim3 grad((unsigned)((q.ntiles + 7) / 8 * 8)), trad(RT_T);
     const double dt = c->p.dt;
     auto work = [&]() -> int {
-      if (q.rows == 8) LAUNCH(k_transport_rad<8>, grad, tb, 0, d->g, q, dt);
-      else if (q.rows == 32) LAUNCH(k_transport_rad<32>, grad, tb, 0, d->g, q, dt);
-      else LAUNCH(k_transport_rad<16>, grad, tb, 0, d->g, q, dt);
+      if (q.rows == 8) LAUNCH(k_transport_rad<8>, grad, trad, 0, d->g, q, dt);
+      else if (q.rows == 32) LAUNCH(k_transport_rad<32>, grad, trad, 0, d->g, q, dt);
+      else LAUNCH(k_transport_rad<16>, grad, trad, 0, d->g, q, dt);
       HCHECK(hipGetLastError());
       return 0;
     };
