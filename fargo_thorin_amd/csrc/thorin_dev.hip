@@ -87,6 +87,7 @@ struct DevCtl {                 // device-side control block, mirrored to pinned
   int sor_done, sor_niter, sor_iter;
   int sor_cur;                  // T buffer holding the latest iterate (0: B_TEMPER, 1: B_TEMPER2, 2: B_TEMPER3,
                                 // 3: B_TEMPER4)
+  int sor_ticket, sor_pad;      // workgroups of a running single-iteration SOR launch that have finished
   double sor_normres0, sor_normres;
   double axstar, aystar;
   double axpl[MAXPL], aypl[MAXPL];
@@ -130,6 +131,12 @@ struct Geo {                    // passed by value to every kernel
   const double *Rinf, *Rsup, *Rmed, *Surf, *InvRmed, *InvSurf, *InvDiffRmed, *InvDiffRsup, *InvRinf;
   const double *OmegaInv, *SigmaMed, *SigmaInf, *EnergyMed, *VthetaMed, *WaveKiller;
   const double *pow_rmed15, *pow_rinf15, *cav_med, *cav_inf, *cfl_pow2, *supp_torque;
+  // ring constants of the azimuthal spacing, formed on the host by the
+  // expressions the kernels would evaluate per cell:
+  //   InvDxTheta[i] = 1 / (2 pi / NSEC * Rmed[i])      InvDxTheta2[i] = 1 / dxtheta / dxtheta
+  //   DxThetaA[i] = Rmed[i] * 2 * pi / NSEC             invdphi = 1 / (2 pi / NSEC)
+  const double *InvDxTheta, *InvDxTheta2, *DxThetaA;
+  double invdphi;
   const int *dampring;
   // cos and sin of the sector angles 2 pi j / NSEC (host libm): a cell's
   // x = Rmed[i] cos_j and y = Rmed[i] sin_j are the reference's
@@ -743,7 +750,7 @@ __device__ __forceinline__ void b_eos_cfl_t(const Geo &g, const double *__restri
   const size_t base = (size_t)i * ns;
   const bool cfl = (i >= 1 && i < nr);
   const double dxrad = g.Rsup[i] - g.Rinf[i];
-  const double dxtheta = g.Rmed[i] * 2.0 * DPI / (double)ns;
+  const double dxtheta = g.DxThetaA[i];
   const double mn = (dxtheta < dxrad) ? dxtheta : dxrad;
   double pre[EC_MAX], q4[EC_MAX];
 #pragma unroll
@@ -1413,7 +1420,7 @@ __device__ __forceinline__ void b_peb_cfl(const Geo &g, const double *__restrict
   double m = -1.0;
   if (i >= 1) {
     const double dxrad = g.Rsup[i] - g.Rinf[i];
-    const double dxtheta = g.Rmed[i] * 2.0 * DPI / (double)ns;
+    const double dxtheta = g.DxThetaA[i];
     const double pm = pvtmed;
     for (int j = tid; j < ns; j += RB) {
       const size_t l = base + j;
@@ -1542,8 +1549,7 @@ __device__ __forceinline__ void b_peb_substep1(const Geo &g, double *__restrict_
       corr *= (pdens[l] / rho[l] - pdens[lim] / rho[lim]) * g.InvDiffRmed[i];
       nvr += corr;
     }
-    const double dxtheta = 2.0 * DPI / (double)ns * g.Rmed[i];
-    const double invdxtheta = 1.0 / dxtheta;
+    const double invdxtheta = g.InvDxTheta[i];
     double corr = -fvisc_med(g, i) / g.schmidt;
     corr *= (rho[l] + rho[ljm]) / (pdens[l] + pdens[ljm]);
     corr *= (pdens[l] / rho[l] - pdens[ljm] / rho[ljm]) * invdxtheta;
@@ -1951,8 +1957,7 @@ __device__ __forceinline__ void b_substep1(const Geo &g, const double *__restric
   }
   {
     const double supp_torque = g.supp_torque[i];
-    const double dxtheta = 2.0 * DPI / (double)ns * g.Rmed[i];
-    const double invdxtheta = 1.0 / dxtheta;
+    const double invdxtheta = g.InvDxTheta[i];
     const double pljm = gm1 * energy[ljm];
     double gradp = (pl - pljm) * 2.0 / (rho[l] + rho[ljm]) * invdxtheta;
     double t_l = agt[l], t_jm = agt[ljm];
@@ -2007,7 +2012,7 @@ __device__ __forceinline__ Stress stress_cell(const Geo &g, const double *__rest
   const size_t l = (size_t)i * ns + j, lip = l + ns;
   const size_t ljp = (j == ns - 1) ? (size_t)i * ns : l + 1;
   const size_t ljm = (j == 0) ? (size_t)i * ns + ns - 1 : l - 1;
-  const double dphi = 2.0 * M_PI / (double)ns, invdphi = 1.0 / dphi, onethird = 1.0 / 3.0;
+  const double invdphi = g.invdphi, onethird = 1.0 / 3.0;
   const double Drr = (vr[lip] - vr[l]) * g.InvDiffRsup[i];
   const double Dpp = (vt[ljp] - vt[l]) * invdphi * g.InvRmed[i] + 0.5 * (vr[lip] + vr[l]) * g.InvRmed[i];
   double dv = (vr[lip] * g.Rsup[i] - vr[l] * g.Rinf[i]) * g.InvDiffRsup[i] * g.InvRmed[i];
@@ -2086,7 +2091,7 @@ __device__ __forceinline__ void b_visc_update(const Geo &g, double *__restrict__
   const size_t l = (size_t)i * ns + j, lip = l + ns;
   const size_t ljp = (j == ns - 1) ? (size_t)i * ns : l + 1;
   const size_t ljm = (j == 0) ? (size_t)i * ns + ns - 1 : l - 1;
-  const double invdphi = 1.0 / (2.0 * M_PI / (double)ns);
+  const double invdphi = g.invdphi;
   if (i >= 1 && i < nr - 1)
     vt[l] += DeltaT * g.InvRmed[i] *
              ((g.Rsup[i] * Trp[lip] - g.Rinf[i] * Trp[l]) * g.InvDiffRsup[i] + (Tpp[l] - Tpp[ljm]) * invdphi +
@@ -2144,8 +2149,7 @@ __device__ __forceinline__ void b_substep2(const Geo &g, const double *__restric
     vradnew[(size_t)nr * ns + j] = 0.0;     // ghost ring nr likewise
     vthetanew[(size_t)nr * ns + j] = 0.0;
   }
-  const double dxtheta = 2.0 * DPI / (double)ns * g.Rmed[i];
-  const double invdxtheta = 1.0 / dxtheta;
+  const double invdxtheta = g.InvDxTheta[i];
   const size_t ljmjp = (size_t)i * ns + j;   // (ljm)+1 == l
   const double qtjm = qvnr(rho[ljm], vtheta[ljmjp] - vtheta[ljm]);
   vthetanew[l] = vtheta[l] - dt * 2.0 / (rho[l] + rho[ljm]) * (qt - qtjm) * invdxtheta;
@@ -2274,7 +2278,7 @@ __device__ __forceinline__ void b_substep3(const Geo &g, const double *__restric
     const double t_im = temper_of(g, eint, rho, lg - ns), t_ip = temper_of(g, eint, rho, lg + ns);
     const double t_jm = temper_of(g, eint, rho, (size_t)ig * ns + jm);
     const double t_jp = temper_of(g, eint, rho, (size_t)ig * ns + jp);
-    const double invdxtheta = 1.0 / (2.0 * DPI / (double)ns * g.Rmed[ig]);
+    const double invdxtheta = g.InvDxTheta[ig];
     const double gtr_l = (tg - t_im) * g.InvDiffRmed[ig];
     const double gtr_lip = (t_ip - tg) * g.InvDiffRmed[ig + 1];
     const double gtt_l = (tg - t_jm) * invdxtheta;
@@ -2316,8 +2320,7 @@ __device__ __forceinline__ double coefB(const Geo &g, const double *D, const dou
   const int ns = g.ns;
   const size_t l = (size_t)i * ns + j;
   const size_t ljm = (j == 0) ? (size_t)i * ns + ns - 1 : l - 1;
-  const double dxtheta = 2.0 * DPI / (double)ns * g.Rmed[i];
-  const double invdxtheta2 = 1.0 / dxtheta / dxtheta;
+  const double invdxtheta2 = g.InvDxTheta2[i];
   const double H = cs[l] * g.OmegaInv[i] * g.sqrt_adiabind_inv;
   return 0.5 * (D[l] + D[ljm]) * invdxtheta2 * 2.0 * H;
 }
@@ -2528,7 +2531,66 @@ __device__ __forceinline__ bool sor_cell(int sl, int h, bool odd, int i0, int j0
   return kappa < tw + h;
 }
 
-template <int NIT>
+template <int NIT, int NT>
+__device__ __forceinline__ void sor_stop_test(const double *__restrict__ psum, const int *__restrict__ pcnt, int nb,
+                                              DevCtl *ctl, int maxiter, int it, double eps, double *red) {
+  // the partial sums of all NIT iterations are read together (eight loads
+  // of each array in flight per thread); each thread adds its partials in
+  // block order as a strided sum does
+  double s[NIT], c[NIT];
+  int ci[NIT];
+#pragma unroll
+  for (int k = 0; k < NIT; k++) { s[k] = 0.0; ci[k] = 0; }
+  int b = threadIdx.x;
+  for (; b + 7 * NT < nb; b += 8 * NT) {
+    double v[NIT][8];
+    int w[NIT][8];
+#pragma unroll
+    for (int k = 0; k < NIT; k++)
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        v[k][u] = psum[(size_t)k * nb + b + u * NT];
+        w[k][u] = pcnt[(size_t)k * nb + b + u * NT];
+      }
+#pragma unroll
+    for (int k = 0; k < NIT; k++)
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        s[k] += v[k][u];
+        ci[k] += w[k][u];
+      }
+  }
+  for (; b < nb; b += NT)
+#pragma unroll
+    for (int k = 0; k < NIT; k++) {
+      s[k] += psum[(size_t)k * nb + b];
+      ci[k] += pcnt[(size_t)k * nb + b];
+    }
+#pragma unroll
+  for (int k = 0; k < NIT; k++) {
+    s[k] = block_sum<NT>(s[k], red);
+    c[k] = block_sum<NT>((double)ci[k], red);
+  }
+  if (threadIdx.x == 0) {
+    for (int k = 0; k < NIT; k++) {
+      const int n = it + k + 1;
+      ctl->sor_iter = n;
+      ctl->sor_cur = n % 4;
+      ctl->sor_normres = s[k];
+      if (s[k] < eps * ctl->sor_normres0 || s[k] < SORMIN || c[k] == 0.0) {
+        ctl->sor_done = 1;
+        ctl->sor_niter = n;
+        break;
+      } else if (n >= maxiter) {
+        ctl->sor_done = 1;
+        ctl->sor_niter = maxiter + 1;
+        break;
+      }
+    }
+  }
+}
+
+template <int NIT, bool CHK = false>
 __device__ __forceinline__ void b_sor_iter(const Geo &g, double *__restrict__ T0, double *__restrict__ T1,
                                            double *__restrict__ T2, double *__restrict__ T3,
                                            const double *__restrict__ Ml,
@@ -2536,7 +2598,8 @@ __device__ __forceinline__ void b_sor_iter(const Geo &g, double *__restrict__ T0
                                            const double *__restrict__ Mljp, const double *__restrict__ Mljm,
                                            const double *__restrict__ RHS, double omega, double *__restrict__ psum,
                                            int *__restrict__ pcnt, DevCtl *__restrict__ ctl, int it,
-                                           const double *__restrict__ mresp, int nbm, int nbs) {
+                                           const double *__restrict__ mresp, int nbm, int nbs, int maxiter = 0,
+                                           double eps = 0.0) {
   static_assert(NIT >= 1 && NIT <= 3, "four iterate buffers hold at most three iterations per launch");
   typedef SorShape<NIT> S;
   __shared__ double t[S::LR * S::LW];
@@ -2652,9 +2715,28 @@ __device__ __forceinline__ void b_sor_iter(const Geo &g, double *__restrict__ T0
       pcnt[(size_t)k * nbs + bid] = (int)nch;
     }
   }
+  if (CHK) {
+    // the stop test in the workgroup that finishes last: each workgroup
+    // publishes its partials (device-scope fence) before it takes a ticket;
+    // the last one reads them all after its own fence, adds them in block
+    // order as k_sor_check does, and rearms the ticket for the next launch.
+    // (Used for the single-iteration launches that end a queue: while the
+    // solve has converged they return at once and need no check launch.)
+    __shared__ int s_last;
+    if (tid == 0) {
+      __threadfence();
+      s_last = atomicAdd(&ctl->sor_ticket, 1) == nbs - 1;
+    }
+    __syncthreads();
+    if (s_last) {
+      __threadfence();
+      sor_stop_test<NIT, TB>(psum, pcnt, nbs, ctl, maxiter, it, eps, red);
+      if (tid == 0) ctl->sor_ticket = 0;
+    }
+  }
 }
 
-template <int NIT>
+template <int NIT, bool CHK = false>
 __global__ void __launch_bounds__(TB) k_sor_iter(Geo g, double *__restrict__ T0, double *__restrict__ T1,
                                                  double *__restrict__ T2, double *__restrict__ T3,
                                                  const double *__restrict__ Ml,
@@ -2662,8 +2744,10 @@ __global__ void __launch_bounds__(TB) k_sor_iter(Geo g, double *__restrict__ T0,
                                                  const double *__restrict__ Mljp, const double *__restrict__ Mljm,
                                                  const double *__restrict__ RHS, double omega, double *__restrict__ psum,
                                                  int *__restrict__ pcnt, DevCtl *__restrict__ ctl, int it,
-                                                 const double *__restrict__ mresp, int nbm, int nbs) {
-  b_sor_iter<NIT>(g, T0, T1, T2, T3, Ml, Mlip, Mlim, Mljp, Mljm, RHS, omega, psum, pcnt, ctl, it, mresp, nbm, nbs);
+                                                 const double *__restrict__ mresp, int nbm, int nbs, int maxiter,
+                                                 double eps) {
+  b_sor_iter<NIT, CHK>(g, T0, T1, T2, T3, Ml, Mlip, Mlim, Mljp, Mljm, RHS, omega, psum, pcnt, ctl, it, mresp, nbm,
+                       nbs, maxiter, eps);
 }
 
 // stop test of iterations it+1 .. it+NIT (EnergySources.c:487-493), partial
@@ -2674,60 +2758,7 @@ __device__ __forceinline__ void b_sor_check(const double *__restrict__ psum, con
                                             DevCtl *ctl, int maxiter, int it, double eps) {
   __shared__ double red[RB];
   if (ctl->sor_done) return;
-  // the partial sums of all NIT iterations are read together (eight loads
-  // of each array in flight per thread); each thread adds its partials in
-  // block order as a strided sum does
-  double s[NIT], c[NIT];
-  int ci[NIT];
-#pragma unroll
-  for (int k = 0; k < NIT; k++) { s[k] = 0.0; ci[k] = 0; }
-  int b = threadIdx.x;
-  for (; b + 7 * RB < nb; b += 8 * RB) {
-    double v[NIT][8];
-    int w[NIT][8];
-#pragma unroll
-    for (int k = 0; k < NIT; k++)
-#pragma unroll
-      for (int u = 0; u < 8; u++) {
-        v[k][u] = psum[(size_t)k * nb + b + u * RB];
-        w[k][u] = pcnt[(size_t)k * nb + b + u * RB];
-      }
-#pragma unroll
-    for (int k = 0; k < NIT; k++)
-#pragma unroll
-      for (int u = 0; u < 8; u++) {
-        s[k] += v[k][u];
-        ci[k] += w[k][u];
-      }
-  }
-  for (; b < nb; b += RB)
-#pragma unroll
-    for (int k = 0; k < NIT; k++) {
-      s[k] += psum[(size_t)k * nb + b];
-      ci[k] += pcnt[(size_t)k * nb + b];
-    }
-#pragma unroll
-  for (int k = 0; k < NIT; k++) {
-    s[k] = block_sum<RB>(s[k], red);
-    c[k] = block_sum<RB>((double)ci[k], red);
-  }
-  if (threadIdx.x == 0) {
-    for (int k = 0; k < NIT; k++) {
-      const int n = it + k + 1;
-      ctl->sor_iter = n;
-      ctl->sor_cur = n % 4;
-      ctl->sor_normres = s[k];
-      if (s[k] < eps * ctl->sor_normres0 || s[k] < SORMIN || c[k] == 0.0) {
-        ctl->sor_done = 1;
-        ctl->sor_niter = n;
-        break;
-      } else if (n >= maxiter) {
-        ctl->sor_done = 1;
-        ctl->sor_niter = maxiter + 1;
-        break;
-      }
-    }
-  }
+  sor_stop_test<NIT, RB>(psum, pcnt, nb, ctl, maxiter, it, eps, red);
 }
 
 template <int NIT>
@@ -3344,10 +3375,10 @@ extern "C" int thorin_dev_create(thorin_ctx *c) {
     HCHECK(hipMalloc(&d->b[k], fb));
     HCHECK(hipMemsetAsync(d->b[k], 0, fb, d->s));
   }
-  // 1D arrays: 27 slots of (nr+3) doubles
+  // 1D arrays: 43 slots of (nr+3) doubles
   const int n1 = c->nr + 3;
-  HCHECK(hipMalloc(&d->d1base, sizeof(double) * (size_t)n1 * 40));
-  HCHECK(hipMemsetAsync(d->d1base, 0, sizeof(double) * (size_t)n1 * 40, d->s));
+  HCHECK(hipMalloc(&d->d1base, sizeof(double) * (size_t)n1 * 43));
+  HCHECK(hipMemsetAsync(d->d1base, 0, sizeof(double) * (size_t)n1 * 43, d->s));
   double *p1 = d->d1base;
   const double *src[21] = {c->Rinf, c->Rsup, c->Rmed, c->Surf, c->InvRmed, c->InvSurf, c->InvDiffRmed,
                            c->InvDiffRsup, c->InvRinf, c->OmegaInv, c->SigmaMed, c->SigmaInf, c->EnergyMed,
@@ -3378,6 +3409,22 @@ extern "C" int thorin_dev_create(thorin_ctx *c) {
   g.pist_avr = p1 + 34 * n1;
   g.pebmax = p1 + 35 * n1;
   g.pvmed = p1 + 36 * n1;
+  {
+    std::vector<double> az((size_t)3 * n1, 0.0);
+    const int ns = c->ns;
+    for (int i = 0; i < n1; i++) {
+      const double dxtheta = 2.0 * DPI / (double)ns * c->Rmed[i];
+      az[i] = 1.0 / dxtheta;
+      az[(size_t)n1 + i] = 1.0 / dxtheta / dxtheta;
+      az[(size_t)2 * n1 + i] = c->Rmed[i] * 2.0 * DPI / (double)ns;
+    }
+    HCHECK(hipMemcpyAsync(p1 + (size_t)37 * n1, az.data(), sizeof(double) * 3 * n1, hipMemcpyHostToDevice, d->s));
+    HCHECK(hipStreamSynchronize(d->s));        // az is freed at the end of this block
+    g.InvDxTheta = p1 + 37 * n1;
+    g.InvDxTheta2 = p1 + 38 * n1;
+    g.DxThetaA = p1 + 39 * n1;
+    g.invdphi = 1.0 / (2.0 * M_PI / (double)ns);
+  }
   int *ip;
   HCHECK(hipMalloc(&ip, sizeof(int) * (size_t)n1 * 3));
   g.Nshift = ip;
@@ -3860,9 +3907,10 @@ BATCH_BODY(k_substep2, b_substep2, TB)
 BATCH_BODY(k_substep3, b_substep3, TB)
 BATCH_BODY(k_matrix, b_matrix, TB)
 BATCH_BODY(k_sor_resid0, b_sor_resid0, TB)
-BATCH_BODY(k_sor_iter<1>, b_sor_iter<1>, TB)
-BATCH_BODY(k_sor_iter<2>, b_sor_iter<2>, TB)
-BATCH_BODY(k_sor_iter<3>, b_sor_iter<3>, TB)
+BATCH_BODY((k_sor_iter<1, false>), (b_sor_iter<1, false>), TB)
+BATCH_BODY((k_sor_iter<1, true>), (b_sor_iter<1, true>), TB)
+BATCH_BODY((k_sor_iter<2, false>), (b_sor_iter<2, false>), TB)
+BATCH_BODY((k_sor_iter<3, false>), (b_sor_iter<3, false>), TB)
 BATCH_BODY(k_sor_check<1>, b_sor_check<1>, RB)
 BATCH_BODY(k_sor_check<2>, b_sor_check<2>, RB)
 BATCH_BODY(k_sor_check<3>, b_sor_check<3>, RB)
@@ -4150,19 +4198,26 @@ static int sor_launch(thorin_ctx *c, double omega, int it0, int &count, double e
   const int nmax = (d->sor_nit >= 1 && d->sor_nit <= 3) ? d->sor_nit : 1;
 #define SOR_LAUNCH(N)                                                                                             \
   do {                                                                                                            \
-    LAUNCH(k_sor_iter<N>, gs, dim3(TB), 0, g, d->b[B_TEMPER], d->b[B_TEMPER2], d->b[B_TEMPER3], d->b[B_TEMPER4],  \
-           d->b[B_ML], d->b[B_MLIP], d->b[B_MLIM], d->b[B_MLJP], d->b[B_MLJM], d->b[B_RHS], omega, d->sorp,        \
-           d->sorc, d->ctl, it, (const double *)d->mresp, d->nb_mresp, nbs);                                     \
+    LAUNCH((k_sor_iter<N, false>), gs, dim3(TB), 0, g, d->b[B_TEMPER], d->b[B_TEMPER2], d->b[B_TEMPER3],          \
+           d->b[B_TEMPER4], d->b[B_ML], d->b[B_MLIP], d->b[B_MLIM], d->b[B_MLJP], d->b[B_MLJM], d->b[B_RHS], omega, \
+           d->sorp, d->sorc, d->ctl, it, (const double *)d->mresp, d->nb_mresp, nbs, maxiter, eps);              \
     LAUNCH(k_sor_check<N>, dim3(1), dim3(RB), 0, d->sorp, d->sorc, nbs, d->ctl, maxiter, it, eps);               \
   } while (0)
+  // single iterations after the first launch carry their own stop test
+#define SOR_LAUNCH1C()                                                                                            \
+  LAUNCH((k_sor_iter<1, true>), gs, dim3(TB), 0, g, d->b[B_TEMPER], d->b[B_TEMPER2], d->b[B_TEMPER3],             \
+         d->b[B_TEMPER4], d->b[B_ML], d->b[B_MLIP], d->b[B_MLIM], d->b[B_MLJP], d->b[B_MLJM], d->b[B_RHS], omega,   \
+         d->sorp, d->sorc, d->ctl, it, (const double *)d->mresp, d->nb_mresp, nbs, maxiter, eps)
   for (int it = it0; it < it0 + count;) {
     const int nit = (it0 + count - it >= nmax) ? nmax : 1;
     if (nit == 3) SOR_LAUNCH(3);
     else if (nit == 2) SOR_LAUNCH(2);
+    else if (it > 0) SOR_LAUNCH1C();
     else SOR_LAUNCH(1);
     it += nit;
   }
 #undef SOR_LAUNCH
+#undef SOR_LAUNCH1C
   HCHECK(hipGetLastError());
   return 0;
 }
@@ -4900,9 +4955,9 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
     if (c->st.sor_omega > OMEGAUPPER) { c->st.sor_omega = OMEGAUPPER; c->st.sor_domega = -c->st.sor_domega; }
     if (c->st.sor_omega < 1.0) { c->st.sor_omega = 1.0; c->st.sor_domega = -c->st.sor_domega; }
     omega_used = c->st.sor_omega;
-    // The iterations are enqueued without a host round trip: the previous
-    // solve's count + 2 of them, each ending in the device-side stop test
-    // (iterations after convergence do nothing).  Everything after the solve
+    // The iterations are enqueued without a host round trip: one more than
+    // the longest of the last four solves took (at least 2), each ending in
+    // the device-side stop test (iterations after convergence do nothing).  Everything after the solve
     // is enqueued behind them and does nothing unless the solve converged
     // (guard = ctl->sor_done); in the rare case it needed more iterations the
     // solve is finished with host checks and the rest of the substep is
