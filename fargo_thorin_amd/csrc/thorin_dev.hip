@@ -671,28 +671,33 @@ __device__ __forceinline__ double strided_min(const double *__restrict__ p, int 
   return m;
 }
 
-// sequential ring sum in the reference's order (one thread walks the ring);
-// the LDS reads of the next 16 values are issued before the 16 dependent
-// adds of the current batch, so the chain runs at the FP64 add latency.
+// sequential ring sum in the reference's order (one thread walks the ring).
+// The walk goes in batches of 16 LDS values held in two register buffers:
+// the reads of batch b+1 are issued before the 16 dependent adds of batch b,
+// so each batch's reads have the previous batch's adds to arrive in and the
+// chain runs at the FP64 add latency.
+__device__ __forceinline__ void ring_ld16(double *r, const double *sh) {
+#pragma unroll
+  for (int k = 0; k < 16; k++) r[k] = sh[k];
+}
+__device__ __forceinline__ void ring_add16(double &s, const double *r) {
+#pragma unroll
+  for (int k = 0; k < 16; k++) s += r[k];
+}
 __device__ __forceinline__ double ordered_ring_sum(const double *sh, int ns) {
   double s = 0.0;
+  const int nb = ns >> 4;                      // full batches of 16
   double a[16], b[16];
-  int j = 0;
-  if (ns >= 16) {
-#pragma unroll
-    for (int k = 0; k < 16; k++) a[k] = sh[k];
-    for (j = 16; j + 16 <= ns; j += 16) {
-#pragma unroll
-      for (int k = 0; k < 16; k++) b[k] = sh[j + k];
-#pragma unroll
-      for (int k = 0; k < 16; k++) s += a[k];
-#pragma unroll
-      for (int k = 0; k < 16; k++) a[k] = b[k];
-    }
-#pragma unroll
-    for (int k = 0; k < 16; k++) s += a[k];
+  if (nb > 0) ring_ld16(a, sh);
+  int q = 0;
+  for (; q + 1 < nb; q += 2) {
+    ring_ld16(b, sh + (q + 1) * 16);
+    ring_add16(s, a);
+    if (q + 2 < nb) ring_ld16(a, sh + (q + 2) * 16);
+    ring_add16(s, b);
   }
-  for (; j < ns; j++) s += sh[j];
+  if (q < nb) ring_add16(s, a);                // an odd batch count: the last one
+  for (int j = nb << 4; j < ns; j++) s += sh[j];
   return s;
 }
 
@@ -742,7 +747,12 @@ __device__ __forceinline__ void b_eos_cfl_t(const Geo &g, const double *__restri
   __shared__ double red[RB / 64];
   __shared__ double vmoy, gcs_i, gtemp_i;
   if (guard && *guard == 0) return;
-  const int i = (int)blockIdx.x - g.blk0, ns = g.ns, nr = g.nr, tid = threadIdx.x;
+  // rings in XCD bands (XCD x = block % 8 takes rings x*per .. x*per+per-1):
+  // the neighbouring rings this ring reads are then mostly in the same L2
+  const int bb = (int)blockIdx.x - g.blk0, per = (g.nr + 7) >> 3;
+  const int i = (bb & 7) * per + (bb >> 3);
+  if (i >= g.nr) return;
+  const int ns = g.ns, nr = g.nr, tid = threadIdx.x;
   const bool want_t = g.alpha_flock && do_eos;
   // rows EC_PAD doubles longer than NSEC: lanes 0, 1, 2 read distinct banks
   double *s_vt = sh, *s_cs = sh + (ns + EC_PAD), *s_t = sh + 2 * (ns + EC_PAD);
@@ -752,21 +762,22 @@ __device__ __forceinline__ void b_eos_cfl_t(const Geo &g, const double *__restri
   const double dxrad = g.Rsup[i] - g.Rinf[i];
   const double dxtheta = g.DxThetaA[i];
   const double mn = (dxtheta < dxrad) ? dxtheta : dxrad;
-  double pre[EC_MAX], q4[EC_MAX];
+  // pre: the CFL terms of the sound speed and of vr; adr: the radial
+  // shear term; the azimuthal shear needs vtheta of the next sector, taken
+  // from LDS after the barrier
+  double pre[EC_MAX], adr[EC_MAX];
 #pragma unroll
   for (int k = 0; k < EC_MAX; k++) {
     const int j = tid + k * RB;
     pre[k] = 0.0;
-    q4[k] = 0.0;
+    adr[k] = 0.0;
     if (j < ns) {
       const size_t l = base + j;
-      double vtl, vrl = 0.0, vrp = 0.0, vtp = 0.0;
+      double vtl, vrl = 0.0, vrp = 0.0;
       if (VEL) {
         const double rl = rho[l];
         const size_t ljm = (j == 0) ? base + ns - 1 : l - 1;
-        const size_t ljp = (j == ns - 1) ? base : l + 1;
         vtl = (vs.tp[ljm] + vs.tm[l]) / (rl + rho[ljm] + 1e-15) / g.Rmed[i] - g.Rmed[i] * g.omega_frame;
-        vtp = (vs.tp[l] + vs.tm[ljp]) / (rho[ljp] + rl + 1e-15) / g.Rmed[i] - g.Rmed[i] * g.omega_frame;
         if (i > 0) vrl = (vs.rp[l - ns] + vs.rm[l]) / (rl + rho[l - ns] + 1e-20);
         if (vs.damp && g.dampring[i]) {
           const double lam = g.WaveKiller[i] * vs.step;
@@ -801,23 +812,16 @@ __device__ __forceinline__ void b_eos_cfl_t(const Geo &g, const double *__restri
       }
       s_cs[j] = c;
       if (cfl) {
-        const int jp = (j == ns - 1) ? 0 : j + 1;
         const double invdt1 = c / mn;
         if (!VEL) {
           vrl = vr[l];
           vrp = vr[l + ns];
-          vtp = vt[base + jp];
         }
         const double invdt2 = fabs(vrl) / dxrad;
         double dvr = vrp - vrl;
-        double dvt = vtp - vtl;
         if (dvr >= 0.0) dvr = 1e-10; else dvr = -dvr;
-        if (dvt >= 0.0) dvt = 1e-10; else dvt = -dvt;
-        const double a = dvr / dxrad, b = dvt / dxtheta;
-        double invdt4 = (b > a) ? b : a;
-        invdt4 *= 4.0 * CVNR * CVNR;
+        adr[k] = dvr / dxrad;
         pre[k] = invdt1 * invdt1 + invdt2 * invdt2;
-        q4[k] = invdt4 * invdt4;
       }
     }
   }
@@ -899,7 +903,15 @@ __device__ __forceinline__ void b_eos_cfl_t(const Geo &g, const double *__restri
       if (j < ns) {
         const double vres = g.fast_transport ? s_vt[j] - vm : s_vt[j];
         const double invdt3 = fabs(vres) / dxtheta;
-        const double S = pre[k] + invdt3 * invdt3 + q4[k] + invdt5 * invdt5;
+        // vtheta of the next sector: the value its owner formed (the same
+        // expression) or read
+        const int jp = (j == ns - 1) ? 0 : j + 1;
+        double dvt = s_vt[jp] - s_vt[j];
+        if (dvt >= 0.0) dvt = 1e-10; else dvt = -dvt;
+        const double a = adr[k], b = dvt / dxtheta;
+        double invdt4 = (b > a) ? b : a;
+        invdt4 *= 4.0 * CVNR * CVNR;
+        const double S = pre[k] + invdt3 * invdt3 + invdt4 * invdt4 + invdt5 * invdt5;
         if (S > m) m = S;
       }
     }
@@ -4679,7 +4691,7 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
       if (peb)   // PebbleStokesNumbers + CriticalCharTime (SourceEuler.c:236, 241)
         LAUNCH(k_peb_cfl, dim3(nr), rb, sizeof(double) * 2 * ns, d->g, d->b[B_DENS], d->b[B_PSIZE], d->b[B_TAU],
                d->b[B_PVR], d->b[B_PVT], d->b[B_VRAD], d->b[B_VTHETA], (const int *)nullptr);
-      LAUNCH(k_eos_cfl, dim3(nr), rb, eos_lds(d, 0), d->g, d->b[B_DENS], d->b[B_ENERGY],
+      LAUNCH(k_eos_cfl, dim3((nr + 7) / 8 * 8), rb, eos_lds(d, 0), d->g, d->b[B_DENS], d->b[B_ENERGY],
              d->b[B_TEMPER], d->b[B_CS], d->b[B_VRAD], d->b[B_VTHETA], 0, (const int *)nullptr);
       LAUNCH(k_cfl_final, dim3(1), rb, 0, d->g, d->ctl, (const double *)nullptr, 0, invdtreb_sq,
              (const int *)nullptr, (DevCtl *)nullptr);
@@ -5040,10 +5052,10 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
       // minima of the next substep (its planets are those of now)
       if (fuse_vel) {
         const VelSrc vs = {d->b[B_RP], d->b[B_RM], d->b[B_TP], d->b[B_TM], dt, p->damping_on};
-        LAUNCH(k_eos_cfl_vel, dim3(nr), rb, eos_lds(d, 1), d->g, d->b[B_DENS], d->b[B_ENERGY], d->b[B_TEMPER],
+        LAUNCH(k_eos_cfl_vel, dim3((nr + 7) / 8 * 8), rb, eos_lds(d, 1), d->g, d->b[B_DENS], d->b[B_ENERGY], d->b[B_TEMPER],
                d->b[B_CS], d->b[B_VRAD], d->b[B_VTHETA], guard, vs);
       } else {
-        LAUNCH(k_eos_cfl, dim3(nr), rb, eos_lds(d, 1), d->g, d->b[B_DENS], d->b[B_ENERGY],
+        LAUNCH(k_eos_cfl, dim3((nr + 7) / 8 * 8), rb, eos_lds(d, 1), d->g, d->b[B_DENS], d->b[B_ENERGY],
                d->b[B_TEMPER], d->b[B_CS], d->b[B_VRAD], d->b[B_VTHETA], 1, guard);
       }
       // the planets' step, overlapped with the kernels enqueued so far
@@ -5888,7 +5900,7 @@ extern "C" int thorin_rad_system(thorin_ctx *c, double dt, double *dt_used) {
     // the hydro CFL step of the current fields (ConditionCFL without planets)
     Geo gg = d->g;
     gg.pebbles = 0;
-    LAUNCH(k_eos_cfl, dim3(nr), rb, eos_lds(d, 0), gg, d->b[B_DENS], d->b[B_ENERGY], d->b[B_TEMPER],
+    LAUNCH(k_eos_cfl, dim3((nr + 7) / 8 * 8), rb, eos_lds(d, 0), gg, d->b[B_DENS], d->b[B_ENERGY], d->b[B_TEMPER],
            d->b[B_CS], d->b[B_VRAD], d->b[B_VTHETA], 0, (const int *)nullptr);
     LAUNCH(k_cfl_final, dim3(1), rb, 0, gg, d->ctl, (const double *)nullptr, 0, 0.0, (const int *)nullptr,
            (DevCtl *)nullptr);
