@@ -2479,6 +2479,11 @@ __global__ void __launch_bounds__(TB) k_sor_resid0(Geo g, const double *__restri
 // does not depend on how many iterations a launch runs.
 #define SOR_TI 8
 #define SOR_TJ 128
+// threads per workgroup of k_sor_iter: the three-iteration launch holds 10
+// cells' matrix rows per thread at 256 threads, which leaves two waves per
+// SIMD; at 512 threads (6 rows each, 128 registers) two workgroups of eight
+// waves share each CU
+#define SOR_NT(nit) ((nit) == 3 ? 2 * TB : TB)
 #define SOR_LW (SOR_TJ + 4)
 // cells per thread of the one-sweep tiles (FP32 sweeps): red with a one-cell halo, black owned
 #define SOR_NRED (((SOR_TI + 2) * ((SOR_TJ + 3) / 2) + TB - 1) / TB)
@@ -2488,10 +2493,16 @@ __global__ void __launch_bounds__(TB) k_sor_resid0(Geo g, const double *__restri
 // staged, so that all global loads of the workgroup are in flight together
 struct SorCell {
   double ml, mip, mim, mjp, mjm, rhs;
-  int k;                        // LDS index of the cell, -1: no cell
-  int d;                        // distance from the owned tile (0: owned)
-  int o;                        // index among the owned cells (owned only)
+  // packed: bits 0-11 the LDS index of the cell, 12-14 its distance from
+  // the owned tile (0: owned), 16-25 its index among the owned cells (owned
+  // only); -1: no cell
+  int pk;
+  __device__ __forceinline__ int k() const { return pk & 0xFFF; }
+  __device__ __forceinline__ int d() const { return (pk >> 12) & 7; }
+  __device__ __forceinline__ int o() const { return (pk >> 16) & 0x3FF; }
 };
+static_assert((SOR_TI + 12) * (SOR_TJ + 12) <= 0xFFF + 1 && SOR_TI * SOR_TJ <= 0x3FF + 1,
+              "SorCell packs the LDS index in 12 bits and the owned index in 10");
 
 __device__ __forceinline__ void sor_load(SorCell &x, size_t l, const double *__restrict__ Ml,
                                          const double *__restrict__ Mlip, const double *__restrict__ Mlim,
@@ -2507,26 +2518,26 @@ __device__ __forceinline__ void sor_load(SorCell &x, size_t l, const double *__r
 
 template <int LW>
 __device__ __forceinline__ void sor_update(const SorCell &x, double *t, double omega, double *sacc, int *schg) {
-  const int k = x.k;
+  const int k = x.k();
   const double resid = t[k] * x.ml + t[k + LW] * x.mip + t[k - LW] * x.mim + t[k + 1] * x.mjp +
                        t[k - 1] * x.mjm - x.rhs;
   const double dtemp = omega * resid / x.ml;
   const double tn = t[k] - dtemp;
   t[k] = tn;
-  if (x.d == 0) {
-    sacc[x.o] = fabs(resid);
-    schg[x.o] = (dtemp > tn * SORPRECISION) ? 1 : 0;
+  if (x.d() == 0) {
+    sacc[x.o()] = fabs(resid);
+    schg[x.o()] = (dtemp > tn * SORPRECISION) ? 1 : 0;
   }
 }
 
-template <int NIT>
+template <int NIT, int NT = TB>
 struct SorShape {
   static constexpr int H = 2 * NIT;                        // staged halo
   static constexpr int LW = SOR_TJ + 2 * H;                // LDS row
   static constexpr int LR = SOR_TI + 2 * H;                // LDS rows
   static constexpr int HR = 2 * NIT - 1, HB = 2 * NIT - 2; // halo of the red / black cells
-  static constexpr int NRED = ((SOR_TI + 2 * HR) * ((SOR_TJ + 2 * HR + 1) / 2) + TB - 1) / TB;
-  static constexpr int NBLK = ((SOR_TI + 2 * HB) * ((SOR_TJ + 2 * HB + 1) / 2) + TB - 1) / TB;
+  static constexpr int NRED = ((SOR_TI + 2 * HR) * ((SOR_TJ + 2 * HR + 1) / 2) + NT - 1) / NT;
+  static constexpr int NBLK = ((SOR_TI + 2 * HB) * ((SOR_TJ + 2 * HB + 1) / 2) + NT - 1) / NT;
 };
 
 // slot sl of the cells of one colour (odd: (i+j) odd, red) within halo h of
@@ -2602,7 +2613,7 @@ __device__ __forceinline__ void sor_stop_test(const double *__restrict__ psum, c
   }
 }
 
-template <int NIT, bool CHK = false>
+template <int NIT, bool CHK = false, int NT = TB>
 __device__ __forceinline__ void b_sor_iter(const Geo &g, double *__restrict__ T0, double *__restrict__ T1,
                                            double *__restrict__ T2, double *__restrict__ T3,
                                            const double *__restrict__ Ml,
@@ -2613,11 +2624,12 @@ __device__ __forceinline__ void b_sor_iter(const Geo &g, double *__restrict__ T0
                                            const double *__restrict__ mresp, int nbm, int nbs, int maxiter = 0,
                                            double eps = 0.0) {
   static_assert(NIT >= 1 && NIT <= 3, "four iterate buffers hold at most three iterations per launch");
-  typedef SorShape<NIT> S;
+  static_assert(NT % TB == 0, "the partial sums keep the pattern of TB threads");
+  typedef SorShape<NIT, NT> S;
   __shared__ double t[S::LR * S::LW];
   __shared__ double sacc[SOR_TI * SOR_TJ];
   __shared__ int schg[SOR_TI * SOR_TJ];
-  __shared__ double red[TB];
+  __shared__ double red[NT];
   const int ns = g.ns, nr = g.nr;
   const int ntx = (ns + SOR_TJ - 1) / SOR_TJ;
   int tx, ty, bid;
@@ -2634,8 +2646,10 @@ __device__ __forceinline__ void b_sor_iter(const Geo &g, double *__restrict__ T0
     if (bid == 0) {
       // initial residual normres0 (EnergySources.c:435-451) from the matrix
       // kernel's block partials
-      double r = strided_sum<TB>(mresp, nbm);
-      r = block_sum<TB>(r, red);
+      // in the pattern of TB threads whatever NT is (the other threads add
+      // +0.0, which leaves every partial as it is)
+      double r = tid < TB ? strided_sum<TB>(mresp, nbm) : 0.0;
+      r = block_sum<NT>(r, red);
       if (tid == 0) {
         ctl->sor_normres0 = r;
         ctl->sor_done = 0;
@@ -2645,13 +2659,13 @@ __device__ __forceinline__ void b_sor_iter(const Geo &g, double *__restrict__ T0
       }
     }
     if (ty == 0)
-      for (int j = j0 + tid; j < j0 + tw; j += TB) {
+      for (int j = j0 + tid; j < j0 + tw; j += NT) {
         T1[j] = T0[j];
         T2[j] = T0[j];
         T3[j] = T0[j];
       }
     if (i1 == nr - 1)
-      for (int j = j0 + tid; j < j0 + tw; j += TB)
+      for (int j = j0 + tid; j < j0 + tw; j += NT)
         for (int r = nr - 1; r <= nr; r++) {
           T1[(size_t)r * ns + j] = T0[(size_t)r * ns + j];
           T2[(size_t)r * ns + j] = T0[(size_t)r * ns + j];
@@ -2666,9 +2680,9 @@ __device__ __forceinline__ void b_sor_iter(const Geo &g, double *__restrict__ T0
   auto setup = [&](SorCell *cs, int ncell, int h, bool odd) {
 #pragma unroll
     for (int q = 0; q < ncell; q++) {
-      cs[q].k = -1;
+      cs[q].pk = -1;
       int rho, kappa;
-      if (!sor_cell(tid + q * TB, h, odd, i0, j0, ti, tw, rho, kappa)) continue;
+      if (!sor_cell(tid + q * NT, h, odd, i0, j0, ti, tw, rho, kappa)) continue;
       const int i = i0 + rho;
       if (i < 1 || i > nr - 2) continue;
       int j = j0 + kappa;
@@ -2676,9 +2690,8 @@ __device__ __forceinline__ void b_sor_iter(const Geo &g, double *__restrict__ T0
       while (j >= ns) j -= ns;
       const int dr = rho < 0 ? -rho : (rho >= ti ? rho - ti + 1 : 0);
       const int dc = kappa < 0 ? -kappa : (kappa >= tw ? kappa - tw + 1 : 0);
-      cs[q].k = (rho + S::H) * S::LW + (kappa + S::H);
-      cs[q].d = dr > dc ? dr : dc;
-      cs[q].o = cs[q].d == 0 ? rho * tw + kappa : -1;
+      const int dd = dr > dc ? dr : dc;
+      cs[q].pk = ((rho + S::H) * S::LW + (kappa + S::H)) | (dd << 12) | ((dd == 0 ? rho * tw + kappa : 0) << 16);
       sor_load(cs[q], (size_t)i * ns + j, Ml, Mlip, Mlim, Mljp, Mljm, RHS);
     }
   };
@@ -2686,7 +2699,7 @@ __device__ __forceinline__ void b_sor_iter(const Geo &g, double *__restrict__ T0
   setup(bc, S::NBLK, S::HB, false);
   // stage rings i0-H .. i1+H-1 and sectors j0-H .. j0+tw+H-1 (periodic in phi)
   const int sw = tw + 2 * S::H;
-  for (int idx = tid; idx < (ti + 2 * S::H) * sw; idx += TB) {
+  for (int idx = tid; idx < (ti + 2 * S::H) * sw; idx += NT) {
     const int r = idx / sw, c = idx - r * sw;
     const int i = i0 - S::H + r;
     if (i < 0 || i > nr - 1) continue;
@@ -2702,26 +2715,26 @@ __device__ __forceinline__ void b_sor_iter(const Geo &g, double *__restrict__ T0
     const int hr = 2 * NIT - 1 - 2 * k, hb = hr - 1;
 #pragma unroll
     for (int q = 0; q < S::NRED; q++)
-      if (rc[q].k >= 0 && rc[q].d <= hr) sor_update<S::LW>(rc[q], t, omega, sacc, schg);
+      if (rc[q].pk >= 0 && rc[q].d() <= hr) sor_update<S::LW>(rc[q], t, omega, sacc, schg);
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < S::NBLK; q++)
-      if (bc[q].k >= 0 && bc[q].d <= hb) sor_update<S::LW>(bc[q], t, omega, sacc, schg);
+      if (bc[q].pk >= 0 && bc[q].d() <= hb) sor_update<S::LW>(bc[q], t, omega, sacc, schg);
     __syncthreads();
     // iterate it+k+1 of the owned cells, and its stop-test partials
     double *__restrict__ Tout = X[(it + k + 1) % 4];
-    for (int idx = tid; idx < nown; idx += TB) {
+    for (int idx = tid; idx < nown; idx += NT) {
       const int r = idx / tw, c = idx - r * tw;
       Tout[(size_t)(i0 + r) * ns + (j0 + c)] = t[(r + S::H) * S::LW + (c + S::H)];
     }
     double acc = 0.0;
     int chg = 0;
-    for (int idx = tid; idx < nown; idx += TB) {
+    for (int idx = tid; tid < TB && idx < nown; idx += TB) {
       acc += sacc[idx];
       chg += schg[idx];
     }
-    acc = block_sum<TB>(acc, red);
-    const double nch = block_sum<TB>((double)chg, red);
+    acc = block_sum<NT>(acc, red);
+    const double nch = block_sum<NT>((double)chg, red);
     if (tid == 0) {
       psum[(size_t)k * nbs + bid] = acc;
       pcnt[(size_t)k * nbs + bid] = (int)nch;
@@ -2742,14 +2755,14 @@ __device__ __forceinline__ void b_sor_iter(const Geo &g, double *__restrict__ T0
     __syncthreads();
     if (s_last) {
       __threadfence();
-      sor_stop_test<NIT, TB>(psum, pcnt, nbs, ctl, maxiter, it, eps, red);
+      sor_stop_test<NIT, NT>(psum, pcnt, nbs, ctl, maxiter, it, eps, red);
       if (tid == 0) ctl->sor_ticket = 0;
     }
   }
 }
 
-template <int NIT, bool CHK = false>
-__global__ void __launch_bounds__(TB) k_sor_iter(Geo g, double *__restrict__ T0, double *__restrict__ T1,
+template <int NIT, bool CHK = false, int NT = TB>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 2 * TB ? 4 : 1))) k_sor_iter(Geo g, double *__restrict__ T0, double *__restrict__ T1,
                                                  double *__restrict__ T2, double *__restrict__ T3,
                                                  const double *__restrict__ Ml,
                                                  const double *__restrict__ Mlip, const double *__restrict__ Mlim,
@@ -2758,8 +2771,8 @@ __global__ void __launch_bounds__(TB) k_sor_iter(Geo g, double *__restrict__ T0,
                                                  int *__restrict__ pcnt, DevCtl *__restrict__ ctl, int it,
                                                  const double *__restrict__ mresp, int nbm, int nbs, int maxiter,
                                                  double eps) {
-  b_sor_iter<NIT, CHK>(g, T0, T1, T2, T3, Ml, Mlip, Mlim, Mljp, Mljm, RHS, omega, psum, pcnt, ctl, it, mresp, nbm,
-                       nbs, maxiter, eps);
+  b_sor_iter<NIT, CHK, NT>(g, T0, T1, T2, T3, Ml, Mlip, Mlim, Mljp, Mljm, RHS, omega, psum, pcnt, ctl, it, mresp,
+                           nbm, nbs, maxiter, eps);
 }
 
 // stop test of iterations it+1 .. it+NIT (EnergySources.c:487-493), partial
@@ -3919,10 +3932,10 @@ BATCH_BODY(k_substep2, b_substep2, TB)
 BATCH_BODY(k_substep3, b_substep3, TB)
 BATCH_BODY(k_matrix, b_matrix, TB)
 BATCH_BODY(k_sor_resid0, b_sor_resid0, TB)
-BATCH_BODY((k_sor_iter<1, false>), (b_sor_iter<1, false>), TB)
+BATCH_BODY((k_sor_iter<1, false, SOR_NT(1)>), (b_sor_iter<1, false, SOR_NT(1)>), SOR_NT(1))
 BATCH_BODY((k_sor_iter<1, true>), (b_sor_iter<1, true>), TB)
-BATCH_BODY((k_sor_iter<2, false>), (b_sor_iter<2, false>), TB)
-BATCH_BODY((k_sor_iter<3, false>), (b_sor_iter<3, false>), TB)
+BATCH_BODY((k_sor_iter<2, false, SOR_NT(2)>), (b_sor_iter<2, false, SOR_NT(2)>), SOR_NT(2))
+BATCH_BODY((k_sor_iter<3, false, SOR_NT(3)>), (b_sor_iter<3, false, SOR_NT(3)>), SOR_NT(3))
 BATCH_BODY(k_sor_check<1>, b_sor_check<1>, RB)
 BATCH_BODY(k_sor_check<2>, b_sor_check<2>, RB)
 BATCH_BODY(k_sor_check<3>, b_sor_check<3>, RB)
@@ -4210,7 +4223,8 @@ static int sor_launch(thorin_ctx *c, double omega, int it0, int &count, double e
   const int nmax = (d->sor_nit >= 1 && d->sor_nit <= 3) ? d->sor_nit : 1;
 #define SOR_LAUNCH(N)                                                                                             \
   do {                                                                                                            \
-    LAUNCH((k_sor_iter<N, false>), gs, dim3(TB), 0, g, d->b[B_TEMPER], d->b[B_TEMPER2], d->b[B_TEMPER3],          \
+    LAUNCH((k_sor_iter<N, false, SOR_NT(N)>), gs, dim3(SOR_NT(N)), 0, g, d->b[B_TEMPER], d->b[B_TEMPER2],         \
+           d->b[B_TEMPER3],                                                                                       \
            d->b[B_TEMPER4], d->b[B_ML], d->b[B_MLIP], d->b[B_MLIM], d->b[B_MLJP], d->b[B_MLJM], d->b[B_RHS], omega, \
            d->sorp, d->sorc, d->ctl, it, (const double *)d->mresp, d->nb_mresp, nbs, maxiter, eps);              \
     LAUNCH(k_sor_check<N>, dim3(1), dim3(RB), 0, d->sorp, d->sorc, nbs, d->ctl, maxiter, it, eps);               \
