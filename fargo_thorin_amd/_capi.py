@@ -134,6 +134,7 @@ def lib() -> C.CDLL:
         "thorin_rad_solve": (ip, [vp, C.POINTER(SolveOpts), C.POINTER(SolveResult)]),
         "thorin_rad_solve_noalloc": (ip, [vp, C.POINTER(SolveOpts), C.POINTER(SolveResult)]),
         "thorin_math_eval": (ip, [ip, dp, dp, dp, sz]),
+        "thorin_opacity_eval": (ip, [vp, dp, dp, dp, sz]),
         "thorin_orbit_elements": (ip, [C.POINTER(Planets), ip, C.c_double, dp]),
         "thorin_cnn_create": (ip, [C.c_ulonglong, C.POINTER(vp)]),
         "thorin_cnn_destroy": (ip, [vp]),

@@ -310,14 +310,29 @@ __device__ __forceinline__ double opacity_bl94(double rho, double T) {
   return k_;
 }
 
+// T < k * pow(rho, e), decided from a single-precision estimate of the right
+// side where T is clear of it by more than 1e-5 (relative), and from the
+// double expression itself otherwise.  The estimate's error is below 1e-6
+// for 1e-30 < rho < 1e30 (float rounding of rho, log2f and exp2f within an
+// ulp each, e within an ulp), so every decision is the double expression's.
+__device__ __forceinline__ bool lt_kpow(double T, double k, double rho, double e, float ef) {
+  const float rf = (float)rho;
+  if (rf > 1e-30f && rf < 1e30f) {
+    const double est = k * (double)exp2f(log2f(rf) * ef);
+    if (T < est * (1.0 - 1e-5)) return true;
+    if (T > est * (1.0 + 1e-5)) return false;
+  }
+  return T < k * pow(rho, e);
+}
+
 __device__ __forceinline__ double opacity_lp85(const Geo &g, double rho, double T) {
   // Opacity.c:201-227; the constant thresholds come folded from the host
   if (T < g.lp_t1) return 2.0e-4 * T * T;
   if (T < g.lp_t2) return 2.0e16 * pow(T, -7.0);
-  if (T < g.lp_t3a * pow(rho, 1.0 / 15.0)) return 5.0e-3 * T;
+  if (lt_kpow(T, g.lp_t3a, rho, 1.0 / 15.0, 1.0f / 15.0f)) return 5.0e-3 * T;
   if (T < g.lp_t4) return 2.0e34 * pow(rho, 2.0 / 3.0) * pow(T, -9.0);
-  if (T < g.lp_t5a * pow(rho, 1.0 / 21.0)) return 2.0e-8 * pow(rho, 2.0 / 3.0) * T * T * T;
-  if (T < g.lp_t6a * pow(rho, 4.0 / 75.0)) return 1.0e-36 * pow(rho, 1.0 / 3.0) * pow(T, 10.0);
+  if (lt_kpow(T, g.lp_t5a, rho, 1.0 / 21.0, 1.0f / 21.0f)) return 2.0e-8 * pow(rho, 2.0 / 3.0) * T * T * T;
+  if (lt_kpow(T, g.lp_t6a, rho, 4.0 / 75.0, 4.0f / 75.0f)) return 1.0e-36 * pow(rho, 1.0 / 3.0) * pow(T, 10.0);
   return 1.5e20 * rho * pow(T, -5.0 / 2.0);
 }
 
@@ -3729,6 +3744,50 @@ __global__ void k_math_eval(int fn, const double *__restrict__ x, const double *
     default: r = sqrt(a); break;
   }
   out[k] = r;
+}
+
+// the context's opacity law (cgs in, cgs out), as the energy kernels call it
+__global__ void k_opacity_eval(Geo g, const double *__restrict__ rho, const double *__restrict__ T,
+                               double *__restrict__ out, int n) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) out[k] = opacity_law(g, rho[k], T[k]);
+}
+
+extern "C" int thorin_opacity_eval(thorin_ctx *c, const double *rho_cgs, const double *T_K, double *kappa_cgs,
+                                   size_t n) {
+  if (need_dev(c)) return -1;
+  if (!rho_cgs || !T_K || !kappa_cgs || n > (size_t)1 << 26) {
+    thorin_set_error("thorin_opacity_eval: needs rho, T, out and at most 2^26 values");
+    return -1;
+  }
+  if (n == 0) return 0;
+  thorin_dev *d = c->dev;
+  const size_t nb = n * sizeof(double);
+  double *dr = nullptr, *dt = nullptr, *dk = nullptr;
+  int rc = -1;
+  hipError_t e = hipMalloc((void **)&dr, nb);
+  if (e == hipSuccess) e = hipMalloc((void **)&dt, nb);
+  if (e == hipSuccess) e = hipMalloc((void **)&dk, nb);
+  if (e == hipSuccess) e = hipMemcpyAsync(dr, rho_cgs, nb, hipMemcpyHostToDevice, d->s);
+  if (e == hipSuccess) e = hipMemcpyAsync(dt, T_K, nb, hipMemcpyHostToDevice, d->s);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_opacity_eval, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, d->s, d->g, dr, dt, dk,
+                       (int)n);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(kappa_cgs, dk, nb, hipMemcpyDeviceToHost, d->s);
+  if (e == hipSuccess) e = hipStreamSynchronize(d->s);
+  if (e == hipSuccess) {
+    rc = 0;
+  } else {
+    char m[200];
+    snprintf(m, sizeof m, "thorin_opacity_eval: HIP error %s", hipGetErrorString(e));
+    thorin_set_error(m);
+  }
+  if (dr) (void)hipFree(dr);
+  if (dt) (void)hipFree(dt);
+  if (dk) (void)hipFree(dk);
+  return rc;
 }
 
 extern "C" int thorin_math_eval(int fn, const double *x, const double *y, double *out, size_t n) {

@@ -405,6 +405,11 @@ enum { THORIN_MATH_POW = 0, THORIN_MATH_EXP, THORIN_MATH_LOG, THORIN_MATH_LOG10,
        THORIN_MATH_SIN, THORIN_MATH_ATAN2, THORIN_MATH_ASIN, THORIN_MATH_ATAN, THORIN_MATH_SQRT,
        THORIN_MATH_N };
 int  thorin_math_eval(int fn, const double *x, const double *y, double *out, size_t n);
+/* The context's opacity law evaluated on the GPU as the energy kernels
+ * evaluate it (OpacityProfile, Opacity.c:231-262, called from EnergySources.c:237): kappa in
+ * cm^2/g from the volume density in g/cm^3 and T in K; host arrays of
+ * n <= 2^26 values.  No counterpart entry point in the reference. */
+int  thorin_opacity_eval(thorin_ctx *ctx, const double *rho_cgs, const double *T_K, double *kappa_cgs, size_t n);
 
 /* Synchronise the device and return the library's last error text. */
 int  thorin_sync(thorin_ctx *ctx);
