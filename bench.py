@@ -51,14 +51,21 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_damped_1024x1536.txt"
 FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
 
 
-def measured_traffic(kname):
-    """Bytes per launch of kname from the committed counter summary, or None."""
+def measured_traffic(kname, nit=None):
+    """Bytes per launch of kname from the committed counter summary, or None.
+    For k_sor_iter the row of the launch shape with nit iterations."""
     try:
         with open(TRAFFIC_FILE) as f:
             for line in f:
-                parts = line.split()
-                if len(parts) >= 3 and parts[0].split("<")[0] == kname:
-                    return FETCH_SCALE * float(parts[1]) * 1024 + WRITE_SCALE * float(parts[2]) * 1024
+                if line.startswith("#"):
+                    continue
+                name = line[:35].strip()
+                parts = line[35:].split()
+                if len(parts) < 2 or name.split("<")[0] != kname:
+                    continue
+                if nit is not None and not name.startswith(f"{kname}<{nit},"):
+                    continue
+                return FETCH_SCALE * float(parts[0]) * 1024 + WRITE_SCALE * float(parts[1]) * 1024
     except OSError:
         return None
     return None
@@ -573,7 +580,8 @@ def main():
     achieved = alg * work / (ms_k * 1e-3) / 1e9 if ms_k > 0 else 0.0
     swept = sor_swept_ev if kname == "k_sor_iter" else n_k
     achieved_swept = alg * swept / (ms_k * 1e-3) / 1e9 if ms_k > 0 else 0.0
-    traffic = measured_traffic(kname) if (nrad, nsec) == (1024, 1536) else None
+    traffic = (measured_traffic(kname, nit if kname == "k_sor_iter" else None)
+               if (nrad, nsec) == (1024, 1536) else None)
     step_bytes = (STEP_BYTES_PER_CELL * nrad * nsec +
                   SOR_BYTES_PER_CELL_ITER * (nrad - 2) * nsec * sor_iters / args.steps)
     out = {
