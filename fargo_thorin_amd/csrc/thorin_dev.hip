@@ -746,6 +746,10 @@ __device__ __forceinline__ double ordered_ring_sum(const double *sh, int ns) {
 // sector) and vr of rings i and i+1 with the same expressions as the owner
 // of each cell, so every value equals what the separate kernels give.
 #define EC_MAX 12              // cells per thread: NSEC <= RB * EC_MAX
+// The kernel runs at four waves per SIMD (at most 128 registers): with four
+// workgroups of four waves per CU, 1024 rings are all resident at once, and
+// the sequential ring sums of every ring run in one overlapped round rather
+// than two.
 #define EC_PAD 4
 struct VelSrc {                 // VEL: the momenta of the transport and the damping
   const double *rp, *rm, *tp, *tm;
@@ -940,7 +944,7 @@ __device__ __forceinline__ void b_eos_cfl(const Geo &g, const double *__restrict
   const VelSrc none = {nullptr, nullptr, nullptr, nullptr, 0.0, 0};
   b_eos_cfl_t<false>(g, rho, e, T, cs, const_cast<double *>(vr), const_cast<double *>(vt), do_eos, guard, none);
 }
-__global__ void __launch_bounds__(RB) k_eos_cfl(Geo g, const double *__restrict__ rho, const double *__restrict__ e,
+__global__ void __launch_bounds__(RB) __attribute__((amdgpu_waves_per_eu(4))) k_eos_cfl(Geo g, const double *__restrict__ rho, const double *__restrict__ e,
                                                 double *__restrict__ T, double *__restrict__ cs,
                                                 const double *__restrict__ vr, const double *__restrict__ vt,
                                                 int do_eos, const int *__restrict__ guard) {
@@ -952,7 +956,7 @@ __device__ __forceinline__ void b_eos_cfl_vel(const Geo &g, const double *__rest
                                               const int *__restrict__ guard, const VelSrc &vs) {
   b_eos_cfl_t<true>(g, rho, e, T, cs, vr, vt, 1, guard, vs);
 }
-__global__ void __launch_bounds__(RB) k_eos_cfl_vel(Geo g, const double *__restrict__ rho,
+__global__ void __launch_bounds__(RB) __attribute__((amdgpu_waves_per_eu(4))) k_eos_cfl_vel(Geo g, const double *__restrict__ rho,
                                                     const double *__restrict__ e, double *__restrict__ T,
                                                     double *__restrict__ cs, double *__restrict__ vr,
                                                     double *__restrict__ vt, const int *__restrict__ guard,
