@@ -203,7 +203,23 @@ def cpu_baseline_reference(nrad, nsec):
             "builds": res}
 
 
-def sor_study(nrad=2048, nsec=3072, train_s=25.0):
+def train_for(net, seconds, batch):
+    """Adam on the network's pairs for about `seconds`, the learning rate
+    stepped down over the run (1e-3, 3e-4, 1e-4, 3e-5 at 30, 60 and 85% of
+    it): (iterations, loss of the first batch's first pass, loss of the last
+    batch's last pass)"""
+    t0 = time.perf_counter()
+    iters, first, last = 0, None, 0.0
+    while time.perf_counter() - t0 < seconds:
+        fr = (time.perf_counter() - t0) / seconds
+        lr = 1e-3 if fr < 0.3 else 3e-4 if fr < 0.6 else 1e-4 if fr < 0.85 else 3e-5
+        lf, last = net.train(batch, lr)
+        first = lf if first is None else first
+        iters += batch
+    return iters, first, last
+
+
+def sor_study(nrad=2048, nsec=3072, train_s=25.0, coarsen=2):
     """The energy equation's implicit solve at nrad x nsec (BL94 opacity,
     T 20-2000 K, dt = the hydro CFL step), all to a 1e-10 relative FP64
     residual at omega = 1.8:
@@ -211,10 +227,13 @@ def sor_study(nrad=2048, nsec=3072, train_s=25.0):
       * FP32 red-black correction sweeps in FP64 iterative refinement, four
         sweeps per launch on LDS tiles; sweeps per round picked by timing 32
         and 64 once;
-      * the same from the learned warm start: the U-Net trained here on the
-        GPU (Adam, residual loss) for about train_s seconds on 12 pairs the
-        solver generates from other seeds and steps (0.5, 1, 2 x CFL dt);
-        the study disk itself is not among them.
+      * the same from the learned warm start: the U-Net on the grid
+        coarsened `coarsen` times, trained here on the GPU (Adam with a
+        stepped learning rate, residual loss of the interpolated guess) for
+        about train_s seconds on 12 pairs the solver generates from other
+        seeds and steps (0.5, 1, 2 x CFL dt); the study disk itself is not
+        among them.  Its time includes the network's inference and the
+        residual of its guess.
     Errors are stated against the FP64 SOR result."""
     import numpy as np
 
@@ -234,6 +253,7 @@ def sor_study(nrad=2048, nsec=3072, train_s=25.0):
     # training pairs (other disks) and the network
     t0 = time.perf_counter()
     net = WarmStart(seed=3)
+    net.set_coarsen(coarsen)
     net.set_loss(1)
     for seed in range(4):
         for mult in (0.5, 1.0, 2.0):
@@ -242,10 +262,7 @@ def sor_study(nrad=2048, nsec=3072, train_s=25.0):
             net.add_pair(d)
     t_pairs = time.perf_counter() - t0
     t1 = time.perf_counter()
-    iters, loss = 0, (0.0, 0.0)
-    while time.perf_counter() - t1 < train_s:
-        loss = net.train(48, 3e-4)
-        iters += 48
+    iters, *loss = train_for(net, train_s, 48)
     t_train = time.perf_counter() - t1
     # the study disk
     dt = assemble(12345)
@@ -271,6 +288,7 @@ def sor_study(nrad=2048, nsec=3072, train_s=25.0):
                                     "warm_used": warm["warm_used"],
                                     "guess_rel_resid": warm["resid_warm"] / warm["resid0"],
                                     "rel_resid": warm["resid"] / warm["resid0"], "rel_err_vs_fp64_sor": err(t_warm),
+                                    "coarsen": coarsen,
                                     "training": {"pairs": net.n_pairs(), "iters": iters, "seconds": t_train,
                                                  "pair_generation_s": t_pairs, "loss_first": loss[0],
                                                  "loss_last": loss[1], "loss": "residual of the guess"}},
@@ -402,14 +420,15 @@ def ensemble_study(n=16, steps=10, warmup=2, cnn=True):
     return out
 
 
-def ensemble_cnn_study(E, disks, planets, train_s=20.0, tol=1e-10, omega=1.8, inner=16):
+def ensemble_cnn_study(E, disks, planets, train_s=20.0, tol=1e-10, omega=1.8, inner=16, coarsen=2):
     """The learned warm start on the ensemble: the network is trained on the
     GPU on pairs that the ensemble's own disks give (the first half of the
     disks, at two times of the run: each pair is the system of the disk's
     substep and its solution to 1e-11), then the energy solve of every disk
     of the other half is timed to a 1e-10 relative residual three ways: the
     reference's FP64 SOR, FP32 sweeps in FP64 refinement, and the same from
-    the network's guess.  The solves use the disk's own substep dt."""
+    the network's guess (the network on the grid coarsened `coarsen`
+    times).  The solves use the disk's own substep dt."""
     from fargo_thorin_amd import WarmStart
     n = len(disks)
     train, evald = list(range(n // 2)), list(range(n // 2, n))
@@ -418,6 +437,7 @@ def ensemble_cnn_study(E, disks, planets, train_s=20.0, tol=1e-10, omega=1.8, in
         lg = E.logs[k]
         return lg.dt[0] if lg.nsub else 1e-3
     net = WarmStart(seed=5)
+    net.set_coarsen(coarsen)
     net.set_loss(1)
     t0 = time.perf_counter()
     for rnd in range(2):
@@ -431,10 +451,7 @@ def ensemble_cnn_study(E, disks, planets, train_s=20.0, tol=1e-10, omega=1.8, in
             net.add_pair(d)
     t_pairs = time.perf_counter() - t0
     t1 = time.perf_counter()
-    iters, loss = 0, (0.0, 0.0)
-    while time.perf_counter() - t1 < train_s:
-        loss = net.train(32, 3e-4)
-        iters += 32
+    iters, *loss = train_for(net, train_s, 32)
     t_train = time.perf_counter() - t1
     rows = []
     for k in evald:
@@ -460,7 +477,7 @@ def ensemble_cnn_study(E, disks, planets, train_s=20.0, tol=1e-10, omega=1.8, in
     net.close()
     return {"train_disks": len(train), "eval_disks": len(evald), "pairs": net_pairs, "pair_generation_s": t_pairs,
             "train_iters": iters, "train_s": t_train, "loss_first": loss[0], "loss_last": loss[1],
-            "tol": tol, "omega": omega, "inner_sweeps": inner,
+            "tol": tol, "omega": omega, "inner_sweeps": inner, "coarsen": coarsen,
             "ms_per_disk_step": {k: v / len(rows) for k, v in tot.items()},
             "speedup_vs_plain": {k: tot["plain_fp64"] / v for k, v in tot.items()},
             "disks": rows}

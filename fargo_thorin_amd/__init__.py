@@ -371,9 +371,12 @@ class WarmStart:
 
     def grad(self, x, t=None, aux=None):
         """forward + backward on host arrays: (loss, gradient, prediction);
-        with aux (5 x H x W) the residual loss"""
+        with aux (5 x H x W, the system's grid) the residual loss.  x, t and
+        the prediction are on the network's grid, H/f x W/f."""
         x = np.ascontiguousarray(x, dtype=np.float32)
-        H, W = x.shape[:2]
+        f = self.coarsen()
+        Hn, Wn = x.shape[:2]
+        H, W = Hn * f, Wn * f
         fp = C.POINTER(C.c_float)
         tp = ap = None
         if t is not None:
@@ -383,7 +386,7 @@ class WarmStart:
             aux = np.ascontiguousarray(aux, dtype=np.float32)
             ap = aux.ctypes.data_as(fp)
         g = np.empty(self.n_params(), dtype=np.float32)
-        y = np.empty((H, W), dtype=np.float32)
+        y = np.empty((Hn, Wn), dtype=np.float32)
         L = C.c_double()
         _check(lib().thorin_cnn_grad2(self._h, H, W, x.ctypes.data_as(fp), tp, ap, C.byref(L),
                                       g.ctypes.data_as(fp), y.ctypes.data_as(fp)), "thorin_cnn_grad")
@@ -392,6 +395,14 @@ class WarmStart:
     def set_loss(self, mode: int) -> None:
         """0: error of the normalised correction; 1: residual of the guess"""
         _check(lib().thorin_cnn_set_loss(self._h, mode), "set_loss")
+
+    def set_coarsen(self, f: int) -> None:
+        """run the network on the system's grid coarsened f = 1, 2 or 4 times
+        (before the first pair)"""
+        _check(lib().thorin_cnn_set_coarsen(self._h, f), "set_coarsen")
+
+    def coarsen(self) -> int:
+        return int(lib().thorin_cnn_coarsen(self._h))
 
     def add_pair(self, disk: Disk) -> None:
         _check(lib().thorin_cnn_add_pair(self._h, disk._h), "thorin_cnn_add_pair")

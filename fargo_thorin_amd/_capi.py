@@ -149,6 +149,8 @@ def lib() -> C.CDLL:
         "thorin_cnn_grad2": (ip, [vp, ip, ip, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float), dp,
                                   C.POINTER(C.c_float), C.POINTER(C.c_float)]),
         "thorin_cnn_set_loss": (ip, [vp, ip]),
+        "thorin_cnn_set_coarsen": (ip, [vp, ip]),
+        "thorin_cnn_coarsen": (ip, [vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -20,6 +20,15 @@
 //   y  = conv(a8, 32->16), channel 0                           level 0
 // Radial padding is zero, azimuthal padding periodic (the polar grid).
 //
+// Coarse network (thorin_cnn_set_coarsen, f = 2 or 4): the features are
+// formed on the NRAD x NSEC grid and averaged over f x f cells, the U-Net
+// runs on the NRAD/f x NSEC/f grid, and its output is interpolated back
+// bilinearly (cell centres; clamped across the rings, periodic along the
+// sectors).  The residual loss is taken on the full grid and its gradient
+// reaches the network through the transpose of that interpolation.  The
+// smooth part of the correction is what the FP32 sweeps remove slowest, and
+// the network then costs about 1/f^2 as much.
+//
 // Arithmetic: activations and weights in BF16 (NHWC, 16 or 32 channels per
 // pixel), convolutions as implicit GEMMs on v_mfma_f32_16x16x16_bf16 with
 // FP32 accumulation, FP32 master weights and Adam.  Every activation (and
@@ -629,8 +638,9 @@ __global__ void k_target(const double *__restrict__ Tsol, const double *__restri
 //   rho_l = -f0_l + tau (y_l + wip_l y_{l+W} + wim_l y_{l-W} + wjp_l y_{l+1} + wjm_l y_{l-1})
 // with f0 = -(A.T0 - RHS)/(Ml T0 s) and w_nb = (M_nb/Ml)(T0_nb/T0_l) (aux
 // planes 0..4); loss = sum rho^2 / n over rings 1..H-2.
-__global__ void __launch_bounds__(256) k_rloss(const float *__restrict__ y, const float *__restrict__ aux, float tau,
-                                               int H, int W, float *__restrict__ rho, double *__restrict__ part) {
+__global__ void __launch_bounds__(256) k_rloss(const float *__restrict__ y, int ys, const float *__restrict__ aux,
+                                               float tau, int H, int W, float *__restrict__ rho,
+                                               double *__restrict__ part) {
   __shared__ double red[256];
   const long n = (long)H * W;
   const double inv = 1.0 / ((double)(H - 2) * W);
@@ -640,10 +650,10 @@ __global__ void __launch_bounds__(256) k_rloss(const float *__restrict__ y, cons
     float r = 0.f;
     if (i >= 1 && i <= H - 2) {
       const long ljp = (j == W - 1) ? l - j : l + 1, ljm = (j == 0) ? l + W - 1 : l - 1;
-      const float yip = (i + 1 <= H - 2) ? y[(l + W) * COUTL] : 0.f;
-      const float yim = (i - 1 >= 1) ? y[(l - W) * COUTL] : 0.f;
-      const float a = y[l * COUTL] + aux[n + l] * yip + aux[2 * n + l] * yim + aux[3 * n + l] * y[ljp * COUTL] +
-                      aux[4 * n + l] * y[ljm * COUTL];
+      const float yip = (i + 1 <= H - 2) ? y[(l + W) * ys] : 0.f;
+      const float yim = (i - 1 >= 1) ? y[(l - W) * ys] : 0.f;
+      const float a = y[l * ys] + aux[n + l] * yip + aux[2 * n + l] * yim + aux[3 * n + l] * y[ljp * ys] +
+                      aux[4 * n + l] * y[ljm * ys];
       r = -aux[l] + tau * a;
       s += (double)r * r;
     }
@@ -659,8 +669,9 @@ __global__ void __launch_bounds__(256) k_rloss(const float *__restrict__ y, cons
 }
 // its gradient: dy_m = 2 tau / n (rho_m + wip_{m-W} rho_{m-W} + wim_{m+W} rho_{m+W}
 //                                 + wjp_{m-1} rho_{m-1} + wjm_{m+1} rho_{m+1}) on rings 1..H-2
+// (into dy as BF16 channel 0 of COUTL, or with gf != NULL into the float plane gf)
 __global__ void k_rgrad(const float *__restrict__ rho, const float *__restrict__ aux, float tau, int H, int W,
-                        bf16_t *__restrict__ dy) {
+                        bf16_t *__restrict__ dy, float *__restrict__ gf) {
   const long n = (long)H * W;
   const float c = (float)(2.0 / ((double)(H - 2) * W)) * tau;
   for (long m = (long)blockIdx.x * blockDim.x + threadIdx.x; m < n; m += (long)gridDim.x * blockDim.x) {
@@ -671,6 +682,10 @@ __global__ void k_rgrad(const float *__restrict__ rho, const float *__restrict__
       g = rho[m] + aux[n + m - W] * rho[m - W] + aux[2 * n + m + W] * rho[m + W] + aux[3 * n + mjm] * rho[mjm] +
           aux[4 * n + mjp] * rho[mjp];
       g *= c;
+    }
+    if (gf) {
+      gf[m] = g;
+      continue;
     }
     *(s4 *)(dy + m * COUTL) = (s4){(short)f2bf(g), 0, 0, 0};
     *(s4 *)(dy + m * COUTL + 4) = (s4){0, 0, 0, 0};
@@ -728,6 +743,151 @@ __global__ void k_apply(const float *__restrict__ y, float tscale, const double 
 }
 
 // ---------------------------------------------------------------------------
+// Coarse network: restriction of the features and of the target (f x f
+// means), bilinear interpolation of the output and its transpose.
+
+// fine index i of an axis -> the two coarse cells whose centres bracket the
+// fine cell's centre, u = (i + 0.5)/f - 0.5 in coarse units, and the weight
+// w of the second (exact in FP32 for f = 1, 2, 4); across the rings the cells
+// are clamped to 0..nc-1, along the sectors taken modulo nc
+__device__ __forceinline__ void bil_axis(int i, int f, int nc, bool periodic, int &c0, int &c1, float &w) {
+  const float u = ((float)i + 0.5f) / (float)f - 0.5f;
+  const float fl = floorf(u);
+  w = u - fl;
+  int a = (int)fl, b = a + 1;
+  if (periodic) {
+    a = (a + nc) % nc;
+    b = b % nc;
+  } else {
+    a = min(max(a, 0), nc - 1);
+    b = min(max(b, 0), nc - 1);
+  }
+  c0 = a;
+  c1 = b;
+}
+// bilinear value at fine cell (i, j) of the coarse plane y (stride ys)
+__device__ __forceinline__ float bil_at(const float *__restrict__ y, int ys, int i, int j, int f, int Hc, int Wc) {
+  int i0, i1, j0, j1;
+  float wi, wj;
+  bil_axis(i, f, Hc, false, i0, i1, wi);
+  bil_axis(j, f, Wc, true, j0, j1, wj);
+  const float r0 = (1.f - wj) * y[((size_t)i0 * Wc + j0) * ys] + wj * y[((size_t)i0 * Wc + j1) * ys];
+  const float r1 = (1.f - wj) * y[((size_t)i1 * Wc + j0) * ys] + wj * y[((size_t)i1 * Wc + j1) * ys];
+  return (1.f - wi) * r0 + wi * r1;
+}
+// features averaged over the f x f fine cells of each coarse cell (the
+// fine features as k_feat2 forms them; 0 on rings 0 and H-1)
+__global__ void k_feat2c(const float *__restrict__ jac, const double *__restrict__ Ml, const double *__restrict__ Mlip,
+                         const double *__restrict__ Mlim, const double *__restrict__ Mljp,
+                         const double *__restrict__ Mljm, int H, int W, int f, const double *__restrict__ sc,
+                         bf16_t *__restrict__ x) {
+  const int Hc = H / f, Wc = W / f;
+  const long n = (long)Hc * Wc;
+  const float is = (float)(1.0 / sc[0]);
+  const float inv = 1.f / (float)(f * f);
+  for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (long)gridDim.x * blockDim.x) {
+    const int I = (int)(p / Wc), J = (int)(p % Wc);
+    float acc[NFEAT];
+    for (int c = 0; c < NFEAT; c++) acc[c] = 0.f;
+    for (int di = 0; di < f; di++) {
+      const int i = I * f + di;
+      if (i < 1 || i > H - 2) continue;
+      for (int dj = 0; dj < f; dj++) {
+        const long l = (long)i * W + J * f + dj;
+        const double ml = Ml[l];
+        acc[0] += jac[l] * is;
+        acc[1] += (float)(4.0 * Mlip[l] / ml);
+        acc[2] += (float)(4.0 * Mlim[l] / ml);
+        acc[3] += (float)(4.0 * Mljp[l] / ml);
+        acc[4] += (float)(4.0 * Mljm[l] / ml);
+        acc[5] += (float)log(ml);
+      }
+    }
+    s4 o0, o1;
+    for (int k = 0; k < 4; k++) o0[k] = (short)f2bf(acc[k] * inv);
+    o1[0] = (short)f2bf(acc[4] * inv);
+    o1[1] = (short)f2bf(acc[5] * inv);
+    o1[2] = 0;
+    o1[3] = 0;
+    *(s4 *)(x + p * CIN0) = o0;
+    *(s4 *)(x + p * CIN0 + 4) = o1;
+    *(s4 *)(x + p * CIN0 + 8) = (s4){0, 0, 0, 0};
+    *(s4 *)(x + p * CIN0 + 12) = (s4){0, 0, 0, 0};
+  }
+}
+// target (T* - T0)/(T0 s) averaged over the f x f fine cells (0 on rings 0, H-1)
+__global__ void k_target_c(const double *__restrict__ Tsol, const double *__restrict__ T0, int H, int W, int f,
+                           const double *__restrict__ sc, float *__restrict__ t) {
+  const int Hc = H / f, Wc = W / f;
+  const long n = (long)Hc * Wc;
+  const double s = sc[0];
+  for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (long)gridDim.x * blockDim.x) {
+    const int I = (int)(p / Wc), J = (int)(p % Wc);
+    double acc = 0.0;
+    for (int di = 0; di < f; di++) {
+      const int i = I * f + di;
+      if (i < 1 || i > H - 2) continue;
+      for (int dj = 0; dj < f; dj++) {
+        const long l = (long)i * W + J * f + dj;
+        acc += (Tsol[l] - T0[l]) / (T0[l] * s);
+      }
+    }
+    t[p] = (float)(acc / (double)(f * f));
+  }
+}
+// yf = bilinear interpolation of channel 0 of y (Hc x Wc, COUTL channels) on the H x W grid
+__global__ void k_upy(const float *__restrict__ y, int f, int H, int W, float *__restrict__ yf) {
+  const int Hc = H / f, Wc = W / f;
+  const long n = (long)H * W;
+  for (long l = (long)blockIdx.x * blockDim.x + threadIdx.x; l < n; l += (long)gridDim.x * blockDim.x)
+    yf[l] = bil_at(y, COUTL, (int)(l / W), (int)(l % W), f, Hc, Wc);
+}
+// transpose of k_upy: dy[p] (channel 0, BF16) = sum over the fine cells of
+// their interpolation weight on coarse cell p times g; the other channels 0
+__global__ void k_upy_adj(const float *__restrict__ g, int f, int H, int W, bf16_t *__restrict__ dy) {
+  const int Hc = H / f, Wc = W / f;
+  const long n = (long)Hc * Wc;
+  for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (long)gridDim.x * blockDim.x) {
+    const int I = (int)(p / Wc), J = (int)(p % Wc);
+    float acc = 0.f;
+    for (int di = -f; di < 2 * f; di++) {
+      const int i = I * f + di;
+      if (i < 0 || i >= H) continue;
+      int a0, a1;
+      float wi;
+      bil_axis(i, f, Hc, false, a0, a1, wi);
+      const float wr = (a0 == I ? 1.f - wi : 0.f) + (a1 == I ? wi : 0.f);
+      if (wr == 0.f) continue;
+      for (int dj = -f; dj < 2 * f; dj++) {
+        const int j = (J * f + dj + W) % W;
+        int b0, b1;
+        float wj;
+        bil_axis(j, f, Wc, true, b0, b1, wj);
+        const float wc = (b0 == J ? 1.f - wj : 0.f) + (b1 == J ? wj : 0.f);
+        if (wc == 0.f) continue;
+        acc += (wr * wc) * g[(size_t)i * W + j];
+      }
+    }
+    *(s4 *)(dy + p * COUTL) = (s4){(short)f2bf(acc), 0, 0, 0};
+    *(s4 *)(dy + p * COUTL + 4) = (s4){0, 0, 0, 0};
+    *(s4 *)(dy + p * COUTL + 8) = (s4){0, 0, 0, 0};
+    *(s4 *)(dy + p * COUTL + 12) = (s4){0, 0, 0, 0};
+  }
+}
+// warm start of the coarse network: Tg = T0 + up(y) tscale T0 s on rings 1..H-2
+__global__ void k_apply_up(const float *__restrict__ y, float tscale, const double *__restrict__ T0, int H, int W,
+                           int f, const double *__restrict__ sc, double *__restrict__ Tg) {
+  const int Hc = H / f, Wc = W / f;
+  const long n = (long)H * W;
+  const double s = sc[0] * (double)tscale;
+  for (long l = (long)blockIdx.x * blockDim.x + threadIdx.x; l < n; l += (long)gridDim.x * blockDim.x) {
+    const int i = (int)(l / W), j = (int)(l % W);
+    const double t0 = T0[l];
+    Tg[l] = (i >= 1 && i <= H - 2) ? t0 + (double)bil_at(y, COUTL, i, j, f, Hc, Wc) * t0 * s : t0;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // host side
 struct Layer {
   int cin, cout, lvl;
@@ -735,10 +895,10 @@ struct Layer {
 };
 
 struct Sample {
-  int H, W;
-  bf16_t *x;
-  float *t;
-  float *aux;                 // residual-loss planes (f0, wip, wim, wjp, wjm)
+  int H, W;                   // the system's grid (the network's is H/f x W/f)
+  bf16_t *x;                  // features on the network's grid
+  float *t;                   // correction target on the network's grid
+  float *aux;                 // residual-loss planes (f0, wip, wim, wjp, wjm) on the system's grid
 };
 
 struct thorin_cnn {
@@ -751,8 +911,12 @@ struct thorin_cnn {
   float tscale;               // RMS of the training targets (standardisation), 1 before any pair
   double tss;                 // running sum of the pairs' mean squared targets
   float tinv;                 // 1/tscale during training, 1 for thorin_cnn_grad
-  int H, W;                   // activation grid of the current pass (0: none)
+  int f;                      // coarsening of the network's grid (1, 2 or 4)
+  int H, W;                   // network grid of the current pass (0: none)
+  int HF, WF;                 // system grid of the current pass (H = HF/f, W = WF/f)
   size_t cap;                 // pixels the activation buffers hold (grids up to that size)
+  size_t capf;                // pixels the system-grid planes hold
+  float *yf, *gf;             // f > 1: interpolated output and loss gradient on the system grid
   bf16_t *x, *a[9], *p1, *p2, *p3, *u0, *u1, *u2, *dy, *dz[9], *du0, *du1, *du2, *dp1, *dp2, *dp3;
   float *y, *t, *jac, *wpart, *bpart, *aux, *rho;
   int lossmode;               // 0: mean square of the correction error; 1: of the guess' residual
@@ -779,27 +943,34 @@ static void free_acts(thorin_cnn *n) {
     if (n->a[k]) { (void)hipFree(n->a[k]); n->a[k] = nullptr; }
     if (n->dz[k]) { (void)hipFree(n->dz[k]); n->dz[k] = nullptr; }
   }
-  float **fs[] = {&n->y, &n->t, &n->jac, &n->wpart, &n->bpart, &n->aux, &n->rho};
+  float **fs[] = {&n->y, &n->t, &n->jac, &n->wpart, &n->bpart, &n->aux, &n->rho, &n->yf, &n->gf};
   for (auto p : fs)
     if (*p) { (void)hipFree(*p); *p = nullptr; }
   double **ds[] = {&n->lpart, &n->fpart};
   for (auto p : ds)
     if (*p) { (void)hipFree(*p); *p = nullptr; }
-  n->H = n->W = 0;
-  n->cap = 0;
+  n->H = n->W = n->HF = n->WF = 0;
+  n->cap = n->capf = 0;
 }
 
-// activation buffers for an H x W grid; they grow to the largest grid met
-// (the pairs of an ensemble have several sizes) and are reused for smaller
-static int alloc_acts(thorin_cnn *n, int H, int W) {
-  if (H % 8 || W % 8 || H < 16 || W < 16) {
-    thorin_set_error("the warm-start network needs NRAD and NSEC divisible by 8 (three 2x2 poolings), >= 16");
+// buffers for a system grid HF x WF: activations on the network's grid
+// HF/f x WF/f, the features' and the loss's planes on the system grid; they
+// grow to the largest grid met (the pairs of an ensemble have several sizes)
+// and are reused for smaller ones
+static int alloc_acts(thorin_cnn *n, int HF, int WF) {
+  const int f = n->f;
+  if (HF % f || WF % f || (HF / f) % 8 || (WF / f) % 8 || HF / f < 16 || WF / f < 16) {
+    thorin_set_errorf("the warm-start network needs NRAD/%d and NSEC/%d divisible by 8 (three 2x2 poolings) "
+                      "and >= 16; the grid is %dx%d", f, f, HF, WF);
     return -1;
   }
-  const size_t px0 = (size_t)H * W;
-  if (px0 <= n->cap) {
+  const int H = HF / f, W = WF / f;
+  const size_t px0 = (size_t)H * W, pxf = (size_t)HF * WF;
+  if (px0 <= n->cap && pxf <= n->capf) {
     n->H = H;
     n->W = W;
+    n->HF = HF;
+    n->WF = WF;
     return 0;
   }
   free_acts(n);
@@ -815,9 +986,13 @@ static int alloc_acts(thorin_cnn *n, int H, int W) {
   if (A(&n->dy, px0 * COUTL)) return -1;
   HCK(hipMalloc(&n->y, px0 * COUTL * sizeof(float)));
   HCK(hipMalloc(&n->t, px0 * sizeof(float)));
-  HCK(hipMalloc(&n->jac, px0 * sizeof(float)));
-  HCK(hipMalloc(&n->aux, 5 * px0 * sizeof(float)));
-  HCK(hipMalloc(&n->rho, px0 * sizeof(float)));
+  HCK(hipMalloc(&n->jac, pxf * sizeof(float)));
+  HCK(hipMalloc(&n->aux, 5 * pxf * sizeof(float)));
+  HCK(hipMalloc(&n->rho, pxf * sizeof(float)));
+  if (f > 1) {
+    HCK(hipMalloc(&n->yf, pxf * sizeof(float)));
+    HCK(hipMalloc(&n->gf, pxf * sizeof(float)));
+  }
   n->nwblk = 512;
   HCK(hipMalloc(&n->wpart, (size_t)n->nwblk * 9 * 4 * 256 * sizeof(float)));
   HCK(hipMalloc(&n->bpart, (size_t)8192 * WID * sizeof(float)));
@@ -825,7 +1000,10 @@ static int alloc_acts(thorin_cnn *n, int H, int W) {
   HCK(hipMalloc(&n->fpart, 8192 * sizeof(double)));
   n->H = H;
   n->W = W;
+  n->HF = HF;
+  n->WF = WF;
   n->cap = px0;
+  n->capf = pxf;
   return 0;
 }
 
@@ -934,16 +1112,30 @@ static int backward(thorin_cnn *n, double *loss) {
   const int H = n->H, W = n->W;
   auto Hl = [&](int l) { return H >> l; };
   auto Wl = [&](int l) { return W >> l; };
-  const int lb = grid_for((long)H * W) < 8192 ? grid_for((long)H * W) : 8192;
+  // the residual loss of a coarse network is taken on the system grid
+  const long nl = (n->lossmode == 1 && n->f > 1) ? (long)n->HF * n->WF : (long)H * W;
+  const int lb = grid_for(nl) < 8192 ? grid_for(nl) : 8192;
   if (n->lossmode == 0) {
     hipLaunchKernelGGL(k_loss, dim3(lb), dim3(256), 0, n->s, (const float *)n->y, (const float *)n->t, n->tinv, H, W,
                        n->dy, n->lpart);
-  } else {
+  } else if (n->f == 1) {
     const float tau = 1.0f / n->tinv;
-    hipLaunchKernelGGL(k_rloss, dim3(lb), dim3(256), 0, n->s, (const float *)n->y, (const float *)n->aux, tau, H, W,
-                       n->rho, n->lpart);
+    hipLaunchKernelGGL(k_rloss, dim3(lb), dim3(256), 0, n->s, (const float *)n->y, COUTL, (const float *)n->aux, tau,
+                       H, W, n->rho, n->lpart);
     hipLaunchKernelGGL(k_rgrad, dim3(grid_for((long)H * W)), dim3(256), 0, n->s, (const float *)n->rho,
-                       (const float *)n->aux, tau, H, W, n->dy);
+                       (const float *)n->aux, tau, H, W, n->dy, (float *)nullptr);
+  } else {
+    // y -> yf (bilinear), residual loss and its gradient gf on the system
+    // grid, then dy = transpose of the interpolation applied to gf
+    const float tau = 1.0f / n->tinv;
+    const int HF = n->HF, WF = n->WF;
+    hipLaunchKernelGGL(k_upy, dim3(grid_for(nl)), dim3(256), 0, n->s, (const float *)n->y, n->f, HF, WF, n->yf);
+    hipLaunchKernelGGL(k_rloss, dim3(lb), dim3(256), 0, n->s, (const float *)n->yf, 1, (const float *)n->aux, tau,
+                       HF, WF, n->rho, n->lpart);
+    hipLaunchKernelGGL(k_rgrad, dim3(grid_for(nl)), dim3(256), 0, n->s, (const float *)n->rho,
+                       (const float *)n->aux, tau, HF, WF, (bf16_t *)nullptr, n->gf);
+    hipLaunchKernelGGL(k_upy_adj, dim3(grid_for((long)H * W)), dim3(256), 0, n->s, (const float *)n->gf, n->f, HF,
+                       WF, n->dy);
   }
   // layer 9 (32 -> 16): weights from (a8, dy); input gradient masked by a8 > 0
   if (wgrad(n, 8, n->a[8], n->dy, H, W)) return -1;
@@ -1004,8 +1196,12 @@ static int features(thorin_cnn *n, const thorin_sysptrs &sp) {
   hipLaunchKernelGGL(k_feat1, dim3(nb), dim3(256), 0, n->s, sp.T, sp.ml, sp.mlip, sp.mlim, sp.mljp, sp.mljm, sp.rhs,
                      H, W, n->jac, n->fpart);
   hipLaunchKernelGGL(k_scale, dim3(1), dim3(256), 0, n->s, (const double *)n->fpart, nb, (long)(H - 2) * W, n->sc);
-  hipLaunchKernelGGL(k_feat2, dim3(grid_for(npx)), dim3(256), 0, n->s, (const float *)n->jac, sp.ml, sp.mlip, sp.mlim,
-                     sp.mljp, sp.mljm, H, W, (const double *)n->sc, n->x);
+  if (n->f == 1)
+    hipLaunchKernelGGL(k_feat2, dim3(grid_for(npx)), dim3(256), 0, n->s, (const float *)n->jac, sp.ml, sp.mlip,
+                       sp.mlim, sp.mljp, sp.mljm, H, W, (const double *)n->sc, n->x);
+  else
+    hipLaunchKernelGGL(k_feat2c, dim3(grid_for((long)n->H * n->W)), dim3(256), 0, n->s, (const float *)n->jac, sp.ml,
+                       sp.mlip, sp.mlim, sp.mljp, sp.mljm, H, W, n->f, (const double *)n->sc, n->x);
   HCK(hipGetLastError());
   return 0;
 }
@@ -1023,8 +1219,10 @@ extern "C" int thorin_cnn_create(unsigned long long seed, thorin_cnn **out) {
   n->y = n->t = n->jac = n->wpart = n->bpart = n->aux = n->rho = nullptr;
   n->lossmode = 0;
   n->lpart = n->fpart = nullptr;
-  n->H = n->W = 0;
-  n->cap = 0;
+  n->f = 1;
+  n->yf = n->gf = nullptr;
+  n->H = n->W = n->HF = n->WF = 0;
+  n->cap = n->capf = 0;
   n->adam_t = 0;
   n->tscale = 1.0f;
   n->tss = 0.0;
@@ -1136,12 +1334,12 @@ extern "C" int thorin_cnn_grad2(thorin_cnn *n, int H, int W, const float *x, con
                                 double *loss, float *grad, float *y) {
   if (!n || !x || (!t && !aux)) { thorin_set_error("thorin_cnn_grad: null argument"); return -1; }
   if (alloc_acts(n, H, W)) return -1;
-  const size_t npx = (size_t)H * W;
+  const size_t npx = (size_t)n->H * n->W;        // the network's grid
   std::vector<bf16_t> xb(npx * CIN0);
   for (size_t e = 0; e < xb.size(); e++) xb[e] = h_f2bf(x[e]);
   HCK(hipMemcpyAsync(n->x, xb.data(), xb.size() * sizeof(bf16_t), hipMemcpyHostToDevice, n->s));
   if (t) HCK(hipMemcpyAsync(n->t, t, npx * sizeof(float), hipMemcpyHostToDevice, n->s));
-  if (aux) HCK(hipMemcpyAsync(n->aux, aux, 5 * npx * sizeof(float), hipMemcpyHostToDevice, n->s));
+  if (aux) HCK(hipMemcpyAsync(n->aux, aux, 5 * (size_t)H * W * sizeof(float), hipMemcpyHostToDevice, n->s));
   n->tinv = 1.0f;
   const int keep = n->lossmode;
   n->lossmode = aux ? 1 : 0;
@@ -1173,15 +1371,20 @@ extern "C" int thorin_cnn_add_pair(thorin_cnn *n, thorin_ctx *c) {
   Sample sm;
   sm.H = sp.nr;
   sm.W = sp.ns;
-  const size_t npx = (size_t)sp.nr * sp.ns;
+  const size_t npf = (size_t)sp.nr * sp.ns;      // the system's grid
+  const size_t npx = (size_t)n->H * n->W;        // the network's grid
   HCK(hipMalloc(&sm.x, npx * CIN0 * sizeof(bf16_t)));
   HCK(hipMalloc(&sm.t, npx * sizeof(float)));
-  HCK(hipMalloc(&sm.aux, 5 * npx * sizeof(float)));
-  hipLaunchKernelGGL(k_aux, dim3(grid_for((long)npx)), dim3(256), 0, n->s, (const float *)n->jac, sp.T0, sp.ml, sp.mlip,
+  HCK(hipMalloc(&sm.aux, 5 * npf * sizeof(float)));
+  hipLaunchKernelGGL(k_aux, dim3(grid_for((long)npf)), dim3(256), 0, n->s, (const float *)n->jac, sp.T0, sp.ml, sp.mlip,
                      sp.mlim, sp.mljp, sp.mljm, sp.nr, sp.ns, (const double *)n->sc, sm.aux);
   HCK(hipMemcpyAsync(sm.x, n->x, npx * CIN0 * sizeof(bf16_t), hipMemcpyDeviceToDevice, n->s));
-  hipLaunchKernelGGL(k_target, dim3(grid_for((long)npx)), dim3(256), 0, n->s, sp.T, sp.T0, sp.nr, sp.ns,
-                     (const double *)n->sc, sm.t);
+  if (n->f == 1)
+    hipLaunchKernelGGL(k_target, dim3(grid_for((long)npx)), dim3(256), 0, n->s, sp.T, sp.T0, sp.nr, sp.ns,
+                       (const double *)n->sc, sm.t);
+  else
+    hipLaunchKernelGGL(k_target_c, dim3(grid_for((long)npx)), dim3(256), 0, n->s, sp.T, sp.T0, sp.nr, sp.ns, n->f,
+                       (const double *)n->sc, sm.t);
   // running RMS of the targets
   const int nb = grid_for((long)npx) < 8192 ? grid_for((long)npx) : 8192;
   hipLaunchKernelGGL(k_sumsq, dim3(nb), dim3(256), 0, n->s, (const float *)sm.t, (long)npx, n->lpart);
@@ -1192,7 +1395,7 @@ extern "C" int thorin_cnn_add_pair(thorin_cnn *n, thorin_ctx *c) {
   double ss = 0.0;
   for (int b = 0; b < nb; b++) ss += part[b];
   n->samples.push_back(sm);
-  n->tss += ss / ((double)(sm.H - 2) * sm.W);
+  n->tss += ss / ((double)(n->H - 2) * n->W);
   const double rms = sqrt(n->tss / (double)n->samples.size());
   n->tscale = rms > 0.0 ? (float)rms : 1.0f;
   return 0;
@@ -1218,10 +1421,10 @@ extern "C" int thorin_cnn_train(thorin_cnn *n, int iters, float lr, double *loss
   for (int it = 0; it < iters; it++) {
     const Sample &sm = n->samples[it % ns];
     if (alloc_acts(n, sm.H, sm.W)) return -1;
-    const size_t npx = (size_t)n->H * n->W;
+    const size_t npx = (size_t)n->H * n->W, npf = (size_t)sm.H * sm.W;
     HCK(hipMemcpyAsync(n->x, sm.x, npx * CIN0 * sizeof(bf16_t), hipMemcpyDeviceToDevice, n->s));
     HCK(hipMemcpyAsync(n->t, sm.t, npx * sizeof(float), hipMemcpyDeviceToDevice, n->s));
-    if (n->lossmode) HCK(hipMemcpyAsync(n->aux, sm.aux, 5 * npx * sizeof(float), hipMemcpyDeviceToDevice, n->s));
+    if (n->lossmode) HCK(hipMemcpyAsync(n->aux, sm.aux, 5 * npf * sizeof(float), hipMemcpyDeviceToDevice, n->s));
     if (forward(n)) return -1;
     const bool want = it < ns || it >= iters - ns;
     double L = 0.0;
@@ -1252,8 +1455,12 @@ extern "C" int thorin_cnn_predict_dev(thorin_cnn *n, thorin_ctx *c, double *Tg) 
   if (!rc) rc = forward(n);
   if (!rc) {
     const long npx = (long)sp.nr * sp.ns;
-    hipLaunchKernelGGL(k_apply, dim3(grid_for(npx)), dim3(256), 0, n->s, (const float *)n->y, n->tscale, sp.T, sp.nr,
-                       sp.ns, (const double *)n->sc, Tg);
+    if (n->f == 1)
+      hipLaunchKernelGGL(k_apply, dim3(grid_for(npx)), dim3(256), 0, n->s, (const float *)n->y, n->tscale, sp.T,
+                         sp.nr, sp.ns, (const double *)n->sc, Tg);
+    else
+      hipLaunchKernelGGL(k_apply_up, dim3(grid_for(npx)), dim3(256), 0, n->s, (const float *)n->y, n->tscale, sp.T,
+                         sp.nr, sp.ns, n->f, (const double *)n->sc, Tg);
     if (hipGetLastError() != hipSuccess) { thorin_set_error("k_apply launch failed"); rc = -1; }
   }
   n->s = keep;
@@ -1267,3 +1474,20 @@ extern "C" int thorin_cnn_set_loss(thorin_cnn *n, int mode) {
   n->lossmode = mode;
   return 0;
 }
+
+// the network's grid: the system's grid coarsened f = 1, 2 or 4 times along
+// both axes; set before the first pair (pairs keep features of that grid)
+extern "C" int thorin_cnn_set_coarsen(thorin_cnn *n, int f) {
+  if (!n || (f != 1 && f != 2 && f != 4)) { thorin_set_error("thorin_cnn_set_coarsen: f is 1, 2 or 4"); return -1; }
+  if (f == n->f) return 0;
+  if (!n->samples.empty()) {
+    thorin_set_error("thorin_cnn_set_coarsen: the network already holds pairs of another grid");
+    return -1;
+  }
+  HCK(hipStreamSynchronize(n->s));
+  free_acts(n);
+  n->f = f;
+  return 0;
+}
+
+extern "C" int thorin_cnn_coarsen(const thorin_cnn *n) { return n ? n->f : 0; }

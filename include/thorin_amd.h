@@ -395,6 +395,16 @@ int    thorin_cnn_grad2(thorin_cnn *n, int H, int W, const float *x, const float
 /* training objective: 0 = mean square error of the normalised correction
  * (default); 1 = mean square of the normalised residual of the guess */
 int    thorin_cnn_set_loss(thorin_cnn *n, int mode);
+/* the network's grid: the system's NRAD x NSEC coarsened f = 1 (default), 2
+ * or 4 times along both axes.  With f > 1 the features are averaged over
+ * f x f cells, the output is interpolated bilinearly back to the system's
+ * grid (cell centres; clamped across the rings, periodic along the sectors)
+ * and the residual loss is taken there.  NRAD/f and NSEC/f must be divisible
+ * by 8.  Set before the first pair.  In thorin_cnn_grad / _grad2, H and W
+ * are then the system's grid: x and t hold (H/f)(W/f) pixels, aux 5 H W, and
+ * the prediction comes back on the network's grid. */
+int    thorin_cnn_set_coarsen(thorin_cnn *n, int f);
+int    thorin_cnn_coarsen(const thorin_cnn *n);
 
 /* ---- the device math library, for checking it.  No counterpart in the
  * reference (whose host builds call glibc's).  out[i] = f(x[i]), or

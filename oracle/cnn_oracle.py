@@ -17,6 +17,11 @@ rings, periodic padding along the sectors), NHWC float arrays:
     y  = conv(a8, 32->16)[..., 0]                              level 0
     loss = mean over rings 1..H-2 of (y - t)^2
 
+Coarse network (f = 2 or 4): x and the prediction y live on the H/f x W/f
+grid; y is interpolated bilinearly to the H x W grid (cell centres, clamped
+across the rings, periodic along the sectors), where the residual loss is
+taken, and its gradient returns through the transpose of the interpolation.
+
 With bf16=True every stored activation, activation gradient, the input and
 the weights used by the convolutions are rounded to BF16 (round to nearest
 even) where the GPU stores them; sums are formed in float64.  With
@@ -125,10 +130,56 @@ def residual_grad(rho, aux):
     return g
 
 
-def forward_backward(params, x, t, bf16=True, aux=None):
+def block_mean(a, f):
+    """mean over f x f cells (the coarse network's restriction)"""
+    H, W = a.shape[:2]
+    return a.reshape(H // f, f, W // f, f, *a.shape[2:]).mean(axis=(1, 3))
+
+
+def bilinear_axis(n, f, nc, periodic):
+    """for the n fine cells of an axis: the coarse cells c0, c1 whose centres
+    bracket the fine cell's centre u = (i + 0.5)/f - 0.5 and the weight w of
+    c1; clamped to 0..nc-1 or taken modulo nc"""
+    u = (np.arange(n) + 0.5) / f - 0.5
+    c0 = np.floor(u).astype(int)
+    w = u - c0
+    c1 = c0 + 1
+    if periodic:
+        c0, c1 = c0 % nc, c1 % nc
+    else:
+        c0, c1 = np.clip(c0, 0, nc - 1), np.clip(c1, 0, nc - 1)
+    return c0, c1, w
+
+
+def upsample(yc, f):
+    """bilinear interpolation of the coarse plane yc to the f-times finer grid"""
+    Hc, Wc = yc.shape
+    i0, i1, wi = bilinear_axis(Hc * f, f, Hc, False)
+    j0, j1, wj = bilinear_axis(Wc * f, f, Wc, True)
+    r0 = yc[i0][:, j0] * (1 - wj) + yc[i0][:, j1] * wj
+    r1 = yc[i1][:, j0] * (1 - wj) + yc[i1][:, j1] * wj
+    return r0 * (1 - wi)[:, None] + r1 * wi[:, None]
+
+
+def upsample_adj(g, f):
+    """transpose of upsample: each fine value spread back to its four coarse
+    cells with its interpolation weights"""
+    H, W = g.shape
+    Hc, Wc = H // f, W // f
+    i0, i1, wi = bilinear_axis(H, f, Hc, False)
+    j0, j1, wj = bilinear_axis(W, f, Wc, True)
+    out = np.zeros((Hc, Wc))
+    for ci, wr in ((i0, 1 - wi), (i1, wi)):
+        for cj, wc in ((j0, 1 - wj), (j1, wj)):
+            np.add.at(out, (ci[:, None], cj[None, :]), wr[:, None] * wc[None, :] * g)
+    return out
+
+
+def forward_backward(params, x, t, bf16=True, aux=None, f=1):
     """loss, flat gradient, prediction y (H x W) and the activations; with
     aux the residual loss mean(rho^2) over rings 1..H-2 replaces the
-    correction loss"""
+    correction loss; with f > 1 (and aux on the f-times finer grid) the
+    residual loss is that of the interpolated prediction"""
     r = to_bf16 if bf16 else (lambda a: np.asarray(a, dtype=np.float64))
     relu = lambda a: np.maximum(a, 0.0)
     L = unpack(params)
@@ -159,10 +210,16 @@ def forward_backward(params, x, t, bf16=True, aux=None):
         d[1:H - 1] = y[1:H - 1] - t[1:H - 1]
         loss = float(np.sum(d * d) / n)
         dy[..., 0] = r(2.0 * d / n)
-    else:
+    elif f == 1:
         rho = residual(y, aux)
         loss = float(np.sum(rho * rho) / n)
         dy[..., 0] = r(2.0 * residual_grad(rho, aux) / n)
+    else:
+        yf = upsample(y, f)
+        nf = (H * f - 2) * W * f
+        rho = residual(yf, aux)
+        loss = float(np.sum(rho * rho) / nf)
+        dy[..., 0] = r(upsample_adj(2.0 * residual_grad(rho, aux) / nf, f))
     g = [None] * 9
     g[8] = conv_weight_grad(a[8], dy)
     dz8 = r(conv_input_grad(dy, Wq[8]) * (a[8] > 0))
@@ -191,6 +248,6 @@ def forward_backward(params, x, t, bf16=True, aux=None):
     return loss, flat, y, a
 
 
-def loss_only(params, x, t, aux=None):
+def loss_only(params, x, t, aux=None, f=1):
     """float64 loss without rounding (for finite differences)"""
-    return forward_backward(params, x, t, bf16=False, aux=aux)[0]
+    return forward_backward(params, x, t, bf16=False, aux=aux, f=f)[0]
