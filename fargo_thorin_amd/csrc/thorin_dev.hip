@@ -4292,7 +4292,10 @@ static int sor_launch(thorin_ctx *c, double omega, int it0, int &count, double e
            d->sorp, d->sorc, d->ctl, it, (const double *)d->mresp, d->nb_mresp, nbs, maxiter, eps);              \
     LAUNCH(k_sor_check<N>, dim3(1), dim3(RB), 0, d->sorp, d->sorc, nbs, d->ctl, maxiter, it, eps);               \
   } while (0)
-  // single iterations after the first launch carry their own stop test
+  // single iterations that follow multi-iteration launches (the short tail of
+  // the queue, most of which return at once) carry their own stop test in the
+  // last workgroup; its device-scope fence per workgroup costs more than a
+  // separate check where single iterations are the whole queue
 #define SOR_LAUNCH1C()                                                                                            \
   LAUNCH((k_sor_iter<1, true>), gs, dim3(TB), 0, g, d->b[B_TEMPER], d->b[B_TEMPER2], d->b[B_TEMPER3],             \
          d->b[B_TEMPER4], d->b[B_ML], d->b[B_MLIP], d->b[B_MLIM], d->b[B_MLJP], d->b[B_MLJM], d->b[B_RHS], omega,   \
@@ -4301,7 +4304,7 @@ static int sor_launch(thorin_ctx *c, double omega, int it0, int &count, double e
     const int nit = (it0 + count - it >= nmax) ? nmax : 1;
     if (nit == 3) SOR_LAUNCH(3);
     else if (nit == 2) SOR_LAUNCH(2);
-    else if (it > 0) SOR_LAUNCH1C();
+    else if (it > 0 && nmax > 1) SOR_LAUNCH1C();
     else SOR_LAUNCH(1);
     it += nit;
   }
