@@ -24,6 +24,7 @@
 #include <string>
 #include <thread>
 #include <tuple>
+#include <type_traits>
 #include <unordered_map>
 #include <utility>
 #include <vector>
@@ -211,6 +212,7 @@ struct thorin_dev {
   int niter_hist[4], nsolve;    // SOR iteration counts of the last four solves
   int rs_rows;                  // rings per workgroup of k_ring_means (LDS holds them whole)
   int sor_nit;                  // SOR iterations per launch (1, 2 or 3; thorin_tune)
+  int visc_chain;               // SubStep1 .. SubStep2 as one kernel where the options allow it
   int sor_queue_min;            // ensemble: SOR iterations to queue at least
   ForceSet spec;                // force kernels enqueued ahead for the next substep
   int heat_max;                 // heatsrc_max: set by the pebble accretion when AccretionalHeating is on
@@ -242,7 +244,7 @@ static const char *const KNAMES[] = {
     "k_divstress", "k_visc_update", "k_substep2",
     "k_matrix", "k_sor_resid0", "k_sor_iter", "k_sor_check", "k_energy_new", "k_transport_rad",
     "k_transport_theta", "k_velocities", "k_substep3", "k_peb_cfl", "k_peb_damp", "k_peb_source",
-    "k_peb_substep1", "k_peb_accrete", "k_peb_evaporate", "k_ring_means", "k_eos_cfl_vel", NULL};
+    "k_peb_substep1", "k_peb_accrete", "k_peb_evaporate", "k_ring_means", "k_eos_cfl_vel", "k_visc_chain", NULL};
 
 // a template instance ("k_name<3>") counts as its kernel
 static int kid_of(const char *name) {
@@ -493,6 +495,30 @@ __device__ __forceinline__ double block_sum(double v, double *sh) {
   for (int k = 1; k < N / 64; k++) r += sh[k];
   __syncthreads();
   return r;
+}
+
+// M block sums at once, each added in the order block_sum forms it (the
+// butterfly in each wave, then the waves in order): one barrier pair for all
+template <int N, int M>
+__device__ __forceinline__ void block_sums(double (&v)[M], double *sh) {
+#pragma unroll
+  for (int m = 0; m < M; m++)
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v[m] += __shfl_xor(v[m], o, 64);
+  if (N == 64) return;
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0)
+#pragma unroll
+    for (int m = 0; m < M; m++) sh[m * (N / 64) + w] = v[m];
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < M; m++) {
+    double r = sh[m * (N / 64)];
+#pragma unroll
+    for (int k = 1; k < N / 64; k++) r += sh[m * (N / 64) + k];
+    v[m] = r;
+  }
+  __syncthreads();
 }
 
 template <int N>
@@ -2077,7 +2103,19 @@ struct CrashSnap {
   int peb;
   const double *src[4];
   double *dst[4];
+  // taken by SubStep3 after the fused SubStep1-2 kernel (k_visc_chain): src[1]
+  // is vrad before the damping boundary, applied again here where damp is set
+  int damp;
+  double step;
 };
+
+// the damping boundary on vrad alone (k_boundary without DampInit and the
+// flux-based damping: the target vrad is 0)
+__device__ __forceinline__ double damp_vrad_zero(const Geo &g, int i, double vr, double step) {
+  const double vrad0 = 0.0;
+  const double lambda = g.WaveKiller[i] * step;
+  return (vr + lambda * vrad0) / (1.0 + lambda);
+}
 
 __device__ __forceinline__ void b_divstress(const Geo &g, const double *__restrict__ vr,
                                                   const double *__restrict__ vt,
@@ -2197,6 +2235,338 @@ __global__ void __launch_bounds__(TB) k_substep2(Geo g, const double *__restrict
 }
 
 
+// ---------------------------------------------------------------------------
+// SubStep1, ViscousTerms and SubStep2 in one pass, for the gas alone with a
+// damping boundary that acts on vrad only (none, or DAMPING without DampInit
+// and without the flux-based target): k_boundary, k_substep1, k_divstress,
+// k_visc_update, k_substep2 and k_boundary again, fused.  A workgroup owns a
+// tile of VC_TI rings x VC_TJ sectors and stages rho, energy, vtheta and the
+// accelerations (the indirect term added as SubStep1 adds it) of the tile
+// and a 4-cell halo in LDS.  It then forms, each on a region one cell
+// smaller than the last: VradInt and VthetaInt (SubStep1, the vrad of the
+// damped rings damped first), the stress tensor, the viscous update with
+// ImposeKeplerianEdges, and SubStep2 of its own cells, whose vrad is damped
+// as the second k_boundary does.  Every value is formed by the expression of
+// the separate kernels, so a halo cell holds what its owner computes.  Rings
+// 0 and NRAD of VradInt, which SubStep1 does not write, come from the VradInt
+// buffer as the separate kernels read them there.  The stress tensor,
+// VradInt and VthetaInt stay in LDS.  DetectCrash's flags are set here; the
+// crash snapshot is taken by SubStep3 (CrashSnap.damp).
+#define VC_TI 16
+#define VC_TJ 48
+#define VC_T 512                  // threads per workgroup
+#define VC_LR (VC_TI + 7)         // rings ib .. ib+VC_LR-1, ib = i0 - 4
+#define VC_LW (VC_TJ + 8)         // sectors jb .. jb+VC_LW-1, jb = j0 - 4
+#define VC_N (VC_LR * VC_LW)
+
+struct ViscChain {
+  const double *rho, *energy, *vrad, *vtheta, *agr, *agt;
+  const double *vradint;        // VradInt: rings 0 and NRAD
+  double *vradnew, *vthetanew, *energyint;
+  DevCtl *ctl;
+  double dt;
+  int damp;                     // the damping boundary acts on vrad (rings with dampring set)
+  int fold_ind;                 // add the indirect term to the accelerations
+  double ind_px, ind_py;
+  int kepler;                   // ImposeKeplerianEdges
+  double vkin, vkout;
+};
+
+// ring constants of the tile's rings, held in LDS (index: row of the tile)
+enum {
+  VR_RINF, VR_RSUP, VR_RMED, VR_IRMED, VR_IRINF, VR_IDRMED, VR_IDRSUP, VR_IDXT, VR_SUPP, VR_VMED, VR_VINF, VR_LAMBDA,
+  VR_N
+};
+
+// divergence and stress of the cell at LDS index k (ring i, tile row r), as
+// stress_cell forms them from the global fields
+__device__ __forceinline__ Stress stress_lds(const Geo &g, const double *vr, const double *vt, const double *rho,
+                                             const double (*rc)[VC_LR], int i, int r, int k) {
+  const int kip = k + VC_LW, kjp = k + 1, kjm = k - 1;
+  const double invdphi = g.invdphi, onethird = 1.0 / 3.0;
+  const double irmed = rc[VR_IRMED][r], idrsup = rc[VR_IDRSUP][r];
+  const double Drr = (vr[kip] - vr[k]) * idrsup;
+  const double Dpp = (vt[kjp] - vt[k]) * invdphi * irmed + 0.5 * (vr[kip] + vr[k]) * irmed;
+  double dv = (vr[kip] * rc[VR_RSUP][r] - vr[k] * rc[VR_RINF][r]) * idrsup * irmed;
+  dv += (vt[kjp] - vt[k]) * invdphi * irmed;
+  Stress st;
+  st.dv = dv;
+  const double visc = rc[VR_VMED][r];
+  st.trr = 2.0 * rho[k] * visc * (Drr - onethird * dv);
+  st.tpp = 2.0 * rho[k] * visc * (Dpp - onethird * dv);
+  st.has_trp = i >= 1;
+  st.trp = 0.0;
+  if (i >= 1) {
+    const int kim = k - VC_LW, kjmim = kjm - VC_LW;
+    const double Drp = 0.5 * (rc[VR_RINF][r] * (vt[k] * irmed - vt[kim] * rc[VR_IRMED][r - 1]) * rc[VR_IDRMED][r] +
+                              (vr[k] - vr[kjm]) * invdphi * rc[VR_IRINF][r]);
+    const double vinf = rc[VR_VINF][r];
+    st.trp = 2.0 * 0.25 * (rho[k] + rho[kim] + rho[kjm] + rho[kjmim]) * vinf * Drp;
+  }
+  return st;
+}
+
+__device__ __forceinline__ void b_visc_chain(const Geo &g, const ViscChain &v) {
+  // six field arrays over the tile and its halo: rho throughout; the others
+  // hold energy, vtheta, agr, agt for SubStep1, then VradInt (in the slot
+  // nothing uses before), VthetaInt (energy's), Trp (vtheta's), Trr (agr's)
+  // and Tpp (agt's)
+  __shared__ double s_rho[VC_N], s_a1[VC_N], s_a2[VC_N], s_a3[VC_N], s_a4[VC_N], s_a5[VC_N];
+  __shared__ double s_rc[VR_N][VC_LR];
+  __shared__ double s_cos[VC_LW], s_sin[VC_LW];
+  const int ns = g.ns, nr = g.nr, tid = threadIdx.x;
+  const int ntx = (ns + VC_TJ - 1) / VC_TJ;
+  int tx, ty, bid;
+  if (!xcd_tile(ntx, (nr + VC_TI - 1) / VC_TI, tx, ty, bid, g.blk0)) return;
+  const int i0 = ty * VC_TI, j0 = tx * VC_TJ;
+  const int tw = min(VC_TJ, ns - j0);
+  const int ib = i0 - 4, jb = j0 - 4;
+  const double dt = v.dt, gm1 = g.adiabind - 1.0;
+  auto wrapj = [ns](int j) {
+    while (j < 0) j += ns;
+    while (j >= ns) j -= ns;
+    return j;
+  };
+  // 1. loads: the ring and sector constants first, then every field value of
+  // the thread (all in flight together)
+  double rcv[VR_N];
+  if (tid < VC_LR) {
+    const int gi = ib + tid;
+#pragma unroll
+    for (int q = 0; q < VR_N; q++) rcv[q] = 0.0;
+    if (gi >= 0 && gi < nr) {
+      rcv[VR_RINF] = g.Rinf[gi];
+      rcv[VR_RSUP] = g.Rsup[gi];
+      rcv[VR_RMED] = g.Rmed[gi];
+      rcv[VR_IRMED] = g.InvRmed[gi];
+      rcv[VR_IRINF] = g.InvRinf[gi];
+      rcv[VR_IDRMED] = g.InvDiffRmed[gi];
+      rcv[VR_IDRSUP] = g.InvDiffRsup[gi];
+      rcv[VR_IDXT] = g.InvDxTheta[gi];
+      rcv[VR_SUPP] = g.supp_torque[gi];
+      rcv[VR_VMED] = fvisc_med(g, gi);
+      rcv[VR_VINF] = fvisc_inf(g, gi);
+      rcv[VR_LAMBDA] = (v.damp && g.dampring[gi]) ? g.WaveKiller[gi] * dt : -1.0;
+    }
+  }
+  double csj = 0.0, snj = 0.0;
+  const int tc = tid - 64;                          // the second wave loads the sector constants
+  if (tc >= 0 && tc < VC_LW) {
+    const int j = wrapj(jb + tc);
+    csj = g.cosj[j];
+    snj = g.sinj[j];
+  }
+  constexpr int N0 = (VC_N + VC_T - 1) / VC_T;
+  constexpr int W1 = VC_LW - 2, N1C = W1 * (VC_LR - 1), N1 = (N1C + VC_T - 1) / VC_T;   // SubStep1's region
+  constexpr int N4 = (VC_TI * VC_TJ + VC_T - 1) / VC_T;                               // owned cells
+  double x0[N0], x1[N0], x2[N0], x3[N0], x4[N0], vr0[N1], e4[N4];
+#pragma unroll
+  for (int q = 0; q < N0; q++) {
+    const int idx = tid + q * VC_T;
+    const int r = idx / VC_LW, c = idx - r * VC_LW, gi = ib + r;
+    x0[q] = x1[q] = x2[q] = x3[q] = x4[q] = 0.0;
+    if (idx < VC_N && gi >= 0 && gi < nr) {
+      const size_t l = (size_t)gi * ns + wrapj(jb + c);
+      x0[q] = v.rho[l];
+      x1[q] = v.energy[l];
+      x2[q] = v.vtheta[l];
+      x3[q] = v.agr[l];
+      x4[q] = v.agt[l];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < N1; q++) {
+    const int idx = tid + q * VC_T;
+    const int r = 1 + idx / W1, c = 1 + idx % W1, gi = ib + r;
+    vr0[q] = 0.0;
+    if (idx < N1C && gi >= 0 && gi <= nr) {
+      const size_t l = (size_t)gi * ns + wrapj(jb + c);
+      vr0[q] = (gi >= 1 && gi < nr) ? v.vrad[l] : v.vradint[l];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < N4; q++) {
+    const int idx = tid + q * VC_T;
+    const int r = 4 + idx / VC_TJ, c = idx % VC_TJ, gi = ib + r;
+    e4[q] = 0.0;
+    if (idx < VC_TI * VC_TJ && gi < nr && c < tw) e4[q] = v.energy[(size_t)gi * ns + j0 + c];
+  }
+  if (tid < VC_LR)
+#pragma unroll
+    for (int q = 0; q < VR_N; q++) s_rc[q][tid] = rcv[q];
+  if (tc >= 0 && tc < VC_LW) {
+    s_cos[tc] = csj;
+    s_sin[tc] = snj;
+  }
+  __syncthreads();
+  // fields into LDS, the accelerations with the indirect term as SubStep1
+  // adds it to each value it reads
+  double axindir = 0.0, ayindir = 0.0;
+  if (v.fold_ind) {
+    axindir = -v.ctl->axstar;
+    ayindir = -v.ctl->aystar;
+    if (g.indirect_term) { axindir += v.ind_px; ayindir += v.ind_py; }
+  }
+#pragma unroll
+  for (int q = 0; q < N0; q++) {
+    const int idx = tid + q * VC_T;
+    if (idx >= VC_N) continue;
+    const int r = idx / VC_LW, c = idx - r * VC_LW;
+    double a = x3[q], t = x4[q];
+    if (v.fold_ind) {
+      const double rm = s_rc[VR_RMED][r];
+      const double x = rm * s_cos[c], y = rm * s_sin[c];
+      a += (x * axindir + y * ayindir) * s_rc[VR_IRMED][r];
+      t += (x * ayindir - y * axindir) * s_rc[VR_IRMED][r];
+    }
+    s_rho[idx] = x0[q];
+    s_a1[idx] = x1[q];
+    s_a2[idx] = x2[q];
+    s_a3[idx] = a;
+    s_a4[idx] = t;
+  }
+  __syncthreads();
+  // 2. SubStep1 (SourceEuler.c:319-406) on rows 1 .. VC_LR-1, columns 1 ..
+  // VC_LW-2, after the damping boundary on vrad; VradInt into s_a5,
+  // VthetaInt held until energy's array is free
+  {
+    const double *s_e = s_a1, *s_vt = s_a2, *s_ar = s_a3, *s_at = s_a4;
+    double vti[N1];
+#pragma unroll
+    for (int q = 0; q < N1; q++) {
+      const int idx = tid + q * VC_T;
+      const int r = 1 + idx / W1, c = 1 + idx % W1, gi = ib + r;
+      vti[q] = 0.0;
+      if (idx >= N1C || gi < 0 || gi > nr) continue;
+      const int k = r * VC_LW + c;
+      if (gi == 0 || gi == nr) {
+        s_a5[k] = vr0[q];
+      } else {
+        const int km = k - VC_LW;
+        double vr = vr0[q];
+        const double lambda = s_rc[VR_LAMBDA][r];
+        if (lambda >= 0.0) {
+          const double vrad0 = 0.0;
+          vr = (vr + lambda * vrad0) / (1.0 + lambda);
+        }
+        const double pl = gm1 * s_e[k], plim = gm1 * s_e[km];
+        double gradp = (pl - plim) * 2.0 / (s_rho[k] + s_rho[km]) * s_rc[VR_IDRMED][r];
+        double vt2 = s_vt[k] + s_vt[k + 1] + s_vt[km] + s_vt[km + 1];
+        vt2 = vt2 / 4.0 + s_rc[VR_RINF][r] * g.omega_frame;
+        vt2 = vt2 * vt2;
+        const double tmp = -gradp + 0.5 * (s_ar[k] + s_ar[km]) + vt2 * s_rc[VR_IRINF][r];
+        s_a5[k] = vr + dt * tmp;
+      }
+      if (gi < nr) {
+        const double pl = gm1 * s_e[k], pljm = gm1 * s_e[k - 1];
+        const double gradp = (pl - pljm) * 2.0 / (s_rho[k] + s_rho[k - 1]) * s_rc[VR_IDXT][r];
+        const double tmp = -gradp + 0.5 * (s_at[k] + s_at[k - 1]);
+        double vv = s_vt[k] + dt * tmp;
+        vv += dt * s_rc[VR_SUPP][r];
+        vti[q] = vv;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < N1; q++) {
+      const int idx = tid + q * VC_T;
+      const int r = 1 + idx / W1, c = 1 + idx % W1, gi = ib + r;
+      if (idx < N1C && gi >= 0 && gi < nr) s_a1[r * VC_LW + c] = vti[q];
+    }
+  }
+  __syncthreads();
+  double *s_vri = s_a5, *s_vti = s_a1, *s_trp = s_a2, *s_trr = s_a3, *s_tpp = s_a4;
+  // 3. stress tensor (Viscosity.c:114-160) on rows 2 .. VC_LR-2, columns 2 .. VC_LW-3
+  {
+    constexpr int W2 = VC_LW - 4, N2C = W2 * (VC_LR - 3), N2 = (N2C + VC_T - 1) / VC_T;
+#pragma unroll
+    for (int q = 0; q < N2; q++) {
+      const int idx = tid + q * VC_T;
+      const int r = 2 + idx / W2, c = 2 + idx % W2, gi = ib + r;
+      if (idx >= N2C || gi < 0 || gi >= nr) continue;
+      const int k = r * VC_LW + c;
+      const Stress st = stress_lds(g, s_vri, s_vti, s_rho, s_rc, gi, r, k);
+      s_trr[k] = st.trr;
+      s_tpp[k] = st.tpp;
+      if (st.has_trp) s_trp[k] = st.trp;
+    }
+  }
+  __syncthreads();
+  // 4. UpdateVelocityWithViscousTerms (Viscosity.c:165-218) + ImposeKeplerianEdges
+  // on rows 3 .. VC_LR-3, columns 3 .. VC_LW-4, in place
+  {
+    constexpr int W3 = VC_LW - 6, N3C = W3 * (VC_LR - 5), N3 = (N3C + VC_T - 1) / VC_T;
+    const double invdphi = g.invdphi, DeltaT = dt;
+#pragma unroll
+    for (int q = 0; q < N3; q++) {
+      const int idx = tid + q * VC_T;
+      const int r = 3 + idx / W3, c = 3 + idx % W3, gi = ib + r;
+      if (idx >= N3C || gi < 0 || gi >= nr) continue;
+      const int k = r * VC_LW + c, kip = k + VC_LW, kim = k - VC_LW;
+      if (gi >= 1 && gi < nr - 1)
+        s_vti[k] += DeltaT * s_rc[VR_IRMED][r] *
+                    ((s_rc[VR_RSUP][r] * s_trp[kip] - s_rc[VR_RINF][r] * s_trp[k]) * s_rc[VR_IDRSUP][r] +
+                     (s_tpp[k] - s_tpp[k - 1]) * invdphi + 0.5 * (s_trp[k] + s_trp[kip])) /
+                    (0.5 * (s_rho[k] + s_rho[k - 1]));
+      if (gi >= 1)
+        s_vri[k] += DeltaT * s_rc[VR_IRINF][r] *
+                    ((s_rc[VR_RMED][r] * s_trr[k] - s_rc[VR_RMED][r - 1] * s_trr[kim]) * s_rc[VR_IDRMED][r] +
+                     (s_trp[k + 1] - s_trp[k]) * invdphi - 0.5 * (s_tpp[k] + s_tpp[kim])) /
+                    (0.5 * (s_rho[k] + s_rho[kim]));
+      if (v.kepler) {
+        if (gi == 0) s_vti[k] = v.vkin - g.Rmed[0] * g.omega_frame;
+        if (gi == nr - 1) s_vti[k] = v.vkout - g.Rmed[nr - 1] * g.omega_frame;
+      }
+    }
+  }
+  __syncthreads();
+  // 5. SubStep2 (SourceEuler.c:417-504) of the owned cells, DetectCrash's
+  // flags, and the damping boundary on the new vrad
+  {
+    const double *vr = s_vri, *vt = s_vti;
+    int ce = 0, cr = 0;
+#pragma unroll
+    for (int q = 0; q < N4; q++) {
+      const int idx = tid + q * VC_T;
+      const int r = 4 + idx / VC_TJ, c = 4 + idx % VC_TJ, gi = ib + r;
+      if (idx >= VC_TI * VC_TJ || gi >= nr || c - 4 >= tw) continue;
+      const int k = r * VC_LW + c, kip = k + VC_LW, kim = k - VC_LW;
+      const int j = j0 + c - 4;
+      const size_t l = (size_t)gi * ns + j;
+      const double el = e4[q];
+      if (el < 0.0) ce = 1;
+      if (s_rho[k] < 0.0) cr = 1;
+      const double dvr = vr[kip] - vr[k];
+      const double dvt = vt[k + 1] - vt[k];
+      const double qr = qvnr(s_rho[k], dvr);
+      const double qt = qvnr(s_rho[k], dvt);
+      double vrn;
+      if (gi >= 1) {
+        const double qrim = qvnr(s_rho[kim], vr[k] - vr[kim]);
+        vrn = vr[k] - dt * 2.0 / (s_rho[k] + s_rho[kim]) * (qr - qrim) * s_rc[VR_IDRMED][r];
+      } else {
+        vrn = 0.0;
+        v.vradnew[(size_t)nr * ns + j] = 0.0;
+        v.vthetanew[(size_t)nr * ns + j] = 0.0;
+      }
+      const double invdxtheta = s_rc[VR_IDXT][r];
+      const double qtjm = qvnr(s_rho[k - 1], vt[k] - vt[k - 1]);
+      v.vthetanew[l] = vt[k] - dt * 2.0 / (s_rho[k] + s_rho[k - 1]) * (qt - qtjm) * invdxtheta;
+      v.energyint[l] = el - dt * qr * (vr[kip] - vr[k]) * s_rc[VR_IDRSUP][r] -
+                       dt * qt * (vt[k + 1] - vt[k]) * invdxtheta;
+      const double lambda = s_rc[VR_LAMBDA][r];
+      if (lambda >= 0.0) {
+        const double vrad0 = 0.0;
+        vrn = (vrn + lambda * vrad0) / (1.0 + lambda);
+      }
+      v.vradnew[l] = vrn;
+    }
+    if (ce) v.ctl->crash_e = 1;
+    if (cr) v.ctl->crash_rho = 1;
+  }
+}
+__global__ void __launch_bounds__(VC_T) k_visc_chain(Geo g, ViscChain v) { b_visc_chain(g, v); }
+
 __device__ __forceinline__ double flux_limiter(double s) {   // EnergySources.c:684-696
   double tmp;
   if (s <= 2.0) { tmp = 3.0 + sqrt(9.0 + 10.0 * s * s); return 2.0 / tmp; }
@@ -2237,7 +2607,8 @@ __device__ __forceinline__ void b_substep3(const Geo &g, const double *__restric
                                                  double *__restrict__ qplus, double *__restrict__ T,
                                                  double *__restrict__ cs, double *__restrict__ opa,
                                                  double *__restrict__ qminus, double *__restrict__ D,
-                                                 double *__restrict__ pmin, int fill_opacity, int nsum) {
+                                                 double *__restrict__ pmin, int fill_opacity, int nsum,
+                                                 const CrashSnap &sn) {
   __shared__ double red[TB];
   const int ns = g.ns, nr = g.nr;
   if ((int)blockIdx.x - g.blk0 < nsum) {
@@ -2284,6 +2655,16 @@ __device__ __forceinline__ void b_substep3(const Geo &g, const double *__restric
     const double gam = g.adiabind;
     const double tl = temper_of(g, eint, rho, l);
     const double cl = sqrt(gam * (gam - 1.0) * fabs(eint[l]) / rl);
+    if (sn.ctl && sn.ctl->crash_e) {
+      // the first crash's snapshot after the fused SubStep1-2 kernel: rho,
+      // vrad with the damping boundary, vtheta, and T before it is formed here
+      double vrs = sn.src[1][l];
+      if (sn.damp && g.dampring[i]) vrs = damp_vrad_zero(g, i, vrs, sn.step);
+      sn.dst[0][l] = sn.src[0][l];
+      sn.dst[1][l] = vrs;
+      sn.dst[2][l] = sn.src[2][l];
+      sn.dst[3][l] = sn.src[3][l];
+    }
     T[l] = tl;
     cs[l] = cl;
     // opacity and Q-
@@ -2338,8 +2719,9 @@ __global__ void __launch_bounds__(TB) k_substep3(Geo g, const double *__restrict
                                                  double *__restrict__ qplus, double *__restrict__ T,
                                                  double *__restrict__ cs, double *__restrict__ opa,
                                                  double *__restrict__ qminus, double *__restrict__ D,
-                                                 double *__restrict__ pmin, int fill_opacity, int nsum) {
-  b_substep3(g, vr, vt, rho, eint, divv, Trr, Trp, Tpp, qplus, T, cs, opa, qminus, D, pmin, fill_opacity, nsum);
+                                                 double *__restrict__ pmin, int fill_opacity, int nsum,
+                                                 CrashSnap sn) {
+  b_substep3(g, vr, vt, rho, eint, divv, Trr, Trp, Tpp, qplus, T, cs, opa, qminus, D, pmin, fill_opacity, nsum, sn);
 }
 
 
@@ -2482,7 +2864,7 @@ __global__ void __launch_bounds__(TB) k_sor_resid0(Geo g, const double *__restri
 
 
 // NIT red-black SOR iterations (EnergySources.c:455-493) per launch.  A
-// workgroup owns a tile of SOR_TI rings x SOR_TJ sectors and stages T with a
+// workgroup owns a tile of SR_TI rings x SR_TJ sectors and stages T with a
 // halo of 2*NIT cells in LDS.  Half-sweep s (red for even s, black for odd s)
 // is applied to the cells of its colour within 2*NIT-1-s cells of the tile:
 // a halo cell is updated with the same inputs and arithmetic as its owner
@@ -2496,12 +2878,21 @@ __global__ void __launch_bounds__(TB) k_sor_resid0(Geo g, const double *__restri
 // stop-test partials of each iteration are summed over the owned cells in a
 // fixed order (through LDS), the same for every NIT, so the iteration count
 // does not depend on how many iterations a launch runs.
+//
+// A square tile keeps the halo small (a 32x32 tile with a 6-cell halo
+// updates 1.35 cells per owned cell over three iterations, an 8x128 one
+// 1.70).  Each thread holds the matrix rows of its cells of either colour
+// in registers.  A colour's cells are numbered owned cells first, then the
+// rings of cells at Chebyshev distance 1, 2, ... from the tile, so that the
+// cells a half-sweep updates (distance <= its halo) are a prefix of the
+// numbering: the waves past that prefix skip the half-sweep as a whole.
+#define SR_TI 32
+#define SR_TJ 32
+// the FP32 one-sweep tiles (k_mp_sweep) keep their own shape
 #define SOR_TI 8
 #define SOR_TJ 128
-// threads per workgroup of k_sor_iter: the three-iteration launch holds 10
-// cells' matrix rows per thread at 256 threads, which leaves two waves per
-// SIMD; at 512 threads (6 rows each, 128 registers) two workgroups of eight
-// waves share each CU
+// threads per workgroup of k_sor_iter: 512 for three iterations per launch
+// (two cells of each colour per thread), 256 otherwise
 #define SOR_NT(nit) ((nit) == 3 ? 2 * TB : TB)
 #define SOR_LW (SOR_TJ + 4)
 // cells per thread of the one-sweep tiles (FP32 sweeps): red with a one-cell halo, black owned
@@ -2512,6 +2903,7 @@ __global__ void __launch_bounds__(TB) k_sor_resid0(Geo g, const double *__restri
 // staged, so that all global loads of the workgroup are in flight together
 struct SorCell {
   double ml, mip, mim, mjp, mjm, rhs;
+  int gl;   // offset of the cell in the field (written for owned cells)
   // packed: bits 0-11 the LDS index of the cell, 12-14 its distance from
   // the owned tile (0: owned), 16-25 its index among the owned cells (owned
   // only); -1: no cell
@@ -2520,8 +2912,9 @@ struct SorCell {
   __device__ __forceinline__ int d() const { return (pk >> 12) & 7; }
   __device__ __forceinline__ int o() const { return (pk >> 16) & 0x3FF; }
 };
-static_assert((SOR_TI + 12) * (SOR_TJ + 12) <= 0xFFF + 1 && SOR_TI * SOR_TJ <= 0x3FF + 1,
+static_assert((SR_TI + 12) * (SR_TJ + 12) <= 0xFFF + 1 && SR_TI * SR_TJ <= 0x3FF + 1,
               "SorCell packs the LDS index in 12 bits and the owned index in 10");
+static_assert(SR_TI % 2 == 0 && SR_TJ % 2 == 0, "the slot counts of a colour assume even tile sides");
 
 __device__ __forceinline__ void sor_load(SorCell &x, size_t l, const double *__restrict__ Ml,
                                          const double *__restrict__ Mlip, const double *__restrict__ Mlim,
@@ -2535,8 +2928,12 @@ __device__ __forceinline__ void sor_load(SorCell &x, size_t l, const double *__r
   x.rhs = RHS[l];
 }
 
+// one cell of a half-sweep; an owned cell's new value is also its iterate in
+// Tout (no later half-sweep of the iteration changes it) and its stop-test
+// terms go to sacc / schg
 template <int LW>
-__device__ __forceinline__ void sor_update(const SorCell &x, double *t, double omega, double *sacc, int *schg) {
+__device__ __forceinline__ void sor_update(const SorCell &x, double *t, double omega, double *sacc, int *schg,
+                                           double *__restrict__ Tout) {
   const int k = x.k();
   const double resid = t[k] * x.ml + t[k + LW] * x.mip + t[k - LW] * x.mim + t[k + 1] * x.mjp +
                        t[k - 1] * x.mjm - x.rhs;
@@ -2546,31 +2943,78 @@ __device__ __forceinline__ void sor_update(const SorCell &x, double *t, double o
   if (x.d() == 0) {
     sacc[x.o()] = fabs(resid);
     schg[x.o()] = (dtemp > tn * SORPRECISION) ? 1 : 0;
+    Tout[x.gl] = tn;
   }
+}
+
+// cells of one colour of a full tile: SR_TI*SR_TJ/2 owned ones, and in the
+// ring at distance d two rows of SR_TJ+2d cells and two columns of
+// SR_TI+2d-2 cells, half of each of that colour
+__host__ __device__ constexpr int sr_upto(int h) {
+  int n = SR_TI * SR_TJ / 2;
+  for (int d = 1; d <= h; d++) n += SR_TJ + SR_TI + 4 * d - 2;
+  return n;
 }
 
 template <int NIT, int NT = TB>
 struct SorShape {
   static constexpr int H = 2 * NIT;                        // staged halo
-  static constexpr int LW = SOR_TJ + 2 * H;                // LDS row
-  static constexpr int LR = SOR_TI + 2 * H;                // LDS rows
+  static constexpr int LW = SR_TJ + 2 * H;                 // LDS row
+  static constexpr int LR = SR_TI + 2 * H;                 // LDS rows
   static constexpr int HR = 2 * NIT - 1, HB = 2 * NIT - 2; // halo of the red / black cells
-  static constexpr int NRED = ((SOR_TI + 2 * HR) * ((SOR_TJ + 2 * HR + 1) / 2) + NT - 1) / NT;
-  static constexpr int NBLK = ((SOR_TI + 2 * HB) * ((SOR_TJ + 2 * HB + 1) / 2) + NT - 1) / NT;
+  static constexpr int NRED = (sr_upto(HR) + NT - 1) / NT;
+  static constexpr int NBLK = (sr_upto(HB) + NT - 1) / NT;
 };
 
-// slot sl of the cells of one colour (odd: (i+j) odd, red) within halo h of
-// the tile: row rho in [-h, ti+h), columns kappa in [-h, tw+h) of that colour
-__device__ __forceinline__ bool sor_cell(int sl, int h, bool odd, int i0, int j0, int ti, int tw, int &rho,
-                                         int &kappa) {
-  const int rw = (tw + 2 * h + 1) / 2;
-  if (sl >= (ti + 2 * h) * rw) return false;
-  const int r = sl / rw, m = sl - r * rw;
-  rho = r - h;
-  const int i = i0 + rho;
-  const int par = (i + j0 - h) & 1;           // parity of the first column
-  kappa = -h + ((par == (odd ? 1 : 0)) ? 0 : 1) + 2 * m;
-  return kappa < tw + h;
+// Cell s of colour c (c = 1: i+j odd, red) of the tile at rings i0.., sectors
+// j0.. (P0 = (i0+j0) & 1) with ti x tw owned cells: row rho and column kappa
+// relative to the tile's first cell, distance d from the tile.  The slots
+// follow a full tile; a narrower or shorter last tile leaves some empty
+// (false).  Rows of a ring run over columns -d .. tw+d-1 (top, then bottom),
+// its columns over rows -d+1 .. ti+d-2 (left, then right); in each, the
+// first cell of colour c is at offset 0 or 1 and the others follow every
+// second cell.
+template <int H>
+__device__ __forceinline__ bool sr_slot(int s, int c, int P0, int ti, int tw, int &rho, int &kappa, int &d) {
+  constexpr int OW = SR_TJ / 2;
+  if (s < SR_TI * OW) {
+    const int r = s / OW, m = s - r * OW;
+    rho = r;
+    d = 0;
+    kappa = ((P0 + r + c) & 1) + 2 * m;
+    return r < ti && kappa < tw;
+  }
+  s -= SR_TI * OW;
+#pragma unroll
+  for (int dd = 1; dd <= H; dd++) {
+    const int nrow = SR_TJ / 2 + dd, ncol = SR_TI / 2 + dd - 1;
+    if (s < 2 * nrow + 2 * ncol) {
+      d = dd;
+      if (s < nrow) {
+        rho = -dd;
+        kappa = -dd + ((P0 + c) & 1) + 2 * s;
+        return kappa < tw + dd;
+      }
+      s -= nrow;
+      if (s < nrow) {
+        rho = ti + dd - 1;
+        kappa = -dd + ((P0 + ti - 1 + c) & 1) + 2 * s;
+        return kappa < tw + dd;
+      }
+      s -= nrow;
+      if (s < ncol) {
+        kappa = -dd;
+        rho = -dd + 1 + ((P0 + 1 + c) & 1) + 2 * s;
+        return rho < ti + dd - 1;
+      }
+      s -= ncol;
+      kappa = tw + dd - 1;
+      rho = -dd + 1 + ((P0 + tw + c) & 1) + 2 * s;
+      return rho < ti + dd - 1;
+    }
+    s -= 2 * nrow + 2 * ncol;
+  }
+  return false;
 }
 
 template <int NIT, int NT>
@@ -2646,17 +3090,18 @@ __device__ __forceinline__ void b_sor_iter(const Geo &g, double *__restrict__ T0
   static_assert(NT % TB == 0, "the partial sums keep the pattern of TB threads");
   typedef SorShape<NIT, NT> S;
   __shared__ double t[S::LR * S::LW];
-  __shared__ double sacc[SOR_TI * SOR_TJ];
-  __shared__ int schg[SOR_TI * SOR_TJ];
+  __shared__ double sacc[NIT][SR_TI * SR_TJ];
+  __shared__ int schg[NIT][SR_TI * SR_TJ];
   __shared__ double red[NT];
+  static_assert(2 * NIT * (NT / 64) <= NT, "red holds the wave partials of the stop-test sums");
   const int ns = g.ns, nr = g.nr;
-  const int ntx = (ns + SOR_TJ - 1) / SOR_TJ;
+  const int ntx = (ns + SR_TJ - 1) / SR_TJ;
   int tx, ty, bid;
-  if (!xcd_tile(ntx, (nr - 2 + SOR_TI - 1) / SOR_TI, tx, ty, bid, g.blk0)) return;
-  const int i0 = 1 + ty * SOR_TI;
-  const int i1 = min(i0 + SOR_TI, nr - 1);          // owned rings [i0, i1) within [1, nr-2]
-  const int j0 = tx * SOR_TJ;
-  const int tw = min(SOR_TJ, ns - j0);               // owned sectors [j0, j0+tw)
+  if (!xcd_tile(ntx, (nr - 2 + SR_TI - 1) / SR_TI, tx, ty, bid, g.blk0)) return;
+  const int i0 = 1 + ty * SR_TI;
+  const int i1 = min(i0 + SR_TI, nr - 1);           // owned rings [i0, i1) within [1, nr-2]
+  const int j0 = tx * SR_TJ;
+  const int tw = min(SR_TJ, ns - j0);                // owned sectors [j0, j0+tw)
   const int ti = i1 - i0;
   const int tid = threadIdx.x;
   if (it == 0) {
@@ -2696,68 +3141,89 @@ __device__ __forceinline__ void b_sor_iter(const Geo &g, double *__restrict__ T0
   double *X[4] = {T0, T1, T2, T3};
   const double *__restrict__ Tin = X[it % 4];
   SorCell rc[S::NRED], bc[S::NBLK];
-  auto setup = [&](SorCell *cs, int ncell, int h, bool odd) {
+  const int P0 = (i0 + j0) & 1;
+  auto setup = [&](SorCell *cs, int ncell, auto hc, int c) {
+    constexpr int h = decltype(hc)::value;
 #pragma unroll
     for (int q = 0; q < ncell; q++) {
       cs[q].pk = -1;
-      int rho, kappa;
-      if (!sor_cell(tid + q * NT, h, odd, i0, j0, ti, tw, rho, kappa)) continue;
+      int rho, kappa, dd;
+      if (!sr_slot<h>(tid + q * NT, c, P0, ti, tw, rho, kappa, dd)) continue;
       const int i = i0 + rho;
       if (i < 1 || i > nr - 2) continue;
       int j = j0 + kappa;
       while (j < 0) j += ns;
       while (j >= ns) j -= ns;
-      const int dr = rho < 0 ? -rho : (rho >= ti ? rho - ti + 1 : 0);
-      const int dc = kappa < 0 ? -kappa : (kappa >= tw ? kappa - tw + 1 : 0);
-      const int dd = dr > dc ? dr : dc;
       cs[q].pk = ((rho + S::H) * S::LW + (kappa + S::H)) | (dd << 12) | ((dd == 0 ? rho * tw + kappa : 0) << 16);
+      cs[q].gl = i * ns + j;
       sor_load(cs[q], (size_t)i * ns + j, Ml, Mlip, Mlim, Mljp, Mljm, RHS);
     }
   };
-  setup(rc, S::NRED, S::HR, true);
-  setup(bc, S::NBLK, S::HB, false);
-  // stage rings i0-H .. i1+H-1 and sectors j0-H .. j0+tw+H-1 (periodic in phi)
+  setup(rc, S::NRED, std::integral_constant<int, S::HR>(), 1);
+  setup(bc, S::NBLK, std::integral_constant<int, S::HB>(), 0);
+  // stage rings i0-H .. i1+H-1 and sectors j0-H .. j0+tw+H-1 (periodic in
+  // phi) over LDS rows of the full width: every load of the thread is issued
+  // before the first LDS store (one memory latency, not one per load).
+  // Places outside the tile's rings or the grid are never read; they get 0.
+  constexpr int NSTG = (S::LR * S::LW + NT - 1) / NT;
   const int sw = tw + 2 * S::H;
-  for (int idx = tid; idx < (ti + 2 * S::H) * sw; idx += NT) {
-    const int r = idx / sw, c = idx - r * sw;
+  double sv[NSTG];
+#pragma unroll
+  for (int q = 0; q < NSTG; q++) {
+    const int idx = tid + q * NT;
+    const int r = idx / S::LW, c = idx - r * S::LW;
     const int i = i0 - S::H + r;
-    if (i < 0 || i > nr - 1) continue;
-    int j = j0 - S::H + c;
-    while (j < 0) j += ns;
-    while (j >= ns) j -= ns;
-    t[r * S::LW + c] = Tin[(size_t)i * ns + j];
+    sv[q] = 0.0;
+    if (r < ti + 2 * S::H && c < sw && i >= 0 && i <= nr - 1) {
+      int j = j0 - S::H + c;
+      while (j < 0) j += ns;
+      while (j >= ns) j -= ns;
+      sv[q] = Tin[(size_t)i * ns + j];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < NSTG; q++) {
+    const int idx = tid + q * NT;
+    if (idx < S::LR * S::LW) t[idx] = sv[q];
   }
   __syncthreads();
   const int nown = ti * tw;
 #pragma unroll
   for (int k = 0; k < NIT; k++) {
     const int hr = 2 * NIT - 1 - 2 * k, hb = hr - 1;
+    // iterate it+k+1 of the owned cells goes to Tout as they are updated
+    double *__restrict__ Tout = X[(it + k + 1) % 4];
 #pragma unroll
     for (int q = 0; q < S::NRED; q++)
-      if (rc[q].pk >= 0 && rc[q].d() <= hr) sor_update<S::LW>(rc[q], t, omega, sacc, schg);
+      if (rc[q].pk >= 0 && rc[q].d() <= hr) sor_update<S::LW>(rc[q], t, omega, sacc[k], schg[k], Tout);
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < S::NBLK; q++)
-      if (bc[q].pk >= 0 && bc[q].d() <= hb) sor_update<S::LW>(bc[q], t, omega, sacc, schg);
+      if (bc[q].pk >= 0 && bc[q].d() <= hb) sor_update<S::LW>(bc[q], t, omega, sacc[k], schg[k], Tout);
     __syncthreads();
-    // iterate it+k+1 of the owned cells, and its stop-test partials
-    double *__restrict__ Tout = X[(it + k + 1) % 4];
-    for (int idx = tid; idx < nown; idx += NT) {
-      const int r = idx / tw, c = idx - r * tw;
-      Tout[(size_t)(i0 + r) * ns + (j0 + c)] = t[(r + S::H) * S::LW + (c + S::H)];
+  }
+  // stop-test partials of the NIT iterations: each of the first TB threads
+  // adds the owned cells tid, tid+TB, ... in order, then the block sums
+  {
+    double v[2 * NIT];
+#pragma unroll
+    for (int k = 0; k < NIT; k++) {
+      double acc = 0.0;
+      int chg = 0;
+      for (int idx = tid; tid < TB && idx < nown; idx += TB) {
+        acc += sacc[k][idx];
+        chg += schg[k][idx];
+      }
+      v[2 * k] = acc;
+      v[2 * k + 1] = (double)chg;
     }
-    double acc = 0.0;
-    int chg = 0;
-    for (int idx = tid; tid < TB && idx < nown; idx += TB) {
-      acc += sacc[idx];
-      chg += schg[idx];
-    }
-    acc = block_sum<NT>(acc, red);
-    const double nch = block_sum<NT>((double)chg, red);
-    if (tid == 0) {
-      psum[(size_t)k * nbs + bid] = acc;
-      pcnt[(size_t)k * nbs + bid] = (int)nch;
-    }
+    block_sums<NT, 2 * NIT>(v, red);
+    if (tid == 0)
+#pragma unroll
+      for (int k = 0; k < NIT; k++) {
+        psum[(size_t)k * nbs + bid] = v[2 * k];
+        pcnt[(size_t)k * nbs + bid] = (int)v[2 * k + 1];
+      }
   }
   if (CHK) {
     // the stop test in the workgroup that finishes last: each workgroup
@@ -3393,6 +3859,7 @@ extern "C" int thorin_dev_create(thorin_ctx *c) {
     d->rs_rows = c->ns < 512 ? 2 : 1;
     d->sor_nit = 1;
     d->rad_rows = 16;
+    d->visc_chain = 1;
     const size_t rs_lds = sizeof(double) * (size_t)d->rs_rows * c->ns;
     if (rs_lds > (size_t)prop.sharedMemPerBlock) {
       thorin_set_error("NSEC too large for the ring-mean kernel's LDS");
@@ -3562,8 +4029,13 @@ extern "C" int thorin_dev_create(thorin_ctx *c) {
   HCHECK(hipMalloc(&d->red, sizeof(double) * (size_t)d->nblk_cells * (2 + 2 * 8)));
   const int half = c->ns / 2;
   d->nblk_sor = ((half + TB - 1) / TB) * (c->nr - 2);
-  HCHECK(hipMalloc(&d->sorp, sizeof(double) * (size_t)d->nblk_cells * 3));   // up to 3 iterations per launch
-  HCHECK(hipMalloc(&d->sorc, sizeof(int) * (size_t)d->nblk_cells * 3));
+  // the SOR's stop-test partials: one per tile and iteration of a launch (up to 3)
+  {
+    const size_t nbs = (size_t)((c->ns + SR_TJ - 1) / SR_TJ) * (size_t)((c->nr - 2 + SR_TI - 1) / SR_TI);
+    const size_t np = 3 * (nbs > (size_t)d->nblk_cells ? nbs : (size_t)d->nblk_cells);
+    HCHECK(hipMalloc(&d->sorp, sizeof(double) * np));
+    HCHECK(hipMalloc(&d->sorc, sizeof(int) * np));
+  }
   HCHECK(hipMalloc(&d->pstellar, sizeof(double) * (size_t)d->nblk_cells));
   HCHECK(hipMalloc(&d->mresp, sizeof(double) * (size_t)d->nblk_cells));
   HCHECK(hipHostMalloc(&d->h_part, sizeof(double) * (size_t)d->nblk_cells, hipHostMallocDefault));
@@ -3992,6 +4464,7 @@ BATCH_BODY(k_substep1, b_substep1, TB)
 BATCH_BODY(k_divstress, b_divstress, TB)
 BATCH_BODY(k_visc_update, b_visc_update, TB)
 BATCH_BODY(k_substep2, b_substep2, TB)
+BATCH_BODY(k_visc_chain, b_visc_chain, VC_T)
 BATCH_BODY(k_substep3, b_substep3, TB)
 BATCH_BODY(k_matrix, b_matrix, TB)
 BATCH_BODY(k_sor_resid0, b_sor_resid0, TB)
@@ -4279,7 +4752,7 @@ static int sor_launch(thorin_ctx *c, double omega, int it0, int &count, double e
     d->nb_mresp = cell_count(d, nr - 2);
     d->normres0_valid = 1;
   }
-  const int nbs = ((ns + SOR_TJ - 1) / SOR_TJ) * ((nr - 2 + SOR_TI - 1) / SOR_TI);
+  const int nbs = ((ns + SR_TJ - 1) / SR_TJ) * ((nr - 2 + SR_TI - 1) / SR_TI);
   const dim3 gs((unsigned)((nbs + 7) / 8 * 8));
   // launches of sor_nit iterations while that many remain, then single ones
   // (the iterations queued beyond the expected count are a margin)
@@ -4960,60 +5433,100 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
       }
       return 0;
     };
-    // ---- ApplyBoundaryCondition, DetectCrash, ComputePressureField, SubStep1
-    if (p->damping_on)
-      LAUNCH(k_boundary, bnd_grid(d), tb, 0, d->g, d->b[B_VRAD], d->b[B_VTHETA], d->b[B_DENS], d->b[B_ENERGY], dt,
-             (const int *)nullptr, d->damp_nlo, d->damp_hi0);
-    if (peb) {
-      // DetectCrashPebbles, then SourceTermsPebbles (SourceEuler.c:274, 291)
-      LAUNCH(k_peb_damp, gall, tb, 0, d->g, d->b[B_PDENS], d->b[B_PVR], d->b[B_PVT], dt, 1, d->ctl,
-             (const int *)nullptr);
-      LAUNCH(k_peb_source, gall, tb, 0, d->g, d->b[B_PDENS], d->b[B_PVR], d->b[B_PVT], d->b[B_TAU], d->b[B_VRAD],
-             d->b[B_VTHETA], d->b[B_PAGR], d->b[B_PAGT], d->b[B_PACCR], d->b[B_PACCT], d->b[B_DRAGR], d->b[B_DRAGT]);
+    // ---- ApplyBoundaryCondition, DetectCrash, ComputePressureField, SubStep1,
+    // ViscousTerms, SubStep2, ActualiseGas, ApplyBoundaryCondition.  The gas
+    // alone with a damping boundary on vrad only runs them as one kernel
+    // (k_visc_chain), which leaves the crash snapshot to SubStep3.
+    const int kepler = (!p->damping_on) || (p->damping_on && p->damp_vrad);
+    double vkin = 0.0, vkout = 0.0;
+    if (kepler) {
+      // ImposeKeplerianEdges (Viscosity.c:236-243), host libm
+      vkin = sqrt(1.0 * 1.0 / c->Rmed[0]) *
+             sqrt(1.0 - pow(thorin_aspectratio_host(c, c->Rmed[0]), 2.0) * pow(c->Rmed[0], 2.0 * p->flaringindex) *
+                            (1. + p->sigmaslope - 2.0 * p->flaringindex));
+      vkout = sqrt(1.0 * 1.0 / c->Rmed[nr - 1]) *
+              sqrt(1.0 - pow(thorin_aspectratio_host(c, c->Rmed[nr - 1]), 2.0) *
+                             pow(c->Rmed[nr - 1], 2.0 * p->flaringindex) * (1. + p->sigmaslope - 2.0 * p->flaringindex));
     }
-    const int br = peb && p->back_reaction;
-    LAUNCH(k_substep1, gall, tb, 0, d->g, d->b[B_DENS], d->b[B_ENERGY], d->b[B_VRAD], d->b[B_VTHETA], d->b[B_AGR],
-           d->b[B_AGT], d->b[B_VRADINT], d->b[B_VTHETAINT], dt, d->ctl, peb ? d->b[B_GACCR] : (double *)nullptr,
-           peb ? d->b[B_GACCT] : (double *)nullptr, br ? d->b[B_DRAGR] : (const double *)nullptr,
-           br ? d->b[B_DRAGT] : (const double *)nullptr, peb ? 0 : 1, fs->indir_px, fs->indir_py);
-    if (peb)   // SubStep1Pebbles + ParticleDiffusion (SourceEuler.c:293-296)
-      LAUNCH(k_peb_substep1, gall, tb, 0, d->g, d->b[B_PVR], d->b[B_PVT], d->b[B_TAU], d->b[B_PACCR], d->b[B_PACCT],
-             d->b[B_GACCR], d->b[B_GACCT], d->b[B_VRAD], d->b[B_VTHETA], d->b[B_DENS], d->b[B_PDENS], dt,
-             p->diffusive_particles);
+    const bool fuse_chain = !peb && !(p->damping_on && (p->damp_init || p->damp_flux)) && d->visc_chain;
     CrashSnap snap;
     memset(&snap, 0, sizeof snap);
-    if (c->st.already_crashed == 0) {
-      snap.ctl = d->ctl;
-      snap.peb = peb;
-      const int sb[4] = {B_DENS, B_VRAD, B_VTHETA, B_TEMPER}, db[4] = {B_CRASH_DENS, B_CRASH_VRAD, B_CRASH_VTHETA,
-                                                                       B_CRASH_TEMPER};
-      for (int k = 0; k < 4; k++) { snap.src[k] = d->b[sb[k]]; snap.dst[k] = d->b[db[k]]; }
-    }
-    LAUNCH(k_divstress, gall, tb, 0, d->g, d->b[B_VRADINT], d->b[B_VTHETAINT], d->b[B_DENS], (double *)nullptr,
-           d->b[B_TAURR], d->b[B_TAURP], d->b[B_TAUPP], snap);
-    {
-      const int kepler = (!p->damping_on) || (p->damping_on && p->damp_vrad);
-      double vkin = 0.0, vkout = 0.0;
-      if (kepler) {
-        // ImposeKeplerianEdges (Viscosity.c:236-243), host libm
-        vkin = sqrt(1.0 * 1.0 / c->Rmed[0]) *
-               sqrt(1.0 - pow(thorin_aspectratio_host(c, c->Rmed[0]), 2.0) * pow(c->Rmed[0], 2.0 * p->flaringindex) *
-                              (1. + p->sigmaslope - 2.0 * p->flaringindex));
-        vkout = sqrt(1.0 * 1.0 / c->Rmed[nr - 1]) *
-                sqrt(1.0 - pow(thorin_aspectratio_host(c, c->Rmed[nr - 1]), 2.0) *
-                               pow(c->Rmed[nr - 1], 2.0 * p->flaringindex) * (1. + p->sigmaslope - 2.0 * p->flaringindex));
+    CrashSnap snap3;              // taken by SubStep3 after the fused kernel
+    memset(&snap3, 0, sizeof snap3);
+    if (fuse_chain) {
+      ViscChain vc;
+      vc.rho = d->b[B_DENS];
+      vc.energy = d->b[B_ENERGY];
+      vc.vrad = d->b[B_VRAD];
+      vc.vtheta = d->b[B_VTHETA];
+      vc.agr = d->b[B_AGR];
+      vc.agt = d->b[B_AGT];
+      vc.vradint = d->b[B_VRADINT];
+      vc.vradnew = d->b[B_VRADNEW];
+      vc.vthetanew = d->b[B_VTHETANEW];
+      vc.energyint = d->b[B_EINT];
+      vc.ctl = d->ctl;
+      vc.dt = dt;
+      vc.damp = p->damping_on;
+      vc.fold_ind = 1;
+      vc.ind_px = fs->indir_px;
+      vc.ind_py = fs->indir_py;
+      vc.kepler = kepler;
+      vc.vkin = vkin;
+      vc.vkout = vkout;
+      const int ntile = ((ns + VC_TJ - 1) / VC_TJ) * ((nr + VC_TI - 1) / VC_TI);
+      LAUNCH(k_visc_chain, dim3((unsigned)((ntile + 7) / 8 * 8)), dim3(VC_T), 0, d->g, vc);
+      if (c->st.already_crashed == 0) {
+        // after the swaps below the substep's old vrad and vtheta are in B_VRADNEW and B_VTHETANEW
+        snap3.ctl = d->ctl;
+        snap3.damp = p->damping_on;
+        snap3.step = dt;
+        const int sb[4] = {B_DENS, B_VRAD, B_VTHETA, B_TEMPER}, db[4] = {B_CRASH_DENS, B_CRASH_VRAD, B_CRASH_VTHETA,
+                                                                         B_CRASH_TEMPER};
+        for (int k = 0; k < 4; k++) { snap3.src[k] = d->b[sb[k]]; snap3.dst[k] = d->b[db[k]]; }
       }
+      swapb(d, B_VRAD, B_VRADNEW);
+      swapb(d, B_VTHETA, B_VTHETANEW);
+    } else {
+      if (p->damping_on)
+        LAUNCH(k_boundary, bnd_grid(d), tb, 0, d->g, d->b[B_VRAD], d->b[B_VTHETA], d->b[B_DENS], d->b[B_ENERGY], dt,
+               (const int *)nullptr, d->damp_nlo, d->damp_hi0);
+      if (peb) {
+        // DetectCrashPebbles, then SourceTermsPebbles (SourceEuler.c:274, 291)
+        LAUNCH(k_peb_damp, gall, tb, 0, d->g, d->b[B_PDENS], d->b[B_PVR], d->b[B_PVT], dt, 1, d->ctl,
+               (const int *)nullptr);
+        LAUNCH(k_peb_source, gall, tb, 0, d->g, d->b[B_PDENS], d->b[B_PVR], d->b[B_PVT], d->b[B_TAU], d->b[B_VRAD],
+               d->b[B_VTHETA], d->b[B_PAGR], d->b[B_PAGT], d->b[B_PACCR], d->b[B_PACCT], d->b[B_DRAGR], d->b[B_DRAGT]);
+      }
+      const int br = peb && p->back_reaction;
+      LAUNCH(k_substep1, gall, tb, 0, d->g, d->b[B_DENS], d->b[B_ENERGY], d->b[B_VRAD], d->b[B_VTHETA], d->b[B_AGR],
+             d->b[B_AGT], d->b[B_VRADINT], d->b[B_VTHETAINT], dt, d->ctl, peb ? d->b[B_GACCR] : (double *)nullptr,
+             peb ? d->b[B_GACCT] : (double *)nullptr, br ? d->b[B_DRAGR] : (const double *)nullptr,
+             br ? d->b[B_DRAGT] : (const double *)nullptr, peb ? 0 : 1, fs->indir_px, fs->indir_py);
+      if (peb)   // SubStep1Pebbles + ParticleDiffusion (SourceEuler.c:293-296)
+        LAUNCH(k_peb_substep1, gall, tb, 0, d->g, d->b[B_PVR], d->b[B_PVT], d->b[B_TAU], d->b[B_PACCR], d->b[B_PACCT],
+               d->b[B_GACCR], d->b[B_GACCT], d->b[B_VRAD], d->b[B_VTHETA], d->b[B_DENS], d->b[B_PDENS], dt,
+               p->diffusive_particles);
+      if (c->st.already_crashed == 0) {
+        snap.ctl = d->ctl;
+        snap.peb = peb;
+        const int sb[4] = {B_DENS, B_VRAD, B_VTHETA, B_TEMPER}, db[4] = {B_CRASH_DENS, B_CRASH_VRAD, B_CRASH_VTHETA,
+                                                                         B_CRASH_TEMPER};
+        for (int k = 0; k < 4; k++) { snap.src[k] = d->b[sb[k]]; snap.dst[k] = d->b[db[k]]; }
+      }
+      LAUNCH(k_divstress, gall, tb, 0, d->g, d->b[B_VRADINT], d->b[B_VTHETAINT], d->b[B_DENS], (double *)nullptr,
+             d->b[B_TAURR], d->b[B_TAURP], d->b[B_TAUPP], snap);
       LAUNCH(k_visc_update, gall, tb, 0, d->g, d->b[B_VRADINT], d->b[B_VTHETAINT], d->b[B_DENS], d->b[B_TAURR],
              d->b[B_TAURP], d->b[B_TAUPP], dt, kepler, vkin, vkout);
+      // ---- SubStep2, ActualiseGas, ApplyBoundaryCondition
+      LAUNCH(k_substep2, gall, tb, 0, d->g, d->b[B_DENS], d->b[B_VRADINT], d->b[B_VTHETAINT], d->b[B_ENERGY],
+             d->b[B_VRADNEW], d->b[B_VTHETANEW], d->b[B_EINT], dt);
+      swapb(d, B_VRAD, B_VRADNEW);
+      swapb(d, B_VTHETA, B_VTHETANEW);
+      if (p->damping_on)
+        LAUNCH(k_boundary, bnd_grid(d), tb, 0, d->g, d->b[B_VRAD], d->b[B_VTHETA], d->b[B_DENS], d->b[B_ENERGY], dt,
+               (const int *)nullptr, d->damp_nlo, d->damp_hi0);
     }
-    // ---- SubStep2, ActualiseGas, ApplyBoundaryCondition
-    LAUNCH(k_substep2, gall, tb, 0, d->g, d->b[B_DENS], d->b[B_VRADINT], d->b[B_VTHETAINT], d->b[B_ENERGY],
-           d->b[B_VRADNEW], d->b[B_VTHETANEW], d->b[B_EINT], dt);
-    swapb(d, B_VRAD, B_VRADNEW);
-    swapb(d, B_VTHETA, B_VTHETANEW);
-    if (p->damping_on)
-      LAUNCH(k_boundary, bnd_grid(d), tb, 0, d->g, d->b[B_VRAD], d->b[B_VTHETA], d->b[B_DENS], d->b[B_ENERGY], dt,
-             (const int *)nullptr, d->damp_nlo, d->damp_hi0);
     HCHECK(hipGetLastError());
     // ---- SubStep3: Q+, implicit radiative diffusion
     int niter = 0;
@@ -5024,7 +5537,7 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
       LAUNCH(k_substep3, dim3(gall.x + (unsigned)nsum8), tb, 0, d->g, d->b[B_VRAD], d->b[B_VTHETA], d->b[B_DENS],
              d->b[B_EINT], d->b[B_DIVV], (double *)nullptr, (double *)nullptr, (double *)nullptr, d->b[B_QPLUS],
              d->b[B_TEMPER], d->b[B_CS], d->b[B_OPACITY], d->b[B_QMINUS], d->b[B_DIFF], d->pstellar,
-             d->opa_filled ? 0 : 1, nsum8);
+             d->opa_filled ? 0 : 1, nsum8, snap3);
     }
     d->opa_filled = 1;
     {
@@ -5686,6 +6199,7 @@ extern "C" int thorin_autotune(thorin_ctx *c, thorin_tuning *out) {
   for (int v = 1; v < 3; v++)
     if (t.rad_us[v] < t.rad_us[bi]) bi = v;
   t.rad_rows = rows[bi];
+  t.fused_chain = d->visc_chain;
   d->sor_nit = t.sor_iters_per_launch;
   d->rad_rows = t.rad_rows;
   if (out) *out = t;
@@ -5695,12 +6209,14 @@ extern "C" int thorin_autotune(thorin_ctx *c, thorin_tuning *out) {
 extern "C" int thorin_set_tuning(thorin_ctx *c, const thorin_tuning *t) {
   if (need_dev(c)) return -1;
   if (!t || t->sor_iters_per_launch < 1 || t->sor_iters_per_launch > 3 ||
-      (t->rad_rows != 8 && t->rad_rows != 16 && t->rad_rows != 32)) {
-    thorin_set_error("thorin_set_tuning: sor_iters_per_launch must be 1, 2 or 3, rad_rows 8, 16 or 32");
+      (t->rad_rows != 8 && t->rad_rows != 16 && t->rad_rows != 32) || (t->fused_chain != 0 && t->fused_chain != 1)) {
+    thorin_set_error("thorin_set_tuning: sor_iters_per_launch must be 1, 2 or 3, rad_rows 8, 16 or 32, "
+                     "fused_chain 0 or 1");
     return -1;
   }
   c->dev->sor_nit = t->sor_iters_per_launch;
   c->dev->rad_rows = t->rad_rows;
+  c->dev->visc_chain = t->fused_chain;
   return 0;
 }
 
@@ -5709,6 +6225,7 @@ extern "C" int thorin_get_tuning(thorin_ctx *c, thorin_tuning *t) {
   memset(t, 0, sizeof *t);
   t->sor_iters_per_launch = c->dev->sor_nit;
   t->rad_rows = c->dev->rad_rows;
+  t->fused_chain = c->dev->visc_chain;
   return 0;
 }
 
@@ -5732,6 +6249,7 @@ extern "C" int thorin_ensemble_autotune(thorin_ensemble *e, thorin_tuning *out) 
   for (int v = 1; v < 3; v++)
     if (sum.rad_us[v] < sum.rad_us[bi]) bi = v;
   sum.rad_rows = rows[bi];
+  sum.fused_chain = e->n > 0 ? e->c[0]->dev->visc_chain : 1;
   for (int k = 0; k < e->n; k++)
     if (thorin_set_tuning(e->c[k], &sum)) return -1;
   if (out) *out = sum;
@@ -5993,7 +6511,7 @@ extern "C" int thorin_rad_system(thorin_ctx *c, double dt, double *dt_used) {
   HCHECK(hipMemcpyAsync(d->b[B_EINT], d->b[B_ENERGY], nb, hipMemcpyDeviceToDevice, d->s));
   LAUNCH(k_substep3, gall, tb, 0, d->g, d->b[B_VRAD], d->b[B_VTHETA], d->b[B_DENS], d->b[B_EINT], d->b[B_DIVV],
          d->b[B_TAURR], d->b[B_TAURP], d->b[B_TAUPP], d->b[B_QPLUS], d->b[B_TEMPER], d->b[B_CS], d->b[B_OPACITY],
-         d->b[B_QMINUS], d->b[B_DIFF], d->pstellar, d->opa_filled ? 0 : 1, 0);
+         d->b[B_QMINUS], d->b[B_DIFF], d->pstellar, d->opa_filled ? 0 : 1, 0, CrashSnap{});
   d->opa_filled = 1;
   const dim3 gm = cell_grid(d, nr - 2);
   LAUNCH(k_matrix, gm, tb, 0, d->g, d->b[B_DENS], d->b[B_CS], d->b[B_DIFF], d->b[B_DIVV], d->b[B_QMINUS],

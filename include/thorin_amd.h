@@ -289,6 +289,10 @@ typedef struct thorin_tuning {
   int rad_rows;                   /* 8, 16 or 32 (default 16) */
   double sor_us_per_iter[3];      /* measured: per SOR iteration, 1 / 2 / 3 per launch */
   double rad_us[3];               /* measured: one radial transport launch, 8 / 16 / 32 rings */
+  int fused_chain;                /* 1 (default): SubStep1, ViscousTerms and SubStep2 (with the damping
+                                     boundary around them) as one kernel where the options allow it (the
+                                     gas alone, a damping boundary on vrad only); 0: one kernel per step.
+                                     Same bits either way; the autotuner leaves it as it is */
 } thorin_tuning;
 int  thorin_autotune(thorin_ctx *ctx, thorin_tuning *out);
 int  thorin_set_tuning(thorin_ctx *ctx, const thorin_tuning *t);

@@ -365,3 +365,36 @@ def test_gpu_crash_run_and_dump_vs_reference():
     for f in ("dens", "vrad", "vtheta", "temper", "energy"):
         err = rel_linf(d.field(f), g[f"{f}1"])
         assert err <= TOL, f"{f}1: {err:.3e}"
+
+
+@pytest.mark.parametrize("nrad,nsec,wall", [(64, 128, False), (21, 1100, False), (37, 96, True), (256, 384, False),
+                                            (12, 64, True)])
+def test_gpu_fused_substep_chain_equals_separate_kernels(nrad, nsec, wall):
+    """k_visc_chain (SubStep1, ViscousTerms, SubStep2 and the damping boundary
+    around them in one kernel) against the separate kernels on the same disk:
+    every field, the SOR iterations and the CFL counts bit for bit over
+    several DTs.  The grids leave partial tiles in both directions; the walled
+    disk has no damping and ImposeKeplerianEdges."""
+    _print_case(f"fused chain {nrad}x{nsec} wall={wall}")
+    from fargo_thorin_amd import configs, seeded_disk
+    cfg = configs.with_grid(configs.DAMPED_1024x1536, nrad, nsec)
+    if wall:
+        cfg = dict(cfg, InnerBoundary="WALL")
+    disk = seeded_disk(nrad, nsec, seed=4242)
+    P = params_for(cfg)
+    runs = []
+    for fused in (1, 0):
+        d = Disk(P)
+        d.set_tuning(3, 8, fused_chain=fused)
+        d.init(disk["dens"], disk["vrad"], disk["vtheta"], disk["temper"])
+        sp = planets_for([(1e-5, 1.0)])
+        nb = host_nbody_fn()
+        logs = []
+        for _ in range(4):
+            lg = d.algogas(sp, nb)
+            logs.append(([lg.cfl_n[k] for k in range(lg.nsub)], [lg.sor_niter[k] for k in range(lg.nsub)]))
+        assert d.tuning()["fused_chain"] == fused
+        runs.append((logs, {f: d.field(f, ghost=f in ("vrad", "vtheta")) for f in STATE_FIELDS + ["energyint"]}))
+    assert runs[0][0] == runs[1][0]
+    for f, a in runs[0][1].items():
+        assert np.array_equal(a, runs[1][1][f]), f
