@@ -700,15 +700,15 @@ __device__ __forceinline__ double strided_sum(const double *__restrict__ p, int 
 
 template <int NT>
 __device__ __forceinline__ double strided_min(const double *__restrict__ p, int n, double m) {
-  int b = threadIdx.x;
-  for (; b + 7 * NT < n; b += 8 * NT) {
+  // eight loads in flight per round, the last round's guarded (a minimum
+  // does not depend on the order)
+  for (int b = threadIdx.x; b < n; b += 8 * NT) {
     double v[8];
 #pragma unroll
-    for (int u = 0; u < 8; u++) v[u] = p[b + u * NT];
+    for (int u = 0; u < 8; u++) v[u] = (b + u * NT < n) ? p[b + u * NT] : m;
 #pragma unroll
     for (int u = 0; u < 8; u++) m = v[u] < m ? v[u] : m;
   }
-  for (; b < n; b += NT) { const double v = p[b]; m = v < m ? v : m; }
   return m;
 }
 
@@ -998,23 +998,26 @@ __global__ void __launch_bounds__(RB) __attribute__((amdgpu_waves_per_eu(4))) k_
 // between rings.  The minimum of the DiffusionTimestep partials of this
 // substep's k_substep3 (nstellar of them; 0: none formed) goes to
 // ctl->dt_stellar.
+// one workgroup of CF_T threads: the partials and ring values are few per
+// thread, so their loads take about one memory latency
+#define CF_T 1024
 __device__ __forceinline__ void b_cfl_final(const Geo &g, DevCtl *ctl, const double *__restrict__ pstellar,
                                                   int nstellar, double invdtreb_sq, const int *__restrict__ guard,
                                              DevCtl *__restrict__ hout) {
   // the four extrema (largest S and pebble term, smallest stellar partial and
   // shear limit) are independent: each thread forms its share of all four,
   // then one reduction; min and max are exact, so the order does not matter
-  __shared__ double red[4][RB / 64];
+  __shared__ double red[4][CF_T / 64];
   if (guard && *guard == 0) return;
   double ms = 1.0e38;
-  if (nstellar > 0) ms = strided_min<RB>(pstellar, nstellar, ms);
+  if (nstellar > 0) ms = strided_min<CF_T>(pstellar, nstellar, ms);
   double S = -1.0, P = -1.e30, msh = 1e30;
-  for (int i = 1 + threadIdx.x; i < g.nr; i += RB) {
+  for (int i = 1 + threadIdx.x; i < g.nr; i += CF_T) {
     const double v = g.ringmax[i];
     S = v > S ? v : S;
     if (g.pebbles) { const double u = g.pebmax[i]; P = u > P ? u : P; }
   }
-  for (int i = threadIdx.x; i < g.nr - 1; i += RB) {
+  for (int i = threadIdx.x; i < g.nr - 1; i += CF_T) {
     double dt = 2.0 * DPI * CFLSECURITY / (double)g.ns /
                 fabs(g.Vmoy[i] * g.InvRmed[i] - g.Vmoy[i + 1] * g.InvRmed[i + 1]);
     if (dt < msh) msh = dt;
@@ -1044,7 +1047,7 @@ __device__ __forceinline__ void b_cfl_final(const Geo &g, DevCtl *ctl, const dou
     ms = red[2][0];
     msh = red[3][0];
 #pragma unroll
-    for (int k = 1; k < RB / 64; k++) {
+    for (int k = 1; k < CF_T / 64; k++) {
       S = red[0][k] > S ? red[0][k] : S;
       P = red[1][k] > P ? red[1][k] : P;
       ms = red[2][k] < ms ? red[2][k] : ms;
@@ -1068,10 +1071,10 @@ __device__ __forceinline__ void b_cfl_final(const Geo &g, DevCtl *ctl, const dou
     __syncthreads();
     const unsigned long long *src = (const unsigned long long *)ctl;
     unsigned long long *dst = (unsigned long long *)hout;
-    for (int q = threadIdx.x; q < (int)(sizeof(DevCtl) / 8); q += RB) dst[q] = src[q];
+    for (int q = threadIdx.x; q < (int)(sizeof(DevCtl) / 8); q += CF_T) dst[q] = src[q];
   }
 }
-__global__ void __launch_bounds__(RB) k_cfl_final(Geo g, DevCtl *ctl, const double *__restrict__ pstellar,
+__global__ void __launch_bounds__(CF_T) k_cfl_final(Geo g, DevCtl *ctl, const double *__restrict__ pstellar,
                                                   int nstellar, double invdtreb_sq, const int *__restrict__ guard,
                                                   DevCtl *__restrict__ hout) {
   b_cfl_final(g, ctl, pstellar, nstellar, invdtreb_sq, guard, hout);
@@ -3332,6 +3335,12 @@ __global__ void __launch_bounds__(TB) k_energy_new(Geo g, const double *__restri
 
 struct RadQ {
   const double *rho, *vr, *vt, *e;
+  // the energy from the SOR's last iterate (EnergyNew = rho*T*CV with the
+  // edge rings' T taken from rings 1 and NRAD-2, EnergySources.c:319-334):
+  // T buffer ctl->sor_cur of tsrc, instead of e (tsrc[0] null: read e)
+  const double *tsrc[4];
+  const DevCtl *ctl;
+  double cv;
   double *out[RC_NQ];          // RadMomP, RadMomM, ThetaMomP, ThetaMomM, Energy, Rho
   const int *guard;             // skip the launch while *guard == 0
   double of;
@@ -3383,6 +3392,8 @@ __device__ __forceinline__ void rad_column(const Geo &g, const RadQ &q, const Ra
   const double dtheta = 2.0 * DPI / (double)ns;
   const double *__restrict__ pvt = q.vt;
   const double *__restrict__ pe = q.e;
+  const double *__restrict__ pt = nullptr;
+  if (W == 4 && q.tsrc[0]) pt = q.tsrc[q.ctl->sor_cur];
   double *__restrict__ out = q.out[W];
   // rolling state for output ring i: W at i, i+1; Q at i, i+1; dq at i;
   // flux at i
@@ -3396,14 +3407,21 @@ __device__ __forceinline__ void rad_column(const Geo &g, const RadQ &q, const Ra
     double x_l = 0.0;
     if (W == 2) x_l = pvt[kc * ns + jp];
     if (W == 3) x_l = pvt[kc * ns + j];
-    if (W == 4) x_l = pe[kc * ns + j];
+    if (W == 4) {
+      if (pt) {
+        const int kt = kc == 0 ? 1 : (kc == nr - 1 ? nr - 2 : kc);
+        x_l = pt[kt * ns + j];
+      } else {
+        x_l = pe[kc * ns + j];
+      }
+    }
     const int r = t + 1;                       // geometry entry of ring i+1
     double Q2 = 0.0;
     if (k >= 0 && k < nr) {
       if (W == 0) Q2 = rho2 * sg.vr[t + 2][c];
       if (W == 1) Q2 = rho2 * sg.vr[t + 1][c];
       if (W == 2 || W == 3) Q2 = rho2 * (x_l + sg.rmed[r + 1] * q.of) * sg.rmed[r + 1];
-      if (W == 4) Q2 = x_l;
+      if (W == 4) Q2 = pt ? rho2 * x_l * q.cv : x_l;
       if (W == 5) Q2 = rho2;
     }
     const double w2 = Q2 / (rho2 + 1e-20);
@@ -4446,7 +4464,7 @@ struct Batch {
   };
 BATCH_BODY(k_eos_cfl, b_eos_cfl, RB)
 BATCH_BODY(k_eos_cfl_vel, b_eos_cfl_vel, RB)
-BATCH_BODY(k_cfl_final, b_cfl_final, RB)
+BATCH_BODY(k_cfl_final, b_cfl_final, CF_T)
 BATCH_BODY((k_forces<false, 1>), (b_forces<false, 1>), TB)
 BATCH_BODY((k_forces<false, 8>), (b_forces<false, 8>), TB)
 BATCH_BODY((k_forces<true, 1>), (b_forces<true, 1>), TB)
@@ -5246,7 +5264,7 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
                d->b[B_PVR], d->b[B_PVT], d->b[B_VRAD], d->b[B_VTHETA], (const int *)nullptr);
       LAUNCH(k_eos_cfl, dim3((nr + 7) / 8 * 8), rb, eos_lds(d, 0), d->g, d->b[B_DENS], d->b[B_ENERGY],
              d->b[B_TEMPER], d->b[B_CS], d->b[B_VRAD], d->b[B_VTHETA], 0, (const int *)nullptr);
-      LAUNCH(k_cfl_final, dim3(1), rb, 0, d->g, d->ctl, (const double *)nullptr, 0, invdtreb_sq,
+      LAUNCH(k_cfl_final, dim3(1), dim3(CF_T), 0, d->g, d->ctl, (const double *)nullptr, 0, invdtreb_sq,
              (const int *)nullptr, (DevCtl *)nullptr);
       HCHECK(hipGetLastError());
       DCOPY(d->hctl, d->ctl, sizeof(DevCtl), hipMemcpyDeviceToHost);
@@ -5588,16 +5606,20 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
     double *bsave[NBUF];
     memcpy(bsave, d->b, sizeof bsave);
     auto after_sor = [&]() -> int {
-      LAUNCH(k_energy_new, gall, tb, 0, d->g, d->b[B_DENS], d->b[B_TEMPER], d->b[B_TEMPER2], d->b[B_TEMPER3],
-             d->b[B_TEMPER4], d->b[B_ENEW], d->ctl);
-      swapb(d, B_ENERGY, B_ENEW);
-      // ---- Transport
+      // ---- EnergyNew = rho T CV from the SOR's last iterate, formed where the
+      // radial transport reads the energy; Transport
       {
-        RadQ q;
+        RadQ q = {};
         q.rho = d->b[B_DENS];
         q.vr = d->b[B_VRAD];
         q.vt = d->b[B_VTHETA];
-        q.e = d->b[B_ENERGY];
+        q.e = nullptr;
+        q.tsrc[0] = d->b[B_TEMPER];
+        q.tsrc[1] = d->b[B_TEMPER2];
+        q.tsrc[2] = d->b[B_TEMPER3];
+        q.tsrc[3] = d->b[B_TEMPER4];
+        q.ctl = d->ctl;
+        q.cv = d->g.cv;
         const int outs[RC_NQ] = {B_XRP, B_XRM, B_XTP, B_XTM, B_XE, B_XRHO};
         for (int k = 0; k < RC_NQ; k++) q.out[k] = d->b[outs[k]];
         q.guard = guard;
@@ -5623,7 +5645,7 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
                d->g.pvmed, guard, BOff{0});
         Geo gp = d->g;
         gp.VMed = d->g.pvmed;
-        RadQ q;
+        RadQ q = {};
         q.rho = d->b[B_PDENS];
         q.vr = d->b[B_PVR];
         q.vt = d->b[B_PVT];
@@ -5660,7 +5682,7 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
       // the mirror then reads sor_done = 0 and the solve is finished from
       // the host.
       d->hctl->sor_done = 0;
-      LAUNCH(k_cfl_final, dim3(1), rb, 0, d->g, d->ctl, (const double *)d->pstellar, d->nblk_cells,
+      LAUNCH(k_cfl_final, dim3(1), dim3(CF_T), 0, d->g, d->ctl, (const double *)d->pstellar, d->nblk_cells,
              d->cfl_invdtreb, guard, d->hctl);
       HCHECK(hipGetLastError());
       d->spec.valid = 0;
@@ -6160,7 +6182,7 @@ extern "C" int thorin_autotune(thorin_ctx *c, thorin_tuning *out) {
     // the copy of T (one pass) is timed too; it is the same for all
     t.sor_us_per_iter[v] = 1e3 * ms / iters;
   }
-  RadQ q;
+  RadQ q = {};
   q.rho = d->b[B_DENS];
   q.vr = d->b[B_VRAD];
   q.vt = d->b[B_VTHETA];
@@ -6500,7 +6522,7 @@ extern "C" int thorin_rad_system(thorin_ctx *c, double dt, double *dt_used) {
     gg.pebbles = 0;
     LAUNCH(k_eos_cfl, dim3((nr + 7) / 8 * 8), rb, eos_lds(d, 0), gg, d->b[B_DENS], d->b[B_ENERGY], d->b[B_TEMPER],
            d->b[B_CS], d->b[B_VRAD], d->b[B_VTHETA], 0, (const int *)nullptr);
-    LAUNCH(k_cfl_final, dim3(1), rb, 0, gg, d->ctl, (const double *)nullptr, 0, 0.0, (const int *)nullptr,
+    LAUNCH(k_cfl_final, dim3(1), dim3(CF_T), 0, gg, d->ctl, (const double *)nullptr, 0, 0.0, (const int *)nullptr,
            (DevCtl *)nullptr);
     HCHECK(hipMemcpyAsync(d->hctl, d->ctl, sizeof(DevCtl), hipMemcpyDeviceToHost, d->s));
     HCHECK(hipStreamSynchronize(d->s));
