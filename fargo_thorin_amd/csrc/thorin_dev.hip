@@ -3695,22 +3695,21 @@ __device__ __forceinline__ void b_transport_theta(const Geo &g, const double *__
     th_pass<NPT>(t, qv, r1, rs2, nullptr, vt2, true, s_w, s_f);       // VanLeerTheta(Rho), uniform velocity
   }
   const int sh_n = fast ? nshift : 0;
-  // AdvectSHIFT through LDS: out[j] = q[j - Nshift]
+  // AdvectSHIFT: out[j] = q[j - Nshift], written as out[j + Nshift] = q[j]
+  // (a rotation of the ring: each cell goes to one place, the stores of a
+  // wave stay contiguous but for the wrap)
+  int sh0 = sh_n % ns;
+  if (sh0 < 0) sh0 += ns;
   auto store_shifted = [&](double *out) {
-    __syncthreads();
 #pragma unroll
     for (int k = 0; k < NPT; k++) {
       const int j = tid + k * TT_T;
-      if (j < ns) s_w[j] = qv[k];
+      if (j < ns) {
+        int jo = j + sh0;
+        if (jo >= ns) jo -= ns;
+        out[base + jo] = qv[k];
+      }
     }
-    __syncthreads();
-    for (int j = tid; j < ns; j += TT_T) {
-      int ji = j - sh_n;
-      while (ji < 0) ji += ns;
-      while (ji >= ns) ji -= ns;
-      out[base + j] = s_w[ji];
-    }
-    __syncthreads();
   };
   store_shifted(orho);
   const double *ins[5] = {xrp, xrm, xtp, xtm, xe};
@@ -3732,11 +3731,10 @@ __device__ __forceinline__ void b_transport_theta(const Geo &g, const double *__
         nxt[k] = (j < ns) ? src[base + j] : 0.0;
       }
     }
+    // (the passes' LDS arrays are each written only after the barrier that
+    // follows every read of them in the pass before)
     th_pass<NPT>(t, qv, r0, rs1, vtr, 0.0, false, s_w, s_f);
-    if (fast) {
-      __syncthreads();
-      th_pass<NPT>(t, qv, r1, rs2, nullptr, vt2, true, s_w, s_f);
-    }
+    if (fast) th_pass<NPT>(t, qv, r1, rs2, nullptr, vt2, true, s_w, s_f);
     store_shifted(outs[which]);
   }
 }
