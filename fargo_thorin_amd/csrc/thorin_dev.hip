@@ -3393,7 +3393,10 @@ __device__ __forceinline__ void rad_column(const Geo &g, const RadQ &q, const Ra
   const double *__restrict__ pvt = q.vt;
   const double *__restrict__ pe = q.e;
   const double *__restrict__ pt = nullptr;
-  if (W == 4 && q.tsrc[0]) pt = q.tsrc[q.ctl->sor_cur];
+  if (W == 4 && q.tsrc[0]) {
+    const int cur = q.ctl->sor_cur;   // (a select: no indexing of the argument block)
+    pt = cur == 0 ? q.tsrc[0] : (cur == 1 ? q.tsrc[1] : (cur == 2 ? q.tsrc[2] : q.tsrc[3]));
+  }
   double *__restrict__ out = q.out[W];
   // rolling state for output ring i: W at i, i+1; Q at i, i+1; dq at i;
   // flux at i
@@ -3445,7 +3448,7 @@ __device__ __forceinline__ void rad_column(const Geo &g, const RadQ &q, const Ra
 }
 
 template <int RI>
-__device__ __forceinline__ void b_transport_rad(const Geo &g, RadQ q, double dt) {
+__device__ __forceinline__ void b_transport_rad(const Geo &g, const RadQ &q, double dt) {
   __shared__ RadGeo<RI> sg;
   if (q.guard && *q.guard == 0) return;
   const int b = (int)blockIdx.x - g.blk0, slot = b & 7;
@@ -4546,6 +4549,12 @@ struct RecOp {
   hipMemcpyKind kind;
 };
 
+// The end-of-substep marker of a recorded queue: what comes before it is what
+// the disk's host side waits for; what follows (the next substep's forces)
+// may still run while the host works.  Merged like a kernel (one group when
+// every disk has reached it), issued as an event the flush then waits for.
+static const int ens_marker_key = 0;
+
 struct EnsGraph {               // a captured flush
   std::vector<unsigned long long> sig;
   hipGraph_t g;
@@ -4577,6 +4586,20 @@ struct thorin_ensemble {
   std::vector<std::vector<unsigned long long>> seen;   // signatures met once
   unsigned long long tick;
   long long captures, replays;
+  // the disks' host threads, one per disk for the ensemble's life: woken
+  // for each DT (job), they run thorin_algogas on their disk
+  hipEvent_t mark;               // the end-of-substep markers' event
+  int marked;                    // markers in the flush being planned (one per disk at most)
+  std::vector<std::thread> workers;
+  std::condition_variable cv_job;
+  unsigned long job;             // DTs started
+  bool quit;
+  thorin_planets *const *j_sys;
+  thorin_nbody_fn j_nbody;
+  void *const *j_user;
+  thorin_step_log *j_logs;
+  std::vector<int> j_rc;
+  std::string j_err;             // the first failing disk's error text
 };
 
 // the table of a group: n argument tuples, then n+1 block offsets; the host
@@ -4695,6 +4718,31 @@ static int dev_copy(thorin_dev *d, void *dst, const void *src, size_t bytes, hip
 static int dev_sync(thorin_dev *d) {
   if (d->rec) return rec_wait(d->rec);
   HCHECK(hipStreamSynchronize(d->s));
+  return 0;
+}
+
+// the end of the substep's own work: the event ev_end on the disk's stream,
+// or a marker in the recorded queue
+static int dev_mark(thorin_dev *d) {
+  if (d->rec) {
+    RecOp op;
+    op.type = 0;
+    op.key = (const void *)&ens_marker_key;
+    op.plan = nullptr;            // issued by the flush itself
+    op.grid = dim3(1);
+    op.block = dim3(1);
+    op.shmem = 0;
+    d->rec->q.push_back(std::move(op));
+    return 0;
+  }
+  HCHECK(hipEventRecord(d->ev_end, d->s));
+  return 0;
+}
+
+// wait for the marked end of the substep (dev_mark)
+static int dev_wait_mark(thorin_dev *d) {
+  if (d->rec) return rec_wait(d->rec);
+  HCHECK(hipEventSynchronize(d->ev_end));
   return 0;
 }
 
@@ -5391,8 +5439,7 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
     // ---- FillForcesArrays (the planets' constants go to the kernel by value)
     const int want_tq = p->write_torque && log && log->nsub == 0 && sys->nb > 0;
     ForceSet fset;
-    if (!peb && !gacc && !d->rec && d->spec.valid &&
-        force_set_matches(d->spec, sys, mt, want_tq, c->st.physical_time)) {
+    if (!peb && !gacc && d->spec.valid && force_set_matches(d->spec, sys, mt, want_tq, c->st.physical_time)) {
       fset = d->spec;           // enqueued at the end of the previous substep
     } else {
       if (launch_forces(c, sys, mt, want_tq, nullptr, &fset)) return -1;
@@ -5684,7 +5731,7 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
              d->cfl_invdtreb, guard, d->hctl);
       HCHECK(hipGetLastError());
       d->spec.valid = 0;
-      if (peb || gacc || d->rec) {   // (the gas accretion changes the density before the forces)
+      if (peb || gacc) {   // (the gas accretion changes the density before the forces)
         DSYNC();
         return 0;
       }
@@ -5694,14 +5741,14 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
       // are now, MassTaper at the new time) are those the next substep would
       // use unless the host changes the planets, which the next substep
       // detects by comparing the planet set before it uses these forces.
-      HCHECK(hipEventRecord(d->ev_end, d->s));
+      if (dev_mark(d)) return -1;
       {
         double mtn = c->st.physical_time / (p->masstaper * 2.0 * M_PI);
         mtn = (mtn > 1.0 ? 1.0 : pow(sin(mtn * M_PI / 2.0), 2.0));
         const int tqn = !(dtemp < 0.999999999 * DT) && p->write_torque && sys->nb > 0;
         if (launch_forces(c, sys, mtn, tqn, guard, &d->spec)) return -1;
       }
-      HCHECK(hipEventSynchronize(d->ev_end));
+      if (dev_wait_mark(d)) return -1;
       if (!d->hctl->sor_done) d->spec.valid = 0;   // the guarded launches did nothing
       return 0;
     };
@@ -5783,6 +5830,12 @@ static int rec_wait(Rec *r) {
 // the rest of the substep is batched again.
 static int ens_flush(thorin_ensemble *e) {
   const int n = e->n;
+  e->marked = 0;
+  // disks with work in this flush: the flush may end at the markers' event
+  // only when each of them ends its part with a marker (else a disk waits
+  // for work, such as a copy of its control block, queued after the event)
+  int active = 0;
+  for (int k = 0; k < n; k++) active += e->recs[k].q.empty() ? 0 : 1;
   std::vector<size_t> pos((size_t)n, 0);
   std::vector<std::unordered_map<const void *, int>> left((size_t)n);
   for (int k = 0; k < n; k++)
@@ -5841,9 +5894,20 @@ static int ens_flush(thorin_ensemble *e) {
         left[k][best]--;
       }
     }
-    issue.push_back(grp[0]->plan(grp.data(), (int)grp.size(), tables, sig));
-    e->launches++;
-    nops += (long long)grp.size();
+    if (best == (const void *)&ens_marker_key) {
+      // an event record (the last one recorded follows every disk's work
+      // before its marker); such a flush is not captured as a graph
+      e->marked += (int)grp.size();
+      sig.push_back(~0ull);
+      issue.push_back([e](hipStream_t s, const char *) -> int {
+        HCHECK(hipEventRecord(e->mark, s));
+        return 0;
+      });
+    } else {
+      issue.push_back(grp[0]->plan(grp.data(), (int)grp.size(), tables, sig));
+      e->launches++;
+      nops += (long long)grp.size();
+    }
   }
   e->batched_ops += nops;
   if (tables.size() > e->st.cap) {
@@ -5922,9 +5986,35 @@ static int ens_flush(thorin_ensemble *e) {
       HCHECK(hipGraphLaunch(G->x, e->s));
     }
   }
-  HCHECK(hipStreamSynchronize(e->s));
+  // every disk's part ending in its end-of-substep marker: the disks wait
+  // for the last marker (the next substeps' forces queued behind it keep
+  // running); else for everything issued
+  if (e->marked > 0 && e->marked == active) HCHECK(hipEventSynchronize(e->mark));
+  else HCHECK(hipStreamSynchronize(e->s));
   for (int k = 0; k < n; k++) e->recs[k].q.clear();
   return 0;
+}
+
+
+// disk k's host thread: one thorin_algogas per job; its errors (kept per
+// thread) are copied out under the lock
+static void ens_worker(thorin_ensemble *e, int k) {
+  unsigned long seen = 0;
+  for (;;) {
+    {
+      std::unique_lock<std::mutex> lk(e->m);
+      e->cv_job.wait(lk, [&] { return e->quit || e->job != seen; });
+      if (e->quit) return;
+      seen = e->job;
+    }
+    const int r = thorin_algogas(e->c[k], e->j_sys[k], e->j_nbody, e->j_user ? e->j_user[k] : nullptr,
+                                 e->j_logs ? &e->j_logs[k] : nullptr);
+    std::lock_guard<std::mutex> lk(e->m);
+    e->j_rc[k] = r;
+    if (r < 0 && e->j_err.empty()) e->j_err = "disk " + std::to_string(k) + ": " + thorin_last_error();
+    e->nfinished++;
+    e->cv_coord.notify_one();
+  }
 }
 
 extern "C" int thorin_ensemble_create(thorin_ctx *const *disks, int n, thorin_ensemble **out) {
@@ -5961,21 +6051,54 @@ extern "C" int thorin_ensemble_create(thorin_ctx *const *disks, int n, thorin_en
     delete e;
     return -1;
   }
+  if (hipEventCreateWithFlags(&e->mark, hipEventDisableTiming) != hipSuccess) {
+    thorin_set_error("thorin_ensemble_create: event creation failed");
+    (void)hipStreamDestroy(e->s);
+    delete[] e->c;
+    delete e;
+    return -1;
+  }
   // work queued on the disks' own streams comes first
   for (int k = 0; k < n; k++)
     if (hipStreamSynchronize(disks[k]->dev->s) != hipSuccess) {
       thorin_set_error("thorin_ensemble_create: a disk's stream failed");
+      (void)hipEventDestroy(e->mark);
       (void)hipStreamDestroy(e->s);
       delete[] e->c;
       delete e;
       return -1;
     }
+  e->job = 0;
+  e->quit = false;
+  e->j_rc.assign((size_t)n, 0);
+  try {
+    for (int k = 0; k < n; k++) e->workers.emplace_back(ens_worker, e, k);
+  } catch (...) {
+    {
+      std::lock_guard<std::mutex> lk(e->m);
+      e->quit = true;
+    }
+    e->cv_job.notify_all();
+    for (auto &t : e->workers) t.join();
+    (void)hipEventDestroy(e->mark);
+    (void)hipStreamDestroy(e->s);
+    delete[] e->c;
+    delete e;
+    thorin_set_error("thorin_ensemble_create: the disks' threads could not be started");
+    return -1;
+  }
   *out = e;
   return 0;
 }
 
 extern "C" void thorin_ensemble_destroy(thorin_ensemble *e) {
   if (!e) return;
+  {
+    std::lock_guard<std::mutex> lk(e->m);
+    e->quit = true;
+  }
+  e->cv_job.notify_all();
+  for (auto &t : e->workers) t.join();
   (void)hipStreamSynchronize(e->s);
   for (int k = 0; k < e->n; k++)
     if (e->c[k] && e->c[k]->dev) e->c[k]->dev->tt_npt_min = 0;
@@ -5985,6 +6108,7 @@ extern "C" void thorin_ensemble_destroy(thorin_ensemble *e) {
   }
   if (e->st.h) (void)hipHostFree(e->st.h);
   if (e->st.dv) (void)hipFree(e->st.dv);
+  (void)hipEventDestroy(e->mark);
   (void)hipStreamDestroy(e->s);
   delete[] e->c;
   delete e;
@@ -6016,18 +6140,17 @@ extern "C" int thorin_ensemble_algogas(thorin_ensemble *e, thorin_planets *const
     e->c[k]->dev->rec = &e->recs[k];
     e->c[k]->dev->sor_queue_min = qmin;
   }
-  std::vector<int> rc((size_t)n, 0);
-  std::vector<std::thread> th;
-  std::string disk_err;          // the first failing disk's error text (set on its thread)
-  th.reserve((size_t)n);
-  for (int k = 0; k < n; k++)
-    th.emplace_back([&, k] {
-      rc[k] = thorin_algogas(e->c[k], sys[k], nbody, nbody_user ? nbody_user[k] : nullptr, logs ? &logs[k] : nullptr);
-      std::lock_guard<std::mutex> lk(e->m);
-      if (rc[k] < 0 && disk_err.empty()) disk_err = "disk " + std::to_string(k) + ": " + thorin_last_error();
-      e->nfinished++;
-      e->cv_coord.notify_one();
-    });
+  {
+    std::lock_guard<std::mutex> lk(e->m);
+    e->j_sys = sys;
+    e->j_nbody = nbody;
+    e->j_user = nbody_user;
+    e->j_logs = logs;
+    e->j_err.clear();
+    for (int k = 0; k < n; k++) e->j_rc[k] = 0;
+    e->job++;
+  }
+  e->cv_job.notify_all();
   int status = 0;
   for (;;) {
     std::unique_lock<std::mutex> lk(e->m);
@@ -6048,18 +6171,18 @@ extern "C" int thorin_ensemble_algogas(thorin_ensemble *e, thorin_planets *const
     e->gen++;
     e->cv_disk.notify_all();
   }
-  for (auto &t : th) t.join();
   for (int k = 0; k < n; k++) {
     e->c[k]->dev->rec = nullptr;
     e->c[k]->dev->sor_queue_min = 0;
   }
   int worst = 0;
   for (int k = 0; k < n; k++) {
-    if (rc_out) rc_out[k] = rc[k];
-    if (rc[k] < 0) status = -1;
-    else if (rc[k] > worst) worst = rc[k];
+    const int r = e->j_rc[k];
+    if (rc_out) rc_out[k] = r;
+    if (r < 0) status = -1;
+    else if (r > worst) worst = r;
   }
-  if (!disk_err.empty()) thorin_set_error(disk_err.c_str());
+  if (!e->j_err.empty()) thorin_set_error(e->j_err.c_str());
   return status ? -1 : worst;
 }
 
