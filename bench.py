@@ -38,7 +38,7 @@ ALG_BYTES_PER_CELL = {
     "k_sor_iter": 64, "k_transport_theta": 104, "k_transport_rad": 80, "k_forces": 48,
     "k_substep1": 64, "k_divstress": 56, "k_visc_update": 64, "k_substep2": 56, "k_matrix": 104, "k_velocities": 56,
     "k_energy_new": 24, "k_indirect": 48, "k_eos_cfl": 48, "k_eos_cfl_vel": 80, "k_sor_resid0": 56,
-    "k_substep3": 96,
+    "k_substep3": 96, "k_visc_chain": 72,
 }
 # Whole step: the fused-dataflow minimum of the hydro step, 82 field passes
 # per substep (656 B/cell) plus 64 B/cell per SOR iteration.
@@ -82,6 +82,8 @@ def parse():
     ap.add_argument("--no-sor-study", action="store_true")
     ap.add_argument("--no-two-fluid", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--no-cnn", action="store_true",
+                    help="ensemble study without its warm-start part (the disks' stepping alone)")
     ap.add_argument("--study", choices=["sor", "two-fluid", "ensemble"], default=None,
                     help="run only this study and print its JSON (for profiling)")
     ap.add_argument("--roofline-kernel", default="auto")
@@ -526,7 +528,7 @@ def main():
         print(json.dumps(two_fluid_study()))
         return
     if args.study == "ensemble":
-        print(json.dumps(ensemble_study()))
+        print(json.dumps(ensemble_study(cnn=not args.no_cnn)))
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this library runs on one GPU; --gpus must be 1 (radial slabs over several "

@@ -88,7 +88,6 @@ struct DevCtl {                 // device-side control block, mirrored to pinned
   int sor_done, sor_niter, sor_iter;
   int sor_cur;                  // T buffer holding the latest iterate (0: B_TEMPER, 1: B_TEMPER2, 2: B_TEMPER3,
                                 // 3: B_TEMPER4)
-  int sor_ticket, sor_pad;      // workgroups of a running single-iteration SOR launch that have finished
   double sor_normres0, sor_normres;
   double axstar, aystar;
   double axpl[MAXPL], aypl[MAXPL];
@@ -3079,7 +3078,7 @@ __device__ __forceinline__ void sor_stop_test(const double *__restrict__ psum, c
   }
 }
 
-template <int NIT, bool CHK = false, int NT = TB>
+template <int NIT, int NT = TB>
 __device__ __forceinline__ void b_sor_iter(const Geo &g, double *__restrict__ T0, double *__restrict__ T1,
                                            double *__restrict__ T2, double *__restrict__ T3,
                                            const double *__restrict__ Ml,
@@ -3087,8 +3086,7 @@ __device__ __forceinline__ void b_sor_iter(const Geo &g, double *__restrict__ T0
                                            const double *__restrict__ Mljp, const double *__restrict__ Mljm,
                                            const double *__restrict__ RHS, double omega, double *__restrict__ psum,
                                            int *__restrict__ pcnt, DevCtl *__restrict__ ctl, int it,
-                                           const double *__restrict__ mresp, int nbm, int nbs, int maxiter = 0,
-                                           double eps = 0.0) {
+                                           const double *__restrict__ mresp, int nbm, int nbs) {
   static_assert(NIT >= 1 && NIT <= 3, "four iterate buffers hold at most three iterations per launch");
   static_assert(NT % TB == 0, "the partial sums keep the pattern of TB threads");
   typedef SorShape<NIT, NT> S;
@@ -3228,28 +3226,9 @@ __device__ __forceinline__ void b_sor_iter(const Geo &g, double *__restrict__ T0
         pcnt[(size_t)k * nbs + bid] = (int)v[2 * k + 1];
       }
   }
-  if (CHK) {
-    // the stop test in the workgroup that finishes last: each workgroup
-    // publishes its partials (device-scope fence) before it takes a ticket;
-    // the last one reads them all after its own fence, adds them in block
-    // order as k_sor_check does, and rearms the ticket for the next launch.
-    // (Used for the single-iteration launches that end a queue: while the
-    // solve has converged they return at once and need no check launch.)
-    __shared__ int s_last;
-    if (tid == 0) {
-      __threadfence();
-      s_last = atomicAdd(&ctl->sor_ticket, 1) == nbs - 1;
-    }
-    __syncthreads();
-    if (s_last) {
-      __threadfence();
-      sor_stop_test<NIT, NT>(psum, pcnt, nbs, ctl, maxiter, it, eps, red);
-      if (tid == 0) ctl->sor_ticket = 0;
-    }
-  }
 }
 
-template <int NIT, bool CHK = false, int NT = TB>
+template <int NIT, int NT = TB>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 2 * TB ? 4 : 1))) k_sor_iter(Geo g, double *__restrict__ T0, double *__restrict__ T1,
                                                  double *__restrict__ T2, double *__restrict__ T3,
                                                  const double *__restrict__ Ml,
@@ -3257,10 +3236,9 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 2
                                                  const double *__restrict__ Mljp, const double *__restrict__ Mljm,
                                                  const double *__restrict__ RHS, double omega, double *__restrict__ psum,
                                                  int *__restrict__ pcnt, DevCtl *__restrict__ ctl, int it,
-                                                 const double *__restrict__ mresp, int nbm, int nbs, int maxiter,
-                                                 double eps) {
-  b_sor_iter<NIT, CHK, NT>(g, T0, T1, T2, T3, Ml, Mlip, Mlim, Mljp, Mljm, RHS, omega, psum, pcnt, ctl, it, mresp,
-                           nbm, nbs, maxiter, eps);
+                                                 const double *__restrict__ mresp, int nbm, int nbs) {
+  b_sor_iter<NIT, NT>(g, T0, T1, T2, T3, Ml, Mlip, Mlim, Mljp, Mljm, RHS, omega, psum, pcnt, ctl, it, mresp,
+                           nbm, nbs);
 }
 
 // stop test of iterations it+1 .. it+NIT (EnergySources.c:487-493), partial
@@ -4487,10 +4465,9 @@ BATCH_BODY(k_visc_chain, b_visc_chain, VC_T)
 BATCH_BODY(k_substep3, b_substep3, TB)
 BATCH_BODY(k_matrix, b_matrix, TB)
 BATCH_BODY(k_sor_resid0, b_sor_resid0, TB)
-BATCH_BODY((k_sor_iter<1, false, SOR_NT(1)>), (b_sor_iter<1, false, SOR_NT(1)>), SOR_NT(1))
-BATCH_BODY((k_sor_iter<1, true>), (b_sor_iter<1, true>), TB)
-BATCH_BODY((k_sor_iter<2, false, SOR_NT(2)>), (b_sor_iter<2, false, SOR_NT(2)>), SOR_NT(2))
-BATCH_BODY((k_sor_iter<3, false, SOR_NT(3)>), (b_sor_iter<3, false, SOR_NT(3)>), SOR_NT(3))
+BATCH_BODY((k_sor_iter<1, SOR_NT(1)>), (b_sor_iter<1, SOR_NT(1)>), SOR_NT(1))
+BATCH_BODY((k_sor_iter<2, SOR_NT(2)>), (b_sor_iter<2, SOR_NT(2)>), SOR_NT(2))
+BATCH_BODY((k_sor_iter<3, SOR_NT(3)>), (b_sor_iter<3, SOR_NT(3)>), SOR_NT(3))
 BATCH_BODY(k_sor_check<1>, b_sor_check<1>, RB)
 BATCH_BODY(k_sor_check<2>, b_sor_check<2>, RB)
 BATCH_BODY(k_sor_check<3>, b_sor_check<3>, RB)
@@ -4823,30 +4800,23 @@ static int sor_launch(thorin_ctx *c, double omega, int it0, int &count, double e
   const int nmax = (d->sor_nit >= 1 && d->sor_nit <= 3) ? d->sor_nit : 1;
 #define SOR_LAUNCH(N)                                                                                             \
   do {                                                                                                            \
-    LAUNCH((k_sor_iter<N, false, SOR_NT(N)>), gs, dim3(SOR_NT(N)), 0, g, d->b[B_TEMPER], d->b[B_TEMPER2],         \
+    LAUNCH((k_sor_iter<N, SOR_NT(N)>), gs, dim3(SOR_NT(N)), 0, g, d->b[B_TEMPER], d->b[B_TEMPER2],         \
            d->b[B_TEMPER3],                                                                                       \
            d->b[B_TEMPER4], d->b[B_ML], d->b[B_MLIP], d->b[B_MLIM], d->b[B_MLJP], d->b[B_MLJM], d->b[B_RHS], omega, \
-           d->sorp, d->sorc, d->ctl, it, (const double *)d->mresp, d->nb_mresp, nbs, maxiter, eps);              \
+           d->sorp, d->sorc, d->ctl, it, (const double *)d->mresp, d->nb_mresp, nbs);                            \
     LAUNCH(k_sor_check<N>, dim3(1), dim3(RB), 0, d->sorp, d->sorc, nbs, d->ctl, maxiter, it, eps);               \
   } while (0)
-  // single iterations that follow multi-iteration launches (the short tail of
-  // the queue, most of which return at once) carry their own stop test in the
-  // last workgroup; its device-scope fence per workgroup costs more than a
-  // separate check where single iterations are the whole queue
-#define SOR_LAUNCH1C()                                                                                            \
-  LAUNCH((k_sor_iter<1, true>), gs, dim3(TB), 0, g, d->b[B_TEMPER], d->b[B_TEMPER2], d->b[B_TEMPER3],             \
-         d->b[B_TEMPER4], d->b[B_ML], d->b[B_MLIP], d->b[B_MLIM], d->b[B_MLJP], d->b[B_MLJM], d->b[B_RHS], omega,   \
-         d->sorp, d->sorc, d->ctl, it, (const double *)d->mresp, d->nb_mresp, nbs, maxiter, eps)
+  // (a stop test made by the last workgroup of each single-iteration launch
+  // needs a device-scope fence per workgroup: a single iteration then took
+  // three times as long as with the separate check)
   for (int it = it0; it < it0 + count;) {
     const int nit = (it0 + count - it >= nmax) ? nmax : 1;
     if (nit == 3) SOR_LAUNCH(3);
     else if (nit == 2) SOR_LAUNCH(2);
-    else if (it > 0 && nmax > 1) SOR_LAUNCH1C();
     else SOR_LAUNCH(1);
     it += nit;
   }
 #undef SOR_LAUNCH
-#undef SOR_LAUNCH1C
   HCHECK(hipGetLastError());
   return 0;
 }
