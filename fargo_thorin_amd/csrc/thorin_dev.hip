@@ -4776,9 +4776,9 @@ extern "C" const char *thorin_prof_kernel_name(int kid) {
 // --------------------------------------------------------------------------
 // SOR driver
 
-// enqueue iterations it0 .. it0+count-1 (iteration + stop test each); the
-// first one also forms the initial residual when the matrix kernel has not
-// launches of sor_nit iterations while that many remain, then single ones
+// enqueue iterations it0 .. it0+count-1, each launch of iterations followed
+// by its stop test: launches of sor_nit iterations while that many remain,
+// then one launch of the rest
 static int sor_launch(thorin_ctx *c, double omega, int it0, int &count, double eps = SOREPS,
                       int maxiter = SORMAXITERS) {
   thorin_dev *d = c->dev;
@@ -4795,8 +4795,9 @@ static int sor_launch(thorin_ctx *c, double omega, int it0, int &count, double e
   }
   const int nbs = ((ns + SR_TJ - 1) / SR_TJ) * ((nr - 2 + SR_TI - 1) / SR_TI);
   const dim3 gs((unsigned)((nbs + 7) / 8 * 8));
-  // launches of sor_nit iterations while that many remain, then single ones
-  // (the iterations queued beyond the expected count are a margin)
+  // the iterations queued beyond the expected count are a margin: a launch
+  // after the solve's end returns at once, so the rest goes in one launch
+  // (an empty launch and its stop test cost about 9 us at 1024x1536)
   const int nmax = (d->sor_nit >= 1 && d->sor_nit <= 3) ? d->sor_nit : 1;
 #define SOR_LAUNCH(N)                                                                                             \
   do {                                                                                                            \
@@ -4810,7 +4811,7 @@ static int sor_launch(thorin_ctx *c, double omega, int it0, int &count, double e
   // needs a device-scope fence per workgroup: a single iteration then took
   // three times as long as with the separate check)
   for (int it = it0; it < it0 + count;) {
-    const int nit = (it0 + count - it >= nmax) ? nmax : 1;
+    const int nit = std::min(nmax, it0 + count - it);
     if (nit == 3) SOR_LAUNCH(3);
     else if (nit == 2) SOR_LAUNCH(2);
     else SOR_LAUNCH(1);
@@ -6226,13 +6227,14 @@ static int sor_expected(const thorin_ctx *c) {
 }
 
 // The launch shape that finishes a solve of `expect` iterations first: the
-// queue runs expect / n launches of n iterations and the rest one at a time
+// queue runs expect / n launches of n iterations and one launch of the rest
 // (launches queued after the stop return at once).
 static int sor_pick(const double *us_per_iter, int expect) {
   int best = 1;
   double tbest = 0.0;
   for (int n = 1; n <= 3; n++) {
-    const double t = us_per_iter[n - 1] * (double)((expect / n) * n) + us_per_iter[0] * (double)(expect % n);
+    const int rest = expect % n;
+    const double t = us_per_iter[n - 1] * (double)((expect / n) * n) + (rest ? us_per_iter[rest - 1] * rest : 0.0);
     if (n == 1 || t < tbest) { best = n; tbest = t; }
   }
   return best;
