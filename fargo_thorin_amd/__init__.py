@@ -161,7 +161,7 @@ class Disk:
         _check(lib().thorin_autotune(self._h, C.byref(t)), "thorin_autotune")
         return _tuning_dict(t)
 
-    def set_tuning(self, sor_iters_per_launch: int = 1, rad_rows: int = 16, fused_chain: int = 1) -> None:
+    def set_tuning(self, sor_iters_per_launch: int = 3, rad_rows: int = 8, fused_chain: int = 1) -> None:
         t = _capi.Tuning(sor_iters_per_launch=sor_iters_per_launch, rad_rows=rad_rows, fused_chain=fused_chain)
         _check(lib().thorin_set_tuning(self._h, C.byref(t)), "thorin_set_tuning")
 

@@ -3854,8 +3854,10 @@ extern "C" int thorin_dev_create(thorin_ctx *c) {
     // k_ring_means: one ring per workgroup (the most workgroups staging and
     // walking at once), two for the narrow rings of small grids
     d->rs_rows = c->ns < 512 ? 2 : 1;
-    d->sor_nit = 1;
-    d->rad_rows = 16;
+    // launch shapes (the results do not depend on them): what the autotuner
+    // picks at 1024x1536 and for the 16-disk ensemble
+    d->sor_nit = 3;
+    d->rad_rows = 8;
     d->visc_chain = 1;
     const size_t rs_lds = sizeof(double) * (size_t)d->rs_rows * c->ns;
     if (rs_lds > (size_t)prop.sharedMemPerBlock) {

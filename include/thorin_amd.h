@@ -285,8 +285,8 @@ void thorin_ensemble_destroy(thorin_ensemble *e);
  * any; a solve runs whole launches); thorin_ensemble_autotune picks one shape
  * for all disks (least time summed over them).  Call them between steps. */
 typedef struct thorin_tuning {
-  int sor_iters_per_launch;       /* 1, 2 or 3 (default 1) */
-  int rad_rows;                   /* 8, 16 or 32 (default 16) */
+  int sor_iters_per_launch;       /* 1, 2 or 3 (default 3) */
+  int rad_rows;                   /* 8, 16 or 32 (default 8) */
   double sor_us_per_iter[3];      /* measured: per SOR iteration, 1 / 2 / 3 per launch */
   double rad_us[3];               /* measured: one radial transport launch, 8 / 16 / 32 rings */
   int fused_chain;                /* 1 (default): SubStep1, ViscousTerms and SubStep2 (with the damping
