@@ -17,6 +17,8 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
+#include <chrono>
 #include <condition_variable>
 #include <functional>
 #include <mutex>
@@ -215,7 +217,9 @@ struct thorin_dev {
   int sor_queue_min;            // ensemble: SOR iterations to queue at least
   ForceSet spec;                // force kernels enqueued ahead for the next substep
   int heat_max;                 // heatsrc_max: set by the pebble accretion when AccretionalHeating is on
-  hipEvent_t ev_end;            // end of a substep's own work (the host waits for it)
+  unsigned *hflag;              // pinned host word: k_cfl_final writes the substep's sequence number
+                                // there after the control block's mirror (the host polls it)
+  unsigned flag_seq;            // the last sequence number given to k_cfl_final
   hipEvent_t ev_t0, ev_t1;      // the energy solve's timing (thorin_rad_solve*)
   double *h_part;               // pinned: block partials of the FP64 residual (nblk_cells)
   int device;                   // the HIP device of the context's stream
@@ -1000,14 +1004,12 @@ __global__ void __launch_bounds__(RB) __attribute__((amdgpu_waves_per_eu(4))) k_
 // one workgroup of CF_T threads: the partials and ring values are few per
 // thread, so their loads take about one memory latency
 #define CF_T 1024
-__device__ __forceinline__ void b_cfl_final(const Geo &g, DevCtl *ctl, const double *__restrict__ pstellar,
-                                                  int nstellar, double invdtreb_sq, const int *__restrict__ guard,
-                                             DevCtl *__restrict__ hout) {
+__device__ __forceinline__ void cfl_final_body(const Geo &g, DevCtl *ctl, const double *__restrict__ pstellar,
+                                               int nstellar, double invdtreb_sq, DevCtl *__restrict__ hout) {
   // the four extrema (largest S and pebble term, smallest stellar partial and
   // shear limit) are independent: each thread forms its share of all four,
   // then one reduction; min and max are exact, so the order does not matter
   __shared__ double red[4][CF_T / 64];
-  if (guard && *guard == 0) return;
   double ms = 1.0e38;
   if (nstellar > 0) ms = strided_min<CF_T>(pstellar, nstellar, ms);
   double S = -1.0, P = -1.e30, msh = 1e30;
@@ -1040,18 +1042,27 @@ __device__ __forceinline__ void b_cfl_final(const Geo &g, DevCtl *ctl, const dou
     red[3][w] = msh;
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    S = red[0][0];
-    P = red[1][0];
-    ms = red[2][0];
-    msh = red[3][0];
+  if (threadIdx.x < 64) {
+    // the waves' extrema across the first wave's lanes (the other lanes hold
+    // the values of a thread without data)
+    const int k = threadIdx.x;
+    S = k < CF_T / 64 ? red[0][k] : -1.0;
+    P = k < CF_T / 64 ? red[1][k] : -1.e30;
+    ms = k < CF_T / 64 ? red[2][k] : 1.0e38;
+    msh = k < CF_T / 64 ? red[3][k] : 1e30;
 #pragma unroll
-    for (int k = 1; k < CF_T / 64; k++) {
-      S = red[0][k] > S ? red[0][k] : S;
-      P = red[1][k] > P ? red[1][k] : P;
-      ms = red[2][k] < ms ? red[2][k] : ms;
-      msh = red[3][k] < msh ? red[3][k] : msh;
+    for (int o = 32; o > 0; o >>= 1) {
+      double u = __shfl_xor(S, o, 64);
+      S = u > S ? u : S;
+      u = __shfl_xor(P, o, 64);
+      P = u > P ? u : P;
+      u = __shfl_xor(ms, o, 64);
+      ms = u < ms ? u : ms;
+      u = __shfl_xor(msh, o, 64);
+      msh = u < msh ? u : msh;
     }
+  }
+  if (threadIdx.x == 0) {
     if (nstellar > 0) ctl->dt_stellar = ms;
     double m = 1e30;
     if (g.pebbles) {
@@ -1065,18 +1076,35 @@ __device__ __forceinline__ void b_cfl_final(const Geo &g, DevCtl *ctl, const dou
     ctl->newdt_raw = m;
   }
   if (hout) {
-    // the control block straight into the host's pinned mirror: the end of
-    // the substep needs no copy of its own before the host's sync
+    // the control block straight into the host's pinned mirror by the first
+    // wave: the end of the substep needs no copy of its own
     __syncthreads();
     const unsigned long long *src = (const unsigned long long *)ctl;
     unsigned long long *dst = (unsigned long long *)hout;
-    for (int q = threadIdx.x; q < (int)(sizeof(DevCtl) / 8); q += CF_T) dst[q] = src[q];
+    if (threadIdx.x < 64)
+      for (int q = threadIdx.x; q < (int)(sizeof(DevCtl) / 8); q += 64) dst[q] = src[q];
+  }
+}
+
+// ConditionCFL's final reduction (nothing while *guard == 0: the SOR has not
+// converged) and, with hflag, the substep's sequence number written into the
+// host's flag word after the mirror by the same wave, behind a system-scope
+// fence: the host polls the word instead of waiting for an event, whose
+// system-scope release at this point held the next kernel back by ~6 us
+__device__ __forceinline__ void b_cfl_final(const Geo &g, DevCtl *ctl, const double *__restrict__ pstellar,
+                                            int nstellar, double invdtreb_sq, const int *__restrict__ guard,
+                                            DevCtl *__restrict__ hout, unsigned *__restrict__ hflag, unsigned seq) {
+  if (!guard || *guard != 0) cfl_final_body(g, ctl, pstellar, nstellar, invdtreb_sq, hout);
+  if (hflag && threadIdx.x < 64) {
+    __threadfence_system();
+    if (threadIdx.x == 0) __hip_atomic_store(hflag, seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
   }
 }
 __global__ void __launch_bounds__(CF_T) k_cfl_final(Geo g, DevCtl *ctl, const double *__restrict__ pstellar,
                                                   int nstellar, double invdtreb_sq, const int *__restrict__ guard,
-                                                  DevCtl *__restrict__ hout) {
-  b_cfl_final(g, ctl, pstellar, nstellar, invdtreb_sq, guard, hout);
+                                                  DevCtl *__restrict__ hout, unsigned *__restrict__ hflag,
+                                                  unsigned seq) {
+  b_cfl_final(g, ctl, pstellar, nstellar, invdtreb_sq, guard, hout, hflag, seq);
 }
 
 
@@ -3877,7 +3905,6 @@ extern "C" int thorin_dev_create(thorin_ctx *c) {
                                (int)eos_need));
   }
   HCHECK(hipStreamCreateWithFlags(&d->s, hipStreamNonBlocking));
-  HCHECK(hipEventCreateWithFlags(&d->ev_end, hipEventDisableTiming));
   d->prof_pend = new std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>>();
   d->prof_pool = new std::vector<hipEvent_t>();
   const size_t fb = (d->nf + 64) * sizeof(double);
@@ -4023,6 +4050,9 @@ extern "C" int thorin_dev_create(thorin_ctx *c) {
   HCHECK(hipMalloc(&d->ctl, sizeof(DevCtl)));
   HCHECK(hipMemsetAsync(d->ctl, 0, sizeof(DevCtl), d->s));
   HCHECK(hipHostMalloc(&d->hctl, sizeof(DevCtl), hipHostMallocDefault));
+  HCHECK(hipHostMalloc((void **)&d->hflag, 64, hipHostMallocDefault));
+  *(volatile unsigned *)d->hflag = 0;
+  d->flag_seq = 0;
   memset(d->hctl, 0, sizeof(DevCtl));
   d->nblk_cells = ((c->ns + TB - 1) / TB) * c->nr;
   HCHECK(hipMalloc(&d->red, sizeof(double) * (size_t)d->nblk_cells * (2 + 2 * 8)));
@@ -4074,6 +4104,7 @@ extern "C" void thorin_dev_destroy(thorin_ctx *c) {
   if (d->g.Nshift) (void)hipFree(d->g.Nshift);
   if (d->ctl) (void)hipFree(d->ctl);
   if (d->hctl) (void)hipHostFree(d->hctl);
+  if (d->hflag) (void)hipHostFree(d->hflag);
   if (d->red) (void)hipFree(d->red);
   if (d->sorp) (void)hipFree(d->sorp);
   if (d->trig) (void)hipFree(d->trig);
@@ -4095,7 +4126,6 @@ extern "C" void thorin_dev_destroy(thorin_ctx *c) {
     for (auto e : *d->prof_pool) (void)hipEventDestroy(e);
     delete d->prof_pool;
   }
-  if (d->ev_end) (void)hipEventDestroy(d->ev_end);
   if (d->ev_t0) (void)hipEventDestroy(d->ev_t0);
   if (d->ev_t1) (void)hipEventDestroy(d->ev_t1);
   if (d->h_part) (void)hipHostFree(d->h_part);
@@ -4700,8 +4730,11 @@ static int dev_sync(thorin_dev *d) {
   return 0;
 }
 
-// the end of the substep's own work: the event ev_end on the disk's stream,
-// or a marker in the recorded queue
+// the end of the substep's own work.  One disk: k_cfl_final writes the
+// substep's sequence number into the pinned flag word, and dev_wait_mark
+// polls it (no event: recording one there gives k_cfl_final a system-scope
+// release that holds the next kernel back).  In an ensemble: a marker op,
+// recorded as an event by the merged flush.
 static int dev_mark(thorin_dev *d) {
   if (d->rec) {
     RecOp op;
@@ -4712,16 +4745,30 @@ static int dev_mark(thorin_dev *d) {
     op.block = dim3(1);
     op.shmem = 0;
     d->rec->q.push_back(std::move(op));
-    return 0;
   }
-  HCHECK(hipEventRecord(d->ev_end, d->s));
   return 0;
 }
 
-// wait for the marked end of the substep (dev_mark)
+// wait for the marked end of the substep (dev_mark).  Should the stream
+// finish without the flag word showing the substep's number, that is an
+// error (checked after 2 ms, then every ~1000 polls).
 static int dev_wait_mark(thorin_dev *d) {
   if (d->rec) return rec_wait(d->rec);
-  HCHECK(hipEventSynchronize(d->ev_end));
+  const unsigned seq = d->flag_seq;
+  volatile unsigned *f = d->hflag;
+  const auto t0 = std::chrono::steady_clock::now();
+  for (unsigned long k = 1; *f != seq; k++) {
+    __builtin_ia32_pause();
+    if ((k & 1023) == 0 && std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(2)) {
+      const hipError_t e = hipStreamQuery(d->s);
+      if (e == hipSuccess && *f != seq) {
+        thorin_set_error("the end-of-substep kernel finished without writing the host flag");
+        return -1;
+      }
+      if (e != hipSuccess && e != hipErrorNotReady) HCHECK(e);
+    }
+  }
+  std::atomic_thread_fence(std::memory_order_acquire);
   return 0;
 }
 
@@ -5284,7 +5331,7 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
       LAUNCH(k_eos_cfl, dim3((nr + 7) / 8 * 8), rb, eos_lds(d, 0), d->g, d->b[B_DENS], d->b[B_ENERGY],
              d->b[B_TEMPER], d->b[B_CS], d->b[B_VRAD], d->b[B_VTHETA], 0, (const int *)nullptr);
       LAUNCH(k_cfl_final, dim3(1), dim3(CF_T), 0, d->g, d->ctl, (const double *)nullptr, 0, invdtreb_sq,
-             (const int *)nullptr, (DevCtl *)nullptr);
+             (const int *)nullptr, (DevCtl *)nullptr, (unsigned *)nullptr, 0u);
       HCHECK(hipGetLastError());
       DCOPY(d->hctl, d->ctl, sizeof(DevCtl), hipMemcpyDeviceToHost);
       DSYNC();
@@ -5700,8 +5747,11 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
       // the mirror then reads sor_done = 0 and the solve is finished from
       // the host.
       d->hctl->sor_done = 0;
+      // outside an ensemble the host polls the flag word (dev_wait_mark)
+      unsigned *hflag = d->rec ? nullptr : d->hflag;
+      const unsigned seq = d->rec ? 0u : ++d->flag_seq;
       LAUNCH(k_cfl_final, dim3(1), dim3(CF_T), 0, d->g, d->ctl, (const double *)d->pstellar, d->nblk_cells,
-             d->cfl_invdtreb, guard, d->hctl);
+             d->cfl_invdtreb, guard, d->hctl, hflag, seq);
       HCHECK(hipGetLastError());
       d->spec.valid = 0;
       if (peb || gacc) {   // (the gas accretion changes the density before the forces)
@@ -6618,7 +6668,7 @@ extern "C" int thorin_rad_system(thorin_ctx *c, double dt, double *dt_used) {
     LAUNCH(k_eos_cfl, dim3((nr + 7) / 8 * 8), rb, eos_lds(d, 0), gg, d->b[B_DENS], d->b[B_ENERGY], d->b[B_TEMPER],
            d->b[B_CS], d->b[B_VRAD], d->b[B_VTHETA], 0, (const int *)nullptr);
     LAUNCH(k_cfl_final, dim3(1), dim3(CF_T), 0, gg, d->ctl, (const double *)nullptr, 0, 0.0, (const int *)nullptr,
-           (DevCtl *)nullptr);
+           (DevCtl *)nullptr, (unsigned *)nullptr, 0u);
     HCHECK(hipMemcpyAsync(d->hctl, d->ctl, sizeof(DevCtl), hipMemcpyDeviceToHost, d->s));
     HCHECK(hipStreamSynchronize(d->s));
     dt = d->hctl->newdt_raw;
