@@ -10,9 +10,11 @@ in HBM.  The first warmup step includes the reference's one-off search of
 the SOR relaxation parameter and is never timed.
 
 Printed: one JSON line with steps/s, the HIP-event roofline of the dominant
-kernel measured in the timed region, and the reference's own sequential CPU
-build (oracle/_ref/thorin) timed on this host on two DTs of the same
-workload.
+kernel (the one moving the most algorithmic bytes per step, timed in a
+region of its own after the throughput region), the whole step's roofline,
+the reference's own CPU builds (oracle/_ref: sequential, OpenMP, MPI) timed
+on this host on a few DTs of the same workload, and the studies (energy
+solve at 2048x3072, two-fluid step, 16-disk ensemble).
 """
 from __future__ import annotations
 
@@ -86,7 +88,8 @@ def parse():
                     help="ensemble study without its warm-start part (the disks' stepping alone)")
     ap.add_argument("--study", choices=["sor", "two-fluid", "ensemble"], default=None,
                     help="run only this study and print its JSON (for profiling)")
-    ap.add_argument("--roofline-kernel", default="auto")
+    ap.add_argument("--roofline-kernel", default="auto",
+                    help="kernel of the roofline line (auto: the most algorithmic bytes per step)")
     ap.add_argument("--copy-bandwidth", action="store_true",
                     help="only time a 1 GiB device copy with HIP events and print its GB/s")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -554,11 +557,17 @@ def main():
     for _ in range(warm - 1):
         d.algogas(sysp)
     warm_prof = d.profile_read()
+    nit = tuning["sor_iters_per_launch"]
     kname = args.roofline_kernel
     if kname == "auto":
-        kname = max(warm_prof, key=lambda k: warm_prof[k][0])
+        # the kernel that moves the most algorithmic bytes per step (the
+        # roofline is an HBM one; by time the three largest kernels are
+        # within a few microseconds of each other)
+        def bytes_per_step(k):
+            per_launch = ALG_BYTES_PER_CELL.get(k, 0) * (nit if k == "k_sor_iter" else 1)
+            return per_launch * warm_prof[k][1]
+        kname = max(warm_prof, key=bytes_per_step)
     d.profile_off()
-    nit = tuning["sor_iters_per_launch"]
 
     def run(n):
         """n DTs between two device syncs: wall seconds, substeps, SOR

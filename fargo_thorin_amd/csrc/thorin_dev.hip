@@ -239,6 +239,7 @@ struct thorin_dev {
   long long prof_cnt[64];
   struct Rec *rec;              // ensemble mode: operations are recorded, not issued (thorin_ensemble_*)
   int tt_npt_min;               // smallest register count per thread of k_transport_theta (ensemble)
+  int ec_min;                   // smallest cells per thread of the end-of-substep kernels (ensemble)
 };
 
 // kernels that the event timer knows, by launch name
@@ -763,7 +764,7 @@ __device__ __forceinline__ double ordered_ring_sum(const double *sh, int ns) {
 //    pebble counterparts from the Stokes-number ring means of k_peb_cfl.
 // The ring sums are sequential in j; lanes 0, 1, 2 of the first wave walk
 // them side by side while the other waves wait.  The CFL terms that need no
-// ring mean are formed before the sums (EC_MAX cells per thread) and the sum
+// ring mean are formed before the sums (EC cells per thread) and the sum
 // under the square root is completed afterwards in the reference's order.
 // With do_eos = 0 the temperature and sound speed already in memory are used.
 //
@@ -775,6 +776,8 @@ __device__ __forceinline__ double ordered_ring_sum(const double *sh, int ns) {
 // sector) and vr of rings i and i+1 with the same expressions as the owner
 // of each cell, so every value equals what the separate kernels give.
 #define EC_MAX 12              // cells per thread: NSEC <= RB * EC_MAX
+// instances of EC cells per thread (3, 6, 12): the per-cell CFL terms live
+// in registers, so the smallest instance with NSEC <= RB * EC runs
 // The kernel runs at four waves per SIMD (at most 128 registers): with four
 // workgroups of four waves per CU, 1024 rings are all resident at once, and
 // the sequential ring sums of every ring run in one overlapped round rather
@@ -786,7 +789,7 @@ struct VelSrc {                 // VEL: the momenta of the transport and the dam
   int damp;                     // ZEROVRAD damping on
 };
 
-template <bool VEL>
+template <bool VEL, int EC>
 __device__ __forceinline__ void b_eos_cfl_t(const Geo &g, const double *__restrict__ rho, const double *__restrict__ e,
                                             double *__restrict__ T, double *__restrict__ cs, double *__restrict__ vr,
                                             double *__restrict__ vt, int do_eos, const int *__restrict__ guard,
@@ -813,9 +816,9 @@ __device__ __forceinline__ void b_eos_cfl_t(const Geo &g, const double *__restri
   // pre: the CFL terms of the sound speed and of vr; adr: the radial
   // shear term; the azimuthal shear needs vtheta of the next sector, taken
   // from LDS after the barrier
-  double pre[EC_MAX], adr[EC_MAX];
+  double pre[EC], adr[EC];
 #pragma unroll
-  for (int k = 0; k < EC_MAX; k++) {
+  for (int k = 0; k < EC; k++) {
     const int j = tid + k * RB;
     pre[k] = 0.0;
     adr[k] = 0.0;
@@ -946,7 +949,7 @@ __device__ __forceinline__ void b_eos_cfl_t(const Geo &g, const double *__restri
     const double invdt5 = viscous ? visc * 4. / g.cfl_pow2[i] : 1e-30;
     const double vm = vmoy;
 #pragma unroll
-    for (int k = 0; k < EC_MAX; k++) {
+    for (int k = 0; k < EC; k++) {
       const int j = tid + k * RB;
       if (j < ns) {
         const double vres = g.fast_transport ? s_vt[j] - vm : s_vt[j];
@@ -967,32 +970,34 @@ __device__ __forceinline__ void b_eos_cfl_t(const Geo &g, const double *__restri
   m = block_max<RB>(m, red);
   if (tid == 0) g.ringmax[i] = m;
 }
+template <int EC>
 __device__ __forceinline__ void b_eos_cfl(const Geo &g, const double *__restrict__ rho, const double *__restrict__ e,
                                           double *__restrict__ T, double *__restrict__ cs, const double *__restrict__ vr,
                                           const double *__restrict__ vt, int do_eos, const int *__restrict__ guard) {
   const VelSrc none = {nullptr, nullptr, nullptr, nullptr, 0.0, 0};
-  b_eos_cfl_t<false>(g, rho, e, T, cs, const_cast<double *>(vr), const_cast<double *>(vt), do_eos, guard, none);
+  b_eos_cfl_t<false, EC>(g, rho, e, T, cs, const_cast<double *>(vr), const_cast<double *>(vt), do_eos, guard, none);
 }
-__global__ void __launch_bounds__(RB) __attribute__((amdgpu_waves_per_eu(4))) k_eos_cfl(Geo g, const double *__restrict__ rho, const double *__restrict__ e,
-                                                double *__restrict__ T, double *__restrict__ cs,
-                                                const double *__restrict__ vr, const double *__restrict__ vt,
-                                                int do_eos, const int *__restrict__ guard) {
-  b_eos_cfl(g, rho, e, T, cs, vr, vt, do_eos, guard);
+template <int EC>
+__global__ void __launch_bounds__(RB) __attribute__((amdgpu_waves_per_eu(4)))
+k_eos_cfl(Geo g, const double *__restrict__ rho, const double *__restrict__ e, double *__restrict__ T,
+          double *__restrict__ cs, const double *__restrict__ vr, const double *__restrict__ vt, int do_eos,
+          const int *__restrict__ guard) {
+  b_eos_cfl<EC>(g, rho, e, T, cs, vr, vt, do_eos, guard);
 }
+template <int EC>
 __device__ __forceinline__ void b_eos_cfl_vel(const Geo &g, const double *__restrict__ rho,
                                               const double *__restrict__ e, double *__restrict__ T,
                                               double *__restrict__ cs, double *__restrict__ vr, double *__restrict__ vt,
                                               const int *__restrict__ guard, const VelSrc &vs) {
-  b_eos_cfl_t<true>(g, rho, e, T, cs, vr, vt, 1, guard, vs);
+  b_eos_cfl_t<true, EC>(g, rho, e, T, cs, vr, vt, 1, guard, vs);
 }
-__global__ void __launch_bounds__(RB) __attribute__((amdgpu_waves_per_eu(4))) k_eos_cfl_vel(Geo g, const double *__restrict__ rho,
-                                                    const double *__restrict__ e, double *__restrict__ T,
-                                                    double *__restrict__ cs, double *__restrict__ vr,
-                                                    double *__restrict__ vt, const int *__restrict__ guard,
-                                                    VelSrc vs) {
-  b_eos_cfl_vel(g, rho, e, T, cs, vr, vt, guard, vs);
+template <int EC>
+__global__ void __launch_bounds__(RB) __attribute__((amdgpu_waves_per_eu(4)))
+k_eos_cfl_vel(Geo g, const double *__restrict__ rho, const double *__restrict__ e, double *__restrict__ T,
+              double *__restrict__ cs, double *__restrict__ vr, double *__restrict__ vt,
+              const int *__restrict__ guard, VelSrc vs) {
+  b_eos_cfl_vel<EC>(g, rho, e, T, cs, vr, vt, guard, vs);
 }
-
 
 // ConditionCFL's minimum (SourceEuler.c:620-676): the cell term at the
 // largest S of rings 1..NRAD-1, with the pebble limit invdtpeb_sq
@@ -3825,6 +3830,21 @@ static inline size_t eos_lds(const thorin_dev *d, int do_eos) {
   return sizeof(double) * (size_t)(d->ns + EC_PAD) * ((d->g.alpha_flock && do_eos) ? 3 : 2);
 }
 
+// cells per thread of the end-of-substep kernels: the smallest instance that
+// holds NSEC (an ensemble runs its disks with one instance)
+static inline int eos_ec(const thorin_dev *d) {
+  int ec = d->ns <= 3 * RB ? 3 : (d->ns <= 6 * RB ? 6 : 12);
+  if (ec < d->ec_min) ec = d->ec_min;
+  return ec;
+}
+#define EOS_LAUNCH(KERN, grid, block, shmem, ...)                          \
+  do {                                                                    \
+    const int ec_ = eos_ec(d);                                            \
+    if (ec_ == 3) LAUNCH(KERN<3>, grid, block, shmem, __VA_ARGS__);       \
+    else if (ec_ == 6) LAUNCH(KERN<6>, grid, block, shmem, __VA_ARGS__);  \
+    else LAUNCH(KERN<12>, grid, block, shmem, __VA_ARGS__);               \
+  } while (0)
+
 // 1D grid of the XCD-banded cell blocks over `rings` rings (xcd_cell), and
 // the number of logical blocks (= block partials a reduction kernel writes)
 static inline int cell_count(const thorin_dev *d, int rings) { return d->g.ncb * rings; }
@@ -3900,9 +3920,11 @@ extern "C" int thorin_dev_create(thorin_ctx *c) {
       thorin_set_error("NSEC too large for the end-of-substep kernel's LDS");
       return -1;
     }
-    HCHECK(hipFuncSetAttribute((const void *)k_eos_cfl, hipFuncAttributeMaxDynamicSharedMemorySize, (int)eos_need));
-    HCHECK(hipFuncSetAttribute((const void *)k_eos_cfl_vel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)eos_need));
+    const void *ek[6] = {(const void *)k_eos_cfl<3>,     (const void *)k_eos_cfl<6>,
+                         (const void *)k_eos_cfl<12>,    (const void *)k_eos_cfl_vel<3>,
+                         (const void *)k_eos_cfl_vel<6>, (const void *)k_eos_cfl_vel<12>};
+    for (int k = 0; k < 6; k++)
+      HCHECK(hipFuncSetAttribute(ek[k], hipFuncAttributeMaxDynamicSharedMemorySize, (int)eos_need));
   }
   HCHECK(hipStreamCreateWithFlags(&d->s, hipStreamNonBlocking));
   d->prof_pend = new std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>>();
@@ -4473,8 +4495,12 @@ struct Batch {
     static constexpr auto body = &BODY;        \
     static constexpr int nt = NT;              \
   };
-BATCH_BODY(k_eos_cfl, b_eos_cfl, RB)
-BATCH_BODY(k_eos_cfl_vel, b_eos_cfl_vel, RB)
+BATCH_BODY(k_eos_cfl<3>, b_eos_cfl<3>, RB)
+BATCH_BODY(k_eos_cfl<6>, b_eos_cfl<6>, RB)
+BATCH_BODY(k_eos_cfl<12>, b_eos_cfl<12>, RB)
+BATCH_BODY(k_eos_cfl_vel<3>, b_eos_cfl_vel<3>, RB)
+BATCH_BODY(k_eos_cfl_vel<6>, b_eos_cfl_vel<6>, RB)
+BATCH_BODY(k_eos_cfl_vel<12>, b_eos_cfl_vel<12>, RB)
 BATCH_BODY(k_cfl_final, b_cfl_final, CF_T)
 BATCH_BODY((k_forces<false, 1>), (b_forces<false, 1>), TB)
 BATCH_BODY((k_forces<false, 8>), (b_forces<false, 8>), TB)
@@ -5328,7 +5354,7 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
       if (peb)   // PebbleStokesNumbers + CriticalCharTime (SourceEuler.c:236, 241)
         LAUNCH(k_peb_cfl, dim3(nr), rb, sizeof(double) * 2 * ns, d->g, d->b[B_DENS], d->b[B_PSIZE], d->b[B_TAU],
                d->b[B_PVR], d->b[B_PVT], d->b[B_VRAD], d->b[B_VTHETA], (const int *)nullptr);
-      LAUNCH(k_eos_cfl, dim3((nr + 7) / 8 * 8), rb, eos_lds(d, 0), d->g, d->b[B_DENS], d->b[B_ENERGY],
+      EOS_LAUNCH(k_eos_cfl, dim3((nr + 7) / 8 * 8), rb, eos_lds(d, 0), d->g, d->b[B_DENS], d->b[B_ENERGY],
              d->b[B_TEMPER], d->b[B_CS], d->b[B_VRAD], d->b[B_VTHETA], 0, (const int *)nullptr);
       LAUNCH(k_cfl_final, dim3(1), dim3(CF_T), 0, d->g, d->ctl, (const double *)nullptr, 0, invdtreb_sq,
              (const int *)nullptr, (DevCtl *)nullptr, (unsigned *)nullptr, 0u);
@@ -5732,10 +5758,10 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
       // minima of the next substep (its planets are those of now)
       if (fuse_vel) {
         const VelSrc vs = {d->b[B_RP], d->b[B_RM], d->b[B_TP], d->b[B_TM], dt, p->damping_on};
-        LAUNCH(k_eos_cfl_vel, dim3((nr + 7) / 8 * 8), rb, eos_lds(d, 1), d->g, d->b[B_DENS], d->b[B_ENERGY], d->b[B_TEMPER],
+        EOS_LAUNCH(k_eos_cfl_vel, dim3((nr + 7) / 8 * 8), rb, eos_lds(d, 1), d->g, d->b[B_DENS], d->b[B_ENERGY], d->b[B_TEMPER],
                d->b[B_CS], d->b[B_VRAD], d->b[B_VTHETA], guard, vs);
       } else {
-        LAUNCH(k_eos_cfl, dim3((nr + 7) / 8 * 8), rb, eos_lds(d, 1), d->g, d->b[B_DENS], d->b[B_ENERGY],
+        EOS_LAUNCH(k_eos_cfl, dim3((nr + 7) / 8 * 8), rb, eos_lds(d, 1), d->g, d->b[B_DENS], d->b[B_ENERGY],
                d->b[B_TEMPER], d->b[B_CS], d->b[B_VRAD], d->b[B_VTHETA], 1, guard);
       }
       // the planets' step, overlapped with the kernels enqueued so far
@@ -6068,6 +6094,10 @@ extern "C" int thorin_ensemble_create(thorin_ctx *const *disks, int n, thorin_en
     npt = v > npt ? v : npt;
   }
   for (int k = 0; k < n; k++) disks[k]->dev->tt_npt_min = npt;
+  // and of the end-of-substep kernels
+  int ec = 3;
+  for (int k = 0; k < n; k++) ec = std::max(ec, eos_ec(disks[k]->dev));
+  for (int k = 0; k < n; k++) disks[k]->dev->ec_min = ec;
   if (hipStreamCreateWithFlags(&e->s, hipStreamNonBlocking) != hipSuccess) {
     thorin_set_error("thorin_ensemble_create: stream creation failed");
     delete[] e->c;
@@ -6124,7 +6154,10 @@ extern "C" void thorin_ensemble_destroy(thorin_ensemble *e) {
   for (auto &t : e->workers) t.join();
   (void)hipStreamSynchronize(e->s);
   for (int k = 0; k < e->n; k++)
-    if (e->c[k] && e->c[k]->dev) e->c[k]->dev->tt_npt_min = 0;
+    if (e->c[k] && e->c[k]->dev) {
+      e->c[k]->dev->tt_npt_min = 0;
+      e->c[k]->dev->ec_min = 0;
+    }
   for (auto &x : e->graphs) {
     (void)hipGraphExecDestroy(x.x);
     (void)hipGraphDestroy(x.g);
@@ -6665,7 +6698,7 @@ extern "C" int thorin_rad_system(thorin_ctx *c, double dt, double *dt_used) {
     // the hydro CFL step of the current fields (ConditionCFL without planets)
     Geo gg = d->g;
     gg.pebbles = 0;
-    LAUNCH(k_eos_cfl, dim3((nr + 7) / 8 * 8), rb, eos_lds(d, 0), gg, d->b[B_DENS], d->b[B_ENERGY], d->b[B_TEMPER],
+    EOS_LAUNCH(k_eos_cfl, dim3((nr + 7) / 8 * 8), rb, eos_lds(d, 0), gg, d->b[B_DENS], d->b[B_ENERGY], d->b[B_TEMPER],
            d->b[B_CS], d->b[B_VRAD], d->b[B_VTHETA], 0, (const int *)nullptr);
     LAUNCH(k_cfl_final, dim3(1), dim3(CF_T), 0, gg, d->ctl, (const double *)nullptr, 0, 0.0, (const int *)nullptr,
            (DevCtl *)nullptr, (unsigned *)nullptr, 0u);
