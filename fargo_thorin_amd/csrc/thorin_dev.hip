@@ -369,13 +369,17 @@ __device__ __forceinline__ double opacity_zhu12(double rho, double T) {
 // al. (2003) dust and Malygin et al. (2014) gas.  The dust coefficients and
 // evaporation temperatures are the published tables the reference holds
 // (Opacity2.c:107-137).
-__device__ __forceinline__ double sm_interp(const double *x, const double *y, int n, double x0) {   // Opacity2.c:28-40
-  if (n <= 1) return y[0];
-  if (x0 < x[0]) return y[0];
-  if (x0 > x[n - 1]) return y[n - 1];
+// interpolate (Opacity2.c:28-40) of x0 in x[0..n-1] with y = column c of
+// the row-major table tab of ncol columns
+__device__ __forceinline__ double sm_interp_col(const double *x, const double *tab, int ncol, int c, int n,
+                                                double x0) {
+  if (n <= 1) return tab[c];
+  if (x0 < x[0]) return tab[c];
+  if (x0 > x[n - 1]) return tab[(n - 1) * ncol + c];
   int i = 1;
   while ((x[i] < x0) && (i < n - 1)) i++;
-  return y[i - 1] + (y[i] - y[i - 1]) * (x0 - x[i - 1]) / (x[i] - x[i - 1]);
+  const double y0 = tab[(i - 1) * ncol + c], y1 = tab[i * ncol + c];
+  return y0 + (y1 - y0) * (x0 - x[i - 1]) / (x[i] - x[i - 1]);
 }
 
 // opacity_malygin (Opacity2.c:44-95), Rosseland table: bilinear in the
@@ -427,28 +431,32 @@ __device__ __forceinline__ double opacity_sm03(const Geo &g, double rho, double 
   static const double logrr[8] = {-18.0, -16.0, -14.0, -12.0, -10.0, -8.0, -6.0, -4.0};
   if (T_dust <= 0.9) return 0.9;
   const double logrho = log10(rho);
-  double T_ev[4], temp[8];
-  for (int i = 0; i < 4; i++) {
-    for (int j = 0; j < 8; j++) temp[j] = logtt[j][i];
-    T_ev[i] = pow(10.0, sm_interp(logrr, temp, 8, logrho));
-  }
-  double T[5], dT[5];
-  T[0] = T_ev[0]; T[1] = 275.0; T[2] = 425.0; T[3] = 680.0;
+  // the evaporation temperatures: column c of logtt interpolated in log rho
+  // (the arrays below are indexed by constants or selected, so they stay in
+  // registers)
+  double T_ev[4];
+#pragma unroll
+  for (int c = 0; c < 4; c++) T_ev[c] = pow(10.0, sm_interp_col(logrr, &logtt[0][0], 4, c, 8, logrho));
   const double tmax1 = T_ev[1] > T_ev[2] ? T_ev[1] : T_ev[2];
   const double tmax2 = T_ev[2] > T_ev[3] ? T_ev[2] : T_ev[3];
-  T[4] = tmax1 < tmax2 ? tmax1 : tmax2;
-  dT[0] = 5.0; dT[1] = 5.0; dT[2] = 15.0; dT[3] = 5.0; dT[4] = 100.0;
+  const double T4 = tmax1 < tmax2 ? tmax1 : tmax2;
+  // T[k] and dT[k] of the reference's five regime borders
+  auto Tk = [&](int k) { return k == 0 ? T_ev[0] : (k == 1 ? 275.0 : (k == 2 ? 425.0 : (k == 3 ? 680.0 : T4))); };
+  auto dTk = [](int k) { return k == 2 ? 15.0 : (k == 4 ? 100.0 : 5.0); };
   int KK = 5;
-  if (T_dust <= (T[0] + dT[0])) KK = 0;
+  if (T_dust <= (Tk(0) + dTk(0))) KK = 0;
+#pragma unroll
   for (int i = 1; i < 5; i++)
-    if ((T_dust > (T[i - 1] + dT[i - 1])) && (T_dust <= (T[i] + dT[i]))) KK = i;
+    if ((T_dust > (Tk(i - 1) + dTk(i - 1))) && (T_dust <= (Tk(i) + dTk(i)))) KK = i;
   double aKext = 0.0;
   if (KK != 5) {
     int smooth = 0;
+#pragma unroll
     for (int i = 0; i < 5; i++)       // the reference passes the double difference to the int abs()
-      if (abs((int)(T_dust - T[i])) < dT[i]) smooth = 1;
+      if (abs((int)(T_dust - Tk(i))) < dTk(i)) smooth = 1;
+    const double TK = Tk(KK), dTK = dTk(KK);
     if (smooth == 1) {
-      const double T1 = T[KK] - dT[KK], T2 = T[KK] + dT[KK], TD = T_dust - T[KK];
+      const double T1 = TK - dTK, T2 = TK + dTK, TD = T_dust - TK;
       int j = (KK + 1) * 6;
       const double aKrL = ((((eD[j - 1] * T1 + eD[j - 2]) * T1 + eD[j - 3]) * T1 + eD[j - 4]) * T1 + eD[j - 5]) * T1 +
                           eD[j - 6];
@@ -458,7 +466,7 @@ __device__ __forceinline__ double opacity_sm03(const Geo &g, double rho, double 
         aKrR = ((((eD[j - 1] * T2 + eD[j - 2]) * T2 + eD[j - 3]) * T2 + eD[j - 4]) * T2 + eD[j - 5]) * T2 + eD[j - 6];
       }
       const double AA = 0.5e0 * (aKrL - aKrR), BB = 0.5e0 * (aKrL + aKrR);
-      const double FF = DPI / 2.0 / dT[KK];
+      const double FF = DPI / 2.0 / dTK;
       aKext = BB - AA * sin(FF * TD);
     } else {
       const int j = (KK + 1) * 6;
@@ -470,9 +478,13 @@ __device__ __forceinline__ double opacity_sm03(const Geo &g, double rho, double 
   return aKext > kappa_gas ? aKext : kappa_gas;
 }
 
+// SM: the SM03 table law compiled in (its table lookups need more scalar
+// registers than SubStep3 has to spare; the other laws run an instance
+// without it)
+template <bool SM = true>
 __device__ __forceinline__ double opacity_law(const Geo &g, double rho, double T) {
   switch (g.opacity_law) {
-    case THORIN_OPA_SM03: return opacity_sm03(g, rho, T);
+    case THORIN_OPA_SM03: return SM ? opacity_sm03(g, rho, T) : 0.0;
     case THORIN_OPA_LP85: return opacity_lp85(g, rho, T);
     case THORIN_OPA_ZHU12: return opacity_zhu12(rho, T);
     case THORIN_OPA_CONST: return g.parametricopacity;
@@ -2635,6 +2647,7 @@ __device__ __forceinline__ double qplus_of(const Geo &g, int i, double rho, cons
   return 0.0;
 }
 
+template <bool SM>
 __device__ __forceinline__ void b_substep3(const Geo &g, const double *__restrict__ vr, const double *__restrict__ vt,
                                                  const double *__restrict__ rho, const double *__restrict__ eint,
                                                  double *__restrict__ divv, double *__restrict__ Trr,
@@ -2709,7 +2722,7 @@ __device__ __forceinline__ void b_substep3(const Geo &g, const double *__restric
     } else {
       const double H = cl * g.OmegaInv[i] * g.sqrt_adiabind_inv;
       const double voldens = rl / H * g.sqrt2pi_inv;
-      o = opacity_law(g, voldens * g.rho2cgs, tl * g.t2si) * g.opa2cu;
+      o = opacity_law<SM>(g, voldens * g.rho2cgs, tl * g.t2si) * g.opa2cu;
     }
     opa[l] = o;
     const double tau = g.opacitydrop * 0.5 * o * rl;
@@ -2747,6 +2760,7 @@ __device__ __forceinline__ void b_substep3(const Geo &g, const double *__restric
   dtmin = block_min<TB>(dtmin, red);
   if (threadIdx.x == 0) pmin[L] = dtmin;
 }
+template <bool SM>
 __global__ void __launch_bounds__(TB) k_substep3(Geo g, const double *__restrict__ vr, const double *__restrict__ vt,
                                                  const double *__restrict__ rho, const double *__restrict__ eint,
                                                  double *__restrict__ divv, double *__restrict__ Trr,
@@ -2756,7 +2770,8 @@ __global__ void __launch_bounds__(TB) k_substep3(Geo g, const double *__restrict
                                                  double *__restrict__ qminus, double *__restrict__ D,
                                                  double *__restrict__ pmin, int fill_opacity, int nsum,
                                                  CrashSnap sn) {
-  b_substep3(g, vr, vt, rho, eint, divv, Trr, Trp, Tpp, qplus, T, cs, opa, qminus, D, pmin, fill_opacity, nsum, sn);
+  b_substep3<SM>(g, vr, vt, rho, eint, divv, Trr, Trp, Tpp, qplus, T, cs, opa, qminus, D, pmin, fill_opacity, nsum,
+                 sn);
 }
 
 
@@ -3837,6 +3852,12 @@ static inline int eos_ec(const thorin_dev *d) {
   if (ec < d->ec_min) ec = d->ec_min;
   return ec;
 }
+// SubStep3 with the SM03 law compiled in only where the disk uses it
+#define SUB3_LAUNCH(grid, block, shmem, ...)                                              \
+  do {                                                                                    \
+    if (d->g.opacity_law == THORIN_OPA_SM03) LAUNCH(k_substep3<true>, grid, block, shmem, __VA_ARGS__); \
+    else LAUNCH(k_substep3<false>, grid, block, shmem, __VA_ARGS__);                      \
+  } while (0)
 #define EOS_LAUNCH(KERN, grid, block, shmem, ...)                          \
   do {                                                                    \
     const int ec_ = eos_ec(d);                                            \
@@ -4520,7 +4541,8 @@ BATCH_BODY(k_divstress, b_divstress, TB)
 BATCH_BODY(k_visc_update, b_visc_update, TB)
 BATCH_BODY(k_substep2, b_substep2, TB)
 BATCH_BODY(k_visc_chain, b_visc_chain, VC_T)
-BATCH_BODY(k_substep3, b_substep3, TB)
+BATCH_BODY(k_substep3<false>, b_substep3<false>, TB)
+BATCH_BODY(k_substep3<true>, b_substep3<true>, TB)
 BATCH_BODY(k_matrix, b_matrix, TB)
 BATCH_BODY(k_sor_resid0, b_sor_resid0, TB)
 BATCH_BODY((k_sor_iter<1, SOR_NT(1)>), (b_sor_iter<1, SOR_NT(1)>), SOR_NT(1))
@@ -5643,10 +5665,10 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
     {
       const int nsum = (nr + TB / 64 - 1) / (TB / 64);   // ring-mean blocks, one ring per wave
       const int nsum8 = (nsum + 7) / 8 * 8;
-      LAUNCH(k_substep3, dim3(gall.x + (unsigned)nsum8), tb, 0, d->g, d->b[B_VRAD], d->b[B_VTHETA], d->b[B_DENS],
-             d->b[B_EINT], d->b[B_DIVV], (double *)nullptr, (double *)nullptr, (double *)nullptr, d->b[B_QPLUS],
-             d->b[B_TEMPER], d->b[B_CS], d->b[B_OPACITY], d->b[B_QMINUS], d->b[B_DIFF], d->pstellar,
-             d->opa_filled ? 0 : 1, nsum8, snap3);
+      SUB3_LAUNCH(dim3(gall.x + (unsigned)nsum8), tb, 0, d->g, d->b[B_VRAD], d->b[B_VTHETA], d->b[B_DENS],
+                  d->b[B_EINT], d->b[B_DIVV], (double *)nullptr, (double *)nullptr, (double *)nullptr, d->b[B_QPLUS],
+                  d->b[B_TEMPER], d->b[B_CS], d->b[B_OPACITY], d->b[B_QMINUS], d->b[B_DIFF], d->pstellar,
+                  d->opa_filled ? 0 : 1, nsum8, snap3);
     }
     d->opa_filled = 1;
     {
@@ -6709,9 +6731,9 @@ extern "C" int thorin_rad_system(thorin_ctx *c, double dt, double *dt_used) {
   }
   if (dt_used) *dt_used = dt;
   HCHECK(hipMemcpyAsync(d->b[B_EINT], d->b[B_ENERGY], nb, hipMemcpyDeviceToDevice, d->s));
-  LAUNCH(k_substep3, gall, tb, 0, d->g, d->b[B_VRAD], d->b[B_VTHETA], d->b[B_DENS], d->b[B_EINT], d->b[B_DIVV],
-         d->b[B_TAURR], d->b[B_TAURP], d->b[B_TAUPP], d->b[B_QPLUS], d->b[B_TEMPER], d->b[B_CS], d->b[B_OPACITY],
-         d->b[B_QMINUS], d->b[B_DIFF], d->pstellar, d->opa_filled ? 0 : 1, 0, CrashSnap{});
+  SUB3_LAUNCH(gall, tb, 0, d->g, d->b[B_VRAD], d->b[B_VTHETA], d->b[B_DENS], d->b[B_EINT], d->b[B_DIVV],
+              d->b[B_TAURR], d->b[B_TAURP], d->b[B_TAUPP], d->b[B_QPLUS], d->b[B_TEMPER], d->b[B_CS],
+              d->b[B_OPACITY], d->b[B_QMINUS], d->b[B_DIFF], d->pstellar, d->opa_filled ? 0 : 1, 0, CrashSnap{});
   d->opa_filled = 1;
   const dim3 gm = cell_grid(d, nr - 2);
   LAUNCH(k_matrix, gm, tb, 0, d->g, d->b[B_DENS], d->b[B_CS], d->b[B_DIFF], d->b[B_DIVV], d->b[B_QMINUS],
