@@ -187,6 +187,15 @@ struct ForceSet {
   double indir_px, indir_py;
 };
 
+// accretion heating sources of one substep (heatsrc, heatsrc_index,
+// heatsrc_max of global.h:21-28): k_matrix adds bfac*q[k] to the RHS of cell
+// idx[k] for k < n, in k order (EnergySources.c:291-294)
+struct HeatK {
+  int n, pad;
+  int idx[8];
+  double q[8];
+};
+
 struct thorin_dev {
   hipStream_t s;
   int nr, ns;
@@ -203,12 +212,14 @@ struct thorin_dev {
   double *sorp;                 // SOR partial sums (2 passes)
   int *sorc;                    // SOR partial counts
   double *pstellar;             // DiffusionTimestep block minima
-  double *mresp;                // block partials of the initial SOR residual
-  int nb_mresp;
+  double *mresp;                // tile partials of the initial SOR residual (and k_mp_resid's block partials)
   double *ringtmp;              // per-ring scratch
   int nblk_cells, nblk_sor;
   int opa_filled;
-  int normres0_valid;           // ctl->sor_normres0 holds the residual of the current matrix
+  int sys_form;                 // 1: the SOR's first launch forms the energy system's rows (gas step);
+                                // 0: B_ML .. B_RHS hold them (thorin_rad_system, the autotuner)
+  double sys_dt;                // the system's dt and accretion heating sources (sys_form)
+  HeatK sys_heat;
   int cfl_ready;                // hctl->newdt_raw holds the CFL minimum of the current fields
   int niter_hist[4], nsolve;    // SOR iteration counts of the last four solves
   int rs_rows;                  // rings per workgroup of k_ring_means (LDS holds them whole)
@@ -246,7 +257,7 @@ struct thorin_dev {
 static const char *const KNAMES[] = {
     "k_eos_cfl", "k_cfl_final", "k_forces", "k_forces_reduce", "k_indirect", "k_boundary", "k_substep1",
     "k_divstress", "k_visc_update", "k_substep2",
-    "k_matrix", "k_sor_resid0", "k_sor_iter", "k_sor_check", "k_energy_new", "k_transport_rad",
+    "k_matrix", "k_sor_iter", "k_sor_check", "k_energy_new", "k_transport_rad",
     "k_transport_theta", "k_velocities", "k_substep3", "k_peb_cfl", "k_peb_damp", "k_peb_source",
     "k_peb_substep1", "k_peb_accrete", "k_peb_evaporate", "k_ring_means", "k_eos_cfl_vel", "k_visc_chain", NULL};
 
@@ -1726,15 +1737,6 @@ __device__ double column_mass(double a, double b, double Hpeb, int &ok) {
 // left).  The window, Roche radius and Kley's fractions come from the host
 // with the reference's expressions; every cell of the window is one thread's.
 // qtrap/trapzd (Planet.c:176-214) integrate exp(-z^2) to QTRAPEPS = 1e-4.
-// accretion heating sources of one substep (heatsrc, heatsrc_index,
-// heatsrc_max of global.h:21-28): k_matrix adds bfac*q[k] to the RHS of cell
-// idx[k] for k < n, in k order (EnergySources.c:291-294)
-struct HeatK {
-  int n, pad;
-  int idx[8];
-  double q[8];
-};
-
 struct GasAccK {
   int np, pad;
   double x[8], y[8], z[8], rroche[8], facc1[8], facc2[8];
@@ -2796,23 +2798,26 @@ __device__ __forceinline__ double coefB(const Geo &g, const double *D, const dou
 // derivative of the pressure scale height, 0 for surfaces tilted away; the
 // edge rings' values feed no matrix row) and CalculateQirr
 // (EnergySources.c:103-135).  pow(Rsup[i], 1.5) is pow_rinf15[i + 1].
-__device__ __forceinline__ double qirr_cell(const Geo &g, int i, size_t l, const double *__restrict__ cs,
-                                            const double *__restrict__ opa, const double *__restrict__ rho) {
-  const size_t lip = l + g.ns, lim = l - g.ns;
-  const double csin = 0.5 * (cs[l] + cs[lim]);
-  const double csout = 0.5 * (cs[l] + cs[lip]);
+__device__ __forceinline__ double qirr_vals(const Geo &g, int i, double csl, double csip, double csim, double opal,
+                                            double rhol) {
+  const double csin = 0.5 * (csl + csim);
+  const double csout = 0.5 * (csl + csip);
   const double Hin = csin * g.pow_rinf15[i] * g.sqrt_adiabind_inv;
   const double Hout = csout * g.pow_rinf15[i + 1] * g.sqrt_adiabind_inv;
   const double dHdr = (Hout - Hin) * g.InvDiffRsup[i];
-  const double H = cs[l] * g.OmegaInv[i] * g.sqrt_adiabind_inv;
+  const double H = csl * g.OmegaInv[i] * g.sqrt_adiabind_inv;
   double fl = atan(dHdr) - atan((H - 0.4 * g.rstar) * g.InvRmed[i]);
   fl = (fl >= 0.0) ? sin(fl) : 0.0;
   if (!(fl > 0.0)) return 0.0;
   const double r2inv = 1.0 / (g.Rmed[i] * g.Rmed[i]);
-  const double tau = g.opacitydrop * 0.5 * opa[l] * rho[l];
+  const double tau = g.opacitydrop * 0.5 * opal * rhol;
   const double taueff = 0.375 * tau + 0.5 + 0.25 / tau;
   const double Tirr4 = g.tirr4fac * r2inv * fl;
   return 2.0 * g.stefan * Tirr4 / taueff;
+}
+__device__ __forceinline__ double qirr_cell(const Geo &g, int i, size_t l, const double *__restrict__ cs,
+                                            const double *__restrict__ opa, const double *__restrict__ rho) {
+  return qirr_vals(g, i, cs[l], cs[l + g.ns], cs[l - g.ns], opa[l], rho[l]);
 }
 
 __device__ __forceinline__ void b_matrix(const Geo &g, const double *__restrict__ rho,
@@ -2884,32 +2889,6 @@ __device__ __forceinline__ double sor_resid(const double *T, const double *Ml, c
   return T[l] * Ml[l] + T[lip] * Mlip[l] + T[lim] * Mlim[l] + T[ljp] * Mljp[l] + T[ljm] * Mljm[l] - RHS[l];
 }
 
-__device__ __forceinline__ void b_sor_resid0(const Geo &g, const double *__restrict__ T,
-                                                   const double *__restrict__ Ml, const double *__restrict__ Mlip,
-                                                   const double *__restrict__ Mlim, const double *__restrict__ Mljp,
-                                                   const double *__restrict__ Mljm, const double *__restrict__ RHS,
-                                                   double *__restrict__ partial) {
-  __shared__ double red[TB];
-  const int ns = g.ns;
-  int i, j, L;
-  if (!xcd_cell(g, 1, g.nr - 2, i, j, L)) return;
-  double r = 0.0;
-  if (j < ns) {
-    const size_t l = (size_t)i * ns + j;
-    const size_t ljp = (j == ns - 1) ? (size_t)i * ns : l + 1;
-    const size_t ljm = (j == 0) ? (size_t)i * ns + ns - 1 : l - 1;
-    r = fabs(sor_resid(T, Ml, Mlip, Mlim, Mljp, Mljm, RHS, l, l + ns, l - ns, ljp, ljm));
-  }
-  r = block_sum<TB>(r, red);
-  if (threadIdx.x == 0) partial[L] = r;
-}
-__global__ void __launch_bounds__(TB) k_sor_resid0(Geo g, const double *__restrict__ T,
-                                                   const double *__restrict__ Ml, const double *__restrict__ Mlip,
-                                                   const double *__restrict__ Mlim, const double *__restrict__ Mljp,
-                                                   const double *__restrict__ Mljm, const double *__restrict__ RHS,
-                                                   double *__restrict__ partial) {
-  b_sor_resid0(g, T, Ml, Mlip, Mlim, Mljp, Mljm, RHS, partial);
-}
 
 
 
@@ -2966,16 +2945,73 @@ static_assert((SR_TI + 12) * (SR_TJ + 12) <= 0xFFF + 1 && SR_TI * SR_TJ <= 0x3FF
               "SorCell packs the LDS index in 12 bits and the owned index in 10");
 static_assert(SR_TI % 2 == 0 && SR_TJ % 2 == 0, "the slot counts of a colour assume even tile sides");
 
-__device__ __forceinline__ void sor_load(SorCell &x, size_t l, const double *__restrict__ Ml,
-                                         const double *__restrict__ Mlip, const double *__restrict__ Mlim,
-                                         const double *__restrict__ Mljp, const double *__restrict__ Mljm,
-                                         const double *__restrict__ RHS) {
-  x.ml = Ml[l];
-  x.mip = Mlip[l];
-  x.mim = Mlim[l];
-  x.mjp = Mljp[l];
-  x.mjm = Mljm[l];
-  x.rhs = RHS[l];
+// The energy system the SOR solves (EnergySources.c:256-297): rows ml..mjm
+// and right-hand side rhs.  With form, the first launch of a solve forms the
+// rows of its staged cells from the system's inputs with k_matrix's
+// expressions (rho, cs, D, div v, Q-, Q+, T, the opacity for the stellar
+// irradiation, the accretion heating sources; D and cs staged in LDS with
+// T) and stores those of its owned cells, which the later launches read.
+struct SorSys {
+  double *ml, *mip, *mim, *mjp, *mjm, *rhs;
+  const double *rho, *cs, *D, *divv, *qminus, *qplus, *opa;
+  double dt;
+  HeatK heat;
+  int form;
+};
+
+__device__ __forceinline__ void sor_load(SorCell &x, const SorSys &S, size_t l) {
+  x.ml = S.ml[l];
+  x.mip = S.mip[l];
+  x.mim = S.mim[l];
+  x.mjp = S.mjp[l];
+  x.mjm = S.mjm[l];
+  x.rhs = S.rhs[l];
+}
+
+// k_matrix's row of cell l (ring i) from D, cs and T staged in LDS (index k,
+// rows of LW) and the cell's own inputs: A at the radial interfaces i and
+// i+1, B at the azimuthal interfaces j and j+1 (the scale height of the
+// cell on the interface's upper side), the same expressions in the same
+// order as coefA, coefB, b_matrix and qirr_cell
+struct SorIn {                  // a cell's own inputs of its row
+  double rho, qminus, divv, qplus, opa;
+};
+__device__ __forceinline__ SorIn sor_in(const Geo &g, const SorSys &S, size_t l) {
+  SorIn v;
+  v.rho = S.rho[l];
+  v.qminus = S.qminus[l];
+  v.divv = S.divv[l];
+  v.qplus = S.qplus[l];
+  v.opa = g.stellar_irr ? S.opa[l] : 0.0;
+  return v;
+}
+template <int LW>
+__device__ __forceinline__ void sor_form_lds(SorCell &x, const Geo &g, const SorSys &S, const SorIn &in,
+                                             const double *sd, const double *sc, const double *t, int i, size_t l,
+                                             int k) {
+  const double Dl = sd[k];
+  const double Al = 0.5 * (Dl + sd[k - LW]) * g.Rinf[i] * g.InvDiffRmed[i];
+  const double Alip = 0.5 * (sd[k + LW] + Dl) * g.Rinf[i + 1] * g.InvDiffRmed[i + 1];
+  const double invdxtheta2 = g.InvDxTheta2[i];
+  const double csl = sc[k];
+  const double H = csl * g.OmegaInv[i] * g.sqrt_adiabind_inv;
+  const double Bl = 0.5 * (Dl + sd[k - 1]) * invdxtheta2 * 2.0 * H;
+  const double Hjp = sc[k + 1] * g.OmegaInv[i] * g.sqrt_adiabind_inv;
+  const double Bljp = 0.5 * (sd[k + 1] + Dl) * invdxtheta2 * 2.0 * Hjp;
+  const double rl = in.rho, qm = in.qminus;
+  const double bfac = S.dt / (rl * g.cv);
+  const double afac = bfac * g.InvDiffRsup[i] * 2.0 * H / g.Rmed[i];
+  x.ml = 1.0 + in.divv * (g.adiabind - 1.0) * S.dt + 4.0 * bfac * qm + afac * (Alip + Al) + bfac * (Bljp + Bl);
+  x.mip = -afac * Alip;
+  x.mim = -afac * Al;
+  x.mjp = -bfac * Bljp;
+  x.mjm = -bfac * Bl;
+  const double tl = t[k];
+  double rhs = tl + bfac * (in.qplus + 3.0 * qm * tl);
+  if (g.stellar_irr) rhs += bfac * qirr_vals(g, i, csl, sc[k + LW], sc[k - LW], in.opa, rl);   // EnergySources.c:290
+  for (int q = 0; q < S.heat.n; q++)                                                             // EnergySources.c:291-294
+    if ((int)l == S.heat.idx[q]) rhs += bfac * S.heat.q[q];
+  x.rhs = rhs;
 }
 
 // one cell of a half-sweep; an owned cell's new value is also its iterate in
@@ -3126,22 +3162,35 @@ __device__ __forceinline__ void sor_stop_test(const double *__restrict__ psum, c
   }
 }
 
-template <int NIT, int NT = TB>
+// residual T.M - RHS of a cell from the T in LDS (EnergySources.c:435-449)
+template <int LW>
+__device__ __forceinline__ double sor_resid_lds(const SorCell &x, const double *t) {
+  const int k = x.k();
+  return t[k] * x.ml + t[k + LW] * x.mip + t[k - LW] * x.mim + t[k + 1] * x.mjp + t[k - 1] * x.mjm - x.rhs;
+}
+
+// FORM: the first launch of a solve that forms the rows (its own instance,
+// so that the other launches keep their registers)
+template <int NIT, int NT = TB, bool FORM = false>
 __device__ __forceinline__ void b_sor_iter(const Geo &g, double *__restrict__ T0, double *__restrict__ T1,
-                                           double *__restrict__ T2, double *__restrict__ T3,
-                                           const double *__restrict__ Ml,
-                                           const double *__restrict__ Mlip, const double *__restrict__ Mlim,
-                                           const double *__restrict__ Mljp, const double *__restrict__ Mljm,
-                                           const double *__restrict__ RHS, double omega, double *__restrict__ psum,
-                                           int *__restrict__ pcnt, DevCtl *__restrict__ ctl, int it,
-                                           const double *__restrict__ mresp, int nbm, int nbs) {
+                                           double *__restrict__ T2, double *__restrict__ T3, const SorSys &Sy,
+                                           double omega, double *__restrict__ psum, int *__restrict__ pcnt,
+                                           DevCtl *__restrict__ ctl, int it, double *__restrict__ mresp, int nbs) {
   static_assert(NIT >= 1 && NIT <= 3, "four iterate buffers hold at most three iterations per launch");
   static_assert(NT % TB == 0, "the partial sums keep the pattern of TB threads");
   typedef SorShape<NIT, NT> S;
-  __shared__ double t[S::LR * S::LW];
-  __shared__ double sacc[NIT][SR_TI * SR_TJ];
-  __shared__ int schg[NIT][SR_TI * SR_TJ];
+  constexpr int STG = S::LR * S::LW;                 // staged places
+  constexpr int NOWN = SR_TI * SR_TJ;
+  // the stop-test arrays (|residual| and changed flags of the owned cells
+  // per iteration) share their LDS with the staged D and cs of a forming
+  // launch, which are read before the first of them is written
+  constexpr int RAW = (NIT * NOWN * 3 / 2 > 2 * STG) ? NIT * NOWN * 3 / 2 : 2 * STG;
+  __shared__ double t[STG];
+  __shared__ double sraw[RAW];
   __shared__ double red[NT];
+  double(*sacc)[NOWN] = reinterpret_cast<double(*)[NOWN]>(sraw);
+  int(*schg)[NOWN] = reinterpret_cast<int(*)[NOWN]>(sraw + NIT * NOWN);
+  double *sd = sraw, *sc = sraw + STG;
   static_assert(2 * NIT * (NT / 64) <= NT, "red holds the wave partials of the stop-test sums");
   const int ns = g.ns, nr = g.nr;
   const int ntx = (ns + SR_TJ - 1) / SR_TJ;
@@ -3154,22 +3203,14 @@ __device__ __forceinline__ void b_sor_iter(const Geo &g, double *__restrict__ T0
   const int ti = i1 - i0;
   const int tid = threadIdx.x;
   if (it == 0) {
-    // first iteration of a solve: reset the control words, and give T1 and
-    // T2 the rings no sweep writes (0, NRAD-1 and the ghost ring)
-    if (bid == 0) {
-      // initial residual normres0 (EnergySources.c:435-451) from the matrix
-      // kernel's block partials
-      // in the pattern of TB threads whatever NT is (the other threads add
-      // +0.0, which leaves every partial as it is)
-      double r = tid < TB ? strided_sum<TB>(mresp, nbm) : 0.0;
-      r = block_sum<NT>(r, red);
-      if (tid == 0) {
-        ctl->sor_normres0 = r;
-        ctl->sor_done = 0;
-        ctl->sor_niter = 0;
-        ctl->sor_iter = 0;
-        ctl->sor_cur = 0;
-      }
+    // first iteration of a solve: reset the control words (k_sor_check forms
+    // normres0 from this launch's tile partials), and give T1 .. T3 the rings
+    // no sweep writes (0, NRAD-1 and the ghost ring)
+    if (bid == 0 && tid == 0) {
+      ctl->sor_done = 0;
+      ctl->sor_niter = 0;
+      ctl->sor_iter = 0;
+      ctl->sor_cur = 0;
     }
     if (ty == 0)
       for (int j = j0 + tid; j < j0 + tw; j += NT) {
@@ -3189,8 +3230,11 @@ __device__ __forceinline__ void b_sor_iter(const Geo &g, double *__restrict__ T0
   }
   double *X[4] = {T0, T1, T2, T3};
   const double *__restrict__ Tin = X[it % 4];
+  constexpr bool form = FORM;
   SorCell rc[S::NRED], bc[S::NBLK];
   const int P0 = (i0 + j0) & 1;
+  // the cells of the thread's slots; their rows are loaded here (issued
+  // before the staging loads), or formed after the staging
   auto setup = [&](SorCell *cs, int ncell, auto hc, int c) {
     constexpr int h = decltype(hc)::value;
 #pragma unroll
@@ -3205,7 +3249,7 @@ __device__ __forceinline__ void b_sor_iter(const Geo &g, double *__restrict__ T0
       while (j >= ns) j -= ns;
       cs[q].pk = ((rho + S::H) * S::LW + (kappa + S::H)) | (dd << 12) | ((dd == 0 ? rho * tw + kappa : 0) << 16);
       cs[q].gl = i * ns + j;
-      sor_load(cs[q], (size_t)i * ns + j, Ml, Mlip, Mlim, Mljp, Mljm, RHS);
+      if (!form) sor_load(cs[q], Sy, (size_t)i * ns + j);
     }
   };
   setup(rc, S::NRED, std::integral_constant<int, S::HR>(), 1);
@@ -3214,29 +3258,87 @@ __device__ __forceinline__ void b_sor_iter(const Geo &g, double *__restrict__ T0
   // phi) over LDS rows of the full width: every load of the thread is issued
   // before the first LDS store (one memory latency, not one per load).
   // Places outside the tile's rings or the grid are never read; they get 0.
-  constexpr int NSTG = (S::LR * S::LW + NT - 1) / NT;
+  constexpr int NSTG = (STG + NT - 1) / NT;
   const int sw = tw + 2 * S::H;
-  double sv[NSTG];
+  // (forming: T, D and cs in one batch, then the inputs of every slot cell
+  // before the first row is formed)
+  constexpr int NSRC = FORM ? 3 : 1;
+  {
+    const double *src[3] = {Tin, Sy.D, Sy.cs};
+    double *dst[3] = {t, sd, sc};
+    double sv[NSRC][NSTG];
 #pragma unroll
-  for (int q = 0; q < NSTG; q++) {
-    const int idx = tid + q * NT;
-    const int r = idx / S::LW, c = idx - r * S::LW;
-    const int i = i0 - S::H + r;
-    sv[q] = 0.0;
-    if (r < ti + 2 * S::H && c < sw && i >= 0 && i <= nr - 1) {
-      int j = j0 - S::H + c;
-      while (j < 0) j += ns;
-      while (j >= ns) j -= ns;
-      sv[q] = Tin[(size_t)i * ns + j];
+    for (int q = 0; q < NSTG; q++) {
+      const int idx = tid + q * NT;
+      const int r = idx / S::LW, c = idx - r * S::LW;
+      const int i = i0 - S::H + r;
+#pragma unroll
+      for (int a = 0; a < NSRC; a++) sv[a][q] = 0.0;
+      if (r < ti + 2 * S::H && c < sw && i >= 0 && i <= nr - 1) {
+        int j = j0 - S::H + c;
+        while (j < 0) j += ns;
+        while (j >= ns) j -= ns;
+#pragma unroll
+        for (int a = 0; a < NSRC; a++) sv[a][q] = src[a][(size_t)i * ns + j];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < NSTG; q++) {
+      const int idx = tid + q * NT;
+      if (idx < STG)
+#pragma unroll
+        for (int a = 0; a < NSRC; a++) dst[a][idx] = sv[a][q];
     }
   }
+  if constexpr (FORM) {
+    SorIn ri[S::NRED], bi[S::NBLK];
 #pragma unroll
-  for (int q = 0; q < NSTG; q++) {
-    const int idx = tid + q * NT;
-    if (idx < S::LR * S::LW) t[idx] = sv[q];
+    for (int q = 0; q < S::NRED; q++)
+      if (rc[q].pk >= 0) ri[q] = sor_in(g, Sy, (size_t)rc[q].gl);
+#pragma unroll
+    for (int q = 0; q < S::NBLK; q++)
+      if (bc[q].pk >= 0) bi[q] = sor_in(g, Sy, (size_t)bc[q].gl);
+    __syncthreads();
+    // the rows of the staged cells; the owned cells' rows are stored
+    auto form_rows = [&](SorCell *cs, const SorIn *in, int ncell) {
+#pragma unroll
+      for (int q = 0; q < ncell; q++) {
+        if (cs[q].pk < 0) continue;
+        const size_t l = (size_t)cs[q].gl;
+        const int k = cs[q].k();
+        const int i = i0 + k / S::LW - S::H;        // the cell's ring from its LDS row
+        sor_form_lds<S::LW>(cs[q], g, Sy, in[q], sd, sc, t, i, l, k);
+        if (cs[q].d() == 0) {
+          Sy.ml[l] = cs[q].ml;
+          Sy.mip[l] = cs[q].mip;
+          Sy.mim[l] = cs[q].mim;
+          Sy.mjp[l] = cs[q].mjp;
+          Sy.mjm[l] = cs[q].mjm;
+          Sy.rhs[l] = cs[q].rhs;
+        }
+      }
+    };
+    form_rows(rc, ri, S::NRED);
+    form_rows(bc, bi, S::NBLK);
   }
-  __syncthreads();
+  __syncthreads();              // (forming: every read of sd, sc before sacc is written)
   const int nown = ti * tw;
+  if (it == 0) {
+    // the solve's initial residual sum|T.M - RHS| (EnergySources.c:435-451)
+    // over the owned cells from the staged T, added as the stop-test sums
+    // are; k_sor_check adds the tiles' partials in tile order
+#pragma unroll
+    for (int q = 0; q < S::NRED; q++)
+      if (rc[q].pk >= 0 && rc[q].d() == 0) sacc[0][rc[q].o()] = fabs(sor_resid_lds<S::LW>(rc[q], t));
+#pragma unroll
+    for (int q = 0; q < S::NBLK; q++)
+      if (bc[q].pk >= 0 && bc[q].d() == 0) sacc[0][bc[q].o()] = fabs(sor_resid_lds<S::LW>(bc[q], t));
+    __syncthreads();
+    double r[1] = {0.0};
+    for (int idx = tid; tid < TB && idx < nown; idx += TB) r[0] += sacc[0][idx];
+    block_sums<NT, 1>(r, red);      // (its closing barrier: every read of sacc done before the sweeps write it)
+    if (tid == 0) mresp[bid] = r[0];
+  }
 #pragma unroll
   for (int k = 0; k < NIT; k++) {
     const int hr = 2 * NIT - 1 - 2 * k, hb = hr - 1;
@@ -3276,17 +3378,12 @@ __device__ __forceinline__ void b_sor_iter(const Geo &g, double *__restrict__ T0
   }
 }
 
-template <int NIT, int NT = TB>
-__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 2 * TB ? 4 : 1))) k_sor_iter(Geo g, double *__restrict__ T0, double *__restrict__ T1,
-                                                 double *__restrict__ T2, double *__restrict__ T3,
-                                                 const double *__restrict__ Ml,
-                                                 const double *__restrict__ Mlip, const double *__restrict__ Mlim,
-                                                 const double *__restrict__ Mljp, const double *__restrict__ Mljm,
-                                                 const double *__restrict__ RHS, double omega, double *__restrict__ psum,
-                                                 int *__restrict__ pcnt, DevCtl *__restrict__ ctl, int it,
-                                                 const double *__restrict__ mresp, int nbm, int nbs) {
-  b_sor_iter<NIT, NT>(g, T0, T1, T2, T3, Ml, Mlip, Mlim, Mljp, Mljm, RHS, omega, psum, pcnt, ctl, it, mresp,
-                           nbm, nbs);
+template <int NIT, int NT = TB, bool FORM = false>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 2 * TB ? 4 : 1)))
+k_sor_iter(Geo g, double *__restrict__ T0, double *__restrict__ T1, double *__restrict__ T2, double *__restrict__ T3,
+           SorSys sy, double omega, double *__restrict__ psum, int *__restrict__ pcnt, DevCtl *__restrict__ ctl, int it,
+           double *__restrict__ mresp, int nbs) {
+  b_sor_iter<NIT, NT, FORM>(g, T0, T1, T2, T3, sy, omega, psum, pcnt, ctl, it, mresp, nbs);
 }
 
 // stop test of iterations it+1 .. it+NIT (EnergySources.c:487-493), partial
@@ -3294,16 +3391,24 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 2
 // meets it
 template <int NIT>
 __device__ __forceinline__ void b_sor_check(const double *__restrict__ psum, const int *__restrict__ pcnt, int nb,
-                                            DevCtl *ctl, int maxiter, int it, double eps) {
+                                            DevCtl *ctl, int maxiter, int it, double eps,
+                                            const double *__restrict__ mresp) {
   __shared__ double red[RB];
+  if (it == 0) {
+    // the initial residual normres0 (EnergySources.c:435-451): the first
+    // launch's tile partials added in tile order
+    const double r = block_sum<RB>(strided_sum<RB>(mresp, nb), red);
+    if (threadIdx.x == 0) ctl->sor_normres0 = r;
+  }
   if (ctl->sor_done) return;
   sor_stop_test<NIT, RB>(psum, pcnt, nb, ctl, maxiter, it, eps, red);
 }
 
 template <int NIT>
 __global__ void __launch_bounds__(RB) k_sor_check(const double *__restrict__ psum, const int *__restrict__ pcnt,
-                                                  int nb, DevCtl *ctl, int maxiter, int it, double eps) {
-  b_sor_check<NIT>(psum, pcnt, nb, ctl, maxiter, it, eps);
+                                                  int nb, DevCtl *ctl, int maxiter, int it, double eps,
+                                                  const double *__restrict__ mresp) {
+  b_sor_check<NIT>(psum, pcnt, nb, ctl, maxiter, it, eps, mresp);
 }
 
 // edges of T and EnergyNew = rho*T*CV (EnergySources.c:319-334).  The SOR's
@@ -4109,7 +4214,10 @@ extern "C" int thorin_dev_create(thorin_ctx *c) {
     HCHECK(hipMalloc(&d->sorc, sizeof(int) * np));
   }
   HCHECK(hipMalloc(&d->pstellar, sizeof(double) * (size_t)d->nblk_cells));
-  HCHECK(hipMalloc(&d->mresp, sizeof(double) * (size_t)d->nblk_cells));
+  {
+    const int nbs = ((c->ns + SR_TJ - 1) / SR_TJ) * ((c->nr - 2 + SR_TI - 1) / SR_TI);
+    HCHECK(hipMalloc(&d->mresp, sizeof(double) * (size_t)std::max(d->nblk_cells, nbs)));
+  }
   HCHECK(hipHostMalloc(&d->h_part, sizeof(double) * (size_t)d->nblk_cells, hipHostMallocDefault));
   HCHECK(hipEventCreate(&d->ev_t0));
   HCHECK(hipEventCreate(&d->ev_t1));
@@ -4544,10 +4652,12 @@ BATCH_BODY(k_visc_chain, b_visc_chain, VC_T)
 BATCH_BODY(k_substep3<false>, b_substep3<false>, TB)
 BATCH_BODY(k_substep3<true>, b_substep3<true>, TB)
 BATCH_BODY(k_matrix, b_matrix, TB)
-BATCH_BODY(k_sor_resid0, b_sor_resid0, TB)
-BATCH_BODY((k_sor_iter<1, SOR_NT(1)>), (b_sor_iter<1, SOR_NT(1)>), SOR_NT(1))
-BATCH_BODY((k_sor_iter<2, SOR_NT(2)>), (b_sor_iter<2, SOR_NT(2)>), SOR_NT(2))
-BATCH_BODY((k_sor_iter<3, SOR_NT(3)>), (b_sor_iter<3, SOR_NT(3)>), SOR_NT(3))
+BATCH_BODY((k_sor_iter<1, SOR_NT(1), false>), (b_sor_iter<1, SOR_NT(1), false>), SOR_NT(1))
+BATCH_BODY((k_sor_iter<2, SOR_NT(2), false>), (b_sor_iter<2, SOR_NT(2), false>), SOR_NT(2))
+BATCH_BODY((k_sor_iter<3, SOR_NT(3), false>), (b_sor_iter<3, SOR_NT(3), false>), SOR_NT(3))
+BATCH_BODY((k_sor_iter<1, SOR_NT(1), true>), (b_sor_iter<1, SOR_NT(1), true>), SOR_NT(1))
+BATCH_BODY((k_sor_iter<2, SOR_NT(2), true>), (b_sor_iter<2, SOR_NT(2), true>), SOR_NT(2))
+BATCH_BODY((k_sor_iter<3, SOR_NT(3), true>), (b_sor_iter<3, SOR_NT(3), true>), SOR_NT(3))
 BATCH_BODY(k_sor_check<1>, b_sor_check<1>, RB)
 BATCH_BODY(k_sor_check<2>, b_sor_check<2>, RB)
 BATCH_BODY(k_sor_check<3>, b_sor_check<3>, RB)
@@ -4881,37 +4991,52 @@ static int sor_launch(thorin_ctx *c, double omega, int it0, int &count, double e
   thorin_dev *d = c->dev;
   const Geo &g = d->g;
   const int nr = d->nr, ns = d->ns;
-  // initial residual (EnergySources.c:435-451); the matrix is unchanged and
-  // T is the same initial guess for every solve of a substep
-  if (it0 == 0 && !d->normres0_valid) {
-    dim3 gc = cell_grid(d, nr - 2);
-    LAUNCH(k_sor_resid0, gc, dim3(TB), 0, g, d->b[B_TEMPER], d->b[B_ML], d->b[B_MLIP], d->b[B_MLIM], d->b[B_MLJP],
-           d->b[B_MLJM], d->b[B_RHS], d->mresp);
-    d->nb_mresp = cell_count(d, nr - 2);
-    d->normres0_valid = 1;
-  }
+  // the energy system: its rows, or (sys_form) their inputs for the first
+  // launch of a solve; every solve's first launch forms the initial residual
+  SorSys sy;
+  sy.ml = d->b[B_ML];
+  sy.mip = d->b[B_MLIP];
+  sy.mim = d->b[B_MLIM];
+  sy.mjp = d->b[B_MLJP];
+  sy.mjm = d->b[B_MLJM];
+  sy.rhs = d->b[B_RHS];
+  sy.rho = d->b[B_DENS];
+  sy.cs = d->b[B_CS];
+  sy.D = d->b[B_DIFF];
+  sy.divv = d->b[B_DIVV];
+  sy.qminus = d->b[B_QMINUS];
+  sy.qplus = d->b[B_QPLUS];
+  sy.opa = d->b[B_OPACITY];
+  sy.dt = d->sys_dt;
+  sy.heat = d->sys_heat;
+  sy.form = d->sys_form;
   const int nbs = ((ns + SR_TJ - 1) / SR_TJ) * ((nr - 2 + SR_TI - 1) / SR_TI);
   const dim3 gs((unsigned)((nbs + 7) / 8 * 8));
   // the iterations queued beyond the expected count are a margin: a launch
   // after the solve's end returns at once, so the rest goes in one launch
   // (an empty launch and its stop test cost about 9 us at 1024x1536)
   const int nmax = (d->sor_nit >= 1 && d->sor_nit <= 3) ? d->sor_nit : 1;
-#define SOR_LAUNCH(N)                                                                                             \
+#define SOR_LAUNCH(N, F)                                                                                          \
   do {                                                                                                            \
-    LAUNCH((k_sor_iter<N, SOR_NT(N)>), gs, dim3(SOR_NT(N)), 0, g, d->b[B_TEMPER], d->b[B_TEMPER2],         \
-           d->b[B_TEMPER3],                                                                                       \
-           d->b[B_TEMPER4], d->b[B_ML], d->b[B_MLIP], d->b[B_MLIM], d->b[B_MLJP], d->b[B_MLJM], d->b[B_RHS], omega, \
-           d->sorp, d->sorc, d->ctl, it, (const double *)d->mresp, d->nb_mresp, nbs);                            \
-    LAUNCH(k_sor_check<N>, dim3(1), dim3(RB), 0, d->sorp, d->sorc, nbs, d->ctl, maxiter, it, eps);               \
+    LAUNCH((k_sor_iter<N, SOR_NT(N), F>), gs, dim3(SOR_NT(N)), 0, g, d->b[B_TEMPER], d->b[B_TEMPER2],      \
+           d->b[B_TEMPER3], d->b[B_TEMPER4], sy, omega, d->sorp, d->sorc, d->ctl, it, d->mresp, nbs);             \
+    LAUNCH(k_sor_check<N>, dim3(1), dim3(RB), 0, d->sorp, d->sorc, nbs, d->ctl, maxiter, it, eps,                  \
+           (const double *)d->mresp);                                                                             \
   } while (0)
   // (a stop test made by the last workgroup of each single-iteration launch
   // needs a device-scope fence per workgroup: a single iteration then took
   // three times as long as with the separate check)
   for (int it = it0; it < it0 + count;) {
     const int nit = std::min(nmax, it0 + count - it);
-    if (nit == 3) SOR_LAUNCH(3);
-    else if (nit == 2) SOR_LAUNCH(2);
-    else SOR_LAUNCH(1);
+    if (it == 0 && sy.form) {       // the solve's first launch forms the rows
+      if (nit == 3) SOR_LAUNCH(3, true);
+      else if (nit == 2) SOR_LAUNCH(2, true);
+      else SOR_LAUNCH(1, true);
+    } else {
+      if (nit == 3) SOR_LAUNCH(3, false);
+      else if (nit == 2) SOR_LAUNCH(2, false);
+      else SOR_LAUNCH(1, false);
+    }
     it += nit;
   }
 #undef SOR_LAUNCH
@@ -5674,11 +5799,12 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
     {
       const dim3 gm = cell_grid(d, nr - 2);
       heat.n = d->heat_max < 8 ? d->heat_max : 8;       // heatsrc_max as the matrix reads it
-      LAUNCH(k_matrix, gm, tb, 0, d->g, d->b[B_DENS], d->b[B_CS], d->b[B_DIFF], d->b[B_DIVV], d->b[B_QMINUS],
-             d->b[B_QPLUS], d->b[B_TEMPER], d->b[B_ML], d->b[B_MLIP], d->b[B_MLIM], d->b[B_MLJP], d->b[B_MLJM],
-             d->b[B_RHS], dt, d->mresp, (const double *)d->b[B_OPACITY], heat);
-      d->nb_mresp = cell_count(d, nr - 2);
-      d->normres0_valid = 1;
+      // the energy system (EnergySources.c:256-297): the first SOR launch of
+      // each solve forms its rows from SubStep3's fields (no matrix pass)
+      d->sys_form = 1;
+      d->sys_dt = dt;
+      d->sys_heat = heat;
+      (void)gm;
     }
     HCHECK(hipGetLastError());
     if (c->st.sor_first) {
@@ -6365,7 +6491,7 @@ extern "C" int thorin_autotune(thorin_ctx *c, thorin_tuning *out) {
                      d->b[B_MLIP], d->b[B_MLIM], d->b[B_MLJP], d->b[B_MLJM], d->b[B_RHS],
                      (const double *)d->b[B_TEMPER], d->nf);
   HCHECK(hipGetLastError());
-  d->normres0_valid = 0;
+  d->sys_form = 0;
   d->rad_ready = 0;
   const int saved_nit = d->sor_nit, saved_rows = d->rad_rows;
   const int iters = 6;          // whole launches of 1, 2 and 3 iterations
@@ -6413,7 +6539,6 @@ extern "C" int thorin_autotune(thorin_ctx *c, thorin_tuning *out) {
   }
   HCHECK(hipMemcpyAsync(d->b[B_TEMPER], d->b[B_TBACK], nb, hipMemcpyDeviceToDevice, d->s));
   HCHECK(hipStreamSynchronize(d->s));
-  d->normres0_valid = 0;
   (void)saved_nit;
   (void)saved_rows;
   t.sor_iters_per_launch = sor_pick(t.sor_us_per_iter, sor_expected(c));
@@ -6739,8 +6864,7 @@ extern "C" int thorin_rad_system(thorin_ctx *c, double dt, double *dt_used) {
   LAUNCH(k_matrix, gm, tb, 0, d->g, d->b[B_DENS], d->b[B_CS], d->b[B_DIFF], d->b[B_DIVV], d->b[B_QMINUS],
          d->b[B_QPLUS], d->b[B_TEMPER], d->b[B_ML], d->b[B_MLIP], d->b[B_MLIM], d->b[B_MLJP], d->b[B_MLJM],
          d->b[B_RHS], dt, d->mresp, (const double *)d->b[B_OPACITY], HeatK{});
-  d->nb_mresp = cell_count(d, nr - 2);
-  d->normres0_valid = 1;
+  d->sys_form = 0;
   HCHECK(hipMemcpyAsync(d->b[B_TBACK], d->b[B_TEMPER], nb, hipMemcpyDeviceToDevice, d->s));
   HCHECK(hipGetLastError());
   HCHECK(hipStreamSynchronize(d->s));
@@ -6811,7 +6935,6 @@ static int rad_solve_on(thorin_ctx *c, const thorin_solve_opts *o, thorin_solve_
   }
   if (o->method == 0) {
     // the reference's SOR with stop test sum|r| (in-sweep) < tol * normres0
-    d->normres0_valid = 0;
     int niter = 0;
     if (sor_finish(c, o->omega, 0, &niter, o->tol, maxit, 16)) return -1;
     res->sweeps = niter > maxit ? maxit : niter;
