@@ -2946,11 +2946,13 @@ static_assert((SR_TI + 12) * (SR_TJ + 12) <= 0xFFF + 1 && SR_TI * SR_TJ <= 0x3FF
 static_assert(SR_TI % 2 == 0 && SR_TJ % 2 == 0, "the slot counts of a colour assume even tile sides");
 
 // The energy system the SOR solves (EnergySources.c:256-297): rows ml..mjm
-// and right-hand side rhs.  With form, the first launch of a solve forms the
+// and right-hand side rhs.  With form, every launch of a solve forms the
 // rows of its staged cells from the system's inputs with k_matrix's
 // expressions (rho, cs, D, div v, Q-, Q+, T, the opacity for the stellar
 // irradiation, the accretion heating sources; D and cs staged in LDS with
-// T) and stores those of its owned cells, which the later launches read.
+// T): the first one its right-hand side too, from the initial T, which it
+// stores for the owned cells (the later launches load it; their T is an
+// iterate).  The rows themselves are never stored.
 struct SorSys {
   double *ml, *mip, *mim, *mjp, *mjm, *rhs;
   const double *rho, *cs, *D, *divv, *qminus, *qplus, *opa;
@@ -2985,7 +2987,7 @@ __device__ __forceinline__ SorIn sor_in(const Geo &g, const SorSys &S, size_t l)
   v.opa = g.stellar_irr ? S.opa[l] : 0.0;
   return v;
 }
-template <int LW>
+template <int LW, bool RHS>
 __device__ __forceinline__ void sor_form_lds(SorCell &x, const Geo &g, const SorSys &S, const SorIn &in,
                                              const double *sd, const double *sc, const double *t, int i, size_t l,
                                              int k) {
@@ -3006,6 +3008,7 @@ __device__ __forceinline__ void sor_form_lds(SorCell &x, const Geo &g, const Sor
   x.mim = -afac * Al;
   x.mjp = -bfac * Bljp;
   x.mjm = -bfac * Bl;
+  if (!RHS) return;
   const double tl = t[k];
   double rhs = tl + bfac * (in.qplus + 3.0 * qm * tl);
   if (g.stellar_irr) rhs += bfac * qirr_vals(g, i, csl, sc[k + LW], sc[k - LW], in.opa, rl);   // EnergySources.c:290
@@ -3169,9 +3172,11 @@ __device__ __forceinline__ double sor_resid_lds(const SorCell &x, const double *
   return t[k] * x.ml + t[k + LW] * x.mip + t[k - LW] * x.mim + t[k + 1] * x.mjp + t[k - 1] * x.mjm - x.rhs;
 }
 
-// FORM: the first launch of a solve that forms the rows (its own instance,
-// so that the other launches keep their registers)
-template <int NIT, int NT = TB, bool FORM = false>
+// FM: 0 the rows are loaded; 1 the first launch of a solve that forms the
+// rows and right-hand sides (storing the owned cells' right-hand sides); 2 a
+// later launch that forms the rows and loads the right-hand sides.  Each its
+// own instance, so that the launches that load keep their registers.
+template <int NIT, int NT = TB, int FM = 0>
 __device__ __forceinline__ void b_sor_iter(const Geo &g, double *__restrict__ T0, double *__restrict__ T1,
                                            double *__restrict__ T2, double *__restrict__ T3, const SorSys &Sy,
                                            double omega, double *__restrict__ psum, int *__restrict__ pcnt,
@@ -3230,7 +3235,7 @@ __device__ __forceinline__ void b_sor_iter(const Geo &g, double *__restrict__ T0
   }
   double *X[4] = {T0, T1, T2, T3};
   const double *__restrict__ Tin = X[it % 4];
-  constexpr bool form = FORM;
+  constexpr bool form = FM != 0;
   SorCell rc[S::NRED], bc[S::NBLK];
   const int P0 = (i0 + j0) & 1;
   // the cells of the thread's slots; their rows are loaded here (issued
@@ -3249,7 +3254,8 @@ __device__ __forceinline__ void b_sor_iter(const Geo &g, double *__restrict__ T0
       while (j >= ns) j -= ns;
       cs[q].pk = ((rho + S::H) * S::LW + (kappa + S::H)) | (dd << 12) | ((dd == 0 ? rho * tw + kappa : 0) << 16);
       cs[q].gl = i * ns + j;
-      if (!form) sor_load(cs[q], Sy, (size_t)i * ns + j);
+      if (FM == 0) sor_load(cs[q], Sy, (size_t)i * ns + j);
+      if (FM == 2) cs[q].rhs = Sy.rhs[(size_t)i * ns + j];
     }
   };
   setup(rc, S::NRED, std::integral_constant<int, S::HR>(), 1);
@@ -3262,7 +3268,7 @@ __device__ __forceinline__ void b_sor_iter(const Geo &g, double *__restrict__ T0
   const int sw = tw + 2 * S::H;
   // (forming: T, D and cs in one batch, then the inputs of every slot cell
   // before the first row is formed)
-  constexpr int NSRC = FORM ? 3 : 1;
+  constexpr int NSRC = form ? 3 : 1;
   {
     const double *src[3] = {Tin, Sy.D, Sy.cs};
     double *dst[3] = {t, sd, sc};
@@ -3290,7 +3296,7 @@ __device__ __forceinline__ void b_sor_iter(const Geo &g, double *__restrict__ T0
         for (int a = 0; a < NSRC; a++) dst[a][idx] = sv[a][q];
     }
   }
-  if constexpr (FORM) {
+  if constexpr (form) {
     SorIn ri[S::NRED], bi[S::NBLK];
 #pragma unroll
     for (int q = 0; q < S::NRED; q++)
@@ -3299,7 +3305,8 @@ __device__ __forceinline__ void b_sor_iter(const Geo &g, double *__restrict__ T0
     for (int q = 0; q < S::NBLK; q++)
       if (bc[q].pk >= 0) bi[q] = sor_in(g, Sy, (size_t)bc[q].gl);
     __syncthreads();
-    // the rows of the staged cells; the owned cells' rows are stored
+    // the rows of the staged cells (and, first launch, their right-hand
+    // sides: the owned cells' are stored)
     auto form_rows = [&](SorCell *cs, const SorIn *in, int ncell) {
 #pragma unroll
       for (int q = 0; q < ncell; q++) {
@@ -3307,15 +3314,8 @@ __device__ __forceinline__ void b_sor_iter(const Geo &g, double *__restrict__ T0
         const size_t l = (size_t)cs[q].gl;
         const int k = cs[q].k();
         const int i = i0 + k / S::LW - S::H;        // the cell's ring from its LDS row
-        sor_form_lds<S::LW>(cs[q], g, Sy, in[q], sd, sc, t, i, l, k);
-        if (cs[q].d() == 0) {
-          Sy.ml[l] = cs[q].ml;
-          Sy.mip[l] = cs[q].mip;
-          Sy.mim[l] = cs[q].mim;
-          Sy.mjp[l] = cs[q].mjp;
-          Sy.mjm[l] = cs[q].mjm;
-          Sy.rhs[l] = cs[q].rhs;
-        }
+        sor_form_lds<S::LW, FM == 1>(cs[q], g, Sy, in[q], sd, sc, t, i, l, k);
+        if (FM == 1 && cs[q].d() == 0) Sy.rhs[l] = cs[q].rhs;
       }
     };
     form_rows(rc, ri, S::NRED);
@@ -3378,12 +3378,12 @@ __device__ __forceinline__ void b_sor_iter(const Geo &g, double *__restrict__ T0
   }
 }
 
-template <int NIT, int NT = TB, bool FORM = false>
+template <int NIT, int NT = TB, int FM = 0>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 2 * TB ? 4 : 1)))
 k_sor_iter(Geo g, double *__restrict__ T0, double *__restrict__ T1, double *__restrict__ T2, double *__restrict__ T3,
            SorSys sy, double omega, double *__restrict__ psum, int *__restrict__ pcnt, DevCtl *__restrict__ ctl, int it,
            double *__restrict__ mresp, int nbs) {
-  b_sor_iter<NIT, NT, FORM>(g, T0, T1, T2, T3, sy, omega, psum, pcnt, ctl, it, mresp, nbs);
+  b_sor_iter<NIT, NT, FM>(g, T0, T1, T2, T3, sy, omega, psum, pcnt, ctl, it, mresp, nbs);
 }
 
 // stop test of iterations it+1 .. it+NIT (EnergySources.c:487-493), partial
@@ -4652,12 +4652,15 @@ BATCH_BODY(k_visc_chain, b_visc_chain, VC_T)
 BATCH_BODY(k_substep3<false>, b_substep3<false>, TB)
 BATCH_BODY(k_substep3<true>, b_substep3<true>, TB)
 BATCH_BODY(k_matrix, b_matrix, TB)
-BATCH_BODY((k_sor_iter<1, SOR_NT(1), false>), (b_sor_iter<1, SOR_NT(1), false>), SOR_NT(1))
-BATCH_BODY((k_sor_iter<2, SOR_NT(2), false>), (b_sor_iter<2, SOR_NT(2), false>), SOR_NT(2))
-BATCH_BODY((k_sor_iter<3, SOR_NT(3), false>), (b_sor_iter<3, SOR_NT(3), false>), SOR_NT(3))
-BATCH_BODY((k_sor_iter<1, SOR_NT(1), true>), (b_sor_iter<1, SOR_NT(1), true>), SOR_NT(1))
-BATCH_BODY((k_sor_iter<2, SOR_NT(2), true>), (b_sor_iter<2, SOR_NT(2), true>), SOR_NT(2))
-BATCH_BODY((k_sor_iter<3, SOR_NT(3), true>), (b_sor_iter<3, SOR_NT(3), true>), SOR_NT(3))
+BATCH_BODY((k_sor_iter<1, SOR_NT(1), 0>), (b_sor_iter<1, SOR_NT(1), 0>), SOR_NT(1))
+BATCH_BODY((k_sor_iter<2, SOR_NT(2), 0>), (b_sor_iter<2, SOR_NT(2), 0>), SOR_NT(2))
+BATCH_BODY((k_sor_iter<3, SOR_NT(3), 0>), (b_sor_iter<3, SOR_NT(3), 0>), SOR_NT(3))
+BATCH_BODY((k_sor_iter<1, SOR_NT(1), 1>), (b_sor_iter<1, SOR_NT(1), 1>), SOR_NT(1))
+BATCH_BODY((k_sor_iter<2, SOR_NT(2), 1>), (b_sor_iter<2, SOR_NT(2), 1>), SOR_NT(2))
+BATCH_BODY((k_sor_iter<3, SOR_NT(3), 1>), (b_sor_iter<3, SOR_NT(3), 1>), SOR_NT(3))
+BATCH_BODY((k_sor_iter<1, SOR_NT(1), 2>), (b_sor_iter<1, SOR_NT(1), 2>), SOR_NT(1))
+BATCH_BODY((k_sor_iter<2, SOR_NT(2), 2>), (b_sor_iter<2, SOR_NT(2), 2>), SOR_NT(2))
+BATCH_BODY((k_sor_iter<3, SOR_NT(3), 2>), (b_sor_iter<3, SOR_NT(3), 2>), SOR_NT(3))
 BATCH_BODY(k_sor_check<1>, b_sor_check<1>, RB)
 BATCH_BODY(k_sor_check<2>, b_sor_check<2>, RB)
 BATCH_BODY(k_sor_check<3>, b_sor_check<3>, RB)
@@ -5028,14 +5031,18 @@ static int sor_launch(thorin_ctx *c, double omega, int it0, int &count, double e
   // three times as long as with the separate check)
   for (int it = it0; it < it0 + count;) {
     const int nit = std::min(nmax, it0 + count - it);
-    if (it == 0 && sy.form) {       // the solve's first launch forms the rows
-      if (nit == 3) SOR_LAUNCH(3, true);
-      else if (nit == 2) SOR_LAUNCH(2, true);
-      else SOR_LAUNCH(1, true);
+    if (!sy.form) {
+      if (nit == 3) SOR_LAUNCH(3, 0);
+      else if (nit == 2) SOR_LAUNCH(2, 0);
+      else SOR_LAUNCH(1, 0);
+    } else if (it == 0) {           // the solve's first launch: rows and right-hand sides
+      if (nit == 3) SOR_LAUNCH(3, 1);
+      else if (nit == 2) SOR_LAUNCH(2, 1);
+      else SOR_LAUNCH(1, 1);
     } else {
-      if (nit == 3) SOR_LAUNCH(3, false);
-      else if (nit == 2) SOR_LAUNCH(2, false);
-      else SOR_LAUNCH(1, false);
+      if (nit == 3) SOR_LAUNCH(3, 2);
+      else if (nit == 2) SOR_LAUNCH(2, 2);
+      else SOR_LAUNCH(1, 2);
     }
     it += nit;
   }

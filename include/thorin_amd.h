@@ -170,8 +170,11 @@ enum {
 };
 
 /* Read-only diagnostics for thorin_download_field: the energy equation's
- * five-point system of the last assembly (EnergySources.c:256-297, rings
- * 1..nrad-2) and the diffusion coefficient it was built from. */
+ * five-point system (EnergySources.c:256-297, rings 1..nrad-2) as the last
+ * thorin_rad_system assembled it, and the diffusion coefficient.  A gas
+ * step forms the rows inside its SOR launches and stores only the
+ * right-hand side: after thorin_algogas, THORIN_X_RHS and THORIN_X_DIFF are
+ * its last solve's, the five diagonals are left as they were. */
 enum { THORIN_X_ML = 64, THORIN_X_MLIP, THORIN_X_MLIM, THORIN_X_MLJP, THORIN_X_MLJM, THORIN_X_RHS,
        THORIN_X_DIFF,
        /* the fields of the run's first crash, taken where DetectCrash tests
