@@ -1039,17 +1039,30 @@ __device__ __forceinline__ void cfl_final_body(const Geo &g, DevCtl *ctl, const 
   // then one reduction; min and max are exact, so the order does not matter
   __shared__ double red[4][CF_T / 64];
   double ms = 1.0e38;
-  if (nstellar > 0) ms = strided_min<CF_T>(pstellar, nstellar, ms);
   double S = -1.0, P = -1.e30, msh = 1e30;
-  for (int i = 1 + threadIdx.x; i < g.nr; i += CF_T) {
-    const double v = g.ringmax[i];
-    S = v > S ? v : S;
-    if (g.pebbles) { const double u = g.pebmax[i]; P = u > P ? u : P; }
-  }
-  for (int i = threadIdx.x; i < g.nr - 1; i += CF_T) {
-    double dt = 2.0 * DPI * CFLSECURITY / (double)g.ns /
-                fabs(g.Vmoy[i] * g.InvRmed[i] - g.Vmoy[i + 1] * g.InvRmed[i + 1]);
+  const int tid = threadIdx.x, nr = g.nr;
+  auto shear = [&](double va, double vb, int i) {
+    const double dt = 2.0 * DPI * CFLSECURITY / (double)g.ns / fabs(va * g.InvRmed[i] - vb * g.InvRmed[i + 1]);
     if (dt < msh) msh = dt;
+  };
+  if (nr <= CF_T) {
+    // one ring per thread: the ring values are loaded with the first batch
+    // of stellar partials, one memory latency for all
+    const double v = (tid >= 1 && tid < nr) ? g.ringmax[tid] : -1.0;
+    const double u = (g.pebbles && tid >= 1 && tid < nr) ? g.pebmax[tid] : -1.e30;
+    const double va = (tid < nr - 1) ? g.Vmoy[tid] : 0.0, vb = (tid < nr - 1) ? g.Vmoy[tid + 1] : 0.0;
+    if (nstellar > 0) ms = strided_min<CF_T>(pstellar, nstellar, ms);
+    S = v > S ? v : S;
+    P = u > P ? u : P;
+    if (tid < nr - 1) shear(va, vb, tid);
+  } else {
+    if (nstellar > 0) ms = strided_min<CF_T>(pstellar, nstellar, ms);
+    for (int i = 1 + tid; i < nr; i += CF_T) {
+      const double v = g.ringmax[i];
+      S = v > S ? v : S;
+      if (g.pebbles) { const double u = g.pebmax[i]; P = u > P ? u : P; }
+    }
+    for (int i = tid; i < nr - 1; i += CF_T) shear(g.Vmoy[i], g.Vmoy[i + 1], i);
   }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) {
@@ -3106,20 +3119,27 @@ __device__ __forceinline__ bool sr_slot(int s, int c, int P0, int ti, int tw, in
   return false;
 }
 
+// (mresp non-null, a solve's first launch: the tiles' partials of the
+// initial residual are read with the others and added into normres0 the
+// same way)
 template <int NIT, int NT>
 __device__ __forceinline__ void sor_stop_test(const double *__restrict__ psum, const int *__restrict__ pcnt, int nb,
-                                              DevCtl *ctl, int maxiter, int it, double eps, double *red) {
+                                              DevCtl *ctl, int maxiter, int it, double eps, double *red,
+                                              const double *__restrict__ mresp) {
   // the partial sums of all NIT iterations are read together (eight loads
   // of each array in flight per thread); each thread adds its partials in
   // block order as a strided sum does
   double s[NIT], c[NIT];
   int ci[NIT];
+  double r0 = 0.0;
 #pragma unroll
   for (int k = 0; k < NIT; k++) { s[k] = 0.0; ci[k] = 0; }
   int b = threadIdx.x;
   for (; b + 7 * NT < nb; b += 8 * NT) {
-    double v[NIT][8];
+    double v[NIT][8], v0[8];
     int w[NIT][8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) v0[u] = mresp ? mresp[b + u * NT] : 0.0;
 #pragma unroll
     for (int k = 0; k < NIT; k++)
 #pragma unroll
@@ -3128,6 +3148,8 @@ __device__ __forceinline__ void sor_stop_test(const double *__restrict__ psum, c
         w[k][u] = pcnt[(size_t)k * nb + b + u * NT];
       }
 #pragma unroll
+    for (int u = 0; u < 8; u++) r0 += v0[u];
+#pragma unroll
     for (int k = 0; k < NIT; k++)
 #pragma unroll
       for (int u = 0; u < 8; u++) {
@@ -3135,16 +3157,22 @@ __device__ __forceinline__ void sor_stop_test(const double *__restrict__ psum, c
         ci[k] += w[k][u];
       }
   }
-  for (; b < nb; b += NT)
+  for (; b < nb; b += NT) {
+    if (mresp) r0 += mresp[b];
 #pragma unroll
     for (int k = 0; k < NIT; k++) {
       s[k] += psum[(size_t)k * nb + b];
       ci[k] += pcnt[(size_t)k * nb + b];
     }
+  }
 #pragma unroll
   for (int k = 0; k < NIT; k++) {
     s[k] = block_sum<NT>(s[k], red);
     c[k] = block_sum<NT>((double)ci[k], red);
+  }
+  if (mresp) {
+    r0 = block_sum<NT>(r0, red);
+    if (threadIdx.x == 0) ctl->sor_normres0 = r0;
   }
   if (threadIdx.x == 0) {
     for (int k = 0; k < NIT; k++) {
@@ -3394,14 +3422,11 @@ __device__ __forceinline__ void b_sor_check(const double *__restrict__ psum, con
                                             DevCtl *ctl, int maxiter, int it, double eps,
                                             const double *__restrict__ mresp) {
   __shared__ double red[RB];
-  if (it == 0) {
-    // the initial residual normres0 (EnergySources.c:435-451): the first
-    // launch's tile partials added in tile order
-    const double r = block_sum<RB>(strided_sum<RB>(mresp, nb), red);
-    if (threadIdx.x == 0) ctl->sor_normres0 = r;
-  }
-  if (ctl->sor_done) return;
-  sor_stop_test<NIT, RB>(psum, pcnt, nb, ctl, maxiter, it, eps, red);
+  // at iteration 0 the launch has just cleared sor_done, and the initial
+  // residual normres0 (EnergySources.c:435-451) is formed from its tile
+  // partials in tile order
+  if (it != 0 && ctl->sor_done) return;
+  sor_stop_test<NIT, RB>(psum, pcnt, nb, ctl, maxiter, it, eps, red, it == 0 ? mresp : nullptr);
 }
 
 template <int NIT>
