@@ -41,7 +41,7 @@ HBM_PEAK_GBS = 8000.0
 ALG_BYTES_PER_CELL = {
     "k_sor_iter": 64, "k_transport_theta": 104, "k_transport_rad": 80, "k_forces": 48,
     "k_substep1": 64, "k_divstress": 56, "k_visc_update": 64, "k_substep2": 56, "k_matrix": 104, "k_velocities": 56,
-    "k_energy_new": 24, "k_indirect": 48, "k_eos_cfl": 48, "k_eos_cfl_vel": 80, "k_sor_resid0": 56,
+    "k_indirect": 48, "k_eos_cfl": 48, "k_eos_cfl_vel": 80,
     "k_substep3": 96, "k_visc_chain": 72,
 }
 # Whole step: the fused-dataflow minimum of the hydro step, 82 field passes

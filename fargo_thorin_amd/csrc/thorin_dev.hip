@@ -257,7 +257,7 @@ struct thorin_dev {
 static const char *const KNAMES[] = {
     "k_eos_cfl", "k_cfl_final", "k_forces", "k_forces_reduce", "k_indirect", "k_boundary", "k_substep1",
     "k_divstress", "k_visc_update", "k_substep2",
-    "k_matrix", "k_sor_iter", "k_sor_check", "k_energy_new", "k_transport_rad",
+    "k_matrix", "k_sor_iter", "k_sor_check", "k_transport_rad",
     "k_transport_theta", "k_velocities", "k_substep3", "k_peb_cfl", "k_peb_damp", "k_peb_source",
     "k_peb_substep1", "k_peb_accrete", "k_peb_evaporate", "k_ring_means", "k_eos_cfl_vel", "k_visc_chain", NULL};
 
@@ -3436,34 +3436,6 @@ __global__ void __launch_bounds__(RB) k_sor_check(const double *__restrict__ psu
   b_sor_check<NIT>(psum, pcnt, nb, ctl, maxiter, it, eps, mresp);
 }
 
-// edges of T and EnergyNew = rho*T*CV (EnergySources.c:319-334).  The SOR's
-// last iterate is in T0 .. T3 (ctl->sor_cur); T0 receives it, so that T0
-// holds the solved temperature afterwards.  Nothing is done until the SOR
-// has converged (ctl->sor_done).
-__device__ __forceinline__ void b_energy_new(const Geo &g, const double *__restrict__ rho, double *__restrict__ T0,
-                                             const double *__restrict__ T1, const double *__restrict__ T2,
-                                             const double *__restrict__ T3, double *__restrict__ enew,
-                                             const DevCtl *__restrict__ ctl) {
-  const int ns = g.ns, nr = g.nr;
-  int i, j, L_;
-  if (ctl->sor_done == 0) return;
-  if (!xcd_cell(g, 0, g.nr, i, j, L_) || j >= ns) return;
-  const int cur = ctl->sor_cur;
-  const double *__restrict__ Ts = cur == 0 ? T0 : (cur == 1 ? T1 : (cur == 2 ? T2 : T3));
-  const size_t l = (size_t)i * ns + j;
-  double t = Ts[l];
-  if (i == 0) t = Ts[l + ns];
-  if (i == nr - 1) t = Ts[l - ns];
-  if (cur || i == 0 || i == nr - 1) T0[l] = t;
-  enew[l] = rho[l] * t * g.cv;
-  if (i == 0) enew[(size_t)nr * ns + j] = 0.0;
-}
-__global__ void __launch_bounds__(TB) k_energy_new(Geo g, const double *__restrict__ rho, double *__restrict__ T0,
-                                                   const double *__restrict__ T1, const double *__restrict__ T2,
-                                                   const double *__restrict__ T3, double *__restrict__ enew,
-                                                   const DevCtl *__restrict__ ctl) {
-  b_energy_new(g, rho, T0, T1, T2, T3, enew, ctl);
-}
 
 
 // ---------------------------------------------------------------------------
@@ -4689,7 +4661,6 @@ BATCH_BODY((k_sor_iter<3, SOR_NT(3), 2>), (b_sor_iter<3, SOR_NT(3), 2>), SOR_NT(
 BATCH_BODY(k_sor_check<1>, b_sor_check<1>, RB)
 BATCH_BODY(k_sor_check<2>, b_sor_check<2>, RB)
 BATCH_BODY(k_sor_check<3>, b_sor_check<3>, RB)
-BATCH_BODY(k_energy_new, b_energy_new, TB)
 BATCH_BODY(k_transport_rad<8>, b_transport_rad<8>, RT_T)
 BATCH_BODY(k_transport_rad<16>, b_transport_rad<16>, RT_T)
 BATCH_BODY(k_transport_rad<32>, b_transport_rad<32>, RT_T)
@@ -5078,7 +5049,8 @@ static int sor_launch(thorin_ctx *c, double omega, int it0, int &count, double e
 
 // Run the solve to its end with host checks between batches, the first
 // `launched` iterations being already enqueued.  The last iterate stays in
-// B_TEMPER, B_TEMPER2 or B_TEMPER3 as ctl->sor_cur says (k_energy_new picks it up).
+// B_TEMPER .. B_TEMPER4 as ctl->sor_cur says (the radial transport forms
+// EnergyNew from it).
 static int sor_finish(thorin_ctx *c, double omega, int launched, int *niter_out, double eps = SOREPS,
                       int maxiter = SORMAXITERS, int first_batch = 0) {
   thorin_dev *d = c->dev;
