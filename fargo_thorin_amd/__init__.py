@@ -100,7 +100,8 @@ def make_planets(masses, radii_au, feel_disk: bool = False) -> Planets:
 
 def _tuning_dict(t) -> dict:
     return {"sor_iters_per_launch": t.sor_iters_per_launch, "rad_rows": t.rad_rows,
-            "sor_us_per_iter": list(t.sor_us_per_iter), "rad_us": list(t.rad_us), "fused_chain": t.fused_chain}
+            "sor_us_per_iter": list(t.sor_us_per_iter), "rad_us": list(t.rad_us), "fused_chain": t.fused_chain,
+            "sor_assembly": t.sor_assembly}
 
 
 class Disk:
@@ -161,8 +162,10 @@ class Disk:
         _check(lib().thorin_autotune(self._h, C.byref(t)), "thorin_autotune")
         return _tuning_dict(t)
 
-    def set_tuning(self, sor_iters_per_launch: int = 3, rad_rows: int = 8, fused_chain: int = 1) -> None:
-        t = _capi.Tuning(sor_iters_per_launch=sor_iters_per_launch, rad_rows=rad_rows, fused_chain=fused_chain)
+    def set_tuning(self, sor_iters_per_launch: int = 3, rad_rows: int = 8, fused_chain: int = 1,
+                   sor_assembly: int = 1) -> None:
+        t = _capi.Tuning(sor_iters_per_launch=sor_iters_per_launch, rad_rows=rad_rows, fused_chain=fused_chain,
+                         sor_assembly=sor_assembly)
         _check(lib().thorin_set_tuning(self._h, C.byref(t)), "thorin_set_tuning")
 
     def tuning(self) -> dict:

@@ -68,7 +68,8 @@ class SolveResult(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("sor_iters_per_launch", C.c_int), ("rad_rows", C.c_int),
-                ("sor_us_per_iter", C.c_double * 3), ("rad_us", C.c_double * 3), ("fused_chain", C.c_int)]
+                ("sor_us_per_iter", C.c_double * 3), ("rad_us", C.c_double * 3), ("fused_chain", C.c_int),
+                ("sor_assembly", C.c_int)]
 
 
 NBODY_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(Planets), C.c_double, C.c_double, C.c_double)

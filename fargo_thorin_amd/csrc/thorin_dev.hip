@@ -225,6 +225,7 @@ struct thorin_dev {
   int rs_rows;                  // rings per workgroup of k_ring_means (LDS holds them whole)
   int sor_nit;                  // SOR iterations per launch (1, 2 or 3; thorin_tune)
   int visc_chain;               // SubStep1 .. SubStep2 as one kernel where the options allow it
+  int sor_assembly;             // the gas step's energy system formed inside its SOR launches
   int sor_queue_min;            // ensemble: SOR iterations to queue at least
   ForceSet spec;                // force kernels enqueued ahead for the next substep
   int heat_max;                 // heatsrc_max: set by the pebble accretion when AccretionalHeating is on
@@ -4030,6 +4031,7 @@ extern "C" int thorin_dev_create(thorin_ctx *c) {
     d->sor_nit = 3;
     d->rad_rows = 8;
     d->visc_chain = 1;
+    d->sor_assembly = 1;
     const size_t rs_lds = sizeof(double) * (size_t)d->rs_rows * c->ns;
     if (rs_lds > (size_t)prop.sharedMemPerBlock) {
       thorin_set_error("NSEC too large for the ring-mean kernel's LDS");
@@ -5803,12 +5805,19 @@ extern "C" int thorin_algogas(thorin_ctx *c, thorin_planets *sys, thorin_nbody_f
     {
       const dim3 gm = cell_grid(d, nr - 2);
       heat.n = d->heat_max < 8 ? d->heat_max : 8;       // heatsrc_max as the matrix reads it
-      // the energy system (EnergySources.c:256-297): the first SOR launch of
-      // each solve forms its rows from SubStep3's fields (no matrix pass)
-      d->sys_form = 1;
+      // the energy system (EnergySources.c:256-297): the SOR launches form
+      // its rows from SubStep3's fields (no matrix pass), or (sor_assembly 0)
+      // the matrix pass assembles it
       d->sys_dt = dt;
       d->sys_heat = heat;
-      (void)gm;
+      if (d->sor_assembly) {
+        d->sys_form = 1;
+      } else {
+        LAUNCH(k_matrix, gm, tb, 0, d->g, d->b[B_DENS], d->b[B_CS], d->b[B_DIFF], d->b[B_DIVV], d->b[B_QMINUS],
+               d->b[B_QPLUS], d->b[B_TEMPER], d->b[B_ML], d->b[B_MLIP], d->b[B_MLIM], d->b[B_MLJP], d->b[B_MLJM],
+               d->b[B_RHS], dt, d->mresp, (const double *)d->b[B_OPACITY], heat);
+        d->sys_form = 0;
+      }
     }
     HCHECK(hipGetLastError());
     if (c->st.sor_first) {
@@ -6551,6 +6560,7 @@ extern "C" int thorin_autotune(thorin_ctx *c, thorin_tuning *out) {
     if (t.rad_us[v] < t.rad_us[bi]) bi = v;
   t.rad_rows = rows[bi];
   t.fused_chain = d->visc_chain;
+  t.sor_assembly = d->sor_assembly;
   d->sor_nit = t.sor_iters_per_launch;
   d->rad_rows = t.rad_rows;
   if (out) *out = t;
@@ -6560,14 +6570,16 @@ extern "C" int thorin_autotune(thorin_ctx *c, thorin_tuning *out) {
 extern "C" int thorin_set_tuning(thorin_ctx *c, const thorin_tuning *t) {
   if (need_dev(c)) return -1;
   if (!t || t->sor_iters_per_launch < 1 || t->sor_iters_per_launch > 3 ||
-      (t->rad_rows != 8 && t->rad_rows != 16 && t->rad_rows != 32) || (t->fused_chain != 0 && t->fused_chain != 1)) {
+      (t->rad_rows != 8 && t->rad_rows != 16 && t->rad_rows != 32) || (t->fused_chain != 0 && t->fused_chain != 1) ||
+      (t->sor_assembly != 0 && t->sor_assembly != 1)) {
     thorin_set_error("thorin_set_tuning: sor_iters_per_launch must be 1, 2 or 3, rad_rows 8, 16 or 32, "
-                     "fused_chain 0 or 1");
+                     "fused_chain and sor_assembly 0 or 1");
     return -1;
   }
   c->dev->sor_nit = t->sor_iters_per_launch;
   c->dev->rad_rows = t->rad_rows;
   c->dev->visc_chain = t->fused_chain;
+  c->dev->sor_assembly = t->sor_assembly;
   return 0;
 }
 
@@ -6577,6 +6589,7 @@ extern "C" int thorin_get_tuning(thorin_ctx *c, thorin_tuning *t) {
   t->sor_iters_per_launch = c->dev->sor_nit;
   t->rad_rows = c->dev->rad_rows;
   t->fused_chain = c->dev->visc_chain;
+  t->sor_assembly = c->dev->sor_assembly;
   return 0;
 }
 
@@ -6601,6 +6614,7 @@ extern "C" int thorin_ensemble_autotune(thorin_ensemble *e, thorin_tuning *out) 
     if (sum.rad_us[v] < sum.rad_us[bi]) bi = v;
   sum.rad_rows = rows[bi];
   sum.fused_chain = e->n > 0 ? e->c[0]->dev->visc_chain : 1;
+  sum.sor_assembly = e->n > 0 ? e->c[0]->dev->sor_assembly : 1;
   for (int k = 0; k < e->n; k++)
     if (thorin_set_tuning(e->c[k], &sum)) return -1;
   if (out) *out = sum;

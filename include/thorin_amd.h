@@ -296,6 +296,9 @@ typedef struct thorin_tuning {
                                      boundary around them) as one kernel where the options allow it (the
                                      gas alone, a damping boundary on vrad only); 0: one kernel per step.
                                      Same bits either way; the autotuner leaves it as it is */
+  int sor_assembly;               /* 1 (default): the gas step's energy system is formed inside its SOR
+                                     launches; 0: a separate matrix pass assembles it and the SOR loads
+                                     it.  Same bits either way; the autotuner leaves it as it is */
 } thorin_tuning;
 int  thorin_autotune(thorin_ctx *ctx, thorin_tuning *out);
 int  thorin_set_tuning(thorin_ctx *ctx, const thorin_tuning *t);

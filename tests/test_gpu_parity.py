@@ -398,3 +398,38 @@ def test_gpu_fused_substep_chain_equals_separate_kernels(nrad, nsec, wall):
     assert runs[0][0] == runs[1][0]
     for f, a in runs[0][1].items():
         assert np.array_equal(a, runs[1][1][f]), f
+
+
+@pytest.mark.parametrize("name", ["wall_lp85_64x128", "irr_lp85_48x96", "damp_sm03_48x96", "pebheat_q3e-5_64x128"])
+def test_gpu_sor_assembly_equals_matrix_pass(name):
+    """The energy system formed inside the SOR launches (the default) against
+    the separate matrix pass that the SOR loads (sor_assembly 0), on the same
+    disk: every field, the SOR iterations, the CFL counts and the planet mass
+    bit for bit over 6 DTs.  The cases cover the stellar irradiation term of
+    the right-hand side, the SM03 opacity, and a two-fluid run whose pebble
+    accretion heating adds sources to it."""
+    _print_case("sor assembly", name)
+    cfg, planets, _ = case(name)
+    g = golden(name)
+    peb = name.startswith("pebheat")
+    fields = STATE_FIELDS + (PEB_FIELDS if peb else [])
+    runs = []
+    for asm in (1, 0):
+        d = Disk(params_for(cfg))
+        d.set_tuning(3, 8, sor_assembly=asm)
+        d.init(g["init_dens"], g["init_vrad"], g["init_vtheta"], g["init_temper"])
+        if peb:
+            d.init_pebbles(0)
+        sysp = planets_for(planets, cfg)
+        nb = host_nbody_fn()
+        logs = []
+        for _ in range(6):
+            lg = d.algogas(sysp, nb)
+            logs.append(([lg.cfl_n[k] for k in range(lg.nsub)], [lg.sor_niter[k] for k in range(lg.nsub)]))
+        assert d.tuning()["sor_assembly"] == asm
+        runs.append((logs, {f: d.field(f) for f in fields}, sysp.mass[0]))
+        d.close()
+    assert runs[0][0] == runs[1][0]
+    for f, a in runs[0][1].items():
+        assert np.array_equal(a, runs[1][1][f]), f
+    assert runs[0][2] == runs[1][2]
