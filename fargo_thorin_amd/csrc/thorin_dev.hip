@@ -3166,15 +3166,22 @@ __device__ __forceinline__ void sor_stop_test(const double *__restrict__ psum, c
       ci[k] += pcnt[(size_t)k * nb + b];
     }
   }
+  // every sum in one pass, each in block_sum's order (red holds NT doubles)
+  static_assert((2 * NIT + 1) * (NT / 64) <= NT, "red holds the wave partials of every stop-test sum");
+  double v[2 * NIT + 1];
 #pragma unroll
   for (int k = 0; k < NIT; k++) {
-    s[k] = block_sum<NT>(s[k], red);
-    c[k] = block_sum<NT>((double)ci[k], red);
+    v[2 * k] = s[k];
+    v[2 * k + 1] = (double)ci[k];
   }
-  if (mresp) {
-    r0 = block_sum<NT>(r0, red);
-    if (threadIdx.x == 0) ctl->sor_normres0 = r0;
+  v[2 * NIT] = r0;
+  block_sums<NT, 2 * NIT + 1>(v, red);
+#pragma unroll
+  for (int k = 0; k < NIT; k++) {
+    s[k] = v[2 * k];
+    c[k] = v[2 * k + 1];
   }
+  if (mresp && threadIdx.x == 0) ctl->sor_normres0 = v[2 * NIT];
   if (threadIdx.x == 0) {
     for (int k = 0; k < NIT; k++) {
       const int n = it + k + 1;
